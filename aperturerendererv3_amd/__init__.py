@@ -1,0 +1,202 @@
+"""Aperture MI355X path-tracing core: Python host mirror of include/ar.h.
+
+The compute path is libar_hip.so (hand-written HIP kernels for gfx950 behind
+a C ABI).  This module only marshals arguments: device memory comes from
+torch tensors on a ROCm device, and every call goes through the C ABI.  If
+the library is not built, or no GPU is present, calls raise; nothing here
+computes a result on the CPU.
+
+Naming follows the reference's host API (SceneRenderer / Camera in
+core/scene.hpp:18-135 and core/camera.hpp:14-99): a Context owns the scene,
+its BVH and the device state.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import (AR_MISS, AR_MODEL_GGX, AR_MODEL_LAMBERT, AR_OK, ar_bvh_info, ar_bvh_opts, ar_device_opts,
+                   ar_hits, ar_material, ar_mesh, ar_rays)
+
+__all__ = ["Context", "ArError", "AR_MISS", "AR_MODEL_LAMBERT", "AR_MODEL_GGX", "Material", "RAY_FIELDS"]
+
+RAY_FIELDS = ("ox", "oy", "oz", "tmin", "dx", "dy", "dz", "tmax")
+
+
+class ArError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class Material:
+    """Per-triangle material (triangle counterpart of the reference GBuffer)."""
+
+    def __init__(self, albedo=(0.75, 0.75, 0.75), emission=(0.0, 0.0, 0.0), specular=0.5, metallic=0.0,
+                 roughness=1.0, model=AR_MODEL_LAMBERT):
+        self.albedo = tuple(float(x) for x in albedo)
+        self.emission = tuple(float(x) for x in emission)
+        self.specular = float(specular)
+        self.metallic = float(metallic)
+        self.roughness = float(roughness)
+        self.model = int(model)
+
+    def to_c(self) -> ar_material:
+        m = ar_material()
+        m.albedo[:] = self.albedo
+        m.emission[:] = self.emission
+        m.specular, m.metallic, m.roughness, m.model = self.specular, self.metallic, self.roughness, self.model
+        return m
+
+
+def _ptr(t) -> int:
+    return int(t.data_ptr())
+
+
+def _stream_ptr(stream) -> int | None:
+    if stream is None:
+        import torch
+        return int(torch.cuda.current_stream().cuda_stream)
+    return int(stream.cuda_stream)
+
+
+class Context:
+    """One device context (ar_ctx).  Thread-compatible, not thread-safe."""
+
+    def __init__(self, device: int = 0):
+        self._lib = _lib.load()
+        h = C.c_void_p()
+        opts = ar_device_opts(device, 0)
+        rc = self._lib.ar_create(C.byref(h), C.byref(opts))
+        if rc != AR_OK:
+            raise ArError(rc, f"ar_create(device={device}) failed")
+        self._h = h
+        self.device = device
+        self._keep = []
+
+    # -- lifetime ----------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.ar_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc: int, what: str):
+        if rc != AR_OK:
+            msg = self._lib.ar_last_error(self._h)
+            raise ArError(rc, f"{what}: {msg.decode() if msg else ''}")
+
+    # -- scene -------------------------------------------------------------
+    def set_meshes(self, meshes):
+        """meshes: list of dicts with 'vertices' (nv,3) f32, 'indices' (ntri,3) u32,
+        optional 'tri_material' (ntri,) u32 and 'materials' (list of Material)."""
+        cms = (ar_mesh * max(1, len(meshes)))()
+        keep = []
+        for i, m in enumerate(meshes):
+            v = np.ascontiguousarray(m["vertices"], dtype=np.float32).reshape(-1, 3)
+            ix = np.ascontiguousarray(m["indices"], dtype=np.uint32).reshape(-1, 3)
+            keep += [v, ix]
+            cms[i].vertices = v.ctypes.data
+            cms[i].nv = v.shape[0]
+            cms[i].indices = ix.ctypes.data
+            cms[i].ntri = ix.shape[0]
+            tm = m.get("tri_material")
+            if tm is not None:
+                tm = np.ascontiguousarray(tm, dtype=np.uint32)
+                keep.append(tm)
+                cms[i].tri_material = tm.ctypes.data
+            mats = m.get("materials") or []
+            if mats:
+                arr = (ar_material * len(mats))(*[mm.to_c() for mm in mats])
+                keep.append(arr)
+                cms[i].materials = C.cast(arr, C.c_void_p)
+                cms[i].nmaterials = len(mats)
+        self._check(self._lib.ar_scene_set(self._h, cms, len(meshes)), "ar_scene_set")
+
+    def build_bvh(self, width=2, leaf_size=4, bin_count=16, max_depth=60) -> dict:
+        o = ar_bvh_opts(width, leaf_size, bin_count, max_depth)
+        self._check(self._lib.ar_bvh_build(self._h, C.byref(o)), "ar_bvh_build")
+        return self.bvh_info()
+
+    def bvh_info(self) -> dict:
+        info = ar_bvh_info()
+        self._check(self._lib.ar_bvh_info_get(self._h, C.byref(info)), "ar_bvh_info_get")
+        return {k: getattr(info, k) for k, _ in info._fields_}
+
+    # -- traversal -----------------------------------------------------------
+    @staticmethod
+    def _rays_c(rays) -> ar_rays:
+        """rays: float32 CUDA tensor of shape (8, n) with rows RAY_FIELDS."""
+        assert rays.dim() == 2 and rays.shape[0] == 8 and rays.is_contiguous() and rays.is_cuda
+        import torch
+        assert rays.dtype == torch.float32
+        r = ar_rays()
+        for k, name in enumerate(RAY_FIELDS):
+            setattr(r, name, _ptr(rays[k]))
+        return r
+
+    @staticmethod
+    def alloc_hits(n: int, device="cuda"):
+        import torch
+        return {"prim": torch.empty(n, dtype=torch.int32, device=device),
+                "tuv": torch.empty((3, n), dtype=torch.float32, device=device),
+                "flags": torch.empty(n, dtype=torch.int32, device=device)}
+
+    @staticmethod
+    def _hits_c(h) -> ar_hits:
+        c = ar_hits()
+        c.prim = _ptr(h["prim"])
+        c.t = _ptr(h["tuv"][0])
+        c.u = _ptr(h["tuv"][1])
+        c.v = _ptr(h["tuv"][2])
+        c.flags = _ptr(h["flags"])
+        return c
+
+    def trace_closest(self, rays, hits=None, stream=None, counters=None):
+        n = rays.shape[1]
+        if hits is None:
+            hits = self.alloc_hits(n, rays.device)
+        rc, hc = self._rays_c(rays), self._hits_c(hits)
+        if counters is not None:
+            self._check(self._lib.ar_trace_closest_counted(self._h, C.byref(rc), n, C.byref(hc),
+                                                           _ptr(counters[0]), _ptr(counters[1]),
+                                                           _stream_ptr(stream)), "ar_trace_closest_counted")
+        else:
+            self._check(self._lib.ar_trace_closest(self._h, C.byref(rc), n, C.byref(hc), _stream_ptr(stream)),
+                        "ar_trace_closest")
+        return hits
+
+    def trace_any(self, rays, occluded=None, stream=None):
+        import torch
+        n = rays.shape[1]
+        if occluded is None:
+            occluded = torch.empty((n + 31) // 32, dtype=torch.int32, device=rays.device)
+        rc = self._rays_c(rays)
+        self._check(self._lib.ar_trace_any(self._h, C.byref(rc), n, _ptr(occluded), _stream_ptr(stream)),
+                    "ar_trace_any")
+        return occluded
+
+    def make_shadow_rays(self, rays, hits, target, tmin=1e-4, tmax_scale=1e-4, out=None, stream=None):
+        import torch
+        n = rays.shape[1]
+        if out is None:
+            out = torch.empty_like(rays)
+        rc, hc, oc = self._rays_c(rays), self._hits_c(hits), self._rays_c(out)
+        tgt = (C.c_float * 3)(*[float(x) for x in target])
+        self._check(self._lib.ar_make_shadow_rays(self._h, C.byref(rc), C.byref(hc), n, C.byref(tgt), tmin,
+                                                  tmax_scale, C.byref(oc), _stream_ptr(stream)),
+                    "ar_make_shadow_rays")
+        return out

@@ -1,0 +1,104 @@
+"""ctypes declarations of include/ar.h and the loader of libar_hip.so.
+
+The library is loaded from the package directory only.  When it is missing
+the import of the GPU API raises: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libar_hip.so"
+HEADER = Path(__file__).resolve().parent.parent / "include" / "ar.h"
+
+AR_OK = 0
+AR_E_INVALID = -1
+AR_E_DEVICE = -2
+AR_E_NOSCENE = -3
+AR_E_OOM = -4
+AR_E_UNSUPPORTED = -5
+AR_E_CAPACITY = -6
+AR_MISS = 0xFFFFFFFF
+
+AR_MODEL_LAMBERT = 0
+AR_MODEL_GGX = 1
+
+FP = C.POINTER(C.c_float)
+UP = C.POINTER(C.c_uint32)
+
+
+class ar_device_opts(C.Structure):
+    _fields_ = [("device", C.c_int), ("flags", C.c_uint32)]
+
+
+class ar_material(C.Structure):
+    _fields_ = [("albedo", C.c_float * 3), ("emission", C.c_float * 3), ("specular", C.c_float),
+                ("metallic", C.c_float), ("roughness", C.c_float), ("model", C.c_uint32)]
+
+
+class ar_mesh(C.Structure):
+    _fields_ = [("vertices", C.c_void_p), ("nv", C.c_uint32), ("indices", C.c_void_p), ("ntri", C.c_uint32),
+                ("tri_material", C.c_void_p), ("materials", C.c_void_p), ("nmaterials", C.c_uint32)]
+
+
+class ar_rays(C.Structure):
+    _fields_ = [(k, C.c_void_p) for k in ("ox", "oy", "oz", "tmin", "dx", "dy", "dz", "tmax")]
+
+
+class ar_hits(C.Structure):
+    _fields_ = [(k, C.c_void_p) for k in ("prim", "t", "u", "v", "flags")]
+
+
+class ar_bvh_opts(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("leaf_size", C.c_uint32), ("bin_count", C.c_uint32),
+                ("max_depth", C.c_uint32)]
+
+
+class ar_bvh_info(C.Structure):
+    _fields_ = [("node_count", C.c_uint64), ("leaf_count", C.c_uint64), ("depth", C.c_uint32),
+                ("width", C.c_uint32), ("device_bytes", C.c_uint64), ("build_seconds", C.c_double),
+                ("sah_cost", C.c_double)]
+
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "ar_abi_version": (C.c_int, []),
+    "ar_create": (C.c_int, [C.POINTER(C.c_void_p), C.POINTER(ar_device_opts)]),
+    "ar_destroy": (C.c_int, [C.c_void_p]),
+    "ar_last_error": (C.c_char_p, [C.c_void_p]),
+    "ar_scene_set": (C.c_int, [C.c_void_p, C.POINTER(ar_mesh), C.c_size_t]),
+    "ar_bvh_build": (C.c_int, [C.c_void_p, C.POINTER(ar_bvh_opts)]),
+    "ar_bvh_info_get": (C.c_int, [C.c_void_p, C.POINTER(ar_bvh_info)]),
+    "ar_trace_closest": (C.c_int, [C.c_void_p, C.POINTER(ar_rays), C.c_size_t, C.POINTER(ar_hits), C.c_void_p]),
+    "ar_trace_any": (C.c_int, [C.c_void_p, C.POINTER(ar_rays), C.c_size_t, C.c_void_p, C.c_void_p]),
+    "ar_trace_closest_counted": (C.c_int, [C.c_void_p, C.POINTER(ar_rays), C.c_size_t, C.POINTER(ar_hits),
+                                           C.c_void_p, C.c_void_p, C.c_void_p]),
+    "ar_make_shadow_rays": (C.c_int, [C.c_void_p, C.POINTER(ar_rays), C.POINTER(ar_hits), C.c_size_t,
+                                      C.POINTER(C.c_float * 3), C.c_float, C.c_float, C.POINTER(ar_rays),
+                                      C.c_void_p]),
+}
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Load libar_hip.so from the package directory (raises if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(f"{LIB_PATH} is not built: run python -m aperturerendererv3_amd.build")
+    lib = C.CDLL(str(LIB_PATH))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def declared_symbols() -> list[str]:
+    """Function names declared in include/ar.h."""
+    import re
+    text = HEADER.read_text()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(ar_[a-z_0-9]+)\s*\(", text, re.M)))

@@ -1,0 +1,362 @@
+// C ABI of libar_hip: context, scene upload, BVH build and the traversal
+// entry points.  Declarations and reference citations are in include/ar.h.
+#include "ar_internal.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <new>
+
+namespace ar {
+
+int set_error(ar_ctx* ctx, int code, const std::string& msg) {
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+int check_hip(ar_ctx* ctx, hipError_t e, const char* what) {
+    if (e == hipSuccess) return AR_OK;
+    return set_error(ctx, e == hipErrorOutOfMemory ? AR_E_OOM : AR_E_DEVICE,
+                     std::string(what) + ": " + hipGetErrorString(e));
+}
+
+namespace {
+
+void free_bvh(ar_ctx* ctx) {
+    if (ctx->bvh.nodes) (void)hipFree(ctx->bvh.nodes);
+    if (ctx->bvh.tris) (void)hipFree(ctx->bvh.tris);
+    ctx->bvh = DeviceBVH{};
+    ctx->bvh_ready = false;
+}
+
+struct Converter {
+    const BuildResult& br;
+    const std::vector<float>& verts;
+    const std::vector<uint32_t>& idx;
+    float pad;
+    std::vector<GpuNode2> nodes;
+    std::vector<float> tris;   // 12 floats per slot
+    uint64_t leaves = 0;
+
+    uint32_t leaf_ref(const BuildNode& n) {
+        const uint32_t first = (uint32_t)(tris.size() / 12);
+        for (uint32_t k = 0; k < n.count; ++k) {
+            const uint32_t prim = br.order[n.first + k];
+            for (int v = 0; v < 3; ++v) {
+                const float* p = &verts[3 * (size_t)idx[3 * (size_t)prim + v]];
+                tris.push_back(p[0]);
+                tris.push_back(p[1]);
+                tris.push_back(p[2]);
+                float w = 0.0f;
+                if (v == 0) std::memcpy(&w, &prim, 4);
+                tris.push_back(w);
+            }
+        }
+        ++leaves;
+        return make_leaf_ref(first, n.count);
+    }
+
+    void set_box(GpuNode2& g, int which, const BuildNode& c) {
+        const float lx = c.lo[0] - pad, hx = c.hi[0] + pad;
+        const float ly = c.lo[1] - pad, hy = c.hi[1] + pad;
+        const float lz = c.lo[2] - pad, hz = c.hi[2] + pad;
+        float* xy = which == 0 ? g.a : g.b;
+        xy[0] = lx; xy[1] = hx; xy[2] = ly; xy[3] = hy;
+        g.c[2 * which + 0] = lz;
+        g.c[2 * which + 1] = hz;
+    }
+
+    uint32_t child(int bi) {
+        const BuildNode& n = br.nodes[bi];
+        return n.left < 0 ? leaf_ref(n) : interior(bi);
+    }
+
+    uint32_t interior(int bi) {
+        const uint32_t gi = (uint32_t)nodes.size();
+        nodes.emplace_back();
+        const BuildNode& n = br.nodes[bi];
+        const uint32_t r0 = child(n.left);
+        const uint32_t r1 = child(n.right);
+        GpuNode2& g = nodes[gi];
+        set_box(g, 0, br.nodes[n.left]);
+        set_box(g, 1, br.nodes[n.right]);
+        g.d[0] = r0; g.d[1] = r1; g.d[2] = 0; g.d[3] = 0;
+        return gi;
+    }
+
+    void run() {
+        const BuildNode& root = br.nodes[0];
+        if (root.left < 0) {
+            // a single leaf: both children reference it (same answer, tested twice)
+            nodes.emplace_back();
+            const uint32_t r = leaf_ref(root);
+            GpuNode2& g = nodes[0];
+            set_box(g, 0, root);
+            set_box(g, 1, root);
+            g.d[0] = r; g.d[1] = r; g.d[2] = 0; g.d[3] = 0;
+        } else {
+            interior(0);
+        }
+    }
+};
+
+int ensure_spill(ar_ctx* ctx) {
+    if (ctx->spill) return AR_OK;
+    const int bpc = trace_blocks_per_cu();
+    const uint32_t blocks = (uint32_t)ctx->num_cus * (uint32_t)bpc;
+    ctx->trace_threads = blocks * kTraceBlock;
+    ctx->spill_entries = (size_t)(kStackMax - kLdsStack) * ctx->trace_threads;
+    int rc = check_hip(ctx, hipMalloc(&ctx->spill, ctx->spill_entries * sizeof(int)), "hipMalloc(stack spill)");
+    if (rc != AR_OK) { ctx->spill = nullptr; return rc; }
+    return AR_OK;
+}
+
+bool rays_ok(const ar_rays* r) {
+    return r && r->ox && r->oy && r->oz && r->tmin && r->dx && r->dy && r->dz && r->tmax;
+}
+
+bool hits_ok(const ar_hits* h) { return h && h->prim && h->t && h->u && h->v && h->flags; }
+
+int trace_prologue(ar_ctx* ctx, TraceArgs& ta, size_t n, uint32_t& blocks) {
+    if (!ctx->bvh_ready) return set_error(ctx, AR_E_NOSCENE, "no BVH: call ar_scene_set and ar_bvh_build first");
+    int rc = ensure_spill(ctx);
+    if (rc != AR_OK) return rc;
+    const uint32_t full_blocks = ctx->trace_threads / kTraceBlock;
+    const size_t need = (n + kTraceBlock - 1) / kTraceBlock;
+    blocks = (uint32_t)std::min<size_t>(full_blocks, std::max<size_t>(need, 1));
+    ta.nodes = ctx->bvh.nodes;
+    ta.tris = reinterpret_cast<const float4*>(ctx->bvh.tris);
+    ta.root = ctx->bvh.root;
+    ta.origin_limit = ctx->bvh.origin_limit;
+    ta.spill = ctx->spill;
+    ta.threads = blocks * kTraceBlock;
+    return AR_OK;
+}
+
+hipStream_t pick_stream(ar_ctx* ctx, void* stream) {
+    return stream ? reinterpret_cast<hipStream_t>(stream) : ctx->stream;
+}
+
+} // namespace
+} // namespace ar
+
+using namespace ar;
+
+extern "C" {
+
+int ar_abi_version(void) { return AR_ABI_VERSION; }
+
+int ar_create(ar_ctx** out, const ar_device_opts* opts) {
+    if (!out) return AR_E_INVALID;
+    *out = nullptr;
+    ar_ctx* ctx = new (std::nothrow) ar_ctx();
+    if (!ctx) return AR_E_OOM;
+    ctx->device = opts ? opts->device : 0;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        delete ctx;
+        return AR_E_DEVICE;
+    }
+    if (ctx->device < 0 || ctx->device >= ndev) {
+        delete ctx;
+        return AR_E_INVALID;
+    }
+    if (hipSetDevice(ctx->device) != hipSuccess ||
+        hipDeviceGetAttribute(&ctx->num_cus, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess ||
+        hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return AR_E_DEVICE;
+    }
+    *out = ctx;
+    return AR_OK;
+}
+
+int ar_destroy(ar_ctx* ctx) {
+    if (!ctx) return AR_E_INVALID;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    free_bvh(ctx);
+    if (ctx->spill) (void)hipFree(ctx->spill);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return AR_OK;
+}
+
+const char* ar_last_error(const ar_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int ar_scene_set(ar_ctx* ctx, const ar_mesh* meshes, size_t nmesh) {
+    if (!ctx) return AR_E_INVALID;
+    if (nmesh && !meshes) return set_error(ctx, AR_E_INVALID, "meshes is NULL");
+    std::vector<float> verts;
+    std::vector<uint32_t> idx, tri_mat;
+    std::vector<ar_material> mats;
+    uint64_t total = 0;
+    for (size_t m = 0; m < nmesh; ++m) total += meshes[m].ntri;
+    if (total > kMaxTriSlots) return set_error(ctx, AR_E_CAPACITY, "more than 2^27-1 triangles");
+    for (size_t m = 0; m < nmesh; ++m) {
+        const ar_mesh& me = meshes[m];
+        if (me.ntri && (!me.vertices || !me.indices))
+            return set_error(ctx, AR_E_INVALID, "mesh " + std::to_string(m) + ": NULL vertices or indices");
+        const uint32_t vbase = (uint32_t)(verts.size() / 3);
+        const uint32_t mbase = (uint32_t)mats.size();
+        verts.insert(verts.end(), me.vertices, me.vertices + 3 * (size_t)me.nv);
+        for (uint32_t k = 0; k < 3 * me.nv; ++k)
+            if (!std::isfinite(me.vertices[k]))
+                return set_error(ctx, AR_E_INVALID, "mesh " + std::to_string(m) + ": non-finite vertex");
+        for (uint32_t t = 0; t < me.ntri; ++t) {
+            for (int k = 0; k < 3; ++k) {
+                const uint32_t v = me.indices[3 * (size_t)t + k];
+                if (v >= me.nv)
+                    return set_error(ctx, AR_E_INVALID, "mesh " + std::to_string(m) + ": index out of range");
+                idx.push_back(vbase + v);
+            }
+            uint32_t mi = me.tri_material ? me.tri_material[t] : 0;
+            if (me.nmaterials && mi >= me.nmaterials)
+                return set_error(ctx, AR_E_INVALID, "mesh " + std::to_string(m) + ": material out of range");
+            tri_mat.push_back(me.nmaterials ? mbase + mi : 0);
+        }
+        if (me.nmaterials) {
+            if (!me.materials) return set_error(ctx, AR_E_INVALID, "materials is NULL");
+            mats.insert(mats.end(), me.materials, me.materials + me.nmaterials);
+        }
+    }
+    if (mats.empty()) {
+        ar_material def{};
+        def.albedo[0] = def.albedo[1] = def.albedo[2] = 0.75f;
+        def.roughness = 1.0f;
+        mats.push_back(def);
+    }
+    ctx->verts.swap(verts);
+    ctx->idx.swap(idx);
+    ctx->tri_mat.swap(tri_mat);
+    ctx->mats.swap(mats);
+    ctx->scene_set = true;
+    (void)hipSetDevice(ctx->device);
+    free_bvh(ctx);
+    return AR_OK;
+}
+
+int ar_bvh_build(ar_ctx* ctx, const ar_bvh_opts* opts) {
+    if (!ctx) return AR_E_INVALID;
+    if (!ctx->scene_set) return set_error(ctx, AR_E_NOSCENE, "no scene: call ar_scene_set first");
+    ar_bvh_opts o{2, 4, 16, (uint32_t)kMaxDepth};
+    if (opts) o = *opts;
+    if (o.width != 2) return set_error(ctx, AR_E_UNSUPPORTED, "only BVH width 2 is implemented");
+    if (o.leaf_size < 1 || o.leaf_size > kMaxLeafTris) return set_error(ctx, AR_E_INVALID, "leaf_size must be 1..15");
+    if (o.bin_count < 4 || o.bin_count > 64) return set_error(ctx, AR_E_INVALID, "bin_count must be 4..64");
+    if (o.max_depth == 0 || o.max_depth > (uint32_t)kMaxDepth) o.max_depth = kMaxDepth;
+    (void)hipSetDevice(ctx->device);
+    free_bvh(ctx);
+    const auto t0 = std::chrono::steady_clock::now();
+    const uint32_t n = (uint32_t)(ctx->idx.size() / 3);
+    ctx->info = ar_bvh_info{};
+    ctx->info.width = 2;
+    if (n == 0) {
+        ctx->bvh.root = kRefEmpty;
+        ctx->bvh_ready = true;
+        return AR_OK;
+    }
+    std::vector<float> lo(3 * (size_t)n), hi(3 * (size_t)n), cen(3 * (size_t)n);
+    float mag = 0.0f;
+    for (uint32_t t = 0; t < n; ++t) {
+        for (int a = 0; a < 3; ++a) {
+            const float x = ctx->verts[3 * (size_t)ctx->idx[3 * (size_t)t + 0] + a];
+            const float y = ctx->verts[3 * (size_t)ctx->idx[3 * (size_t)t + 1] + a];
+            const float z = ctx->verts[3 * (size_t)ctx->idx[3 * (size_t)t + 2] + a];
+            const float l = std::min(x, std::min(y, z)), h = std::max(x, std::max(y, z));
+            lo[3 * (size_t)t + a] = l;
+            hi[3 * (size_t)t + a] = h;
+            cen[3 * (size_t)t + a] = 0.5f * (l + h);
+            mag = std::max(mag, std::max(std::fabs(l), std::fabs(h)));
+        }
+    }
+    BuildResult br = build_bvh2(lo.data(), hi.data(), cen.data(), n, o.leaf_size, o.bin_count, (int)o.max_depth);
+    if (br.depth > kMaxDepth)
+        return set_error(ctx, AR_E_CAPACITY, "BVH deeper than the traversal stack allows");
+    Converter cv{br, ctx->verts, ctx->idx, mag * kPadRel + 1e-30f, {}, {}, 0};
+    cv.nodes.reserve(br.nodes.size() / 2 + 1);
+    cv.tris.reserve(12 * (size_t)n);
+    cv.run();
+    int rc = check_hip(ctx, hipMalloc(&ctx->bvh.nodes, cv.nodes.size() * sizeof(GpuNode2)), "hipMalloc(nodes)");
+    if (rc != AR_OK) { ctx->bvh.nodes = nullptr; return rc; }
+    rc = check_hip(ctx, hipMalloc(&ctx->bvh.tris, cv.tris.size() * sizeof(float)), "hipMalloc(tris)");
+    if (rc != AR_OK) { ctx->bvh.tris = nullptr; free_bvh(ctx); return rc; }
+    rc = check_hip(ctx, hipMemcpy(ctx->bvh.nodes, cv.nodes.data(), cv.nodes.size() * sizeof(GpuNode2),
+                                  hipMemcpyHostToDevice), "upload nodes");
+    if (rc == AR_OK)
+        rc = check_hip(ctx, hipMemcpy(ctx->bvh.tris, cv.tris.data(), cv.tris.size() * sizeof(float),
+                                      hipMemcpyHostToDevice), "upload triangles");
+    if (rc != AR_OK) { free_bvh(ctx); return rc; }
+    ctx->bvh.node_count = (uint32_t)cv.nodes.size();
+    ctx->bvh.tri_slots = (uint32_t)(cv.tris.size() / 12);
+    ctx->bvh.root = 0;
+    ctx->bvh.origin_limit = kOriginLimit * mag;
+    ctx->bvh_ready = true;
+    const auto t1 = std::chrono::steady_clock::now();
+    ctx->info.node_count = cv.nodes.size();
+    ctx->info.leaf_count = cv.leaves;
+    ctx->info.depth = (uint32_t)br.depth;
+    ctx->info.device_bytes = cv.nodes.size() * sizeof(GpuNode2) + cv.tris.size() * sizeof(float);
+    ctx->info.build_seconds = std::chrono::duration<double>(t1 - t0).count();
+    ctx->info.sah_cost = br.sah_cost;
+    return AR_OK;
+}
+
+int ar_bvh_info_get(const ar_ctx* ctx, ar_bvh_info* info) {
+    if (!ctx || !info) return AR_E_INVALID;
+    if (!ctx->bvh_ready) return AR_E_NOSCENE;
+    *info = ctx->info;
+    return AR_OK;
+}
+
+int ar_trace_closest(ar_ctx* ctx, const ar_rays* rays, size_t n, const ar_hits* hits, void* stream) {
+    if (!ctx) return AR_E_INVALID;
+    if (n == 0) return AR_OK;
+    if (!rays_ok(rays) || !hits_ok(hits)) return set_error(ctx, AR_E_INVALID, "NULL ray or hit array");
+    TraceArgs ta;
+    uint32_t blocks;
+    int rc = trace_prologue(ctx, ta, n, blocks);
+    if (rc != AR_OK) return rc;
+    return check_hip(ctx, launch_trace_closest(ta, *rays, n, *hits, nullptr, nullptr, blocks, pick_stream(ctx, stream)),
+                     "trace_closest launch");
+}
+
+int ar_trace_closest_counted(ar_ctx* ctx, const ar_rays* rays, size_t n, const ar_hits* hits,
+                             uint32_t* box_tests, uint32_t* tri_tests, void* stream) {
+    if (!ctx) return AR_E_INVALID;
+    if (n == 0) return AR_OK;
+    if (!rays_ok(rays) || !hits_ok(hits) || !box_tests || !tri_tests)
+        return set_error(ctx, AR_E_INVALID, "NULL ray, hit or counter array");
+    TraceArgs ta;
+    uint32_t blocks;
+    int rc = trace_prologue(ctx, ta, n, blocks);
+    if (rc != AR_OK) return rc;
+    return check_hip(ctx, launch_trace_closest(ta, *rays, n, *hits, box_tests, tri_tests, blocks,
+                                               pick_stream(ctx, stream)), "trace_closest_counted launch");
+}
+
+int ar_trace_any(ar_ctx* ctx, const ar_rays* rays, size_t n, uint32_t* occluded, void* stream) {
+    if (!ctx) return AR_E_INVALID;
+    if (n == 0) return AR_OK;
+    if (!rays_ok(rays) || !occluded) return set_error(ctx, AR_E_INVALID, "NULL ray or occlusion array");
+    TraceArgs ta;
+    uint32_t blocks;
+    int rc = trace_prologue(ctx, ta, n, blocks);
+    if (rc != AR_OK) return rc;
+    return check_hip(ctx, launch_trace_any(ta, *rays, n, occluded, blocks, pick_stream(ctx, stream)),
+                     "trace_any launch");
+}
+
+int ar_make_shadow_rays(ar_ctx* ctx, const ar_rays* rays, const ar_hits* hits, size_t n, const float target[3],
+                        float tmin, float tmax_scale, const ar_rays* out, void* stream) {
+    if (!ctx) return AR_E_INVALID;
+    if (n == 0) return AR_OK;
+    if (!rays_ok(rays) || !hits_ok(hits) || !rays_ok(out) || !target)
+        return set_error(ctx, AR_E_INVALID, "NULL argument");
+    return check_hip(ctx, launch_shadow_rays(*rays, *hits, n, target[0], target[1], target[2], tmin, tmax_scale, *out,
+                                             pick_stream(ctx, stream)), "shadow ray launch");
+}
+
+} // extern "C"

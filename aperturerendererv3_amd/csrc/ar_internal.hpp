@@ -1,0 +1,131 @@
+// Internal types of libar_hip: the context, the device BVH layout and the
+// host builder's interface.  Not part of the ABI (include/ar.h is).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/ar.h"
+
+namespace ar {
+
+// ---------------------------------------------------------------------------
+// Device BVH2 layout ("both child boxes in the parent", one 64-byte node).
+//
+//   float4 a = (c0.lo.x, c0.hi.x, c0.lo.y, c0.hi.y)
+//   float4 b = (c1.lo.x, c1.hi.x, c1.lo.y, c1.hi.y)
+//   float4 c = (c0.lo.z, c0.hi.z, c1.lo.z, c1.hi.z)
+//   uint4  d = (ref0, ref1, 0, 0)
+//
+// A child reference with bit 31 clear is an interior node index.  With bit
+// 31 set it is a leaf: bits 0..26 = first triangle slot, bits 27..30 =
+// triangle count - 1.  kRefEmpty (a leaf of count 16 at the last slot, never
+// produced by the builder) marks an empty child and the stack sentinel.
+//
+// Leaf triangles are stored in leaf order, three float4 per slot:
+//   (v0.xyz, prim id bits), (v1.xyz, 0), (v2.xyz, 0).
+// Child boxes are padded outward by kPadRel * (largest |coordinate| of the
+// scene) so that the box test never culls a triangle the watertight test
+// would report (rounding of either test stays far below the pad for ray
+// origins within kOriginLimit scene magnitudes; beyond that the kernel adds a
+// per-ray pad in t).
+// ---------------------------------------------------------------------------
+constexpr uint32_t kLeafBit = 0x80000000u;
+constexpr uint32_t kRefEmpty = 0xFFFFFFFFu;
+constexpr uint32_t kMaxLeafTris = 15;            // count-1 must fit 4 bits and differ from kRefEmpty
+constexpr uint32_t kMaxTriSlots = (1u << 27) - 1;
+constexpr float kPadRel = 1.0f / 65536.0f;       // 2^-16
+constexpr float kOriginLimit = 8.0f;
+constexpr int kMaxDepth = 60;                    // tree depth bound (stack <= depth + 1)
+constexpr int kStackMax = 64;                    // per-ray stack entries incl. sentinel
+constexpr int kLdsStack = 16;                    // entries held in LDS, the rest spill to HBM
+constexpr int kTraceBlock = 256;
+
+inline uint32_t make_leaf_ref(uint32_t first, uint32_t count) {
+    return kLeafBit | ((count - 1u) << 27) | first;
+}
+
+struct GpuNode2 {
+    float a[4];
+    float b[4];
+    float c[4];
+    uint32_t d[4];
+};
+static_assert(sizeof(GpuNode2) == 64, "GpuNode2 must be 64 bytes");
+
+// Host-side binary tree produced by the builder (pre-order, left = parent+1).
+struct BuildNode {
+    float lo[3], hi[3];
+    int32_t left, right;   // -1 for leaves
+    uint32_t first, count; // leaf range in the primitive order
+};
+
+struct BuildResult {
+    std::vector<BuildNode> nodes;
+    std::vector<uint32_t> order;  // primitive ids in leaf order
+    int depth = 0;
+    double sah_cost = 0.0;
+};
+
+// Binned-SAH builder (bvh_build.cpp).  lo/hi/cen: n x 3 floats.
+BuildResult build_bvh2(const float* lo, const float* hi, const float* cen, uint32_t n,
+                       uint32_t leaf_size, uint32_t bins, int max_depth);
+
+struct DeviceBVH {
+    GpuNode2* nodes = nullptr;   // device
+    float* tris = nullptr;       // device, 12 floats per slot
+    uint32_t node_count = 0;
+    uint32_t tri_slots = 0;
+    uint32_t root = kRefEmpty;   // root interior node index (0) or kRefEmpty
+    float origin_limit = 0.0f;   // kOriginLimit * scene magnitude
+};
+
+} // namespace ar
+
+struct ar_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    int num_cus = 0;
+
+    // scene, host copy
+    std::vector<float> verts;         // 3 per vertex
+    std::vector<uint32_t> idx;        // 3 per triangle (global vertex ids)
+    std::vector<uint32_t> tri_mat;    // per triangle
+    std::vector<ar_material> mats;
+    bool scene_set = false;
+
+    ar::DeviceBVH bvh;
+    bool bvh_ready = false;
+    ar_bvh_info info{};
+
+    // traversal scratch: per-thread stack spill, sized for the persistent grid
+    int* spill = nullptr;
+    size_t spill_entries = 0;
+    uint32_t trace_threads = 0;       // persistent grid size in threads
+};
+
+namespace ar {
+int set_error(ar_ctx* ctx, int code, const std::string& msg);
+int check_hip(ar_ctx* ctx, hipError_t e, const char* what);
+
+// trace.hip launchers
+struct TraceArgs {
+    const GpuNode2* nodes;
+    const float4* tris;
+    uint32_t root;
+    float origin_limit;
+    int* spill;
+    uint32_t threads;
+};
+hipError_t launch_trace_closest(const TraceArgs& ta, const ar_rays& r, size_t n, const ar_hits& h,
+                                uint32_t* cnt_box, uint32_t* cnt_tri, uint32_t blocks, hipStream_t s);
+hipError_t launch_trace_any(const TraceArgs& ta, const ar_rays& r, size_t n, uint32_t* occ,
+                            uint32_t blocks, hipStream_t s);
+hipError_t launch_shadow_rays(const ar_rays& r, const ar_hits& h, size_t n, float tx, float ty, float tz,
+                              float tmin, float tmax_scale, const ar_rays& out, hipStream_t s);
+int trace_blocks_per_cu();
+} // namespace ar

@@ -1,0 +1,159 @@
+"""Deterministic synthetic workloads (numpy, seeded).
+
+traversal_mesh / traversal_rays build the traversal microbenchmark: a
+tetrahedron subdivided `levels` times (4 * 4**levels triangles; 9 levels =
+1,048,576), projected to the unit sphere and displaced radially by
+1 + amp * n(v), n = seeded 3-D value noise in [-1, 1] at frequency 8; and a
+square grid of rays where slots with x+y even are pinhole primary rays from
+(0, 0, 3) toward the origin (vertical fov 60 degrees, pixel centres) and slots
+with x+y odd are incoherent rays: a uniform point on an area-weighted random
+triangle, offset 1e-4 along the inward geometric normal, with a
+cosine-weighted direction about that inward normal (so the ray crosses the
+sphere's interior and traverses the whole tree).
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+_TETRA_V = np.array([[1, 1, 1], [1, -1, -1], [-1, 1, -1], [-1, -1, 1]], dtype=np.float64)
+_TETRA_F = np.array([[0, 1, 2], [0, 3, 1], [0, 2, 3], [1, 3, 2]], dtype=np.int64)
+
+
+def _hash_u32(x: np.ndarray) -> np.ndarray:
+    """32-bit avalanche hash (lowbias32 by C. Wellons), vectorised."""
+    x = x.astype(np.uint32)
+    x ^= x >> np.uint32(16)
+    x *= np.uint32(0x7FEB352D)
+    x ^= x >> np.uint32(15)
+    x *= np.uint32(0x846CA68B)
+    x ^= x >> np.uint32(16)
+    return x
+
+
+def _lattice(ix, iy, iz, seed):
+    h = _hash_u32(ix.astype(np.uint32) * np.uint32(73856093) ^ iy.astype(np.uint32) * np.uint32(19349663)
+                  ^ iz.astype(np.uint32) * np.uint32(83492791) ^ np.uint32(seed * 0x9E3779B9 & 0xFFFFFFFF))
+    return (h >> np.uint32(8)).astype(np.float64) * (2.0 / (1 << 24)) - 1.0
+
+
+def value_noise(p: np.ndarray, seed: int) -> np.ndarray:
+    """Trilinear value noise with smoothstep fade; p (n,3) in lattice units."""
+    f = np.floor(p)
+    i = f.astype(np.int64)
+    t = p - f
+    t = t * t * (3.0 - 2.0 * t)
+    out = np.zeros(p.shape[0])
+    for dz in (0, 1):
+        for dy in (0, 1):
+            for dx in (0, 1):
+                w = ((t[:, 0] if dx else 1 - t[:, 0]) * (t[:, 1] if dy else 1 - t[:, 1])
+                     * (t[:, 2] if dz else 1 - t[:, 2]))
+                out += w * _lattice(i[:, 0] + dx, i[:, 1] + dy, i[:, 2] + dz, seed)
+    return out
+
+
+def subdivided_tetra(levels: int):
+    """Unit-sphere geodesic tetrahedron: (V (nv,3) f64, F (nf,3) i64), outward winding."""
+    V = _TETRA_V / np.sqrt(3.0)
+    F = _TETRA_F.copy()
+    for _ in range(levels):
+        e = np.concatenate([F[:, [0, 1]], F[:, [1, 2]], F[:, [2, 0]]])
+        lo, hi = np.minimum(e[:, 0], e[:, 1]), np.maximum(e[:, 0], e[:, 1])
+        key = lo * (V.shape[0] + 1) + hi
+        uk, inv = np.unique(key, return_inverse=True)
+        a, b = uk // (V.shape[0] + 1), uk % (V.shape[0] + 1)
+        mid = V[a] + V[b]
+        mid /= np.linalg.norm(mid, axis=1, keepdims=True)
+        base = V.shape[0]
+        V = np.concatenate([V, mid])
+        m = F.shape[0]
+        ab, bc, ca = base + inv[:m], base + inv[m:2 * m], base + inv[2 * m:]
+        F = np.concatenate([
+            np.stack([F[:, 0], ab, ca], 1), np.stack([F[:, 1], bc, ab], 1),
+            np.stack([F[:, 2], ca, bc], 1), np.stack([ab, bc, ca], 1)])
+    return V, F
+
+
+def traversal_mesh(levels: int = 9, amp: float = 0.05, freq: float = 8.0, seed: int = 1):
+    """Displaced subdivided tetrahedron: vertices (nv,3) f32, indices (nf,3) u32."""
+    V, F = subdivided_tetra(levels)
+    r = 1.0 + amp * value_noise(V * freq, seed)
+    Vd = (V * r[:, None]).astype(np.float32)
+    return Vd, F.astype(np.uint32)
+
+
+def camera_dirs(width, height, fwd, vfov_deg=60.0, jitter=None):
+    """Pinhole directions with the reference camera convention
+    (core/scene.cu:145-164): right = norm(fwd x +Y), up = norm(fwd x right),
+    ndc.y = (1-v)*2-1.  jitter: (h,w,2) offsets in [0,1) or None (centres)."""
+    fwd = np.asarray(fwd, dtype=np.float64)
+    fwd = fwd / np.linalg.norm(fwd)
+    right = np.cross(fwd, [0.0, 1.0, 0.0])
+    right /= np.linalg.norm(right)
+    up = np.cross(fwd, right)
+    up /= np.linalg.norm(up)
+    jj, ii = np.meshgrid(np.arange(height), np.arange(width), indexing="ij")
+    ox = 0.5 if jitter is None else jitter[..., 0]
+    oy = 0.5 if jitter is None else jitter[..., 1]
+    u = (ii + ox) / width
+    v = (jj + oy) / height
+    th = math.tan(math.radians(vfov_deg) * 0.5)
+    sx = (u * 2 - 1) * th * (width / max(1.0, height))
+    sy = ((1 - v) * 2 - 1) * th
+    d = fwd[None, None, :] + right[None, None, :] * sx[..., None] + up[None, None, :] * sy[..., None]
+    d /= np.linalg.norm(d, axis=-1, keepdims=True)
+    return d
+
+
+def _onb(n):
+    """Branchless orthonormal basis (Duff et al., JCGT 6(1) 2017)."""
+    s = np.where(n[:, 2] >= 0, 1.0, -1.0)
+    a = -1.0 / (s + n[:, 2])
+    b = n[:, 0] * n[:, 1] * a
+    t = np.stack([1 + s * n[:, 0] ** 2 * a, s * b, -s * n[:, 0]], 1)
+    bt = np.stack([b, s + n[:, 1] ** 2 * a, -n[:, 1]], 1)
+    return t, bt
+
+
+def traversal_rays(V, F, res: int = 4096, seed: int = 1, cam_pos=(0.0, 0.0, 3.0)):
+    """Ray slots of a res x res grid as an (8, res*res) float32 array
+    (rows ox, oy, oz, tmin, dx, dy, dz, tmax)."""
+    n = res * res
+    rng = np.random.Generator(np.random.PCG64(seed))
+    rays = np.empty((8, n), dtype=np.float32)
+    yy, xx = np.divmod(np.arange(n), res)
+    primary = ((xx + yy) & 1) == 0
+    cam = np.asarray(cam_pos, dtype=np.float64)
+    d = camera_dirs(res, res, -cam).reshape(n, 3)
+    pi = np.nonzero(primary)[0]
+    rays[0:3, pi] = cam[:, None]
+    rays[4:7, pi] = d[pi].T
+    qi = np.nonzero(~primary)[0]
+    m = qi.size
+    a = V[F[:, 0]].astype(np.float64)
+    e1 = V[F[:, 1]].astype(np.float64) - a
+    e2 = V[F[:, 2]].astype(np.float64) - a
+    cr = np.cross(e1, e2)
+    area = np.linalg.norm(cr, axis=1)
+    cdf = np.cumsum(area)
+    cdf /= cdf[-1]
+    tri = np.minimum(np.searchsorted(cdf, rng.random(m), side="right"), F.shape[0] - 1)
+    r1, r2 = rng.random(m), rng.random(m)
+    s1 = np.sqrt(r1)
+    p = a[tri] + e1[tri] * (s1 * (1 - r2))[:, None] + e2[tri] * (s1 * r2)[:, None]
+    nin = -cr[tri] / area[tri, None]
+    p = p + 1e-4 * nin
+    u1, u2 = rng.random(m), rng.random(m)
+    rr = np.sqrt(u1)
+    phi = 2 * math.pi * u2
+    lx, ly, lz = rr * np.cos(phi), rr * np.sin(phi), np.sqrt(np.maximum(0.0, 1 - u1))
+    t, b = _onb(nin)
+    dd = t * lx[:, None] + b * ly[:, None] + nin * lz[:, None]
+    dd /= np.linalg.norm(dd, axis=1, keepdims=True)
+    rays[0:3, qi] = p.T
+    rays[4:7, qi] = dd.T
+    rays[3] = 0.0
+    rays[7] = np.inf
+    return rays

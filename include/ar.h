@@ -1,0 +1,168 @@
+/*
+ * ar.h — C ABI of the Aperture MI355X path-tracing core (libar_hip.so).
+ *
+ * Plain C types only: pointers, sizes, PODs.  No HIP, torch or C++ types
+ * cross this boundary.  Every entry point returns an int status
+ * (AR_OK = 0, negative AR_E_* on failure); ar_last_error() gives the text.
+ * No exception crosses the ABI.
+ *
+ * Each entry point cites the reference interface it replaces (paths are
+ * relative to the reference checkout of ApertureRendererV3).
+ *
+ * Threading: one ar_ctx per host thread (thread-compatible, not
+ * thread-safe).  Device work is queued on the stream passed in (a
+ * hipStream_t, NULL = the context's own stream) and is not waited for
+ * unless the entry point says so.
+ */
+#ifndef AR_H
+#define AR_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AR_ABI_VERSION 1
+
+#define AR_OK 0
+#define AR_E_INVALID (-1)     /* bad argument */
+#define AR_E_DEVICE (-2)      /* HIP runtime error */
+#define AR_E_NOSCENE (-3)     /* no scene / acceleration structure yet */
+#define AR_E_OOM (-4)         /* device or host allocation failed */
+#define AR_E_UNSUPPORTED (-5) /* option not supported */
+#define AR_E_CAPACITY (-6)    /* a fixed capacity would be exceeded */
+
+#define AR_MISS 0xFFFFFFFFu   /* ar_hits.prim of a ray that hit nothing */
+
+/* Hit flags (ar_hits.flags): barycentric weight of v0 / v1 / v2 exactly zero
+ * (the ray passed through an edge or a vertex), and det < 0 (the triangle's
+ * vertices wind clockwise as seen along the ray). */
+#define AR_HIT_W0_ZERO 1u
+#define AR_HIT_W1_ZERO 2u
+#define AR_HIT_W2_ZERO 4u
+#define AR_HIT_BACKFACE 8u
+
+typedef struct ar_ctx ar_ctx;
+
+typedef struct {
+    int device;        /* HIP device ordinal */
+    uint32_t flags;    /* reserved, 0 */
+} ar_device_opts;
+
+/* Material of a triangle (the triangle-mesh counterpart of the reference's
+ * GBuffer / Material, core/render/object.cuh:36-150). */
+typedef struct {
+    float albedo[3];
+    float emission[3];   /* radiance; > 0 makes the triangle a light */
+    float specular;      /* UE-style specular (F0 = 0.08*specular for dielectrics) */
+    float metallic;
+    float roughness;     /* GGX roughness, clamped to [0.01, 1] */
+    uint32_t model;      /* AR_MODEL_* */
+} ar_material;
+
+#define AR_MODEL_LAMBERT 0u
+#define AR_MODEL_GGX 1u      /* GGX + Lambert mix (core/render/material.cuh:47-165) */
+
+/* A triangle mesh in host memory.  vertices: nv x 3 floats; indices:
+ * ntri x 3 u32; tri_material: ntri u32 indices into materials (NULL: all
+ * triangles use materials[0]; materials may be NULL when nmaterials = 0). */
+typedef struct {
+    const float* vertices;
+    uint32_t nv;
+    const uint32_t* indices;
+    uint32_t ntri;
+    const uint32_t* tri_material;
+    const ar_material* materials;
+    uint32_t nmaterials;
+} ar_mesh;
+
+/* Rays and hits in structure-of-arrays form, device pointers, one element
+ * per ray.  A ray is valid for t in [tmin, tmax). */
+typedef struct {
+    const float *ox, *oy, *oz, *tmin, *dx, *dy, *dz, *tmax;
+} ar_rays;
+
+/* Closest-hit output.  prim = AR_MISS on a miss (then t = +inf, u = v = 0,
+ * flags = 0).  Otherwise prim is the scene triangle index (meshes are
+ * numbered in the order given to ar_scene_set, triangles consecutively),
+ * hit point = (1-u-v)*v0 + u*v1 + v*v2.  The hit is the one with the
+ * smallest t; among equal t, the smallest prim. */
+typedef struct {
+    uint32_t* prim;
+    float* t;
+    float* u;
+    float* v;
+    uint32_t* flags;
+} ar_hits;
+
+typedef struct {
+    uint32_t width;      /* branching factor; 2 supported */
+    uint32_t leaf_size;  /* max triangles per leaf, 1..15 */
+    uint32_t bin_count;  /* SAH bins, 4..64 */
+    uint32_t max_depth;  /* tree depth limit, <= 64 */
+} ar_bvh_opts;
+
+typedef struct {
+    uint64_t node_count;
+    uint64_t leaf_count;
+    uint32_t depth;
+    uint32_t width;
+    uint64_t device_bytes;   /* nodes + leaf triangles */
+    double build_seconds;    /* host build + upload */
+    double sah_cost;
+} ar_bvh_info;
+
+/* --- context --------------------------------------------------------------
+ * Replaces SceneRenderer's constructor / destructor (core/scene.hpp:94-96,
+ * core/scene.cu:609-654, 745-750). */
+int ar_create(ar_ctx** out, const ar_device_opts* opts);
+int ar_destroy(ar_ctx* ctx);
+const char* ar_last_error(const ar_ctx* ctx);
+int ar_abi_version(void);
+
+/* --- scene ----------------------------------------------------------------
+ * Replaces SceneRenderer(const std::vector<SceneObject>&) + MarkAsDirty
+ * (core/scene.hpp:94-101) and the upload in UpdateSceneToDeviceIfDirty
+ * (core/scene.cu:656-732).  Copies the meshes; the caller keeps ownership of
+ * its arrays.  Invalidates the BVH. */
+int ar_scene_set(ar_ctx* ctx, const ar_mesh* meshes, size_t nmesh);
+
+/* --- acceleration structure ----------------------------------------------
+ * Replaces BuildBVH (core/bvh/bvh.cuh:142-422) as called from
+ * UpdateSceneToDeviceIfDirty (core/scene.cu:669-675).  opts may be NULL
+ * (defaults).  Synchronous. */
+int ar_bvh_build(ar_ctx* ctx, const ar_bvh_opts* opts);
+int ar_bvh_info_get(const ar_ctx* ctx, ar_bvh_info* info);
+
+/* --- traversal ------------------------------------------------------------
+ * Replace TraverseBVH<false> (closest hit) and TraverseBVH<true> /
+ * AnyHitSegmentBVH (any hit) of core/render/intersection.cuh:148-310.
+ * All pointers are device pointers; no allocation happens in these calls
+ * once the context has traced once. */
+int ar_trace_closest(ar_ctx* ctx, const ar_rays* rays, size_t n, const ar_hits* hits, void* stream);
+
+/* occluded: ceil(n/32) u32 words; bit (r & 31) of word r/32 is set iff some
+ * triangle is hit with tmin <= t < tmax. */
+int ar_trace_any(ar_ctx* ctx, const ar_rays* rays, size_t n, uint32_t* occluded, void* stream);
+
+/* Closest hit that also counts, per ray, the box tests and triangle tests
+ * the traversal made (the ops model of the traversal roofline).  Slower
+ * than ar_trace_closest; for measurement only.  box_tests/tri_tests: n u32. */
+int ar_trace_closest_counted(ar_ctx* ctx, const ar_rays* rays, size_t n, const ar_hits* hits,
+                             uint32_t* box_tests, uint32_t* tri_tests, void* stream);
+
+/* Shadow segments toward a point light for every hit of a closest-hit pass
+ * (the traversal benchmark's any-hit workload): for ray r with a hit, the
+ * segment starts at o + t*d, points at `target` and ends at
+ * dist*(1 - tmax_scale); misses get an empty segment (tmax = 0).  Output
+ * rays are device SoA arrays of n entries. */
+int ar_make_shadow_rays(ar_ctx* ctx, const ar_rays* rays, const ar_hits* hits, size_t n,
+                        const float target[3], float tmin, float tmax_scale,
+                        const ar_rays* out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* AR_H */

@@ -1,0 +1,105 @@
+/*
+ * CPU oracle for the Aperture path-tracing core.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the shipped library links, loads or
+ * calls this code: only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg use it, and there only as the checker / CPU baseline.
+ *
+ * It restates, in plain C99, the algorithms of the reference renderer that
+ * sit on the hot path (citations are relative to the reference checkout):
+ *
+ *   orc_bvh_build        core/bvh/bvh.cuh:142-422   binned-SAH BVH2 builder
+ *   box_hit (static)     core/bvh/bvh.cuh:424-470   FastAABBHit(WithEnter)
+ *   orc_trace_*          core/render/intersection.cuh:148-310 TraverseBVH
+ *   orc_tri_test         (new: the reference has no triangles; the
+ *                        watertight test of Woop, Benthin & Wald,
+ *                        JCGT 2(1) 2013, is restated from the paper)
+ *
+ * Parity pins (what checks each function, see tests/):
+ *   orc_tri_test         closed-form FP64 ray/plane hits, known-answer rays,
+ *                        watertightness on closed meshes (tests/test_oracle_trace.py)
+ *   orc_bvh_build        structural invariants (each primitive in exactly one
+ *                        leaf, bounds containment, pre-order left=parent+1,
+ *                        leaf size, SAH accept rule)
+ *   orc_trace_closest    equality with orc_trace_brute on every ray
+ *   orc_trace_any        equality with (closest hit t < tmax)
+ *
+ * Build: oracle/Makefile (gcc -O2 -ffp-contract=off -fopenmp, no fast-math).
+ */
+#ifndef AR_ORACLE_H
+#define AR_ORACLE_H
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_MISS 0xFFFFFFFFu
+
+/* 32-bit reference node layout (core/bvh/bvh.cuh:36-43): leaf iff count > 0,
+ * pre-order, left child = parent + 1. */
+typedef struct {
+    float bmin[3];
+    float bmax[3];
+    int32_t left, right, first, count;
+} orc_node;
+
+/* Binned-SAH build over primitive proxies (bvh.cuh:142-422).  Arrays are
+ * n x 3 floats.  nodes_out must hold 2n entries, indices_out n entries.
+ * Returns the node count (0 for an empty input). */
+int orc_bvh_build(const float* pmin, const float* pmax, const float* pcen,
+                  int n, int leaf_size, int bin_count,
+                  orc_node* nodes_out, int32_t* indices_out);
+
+/* Proxies of a triangle mesh: box of the three vertices and its centre. */
+void orc_tri_proxies(const float* verts, const uint32_t* idx, int ntri,
+                     float* pmin, float* pmax, float* pcen);
+
+/* Watertight ray/triangle test.  Returns 1 when the ray's line crosses the
+ * triangle (any t; callers apply their own [tmin, tmax) rule) and writes
+ * t, the barycentric weights of v1 and v2, and flags: bit0/1/2 = weight of
+ * v0/v1/v2 exactly zero (edge or vertex hit), bit3 = det < 0. */
+int orc_tri_test(const float org[3], const float dir[3],
+                 const float v0[3], const float v1[3], const float v2[3],
+                 float* t, float* u, float* v, uint32_t* flags);
+
+/* Ray batch in structure-of-arrays form (same as the library's ar_rays). */
+typedef struct {
+    const float *ox, *oy, *oz, *tmin, *dx, *dy, *dz, *tmax;
+} orc_rays;
+
+typedef struct {
+    uint32_t* prim; float* t; float* u; float* v; uint32_t* flags;
+} orc_hits;
+
+/* mode 0 = canonical: padded (conservative) box test and the rule
+ *          "smallest t, then smallest primitive id" (the brute-force answer);
+ * mode 1 = reference-faithful: FastAABBHit as written and the first hit
+ *          found in traversal order wins (strict <).
+ * boxes/tris: optional per-ray counters of box and triangle tests. */
+void orc_trace_closest(const orc_node* nodes, int node_count, const int32_t* indices,
+                       const float* verts, const uint32_t* idx,
+                       orc_rays rays, size_t n, orc_hits hits, int mode,
+                       uint32_t* boxes, uint32_t* tris, int threads);
+
+/* occluded bit r set iff some triangle has tmin <= t < tmax. */
+void orc_trace_any(const orc_node* nodes, int node_count, const int32_t* indices,
+                   const float* verts, const uint32_t* idx,
+                   orc_rays rays, size_t n, uint32_t* occluded_bits, int mode,
+                   uint32_t* boxes, uint32_t* tris, int threads);
+
+/* O(N*M) closest hit over all triangles, canonical rule. */
+void orc_trace_brute(const float* verts, const uint32_t* idx, int ntri,
+                     orc_rays rays, size_t n, orc_hits hits, int threads);
+
+void orc_trace_any_brute(const float* verts, const uint32_t* idx, int ntri,
+                         orc_rays rays, size_t n, uint32_t* occluded_bits, int threads);
+
+/* Maximum depth (root = 0) of a built tree. */
+int orc_bvh_depth(const orc_node* nodes, int node_count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

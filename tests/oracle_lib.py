@@ -1,0 +1,167 @@
+"""ctypes wrapper of the CPU oracle (oracle/liboracle.so) for the tests.
+
+Test infrastructure only: the shipped library never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent.parent
+ORACLE_DIR = ROOT / "oracle"
+LIB = ORACLE_DIR / "liboracle.so"
+MISS = 0xFFFFFFFF
+
+
+class orc_node(C.Structure):
+    _fields_ = [("bmin", C.c_float * 3), ("bmax", C.c_float * 3), ("left", C.c_int32), ("right", C.c_int32),
+                ("first", C.c_int32), ("count", C.c_int32)]
+
+
+class orc_rays(C.Structure):
+    _fields_ = [(k, C.c_void_p) for k in ("ox", "oy", "oz", "tmin", "dx", "dy", "dz", "tmax")]
+
+
+class orc_hits(C.Structure):
+    _fields_ = [(k, C.c_void_p) for k in ("prim", "t", "u", "v", "flags")]
+
+
+NODE_DTYPE = np.dtype([("bmin", "<f4", 3), ("bmax", "<f4", 3), ("left", "<i4"), ("right", "<i4"),
+                       ("first", "<i4"), ("count", "<i4")])
+
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    srcs = [ORACLE_DIR / "trace.c", ORACLE_DIR / "oracle.h"]
+    if not LIB.exists() or any(s.stat().st_mtime > LIB.stat().st_mtime for s in srcs if s.exists()):
+        subprocess.run(["make", "-s", "-C", str(ORACLE_DIR)], check=True)
+    lib = C.CDLL(str(LIB))
+    vp = C.c_void_p
+    lib.orc_bvh_build.restype = C.c_int
+    lib.orc_bvh_build.argtypes = [vp, vp, vp, C.c_int, C.c_int, C.c_int, vp, vp]
+    lib.orc_tri_proxies.restype = None
+    lib.orc_tri_proxies.argtypes = [vp, vp, C.c_int, vp, vp, vp]
+    lib.orc_tri_test.restype = C.c_int
+    lib.orc_tri_test.argtypes = [vp] * 5 + [C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                          C.POINTER(C.c_uint32)]
+    lib.orc_trace_closest.restype = None
+    lib.orc_trace_closest.argtypes = [vp, C.c_int, vp, vp, vp, orc_rays, C.c_size_t, orc_hits, C.c_int, vp, vp,
+                                      C.c_int]
+    lib.orc_trace_any.restype = None
+    lib.orc_trace_any.argtypes = [vp, C.c_int, vp, vp, vp, orc_rays, C.c_size_t, vp, C.c_int, vp, vp, C.c_int]
+    lib.orc_trace_brute.restype = None
+    lib.orc_trace_brute.argtypes = [vp, vp, C.c_int, orc_rays, C.c_size_t, orc_hits, C.c_int]
+    lib.orc_trace_any_brute.restype = None
+    lib.orc_trace_any_brute.argtypes = [vp, vp, C.c_int, orc_rays, C.c_size_t, vp, C.c_int]
+    lib.orc_bvh_depth.restype = C.c_int
+    lib.orc_bvh_depth.argtypes = [vp, C.c_int]
+    _lib = lib
+    return lib
+
+
+def _p(a: np.ndarray) -> int:
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data
+
+
+def tri_test(o, d, v0, v1, v2):
+    """Returns (hit, t, u, v, flags)."""
+    lib = load()
+    arrs = [np.ascontiguousarray(x, dtype=np.float32) for x in (o, d, v0, v1, v2)]
+    t, u, v, f = C.c_float(), C.c_float(), C.c_float(), C.c_uint32()
+    hit = lib.orc_tri_test(*[_p(a) for a in arrs], C.byref(t), C.byref(u), C.byref(v), C.byref(f))
+    return bool(hit), t.value, u.value, v.value, f.value
+
+
+class OracleBVH:
+    """Reference-semantics BVH2 over a triangle mesh (oracle)."""
+
+    def __init__(self, V, F, leaf_size=8, bins=12):
+        lib = load()
+        self.V = np.ascontiguousarray(V, dtype=np.float32)
+        self.F = np.ascontiguousarray(F, dtype=np.uint32)
+        n = self.F.shape[0]
+        self.pmin = np.empty((n, 3), np.float32)
+        self.pmax = np.empty((n, 3), np.float32)
+        self.pcen = np.empty((n, 3), np.float32)
+        lib.orc_tri_proxies(_p(self.V), _p(self.F), n, _p(self.pmin), _p(self.pmax), _p(self.pcen))
+        self.nodes = np.zeros(max(1, 2 * n), dtype=NODE_DTYPE)
+        self.indices = np.zeros(max(1, n), dtype=np.int32)
+        self.count = lib.orc_bvh_build(_p(self.pmin), _p(self.pmax), _p(self.pcen), n, leaf_size, bins,
+                                       _p(self.nodes), _p(self.indices))
+        self.nodes = self.nodes[:self.count].copy()
+        self.depth = lib.orc_bvh_depth(_p(self.nodes), self.count) if self.count else 0
+
+    def closest(self, rays: np.ndarray, mode=0, counters=False, threads=0):
+        lib = load()
+        rays = np.ascontiguousarray(rays, dtype=np.float32)
+        n = rays.shape[1]
+        out = {"prim": np.empty(n, np.uint32), "t": np.empty(n, np.float32), "u": np.empty(n, np.float32),
+               "v": np.empty(n, np.float32), "flags": np.empty(n, np.uint32)}
+        b = np.zeros(n, np.uint32) if counters else None
+        t = np.zeros(n, np.uint32) if counters else None
+        lib.orc_trace_closest(_p(self.nodes) if self.count else None, self.count, _p(self.indices), _p(self.V),
+                              _p(self.F), _rays(rays), n, _hits(out), mode,
+                              _p(b) if counters else None, _p(t) if counters else None, threads)
+        if counters:
+            out["box_tests"], out["tri_tests"] = b, t
+        return out
+
+    def any(self, rays: np.ndarray, mode=0, threads=0):
+        lib = load()
+        rays = np.ascontiguousarray(rays, dtype=np.float32)
+        n = rays.shape[1]
+        occ = np.zeros((n + 31) // 32, np.uint32)
+        lib.orc_trace_any(_p(self.nodes) if self.count else None, self.count, _p(self.indices), _p(self.V),
+                          _p(self.F), _rays(rays), n, _p(occ), mode, None, None, threads)
+        return occ
+
+
+def brute_closest(V, F, rays, threads=0):
+    lib = load()
+    V = np.ascontiguousarray(V, dtype=np.float32)
+    F = np.ascontiguousarray(F, dtype=np.uint32)
+    rays = np.ascontiguousarray(rays, dtype=np.float32)
+    n = rays.shape[1]
+    out = {"prim": np.empty(n, np.uint32), "t": np.empty(n, np.float32), "u": np.empty(n, np.float32),
+           "v": np.empty(n, np.float32), "flags": np.empty(n, np.uint32)}
+    lib.orc_trace_brute(_p(V), _p(F), F.shape[0], _rays(rays), n, _hits(out), threads)
+    return out
+
+
+def brute_any(V, F, rays, threads=0):
+    lib = load()
+    V = np.ascontiguousarray(V, dtype=np.float32)
+    F = np.ascontiguousarray(F, dtype=np.uint32)
+    rays = np.ascontiguousarray(rays, dtype=np.float32)
+    n = rays.shape[1]
+    occ = np.zeros((n + 31) // 32, np.uint32)
+    lib.orc_trace_any_brute(_p(V), _p(F), F.shape[0], _rays(rays), n, _p(occ), threads)
+    return occ
+
+
+def _rays(r):
+    o = orc_rays()
+    for k, name in enumerate(("ox", "oy", "oz", "tmin", "dx", "dy", "dz", "tmax")):
+        setattr(o, name, r[k].ctypes.data)
+    return o
+
+
+def _hits(h):
+    o = orc_hits()
+    for name in ("prim", "t", "u", "v", "flags"):
+        setattr(o, name, h[name].ctypes.data)
+    return o
+
+
+def unpack_bits(words: np.ndarray, n: int) -> np.ndarray:
+    w = np.asarray(words).view(np.uint32)
+    bits = (w[:, None] >> np.arange(32, dtype=np.uint32)[None, :]) & 1
+    return bits.reshape(-1)[:n].astype(bool)
