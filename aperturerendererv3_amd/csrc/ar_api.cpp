@@ -135,8 +135,10 @@ int trace_prologue(ar_ctx* ctx, TraceArgs& ta, size_t n, uint32_t& blocks) {
     return AR_OK;
 }
 
-hipStream_t pick_stream(ar_ctx* ctx, void* stream) {
-    return stream ? reinterpret_cast<hipStream_t>(stream) : ctx->stream;
+// NULL selects the HIP default (null) stream, which orders with every other
+// blocking stream of the process (torch's default stream is that stream).
+hipStream_t pick_stream(ar_ctx*, void* stream) {
+    return reinterpret_cast<hipStream_t>(stream);
 }
 
 } // namespace

@@ -1,0 +1,214 @@
+"""Traversal benchmark of the Aperture MI355X core (one JSON line on stdout).
+
+Workload (the traversal microbenchmark): a displaced subdivided tetrahedron of
+1,048,576 FP32 triangles; 4096 x 4096 ray slots, half pinhole primary rays
+and half cosine-weighted incoherent rays from random surface points; one
+step = closest hit over all 16,777,216 rays, shadow-segment generation from
+the hits toward (2, 2, 2), and any hit over those segments.  Inputs are
+resident in HBM before the timed region.
+
+value = (closest queries + any-hit queries) / second, in Grays/s (1e9 ray
+queries per second).
+
+roofline: the dominant kernel is the closest-hit traversal.  Its work model
+is 16 ops per box test + 40 ops per triangle test (counted per ray by the
+kernel's counting variant on the same rays), priced against the non-packed
+VALU lane-op peak of MI355X (256 CUs x 64 lanes x 2.4 GHz = 39.3 Tops/s);
+its compulsory HBM traffic is 32 B per ray read + 20 B per hit written.
+The kernel's average duration comes from HIP events recorded on the stream
+the kernel runs on, over the timed region.
+
+cpu_baseline: the oracle's BVH2 traversal (the reference builder's binned
+SAH with 12 bins and leaves of 8, the reference's sorted 128-entry stack, and
+a slab test that also requires entry <= exit; the reference's own slab test
+omits that check and visits ~1e5 boxes per ray on this mesh) on a bounded
+sample of the same rays (every 2nd slot: 8,388,608 closest hits + their shadow
+any hits), with OpenMP over the host cores available to this process.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+METRIC = "Grays/s traversal; paths/s at equal FP64-oracle relMSE, plain vs batched+MLP"
+VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12      # 39.3 Tops/s
+HBM_PEAK_GBPS = 8000.0
+LIGHT = (2.0, 2.0, 2.0)
+TRAFFIC_FILE = ROOT / "profiles" / "trace_closest_traversal_1M_tris_16M_rays.md"
+
+
+def parse_args():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--levels", type=int, default=9, help="subdivision levels (9 = 1,048,576 triangles)")
+    ap.add_argument("--res", type=int, default=4096, help="ray grid side (4096 = 16,777,216 slots)")
+    ap.add_argument("--leaf-size", type=int, default=4)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample-stride", type=int, default=2)
+    return ap.parse_args()
+
+
+def read_traffic():
+    """HBM bytes per closest-hit launch, from the committed rocprofv3 --pmc summary."""
+    if not TRAFFIC_FILE.exists():
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", TRAFFIC_FILE.read_text())
+    return float(m.group(1)) if m else None
+
+
+def cpu_baseline(V, F, rays_np, stride, shadow_np):
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_lib as orc
+    cores = len(os.sched_getaffinity(0))
+    bvh = orc.OracleBVH(V, F, leaf_size=8, bins=12)
+    sub = rays_np[:, ::stride].copy()
+    sh = shadow_np[:, ::stride].copy()
+    n_any = int((sh[7] > 0).sum())
+    t0 = time.perf_counter()
+    bvh.closest(sub, mode=0, threads=cores)
+    bvh.any(sh, mode=0, threads=cores)
+    dt = time.perf_counter() - t0
+    return {"value": (sub.shape[1] + n_any) / dt / 1e9, "unit": "Grays/s", "cores": cores, "kind": "port",
+            "sample": f"every {stride}th ray slot of the same workload: {sub.shape[1]} closest-hit rays + "
+                      f"{n_any} shadow any-hit segments, oracle BVH2 traversal (reference builder: binned SAH, "
+                      f"12 bins, leaf 8; sorted 128-entry stack; strict slab test), OpenMP, {dt:.2f} s"}
+
+
+def main():
+    args = parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(the traversal shards by ray slots with the BVH replicated, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from aperturerendererv3_amd import AR_MISS, Context
+    from aperturerendererv3_amd.workloads import traversal_mesh, traversal_rays
+
+    V, F = traversal_mesh(levels=args.levels)
+    rays_np = traversal_rays(V, F, res=args.res)
+    n = rays_np.shape[1]
+    ctx = Context(0)
+    ctx.set_meshes([{"vertices": V, "indices": F}])
+    info = ctx.build_bvh(leaf_size=args.leaf_size)
+    dev = torch.device("cuda:0")
+    rays = torch.from_numpy(rays_np).to(dev)
+    hits = ctx.alloc_hits(n, dev)
+    shadow = torch.empty_like(rays)
+    occ = torch.empty((n + 31) // 32, dtype=torch.int32, device=dev)
+    stream = torch.cuda.current_stream()
+
+    def step(ev=None):
+        if ev is not None:
+            ev[0].record(stream)
+        ctx.trace_closest(rays, hits)
+        if ev is not None:
+            ev[1].record(stream)
+        ctx.make_shadow_rays(rays, hits, LIGHT, out=shadow)
+        if ev is not None:
+            ev[2].record(stream)
+        ctx.trace_any(shadow, occ)
+        if ev is not None:
+            ev[3].record(stream)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    n_hit = int((hits["prim"] != -1).sum().item())
+    n_any = int((shadow[7] > 0).sum().item())
+    queries = n + n_any
+
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        step(evs[k])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    closest_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in evs]))
+    shadow_ms = float(np.mean([e[1].elapsed_time(e[2]) for e in evs]))
+    any_ms = float(np.mean([e[2].elapsed_time(e[3]) for e in evs]))
+    ms_per_step = wall * 1e3 / args.steps
+    value = queries / (wall / args.steps) / 1e9
+
+    # ops model of the closest-hit kernel on these rays (counting variant, untimed)
+    cb = torch.empty(n, dtype=torch.int32, device=dev)
+    ct = torch.empty(n, dtype=torch.int32, device=dev)
+    h2 = ctx.alloc_hits(n, dev)
+    ctx.trace_closest(rays, h2, counters=(cb, ct))
+    torch.cuda.synchronize()
+    boxes = int(cb.to(torch.int64).sum().item())
+    tris = int(ct.to(torch.int64).sum().item())
+    assert torch.equal(h2["prim"], hits["prim"]), "counting variant disagrees with the timed kernel"
+    ops = 16 * boxes + 40 * tris
+    achieved_tops = ops / (closest_ms * 1e-3) / 1e12
+    compulsory = n * 32 + n * 20
+    hbm_gbps = compulsory / (closest_ms * 1e-3) / 1e9
+    traffic = read_traffic()
+
+    # per-class breakdown (untimed for value): primary vs incoherent closest hit
+    yy, xx = np.divmod(np.arange(n), args.res)
+    prim_idx = torch.from_numpy(np.nonzero(((xx + yy) & 1) == 0)[0]).to(dev)
+    inc_idx = torch.from_numpy(np.nonzero(((xx + yy) & 1) == 1)[0]).to(dev)
+    breakdown = {}
+    for name, idx in (("primary", prim_idx), ("incoherent", inc_idx)):
+        sub = rays[:, idx].contiguous()
+        hs = ctx.alloc_hits(sub.shape[1], dev)
+        for _ in range(2):
+            ctx.trace_closest(sub, hs)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        reps = 5
+        e0.record(stream)
+        for _ in range(reps):
+            ctx.trace_closest(sub, hs)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        breakdown[f"{name}_closest_grays"] = sub.shape[1] / (e0.elapsed_time(e1) / reps * 1e-3) / 1e9
+    breakdown["closest_grays"] = n / (closest_ms * 1e-3) / 1e9
+    breakdown["any_grays"] = n_any / (any_ms * 1e-3) / 1e9
+    breakdown["closest_ms"] = closest_ms
+    breakdown["shadow_gen_ms"] = shadow_ms
+    breakdown["any_ms"] = any_ms
+    breakdown["hit_fraction"] = n_hit / n
+    breakdown["box_tests_per_ray"] = boxes / n
+    breakdown["tri_tests_per_ray"] = tris / n
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(V, F, rays_np, args.cpu_sample_stride, shadow.cpu().numpy())
+
+    valu_frac = achieved_tops / VALU_PEAK_TOPS
+    out = {
+        "metric": METRIC, "value": round(value, 4), "unit": "Grays/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "traversal microbenchmark: displaced subdivided tetrahedron, "
+                               f"{F.shape[0]} triangles, {args.res}x{args.res} ray slots (50% pinhole primary, "
+                               "50% cosine-incoherent), closest hit + any-hit shadow segments",
+                   "triangles": int(F.shape[0]), "rays": n, "any_hit_queries": n_any,
+                   "bvh": {"width": 2, "leaf_size": args.leaf_size, "nodes": info["node_count"],
+                           "depth": info["depth"], "build_s": round(info["build_seconds"], 3)},
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "valu", "kernel": "k_trace_closest", "achieved": round(achieved_tops, 4),
+                     "peak": round(VALU_PEAK_TOPS, 2), "unit": "Tops/s", "frac": round(valu_frac, 5),
+                     "traffic": traffic, "compulsory_bytes": compulsory,
+                     "hbm_achieved_GBps": round(hbm_gbps, 2), "hbm_frac": round(hbm_gbps / HBM_PEAK_GBPS, 5),
+                     "ops_per_launch": ops, "avg_launch_ms": round(closest_ms, 4)},
+        "cpu_baseline": cpu,
+        "breakdown": {k: round(v, 4) for k, v in breakdown.items()},
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

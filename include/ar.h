@@ -11,8 +11,8 @@
  *
  * Threading: one ar_ctx per host thread (thread-compatible, not
  * thread-safe).  Device work is queued on the stream passed in (a
- * hipStream_t, NULL = the context's own stream) and is not waited for
- * unless the entry point says so.
+ * hipStream_t; NULL = the HIP default stream) and is not waited for unless
+ * the entry point says so.
  */
 #ifndef AR_H
 #define AR_H

@@ -329,11 +329,15 @@ static float inv_safe(float x)
     return (x == 0.0f) ? (x >= 0.0f ? FLT_MAX : -FLT_MAX) : 1.0f / x;
 }
 
-/* FastAABBHitWithEnter (bvh.cuh:446-470).  With pad != 0 the box is grown by
- * pad on every side first; the canonical traversal uses that to make the
- * box test conservative with respect to the triangle test's rounding. */
+/* FastAABBHitWithEnter (bvh.cuh:446-470).  The reference accepts a box when
+ * tExit >= max(0, tMin) and tEnter <= tMax, without requiring
+ * tEnter <= tExit, so a ray that passes beside a box still "hits" it until a
+ * closer hit shrinks tMax (mode 1 keeps that, mode 0 adds the check).  With
+ * pad != 0 the box is grown by pad on every side first; the canonical
+ * traversal uses that to make the test conservative with respect to the
+ * triangle test's rounding. */
 static int box_hit(const float o[3], const float inv[3], const float mn[3], const float mx[3],
-                   float tmin, float tmax, float pad, float* tenter)
+                   float tmin, float tmax, float pad, int strict, float* tenter)
 {
     const float lx = mn[0] - pad, hx = mx[0] + pad;
     const float ly = mn[1] - pad, hy = mx[1] + pad;
@@ -344,7 +348,7 @@ static int box_hit(const float o[3], const float inv[3], const float mn[3], cons
     te = fmaxf(te, fminf(ty0, ty1)); tx = fminf(tx, fmaxf(ty0, ty1));
     const float tz0 = (lz - o[2]) * inv[2], tz1 = (hz - o[2]) * inv[2];
     te = fmaxf(te, fminf(tz0, tz1)); tx = fminf(tx, fmaxf(tz0, tz1));
-    if (tx >= fmaxf(0.0f, tmin) && te <= tmax) { *tenter = te; return 1; }
+    if (tx >= fmaxf(0.0f, tmin) && te <= tmax && (!strict || te <= tx)) { *tenter = te; return 1; }
     return 0;
 }
 
@@ -400,7 +404,7 @@ static int traverse_one(const orc_node* nodes, int node_count, const int32_t* in
             float te;
             nb++;
             const float pad = mode == 0 ? box_pad(nd->bmin, nd->bmax, omax) : 0.0f;
-            if (!box_hit(o, inv, nd->bmin, nd->bmax, tmin, best_t, pad, &te)) continue;
+            if (!box_hit(o, inv, nd->bmin, nd->bmax, tmin, best_t, pad, mode == 0, &te)) continue;
             if (nd->count > 0) {
                 for (int k = 0; k < nd->count; ++k) {
                     const int p = indices[nd->first + k];
@@ -426,8 +430,8 @@ static int traverse_one(const orc_node* nodes, int node_count, const int32_t* in
                 nb += 2;
                 const float pl = mode == 0 ? box_pad(L->bmin, L->bmax, omax) : 0.0f;
                 const float pr = mode == 0 ? box_pad(R->bmin, R->bmax, omax) : 0.0f;
-                const int hl = box_hit(o, inv, L->bmin, L->bmax, tmin, best_t, pl, &tl);
-                const int hr = box_hit(o, inv, R->bmin, R->bmax, tmin, best_t, pr, &tr);
+                const int hl = box_hit(o, inv, L->bmin, L->bmax, tmin, best_t, pl, mode == 0, &tl);
+                const int hr = box_hit(o, inv, R->bmin, R->bmax, tmin, best_t, pr, mode == 0, &tr);
                 /* sorted push, intersection.cuh:218-243 (near child popped first) */
                 if (hl && hr) {
                     if (sp + 2 > ORC_STACK) abort();
