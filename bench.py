@@ -55,6 +55,8 @@ def parse_args():
     ap.add_argument("--leaf-size", type=int, default=4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-stride", type=int, default=2)
+    ap.add_argument("--profile", action="store_true",
+                    help="timed steps only, plus one 256 MiB device copy used to scale HBM counters")
     return ap.parse_args()
 
 
@@ -139,6 +141,16 @@ def main():
     any_ms = float(np.mean([e[2].elapsed_time(e[3]) for e in evs]))
     ms_per_step = wall * 1e3 / args.steps
     value = queries / (wall / args.steps) / 1e9
+
+    if args.profile:
+        src = torch.empty(64 << 20, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"profile_run": True, "closest_ms": closest_ms, "any_ms": any_ms,
+                          "copy_bytes_each_way": src.numel() * 4}), flush=True)
+        ctx.close()
+        return
 
     # ops model of the closest-hit kernel on these rays (counting variant, untimed)
     cb = torch.empty(n, dtype=torch.int32, device=dev)
