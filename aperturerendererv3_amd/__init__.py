@@ -17,10 +17,12 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import (AR_MISS, AR_MODEL_GGX, AR_MODEL_LAMBERT, AR_OK, ar_bvh_info, ar_bvh_opts, ar_device_opts,
-                   ar_hits, ar_material, ar_mesh, ar_rays)
+from ._lib import (AR_MISS, AR_MODEL_GGX, AR_MODEL_LAMBERT, AR_OK, AR_SKY_CONSTANT, AR_SKY_GRADIENT, AR_SKY_NONE,
+                   ar_bvh_info, ar_bvh_opts, ar_camera, ar_device_opts, ar_hits, ar_material, ar_mesh, ar_rays,
+                   ar_render_opts)
 
-__all__ = ["Context", "ArError", "AR_MISS", "AR_MODEL_LAMBERT", "AR_MODEL_GGX", "Material", "RAY_FIELDS"]
+__all__ = ["Context", "ArError", "AR_MISS", "AR_MODEL_LAMBERT", "AR_MODEL_GGX", "AR_SKY_NONE", "AR_SKY_GRADIENT",
+           "AR_SKY_CONSTANT", "Material", "RAY_FIELDS"]
 
 RAY_FIELDS = ("ox", "oy", "oz", "tmin", "dx", "dy", "dz", "tmax")
 
@@ -118,7 +120,7 @@ class Context:
                 tm = np.ascontiguousarray(tm, dtype=np.uint32)
                 keep.append(tm)
                 cms[i].tri_material = tm.ctypes.data
-            mats = m.get("materials") or []
+            mats = [mm if isinstance(mm, Material) else Material(**mm) for mm in (m.get("materials") or [])]
             if mats:
                 arr = (ar_material * len(mats))(*[mm.to_c() for mm in mats])
                 keep.append(arr)
@@ -199,4 +201,25 @@ class Context:
         self._check(self._lib.ar_make_shadow_rays(self._h, C.byref(rc), C.byref(hc), n, C.byref(tgt), tmin,
                                                   tmax_scale, C.byref(oc), _stream_ptr(stream)),
                     "ar_make_shadow_rays")
+        return out
+
+    # -- integrator ----------------------------------------------------------
+    def render(self, width, height, spp, cam_pos=(0.0, 0.0, 1.1), cam_dir=(0.0, 0.0, -1.0), vfov=60.0, seed=1,
+               max_depth=5, nee=True, sky=AR_SKY_NONE, env=(0.0, 0.0, 0.0), rr_min_depth=3, sample_offset=0,
+               max_paths_per_pass=0, out=None, stream=None):
+        """Mean radiance image (height, width, 3) float32 on the GPU (ar_render)."""
+        import torch
+        cam = ar_camera()
+        cam.pos[:] = [float(x) for x in cam_pos]
+        cam.dir[:] = [float(x) for x in cam_dir]
+        cam.vfov_deg = float(vfov)
+        o = ar_render_opts()
+        o.width, o.height, o.spp, o.sample_offset = width, height, spp, sample_offset
+        o.seed, o.max_depth, o.rr_min_depth, o.nee, o.sky = seed, max_depth, rr_min_depth, int(bool(nee)), sky
+        o.env[:] = [float(x) for x in env]
+        o.max_paths_per_pass = max_paths_per_pass
+        if out is None:
+            out = torch.empty((height, width, 3), dtype=torch.float32, device=f"cuda:{self.device}")
+        self._check(self._lib.ar_render(self._h, C.byref(cam), C.byref(o), _ptr(out), _stream_ptr(stream)),
+                    "ar_render")
         return out

@@ -60,6 +60,20 @@ class ar_bvh_info(C.Structure):
                 ("sah_cost", C.c_double)]
 
 
+class ar_camera(C.Structure):
+    _fields_ = [("pos", C.c_float * 3), ("dir", C.c_float * 3), ("vfov_deg", C.c_float)]
+
+
+class ar_render_opts(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32), ("sample_offset", C.c_uint32),
+                ("seed", C.c_uint64), ("max_depth", C.c_uint32), ("rr_min_depth", C.c_uint32), ("nee", C.c_uint32),
+                ("sky", C.c_uint32), ("env", C.c_float * 3), ("max_paths_per_pass", C.c_uint32)]
+
+
+AR_SKY_NONE = 0
+AR_SKY_GRADIENT = 1
+AR_SKY_CONSTANT = 2
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     "ar_abi_version": (C.c_int, []),
@@ -76,6 +90,7 @@ SIGNATURES = {
     "ar_make_shadow_rays": (C.c_int, [C.c_void_p, C.POINTER(ar_rays), C.POINTER(ar_hits), C.c_size_t,
                                       C.POINTER(C.c_float * 3), C.c_float, C.c_float, C.POINTER(ar_rays),
                                       C.c_void_p]),
+    "ar_render": (C.c_int, [C.c_void_p, C.POINTER(ar_camera), C.POINTER(ar_render_opts), C.c_void_p, C.c_void_p]),
 }
 
 _lib = None

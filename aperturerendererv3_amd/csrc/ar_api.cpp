@@ -119,6 +119,8 @@ bool rays_ok(const ar_rays* r) {
 
 bool hits_ok(const ar_hits* h) { return h && h->prim && h->t && h->u && h->v && h->flags; }
 
+} // namespace
+
 int trace_prologue(ar_ctx* ctx, TraceArgs& ta, size_t n, uint32_t& blocks) {
     if (!ctx->bvh_ready) return set_error(ctx, AR_E_NOSCENE, "no BVH: call ar_scene_set and ar_bvh_build first");
     int rc = ensure_spill(ctx);
@@ -134,6 +136,8 @@ int trace_prologue(ar_ctx* ctx, TraceArgs& ta, size_t n, uint32_t& blocks) {
     ta.threads = blocks * kTraceBlock;
     return AR_OK;
 }
+
+namespace {
 
 // NULL selects the HIP default (null) stream, which orders with every other
 // blocking stream of the process (torch's default stream is that stream).
@@ -180,6 +184,8 @@ int ar_destroy(ar_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     free_bvh(ctx);
+    free_scene(ctx);
+    free_render_buffers(ctx);
     if (ctx->spill) (void)hipFree(ctx->spill);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -237,6 +243,7 @@ int ar_scene_set(ar_ctx* ctx, const ar_mesh* meshes, size_t nmesh) {
     ctx->scene_set = true;
     (void)hipSetDevice(ctx->device);
     free_bvh(ctx);
+    free_scene(ctx);
     return AR_OK;
 }
 
@@ -258,7 +265,7 @@ int ar_bvh_build(ar_ctx* ctx, const ar_bvh_opts* opts) {
     if (n == 0) {
         ctx->bvh.root = kRefEmpty;
         ctx->bvh_ready = true;
-        return AR_OK;
+        return upload_scene(ctx);
     }
     std::vector<float> lo(3 * (size_t)n), hi(3 * (size_t)n), cen(3 * (size_t)n);
     float mag = 0.0f;
@@ -303,7 +310,7 @@ int ar_bvh_build(ar_ctx* ctx, const ar_bvh_opts* opts) {
     ctx->info.device_bytes = cv.nodes.size() * sizeof(GpuNode2) + cv.tris.size() * sizeof(float);
     ctx->info.build_seconds = std::chrono::duration<double>(t1 - t0).count();
     ctx->info.sah_cost = br.sah_cost;
-    return AR_OK;
+    return upload_scene(ctx);
 }
 
 int ar_bvh_info_get(const ar_ctx* ctx, ar_bvh_info* info) {
@@ -359,6 +366,16 @@ int ar_make_shadow_rays(ar_ctx* ctx, const ar_rays* rays, const ar_hits* hits, s
         return set_error(ctx, AR_E_INVALID, "NULL argument");
     return check_hip(ctx, launch_shadow_rays(*rays, *hits, n, target[0], target[1], target[2], tmin, tmax_scale, *out,
                                              pick_stream(ctx, stream)), "shadow ray launch");
+}
+
+int ar_render(ar_ctx* ctx, const ar_camera* cam, const ar_render_opts* opts, float* out_rgb, void* stream) {
+    if (!ctx) return AR_E_INVALID;
+    if (!cam || !opts || !out_rgb) return set_error(ctx, AR_E_INVALID, "NULL camera, options or output");
+    if (opts->width == 0 || opts->height == 0 || opts->spp == 0) return set_error(ctx, AR_E_INVALID, "empty image or spp = 0");
+    if (opts->max_depth == 0 || opts->max_depth > 1000) return set_error(ctx, AR_E_INVALID, "max_depth must be 1..1000");
+    if (opts->sky > AR_SKY_CONSTANT) return set_error(ctx, AR_E_INVALID, "unknown sky mode");
+    if (!ctx->bvh_ready) return set_error(ctx, AR_E_NOSCENE, "no BVH: call ar_scene_set and ar_bvh_build first");
+    return render(ctx, cam, opts, out_rgb, pick_stream(ctx, stream));
 }
 
 } // extern "C"

@@ -83,6 +83,52 @@ struct DeviceBVH {
     float origin_limit = 0.0f;   // kOriginLimit * scene magnitude
 };
 
+// Shading data of the scene, prim order (uploaded with the BVH).
+struct GpuMaterial {
+    float albedo[3];
+    float specular;
+    float emission[3];
+    float metallic;
+    float roughness;
+    uint32_t model;
+    float pad[2];
+};
+static_assert(sizeof(GpuMaterial) == 48, "GpuMaterial must be 48 bytes");
+
+struct SceneDev {
+    float4* prims = nullptr;      // 3 per triangle: (v0, material bits), (v1, light index bits), (v2, 0)
+    GpuMaterial* mats = nullptr;
+    int32_t* light_tri = nullptr;
+    float* light_area = nullptr;
+    float* light_prob = nullptr;
+    float* light_cdf = nullptr;
+    uint32_t nlights = 0;
+    uint32_t ntri = 0;
+};
+
+// Wavefront state of the integrator, sized for `capacity` paths.
+struct RenderBuffers {
+    size_t capacity = 0;
+    void* block = nullptr;        // one allocation, carved below
+    float* rays[2][8];            // queue rays (current / next), SoA
+    uint32_t* qpath[2];           // queue slot -> path id
+    uint32_t* hit_prim; float* hit_t; float* hit_u; float* hit_v; uint32_t* hit_flags;
+    float* srays[8];              // shadow rays
+    uint32_t* spath;
+    float* scontrib[3];
+    uint32_t* sflags;             // bit0: the sampled light point faced the surface (both cosines > 0)
+    uint32_t* occ;                // shadow occlusion bits
+    float* mask[3];
+    float* rad[3];
+    float* last_pos[3];
+    float* last_pdf;
+    uint32_t* meta;               // non-delta depth (bits 0..15), previous vertex took NEE (bit 16)
+    uint32_t* counters;           // [0] current queue, [1] next queue, [2] shadow queue
+    float* accum;                 // width*height*3 running sums
+    size_t accum_pixels = 0;
+    float* eps_table;             // INTERSECT_EPS(d) for d < 256
+};
+
 } // namespace ar
 
 struct ar_ctx {
@@ -101,6 +147,9 @@ struct ar_ctx {
     ar::DeviceBVH bvh;
     bool bvh_ready = false;
     ar_bvh_info info{};
+
+    ar::SceneDev scene;
+    ar::RenderBuffers rb;
 
     // traversal scratch: per-thread stack spill, sized for the persistent grid
     int* spill = nullptr;
@@ -121,11 +170,21 @@ struct TraceArgs {
     int* spill;
     uint32_t threads;
 };
+// n_dev (optional): device pointer to the ray count, read by the kernel
+// (then n is only the capacity used to size the grid).
 hipError_t launch_trace_closest(const TraceArgs& ta, const ar_rays& r, size_t n, const ar_hits& h,
-                                uint32_t* cnt_box, uint32_t* cnt_tri, uint32_t blocks, hipStream_t s);
+                                uint32_t* cnt_box, uint32_t* cnt_tri, uint32_t blocks, hipStream_t s,
+                                const uint32_t* n_dev = nullptr);
 hipError_t launch_trace_any(const TraceArgs& ta, const ar_rays& r, size_t n, uint32_t* occ,
-                            uint32_t blocks, hipStream_t s);
+                            uint32_t blocks, hipStream_t s, const uint32_t* n_dev = nullptr);
 hipError_t launch_shadow_rays(const ar_rays& r, const ar_hits& h, size_t n, float tx, float ty, float tz,
                               float tmin, float tmax_scale, const ar_rays& out, hipStream_t s);
 int trace_blocks_per_cu();
+
+// integrator.hip
+int upload_scene(ar_ctx* ctx);
+void free_scene(ar_ctx* ctx);
+void free_render_buffers(ar_ctx* ctx);
+int render(ar_ctx* ctx, const ar_camera* cam, const ar_render_opts* opts, float* out, hipStream_t s);
+int trace_prologue(ar_ctx* ctx, TraceArgs& ta, size_t n, uint32_t& blocks);
 } // namespace ar

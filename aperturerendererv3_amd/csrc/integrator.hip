@@ -1,0 +1,855 @@
+// Wavefront path integrator for gfx950.
+//
+// Replaces the reference's megakernel RenderCamera -> CalculateRadiance
+// (core/scene.cu:135-189, core/render/render.cuh:460-998) with one kernel per
+// stage and compacted queues between them:
+//   camera -> [ trace closest -> shade -> trace any (shadow) -> resolve ] x depth
+//   -> accumulate.
+// Queues are structure-of-arrays; shade appends surviving paths and shadow
+// segments with one atomic per wave (ballot + popcount prefix), so every
+// traversal launch sees a dense ray array.  Ray counts live in device memory
+// and the traversal kernels read them, so the host loop issues a fixed
+// sequence of launches with no synchronisation (it can be captured in a
+// HIP graph).
+//
+// Per-path arithmetic follows the reference step by step (see the citations
+// on each stage) in the same floating-point order as the CPU oracle, so at
+// equal seed the GPU and the oracle trace the same paths except where a
+// transcendental (sinf/cosf) differs in its last bit.
+#include "ar_internal.hpp"
+
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+
+namespace ar {
+namespace {
+
+constexpr uint32_t kCameraBounce = 0xFFFFFFFFu;
+constexpr int kBlock = 256;
+constexpr uint32_t kMetaNee = 1u << 16;
+
+// ---------------------------------------------------------------- helpers
+struct F3 { float x, y, z; };
+__device__ __forceinline__ F3 f3(float x, float y, float z) { return {x, y, z}; }
+__device__ __forceinline__ F3 add(F3 a, F3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ F3 sub(F3 a, F3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ F3 mul(F3 a, F3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+__device__ __forceinline__ F3 scale(F3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ float dot(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ F3 cross(F3 a, F3 b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ float length(F3 a) { return sqrtf(dot(a, a)); }
+__device__ __forceinline__ F3 normalize(F3 v) {
+    const float l2 = dot(v, v);
+    if (l2 <= 1e-20f) return {0.0f, 0.0f, 0.0f};
+    const float inv = 1.0f / sqrtf(l2);
+    return scale(v, inv);
+}
+__device__ __forceinline__ float maxr(float a, float b) { return a > b ? a : b; }
+__device__ __forceinline__ float minr(float a, float b) { return a < b ? a : b; }
+__device__ __forceinline__ float clampr(float x, float a, float b) { return minr(maxr(x, a), b); }
+__device__ __forceinline__ float lum(F3 c) { return c.x * 0.2126f + c.y * 0.7152f + c.z * 0.0722f; }
+__device__ __forceinline__ float max3(F3 c) { return maxr(c.x, maxr(c.y, c.z)); }
+__device__ __forceinline__ float fixnan(float x) { return (isnan(x) || isinf(x)) ? 0.0f : x; }
+__device__ __forceinline__ F3 xyz(float4 v) { return {v.x, v.y, v.z}; }
+
+constexpr float kPi = 3.14159265358979323846f;
+
+// Philox4x32-10 (Salmon et al., SC'11)
+__device__ __forceinline__ void philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1,
+                                       uint32_t out[4]) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
+        const uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
+        const uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+__device__ __forceinline__ void uniforms(uint64_t seed, uint32_t pix, uint32_t smp, uint32_t bounce, uint32_t group,
+                                         float* u) {
+    uint32_t x[4];
+    philox(pix, smp, bounce, group, (uint32_t)seed, (uint32_t)(seed >> 32), x);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) u[k] = (float)(x[k] >> 8) * 0x1p-24f;
+}
+
+// Appends `pred` lanes to a queue with one atomic per wave; returns the slot.
+__device__ __forceinline__ uint32_t wave_append(bool pred, uint32_t* counter) {
+    const uint64_t m = __ballot(pred);
+    if (m == 0) return 0;
+    const uint32_t lane = __lane_id();
+    const uint32_t leader = (uint32_t)__ffsll((unsigned long long)m) - 1u;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
+    base = __shfl(base, (int)leader);
+    return base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+}
+
+// ---------------------------------------------------------------- sampling
+// BuildTangentBasisRandom (core/vector.cuh:284-294)
+__device__ __forceinline__ void tangent_basis_random(F3 n, float e, F3& t, F3& b) {
+    const F3 a = (fabsf(n.z) < 0.999f) ? f3(0, 0, 1) : f3(1, 0, 0);
+    const F3 t0 = normalize(cross(a, n));
+    const F3 b0 = cross(n, t0);
+    const float ang = e * kPi * 2.0f;
+    const float s = sinf(ang), c = cosf(ang);
+    t = add(scale(t0, c), scale(b0, s));
+    b = add(scale(t0, -s), scale(b0, c));
+}
+
+// ConcentricSampleDisk (core/render/sample.cuh:63-86)
+__device__ __forceinline__ void concentric_disk(float u0, float u1, float& dx, float& dy) {
+    const float sx = 2.0f * u0 - 1.0f, sy = 2.0f * u1 - 1.0f;
+    if (sx == 0.0f && sy == 0.0f) { dx = 0.0f; dy = 0.0f; return; }
+    float r, phi;
+    if (fabsf(sx) > fabsf(sy)) {
+        r = sx;
+        phi = (kPi / 4.0f) * (sy / maxr(fabsf(sx), 1e-20f));
+    } else {
+        r = sy;
+        phi = (kPi / 2.0f) - (kPi / 4.0f) * (sx / maxr(fabsf(sy), 1e-20f));
+    }
+    dx = r * cosf(phi);
+    dy = r * sinf(phi);
+}
+
+// SampleCosineWeightedHemisphere (core/render/sample.cuh:88-108)
+__device__ __forceinline__ F3 sample_cosine(float e0, float e1, float e2, F3 n, float& pdf) {
+    F3 t, b;
+    tangent_basis_random(n, e2, t, b);
+    float x, y;
+    concentric_disk(e0, e1, x, y);
+    const float z = sqrtf(maxr(0.0f, 1.0f - x * x - y * y));
+    const F3 dir = add(add(scale(t, x), scale(b, y)), scale(n, z));
+    pdf = z / kPi;
+    return normalize(dir);
+}
+
+// GGX_D (core/render/brdf.cuh:82-87)
+__device__ __forceinline__ float ggx_d(float alpha, float noh) {
+    const float a2 = alpha * alpha;
+    const float d = noh * noh * (a2 - 1.0f) + 1.0f;
+    return a2 / maxr(kPi * d * d, 1e-12f);
+}
+
+// SampleGGXBoundedVNDF (core/render/sample.cuh:179-224)
+__device__ __forceinline__ F3 sample_vndf(F3 v, F3 n, float alpha, float r0, float r1, float r2, float& pdf) {
+    F3 X, Y;
+    tangent_basis_random(n, r2, X, Y);
+    const float xov = dot(X, v), yov = dot(Y, v), nov = dot(n, v);
+    const float nov2 = minr(1.0f, nov * nov);
+    const float a2 = alpha * alpha;
+    const float inv = 1.0f / sqrtf(a2 + (1.0f - a2) * nov2);
+    const F3 vstd = f3(xov * alpha * inv, yov * alpha * inv, nov * inv);
+    const float phi = 2.0f * kPi * r0;
+    const float s = 1.0f + sqrtf(1.0f - nov2);
+    const float s2 = s * s;
+    const float k = (1.0f - a2) * s2 / (s2 + a2 * nov2);
+    const float b = nov > 0.0f ? k * vstd.z : vstd.z;
+    const float z = (1.0f - r1) * (1.0f + b) - b;
+    const float st = sqrtf(clampr(1.0f - z * z, 0.0f, 1.0f));
+    const F3 mstd = f3(vstd.x + st * cosf(phi), vstd.y + st * sinf(phi), vstd.z + z);
+    const F3 h = normalize(f3(mstd.x * alpha, mstd.y * alpha, mstd.z));
+    const F3 lv = f3(xov, yov, nov);
+    const float d2 = 2.0f * dot(lv, h);
+    const F3 ll = f3(d2 * h.x - xov, d2 * h.y - yov, d2 * h.z - nov);
+    const F3 dir = add(add(scale(X, ll.x), scale(Y, ll.y)), scale(n, ll.z));
+    const float ndf = ggx_d(alpha, h.z);
+    const float len2 = (1.0f - nov2) * a2;
+    const float t = sqrtf(len2 + nov2);
+    pdf = ndf / (2.0f * (k * nov + t));
+    return dir;
+}
+
+// GGXBoundedVNDF_PDF (core/render/sample.cuh:226-248)
+__device__ __forceinline__ float vndf_pdf(F3 v, F3 n, F3 l, float alpha) {
+    const float nov = dot(n, v), nol = dot(n, l);
+    if (nov <= 0.0f || nol <= 0.0f) return 0.0f;
+    const F3 h = normalize(add(v, l));
+    const float nov2 = minr(1.0f, nov * nov);
+    const float a2 = alpha * alpha;
+    const float s = 1.0f + sqrtf(1.0f - nov2);
+    const float s2 = s * s;
+    const float k = (1.0f - a2) * s2 / (s2 + a2 * nov2);
+    const float noh = maxr(0.0f, dot(n, h));
+    const float ndf = ggx_d(alpha, noh);
+    const float len2 = (1.0f - nov2) * a2;
+    const float t = sqrtf(len2 + nov2);
+    return maxr(ndf / (2.0f * (k * nov + t)), 1e-12f);
+}
+
+// ---------------------------------------------------------------- BSDF
+struct Mat {
+    F3 albedo, emission;
+    float specular, metallic, roughness;
+    uint32_t model;
+};
+
+__device__ __forceinline__ Mat load_mat(const GpuMaterial* __restrict__ mats, uint32_t id) {
+    const float4* m4 = reinterpret_cast<const float4*>(mats + id);
+    const float4 a = m4[0], b = m4[1], c = m4[2];
+    Mat m;
+    m.albedo = f3(a.x, a.y, a.z);
+    m.specular = a.w;
+    m.emission = f3(b.x, b.y, b.z);
+    m.metallic = b.w;
+    m.roughness = c.x;
+    m.model = __float_as_uint(c.y);
+    return m;
+}
+
+__device__ __forceinline__ float f0_ue(float spec) { return 0.08f * clampr(spec, 0.0f, 1.0f); }
+
+__device__ __forceinline__ float g_correlated(float alpha, float nol, float nov) {
+    const float a2 = alpha * alpha;
+    const float gv = nol * sqrtf(a2 + (1.0f - a2) * nov * nov);
+    const float gl = nov * sqrtf(a2 + (1.0f - a2) * nol * nol);
+    const float den = gv + gl;
+    return den > 0.0f ? (2.0f * nol * nov) / den : 0.0f;
+}
+
+__device__ __forceinline__ F3 f0_of(const Mat& m) {
+    const float f = f0_ue(m.specular);
+    return f3(f + (m.albedo.x - f) * m.metallic, f + (m.albedo.y - f) * m.metallic,
+              f + (m.albedo.z - f) * m.metallic);
+}
+
+__device__ __forceinline__ float alpha_of(const Mat& m) {
+    const float r = clampr(m.roughness, 0.01f, 1.0f);
+    return r * r;
+}
+
+// EvaluateBSDFWithCos (core/render/material.cuh:47-165), rd toward the surface
+__device__ __forceinline__ F3 eval_bsdf(const Mat& m, F3 rd, F3 l, F3 n) {
+    const float nol = maxr(0.0f, dot(n, l));
+    if (m.model == AR_MODEL_LAMBERT) return scale(m.albedo, nol / kPi);
+    const F3 v = normalize(scale(rd, -1.0f));
+    const float nov = maxr(0.0f, dot(n, v));
+    if (nov <= 0.0f || nol <= 0.0f) return f3(0, 0, 0);
+    const float alpha = alpha_of(m);
+    const F3 f0 = f0_of(m);
+    const float kd = 1.0f - m.metallic;
+    const F3 h = normalize(add(v, l));
+    const float noh = maxr(0.0f, dot(n, h));
+    const float loh = maxr(0.0f, dot(l, h));
+    const float c = clampr(1.0f - loh, 0.0f, 1.0f);
+    const float c5 = c * c * c * c * c;
+    const F3 F = f3(f0.x + (1.0f - f0.x) * c5, f0.y + (1.0f - f0.y) * c5, f0.z + (1.0f - f0.z) * c5);
+    const float D = ggx_d(alpha, noh);
+    const float G = g_correlated(alpha, nol, nov);
+    const float sden = maxr(4.0f * nov, 1e-7f);
+    const float dcos = nol / kPi;
+    return f3(F.x * D * G / sden + kd * (1.0f - F.x) * m.albedo.x * dcos,
+              F.y * D * G / sden + kd * (1.0f - F.y) * m.albedo.y * dcos,
+              F.z * D * G / sden + kd * (1.0f - F.z) * m.albedo.z * dcos);
+}
+
+// BSDF_PDF (core/render/material.cuh:316-371)
+__device__ __forceinline__ float bsdf_pdf(const Mat& m, F3 rd, F3 l, F3 n) {
+    const float nol = maxr(0.0f, dot(n, l));
+    if (m.model == AR_MODEL_LAMBERT) return nol > 0.0f ? nol * (1.0f / kPi) : 0.0f;
+    const F3 v = scale(rd, -1.0f);
+    const float nov = maxr(0.0f, dot(n, v));
+    if (nov <= 0.0f || nol <= 0.0f) return 0.0f;
+    const float alpha = alpha_of(m);
+    const float pspec = clampr(lum(f0_of(m)), 0.1f, 0.9f);
+    const float ps = vndf_pdf(v, n, l, alpha);
+    const float pd = nol * (1.0f / kPi);
+    return maxr(0.0f, pspec * ps + (1.0f - pspec) * pd);
+}
+
+// SampleBSDF (core/render/material.cuh:167-314); false ends the path
+__device__ __forceinline__ bool sample_bsdf(const Mat& m, F3 rd, F3 n, const float* u, F3& next, float& pdf) {
+    if (m.model == AR_MODEL_LAMBERT) {
+        next = sample_cosine(u[1], u[2], u[3], n, pdf);
+        return true;
+    }
+    const F3 v = scale(rd, -1.0f);
+    const float nov = maxr(0.0f, dot(n, v));
+    if (nov <= 0.0f) return false;
+    const float alpha = alpha_of(m);
+    const float pspec = clampr(lum(f0_of(m)), 0.1f, 0.9f);
+    F3 l;
+    float ps, pd;
+    if (u[0] < pspec) {
+        l = sample_vndf(v, n, alpha, u[1], u[2], u[3], ps);
+        const float nol = maxr(0.0f, dot(n, l));
+        if (nol <= 0.0f) return false;
+        pd = nol / kPi;
+    } else {
+        l = sample_cosine(u[1], u[2], u[3], n, pd);
+        ps = vndf_pdf(v, n, l, alpha);
+    }
+    next = l;
+    pdf = pspec * ps + (1.0f - pspec) * pd;
+    return true;
+}
+
+__device__ __forceinline__ float power_heuristic(float pa, float pb) {
+    const float a = pa * pa, b = pb * pb;
+    return a / maxr(1e-24f, a + b);
+}
+
+// SkyLighting (core/render/render.cuh:58-73)
+__device__ __forceinline__ F3 sky_gradient(F3 d) {
+    const F3 white = f3(1, 1, 1), blue = f3(0.05f, 0.45f, 1.0f);
+    const F3 down = scale(f3(0.2f, 0.6f, 1.0f), 0.125f);
+    const float t1 = powf(maxr(d.y, 0.0f), 0.75f);
+    const F3 a = add(blue, scale(sub(white, blue), t1));
+    const float t2 = minr(maxr(-d.y * 0.5f + 0.5f, 0.0f), 1.0f);
+    const F3 e = add(a, scale(sub(down, a), t2));
+    return scale(e, 0.5f);
+}
+
+struct RenderParams {
+    uint64_t seed;
+    uint32_t width, height, npix;
+    uint32_t sample_base;        // sample index of path slot group 0 in this pass
+    uint32_t capacity;           // paths in this pass
+    uint32_t max_depth;
+    uint32_t rr_min_depth;
+    uint32_t nee;
+    uint32_t sky;
+    float env[3];
+    float cam_pos[3];
+    float fwd[3], right[3], up[3];
+    float thx, thy;
+};
+
+struct QueueRays {
+    float *ox, *oy, *oz, *tmin, *dx, *dy, *dz, *tmax;
+};
+
+__device__ __forceinline__ QueueRays qrays(float* const* r) {
+    return {r[0], r[1], r[2], r[3], r[4], r[5], r[6], r[7]};
+}
+
+struct SceneArgs {
+    const float4* __restrict__ prims;
+    const GpuMaterial* __restrict__ mats;
+    const int32_t* __restrict__ light_tri;
+    const float* __restrict__ light_area;
+    const float* __restrict__ light_prob;
+    const float* __restrict__ light_cdf;
+    uint32_t nlights;
+};
+
+struct PathArgs {
+    float* mask0; float* mask1; float* mask2;
+    float* rad0; float* rad1; float* rad2;
+    float* lp0; float* lp1; float* lp2;
+    float* last_pdf;
+    uint32_t* meta;
+};
+
+// ---------------------------------------------------------------- kernels
+
+// RenderCamera's ray generation (core/scene.cu:135-164) for every path slot
+// of the pass: slot p = s_local * npix + pixel.
+__global__ __launch_bounds__(kBlock) void k_camera(RenderParams P, PathArgs S, QueueRays q, uint32_t* __restrict__ qpath,
+                                                   uint32_t* __restrict__ counters, const float* __restrict__ eps_table) {
+    const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+    if (p == 0) { counters[0] = P.capacity; counters[1] = 0; counters[2] = 0; }
+    if (p >= P.capacity) return;
+    const uint32_t pix = p % P.npix, sl = p / P.npix;
+    const uint32_t i = pix % P.width, j = pix / P.width;
+    float u[4];
+    uniforms(P.seed, pix, P.sample_base + sl, kCameraBounce, 0, u);
+    const float uu = ((float)i + u[0]) / (float)P.width;
+    const float vv = ((float)j + u[1]) / (float)P.height;
+    const float ndx = uu * 2.0f - 1.0f, ndy = (1.0f - vv) * 2.0f - 1.0f;
+    const F3 fwd = f3(P.fwd[0], P.fwd[1], P.fwd[2]);
+    const F3 right = f3(P.right[0], P.right[1], P.right[2]);
+    const F3 up = f3(P.up[0], P.up[1], P.up[2]);
+    F3 d = normalize(add(add(fwd, scale(right, ndx * P.thx)), scale(up, ndy * P.thy)));
+    d = normalize(d);   // CalculateRadiance normalises the incoming direction again
+    q.ox[p] = P.cam_pos[0]; q.oy[p] = P.cam_pos[1]; q.oz[p] = P.cam_pos[2];
+    q.tmin[p] = eps_table[0];
+    q.dx[p] = d.x; q.dy[p] = d.y; q.dz[p] = d.z;
+    q.tmax[p] = __int_as_float(0x7f800000);
+    qpath[p] = p;
+    S.mask0[p] = 1.0f; S.mask1[p] = 1.0f; S.mask2[p] = 1.0f;
+    S.rad0[p] = 0.0f; S.rad1[p] = 0.0f; S.rad2[p] = 0.0f;
+    S.last_pdf[p] = 0.0f;
+    S.lp0[p] = 0.0f; S.lp1[p] = 0.0f; S.lp2[p] = 0.0f;
+    S.meta[p] = 0u;
+}
+
+// One bounce of CalculateRadiance for every queued path: emission with MIS
+// (render.cuh:664-710), BSDF sampling and the hit-point offset
+// (render.cuh:75-163), NEE light sampling (render.cuh:205-343, 766-827;
+// the shadow segment is queued and resolved after the any-hit pass), Russian
+// roulette (render.cuh:838-859) and the next ray.
+__global__ __launch_bounds__(kBlock) void k_shade(
+    RenderParams P, uint32_t depth, SceneArgs sc, PathArgs S,
+    QueueRays cur, const uint32_t* __restrict__ qcur,
+    const uint32_t* __restrict__ hprim, const float* __restrict__ ht, const float* __restrict__ hu,
+    const float* __restrict__ hv,
+    QueueRays nxt, uint32_t* __restrict__ qnext,
+    QueueRays sh, uint32_t* __restrict__ spath, float* __restrict__ sc0, float* __restrict__ sc1,
+    float* __restrict__ sc2, uint32_t* __restrict__ sflags,
+    uint32_t* __restrict__ counters, const float* __restrict__ eps_table) {
+    const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t n = counters[0];
+    bool cont = false, shadow = false;
+    uint32_t p = 0;
+    F3 next_o = f3(0, 0, 0), next_d = f3(0, 0, 0);
+    float next_tmin = 0.0f;
+    F3 s_o = f3(0, 0, 0), s_d = f3(0, 0, 1), s_contrib = f3(0, 0, 0);
+    float s_tmax = 0.0f;
+    uint32_t s_flag = 0;
+    if (q < n) {
+        p = qcur[q];
+        const uint32_t pix = p % P.npix, sl = p / P.npix;
+        const uint32_t smp = P.sample_base + sl;
+        float u[12];
+        uniforms(P.seed, pix, smp, depth, 0, u);
+        uniforms(P.seed, pix, smp, depth, 1, u + 4);
+        uniforms(P.seed, pix, smp, depth, 2, u + 8);
+        const uint32_t meta = S.meta[p];
+        const uint32_t non_delta = meta & 0xFFFFu;
+        const bool prev_nee = (meta & kMetaNee) != 0;
+        F3 mask = f3(S.mask0[p], S.mask1[p], S.mask2[p]);
+        F3 rad = f3(S.rad0[p], S.rad1[p], S.rad2[p]);
+        const F3 o = f3(cur.ox[q], cur.oy[q], cur.oz[q]);
+        const F3 d = f3(cur.dx[q], cur.dy[q], cur.dz[q]);
+        const float eps = eps_table[non_delta < 255 ? non_delta : 255];
+        const uint32_t prim = hprim[q];
+        uint32_t new_meta = non_delta;
+        if (prim == AR_MISS) {
+            if (P.sky == AR_SKY_GRADIENT) rad = add(rad, mul(mask, sky_gradient(d)));
+            else if (P.sky == AR_SKY_CONSTANT) rad = add(rad, mul(mask, f3(P.env[0], P.env[1], P.env[2])));
+        } else {
+            const float4 a4 = sc.prims[3 * (size_t)prim + 0];
+            const float4 b4 = sc.prims[3 * (size_t)prim + 1];
+            const float4 c4 = sc.prims[3 * (size_t)prim + 2];
+            const F3 a = xyz(a4), b = xyz(b4), c = xyz(c4);
+            const uint32_t mat_id = __float_as_uint(a4.w);
+            const int light_id = __float_as_int(b4.w);
+            const float bu = hu[q], bv = hv[q];
+            const float bw = 1.0f - bu - bv;
+            F3 pos = add(add(scale(a, bw), scale(b, bu)), scale(c, bv));
+            const F3 ng = normalize(cross(sub(b, a), sub(c, a)));
+            const float ndd = dot(ng, d);
+            const F3 nn = ndd < 0.0f ? ng : scale(ng, -1.0f);
+            const Mat m = load_mat(sc.mats, mat_id);
+            if (light_id >= 0) {
+                if (ndd < 0.0f) {
+                    float w = 1.0f;
+                    if (P.nee && depth > 0 && prev_nee) {
+                        const F3 lp = f3(S.lp0[p], S.lp1[p], S.lp2[p]);
+                        const F3 tl = sub(pos, lp);
+                        const float dist2 = maxr(dot(tl, tl), 1e-20f);
+                        const F3 dir = scale(tl, 1.0f / sqrtf(dist2));
+                        const float cosy = maxr(0.0f, -dot(ng, dir));
+                        const float pa = sc.light_prob[light_id] / sc.light_area[light_id];
+                        const float pl = cosy <= 0.0f ? 0.0f : pa * (dist2 / cosy);
+                        const float last_pdf = S.last_pdf[p];
+                        if (pl >= 0.0f && last_pdf > 0.0f) w = power_heuristic(last_pdf, pl);
+                    }
+                    rad = add(rad, scale(mul(mask, m.emission), w));
+                }
+            } else {
+                F3 next;
+                float pdf;
+                if (sample_bsdf(m, d, nn, u, next, pdf)) {
+                    const F3 fcos = eval_bsdf(m, d, next, nn);
+                    const float off = eps * maxr(1.0f, length(pos)) * (u[4] + 0.5f);
+                    const float dn = dot(nn, next);
+                    const float sgn = dn > 0.0f ? 1.0f : (dn < 0.0f ? -1.0f : 0.0f);
+                    pos = add(pos, scale(nn, sgn * off));
+                    bool nee_now = false;
+                    if (P.nee && sc.nlights > 0 && depth + 1 < P.max_depth && non_delta < 10 &&
+                        u[5] < sqrtf(clampr(lum(mask), 0.0f, 1.0f))) {
+                        // pick by the CDF (first entry with u < cdf; binary search)
+                        uint32_t lo = 0, hi = sc.nlights - 1;
+                        while (lo < hi) {
+                            const uint32_t mid = (lo + hi) >> 1;
+                            if (u[6] < sc.light_cdf[mid]) hi = mid; else lo = mid + 1;
+                        }
+                        const uint32_t li = lo;
+                        const uint32_t lp = (uint32_t)sc.light_tri[li];
+                        const float4 la4 = sc.prims[3 * (size_t)lp + 0];
+                        const F3 la = xyz(la4), lb = xyz(sc.prims[3 * (size_t)lp + 1]), lc = xyz(sc.prims[3 * (size_t)lp + 2]);
+                        const float su = sqrtf(u[7]);
+                        const F3 qpt = add(add(scale(la, 1.0f - su), scale(lb, su * (1.0f - u[8]))), scale(lc, su * u[8]));
+                        const F3 ln = normalize(cross(sub(lb, la), sub(lc, la)));
+                        const Mat lm = load_mat(sc.mats, __float_as_uint(la4.w));
+                        const float pa_total = maxr(1e-20f, sc.light_prob[li] / sc.light_area[li]);
+                        const F3 lu = sub(qpt, pos);
+                        const float dist2 = maxr(dot(lu, lu), 1e-12f);
+                        const F3 ldir = scale(lu, 1.0f / sqrtf(dist2));
+                        const float dist = sqrtf(dot(lu, lu));
+                        const float cosx = maxr(0.0f, dot(nn, ldir));
+                        const float cosy = maxr(0.0f, -dot(ln, ldir));
+                        const bool faces = cosx > 0.0f && cosy > 0.0f;
+                        F3 contrib_w = f3(0, 0, 0);
+                        if (faces) {
+                            const F3 f = eval_bsdf(m, d, ldir, nn);
+                            const float g = cosy / dist2;
+                            const F3 contrib = scale(mul(f, lm.emission), g / pa_total);
+                            const float pl = pa_total * (dist2 / cosy);
+                            const float pb = maxr(bsdf_pdf(m, d, ldir, nn), 0.0f);
+                            const float w = power_heuristic(pl, pb);
+                            contrib_w = mul(mask, scale(contrib, w));
+                        }
+                        // AnyHitSegmentBVH (intersection.cuh:285-310)
+                        float tmx = 0.0f;
+                        F3 sdir = f3(0, 0, 1);
+                        if (dist > 0.0f) {
+                            sdir = scale(lu, 1.0f / dist);
+                            const float seps = eps_table[depth < 255 ? depth : 255] *
+                                               maxr(1.0f, maxr(length(pos), length(qpt)));
+                            tmx = maxr(0.0f, dist - seps);
+                        }
+                        if (tmx > 0.0f) {
+                            shadow = true;
+                            s_o = pos; s_d = sdir; s_tmax = tmx; s_contrib = contrib_w; s_flag = faces ? 1u : 0u;
+                        } else {
+                            // nothing to test: visible
+                            if (faces) { rad = add(rad, contrib_w); nee_now = true; }
+                        }
+                    }
+                    // the resolve pass sets the NEE bit for queued shadow segments
+                    new_meta = non_delta | (nee_now ? kMetaNee : 0u);
+                    S.last_pdf[p] = pdf;
+                    S.lp0[p] = pos.x; S.lp1[p] = pos.y; S.lp2[p] = pos.z;
+                    const float inv_pdf = 1.0f / maxr(pdf, 1e-12f);
+                    const F3 contrib = scale(fcos, inv_pdf);
+                    float qrr = 1.0f;
+                    bool alive = true;
+                    if (non_delta >= P.rr_min_depth) {
+                        qrr = clampr(lum(mask) * 0.5f + max3(mask) * 0.5f, 0.05f, 0.95f);
+                        if (u[9] > qrr) alive = false;
+                    }
+                    if (alive && depth + 1 < P.max_depth) {
+                        mask = scale(mul(mask, contrib), 1.0f / qrr);
+                        cont = true;
+                        next_o = pos;
+                        next_d = next;
+                        const uint32_t nd = non_delta + 1;
+                        next_tmin = eps_table[nd < 255 ? nd : 255];
+                        new_meta = (new_meta & kMetaNee) | nd;
+                    }
+                }
+            }
+        }
+        S.mask0[p] = mask.x; S.mask1[p] = mask.y; S.mask2[p] = mask.z;
+        S.rad0[p] = rad.x; S.rad1[p] = rad.y; S.rad2[p] = rad.z;
+        S.meta[p] = new_meta;
+    }
+    const uint32_t slot = wave_append(cont, &counters[1]);
+    if (cont) {
+        nxt.ox[slot] = next_o.x; nxt.oy[slot] = next_o.y; nxt.oz[slot] = next_o.z;
+        nxt.tmin[slot] = next_tmin;
+        nxt.dx[slot] = next_d.x; nxt.dy[slot] = next_d.y; nxt.dz[slot] = next_d.z;
+        nxt.tmax[slot] = __int_as_float(0x7f800000);
+        qnext[slot] = p;
+    }
+    const uint32_t ss = wave_append(shadow, &counters[2]);
+    if (shadow) {
+        sh.ox[ss] = s_o.x; sh.oy[ss] = s_o.y; sh.oz[ss] = s_o.z; sh.tmin[ss] = 0.0f;
+        sh.dx[ss] = s_d.x; sh.dy[ss] = s_d.y; sh.dz[ss] = s_d.z; sh.tmax[ss] = s_tmax;
+        spath[ss] = p;
+        sc0[ss] = s_contrib.x; sc1[ss] = s_contrib.y; sc2[ss] = s_contrib.z;
+        sflags[ss] = s_flag;
+    }
+}
+
+// Shadow results: an occluded segment still counts as a taken NEE sample
+// (render.cuh:297-305); an unoccluded one adds its MIS-weighted
+// contribution when the light point faced the surface.
+__global__ __launch_bounds__(kBlock) void k_resolve(PathArgs S, const uint32_t* __restrict__ spath,
+                                                    const float* __restrict__ sc0, const float* __restrict__ sc1,
+                                                    const float* __restrict__ sc2, const uint32_t* __restrict__ sflags,
+                                                    const uint32_t* __restrict__ occ,
+                                                    const uint32_t* __restrict__ counters) {
+    const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+    if (s >= counters[2]) return;
+    const uint32_t p = spath[s];
+    const bool occluded = (occ[s >> 5] >> (s & 31u)) & 1u;
+    const bool faces = sflags[s] & 1u;
+    if (occluded || faces) S.meta[p] |= kMetaNee;
+    if (!occluded && faces) {
+        S.rad0[p] = S.rad0[p] + sc0[s];
+        S.rad1[p] = S.rad1[p] + sc1[s];
+        S.rad2[p] = S.rad2[p] + sc2[s];
+    }
+}
+
+// Next bounce: the appended queue becomes the current one.
+__global__ void k_advance(uint32_t* counters) {
+    counters[0] = counters[1];
+    counters[1] = 0;
+    counters[2] = 0;
+}
+
+// Per-pixel sum over the pass's samples in sample order, then the mean.
+__global__ __launch_bounds__(kBlock) void k_accumulate(RenderParams P, PathArgs S, float* __restrict__ accum,
+                                                       uint32_t spp_pass, bool first, bool last, float spp_total,
+                                                       float* __restrict__ out) {
+    const uint32_t pix = blockIdx.x * kBlock + threadIdx.x;
+    if (pix >= P.npix) return;
+    float ax = first ? 0.0f : accum[3 * pix + 0];
+    float ay = first ? 0.0f : accum[3 * pix + 1];
+    float az = first ? 0.0f : accum[3 * pix + 2];
+    for (uint32_t s = 0; s < spp_pass; ++s) {
+        const uint32_t p = s * P.npix + pix;
+        const float rx = clampr(fixnan(S.rad0[p]), 0.0f, 1e3f);
+        const float ry = clampr(fixnan(S.rad1[p]), 0.0f, 1e3f);
+        const float rz = clampr(fixnan(S.rad2[p]), 0.0f, 1e3f);
+        ax = ax + rx; ay = ay + ry; az = az + rz;
+    }
+    if (last) {
+        out[3 * pix + 0] = ax / spp_total;
+        out[3 * pix + 1] = ay / spp_total;
+        out[3 * pix + 2] = az / spp_total;
+    } else {
+        accum[3 * pix + 0] = ax; accum[3 * pix + 1] = ay; accum[3 * pix + 2] = az;
+    }
+}
+
+uint32_t grid_for(size_t n) { return (uint32_t)((n + kBlock - 1) / kBlock); }
+
+} // namespace
+
+// ---------------------------------------------------------------- host side
+
+void free_scene(ar_ctx* ctx) {
+    SceneDev& s = ctx->scene;
+    for (void* ptr : {(void*)s.prims, (void*)s.mats, (void*)s.light_tri, (void*)s.light_area, (void*)s.light_prob,
+                      (void*)s.light_cdf})
+        if (ptr) (void)hipFree(ptr);
+    s = SceneDev{};
+}
+
+void free_render_buffers(ar_ctx* ctx) {
+    if (ctx->rb.block) (void)hipFree(ctx->rb.block);
+    if (ctx->rb.accum) (void)hipFree(ctx->rb.accum);
+    if (ctx->rb.eps_table) (void)hipFree(ctx->rb.eps_table);
+    ctx->rb = RenderBuffers{};
+}
+
+static float tri_area_host(const float* a, const float* b, const float* c) {
+    const float e1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+    const float e2[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+    const float cx = e1[1] * e2[2] - e1[2] * e2[1];
+    const float cy = e1[2] * e2[0] - e1[0] * e2[2];
+    const float cz = e1[0] * e2[1] - e1[1] * e2[0];
+    return 0.5f * std::sqrt(cx * cx + cy * cy + cz * cz);
+}
+
+// Uploads prim-order triangles, materials and the light table (the light
+// selection CDF of core/scene.cu:677-701: w = max(area * lum(Le), 1e-20),
+// prob = w / sum(w), cdf = running FP32 sum of prob).
+int upload_scene(ar_ctx* ctx) {
+    free_scene(ctx);
+    SceneDev& s = ctx->scene;
+    const uint32_t n = (uint32_t)(ctx->idx.size() / 3);
+    s.ntri = n;
+    std::vector<float> prims(12 * (size_t)std::max(n, 1u), 0.0f);
+    std::vector<int32_t> ltri;
+    std::vector<float> larea, lprob, lcdf;
+    float total = 0.0f;
+    for (uint32_t t = 0; t < n; ++t) {
+        const float* v[3];
+        for (int k = 0; k < 3; ++k) v[k] = &ctx->verts[3 * (size_t)ctx->idx[3 * (size_t)t + k]];
+        const ar_material& m = ctx->mats[ctx->tri_mat[t]];
+        int32_t light = -1;
+        if (m.emission[0] > 0.0f || m.emission[1] > 0.0f || m.emission[2] > 0.0f) {
+            const float area = tri_area_host(v[0], v[1], v[2]);
+            const float l = m.emission[0] * 0.2126f + m.emission[1] * 0.7152f + m.emission[2] * 0.0722f;
+            const float w = std::max(area * l, 1.0e-20f);
+            light = (int32_t)ltri.size();
+            ltri.push_back((int32_t)t);
+            larea.push_back(area);
+            lprob.push_back(w);
+            total += w;
+        }
+        for (int k = 0; k < 3; ++k) {
+            float* dst = &prims[12 * (size_t)t + 4 * k];
+            dst[0] = v[k][0]; dst[1] = v[k][1]; dst[2] = v[k][2];
+        }
+        std::memcpy(&prims[12 * (size_t)t + 3], &ctx->tri_mat[t], 4);
+        std::memcpy(&prims[12 * (size_t)t + 7], &light, 4);
+    }
+    float acc = 0.0f;
+    for (size_t i = 0; i < lprob.size(); ++i) {
+        lprob[i] = lprob[i] / total;
+        acc += lprob[i];
+        lcdf.push_back(acc);
+    }
+    std::vector<GpuMaterial> mats(ctx->mats.size());
+    for (size_t i = 0; i < mats.size(); ++i) {
+        const ar_material& m = ctx->mats[i];
+        GpuMaterial g{};
+        for (int k = 0; k < 3; ++k) { g.albedo[k] = m.albedo[k]; g.emission[k] = m.emission[k]; }
+        g.specular = m.specular; g.metallic = m.metallic; g.roughness = m.roughness; g.model = m.model;
+        mats[i] = g;
+    }
+    s.nlights = (uint32_t)ltri.size();
+    auto up = [&](void** dst, const void* src, size_t bytes, const char* what) -> int {
+        if (bytes == 0) bytes = 16;
+        int rc = check_hip(ctx, hipMalloc(dst, bytes), what);
+        if (rc != AR_OK) { *dst = nullptr; return rc; }
+        if (src) rc = check_hip(ctx, hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice), what);
+        return rc;
+    };
+    int rc = up((void**)&s.prims, prims.data(), prims.size() * sizeof(float), "upload prims");
+    if (rc == AR_OK) rc = up((void**)&s.mats, mats.data(), mats.size() * sizeof(GpuMaterial), "upload materials");
+    if (rc == AR_OK) rc = up((void**)&s.light_tri, ltri.empty() ? nullptr : ltri.data(), ltri.size() * 4, "upload lights");
+    if (rc == AR_OK) rc = up((void**)&s.light_area, larea.empty() ? nullptr : larea.data(), larea.size() * 4, "upload lights");
+    if (rc == AR_OK) rc = up((void**)&s.light_prob, lprob.empty() ? nullptr : lprob.data(), lprob.size() * 4, "upload lights");
+    if (rc == AR_OK) rc = up((void**)&s.light_cdf, lcdf.empty() ? nullptr : lcdf.data(), lcdf.size() * 4, "upload lights");
+    if (rc != AR_OK) free_scene(ctx);
+    return rc;
+}
+
+static int ensure_render_buffers(ar_ctx* ctx, size_t cap, size_t npix) {
+    RenderBuffers& b = ctx->rb;
+    if (!b.eps_table) {
+        float eps[256];
+        for (int d = 0; d < 256; ++d) eps[d] = 1.0e-3f * std::exp(std::min((float)d * 0.5f, 4.0f));  // INTERSECT_EPS
+        int rc = check_hip(ctx, hipMalloc((void**)&b.eps_table, sizeof eps), "hipMalloc(eps)");
+        if (rc != AR_OK) { b.eps_table = nullptr; return rc; }
+        rc = check_hip(ctx, hipMemcpy(b.eps_table, eps, sizeof eps, hipMemcpyHostToDevice), "upload eps");
+        if (rc != AR_OK) return rc;
+    }
+    if (b.accum_pixels < npix) {
+        if (b.accum) (void)hipFree(b.accum);
+        b.accum = nullptr;
+        int rc = check_hip(ctx, hipMalloc((void**)&b.accum, npix * 3 * sizeof(float)), "hipMalloc(accum)");
+        if (rc != AR_OK) { b.accum_pixels = 0; return rc; }
+        b.accum_pixels = npix;
+    }
+    if (b.capacity >= cap) return AR_OK;
+    if (b.block) (void)hipFree(b.block);
+    b.block = nullptr;
+    b.capacity = 0;
+    // 4-byte arrays of cap entries: 2x8 queue rays, 2 qpath, 5 hits, 8 shadow rays,
+    // spath, 3 scontrib, sflags, 3 mask, 3 rad, 3 last_pos, last_pdf, meta = 47
+    const size_t arrays = 47;
+    const size_t occ_words = (cap + 31) / 32 + 64;
+    const size_t bytes = arrays * cap * 4 + occ_words * 4 + 64;
+    int rc = check_hip(ctx, hipMalloc(&b.block, bytes), "hipMalloc(render state)");
+    if (rc != AR_OK) { b.block = nullptr; return rc; }
+    char* base = (char*)b.block;
+    auto take = [&](size_t count) { void* r = base; base += count * 4; return r; };
+    for (int k = 0; k < 2; ++k)
+        for (int f = 0; f < 8; ++f) b.rays[k][f] = (float*)take(cap);
+    b.qpath[0] = (uint32_t*)take(cap);
+    b.qpath[1] = (uint32_t*)take(cap);
+    b.hit_prim = (uint32_t*)take(cap);
+    b.hit_t = (float*)take(cap);
+    b.hit_u = (float*)take(cap);
+    b.hit_v = (float*)take(cap);
+    b.hit_flags = (uint32_t*)take(cap);
+    for (int f = 0; f < 8; ++f) b.srays[f] = (float*)take(cap);
+    b.spath = (uint32_t*)take(cap);
+    for (int k = 0; k < 3; ++k) b.scontrib[k] = (float*)take(cap);
+    b.sflags = (uint32_t*)take(cap);
+    for (int k = 0; k < 3; ++k) b.mask[k] = (float*)take(cap);
+    for (int k = 0; k < 3; ++k) b.rad[k] = (float*)take(cap);
+    for (int k = 0; k < 3; ++k) b.last_pos[k] = (float*)take(cap);
+    b.last_pdf = (float*)take(cap);
+    b.meta = (uint32_t*)take(cap);
+    b.occ = (uint32_t*)take(occ_words);
+    b.counters = (uint32_t*)take(16);
+    b.capacity = cap;
+    return AR_OK;
+}
+
+int render(ar_ctx* ctx, const ar_camera* cam, const ar_render_opts* opts, float* out, hipStream_t stream) {
+    (void)hipSetDevice(ctx->device);
+    const size_t npix = (size_t)opts->width * opts->height;
+    const size_t max_paths = opts->max_paths_per_pass ? opts->max_paths_per_pass : ((size_t)1 << 22);
+    if (npix > max_paths && npix > ((size_t)1 << 26))
+        return set_error(ctx, AR_E_CAPACITY, "image larger than 2^26 pixels");
+    const uint32_t spp_pass = (uint32_t)std::max<size_t>(1, std::min<size_t>(opts->spp, max_paths / std::max<size_t>(npix, 1)));
+    const size_t cap = npix * spp_pass;
+    int rc = ensure_render_buffers(ctx, cap, npix);
+    if (rc != AR_OK) return rc;
+    TraceArgs ta;
+    uint32_t tblocks;
+    rc = trace_prologue(ctx, ta, cap, tblocks);
+    if (rc != AR_OK) return rc;
+
+    RenderParams P{};
+    P.seed = opts->seed;
+    P.width = opts->width;
+    P.height = opts->height;
+    P.npix = (uint32_t)npix;
+    P.max_depth = opts->max_depth;
+    P.rr_min_depth = opts->rr_min_depth;
+    P.nee = opts->nee;
+    P.sky = opts->sky;
+    for (int k = 0; k < 3; ++k) { P.env[k] = opts->env[k]; P.cam_pos[k] = cam->pos[k]; P.fwd[k] = cam->dir[k]; }
+    // camera basis on the host, as the reference computes it per pixel
+    // (core/scene.cu:154-164): FOV = vfov * 3.1415926 / 180
+    {
+        const float fx = cam->dir[0], fy = cam->dir[1], fz = cam->dir[2];
+        float rx = fy * 0.0f - fz * 1.0f, ry = fz * 0.0f - fx * 0.0f, rz = fx * 1.0f - fy * 0.0f;
+        float l2 = rx * rx + ry * ry + rz * rz;
+        float inv = l2 <= 1e-20f ? 0.0f : 1.0f / std::sqrt(l2);
+        rx *= inv; ry *= inv; rz *= inv;
+        if (l2 <= 1e-20f) { rx = ry = rz = 0.0f; }
+        float ux = fy * rz - fz * ry, uy = fz * rx - fx * rz, uz = fx * ry - fy * rx;
+        l2 = ux * ux + uy * uy + uz * uz;
+        inv = l2 <= 1e-20f ? 0.0f : 1.0f / std::sqrt(l2);
+        ux *= inv; uy *= inv; uz *= inv;
+        P.right[0] = rx; P.right[1] = ry; P.right[2] = rz;
+        P.up[0] = ux; P.up[1] = uy; P.up[2] = uz;
+        const float fov = cam->vfov_deg * 3.1415926f / 180.0f;
+        const float aspect = (float)opts->width / std::max(1.0f, (float)opts->height);
+        P.thy = std::tan(0.5f * fov);
+        P.thx = P.thy * aspect;
+    }
+    RenderBuffers& b = ctx->rb;
+    PathArgs S{b.mask[0], b.mask[1], b.mask[2], b.rad[0], b.rad[1], b.rad[2],
+               b.last_pos[0], b.last_pos[1], b.last_pos[2], b.last_pdf, b.meta};
+    SceneArgs sc{ctx->scene.prims, ctx->scene.mats, ctx->scene.light_tri, ctx->scene.light_area,
+                 ctx->scene.light_prob, ctx->scene.light_cdf, ctx->scene.nlights};
+    QueueRays q[2] = {{b.rays[0][0], b.rays[0][1], b.rays[0][2], b.rays[0][3], b.rays[0][4], b.rays[0][5], b.rays[0][6], b.rays[0][7]},
+                      {b.rays[1][0], b.rays[1][1], b.rays[1][2], b.rays[1][3], b.rays[1][4], b.rays[1][5], b.rays[1][6], b.rays[1][7]}};
+    QueueRays shq{b.srays[0], b.srays[1], b.srays[2], b.srays[3], b.srays[4], b.srays[5], b.srays[6], b.srays[7]};
+    ar_hits hits{b.hit_prim, b.hit_t, b.hit_u, b.hit_v, b.hit_flags};
+    const float spp_total = (float)opts->spp;
+    for (uint32_t done = 0; done < opts->spp;) {
+        const uint32_t this_spp = std::min(spp_pass, opts->spp - done);
+        P.sample_base = opts->sample_offset + done;
+        P.capacity = (uint32_t)(npix * this_spp);
+        hipLaunchKernelGGL(k_camera, dim3(grid_for(P.capacity)), dim3(kBlock), 0, stream, P, S, q[0], b.qpath[0],
+                           b.counters, b.eps_table);
+        int c = 0;
+        for (uint32_t depth = 0; depth < opts->max_depth; ++depth) {
+            ar_rays rr{q[c].ox, q[c].oy, q[c].oz, q[c].tmin, q[c].dx, q[c].dy, q[c].dz, q[c].tmax};
+            hipError_t e = launch_trace_closest(ta, rr, P.capacity, hits, nullptr, nullptr, tblocks, stream, &b.counters[0]);
+            if (e != hipSuccess) return check_hip(ctx, e, "integrator trace");
+            hipLaunchKernelGGL(k_shade, dim3(grid_for(P.capacity)), dim3(kBlock), 0, stream, P, depth, sc, S, q[c],
+                               b.qpath[c], b.hit_prim, b.hit_t, b.hit_u, b.hit_v, q[1 - c], b.qpath[1 - c], shq,
+                               b.spath, b.scontrib[0], b.scontrib[1], b.scontrib[2], b.sflags, b.counters, b.eps_table);
+            if (opts->nee) {
+                ar_rays sr{shq.ox, shq.oy, shq.oz, shq.tmin, shq.dx, shq.dy, shq.dz, shq.tmax};
+                e = launch_trace_any(ta, sr, P.capacity, b.occ, tblocks, stream, &b.counters[2]);
+                if (e != hipSuccess) return check_hip(ctx, e, "integrator shadow trace");
+                hipLaunchKernelGGL(k_resolve, dim3(grid_for(P.capacity)), dim3(kBlock), 0, stream, S, b.spath,
+                                   b.scontrib[0], b.scontrib[1], b.scontrib[2], b.sflags, b.occ, b.counters);
+            }
+            hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, stream, b.counters);
+            c = 1 - c;
+        }
+        done += this_spp;
+        hipLaunchKernelGGL(k_accumulate, dim3(grid_for(npix)), dim3(kBlock), 0, stream, P, S, b.accum, this_spp,
+                           P.sample_base == opts->sample_offset, done >= opts->spp, spp_total, out);
+    }
+    return check_hip(ctx, hipGetLastError(), "integrator launch");
+}
+
+} // namespace ar

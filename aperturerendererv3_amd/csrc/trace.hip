@@ -215,9 +215,10 @@ __device__ __forceinline__ uint32_t traverse(const GpuNode2* __restrict__ nodes,
 template <bool COUNT>
 __global__ __launch_bounds__(kTraceBlock) void k_trace_closest(
     const GpuNode2* __restrict__ nodes, const float4* __restrict__ tris, uint32_t root, float origin_limit,
-    ar_rays r, size_t n, ar_hits h, int* __restrict__ spill, uint32_t threads,
+    ar_rays r, size_t n, const uint32_t* __restrict__ n_dev, ar_hits h, int* __restrict__ spill, uint32_t threads,
     uint32_t* __restrict__ cnt_box, uint32_t* __restrict__ cnt_tri) {
     __shared__ int lds_stack[kLdsStack * kTraceBlock];
+    if (n_dev) n = *n_dev;
     const uint32_t gtid = blockIdx.x * kTraceBlock + threadIdx.x;
     Stack st;
     st.lds = lds_stack + threadIdx.x;
@@ -252,8 +253,10 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_closest(
 // multiple of 64), so the wave's ballot is exactly two occlusion words.
 __global__ __launch_bounds__(kTraceBlock) void k_trace_any(
     const GpuNode2* __restrict__ nodes, const float4* __restrict__ tris, uint32_t root, float origin_limit,
-    ar_rays r, size_t n, uint32_t* __restrict__ occ, int* __restrict__ spill, uint32_t threads) {
+    ar_rays r, size_t n, const uint32_t* __restrict__ n_dev, uint32_t* __restrict__ occ, int* __restrict__ spill,
+    uint32_t threads) {
     __shared__ int lds_stack[kLdsStack * kTraceBlock];
+    if (n_dev) n = *n_dev;
     const uint32_t gtid = blockIdx.x * kTraceBlock + threadIdx.x;
     const uint32_t lane = threadIdx.x & 63u;
     Stack st;
@@ -322,21 +325,22 @@ int trace_blocks_per_cu() {
 }
 
 hipError_t launch_trace_closest(const TraceArgs& ta, const ar_rays& r, size_t n, const ar_hits& h,
-                                uint32_t* cnt_box, uint32_t* cnt_tri, uint32_t blocks, hipStream_t s) {
+                                uint32_t* cnt_box, uint32_t* cnt_tri, uint32_t blocks, hipStream_t s,
+                                const uint32_t* n_dev) {
     if (cnt_box) {
         hipLaunchKernelGGL(k_trace_closest<true>, dim3(blocks), dim3(kTraceBlock), 0, s, ta.nodes, ta.tris, ta.root,
-                           ta.origin_limit, r, n, h, ta.spill, ta.threads, cnt_box, cnt_tri);
+                           ta.origin_limit, r, n, n_dev, h, ta.spill, ta.threads, cnt_box, cnt_tri);
     } else {
         hipLaunchKernelGGL(k_trace_closest<false>, dim3(blocks), dim3(kTraceBlock), 0, s, ta.nodes, ta.tris, ta.root,
-                           ta.origin_limit, r, n, h, ta.spill, ta.threads, nullptr, nullptr);
+                           ta.origin_limit, r, n, n_dev, h, ta.spill, ta.threads, nullptr, nullptr);
     }
     return hipGetLastError();
 }
 
 hipError_t launch_trace_any(const TraceArgs& ta, const ar_rays& r, size_t n, uint32_t* occ, uint32_t blocks,
-                            hipStream_t s) {
+                            hipStream_t s, const uint32_t* n_dev) {
     hipLaunchKernelGGL(k_trace_any, dim3(blocks), dim3(kTraceBlock), 0, s, ta.nodes, ta.tris, ta.root,
-                       ta.origin_limit, r, n, occ, ta.spill, ta.threads);
+                       ta.origin_limit, r, n, n_dev, occ, ta.spill, ta.threads);
     return hipGetLastError();
 }
 

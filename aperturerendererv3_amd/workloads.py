@@ -157,3 +157,63 @@ def traversal_rays(V, F, res: int = 4096, seed: int = 1, cam_pos=(0.0, 0.0, 3.0)
     rays[3] = 0.0
     rays[7] = np.inf
     return rays
+
+
+# --------------------------------------------------------------------------
+# Cornell-box-like scene of 32 triangles (the plain-integrator workload)
+# --------------------------------------------------------------------------
+
+CORNELL_CAMERA = {"pos": (0.0, 0.0, 1.1), "dir": (0.0, 0.0, -1.0), "vfov": 60.0}
+
+
+def _quad(V, F, M, corners, mat):
+    """Append a quad (4 corners, counter-clockwise seen from the side the
+    normal (b-a)x(c-a) points to) as two triangles."""
+    base = len(V)
+    V.extend(corners)
+    F.extend([[base, base + 1, base + 2], [base, base + 2, base + 3]])
+    M.extend([mat, mat])
+
+
+def _box(V, F, M, center, size, yrot_deg, mat):
+    """Five faces (no bottom) of an axis-aligned box rotated about +Y,
+    outward winding."""
+    hx, hy, hz = (s * 0.5 for s in size)
+    c, s = math.cos(math.radians(yrot_deg)), math.sin(math.radians(yrot_deg))
+
+    def P(x, y, z):
+        return (center[0] + c * x + s * z, center[1] + y, center[2] - s * x + c * z)
+    # top (+y), then +x, -x, +z, -z sides, each counter-clockwise from outside
+    _quad(V, F, M, [P(-hx, hy, -hz), P(-hx, hy, hz), P(hx, hy, hz), P(hx, hy, -hz)], mat)
+    _quad(V, F, M, [P(hx, -hy, -hz), P(hx, hy, -hz), P(hx, hy, hz), P(hx, -hy, hz)], mat)
+    _quad(V, F, M, [P(-hx, -hy, -hz), P(-hx, -hy, hz), P(-hx, hy, hz), P(-hx, hy, -hz)], mat)
+    _quad(V, F, M, [P(-hx, -hy, hz), P(hx, -hy, hz), P(hx, hy, hz), P(-hx, hy, hz)], mat)
+    _quad(V, F, M, [P(-hx, -hy, -hz), P(-hx, hy, -hz), P(hx, hy, -hz), P(hx, -hy, -hz)], mat)
+
+
+def cornell_box(light_le=(17.0, 12.0, 4.0)):
+    """Unit box [-0.5, 0.5]^3 open at +z: floor, ceiling and back wall grey 0.75,
+    left wall red (0.63, 0.065, 0.05), right wall green (0.14, 0.45, 0.09); a tall
+    0.3x0.6x0.3 box turned +17 degrees and a 0.3^3 box turned -17 degrees (five
+    faces each, grey 0.75); a 0.25 x 0.25 light quad facing down just below the
+    ceiling.  All Lambert.  Returns (V f32 (n,3), F u32 (32,3), tri_material u32,
+    materials as dicts of ar_material fields)."""
+    V, F, M = [], [], []
+    g, r, gr, lt = 0, 1, 2, 3
+    h = 0.5
+    _quad(V, F, M, [(-h, -h, -h), (-h, -h, h), (h, -h, h), (h, -h, -h)], g)      # floor, +y
+    _quad(V, F, M, [(-h, h, -h), (h, h, -h), (h, h, h), (-h, h, h)], g)          # ceiling, -y
+    _quad(V, F, M, [(-h, -h, -h), (h, -h, -h), (h, h, -h), (-h, h, -h)], g)      # back, +z
+    _quad(V, F, M, [(-h, -h, -h), (-h, h, -h), (-h, h, h), (-h, -h, h)], r)      # left, +x
+    _quad(V, F, M, [(h, -h, -h), (h, -h, h), (h, h, h), (h, h, -h)], gr)         # right, -x
+    _box(V, F, M, (-0.16, -0.2, -0.14), (0.3, 0.6, 0.3), 17.0, g)
+    _box(V, F, M, (0.17, -0.35, 0.12), (0.3, 0.3, 0.3), -17.0, g)
+    ly, l = h - 1e-3, 0.125
+    _quad(V, F, M, [(-l, ly, -l), (l, ly, -l), (l, ly, l), (-l, ly, l)], lt)      # light, -y
+    mats = [
+        {"albedo": (0.75, 0.75, 0.75)},
+        {"albedo": (0.63, 0.065, 0.05)},
+        {"albedo": (0.14, 0.45, 0.09)},
+        {"albedo": (0.0, 0.0, 0.0), "emission": tuple(light_le)},
+    ]
+    return (np.asarray(V, np.float32), np.asarray(F, np.uint32), np.asarray(M, np.uint32), mats)

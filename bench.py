@@ -86,6 +86,49 @@ def cpu_baseline(V, F, rays_np, stride, shadow_np):
                       f"12 bins, leaf 8; sorted 128-entry stack; strict slab test), OpenMP, {dt:.2f} s"}
 
 
+def integrator_bench(ctx_cls, reps=5, cpu=True):
+    """Plain integrator on the Cornell-box workload (32 triangles, 256x256,
+    16 spp, max depth 5, NEE on, seed 1): GPU paths/s through ar_render, the
+    oracle's FP32 integrator on the host cores on the same frame, and the
+    relMSE between the two images (same seed: same paths)."""
+    import torch
+    from aperturerendererv3_amd.workloads import CORNELL_CAMERA, cornell_box
+    V, F, M, mats = cornell_box()
+    cam = CORNELL_CAMERA
+    kw = dict(cam_pos=cam["pos"], cam_dir=cam["dir"], vfov=cam["vfov"], seed=1, max_depth=5)
+    w = h = 256
+    spp = 16
+    with ctx_cls(0) as ctx:
+        ctx.set_meshes([{"vertices": V, "indices": F, "tri_material": M, "materials": mats}])
+        ctx.build_bvh()
+        out = torch.empty((h, w, 3), dtype=torch.float32, device="cuda:0")
+        for _ in range(2):
+            ctx.render(w, h, spp, out=out, **kw)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            ctx.render(w, h, spp, out=out, **kw)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        img = out.cpu().numpy()
+    paths = w * h * spp
+    res = {"workload": "Cornell box, 32 triangles, 256x256, 16 spp, max depth 5, NEE on, seed 1",
+           "gpu_ms_per_frame": round(ms, 4), "gpu_paths_per_s": round(paths / (ms * 1e-3), 1)}
+    if cpu:
+        sys.path.insert(0, str(ROOT / "tests"))
+        import oracle_lib as orc
+        cores = len(os.sched_getaffinity(0))
+        t0 = time.perf_counter()
+        ref = orc.render(V, F, M, mats, w, h, spp, precision=32, threads=cores, **kw)
+        dt = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": round(paths / dt, 1), "unit": "paths/s", "cores": cores, "kind": "port",
+                               "sample": f"the full frame ({paths} paths) with the oracle FP32 integrator, {dt:.2f} s"}
+        res["relmse_vs_oracle_same_seed"] = orc.rel_mse(img, ref)
+    return res
+
+
 def main():
     args = parse_args()
     if args.gpus != 1:
@@ -197,6 +240,7 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(V, F, rays_np, args.cpu_sample_stride, shadow.cpu().numpy())
+    integ = integrator_bench(Context, cpu=not args.no_cpu_baseline)
 
     valu_frac = achieved_tops / VALU_PEAK_TOPS
     out = {
@@ -217,6 +261,7 @@ def main():
                      "ops_per_launch": ops, "avg_launch_ms": round(closest_ms, 4)},
         "cpu_baseline": cpu,
         "breakdown": {k: round(v, 4) for k, v in breakdown.items()},
+        "integrator": integ,
     }
     print(json.dumps(out), flush=True)
     ctx.close()

@@ -162,6 +162,49 @@ int ar_make_shadow_rays(ar_ctx* ctx, const ar_rays* rays, const ar_hits* hits, s
                         const float target[3], float tmin, float tmax_scale,
                         const ar_rays* out, void* stream);
 
+/* --- path integrator ------------------------------------------------------
+ * Replaces SceneRenderer::Render (core/scene.hpp:104-110, core/scene.cu:
+ * 210-238) with its RenderCamera kernel (core/scene.cu:135-189) and the
+ * CalculateRadiance integrator (core/render/render.cuh:460-998), for the
+ * triangle scene given to ar_scene_set (the BVH must be built).
+ *
+ * Camera: the reference convention (core/scene.cu:145-164): right =
+ * normalize(dir x +Y), up = normalize(dir x right), pixel (i, j) jittered
+ * and mapped to ndc ((u*2-1) * tan(fov/2) * aspect, ((1-v)*2-1) * tan(fov/2)).
+ * Row j = 0 therefore looks along -up; the reference's BMP writer stores it
+ * as the bottom row.
+ *
+ * Random numbers: Philox4x32-10, key = seed, counter = (pixel, sample,
+ * bounce, group), so a render is a pure function of its options and
+ * sample_offset makes progressive rendering reproducible. */
+typedef struct {
+    float pos[3];
+    float dir[3];
+    float vfov_deg;       /* 60 in the reference */
+} ar_camera;
+
+#define AR_SKY_NONE 0u
+#define AR_SKY_GRADIENT 1u    /* SkyLighting, core/render/render.cuh:58-73 */
+#define AR_SKY_CONSTANT 2u    /* env[] radiance in every direction */
+
+typedef struct {
+    uint32_t width, height;
+    uint32_t spp;
+    uint32_t sample_offset;       /* index of the first sample */
+    uint64_t seed;
+    uint32_t max_depth;           /* bounce-loop cap (MAX_TRACING_DEPTH = 1000 in the reference) */
+    uint32_t rr_min_depth;        /* Russian roulette from this non-delta depth (3) */
+    uint32_t nee;                 /* next-event estimation on/off */
+    uint32_t sky;                 /* AR_SKY_* */
+    float env[3];
+    uint32_t max_paths_per_pass;  /* 0: 4,194,304 */
+} ar_render_opts;
+
+/* out_rgb: device array of width*height*3 floats (pixel j*width + i),
+ * receives the mean over spp of clamp(checkNanInf(L), 0, 1e3) per channel.
+ * Queued on `stream`; the call does not wait. */
+int ar_render(ar_ctx* ctx, const ar_camera* cam, const ar_render_opts* opts, float* out_rgb, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -99,6 +99,76 @@ void orc_trace_any_brute(const float* verts, const uint32_t* idx, int ntri,
 /* Maximum depth (root = 0) of a built tree. */
 int orc_bvh_depth(const orc_node* nodes, int node_count);
 
+/* Single-ray closest / any hit (canonical rule), for the oracle integrator. */
+int orc_closest1(const orc_node* nodes, int node_count, const int32_t* indices, const float* verts,
+                 const uint32_t* idx, const float o[3], const float d[3], float tmin, float tmax,
+                 uint32_t* prim, float* t, float* u, float* v);
+int orc_any1(const orc_node* nodes, int node_count, const int32_t* indices, const float* verts,
+             const uint32_t* idx, const float o[3], const float d[3], float tmin, float tmax);
+
+/* ---------------- random numbers (render.c) ---------------- */
+
+/* Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11): 4 x u32 from a 128-bit
+ * counter and a 64-bit key. */
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+/* 4*ngroups uniforms in [0,1) for counter (pix, smp, bounce, group g),
+ * key = seed: u = (x >> 8) * 2^-24. */
+void orc_philox_uniforms(uint64_t seed, uint32_t pix, uint32_t smp, uint32_t bounce, int ngroups, float* out);
+
+#define ORC_CAMERA_BOUNCE 0xFFFFFFFFu
+
+/* ---------------- integrator (render.c) ---------------- */
+
+/* Same layout as ar_material (include/ar.h). */
+typedef struct {
+    float albedo[3];
+    float emission[3];
+    float specular, metallic, roughness;
+    uint32_t model;       /* 0 Lambert, 1 GGX + Lambert */
+} orc_material;
+
+typedef struct {
+    const float* verts;           /* nv x 3 */
+    const uint32_t* idx;          /* ntri x 3 */
+    uint32_t ntri;
+    const uint32_t* tri_mat;      /* ntri, or NULL: material 0 */
+    const orc_material* mats;
+    /* filled by orc_render */
+    const orc_node* nodes;
+    int node_count;
+    const int32_t* indices;
+    const int32_t* light_index;   /* ntri: index into the light table or -1 */
+} orc_render_scene;
+
+typedef struct {
+    int count;
+    const int32_t* tri;    /* light triangle ids */
+    const float* area;
+    const float* prob;     /* select probability = w / sum(w), w = max(area * lum(Le), 1e-20) */
+    const float* cdf;      /* running FP32 sum of prob */
+} orc_render_lights;
+
+typedef struct {
+    uint32_t width, height, spp, sample_offset;
+    uint64_t seed;
+    int max_depth;
+    int rr_min_depth;
+    int nee;
+    int sky;               /* 0 none, 1 reference sky gradient, 2 constant env */
+    float env[3];
+    float cam_pos[3], cam_dir[3];
+    float vfov_deg;
+} orc_render_opts;
+
+/* Renders the mean of spp samples per pixel into out (width*height*3,
+ * row-major, pixel index = j*width + i).  precision 32: float out, FP32
+ * arithmetic with the same triangle test and hit rule as the GPU; 64:
+ * double out, FP64 arithmetic throughout.  Returns 0 on success. */
+int orc_render(const float* verts, uint32_t nv, const uint32_t* idx, uint32_t ntri, const uint32_t* tri_mat,
+               const orc_material* mats, uint32_t nmats, const orc_render_opts* opts, int precision,
+               void* out, int threads);
+
 #ifdef __cplusplus
 }
 #endif

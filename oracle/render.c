@@ -1,0 +1,267 @@
+/*
+ * Oracle integrator (test infrastructure only): Philox RNG, light table,
+ * FP32 / FP64 traversal glue and the two instantiations of render_impl.h.
+ */
+#include "oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ---------------- Philox4x32-10 ---------------- */
+
+static void mulhilo(uint32_t a, uint32_t b, uint32_t* hi, uint32_t* lo)
+{
+    const uint64_t p = (uint64_t)a * (uint64_t)b;
+    *hi = (uint32_t)(p >> 32);
+    *lo = (uint32_t)p;
+}
+
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4])
+{
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+    uint32_t k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; ++r) {
+        uint32_t hi0, lo0, hi1, lo1;
+        mulhilo(0xD2511F53u, c0, &hi0, &lo0);
+        mulhilo(0xCD9E8D57u, c2, &hi1, &lo1);
+        const uint32_t n0 = hi1 ^ c1 ^ k0;
+        const uint32_t n1 = lo1;
+        const uint32_t n2 = hi0 ^ c3 ^ k1;
+        const uint32_t n3 = lo0;
+        c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+void orc_philox_uniforms(uint64_t seed, uint32_t pix, uint32_t smp, uint32_t bounce, int ngroups, float* out)
+{
+    const uint32_t key[2] = { (uint32_t)seed, (uint32_t)(seed >> 32) };
+    for (int g = 0; g < ngroups; ++g) {
+        const uint32_t ctr[4] = { pix, smp, bounce, (uint32_t)g };
+        uint32_t x[4];
+        orc_philox4x32_10(ctr, key, x);
+        for (int k = 0; k < 4; ++k) out[4 * g + k] = (float)(x[k] >> 8) * 0x1p-24f;
+    }
+}
+
+/* ---------------- FP64 traversal (watertight test in double) ---------------- */
+
+static int tri_test_d(const double o[3], const double d[3], const float* v0, const float* v1, const float* v2,
+                      double* t, double* u, double* v)
+{
+    const double ax = fabs(d[0]), ay = fabs(d[1]), az = fabs(d[2]);
+    int kz = 0;
+    if (ay > ax) kz = 1;
+    if (az > fabs(d[kz])) kz = 2;
+    int kx = (kz + 1) % 3, ky = (kx + 1) % 3;
+    if (d[kz] < 0) { const int tmp = kx; kx = ky; ky = tmp; }
+    if (d[kz] == 0) return 0;
+    const double sx = d[kx] / d[kz], sy = d[ky] / d[kz], sz = 1.0 / d[kz];
+    const double A[3] = { v0[0] - o[0], v0[1] - o[1], v0[2] - o[2] };
+    const double B[3] = { v1[0] - o[0], v1[1] - o[1], v1[2] - o[2] };
+    const double C[3] = { v2[0] - o[0], v2[1] - o[1], v2[2] - o[2] };
+    const double Ax = A[kx] - sx * A[kz], Ay = A[ky] - sy * A[kz];
+    const double Bx = B[kx] - sx * B[kz], By = B[ky] - sy * B[kz];
+    const double Cx = C[kx] - sx * C[kz], Cy = C[ky] - sy * C[kz];
+    const double U = Cx * By - Cy * Bx, V = Ax * Cy - Ay * Cx, W = Bx * Ay - By * Ax;
+    if ((U < 0 || V < 0 || W < 0) && (U > 0 || V > 0 || W > 0)) return 0;
+    const double det = U + V + W;
+    if (det == 0) return 0;
+    const double T = U * (sz * A[kz]) + V * (sz * B[kz]) + W * (sz * C[kz]);
+    *t = T / det;
+    *u = V / det;
+    *v = W / det;
+    return 1;
+}
+
+static int box_hit_d(const double o[3], const double inv[3], const orc_node* nd, double tmin, double tmax)
+{
+    double m = fmax(fabs(o[0]), fmax(fabs(o[1]), fabs(o[2])));
+    for (int a = 0; a < 3; ++a) m = fmax(m, fmax(fabs(nd->bmin[a]), fabs(nd->bmax[a])));
+    const double pad = m * 0x1p-16 + 1e-30;
+    double te = -INFINITY, tx = INFINITY;
+    for (int a = 0; a < 3; ++a) {
+        const double t0 = (nd->bmin[a] - pad - o[a]) * inv[a], t1 = (nd->bmax[a] + pad - o[a]) * inv[a];
+        te = fmax(te, fmin(t0, t1));
+        tx = fmin(tx, fmax(t0, t1));
+    }
+    return tx >= fmax(0.0, tmin) && te <= tmax && te <= tx;
+}
+
+/* any = 1: first accepted hit.  Canonical rule (min t, then min id). */
+static uint32_t trace_d(const orc_render_scene* sc, const double o[3], const double d[3], double tmin, double tmax,
+                        int any, double* tout, double* uout, double* vout)
+{
+    if (sc->node_count == 0 || !(tmax >= tmin)) return ORC_MISS;
+    double inv[3];
+    for (int a = 0; a < 3; ++a) inv[a] = d[a] == 0 ? (signbit(d[a]) ? -DBL_MAX : DBL_MAX) : 1.0 / d[a];
+    int stack[128];
+    int sp = 0;
+    stack[sp++] = 0;
+    uint32_t best = ORC_MISS;
+    double bt = tmax, bu = 0, bv = 0;
+    while (sp) {
+        const orc_node* nd = &sc->nodes[stack[--sp]];
+        if (!box_hit_d(o, inv, nd, tmin, bt)) continue;
+        if (nd->count > 0) {
+            for (int k = 0; k < nd->count; ++k) {
+                const uint32_t p = (uint32_t)sc->indices[nd->first + k];
+                const float* a = &sc->verts[3 * sc->idx[3 * p]];
+                const float* b = &sc->verts[3 * sc->idx[3 * p + 1]];
+                const float* c = &sc->verts[3 * sc->idx[3 * p + 2]];
+                double t, u, v;
+                if (!tri_test_d(o, d, a, b, c, &t, &u, &v)) continue;
+                const int acc = t >= tmin && (best != ORC_MISS ? (t < bt || (t == bt && p < best)) : t < tmax);
+                if (acc) {
+                    best = p; bt = t; bu = u; bv = v;
+                    if (any) { *tout = bt; return best; }
+                }
+            }
+        } else {
+            if (sp + 2 > 128) abort();
+            stack[sp++] = nd->right;
+            stack[sp++] = nd->left;
+        }
+    }
+    *tout = bt; *uout = bu; *vout = bv;
+    return best;
+}
+
+/* ---------------- instantiations ---------------- */
+
+#define RF_CAT(a, b) RF_CAT2(a, b)
+#define RF_CAT2(a, b) a##_##b
+
+#define R float
+#define SUF f32
+#define SQRT sqrtf
+#define SIN sinf
+#define COS cosf
+#define TAN tanf
+#define EXP expf
+#define POW powf
+#define FABS fabsf
+#define PI_R 3.14159265358979323846f
+#include "render_impl.h"
+
+#define R double
+#define SUF f64
+#define SQRT sqrt
+#define SIN sin
+#define COS cos
+#define TAN tan
+#define EXP exp
+#define POW pow
+#define FABS fabs
+#define PI_R 3.14159265358979323846
+#include "render_impl.h"
+
+static uint32_t closest_f32(const orc_render_scene* sc, vec3_f32 o, vec3_f32 d, float tmin, float tmax,
+                            float* t, float* bu, float* bv)
+{
+    const float oo[3] = { o.x, o.y, o.z }, dd[3] = { d.x, d.y, d.z };
+    uint32_t p;
+    *bu = *bv = 0.0f;
+    orc_closest1(sc->nodes, sc->node_count, sc->indices, sc->verts, sc->idx, oo, dd, tmin, tmax, &p, t, bu, bv);
+    return p;
+}
+
+static int occluded_f32(const orc_render_scene* sc, vec3_f32 o, vec3_f32 d, float tmin, float tmax)
+{
+    const float oo[3] = { o.x, o.y, o.z }, dd[3] = { d.x, d.y, d.z };
+    return orc_any1(sc->nodes, sc->node_count, sc->indices, sc->verts, sc->idx, oo, dd, tmin, tmax);
+}
+
+static uint32_t closest_f64(const orc_render_scene* sc, vec3_f64 o, vec3_f64 d, double tmin, double tmax,
+                            double* t, double* bu, double* bv)
+{
+    const double oo[3] = { o.x, o.y, o.z }, dd[3] = { d.x, d.y, d.z };
+    *bu = *bv = 0.0;
+    return trace_d(sc, oo, dd, tmin, tmax, 0, t, bu, bv);
+}
+
+static int occluded_f64(const orc_render_scene* sc, vec3_f64 o, vec3_f64 d, double tmin, double tmax)
+{
+    const double oo[3] = { o.x, o.y, o.z }, dd[3] = { d.x, d.y, d.z };
+    double t, u, v;
+    return trace_d(sc, oo, dd, tmin, tmax, 1, &t, &u, &v) != ORC_MISS;
+}
+
+/* ---------------- light table (core/scene.cu:677-701) ---------------- */
+
+static float tri_area_f(const float* a, const float* b, const float* c)
+{
+    const float e1[3] = { b[0] - a[0], b[1] - a[1], b[2] - a[2] };
+    const float e2[3] = { c[0] - a[0], c[1] - a[1], c[2] - a[2] };
+    const float cx = e1[1] * e2[2] - e1[2] * e2[1];
+    const float cy = e1[2] * e2[0] - e1[0] * e2[2];
+    const float cz = e1[0] * e2[1] - e1[1] * e2[0];
+    return 0.5f * sqrtf(cx * cx + cy * cy + cz * cz);
+}
+
+int orc_render(const float* verts, uint32_t nv, const uint32_t* idx, uint32_t ntri, const uint32_t* tri_mat,
+               const orc_material* mats, uint32_t nmats, const orc_render_opts* opts, int precision,
+               void* out, int threads)
+{
+    (void)nv; (void)nmats;
+    orc_render_scene sc;
+    memset(&sc, 0, sizeof sc);
+    sc.verts = verts; sc.idx = idx; sc.ntri = ntri; sc.tri_mat = tri_mat; sc.mats = mats;
+    float* pmin = malloc(sizeof(float) * 3 * (ntri ? ntri : 1));
+    float* pmax = malloc(sizeof(float) * 3 * (ntri ? ntri : 1));
+    float* pcen = malloc(sizeof(float) * 3 * (ntri ? ntri : 1));
+    orc_node* nodes = malloc(sizeof(orc_node) * 2 * (ntri ? ntri : 1));
+    int32_t* indices = malloc(sizeof(int32_t) * (ntri ? ntri : 1));
+    int32_t* light_index = malloc(sizeof(int32_t) * (ntri ? ntri : 1));
+    int32_t* ltri = malloc(sizeof(int32_t) * (ntri ? ntri : 1));
+    float* larea = malloc(sizeof(float) * (ntri ? ntri : 1));
+    float* lprob = malloc(sizeof(float) * (ntri ? ntri : 1));
+    float* lcdf = malloc(sizeof(float) * (ntri ? ntri : 1));
+    orc_tri_proxies(verts, idx, (int)ntri, pmin, pmax, pcen);
+    sc.node_count = orc_bvh_build(pmin, pmax, pcen, (int)ntri, 8, 12, nodes, indices);
+    sc.nodes = nodes;
+    sc.indices = indices;
+    int nl = 0;
+    float total = 0.0f;
+    for (uint32_t p = 0; p < ntri; ++p) {
+        const orc_material* m = &mats[tri_mat ? tri_mat[p] : 0];
+        light_index[p] = -1;
+        if (m->emission[0] > 0.0f || m->emission[1] > 0.0f || m->emission[2] > 0.0f) {
+            const float area = tri_area_f(&verts[3 * idx[3 * p]], &verts[3 * idx[3 * p + 1]], &verts[3 * idx[3 * p + 2]]);
+            const float lum = m->emission[0] * 0.2126f + m->emission[1] * 0.7152f + m->emission[2] * 0.0722f;
+            const float w = fmaxf(area * lum, 1.0e-20f);
+            light_index[p] = nl;
+            ltri[nl] = (int32_t)p;
+            larea[nl] = area;
+            lprob[nl] = w;
+            total += w;
+            ++nl;
+        }
+    }
+    float acc = 0.0f;
+    for (int i = 0; i < nl; ++i) {
+        lprob[i] = lprob[i] / total;
+        acc += lprob[i];
+        lcdf[i] = acc;
+    }
+    sc.light_index = light_index;
+    orc_render_lights L = { nl, ltri, larea, lprob, lcdf };
+    int nt = threads;
+#ifdef _OPENMP
+    if (nt <= 0) nt = omp_get_max_threads();
+#else
+    nt = 1;
+#endif
+    if (precision == 64) render_f64(&sc, &L, opts, (double*)out, nt);
+    else render_f32(&sc, &L, opts, (float*)out, nt);
+    free(pmin); free(pmax); free(pcen); free(nodes); free(indices); free(light_index);
+    free(ltri); free(larea); free(lprob); free(lcdf);
+    return 0;
+}

@@ -583,3 +583,31 @@ void orc_trace_any_brute(const float* verts, const uint32_t* idx, int ntri,
         occluded_bits[w] = word;
     }
 }
+
+/* Single-ray entry points used by the oracle integrator (render.c). */
+int orc_closest1(const orc_node* nodes, int node_count, const int32_t* indices, const float* verts,
+                 const uint32_t* idx, const float o[3], const float d[3], float tmin, float tmax,
+                 uint32_t* prim, float* t, float* u, float* v)
+{
+    float tt;
+    uint32_t p;
+    const int hit = traverse_one(nodes, node_count, indices, verts, idx, o, d, tmin, tmax, 0, 0, &p, &tt, NULL, NULL);
+    *prim = p;
+    *t = tt;
+    if (hit) {
+        const float *a, *b, *c;
+        tri_verts(verts, idx, (int)p, &a, &b, &c);
+        const ray_pre pre = ray_precompute(d);
+        uint32_t f;
+        tri_test_pre(o, &pre, a, b, c, t, u, v, &f);
+    }
+    return hit;
+}
+
+int orc_any1(const orc_node* nodes, int node_count, const int32_t* indices, const float* verts,
+             const uint32_t* idx, const float o[3], const float d[3], float tmin, float tmax)
+{
+    float tt;
+    uint32_t p;
+    return traverse_one(nodes, node_count, indices, verts, idx, o, d, tmin, tmax, 0, 1, &p, &tt, NULL, NULL);
+}

@@ -109,3 +109,36 @@ def edge_case_rays():
     add((0.5, 0.25, 2.0), (0.0, 0.0, -1.0))                 # along a grid edge line
     add((-1.0, 0.5, 0.0), (1.0, 0.0, 0.0))                  # in the plane of the grid
     return np.array(rows, dtype=np.float32).T.copy()
+
+
+def floor_and_light(albedo=0.5, le=(10.0, 10.0, 10.0), light_h=1.0, light_size=0.4):
+    """A large upward floor quad at y=0 (grey `albedo`) and one emissive
+    triangle at height light_h facing down.  Returns (V, F, tri_mat, mats, tri)
+    with tri the light triangle's vertices (3,3)."""
+    S = 50.0
+    V = [(-S, 0, -S), (-S, 0, S), (S, 0, S), (S, 0, -S)]
+    F = [[0, 1, 2], [0, 2, 3]]
+    M = [0, 0]
+    s = light_size
+    tri = np.array([(-s, light_h, -s), (s, light_h, -s), (0.0, light_h, s)], np.float64)
+    base = len(V)
+    V += [tuple(t) for t in tri]
+    F.append([base, base + 1, base + 2])        # (b-a)x(c-a) points down: front face toward the floor
+    M.append(1)
+    mats = [{"albedo": (albedo,) * 3}, {"albedo": (0, 0, 0), "emission": tuple(le)}]
+    return np.array(V, np.float32), np.array(F, np.uint32), np.array(M, np.uint32), mats, tri
+
+
+def polygon_irradiance(p, n, poly):
+    """Irradiance at p (normal n) from a unit-radiance polygon (Lambert's
+    formula): E = 1/2 * sum_i theta_i * (n . g_i), g_i the unit normal of the
+    plane through p and edge (v_i, v_{i+1}), theta_i the angle subtended."""
+    p, n = np.asarray(p, np.float64), np.asarray(n, np.float64)
+    R = [np.asarray(v, np.float64) - p for v in poly]
+    E = 0.0
+    for i in range(len(R)):
+        a, b = R[i], R[(i + 1) % len(R)]
+        cr = np.cross(a, b)
+        theta = np.arctan2(np.linalg.norm(cr), np.dot(a, b))
+        E += theta * np.dot(n, cr / np.linalg.norm(cr))
+    return 0.5 * abs(E)

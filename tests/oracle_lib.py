@@ -165,3 +165,81 @@ def unpack_bits(words: np.ndarray, n: int) -> np.ndarray:
     w = np.asarray(words).view(np.uint32)
     bits = (w[:, None] >> np.arange(32, dtype=np.uint32)[None, :]) & 1
     return bits.reshape(-1)[:n].astype(bool)
+
+
+# ---------------------------------------------------------------- integrator
+
+class orc_material(C.Structure):
+    _fields_ = [("albedo", C.c_float * 3), ("emission", C.c_float * 3), ("specular", C.c_float),
+                ("metallic", C.c_float), ("roughness", C.c_float), ("model", C.c_uint32)]
+
+
+class orc_render_opts(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32), ("sample_offset", C.c_uint32),
+                ("seed", C.c_uint64), ("max_depth", C.c_int), ("rr_min_depth", C.c_int), ("nee", C.c_int),
+                ("sky", C.c_int), ("env", C.c_float * 3), ("cam_pos", C.c_float * 3), ("cam_dir", C.c_float * 3),
+                ("vfov_deg", C.c_float)]
+
+
+def philox(ctr, key):
+    lib = load()
+    c = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    o = (C.c_uint32 * 4)()
+    lib.orc_philox4x32_10.restype = None
+    lib.orc_philox4x32_10(c, k, o)
+    return list(o)
+
+
+def philox_uniforms(seed, pix, smp, bounce, ngroups=3):
+    lib = load()
+    out = (C.c_float * (4 * ngroups))()
+    lib.orc_philox_uniforms.restype = None
+    lib.orc_philox_uniforms.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, C.c_void_p]
+    lib.orc_philox_uniforms(seed, pix, smp, bounce, ngroups, C.cast(out, C.c_void_p))
+    return np.array(out[:], np.float32)
+
+
+def make_materials(mats):
+    arr = (orc_material * max(1, len(mats)))()
+    for i, m in enumerate(mats):
+        arr[i].albedo[:] = m.get("albedo", (0.75, 0.75, 0.75))
+        arr[i].emission[:] = m.get("emission", (0.0, 0.0, 0.0))
+        arr[i].specular = m.get("specular", 0.5)
+        arr[i].metallic = m.get("metallic", 0.0)
+        arr[i].roughness = m.get("roughness", 1.0)
+        arr[i].model = m.get("model", 0)
+    return arr
+
+
+def render(V, F, tri_mat, mats, width, height, spp, seed=1, max_depth=5, nee=1, sky=0, env=(0, 0, 0),
+           cam_pos=(0, 0, 1.1), cam_dir=(0, 0, -1), vfov=60.0, precision=32, rr_min_depth=3, sample_offset=0,
+           threads=0):
+    """Oracle render; returns (height, width, 3) float32 or float64 mean image."""
+    lib = load()
+    V = np.ascontiguousarray(V, np.float32)
+    F = np.ascontiguousarray(F, np.uint32)
+    tm = None if tri_mat is None else np.ascontiguousarray(tri_mat, np.uint32)
+    marr = make_materials(mats)
+    o = orc_render_opts()
+    o.width, o.height, o.spp, o.sample_offset = width, height, spp, sample_offset
+    o.seed, o.max_depth, o.rr_min_depth, o.nee, o.sky = seed, max_depth, rr_min_depth, nee, sky
+    o.env[:] = env
+    o.cam_pos[:] = cam_pos
+    o.cam_dir[:] = cam_dir
+    o.vfov_deg = vfov
+    out = np.zeros((height, width, 3), np.float64 if precision == 64 else np.float32)
+    lib.orc_render.restype = C.c_int
+    lib.orc_render.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p,
+                               C.c_uint32, C.POINTER(orc_render_opts), C.c_int, C.c_void_p, C.c_int]
+    rc = lib.orc_render(V.ctypes.data, V.shape[0], F.ctypes.data, F.shape[0], None if tm is None else tm.ctypes.data,
+                        C.cast(marr, C.c_void_p), len(mats), C.byref(o), precision, out.ctypes.data, threads)
+    assert rc == 0
+    return out
+
+
+def rel_mse(img, ref):
+    """mean over pixels and channels of (I - I_ref)^2 / (I_ref^2 + 0.01)"""
+    img = np.asarray(img, np.float64)
+    ref = np.asarray(ref, np.float64)
+    return float(np.mean((img - ref) ** 2 / (ref ** 2 + 0.01)))
