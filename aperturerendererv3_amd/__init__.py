@@ -128,7 +128,7 @@ class Context:
                 cms[i].nmaterials = len(mats)
         self._check(self._lib.ar_scene_set(self._h, cms, len(meshes)), "ar_scene_set")
 
-    def build_bvh(self, width=2, leaf_size=4, bin_count=16, max_depth=60) -> dict:
+    def build_bvh(self, width=8, leaf_size=3, bin_count=16, max_depth=60) -> dict:
         o = ar_bvh_opts(width, leaf_size, bin_count, max_depth)
         self._check(self._lib.ar_bvh_build(self._h, C.byref(o)), "ar_bvh_build")
         return self.bvh_info()

@@ -102,14 +102,22 @@ struct Converter {
     }
 };
 
-int ensure_spill(ar_ctx* ctx) {
-    if (ctx->spill) return AR_OK;
+int ensure_spill(ar_ctx* ctx, uint32_t stack_entries) {
+    if (ctx->spill && ctx->spill_stack >= stack_entries) return AR_OK;
+    if (ctx->spill) (void)hipFree(ctx->spill);
+    ctx->spill = nullptr;
     const int bpc = trace_blocks_per_cu();
     const uint32_t blocks = (uint32_t)ctx->num_cus * (uint32_t)bpc;
     ctx->trace_threads = blocks * kTraceBlock;
-    ctx->spill_entries = (size_t)(kStackMax - kLdsStack) * ctx->trace_threads;
+    const uint32_t deep = stack_entries > (uint32_t)kLdsStack ? stack_entries - (uint32_t)kLdsStack : 1u;
+    ctx->spill_entries = (size_t)deep * ctx->trace_threads;
     int rc = check_hip(ctx, hipMalloc(&ctx->spill, ctx->spill_entries * sizeof(int)), "hipMalloc(stack spill)");
-    if (rc != AR_OK) { ctx->spill = nullptr; return rc; }
+    if (rc != AR_OK) { ctx->spill = nullptr; ctx->spill_stack = 0; return rc; }
+    ctx->spill_stack = stack_entries;
+    if (!ctx->fetch) {
+        rc = check_hip(ctx, hipMalloc((void**)&ctx->fetch, 256), "hipMalloc(refill counter)");
+        if (rc != AR_OK) { ctx->fetch = nullptr; return rc; }
+    }
     return AR_OK;
 }
 
@@ -123,17 +131,21 @@ bool hits_ok(const ar_hits* h) { return h && h->prim && h->t && h->u && h->v && 
 
 int trace_prologue(ar_ctx* ctx, TraceArgs& ta, size_t n, uint32_t& blocks) {
     if (!ctx->bvh_ready) return set_error(ctx, AR_E_NOSCENE, "no BVH: call ar_scene_set and ar_bvh_build first");
-    int rc = ensure_spill(ctx);
+    int rc = ensure_spill(ctx, ctx->bvh.stack_entries);
     if (rc != AR_OK) return rc;
-    const uint32_t full_blocks = ctx->trace_threads / kTraceBlock;
+    const uint32_t full_blocks = std::min(ctx->trace_threads / kTraceBlock,
+                                          (uint32_t)ctx->num_cus * (uint32_t)trace_blocks_per_cu((int)ctx->bvh.width, false));
     const size_t need = (n + kTraceBlock - 1) / kTraceBlock;
     blocks = (uint32_t)std::min<size_t>(full_blocks, std::max<size_t>(need, 1));
+    ta.width = ctx->bvh.width;
+    ta.stack_entries = ctx->bvh.stack_entries;
     ta.nodes = ctx->bvh.nodes;
     ta.tris = reinterpret_cast<const float4*>(ctx->bvh.tris);
     ta.root = ctx->bvh.root;
     ta.origin_limit = ctx->bvh.origin_limit;
     ta.spill = ctx->spill;
     ta.threads = blocks * kTraceBlock;
+    ta.fetch = ctx->fetch;
     return AR_OK;
 }
 
@@ -187,6 +199,7 @@ int ar_destroy(ar_ctx* ctx) {
     free_scene(ctx);
     free_render_buffers(ctx);
     if (ctx->spill) (void)hipFree(ctx->spill);
+    if (ctx->fetch) (void)hipFree(ctx->fetch);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return AR_OK;
@@ -250,9 +263,9 @@ int ar_scene_set(ar_ctx* ctx, const ar_mesh* meshes, size_t nmesh) {
 int ar_bvh_build(ar_ctx* ctx, const ar_bvh_opts* opts) {
     if (!ctx) return AR_E_INVALID;
     if (!ctx->scene_set) return set_error(ctx, AR_E_NOSCENE, "no scene: call ar_scene_set first");
-    ar_bvh_opts o{2, 4, 16, (uint32_t)kMaxDepth};
+    ar_bvh_opts o{8, 3, 16, (uint32_t)kMaxDepth};
     if (opts) o = *opts;
-    if (o.width != 2) return set_error(ctx, AR_E_UNSUPPORTED, "only BVH width 2 is implemented");
+    if (o.width != 2 && o.width != 8) return set_error(ctx, AR_E_UNSUPPORTED, "BVH width must be 2 or 8");
     if (o.leaf_size < 1 || o.leaf_size > kMaxLeafTris) return set_error(ctx, AR_E_INVALID, "leaf_size must be 1..15");
     if (o.bin_count < 4 || o.bin_count > 64) return set_error(ctx, AR_E_INVALID, "bin_count must be 4..64");
     if (o.max_depth == 0 || o.max_depth > (uint32_t)kMaxDepth) o.max_depth = kMaxDepth;
@@ -261,7 +274,7 @@ int ar_bvh_build(ar_ctx* ctx, const ar_bvh_opts* opts) {
     const auto t0 = std::chrono::steady_clock::now();
     const uint32_t n = (uint32_t)(ctx->idx.size() / 3);
     ctx->info = ar_bvh_info{};
-    ctx->info.width = 2;
+    ctx->info.width = o.width;
     if (n == 0) {
         ctx->bvh.root = kRefEmpty;
         ctx->bvh_ready = true;
@@ -284,30 +297,50 @@ int ar_bvh_build(ar_ctx* ctx, const ar_bvh_opts* opts) {
     BuildResult br = build_bvh2(lo.data(), hi.data(), cen.data(), n, o.leaf_size, o.bin_count, (int)o.max_depth);
     if (br.depth > kMaxDepth)
         return set_error(ctx, AR_E_CAPACITY, "BVH deeper than the traversal stack allows");
-    Converter cv{br, ctx->verts, ctx->idx, mag * kPadRel + 1e-30f, {}, {}, 0};
-    cv.nodes.reserve(br.nodes.size() / 2 + 1);
-    cv.tris.reserve(12 * (size_t)n);
-    cv.run();
-    int rc = check_hip(ctx, hipMalloc(&ctx->bvh.nodes, cv.nodes.size() * sizeof(GpuNode2)), "hipMalloc(nodes)");
+    const float pad = mag * kPadRel + 1e-30f;
+    std::vector<GpuNode8> nodes8;
+    std::vector<float> tris;
+    size_t node_bytes = 0;
+    uint64_t leaves = 0;
+    int depth = br.depth;
+    const void* node_src = nullptr;
+    Converter cv{br, ctx->verts, ctx->idx, pad, {}, {}, 0};
+    if (o.width == 8) {
+        convert_bvh8(br, ctx->verts, ctx->idx, pad, nodes8, tris, leaves, depth);
+        node_src = nodes8.data();
+        node_bytes = nodes8.size() * sizeof(GpuNode8);
+        ctx->bvh.stack_entries = 7u * (uint32_t)(depth + 1) + 2u;
+        ctx->info.node_count = nodes8.size();
+    } else {
+        cv.nodes.reserve(br.nodes.size() / 2 + 1);
+        cv.tris.reserve(12 * (size_t)n);
+        cv.run();
+        tris.swap(cv.tris);
+        leaves = cv.leaves;
+        node_src = cv.nodes.data();
+        node_bytes = cv.nodes.size() * sizeof(GpuNode2);
+        ctx->bvh.stack_entries = (uint32_t)br.depth + 2u;
+        ctx->info.node_count = cv.nodes.size();
+    }
+    int rc = check_hip(ctx, hipMalloc((void**)&ctx->bvh.nodes, node_bytes), "hipMalloc(nodes)");
     if (rc != AR_OK) { ctx->bvh.nodes = nullptr; return rc; }
-    rc = check_hip(ctx, hipMalloc(&ctx->bvh.tris, cv.tris.size() * sizeof(float)), "hipMalloc(tris)");
+    rc = check_hip(ctx, hipMalloc(&ctx->bvh.tris, tris.size() * sizeof(float)), "hipMalloc(tris)");
     if (rc != AR_OK) { ctx->bvh.tris = nullptr; free_bvh(ctx); return rc; }
-    rc = check_hip(ctx, hipMemcpy(ctx->bvh.nodes, cv.nodes.data(), cv.nodes.size() * sizeof(GpuNode2),
-                                  hipMemcpyHostToDevice), "upload nodes");
+    rc = check_hip(ctx, hipMemcpy(ctx->bvh.nodes, node_src, node_bytes, hipMemcpyHostToDevice), "upload nodes");
     if (rc == AR_OK)
-        rc = check_hip(ctx, hipMemcpy(ctx->bvh.tris, cv.tris.data(), cv.tris.size() * sizeof(float),
+        rc = check_hip(ctx, hipMemcpy(ctx->bvh.tris, tris.data(), tris.size() * sizeof(float),
                                       hipMemcpyHostToDevice), "upload triangles");
     if (rc != AR_OK) { free_bvh(ctx); return rc; }
-    ctx->bvh.node_count = (uint32_t)cv.nodes.size();
-    ctx->bvh.tri_slots = (uint32_t)(cv.tris.size() / 12);
+    ctx->bvh.width = o.width;
+    ctx->bvh.node_count = (uint32_t)ctx->info.node_count;
+    ctx->bvh.tri_slots = (uint32_t)(tris.size() / 12);
     ctx->bvh.root = 0;
     ctx->bvh.origin_limit = kOriginLimit * mag;
     ctx->bvh_ready = true;
     const auto t1 = std::chrono::steady_clock::now();
-    ctx->info.node_count = cv.nodes.size();
-    ctx->info.leaf_count = cv.leaves;
-    ctx->info.depth = (uint32_t)br.depth;
-    ctx->info.device_bytes = cv.nodes.size() * sizeof(GpuNode2) + cv.tris.size() * sizeof(float);
+    ctx->info.leaf_count = leaves;
+    ctx->info.depth = (uint32_t)depth;
+    ctx->info.device_bytes = node_bytes + tris.size() * sizeof(float);
     ctx->info.build_seconds = std::chrono::duration<double>(t1 - t0).count();
     ctx->info.sah_cost = br.sah_cost;
     return upload_scene(ctx);

@@ -56,6 +56,28 @@ struct GpuNode2 {
 };
 static_assert(sizeof(GpuNode2) == 64, "GpuNode2 must be 64 bytes");
 
+// ---------------------------------------------------------------------------
+// Device BVH8 layout (quantised 8-wide node, 96 bytes = 6 x 16 B).
+//
+//   float4 h  = (p.x, p.y, p.z, bits: e.x | e.y << 8 | e.z << 16 | nchild << 24)
+//   uint4 r0  = ref[0..3],  uint4 r1 = ref[4..7]   (same encoding as BVH2 refs)
+//   uint4 q0  = lo.x[0..7] bytes, lo.y[0..7] bytes
+//   uint4 q1  = lo.z[0..7] bytes, hi.x[0..7] bytes
+//   uint4 q2  = hi.y[0..7] bytes, hi.z[0..7] bytes
+//
+// Child c's box is p + q * 2^(e - 127) per axis (e is a biased exponent, so
+// 2^(e-127) is built exactly from bits).  The builder rounds q outward and
+// checks the decoded box in float against the padded child box, so the
+// decoded box always contains it.
+// ---------------------------------------------------------------------------
+struct GpuNode8 {
+    float p[3];
+    uint32_t meta;
+    uint32_t ref[8];
+    uint8_t q[48];
+};
+static_assert(sizeof(GpuNode8) == 96, "GpuNode8 must be 96 bytes");
+
 // Host-side binary tree produced by the builder (pre-order, left = parent+1).
 struct BuildNode {
     float lo[3], hi[3];
@@ -74,13 +96,19 @@ struct BuildResult {
 BuildResult build_bvh2(const float* lo, const float* hi, const float* cen, uint32_t n,
                        uint32_t leaf_size, uint32_t bins, int max_depth);
 
+// Collapse into quantised 8-wide nodes (bvh_wide.cpp); depth = wide-tree depth.
+bool convert_bvh8(const BuildResult& br, const std::vector<float>& verts, const std::vector<uint32_t>& idx, float pad,
+                  std::vector<GpuNode8>& nodes, std::vector<float>& tris, uint64_t& leaves, int& depth);
+
 struct DeviceBVH {
-    GpuNode2* nodes = nullptr;   // device
+    uint32_t width = 2;          // 2: GpuNode2 nodes, 8: GpuNode8 nodes
+    GpuNode2* nodes = nullptr;   // device (GpuNode8* when width == 8)
     float* tris = nullptr;       // device, 12 floats per slot
     uint32_t node_count = 0;
     uint32_t tri_slots = 0;
     uint32_t root = kRefEmpty;   // root interior node index (0) or kRefEmpty
     float origin_limit = 0.0f;   // kOriginLimit * scene magnitude
+    uint32_t stack_entries = kStackMax;  // per-ray traversal stack bound (incl. sentinel)
 };
 
 // Shading data of the scene, prim order (uploaded with the BVH).
@@ -153,7 +181,9 @@ struct ar_ctx {
 
     // traversal scratch: per-thread stack spill, sized for the persistent grid
     int* spill = nullptr;
-    size_t spill_entries = 0;
+    size_t spill_entries = 0;          // total ints allocated
+    uint32_t spill_stack = 0;          // per-thread stack entries the spill buffer covers
+    uint32_t* fetch = nullptr;         // ray-refill counter of the traversal kernels
     uint32_t trace_threads = 0;       // persistent grid size in threads
 };
 
@@ -163,12 +193,15 @@ int check_hip(ar_ctx* ctx, hipError_t e, const char* what);
 
 // trace.hip launchers
 struct TraceArgs {
+    uint32_t width;
+    uint32_t stack_entries;
     const GpuNode2* nodes;
     const float4* tris;
     uint32_t root;
     float origin_limit;
     int* spill;
     uint32_t threads;
+    uint32_t* fetch;     // device ray-refill counter
 };
 // n_dev (optional): device pointer to the ray count, read by the kernel
 // (then n is only the capacity used to size the grid).
@@ -179,7 +212,7 @@ hipError_t launch_trace_any(const TraceArgs& ta, const ar_rays& r, size_t n, uin
                             uint32_t blocks, hipStream_t s, const uint32_t* n_dev = nullptr);
 hipError_t launch_shadow_rays(const ar_rays& r, const ar_hits& h, size_t n, float tx, float ty, float tz,
                               float tmin, float tmax_scale, const ar_rays& out, hipStream_t s);
-int trace_blocks_per_cu();
+int trace_blocks_per_cu(int width = 0, bool any = false);
 
 // integrator.hip
 int upload_scene(ar_ctx* ctx);

@@ -1,0 +1,152 @@
+// Collapse of the binary SAH tree into the quantised 8-wide device layout.
+//
+// Each wide node takes a binary node's two children and repeatedly opens the
+// interior child of largest surface area (its two children replace it) until
+// eight children or only leaves remain, as in the wide-BVH construction of
+// Wald et al. (RT'08) and Ylitie et al. (HPG'17).  Child boxes are padded
+// by the scene pad, then quantised outward to 8 bits on a per-node grid
+// p + q * 2^e; every decoded bound is checked in float against the padded
+// box and nudged outward if rounding put it inside.
+#include "ar_internal.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace ar {
+namespace {
+
+float area_of(const BuildNode& n) {
+    const float dx = std::max(0.0f, n.hi[0] - n.lo[0]);
+    const float dy = std::max(0.0f, n.hi[1] - n.lo[1]);
+    const float dz = std::max(0.0f, n.hi[2] - n.lo[2]);
+    return 2.0f * (dx * dy + dy * dz + dz * dx);
+}
+
+struct Collapser {
+    const BuildResult& br;
+    const std::vector<float>& verts;
+    const std::vector<uint32_t>& idx;
+    float pad;
+    std::vector<GpuNode8> nodes;
+    std::vector<float> tris;
+    uint64_t leaves = 0;
+    int max_depth = 0;
+
+    uint32_t leaf_ref(const BuildNode& n) {
+        const uint32_t first = (uint32_t)(tris.size() / 12);
+        for (uint32_t k = 0; k < n.count; ++k) {
+            const uint32_t prim = br.order[n.first + k];
+            for (int v = 0; v < 3; ++v) {
+                const float* p = &verts[3 * (size_t)idx[3 * (size_t)prim + v]];
+                tris.push_back(p[0]);
+                tris.push_back(p[1]);
+                tris.push_back(p[2]);
+                float w = 0.0f;
+                if (v == 0) std::memcpy(&w, &prim, 4);
+                tris.push_back(w);
+            }
+        }
+        ++leaves;
+        return make_leaf_ref(first, n.count);
+    }
+
+    // biased exponent e (1..254) with 255 * 2^(e-127) >= ext
+    static int exponent_for(float ext) {
+        if (!(ext > 0.0f)) return 1;
+        int e = (int)std::ceil(std::log2((double)ext / 255.0));
+        while (std::ldexp(255.0, e) < (double)ext) ++e;
+        return std::min(254, std::max(1, e + 127));
+    }
+
+    static float scale_of(int biased) {
+        uint32_t bits = (uint32_t)biased << 23;
+        float s;
+        std::memcpy(&s, &bits, 4);
+        return s;
+    }
+
+    // children: binary-node indices of the wide node's children
+    uint32_t build(int bi, int depth) {
+        max_depth = std::max(max_depth, depth);
+        std::vector<int> ch;
+        const BuildNode& n = br.nodes[bi];
+        if (n.left < 0) {
+            ch.push_back(bi);
+        } else {
+            ch.push_back(n.left);
+            ch.push_back(n.right);
+            while (ch.size() < 8) {
+                int best = -1;
+                float best_a = -1.0f;
+                for (size_t k = 0; k < ch.size(); ++k) {
+                    const BuildNode& c = br.nodes[ch[k]];
+                    if (c.left >= 0 && area_of(c) > best_a) { best_a = area_of(c); best = (int)k; }
+                }
+                if (best < 0) break;
+                const BuildNode& c = br.nodes[ch[best]];
+                ch[best] = c.left;
+                ch.push_back(c.right);
+            }
+        }
+        const uint32_t gi = (uint32_t)nodes.size();
+        nodes.emplace_back();
+        uint32_t refs[8];
+        for (size_t k = 0; k < 8; ++k) refs[k] = kRefEmpty;
+        for (size_t k = 0; k < ch.size(); ++k) {
+            const BuildNode& c = br.nodes[ch[k]];
+            refs[k] = c.left < 0 ? leaf_ref(c) : build(ch[k], depth + 1);
+        }
+        // quantisation grid over the union of the padded child boxes
+        float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (int c : ch)
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = std::min(lo[a], br.nodes[c].lo[a] - pad);
+                hi[a] = std::max(hi[a], br.nodes[c].hi[a] + pad);
+            }
+        GpuNode8 g{};
+        int eb[3];
+        float s[3];
+        for (int a = 0; a < 3; ++a) {
+            g.p[a] = lo[a];
+            eb[a] = exponent_for(hi[a] - lo[a]);
+            s[a] = scale_of(eb[a]);
+        }
+        g.meta = (uint32_t)eb[0] | ((uint32_t)eb[1] << 8) | ((uint32_t)eb[2] << 16) | ((uint32_t)ch.size() << 24);
+        for (int k = 0; k < 8; ++k) g.ref[k] = refs[k];
+        for (size_t k = 0; k < ch.size(); ++k) {
+            const BuildNode& c = br.nodes[ch[k]];
+            for (int a = 0; a < 3; ++a) {
+                const float clo = c.lo[a] - pad, chi = c.hi[a] + pad;
+                long ql = (long)std::floor(((double)clo - (double)g.p[a]) / (double)s[a]);
+                long qh = (long)std::ceil(((double)chi - (double)g.p[a]) / (double)s[a]);
+                ql = std::min(255L, std::max(0L, ql));
+                qh = std::min(255L, std::max(0L, qh));
+                while (ql > 0 && std::fma((float)ql, s[a], g.p[a]) > clo) --ql;
+                while (qh < 255 && std::fma((float)qh, s[a], g.p[a]) < chi) ++qh;
+                // byte positions: lo.x 0-7, lo.y 8-15, lo.z 16-23, hi.x 24-31, hi.y 32-39, hi.z 40-47
+                g.q[8 * a + k] = (uint8_t)ql;
+                g.q[24 + 8 * a + k] = (uint8_t)qh;
+            }
+        }
+        nodes[gi] = g;
+        return gi;
+    }
+};
+
+} // namespace
+
+bool convert_bvh8(const BuildResult& br, const std::vector<float>& verts, const std::vector<uint32_t>& idx, float pad,
+                  std::vector<GpuNode8>& nodes, std::vector<float>& tris, uint64_t& leaves, int& depth) {
+    Collapser c{br, verts, idx, pad, {}, {}, 0, 0};
+    c.nodes.reserve(br.nodes.size() / 4 + 1);
+    c.tris.reserve(12 * br.order.size());
+    c.build(0, 0);
+    nodes.swap(c.nodes);
+    tris.swap(c.tris);
+    leaves = c.leaves;
+    depth = c.max_depth;
+    return true;
+}
+
+} // namespace ar

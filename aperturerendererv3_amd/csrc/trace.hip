@@ -28,8 +28,15 @@ struct RayPre {
     bool valid;
 };
 
+// 1/x for the box tests, magnitude clamped to 2^100 (x = 0 gives +-2^100 by
+// the sign of the zero).  Bounded so that products with the quantised grid
+// scale and with (p - o) stay finite for coordinates within +-2^26: an
+// infinite factor would turn 0 * inf into NaN.
 __device__ __forceinline__ float inv_safe(float x) {
-    return (x == 0.0f) ? (x >= 0.0f ? FLT_MAX : -FLT_MAX) : 1.0f / x;
+    const float big = 0x1p100f;
+    if (x == 0.0f) return __builtin_signbit(x) ? -big : big;
+    const float r = 1.0f / x;
+    return fminf(fmaxf(r, -big), big);
 }
 
 __device__ __forceinline__ float sel3(int k, float x, float y, float z) {
@@ -127,7 +134,7 @@ __device__ __forceinline__ bool box_test(const RayPre& p, float lx, float hx, fl
     const float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), tmin)) - p.tpad;
     const float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tmax)) + p.tpad;
     tnear = tn;
-    return tn <= tf;
+    return !(tn > tf);   // NaN counts as a hit (conservative)
 }
 
 struct Stack {
@@ -146,140 +153,247 @@ struct Stack {
     }
 };
 
-// Traverses one ray.  ANY: stop at the first accepted hit.  Returns the leaf
-// slot of the hit triangle (kRefEmpty on a miss); its t goes to best_t.
-template <bool ANY, bool COUNT>
-__device__ __forceinline__ uint32_t traverse(const GpuNode2* __restrict__ nodes, const float4* __restrict__ tris,
-                                             uint32_t root, const RayPre& p, float tmin, float tmax, Stack& st,
-                                             float& best_t, uint32_t& nbox, uint32_t& ntri) {
-    uint32_t best_p = kRefEmpty;
-    uint32_t best_slot = kRefEmpty;
-    bool have = false;
-    best_t = tmax;
-    if (!p.valid || !(tmax >= tmin) || root == kRefEmpty) return kRefEmpty;
-    st.sp = 0;
-    st.push(kRefEmpty);
-    uint32_t ref = root;
-    while (ref != kRefEmpty) {
-        // interior nodes until a leaf (or nothing) is next
-        while ((int)ref >= 0) {
-            const GpuNode2* nd = nodes + ref;
-            const float4 a = *reinterpret_cast<const float4*>(nd->a);
-            const float4 b = *reinterpret_cast<const float4*>(nd->b);
-            const float4 c = *reinterpret_cast<const float4*>(nd->c);
-            const uint4 d = *reinterpret_cast<const uint4*>(nd->d);
-            float t0, t1;
-            const bool h0 = box_test(p, a.x, a.y, a.z, a.w, c.x, c.y, tmin, best_t, t0);
-            const bool h1 = box_test(p, b.x, b.y, b.z, b.w, c.z, c.w, tmin, best_t, t1);
-            if (COUNT) nbox += 2;
-            if (h0 && h1) {
-                uint32_t nr = d.x, fr = d.y;
-                if (t1 < t0) { nr = d.y; fr = d.x; }
-                st.push(fr);
-                ref = nr;
-            } else if (h0) {
-                ref = d.x;
-            } else if (h1) {
-                ref = d.y;
-            } else {
-                ref = st.pop();
+// Per-lane traversal state, kept in registers across refill rounds.
+struct Trav {
+    RayPre p;
+    float tmin, tmax, best_t;
+    uint32_t ref, best_p, best_slot, idx;
+    bool have;
+};
+
+__device__ __forceinline__ float byte_f(uint32_t w, int k) { return (float)((w >> (8 * k)) & 0xFFu); }
+
+// Interior node of the binary layout: both child boxes from one 64 B node;
+// the nearer hit child is next, the farther one is pushed.
+__device__ __forceinline__ void interior2(const GpuNode2* __restrict__ nodes, Trav& s, Stack& st) {
+    const GpuNode2* nd = nodes + s.ref;
+    const float4 a = *reinterpret_cast<const float4*>(nd->a);
+    const float4 b = *reinterpret_cast<const float4*>(nd->b);
+    const float4 c = *reinterpret_cast<const float4*>(nd->c);
+    const uint4 d = *reinterpret_cast<const uint4*>(nd->d);
+    float t0, t1;
+    const bool h0 = box_test(s.p, a.x, a.y, a.z, a.w, c.x, c.y, s.tmin, s.best_t, t0);
+    const bool h1 = box_test(s.p, b.x, b.y, b.z, b.w, c.z, c.w, s.tmin, s.best_t, t1);
+    if (h0 && h1) {
+        uint32_t nr = d.x, fr = d.y;
+        if (t1 < t0) { nr = d.y; fr = d.x; }
+        st.push(fr);
+        s.ref = nr;
+    } else if (h0) {
+        s.ref = d.x;
+    } else if (h1) {
+        s.ref = d.y;
+    } else {
+        s.ref = st.pop();
+    }
+}
+
+// Interior node of the quantised 8-wide layout.  Hit children's entry
+// distances stay in registers (static indices only); the nearest is next and
+// the rest are pushed far-to-near (closest hit) or in slot order (any hit).
+template <bool ANY>
+__device__ __forceinline__ uint32_t interior8(const GpuNode8* __restrict__ nodes, Trav& s, Stack& st) {
+    const float4* nd = reinterpret_cast<const float4*>(nodes + s.ref);
+    const float4 h = nd[0];
+    const uint4 r0 = *reinterpret_cast<const uint4*>(nd + 1);
+    const uint4 r1 = *reinterpret_cast<const uint4*>(nd + 2);
+    const uint4 q0 = *reinterpret_cast<const uint4*>(nd + 3);
+    const uint4 q1 = *reinterpret_cast<const uint4*>(nd + 4);
+    const uint4 q2 = *reinterpret_cast<const uint4*>(nd + 5);
+    const uint32_t meta = __float_as_uint(h.w);
+    const uint32_t nchild = meta >> 24;
+    const RayPre& p = s.p;
+    const float ax = __uint_as_float((meta & 0xFFu) << 23) * p.idx;
+    const float ay = __uint_as_float(((meta >> 8) & 0xFFu) << 23) * p.idy;
+    const float az = __uint_as_float(((meta >> 16) & 0xFFu) << 23) * p.idz;
+    const float bx = (h.x - p.ox) * p.idx;
+    const float by = (h.y - p.oy) * p.idy;
+    const float bz = (h.z - p.oz) * p.idz;
+    const uint32_t refs[8] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w};
+    const uint32_t lox[2] = {q0.x, q0.y}, loy[2] = {q0.z, q0.w}, loz[2] = {q1.x, q1.y};
+    const uint32_t hix[2] = {q1.z, q1.w}, hiy[2] = {q2.x, q2.y}, hiz[2] = {q2.z, q2.w};
+    float tn[8];
+    uint32_t mask = 0;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        const int w = c >> 2, k = c & 3;
+        const float x0 = fmaf(byte_f(lox[w], k), ax, bx), x1 = fmaf(byte_f(hix[w], k), ax, bx);
+        const float y0 = fmaf(byte_f(loy[w], k), ay, by), y1 = fmaf(byte_f(hiy[w], k), ay, by);
+        const float z0 = fmaf(byte_f(loz[w], k), az, bz), z1 = fmaf(byte_f(hiz[w], k), az, bz);
+        const float a = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), s.tmin)) - p.tpad;
+        const float b = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), s.best_t)) + p.tpad;
+        tn[c] = a;
+        if ((uint32_t)c < nchild && !(a > b)) mask |= 1u << c;   // NaN counts as a hit
+    }
+    if (mask == 0) {
+        s.ref = st.pop();
+        return nchild;
+    }
+    if (ANY) {
+        uint32_t next = kRefEmpty;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            if ((mask >> c) & 1u) {
+                if (next == kRefEmpty) next = refs[c];
+                else st.push(refs[c]);
             }
         }
-        // leaves until an interior node (or the sentinel) is next
-        while ((int)ref < 0 && ref != kRefEmpty) {
-            const uint32_t first = ref & kMaxTriSlots;
-            const uint32_t cnt = ((ref >> 27) & 15u) + 1u;
-            for (uint32_t k = 0; k < cnt; ++k) {
-                const float4* tp = tris + 3 * (size_t)(first + k);
-                const float4 va = tp[0], vb = tp[1], vc = tp[2];
-                const uint32_t prim = __float_as_uint(va.w);
-                float t;
-                if (COUNT) ++ntri;
-                if (!tri_test<false>(p, va, vb, vc, t)) continue;
-                const bool accept = t >= tmin &&
-                    (have ? (t < best_t || (t == best_t && prim < best_p)) : t < tmax);
-                if (accept) {
-                    have = true;
-                    best_t = t;
-                    best_p = prim;
-                    best_slot = first + k;
-                    if (ANY) return best_slot;
+        s.ref = next;
+    } else {
+        float nt = INFINITY;
+        uint32_t next = kRefEmpty;
+        int ni = 0;
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+            if (((mask >> c) & 1u) && tn[c] < nt) { nt = tn[c]; next = refs[c]; ni = c; }
+        mask &= ~(1u << ni);
+        while (mask) {
+            float ft = -INFINITY;
+            uint32_t fr = 0;
+            int fi = 0;
+#pragma unroll
+            for (int c = 0; c < 8; ++c)
+                if (((mask >> c) & 1u) && tn[c] >= ft) { ft = tn[c]; fr = refs[c]; fi = c; }
+            st.push(fr);
+            mask &= ~(1u << fi);
+        }
+        s.ref = next;
+    }
+    return nchild;
+}
+
+// Leaf: every triangle in it, hit rule "smallest t, then smallest id".
+// Any hit ends the traversal on its first accepted triangle.
+template <bool ANY>
+__device__ __forceinline__ uint32_t leaf(const float4* __restrict__ tris, Trav& s, Stack& st) {
+    const uint32_t first = s.ref & kMaxTriSlots;
+    const uint32_t cnt = ((s.ref >> 27) & 15u) + 1u;
+    for (uint32_t k = 0; k < cnt; ++k) {
+        const float4* tp = tris + 3 * (size_t)(first + k);
+        const float4 va = tp[0], vb = tp[1], vc = tp[2];
+        const uint32_t prim = __float_as_uint(va.w);
+        float t;
+        if (!tri_test<false>(s.p, va, vb, vc, t)) continue;
+        const bool accept = t >= s.tmin &&
+            (s.have ? (t < s.best_t || (t == s.best_t && prim < s.best_p)) : t < s.tmax);
+        if (accept) {
+            s.have = true;
+            s.best_t = t;
+            s.best_p = prim;
+            s.best_slot = first + k;
+            if (ANY) { s.ref = kRefEmpty; return k + 1; }
+        }
+    }
+    s.ref = st.pop();
+    return cnt;
+}
+
+constexpr int kStepBudget = 12;   // traversal steps between two ray-refill rounds
+
+// Persistent traversal with per-lane ray refill (Aila & Laine, HPG'09): a
+// lane whose ray is finished takes the next ray index from a global counter
+// (one atomic per wave and round), so a wave never idles on its slowest ray.
+// ANY: sets bit idx of occ (zeroed beforehand) for occluded rays.
+template <int W, bool ANY, bool COUNT>
+__global__ __launch_bounds__(kTraceBlock) void k_trace(
+    const void* __restrict__ nodes, const float4* __restrict__ tris, uint32_t root, float origin_limit,
+    ar_rays r, size_t n, const uint32_t* __restrict__ n_dev, ar_hits h, uint32_t* __restrict__ occ,
+    int* __restrict__ spill, uint32_t threads, uint32_t* __restrict__ cnt_box, uint32_t* __restrict__ cnt_tri,
+    uint32_t* __restrict__ fetch) {
+    __shared__ int lds_stack[kLdsStack * kTraceBlock];
+    if (n_dev) n = *n_dev;
+    const uint32_t gtid = blockIdx.x * kTraceBlock + threadIdx.x;
+    const uint32_t lane = __lane_id();
+    Stack st;
+    st.lds = lds_stack + threadIdx.x;
+    st.spill = spill + gtid;
+    st.threads = threads;
+    st.sp = 0;
+    Trav s;
+    s.ref = kRefEmpty;
+    bool active = false, exhausted = false;
+    uint32_t nb = 0, nt = 0;
+    while (true) {
+        const bool need = !active && !exhausted;
+        const uint64_t nm = __ballot(need);
+        if (nm) {
+            const uint32_t leader = (uint32_t)__ffsll((unsigned long long)nm) - 1u;
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(fetch, (uint32_t)__popcll(nm));
+            base = __shfl(base, (int)leader);
+            if (need) {
+                const uint32_t idx = base + (uint32_t)__popcll(nm & ((1ull << lane) - 1ull));
+                if (idx < n) {
+                    s.idx = idx;
+                    s.p = ray_setup(r.ox[idx], r.oy[idx], r.oz[idx], r.dx[idx], r.dy[idx], r.dz[idx], origin_limit);
+                    s.tmin = r.tmin[idx];
+                    s.tmax = r.tmax[idx];
+                    s.best_t = s.tmax;
+                    s.best_p = kRefEmpty;
+                    s.best_slot = kRefEmpty;
+                    s.have = false;
+                    st.sp = 0;
+                    st.push(kRefEmpty);
+                    s.ref = (s.p.valid && s.tmax >= s.tmin) ? root : kRefEmpty;
+                    nb = 0;
+                    nt = 0;
+                    active = true;
+                } else {
+                    exhausted = true;
                 }
             }
-            ref = st.pop();
+        }
+        if (__ballot(active) == 0) break;
+        if (active) {
+            for (int it = 0; it < kStepBudget && s.ref != kRefEmpty; ++it) {
+                if ((int)s.ref >= 0) {
+                    if constexpr (W == 8) {
+                        const uint32_t c = interior8<ANY>(reinterpret_cast<const GpuNode8*>(nodes), s, st);
+                        if (COUNT) nb += c;
+                    } else {
+                        interior2(reinterpret_cast<const GpuNode2*>(nodes), s, st);
+                        if (COUNT) nb += 2;
+                    }
+                } else {
+                    const uint32_t c = leaf<ANY>(tris, s, st);
+                    if (COUNT) nt += c;
+                }
+            }
+            if (s.ref == kRefEmpty) {
+                const uint32_t i = s.idx;
+                if constexpr (ANY) {
+                    if (s.have) atomicOr(&occ[i >> 5], 1u << (i & 31u));
+                } else {
+                    if (!s.have) {
+                        h.prim[i] = AR_MISS;
+                        h.t[i] = __int_as_float(0x7f800000);
+                        h.u[i] = 0.0f; h.v[i] = 0.0f; h.flags[i] = 0u;
+                    } else {
+                        const float4* tp = tris + 3 * (size_t)s.best_slot;
+                        const float4 va = tp[0], vb = tp[1], vc = tp[2];
+                        float tt, u = 0.0f, v = 0.0f;
+                        uint32_t flags = 0;
+                        tri_test<true>(s.p, va, vb, vc, tt, &u, &v, &flags);
+                        h.prim[i] = __float_as_uint(va.w);
+                        h.t[i] = s.best_t;
+                        h.u[i] = u; h.v[i] = v; h.flags[i] = flags;
+                    }
+                    if (COUNT) { cnt_box[i] = nb; cnt_tri[i] = nt; }
+                }
+                active = false;
+            }
         }
     }
-    return have ? best_slot : kRefEmpty;
 }
 
-template <bool COUNT>
-__global__ __launch_bounds__(kTraceBlock) void k_trace_closest(
-    const GpuNode2* __restrict__ nodes, const float4* __restrict__ tris, uint32_t root, float origin_limit,
-    ar_rays r, size_t n, const uint32_t* __restrict__ n_dev, ar_hits h, int* __restrict__ spill, uint32_t threads,
-    uint32_t* __restrict__ cnt_box, uint32_t* __restrict__ cnt_tri) {
-    __shared__ int lds_stack[kLdsStack * kTraceBlock];
+// Clears the refill counter and, for any-hit, the occlusion words of the
+// rays about to be traced (n read from n_dev when given).
+__global__ void k_trace_prepare(uint32_t* __restrict__ fetch, uint32_t* __restrict__ occ, size_t n,
+                                const uint32_t* __restrict__ n_dev, size_t cap_words) {
+    const size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (n_dev) n = *n_dev;
-    const uint32_t gtid = blockIdx.x * kTraceBlock + threadIdx.x;
-    Stack st;
-    st.lds = lds_stack + threadIdx.x;
-    st.spill = spill + gtid;
-    st.threads = threads;
-    for (size_t i = gtid; i < n; i += threads) {
-        const float ox = r.ox[i], oy = r.oy[i], oz = r.oz[i], tmin = r.tmin[i];
-        const float dx = r.dx[i], dy = r.dy[i], dz = r.dz[i], tmax = r.tmax[i];
-        const RayPre p = ray_setup(ox, oy, oz, dx, dy, dz, origin_limit);
-        float t;
-        uint32_t nb = 0, nt = 0;
-        const uint32_t slot = traverse<false, COUNT>(nodes, tris, root, p, tmin, tmax, st, t, nb, nt);
-        if (slot == kRefEmpty) {
-            h.prim[i] = AR_MISS;
-            h.t[i] = __int_as_float(0x7f800000);
-            h.u[i] = 0.0f; h.v[i] = 0.0f; h.flags[i] = 0u;
-        } else {
-            const float4* tp = tris + 3 * (size_t)slot;
-            const float4 va = tp[0], vb = tp[1], vc = tp[2];
-            float tt, u = 0.0f, v = 0.0f;
-            uint32_t flags = 0;
-            tri_test<true>(p, va, vb, vc, tt, &u, &v, &flags);
-            h.prim[i] = __float_as_uint(va.w);
-            h.t[i] = t;
-            h.u[i] = u; h.v[i] = v; h.flags[i] = flags;
-        }
-        if (COUNT) { cnt_box[i] = nb; cnt_tri[i] = nt; }
-    }
-}
-
-// Any hit.  Each wave owns 64 consecutive rays per iteration (threads is a
-// multiple of 64), so the wave's ballot is exactly two occlusion words.
-__global__ __launch_bounds__(kTraceBlock) void k_trace_any(
-    const GpuNode2* __restrict__ nodes, const float4* __restrict__ tris, uint32_t root, float origin_limit,
-    ar_rays r, size_t n, const uint32_t* __restrict__ n_dev, uint32_t* __restrict__ occ, int* __restrict__ spill,
-    uint32_t threads) {
-    __shared__ int lds_stack[kLdsStack * kTraceBlock];
-    if (n_dev) n = *n_dev;
-    const uint32_t gtid = blockIdx.x * kTraceBlock + threadIdx.x;
-    const uint32_t lane = threadIdx.x & 63u;
-    Stack st;
-    st.lds = lds_stack + threadIdx.x;
-    st.spill = spill + gtid;
-    st.threads = threads;
-    const size_t nwords = (n + 31) / 32;
-    for (size_t base = gtid - lane; base < n; base += threads) {
-        const size_t i = base + lane;
-        bool hit = false;
-        if (i < n) {
-            const float ox = r.ox[i], oy = r.oy[i], oz = r.oz[i], tmin = r.tmin[i];
-            const float dx = r.dx[i], dy = r.dy[i], dz = r.dz[i], tmax = r.tmax[i];
-            const RayPre p = ray_setup(ox, oy, oz, dx, dy, dz, origin_limit);
-            float t;
-            uint32_t nb = 0, nt = 0;
-            hit = traverse<true, false>(nodes, tris, root, p, tmin, tmax, st, t, nb, nt) != kRefEmpty;
-        }
-        const uint64_t m = __ballot(hit);
-        const size_t w = base / 32;
-        if (lane == 0) occ[w] = (uint32_t)m;
-        if (lane == 32 && w + 1 < nwords) occ[w + 1] = (uint32_t)(m >> 32);
-    }
+    if (w == 0) *fetch = 0u;
+    if (occ && w < cap_words && w < (n + 31) / 32) occ[w] = 0u;
 }
 
 // Shadow segment from each closest hit toward a point (the traversal
@@ -317,30 +431,61 @@ __global__ void k_shadow_rays(ar_rays r, ar_hits h, size_t n, float tx, float ty
 
 } // namespace
 
-int trace_blocks_per_cu() {
-    int b0 = 0, b1 = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b0, k_trace_closest<false>, kTraceBlock, 0) != hipSuccess) b0 = 1;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b1, k_trace_any, kTraceBlock, 0) != hipSuccess) b1 = 1;
-    return std::max(1, std::max(b0, b1));
+// Resident blocks per CU of a traversal kernel (0: the largest over every
+// variant, used to size the stack spill buffer).
+int trace_blocks_per_cu(int width, bool any) {
+    int b = 0;
+    hipError_t e;
+    if (width == 8) e = any ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_trace<8, true, false>, kTraceBlock, 0)
+                            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_trace<8, false, false>, kTraceBlock, 0);
+    else if (width == 2) e = any ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_trace<2, true, false>, kTraceBlock, 0)
+                                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_trace<2, false, false>, kTraceBlock, 0);
+    else {
+        int m = 1;
+        for (int w : {2, 8})
+            for (bool a : {false, true}) m = std::max(m, trace_blocks_per_cu(w, a));
+        return m;
+    }
+    return (e == hipSuccess && b > 0) ? b : 1;
+}
+
+static hipError_t prepare(const TraceArgs& ta, uint32_t* occ, size_t n, const uint32_t* n_dev, hipStream_t s) {
+    const size_t words = (n + 31) / 32;
+    const size_t grid = std::max<size_t>(1, (words + 255) / 256);
+    hipLaunchKernelGGL(k_trace_prepare, dim3((uint32_t)grid), dim3(256), 0, s, ta.fetch, occ, n, n_dev, words);
+    return hipGetLastError();
 }
 
 hipError_t launch_trace_closest(const TraceArgs& ta, const ar_rays& r, size_t n, const ar_hits& h,
                                 uint32_t* cnt_box, uint32_t* cnt_tri, uint32_t blocks, hipStream_t s,
                                 const uint32_t* n_dev) {
-    if (cnt_box) {
-        hipLaunchKernelGGL(k_trace_closest<true>, dim3(blocks), dim3(kTraceBlock), 0, s, ta.nodes, ta.tris, ta.root,
-                           ta.origin_limit, r, n, n_dev, h, ta.spill, ta.threads, cnt_box, cnt_tri);
+    hipError_t e = prepare(ta, nullptr, n, n_dev, s);
+    if (e != hipSuccess) return e;
+    const void* nd = ta.nodes;
+#define AR_LAUNCH_CLOSEST(W, C)                                                                                  \
+    hipLaunchKernelGGL((k_trace<W, false, C>), dim3(blocks), dim3(kTraceBlock), 0, s, nd, ta.tris, ta.root,     \
+                       ta.origin_limit, r, n, n_dev, h, nullptr, ta.spill, ta.threads, cnt_box, cnt_tri, ta.fetch)
+    if (ta.width == 8) {
+        if (cnt_box) AR_LAUNCH_CLOSEST(8, true); else AR_LAUNCH_CLOSEST(8, false);
     } else {
-        hipLaunchKernelGGL(k_trace_closest<false>, dim3(blocks), dim3(kTraceBlock), 0, s, ta.nodes, ta.tris, ta.root,
-                           ta.origin_limit, r, n, n_dev, h, ta.spill, ta.threads, nullptr, nullptr);
+        if (cnt_box) AR_LAUNCH_CLOSEST(2, true); else AR_LAUNCH_CLOSEST(2, false);
     }
+#undef AR_LAUNCH_CLOSEST
     return hipGetLastError();
 }
 
 hipError_t launch_trace_any(const TraceArgs& ta, const ar_rays& r, size_t n, uint32_t* occ, uint32_t blocks,
                             hipStream_t s, const uint32_t* n_dev) {
-    hipLaunchKernelGGL(k_trace_any, dim3(blocks), dim3(kTraceBlock), 0, s, ta.nodes, ta.tris, ta.root,
-                       ta.origin_limit, r, n, n_dev, occ, ta.spill, ta.threads);
+    hipError_t e = prepare(ta, occ, n, n_dev, s);
+    if (e != hipSuccess) return e;
+    const void* nd = ta.nodes;
+    const ar_hits none{nullptr, nullptr, nullptr, nullptr, nullptr};
+    if (ta.width == 8)
+        hipLaunchKernelGGL((k_trace<8, true, false>), dim3(blocks), dim3(kTraceBlock), 0, s, nd, ta.tris, ta.root,
+                           ta.origin_limit, r, n, n_dev, none, occ, ta.spill, ta.threads, nullptr, nullptr, ta.fetch);
+    else
+        hipLaunchKernelGGL((k_trace<2, true, false>), dim3(blocks), dim3(kTraceBlock), 0, s, nd, ta.tris, ta.root,
+                           ta.origin_limit, r, n, n_dev, none, occ, ta.spill, ta.threads, nullptr, nullptr, ta.fetch);
     return hipGetLastError();
 }
 

@@ -52,7 +52,8 @@ def parse_args():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--levels", type=int, default=9, help="subdivision levels (9 = 1,048,576 triangles)")
     ap.add_argument("--res", type=int, default=4096, help="ray grid side (4096 = 16,777,216 slots)")
-    ap.add_argument("--leaf-size", type=int, default=4)
+    ap.add_argument("--leaf-size", type=int, default=3)
+    ap.add_argument("--width", type=int, default=8, help="BVH branching factor (2 or 8)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-stride", type=int, default=2)
     ap.add_argument("--profile", action="store_true",
@@ -144,7 +145,7 @@ def main():
     n = rays_np.shape[1]
     ctx = Context(0)
     ctx.set_meshes([{"vertices": V, "indices": F}])
-    info = ctx.build_bvh(leaf_size=args.leaf_size)
+    info = ctx.build_bvh(width=args.width, leaf_size=args.leaf_size)
     dev = torch.device("cuda:0")
     rays = torch.from_numpy(rays_np).to(dev)
     hits = ctx.alloc_hits(n, dev)
@@ -251,7 +252,7 @@ def main():
                                f"{F.shape[0]} triangles, {args.res}x{args.res} ray slots (50% pinhole primary, "
                                "50% cosine-incoherent), closest hit + any-hit shadow segments",
                    "triangles": int(F.shape[0]), "rays": n, "any_hit_queries": n_any,
-                   "bvh": {"width": 2, "leaf_size": args.leaf_size, "nodes": info["node_count"],
+                   "bvh": {"width": args.width, "leaf_size": args.leaf_size, "nodes": info["node_count"],
                            "depth": info["depth"], "build_s": round(info["build_seconds"], 3)},
                    "parallelism": "single GPU"},
         "roofline": {"bound": "valu", "kernel": "k_trace_closest", "achieved": round(achieved_tops, 4),

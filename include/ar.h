@@ -98,8 +98,8 @@ typedef struct {
 } ar_hits;
 
 typedef struct {
-    uint32_t width;      /* branching factor; 2 supported */
-    uint32_t leaf_size;  /* max triangles per leaf, 1..15 */
+    uint32_t width;      /* branching factor: 2 (64 B nodes) or 8 (quantised 96 B nodes, default) */
+    uint32_t leaf_size;  /* max triangles per leaf, 1..15 (default 3) */
     uint32_t bin_count;  /* SAH bins, 4..64 */
     uint32_t max_depth;  /* tree depth limit, <= 64 */
 } ar_bvh_opts;

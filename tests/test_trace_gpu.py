@@ -29,10 +29,10 @@ def ctx():
     c.close()
 
 
-def gpu_closest(ctx, V, F, rays, leaf_size=4, bins=16):
+def gpu_closest(ctx, V, F, rays, leaf_size=4, bins=16, width=8):
     torch = _torch()
     ctx.set_meshes([{"vertices": V, "indices": F}])
-    ctx.build_bvh(leaf_size=leaf_size, bin_count=bins)
+    ctx.build_bvh(width=width, leaf_size=leaf_size, bin_count=bins)
     r = torch.from_numpy(np.ascontiguousarray(rays)).cuda()
     h = ctx.trace_closest(r)
     occ = ctx.trace_any(r)
@@ -65,24 +65,26 @@ CASES = [
 ]
 
 
+@pytest.mark.parametrize("width", [2, 8])
 @pytest.mark.parametrize("case", range(len(CASES)), ids=[c[0] for c in CASES])
-def test_closest_and_any_match_brute_force(ctx, case):
+def test_closest_and_any_match_brute_force(ctx, case, width):
     name, mk_mesh, mk_rays = CASES[case]
     V, F = mk_mesh()
     rays = mk_rays(V, F)
-    print(f"case {name}: {F.shape[0]} tris, {rays.shape[1]} rays", flush=True)
-    got, occ = gpu_closest(ctx, V, F, rays)
+    print(f"case {name} width {width}: {F.shape[0]} tris, {rays.shape[1]} rays", flush=True)
+    got, occ = gpu_closest(ctx, V, F, rays, width=width)
     ref = orc.brute_closest(V, F, rays)
     assert_hits_equal(got, ref, name)
     ref_occ = orc.brute_any(V, F, rays)
     assert np.array_equal(occ, ref_occ), f"{name}: occlusion words differ"
 
 
-@pytest.mark.parametrize("leaf_size,bins", [(1, 4), (8, 12), (15, 64)])
-def test_bvh_parameters_do_not_change_results(ctx, leaf_size, bins):
+@pytest.mark.parametrize("width,leaf_size,bins", [(2, 1, 4), (2, 8, 12), (2, 15, 64), (8, 1, 16), (8, 3, 8),
+                                                  (8, 15, 64)])
+def test_bvh_parameters_do_not_change_results(ctx, width, leaf_size, bins):
     V, F = geom.sphere_mesh(5)
     rays = geom.random_rays(20000, seed=7)
-    got, occ = gpu_closest(ctx, V, F, rays, leaf_size=leaf_size, bins=bins)
+    got, occ = gpu_closest(ctx, V, F, rays, leaf_size=leaf_size, bins=bins, width=width)
     ref = orc.OracleBVH(V, F).closest(rays)
     assert_hits_equal(got, ref, f"leaf={leaf_size} bins={bins}")
 
@@ -118,10 +120,11 @@ def test_scaled_traversal_workload(ctx):
     from aperturerendererv3_amd.workloads import traversal_mesh, traversal_rays
     V, F = traversal_mesh(levels=6)
     rays = traversal_rays(V, F, res=512)
-    got, _ = gpu_closest(ctx, V, F, rays)
-    obvh = orc.OracleBVH(V, F)
-    ref = obvh.closest(rays)
-    assert_hits_equal(got, ref, "scaled workload")
+    for width in (2, 8):
+        got, _ = gpu_closest(ctx, V, F, rays, width=width)
+        obvh = orc.OracleBVH(V, F)
+        ref = obvh.closest(rays)
+        assert_hits_equal(got, ref, f"scaled workload width {width}")
     sub = np.random.default_rng(0).choice(rays.shape[1], 2000, replace=False)
     bref = orc.brute_closest(V, F, rays[:, sub].copy())
     assert np.array_equal(ref["prim"][sub], bref["prim"])
