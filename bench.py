@@ -42,7 +42,7 @@ METRIC = "Grays/s traversal; paths/s at equal FP64-oracle relMSE, plain vs batch
 VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12      # 39.3 Tops/s
 HBM_PEAK_GBPS = 8000.0
 LIGHT = (2.0, 2.0, 2.0)
-TRAFFIC_FILE = ROOT / "profiles" / "trace_closest_traversal_1M_tris_16M_rays.md"
+TRAFFIC_FILE = ROOT / "profiles" / "k_trace_closest_bvh8_traversal_1M_tris_16M_rays.md"
 
 
 def parse_args():
