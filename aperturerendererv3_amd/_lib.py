@@ -67,12 +67,18 @@ class ar_camera(C.Structure):
 class ar_render_opts(C.Structure):
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32), ("sample_offset", C.c_uint32),
                 ("seed", C.c_uint64), ("max_depth", C.c_uint32), ("rr_min_depth", C.c_uint32), ("nee", C.c_uint32),
-                ("sky", C.c_uint32), ("env", C.c_float * 3), ("max_paths_per_pass", C.c_uint32)]
+                ("sky", C.c_uint32), ("env", C.c_float * 3), ("max_paths_per_pass", C.c_uint32),
+                ("spectral", C.c_uint32), ("output", C.c_uint32)]
 
 
 AR_SKY_NONE = 0
 AR_SKY_GRADIENT = 1
 AR_SKY_CONSTANT = 2
+AR_SPECTRAL_RGB = 0
+AR_SPECTRAL_SINGLE = 1
+AR_SPECTRAL_HERO4 = 4
+AR_OUTPUT_XYZ = 0
+AR_OUTPUT_SRGB = 1
 
 # name -> (restype, argtypes)
 SIGNATURES = {
@@ -91,6 +97,8 @@ SIGNATURES = {
                                       C.POINTER(C.c_float * 3), C.c_float, C.c_float, C.POINTER(ar_rays),
                                       C.c_void_p]),
     "ar_render": (C.c_int, [C.c_void_p, C.POINTER(ar_camera), C.POINTER(ar_render_opts), C.c_void_p, C.c_void_p]),
+    "ar_spectrum_lut_build": (C.c_int, [C.c_void_p, C.c_size_t]),
+    "ar_spectrum_uplift": (C.c_int, [C.c_void_p, C.c_size_t, C.c_void_p]),
 }
 
 _lib = None

@@ -26,6 +26,7 @@ COMMON = [
     "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
     "-Wall", "-Wno-unused-function",
     f"-I{PKG.parent / 'include'}",
+    f"-I{PKG / '_build'}",
 ]
 DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-munsafe-fp-atomics"]
@@ -53,8 +54,26 @@ def _compile(src: Path) -> Path:
     return obj
 
 
+DATA = PKG / "data" / "cie1931_d65_1nm.txt"
+
+
+def _gen_tables():
+    """Embeds the CIE 1931 / D65 table (data/cie1931_d65_1nm.txt) as C arrays."""
+    rows = [ln.split() for ln in DATA.read_text().splitlines() if ln.strip() and not ln.startswith("#")]
+    assert len(rows) == 421
+    xyz = ",\n".join(f"{float(r[1])!r}f, {float(r[2])!r}f, {float(r[3])!r}f" for r in rows)
+    d65 = ",\n".join(f"{float(r[4])!r}f" for r in rows)
+    text = ("// generated from data/cie1931_d65_1nm.txt by build.py\n"
+            f"static const float kCieXyz1nm[421 * 3] = {{\n{xyz}\n}};\n"
+            f"static const float kD65_1nm[421] = {{\n{d65}\n}};\n")
+    out = BUILD_DIR / "cie_tables.inc"
+    if not out.exists() or out.read_text() != text:
+        out.write_text(text)
+
+
 def build(verbose: bool = False) -> Path:
     BUILD_DIR.mkdir(exist_ok=True)
+    _gen_tables()
     srcs = _sources()
     with ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
         objs = list(ex.map(_compile, srcs))

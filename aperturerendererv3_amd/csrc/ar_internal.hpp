@@ -111,7 +111,9 @@ struct DeviceBVH {
     uint32_t stack_entries = kStackMax;  // per-ray traversal stack bound (incl. sentinel)
 };
 
-// Shading data of the scene, prim order (uploaded with the BVH).
+// Shading data of the scene, prim order (uploaded with the BVH).  The
+// spectral fields hold the sigmoid coefficients of the albedo and of the
+// emission normalised by its largest channel (escale = that channel if > 1).
 struct GpuMaterial {
     float albedo[3];
     float specular;
@@ -119,9 +121,22 @@ struct GpuMaterial {
     float metallic;
     float roughness;
     uint32_t model;
-    float pad[2];
+    float escale;
+    float pad0;
+    float calb[3];
+    float pad1;
+    float cem[3];
+    float pad2;
 };
-static_assert(sizeof(GpuMaterial) == 48, "GpuMaterial must be 48 bytes");
+static_assert(sizeof(GpuMaterial) == 80, "GpuMaterial must be 80 bytes");
+
+namespace spec {
+size_t lut_floats();
+const std::vector<float>& uplift_lut(const float* xyz, const float* d65);
+void uplift(const float* lut, const float rgb[3], float c[3]);
+const float* cie_xyz_table();   // 421 x 3, 360..780 nm
+const float* d65_table();       // 421
+}
 
 struct SceneDev {
     float4* prims = nullptr;      // 3 per triangle: (v0, material bits), (v1, light index bits), (v2, 0)
@@ -132,6 +147,8 @@ struct SceneDev {
     float* light_cdf = nullptr;
     uint32_t nlights = 0;
     uint32_t ntri = 0;
+    std::vector<GpuMaterial> host_mats;   // host copy (spectral coefficients filled on first spectral render)
+    bool spectral_ready = false;
 };
 
 // Wavefront state of the integrator, sized for `capacity` paths.
@@ -143,18 +160,21 @@ struct RenderBuffers {
     uint32_t* hit_prim; float* hit_t; float* hit_u; float* hit_v; uint32_t* hit_flags;
     float* srays[8];              // shadow rays
     uint32_t* spath;
-    float* scontrib[3];
+    float* scontrib[4];
     uint32_t* sflags;             // bit0: the sampled light point faced the surface (both cosines > 0)
     uint32_t* occ;                // shadow occlusion bits
-    float* mask[3];
-    float* rad[3];
+    float* mask[4];
+    float* rad[4];
     float* last_pos[3];
     float* last_pdf;
     uint32_t* meta;               // non-delta depth (bits 0..15), previous vertex took NEE (bit 16)
     uint32_t* counters;           // [0] current queue, [1] next queue, [2] shadow queue
+    float* lambda0;               // per path: first wavelength (spectral modes)
     float* accum;                 // width*height*3 running sums
     size_t accum_pixels = 0;
     float* eps_table;             // INTERSECT_EPS(d) for d < 256
+    float* cie;                   // device copies of the CIE 1931 / D65 tables
+    float* d65;
 };
 
 } // namespace ar

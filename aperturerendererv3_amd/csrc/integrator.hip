@@ -17,6 +17,7 @@
 // equal seed the GPU and the oracle trace the same paths except where a
 // transcendental (sinf/cosf) differs in its last bit.
 #include "ar_internal.hpp"
+#include "spectral.hpp"
 
 #include <cfloat>
 #include <cmath>
@@ -56,6 +57,18 @@ __device__ __forceinline__ float fixnan(float x) { return (isnan(x) || isinf(x))
 __device__ __forceinline__ F3 xyz(float4 v) { return {v.x, v.y, v.z}; }
 
 constexpr float kPi = 3.14159265358979323846f;
+
+// Per-path spectral quantity: RGB in channels 0..2 (RGB mode), one wavelength
+// in channel 0 (single-wavelength mode) or four hero wavelengths.
+struct S4 { float v[4]; };
+__device__ __forceinline__ S4 s4(float a, float b, float c, float d) { return {{a, b, c, d}}; }
+__device__ __forceinline__ S4 s4mul(const S4& a, const S4& b) {
+    return {{a.v[0] * b.v[0], a.v[1] * b.v[1], a.v[2] * b.v[2], a.v[3] * b.v[3]}};
+}
+__device__ __forceinline__ S4 s4add(const S4& a, const S4& b) {
+    return {{a.v[0] + b.v[0], a.v[1] + b.v[1], a.v[2] + b.v[2], a.v[3] + b.v[3]}};
+}
+__device__ __forceinline__ S4 s4scale(const S4& a, float s) { return {{a.v[0] * s, a.v[1] * s, a.v[2] * s, a.v[3] * s}}; }
 
 // Philox4x32-10 (Salmon et al., SC'11)
 __device__ __forceinline__ void philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1,
@@ -188,13 +201,14 @@ __device__ __forceinline__ float vndf_pdf(F3 v, F3 n, F3 l, float alpha) {
 // ---------------------------------------------------------------- BSDF
 struct Mat {
     F3 albedo, emission;
-    float specular, metallic, roughness;
+    float specular, metallic, roughness, escale;
     uint32_t model;
+    float ca[3], ce[3];
 };
 
 __device__ __forceinline__ Mat load_mat(const GpuMaterial* __restrict__ mats, uint32_t id) {
     const float4* m4 = reinterpret_cast<const float4*>(mats + id);
-    const float4 a = m4[0], b = m4[1], c = m4[2];
+    const float4 a = m4[0], b = m4[1], c = m4[2], d = m4[3], e = m4[4];
     Mat m;
     m.albedo = f3(a.x, a.y, a.z);
     m.specular = a.w;
@@ -202,7 +216,64 @@ __device__ __forceinline__ Mat load_mat(const GpuMaterial* __restrict__ mats, ui
     m.metallic = b.w;
     m.roughness = c.x;
     m.model = __float_as_uint(c.y);
+    m.escale = c.z;
+    m.ca[0] = d.x; m.ca[1] = d.y; m.ca[2] = d.z;
+    m.ce[0] = e.x; m.ce[1] = e.y; m.ce[2] = e.z;
     return m;
+}
+
+// Wavelengths of a path: spectral mode and the first wavelength.
+struct Lam {
+    uint32_t mode;   // AR_SPECTRAL_*
+    float l[4];
+};
+
+__device__ __forceinline__ Lam make_lam(uint32_t mode, float l0) {
+    Lam L;
+    L.mode = mode;
+    L.l[0] = l0;
+#pragma unroll
+    for (int k = 1; k < 4; ++k) {
+        float x = l0 - 380.0f + 100.0f * (float)k;
+        if (x >= 400.0f) x -= 400.0f;
+        L.l[k] = 380.0f + x;
+    }
+    return L;
+}
+
+// Reflectance of an RGB albedo in this path's channels.
+__device__ __forceinline__ S4 albedo_of(const Mat& m, const Lam& L) {
+    if (L.mode == AR_SPECTRAL_RGB) return s4(m.albedo.x, m.albedo.y, m.albedo.z, 0.0f);
+    S4 r;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        r.v[k] = (L.mode == AR_SPECTRAL_HERO4 || k == 0) ? spec::sigmoid_poly(m.ca[0], m.ca[1], m.ca[2], L.l[k]) : 0.0f;
+    return r;
+}
+
+// Emitted radiance (Interact * D65Norm of render.cuh:657-700 in spectral modes)
+__device__ __forceinline__ S4 emission_of(const Mat& m, const Lam& L, const float* __restrict__ d65) {
+    if (L.mode == AR_SPECTRAL_RGB) return s4(m.emission.x, m.emission.y, m.emission.z, 0.0f);
+    S4 r;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        r.v[k] = (L.mode == AR_SPECTRAL_HERO4 || k == 0)
+                     ? spec::sigmoid_poly(m.ce[0], m.ce[1], m.ce[2], L.l[k]) * m.escale * spec::d65_norm(d65, L.l[k])
+                     : 0.0f;
+    return r;
+}
+
+// luminance and largest channel of a throughput (RGB: Rec.709 luminance;
+// one wavelength: the value itself; hero wavelengths: mean and max)
+__device__ __forceinline__ float lum4(const S4& c, uint32_t mode) {
+    if (mode == AR_SPECTRAL_RGB) return c.v[0] * 0.2126f + c.v[1] * 0.7152f + c.v[2] * 0.0722f;
+    if (mode == AR_SPECTRAL_SINGLE) return c.v[0];
+    return (c.v[0] + c.v[1] + c.v[2] + c.v[3]) * 0.25f;
+}
+__device__ __forceinline__ float max4(const S4& c, uint32_t mode) {
+    if (mode == AR_SPECTRAL_RGB) return maxr(c.v[0], maxr(c.v[1], c.v[2]));
+    if (mode == AR_SPECTRAL_SINGLE) return c.v[0];
+    return maxr(maxr(c.v[0], c.v[1]), maxr(c.v[2], c.v[3]));
 }
 
 __device__ __forceinline__ float f0_ue(float spec) { return 0.08f * clampr(spec, 0.0f, 1.0f); }
@@ -215,58 +286,68 @@ __device__ __forceinline__ float g_correlated(float alpha, float nol, float nov)
     return den > 0.0f ? (2.0f * nol * nov) / den : 0.0f;
 }
 
-__device__ __forceinline__ F3 f0_of(const Mat& m) {
-    const float f = f0_ue(m.specular);
-    return f3(f + (m.albedo.x - f) * m.metallic, f + (m.albedo.y - f) * m.metallic,
-              f + (m.albedo.z - f) * m.metallic);
-}
-
 __device__ __forceinline__ float alpha_of(const Mat& m) {
     const float r = clampr(m.roughness, 0.01f, 1.0f);
     return r * r;
 }
 
+// lobe selection probability of the GGX mix: clamp(lum(F0), 0.1, 0.9) in RGB,
+// clamp(F0(lambda), 0.1, 0.9) of the first wavelength otherwise
+__device__ __forceinline__ float pspec_of(const Mat& m, const S4& alb, uint32_t mode) {
+    const float f = f0_ue(m.specular);
+    if (mode == AR_SPECTRAL_RGB) {
+        const F3 f0 = f3(f + (m.albedo.x - f) * m.metallic, f + (m.albedo.y - f) * m.metallic,
+                         f + (m.albedo.z - f) * m.metallic);
+        return clampr(lum(f0), 0.1f, 0.9f);
+    }
+    return clampr(f + (alb.v[0] - f) * m.metallic, 0.1f, 0.9f);
+}
+
 // EvaluateBSDFWithCos (core/render/material.cuh:47-165), rd toward the surface
-__device__ __forceinline__ F3 eval_bsdf(const Mat& m, F3 rd, F3 l, F3 n) {
+__device__ __forceinline__ S4 eval_bsdf(const Mat& m, const S4& alb, F3 rd, F3 l, F3 n) {
     const float nol = maxr(0.0f, dot(n, l));
-    if (m.model == AR_MODEL_LAMBERT) return scale(m.albedo, nol / kPi);
+    if (m.model == AR_MODEL_LAMBERT) return s4scale(alb, nol / kPi);
     const F3 v = normalize(scale(rd, -1.0f));
     const float nov = maxr(0.0f, dot(n, v));
-    if (nov <= 0.0f || nol <= 0.0f) return f3(0, 0, 0);
+    if (nov <= 0.0f || nol <= 0.0f) return s4(0, 0, 0, 0);
     const float alpha = alpha_of(m);
-    const F3 f0 = f0_of(m);
+    const float f = f0_ue(m.specular);
     const float kd = 1.0f - m.metallic;
     const F3 h = normalize(add(v, l));
     const float noh = maxr(0.0f, dot(n, h));
     const float loh = maxr(0.0f, dot(l, h));
     const float c = clampr(1.0f - loh, 0.0f, 1.0f);
     const float c5 = c * c * c * c * c;
-    const F3 F = f3(f0.x + (1.0f - f0.x) * c5, f0.y + (1.0f - f0.y) * c5, f0.z + (1.0f - f0.z) * c5);
     const float D = ggx_d(alpha, noh);
     const float G = g_correlated(alpha, nol, nov);
     const float sden = maxr(4.0f * nov, 1e-7f);
     const float dcos = nol / kPi;
-    return f3(F.x * D * G / sden + kd * (1.0f - F.x) * m.albedo.x * dcos,
-              F.y * D * G / sden + kd * (1.0f - F.y) * m.albedo.y * dcos,
-              F.z * D * G / sden + kd * (1.0f - F.z) * m.albedo.z * dcos);
+    S4 r;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const float f0 = f + (alb.v[k] - f) * m.metallic;
+        const float F = f0 + (1.0f - f0) * c5;
+        r.v[k] = F * D * G / sden + kd * (1.0f - F) * alb.v[k] * dcos;
+    }
+    return r;
 }
 
 // BSDF_PDF (core/render/material.cuh:316-371)
-__device__ __forceinline__ float bsdf_pdf(const Mat& m, F3 rd, F3 l, F3 n) {
+__device__ __forceinline__ float bsdf_pdf(const Mat& m, float pspec, F3 rd, F3 l, F3 n) {
     const float nol = maxr(0.0f, dot(n, l));
     if (m.model == AR_MODEL_LAMBERT) return nol > 0.0f ? nol * (1.0f / kPi) : 0.0f;
     const F3 v = scale(rd, -1.0f);
     const float nov = maxr(0.0f, dot(n, v));
     if (nov <= 0.0f || nol <= 0.0f) return 0.0f;
     const float alpha = alpha_of(m);
-    const float pspec = clampr(lum(f0_of(m)), 0.1f, 0.9f);
     const float ps = vndf_pdf(v, n, l, alpha);
     const float pd = nol * (1.0f / kPi);
     return maxr(0.0f, pspec * ps + (1.0f - pspec) * pd);
 }
 
 // SampleBSDF (core/render/material.cuh:167-314); false ends the path
-__device__ __forceinline__ bool sample_bsdf(const Mat& m, F3 rd, F3 n, const float* u, F3& next, float& pdf) {
+__device__ __forceinline__ bool sample_bsdf(const Mat& m, float pspec, F3 rd, F3 n, const float* u, F3& next,
+                                            float& pdf) {
     if (m.model == AR_MODEL_LAMBERT) {
         next = sample_cosine(u[1], u[2], u[3], n, pdf);
         return true;
@@ -275,7 +356,6 @@ __device__ __forceinline__ bool sample_bsdf(const Mat& m, F3 rd, F3 n, const flo
     const float nov = maxr(0.0f, dot(n, v));
     if (nov <= 0.0f) return false;
     const float alpha = alpha_of(m);
-    const float pspec = clampr(lum(f0_of(m)), 0.1f, 0.9f);
     F3 l;
     float ps, pd;
     if (u[0] < pspec) {
@@ -317,7 +397,10 @@ struct RenderParams {
     uint32_t rr_min_depth;
     uint32_t nee;
     uint32_t sky;
+    uint32_t spectral;           // AR_SPECTRAL_*
+    uint32_t output;             // AR_OUTPUT_*
     float env[3];
+    float env_c[3], env_scale;   // sigmoid coefficients of the constant sky (spectral modes)
     float cam_pos[3];
     float fwd[3], right[3], up[3];
     float thx, thy;
@@ -327,10 +410,6 @@ struct QueueRays {
     float *ox, *oy, *oz, *tmin, *dx, *dy, *dz, *tmax;
 };
 
-__device__ __forceinline__ QueueRays qrays(float* const* r) {
-    return {r[0], r[1], r[2], r[3], r[4], r[5], r[6], r[7]};
-}
-
 struct SceneArgs {
     const float4* __restrict__ prims;
     const GpuMaterial* __restrict__ mats;
@@ -339,20 +418,28 @@ struct SceneArgs {
     const float* __restrict__ light_prob;
     const float* __restrict__ light_cdf;
     uint32_t nlights;
+    const float* __restrict__ cie;
+    const float* __restrict__ d65;
 };
 
 struct PathArgs {
-    float* mask0; float* mask1; float* mask2;
-    float* rad0; float* rad1; float* rad2;
+    float* mask0; float* mask1; float* mask2; float* mask3;
+    float* rad0; float* rad1; float* rad2; float* rad3;
     float* lp0; float* lp1; float* lp2;
     float* last_pdf;
     uint32_t* meta;
+    float* lambda0;
 };
+
+__device__ __forceinline__ S4 load4(const float* a, const float* b, const float* c, const float* d, uint32_t p) {
+    return s4(a[p], b[p], c[p], d[p]);
+}
 
 // ---------------------------------------------------------------- kernels
 
 // RenderCamera's ray generation (core/scene.cu:135-164) for every path slot
-// of the pass: slot p = s_local * npix + pixel.
+// of the pass (slot p = s_local * npix + pixel), plus the path's first
+// wavelength in spectral modes (render.cuh:466-474).
 __global__ __launch_bounds__(kBlock) void k_camera(RenderParams P, PathArgs S, QueueRays q, uint32_t* __restrict__ qpath,
                                                    uint32_t* __restrict__ counters, const float* __restrict__ eps_table) {
     const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
@@ -370,16 +457,23 @@ __global__ __launch_bounds__(kBlock) void k_camera(RenderParams P, PathArgs S, Q
     const F3 up = f3(P.up[0], P.up[1], P.up[2]);
     F3 d = normalize(add(add(fwd, scale(right, ndx * P.thx)), scale(up, ndy * P.thy)));
     d = normalize(d);   // CalculateRadiance normalises the incoming direction again
+    float l0 = 0.0f;
+    if (P.spectral != AR_SPECTRAL_RGB) {
+        float w[4];
+        uniforms(P.seed, pix, P.sample_base + sl, kCameraBounce, 1, w);
+        l0 = P.spectral == AR_SPECTRAL_SINGLE ? spec::sample_lambda(w[0], w[1], w[2]) : 380.0f + 400.0f * w[0];
+    }
     q.ox[p] = P.cam_pos[0]; q.oy[p] = P.cam_pos[1]; q.oz[p] = P.cam_pos[2];
     q.tmin[p] = eps_table[0];
     q.dx[p] = d.x; q.dy[p] = d.y; q.dz[p] = d.z;
     q.tmax[p] = __int_as_float(0x7f800000);
     qpath[p] = p;
-    S.mask0[p] = 1.0f; S.mask1[p] = 1.0f; S.mask2[p] = 1.0f;
-    S.rad0[p] = 0.0f; S.rad1[p] = 0.0f; S.rad2[p] = 0.0f;
+    S.mask0[p] = 1.0f; S.mask1[p] = 1.0f; S.mask2[p] = 1.0f; S.mask3[p] = 1.0f;
+    S.rad0[p] = 0.0f; S.rad1[p] = 0.0f; S.rad2[p] = 0.0f; S.rad3[p] = 0.0f;
     S.last_pdf[p] = 0.0f;
     S.lp0[p] = 0.0f; S.lp1[p] = 0.0f; S.lp2[p] = 0.0f;
     S.meta[p] = 0u;
+    S.lambda0[p] = l0;
 }
 
 // One bounce of CalculateRadiance for every queued path: emission with MIS
@@ -394,7 +488,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(
     const float* __restrict__ hv,
     QueueRays nxt, uint32_t* __restrict__ qnext,
     QueueRays sh, uint32_t* __restrict__ spath, float* __restrict__ sc0, float* __restrict__ sc1,
-    float* __restrict__ sc2, uint32_t* __restrict__ sflags,
+    float* __restrict__ sc2, float* __restrict__ sc3, uint32_t* __restrict__ sflags,
     uint32_t* __restrict__ counters, const float* __restrict__ eps_table) {
     const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
     const uint32_t n = counters[0];
@@ -402,7 +496,8 @@ __global__ __launch_bounds__(kBlock) void k_shade(
     uint32_t p = 0;
     F3 next_o = f3(0, 0, 0), next_d = f3(0, 0, 0);
     float next_tmin = 0.0f;
-    F3 s_o = f3(0, 0, 0), s_d = f3(0, 0, 1), s_contrib = f3(0, 0, 0);
+    F3 s_o = f3(0, 0, 0), s_d = f3(0, 0, 1);
+    S4 s_contrib = s4(0, 0, 0, 0);
     float s_tmax = 0.0f;
     uint32_t s_flag = 0;
     if (q < n) {
@@ -416,16 +511,32 @@ __global__ __launch_bounds__(kBlock) void k_shade(
         const uint32_t meta = S.meta[p];
         const uint32_t non_delta = meta & 0xFFFFu;
         const bool prev_nee = (meta & kMetaNee) != 0;
-        F3 mask = f3(S.mask0[p], S.mask1[p], S.mask2[p]);
-        F3 rad = f3(S.rad0[p], S.rad1[p], S.rad2[p]);
+        const Lam L = make_lam(P.spectral, S.lambda0[p]);
+        S4 mask = load4(S.mask0, S.mask1, S.mask2, S.mask3, p);
+        S4 rad = load4(S.rad0, S.rad1, S.rad2, S.rad3, p);
         const F3 o = f3(cur.ox[q], cur.oy[q], cur.oz[q]);
         const F3 d = f3(cur.dx[q], cur.dy[q], cur.dz[q]);
         const float eps = eps_table[non_delta < 255 ? non_delta : 255];
         const uint32_t prim = hprim[q];
         uint32_t new_meta = non_delta;
         if (prim == AR_MISS) {
-            if (P.sky == AR_SKY_GRADIENT) rad = add(rad, mul(mask, sky_gradient(d)));
-            else if (P.sky == AR_SKY_CONSTANT) rad = add(rad, mul(mask, f3(P.env[0], P.env[1], P.env[2])));
+            if (P.sky == AR_SKY_GRADIENT) {
+                const F3 e = sky_gradient(d);
+                rad = s4add(rad, s4mul(mask, s4(e.x, e.y, e.z, 0.0f)));
+            } else if (P.sky == AR_SKY_CONSTANT) {
+                S4 e;
+                if (P.spectral == AR_SPECTRAL_RGB) {
+                    e = s4(P.env[0], P.env[1], P.env[2], 0.0f);
+                } else {
+#pragma unroll
+                    for (int k = 0; k < 4; ++k)
+                        e.v[k] = (P.spectral == AR_SPECTRAL_HERO4 || k == 0)
+                                     ? spec::sigmoid_poly(P.env_c[0], P.env_c[1], P.env_c[2], L.l[k]) * P.env_scale *
+                                           spec::d65_norm(sc.d65, L.l[k])
+                                     : 0.0f;
+                }
+                rad = s4add(rad, s4mul(mask, e));
+            }
         } else {
             const float4 a4 = sc.prims[3 * (size_t)prim + 0];
             const float4 b4 = sc.prims[3 * (size_t)prim + 1];
@@ -454,20 +565,22 @@ __global__ __launch_bounds__(kBlock) void k_shade(
                         const float last_pdf = S.last_pdf[p];
                         if (pl >= 0.0f && last_pdf > 0.0f) w = power_heuristic(last_pdf, pl);
                     }
-                    rad = add(rad, scale(mul(mask, m.emission), w));
+                    rad = s4add(rad, s4scale(s4mul(mask, emission_of(m, L, sc.d65)), w));
                 }
             } else {
+                const S4 alb = albedo_of(m, L);
+                const float pspec = m.model == AR_MODEL_LAMBERT ? 0.0f : pspec_of(m, alb, P.spectral);
                 F3 next;
                 float pdf;
-                if (sample_bsdf(m, d, nn, u, next, pdf)) {
-                    const F3 fcos = eval_bsdf(m, d, next, nn);
+                if (sample_bsdf(m, pspec, d, nn, u, next, pdf)) {
+                    const S4 fcos = eval_bsdf(m, alb, d, next, nn);
                     const float off = eps * maxr(1.0f, length(pos)) * (u[4] + 0.5f);
                     const float dn = dot(nn, next);
                     const float sgn = dn > 0.0f ? 1.0f : (dn < 0.0f ? -1.0f : 0.0f);
                     pos = add(pos, scale(nn, sgn * off));
                     bool nee_now = false;
                     if (P.nee && sc.nlights > 0 && depth + 1 < P.max_depth && non_delta < 10 &&
-                        u[5] < sqrtf(clampr(lum(mask), 0.0f, 1.0f))) {
+                        u[5] < sqrtf(clampr(lum4(mask, P.spectral), 0.0f, 1.0f))) {
                         // pick by the CDF (first entry with u < cdf; binary search)
                         uint32_t lo = 0, hi = sc.nlights - 1;
                         while (lo < hi) {
@@ -490,15 +603,15 @@ __global__ __launch_bounds__(kBlock) void k_shade(
                         const float cosx = maxr(0.0f, dot(nn, ldir));
                         const float cosy = maxr(0.0f, -dot(ln, ldir));
                         const bool faces = cosx > 0.0f && cosy > 0.0f;
-                        F3 contrib_w = f3(0, 0, 0);
+                        S4 contrib_w = s4(0, 0, 0, 0);
                         if (faces) {
-                            const F3 f = eval_bsdf(m, d, ldir, nn);
+                            const S4 f = eval_bsdf(m, alb, d, ldir, nn);
                             const float g = cosy / dist2;
-                            const F3 contrib = scale(mul(f, lm.emission), g / pa_total);
+                            const S4 contrib = s4scale(s4mul(f, emission_of(lm, L, sc.d65)), g / pa_total);
                             const float pl = pa_total * (dist2 / cosy);
-                            const float pb = maxr(bsdf_pdf(m, d, ldir, nn), 0.0f);
+                            const float pb = maxr(bsdf_pdf(m, pspec, d, ldir, nn), 0.0f);
                             const float w = power_heuristic(pl, pb);
-                            contrib_w = mul(mask, scale(contrib, w));
+                            contrib_w = s4mul(mask, s4scale(contrib, w));
                         }
                         // AnyHitSegmentBVH (intersection.cuh:285-310)
                         float tmx = 0.0f;
@@ -514,7 +627,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(
                             s_o = pos; s_d = sdir; s_tmax = tmx; s_contrib = contrib_w; s_flag = faces ? 1u : 0u;
                         } else {
                             // nothing to test: visible
-                            if (faces) { rad = add(rad, contrib_w); nee_now = true; }
+                            if (faces) { rad = s4add(rad, contrib_w); nee_now = true; }
                         }
                     }
                     // the resolve pass sets the NEE bit for queued shadow segments
@@ -522,15 +635,15 @@ __global__ __launch_bounds__(kBlock) void k_shade(
                     S.last_pdf[p] = pdf;
                     S.lp0[p] = pos.x; S.lp1[p] = pos.y; S.lp2[p] = pos.z;
                     const float inv_pdf = 1.0f / maxr(pdf, 1e-12f);
-                    const F3 contrib = scale(fcos, inv_pdf);
+                    const S4 contrib = s4scale(fcos, inv_pdf);
                     float qrr = 1.0f;
                     bool alive = true;
                     if (non_delta >= P.rr_min_depth) {
-                        qrr = clampr(lum(mask) * 0.5f + max3(mask) * 0.5f, 0.05f, 0.95f);
+                        qrr = clampr(lum4(mask, P.spectral) * 0.5f + max4(mask, P.spectral) * 0.5f, 0.05f, 0.95f);
                         if (u[9] > qrr) alive = false;
                     }
                     if (alive && depth + 1 < P.max_depth) {
-                        mask = scale(mul(mask, contrib), 1.0f / qrr);
+                        mask = s4scale(s4mul(mask, contrib), 1.0f / qrr);
                         cont = true;
                         next_o = pos;
                         next_d = next;
@@ -541,8 +654,8 @@ __global__ __launch_bounds__(kBlock) void k_shade(
                 }
             }
         }
-        S.mask0[p] = mask.x; S.mask1[p] = mask.y; S.mask2[p] = mask.z;
-        S.rad0[p] = rad.x; S.rad1[p] = rad.y; S.rad2[p] = rad.z;
+        S.mask0[p] = mask.v[0]; S.mask1[p] = mask.v[1]; S.mask2[p] = mask.v[2]; S.mask3[p] = mask.v[3];
+        S.rad0[p] = rad.v[0]; S.rad1[p] = rad.v[1]; S.rad2[p] = rad.v[2]; S.rad3[p] = rad.v[3];
         S.meta[p] = new_meta;
     }
     const uint32_t slot = wave_append(cont, &counters[1]);
@@ -558,7 +671,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(
         sh.ox[ss] = s_o.x; sh.oy[ss] = s_o.y; sh.oz[ss] = s_o.z; sh.tmin[ss] = 0.0f;
         sh.dx[ss] = s_d.x; sh.dy[ss] = s_d.y; sh.dz[ss] = s_d.z; sh.tmax[ss] = s_tmax;
         spath[ss] = p;
-        sc0[ss] = s_contrib.x; sc1[ss] = s_contrib.y; sc2[ss] = s_contrib.z;
+        sc0[ss] = s_contrib.v[0]; sc1[ss] = s_contrib.v[1]; sc2[ss] = s_contrib.v[2]; sc3[ss] = s_contrib.v[3];
         sflags[ss] = s_flag;
     }
 }
@@ -568,7 +681,8 @@ __global__ __launch_bounds__(kBlock) void k_shade(
 // contribution when the light point faced the surface.
 __global__ __launch_bounds__(kBlock) void k_resolve(PathArgs S, const uint32_t* __restrict__ spath,
                                                     const float* __restrict__ sc0, const float* __restrict__ sc1,
-                                                    const float* __restrict__ sc2, const uint32_t* __restrict__ sflags,
+                                                    const float* __restrict__ sc2, const float* __restrict__ sc3,
+                                                    const uint32_t* __restrict__ sflags,
                                                     const uint32_t* __restrict__ occ,
                                                     const uint32_t* __restrict__ counters) {
     const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
@@ -581,6 +695,7 @@ __global__ __launch_bounds__(kBlock) void k_resolve(PathArgs S, const uint32_t* 
         S.rad0[p] = S.rad0[p] + sc0[s];
         S.rad1[p] = S.rad1[p] + sc1[s];
         S.rad2[p] = S.rad2[p] + sc2[s];
+        S.rad3[p] = S.rad3[p] + sc3[s];
     }
 }
 
@@ -591,26 +706,62 @@ __global__ void k_advance(uint32_t* counters) {
     counters[2] = 0;
 }
 
-// Per-pixel sum over the pass's samples in sample order, then the mean.
-__global__ __launch_bounds__(kBlock) void k_accumulate(RenderParams P, PathArgs S, float* __restrict__ accum,
-                                                       uint32_t spp_pass, bool first, bool last, float spp_total,
-                                                       float* __restrict__ out) {
+// A path's pixel value: RGB clamp(checkNanInf(L), 0, 1e3), or in spectral
+// modes clamp(XYZ(lambda) * checkNanInf(L) / p(lambda), 0, 1e3)
+// (render.cuh:966-997), summed over the hero wavelengths over the sum of
+// their densities.
+__device__ __forceinline__ F3 path_value(const RenderParams& P, const SceneArgs& sc, const PathArgs& S, uint32_t p) {
+    if (P.spectral == AR_SPECTRAL_RGB)
+        return f3(clampr(fixnan(S.rad0[p]), 0.0f, 1e3f), clampr(fixnan(S.rad1[p]), 0.0f, 1e3f),
+                  clampr(fixnan(S.rad2[p]), 0.0f, 1e3f));
+    const Lam L = make_lam(P.spectral, S.lambda0[p]);
+    const float r[4] = {S.rad0[p], S.rad1[p], S.rad2[p], S.rad3[p]};
+    float X = 0.0f, Y = 0.0f, Z = 0.0f;
+    if (P.spectral == AR_SPECTRAL_SINGLE) {
+        float x, y, z;
+        spec::cie_xyz(sc.cie, L.l[0], x, y, z);
+        const float pdf = spec::lambda_pdf(L.l[0]);
+        const float v = fixnan(r[0]);
+        X = x * v / pdf; Y = y * v / pdf; Z = z * v / pdf;
+    } else {
+        const float psum = 4.0f / 400.0f;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            float x, y, z;
+            spec::cie_xyz(sc.cie, L.l[k], x, y, z);
+            const float v = fixnan(r[k]);
+            X = X + x * v; Y = Y + y * v; Z = Z + z * v;
+        }
+        X = X / psum; Y = Y / psum; Z = Z / psum;
+    }
+    return f3(clampr(X, 0.0f, 1e3f), clampr(Y, 0.0f, 1e3f), clampr(Z, 0.0f, 1e3f));
+}
+
+// Per-pixel sum over the pass's samples in sample order, then the mean
+// (XYZ2SRGBLinearD65, core/tonemap.cuh:116-126, for sRGB output).
+__global__ __launch_bounds__(kBlock) void k_accumulate(RenderParams P, SceneArgs sc, PathArgs S,
+                                                       float* __restrict__ accum, uint32_t spp_pass, bool first,
+                                                       bool last, float spp_total, float* __restrict__ out) {
     const uint32_t pix = blockIdx.x * kBlock + threadIdx.x;
     if (pix >= P.npix) return;
     float ax = first ? 0.0f : accum[3 * pix + 0];
     float ay = first ? 0.0f : accum[3 * pix + 1];
     float az = first ? 0.0f : accum[3 * pix + 2];
     for (uint32_t s = 0; s < spp_pass; ++s) {
-        const uint32_t p = s * P.npix + pix;
-        const float rx = clampr(fixnan(S.rad0[p]), 0.0f, 1e3f);
-        const float ry = clampr(fixnan(S.rad1[p]), 0.0f, 1e3f);
-        const float rz = clampr(fixnan(S.rad2[p]), 0.0f, 1e3f);
-        ax = ax + rx; ay = ay + ry; az = az + rz;
+        const F3 r = path_value(P, sc, S, s * P.npix + pix);
+        ax = ax + r.x; ay = ay + r.y; az = az + r.z;
     }
     if (last) {
-        out[3 * pix + 0] = ax / spp_total;
-        out[3 * pix + 1] = ay / spp_total;
-        out[3 * pix + 2] = az / spp_total;
+        float x = ax / spp_total, y = ay / spp_total, z = az / spp_total;
+        if (P.spectral != AR_SPECTRAL_RGB && P.output == AR_OUTPUT_SRGB) {
+            const float r = 3.2406f * x + -1.5372f * y + -0.4986f * z;
+            const float g = -0.9689f * x + 1.8758f * y + 0.0415f * z;
+            const float b = 0.0557f * x + -0.204f * y + 1.057f * z;
+            x = maxr(0.0f, r); y = maxr(0.0f, g); z = maxr(0.0f, b);
+        }
+        out[3 * pix + 0] = x;
+        out[3 * pix + 1] = y;
+        out[3 * pix + 2] = z;
     } else {
         accum[3 * pix + 0] = ax; accum[3 * pix + 1] = ay; accum[3 * pix + 2] = az;
     }
@@ -634,6 +785,8 @@ void free_render_buffers(ar_ctx* ctx) {
     if (ctx->rb.block) (void)hipFree(ctx->rb.block);
     if (ctx->rb.accum) (void)hipFree(ctx->rb.accum);
     if (ctx->rb.eps_table) (void)hipFree(ctx->rb.eps_table);
+    if (ctx->rb.cie) (void)hipFree(ctx->rb.cie);
+    if (ctx->rb.d65) (void)hipFree(ctx->rb.d65);
     ctx->rb = RenderBuffers{};
 }
 
@@ -695,6 +848,8 @@ int upload_scene(ar_ctx* ctx) {
         mats[i] = g;
     }
     s.nlights = (uint32_t)ltri.size();
+    s.host_mats = mats;
+    s.spectral_ready = false;
     auto up = [&](void** dst, const void* src, size_t bytes, const char* what) -> int {
         if (bytes == 0) bytes = 16;
         int rc = check_hip(ctx, hipMalloc(dst, bytes), what);
@@ -721,6 +876,15 @@ static int ensure_render_buffers(ar_ctx* ctx, size_t cap, size_t npix) {
         if (rc != AR_OK) { b.eps_table = nullptr; return rc; }
         rc = check_hip(ctx, hipMemcpy(b.eps_table, eps, sizeof eps, hipMemcpyHostToDevice), "upload eps");
         if (rc != AR_OK) return rc;
+        rc = check_hip(ctx, hipMalloc((void**)&b.cie, 421 * 3 * sizeof(float)), "hipMalloc(cie)");
+        if (rc != AR_OK) { b.cie = nullptr; return rc; }
+        rc = check_hip(ctx, hipMemcpy(b.cie, spec::cie_xyz_table(), 421 * 3 * sizeof(float), hipMemcpyHostToDevice),
+                       "upload cie");
+        if (rc != AR_OK) return rc;
+        rc = check_hip(ctx, hipMalloc((void**)&b.d65, 421 * sizeof(float)), "hipMalloc(d65)");
+        if (rc != AR_OK) { b.d65 = nullptr; return rc; }
+        rc = check_hip(ctx, hipMemcpy(b.d65, spec::d65_table(), 421 * sizeof(float), hipMemcpyHostToDevice), "upload d65");
+        if (rc != AR_OK) return rc;
     }
     if (b.accum_pixels < npix) {
         if (b.accum) (void)hipFree(b.accum);
@@ -734,8 +898,8 @@ static int ensure_render_buffers(ar_ctx* ctx, size_t cap, size_t npix) {
     b.block = nullptr;
     b.capacity = 0;
     // 4-byte arrays of cap entries: 2x8 queue rays, 2 qpath, 5 hits, 8 shadow rays,
-    // spath, 3 scontrib, sflags, 3 mask, 3 rad, 3 last_pos, last_pdf, meta = 47
-    const size_t arrays = 47;
+    // spath, 4 scontrib, sflags, 4 mask, 4 rad, 3 last_pos, last_pdf, meta, lambda0 = 51
+    const size_t arrays = 51;
     const size_t occ_words = (cap + 31) / 32 + 64;
     const size_t bytes = arrays * cap * 4 + occ_words * 4 + 64;
     int rc = check_hip(ctx, hipMalloc(&b.block, bytes), "hipMalloc(render state)");
@@ -753,17 +917,41 @@ static int ensure_render_buffers(ar_ctx* ctx, size_t cap, size_t npix) {
     b.hit_flags = (uint32_t*)take(cap);
     for (int f = 0; f < 8; ++f) b.srays[f] = (float*)take(cap);
     b.spath = (uint32_t*)take(cap);
-    for (int k = 0; k < 3; ++k) b.scontrib[k] = (float*)take(cap);
+    for (int k = 0; k < 4; ++k) b.scontrib[k] = (float*)take(cap);
     b.sflags = (uint32_t*)take(cap);
-    for (int k = 0; k < 3; ++k) b.mask[k] = (float*)take(cap);
-    for (int k = 0; k < 3; ++k) b.rad[k] = (float*)take(cap);
+    for (int k = 0; k < 4; ++k) b.mask[k] = (float*)take(cap);
+    for (int k = 0; k < 4; ++k) b.rad[k] = (float*)take(cap);
     for (int k = 0; k < 3; ++k) b.last_pos[k] = (float*)take(cap);
     b.last_pdf = (float*)take(cap);
     b.meta = (uint32_t*)take(cap);
+    b.lambda0 = (float*)take(cap);
     b.occ = (uint32_t*)take(occ_words);
     b.counters = (uint32_t*)take(16);
     b.capacity = cap;
     return AR_OK;
+}
+
+// Sigmoid coefficients of every material (albedo; emission normalised by its
+// largest channel when that exceeds 1, as Interact does, spectrum_lut.cuh).
+static int prepare_spectral_materials(ar_ctx* ctx) {
+    SceneDev& s = ctx->scene;
+    if (s.spectral_ready) return AR_OK;
+    const std::vector<float>& lut = spec::uplift_lut(spec::cie_xyz_table(), spec::d65_table());
+    for (GpuMaterial& g : s.host_mats) {
+        spec::uplift(lut.data(), g.albedo, g.calb);
+        const float mx = std::max(g.emission[0], std::max(g.emission[1], g.emission[2]));
+        float e[3] = {g.emission[0], g.emission[1], g.emission[2]};
+        g.escale = 1.0f;
+        if (mx > 1.0f) {
+            for (float& x : e) x /= mx;
+            g.escale = mx;
+        }
+        spec::uplift(lut.data(), e, g.cem);
+    }
+    int rc = check_hip(ctx, hipMemcpy(s.mats, s.host_mats.data(), s.host_mats.size() * sizeof(GpuMaterial),
+                                      hipMemcpyHostToDevice), "upload spectral materials");
+    if (rc == AR_OK) s.spectral_ready = true;
+    return rc;
 }
 
 int render(ar_ctx* ctx, const ar_camera* cam, const ar_render_opts* opts, float* out, hipStream_t stream) {
@@ -781,8 +969,28 @@ int render(ar_ctx* ctx, const ar_camera* cam, const ar_render_opts* opts, float*
     rc = trace_prologue(ctx, ta, cap, tblocks);
     if (rc != AR_OK) return rc;
 
+    if (opts->spectral != AR_SPECTRAL_RGB && opts->spectral != AR_SPECTRAL_SINGLE && opts->spectral != AR_SPECTRAL_HERO4)
+        return set_error(ctx, AR_E_INVALID, "spectral must be AR_SPECTRAL_RGB, _SINGLE or _HERO4");
+    if (opts->spectral != AR_SPECTRAL_RGB && opts->sky == AR_SKY_GRADIENT)
+        return set_error(ctx, AR_E_UNSUPPORTED, "the gradient sky is RGB-only; use AR_SKY_CONSTANT in spectral modes");
+    if (opts->spectral != AR_SPECTRAL_RGB) {
+        rc = prepare_spectral_materials(ctx);
+        if (rc != AR_OK) return rc;
+    }
     RenderParams P{};
     P.seed = opts->seed;
+    P.spectral = opts->spectral;
+    P.output = opts->output;
+    {
+        float e[3] = {opts->env[0], opts->env[1], opts->env[2]};
+        const float mx = std::max(e[0], std::max(e[1], e[2]));
+        P.env_scale = 1.0f;
+        if (mx > 1.0f) { for (float& x : e) x /= mx; P.env_scale = mx; }
+        if (opts->spectral != AR_SPECTRAL_RGB) {
+            const std::vector<float>& lut = spec::uplift_lut(spec::cie_xyz_table(), spec::d65_table());
+            spec::uplift(lut.data(), e, P.env_c);
+        }
+    }
     P.width = opts->width;
     P.height = opts->height;
     P.npix = (uint32_t)npix;
@@ -812,10 +1020,10 @@ int render(ar_ctx* ctx, const ar_camera* cam, const ar_render_opts* opts, float*
         P.thx = P.thy * aspect;
     }
     RenderBuffers& b = ctx->rb;
-    PathArgs S{b.mask[0], b.mask[1], b.mask[2], b.rad[0], b.rad[1], b.rad[2],
-               b.last_pos[0], b.last_pos[1], b.last_pos[2], b.last_pdf, b.meta};
+    PathArgs S{b.mask[0], b.mask[1], b.mask[2], b.mask[3], b.rad[0], b.rad[1], b.rad[2], b.rad[3],
+               b.last_pos[0], b.last_pos[1], b.last_pos[2], b.last_pdf, b.meta, b.lambda0};
     SceneArgs sc{ctx->scene.prims, ctx->scene.mats, ctx->scene.light_tri, ctx->scene.light_area,
-                 ctx->scene.light_prob, ctx->scene.light_cdf, ctx->scene.nlights};
+                 ctx->scene.light_prob, ctx->scene.light_cdf, ctx->scene.nlights, b.cie, b.d65};
     QueueRays q[2] = {{b.rays[0][0], b.rays[0][1], b.rays[0][2], b.rays[0][3], b.rays[0][4], b.rays[0][5], b.rays[0][6], b.rays[0][7]},
                       {b.rays[1][0], b.rays[1][1], b.rays[1][2], b.rays[1][3], b.rays[1][4], b.rays[1][5], b.rays[1][6], b.rays[1][7]}};
     QueueRays shq{b.srays[0], b.srays[1], b.srays[2], b.srays[3], b.srays[4], b.srays[5], b.srays[6], b.srays[7]};
@@ -834,19 +1042,21 @@ int render(ar_ctx* ctx, const ar_camera* cam, const ar_render_opts* opts, float*
             if (e != hipSuccess) return check_hip(ctx, e, "integrator trace");
             hipLaunchKernelGGL(k_shade, dim3(grid_for(P.capacity)), dim3(kBlock), 0, stream, P, depth, sc, S, q[c],
                                b.qpath[c], b.hit_prim, b.hit_t, b.hit_u, b.hit_v, q[1 - c], b.qpath[1 - c], shq,
-                               b.spath, b.scontrib[0], b.scontrib[1], b.scontrib[2], b.sflags, b.counters, b.eps_table);
+                               b.spath, b.scontrib[0], b.scontrib[1], b.scontrib[2], b.scontrib[3], b.sflags,
+                               b.counters, b.eps_table);
             if (opts->nee) {
                 ar_rays sr{shq.ox, shq.oy, shq.oz, shq.tmin, shq.dx, shq.dy, shq.dz, shq.tmax};
                 e = launch_trace_any(ta, sr, P.capacity, b.occ, tblocks, stream, &b.counters[2]);
                 if (e != hipSuccess) return check_hip(ctx, e, "integrator shadow trace");
                 hipLaunchKernelGGL(k_resolve, dim3(grid_for(P.capacity)), dim3(kBlock), 0, stream, S, b.spath,
-                                   b.scontrib[0], b.scontrib[1], b.scontrib[2], b.sflags, b.occ, b.counters);
+                                   b.scontrib[0], b.scontrib[1], b.scontrib[2], b.scontrib[3], b.sflags, b.occ,
+                                   b.counters);
             }
             hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, stream, b.counters);
             c = 1 - c;
         }
         done += this_spp;
-        hipLaunchKernelGGL(k_accumulate, dim3(grid_for(npix)), dim3(kBlock), 0, stream, P, S, b.accum, this_spp,
+        hipLaunchKernelGGL(k_accumulate, dim3(grid_for(npix)), dim3(kBlock), 0, stream, P, sc, S, b.accum, this_spp,
                            P.sample_base == opts->sample_offset, done >= opts->spp, spp_total, out);
     }
     return check_hip(ctx, hipGetLastError(), "integrator launch");

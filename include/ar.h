@@ -198,12 +198,45 @@ typedef struct {
     uint32_t sky;                 /* AR_SKY_* */
     float env[3];
     uint32_t max_paths_per_pass;  /* 0: 4,194,304 */
+    uint32_t spectral;            /* AR_SPECTRAL_* */
+    uint32_t output;              /* AR_OUTPUT_* (spectral modes) */
 } ar_render_opts;
+
+/* Spectral modes.  RGB: three colour channels throughout (the reference with
+ * USE_SPECTRUM_RENDERING off).  SINGLE: one wavelength per path in
+ * [360, 780] nm drawn from the reference's CIE-fitted mixture
+ * (core/render/render.cuh:466-474, core/spectrum/sample_ciexyz.cuh:288-326).
+ * HERO4: four hero wavelengths per path, lambda_k = 380 + (u*400 + 100 k)
+ * mod 400, each path's estimate divided by the sum of their densities.
+ * Reflectances and emitters are uplifted from RGB through the 3 x 64^3
+ * sigmoid-coefficient table (ar_spectrum_lut_build); emitters and a
+ * constant sky are scaled by D65 normalised to integral D65 * ybar = 1. */
+#define AR_SPECTRAL_RGB 0u
+#define AR_SPECTRAL_SINGLE 1u
+#define AR_SPECTRAL_HERO4 4u
+
+/* Output of a spectral render: CIE XYZ (the reference's spectral output), or
+ * linear sRGB through XYZ2SRGBLinearD65 (core/tonemap.cuh:116-126). */
+#define AR_OUTPUT_XYZ 0u
+#define AR_OUTPUT_SRGB 1u
 
 /* out_rgb: device array of width*height*3 floats (pixel j*width + i),
  * receives the mean over spp of clamp(checkNanInf(L), 0, 1e3) per channel.
  * Queued on `stream`; the call does not wait. */
 int ar_render(ar_ctx* ctx, const ar_camera* cam, const ar_render_opts* opts, float* out_rgb, void* stream);
+
+/* --- spectral uplift table --------------------------------------------------
+ * Replaces PrecomputeRGBLUTKernel / SetupSpectrumLUT (core/spectrum/
+ * spectrum_lut.cuh:12-78, core/scene.cu:591-607): instead of evaluating the
+ * reference's RGB->basis network on a 512^3 grid (2 GiB), builds the
+ * 3 x 64^3 x 3 sigmoid-coefficient table once per process on the host.  Needs
+ * no device.  out: 3*64*64*64*3 floats, layout [largest channel][z][y][x][c].
+ * Returns AR_E_INVALID if count is smaller. */
+int ar_spectrum_lut_build(float* out, size_t count);
+
+/* Sigmoid coefficients for n RGB colours (rows of 3, channels clamped to
+ * [0, 1]) from the table (builds it on first use).  No device needed. */
+int ar_spectrum_uplift(const float* rgb, size_t n, float* coeffs);
 
 #ifdef __cplusplus
 }

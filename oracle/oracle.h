@@ -139,6 +139,8 @@ typedef struct {
     int node_count;
     const int32_t* indices;
     const int32_t* light_index;   /* ntri: index into the light table or -1 */
+    const float* mat_coef;        /* spectral modes: per material calb[3], cem[3], escale, 0 */
+    float env_coef[4];            /* spectral constant sky: coefficients, scale */
 } orc_render_scene;
 
 typedef struct {
@@ -159,7 +161,24 @@ typedef struct {
     float env[3];
     float cam_pos[3], cam_dir[3];
     float vfov_deg;
+    int spectral;          /* 0 RGB, 1 one wavelength (reference mixture), 4 hero wavelengths */
+    int output;            /* spectral modes: 0 XYZ, 1 linear sRGB */
+    const float* cie;      /* 421 x 3 CIE 1931 2-degree table, 360..780 nm at 1 nm */
+    const float* d65;      /* 421 D65 values */
+    const float* lut;      /* 3 x 64^3 x 3 sigmoid-coefficient table (spectral modes) */
 } orc_render_opts;
+
+/* Sigmoid coefficients of an RGB colour (channels clamped to [0, 1]) from the
+ * 3 x 64^3 x 3 table: largest channel l, its value z on the smoothstep^2
+ * scale, the other two channels divided by z; trilinear in the coefficients. */
+void orc_uplift(const float* lut, const float rgb[3], float c[3]);
+
+/* FP32 wavelength sampling pieces (sample_ciexyz.cuh): normal quantile,
+ * mixture density and sample, and the sigmoid spectrum at a wavelength. */
+float orc_ndtri_f(float u);
+float orc_lambda_pdf_f(float l);
+float orc_sample_lambda_f(float u0, float u1, float u2);
+float orc_sigmoid_f(const float c[3], float l);
 
 /* Renders the mean of spp samples per pixel into out (width*height*3,
  * row-major, pixel index = j*width + i).  precision 32: float out, FP32

@@ -149,6 +149,11 @@ static uint32_t trace_d(const orc_render_scene* sc, const double o[3], const dou
 #define POW powf
 #define FABS fabsf
 #define PI_R 3.14159265358979323846f
+#define LOG logf
+#define LOG1P log1pf
+#define FMA fmaf
+#define ERFC erfcf
+#define FLOOR floorf
 #include "render_impl.h"
 
 #define R double
@@ -161,6 +166,11 @@ static uint32_t trace_d(const orc_render_scene* sc, const double o[3], const dou
 #define POW pow
 #define FABS fabs
 #define PI_R 3.14159265358979323846
+#define LOG log
+#define LOG1P log1p
+#define FMA fma
+#define ERFC erfc
+#define FLOOR floor
 #include "render_impl.h"
 
 static uint32_t closest_f32(const orc_render_scene* sc, vec3_f32 o, vec3_f32 d, float tmin, float tmax,
@@ -194,6 +204,47 @@ static int occluded_f64(const orc_render_scene* sc, vec3_f64 o, vec3_f64 d, doub
     return trace_d(sc, oo, dd, tmin, tmax, 1, &t, &u, &v) != ORC_MISS;
 }
 
+/* ---------------- RGB uplift lookup ---------------- */
+
+static float lut_scale(int k)
+{
+    float t = (float)k / 63.0f;
+    t = t * t * (3.0f - 2.0f * t);
+    return t * t * (3.0f - 2.0f * t);
+}
+
+void orc_uplift(const float* lut, const float rgb_in[3], float c[3])
+{
+    float rgb[3];
+    for (int k = 0; k < 3; ++k) rgb[k] = fminf(fmaxf(rgb_in[k], 0.0f), 1.0f);
+    int l = 0;
+    if (rgb[1] > rgb[l]) l = 1;
+    if (rgb[2] > rgb[l]) l = 2;
+    const float z = rgb[l];
+    if (!(z > 0.0f)) { c[0] = 0.0f; c[1] = 0.0f; c[2] = -1e30f; return; }
+    const float x = rgb[(l + 1) % 3] / z, y = rgb[(l + 2) % 3] / z;
+    int k = 0;
+    while (k < 62 && lut_scale(k + 1) <= z) ++k;
+    const float z0 = lut_scale(k), z1 = lut_scale(k + 1);
+    float tz = (z - z0) / (z1 - z0);
+    tz = fminf(fmaxf(tz, 0.0f), 1.0f);
+    const float fx = x * 63.0f, fy = y * 63.0f;
+    int i = (int)fx, j = (int)fy;
+    if (i > 62) i = 62;
+    if (j > 62) j = 62;
+    const float tx = fx - (float)i, ty = fy - (float)j;
+    for (int m = 0; m < 3; ++m) {
+        float acc = 0.0f;
+        for (int dz = 0; dz < 2; ++dz)
+            for (int dy = 0; dy < 2; ++dy)
+                for (int dx = 0; dx < 2; ++dx) {
+                    const float w = (dx ? tx : 1.0f - tx) * (dy ? ty : 1.0f - ty) * (dz ? tz : 1.0f - tz);
+                    acc += w * lut[((((size_t)l * 64 + (size_t)(k + dz)) * 64 + (size_t)(j + dy)) * 64 + (size_t)(i + dx)) * 3 + (size_t)m];
+                }
+        c[m] = acc;
+    }
+}
+
 /* ---------------- light table (core/scene.cu:677-701) ---------------- */
 
 static float tri_area_f(const float* a, const float* b, const float* c)
@@ -210,7 +261,7 @@ int orc_render(const float* verts, uint32_t nv, const uint32_t* idx, uint32_t nt
                const orc_material* mats, uint32_t nmats, const orc_render_opts* opts, int precision,
                void* out, int threads)
 {
-    (void)nv; (void)nmats;
+    (void)nv;
     orc_render_scene sc;
     memset(&sc, 0, sizeof sc);
     sc.verts = verts; sc.idx = idx; sc.ntri = ntri; sc.tri_mat = tri_mat; sc.mats = mats;
@@ -252,6 +303,25 @@ int orc_render(const float* verts, uint32_t nv, const uint32_t* idx, uint32_t nt
         lcdf[i] = acc;
     }
     sc.light_index = light_index;
+    float* mat_coef = NULL;
+    if (opts->spectral != 0) {
+        mat_coef = calloc((size_t)8 * (nmats ? nmats : 1), sizeof(float));
+        for (uint32_t i = 0; i < nmats; ++i) {
+            orc_uplift(opts->lut, mats[i].albedo, &mat_coef[8 * i]);
+            float e[3] = { mats[i].emission[0], mats[i].emission[1], mats[i].emission[2] };
+            const float mx = fmaxf(e[0], fmaxf(e[1], e[2]));
+            float scale = 1.0f;
+            if (mx > 1.0f) { for (int k = 0; k < 3; ++k) e[k] /= mx; scale = mx; }
+            orc_uplift(opts->lut, e, &mat_coef[8 * i + 3]);
+            mat_coef[8 * i + 6] = scale;
+        }
+        float e[3] = { opts->env[0], opts->env[1], opts->env[2] };
+        const float mx = fmaxf(e[0], fmaxf(e[1], e[2]));
+        sc.env_coef[3] = 1.0f;
+        if (mx > 1.0f) { for (int k = 0; k < 3; ++k) e[k] /= mx; sc.env_coef[3] = mx; }
+        orc_uplift(opts->lut, e, sc.env_coef);
+    }
+    sc.mat_coef = mat_coef;
     orc_render_lights L = { nl, ltri, larea, lprob, lcdf };
     int nt = threads;
 #ifdef _OPENMP
@@ -262,6 +332,12 @@ int orc_render(const float* verts, uint32_t nv, const uint32_t* idx, uint32_t nt
     if (precision == 64) render_f64(&sc, &L, opts, (double*)out, nt);
     else render_f32(&sc, &L, opts, (float*)out, nt);
     free(pmin); free(pmax); free(pcen); free(nodes); free(indices); free(light_index);
-    free(ltri); free(larea); free(lprob); free(lcdf);
+    free(ltri); free(larea); free(lprob); free(lcdf); free(mat_coef);
     return 0;
 }
+
+/* Wavelength helpers exposed for the tests (FP32 instantiation). */
+float orc_ndtri_f(float u) { return ndtri_f32(u); }
+float orc_lambda_pdf_f(float l) { return lambda_pdf_f32(l); }
+float orc_sample_lambda_f(float u0, float u1, float u2) { return sample_lambda_f32(u0, u1, u2); }
+float orc_sigmoid_f(const float c[3], float l) { return sigmoid_f32(c, l); }

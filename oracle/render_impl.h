@@ -24,6 +24,15 @@
  * NEE is skipped on the last iteration so that every path length up to
  * max_depth gets consistent MIS weights.
  *
+ * Spectral modes restate the reference's wavelength path (render.cuh:466-474,
+ * 657-700, 966-997): one wavelength from the CIE-fitted mixture with
+ * density p (sample_ciexyz.cuh:15-326, Acklam's quantile + one Newton step),
+ * or four hero wavelengths 100 nm apart in [380, 780); reflectance and
+ * emission spectra from sigmoid-polynomial coefficients (Jakob & Hanika 2019)
+ * looked up in the 3 x 64^3 table; emitters times D65 normalised to
+ * integral D65 * ybar = 1 (ciexyz.cuh:67-85); output XYZ = xbar..zbar(lambda)
+ * * L / p (linear interpolation of the 1 nm CIE table, ciexyz.cuh:522-543).
+ *
  * Random numbers: Philox4x32-10 (Salmon et al., SC'11), key = seed, counter
  * = (pixel, sample, bounce, group); the GPU kernels draw the same numbers.
  */
@@ -61,6 +70,151 @@ static inline V3 RF(reflect)(V3 v, V3 n) { return RF(sub)(v, RF(scale)(n, (R)2 *
 static inline R RF(fixnan)(R x) { return (isnan(x) || isinf(x)) ? (R)0 : x; }
 
 static inline V3 RF(load3)(const float* p) { return RF(v3)((R)p[0], (R)p[1], (R)p[2]); }
+
+/* per-path spectral quantity: RGB (channels 0..2), one wavelength (channel 0)
+ * or four hero wavelengths */
+#define S4 RF(s4t)
+typedef struct { R v[4]; } S4;
+static inline S4 RF(s4)(R a, R b, R c, R d) { S4 r; r.v[0] = a; r.v[1] = b; r.v[2] = c; r.v[3] = d; return r; }
+static inline S4 RF(s4mul)(S4 a, S4 b) { return RF(s4)(a.v[0] * b.v[0], a.v[1] * b.v[1], a.v[2] * b.v[2], a.v[3] * b.v[3]); }
+static inline S4 RF(s4add)(S4 a, S4 b) { return RF(s4)(a.v[0] + b.v[0], a.v[1] + b.v[1], a.v[2] + b.v[2], a.v[3] + b.v[3]); }
+static inline S4 RF(s4scale)(S4 a, R s) { return RF(s4)(a.v[0] * s, a.v[1] * s, a.v[2] * s, a.v[3] * s); }
+
+/* ---------------------------------------------------------------------- */
+/* Wavelengths (core/spectrum/sample_ciexyz.cuh, ciexyz.cuh)               */
+/* ---------------------------------------------------------------------- */
+
+static R RF(ndtri)(R u)
+{
+    u = u < 0 ? 0 : (u > 1 ? 1 : u);
+    const R a0 = (R)-3.969683028665376e+01, a1 = (R)2.209460984245205e+02, a2 = (R)-2.759285104469687e+02,
+            a3 = (R)1.383577518672690e+02, a4 = (R)-3.066479806614716e+01, a5 = (R)2.506628277459239e+00;
+    const R b0 = (R)-5.447609879822406e+01, b1 = (R)1.615858368580409e+02, b2 = (R)-1.556989798598866e+02,
+            b3 = (R)6.680131188771972e+01, b4 = (R)-1.328068155288572e+01;
+    const R c0 = (R)-7.784894002430293e-03, c1 = (R)-3.223964580411365e-01, c2 = (R)-2.400758277161838e+00,
+            c3 = (R)-2.549732539343734e+00, c4 = (R)4.374664141464968e+00, c5 = (R)2.938163982698783e+00;
+    const R d0 = (R)7.784695709041462e-03, d1 = (R)3.224671290700398e-01, d2 = (R)2.445134137142996e+00,
+            d3 = (R)3.754408661907416e+00;
+    const R plow = (R)0.02425, phigh = (R)1 - plow;
+    R x;
+    if (u < plow) {
+        const R q = SQRT((R)-2 * LOG(u));
+        const R num = FMA(FMA(FMA(FMA(FMA(c0, q, c1), q, c2), q, c3), q, c4), q, c5);
+        const R den = FMA(FMA(FMA(FMA(d0, q, d1), q, d2), q, d3), q, (R)1);
+        x = num / den;
+    } else if (u > phigh) {
+        const R q = SQRT((R)-2 * LOG1P(-u));
+        const R num = FMA(FMA(FMA(FMA(FMA(c0, q, c1), q, c2), q, c3), q, c4), q, c5);
+        const R den = FMA(FMA(FMA(FMA(d0, q, d1), q, d2), q, d3), q, (R)1);
+        x = -num / den;
+    } else {
+        const R q = u - (R)0.5, r = q * q;
+        const R num = FMA(FMA(FMA(FMA(FMA(a0, r, a1), r, a2), r, a3), r, a4), r, a5) * q;
+        const R den = FMA(FMA(FMA(FMA(FMA(b0, r, b1), r, b2), r, b3), r, b4), r, (R)1);
+        x = num / den;
+    }
+    const R pdf = EXP((R)-0.5 * x * x) * (R)0.3989422804014327;
+    const R cdf = (R)0.5 * ERFC(-x * (R)0.7071067811865476);
+    return x - (cdf - u) / pdf;
+}
+
+static R RF(xfit)(R l) { const R a = (l - (R)595.8) / (R)33.33, b = (l - (R)446.8) / (R)19.44;
+                         return (R)1.065 * EXP((R)-0.5 * a * a) + (R)0.366 * EXP((R)-0.5 * b * b); }
+static R RF(yfit)(R l) { const R k = (LOG(l) - (R)6.3213077) / (R)0.075; return (R)1.014 * EXP((R)-0.5 * k * k); }
+static R RF(zfit)(R l) { const R k = (LOG(l) - (R)6.1088030) / (R)0.051; return (R)1.839 * EXP((R)-0.5 * k * k); }
+
+static R RF(lambda_pdf)(R l)
+{
+    const R total = (R)3.1;
+    const R p = ((R)0.1 / total) * ((R)1 / (R)420) + ((R)1 / total) * (RF(xfit)(l) / (R)106.81109281) +
+                ((R)1 / total) * (RF(yfit)(l) / (R)106.34513640) + ((R)1 / total) * (RF(zfit)(l) / (R)105.88243587);
+    return p > (R)1e-16 ? p : (R)1e-16;
+}
+
+static R RF(sample_lambda)(R u0, R u1, R u2)
+{
+    const R total = (R)3.1, ru = (R)0.1 / total, rx = (R)1 / total, ry = (R)1 / total;
+    R l;
+    if (u0 < ru) l = u1 * (R)420 + (R)360;
+    else if (u0 < ru + rx) {
+        if (u1 < (R)0.8330258723493694) {
+            const R t = (R)7.4884543e-13 + u2 * ((R)0.9999999836707862 - (R)7.4884543e-13);
+            l = (R)595.8 + (R)33.33 * RF(ndtri)(t);
+        } else {
+            const R t = (R)0.0000040030528583 + u2 * ((R)1 - (R)0.0000040030528583);
+            l = (R)446.8 + (R)19.44 * RF(ndtri)(t);
+        }
+    } else if (u0 < ru + rx + ry) {
+        const R t = (R)0.0000000020798312 + u1 * ((R)0.9999953206338883 - (R)0.0000000020798312);
+        l = EXP((R)6.3269327170812479 + (R)0.075 * RF(ndtri)(t));
+    } else {
+        const R t = (R)0.0000049890553249 + u1 * ((R)1 - (R)0.0000049890553249);
+        l = EXP((R)6.1114040395252154 + (R)0.051 * RF(ndtri)(t));
+    }
+    return l < (R)360 ? (R)360 : (l > (R)780 ? (R)780 : l);
+}
+
+static V3 RF(cie)(const float* t, R l)
+{
+    if (l <= (R)360) return RF(load3)(t);
+    if (l >= (R)780) return RF(load3)(t + 3 * 420);
+    const R u = l - (R)360;
+    const int i = (int)u;
+    const R f = u - (R)i;
+    const float* a = t + 3 * i;
+    return RF(v3)((R)a[0] + ((R)a[3] - (R)a[0]) * f, (R)a[1] + ((R)a[4] - (R)a[1]) * f, (R)a[2] + ((R)a[5] - (R)a[2]) * f);
+}
+
+static R RF(d65n)(const float* t, R l)
+{
+    const R k = (R)1 / (R)10567.065f;
+    if (l <= (R)360) return (R)t[0] * k;
+    if (l >= (R)780) return (R)t[420] * k;
+    const R x = l - (R)360;
+    const int i = (int)FLOOR(x);
+    const R f = x - (R)i;
+    return ((R)t[i] + ((R)t[i + 1] - (R)t[i]) * f) * k;
+}
+
+static R RF(sigmoid)(const float* c, R l)
+{
+    const R x = (l - (R)360) * ((R)1 / (R)420);
+    const R q = FMA(FMA((R)c[0], x, (R)c[1]), x, (R)c[2]);
+    if (!(FABS(q) < (R)1e18)) return q > 0 ? (R)1 : (R)0;
+    return (R)0.5 + q / ((R)2 * SQRT((R)1 + q * q));
+}
+
+typedef struct { int mode; R l[4]; } RF(lam);
+
+static RF(lam) RF(make_lam)(int mode, R l0)
+{
+    RF(lam) L;
+    L.mode = mode;
+    L.l[0] = l0;
+    for (int k = 1; k < 4; ++k) {
+        R x = l0 - (R)380 + (R)100 * (R)k;
+        if (x >= (R)400) x -= (R)400;
+        L.l[k] = (R)380 + x;
+    }
+    return L;
+}
+
+static R RF(lum4)(S4 c, int mode)
+{
+    if (mode == 0) return c.v[0] * (R)0.2126 + c.v[1] * (R)0.7152 + c.v[2] * (R)0.0722;
+    if (mode == 1) return c.v[0];
+    return (c.v[0] + c.v[1] + c.v[2] + c.v[3]) * (R)0.25;
+}
+
+static R RF(max4)(S4 c, int mode)
+{
+    R a, b;
+    if (mode == 0) { a = c.v[0] > c.v[1] ? c.v[0] : c.v[1]; return a > c.v[2] ? a : c.v[2]; }
+    if (mode == 1) return c.v[0];
+    a = c.v[0] > c.v[1] ? c.v[0] : c.v[1];
+    b = c.v[2] > c.v[3] ? c.v[2] : c.v[3];
+    return a > b ? a : b;
+}
 
 /* ---------------------------------------------------------------------- */
 /* Sampling (core/render/sample.cuh)                                       */
@@ -170,8 +324,9 @@ static R RF(vndf_pdf)(V3 v, V3 n, V3 l, R alpha)
 
 typedef struct {
     V3 albedo, emission;
-    R specular, metallic, roughness;
+    R specular, metallic, roughness, escale;
     uint32_t model;
+    float ca[3], ce[3];
 } RF(mat);
 
 static R RF(f0_ue)(R spec) { return (R)0.08 * RF(clampr)(spec, 0, 1); }
@@ -185,47 +340,72 @@ static R RF(g_correlated)(R alpha, R nol, R nov)
     return den > 0 ? ((R)2 * nol * nov) / den : 0;
 }
 
-static V3 RF(f0_of)(const RF(mat)* m)
-{
-    const R f = RF(f0_ue)(m->specular);
-    return RF(v3)(f + (m->albedo.x - f) * m->metallic, f + (m->albedo.y - f) * m->metallic,
-                  f + (m->albedo.z - f) * m->metallic);
-}
-
 static R RF(alpha_of)(const RF(mat)* m)
 {
     const R r = RF(clampr)(m->roughness, (R)0.01, (R)1);
     return r * r;
 }
 
+static S4 RF(albedo_of)(const RF(mat)* m, const RF(lam)* L)
+{
+    if (L->mode == 0) return RF(s4)(m->albedo.x, m->albedo.y, m->albedo.z, 0);
+    S4 r;
+    for (int k = 0; k < 4; ++k) r.v[k] = (L->mode == 4 || k == 0) ? RF(sigmoid)(m->ca, L->l[k]) : (R)0;
+    return r;
+}
+
+static S4 RF(emission_of)(const RF(mat)* m, const RF(lam)* L, const float* d65)
+{
+    if (L->mode == 0) return RF(s4)(m->emission.x, m->emission.y, m->emission.z, 0);
+    S4 r;
+    for (int k = 0; k < 4; ++k)
+        r.v[k] = (L->mode == 4 || k == 0) ? RF(sigmoid)(m->ce, L->l[k]) * m->escale * RF(d65n)(d65, L->l[k]) : (R)0;
+    return r;
+}
+
+/* clamp(lum(F0), 0.1, 0.9) in RGB, clamp(F0(lambda_0), 0.1, 0.9) otherwise (material.cuh:257-264) */
+static R RF(pspec_of)(const RF(mat)* m, S4 alb, int mode)
+{
+    const R f = RF(f0_ue)(m->specular);
+    if (mode == 0) {
+        const V3 f0 = RF(v3)(f + (m->albedo.x - f) * m->metallic, f + (m->albedo.y - f) * m->metallic,
+                             f + (m->albedo.z - f) * m->metallic);
+        return RF(clampr)(RF(lum)(f0), (R)0.1, (R)0.9);
+    }
+    return RF(clampr)(f + (alb.v[0] - f) * m->metallic, (R)0.1, (R)0.9);
+}
+
 /* EvaluateBSDFWithCos (material.cuh:47-165), rd toward the surface */
-static V3 RF(eval_bsdf)(const RF(mat)* m, V3 rd, V3 l, V3 n)
+static S4 RF(eval_bsdf)(const RF(mat)* m, S4 alb, V3 rd, V3 l, V3 n)
 {
     const R nol = RF(maxr)((R)0, RF(dot)(n, l));
-    if (m->model == 0) return RF(scale)(m->albedo, nol / PI_R);
+    if (m->model == 0) return RF(s4scale)(alb, nol / PI_R);
     const V3 v = RF(normalize)(RF(scale)(rd, -1));
     const R nov = RF(maxr)((R)0, RF(dot)(n, v));
-    if (nov <= 0 || nol <= 0) return RF(v3)(0, 0, 0);
+    if (nov <= 0 || nol <= 0) return RF(s4)(0, 0, 0, 0);
     const R alpha = RF(alpha_of)(m);
-    const V3 f0 = RF(f0_of)(m);
+    const R f = RF(f0_ue)(m->specular);
     const R kd = (R)1 - m->metallic;
     const V3 h = RF(normalize)(RF(add)(v, l));
     const R noh = RF(maxr)((R)0, RF(dot)(n, h));
     const R loh = RF(maxr)((R)0, RF(dot)(l, h));
     const R c = RF(clampr)((R)1 - loh, 0, 1);
     const R c5 = c * c * c * c * c;
-    const V3 F = RF(v3)(f0.x + ((R)1 - f0.x) * c5, f0.y + ((R)1 - f0.y) * c5, f0.z + ((R)1 - f0.z) * c5);
     const R D = RF(ggx_d)(alpha, noh);
     const R G = RF(g_correlated)(alpha, nol, nov);
     const R sden = RF(maxr)((R)4 * nov, (R)1e-7);
     const R dcos = nol / PI_R;
-    return RF(v3)(F.x * D * G / sden + kd * ((R)1 - F.x) * m->albedo.x * dcos,
-                  F.y * D * G / sden + kd * ((R)1 - F.y) * m->albedo.y * dcos,
-                  F.z * D * G / sden + kd * ((R)1 - F.z) * m->albedo.z * dcos);
+    S4 r;
+    for (int k = 0; k < 4; ++k) {
+        const R f0 = f + (alb.v[k] - f) * m->metallic;
+        const R F = f0 + ((R)1 - f0) * c5;
+        r.v[k] = F * D * G / sden + kd * ((R)1 - F) * alb.v[k] * dcos;
+    }
+    return r;
 }
 
 /* BSDF_PDF (material.cuh:316-371) */
-static R RF(bsdf_pdf)(const RF(mat)* m, V3 rd, V3 l, V3 n)
+static R RF(bsdf_pdf)(const RF(mat)* m, R pspec, V3 rd, V3 l, V3 n)
 {
     const R nol = RF(maxr)((R)0, RF(dot)(n, l));
     if (m->model == 0) return nol > 0 ? nol * ((R)1 / PI_R) : 0;
@@ -233,7 +413,6 @@ static R RF(bsdf_pdf)(const RF(mat)* m, V3 rd, V3 l, V3 n)
     const R nov = RF(maxr)((R)0, RF(dot)(n, v));
     if (nov <= 0 || nol <= 0) return 0;
     const R alpha = RF(alpha_of)(m);
-    const R pspec = RF(clampr)(RF(lum)(RF(f0_of)(m)), (R)0.1, (R)0.9);
     const R ps = RF(vndf_pdf)(v, n, l, alpha);
     const R pd = nol * ((R)1 / PI_R);
     return RF(maxr)((R)0, pspec * ps + ((R)1 - pspec) * pd);
@@ -241,7 +420,7 @@ static R RF(bsdf_pdf)(const RF(mat)* m, V3 rd, V3 l, V3 n)
 
 /* SampleBSDF (material.cuh:167-314).  u[0] lobe, u[1..3] direction.
  * Returns 0 when the path terminates. */
-static int RF(sample_bsdf)(const RF(mat)* m, V3 rd, V3 n, const R* u, V3* next, R* pdf)
+static int RF(sample_bsdf)(const RF(mat)* m, R pspec, V3 rd, V3 n, const R* u, V3* next, R* pdf)
 {
     if (m->model == 0) {
         *next = RF(sample_cosine)(u[1], u[2], u[3], n, pdf);
@@ -251,7 +430,6 @@ static int RF(sample_bsdf)(const RF(mat)* m, V3 rd, V3 n, const R* u, V3* next, 
     const R nov = RF(maxr)((R)0, RF(dot)(n, v));
     if (nov <= 0) return 0;
     const R alpha = RF(alpha_of)(m);
-    const R pspec = RF(clampr)(RF(lum)(RF(f0_of)(m)), (R)0.1, (R)0.9);
     V3 l;
     R ps, pd;
     if (u[0] < pspec) {
@@ -314,6 +492,14 @@ static RF(mat) RF(material)(const orc_render_scene* sc, uint32_t p)
     m.emission = RF(load3)(om->emission);
     m.specular = om->specular; m.metallic = om->metallic; m.roughness = om->roughness;
     m.model = om->model;
+    const uint32_t mi = sc->tri_mat ? sc->tri_mat[p] : 0;
+    m.escale = 1;
+    for (int k = 0; k < 3; ++k) { m.ca[k] = 0.0f; m.ce[k] = 0.0f; }
+    if (sc->mat_coef) {
+        const float* mc = &sc->mat_coef[8 * mi];
+        for (int k = 0; k < 3; ++k) { m.ca[k] = mc[k]; m.ce[k] = mc[3 + k]; }
+        m.escale = (R)mc[6];
+    }
     return m;
 }
 
@@ -331,9 +517,11 @@ static int RF(pick_light)(const orc_render_lights* L, R u)
 /* ---------------------------------------------------------------------- */
 
 static V3 RF(trace_path)(const orc_render_scene* sc, const orc_render_lights* L, const orc_render_opts* op,
-                         uint32_t pix, uint32_t smp, V3 o, V3 d)
+                         uint32_t pix, uint32_t smp, V3 o, V3 d, R lambda0)
 {
-    V3 rad = RF(v3)(0, 0, 0), mask = RF(v3)(1, 1, 1);
+    const int mode = op->spectral;
+    const RF(lam) LM = RF(make_lam)(mode, lambda0);
+    S4 rad = RF(s4)(0, 0, 0, 0), mask = RF(s4)(1, 1, 1, 1);
     int non_delta = 0;
     int prev_nee = 0;
     R last_pdf = 0;
@@ -348,8 +536,20 @@ static V3 RF(trace_path)(const orc_render_scene* sc, const orc_render_lights* L,
         R t, bu, bv;
         const uint32_t prim = RF(closest)(sc, o, d, eps, (R)INFINITY, &t, &bu, &bv);
         if (prim == ORC_MISS) {
-            if (op->sky == 1) rad = RF(add)(rad, RF(mul)(mask, RF(sky)(d)));
-            else if (op->sky == 2) rad = RF(add)(rad, RF(mul)(mask, RF(load3)(op->env)));
+            if (op->sky == 1) {
+                const V3 e = RF(sky)(d);
+                rad = RF(s4add)(rad, RF(s4mul)(mask, RF(s4)(e.x, e.y, e.z, 0)));
+            } else if (op->sky == 2) {
+                S4 e;
+                if (mode == 0) e = RF(s4)((R)op->env[0], (R)op->env[1], (R)op->env[2], 0);
+                else {
+                    RF(mat) em;
+                    em.escale = (R)sc->env_coef[3];
+                    for (int k = 0; k < 3; ++k) em.ce[k] = sc->env_coef[k];
+                    e = RF(emission_of)(&em, &LM, op->d65);
+                }
+                rad = RF(s4add)(rad, RF(s4mul)(mask, e));
+            }
             break;
         }
         V3 a, b, c;
@@ -374,21 +574,23 @@ static V3 RF(trace_path)(const orc_render_scene* sc, const orc_render_lights* L,
                     const R pl = cosy <= 0 ? (R)0 : pa * (dist2 / cosy);
                     if (pl >= 0 && last_pdf > 0) w = RF(power_heuristic)(last_pdf, pl);
                 }
-                rad = RF(add)(rad, RF(scale)(RF(mul)(mask, m.emission), w));
+                rad = RF(s4add)(rad, RF(s4scale)(RF(s4mul)(mask, RF(emission_of)(&m, &LM, op->d65)), w));
             }
             break;
         }
+        const S4 alb = RF(albedo_of)(&m, &LM);
+        const R pspec = m.model == 0 ? (R)0 : RF(pspec_of)(&m, alb, mode);
         V3 next;
         R pdf;
-        if (!RF(sample_bsdf)(&m, d, n, u, &next, &pdf)) break;
-        const V3 fcos = RF(eval_bsdf)(&m, d, next, n);
+        if (!RF(sample_bsdf)(&m, pspec, d, n, u, &next, &pdf)) break;
+        const S4 fcos = RF(eval_bsdf)(&m, alb, d, next, n);
         /* offset along the side of the next direction (render.cuh:155-160) */
         const R off = eps * RF(maxr)((R)1, RF(length)(p)) * (u[4] + (R)0.5);
         const R sgn = RF(dot)(n, next) > 0 ? (R)1 : (RF(dot)(n, next) < 0 ? (R)-1 : (R)0);
         p = RF(add)(p, RF(scale)(n, sgn * off));
         int did_nee = 0;
         if (op->nee && L->count > 0 && depth < op->max_depth - 1 && non_delta < 10 &&
-            u[5] < SQRT(RF(clampr)(RF(lum)(mask), 0, 1))) {
+            u[5] < SQRT(RF(clampr)(RF(lum4)(mask, mode), 0, 1))) {
             const int li = RF(pick_light)(L, u[6]);
             const uint32_t lp = (uint32_t)L->tri[li];
             V3 la, lb, lc;
@@ -417,13 +619,13 @@ static V3 RF(trace_path)(const orc_render_scene* sc, const orc_render_lights* L,
                 const R cosx = RF(maxr)((R)0, RF(dot)(n, ldir));
                 const R cosy = RF(maxr)((R)0, -RF(dot)(ln, ldir));
                 if (cosx > 0 && cosy > 0) {
-                    const V3 f = RF(eval_bsdf)(&m, d, ldir, n);
+                    const S4 f = RF(eval_bsdf)(&m, alb, d, ldir, n);
                     const R g = cosy / dist2;
-                    const V3 contrib = RF(scale)(RF(mul)(f, lm.emission), g / pa_total);
+                    const S4 contrib = RF(s4scale)(RF(s4mul)(f, RF(emission_of)(&lm, &LM, op->d65)), g / pa_total);
                     const R pl = pa_total * (dist2 / cosy);
-                    const R pb = RF(maxr)(RF(bsdf_pdf)(&m, d, ldir, n), (R)0);
+                    const R pb = RF(maxr)(RF(bsdf_pdf)(&m, pspec, d, ldir, n), (R)0);
                     const R w = RF(power_heuristic)(pl, pb);
-                    rad = RF(add)(rad, RF(mul)(mask, RF(scale)(contrib, w)));
+                    rad = RF(s4add)(rad, RF(s4mul)(mask, RF(s4scale)(contrib, w)));
                     did_nee = 1;
                 }
             }
@@ -432,20 +634,38 @@ static V3 RF(trace_path)(const orc_render_scene* sc, const orc_render_lights* L,
         last_pdf = pdf;
         last_pos = p;
         const R inv_pdf = (R)1 / RF(maxr)(pdf, (R)1e-12);
-        const V3 contrib = RF(scale)(fcos, inv_pdf);
-        R q = 1;
+        const S4 contrib = RF(s4scale)(fcos, inv_pdf);
+        R qrr = 1;
         if (non_delta >= op->rr_min_depth) {
-            q = RF(clampr)(RF(lum)(mask) * (R)0.5 + RF(max3)(mask) * (R)0.5, (R)0.05, (R)0.95);
-            if (u[9] > q) break;
+            qrr = RF(clampr)(RF(lum4)(mask, mode) * (R)0.5 + RF(max4)(mask, mode) * (R)0.5, (R)0.05, (R)0.95);
+            if (u[9] > qrr) break;
         }
-        mask = RF(scale)(RF(mul)(mask, contrib), (R)1 / q);
+        mask = RF(s4scale)(RF(s4mul)(mask, contrib), (R)1 / qrr);
         o = p;
         d = next;
         non_delta++;
     }
-    V3 out = RF(v3)(RF(fixnan)(rad.x), RF(fixnan)(rad.y), RF(fixnan)(rad.z));
-    out = RF(v3)(RF(clampr)(out.x, 0, (R)1e3), RF(clampr)(out.y, 0, (R)1e3), RF(clampr)(out.z, 0, (R)1e3));
-    return out;
+    /* pixel value (render.cuh:966-997) */
+    if (mode == 0) {
+        V3 out = RF(v3)(RF(fixnan)(rad.v[0]), RF(fixnan)(rad.v[1]), RF(fixnan)(rad.v[2]));
+        return RF(v3)(RF(clampr)(out.x, 0, (R)1e3), RF(clampr)(out.y, 0, (R)1e3), RF(clampr)(out.z, 0, (R)1e3));
+    }
+    R X = 0, Y = 0, Z = 0;
+    if (mode == 1) {
+        const V3 c = RF(cie)(op->cie, LM.l[0]);
+        const R pdf = RF(lambda_pdf)(LM.l[0]);
+        const R v = RF(fixnan)(rad.v[0]);
+        X = c.x * v / pdf; Y = c.y * v / pdf; Z = c.z * v / pdf;
+    } else {
+        const R psum = (R)4 / (R)400;
+        for (int k = 0; k < 4; ++k) {
+            const V3 c = RF(cie)(op->cie, LM.l[k]);
+            const R v = RF(fixnan)(rad.v[k]);
+            X = X + c.x * v; Y = Y + c.y * v; Z = Z + c.z * v;
+        }
+        X = X / psum; Y = Y / psum; Z = Z / psum;
+    }
+    return RF(v3)(RF(clampr)(X, 0, (R)1e3), RF(clampr)(Y, 0, (R)1e3), RF(clampr)(Z, 0, (R)1e3));
 }
 
 /* RenderCamera (core/scene.cu:135-189): jittered pinhole, vertical fov. */
@@ -473,19 +693,35 @@ static void RF(render)(const orc_render_scene* sc, const orc_render_lights* L, c
         const uint32_t i = (uint32_t)(px % op->width), j = (uint32_t)(px / op->width);
         V3 acc = RF(v3)(0, 0, 0);
         for (uint32_t s = 0; s < (uint32_t)op->spp; ++s) {
-            float uf[4];
+            float uf[8];
             orc_philox_uniforms(op->seed, (uint32_t)px, s + op->sample_offset, ORC_CAMERA_BOUNCE, 1, uf);
             const V3 d = RF(camera_dir)(op, i, j, (R)uf[0], (R)uf[1]);
-            const V3 r = RF(trace_path)(sc, L, op, (uint32_t)px, s + op->sample_offset, RF(load3)(op->cam_pos), d);
+            R l0 = 0;
+            if (op->spectral != 0) {
+                float w[4];
+                orc_philox_uniforms(op->seed, (uint32_t)px, s + op->sample_offset, ORC_CAMERA_BOUNCE, 2, uf);
+                for (int k = 0; k < 4; ++k) w[k] = uf[4 + k];
+                l0 = op->spectral == 1 ? RF(sample_lambda)((R)w[0], (R)w[1], (R)w[2]) : (R)380 + (R)400 * (R)w[0];
+            }
+            const V3 r = RF(trace_path)(sc, L, op, (uint32_t)px, s + op->sample_offset, RF(load3)(op->cam_pos), d, l0);
             acc = RF(add)(acc, r);
         }
-        out[3 * px + 0] = acc.x / (R)op->spp;
-        out[3 * px + 1] = acc.y / (R)op->spp;
-        out[3 * px + 2] = acc.z / (R)op->spp;
+        R x = acc.x / (R)op->spp, y = acc.y / (R)op->spp, z = acc.z / (R)op->spp;
+        if (op->spectral != 0 && op->output == 1) {
+            /* XYZ2SRGBLinearD65 (core/tonemap.cuh:116-126) */
+            const R r = (R)3.2406 * x + (R)-1.5372 * y + (R)-0.4986 * z;
+            const R g = (R)-0.9689 * x + (R)1.8758 * y + (R)0.0415 * z;
+            const R b = (R)0.0557 * x + (R)-0.204 * y + (R)1.057 * z;
+            x = RF(maxr)((R)0, r); y = RF(maxr)((R)0, g); z = RF(maxr)((R)0, b);
+        }
+        out[3 * px + 0] = x;
+        out[3 * px + 1] = y;
+        out[3 * px + 2] = z;
     }
 }
 
 #undef V3
+#undef S4
 #undef RF
 #undef R
 #undef SUF
@@ -497,3 +733,8 @@ static void RF(render)(const orc_render_scene* sc, const orc_render_lights* L, c
 #undef POW
 #undef FABS
 #undef PI_R
+#undef LOG
+#undef LOG1P
+#undef FMA
+#undef ERFC
+#undef FLOOR

@@ -178,7 +178,58 @@ class orc_render_opts(C.Structure):
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32), ("sample_offset", C.c_uint32),
                 ("seed", C.c_uint64), ("max_depth", C.c_int), ("rr_min_depth", C.c_int), ("nee", C.c_int),
                 ("sky", C.c_int), ("env", C.c_float * 3), ("cam_pos", C.c_float * 3), ("cam_dir", C.c_float * 3),
-                ("vfov_deg", C.c_float)]
+                ("vfov_deg", C.c_float), ("spectral", C.c_int), ("output", C.c_int), ("cie", C.c_void_p),
+                ("d65", C.c_void_p), ("lut", C.c_void_p)]
+
+
+CIE_TABLE = ROOT / "aperturerendererv3_amd" / "data" / "cie1931_d65_1nm.txt"
+_cie = None
+
+
+def cie_tables():
+    """(xyz (421,3) f32, d65 (421,) f32) from the committed CIE data file."""
+    global _cie
+    if _cie is None:
+        a = np.loadtxt(CIE_TABLE)
+        _cie = (np.ascontiguousarray(a[:, 1:4], np.float32), np.ascontiguousarray(a[:, 4], np.float32))
+    return _cie
+
+
+_lut = None
+
+
+def uplift_lut():
+    """The 3 x 64^3 x 3 coefficient table, built by the library's host code
+    (ar_spectrum_lut_build: no device needed).  Input data for the oracle."""
+    global _lut
+    if _lut is None:
+        from aperturerendererv3_amd import _lib as L
+        lib = L.load()
+        out = np.zeros(3 * 64 ** 3 * 3, np.float32)
+        rc = lib.ar_spectrum_lut_build(out.ctypes.data, out.size)
+        assert rc == 0
+        _lut = out
+    return _lut
+
+
+def uplift(rgb):
+    lib = load()
+    lut = uplift_lut()
+    lib.orc_uplift.restype = None
+    lib.orc_uplift.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+    rgb = np.ascontiguousarray(rgb, np.float32).reshape(-1, 3)
+    out = np.zeros_like(rgb)
+    for i in range(rgb.shape[0]):
+        lib.orc_uplift(lut.ctypes.data, rgb[i].ctypes.data, out[i].ctypes.data)
+    return out
+
+
+def fn(name, res, args):
+    lib = load()
+    f = getattr(lib, name)
+    f.restype = res
+    f.argtypes = args
+    return f
 
 
 def philox(ctr, key):
@@ -214,7 +265,7 @@ def make_materials(mats):
 
 def render(V, F, tri_mat, mats, width, height, spp, seed=1, max_depth=5, nee=1, sky=0, env=(0, 0, 0),
            cam_pos=(0, 0, 1.1), cam_dir=(0, 0, -1), vfov=60.0, precision=32, rr_min_depth=3, sample_offset=0,
-           threads=0):
+           threads=0, spectral=0, output=0):
     """Oracle render; returns (height, width, 3) float32 or float64 mean image."""
     lib = load()
     V = np.ascontiguousarray(V, np.float32)
@@ -228,6 +279,11 @@ def render(V, F, tri_mat, mats, width, height, spp, seed=1, max_depth=5, nee=1, 
     o.cam_pos[:] = cam_pos
     o.cam_dir[:] = cam_dir
     o.vfov_deg = vfov
+    o.spectral, o.output = spectral, output
+    xyz, d65 = cie_tables()
+    o.cie, o.d65 = xyz.ctypes.data, d65.ctypes.data
+    if spectral:
+        o.lut = uplift_lut().ctypes.data
     out = np.zeros((height, width, 3), np.float64 if precision == 64 else np.float32)
     lib.orc_render.restype = C.c_int
     lib.orc_render.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p,

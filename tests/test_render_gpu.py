@@ -120,3 +120,43 @@ def test_render_errors(ctx):
         ctx.render(0, 4, 1)
     with pytest.raises(ArError):
         ctx.render(4, 4, 1, max_depth=0)
+
+
+@pytest.mark.parametrize("mode,output", [(1, 0), (1, 1), (4, 0), (4, 1)])
+def test_spectral_cornell_matches_oracle(ctx, mode, output):
+    """Spectral modes draw the wavelength from the camera stream at equal seed;
+    the GPU's expf/logf/erfcf may round one ulp away from the host's, which
+    moves a wavelength by ~1e-5 nm, so the per-pixel bar is 1e-4 relative."""
+    V, F, M, mats = cornell_box()
+    mats = [dict(m) for m in mats]
+    mats[2].update({"model": 1, "roughness": 0.3, "specular": 0.5, "metallic": 0.5})
+    kw = dict(_kw(CORNELL_CAMERA), seed=4, max_depth=6, spectral=mode, output=output)
+    g = gpu_render(ctx, V, F, M, mats, 48, 48, 8, **kw)
+    o = orc.render(V, F, M, mats, 48, 48, 8, precision=32, **kw)
+    assert_close_to_oracle(g, o, f"spectral mode={mode} output={output}", rtol=1e-4, atol=1e-5, min_frac=0.99)
+
+
+def test_spectral_furnace_closed_form(ctx):
+    """Gray floor (0.6) under a white constant sky, four hero wavelengths:
+    the sRGB image mean is the closed-form integral of the uplifted spectra
+    (within 4 standard errors) and that integral is 0.6 per channel."""
+    from test_oracle_spectral import _furnace_expect
+    V, F, M, mats, _ = geom.floor_and_light(albedo=0.6)
+    F, M, mats = F[:2], M[:2], mats[:1]
+    g = gpu_render(ctx, V, F, M, mats, 64, 64, 64, max_depth=4, nee=False, sky=2, env=(1.0, 1.0, 1.0),
+                   cam_pos=(0, 0.5, 0.01), cam_dir=(0, -1, 1e-3), vfov=20.0, rr_min_depth=100, spectral=4, output=1)
+    got = g.reshape(-1, 3)
+    mean, sem = got.mean(0), got.std(0) / np.sqrt(got.shape[0])
+    expect = _furnace_expect(0.6, 4, 1)
+    assert np.all(np.abs(mean - expect) < 4 * sem + 1e-4), (mean, expect, sem)
+
+
+def test_spectral_rejects_gradient_sky(ctx):
+    from aperturerendererv3_amd import ArError
+    V, F, M, mats = cornell_box()
+    ctx.set_meshes([{"vertices": V, "indices": F, "tri_material": M, "materials": mats}])
+    ctx.build_bvh()
+    with pytest.raises(ArError):
+        ctx.render(8, 8, 1, sky=1, spectral=4)
+    with pytest.raises(ArError):
+        ctx.render(8, 8, 1, spectral=2)
