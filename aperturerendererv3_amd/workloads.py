@@ -217,3 +217,121 @@ def cornell_box(light_le=(17.0, 12.0, 4.0)):
         {"albedo": (0.0, 0.0, 0.0), "emission": tuple(light_le)},
     ]
     return (np.asarray(V, np.float32), np.asarray(F, np.uint32), np.asarray(M, np.uint32), mats)
+
+
+# --------------------------------------------------------------------------
+# Spectral instanced scene (~500k triangles, 64 triangle lights, Lambert/GGX)
+# --------------------------------------------------------------------------
+
+SPECTRAL_CAMERA = {"pos": (0.0, 0.0, 30.0), "dir": (0.0, 0.0, -1.0), "vfov": 60.0}
+
+
+def _subdivide_sphere(V, F, levels):
+    """Loop-style midpoint subdivision projected to the unit sphere."""
+    for _ in range(levels):
+        e = np.concatenate([F[:, [0, 1]], F[:, [1, 2]], F[:, [2, 0]]])
+        lo, hi = np.minimum(e[:, 0], e[:, 1]), np.maximum(e[:, 0], e[:, 1])
+        key = lo * (V.shape[0] + 1) + hi
+        uk, inv = np.unique(key, return_inverse=True)
+        a, b = uk // (V.shape[0] + 1), uk % (V.shape[0] + 1)
+        mid = V[a] + V[b]
+        mid /= np.linalg.norm(mid, axis=1, keepdims=True)
+        base = V.shape[0]
+        V = np.concatenate([V, mid])
+        m = F.shape[0]
+        ab, bc, ca = base + inv[:m], base + inv[m:2 * m], base + inv[2 * m:]
+        F = np.concatenate([
+            np.stack([F[:, 0], ab, ca], 1), np.stack([F[:, 1], bc, ab], 1),
+            np.stack([F[:, 2], ca, bc], 1), np.stack([ab, bc, ca], 1)])
+    return V, F
+
+
+def icosphere(levels: int = 3):
+    """Unit icosphere with outward winding: 20 * 4**levels triangles
+    (1,280 at 3 levels)."""
+    phi = (1.0 + math.sqrt(5.0)) / 2.0
+    V = []
+    for a in (-1.0, 1.0):
+        for b in (-phi, phi):
+            V += [(a, b, 0.0), (0.0, a, b), (b, 0.0, a)]
+    V = np.asarray(V)
+    n = len(V)
+    d = np.linalg.norm(V[:, None] - V[None], axis=-1)
+    F = []
+    for i in range(n):
+        for j in range(i + 1, n):
+            for k in range(j + 1, n):
+                if abs(d[i, j] - 2) < 1e-9 and abs(d[j, k] - 2) < 1e-9 and abs(d[i, k] - 2) < 1e-9:
+                    nrm = np.cross(V[j] - V[i], V[k] - V[i])
+                    F.append((i, j, k) if nrm @ (V[i] + V[j] + V[k]) > 0 else (i, k, j))
+    V = V / np.linalg.norm(V, axis=1, keepdims=True)
+    return _subdivide_sphere(V, np.asarray(F, np.int64), levels)
+
+
+_BOX_V = np.array([[x, y, z] for x in (-0.5, 0.5) for y in (-0.5, 0.5) for z in (-0.5, 0.5)])
+_BOX_F = np.array([[0, 1, 3], [0, 3, 2], [4, 6, 7], [4, 7, 5], [0, 4, 5], [0, 5, 1],
+                   [2, 3, 7], [2, 7, 6], [0, 2, 6], [0, 6, 4], [1, 5, 7], [1, 7, 3]], np.int64)
+
+
+def spectral_scene(target_tris: int = 500_000, nlights: int = 64, seed: int = 1, sphere_levels: int = 3):
+    """Instances of displaced icospheres (radial 1 + 0.1 n(p), value noise at
+    frequency 4 with a per-instance offset) and boxes (random turn about +Y),
+    scale U[0.2, 1], centres U[-10, 10]^3, one kind or the other with equal
+    probability, added until `target_tris` is reached (within one sphere);
+    above a 100 x 100 two-triangle floor at y = -10 (grey 0.5 Lambert).
+    Each instance has its own material: Lambert or GGX with equal probability,
+    albedo U[0.05, 0.95]^3, roughness U[0.05, 1], metallic 0, specular 0.5.
+    `nlights` distinct instance triangles become white emitters with
+    Le = min(U^(-1/1.5), 1000), each its own material (its instance's
+    material plus the emission).
+    Returns (V f32 (n,3), F u32 (m,3), tri_material u32 (m,), materials)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    SV, SF = icosphere(sphere_levels)
+    BV = _BOX_V
+    parts_v, parts_f, parts_m = [], [], []
+    mats = []
+    nv = 0
+    ntri = 0
+    inst = 0
+    while ntri < target_tris:
+        sphere = rng.random() < 0.5
+        s = 0.2 + 0.8 * rng.random()
+        c = -10.0 + 20.0 * rng.random(3)
+        if sphere:
+            off = rng.random(3) * 1000.0
+            r = 1.0 + 0.1 * value_noise(SV * 4.0 + off, seed + 7)
+            v = SV * (r * s)[:, None] + c
+            f = SF
+        else:
+            a = 2.0 * math.pi * rng.random()
+            ca, sa = math.cos(a), math.sin(a)
+            R = np.array([[ca, 0.0, sa], [0.0, 1.0, 0.0], [-sa, 0.0, ca]])
+            v = (BV * s) @ R.T + c
+            f = _BOX_F
+        ggx = rng.random() < 0.5
+        alb = tuple(float(x) for x in 0.05 + 0.9 * rng.random(3))
+        rough = float(0.05 + 0.95 * rng.random())
+        mats.append({"albedo": alb, "model": 1 if ggx else 0, "roughness": rough, "metallic": 0.0, "specular": 0.5})
+        parts_v.append(v)
+        parts_f.append(f + nv)
+        parts_m.append(np.full(f.shape[0], inst, np.uint32))
+        nv += v.shape[0]
+        ntri += f.shape[0]
+        inst += 1
+    L = 50.0
+    fl = len(mats)
+    mats.append({"albedo": (0.5, 0.5, 0.5)})
+    parts_v.append(np.array([[-L, -10.0, -L], [-L, -10.0, L], [L, -10.0, L], [L, -10.0, -L]]))
+    parts_f.append(np.array([[0, 1, 2], [0, 2, 3]], np.int64) + nv)
+    parts_m.append(np.full(2, fl, np.uint32))
+    V = np.concatenate(parts_v).astype(np.float32)
+    F = np.concatenate(parts_f).astype(np.uint32)
+    M = np.concatenate(parts_m)
+    lights = rng.choice(ntri, size=nlights, replace=False)
+    le = np.minimum(rng.random(nlights) ** (-1.0 / 1.5), 1000.0)
+    for k, t in enumerate(lights):
+        m = dict(mats[M[t]])
+        m["emission"] = (float(le[k]),) * 3
+        M[t] = len(mats)
+        mats.append(m)
+    return V, F, M, mats

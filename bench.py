@@ -55,6 +55,8 @@ def parse_args():
     ap.add_argument("--leaf-size", type=int, default=3)
     ap.add_argument("--width", type=int, default=8, help="BVH branching factor (2 or 8)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--spectral-only", action="store_true", help="run only the spectral 1080p render bench")
+    ap.add_argument("--spectral-spp", type=int, default=64)
     ap.add_argument("--cpu-sample-stride", type=int, default=2)
     ap.add_argument("--profile", action="store_true",
                     help="timed steps only, plus one 256 MiB device copy used to scale HBM counters")
@@ -130,8 +132,63 @@ def integrator_bench(ctx_cls, reps=5, cpu=True):
     return res
 
 
+def spectral_bench(ctx_cls, frames=1, spp=64, cpu=True):
+    """Spectral render of the instanced scene: ~500k triangles (displaced
+    icospheres and boxes above a floor), 64 triangle lights, Lambert/GGX,
+    1920x1080, `spp` samples, max depth 8, four hero wavelengths, NEE on, sky
+    off, CIE XYZ output.  GPU paths/s through ar_render (one frame per step);
+    the oracle's FP32 integrator on the host cores at 8 spp on a 960x540
+    image of the same view as the CPU sample."""
+    import torch
+    from aperturerendererv3_amd.workloads import SPECTRAL_CAMERA, spectral_scene
+    V, F, M, mats = spectral_scene()
+    cam = SPECTRAL_CAMERA
+    kw = dict(cam_pos=cam["pos"], cam_dir=cam["dir"], vfov=cam["vfov"], seed=1, max_depth=8, spectral=4, output=0)
+    w, h = 1920, 1080
+    with ctx_cls(0) as ctx:
+        ctx.set_meshes([{"vertices": V, "indices": F, "tri_material": M, "materials": mats}])
+        t0 = time.perf_counter()
+        info = ctx.build_bvh()
+        build_s = time.perf_counter() - t0
+        out = torch.empty((h, w, 3), dtype=torch.float32, device="cuda:0")
+        ctx.render(w, h, 1, out=out, **kw)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(frames):
+            ctx.render(w, h, spp, out=out, **kw)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / frames
+        mean = float(out.mean().item())
+    paths = w * h * spp
+    res = {"workload": f"instanced spectral scene, {F.shape[0]} triangles, 64 triangle lights, Lambert/GGX, "
+                       f"{w}x{h}, {spp} spp, max depth 8, 4 hero wavelengths, NEE on, sky off, seed 1",
+           "bvh_build_s": round(build_s, 3), "bvh": {"nodes": info["node_count"], "depth": info["depth"]},
+           "gpu_ms_per_frame": round(ms, 2), "gpu_paths_per_s": round(paths / (ms * 1e-3), 1),
+           "image_mean_Y": mean}
+    if cpu:
+        sys.path.insert(0, str(ROOT / "tests"))
+        import oracle_lib as orc
+        cores = len(os.sched_getaffinity(0))
+        cw, ch, cspp = 960, 540, 8
+        t0 = time.perf_counter()
+        orc.render(V, F, M, mats, cw, ch, cspp, precision=32, threads=cores, **kw)
+        dt = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": round(cw * ch * cspp / dt, 1), "unit": "paths/s", "cores": cores,
+                               "kind": "port",
+                               "sample": f"{cw}x{ch} at {cspp} spp ({cw * ch * cspp} paths) of the same view with the "
+                                         f"oracle FP32 integrator, including its BVH build, {dt:.2f} s"}
+    return res
+
+
 def main():
     args = parse_args()
+    if args.spectral_only:
+        from aperturerendererv3_amd import Context
+        print(json.dumps(spectral_bench(Context, frames=max(1, args.steps), spp=args.spectral_spp,
+                                        cpu=not args.no_cpu_baseline)), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(the traversal shards by ray slots with the BVH replicated, see DESIGN.md)", file=sys.stderr)
@@ -242,6 +299,7 @@ def main():
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(V, F, rays_np, args.cpu_sample_stride, shadow.cpu().numpy())
     integ = integrator_bench(Context, cpu=not args.no_cpu_baseline)
+    spectral = spectral_bench(Context, frames=2, spp=args.spectral_spp, cpu=not args.no_cpu_baseline)
 
     valu_frac = achieved_tops / VALU_PEAK_TOPS
     out = {
@@ -263,6 +321,7 @@ def main():
         "cpu_baseline": cpu,
         "breakdown": {k: round(v, 4) for k, v in breakdown.items()},
         "integrator": integ,
+        "spectral_1080p": spectral,
     }
     print(json.dumps(out), flush=True)
     ctx.close()
