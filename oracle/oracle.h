@@ -151,6 +151,44 @@ typedef struct {
     const float* cdf;      /* running FP32 sum of prob */
 } orc_render_lights;
 
+/* ---------------- voxel path guiding (guide.c) ---------------- */
+
+#define ORC_NO_KEY 0xFFFFFFFFFFFFFFFFull
+
+/* Clarberg equal-area octahedral map between [0,1]^2 and the unit sphere. */
+void orc_uv_to_dir(float u, float v, float d[3]);
+void orc_dir_to_uv(const float d[3], float uv[2]);
+/* icosahedral face (argmax of the 20 face normals) of a direction */
+int orc_face20(const float n[3]);
+const float* orc_face20_normals(void);
+/* probe key of (position, direction): 0.05 voxel and face, packed 19/19/19/5
+ * bits; ORC_NO_KEY outside +-2^18 voxels */
+uint64_t orc_voxel_key(const float p[3], const float dir[3]);
+/* leaf texel (y*8 + x) of a direction */
+int orc_leaf_pixel(const float dir[3]);
+
+/* A probe texture is 86 float4 texels: 64 leaves (8x8), 16, 4, 1 mips, pad. */
+void orc_probe_mips(float* tex);
+float orc_probe_pdf(const float* tex, const float dir[3], const float q[3]);
+void orc_probe_sample(const float* tex, const float q[3], const float u[5], float dir[3], float* pdf);
+float orc_probe_hemi_pdf(const float* tex, const float dir[3], const float n[3], const float q[3]);
+void orc_probe_hemi_sample(const float* tex, const float n[3], const float q[3], const float u[5], float dir[3],
+                           float* pdf);
+
+typedef struct orc_guide orc_guide;
+orc_guide* orc_guide_create(int cap);
+void orc_guide_destroy(orc_guide* g);
+int orc_guide_count(const orc_guide* g);
+int orc_guide_find(const orc_guide* g, uint64_t key);
+const float* orc_guide_texels(const orc_guide* g, int i);
+/* keys (count), texels (count x 85 x 4) and sample counts (count) in probe order */
+void orc_guide_export(const orc_guide* g, uint64_t* keys, float* tex, uint64_t* counts);
+/* one frame: ns samples (key, direction, radiance) accumulated in order */
+void orc_guide_update(orc_guide* g, int ns, const uint64_t* keys, const float* dirs, const float* rad);
+
+/* guiding mixture probability of frame f: 0.5 * (min(1, f/64))^0.5 */
+float orc_guide_alpha(uint32_t frame);
+
 typedef struct {
     uint32_t width, height, spp, sample_offset;
     uint64_t seed;
@@ -166,6 +204,8 @@ typedef struct {
     const float* cie;      /* 421 x 3 CIE 1931 2-degree table, 360..780 nm at 1 nm */
     const float* d65;      /* 421 D65 values */
     const float* lut;      /* 3 x 64^3 x 3 sigmoid-coefficient table (spectral modes) */
+    int guiding;           /* 1: voxel guiding, one sample per pixel per frame, probes updated per frame */
+    orc_guide* guide;      /* probe store used and updated when guiding (kept across calls) */
 } orc_render_opts;
 
 /* Sigmoid coefficients of an RGB colour (channels clamped to [0, 1]) from the

@@ -216,6 +216,14 @@ static R RF(max4)(S4 c, int mode)
     return a > b ? a : b;
 }
 
+/* all active channels zero (allZero in render.cuh:364-407) */
+static int RF(s4zero)(S4 c, int mode)
+{
+    if (mode == 0) return c.v[0] == 0 && c.v[1] == 0 && c.v[2] == 0;
+    if (mode == 1) return c.v[0] == 0;
+    return c.v[0] == 0 && c.v[1] == 0 && c.v[2] == 0 && c.v[3] == 0;
+}
+
 /* ---------------------------------------------------------------------- */
 /* Sampling (core/render/sample.cuh)                                       */
 /* ---------------------------------------------------------------------- */
@@ -516,8 +524,62 @@ static int RF(pick_light)(const orc_render_lights* L, R u)
 /* One path (CalculateRadiance, core/render/render.cuh:460-998)            */
 /* ---------------------------------------------------------------------- */
 
+/* guiding: slot radiance over the hero/one wavelength(s) as XYZ, or RGB */
+static V3 RF(slot_xyz)(const orc_render_opts* op, const RF(lam)* LM, S4 r)
+{
+    if (LM->mode == 0) return RF(v3)(r.v[0], r.v[1], r.v[2]);
+    if (LM->mode == 1) {
+        const V3 c = RF(cie)(op->cie, LM->l[0]);
+        const R pdf = RF(lambda_pdf)(LM->l[0]);
+        return RF(v3)(c.x * r.v[0] / pdf, c.y * r.v[0] / pdf, c.z * r.v[0] / pdf);
+    }
+    R X = 0, Y = 0, Z = 0;
+    for (int k = 0; k < 4; ++k) {
+        const V3 c = RF(cie)(op->cie, LM->l[k]);
+        X = X + c.x * r.v[k]; Y = Y + c.y * r.v[k]; Z = Z + c.z * r.v[k];
+    }
+    const R psum = (R)4 / (R)400;
+    return RF(v3)(X / psum, Y / psum, Z / psum);
+}
+
+/* maskForProbeQuery (render.cuh:893-900): max(w, 1e-6) / (sum + 1e-6), with
+ * w = throughput (RGB) or throughput times xbar..zbar of the wavelength(s) */
+static void RF(probe_query)(const orc_render_opts* op, const RF(lam)* LM, S4 mask, float q[3])
+{
+    R w[3];
+    if (LM->mode == 0) { w[0] = mask.v[0]; w[1] = mask.v[1]; w[2] = mask.v[2]; }
+    else if (LM->mode == 1) {
+        const V3 c = RF(cie)(op->cie, LM->l[0]);
+        w[0] = mask.v[0] * c.x; w[1] = mask.v[0] * c.y; w[2] = mask.v[0] * c.z;
+    } else {
+        w[0] = w[1] = w[2] = 0;
+        for (int k = 0; k < 4; ++k) {
+            const V3 c = RF(cie)(op->cie, LM->l[k]);
+            w[0] = w[0] + mask.v[k] * c.x; w[1] = w[1] + mask.v[k] * c.y; w[2] = w[2] + mask.v[k] * c.z;
+        }
+    }
+    for (int k = 0; k < 3; ++k) q[k] = (float)RF(maxr)(w[k], (R)1e-6f);
+    const float sum = (float)((double)(q[0] + q[1] + q[2]) + 1e-6);
+    for (int k = 0; k < 3; ++k) q[k] = q[k] / sum;
+}
+
+static inline V3 RF(reflectv)(V3 v, V3 n) { return RF(sub)(v, RF(scale)(n, (R)2 * RF(dot)(v, n))); }
+
+typedef struct { uint64_t key; float dir[3], rad[3]; } RF(gsample);
+
+/* CollectRadiance with an endpoint emission: every open slot started before
+ * this vertex receives throughput * Le (render.cuh:364-376) */
+static void RF(g_endpoint)(S4* gm, S4* gr, int* gval, S4 le, int non_delta, int mode)
+{
+    if (RF(s4zero)(le, mode)) return;
+    const int hi = non_delta < 5 ? non_delta : 5;
+    for (int i = 0; i < hi; ++i)
+        if (!RF(s4zero)(gm[i], mode)) { gr[i] = RF(s4add)(gr[i], RF(s4mul)(gm[i], le)); gval[i] = 1; }
+}
+
 static V3 RF(trace_path)(const orc_render_scene* sc, const orc_render_lights* L, const orc_render_opts* op,
-                         uint32_t pix, uint32_t smp, V3 o, V3 d, R lambda0)
+                         uint32_t pix, uint32_t smp, V3 o, V3 d, R lambda0,
+                         const orc_guide* G, float alpha, RF(gsample)* gout, int* ngout)
 {
     const int mode = op->spectral;
     const RF(lam) LM = RF(make_lam)(mode, lambda0);
@@ -526,21 +588,34 @@ static V3 RF(trace_path)(const orc_render_scene* sc, const orc_render_lights* L,
     int prev_nee = 0;
     R last_pdf = 0;
     V3 last_pos = RF(v3)(0, 0, 0);
+    /* guiding slots: throughput since the slot's vertex, collected radiance,
+     * sampled direction and probe key (CollectRadiance, render.cuh:345-410) */
+    S4 gm[5], gr[5];
+    V3 gdir[5];
+    uint64_t gkey[5];
+    int gval[5];
+    for (int i = 0; i < 5; ++i) {
+        gm[i] = RF(s4)(0, 0, 0, 0); gr[i] = gm[i]; gdir[i] = RF(v3)(0, 0, 0); gkey[i] = ORC_NO_KEY; gval[i] = 0;
+    }
+    float gq[3] = { 1.0f, 1.0f, 1.0f };
+    const int guided = G != NULL && orc_guide_count(G) > 0;
+    if (ngout) *ngout = 0;
     d = RF(normalize)(d);
     for (int depth = 0; depth < op->max_depth; ++depth) {
-        float uf[12];
-        orc_philox_uniforms(op->seed, pix, smp, (uint32_t)depth, 3, uf);
+        float uf[16];
+        orc_philox_uniforms(op->seed, pix, smp, (uint32_t)depth, guided ? 4 : 3, uf);
         R u[12];
         for (int k = 0; k < 12; ++k) u[k] = (R)uf[k];
         const R eps = RF(intersect_eps)(non_delta);
         R t, bu, bv;
         const uint32_t prim = RF(closest)(sc, o, d, eps, (R)INFINITY, &t, &bu, &bv);
         if (prim == ORC_MISS) {
+            S4 e = RF(s4)(0, 0, 0, 0);
             if (op->sky == 1) {
-                const V3 e = RF(sky)(d);
-                rad = RF(s4add)(rad, RF(s4mul)(mask, RF(s4)(e.x, e.y, e.z, 0)));
+                const V3 sk = RF(sky)(d);
+                e = RF(s4)(sk.x, sk.y, sk.z, 0);
+                rad = RF(s4add)(rad, RF(s4mul)(mask, e));
             } else if (op->sky == 2) {
-                S4 e;
                 if (mode == 0) e = RF(s4)((R)op->env[0], (R)op->env[1], (R)op->env[2], 0);
                 else {
                     RF(mat) em;
@@ -550,6 +625,7 @@ static V3 RF(trace_path)(const orc_render_scene* sc, const orc_render_lights* L,
                 }
                 rad = RF(s4add)(rad, RF(s4mul)(mask, e));
             }
+            RF(g_endpoint)(gm, gr, gval, e, non_delta, mode);
             break;
         }
         V3 a, b, c;
@@ -574,7 +650,9 @@ static V3 RF(trace_path)(const orc_render_scene* sc, const orc_render_lights* L,
                     const R pl = cosy <= 0 ? (R)0 : pa * (dist2 / cosy);
                     if (pl >= 0 && last_pdf > 0) w = RF(power_heuristic)(last_pdf, pl);
                 }
-                rad = RF(s4add)(rad, RF(s4scale)(RF(s4mul)(mask, RF(emission_of)(&m, &LM, op->d65)), w));
+                const S4 le = RF(emission_of)(&m, &LM, op->d65);
+                rad = RF(s4add)(rad, RF(s4scale)(RF(s4mul)(mask, le), w));
+                RF(g_endpoint)(gm, gr, gval, RF(s4scale)(le, w), non_delta, mode);
             }
             break;
         }
@@ -583,11 +661,46 @@ static V3 RF(trace_path)(const orc_render_scene* sc, const orc_render_lights* L,
         V3 next;
         R pdf;
         if (!RF(sample_bsdf)(&m, pspec, d, n, u, &next, &pdf)) break;
-        const S4 fcos = RF(eval_bsdf)(&m, alb, d, next, n);
+        S4 fcos = RF(eval_bsdf)(&m, alb, d, next, n);
         /* offset along the side of the next direction (render.cuh:155-160) */
         const R off = eps * RF(maxr)((R)1, RF(length)(p)) * (u[4] + (R)0.5);
         const R sgn = RF(dot)(n, next) > 0 ? (R)1 : (RF(dot)(n, next) < 0 ? (R)-1 : (R)0);
         p = RF(add)(p, RF(scale)(n, sgn * off));
+        /* guide/BSDF one-sample mixture (render.cuh:553-638) */
+        const float* ptex = NULL;
+        const V3 rvec = RF(reflectv)(d, n);
+        const float nf[3] = { (float)n.x, (float)n.y, (float)n.z };
+        if (guided) {
+            const float pf[3] = { (float)p.x, (float)p.y, (float)p.z };
+            const float rf[3] = { (float)rvec.x, (float)rvec.y, (float)rvec.z };
+            const int pi = orc_guide_find(G, orc_voxel_key(pf, rf));
+            if (pi >= 0) {
+                ptex = orc_guide_texels(G, pi);
+                const float xf[3] = { (float)next.x, (float)next.y, (float)next.z };
+                const R pb = pdf;
+                const R pp = (R)orc_probe_hemi_pdf(ptex, xf, nf, gq);
+                if (pb > (R)1e-12f && pp > (R)1e-12f) {
+                    if (u[10] < (R)alpha) {
+                        const float ug[5] = { uf[11], uf[12], uf[13], uf[14], uf[15] };
+                        float gd[3], gp;
+                        orc_probe_hemi_sample(ptex, nf, gq, ug, gd, &gp);
+                        const V3 gv = RF(v3)((R)gd[0], (R)gd[1], (R)gd[2]);
+                        if (RF(dot)(gv, n) > (R)1e-12f) {
+                            const R pbs = RF(bsdf_pdf)(&m, pspec, d, gv, n);
+                            next = gv;
+                            fcos = RF(eval_bsdf)(&m, alb, d, gv, n);
+                            pdf = ((R)1 - (R)alpha) * RF(maxr)(pbs, (R)1e-12f) + (R)alpha * (R)gp;
+                        } else {
+                            ptex = NULL;
+                        }
+                    } else {
+                        pdf = ((R)1 - (R)alpha) * pb + (R)alpha * pp;
+                    }
+                } else {
+                    ptex = NULL;
+                }
+            }
+        }
         int did_nee = 0;
         if (op->nee && L->count > 0 && depth < op->max_depth - 1 && non_delta < 10 &&
             u[5] < SQRT(RF(clampr)(RF(lum4)(mask, mode), 0, 1))) {
@@ -623,7 +736,11 @@ static V3 RF(trace_path)(const orc_render_scene* sc, const orc_render_lights* L,
                     const R g = cosy / dist2;
                     const S4 contrib = RF(s4scale)(RF(s4mul)(f, RF(emission_of)(&lm, &LM, op->d65)), g / pa_total);
                     const R pl = pa_total * (dist2 / cosy);
-                    const R pb = RF(maxr)(RF(bsdf_pdf)(&m, pspec, d, ldir, n), (R)0);
+                    R pb = RF(maxr)(RF(bsdf_pdf)(&m, pspec, d, ldir, n), (R)0);
+                    if (ptex) {
+                        const float lf[3] = { (float)ldir.x, (float)ldir.y, (float)ldir.z };
+                        pb = ((R)1 - (R)alpha) * pb + (R)alpha * (R)orc_probe_hemi_pdf(ptex, lf, nf, gq);
+                    }
                     const R w = RF(power_heuristic)(pl, pb);
                     rad = RF(s4add)(rad, RF(s4mul)(mask, RF(s4scale)(contrib, w)));
                     did_nee = 1;
@@ -640,10 +757,40 @@ static V3 RF(trace_path)(const orc_render_scene* sc, const orc_render_lights* L,
             qrr = RF(clampr)(RF(lum4)(mask, mode) * (R)0.5 + RF(max4)(mask, mode) * (R)0.5, (R)0.05, (R)0.95);
             if (u[9] > qrr) break;
         }
+        if (gout) {
+            if (non_delta < 5 && RF(s4zero)(gm[non_delta], mode)) {
+                const float pf[3] = { (float)p.x, (float)p.y, (float)p.z };
+                const float rf[3] = { (float)rvec.x, (float)rvec.y, (float)rvec.z };
+                gdir[non_delta] = next;
+                gkey[non_delta] = orc_voxel_key(pf, rf);
+                gm[non_delta] = RF(s4)(1, 1, 1, 1);
+            }
+            const R invq = (R)(1.0 / (double)qrr);
+            const int hi = non_delta + 1 < 5 ? non_delta + 1 : 5;
+            for (int i = 0; i < hi; ++i)
+                if (!RF(s4zero)(gm[i], mode)) gm[i] = RF(s4mul)(gm[i], RF(s4scale)(contrib, invq));
+        }
         mask = RF(s4scale)(RF(s4mul)(mask, contrib), (R)1 / qrr);
+        RF(probe_query)(op, &LM, mask, gq);
         o = p;
         d = next;
         non_delta++;
+    }
+    /* guiding samples of this path (render.cuh:909-964) */
+    if (gout) {
+        int nw = 0;
+        for (int i = 0; i < 5; ++i) {
+            if (!gval[i]) continue;
+            V3 x = RF(slot_xyz)(op, &LM, gr[i]);
+            R strength = RF(max3)(x);
+            if (mode != 0) strength = RF(minr)(strength, (R)1.0e5f);
+            if (strength < (R)1.0e-5f) continue;
+            RF(gsample)* g = &gout[nw++];
+            g->key = gkey[i];
+            g->dir[0] = (float)gdir[i].x; g->dir[1] = (float)gdir[i].y; g->dir[2] = (float)gdir[i].z;
+            g->rad[0] = (float)RF(maxr)(x.x, 0); g->rad[1] = (float)RF(maxr)(x.y, 0); g->rad[2] = (float)RF(maxr)(x.z, 0);
+        }
+        *ngout = nw;
     }
     /* pixel value (render.cuh:966-997) */
     if (mode == 0) {
@@ -684,40 +831,81 @@ static V3 RF(camera_dir)(const orc_render_opts* op, uint32_t i, uint32_t j, R u0
     return RF(normalize)(RF(add)(RF(add)(fwd, RF(scale)(right, ndx * thx)), RF(scale)(up, ndy * thy)));
 }
 
+/* one camera sample of pixel px (RenderCamera, core/scene.cu:135-189) */
+static V3 RF(sample_pixel)(const orc_render_scene* sc, const orc_render_lights* L, const orc_render_opts* op,
+                           long long px, uint32_t smp, const orc_guide* G, float alpha, RF(gsample)* gout,
+                           int* ngout)
+{
+    const uint32_t i = (uint32_t)(px % op->width), j = (uint32_t)(px / op->width);
+    float uf[8];
+    orc_philox_uniforms(op->seed, (uint32_t)px, smp, ORC_CAMERA_BOUNCE, 1, uf);
+    const V3 d = RF(camera_dir)(op, i, j, (R)uf[0], (R)uf[1]);
+    R l0 = 0;
+    if (op->spectral != 0) {
+        float w[4];
+        orc_philox_uniforms(op->seed, (uint32_t)px, smp, ORC_CAMERA_BOUNCE, 2, uf);
+        for (int k = 0; k < 4; ++k) w[k] = uf[4 + k];
+        l0 = op->spectral == 1 ? RF(sample_lambda)((R)w[0], (R)w[1], (R)w[2]) : (R)380 + (R)400 * (R)w[0];
+    }
+    return RF(trace_path)(sc, L, op, (uint32_t)px, smp, RF(load3)(op->cam_pos), d, l0, G, alpha, gout, ngout);
+}
+
+static void RF(finish_pixel)(const orc_render_opts* op, V3 acc, R* o)
+{
+    R x = acc.x / (R)op->spp, y = acc.y / (R)op->spp, z = acc.z / (R)op->spp;
+    if (op->spectral != 0 && op->output == 1) {
+        /* XYZ2SRGBLinearD65 (core/tonemap.cuh:116-126) */
+        const R r = (R)3.2406 * x + (R)-1.5372 * y + (R)-0.4986 * z;
+        const R g = (R)-0.9689 * x + (R)1.8758 * y + (R)0.0415 * z;
+        const R b = (R)0.0557 * x + (R)-0.204 * y + (R)1.057 * z;
+        x = RF(maxr)((R)0, r); y = RF(maxr)((R)0, g); z = RF(maxr)((R)0, b);
+    }
+    o[0] = x;
+    o[1] = y;
+    o[2] = z;
+}
+
 static void RF(render)(const orc_render_scene* sc, const orc_render_lights* L, const orc_render_opts* op,
                        R* out, int threads)
 {
     const long long npix = (long long)op->width * op->height;
+    if (!op->guiding || !op->guide) {
 #pragma omp parallel for schedule(dynamic, 16) num_threads(threads)
-    for (long long px = 0; px < npix; ++px) {
-        const uint32_t i = (uint32_t)(px % op->width), j = (uint32_t)(px / op->width);
-        V3 acc = RF(v3)(0, 0, 0);
-        for (uint32_t s = 0; s < (uint32_t)op->spp; ++s) {
-            float uf[8];
-            orc_philox_uniforms(op->seed, (uint32_t)px, s + op->sample_offset, ORC_CAMERA_BOUNCE, 1, uf);
-            const V3 d = RF(camera_dir)(op, i, j, (R)uf[0], (R)uf[1]);
-            R l0 = 0;
-            if (op->spectral != 0) {
-                float w[4];
-                orc_philox_uniforms(op->seed, (uint32_t)px, s + op->sample_offset, ORC_CAMERA_BOUNCE, 2, uf);
-                for (int k = 0; k < 4; ++k) w[k] = uf[4 + k];
-                l0 = op->spectral == 1 ? RF(sample_lambda)((R)w[0], (R)w[1], (R)w[2]) : (R)380 + (R)400 * (R)w[0];
-            }
-            const V3 r = RF(trace_path)(sc, L, op, (uint32_t)px, s + op->sample_offset, RF(load3)(op->cam_pos), d, l0);
-            acc = RF(add)(acc, r);
+        for (long long px = 0; px < npix; ++px) {
+            V3 acc = RF(v3)(0, 0, 0);
+            for (uint32_t s = 0; s < (uint32_t)op->spp; ++s)
+                acc = RF(add)(acc, RF(sample_pixel)(sc, L, op, px, s + op->sample_offset, NULL, 0.0f, NULL, NULL));
+            RF(finish_pixel)(op, acc, &out[3 * px]);
         }
-        R x = acc.x / (R)op->spp, y = acc.y / (R)op->spp, z = acc.z / (R)op->spp;
-        if (op->spectral != 0 && op->output == 1) {
-            /* XYZ2SRGBLinearD65 (core/tonemap.cuh:116-126) */
-            const R r = (R)3.2406 * x + (R)-1.5372 * y + (R)-0.4986 * z;
-            const R g = (R)-0.9689 * x + (R)1.8758 * y + (R)0.0415 * z;
-            const R b = (R)0.0557 * x + (R)-0.204 * y + (R)1.057 * z;
-            x = RF(maxr)((R)0, r); y = RF(maxr)((R)0, g); z = RF(maxr)((R)0, b);
-        }
-        out[3 * px + 0] = x;
-        out[3 * px + 1] = y;
-        out[3 * px + 2] = z;
+        return;
     }
+    /* voxel guiding: frames of one sample per pixel, probes updated after
+     * each frame with its samples in pixel order (SceneRenderer::Render +
+     * UpdatePathGuidingSample, core/scene.cu:249-270, 409-570) */
+    V3* acc = malloc(sizeof(V3) * (size_t)npix);
+    RF(gsample)* gs = malloc(sizeof(RF(gsample)) * 5 * (size_t)npix);
+    int* gn = malloc(sizeof(int) * (size_t)npix);
+    uint64_t* keys = malloc(sizeof(uint64_t) * 5 * (size_t)npix);
+    float* dirs = malloc(sizeof(float) * 15 * (size_t)npix);
+    float* rads = malloc(sizeof(float) * 15 * (size_t)npix);
+    for (long long px = 0; px < npix; ++px) acc[px] = RF(v3)(0, 0, 0);
+    for (uint32_t s = 0; s < (uint32_t)op->spp; ++s) {
+        const uint32_t smp = s + op->sample_offset;
+        const float alpha = orc_guide_alpha(smp);
+#pragma omp parallel for schedule(dynamic, 16) num_threads(threads)
+        for (long long px = 0; px < npix; ++px)
+            acc[px] = RF(add)(acc[px], RF(sample_pixel)(sc, L, op, px, smp, op->guide, alpha, &gs[5 * px], &gn[px]));
+        int ns = 0;
+        for (long long px = 0; px < npix; ++px)
+            for (int k = 0; k < gn[px]; ++k, ++ns) {
+                const RF(gsample)* g = &gs[5 * px + k];
+                keys[ns] = g->key;
+                for (int c = 0; c < 3; ++c) { dirs[3 * ns + c] = g->dir[c]; rads[3 * ns + c] = g->rad[c]; }
+            }
+        orc_guide_update(op->guide, ns, keys, dirs, rads);
+    }
+    for (long long px = 0; px < npix; ++px) RF(finish_pixel)(op, acc[px], &out[3 * px]);
+    free(acc); free(gs); free(gn); free(keys); free(dirs); free(rads);
 }
 
 #undef V3
