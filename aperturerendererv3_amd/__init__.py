@@ -133,6 +133,40 @@ class Context:
         self._check(self._lib.ar_bvh_build(self._h, C.byref(o)), "ar_bvh_build")
         return self.bvh_info()
 
+    # ---------------------------------------------------------------- guiding
+    def guiding_configure(self, max_probes: int):
+        self._check(self._lib.ar_guiding_configure(self._h, int(max_probes)), "ar_guiding_configure")
+
+    def guiding_reset(self):
+        self._check(self._lib.ar_guiding_reset(self._h), "ar_guiding_reset")
+
+    def guiding_info(self) -> dict:
+        info = _lib.ar_guiding_info()
+        self._check(self._lib.ar_guiding_info_get(self._h, C.byref(info)), "ar_guiding_info_get")
+        return {k: getattr(info, k) for k, _ in info._fields_}
+
+    def guiding_snapshot(self):
+        """(keys u64 (n,), texels f32 (n, 85, 4), counts u64 (n,)) on the host."""
+        import numpy as np
+        n = C.c_uint64()
+        info = self.guiding_info()
+        cap = int(info["probes"])
+        keys = np.zeros(cap, np.uint64)
+        tex = np.zeros((cap, 85, 4), np.float32)
+        cnt = np.zeros(cap, np.uint64)
+        self._check(self._lib.ar_guiding_snapshot(self._h, keys.ctypes.data, tex.ctypes.data, cnt.ctypes.data, cap,
+                                                  C.byref(n)), "ar_guiding_snapshot")
+        m = int(n.value)
+        return keys[:m], tex[:m], cnt[:m]
+
+    def guiding_restore(self, keys, texels, counts):
+        import numpy as np
+        keys = np.ascontiguousarray(keys, np.uint64)
+        texels = np.ascontiguousarray(texels, np.float32)
+        counts = np.ascontiguousarray(counts, np.uint64)
+        self._check(self._lib.ar_guiding_restore(self._h, keys.ctypes.data, texels.ctypes.data, counts.ctypes.data,
+                                                 keys.size), "ar_guiding_restore")
+
     def bvh_info(self) -> dict:
         info = ar_bvh_info()
         self._check(self._lib.ar_bvh_info_get(self._h, C.byref(info)), "ar_bvh_info_get")
@@ -206,7 +240,7 @@ class Context:
     # -- integrator ----------------------------------------------------------
     def render(self, width, height, spp, cam_pos=(0.0, 0.0, 1.1), cam_dir=(0.0, 0.0, -1.0), vfov=60.0, seed=1,
                max_depth=5, nee=True, sky=AR_SKY_NONE, env=(0.0, 0.0, 0.0), rr_min_depth=3, sample_offset=0,
-               max_paths_per_pass=0, spectral=0, output=0, out=None, stream=None):
+               max_paths_per_pass=0, spectral=0, output=0, guiding=0, out=None, stream=None):
         """Mean radiance image (height, width, 3) float32 on the GPU (ar_render)."""
         import torch
         cam = ar_camera()
@@ -218,7 +252,7 @@ class Context:
         o.seed, o.max_depth, o.rr_min_depth, o.nee, o.sky = seed, max_depth, rr_min_depth, int(bool(nee)), sky
         o.env[:] = [float(x) for x in env]
         o.max_paths_per_pass = max_paths_per_pass
-        o.spectral, o.output = spectral, output
+        o.spectral, o.output, o.guiding = spectral, output, guiding
         if out is None:
             out = torch.empty((height, width, 3), dtype=torch.float32, device=f"cuda:{self.device}")
         self._check(self._lib.ar_render(self._h, C.byref(cam), C.byref(o), _ptr(out), _stream_ptr(stream)),

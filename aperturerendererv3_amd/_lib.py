@@ -68,7 +68,12 @@ class ar_render_opts(C.Structure):
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32), ("sample_offset", C.c_uint32),
                 ("seed", C.c_uint64), ("max_depth", C.c_uint32), ("rr_min_depth", C.c_uint32), ("nee", C.c_uint32),
                 ("sky", C.c_uint32), ("env", C.c_float * 3), ("max_paths_per_pass", C.c_uint32),
-                ("spectral", C.c_uint32), ("output", C.c_uint32)]
+                ("spectral", C.c_uint32), ("output", C.c_uint32), ("guiding", C.c_uint32)]
+
+
+class ar_guiding_info(C.Structure):
+    _fields_ = [("probes", C.c_uint64), ("capacity", C.c_uint64), ("samples", C.c_uint64), ("dropped", C.c_uint64),
+                ("device_bytes", C.c_uint64)]
 
 
 AR_SKY_NONE = 0
@@ -79,6 +84,8 @@ AR_SPECTRAL_SINGLE = 1
 AR_SPECTRAL_HERO4 = 4
 AR_OUTPUT_XYZ = 0
 AR_OUTPUT_SRGB = 1
+AR_GUIDING_OFF = 0
+AR_GUIDING_VOXEL = 1
 
 # name -> (restype, argtypes)
 SIGNATURES = {
@@ -98,6 +105,12 @@ SIGNATURES = {
                                       C.c_void_p]),
     "ar_render": (C.c_int, [C.c_void_p, C.POINTER(ar_camera), C.POINTER(ar_render_opts), C.c_void_p, C.c_void_p]),
     "ar_spectrum_lut_build": (C.c_int, [C.c_void_p, C.c_size_t]),
+    "ar_guiding_configure": (C.c_int, [C.c_void_p, C.c_uint64]),
+    "ar_guiding_reset": (C.c_int, [C.c_void_p]),
+    "ar_guiding_info_get": (C.c_int, [C.c_void_p, C.POINTER(ar_guiding_info)]),
+    "ar_guiding_snapshot": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64,
+                                      C.POINTER(C.c_uint64)]),
+    "ar_guiding_restore": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]),
     "ar_spectrum_uplift": (C.c_int, [C.c_void_p, C.c_size_t, C.c_void_p]),
 }
 

@@ -198,6 +198,7 @@ int ar_destroy(ar_ctx* ctx) {
     free_bvh(ctx);
     free_scene(ctx);
     free_render_buffers(ctx);
+    guide_free(ctx);
     if (ctx->spill) (void)hipFree(ctx->spill);
     if (ctx->fetch) (void)hipFree(ctx->fetch);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -257,6 +258,12 @@ int ar_scene_set(ar_ctx* ctx, const ar_mesh* meshes, size_t nmesh) {
     (void)hipSetDevice(ctx->device);
     free_bvh(ctx);
     free_scene(ctx);
+    // probes describe the previous scene's light field
+    if (ctx->guide.cap) {
+        (void)hipDeviceSynchronize();
+        const int rc = guide_clear(ctx, nullptr);
+        if (rc != AR_OK) return rc;
+    }
     return AR_OK;
 }
 

@@ -177,6 +177,36 @@ struct RenderBuffers {
     float* d65;
 };
 
+// Voxel-guiding state: probe pool, key -> probe hash table, this frame's
+// accumulators, the collected samples of a pass and the per-path slots.
+struct GuideDev {
+    uint32_t cap = 0;               // probe capacity
+    uint32_t tcap = 0;              // hash table slots (power of two >= 2 cap)
+    uint64_t* tkey = nullptr;       // table keys (all ones = empty)
+    uint32_t* tval = nullptr;       // probe index per table slot (all ones = none)
+    uint64_t* pkey = nullptr;       // key of each probe
+    float4* tex = nullptr;          // 86 texels per probe
+    uint64_t* cnt = nullptr;        // merged sample count per probe (capped at 1e6)
+    float* acc = nullptr;           // this frame's sums, [probe][channel][leaf]
+    uint32_t* accn = nullptr;       // this frame's sample count per probe
+    uint32_t* touched = nullptr;    // probes with samples this frame
+    uint32_t* newslots = nullptr;   // table slots inserted this frame
+    uint32_t* counters = nullptr;   // see guiding.hip
+    size_t bytes = 0;
+    // samples of one pass
+    size_t scap = 0;
+    uint32_t* s_slot = nullptr;
+    uint32_t* s_leaf = nullptr;
+    float* s_rad = nullptr;         // [3][scap]
+    // per-path collection slots
+    size_t pcap = 0;
+    float* gm = nullptr;            // slot throughput [5*4][pcap]
+    float* gr = nullptr;            // slot radiance [5*4][pcap]
+    uint64_t* gkey = nullptr;       // slot probe key [5][pcap]
+    uint32_t* gleaf = nullptr;      // slot leaf texels, 6 bits each
+    uint32_t* gflag = nullptr;      // bits 0..4: slot received radiance
+};
+
 } // namespace ar
 
 struct ar_ctx {
@@ -205,6 +235,9 @@ struct ar_ctx {
     uint32_t spill_stack = 0;          // per-thread stack entries the spill buffer covers
     uint32_t* fetch = nullptr;         // ray-refill counter of the traversal kernels
     uint32_t trace_threads = 0;       // persistent grid size in threads
+
+    ar::GuideDev guide;
+    uint64_t guide_cap_req = 1u << 20;
 };
 
 namespace ar {
@@ -240,4 +273,13 @@ void free_scene(ar_ctx* ctx);
 void free_render_buffers(ar_ctx* ctx);
 int render(ar_ctx* ctx, const ar_camera* cam, const ar_render_opts* opts, float* out, hipStream_t s);
 int trace_prologue(ar_ctx* ctx, TraceArgs& ta, size_t n, uint32_t& blocks);
+
+// guiding.hip
+int guide_ensure(ar_ctx* ctx, size_t paths);
+void guide_free(ar_ctx* ctx);
+int guide_clear(ar_ctx* ctx, hipStream_t s);
+// merges the samples collected by the paths of one pass (spectral mode and
+// first wavelengths give the XYZ weights)
+int guide_update(ar_ctx* ctx, uint32_t spectral, const float* lambda0, const float* cie, size_t paths,
+                 hipStream_t s);
 } // namespace ar

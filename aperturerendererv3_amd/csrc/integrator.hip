@@ -18,6 +18,8 @@
 // transcendental (sinf/cosf) differs in its last bit.
 #include "ar_internal.hpp"
 #include "spectral.hpp"
+#include "path_common.hpp"
+#include "guide.hpp"
 
 #include <cfloat>
 #include <cmath>
@@ -58,18 +60,6 @@ __device__ __forceinline__ F3 xyz(float4 v) { return {v.x, v.y, v.z}; }
 
 constexpr float kPi = 3.14159265358979323846f;
 
-// Per-path spectral quantity: RGB in channels 0..2 (RGB mode), one wavelength
-// in channel 0 (single-wavelength mode) or four hero wavelengths.
-struct S4 { float v[4]; };
-__device__ __forceinline__ S4 s4(float a, float b, float c, float d) { return {{a, b, c, d}}; }
-__device__ __forceinline__ S4 s4mul(const S4& a, const S4& b) {
-    return {{a.v[0] * b.v[0], a.v[1] * b.v[1], a.v[2] * b.v[2], a.v[3] * b.v[3]}};
-}
-__device__ __forceinline__ S4 s4add(const S4& a, const S4& b) {
-    return {{a.v[0] + b.v[0], a.v[1] + b.v[1], a.v[2] + b.v[2], a.v[3] + b.v[3]}};
-}
-__device__ __forceinline__ S4 s4scale(const S4& a, float s) { return {{a.v[0] * s, a.v[1] * s, a.v[2] * s, a.v[3] * s}}; }
-
 // Philox4x32-10 (Salmon et al., SC'11)
 __device__ __forceinline__ void philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1,
                                        uint32_t out[4]) {
@@ -91,18 +81,6 @@ __device__ __forceinline__ void uniforms(uint64_t seed, uint32_t pix, uint32_t s
     philox(pix, smp, bounce, group, (uint32_t)seed, (uint32_t)(seed >> 32), x);
 #pragma unroll
     for (int k = 0; k < 4; ++k) u[k] = (float)(x[k] >> 8) * 0x1p-24f;
-}
-
-// Appends `pred` lanes to a queue with one atomic per wave; returns the slot.
-__device__ __forceinline__ uint32_t wave_append(bool pred, uint32_t* counter) {
-    const uint64_t m = __ballot(pred);
-    if (m == 0) return 0;
-    const uint32_t lane = __lane_id();
-    const uint32_t leader = (uint32_t)__ffsll((unsigned long long)m) - 1u;
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
-    base = __shfl(base, (int)leader);
-    return base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
 }
 
 // ---------------------------------------------------------------- sampling
@@ -220,25 +198,6 @@ __device__ __forceinline__ Mat load_mat(const GpuMaterial* __restrict__ mats, ui
     m.ca[0] = d.x; m.ca[1] = d.y; m.ca[2] = d.z;
     m.ce[0] = e.x; m.ce[1] = e.y; m.ce[2] = e.z;
     return m;
-}
-
-// Wavelengths of a path: spectral mode and the first wavelength.
-struct Lam {
-    uint32_t mode;   // AR_SPECTRAL_*
-    float l[4];
-};
-
-__device__ __forceinline__ Lam make_lam(uint32_t mode, float l0) {
-    Lam L;
-    L.mode = mode;
-    L.l[0] = l0;
-#pragma unroll
-    for (int k = 1; k < 4; ++k) {
-        float x = l0 - 380.0f + 100.0f * (float)k;
-        if (x >= 400.0f) x -= 400.0f;
-        L.l[k] = 380.0f + x;
-    }
-    return L;
 }
 
 // Reflectance of an RGB albedo in this path's channels.
@@ -399,6 +358,8 @@ struct RenderParams {
     uint32_t sky;
     uint32_t spectral;           // AR_SPECTRAL_*
     uint32_t output;             // AR_OUTPUT_*
+    uint32_t guiding;            // AR_GUIDING_*
+    float alpha;                 // guide/BSDF mixture probability of this frame
     float env[3];
     float env_c[3], env_scale;   // sigmoid coefficients of the constant sky (spectral modes)
     float cam_pos[3];
@@ -431,6 +392,98 @@ struct PathArgs {
     float* lambda0;
 };
 
+// Voxel guiding inputs of the integrator: the probes (read-only during a
+// frame) and the per-path collection slots (CollectRadiance,
+// render.cuh:345-410): throughput since the slot's vertex (gm), collected
+// radiance (gr), probe key and leaf texel of the slot's sampled direction.
+struct GuideArgs {
+    guide::Probes pr;
+    const uint32_t* probe_count;   // device counter; 0 = no probes yet
+    float* gm;                     // [5*4][stride]
+    float* gr;                     // [5*4][stride]
+    uint64_t* gkey;                // [5][stride]
+    uint32_t* gleaf;               // 6 bits per slot
+    uint32_t* gflag;               // bit i: slot i received radiance
+    uint32_t stride;
+};
+
+__device__ __forceinline__ bool s4zero(const S4& c, uint32_t mode) {
+    if (mode == AR_SPECTRAL_RGB) return c.v[0] == 0.0f && c.v[1] == 0.0f && c.v[2] == 0.0f;
+    if (mode == AR_SPECTRAL_SINGLE) return c.v[0] == 0.0f;
+    return c.v[0] == 0.0f && c.v[1] == 0.0f && c.v[2] == 0.0f && c.v[3] == 0.0f;
+}
+
+__device__ __forceinline__ S4 gload(const float* a, uint32_t stride, int i, uint32_t p) {
+    const float* b = a + (size_t)(4 * i) * stride + p;
+    return s4(b[0], b[stride], b[2 * (size_t)stride], b[3 * (size_t)stride]);
+}
+__device__ __forceinline__ void gstore(float* a, uint32_t stride, int i, uint32_t p, const S4& v) {
+    float* b = a + (size_t)(4 * i) * stride + p;
+    b[0] = v.v[0]; b[stride] = v.v[1]; b[2 * (size_t)stride] = v.v[2]; b[3 * (size_t)stride] = v.v[3];
+}
+
+// CollectRadiance with an endpoint emission: every open slot started at an
+// earlier non-delta vertex receives throughput * Le (render.cuh:364-376).
+__device__ __forceinline__ void g_endpoint(const GuideArgs& G, uint32_t p, const S4& le, uint32_t non_delta,
+                                           uint32_t mode) {
+    if (s4zero(le, mode)) return;
+    const int hi = non_delta < 5u ? (int)non_delta : 5;
+    uint32_t flags = 0;
+    for (int i = 0; i < hi; ++i) {
+        const S4 m = gload(G.gm, G.stride, i, p);
+        if (!s4zero(m, mode)) {
+            gstore(G.gr, G.stride, i, p, s4add(gload(G.gr, G.stride, i, p), s4mul(m, le)));
+            flags |= 1u << i;
+        }
+    }
+    if (flags) G.gflag[p] |= flags;
+}
+
+// CollectRadiance at a surviving vertex (render.cuh:378-409): opens the slot
+// of this non-delta depth, then scales the open slots by contrib / q.
+__device__ __forceinline__ void g_advance(const GuideArgs& G, uint32_t p, uint32_t non_delta, uint32_t mode,
+                                          uint64_t key, int leaf, const S4& contrib, float qrr) {
+    if (non_delta < 5u) {
+        const int i = (int)non_delta;
+        if (s4zero(gload(G.gm, G.stride, i, p), mode)) {
+            gstore(G.gm, G.stride, i, p, s4(1.0f, 1.0f, 1.0f, 1.0f));
+            G.gkey[(size_t)i * G.stride + p] = key;
+            G.gleaf[p] = (G.gleaf[p] & ~(63u << (6 * i))) | ((uint32_t)leaf << (6 * i));
+        }
+    }
+    const float invq = (float)(1.0 / (double)qrr);
+    const S4 c = s4scale(contrib, invq);
+    const int hi = non_delta + 1u < 5u ? (int)non_delta + 1 : 5;
+    for (int i = 0; i < hi; ++i) {
+        const S4 m = gload(G.gm, G.stride, i, p);
+        if (!s4zero(m, mode)) gstore(G.gm, G.stride, i, p, s4mul(m, c));
+    }
+}
+
+// maskForProbeQuery (render.cuh:893-900): max(w, 1e-6) / (sum + 1e-6) with w
+// the throughput (RGB) or the throughput times the CMFs of the wavelength(s).
+__device__ __forceinline__ void probe_query(const S4& mask, const Lam& L, const float* __restrict__ cie, float q[3]) {
+    float w0, w1, w2;
+    if (L.mode == AR_SPECTRAL_RGB) {
+        w0 = mask.v[0]; w1 = mask.v[1]; w2 = mask.v[2];
+    } else if (L.mode == AR_SPECTRAL_SINGLE) {
+        float cx, cy, cz;
+        spec::cie_xyz(cie, L.l[0], cx, cy, cz);
+        w0 = mask.v[0] * cx; w1 = mask.v[0] * cy; w2 = mask.v[0] * cz;
+    } else {
+        w0 = w1 = w2 = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            float cx, cy, cz;
+            spec::cie_xyz(cie, L.l[k], cx, cy, cz);
+            w0 = w0 + mask.v[k] * cx; w1 = w1 + mask.v[k] * cy; w2 = w2 + mask.v[k] * cz;
+        }
+    }
+    q[0] = fmaxf(w0, 1e-6f); q[1] = fmaxf(w1, 1e-6f); q[2] = fmaxf(w2, 1e-6f);
+    const float sum = (float)((double)(q[0] + q[1] + q[2]) + 1e-6);
+    q[0] = q[0] / sum; q[1] = q[1] / sum; q[2] = q[2] / sum;
+}
+
 __device__ __forceinline__ S4 load4(const float* a, const float* b, const float* c, const float* d, uint32_t p) {
     return s4(a[p], b[p], c[p], d[p]);
 }
@@ -440,8 +493,9 @@ __device__ __forceinline__ S4 load4(const float* a, const float* b, const float*
 // RenderCamera's ray generation (core/scene.cu:135-164) for every path slot
 // of the pass (slot p = s_local * npix + pixel), plus the path's first
 // wavelength in spectral modes (render.cuh:466-474).
-__global__ __launch_bounds__(kBlock) void k_camera(RenderParams P, PathArgs S, QueueRays q, uint32_t* __restrict__ qpath,
-                                                   uint32_t* __restrict__ counters, const float* __restrict__ eps_table) {
+__global__ __launch_bounds__(kBlock) void k_camera(RenderParams P, PathArgs S, GuideArgs G, QueueRays q,
+                                                   uint32_t* __restrict__ qpath, uint32_t* __restrict__ counters,
+                                                   const float* __restrict__ eps_table) {
     const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
     if (p == 0) { counters[0] = P.capacity; counters[1] = 0; counters[2] = 0; }
     if (p >= P.capacity) return;
@@ -474,6 +528,15 @@ __global__ __launch_bounds__(kBlock) void k_camera(RenderParams P, PathArgs S, Q
     S.lp0[p] = 0.0f; S.lp1[p] = 0.0f; S.lp2[p] = 0.0f;
     S.meta[p] = 0u;
     S.lambda0[p] = l0;
+    if (P.guiding) {
+        for (int i = 0; i < guide::kSlots; ++i) {
+            gstore(G.gm, G.stride, i, p, s4(0, 0, 0, 0));
+            gstore(G.gr, G.stride, i, p, s4(0, 0, 0, 0));
+            G.gkey[(size_t)i * G.stride + p] = guide::kNoKey;
+        }
+        G.gleaf[p] = 0u;
+        G.gflag[p] = 0u;
+    }
 }
 
 // One bounce of CalculateRadiance for every queued path: emission with MIS
@@ -482,7 +545,7 @@ __global__ __launch_bounds__(kBlock) void k_camera(RenderParams P, PathArgs S, Q
 // the shadow segment is queued and resolved after the any-hit pass), Russian
 // roulette (render.cuh:838-859) and the next ray.
 __global__ __launch_bounds__(kBlock) void k_shade(
-    RenderParams P, uint32_t depth, SceneArgs sc, PathArgs S,
+    RenderParams P, uint32_t depth, SceneArgs sc, PathArgs S, GuideArgs G,
     QueueRays cur, const uint32_t* __restrict__ qcur,
     const uint32_t* __restrict__ hprim, const float* __restrict__ ht, const float* __restrict__ hu,
     const float* __restrict__ hv,
@@ -514,17 +577,21 @@ __global__ __launch_bounds__(kBlock) void k_shade(
         const Lam L = make_lam(P.spectral, S.lambda0[p]);
         S4 mask = load4(S.mask0, S.mask1, S.mask2, S.mask3, p);
         S4 rad = load4(S.rad0, S.rad1, S.rad2, S.rad3, p);
+        const bool guided = P.guiding && *G.probe_count > 0u;
+        float gq[3] = {1.0f, 1.0f, 1.0f};
+        if (P.guiding && depth > 0) probe_query(mask, L, sc.cie, gq);
         const F3 o = f3(cur.ox[q], cur.oy[q], cur.oz[q]);
         const F3 d = f3(cur.dx[q], cur.dy[q], cur.dz[q]);
         const float eps = eps_table[non_delta < 255 ? non_delta : 255];
         const uint32_t prim = hprim[q];
         uint32_t new_meta = non_delta;
         if (prim == AR_MISS) {
+            S4 e = s4(0, 0, 0, 0);
             if (P.sky == AR_SKY_GRADIENT) {
-                const F3 e = sky_gradient(d);
-                rad = s4add(rad, s4mul(mask, s4(e.x, e.y, e.z, 0.0f)));
+                const F3 eg = sky_gradient(d);
+                e = s4(eg.x, eg.y, eg.z, 0.0f);
+                rad = s4add(rad, s4mul(mask, e));
             } else if (P.sky == AR_SKY_CONSTANT) {
-                S4 e;
                 if (P.spectral == AR_SPECTRAL_RGB) {
                     e = s4(P.env[0], P.env[1], P.env[2], 0.0f);
                 } else {
@@ -537,6 +604,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(
                 }
                 rad = s4add(rad, s4mul(mask, e));
             }
+            if (P.guiding) g_endpoint(G, p, e, non_delta, P.spectral);
         } else {
             const float4 a4 = sc.prims[3 * (size_t)prim + 0];
             const float4 b4 = sc.prims[3 * (size_t)prim + 1];
@@ -565,7 +633,9 @@ __global__ __launch_bounds__(kBlock) void k_shade(
                         const float last_pdf = S.last_pdf[p];
                         if (pl >= 0.0f && last_pdf > 0.0f) w = power_heuristic(last_pdf, pl);
                     }
-                    rad = s4add(rad, s4scale(s4mul(mask, emission_of(m, L, sc.d65)), w));
+                    const S4 le = emission_of(m, L, sc.d65);
+                    rad = s4add(rad, s4scale(s4mul(mask, le), w));
+                    if (P.guiding) g_endpoint(G, p, s4scale(le, w), non_delta, P.spectral);
                 }
             } else {
                 const S4 alb = albedo_of(m, L);
@@ -573,11 +643,48 @@ __global__ __launch_bounds__(kBlock) void k_shade(
                 F3 next;
                 float pdf;
                 if (sample_bsdf(m, pspec, d, nn, u, next, pdf)) {
-                    const S4 fcos = eval_bsdf(m, alb, d, next, nn);
+                    S4 fcos = eval_bsdf(m, alb, d, next, nn);
                     const float off = eps * maxr(1.0f, length(pos)) * (u[4] + 0.5f);
                     const float dn = dot(nn, next);
                     const float sgn = dn > 0.0f ? 1.0f : (dn < 0.0f ? -1.0f : 0.0f);
                     pos = add(pos, scale(nn, sgn * off));
+                    // guide/BSDF one-sample mixture (render.cuh:553-638)
+                    const float4* ptex = nullptr;
+                    uint64_t gkey = guide::kNoKey;
+                    if (P.guiding) {
+                        const F3 rv = sub(d, scale(nn, 2.0f * dot(d, nn)));
+                        gkey = guide::voxel_key(pos.x, pos.y, pos.z, rv.x, rv.y, rv.z);
+                    }
+                    if (guided) {
+                        const uint32_t pi = guide::find(G.pr, gkey);
+                        if (pi != guide::kNoProbe) {
+                            ptex = G.pr.tex + (size_t)pi * guide::kTexels;
+                            const float pb = pdf;
+                            const float pp = guide::hemi_pdf(ptex, next.x, next.y, next.z, nn.x, nn.y, nn.z, gq);
+                            if (pb > 1e-12f && pp > 1e-12f) {
+                                if (u[10] < P.alpha) {
+                                    float ug[5];
+                                    ug[0] = u[11];
+                                    uniforms(P.seed, pix, smp, depth, 3, ug + 1);
+                                    float gx, gy, gz, gp;
+                                    guide::hemi_sample(ptex, nn.x, nn.y, nn.z, gq, ug, gx, gy, gz, gp);
+                                    const F3 gv = f3(gx, gy, gz);
+                                    if (dot(gv, nn) > 1e-12f) {
+                                        const float pbs = bsdf_pdf(m, pspec, d, gv, nn);
+                                        next = gv;
+                                        fcos = eval_bsdf(m, alb, d, gv, nn);
+                                        pdf = (1.0f - P.alpha) * maxr(pbs, 1e-12f) + P.alpha * gp;
+                                    } else {
+                                        ptex = nullptr;
+                                    }
+                                } else {
+                                    pdf = (1.0f - P.alpha) * pb + P.alpha * pp;
+                                }
+                            } else {
+                                ptex = nullptr;
+                            }
+                        }
+                    }
                     bool nee_now = false;
                     if (P.nee && sc.nlights > 0 && depth + 1 < P.max_depth && non_delta < 10 &&
                         u[5] < sqrtf(clampr(lum4(mask, P.spectral), 0.0f, 1.0f))) {
@@ -609,7 +716,10 @@ __global__ __launch_bounds__(kBlock) void k_shade(
                             const float g = cosy / dist2;
                             const S4 contrib = s4scale(s4mul(f, emission_of(lm, L, sc.d65)), g / pa_total);
                             const float pl = pa_total * (dist2 / cosy);
-                            const float pb = maxr(bsdf_pdf(m, pspec, d, ldir, nn), 0.0f);
+                            float pb = maxr(bsdf_pdf(m, pspec, d, ldir, nn), 0.0f);
+                            if (ptex)
+                                pb = (1.0f - P.alpha) * pb +
+                                     P.alpha * guide::hemi_pdf(ptex, ldir.x, ldir.y, ldir.z, nn.x, nn.y, nn.z, gq);
                             const float w = power_heuristic(pl, pb);
                             contrib_w = s4mul(mask, s4scale(contrib, w));
                         }
@@ -643,6 +753,9 @@ __global__ __launch_bounds__(kBlock) void k_shade(
                         if (u[9] > qrr) alive = false;
                     }
                     if (alive && depth + 1 < P.max_depth) {
+                        if (P.guiding)
+                            g_advance(G, p, non_delta, P.spectral, gkey, guide::leaf_of(next.x, next.y, next.z),
+                                      contrib, qrr);
                         mask = s4scale(s4mul(mask, contrib), 1.0f / qrr);
                         cont = true;
                         next_o = pos;
@@ -960,10 +1073,20 @@ int render(ar_ctx* ctx, const ar_camera* cam, const ar_render_opts* opts, float*
     const size_t max_paths = opts->max_paths_per_pass ? opts->max_paths_per_pass : ((size_t)1 << 22);
     if (npix > max_paths && npix > ((size_t)1 << 26))
         return set_error(ctx, AR_E_CAPACITY, "image larger than 2^26 pixels");
-    const uint32_t spp_pass = (uint32_t)std::max<size_t>(1, std::min<size_t>(opts->spp, max_paths / std::max<size_t>(npix, 1)));
+    if (opts->guiding != AR_GUIDING_OFF && opts->guiding != AR_GUIDING_VOXEL)
+        return set_error(ctx, AR_E_INVALID, "guiding must be AR_GUIDING_OFF or AR_GUIDING_VOXEL");
+    const bool guiding = opts->guiding == AR_GUIDING_VOXEL;
+    // guiding renders frames of one sample per pixel with a probe update
+    // after each (the reference's Render + UpdatePathGuidingSample loop)
+    const uint32_t spp_pass = guiding ? 1u
+        : (uint32_t)std::max<size_t>(1, std::min<size_t>(opts->spp, max_paths / std::max<size_t>(npix, 1)));
     const size_t cap = npix * spp_pass;
     int rc = ensure_render_buffers(ctx, cap, npix);
     if (rc != AR_OK) return rc;
+    if (guiding) {
+        rc = guide_ensure(ctx, cap);
+        if (rc != AR_OK) return rc;
+    }
     TraceArgs ta;
     uint32_t tblocks;
     rc = trace_prologue(ctx, ta, cap, tblocks);
@@ -981,6 +1104,8 @@ int render(ar_ctx* ctx, const ar_camera* cam, const ar_render_opts* opts, float*
     P.seed = opts->seed;
     P.spectral = opts->spectral;
     P.output = opts->output;
+    P.guiding = guiding ? 1u : 0u;
+    P.alpha = 0.0f;
     {
         float e[3] = {opts->env[0], opts->env[1], opts->env[2]};
         const float mx = std::max(e[0], std::max(e[1], e[2]));
@@ -1028,19 +1153,32 @@ int render(ar_ctx* ctx, const ar_camera* cam, const ar_render_opts* opts, float*
                       {b.rays[1][0], b.rays[1][1], b.rays[1][2], b.rays[1][3], b.rays[1][4], b.rays[1][5], b.rays[1][6], b.rays[1][7]}};
     QueueRays shq{b.srays[0], b.srays[1], b.srays[2], b.srays[3], b.srays[4], b.srays[5], b.srays[6], b.srays[7]};
     ar_hits hits{b.hit_prim, b.hit_t, b.hit_u, b.hit_v, b.hit_flags};
+    GuideDev& gd = ctx->guide;
+    GuideArgs G{};
+    if (guiding) {
+        G.pr = guide::Probes{gd.tkey, gd.tval, gd.tcap - 1, gd.tex, 0u};
+        G.probe_count = gd.counters;
+        G.gm = gd.gm; G.gr = gd.gr; G.gkey = gd.gkey; G.gleaf = gd.gleaf; G.gflag = gd.gflag;
+        G.stride = (uint32_t)gd.pcap;
+    }
     const float spp_total = (float)opts->spp;
     for (uint32_t done = 0; done < opts->spp;) {
         const uint32_t this_spp = std::min(spp_pass, opts->spp - done);
         P.sample_base = opts->sample_offset + done;
         P.capacity = (uint32_t)(npix * this_spp);
-        hipLaunchKernelGGL(k_camera, dim3(grid_for(P.capacity)), dim3(kBlock), 0, stream, P, S, q[0], b.qpath[0],
+        if (guiding) {
+            // GetPathGuidingProb (render.cuh:195-200): lerp(0, 0.5, min(1, frame/64)^0.5)
+            const float t = std::pow(std::fmin(1.0f, (float)P.sample_base / 64.0f), 0.5f);
+            P.alpha = std::fmin(std::fmax(0.0f * (1.0f - t) + 0.5f * t, 0.0f), 1.0f);
+        }
+        hipLaunchKernelGGL(k_camera, dim3(grid_for(P.capacity)), dim3(kBlock), 0, stream, P, S, G, q[0], b.qpath[0],
                            b.counters, b.eps_table);
         int c = 0;
         for (uint32_t depth = 0; depth < opts->max_depth; ++depth) {
             ar_rays rr{q[c].ox, q[c].oy, q[c].oz, q[c].tmin, q[c].dx, q[c].dy, q[c].dz, q[c].tmax};
             hipError_t e = launch_trace_closest(ta, rr, P.capacity, hits, nullptr, nullptr, tblocks, stream, &b.counters[0]);
             if (e != hipSuccess) return check_hip(ctx, e, "integrator trace");
-            hipLaunchKernelGGL(k_shade, dim3(grid_for(P.capacity)), dim3(kBlock), 0, stream, P, depth, sc, S, q[c],
+            hipLaunchKernelGGL(k_shade, dim3(grid_for(P.capacity)), dim3(kBlock), 0, stream, P, depth, sc, S, G, q[c],
                                b.qpath[c], b.hit_prim, b.hit_t, b.hit_u, b.hit_v, q[1 - c], b.qpath[1 - c], shq,
                                b.spath, b.scontrib[0], b.scontrib[1], b.scontrib[2], b.scontrib[3], b.sflags,
                                b.counters, b.eps_table);
@@ -1054,6 +1192,10 @@ int render(ar_ctx* ctx, const ar_camera* cam, const ar_render_opts* opts, float*
             }
             hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, stream, b.counters);
             c = 1 - c;
+        }
+        if (guiding) {
+            rc = guide_update(ctx, P.spectral, b.lambda0, b.cie, P.capacity, stream);
+            if (rc != AR_OK) return rc;
         }
         done += this_spp;
         hipLaunchKernelGGL(k_accumulate, dim3(grid_for(npix)), dim3(kBlock), 0, stream, P, sc, S, b.accum, this_spp,

@@ -200,7 +200,19 @@ typedef struct {
     uint32_t max_paths_per_pass;  /* 0: 4,194,304 */
     uint32_t spectral;            /* AR_SPECTRAL_* */
     uint32_t output;              /* AR_OUTPUT_* (spectral modes) */
+    uint32_t guiding;             /* AR_GUIDING_* */
 } ar_render_opts;
+
+/* Path guiding.  VOXEL: the reference's voxel guiding (core/guiding,
+ * core/render/render.cuh:345-458, 553-638, 909-964; core/scene.cu:368-570):
+ * the render runs frames of one sample per pixel; after each frame the paths'
+ * collected radiance (up to 5 non-delta vertices each) is merged into
+ * directional probes, and later frames sample the probe of each vertex with
+ * probability 0.5 * min(1, frame/64)^0.5 in a one-sample mixture with the
+ * BSDF (frame = sample index).  Probes persist in the context across calls
+ * until ar_guiding_reset. */
+#define AR_GUIDING_OFF 0u
+#define AR_GUIDING_VOXEL 1u
 
 /* Spectral modes.  RGB: three colour channels throughout (the reference with
  * USE_SPECTRUM_RENDERING off).  SINGLE: one wavelength per path in
@@ -224,6 +236,39 @@ typedef struct {
  * receives the mean over spp of clamp(checkNanInf(L), 0, 1e3) per channel.
  * Queued on `stream`; the call does not wait. */
 int ar_render(ar_ctx* ctx, const ar_camera* cam, const ar_render_opts* opts, float* out_rgb, void* stream);
+
+/* --- voxel guiding state ------------------------------------------------
+ * Probes are keyed by (voxel = floor(p / 0.05), icosahedral face of the
+ * reflected direction) and hold an 8x8 equal-area octahedral texture of
+ * float4 (XYZ in spectral modes, RGB otherwise; w unused) with 4x4, 2x2, 1x1
+ * mips: 85 texels.  A key packs the voxel as three 19-bit offsets (+2^18) and
+ * the face in bits 57..61. */
+typedef struct {
+    uint64_t probes;          /* probes allocated */
+    uint64_t capacity;        /* probe capacity */
+    uint64_t samples;         /* samples merged by the last frame */
+    uint64_t dropped;         /* samples dropped since the last reset because the capacity was reached */
+    uint64_t device_bytes;
+} ar_guiding_info;
+
+/* Probe capacity (default 1,048,576; about 2.2 KB of device memory per
+ * probe).  Takes effect on the next guided render; discards the probes. */
+int ar_guiding_configure(ar_ctx* ctx, uint64_t max_probes);
+
+/* Drops every probe; replaces SceneRenderer::MarkNeedResetGuiding /
+ * ResetPathGuidingSample (core/scene.hpp:114-116, core/scene.cu:368-407). */
+int ar_guiding_reset(ar_ctx* ctx);
+
+int ar_guiding_info_get(ar_ctx* ctx, ar_guiding_info* out);
+
+/* Copies the probes to host memory in allocation order: keys (n), texels
+ * (n x 85 x 4 floats), sample counts (n).  cap: room in the arrays (probes);
+ * *n receives the probe count (AR_E_CAPACITY if it exceeds cap).  Any array
+ * may be NULL.  Synchronises with the context's work. */
+int ar_guiding_snapshot(ar_ctx* ctx, uint64_t* keys, float* texels, uint64_t* counts, uint64_t cap, uint64_t* n);
+
+/* Replaces the probes with n probes in the snapshot layout (resume). */
+int ar_guiding_restore(ar_ctx* ctx, const uint64_t* keys, const float* texels, const uint64_t* counts, uint64_t n);
 
 /* --- spectral uplift table --------------------------------------------------
  * Replaces PrecomputeRGBLUTKernel / SetupSpectrumLUT (core/spectrum/
