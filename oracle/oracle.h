@@ -189,6 +189,17 @@ void orc_guide_update(orc_guide* g, int ns, const uint64_t* keys, const float* d
 /* guiding mixture probability of frame f: 0.5 * (min(1, f/64))^0.5 */
 float orc_guide_alpha(uint32_t frame);
 
+/* ---------------- learned guiding distribution (mlp.c) ---------------- */
+
+/* parameters: W0 16x64, b0 64, W1 64x64, b1 64, W2 64x64, b2 64, W3 64x48, b3 48 (W[in][out]) */
+int orc_mlp_param_count(void);
+void orc_mlp_forward(const double* params, const float* x5, double* out40);
+double orc_vmf_mixture_pdf(const double* out40, const double* w);
+void orc_vmf_mixture_sample(const double* out40, const double* u3, double* w);
+/* mean over n samples of -weight * log q(dir | x); grad (param count) or NULL */
+double orc_mlp_loss_grad(const double* params, int n, const float* x5, const float* dir, const float* weight,
+                         double* grad);
+
 typedef struct {
     uint32_t width, height, spp, sample_offset;
     uint64_t seed;
