@@ -133,6 +133,49 @@ class Context:
         self._check(self._lib.ar_bvh_build(self._h, C.byref(o)), "ar_bvh_build")
         return self.bvh_info()
 
+    # ---------------------------------------------------------------- learned guiding
+    def mlp_configure(self, learning_rate=5e-3, batch=1 << 18, iters_per_frame=1, seed=1):
+        o = _lib.ar_mlp_opts(float(learning_rate), int(batch), int(iters_per_frame), int(seed))
+        self._check(self._lib.ar_mlp_configure(self._h, C.byref(o)), "ar_mlp_configure")
+
+    def mlp_params(self):
+        import numpy as np
+        p = np.zeros(_lib.MLP_PARAMS, np.float32)
+        self._check(self._lib.ar_mlp_params_get(self._h, p.ctypes.data, p.size), "ar_mlp_params_get")
+        return p
+
+    def mlp_set_params(self, p):
+        import numpy as np
+        p = np.ascontiguousarray(p, np.float32)
+        self._check(self._lib.ar_mlp_params_set(self._h, p.ctypes.data, p.size), "ar_mlp_params_set")
+
+    def mlp_forward(self, x5, stream=None):
+        """Raw outputs (n, 40) of inputs x5 (n, 5), float32 device tensors."""
+        import torch
+        x5 = x5.contiguous()
+        out = torch.empty((x5.shape[0], 40), dtype=torch.float32, device=x5.device)
+        self._check(self._lib.ar_mlp_forward(self._h, _ptr(x5), x5.shape[0], _ptr(out), _stream_ptr(stream)),
+                    "ar_mlp_forward")
+        return out
+
+    def mlp_gradient(self, x5, dirs, weight, stream=None):
+        """(gradient (12720,), loss (1,)) of mean(-weight * log q(dir | x))."""
+        import torch
+        g = torch.empty(_lib.MLP_PARAMS, dtype=torch.float32, device=x5.device)
+        loss = torch.empty(1, dtype=torch.float32, device=x5.device)
+        self._check(self._lib.ar_mlp_gradient(self._h, _ptr(x5.contiguous()), _ptr(dirs.contiguous()),
+                                              _ptr(weight.contiguous()), x5.shape[0], _ptr(g), _ptr(loss),
+                                              _stream_ptr(stream)), "ar_mlp_gradient")
+        return g, loss
+
+    def mlp_train(self, x5, dirs, weight, steps, stream=None):
+        import torch
+        loss = torch.empty(1, dtype=torch.float32, device=x5.device)
+        self._check(self._lib.ar_mlp_train(self._h, _ptr(x5.contiguous()), _ptr(dirs.contiguous()),
+                                           _ptr(weight.contiguous()), x5.shape[0], int(steps), _ptr(loss),
+                                           _stream_ptr(stream)), "ar_mlp_train")
+        return loss
+
     # ---------------------------------------------------------------- guiding
     def guiding_configure(self, max_probes: int):
         self._check(self._lib.ar_guiding_configure(self._h, int(max_probes)), "ar_guiding_configure")

@@ -71,6 +71,14 @@ class ar_render_opts(C.Structure):
                 ("spectral", C.c_uint32), ("output", C.c_uint32), ("guiding", C.c_uint32)]
 
 
+class ar_mlp_opts(C.Structure):
+    _fields_ = [("learning_rate", C.c_float), ("batch", C.c_uint32), ("iters_per_frame", C.c_uint32),
+                ("seed", C.c_uint64)]
+
+
+MLP_PARAMS = 16 * 64 + 64 + 64 * 64 + 64 + 64 * 64 + 64 + 64 * 48 + 48
+
+
 class ar_guiding_info(C.Structure):
     _fields_ = [("probes", C.c_uint64), ("capacity", C.c_uint64), ("samples", C.c_uint64), ("dropped", C.c_uint64),
                 ("device_bytes", C.c_uint64)]
@@ -86,6 +94,7 @@ AR_OUTPUT_XYZ = 0
 AR_OUTPUT_SRGB = 1
 AR_GUIDING_OFF = 0
 AR_GUIDING_VOXEL = 1
+AR_GUIDING_MLP = 2
 
 # name -> (restype, argtypes)
 SIGNATURES = {
@@ -106,6 +115,14 @@ SIGNATURES = {
     "ar_render": (C.c_int, [C.c_void_p, C.POINTER(ar_camera), C.POINTER(ar_render_opts), C.c_void_p, C.c_void_p]),
     "ar_spectrum_lut_build": (C.c_int, [C.c_void_p, C.c_size_t]),
     "ar_guiding_configure": (C.c_int, [C.c_void_p, C.c_uint64]),
+    "ar_mlp_configure": (C.c_int, [C.c_void_p, C.POINTER(ar_mlp_opts)]),
+    "ar_mlp_params_get": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "ar_mlp_params_set": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "ar_mlp_forward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p]),
+    "ar_mlp_gradient": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p,
+                                  C.c_void_p, C.c_void_p]),
+    "ar_mlp_train": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_uint32,
+                               C.c_void_p, C.c_void_p]),
     "ar_guiding_reset": (C.c_int, [C.c_void_p]),
     "ar_guiding_info_get": (C.c_int, [C.c_void_p, C.POINTER(ar_guiding_info)]),
     "ar_guiding_snapshot": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64,

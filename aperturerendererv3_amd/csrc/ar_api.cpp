@@ -199,6 +199,7 @@ int ar_destroy(ar_ctx* ctx) {
     free_scene(ctx);
     free_render_buffers(ctx);
     guide_free(ctx);
+    mlp_free(ctx);
     if (ctx->spill) (void)hipFree(ctx->spill);
     if (ctx->fetch) (void)hipFree(ctx->fetch);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);

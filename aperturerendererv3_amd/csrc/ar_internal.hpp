@@ -207,6 +207,31 @@ struct GuideDev {
     uint32_t* gflag = nullptr;      // bits 0..4: slot received radiance
 };
 
+// Learned guiding network state (mlp.hip)
+struct MlpDev {
+    float* p = nullptr;         // FP32 parameters (one allocation: p, m, v, g, scal, wb, wt)
+    float* m = nullptr;         // Adam moments
+    float* v = nullptr;
+    float* g = nullptr;         // gradient
+    float* scal = nullptr;      // [0] weight scale, [1] loss
+    uint16_t* wb = nullptr;     // BF16 weights [in][out]
+    uint16_t* wt = nullptr;     // BF16 weights [out][in]
+    uint64_t t = 0;             // Adam steps
+    float lr = 5e-3f;
+    uint32_t batch = 1u << 18;  // path vertices per training step
+    uint32_t iters = 1;         // training steps per frame
+    uint64_t seed = 1;
+    // integrator buffers
+    size_t cap = 0;
+    float* feat = nullptr;      // [cap][5] network inputs of the queued vertices
+    float* dec = nullptr;       // [cap][48] decoded mixtures
+    size_t rcap = 0;
+    float* rx = nullptr;        // training records: inputs [rcap][5]
+    float* rd = nullptr;        // directions [rcap][3]
+    float* rw = nullptr;        // weights [rcap]
+    uint32_t* rcount = nullptr; // records collected this frame
+};
+
 } // namespace ar
 
 struct ar_ctx {
@@ -238,6 +263,7 @@ struct ar_ctx {
 
     ar::GuideDev guide;
     uint64_t guide_cap_req = 1u << 20;
+    ar::MlpDev mlp;
 };
 
 namespace ar {
@@ -282,4 +308,15 @@ int guide_clear(ar_ctx* ctx, hipStream_t s);
 // first wavelengths give the XYZ weights)
 int guide_update(ar_ctx* ctx, uint32_t spectral, const float* lambda0, const float* cie, size_t paths,
                  hipStream_t s);
+
+// mlp.hip
+int mlp_ensure(ar_ctx* ctx);
+int mlp_init(ar_ctx* ctx, uint64_t seed);
+int mlp_set_params(ar_ctx* ctx, const float* host);
+void mlp_free(ar_ctx* ctx);
+hipError_t mlp_forward(ar_ctx* ctx, const float* x5, size_t n, const uint32_t* n_dev, float* out, bool decoded,
+                       hipStream_t s);
+hipError_t mlp_gradient(ar_ctx* ctx, const float* x5, const float* dir, const float* w, size_t n,
+                        const uint32_t* n_dev, bool normalize, hipStream_t s);
+hipError_t mlp_adam_step(ar_ctx* ctx, hipStream_t s);
 } // namespace ar

@@ -1,0 +1,108 @@
+// Learned guiding distribution: a 5-64-64-64-40 ReLU MLP whose outputs are an
+// 8-lobe von Mises-Fisher mixture over directions (per lobe: logit, log kappa,
+// mean direction).  Inputs: the vertex position normalised to the scene box
+// and the shading normal as (theta / pi, phi / (2 pi) + 1/2).
+//
+// Parameters are FP32 masters in the layout W0 16x64, b0, W1 64x64, b1,
+// W2 64x64, b2, W3 64x48, b3 (W[in][out]; the 5 inputs and 40 outputs are
+// zero-padded to 16 and 48 for 16x16x16 MFMA tiles).  The kernels use BF16
+// copies of the weights (row-major for the backward pass, transposed for the
+// forward pass) with FP32 accumulation.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+namespace ar {
+namespace mlp {
+
+constexpr int kIn = 16, kHid = 64, kOut = 48, kUsedIn = 5, kUsedOut = 40, kLobes = 8;
+constexpr int OW0 = 0;
+constexpr int OB0 = OW0 + kIn * kHid;
+constexpr int OW1 = OB0 + kHid;
+constexpr int OB1 = OW1 + kHid * kHid;
+constexpr int OW2 = OB1 + kHid;
+constexpr int OB2 = OW2 + kHid * kHid;
+constexpr int OW3 = OB2 + kHid;
+constexpr int OB3 = OW3 + kHid * kOut;
+constexpr int kParams = OB3 + kOut;
+constexpr int kDecoded = 48;   // per vertex: 8 lobes x (pi, norm, kappa, mu xyz)
+constexpr float kTwoPi = 6.28318530717958647692f;
+
+// network inputs of a vertex
+__device__ __forceinline__ void features(float px, float py, float pz, float nx, float ny, float nz, const float lo[3],
+                                         const float inv_ext[3], float x[5]) {
+    x[0] = (px - lo[0]) * inv_ext[0];
+    x[1] = (py - lo[1]) * inv_ext[1];
+    x[2] = (pz - lo[2]) * inv_ext[2];
+    x[3] = acosf(fminf(1.0f, fmaxf(-1.0f, nz))) * 0.318309886183790672f;
+    x[4] = atan2f(ny, nx) * 0.159154943091895336f + 0.5f;
+}
+
+// decoded lobe k of a vertex: d[6k..6k+5] = (pi, pi * kappa / (2 pi (1 - e^-2kappa)), kappa, mu)
+__device__ __forceinline__ float mixture_pdf(const float* __restrict__ d, float wx, float wy, float wz) {
+    float q = 0.0f;
+#pragma unroll
+    for (int k = 0; k < kLobes; ++k) {
+        const float* l = d + 6 * k;
+        q += l[1] * expf(l[2] * (l[3] * wx + l[4] * wy + l[5] * wz - 1.0f));
+    }
+    return q;
+}
+
+__device__ __forceinline__ void mixture_sample(const float* __restrict__ d, float u0, float u1, float u2, float& wx,
+                                               float& wy, float& wz) {
+    int k = 0;
+    float c = 0.0f;
+    for (; k < kLobes - 1; ++k) {
+        c += d[6 * k];
+        if (u0 < c) break;
+    }
+    const float* l = d + 6 * k;
+    const float kap = l[2];
+    // Jakob 2012: cos = 1 + log(u + (1 - u) e^{-2 kappa}) / kappa
+    const float ct = fminf(1.0f, fmaxf(-1.0f, 1.0f + logf(u1 + (1.0f - u1) * expf(-2.0f * kap)) / kap));
+    const float st = sqrtf(fmaxf(0.0f, 1.0f - ct * ct));
+    float sp, cp;
+    sincosf(kTwoPi * u2, &sp, &cp);
+    const float mx = l[3], my = l[4], mz = l[5];
+    const float sg = mz >= 0.0f ? 1.0f : -1.0f;
+    const float a = -1.0f / (sg + mz), b = mx * my * a;
+    const float tx = 1.0f + sg * mx * mx * a, ty = sg * b, tz = -sg * mx;
+    const float bx = b, by = sg + my * my * a, bz = -my;
+    wx = st * cp * tx + st * sp * bx + ct * mx;
+    wy = st * cp * ty + st * sp * by + ct * my;
+    wz = st * cp * tz + st * sp * bz + ct * mz;
+}
+
+// hemisphere fold of the mixture, as for the voxel probes (octahedron.cuh:
+// 291-337): below-horizon samples are mirrored, pdf = q(w) + q(mirror(w)).
+__device__ __forceinline__ void mirror(float wx, float wy, float wz, float nx, float ny, float nz, float& mx, float& my,
+                                       float& mz) {
+    const float d = nx * wx + ny * wy + nz * wz;
+    mx = wx - 2.0f * d * nx;
+    my = wy - 2.0f * d * ny;
+    mz = wz - 2.0f * d * nz;
+}
+
+__device__ __forceinline__ float hemi_pdf(const float* __restrict__ d, float wx, float wy, float wz, float nx, float ny,
+                                          float nz) {
+    if (wx * nx + wy * ny + wz * nz <= 0.0f) return 0.0f;
+    float mx, my, mz;
+    mirror(wx, wy, wz, nx, ny, nz, mx, my, mz);
+    return mixture_pdf(d, wx, wy, wz) + mixture_pdf(d, mx, my, mz);
+}
+
+__device__ __forceinline__ void hemi_sample(const float* __restrict__ d, float nx, float ny, float nz, const float u[3],
+                                            float& wx, float& wy, float& wz, float& pdf) {
+    mixture_sample(d, u[0], u[1], u[2], wx, wy, wz);
+    float mx, my, mz;
+    mirror(wx, wy, wz, nx, ny, nz, mx, my, mz);
+    pdf = mixture_pdf(d, wx, wy, wz) + mixture_pdf(d, mx, my, mz);
+    if (wx * nx + wy * ny + wz * nz < 0.0f) { wx = mx; wy = my; wz = mz; }
+}
+
+} // namespace mlp
+} // namespace ar

@@ -213,6 +213,13 @@ typedef struct {
  * until ar_guiding_reset. */
 #define AR_GUIDING_OFF 0u
 #define AR_GUIDING_VOXEL 1u
+/* MLP: learned guiding (no counterpart in the reference): an MLP maps the
+ * vertex (position in the scene box, shading normal) to an 8-lobe vMF
+ * mixture over directions, sampled in the same one-sample mixture and
+ * hemisphere fold as the voxel probes.  After each frame it is trained on the
+ * frame's path vertices (collected like the voxel samples) with BF16 MFMA
+ * forward/backward and Adam; see ar_mlp_configure. */
+#define AR_GUIDING_MLP 2u
 
 /* Spectral modes.  RGB: three colour channels throughout (the reference with
  * USE_SPECTRUM_RENDERING off).  SINGLE: one wavelength per path in
@@ -269,6 +276,37 @@ int ar_guiding_snapshot(ar_ctx* ctx, uint64_t* keys, float* texels, uint64_t* co
 
 /* Replaces the probes with n probes in the snapshot layout (resume). */
 int ar_guiding_restore(ar_ctx* ctx, const uint64_t* keys, const float* texels, const uint64_t* counts, uint64_t n);
+
+/* --- learned guiding network ----------------------------------------------
+ * No counterpart in the reference (its guiding is the voxel probes).  The
+ * network: 5 inputs (position normalised to the scene box, the shading
+ * normal as theta/pi and phi/(2 pi) + 1/2), three hidden layers of 64 ReLU
+ * units, 40 outputs = 8 lobes x (logit, log kappa clamped to [-3, 7], mean
+ * direction).  Parameters (ar_mlp_params_*): FP32, W0 16x64, b0 64, W1 64x64,
+ * b1 64, W2 64x64, b2 64, W3 64x48, b3 48 (W[in][out], rows 5..15 of W0 and
+ * columns 40..47 of W3 unused and zero): 12,720 floats.  Loss of a sample
+ * (x, w, weight): -weight * log q(w | x). */
+typedef struct {
+    float learning_rate;        /* Adam step (default 5e-3) */
+    uint32_t batch;             /* path vertices per training step (default 2^18) */
+    uint32_t iters_per_frame;   /* training steps after each frame (default 1) */
+    uint64_t seed;              /* He-normal initialisation seed */
+} ar_mlp_opts;
+
+/* Sets the options and re-initialises the parameters. */
+int ar_mlp_configure(ar_ctx* ctx, const ar_mlp_opts* opts);
+int ar_mlp_params_get(ar_ctx* ctx, float* host_out, size_t count);
+int ar_mlp_params_set(ar_ctx* ctx, const float* host_in, size_t count);
+/* Raw network outputs (n x 40, device) of inputs (n x 5, device). */
+int ar_mlp_forward(ar_ctx* ctx, const float* x5, size_t n, float* out40, void* stream);
+/* Gradient (12,720 floats, device) and loss (1 float, device, may be NULL) of
+ * the mean over the n samples of -weight * log q(dir | x); dir: n x 3. */
+int ar_mlp_gradient(ar_ctx* ctx, const float* x5, const float* dir, const float* weight, size_t n, float* grad,
+                    float* loss, void* stream);
+/* `steps` Adam steps on the weighted loss normalised by the sum of weights;
+ * loss (device, may be NULL) receives the last step's loss. */
+int ar_mlp_train(ar_ctx* ctx, const float* x5, const float* dir, const float* weight, size_t n, uint32_t steps,
+                 float* loss, void* stream);
 
 /* --- spectral uplift table --------------------------------------------------
  * Replaces PrecomputeRGBLUTKernel / SetupSpectrumLUT (core/spectrum/

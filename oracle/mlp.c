@@ -15,8 +15,10 @@
  * form of the KL divergence from the sampled radiance distribution to q
  * (Mueller et al., "Neural Importance Sampling", 2019).
  *
- * Everything here is FP64 without rounding to BF16: it is the reference the
- * GPU's BF16 MFMA forward and backward are checked against.
+ * Everything here is FP64.  orc_mlp_*: no rounding, the mathematical network;
+ * orc_mlp_*_bf16: the same with values rounded to BF16 exactly where the GPU
+ * kernels store BF16, so that the GPU's MFMA data flow can be checked to FP32
+ * accumulation accuracy.
  */
 #include "oracle.h"
 
@@ -29,6 +31,23 @@
 #define NLOBE 8
 
 static const double kPi = 3.14159265358979323846;
+
+/* BF16 emulation: when set, values are rounded where the GPU kernels store
+ * BF16 (inputs, weights, post-ReLU activations, the loss gradient and the
+ * masked backward activations), through FP32 with round-to-nearest-even. */
+static int g_bf16 = 0;
+
+static double bf(double x)
+{
+    if (!g_bf16) return x;
+    float f = (float)x;
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    u &= 0xFFFF0000u;
+    memcpy(&f, &u, 4);
+    return (double)f;
+}
 
 /* parameter offsets (floats) */
 enum {
@@ -48,9 +67,10 @@ int orc_mlp_param_count(void) { return NPARAM; }
 static void dense(const double* W, const double* b, const double* x, int nin, int nout, double* y, int relu)
 {
     for (int o = 0; o < nout; ++o) {
-        double s = b[o];
-        for (int i = 0; i < nin; ++i) s += x[i] * W[i * nout + o];
-        y[o] = relu ? (s > 0 ? s : 0) : s;
+        double s = 0;
+        for (int i = 0; i < nin; ++i) s += x[i] * bf(W[i * nout + o]);
+        s += b[o];
+        y[o] = relu ? bf(s > 0 ? s : 0) : s;
     }
 }
 
@@ -65,7 +85,7 @@ static void forward(const double* P, const double* x16, double* h1, double* h2, 
 void orc_mlp_forward(const double* P, const float* x5, double* out40)
 {
     double x[NI] = { 0 }, h1[NH], h2[NH], h3[NH], o[NO];
-    for (int i = 0; i < 5; ++i) x[i] = x5[i];
+    for (int i = 0; i < 5; ++i) x[i] = bf(x5[i]);
     forward(P, x, h1, h2, h3, o);
     for (int i = 0; i < 40; ++i) out40[i] = o[i];
 }
@@ -139,7 +159,7 @@ double orc_mlp_loss_grad(const double* P, int n, const float* x5, const float* d
     double loss = 0;
     for (int s = 0; s < n; ++s) {
         double x[NI] = { 0 }, h1[NH], h2[NH], h3[NH], o[NO];
-        for (int i = 0; i < 5; ++i) x[i] = x5[5 * s + i];
+        for (int i = 0; i < 5; ++i) x[i] = bf(x5[5 * s + i]);
         forward(P, x, h1, h2, h3, o);
         lobes L;
         decode(o, &L);
@@ -167,32 +187,33 @@ double orc_mlp_loss_grad(const double* P, int n, const float* x5, const float* d
                     d_o[5 * k + 2 + i] = -wt * r * kap * (w[i] - mw * L.mu[k][i]) / L.clen[k];
         }
         /* backward */
-        double d3[NH], d2[NH], d1[NH];
+        double d3[NH], d2[NH], d1[NH], dob[NO];
+        for (int o2 = 0; o2 < NO; ++o2) dob[o2] = bf(d_o[o2]);
         for (int i = 0; i < NH; ++i) {
             double s3 = 0;
             for (int o2 = 0; o2 < NO; ++o2) {
-                s3 += P[OW3 + i * NO + o2] * d_o[o2];
-                grad[OW3 + i * NO + o2] += h3[i] * d_o[o2];
+                s3 += bf(P[OW3 + i * NO + o2]) * dob[o2];
+                grad[OW3 + i * NO + o2] += h3[i] * dob[o2];
             }
-            d3[i] = h3[i] > 0 ? s3 : 0;
+            d3[i] = bf(h3[i] > 0 ? s3 : 0);
         }
         for (int o2 = 0; o2 < NO; ++o2) grad[OB3 + o2] += d_o[o2];
         for (int i = 0; i < NH; ++i) {
             double s2 = 0;
             for (int o2 = 0; o2 < NH; ++o2) {
-                s2 += P[OW2 + i * NH + o2] * d3[o2];
+                s2 += bf(P[OW2 + i * NH + o2]) * d3[o2];
                 grad[OW2 + i * NH + o2] += h2[i] * d3[o2];
             }
-            d2[i] = h2[i] > 0 ? s2 : 0;
+            d2[i] = bf(h2[i] > 0 ? s2 : 0);
         }
         for (int o2 = 0; o2 < NH; ++o2) grad[OB2 + o2] += d3[o2];
         for (int i = 0; i < NH; ++i) {
             double s1 = 0;
             for (int o2 = 0; o2 < NH; ++o2) {
-                s1 += P[OW1 + i * NH + o2] * d2[o2];
+                s1 += bf(P[OW1 + i * NH + o2]) * d2[o2];
                 grad[OW1 + i * NH + o2] += h1[i] * d2[o2];
             }
-            d1[i] = h1[i] > 0 ? s1 : 0;
+            d1[i] = bf(h1[i] > 0 ? s1 : 0);
         }
         for (int o2 = 0; o2 < NH; ++o2) grad[OB1 + o2] += d2[o2];
         for (int i = 0; i < NI; ++i)
@@ -200,4 +221,21 @@ double orc_mlp_loss_grad(const double* P, int n, const float* x5, const float* d
         for (int o2 = 0; o2 < NH; ++o2) grad[OB0 + o2] += d1[o2];
     }
     return loss;
+}
+
+/* the same with the GPU kernels' BF16 roundings emulated */
+void orc_mlp_forward_bf16(const double* P, const float* x5, double* out40)
+{
+    g_bf16 = 1;
+    orc_mlp_forward(P, x5, out40);
+    g_bf16 = 0;
+}
+
+double orc_mlp_loss_grad_bf16(const double* P, int n, const float* x5, const float* dir, const float* weight,
+                              double* grad)
+{
+    g_bf16 = 1;
+    const double l = orc_mlp_loss_grad(P, n, x5, dir, weight, grad);
+    g_bf16 = 0;
+    return l;
 }

@@ -199,6 +199,11 @@ void orc_vmf_mixture_sample(const double* out40, const double* u3, double* w);
 /* mean over n samples of -weight * log q(dir | x); grad (param count) or NULL */
 double orc_mlp_loss_grad(const double* params, int n, const float* x5, const float* dir, const float* weight,
                          double* grad);
+/* the same with BF16 rounding where the GPU stores BF16 (inputs, weights,
+ * post-ReLU activations, loss gradient, masked backward activations) */
+void orc_mlp_forward_bf16(const double* params, const float* x5, double* out40);
+double orc_mlp_loss_grad_bf16(const double* params, int n, const float* x5, const float* dir, const float* weight,
+                              double* grad);
 
 typedef struct {
     uint32_t width, height, spp, sample_offset;
