@@ -149,6 +149,8 @@ struct SceneDev {
     uint32_t ntri = 0;
     std::vector<GpuMaterial> host_mats;   // host copy (spectral coefficients filled on first spectral render)
     bool spectral_ready = false;
+    float box_lo[3] = {0, 0, 0};          // scene bounds (guiding-network inputs)
+    float box_inv[3] = {1, 1, 1};         // 1 / extent
 };
 
 // Wavefront state of the integrator, sized for `capacity` paths.
@@ -205,6 +207,9 @@ struct GuideDev {
     uint64_t* gkey = nullptr;       // slot probe key [5][pcap]
     uint32_t* gleaf = nullptr;      // slot leaf texels, 6 bits each
     uint32_t* gflag = nullptr;      // bits 0..4: slot received radiance
+    float* gx = nullptr;            // learned guiding: slot network inputs [5*5][pcap]
+    float* gw = nullptr;            // slot sampled direction [5*3][pcap]
+    float* gp = nullptr;            // slot direction pdf [5][pcap]
 };
 
 // Learned guiding network state (mlp.hip)
@@ -301,7 +306,7 @@ int render(ar_ctx* ctx, const ar_camera* cam, const ar_render_opts* opts, float*
 int trace_prologue(ar_ctx* ctx, TraceArgs& ta, size_t n, uint32_t& blocks);
 
 // guiding.hip
-int guide_ensure(ar_ctx* ctx, size_t paths);
+int guide_ensure(ar_ctx* ctx, size_t paths, bool pool);
 void guide_free(ar_ctx* ctx);
 int guide_clear(ar_ctx* ctx, hipStream_t s);
 // merges the samples collected by the paths of one pass (spectral mode and
@@ -319,4 +324,8 @@ hipError_t mlp_forward(ar_ctx* ctx, const float* x5, size_t n, const uint32_t* n
 hipError_t mlp_gradient(ar_ctx* ctx, const float* x5, const float* dir, const float* w, size_t n,
                         const uint32_t* n_dev, bool normalize, hipStream_t s);
 hipError_t mlp_adam_step(ar_ctx* ctx, hipStream_t s);
+int mlp_ensure_buffers(ar_ctx* ctx, size_t paths);
+// training records from the frame's guiding slots, then the frame's Adam steps
+int mlp_frame_update(ar_ctx* ctx, uint32_t spectral, const float* lambda0, const float* cie, size_t paths,
+                     hipStream_t s);
 } // namespace ar

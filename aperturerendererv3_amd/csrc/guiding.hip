@@ -289,6 +289,7 @@ uint32_t grid_for(size_t n) { return (uint32_t)std::max<size_t>(1, std::min<size
 void free_slots(GuideDev& g) {
     dev_free(g.s_slot); dev_free(g.s_leaf); dev_free(g.s_rad);
     dev_free(g.gm); dev_free(g.gr); dev_free(g.gkey); dev_free(g.gleaf); dev_free(g.gflag);
+    dev_free(g.gx); dev_free(g.gw); dev_free(g.gp);
     g.scap = g.pcap = 0;
 }
 
@@ -317,10 +318,10 @@ int guide_clear(ar_ctx* ctx, hipStream_t s) {
     return check_hip(ctx, hipGetLastError(), "guiding reset");
 }
 
-int guide_ensure(ar_ctx* ctx, size_t paths) {
+int guide_ensure(ar_ctx* ctx, size_t paths, bool pool) {
     GuideDev& g = ctx->guide;
     (void)hipSetDevice(ctx->device);
-    if (g.cap != ctx->guide_cap_req) {
+    if (pool && g.cap != ctx->guide_cap_req) {
         free_pool(g);
         const uint32_t cap = (uint32_t)ctx->guide_cap_req;
         uint32_t tcap = 1;
@@ -361,6 +362,9 @@ int guide_ensure(ar_ctx* ctx, size_t paths) {
         if (rc == AR_OK) rc = dev_alloc(ctx, &g.gkey, 5 * paths, b);
         if (rc == AR_OK) rc = dev_alloc(ctx, &g.gleaf, paths, b);
         if (rc == AR_OK) rc = dev_alloc(ctx, &g.gflag, paths, b);
+        if (rc == AR_OK) rc = dev_alloc(ctx, &g.gx, 25 * paths, b);
+        if (rc == AR_OK) rc = dev_alloc(ctx, &g.gw, 15 * paths, b);
+        if (rc == AR_OK) rc = dev_alloc(ctx, &g.gp, 5 * paths, b);
         if (rc != AR_OK) {
             free_slots(g);
             return rc;
@@ -455,7 +459,7 @@ int ar_guiding_snapshot(ar_ctx* ctx, uint64_t* keys, float* texels, uint64_t* co
 
 int ar_guiding_restore(ar_ctx* ctx, const uint64_t* keys, const float* texels, const uint64_t* counts, uint64_t n) {
     if (!ctx || (n && (!keys || !texels || !counts))) return AR_E_INVALID;
-    int rc = guide_ensure(ctx, 0);
+    int rc = guide_ensure(ctx, 0, true);
     if (rc != AR_OK) return rc;
     GuideDev& g = ctx->guide;
     if (n > g.cap) return set_error(ctx, AR_E_CAPACITY, "more probes than the configured capacity");

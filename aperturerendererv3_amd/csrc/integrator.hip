@@ -20,6 +20,7 @@
 #include "spectral.hpp"
 #include "path_common.hpp"
 #include "guide.hpp"
+#include "mlp.hpp"
 
 #include <cfloat>
 #include <cmath>
@@ -405,6 +406,12 @@ struct GuideArgs {
     uint32_t* gleaf;               // 6 bits per slot
     uint32_t* gflag;               // bit i: slot i received radiance
     uint32_t stride;
+    // learned guiding
+    const float* dec;              // decoded mixture per queue slot [q][48]
+    float* gx;                     // slot network inputs [5*5][stride]
+    float* gw;                     // slot direction [5*3][stride]
+    float* gp;                     // slot direction pdf [5][stride]
+    float box_lo[3], box_inv[3];
 };
 
 __device__ __forceinline__ bool s4zero(const S4& c, uint32_t mode) {
@@ -442,13 +449,22 @@ __device__ __forceinline__ void g_endpoint(const GuideArgs& G, uint32_t p, const
 // CollectRadiance at a surviving vertex (render.cuh:378-409): opens the slot
 // of this non-delta depth, then scales the open slots by contrib / q.
 __device__ __forceinline__ void g_advance(const GuideArgs& G, uint32_t p, uint32_t non_delta, uint32_t mode,
-                                          uint64_t key, int leaf, const S4& contrib, float qrr) {
+                                          uint64_t key, int leaf, const S4& contrib, float qrr, bool learned,
+                                          const float x5[5], F3 dir, float pdf) {
     if (non_delta < 5u) {
         const int i = (int)non_delta;
         if (s4zero(gload(G.gm, G.stride, i, p), mode)) {
             gstore(G.gm, G.stride, i, p, s4(1.0f, 1.0f, 1.0f, 1.0f));
-            G.gkey[(size_t)i * G.stride + p] = key;
-            G.gleaf[p] = (G.gleaf[p] & ~(63u << (6 * i))) | ((uint32_t)leaf << (6 * i));
+            if (learned) {
+                for (int k = 0; k < 5; ++k) G.gx[(size_t)(5 * i + k) * G.stride + p] = x5[k];
+                G.gw[(size_t)(3 * i + 0) * G.stride + p] = dir.x;
+                G.gw[(size_t)(3 * i + 1) * G.stride + p] = dir.y;
+                G.gw[(size_t)(3 * i + 2) * G.stride + p] = dir.z;
+                G.gp[(size_t)i * G.stride + p] = pdf;
+            } else {
+                G.gkey[(size_t)i * G.stride + p] = key;
+                G.gleaf[p] = (G.gleaf[p] & ~(63u << (6 * i))) | ((uint32_t)leaf << (6 * i));
+            }
         }
     }
     const float invq = (float)(1.0 / (double)qrr);
@@ -539,6 +555,35 @@ __global__ __launch_bounds__(kBlock) void k_camera(RenderParams P, PathArgs S, G
     }
 }
 
+// Network inputs of every queued hit (the learned guide's inference batch):
+// the hit point and the shading normal facing the ray, as k_shade computes
+// them.
+__global__ __launch_bounds__(kBlock) void k_mlp_features(const float4* __restrict__ prims, QueueRays cur,
+                                                         const uint32_t* __restrict__ hprim,
+                                                         const float* __restrict__ hu, const float* __restrict__ hv,
+                                                         const uint32_t* __restrict__ counters, float lo0, float lo1,
+                                                         float lo2, float inv0, float inv1, float inv2,
+                                                         float* __restrict__ feat) {
+    const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
+    if (q >= counters[0]) return;
+    float x[5] = {0, 0, 0, 0, 0};
+    const uint32_t prim = hprim[q];
+    if (prim != AR_MISS) {
+        const F3 a = xyz(prims[3 * (size_t)prim + 0]), b = xyz(prims[3 * (size_t)prim + 1]),
+                 c = xyz(prims[3 * (size_t)prim + 2]);
+        const float bu = hu[q], bv = hv[q];
+        const float bw = 1.0f - bu - bv;
+        const F3 pos = add(add(scale(a, bw), scale(b, bu)), scale(c, bv));
+        const F3 ng = normalize(cross(sub(b, a), sub(c, a)));
+        const F3 d = f3(cur.dx[q], cur.dy[q], cur.dz[q]);
+        const F3 nn = dot(ng, d) < 0.0f ? ng : scale(ng, -1.0f);
+        const float lo[3] = {lo0, lo1, lo2}, inv[3] = {inv0, inv1, inv2};
+        mlp::features(pos.x, pos.y, pos.z, nn.x, nn.y, nn.z, lo, inv, x);
+    }
+#pragma unroll
+    for (int k = 0; k < 5; ++k) feat[5 * (size_t)q + k] = x[k];
+}
+
 // One bounce of CalculateRadiance for every queued path: emission with MIS
 // (render.cuh:664-710), BSDF sampling and the hit-point offset
 // (render.cuh:75-163), NEE light sampling (render.cuh:205-343, 766-827;
@@ -577,7 +622,8 @@ __global__ __launch_bounds__(kBlock) void k_shade(
         const Lam L = make_lam(P.spectral, S.lambda0[p]);
         S4 mask = load4(S.mask0, S.mask1, S.mask2, S.mask3, p);
         S4 rad = load4(S.rad0, S.rad1, S.rad2, S.rad3, p);
-        const bool guided = P.guiding && *G.probe_count > 0u;
+        const bool learned = P.guiding == AR_GUIDING_MLP;
+        const bool guided = learned ? P.alpha > 0.0f : (P.guiding && *G.probe_count > 0u);
         float gq[3] = {1.0f, 1.0f, 1.0f};
         if (P.guiding && depth > 0) probe_query(mask, L, sc.cie, gq);
         const F3 o = f3(cur.ox[q], cur.oy[q], cur.oz[q]);
@@ -644,18 +690,47 @@ __global__ __launch_bounds__(kBlock) void k_shade(
                 float pdf;
                 if (sample_bsdf(m, pspec, d, nn, u, next, pdf)) {
                     S4 fcos = eval_bsdf(m, alb, d, next, nn);
+                    float x5[5] = {0, 0, 0, 0, 0};
+                    if (learned) mlp::features(pos.x, pos.y, pos.z, nn.x, nn.y, nn.z, G.box_lo, G.box_inv, x5);
                     const float off = eps * maxr(1.0f, length(pos)) * (u[4] + 0.5f);
                     const float dn = dot(nn, next);
                     const float sgn = dn > 0.0f ? 1.0f : (dn < 0.0f ? -1.0f : 0.0f);
                     pos = add(pos, scale(nn, sgn * off));
                     // guide/BSDF one-sample mixture (render.cuh:553-638)
                     const float4* ptex = nullptr;
+                    const float* pmix = nullptr;
                     uint64_t gkey = guide::kNoKey;
-                    if (P.guiding) {
+                    if (P.guiding == AR_GUIDING_VOXEL) {
                         const F3 rv = sub(d, scale(nn, 2.0f * dot(d, nn)));
                         gkey = guide::voxel_key(pos.x, pos.y, pos.z, rv.x, rv.y, rv.z);
                     }
-                    if (guided) {
+                    if (guided && learned) {
+                        // the network's mixture for this vertex (decoded by the inference pass)
+                        pmix = G.dec + (size_t)q * mlp::kDecoded;
+                        const float pb = pdf;
+                        const float pp = mlp::hemi_pdf(pmix, next.x, next.y, next.z, nn.x, nn.y, nn.z);
+                        if (pb > 1e-12f && pp > 1e-12f) {
+                            if (u[10] < P.alpha) {
+                                float ug[4];
+                                uniforms(P.seed, pix, smp, depth, 3, ug);
+                                float gx, gy, gz, gp;
+                                mlp::hemi_sample(pmix, nn.x, nn.y, nn.z, ug, gx, gy, gz, gp);
+                                const F3 gv = f3(gx, gy, gz);
+                                if (dot(gv, nn) > 1e-12f) {
+                                    const float pbs = bsdf_pdf(m, pspec, d, gv, nn);
+                                    next = gv;
+                                    fcos = eval_bsdf(m, alb, d, gv, nn);
+                                    pdf = (1.0f - P.alpha) * maxr(pbs, 1e-12f) + P.alpha * gp;
+                                } else {
+                                    pmix = nullptr;
+                                }
+                            } else {
+                                pdf = (1.0f - P.alpha) * pb + P.alpha * pp;
+                            }
+                        } else {
+                            pmix = nullptr;
+                        }
+                    } else if (guided) {
                         const uint32_t pi = guide::find(G.pr, gkey);
                         if (pi != guide::kNoProbe) {
                             ptex = G.pr.tex + (size_t)pi * guide::kTexels;
@@ -720,6 +795,9 @@ __global__ __launch_bounds__(kBlock) void k_shade(
                             if (ptex)
                                 pb = (1.0f - P.alpha) * pb +
                                      P.alpha * guide::hemi_pdf(ptex, ldir.x, ldir.y, ldir.z, nn.x, nn.y, nn.z, gq);
+                            else if (pmix)
+                                pb = (1.0f - P.alpha) * pb +
+                                     P.alpha * mlp::hemi_pdf(pmix, ldir.x, ldir.y, ldir.z, nn.x, nn.y, nn.z);
                             const float w = power_heuristic(pl, pb);
                             contrib_w = s4mul(mask, s4scale(contrib, w));
                         }
@@ -754,8 +832,9 @@ __global__ __launch_bounds__(kBlock) void k_shade(
                     }
                     if (alive && depth + 1 < P.max_depth) {
                         if (P.guiding)
-                            g_advance(G, p, non_delta, P.spectral, gkey, guide::leaf_of(next.x, next.y, next.z),
-                                      contrib, qrr);
+                            g_advance(G, p, non_delta, P.spectral, gkey,
+                                      learned ? 0 : guide::leaf_of(next.x, next.y, next.z), contrib, qrr, learned, x5,
+                                      next, pdf);
                         mask = s4scale(s4mul(mask, contrib), 1.0f / qrr);
                         cont = true;
                         next_o = pos;
@@ -924,9 +1003,12 @@ int upload_scene(ar_ctx* ctx) {
     std::vector<int32_t> ltri;
     std::vector<float> larea, lprob, lcdf;
     float total = 0.0f;
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
     for (uint32_t t = 0; t < n; ++t) {
         const float* v[3];
         for (int k = 0; k < 3; ++k) v[k] = &ctx->verts[3 * (size_t)ctx->idx[3 * (size_t)t + k]];
+        for (int k = 0; k < 3; ++k)
+            for (int a = 0; a < 3; ++a) { lo[a] = std::min(lo[a], v[k][a]); hi[a] = std::max(hi[a], v[k][a]); }
         const ar_material& m = ctx->mats[ctx->tri_mat[t]];
         int32_t light = -1;
         if (m.emission[0] > 0.0f || m.emission[1] > 0.0f || m.emission[2] > 0.0f) {
@@ -945,6 +1027,11 @@ int upload_scene(ar_ctx* ctx) {
         }
         std::memcpy(&prims[12 * (size_t)t + 3], &ctx->tri_mat[t], 4);
         std::memcpy(&prims[12 * (size_t)t + 7], &light, 4);
+    }
+    for (int a = 0; a < 3; ++a) {
+        const float ext = n ? hi[a] - lo[a] : 0.0f;
+        s.box_lo[a] = n ? lo[a] : 0.0f;
+        s.box_inv[a] = ext > 0.0f ? 1.0f / ext : 1.0f;
     }
     float acc = 0.0f;
     for (size_t i = 0; i < lprob.size(); ++i) {
@@ -1073,9 +1160,10 @@ int render(ar_ctx* ctx, const ar_camera* cam, const ar_render_opts* opts, float*
     const size_t max_paths = opts->max_paths_per_pass ? opts->max_paths_per_pass : ((size_t)1 << 22);
     if (npix > max_paths && npix > ((size_t)1 << 26))
         return set_error(ctx, AR_E_CAPACITY, "image larger than 2^26 pixels");
-    if (opts->guiding != AR_GUIDING_OFF && opts->guiding != AR_GUIDING_VOXEL)
-        return set_error(ctx, AR_E_INVALID, "guiding must be AR_GUIDING_OFF or AR_GUIDING_VOXEL");
-    const bool guiding = opts->guiding == AR_GUIDING_VOXEL;
+    if (opts->guiding != AR_GUIDING_OFF && opts->guiding != AR_GUIDING_VOXEL && opts->guiding != AR_GUIDING_MLP)
+        return set_error(ctx, AR_E_INVALID, "guiding must be AR_GUIDING_OFF, _VOXEL or _MLP");
+    const bool guiding = opts->guiding != AR_GUIDING_OFF;
+    const bool learned = opts->guiding == AR_GUIDING_MLP;
     // guiding renders frames of one sample per pixel with a probe update
     // after each (the reference's Render + UpdatePathGuidingSample loop)
     const uint32_t spp_pass = guiding ? 1u
@@ -1084,7 +1172,12 @@ int render(ar_ctx* ctx, const ar_camera* cam, const ar_render_opts* opts, float*
     int rc = ensure_render_buffers(ctx, cap, npix);
     if (rc != AR_OK) return rc;
     if (guiding) {
-        rc = guide_ensure(ctx, cap);
+        rc = guide_ensure(ctx, cap, !learned);
+        if (rc != AR_OK) return rc;
+    }
+    if (learned) {
+        rc = mlp_ensure(ctx);
+        if (rc == AR_OK) rc = mlp_ensure_buffers(ctx, cap);
         if (rc != AR_OK) return rc;
     }
     TraceArgs ta;
@@ -1104,7 +1197,7 @@ int render(ar_ctx* ctx, const ar_camera* cam, const ar_render_opts* opts, float*
     P.seed = opts->seed;
     P.spectral = opts->spectral;
     P.output = opts->output;
-    P.guiding = guiding ? 1u : 0u;
+    P.guiding = opts->guiding;
     P.alpha = 0.0f;
     {
         float e[3] = {opts->env[0], opts->env[1], opts->env[2]};
@@ -1156,8 +1249,11 @@ int render(ar_ctx* ctx, const ar_camera* cam, const ar_render_opts* opts, float*
     GuideDev& gd = ctx->guide;
     GuideArgs G{};
     if (guiding) {
-        G.pr = guide::Probes{gd.tkey, gd.tval, gd.tcap - 1, gd.tex, 0u};
+        G.pr = guide::Probes{gd.tkey, gd.tval, gd.tcap ? gd.tcap - 1 : 0u, gd.tex, 0u};
         G.probe_count = gd.counters;
+        G.dec = ctx->mlp.dec;
+        G.gx = gd.gx; G.gw = gd.gw; G.gp = gd.gp;
+        for (int k = 0; k < 3; ++k) { G.box_lo[k] = ctx->scene.box_lo[k]; G.box_inv[k] = ctx->scene.box_inv[k]; }
         G.gm = gd.gm; G.gr = gd.gr; G.gkey = gd.gkey; G.gleaf = gd.gleaf; G.gflag = gd.gflag;
         G.stride = (uint32_t)gd.pcap;
     }
@@ -1178,6 +1274,14 @@ int render(ar_ctx* ctx, const ar_camera* cam, const ar_render_opts* opts, float*
             ar_rays rr{q[c].ox, q[c].oy, q[c].oz, q[c].tmin, q[c].dx, q[c].dy, q[c].dz, q[c].tmax};
             hipError_t e = launch_trace_closest(ta, rr, P.capacity, hits, nullptr, nullptr, tblocks, stream, &b.counters[0]);
             if (e != hipSuccess) return check_hip(ctx, e, "integrator trace");
+            if (learned && P.alpha > 0.0f) {
+                const SceneDev& sd = ctx->scene;
+                hipLaunchKernelGGL(k_mlp_features, dim3(grid_for(P.capacity)), dim3(kBlock), 0, stream, sd.prims, q[c],
+                                   b.hit_prim, b.hit_u, b.hit_v, b.counters, sd.box_lo[0], sd.box_lo[1], sd.box_lo[2],
+                                   sd.box_inv[0], sd.box_inv[1], sd.box_inv[2], ctx->mlp.feat);
+                e = mlp_forward(ctx, ctx->mlp.feat, P.capacity, &b.counters[0], ctx->mlp.dec, true, stream);
+                if (e != hipSuccess) return check_hip(ctx, e, "guiding network inference");
+            }
             hipLaunchKernelGGL(k_shade, dim3(grid_for(P.capacity)), dim3(kBlock), 0, stream, P, depth, sc, S, G, q[c],
                                b.qpath[c], b.hit_prim, b.hit_t, b.hit_u, b.hit_v, q[1 - c], b.qpath[1 - c], shq,
                                b.spath, b.scontrib[0], b.scontrib[1], b.scontrib[2], b.scontrib[3], b.sflags,
@@ -1194,7 +1298,8 @@ int render(ar_ctx* ctx, const ar_camera* cam, const ar_render_opts* opts, float*
             c = 1 - c;
         }
         if (guiding) {
-            rc = guide_update(ctx, P.spectral, b.lambda0, b.cie, P.capacity, stream);
+            rc = learned ? mlp_frame_update(ctx, P.spectral, b.lambda0, b.cie, P.capacity, stream)
+                         : guide_update(ctx, P.spectral, b.lambda0, b.cie, P.capacity, stream);
             if (rc != AR_OK) return rc;
         }
         done += this_spp;

@@ -17,6 +17,8 @@
 // atomics; Adam updates the FP32 masters and rewrites the BF16 copies.
 #include "ar_internal.hpp"
 #include "mlp.hpp"
+#include "path_common.hpp"
+#include "spectral.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -366,7 +368,103 @@ __global__ void k_zero(float* p, size_t n) {
 
 uint32_t tiles(size_t n) { return (uint32_t)std::max<size_t>(1, std::min<size_t>((n + 63) / 64, 8192)); }
 
+struct Slots {
+    const float* gr;       // [5*4][stride] collected radiance
+    const uint32_t* gflag;
+    const float* gx;       // [5*5][stride]
+    const float* gw;       // [5*3][stride]
+    const float* gp;       // [5][stride]
+    uint32_t stride;
+};
+
+// Training records of a frame: every slot that received radiance gives
+// (inputs, direction, luminance(L) / pdf(direction)), the Monte-Carlo weight
+// of the KL divergence from the incident radiance to the guide.
+__global__ __launch_bounds__(256) void k_mlp_records(Slots S, uint32_t npaths, uint32_t spectral,
+                                                     const float* __restrict__ lambda0, const float* __restrict__ cie,
+                                                     float* rx, float* rd, float* rw, uint32_t* count, uint32_t cap) {
+    const uint32_t p = blockIdx.x * 256 + threadIdx.x;
+    const bool live = p < npaths;
+    const uint32_t flags = live ? S.gflag[p] : 0u;
+    const Lam L = make_lam(spectral, live && spectral != AR_SPECTRAL_RGB ? lambda0[p] : 0.0f);
+    for (int i = 0; i < 5; ++i) {
+        float w = 0.0f;
+        if ((flags >> i) & 1u) {
+            const float* r = S.gr + (size_t)(4 * i) * S.stride + p;
+            float lum;
+            if (spectral == AR_SPECTRAL_RGB) {
+                lum = r[0] * 0.2126f + r[S.stride] * 0.7152f + r[2 * (size_t)S.stride] * 0.0722f;
+            } else if (spectral == AR_SPECTRAL_SINGLE) {
+                float cx, cy, cz;
+                spec::cie_xyz(cie, L.l[0], cx, cy, cz);
+                lum = cy * r[0] / spec::lambda_pdf(L.l[0]);
+            } else {
+                lum = 0.0f;
+                for (int k = 0; k < 4; ++k) {
+                    float cx, cy, cz;
+                    spec::cie_xyz(cie, L.l[k], cx, cy, cz);
+                    lum += cy * r[(size_t)k * S.stride];
+                }
+                lum = lum / (4.0f / 400.0f);
+            }
+            const float pdf = S.gp[(size_t)i * S.stride + p];
+            w = pdf > 0.0f ? lum / pdf : 0.0f;
+        }
+        const bool ok = w > 0.0f && w < 3.0e38f;
+        const uint32_t slot = wave_append(ok, count);
+        if (ok && slot < cap) {
+            for (int k = 0; k < 5; ++k) rx[5 * (size_t)slot + k] = S.gx[(size_t)(5 * i + k) * S.stride + p];
+            for (int k = 0; k < 3; ++k) rd[3 * (size_t)slot + k] = S.gw[(size_t)(3 * i + k) * S.stride + p];
+            rw[slot] = w;
+        }
+    }
+}
+
+__global__ void k_clamp_count(uint32_t* count, uint32_t cap) {
+    if (count[0] > cap) count[0] = cap;
+}
+
 } // namespace
+
+int mlp_ensure_buffers(ar_ctx* ctx, size_t paths) {
+    MlpDev& M = ctx->mlp;
+    if (M.cap >= paths) return AR_OK;
+    for (float** q : {&M.feat, &M.dec, &M.rx, &M.rd, &M.rw})
+        if (*q) { (void)hipFree(*q); *q = nullptr; }
+    if (M.rcount) { (void)hipFree(M.rcount); M.rcount = nullptr; }
+    M.cap = M.rcap = 0;
+    const size_t rcap = 5 * paths;
+    hipError_t e = hipMalloc((void**)&M.feat, paths * 5 * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc((void**)&M.dec, paths * kDecoded * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc((void**)&M.rx, rcap * 5 * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc((void**)&M.rd, rcap * 3 * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc((void**)&M.rw, rcap * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc((void**)&M.rcount, 4 * sizeof(uint32_t));
+    if (e != hipSuccess) return set_error(ctx, AR_E_OOM, "mlp: device allocation failed");
+    M.cap = paths;
+    M.rcap = rcap;
+    return AR_OK;
+}
+
+__global__ void k_zero_u32(uint32_t* p) { p[0] = 0u; }
+
+int mlp_frame_update(ar_ctx* ctx, uint32_t spectral, const float* lambda0, const float* cie, size_t paths,
+                     hipStream_t s) {
+    MlpDev& M = ctx->mlp;
+    const GuideDev& g = ctx->guide;
+    Slots S{g.gr, g.gflag, g.gx, g.gw, g.gp, (uint32_t)g.pcap};
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, M.rcount);
+    hipLaunchKernelGGL(k_mlp_records, dim3((uint32_t)((paths + 255) / 256)), dim3(256), 0, s, S, (uint32_t)paths,
+                       spectral, lambda0, cie, M.rx, M.rd, M.rw, M.rcount, (uint32_t)M.rcap);
+    hipLaunchKernelGGL(k_clamp_count, dim3(1), dim3(1), 0, s, M.rcount, (uint32_t)M.rcap);
+    const size_t n = std::min<size_t>(M.rcap, M.batch);
+    hipError_t e = hipSuccess;
+    for (uint32_t it = 0; it < M.iters && e == hipSuccess; ++it) {
+        e = mlp_gradient(ctx, M.rx, M.rd, M.rw, n, M.rcount, true, s);
+        if (e == hipSuccess) e = mlp_adam_step(ctx, s);
+    }
+    return check_hip(ctx, e, "guiding network training");
+}
 
 void mlp_free(ar_ctx* ctx) {
     MlpDev& M = ctx->mlp;

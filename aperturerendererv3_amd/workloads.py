@@ -335,3 +335,61 @@ def spectral_scene(target_tris: int = 500_000, nlights: int = 64, seed: int = 1,
         M[t] = len(mats)
         mats.append(m)
     return V, F, M, mats
+
+
+# --------------------------------------------------------------------------
+# Occluded-light interior: a closed room lit only through a door gap
+# --------------------------------------------------------------------------
+
+OCCLUDED_CAMERA = {"pos": (1.2, 1.4, 1.5), "dir": (-0.8, -0.25, -0.6), "vfov": 70.0}
+
+
+def _grid_quad(origin, eu, ev, nu, nv, keep=None):
+    """Quad origin + s*eu + t*ev split into nu x nv cells of two triangles,
+    wound so the normal is eu x ev; keep(s, t) -> bool drops cells (holes)."""
+    o, eu, ev = (np.asarray(a, np.float64) for a in (origin, eu, ev))
+    s = np.linspace(0, 1, nu + 1)
+    t = np.linspace(0, 1, nv + 1)
+    P = o + s[:, None, None] * eu + t[None, :, None] * ev
+    V = P.reshape(-1, 3)
+    idx = lambda i, j: i * (nv + 1) + j
+    ii, jj = np.meshgrid(np.arange(nu), np.arange(nv), indexing="ij")
+    ii, jj = ii.ravel(), jj.ravel()
+    if keep is not None:
+        m = keep((ii + 0.5) / nu, (jj + 0.5) / nv)
+        ii, jj = ii[m], jj[m]
+    a, b, c, d = idx(ii, jj), idx(ii + 1, jj), idx(ii + 1, jj + 1), idx(ii, jj + 1)
+    F = np.concatenate([np.stack([a, b, c], 1), np.stack([a, c, d], 1)])
+    return V, F
+
+
+def occluded_room(target_tris: int = 200_000, le: float = 40.0):
+    """A 4 x 3 x 4 m room ([-2, 2] x [0, 3] x [-2, 2], all walls facing
+    inward, grey 0.6 Lambert, the floor a GGX 0.3) whose +x wall has a
+    0.5 m wide, 2 m high door gap (z in [-0.25, 0.25]); the only light is a
+    2 x 2 m emitter (radiance `le`, white) outside at x = 3.5 facing the door,
+    so the room is lit by light through the gap.  Walls are tessellated so
+    that the scene has about `target_tris` triangles."""
+    per = max(1, int(math.sqrt(target_tris / 12.0)))
+    parts = []
+    W, H = 4.0, 3.0
+    # floor (+y), ceiling (-y), walls facing inward
+    parts.append((_grid_quad((-2, 0, -2), (0, 0, W), (W, 0, 0), per, per), 1))
+    parts.append((_grid_quad((-2, H, -2), (W, 0, 0), (0, 0, W), per, per), 0))
+    parts.append((_grid_quad((-2, 0, -2), (0, H, 0), (0, 0, W), per, per), 0))            # x = -2, normal +x
+    parts.append((_grid_quad((-2, 0, -2), (W, 0, 0), (0, H, 0), per, per), 0))            # z = -2, normal +z
+    parts.append((_grid_quad((-2, 0, 2), (0, H, 0), (W, 0, 0), per, per), 0))             # z = +2, normal -z
+    door = lambda s, t: ~((np.abs(s * W - 2.0) < 0.25) & (t * H < 2.0))
+    parts.append((_grid_quad((2, 0, -2), (0, 0, W), (0, H, 0), per, per, keep=door), 0))  # x = +2, normal -x
+    parts.append((_grid_quad((3.5, 0.0, -1.0), (0, 0, 2), (0, 2, 0), 1, 1), 2))           # light at x = 3.5, normal -x
+    Vs, Fs, Ms = [], [], []
+    nv = 0
+    for (V, F), m in parts:
+        Vs.append(V)
+        Fs.append(F + nv)
+        Ms.append(np.full(F.shape[0], m, np.uint32))
+        nv += V.shape[0]
+    mats = [{"albedo": (0.6, 0.6, 0.6)},
+            {"albedo": (0.5, 0.45, 0.4), "model": 1, "roughness": 0.3, "specular": 0.5, "metallic": 0.0},
+            {"albedo": (0.0, 0.0, 0.0), "emission": (le, le, le)}]
+    return (np.concatenate(Vs).astype(np.float32), np.concatenate(Fs).astype(np.uint32), np.concatenate(Ms), mats)

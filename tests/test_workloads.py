@@ -44,3 +44,18 @@ def test_spectral_scene_counts_and_materials():
     assert np.all(V[-4:, 1] == -10.0)
     V2, F2, M2, _ = spectral_scene()
     assert np.array_equal(V, V2) and np.array_equal(F, F2) and np.array_equal(M, M2)
+
+
+def test_occluded_room():
+    from aperturerendererv3_amd.workloads import occluded_room
+    V, F, M, mats = occluded_room()
+    assert abs(F.shape[0] - 200_000) < 10_000
+    a, b, c = V[F[:, 0]], V[F[:, 1]], V[F[:, 2]]
+    n = np.cross(b - a, c - a)
+    cen = (a + b + c) / 3
+    walls = M < 2
+    assert ((n[walls] * (np.array([0, 1.5, 0]) - cen[walls])).sum(1) > 0).all()    # all walls face inward
+    assert np.allclose(n[M == 2] / np.linalg.norm(n[M == 2], axis=1, keepdims=True), [-1, 0, 0])
+    # the door gap: no wall triangle covers (2, 1, 0)
+    door = (np.abs(cen[:, 0] - 2) < 1e-6) & (np.abs(cen[:, 2]) < 0.25) & (cen[:, 1] < 2)
+    assert not door.any()
