@@ -64,8 +64,70 @@ def _stream_ptr(stream) -> int | None:
     return int(stream.cuda_stream)
 
 
+def _camera_opts(width, height, spp, cam_pos=(0.0, 0.0, 1.1), cam_dir=(0.0, 0.0, -1.0), vfov=60.0, seed=1,
+                 max_depth=5, nee=True, sky=AR_SKY_NONE, env=(0.0, 0.0, 0.0), rr_min_depth=3, sample_offset=0,
+                 max_paths_per_pass=0, spectral=0, output=0, guiding=0):
+    cam = ar_camera()
+    cam.pos[:] = [float(x) for x in cam_pos]
+    cam.dir[:] = [float(x) for x in cam_dir]
+    cam.vfov_deg = float(vfov)
+    o = ar_render_opts()
+    o.width, o.height, o.spp, o.sample_offset = width, height, spp, sample_offset
+    o.seed, o.max_depth, o.rr_min_depth, o.nee, o.sky = seed, max_depth, rr_min_depth, int(bool(nee)), sky
+    o.env[:] = [float(x) for x in env]
+    o.max_paths_per_pass = max_paths_per_pass
+    o.spectral, o.output, o.guiding = spectral, output, guiding
+    return cam, o
+
+
+def _view_arrays(views, shared):
+    """views: list of dicts (width, height, spp, cam_pos, cam_dir, vfov, sample_offset)."""
+    n = len(views)
+    cams = (ar_camera * n)()
+    opts = (ar_render_opts * n)()
+    for i, v in enumerate(views):
+        kw = dict(shared)
+        kw.update(v)
+        c, o = _camera_opts(**kw)
+        cams[i] = c
+        opts[i] = o
+    return cams, opts
+
+
+class Graph:
+    """A captured render (ar_graph): launch() replays it, writing `outputs`."""
+
+    def __init__(self, lib, handle, outputs):
+        self._lib, self._g, self.outputs = lib, handle, outputs
+
+    def launch(self, stream=None):
+        rc = self._lib.ar_graph_launch(self._g, _stream_ptr(stream))
+        if rc != AR_OK:
+            raise ArError(rc, "ar_graph_launch")
+
+    def close(self):
+        if getattr(self, "_g", None):
+            self._lib.ar_graph_destroy(self._g)
+            self._g = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Context:
     """One device context (ar_ctx).  Thread-compatible, not thread-safe."""
+
+    @classmethod
+    def _adopt(cls, handle, device):
+        c = cls.__new__(cls)
+        c._lib = _lib.load()
+        c._h = handle
+        c.device = device
+        c._keep = []
+        return c
 
     def __init__(self, device: int = 0):
         self._lib = _lib.load()
@@ -132,6 +194,34 @@ class Context:
         o = ar_bvh_opts(width, leaf_size, bin_count, max_depth)
         self._check(self._lib.ar_bvh_build(self._h, C.byref(o)), "ar_bvh_build")
         return self.bvh_info()
+
+    # ---------------------------------------------------------------- batches
+    def set_tile(self, threads: int):
+        self._check(self._lib.ar_set_tile(self._h, int(threads)), "ar_set_tile")
+
+    def _outputs(self, views, shared):
+        import torch
+        return [torch.empty((v["height"], v["width"], 3), dtype=torch.float32, device=f"cuda:{self.device}")
+                for v in views]
+
+    def render_views(self, views, outs=None, stream=None, **shared):
+        """Render every view (batch contexts: one per scene; plain: one) in one
+        call (ar_render_batch).  Returns the list of (h, w, 3) images."""
+        cams, opts = _view_arrays(views, shared)
+        outs = outs or self._outputs(views, shared)
+        ptrs = (C.c_void_p * len(views))(*[_ptr(o) for o in outs])
+        self._check(self._lib.ar_render_batch(self._h, len(views), cams, opts, ptrs, _stream_ptr(stream)),
+                    "ar_render_batch")
+        return outs
+
+    def capture(self, views, **shared) -> Graph:
+        """Capture a render of the views into a HIP graph (ar_graph_capture)."""
+        cams, opts = _view_arrays(views, shared)
+        outs = self._outputs(views, shared)
+        ptrs = (C.c_void_p * len(views))(*[_ptr(o) for o in outs])
+        g = C.c_void_p()
+        self._check(self._lib.ar_graph_capture(self._h, len(views), cams, opts, ptrs, C.byref(g)), "ar_graph_capture")
+        return Graph(self._lib, g, outs)
 
     # ---------------------------------------------------------------- learned guiding
     def mlp_configure(self, learning_rate=5e-3, batch=1 << 18, iters_per_frame=1, seed=1):
@@ -286,18 +376,40 @@ class Context:
                max_paths_per_pass=0, spectral=0, output=0, guiding=0, out=None, stream=None):
         """Mean radiance image (height, width, 3) float32 on the GPU (ar_render)."""
         import torch
-        cam = ar_camera()
-        cam.pos[:] = [float(x) for x in cam_pos]
-        cam.dir[:] = [float(x) for x in cam_dir]
-        cam.vfov_deg = float(vfov)
-        o = ar_render_opts()
-        o.width, o.height, o.spp, o.sample_offset = width, height, spp, sample_offset
-        o.seed, o.max_depth, o.rr_min_depth, o.nee, o.sky = seed, max_depth, rr_min_depth, int(bool(nee)), sky
-        o.env[:] = [float(x) for x in env]
-        o.max_paths_per_pass = max_paths_per_pass
-        o.spectral, o.output, o.guiding = spectral, output, guiding
+        cam, o = _camera_opts(width, height, spp, cam_pos, cam_dir, vfov, seed, max_depth, nee, sky, env,
+                              rr_min_depth, sample_offset, max_paths_per_pass, spectral, output, guiding)
         if out is None:
             out = torch.empty((height, width, 3), dtype=torch.float32, device=f"cuda:{self.device}")
         self._check(self._lib.ar_render(self._h, C.byref(cam), C.byref(o), _ptr(out), _stream_ptr(stream)),
                     "ar_render")
         return out
+
+
+def make_batch(scenes) -> Context:
+    """Merge scene contexts (scene set, BVH built, same width) into a batch
+    context (ar_batch_create)."""
+    lib = _lib.load()
+    hs = (C.c_void_p * len(scenes))(*[c._h.value if hasattr(c._h, "value") else c._h for c in scenes])
+    out = C.c_void_p()
+    rc = lib.ar_batch_create(hs, len(scenes), C.byref(out))
+    if rc != AR_OK:
+        msg = lib.ar_last_error(scenes[0]._h)
+        raise ArError(rc, f"ar_batch_create: {msg.decode() if msg else ''}")
+    b = Context._adopt(out, scenes[0].device)
+    b._keep.append(list(scenes))
+    return b
+
+
+def autotune(scenes, views, probe_spp=1, **shared) -> dict:
+    """ar_autotune_batch: fastest (BVH width, tile) for rendering the views of
+    the scenes as one batch; leaves the scenes built at that width."""
+    lib = _lib.load()
+    cams, opts = _view_arrays(views, shared)
+    hs = (C.c_void_p * len(scenes))(*[c._h.value if hasattr(c._h, "value") else c._h for c in scenes])
+    res = _lib.ar_tune_result()
+    rc = lib.ar_autotune_batch(hs, len(scenes), cams, opts, int(probe_spp), C.byref(res))
+    if rc != AR_OK:
+        msg = lib.ar_last_error(scenes[0]._h)
+        raise ArError(rc, f"ar_autotune_batch: {msg.decode() if msg else ''}")
+    return {"width": res.width, "tile": res.tile,
+            "ms": {f"w{w}_t{t}": res.ms[i][j] for i, w in enumerate((2, 8)) for j, t in enumerate((64, 128, 256, 512))}}

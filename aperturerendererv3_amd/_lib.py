@@ -79,6 +79,10 @@ class ar_mlp_opts(C.Structure):
 MLP_PARAMS = 16 * 64 + 64 + 64 * 64 + 64 + 64 * 64 + 64 + 64 * 48 + 48
 
 
+class ar_tune_result(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("tile", C.c_uint32), ("ms", (C.c_float * 4) * 2)]
+
+
 class ar_guiding_info(C.Structure):
     _fields_ = [("probes", C.c_uint64), ("capacity", C.c_uint64), ("samples", C.c_uint64), ("dropped", C.c_uint64),
                 ("device_bytes", C.c_uint64)]
@@ -115,6 +119,16 @@ SIGNATURES = {
     "ar_render": (C.c_int, [C.c_void_p, C.POINTER(ar_camera), C.POINTER(ar_render_opts), C.c_void_p, C.c_void_p]),
     "ar_spectrum_lut_build": (C.c_int, [C.c_void_p, C.c_size_t]),
     "ar_guiding_configure": (C.c_int, [C.c_void_p, C.c_uint64]),
+    "ar_batch_create": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_void_p)]),
+    "ar_batch_size": (C.c_int, [C.c_void_p]),
+    "ar_render_batch": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "ar_set_tile": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "ar_graph_capture": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                   C.POINTER(C.c_void_p)]),
+    "ar_graph_launch": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "ar_graph_destroy": (C.c_int, [C.c_void_p]),
+    "ar_autotune_batch": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_uint32,
+                                    C.POINTER(ar_tune_result)]),
     "ar_mlp_configure": (C.c_int, [C.c_void_p, C.POINTER(ar_mlp_opts)]),
     "ar_mlp_params_get": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     "ar_mlp_params_set": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),

@@ -200,6 +200,8 @@ int ar_destroy(ar_ctx* ctx) {
     free_render_buffers(ctx);
     guide_free(ctx);
     mlp_free(ctx);
+    free_job(ctx->job);
+    ctx->job = nullptr;
     if (ctx->spill) (void)hipFree(ctx->spill);
     if (ctx->fetch) (void)hipFree(ctx->fetch);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);

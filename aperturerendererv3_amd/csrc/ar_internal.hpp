@@ -174,6 +174,11 @@ struct RenderBuffers {
     float* lambda0;               // per path: first wavelength (spectral modes)
     float* accum;                 // width*height*3 running sums
     size_t accum_pixels = 0;
+    uint32_t* pix;                // per path: pixel, sample index, scene
+    uint32_t* smp;
+    uint32_t* scn;
+    uint32_t* qroot[2];           // queue ray -> BVH root of its scene
+    uint32_t* sroot;              // shadow ray -> BVH root
     float* eps_table;             // INTERSECT_EPS(d) for d < 256
     float* cie;                   // device copies of the CIE 1931 / D65 tables
     float* d65;
@@ -237,6 +242,24 @@ struct MlpDev {
     uint32_t* rcount = nullptr; // records collected this frame
 };
 
+// One scene of a batch context (ar_batch_create): its place in the merged arrays.
+struct BatchEntry {
+    uint32_t root;         // BVH root node
+    uint32_t light_off;    // first light
+    uint32_t light_cnt;
+    uint32_t tri_off;      // first triangle (prim id offset)
+};
+
+struct RenderJob;           // integrator.hip: a planned render and its device plan
+
+// a captured render (ar_graph_capture)
+struct GraphRec {
+    ar_ctx* ctx = nullptr;
+    RenderJob* job = nullptr;
+    hipGraphExec_t exec = nullptr;
+    uint64_t generation = 0;
+};
+
 } // namespace ar
 
 struct ar_ctx {
@@ -269,6 +292,11 @@ struct ar_ctx {
     ar::GuideDev guide;
     uint64_t guide_cap_req = 1u << 20;
     ar::MlpDev mlp;
+
+    std::vector<ar::BatchEntry> batch;   // non-empty for a batch context
+    ar::RenderJob* job = nullptr;         // plan of the last render
+    uint32_t tile = 256;                  // threads per block of the wavefront kernels
+    uint64_t buffer_generation = 0;       // bumped whenever render buffers are reallocated
 };
 
 namespace ar {
@@ -286,6 +314,7 @@ struct TraceArgs {
     int* spill;
     uint32_t threads;
     uint32_t* fetch;     // device ray-refill counter
+    const uint32_t* roots = nullptr;   // per-ray root node (batched scenes), else `root`
 };
 // n_dev (optional): device pointer to the ray count, read by the kernel
 // (then n is only the capacity used to size the grid).
@@ -303,6 +332,13 @@ int upload_scene(ar_ctx* ctx);
 void free_scene(ar_ctx* ctx);
 void free_render_buffers(ar_ctx* ctx);
 int render(ar_ctx* ctx, const ar_camera* cam, const ar_render_opts* opts, float* out, hipStream_t s);
+int render_batch(ar_ctx* ctx, uint32_t n, const ar_camera* cams, const ar_render_opts* opts, float* const* outs,
+                 hipStream_t s);
+int graph_capture(ar_ctx* ctx, uint32_t n, const ar_camera* cams, const ar_render_opts* opts, float* const* outs,
+                  GraphRec* g);
+void free_job(RenderJob* job);
+// batch.cpp
+int batch_merge(ar_ctx* out, ar_ctx* const* scenes, uint32_t n);
 int trace_prologue(ar_ctx* ctx, TraceArgs& ta, size_t n, uint32_t& blocks);
 
 // guiding.hip

@@ -30,7 +30,8 @@ namespace ar {
 namespace {
 
 constexpr uint32_t kCameraBounce = 0xFFFFFFFFu;
-constexpr int kBlock = 256;
+constexpr int kBlock = 256;        // default threads per block of the wavefront kernels
+constexpr int kMaxBlock = 512;     // largest block the wavefront kernels accept
 constexpr uint32_t kMetaNee = 1u << 16;
 
 // ---------------------------------------------------------------- helpers
@@ -348,11 +349,31 @@ __device__ __forceinline__ F3 sky_gradient(F3 d) {
     return scale(e, 0.5f);
 }
 
+// One scene of a render (a batch entry): image, samples, camera basis
+// (core/scene.cu:145-164), BVH root and light range in the merged arrays.
+struct SceneView {
+    uint32_t width, height, npix, spp;
+    uint32_t sample_offset;      // index of the scene's first sample
+    uint32_t out_base;           // first pixel of the scene in the accumulation buffer
+    uint32_t root;               // BVH root of the scene
+    uint32_t light_off, light_cnt;
+    float cam_pos[3];
+    float fwd[3], right[3], up[3];
+    float thx, thy;
+    float* out;                  // output image, width*height*3 floats
+};
+
+// Paths [base, base + count * npix) of a pass: samples sample0 .. sample0 +
+// count - 1 of one scene, sample-major; pixels [pixbase, pixbase + npix) of
+// the pass's accumulation grid.
+struct Segment {
+    uint32_t scene, sample0, count, base, pixbase, pad[3];
+};
+
 struct RenderParams {
     uint64_t seed;
-    uint32_t width, height, npix;
-    uint32_t sample_base;        // sample index of path slot group 0 in this pass
     uint32_t capacity;           // paths in this pass
+    uint32_t npixels;            // pixels of this pass's segments
     uint32_t max_depth;
     uint32_t rr_min_depth;
     uint32_t nee;
@@ -363,13 +384,24 @@ struct RenderParams {
     float alpha;                 // guide/BSDF mixture probability of this frame
     float env[3];
     float env_c[3], env_scale;   // sigmoid coefficients of the constant sky (spectral modes)
-    float cam_pos[3];
-    float fwd[3], right[3], up[3];
-    float thx, thy;
+    const SceneView* views;
+    const Segment* segs;
+    uint32_t nseg;
 };
+
+// segment of path p (segments sorted by base)
+__device__ __forceinline__ uint32_t segment_of(const Segment* segs, uint32_t nseg, uint32_t p) {
+    uint32_t lo = 0, hi = nseg - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (segs[mid].base <= p) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
 
 struct QueueRays {
     float *ox, *oy, *oz, *tmin, *dx, *dy, *dz, *tmax;
+    uint32_t* root;              // BVH root of the ray's scene
 };
 
 struct SceneArgs {
@@ -379,7 +411,6 @@ struct SceneArgs {
     const float* __restrict__ light_area;
     const float* __restrict__ light_prob;
     const float* __restrict__ light_cdf;
-    uint32_t nlights;
     const float* __restrict__ cie;
     const float* __restrict__ d65;
 };
@@ -391,6 +422,9 @@ struct PathArgs {
     float* last_pdf;
     uint32_t* meta;
     float* lambda0;
+    uint32_t* pix;               // pixel of the path
+    uint32_t* smp;               // sample index of the path
+    uint32_t* scn;               // scene of the path
 };
 
 // Voxel guiding inputs of the integrator: the probes (read-only during a
@@ -509,31 +543,38 @@ __device__ __forceinline__ S4 load4(const float* a, const float* b, const float*
 // RenderCamera's ray generation (core/scene.cu:135-164) for every path slot
 // of the pass (slot p = s_local * npix + pixel), plus the path's first
 // wavelength in spectral modes (render.cuh:466-474).
-__global__ __launch_bounds__(kBlock) void k_camera(RenderParams P, PathArgs S, GuideArgs G, QueueRays q,
+__global__ __launch_bounds__(kMaxBlock) void k_camera(RenderParams P, PathArgs S, GuideArgs G, QueueRays q,
                                                    uint32_t* __restrict__ qpath, uint32_t* __restrict__ counters,
                                                    const float* __restrict__ eps_table) {
-    const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p == 0) { counters[0] = P.capacity; counters[1] = 0; counters[2] = 0; }
     if (p >= P.capacity) return;
-    const uint32_t pix = p % P.npix, sl = p / P.npix;
-    const uint32_t i = pix % P.width, j = pix / P.width;
+    const Segment sg = P.segs[segment_of(P.segs, P.nseg, p)];
+    const SceneView& V = P.views[sg.scene];
+    const uint32_t local = p - sg.base;
+    const uint32_t pix = local % V.npix, smp = sg.sample0 + local / V.npix;
+    const uint32_t i = pix % V.width, j = pix / V.width;
     float u[4];
-    uniforms(P.seed, pix, P.sample_base + sl, kCameraBounce, 0, u);
-    const float uu = ((float)i + u[0]) / (float)P.width;
-    const float vv = ((float)j + u[1]) / (float)P.height;
+    uniforms(P.seed, pix, smp, kCameraBounce, 0, u);
+    const float uu = ((float)i + u[0]) / (float)V.width;
+    const float vv = ((float)j + u[1]) / (float)V.height;
     const float ndx = uu * 2.0f - 1.0f, ndy = (1.0f - vv) * 2.0f - 1.0f;
-    const F3 fwd = f3(P.fwd[0], P.fwd[1], P.fwd[2]);
-    const F3 right = f3(P.right[0], P.right[1], P.right[2]);
-    const F3 up = f3(P.up[0], P.up[1], P.up[2]);
-    F3 d = normalize(add(add(fwd, scale(right, ndx * P.thx)), scale(up, ndy * P.thy)));
+    const F3 fwd = f3(V.fwd[0], V.fwd[1], V.fwd[2]);
+    const F3 right = f3(V.right[0], V.right[1], V.right[2]);
+    const F3 up = f3(V.up[0], V.up[1], V.up[2]);
+    F3 d = normalize(add(add(fwd, scale(right, ndx * V.thx)), scale(up, ndy * V.thy)));
     d = normalize(d);   // CalculateRadiance normalises the incoming direction again
     float l0 = 0.0f;
     if (P.spectral != AR_SPECTRAL_RGB) {
         float w[4];
-        uniforms(P.seed, pix, P.sample_base + sl, kCameraBounce, 1, w);
+        uniforms(P.seed, pix, smp, kCameraBounce, 1, w);
         l0 = P.spectral == AR_SPECTRAL_SINGLE ? spec::sample_lambda(w[0], w[1], w[2]) : 380.0f + 400.0f * w[0];
     }
-    q.ox[p] = P.cam_pos[0]; q.oy[p] = P.cam_pos[1]; q.oz[p] = P.cam_pos[2];
+    S.pix[p] = pix;
+    S.smp[p] = smp;
+    S.scn[p] = sg.scene;
+    q.root[p] = V.root;
+    q.ox[p] = V.cam_pos[0]; q.oy[p] = V.cam_pos[1]; q.oz[p] = V.cam_pos[2];
     q.tmin[p] = eps_table[0];
     q.dx[p] = d.x; q.dy[p] = d.y; q.dz[p] = d.z;
     q.tmax[p] = __int_as_float(0x7f800000);
@@ -558,13 +599,13 @@ __global__ __launch_bounds__(kBlock) void k_camera(RenderParams P, PathArgs S, G
 // Network inputs of every queued hit (the learned guide's inference batch):
 // the hit point and the shading normal facing the ray, as k_shade computes
 // them.
-__global__ __launch_bounds__(kBlock) void k_mlp_features(const float4* __restrict__ prims, QueueRays cur,
+__global__ __launch_bounds__(kMaxBlock) void k_mlp_features(const float4* __restrict__ prims, QueueRays cur,
                                                          const uint32_t* __restrict__ hprim,
                                                          const float* __restrict__ hu, const float* __restrict__ hv,
                                                          const uint32_t* __restrict__ counters, float lo0, float lo1,
                                                          float lo2, float inv0, float inv1, float inv2,
                                                          float* __restrict__ feat) {
-    const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= counters[0]) return;
     float x[5] = {0, 0, 0, 0, 0};
     const uint32_t prim = hprim[q];
@@ -589,7 +630,7 @@ __global__ __launch_bounds__(kBlock) void k_mlp_features(const float4* __restric
 // (render.cuh:75-163), NEE light sampling (render.cuh:205-343, 766-827;
 // the shadow segment is queued and resolved after the any-hit pass), Russian
 // roulette (render.cuh:838-859) and the next ray.
-__global__ __launch_bounds__(kBlock) void k_shade(
+__global__ __launch_bounds__(kMaxBlock) void k_shade(
     RenderParams P, uint32_t depth, SceneArgs sc, PathArgs S, GuideArgs G,
     QueueRays cur, const uint32_t* __restrict__ qcur,
     const uint32_t* __restrict__ hprim, const float* __restrict__ ht, const float* __restrict__ hu,
@@ -598,10 +639,10 @@ __global__ __launch_bounds__(kBlock) void k_shade(
     QueueRays sh, uint32_t* __restrict__ spath, float* __restrict__ sc0, float* __restrict__ sc1,
     float* __restrict__ sc2, float* __restrict__ sc3, uint32_t* __restrict__ sflags,
     uint32_t* __restrict__ counters, const float* __restrict__ eps_table) {
-    const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t n = counters[0];
     bool cont = false, shadow = false;
-    uint32_t p = 0;
+    uint32_t p = 0, root = 0;
     F3 next_o = f3(0, 0, 0), next_d = f3(0, 0, 0);
     float next_tmin = 0.0f;
     F3 s_o = f3(0, 0, 0), s_d = f3(0, 0, 1);
@@ -610,8 +651,9 @@ __global__ __launch_bounds__(kBlock) void k_shade(
     uint32_t s_flag = 0;
     if (q < n) {
         p = qcur[q];
-        const uint32_t pix = p % P.npix, sl = p / P.npix;
-        const uint32_t smp = P.sample_base + sl;
+        const uint32_t pix = S.pix[p], smp = S.smp[p];
+        const SceneView& V = P.views[S.scn[p]];
+        root = V.root;
         float u[12];
         uniforms(P.seed, pix, smp, depth, 0, u);
         uniforms(P.seed, pix, smp, depth, 1, u + 4);
@@ -761,10 +803,10 @@ __global__ __launch_bounds__(kBlock) void k_shade(
                         }
                     }
                     bool nee_now = false;
-                    if (P.nee && sc.nlights > 0 && depth + 1 < P.max_depth && non_delta < 10 &&
+                    if (P.nee && V.light_cnt > 0 && depth + 1 < P.max_depth && non_delta < 10 &&
                         u[5] < sqrtf(clampr(lum4(mask, P.spectral), 0.0f, 1.0f))) {
-                        // pick by the CDF (first entry with u < cdf; binary search)
-                        uint32_t lo = 0, hi = sc.nlights - 1;
+                        // pick by the scene's CDF (first entry with u < cdf; binary search)
+                        uint32_t lo = V.light_off, hi = V.light_off + V.light_cnt - 1;
                         while (lo < hi) {
                             const uint32_t mid = (lo + hi) >> 1;
                             if (u[6] < sc.light_cdf[mid]) hi = mid; else lo = mid + 1;
@@ -856,6 +898,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(
         nxt.tmin[slot] = next_tmin;
         nxt.dx[slot] = next_d.x; nxt.dy[slot] = next_d.y; nxt.dz[slot] = next_d.z;
         nxt.tmax[slot] = __int_as_float(0x7f800000);
+        nxt.root[slot] = root;
         qnext[slot] = p;
     }
     const uint32_t ss = wave_append(shadow, &counters[2]);
@@ -863,6 +906,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(
         sh.ox[ss] = s_o.x; sh.oy[ss] = s_o.y; sh.oz[ss] = s_o.z; sh.tmin[ss] = 0.0f;
         sh.dx[ss] = s_d.x; sh.dy[ss] = s_d.y; sh.dz[ss] = s_d.z; sh.tmax[ss] = s_tmax;
         spath[ss] = p;
+        sh.root[ss] = root;
         sc0[ss] = s_contrib.v[0]; sc1[ss] = s_contrib.v[1]; sc2[ss] = s_contrib.v[2]; sc3[ss] = s_contrib.v[3];
         sflags[ss] = s_flag;
     }
@@ -871,13 +915,13 @@ __global__ __launch_bounds__(kBlock) void k_shade(
 // Shadow results: an occluded segment still counts as a taken NEE sample
 // (render.cuh:297-305); an unoccluded one adds its MIS-weighted
 // contribution when the light point faced the surface.
-__global__ __launch_bounds__(kBlock) void k_resolve(PathArgs S, const uint32_t* __restrict__ spath,
+__global__ __launch_bounds__(kMaxBlock) void k_resolve(PathArgs S, const uint32_t* __restrict__ spath,
                                                     const float* __restrict__ sc0, const float* __restrict__ sc1,
                                                     const float* __restrict__ sc2, const float* __restrict__ sc3,
                                                     const uint32_t* __restrict__ sflags,
                                                     const uint32_t* __restrict__ occ,
                                                     const uint32_t* __restrict__ counters) {
-    const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= counters[2]) return;
     const uint32_t p = spath[s];
     const bool occluded = (occ[s >> 5] >> (s & 31u)) & 1u;
@@ -929,21 +973,33 @@ __device__ __forceinline__ F3 path_value(const RenderParams& P, const SceneArgs&
     return f3(clampr(X, 0.0f, 1e3f), clampr(Y, 0.0f, 1e3f), clampr(Z, 0.0f, 1e3f));
 }
 
-// Per-pixel sum over the pass's samples in sample order, then the mean
+// Per-pixel sum over the pass's samples of each scene in sample order, onto
+// the scene's running sum; the scene's last pass writes the mean
 // (XYZ2SRGBLinearD65, core/tonemap.cuh:116-126, for sRGB output).
-__global__ __launch_bounds__(kBlock) void k_accumulate(RenderParams P, SceneArgs sc, PathArgs S,
-                                                       float* __restrict__ accum, uint32_t spp_pass, bool first,
-                                                       bool last, float spp_total, float* __restrict__ out) {
-    const uint32_t pix = blockIdx.x * kBlock + threadIdx.x;
-    if (pix >= P.npix) return;
-    float ax = first ? 0.0f : accum[3 * pix + 0];
-    float ay = first ? 0.0f : accum[3 * pix + 1];
-    float az = first ? 0.0f : accum[3 * pix + 2];
-    for (uint32_t s = 0; s < spp_pass; ++s) {
-        const F3 r = path_value(P, sc, S, s * P.npix + pix);
+__global__ __launch_bounds__(kMaxBlock) void k_accumulate(RenderParams P, SceneArgs sc, PathArgs S,
+                                                          float* __restrict__ accum) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= P.npixels) return;
+    uint32_t lo = 0, hi = P.nseg - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (P.segs[mid].pixbase <= t) lo = mid; else hi = mid - 1;
+    }
+    const Segment sg = P.segs[lo];
+    const SceneView& V = P.views[sg.scene];
+    const uint32_t pix = t - sg.pixbase;
+    const bool first = sg.sample0 == V.sample_offset;
+    const bool last = sg.sample0 + sg.count == V.sample_offset + V.spp;
+    float* acc = accum + 3 * ((size_t)V.out_base + pix);
+    float ax = first ? 0.0f : acc[0];
+    float ay = first ? 0.0f : acc[1];
+    float az = first ? 0.0f : acc[2];
+    for (uint32_t s = 0; s < sg.count; ++s) {
+        const F3 r = path_value(P, sc, S, sg.base + s * V.npix + pix);
         ax = ax + r.x; ay = ay + r.y; az = az + r.z;
     }
     if (last) {
+        const float spp_total = (float)V.spp;
         float x = ax / spp_total, y = ay / spp_total, z = az / spp_total;
         if (P.spectral != AR_SPECTRAL_RGB && P.output == AR_OUTPUT_SRGB) {
             const float r = 3.2406f * x + -1.5372f * y + -0.4986f * z;
@@ -951,15 +1007,15 @@ __global__ __launch_bounds__(kBlock) void k_accumulate(RenderParams P, SceneArgs
             const float b = 0.0557f * x + -0.204f * y + 1.057f * z;
             x = maxr(0.0f, r); y = maxr(0.0f, g); z = maxr(0.0f, b);
         }
-        out[3 * pix + 0] = x;
-        out[3 * pix + 1] = y;
-        out[3 * pix + 2] = z;
+        V.out[3 * pix + 0] = x;
+        V.out[3 * pix + 1] = y;
+        V.out[3 * pix + 2] = z;
     } else {
-        accum[3 * pix + 0] = ax; accum[3 * pix + 1] = ay; accum[3 * pix + 2] = az;
+        acc[0] = ax; acc[1] = ay; acc[2] = az;
     }
 }
 
-uint32_t grid_for(size_t n) { return (uint32_t)((n + kBlock - 1) / kBlock); }
+uint32_t grid_for(size_t n, uint32_t block = kBlock) { return (uint32_t)((n + block - 1) / block); }
 
 } // namespace
 
@@ -1087,6 +1143,7 @@ static int ensure_render_buffers(ar_ctx* ctx, size_t cap, size_t npix) {
         if (rc != AR_OK) return rc;
     }
     if (b.accum_pixels < npix) {
+        ctx->buffer_generation += 1;
         if (b.accum) (void)hipFree(b.accum);
         b.accum = nullptr;
         int rc = check_hip(ctx, hipMalloc((void**)&b.accum, npix * 3 * sizeof(float)), "hipMalloc(accum)");
@@ -1098,8 +1155,10 @@ static int ensure_render_buffers(ar_ctx* ctx, size_t cap, size_t npix) {
     b.block = nullptr;
     b.capacity = 0;
     // 4-byte arrays of cap entries: 2x8 queue rays, 2 qpath, 5 hits, 8 shadow rays,
-    // spath, 4 scontrib, sflags, 4 mask, 4 rad, 3 last_pos, last_pdf, meta, lambda0 = 51
-    const size_t arrays = 51;
+    // spath, 4 scontrib, sflags, 4 mask, 4 rad, 3 last_pos, last_pdf, meta, lambda0,
+    // pix, smp, scn, 2 qroot, sroot = 57
+    const size_t arrays = 57;
+    ctx->buffer_generation += 1;
     const size_t occ_words = (cap + 31) / 32 + 64;
     const size_t bytes = arrays * cap * 4 + occ_words * 4 + 64;
     int rc = check_hip(ctx, hipMalloc(&b.block, bytes), "hipMalloc(render state)");
@@ -1125,6 +1184,12 @@ static int ensure_render_buffers(ar_ctx* ctx, size_t cap, size_t npix) {
     b.last_pdf = (float*)take(cap);
     b.meta = (uint32_t*)take(cap);
     b.lambda0 = (float*)take(cap);
+    b.pix = (uint32_t*)take(cap);
+    b.smp = (uint32_t*)take(cap);
+    b.scn = (uint32_t*)take(cap);
+    b.qroot[0] = (uint32_t*)take(cap);
+    b.qroot[1] = (uint32_t*)take(cap);
+    b.sroot = (uint32_t*)take(cap);
     b.occ = (uint32_t*)take(occ_words);
     b.counters = (uint32_t*)take(16);
     b.capacity = cap;
@@ -1154,159 +1219,359 @@ static int prepare_spectral_materials(ar_ctx* ctx) {
     return rc;
 }
 
-int render(ar_ctx* ctx, const ar_camera* cam, const ar_render_opts* opts, float* out, hipStream_t stream) {
-    (void)hipSetDevice(ctx->device);
-    const size_t npix = (size_t)opts->width * opts->height;
-    const size_t max_paths = opts->max_paths_per_pass ? opts->max_paths_per_pass : ((size_t)1 << 22);
-    if (npix > max_paths && npix > ((size_t)1 << 26))
-        return set_error(ctx, AR_E_CAPACITY, "image larger than 2^26 pixels");
-    if (opts->guiding != AR_GUIDING_OFF && opts->guiding != AR_GUIDING_VOXEL && opts->guiding != AR_GUIDING_MLP)
+// ------------------------------------------------------------ render plans
+//
+// A render (one scene, or a batch of scenes merged into one context) is
+// planned on the host as passes of at most `cap` paths; each pass is a list
+// of segments, each a run of whole samples of one scene.  Scenes are packed
+// in order, so every scene's samples are accumulated in sample order and a
+// batched image equals the scene's own render bit for bit.
+
+struct RenderPlan {
+    std::vector<SceneView> views;
+    std::vector<Segment> segs;       // all passes, concatenated
+    struct Pass { uint32_t seg0, nseg, paths, pixels, sample_base; };
+    std::vector<Pass> passes;
+    size_t cap = 0;                  // largest pass (paths)
+    size_t pixels = 0;               // accumulation pixels (all scenes)
+    size_t pass_pixels = 0;          // largest pass (pixels)
+};
+
+struct RenderJob {
+    RenderPlan plan;
+    void* dev = nullptr;             // device copy of views + segments
+    size_t dev_bytes = 0;
+    void* host = nullptr;            // pinned staging copy (stream-ordered uploads)
+    size_t host_bytes = 0;
+    hipEvent_t uploaded = nullptr;   // the last upload from `host` has been read
+    RenderParams base{};
+    bool guiding = false, learned = false;
+};
+
+static void camera_basis(const ar_camera* cam, uint32_t width, uint32_t height, SceneView& v) {
+    // the reference computes the basis per pixel (core/scene.cu:154-164):
+    // FOV = vfov * 3.1415926 / 180
+    const float fx = cam->dir[0], fy = cam->dir[1], fz = cam->dir[2];
+    float rx = fy * 0.0f - fz * 1.0f, ry = fz * 0.0f - fx * 0.0f, rz = fx * 1.0f - fy * 0.0f;
+    float l2 = rx * rx + ry * ry + rz * rz;
+    float inv = l2 <= 1e-20f ? 0.0f : 1.0f / std::sqrt(l2);
+    rx *= inv; ry *= inv; rz *= inv;
+    if (l2 <= 1e-20f) { rx = ry = rz = 0.0f; }
+    float ux = fy * rz - fz * ry, uy = fz * rx - fx * rz, uz = fx * ry - fy * rx;
+    l2 = ux * ux + uy * uy + uz * uz;
+    inv = l2 <= 1e-20f ? 0.0f : 1.0f / std::sqrt(l2);
+    ux *= inv; uy *= inv; uz *= inv;
+    for (int k = 0; k < 3; ++k) { v.cam_pos[k] = cam->pos[k]; v.fwd[k] = cam->dir[k]; }
+    v.right[0] = rx; v.right[1] = ry; v.right[2] = rz;
+    v.up[0] = ux; v.up[1] = uy; v.up[2] = uz;
+    const float fov = cam->vfov_deg * 3.1415926f / 180.0f;
+    const float aspect = (float)width / std::max(1.0f, (float)height);
+    v.thy = std::tan(0.5f * fov);
+    v.thx = v.thy * aspect;
+}
+
+// shared settings must agree across a batch
+static bool same_settings(const ar_render_opts& a, const ar_render_opts& b) {
+    return a.seed == b.seed && a.max_depth == b.max_depth && a.rr_min_depth == b.rr_min_depth && a.nee == b.nee &&
+           a.sky == b.sky && a.env[0] == b.env[0] && a.env[1] == b.env[1] && a.env[2] == b.env[2] &&
+           a.spectral == b.spectral && a.output == b.output && a.guiding == b.guiding &&
+           a.max_paths_per_pass == b.max_paths_per_pass;
+}
+
+static int make_job(ar_ctx* ctx, uint32_t n, const ar_camera* cams, const ar_render_opts* opts, float* const* outs,
+                    RenderJob& J) {
+    const ar_render_opts& o = opts[0];
+    if (o.guiding != AR_GUIDING_OFF && o.guiding != AR_GUIDING_VOXEL && o.guiding != AR_GUIDING_MLP)
         return set_error(ctx, AR_E_INVALID, "guiding must be AR_GUIDING_OFF, _VOXEL or _MLP");
-    const bool guiding = opts->guiding != AR_GUIDING_OFF;
-    const bool learned = opts->guiding == AR_GUIDING_MLP;
-    // guiding renders frames of one sample per pixel with a probe update
-    // after each (the reference's Render + UpdatePathGuidingSample loop)
-    const uint32_t spp_pass = guiding ? 1u
-        : (uint32_t)std::max<size_t>(1, std::min<size_t>(opts->spp, max_paths / std::max<size_t>(npix, 1)));
-    const size_t cap = npix * spp_pass;
-    int rc = ensure_render_buffers(ctx, cap, npix);
+    if (o.spectral != AR_SPECTRAL_RGB && o.spectral != AR_SPECTRAL_SINGLE && o.spectral != AR_SPECTRAL_HERO4)
+        return set_error(ctx, AR_E_INVALID, "spectral must be AR_SPECTRAL_RGB, _SINGLE or _HERO4");
+    if (o.spectral != AR_SPECTRAL_RGB && o.sky == AR_SKY_GRADIENT)
+        return set_error(ctx, AR_E_UNSUPPORTED, "the gradient sky is RGB-only; use AR_SKY_CONSTANT in spectral modes");
+    J.guiding = o.guiding != AR_GUIDING_OFF;
+    J.learned = o.guiding == AR_GUIDING_MLP;
+    if (J.guiding && n != 1) return set_error(ctx, AR_E_UNSUPPORTED, "guiding renders one scene at a time");
+    const size_t max_paths = o.max_paths_per_pass ? o.max_paths_per_pass : ((size_t)1 << 22);
+    RenderPlan& L = J.plan;
+    L = RenderPlan{};
+    size_t max_npix = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        const ar_render_opts& oi = opts[i];
+        if (i > 0 && !same_settings(o, oi))
+            return set_error(ctx, AR_E_INVALID, "batched scenes must share seed, depth, NEE, sky, spectral and guiding settings");
+        if (oi.width == 0 || oi.height == 0 || oi.spp == 0) return set_error(ctx, AR_E_INVALID, "empty image or spp = 0");
+        if (oi.max_depth == 0 || oi.max_depth > 1000) return set_error(ctx, AR_E_INVALID, "max_depth must be in [1, 1000]");
+        const size_t npix = (size_t)oi.width * oi.height;
+        if (npix > ((size_t)1 << 26)) return set_error(ctx, AR_E_CAPACITY, "image larger than 2^26 pixels");
+        if ((uint64_t)oi.sample_offset + oi.spp > 0xFFFFFFFFull) return set_error(ctx, AR_E_INVALID, "sample index overflow");
+        if (!outs[i]) return set_error(ctx, AR_E_INVALID, "null output image");
+        SceneView v{};
+        v.width = oi.width; v.height = oi.height; v.npix = (uint32_t)npix; v.spp = oi.spp;
+        v.sample_offset = oi.sample_offset;
+        v.out_base = (uint32_t)L.pixels;
+        if (ctx->batch.empty()) {
+            v.root = ctx->bvh.root; v.light_off = 0; v.light_cnt = ctx->scene.nlights;
+        } else {
+            v.root = ctx->batch[i].root; v.light_off = ctx->batch[i].light_off; v.light_cnt = ctx->batch[i].light_cnt;
+        }
+        camera_basis(&cams[i], oi.width, oi.height, v);
+        v.out = outs[i];
+        L.views.push_back(v);
+        L.pixels += npix;
+        max_npix = std::max(max_npix, npix);
+        if (L.pixels > 0xFFFFFFFFull) return set_error(ctx, AR_E_CAPACITY, "batch larger than 2^32 pixels");
+    }
+    // guiding: frames of one sample per pixel with a guide update after each
+    const size_t cap = J.guiding ? max_npix : std::max(max_paths, max_npix);
+    RenderPlan::Pass cur{0, 0, 0, 0, 0};
+    auto close = [&]() {
+        if (cur.nseg) {
+            L.passes.push_back(cur);
+            L.cap = std::max<size_t>(L.cap, cur.paths);
+            L.pass_pixels = std::max<size_t>(L.pass_pixels, cur.pixels);
+        }
+        cur = RenderPlan::Pass{(uint32_t)L.segs.size(), 0, 0, 0, 0};
+    };
+    for (uint32_t i = 0; i < n; ++i) {
+        const SceneView& v = L.views[i];
+        uint32_t done = 0;
+        while (done < v.spp) {
+            size_t fit = (cap - cur.paths) / v.npix;
+            if (J.guiding) fit = std::min<size_t>(fit, 1);
+            if (fit == 0 || (J.guiding && cur.nseg)) { close(); continue; }
+            const uint32_t k = (uint32_t)std::min<size_t>(fit, v.spp - done);
+            Segment sg{};
+            sg.scene = i; sg.sample0 = v.sample_offset + done; sg.count = k;
+            sg.base = cur.paths; sg.pixbase = cur.pixels;
+            if (cur.nseg == 0) cur.sample_base = sg.sample0;
+            L.segs.push_back(sg);
+            cur.nseg += 1;
+            cur.paths += k * v.npix;
+            cur.pixels += v.npix;
+            done += k;
+        }
+    }
+    close();
+    RenderParams& P = J.base;
+    P = RenderParams{};
+    P.seed = o.seed;
+    P.max_depth = o.max_depth;
+    P.rr_min_depth = o.rr_min_depth;
+    P.nee = o.nee;
+    P.sky = o.sky;
+    P.spectral = o.spectral;
+    P.output = o.output;
+    P.guiding = o.guiding;
+    float e[3] = {o.env[0], o.env[1], o.env[2]};
+    for (int k = 0; k < 3; ++k) P.env[k] = o.env[k];
+    const float mx = std::max(e[0], std::max(e[1], e[2]));
+    P.env_scale = 1.0f;
+    if (mx > 1.0f) { for (float& x : e) x /= mx; P.env_scale = mx; }
+    if (o.spectral != AR_SPECTRAL_RGB) {
+        const std::vector<float>& lut = spec::uplift_lut(spec::cie_xyz_table(), spec::d65_table());
+        spec::uplift(lut.data(), e, P.env_c);
+    }
+    return AR_OK;
+}
+
+// allocations and uploads; nothing here may be captured into a graph.
+// stream == nullptr with sync: synchronous upload (graphs); else the plan is
+// copied on `stream`, ordered after the previous render on that stream.
+static int prepare_job(ar_ctx* ctx, RenderJob& J, hipStream_t stream, bool sync) {
+    (void)hipSetDevice(ctx->device);
+    const RenderPlan& L = J.plan;
+    int rc = ensure_render_buffers(ctx, L.cap, L.pixels);
     if (rc != AR_OK) return rc;
-    if (guiding) {
-        rc = guide_ensure(ctx, cap, !learned);
+    if (J.guiding) {
+        rc = guide_ensure(ctx, L.cap, !J.learned);
         if (rc != AR_OK) return rc;
     }
-    if (learned) {
+    if (J.learned) {
         rc = mlp_ensure(ctx);
-        if (rc == AR_OK) rc = mlp_ensure_buffers(ctx, cap);
+        if (rc == AR_OK) rc = mlp_ensure_buffers(ctx, L.cap);
+        if (rc != AR_OK) return rc;
+    }
+    if (J.base.spectral != AR_SPECTRAL_RGB) {
+        rc = prepare_spectral_materials(ctx);
         if (rc != AR_OK) return rc;
     }
     TraceArgs ta;
     uint32_t tblocks;
-    rc = trace_prologue(ctx, ta, cap, tblocks);
+    rc = trace_prologue(ctx, ta, L.cap, tblocks);
     if (rc != AR_OK) return rc;
-
-    if (opts->spectral != AR_SPECTRAL_RGB && opts->spectral != AR_SPECTRAL_SINGLE && opts->spectral != AR_SPECTRAL_HERO4)
-        return set_error(ctx, AR_E_INVALID, "spectral must be AR_SPECTRAL_RGB, _SINGLE or _HERO4");
-    if (opts->spectral != AR_SPECTRAL_RGB && opts->sky == AR_SKY_GRADIENT)
-        return set_error(ctx, AR_E_UNSUPPORTED, "the gradient sky is RGB-only; use AR_SKY_CONSTANT in spectral modes");
-    if (opts->spectral != AR_SPECTRAL_RGB) {
-        rc = prepare_spectral_materials(ctx);
+    const size_t vb = L.views.size() * sizeof(SceneView);
+    const size_t bytes = vb + L.segs.size() * sizeof(Segment);
+    if (J.dev_bytes < bytes) {
+        if (J.dev) (void)hipFree(J.dev);
+        J.dev = nullptr;
+        J.dev_bytes = 0;
+        rc = check_hip(ctx, hipMalloc(&J.dev, bytes), "hipMalloc(render plan)");
         if (rc != AR_OK) return rc;
+        J.dev_bytes = bytes;
     }
-    RenderParams P{};
-    P.seed = opts->seed;
-    P.spectral = opts->spectral;
-    P.output = opts->output;
-    P.guiding = opts->guiding;
-    P.alpha = 0.0f;
-    {
-        float e[3] = {opts->env[0], opts->env[1], opts->env[2]};
-        const float mx = std::max(e[0], std::max(e[1], e[2]));
-        P.env_scale = 1.0f;
-        if (mx > 1.0f) { for (float& x : e) x /= mx; P.env_scale = mx; }
-        if (opts->spectral != AR_SPECTRAL_RGB) {
-            const std::vector<float>& lut = spec::uplift_lut(spec::cie_xyz_table(), spec::d65_table());
-            spec::uplift(lut.data(), e, P.env_c);
-        }
+    if (sync) {
+        rc = check_hip(ctx, hipMemcpy(J.dev, L.views.data(), vb, hipMemcpyHostToDevice), "upload render plan");
+        if (rc == AR_OK)
+            rc = check_hip(ctx, hipMemcpy((char*)J.dev + vb, L.segs.data(), L.segs.size() * sizeof(Segment),
+                                          hipMemcpyHostToDevice), "upload render plan");
+        return rc;
     }
-    P.width = opts->width;
-    P.height = opts->height;
-    P.npix = (uint32_t)npix;
-    P.max_depth = opts->max_depth;
-    P.rr_min_depth = opts->rr_min_depth;
-    P.nee = opts->nee;
-    P.sky = opts->sky;
-    for (int k = 0; k < 3; ++k) { P.env[k] = opts->env[k]; P.cam_pos[k] = cam->pos[k]; P.fwd[k] = cam->dir[k]; }
-    // camera basis on the host, as the reference computes it per pixel
-    // (core/scene.cu:154-164): FOV = vfov * 3.1415926 / 180
-    {
-        const float fx = cam->dir[0], fy = cam->dir[1], fz = cam->dir[2];
-        float rx = fy * 0.0f - fz * 1.0f, ry = fz * 0.0f - fx * 0.0f, rz = fx * 1.0f - fy * 0.0f;
-        float l2 = rx * rx + ry * ry + rz * rz;
-        float inv = l2 <= 1e-20f ? 0.0f : 1.0f / std::sqrt(l2);
-        rx *= inv; ry *= inv; rz *= inv;
-        if (l2 <= 1e-20f) { rx = ry = rz = 0.0f; }
-        float ux = fy * rz - fz * ry, uy = fz * rx - fx * rz, uz = fx * ry - fy * rx;
-        l2 = ux * ux + uy * uy + uz * uz;
-        inv = l2 <= 1e-20f ? 0.0f : 1.0f / std::sqrt(l2);
-        ux *= inv; uy *= inv; uz *= inv;
-        P.right[0] = rx; P.right[1] = ry; P.right[2] = rz;
-        P.up[0] = ux; P.up[1] = uy; P.up[2] = uz;
-        const float fov = cam->vfov_deg * 3.1415926f / 180.0f;
-        const float aspect = (float)opts->width / std::max(1.0f, (float)opts->height);
-        P.thy = std::tan(0.5f * fov);
-        P.thx = P.thy * aspect;
+    if (!J.uploaded) {
+        rc = check_hip(ctx, hipEventCreateWithFlags(&J.uploaded, hipEventDisableTiming), "hipEventCreate");
+        if (rc != AR_OK) return rc;
+    } else {
+        (void)hipEventSynchronize(J.uploaded);
     }
+    if (J.host_bytes < bytes) {
+        if (J.host) (void)hipHostFree(J.host);
+        J.host = nullptr;
+        J.host_bytes = 0;
+        rc = check_hip(ctx, hipHostMalloc(&J.host, bytes, hipHostMallocDefault), "hipHostMalloc(render plan)");
+        if (rc != AR_OK) return rc;
+        J.host_bytes = bytes;
+    }
+    std::memcpy(J.host, L.views.data(), vb);
+    std::memcpy((char*)J.host + vb, L.segs.data(), L.segs.size() * sizeof(Segment));
+    rc = check_hip(ctx, hipMemcpyAsync(J.dev, J.host, bytes, hipMemcpyHostToDevice, stream), "upload render plan");
+    if (rc == AR_OK) rc = check_hip(ctx, hipEventRecord(J.uploaded, stream), "hipEventRecord");
+    return rc;
+}
+
+// kernel launches only (graph-capturable)
+static int enqueue_job(ar_ctx* ctx, RenderJob& J, hipStream_t stream) {
+    const RenderPlan& L = J.plan;
+    TraceArgs ta;
+    uint32_t tblocks;
+    int rc = trace_prologue(ctx, ta, L.cap, tblocks);
+    if (rc != AR_OK) return rc;
     RenderBuffers& b = ctx->rb;
+    const uint32_t blk = ctx->tile;
     PathArgs S{b.mask[0], b.mask[1], b.mask[2], b.mask[3], b.rad[0], b.rad[1], b.rad[2], b.rad[3],
-               b.last_pos[0], b.last_pos[1], b.last_pos[2], b.last_pdf, b.meta, b.lambda0};
+               b.last_pos[0], b.last_pos[1], b.last_pos[2], b.last_pdf, b.meta, b.lambda0, b.pix, b.smp, b.scn};
     SceneArgs sc{ctx->scene.prims, ctx->scene.mats, ctx->scene.light_tri, ctx->scene.light_area,
-                 ctx->scene.light_prob, ctx->scene.light_cdf, ctx->scene.nlights, b.cie, b.d65};
-    QueueRays q[2] = {{b.rays[0][0], b.rays[0][1], b.rays[0][2], b.rays[0][3], b.rays[0][4], b.rays[0][5], b.rays[0][6], b.rays[0][7]},
-                      {b.rays[1][0], b.rays[1][1], b.rays[1][2], b.rays[1][3], b.rays[1][4], b.rays[1][5], b.rays[1][6], b.rays[1][7]}};
-    QueueRays shq{b.srays[0], b.srays[1], b.srays[2], b.srays[3], b.srays[4], b.srays[5], b.srays[6], b.srays[7]};
+                 ctx->scene.light_prob, ctx->scene.light_cdf, b.cie, b.d65};
+    QueueRays q[2] = {{b.rays[0][0], b.rays[0][1], b.rays[0][2], b.rays[0][3], b.rays[0][4], b.rays[0][5], b.rays[0][6],
+                       b.rays[0][7], b.qroot[0]},
+                      {b.rays[1][0], b.rays[1][1], b.rays[1][2], b.rays[1][3], b.rays[1][4], b.rays[1][5], b.rays[1][6],
+                       b.rays[1][7], b.qroot[1]}};
+    QueueRays shq{b.srays[0], b.srays[1], b.srays[2], b.srays[3], b.srays[4], b.srays[5], b.srays[6], b.srays[7],
+                  b.sroot};
     ar_hits hits{b.hit_prim, b.hit_t, b.hit_u, b.hit_v, b.hit_flags};
     GuideDev& gd = ctx->guide;
     GuideArgs G{};
-    if (guiding) {
+    if (J.guiding) {
         G.pr = guide::Probes{gd.tkey, gd.tval, gd.tcap ? gd.tcap - 1 : 0u, gd.tex, 0u};
         G.probe_count = gd.counters;
+        G.gm = gd.gm; G.gr = gd.gr; G.gkey = gd.gkey; G.gleaf = gd.gleaf; G.gflag = gd.gflag;
+        G.stride = (uint32_t)gd.pcap;
         G.dec = ctx->mlp.dec;
         G.gx = gd.gx; G.gw = gd.gw; G.gp = gd.gp;
         for (int k = 0; k < 3; ++k) { G.box_lo[k] = ctx->scene.box_lo[k]; G.box_inv[k] = ctx->scene.box_inv[k]; }
-        G.gm = gd.gm; G.gr = gd.gr; G.gkey = gd.gkey; G.gleaf = gd.gleaf; G.gflag = gd.gflag;
-        G.stride = (uint32_t)gd.pcap;
     }
-    const float spp_total = (float)opts->spp;
-    for (uint32_t done = 0; done < opts->spp;) {
-        const uint32_t this_spp = std::min(spp_pass, opts->spp - done);
-        P.sample_base = opts->sample_offset + done;
-        P.capacity = (uint32_t)(npix * this_spp);
-        if (guiding) {
+    RenderParams P = J.base;
+    P.views = (const SceneView*)J.dev;
+    const Segment* segs = (const Segment*)((const char*)J.dev + L.views.size() * sizeof(SceneView));
+    const uint32_t max_depth = P.max_depth;
+    for (const RenderPlan::Pass& pass : L.passes) {
+        P.segs = segs + pass.seg0;
+        P.nseg = pass.nseg;
+        P.capacity = pass.paths;
+        P.npixels = pass.pixels;
+        if (J.guiding) {
             // GetPathGuidingProb (render.cuh:195-200): lerp(0, 0.5, min(1, frame/64)^0.5)
-            const float t = std::pow(std::fmin(1.0f, (float)P.sample_base / 64.0f), 0.5f);
+            const float t = std::pow(std::fmin(1.0f, (float)pass.sample_base / 64.0f), 0.5f);
             P.alpha = std::fmin(std::fmax(0.0f * (1.0f - t) + 0.5f * t, 0.0f), 1.0f);
         }
-        hipLaunchKernelGGL(k_camera, dim3(grid_for(P.capacity)), dim3(kBlock), 0, stream, P, S, G, q[0], b.qpath[0],
+        hipLaunchKernelGGL(k_camera, dim3(grid_for(P.capacity, blk)), dim3(blk), 0, stream, P, S, G, q[0], b.qpath[0],
                            b.counters, b.eps_table);
         int c = 0;
-        for (uint32_t depth = 0; depth < opts->max_depth; ++depth) {
+        for (uint32_t depth = 0; depth < max_depth; ++depth) {
             ar_rays rr{q[c].ox, q[c].oy, q[c].oz, q[c].tmin, q[c].dx, q[c].dy, q[c].dz, q[c].tmax};
-            hipError_t e = launch_trace_closest(ta, rr, P.capacity, hits, nullptr, nullptr, tblocks, stream, &b.counters[0]);
+            TraceArgs tq = ta;
+            tq.roots = q[c].root;
+            hipError_t e = launch_trace_closest(tq, rr, P.capacity, hits, nullptr, nullptr, tblocks, stream, &b.counters[0]);
             if (e != hipSuccess) return check_hip(ctx, e, "integrator trace");
-            if (learned && P.alpha > 0.0f) {
+            if (J.learned && P.alpha > 0.0f) {
                 const SceneDev& sd = ctx->scene;
-                hipLaunchKernelGGL(k_mlp_features, dim3(grid_for(P.capacity)), dim3(kBlock), 0, stream, sd.prims, q[c],
-                                   b.hit_prim, b.hit_u, b.hit_v, b.counters, sd.box_lo[0], sd.box_lo[1], sd.box_lo[2],
-                                   sd.box_inv[0], sd.box_inv[1], sd.box_inv[2], ctx->mlp.feat);
+                hipLaunchKernelGGL(k_mlp_features, dim3(grid_for(P.capacity, blk)), dim3(blk), 0, stream, sd.prims,
+                                   q[c], b.hit_prim, b.hit_u, b.hit_v, b.counters, sd.box_lo[0], sd.box_lo[1],
+                                   sd.box_lo[2], sd.box_inv[0], sd.box_inv[1], sd.box_inv[2], ctx->mlp.feat);
                 e = mlp_forward(ctx, ctx->mlp.feat, P.capacity, &b.counters[0], ctx->mlp.dec, true, stream);
                 if (e != hipSuccess) return check_hip(ctx, e, "guiding network inference");
             }
-            hipLaunchKernelGGL(k_shade, dim3(grid_for(P.capacity)), dim3(kBlock), 0, stream, P, depth, sc, S, G, q[c],
+            hipLaunchKernelGGL(k_shade, dim3(grid_for(P.capacity, blk)), dim3(blk), 0, stream, P, depth, sc, S, G, q[c],
                                b.qpath[c], b.hit_prim, b.hit_t, b.hit_u, b.hit_v, q[1 - c], b.qpath[1 - c], shq,
                                b.spath, b.scontrib[0], b.scontrib[1], b.scontrib[2], b.scontrib[3], b.sflags,
                                b.counters, b.eps_table);
-            if (opts->nee) {
+            if (P.nee) {
                 ar_rays sr{shq.ox, shq.oy, shq.oz, shq.tmin, shq.dx, shq.dy, shq.dz, shq.tmax};
-                e = launch_trace_any(ta, sr, P.capacity, b.occ, tblocks, stream, &b.counters[2]);
+                TraceArgs ts = ta;
+                ts.roots = shq.root;
+                e = launch_trace_any(ts, sr, P.capacity, b.occ, tblocks, stream, &b.counters[2]);
                 if (e != hipSuccess) return check_hip(ctx, e, "integrator shadow trace");
-                hipLaunchKernelGGL(k_resolve, dim3(grid_for(P.capacity)), dim3(kBlock), 0, stream, S, b.spath,
+                hipLaunchKernelGGL(k_resolve, dim3(grid_for(P.capacity, blk)), dim3(blk), 0, stream, S, b.spath,
                                    b.scontrib[0], b.scontrib[1], b.scontrib[2], b.scontrib[3], b.sflags, b.occ,
                                    b.counters);
             }
             hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, stream, b.counters);
             c = 1 - c;
         }
-        if (guiding) {
-            rc = learned ? mlp_frame_update(ctx, P.spectral, b.lambda0, b.cie, P.capacity, stream)
-                         : guide_update(ctx, P.spectral, b.lambda0, b.cie, P.capacity, stream);
+        if (J.guiding) {
+            rc = J.learned ? mlp_frame_update(ctx, P.spectral, b.lambda0, b.cie, P.capacity, stream)
+                           : guide_update(ctx, P.spectral, b.lambda0, b.cie, P.capacity, stream);
             if (rc != AR_OK) return rc;
         }
-        done += this_spp;
-        hipLaunchKernelGGL(k_accumulate, dim3(grid_for(npix)), dim3(kBlock), 0, stream, P, sc, S, b.accum, this_spp,
-                           P.sample_base == opts->sample_offset, done >= opts->spp, spp_total, out);
+        hipLaunchKernelGGL(k_accumulate, dim3(grid_for(P.npixels, blk)), dim3(blk), 0, stream, P, sc, S, b.accum);
     }
     return check_hip(ctx, hipGetLastError(), "integrator launch");
+}
+
+int render_batch(ar_ctx* ctx, uint32_t n, const ar_camera* cams, const ar_render_opts* opts, float* const* outs,
+                 hipStream_t stream) {
+    if (!ctx->job) ctx->job = new RenderJob();
+    RenderJob& J = *ctx->job;
+    int rc = make_job(ctx, n, cams, opts, outs, J);
+    if (rc == AR_OK) rc = prepare_job(ctx, J, stream, false);
+    if (rc == AR_OK) rc = enqueue_job(ctx, J, stream);
+    return rc;
+}
+
+int render(ar_ctx* ctx, const ar_camera* cam, const ar_render_opts* opts, float* out, hipStream_t stream) {
+    return render_batch(ctx, 1, cam, opts, &out, stream);
+}
+
+void free_job(RenderJob* J) {
+    if (!J) return;
+    if (J->uploaded) { (void)hipEventSynchronize(J->uploaded); (void)hipEventDestroy(J->uploaded); }
+    if (J->dev) (void)hipFree(J->dev);
+    if (J->host) (void)hipHostFree(J->host);
+    delete J;
+}
+
+// ------------------------------------------------------------ graphs
+
+int graph_capture(ar_ctx* ctx, uint32_t n, const ar_camera* cams, const ar_render_opts* opts, float* const* outs,
+                  GraphRec* g) {
+    g->job = new RenderJob();
+    int rc = make_job(ctx, n, cams, opts, outs, *g->job);
+    if (rc == AR_OK) rc = prepare_job(ctx, *g->job, nullptr, true);
+    if (rc != AR_OK) return rc;
+    rc = check_hip(ctx, hipDeviceSynchronize(), "graph capture");
+    if (rc != AR_OK) return rc;
+    hipStream_t cs = nullptr;
+    rc = check_hip(ctx, hipStreamCreateWithFlags(&cs, hipStreamNonBlocking), "graph capture stream");
+    if (rc != AR_OK) return rc;
+    rc = check_hip(ctx, hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    if (rc == AR_OK) {
+        const int rce = enqueue_job(ctx, *g->job, cs);
+        hipGraph_t graph = nullptr;
+        const hipError_t e = hipStreamEndCapture(cs, &graph);
+        rc = rce != AR_OK ? rce : check_hip(ctx, e, "hipStreamEndCapture");
+        if (rc == AR_OK) rc = check_hip(ctx, hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+    (void)hipStreamDestroy(cs);
+    g->generation = ctx->buffer_generation;
+    return rc;
 }
 
 } // namespace ar

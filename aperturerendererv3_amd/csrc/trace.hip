@@ -296,7 +296,8 @@ constexpr int kStepBudget = 12;   // traversal steps between two ray-refill roun
 // ANY: sets bit idx of occ (zeroed beforehand) for occluded rays.
 template <int W, bool ANY, bool COUNT>
 __global__ __launch_bounds__(kTraceBlock) void k_trace(
-    const void* __restrict__ nodes, const float4* __restrict__ tris, uint32_t root, float origin_limit,
+    const void* __restrict__ nodes, const float4* __restrict__ tris, uint32_t root,
+    const uint32_t* __restrict__ roots, float origin_limit,
     ar_rays r, size_t n, const uint32_t* __restrict__ n_dev, ar_hits h, uint32_t* __restrict__ occ,
     int* __restrict__ spill, uint32_t threads, uint32_t* __restrict__ cnt_box, uint32_t* __restrict__ cnt_tri,
     uint32_t* __restrict__ fetch) {
@@ -334,7 +335,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(
                     s.have = false;
                     st.sp = 0;
                     st.push(kRefEmpty);
-                    s.ref = (s.p.valid && s.tmax >= s.tmin) ? root : kRefEmpty;
+                    s.ref = (s.p.valid && s.tmax >= s.tmin) ? (roots ? roots[idx] : root) : kRefEmpty;
                     nb = 0;
                     nt = 0;
                     active = true;
@@ -464,7 +465,8 @@ hipError_t launch_trace_closest(const TraceArgs& ta, const ar_rays& r, size_t n,
     const void* nd = ta.nodes;
 #define AR_LAUNCH_CLOSEST(W, C)                                                                                  \
     hipLaunchKernelGGL((k_trace<W, false, C>), dim3(blocks), dim3(kTraceBlock), 0, s, nd, ta.tris, ta.root,     \
-                       ta.origin_limit, r, n, n_dev, h, nullptr, ta.spill, ta.threads, cnt_box, cnt_tri, ta.fetch)
+                       ta.roots, ta.origin_limit, r, n, n_dev, h, nullptr, ta.spill, ta.threads, cnt_box, cnt_tri,  \
+                       ta.fetch)
     if (ta.width == 8) {
         if (cnt_box) AR_LAUNCH_CLOSEST(8, true); else AR_LAUNCH_CLOSEST(8, false);
     } else {
@@ -482,10 +484,12 @@ hipError_t launch_trace_any(const TraceArgs& ta, const ar_rays& r, size_t n, uin
     const ar_hits none{nullptr, nullptr, nullptr, nullptr, nullptr};
     if (ta.width == 8)
         hipLaunchKernelGGL((k_trace<8, true, false>), dim3(blocks), dim3(kTraceBlock), 0, s, nd, ta.tris, ta.root,
-                           ta.origin_limit, r, n, n_dev, none, occ, ta.spill, ta.threads, nullptr, nullptr, ta.fetch);
+                           ta.roots, ta.origin_limit, r, n, n_dev, none, occ, ta.spill, ta.threads, nullptr, nullptr,
+                           ta.fetch);
     else
         hipLaunchKernelGGL((k_trace<2, true, false>), dim3(blocks), dim3(kTraceBlock), 0, s, nd, ta.tris, ta.root,
-                           ta.origin_limit, r, n, n_dev, none, occ, ta.spill, ta.threads, nullptr, nullptr, ta.fetch);
+                           ta.roots, ta.origin_limit, r, n, n_dev, none, occ, ta.spill, ta.threads, nullptr, nullptr,
+                           ta.fetch);
     return hipGetLastError();
 }
 

@@ -393,3 +393,31 @@ def occluded_room(target_tris: int = 200_000, le: float = 40.0):
             {"albedo": (0.5, 0.45, 0.4), "model": 1, "roughness": 0.3, "specular": 0.5, "metallic": 0.0},
             {"albedo": (0.0, 0.0, 0.0), "emission": (le, le, le)}]
     return (np.concatenate(Vs).astype(np.float32), np.concatenate(Fs).astype(np.uint32), np.concatenate(Ms), mats)
+
+
+# --------------------------------------------------------------------------
+# Batched workload: many scenes of uneven size, resolution and spp
+# --------------------------------------------------------------------------
+
+BATCH_RESOLUTIONS = ((256, 256), (512, 512), (1024, 1024), (1920, 1080))
+BATCH_SPP = (4, 16, 64)
+
+
+def batch_specs(count: int = 48, seed: int = 1, tri_range=(1_000, 2_000_000)):
+    """Per scene k (seed + k): triangle count log-uniform in tri_range,
+    resolution uniform in BATCH_RESOLUTIONS, spp uniform in BATCH_SPP.
+    Returns a list of dicts (seed, tris, width, height, spp)."""
+    out = []
+    lo, hi = math.log(tri_range[0]), math.log(tri_range[1])
+    for k in range(count):
+        rng = np.random.Generator(np.random.PCG64(seed + k))
+        tris = int(round(math.exp(lo + (hi - lo) * rng.random())))
+        w, h = BATCH_RESOLUTIONS[int(rng.integers(len(BATCH_RESOLUTIONS)))]
+        spp = BATCH_SPP[int(rng.integers(len(BATCH_SPP)))]
+        out.append({"seed": seed + k, "tris": tris, "width": w, "height": h, "spp": spp})
+    return out
+
+
+def batch_scene(spec):
+    """The instanced spectral-scene generator at the spec's size and seed."""
+    return spectral_scene(target_tris=max(spec["tris"], 200), seed=spec["seed"])

@@ -244,6 +244,55 @@ typedef struct {
  * Queued on `stream`; the call does not wait. */
 int ar_render(ar_ctx* ctx, const ar_camera* cam, const ar_render_opts* opts, float* out_rgb, void* stream);
 
+/* --- batched rendering ----------------------------------------------------
+ * No counterpart in the reference (one SceneRenderer renders one scene).
+ * ar_batch_create merges scene contexts (each with its scene set and BVH
+ * built, all at the same BVH width) into a batch context whose render traces
+ * the paths of every scene through one set of wavefront queues: each ray
+ * carries its scene's BVH root, each path its scene's camera, image and
+ * lights.  The scene contexts stay valid and independent.  Scenes are packed
+ * into passes in order, whole samples at a time, so each scene's image is
+ * bit-identical to rendering it alone with the same options. */
+typedef struct ar_graph ar_graph;
+
+int ar_batch_create(ar_ctx* const* scenes, uint32_t n, ar_ctx** out);
+int ar_batch_size(const ar_ctx* ctx);
+/* cams, opts, outs: n entries (n = the batch's scene count, or 1 for a plain
+ * context); per scene: width, height, spp, sample_offset; seed, depth, NEE,
+ * sky, spectral, output, guiding and max_paths_per_pass must agree.  Guiding
+ * needs n == 1. */
+int ar_render_batch(ar_ctx* ctx, uint32_t n, const ar_camera* cams, const ar_render_opts* opts, float* const* outs,
+                    void* stream);
+
+/* Threads per block of the wavefront kernels (camera, shade, resolve,
+ * accumulate): 64, 128, 256 (default) or 512. */
+int ar_set_tile(ar_ctx* ctx, uint32_t threads);
+
+/* Captures a whole render (every pass and bounce) into a HIP graph; each
+ * ar_graph_launch replays it on `stream` (same seeds: same image).  The graph
+ * uses the context's render buffers: launches must not overlap other renders
+ * of the context, and a render that reallocates them (a larger image)
+ * invalidates the graph (ar_graph_launch then returns AR_E_INVALID).
+ * Guided renders cannot be captured. */
+int ar_graph_capture(ar_ctx* ctx, uint32_t n, const ar_camera* cams, const ar_render_opts* opts, float* const* outs,
+                     ar_graph** out);
+int ar_graph_launch(ar_graph* graph, void* stream);
+int ar_graph_destroy(ar_graph* graph);
+
+/* Autotuner: for BVH width 2 and 8 (every scene rebuilt, leaf size 3) and
+ * tiles of 64..512 threads, times one batched render of the scenes at
+ * min(spp, probe_spp) samples and picks the fastest; the scenes are left
+ * built at the chosen width.  The caller then creates the batch and sets the
+ * tile. */
+typedef struct {
+    uint32_t width;       /* chosen BVH width */
+    uint32_t tile;        /* chosen threads per block */
+    float ms[2][4];       /* [width 2, 8][tile 64, 128, 256, 512] */
+} ar_tune_result;
+
+int ar_autotune_batch(ar_ctx* const* scenes, uint32_t n, const ar_camera* cams, const ar_render_opts* opts,
+                      uint32_t probe_spp, ar_tune_result* out);
+
 /* --- voxel guiding state ------------------------------------------------
  * Probes are keyed by (voxel = floor(p / 0.05), icosahedral face of the
  * reflected direction) and hold an 8x8 equal-area octahedral texture of
