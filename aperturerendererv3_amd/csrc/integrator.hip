@@ -630,17 +630,16 @@ __global__ __launch_bounds__(kMaxBlock) void k_mlp_features(const float4* __rest
 // (render.cuh:75-163), NEE light sampling (render.cuh:205-343, 766-827;
 // the shadow segment is queued and resolved after the any-hit pass), Russian
 // roulette (render.cuh:838-859) and the next ray.
-__global__ __launch_bounds__(kMaxBlock) void k_shade(
-    RenderParams P, uint32_t depth, SceneArgs sc, PathArgs S, GuideArgs G,
-    QueueRays cur, const uint32_t* __restrict__ qcur,
+template <uint32_t GM>
+__device__ __forceinline__ void shade_one(
+    const RenderParams& P, uint32_t depth, const SceneArgs& sc, const PathArgs& S, const GuideArgs& G,
+    const QueueRays& cur, const uint32_t* __restrict__ qcur,
     const uint32_t* __restrict__ hprim, const float* __restrict__ ht, const float* __restrict__ hu,
     const float* __restrict__ hv,
-    QueueRays nxt, uint32_t* __restrict__ qnext,
-    QueueRays sh, uint32_t* __restrict__ spath, float* __restrict__ sc0, float* __restrict__ sc1,
+    const QueueRays& nxt, uint32_t* __restrict__ qnext,
+    const QueueRays& sh, uint32_t* __restrict__ spath, float* __restrict__ sc0, float* __restrict__ sc1,
     float* __restrict__ sc2, float* __restrict__ sc3, uint32_t* __restrict__ sflags,
-    uint32_t* __restrict__ counters, const float* __restrict__ eps_table) {
-    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t n = counters[0];
+    uint32_t* __restrict__ counters, const float* __restrict__ eps_table, uint32_t q, uint32_t n) {
     bool cont = false, shadow = false;
     uint32_t p = 0, root = 0;
     F3 next_o = f3(0, 0, 0), next_d = f3(0, 0, 0);
@@ -664,10 +663,10 @@ __global__ __launch_bounds__(kMaxBlock) void k_shade(
         const Lam L = make_lam(P.spectral, S.lambda0[p]);
         S4 mask = load4(S.mask0, S.mask1, S.mask2, S.mask3, p);
         S4 rad = load4(S.rad0, S.rad1, S.rad2, S.rad3, p);
-        const bool learned = P.guiding == AR_GUIDING_MLP;
-        const bool guided = learned ? P.alpha > 0.0f : (P.guiding && *G.probe_count > 0u);
+        constexpr bool learned = GM == AR_GUIDING_MLP;
+        const bool guided = learned ? P.alpha > 0.0f : (GM != AR_GUIDING_OFF && *G.probe_count > 0u);
         float gq[3] = {1.0f, 1.0f, 1.0f};
-        if (P.guiding && depth > 0) probe_query(mask, L, sc.cie, gq);
+        if (GM != AR_GUIDING_OFF && depth > 0) probe_query(mask, L, sc.cie, gq);
         const F3 o = f3(cur.ox[q], cur.oy[q], cur.oz[q]);
         const F3 d = f3(cur.dx[q], cur.dy[q], cur.dz[q]);
         const float eps = eps_table[non_delta < 255 ? non_delta : 255];
@@ -692,7 +691,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_shade(
                 }
                 rad = s4add(rad, s4mul(mask, e));
             }
-            if (P.guiding) g_endpoint(G, p, e, non_delta, P.spectral);
+            if (GM != AR_GUIDING_OFF) g_endpoint(G, p, e, non_delta, P.spectral);
         } else {
             const float4 a4 = sc.prims[3 * (size_t)prim + 0];
             const float4 b4 = sc.prims[3 * (size_t)prim + 1];
@@ -723,7 +722,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_shade(
                     }
                     const S4 le = emission_of(m, L, sc.d65);
                     rad = s4add(rad, s4scale(s4mul(mask, le), w));
-                    if (P.guiding) g_endpoint(G, p, s4scale(le, w), non_delta, P.spectral);
+                    if (GM != AR_GUIDING_OFF) g_endpoint(G, p, s4scale(le, w), non_delta, P.spectral);
                 }
             } else {
                 const S4 alb = albedo_of(m, L);
@@ -742,7 +741,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_shade(
                     const float4* ptex = nullptr;
                     const float* pmix = nullptr;
                     uint64_t gkey = guide::kNoKey;
-                    if (P.guiding == AR_GUIDING_VOXEL) {
+                    if (GM == AR_GUIDING_VOXEL) {
                         const F3 rv = sub(d, scale(nn, 2.0f * dot(d, nn)));
                         gkey = guide::voxel_key(pos.x, pos.y, pos.z, rv.x, rv.y, rv.z);
                     }
@@ -873,7 +872,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_shade(
                         if (u[9] > qrr) alive = false;
                     }
                     if (alive && depth + 1 < P.max_depth) {
-                        if (P.guiding)
+                        if (GM != AR_GUIDING_OFF)
                             g_advance(G, p, non_delta, P.spectral, gkey,
                                       learned ? 0 : guide::leaf_of(next.x, next.y, next.z), contrib, qrr, learned, x5,
                                       next, pdf);
@@ -912,6 +911,22 @@ __global__ __launch_bounds__(kMaxBlock) void k_shade(
     }
 }
 
+// One thread per entry of the current queue (grid sized for the pass);
+// specialised per guiding mode so that plain renders carry no guiding state.
+template <uint32_t GM>
+__global__ __launch_bounds__(kMaxBlock) void k_shade(
+    RenderParams P, uint32_t depth, SceneArgs sc, PathArgs S, GuideArgs G,
+    QueueRays cur, const uint32_t* __restrict__ qcur,
+    const uint32_t* __restrict__ hprim, const float* __restrict__ ht, const float* __restrict__ hu,
+    const float* __restrict__ hv,
+    QueueRays nxt, uint32_t* __restrict__ qnext,
+    QueueRays sh, uint32_t* __restrict__ spath, float* __restrict__ sc0, float* __restrict__ sc1,
+    float* __restrict__ sc2, float* __restrict__ sc3, uint32_t* __restrict__ sflags,
+    uint32_t* __restrict__ counters, const float* __restrict__ eps_table) {
+    shade_one<GM>(P, depth, sc, S, G, cur, qcur, hprim, ht, hu, hv, nxt, qnext, sh, spath, sc0, sc1, sc2, sc3, sflags,
+                  counters, eps_table, blockIdx.x * blockDim.x + threadIdx.x, counters[0]);
+}
+
 // Shadow results: an occluded segment still counts as a taken NEE sample
 // (render.cuh:297-305); an unoccluded one adds its MIS-weighted
 // contribution when the light point faced the surface.
@@ -921,17 +936,18 @@ __global__ __launch_bounds__(kMaxBlock) void k_resolve(PathArgs S, const uint32_
                                                     const uint32_t* __restrict__ sflags,
                                                     const uint32_t* __restrict__ occ,
                                                     const uint32_t* __restrict__ counters) {
-    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= counters[2]) return;
-    const uint32_t p = spath[s];
-    const bool occluded = (occ[s >> 5] >> (s & 31u)) & 1u;
-    const bool faces = sflags[s] & 1u;
-    if (occluded || faces) S.meta[p] |= kMetaNee;
-    if (!occluded && faces) {
-        S.rad0[p] = S.rad0[p] + sc0[s];
-        S.rad1[p] = S.rad1[p] + sc1[s];
-        S.rad2[p] = S.rad2[p] + sc2[s];
-        S.rad3[p] = S.rad3[p] + sc3[s];
+    const uint32_t n = counters[2];
+    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
+        const uint32_t p = spath[s];
+        const bool occluded = (occ[s >> 5] >> (s & 31u)) & 1u;
+        const bool faces = sflags[s] & 1u;
+        if (occluded || faces) S.meta[p] |= kMetaNee;
+        if (!occluded && faces) {
+            S.rad0[p] = S.rad0[p] + sc0[s];
+            S.rad1[p] = S.rad1[p] + sc1[s];
+            S.rad2[p] = S.rad2[p] + sc2[s];
+            S.rad3[p] = S.rad3[p] + sc3[s];
+        }
     }
 }
 
@@ -1290,7 +1306,7 @@ static int make_job(ar_ctx* ctx, uint32_t n, const ar_camera* cams, const ar_ren
     J.guiding = o.guiding != AR_GUIDING_OFF;
     J.learned = o.guiding == AR_GUIDING_MLP;
     if (J.guiding && n != 1) return set_error(ctx, AR_E_UNSUPPORTED, "guiding renders one scene at a time");
-    const size_t max_paths = o.max_paths_per_pass ? o.max_paths_per_pass : ((size_t)1 << 22);
+    const size_t max_paths = o.max_paths_per_pass ? o.max_paths_per_pass : ((size_t)1 << 24);
     RenderPlan& L = J.plan;
     L = RenderPlan{};
     size_t max_npix = 0;
@@ -1481,6 +1497,8 @@ static int enqueue_job(ar_ctx* ctx, RenderJob& J, hipStream_t stream) {
             const float t = std::pow(std::fmin(1.0f, (float)pass.sample_base / 64.0f), 0.5f);
             P.alpha = std::fmin(std::fmax(0.0f * (1.0f - t) + 0.5f * t, 0.0f), 1.0f);
         }
+        // grid-stride kernels: enough blocks to fill every CU several times over
+        const uint32_t stride_grid = std::min<uint32_t>(grid_for(P.capacity, blk), (uint32_t)ctx->num_cus * (2048u / blk) * 2u);
         hipLaunchKernelGGL(k_camera, dim3(grid_for(P.capacity, blk)), dim3(blk), 0, stream, P, S, G, q[0], b.qpath[0],
                            b.counters, b.eps_table);
         int c = 0;
@@ -1498,17 +1516,21 @@ static int enqueue_job(ar_ctx* ctx, RenderJob& J, hipStream_t stream) {
                 e = mlp_forward(ctx, ctx->mlp.feat, P.capacity, &b.counters[0], ctx->mlp.dec, true, stream);
                 if (e != hipSuccess) return check_hip(ctx, e, "guiding network inference");
             }
-            hipLaunchKernelGGL(k_shade, dim3(grid_for(P.capacity, blk)), dim3(blk), 0, stream, P, depth, sc, S, G, q[c],
-                               b.qpath[c], b.hit_prim, b.hit_t, b.hit_u, b.hit_v, q[1 - c], b.qpath[1 - c], shq,
-                               b.spath, b.scontrib[0], b.scontrib[1], b.scontrib[2], b.scontrib[3], b.sflags,
-                               b.counters, b.eps_table);
+#define AR_SHADE(GM)                                                                                           \
+    hipLaunchKernelGGL(k_shade<GM>, dim3(grid_for(P.capacity, blk)), dim3(blk), 0, stream, P, depth, sc, S, G, q[c], \
+                       b.qpath[c], b.hit_prim, b.hit_t, b.hit_u, b.hit_v, q[1 - c], b.qpath[1 - c], shq, b.spath,   \
+                       b.scontrib[0], b.scontrib[1], b.scontrib[2], b.scontrib[3], b.sflags, b.counters, b.eps_table)
+            if (P.guiding == AR_GUIDING_VOXEL) AR_SHADE(AR_GUIDING_VOXEL);
+            else if (P.guiding == AR_GUIDING_MLP) AR_SHADE(AR_GUIDING_MLP);
+            else AR_SHADE(AR_GUIDING_OFF);
+#undef AR_SHADE
             if (P.nee) {
                 ar_rays sr{shq.ox, shq.oy, shq.oz, shq.tmin, shq.dx, shq.dy, shq.dz, shq.tmax};
                 TraceArgs ts = ta;
                 ts.roots = shq.root;
                 e = launch_trace_any(ts, sr, P.capacity, b.occ, tblocks, stream, &b.counters[2]);
                 if (e != hipSuccess) return check_hip(ctx, e, "integrator shadow trace");
-                hipLaunchKernelGGL(k_resolve, dim3(grid_for(P.capacity, blk)), dim3(blk), 0, stream, S, b.spath,
+                hipLaunchKernelGGL(k_resolve, dim3(stride_grid), dim3(blk), 0, stream, S, b.spath,
                                    b.scontrib[0], b.scontrib[1], b.scontrib[2], b.scontrib[3], b.sflags, b.occ,
                                    b.counters);
             }

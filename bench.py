@@ -57,6 +57,8 @@ def parse_args():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--spectral-only", action="store_true", help="run only the spectral 1080p render bench")
     ap.add_argument("--spectral-spp", type=int, default=64)
+    ap.add_argument("--batch-only", action="store_true", help="run only the batched multi-scene bench")
+    ap.add_argument("--batch-scenes", type=int, default=48)
     ap.add_argument("--cpu-sample-stride", type=int, default=2)
     ap.add_argument("--profile", action="store_true",
                     help="timed steps only, plus one 256 MiB device copy used to scale HBM counters")
@@ -182,8 +184,98 @@ def spectral_bench(ctx_cls, frames=1, spp=64, cpu=True):
     return res
 
 
+def batch_bench(ctx_cls, count=48, cpu=True, reps=2):
+    """Batched workload: `count` instanced spectral scenes (scene k seeded
+    1 + k), triangles log-uniform in [1e3, 2e6], resolution uniform in
+    {256^2, 512^2, 1024^2, 1920x1080}, spp uniform in {4, 16, 64}, depth 6,
+    RGB, NEE on, sky off.  Autotunes (BVH width, tile) at 1 spp, then times
+    the whole batch as one HIP graph against the same scenes rendered one
+    after another with ar_render (same options, same images)."""
+    import torch
+    from aperturerendererv3_amd import autotune, make_batch
+    from aperturerendererv3_amd.workloads import SPECTRAL_CAMERA, batch_scene, batch_specs
+    specs = batch_specs(count)
+    cam = dict(cam_pos=SPECTRAL_CAMERA["pos"], cam_dir=SPECTRAL_CAMERA["dir"], vfov=SPECTRAL_CAMERA["vfov"])
+    t0 = time.perf_counter()
+    ctxs, views, tris = [], [], 0
+    for sp in specs:
+        V, F, M, mats = batch_scene(sp)
+        c = ctx_cls(0)
+        c.set_meshes([{"vertices": V, "indices": F, "tri_material": M, "materials": mats}])
+        c.build_bvh()
+        ctxs.append(c)
+        views.append(dict(width=sp["width"], height=sp["height"], spp=sp["spp"], **cam))
+        tris += F.shape[0]
+    setup_s = time.perf_counter() - t0
+    shared = dict(max_depth=6, seed=1)
+    t0 = time.perf_counter()
+    tune = autotune(ctxs, views, probe_spp=1, **shared)
+    tune_s = time.perf_counter() - t0
+    paths = sum(v["width"] * v["height"] * v["spp"] for v in views)
+    # sequential: each scene on its own, one ar_render after another
+    outs = [torch.empty((v["height"], v["width"], 3), dtype=torch.float32, device="cuda:0") for v in views]
+    for c in ctxs:
+        c.set_tile(tune["tile"])
+    for c, v, o in zip(ctxs, views, outs):
+        c.render_views([v], outs=[o], **shared)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        for c, v, o in zip(ctxs, views, outs):
+            c.render_views([v], outs=[o], **shared)
+    e1.record()
+    torch.cuda.synchronize()
+    seq_ms = e0.elapsed_time(e1) / reps
+    seq_img = [o.cpu().numpy() for o in outs]
+    b = make_batch(ctxs)
+    b.set_tile(tune["tile"])
+    g = b.capture(views, **shared)
+    g.launch()
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        g.launch()
+    e1.record()
+    torch.cuda.synchronize()
+    batch_ms = e0.elapsed_time(e1) / reps
+    same = all(np.array_equal(a.view(np.uint32), o.cpu().numpy().view(np.uint32)) for a, o in zip(seq_img, g.outputs))
+    res = {"workload": f"{count} instanced spectral scenes, {tris} triangles in total (log-uniform 1e3..2e6 per scene), "
+                       "resolutions in {256^2, 512^2, 1024^2, 1920x1080}, spp in {4, 16, 64}, max depth 6, NEE on, "
+                       "sky off, RGB",
+           "paths": paths, "autotune": tune, "setup_s": round(setup_s, 2), "autotune_s": round(tune_s, 2),
+           "batched_graph_ms": round(batch_ms, 2), "batched_paths_per_s": round(paths / (batch_ms * 1e-3), 1),
+           "sequential_ms": round(seq_ms, 2), "sequential_paths_per_s": round(paths / (seq_ms * 1e-3), 1),
+           "speedup_vs_sequential": round(seq_ms / batch_ms, 3), "images_identical_to_sequential": bool(same)}
+    if cpu:
+        sys.path.insert(0, str(ROOT / "tests"))
+        import oracle_lib as orc
+        cores = len(os.sched_getaffinity(0))
+        order = sorted(range(count), key=lambda k: specs[k]["tris"])[:4]
+        cp, ct = 0, 0.0
+        for k in order:
+            V, F, M, mats = batch_scene(specs[k])
+            w, h = specs[k]["width"] // 4, specs[k]["height"] // 4
+            t0 = time.perf_counter()
+            orc.render(V, F, M, mats, w, h, 1, precision=32, threads=cores, max_depth=6, seed=1, **cam)
+            ct += time.perf_counter() - t0
+            cp += w * h
+        res["cpu_baseline"] = {"value": round(cp / ct, 1), "unit": "paths/s", "cores": cores, "kind": "port",
+                               "sample": f"the 4 smallest scenes at 1 spp and 1/4 resolution ({cp} paths) with the "
+                                         f"oracle FP32 integrator, including its BVH builds, {ct:.2f} s"}
+    g.close()
+    b.close()
+    for c in ctxs:
+        c.close()
+    return res
+
+
 def main():
     args = parse_args()
+    if args.batch_only:
+        from aperturerendererv3_amd import Context
+        print(json.dumps(batch_bench(Context, count=args.batch_scenes, cpu=not args.no_cpu_baseline)), flush=True)
+        return
     if args.spectral_only:
         from aperturerendererv3_amd import Context
         print(json.dumps(spectral_bench(Context, frames=max(1, args.steps), spp=args.spectral_spp,

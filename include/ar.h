@@ -197,7 +197,7 @@ typedef struct {
     uint32_t nee;                 /* next-event estimation on/off */
     uint32_t sky;                 /* AR_SKY_* */
     float env[3];
-    uint32_t max_paths_per_pass;  /* 0: 4,194,304 */
+    uint32_t max_paths_per_pass;  /* 0: 16,777,216 (about 4 GB of path state) */
     uint32_t spectral;            /* AR_SPECTRAL_* */
     uint32_t output;              /* AR_OUTPUT_* (spectral modes) */
     uint32_t guiding;             /* AR_GUIDING_* */
