@@ -66,7 +66,7 @@ def _stream_ptr(stream) -> int | None:
 
 def _camera_opts(width, height, spp, cam_pos=(0.0, 0.0, 1.1), cam_dir=(0.0, 0.0, -1.0), vfov=60.0, seed=1,
                  max_depth=5, nee=True, sky=AR_SKY_NONE, env=(0.0, 0.0, 0.0), rr_min_depth=3, sample_offset=0,
-                 max_paths_per_pass=0, spectral=0, output=0, guiding=0):
+                 max_paths_per_pass=0, spectral=0, output=0, guiding=0, frame_spp=0):
     cam = ar_camera()
     cam.pos[:] = [float(x) for x in cam_pos]
     cam.dir[:] = [float(x) for x in cam_dir]
@@ -76,7 +76,7 @@ def _camera_opts(width, height, spp, cam_pos=(0.0, 0.0, 1.1), cam_dir=(0.0, 0.0,
     o.seed, o.max_depth, o.rr_min_depth, o.nee, o.sky = seed, max_depth, rr_min_depth, int(bool(nee)), sky
     o.env[:] = [float(x) for x in env]
     o.max_paths_per_pass = max_paths_per_pass
-    o.spectral, o.output, o.guiding = spectral, output, guiding
+    o.spectral, o.output, o.guiding, o.frame_spp = spectral, output, guiding, frame_spp
     return cam, o
 
 
@@ -373,11 +373,11 @@ class Context:
     # -- integrator ----------------------------------------------------------
     def render(self, width, height, spp, cam_pos=(0.0, 0.0, 1.1), cam_dir=(0.0, 0.0, -1.0), vfov=60.0, seed=1,
                max_depth=5, nee=True, sky=AR_SKY_NONE, env=(0.0, 0.0, 0.0), rr_min_depth=3, sample_offset=0,
-               max_paths_per_pass=0, spectral=0, output=0, guiding=0, out=None, stream=None):
+               max_paths_per_pass=0, spectral=0, output=0, guiding=0, frame_spp=0, out=None, stream=None):
         """Mean radiance image (height, width, 3) float32 on the GPU (ar_render)."""
         import torch
         cam, o = _camera_opts(width, height, spp, cam_pos, cam_dir, vfov, seed, max_depth, nee, sky, env,
-                              rr_min_depth, sample_offset, max_paths_per_pass, spectral, output, guiding)
+                              rr_min_depth, sample_offset, max_paths_per_pass, spectral, output, guiding, frame_spp)
         if out is None:
             out = torch.empty((height, width, 3), dtype=torch.float32, device=f"cuda:{self.device}")
         self._check(self._lib.ar_render(self._h, C.byref(cam), C.byref(o), _ptr(out), _stream_ptr(stream)),

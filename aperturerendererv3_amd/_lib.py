@@ -68,7 +68,8 @@ class ar_render_opts(C.Structure):
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32), ("sample_offset", C.c_uint32),
                 ("seed", C.c_uint64), ("max_depth", C.c_uint32), ("rr_min_depth", C.c_uint32), ("nee", C.c_uint32),
                 ("sky", C.c_uint32), ("env", C.c_float * 3), ("max_paths_per_pass", C.c_uint32),
-                ("spectral", C.c_uint32), ("output", C.c_uint32), ("guiding", C.c_uint32)]
+                ("spectral", C.c_uint32), ("output", C.c_uint32), ("guiding", C.c_uint32),
+                ("frame_spp", C.c_uint32)]
 
 
 class ar_mlp_opts(C.Structure):

@@ -1290,7 +1290,7 @@ static void camera_basis(const ar_camera* cam, uint32_t width, uint32_t height, 
 static bool same_settings(const ar_render_opts& a, const ar_render_opts& b) {
     return a.seed == b.seed && a.max_depth == b.max_depth && a.rr_min_depth == b.rr_min_depth && a.nee == b.nee &&
            a.sky == b.sky && a.env[0] == b.env[0] && a.env[1] == b.env[1] && a.env[2] == b.env[2] &&
-           a.spectral == b.spectral && a.output == b.output && a.guiding == b.guiding &&
+           a.spectral == b.spectral && a.output == b.output && a.guiding == b.guiding && a.frame_spp == b.frame_spp &&
            a.max_paths_per_pass == b.max_paths_per_pass;
 }
 
@@ -1336,8 +1336,11 @@ static int make_job(ar_ctx* ctx, uint32_t n, const ar_camera* cams, const ar_ren
         max_npix = std::max(max_npix, npix);
         if (L.pixels > 0xFFFFFFFFull) return set_error(ctx, AR_E_CAPACITY, "batch larger than 2^32 pixels");
     }
-    // guiding: frames of one sample per pixel with a guide update after each
-    const size_t cap = J.guiding ? max_npix : std::max(max_paths, max_npix);
+    // guiding: frames of frame_spp samples per pixel with a guide update after each
+    const uint32_t frame_spp = std::max(1u, o.frame_spp);
+    if (J.guiding && max_npix * frame_spp > ((size_t)1 << 26))
+        return set_error(ctx, AR_E_CAPACITY, "guided frame larger than 2^26 paths");
+    const size_t cap = J.guiding ? max_npix * frame_spp : std::max(max_paths, max_npix);
     RenderPlan::Pass cur{0, 0, 0, 0, 0};
     auto close = [&]() {
         if (cur.nseg) {
@@ -1352,7 +1355,7 @@ static int make_job(ar_ctx* ctx, uint32_t n, const ar_camera* cams, const ar_ren
         uint32_t done = 0;
         while (done < v.spp) {
             size_t fit = (cap - cur.paths) / v.npix;
-            if (J.guiding) fit = std::min<size_t>(fit, 1);
+            if (J.guiding) fit = std::min<size_t>(fit, frame_spp);
             if (fit == 0 || (J.guiding && cur.nseg)) { close(); continue; }
             const uint32_t k = (uint32_t)std::min<size_t>(fit, v.spp - done);
             Segment sg{};

@@ -270,6 +270,48 @@ def batch_bench(ctx_cls, count=48, cpu=True, reps=2):
     return res
 
 
+def guiding_bench(ctx_cls):
+    """Equal-time comparison on the occluded-light room (light only through a
+    0.5 m door gap; 196,770 triangles): 64 x 64 pixels of its camera, depth
+    8, scored by relMSE against the committed FP64 oracle reference at 16,384
+    spp.  Plain rendering at 1024 and 4096 spp; learned guiding (MLP, frames
+    of 64 spp, 4 Adam steps per frame) and voxel guiding (frames of 16 spp)
+    at 1024 and 4096 spp; wall time per render with the GPU synchronised."""
+    import torch
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_lib as orc
+    from aperturerendererv3_amd.workloads import OCCLUDED_CAMERA as C, occluded_room
+    ref = np.load(ROOT / "tests" / "golden" / "occluded_room_fp64_64x64.npy")
+    V, F, M, mats = occluded_room()
+    kw = dict(cam_pos=C["pos"], cam_dir=C["dir"], vfov=C["vfov"], max_depth=8)
+    out = {"workload": f"occluded-light room, {F.shape[0]} triangles, 64x64, max depth 8, NEE on; relMSE against "
+                       "the FP64 oracle at 16384 spp (tests/golden/occluded_room_fp64_64x64.npy)"}
+    with ctx_cls(0) as ctx:
+        ctx.set_meshes([{"vertices": V, "indices": F, "tri_material": M, "materials": mats}])
+        ctx.build_bvh()
+
+        def run(spp, seed, prep=None, **extra):
+            if prep:
+                prep()
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            img = ctx.render(64, 64, spp, seed=seed, **kw, **extra)
+            torch.cuda.synchronize()
+            return {"spp": spp, "ms": round((time.perf_counter() - t) * 1e3, 2),
+                    "relmse": orc.rel_mse(img.cpu().numpy(), ref)}
+
+        run(64, 99)
+        out["plain"] = [run(s, 1) for s in (1024, 4096)]
+        mlp = lambda: ctx.mlp_configure(learning_rate=5e-3, iters_per_frame=4, seed=1)
+        out["learned"] = [run(s, 7, prep=mlp, guiding=2, frame_spp=64) for s in (1024, 4096)]
+        out["voxel"] = [run(s, 7, prep=ctx.guiding_reset, guiding=1, frame_spp=16) for s in (1024, 4096)]
+    # time the plain integrator needs for the learned render's error (relMSE ~ 1/spp)
+    p, l = out["plain"][-1], out["learned"][-1]
+    t_equal = p["ms"] * p["relmse"] / l["relmse"]
+    out["learned_vs_plain_time_at_equal_relmse"] = round(l["ms"] / t_equal, 3)
+    return out
+
+
 def main():
     args = parse_args()
     if args.batch_only:
@@ -392,6 +434,8 @@ def main():
         cpu = cpu_baseline(V, F, rays_np, args.cpu_sample_stride, shadow.cpu().numpy())
     integ = integrator_bench(Context, cpu=not args.no_cpu_baseline)
     spectral = spectral_bench(Context, frames=2, spp=args.spectral_spp, cpu=not args.no_cpu_baseline)
+    batched = batch_bench(Context, count=args.batch_scenes, cpu=not args.no_cpu_baseline)
+    guiding = guiding_bench(Context)
 
     valu_frac = achieved_tops / VALU_PEAK_TOPS
     out = {
@@ -414,6 +458,8 @@ def main():
         "breakdown": {k: round(v, 4) for k, v in breakdown.items()},
         "integrator": integ,
         "spectral_1080p": spectral,
+        "batched": batched,
+        "guiding": guiding,
     }
     print(json.dumps(out), flush=True)
     ctx.close()

@@ -201,6 +201,8 @@ typedef struct {
     uint32_t spectral;            /* AR_SPECTRAL_* */
     uint32_t output;              /* AR_OUTPUT_* (spectral modes) */
     uint32_t guiding;             /* AR_GUIDING_* */
+    uint32_t frame_spp;           /* guided renders: samples per pixel per frame, i.e. between guide
+                                     updates (0: 1, the reference's frame) */
 } ar_render_opts;
 
 /* Path guiding.  VOXEL: the reference's voxel guiding (core/guiding,
