@@ -201,25 +201,120 @@ __device__ __forceinline__ void decode(const float* o, Lobes& L) {
 
 __device__ __forceinline__ float vmf_norm(float kap) { return kap / (kTwoPi * (1.0f - expf(-2.0f * kap))); }
 
-// forward of n inputs; DECODE: 48 decoded floats per sample, else the 40 raw outputs
+// Inference: persistent one-wave blocks.  Each lane keeps its A fragments of
+// all four layers in registers (96 VGPRs of BF16 weights) and each 64-sample
+// tile's activations stay in registers too: with the samples as the N
+// dimension, a layer's D tile (rows = units 4g..4g+3, column = sample i)
+// after bias, ReLU and BF16 rounding is exactly the B operand of the next
+// layer's K block, so hidden layers never touch LDS.  DECODE: the 40 outputs
+// of a sample go through LDS to one lane, which writes the 48 decoded floats;
+// else the 40 raw outputs are stored directly.
+__device__ __forceinline__ bf16x4 relu_bf16(f32x4 c, const float b[4]) {
+    bf16x4 h;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) h[r] = (short)f2bf(fmaxf(c[r] + b[r], 0.0f));
+    return h;
+}
+
 template <bool DECODE>
 __global__ __launch_bounds__(64) void k_mlp_forward(Net N, const float* __restrict__ x5, size_t n, float* __restrict__ out,
                                                     const uint32_t* __restrict__ n_dev) {
-    __shared__ FwdLds L;
+    __shared__ float lo[DECODE ? 64 * kOut : 1];
     const int lane = threadIdx.x;
+    const int i = lane & 15, g = lane >> 4;
     const size_t nn = n_dev ? std::min<size_t>(n, *n_dev) : n;
+    if ((size_t)blockIdx.x * 64 >= nn) return;
+    bf16x4 a0[4], a1[4][4], a2[4][4], a3[3][4];
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb) {
+        a0[mb] = ld4(N.wt + OW0 + (mb * 16 + i) * kIn + 4 * g);
+#pragma unroll
+        for (int kb = 0; kb < 4; ++kb) {
+            a1[mb][kb] = ld4(N.wt + OW1 + (mb * 16 + i) * kHid + kb * 16 + 4 * g);
+            a2[mb][kb] = ld4(N.wt + OW2 + (mb * 16 + i) * kHid + kb * 16 + 4 * g);
+        }
+    }
+#pragma unroll
+    for (int mb = 0; mb < 3; ++mb)
+#pragma unroll
+        for (int kb = 0; kb < 4; ++kb) a3[mb][kb] = ld4(N.wt + OW3 + (mb * 16 + i) * kHid + kb * 16 + 4 * g);
     for (size_t s0 = (size_t)blockIdx.x * 64; s0 < nn; s0 += (size_t)gridDim.x * 64) {
-        load_inputs(x5, nn, s0, L.x, lane);
-        __syncthreads();
-        forward_tile(N, L, lane);
-        const size_t s = s0 + lane;
-        if (s < nn) {
-            const float* o = L.o + lane * kOut;
-            if (!DECODE) {
-                for (int k = 0; k < kUsedOut; ++k) out[s * kUsedOut + k] = o[k];
-            } else {
+        // keep the per-tile bias loads inside the loop (hoisting them out would
+        // cost 60 registers and halve the occupancy)
+        asm volatile("" ::: "memory");
+        bf16x4 h[4][4], h2[4][4];
+        // layer 0: B[k = input 4g + r][sample i]
+#pragma unroll
+        for (int nb = 0; nb < 4; ++nb) {
+            const size_t s = s0 + nb * 16 + i;
+            bf16x4 xb;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int k = 4 * g + r;
+                xb[r] = (short)((k < kUsedIn && s < nn) ? f2bf(x5[s * kUsedIn + k]) : (uint16_t)0);
+            }
+#pragma unroll
+            for (int mb = 0; mb < 4; ++mb) {
+                float b[4];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) b[r] = N.p[OB0 + mb * 16 + 4 * g + r];
+                const f32x4 z = {0.0f, 0.0f, 0.0f, 0.0f};
+                h[mb][nb] = relu_bf16(mfma(a0[mb], xb, z), b);
+            }
+        }
+        // layers 1 and 2
+#pragma unroll
+        for (int mb = 0; mb < 4; ++mb) {
+            float b[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) b[r] = N.p[OB1 + mb * 16 + 4 * g + r];
+#pragma unroll
+            for (int nb = 0; nb < 4; ++nb) {
+                f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                for (int kb = 0; kb < 4; ++kb) c = mfma(a1[mb][kb], h[kb][nb], c);
+                h2[mb][nb] = relu_bf16(c, b);
+            }
+        }
+#pragma unroll
+        for (int mb = 0; mb < 4; ++mb) {
+            float b[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) b[r] = N.p[OB2 + mb * 16 + 4 * g + r];
+#pragma unroll
+            for (int nb = 0; nb < 4; ++nb) {
+                f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                for (int kb = 0; kb < 4; ++kb) c = mfma(a2[mb][kb], h2[kb][nb], c);
+                h[mb][nb] = relu_bf16(c, b);
+            }
+        }
+        // output layer
+#pragma unroll
+        for (int mb = 0; mb < 3; ++mb) {
+            float b[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) b[r] = N.p[OB3 + mb * 16 + 4 * g + r];
+#pragma unroll
+            for (int nb = 0; nb < 4; ++nb) {
+                f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                for (int kb = 0; kb < 4; ++kb) c = mfma(a3[mb][kb], h[kb][nb], c);
+                const int u0 = mb * 16 + 4 * g, sl = nb * 16 + i;
+                const float4 v = make_float4(c[0] + b[0], c[1] + b[1], c[2] + b[2], c[3] + b[3]);
+                if (DECODE) {
+                    *reinterpret_cast<float4*>(&lo[sl * kOut + u0]) = v;
+                } else if (u0 < kUsedOut && s0 + sl < nn) {
+                    *reinterpret_cast<float4*>(&out[(s0 + sl) * kUsedOut + u0]) = v;
+                }
+            }
+        }
+        if (DECODE) {
+            __syncthreads();
+            const size_t s = s0 + lane;
+            if (s < nn) {
                 Lobes D;
-                decode(o, D);
+                decode(lo + lane * kOut, D);
                 float* d = out + s * kDecoded;
 #pragma unroll
                 for (int k = 0; k < kLobes; ++k) {
@@ -231,12 +326,11 @@ __global__ __launch_bounds__(64) void k_mlp_forward(Net N, const float* __restri
                     d[6 * k + 5] = D.mu[k][2];
                 }
             }
+            __syncthreads();
         }
-        __syncthreads();
     }
 }
 
-// f.o holds the outputs, then (per lane, after decoding them) the FP32 loss gradient
 struct TrainLds {
     FwdLds f;
     uint16_t dob[64 * kOut];
@@ -524,10 +618,12 @@ hipError_t mlp_forward(ar_ctx* ctx, const float* x5, size_t n, const uint32_t* n
                        hipStream_t s) {
     MlpDev& M = ctx->mlp;
     Net N{M.p, M.wb, M.wt};
+    // persistent: one wave per SIMD (the kernel keeps ~280 registers), each looping over 64-sample tiles
+    const uint32_t grid = (uint32_t)std::min<size_t>((n + 63) / 64, (size_t)std::max(ctx->num_cus, 1) * 4);
     if (decoded)
-        hipLaunchKernelGGL(k_mlp_forward<true>, dim3(tiles(n)), dim3(64), 0, s, N, x5, n, out, n_dev);
+        hipLaunchKernelGGL(k_mlp_forward<true>, dim3(std::max(grid, 1u)), dim3(64), 0, s, N, x5, n, out, n_dev);
     else
-        hipLaunchKernelGGL(k_mlp_forward<false>, dim3(tiles(n)), dim3(64), 0, s, N, x5, n, out, n_dev);
+        hipLaunchKernelGGL(k_mlp_forward<false>, dim3(std::max(grid, 1u)), dim3(64), 0, s, N, x5, n, out, n_dev);
     return hipGetLastError();
 }
 
