@@ -41,6 +41,7 @@ ROOT = Path(__file__).resolve().parent
 METRIC = "Grays/s traversal; paths/s at equal FP64-oracle relMSE, plain vs batched+MLP"
 VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12      # 39.3 Tops/s
 HBM_PEAK_GBPS = 8000.0
+MFMA_BF16_PEAK = 2500.0                          # dense BF16 MFMA, TFLOP/s
 LIGHT = (2.0, 2.0, 2.0)
 TRAFFIC_FILE = ROOT / "profiles" / "k_trace_closest_bvh8_traversal_1M_tris_16M_rays.md"
 
@@ -59,6 +60,7 @@ def parse_args():
     ap.add_argument("--spectral-spp", type=int, default=64)
     ap.add_argument("--batch-only", action="store_true", help="run only the batched multi-scene bench")
     ap.add_argument("--batch-scenes", type=int, default=48)
+    ap.add_argument("--mlp-only", action="store_true", help="run only the guiding-network kernel bench")
     ap.add_argument("--cpu-sample-stride", type=int, default=2)
     ap.add_argument("--profile", action="store_true",
                     help="timed steps only, plus one 256 MiB device copy used to scale HBM counters")
@@ -251,18 +253,19 @@ def batch_bench(ctx_cls, count=48, cpu=True, reps=2):
         sys.path.insert(0, str(ROOT / "tests"))
         import oracle_lib as orc
         cores = len(os.sched_getaffinity(0))
-        order = sorted(range(count), key=lambda k: specs[k]["tris"])[:4]
+        order = sorted(range(count), key=lambda k: specs[k]["tris"])[::max(1, count // 8)][:8]
         cp, ct = 0, 0.0
         for k in order:
             V, F, M, mats = batch_scene(specs[k])
-            w, h = specs[k]["width"] // 4, specs[k]["height"] // 4
+            w, h = specs[k]["width"], specs[k]["height"]
             t0 = time.perf_counter()
-            orc.render(V, F, M, mats, w, h, 1, precision=32, threads=cores, max_depth=6, seed=1, **cam)
+            orc.render(V, F, M, mats, w, h, 6, precision=32, threads=cores, max_depth=6, seed=1, **cam)
             ct += time.perf_counter() - t0
-            cp += w * h
+            cp += 6 * w * h
         res["cpu_baseline"] = {"value": round(cp / ct, 1), "unit": "paths/s", "cores": cores, "kind": "port",
-                               "sample": f"the 4 smallest scenes at 1 spp and 1/4 resolution ({cp} paths) with the "
-                                         f"oracle FP32 integrator, including its BVH builds, {ct:.2f} s"}
+                               "sample": f"8 of the scenes spanning the triangle-count range, full resolution, 6 spp "
+                                         f"({cp} paths), with the oracle FP32 integrator including its BVH builds, "
+                                         f"{ct:.2f} s"}
     g.close()
     b.close()
     for c in ctxs:
@@ -312,8 +315,60 @@ def guiding_bench(ctx_cls):
     return out
 
 
+# algorithmic FLOPs per vertex of the guiding network 5 -> 64 -> 64 -> 64 -> 40
+MLP_FWD_FLOP = 2 * (5 * 64 + 64 * 64 + 64 * 64 + 64 * 40)
+# training: forward, backward through layers 3..1 (no input gradient), weight gradients
+MLP_TRAIN_FLOP = 2 * MLP_FWD_FLOP + 2 * (64 * 64 + 64 * 64 + 64 * 40)
+
+
+def mlp_bench(ctx_cls, reps=10):
+    """Guiding-network kernels on synthetic records: one Adam step over 2^18
+    training records (k_mlp_grad + weight reduction + Adam) and inference of
+    2^21 vertices (k_mlp_forward), each timed with HIP events on the current
+    stream; bound = BF16 dense MFMA."""
+    import torch
+    rng = np.random.default_rng(1)
+    dev = torch.device("cuda:0")
+    n, ni = 1 << 18, 1 << 21
+    x = torch.from_numpy(rng.random((n, 5), dtype=np.float32)).to(dev)
+    d = rng.normal(size=(n, 3))
+    d = torch.from_numpy((d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)).to(dev)
+    w = torch.from_numpy(rng.random(n, dtype=np.float32)).to(dev)
+    xi = torch.from_numpy(rng.random((ni, 5), dtype=np.float32)).to(dev)
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    with ctx_cls(0) as ctx:
+        ctx.mlp_configure(learning_rate=1e-3, seed=1)
+        ctx.mlp_train(x, d, w, 2)
+        torch.cuda.synchronize()
+        e0.record(stream)
+        ctx.mlp_train(x, d, w, reps)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        train_ms = e0.elapsed_time(e1) / reps
+        ctx.mlp_forward(xi)
+        torch.cuda.synchronize()
+        e0.record(stream)
+        for _ in range(reps):
+            ctx.mlp_forward(xi)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        fwd_ms = e0.elapsed_time(e1) / reps
+    train_tf = n * MLP_TRAIN_FLOP / (train_ms * 1e-3) / 1e12
+    fwd_tf = ni * MLP_FWD_FLOP / (fwd_ms * 1e-3) / 1e12
+    return {"workload": "guiding network 5-64-64-64-40 (8 vMF lobes), BF16 MFMA with FP32 accumulation",
+            "train": {"records": n, "ms_per_adam_step": round(train_ms, 4), "flop_per_record": MLP_TRAIN_FLOP,
+                      "achieved_TFLOPs": round(train_tf, 2), "frac_bf16_peak": round(train_tf / MFMA_BF16_PEAK, 5)},
+            "forward": {"vertices": ni, "ms": round(fwd_ms, 4), "flop_per_vertex": MLP_FWD_FLOP,
+                        "achieved_TFLOPs": round(fwd_tf, 2), "frac_bf16_peak": round(fwd_tf / MFMA_BF16_PEAK, 5)}}
+
+
 def main():
     args = parse_args()
+    if args.mlp_only:
+        from aperturerendererv3_amd import Context
+        print(json.dumps(mlp_bench(Context)), flush=True)
+        return
     if args.batch_only:
         from aperturerendererv3_amd import Context
         print(json.dumps(batch_bench(Context, count=args.batch_scenes, cpu=not args.no_cpu_baseline)), flush=True)
@@ -436,6 +491,7 @@ def main():
     spectral = spectral_bench(Context, frames=2, spp=args.spectral_spp, cpu=not args.no_cpu_baseline)
     batched = batch_bench(Context, count=args.batch_scenes, cpu=not args.no_cpu_baseline)
     guiding = guiding_bench(Context)
+    mlp = mlp_bench(Context)
 
     valu_frac = achieved_tops / VALU_PEAK_TOPS
     out = {
@@ -460,6 +516,7 @@ def main():
         "spectral_1080p": spectral,
         "batched": batched,
         "guiding": guiding,
+        "guiding_network": mlp,
     }
     print(json.dumps(out), flush=True)
     ctx.close()
