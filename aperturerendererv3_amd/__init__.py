@@ -17,11 +17,12 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import (AR_MISS, AR_MODEL_GGX, AR_MODEL_LAMBERT, AR_OK, AR_SKY_CONSTANT, AR_SKY_GRADIENT, AR_SKY_NONE,
+from ._lib import (AR_HIT_INSIDE, AR_MISS, AR_MODEL_GGX, AR_MODEL_GLASS, AR_MODEL_LAMBERT, AR_OBJ_BOX, AR_OBJ_SPHERE, AR_OK, AR_SKY_CONSTANT, AR_SKY_GRADIENT, AR_SKY_NONE,
                    ar_bvh_info, ar_bvh_opts, ar_camera, ar_device_opts, ar_hits, ar_material, ar_mesh, ar_rays,
                    ar_render_opts)
 
-__all__ = ["Context", "ArError", "AR_MISS", "AR_MODEL_LAMBERT", "AR_MODEL_GGX", "AR_SKY_NONE", "AR_SKY_GRADIENT",
+__all__ = ["Context", "ArError", "AR_MISS", "AR_MODEL_LAMBERT", "AR_MODEL_GGX", "AR_MODEL_GLASS", "AR_OBJ_BOX",
+           "AR_OBJ_SPHERE", "AR_HIT_INSIDE", "AR_SKY_NONE", "AR_SKY_GRADIENT",
            "AR_SKY_CONSTANT", "Material", "RAY_FIELDS"]
 
 RAY_FIELDS = ("ox", "oy", "oz", "tmin", "dx", "dy", "dz", "tmax")
@@ -34,23 +35,45 @@ class ArError(RuntimeError):
 
 
 class Material:
-    """Per-triangle material (triangle counterpart of the reference GBuffer)."""
+    """Material of a triangle or analytic object (the reference GBuffer);
+    glass parameters default to the reference's (IOR 1.5, sigmaA 100, Abbe
+    20, tilt 0)."""
 
     def __init__(self, albedo=(0.75, 0.75, 0.75), emission=(0.0, 0.0, 0.0), specular=0.5, metallic=0.0,
-                 roughness=1.0, model=AR_MODEL_LAMBERT):
+                 roughness=1.0, model=AR_MODEL_LAMBERT, ior=1.5, sigma_a=100.0, abbe=20.0, tilt=0.0):
         self.albedo = tuple(float(x) for x in albedo)
         self.emission = tuple(float(x) for x in emission)
         self.specular = float(specular)
         self.metallic = float(metallic)
         self.roughness = float(roughness)
         self.model = int(model)
+        self.ior, self.sigma_a, self.abbe, self.tilt = float(ior), float(sigma_a), float(abbe), float(tilt)
 
     def to_c(self) -> ar_material:
         m = ar_material()
         m.albedo[:] = self.albedo
         m.emission[:] = self.emission
         m.specular, m.metallic, m.roughness, m.model = self.specular, self.metallic, self.roughness, self.model
+        m.ior, m.sigma_a, m.abbe, m.tilt = self.ior, self.sigma_a, self.abbe, self.tilt
         return m
+
+
+_OBJ_TYPES = {"box": AR_OBJ_BOX, "sphere": AR_OBJ_SPHERE, AR_OBJ_BOX: AR_OBJ_BOX, AR_OBJ_SPHERE: AR_OBJ_SPHERE}
+
+
+def object_array(objects):
+    """ctypes ar_object array from dicts: type ('box' / 'sphere'), center,
+    extent (half extents; sphere radius = max), rotation (Euler degrees,
+    objectToWorld = Rz Ry Rx), material (Material or its keyword dict)."""
+    arr = (_lib.ar_object * max(1, len(objects)))()
+    for k, o in enumerate(objects):
+        arr[k].type = _OBJ_TYPES[o.get("type", "box")]
+        arr[k].center[:] = [float(x) for x in o["center"]]
+        arr[k].extent[:] = [float(x) for x in o["extent"]]
+        arr[k].rotation[:] = [float(x) for x in o.get("rotation", (0.0, 0.0, 0.0))]
+        m = o.get("material") or Material()
+        arr[k].material = (m if isinstance(m, Material) else Material(**m)).to_c()
+    return arr
 
 
 def _ptr(t) -> int:
@@ -189,6 +212,12 @@ class Context:
                 cms[i].materials = C.cast(arr, C.c_void_p)
                 cms[i].nmaterials = len(mats)
         self._check(self._lib.ar_scene_set(self._h, cms, len(meshes)), "ar_scene_set")
+
+    def set_objects(self, objects):
+        """Analytic boxes and spheres (ar_scene_set_objects); object k is
+        primitive ntri + k.  See object_array for the dict keys."""
+        arr = object_array(objects)
+        self._check(self._lib.ar_scene_set_objects(self._h, arr, len(objects)), "ar_scene_set_objects")
 
     def build_bvh(self, width=8, leaf_size=3, bin_count=16, max_depth=60) -> dict:
         o = ar_bvh_opts(width, leaf_size, bin_count, max_depth)

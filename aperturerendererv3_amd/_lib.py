@@ -22,6 +22,10 @@ AR_MISS = 0xFFFFFFFF
 
 AR_MODEL_LAMBERT = 0
 AR_MODEL_GGX = 1
+AR_MODEL_GLASS = 2
+AR_OBJ_BOX = 0
+AR_OBJ_SPHERE = 1
+AR_HIT_INSIDE = 16
 
 FP = C.POINTER(C.c_float)
 UP = C.POINTER(C.c_uint32)
@@ -33,7 +37,13 @@ class ar_device_opts(C.Structure):
 
 class ar_material(C.Structure):
     _fields_ = [("albedo", C.c_float * 3), ("emission", C.c_float * 3), ("specular", C.c_float),
-                ("metallic", C.c_float), ("roughness", C.c_float), ("model", C.c_uint32)]
+                ("metallic", C.c_float), ("roughness", C.c_float), ("model", C.c_uint32),
+                ("ior", C.c_float), ("sigma_a", C.c_float), ("abbe", C.c_float), ("tilt", C.c_float)]
+
+
+class ar_object(C.Structure):
+    _fields_ = [("type", C.c_uint32), ("center", C.c_float * 3), ("extent", C.c_float * 3),
+                ("rotation", C.c_float * 3), ("material", ar_material)]
 
 
 class ar_mesh(C.Structure):
@@ -108,6 +118,7 @@ SIGNATURES = {
     "ar_destroy": (C.c_int, [C.c_void_p]),
     "ar_last_error": (C.c_char_p, [C.c_void_p]),
     "ar_scene_set": (C.c_int, [C.c_void_p, C.POINTER(ar_mesh), C.c_size_t]),
+    "ar_scene_set_objects": (C.c_int, [C.c_void_p, C.POINTER(ar_object), C.c_size_t]),
     "ar_bvh_build": (C.c_int, [C.c_void_p, C.POINTER(ar_bvh_opts)]),
     "ar_bvh_info_get": (C.c_int, [C.c_void_p, C.POINTER(ar_bvh_info)]),
     "ar_trace_closest": (C.c_int, [C.c_void_p, C.POINTER(ar_rays), C.c_size_t, C.POINTER(ar_hits), C.c_void_p]),

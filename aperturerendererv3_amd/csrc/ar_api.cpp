@@ -1,6 +1,7 @@
 // C ABI of libar_hip: context, scene upload, BVH build and the traversal
 // entry points.  Declarations and reference citations are in include/ar.h.
 #include "ar_internal.hpp"
+#include "objects.hpp"
 
 #include <algorithm>
 #include <chrono>
@@ -33,8 +34,7 @@ void free_bvh(ar_ctx* ctx) {
 
 struct Converter {
     const BuildResult& br;
-    const std::vector<float>& verts;
-    const std::vector<uint32_t>& idx;
+    LeafWriter& lw;
     float pad;
     std::vector<GpuNode2> nodes;
     std::vector<float> tris;   // 12 floats per slot
@@ -42,18 +42,7 @@ struct Converter {
 
     uint32_t leaf_ref(const BuildNode& n) {
         const uint32_t first = (uint32_t)(tris.size() / 12);
-        for (uint32_t k = 0; k < n.count; ++k) {
-            const uint32_t prim = br.order[n.first + k];
-            for (int v = 0; v < 3; ++v) {
-                const float* p = &verts[3 * (size_t)idx[3 * (size_t)prim + v]];
-                tris.push_back(p[0]);
-                tris.push_back(p[1]);
-                tris.push_back(p[2]);
-                float w = 0.0f;
-                if (v == 0) std::memcpy(&w, &prim, 4);
-                tris.push_back(w);
-            }
-        }
+        for (uint32_t k = 0; k < n.count; ++k) lw.write(tris, br.order[n.first + k]);
         ++leaves;
         return make_leaf_ref(first, n.count);
     }
@@ -146,6 +135,8 @@ int trace_prologue(ar_ctx* ctx, TraceArgs& ta, size_t n, uint32_t& blocks) {
     ta.spill = ctx->spill;
     ta.threads = blocks * kTraceBlock;
     ta.fetch = ctx->fetch;
+    ta.objs = ctx->scene.nobj ? ctx->scene.objs : nullptr;
+    ta.obj_base = (uint32_t)(ctx->idx.size() / 3);
     return AR_OK;
 }
 
@@ -237,13 +228,19 @@ int ar_scene_set(ar_ctx* ctx, const ar_mesh* meshes, size_t nmesh) {
                     return set_error(ctx, AR_E_INVALID, "mesh " + std::to_string(m) + ": index out of range");
                 idx.push_back(vbase + v);
             }
-            uint32_t mi = me.tri_material ? me.tri_material[t] : 0;
+            const uint32_t mi = me.tri_material ? me.tri_material[t] : 0;
             if (me.nmaterials && mi >= me.nmaterials)
                 return set_error(ctx, AR_E_INVALID, "mesh " + std::to_string(m) + ": material out of range");
             tri_mat.push_back(me.nmaterials ? mbase + mi : 0);
         }
         if (me.nmaterials) {
             if (!me.materials) return set_error(ctx, AR_E_INVALID, "materials is NULL");
+            for (uint32_t k = 0; k < me.nmaterials; ++k) {
+                if (me.materials[k].model > AR_MODEL_GGX)
+                    return set_error(ctx, me.materials[k].model == AR_MODEL_GLASS ? AR_E_UNSUPPORTED : AR_E_INVALID,
+                                     "mesh " + std::to_string(m) +
+                                         ": triangle materials are Lambert or GGX (glass is for analytic objects)");
+            }
             mats.insert(mats.end(), me.materials, me.materials + me.nmaterials);
         }
     }
@@ -270,6 +267,37 @@ int ar_scene_set(ar_ctx* ctx, const ar_mesh* meshes, size_t nmesh) {
     return AR_OK;
 }
 
+int ar_scene_set_objects(ar_ctx* ctx, const ar_object* objects, size_t nobj) {
+    if (!ctx) return AR_E_INVALID;
+    if (nobj && !objects) return set_error(ctx, AR_E_INVALID, "objects is NULL");
+    if (nobj > (size_t)(kObjSlot - 1u) || nobj > (size_t)INT32_MAX)
+        return set_error(ctx, AR_E_CAPACITY, "too many objects");
+    for (size_t k = 0; k < nobj; ++k) {
+        const ar_object& o = objects[k];
+        const std::string who = "object " + std::to_string(k);
+        if (o.type != AR_OBJ_BOX && o.type != AR_OBJ_SPHERE) return set_error(ctx, AR_E_INVALID, who + ": unknown type");
+        for (int a = 0; a < 3; ++a)
+            if (!std::isfinite(o.center[a]) || !std::isfinite(o.extent[a]) || !std::isfinite(o.rotation[a]))
+                return set_error(ctx, AR_E_INVALID, who + ": non-finite center, extent or rotation");
+        const float ex = std::fabs(o.extent[0]), ey = std::fabs(o.extent[1]), ez = std::fabs(o.extent[2]);
+        if (o.type == AR_OBJ_BOX ? !(ex > 0.0f && ey > 0.0f && ez > 0.0f) : !(std::max(ex, std::max(ey, ez)) > 0.0f))
+            return set_error(ctx, AR_E_INVALID, who + ": empty extent");
+        if (o.material.model > AR_MODEL_GLASS) return set_error(ctx, AR_E_INVALID, who + ": unknown material model");
+        if (o.material.model == AR_MODEL_GLASS && !(o.material.ior > 0.0f))
+            return set_error(ctx, AR_E_INVALID, who + ": glass needs ior > 0");
+    }
+    ctx->objs.assign(objects, objects + nobj);
+    (void)hipSetDevice(ctx->device);
+    free_bvh(ctx);
+    free_scene(ctx);
+    if (ctx->guide.cap) {
+        (void)hipDeviceSynchronize();
+        const int rc = guide_clear(ctx, nullptr);
+        if (rc != AR_OK) return rc;
+    }
+    return AR_OK;
+}
+
 int ar_bvh_build(ar_ctx* ctx, const ar_bvh_opts* opts) {
     if (!ctx) return AR_E_INVALID;
     if (!ctx->scene_set) return set_error(ctx, AR_E_NOSCENE, "no scene: call ar_scene_set first");
@@ -282,7 +310,9 @@ int ar_bvh_build(ar_ctx* ctx, const ar_bvh_opts* opts) {
     (void)hipSetDevice(ctx->device);
     free_bvh(ctx);
     const auto t0 = std::chrono::steady_clock::now();
-    const uint32_t n = (uint32_t)(ctx->idx.size() / 3);
+    const uint32_t ntri = (uint32_t)(ctx->idx.size() / 3);
+    const uint32_t nobj = (uint32_t)ctx->objs.size();
+    const uint32_t n = ntri + nobj;
     ctx->info = ar_bvh_info{};
     ctx->info.width = o.width;
     if (n == 0) {
@@ -292,7 +322,18 @@ int ar_bvh_build(ar_ctx* ctx, const ar_bvh_opts* opts) {
     }
     std::vector<float> lo(3 * (size_t)n), hi(3 * (size_t)n), cen(3 * (size_t)n);
     float mag = 0.0f;
-    for (uint32_t t = 0; t < n; ++t) {
+    for (uint32_t k = 0; k < nobj; ++k) {
+        const ar_object& ob = ctx->objs[k];
+        const float e[3] = {std::fabs(ob.extent[0]), std::fabs(ob.extent[1]), std::fabs(ob.extent[2])};
+        const obj::Basis bs = obj::basis_from_euler(ob.rotation);
+        const size_t i = 3 * ((size_t)ntri + k);
+        obj::proxy(ob.type, ob.center, e, bs, &lo[i], &hi[i]);
+        for (int a = 0; a < 3; ++a) {
+            cen[i + a] = ob.center[a];
+            mag = std::max(mag, std::max(std::fabs(lo[i + a]), std::fabs(hi[i + a])));
+        }
+    }
+    for (uint32_t t = 0; t < ntri; ++t) {
         for (int a = 0; a < 3; ++a) {
             const float x = ctx->verts[3 * (size_t)ctx->idx[3 * (size_t)t + 0] + a];
             const float y = ctx->verts[3 * (size_t)ctx->idx[3 * (size_t)t + 1] + a];
@@ -314,9 +355,10 @@ int ar_bvh_build(ar_ctx* ctx, const ar_bvh_opts* opts) {
     uint64_t leaves = 0;
     int depth = br.depth;
     const void* node_src = nullptr;
-    Converter cv{br, ctx->verts, ctx->idx, pad, {}, {}, 0};
+    LeafWriter lw{ctx->verts, ctx->idx, ntri, {}};
+    Converter cv{br, lw, pad, {}, {}, 0};
     if (o.width == 8) {
-        convert_bvh8(br, ctx->verts, ctx->idx, pad, nodes8, tris, leaves, depth);
+        convert_bvh8(br, lw, pad, nodes8, tris, leaves, depth);
         node_src = nodes8.data();
         node_bytes = nodes8.size() * sizeof(GpuNode8);
         ctx->bvh.stack_entries = 7u * (uint32_t)(depth + 1) + 2u;
@@ -342,6 +384,7 @@ int ar_bvh_build(ar_ctx* ctx, const ar_bvh_opts* opts) {
                                       hipMemcpyHostToDevice), "upload triangles");
     if (rc != AR_OK) { free_bvh(ctx); return rc; }
     ctx->bvh.width = o.width;
+    ctx->bvh.obj_slot.swap(lw.obj_slot);
     ctx->bvh.node_count = (uint32_t)ctx->info.node_count;
     ctx->bvh.tri_slots = (uint32_t)(tris.size() / 12);
     ctx->bvh.root = 0;

@@ -48,6 +48,11 @@ inline uint32_t make_leaf_ref(uint32_t first, uint32_t count) {
     return kLeafBit | ((count - 1u) << 27) | first;
 }
 
+// A leaf slot whose first word carries kObjSlot holds analytic object
+// (word & ~kObjSlot) instead of a triangle: (0, 0, 0, word), (0), (0).
+// Triangle prim ids stay below 2^27, so the bit is free.
+constexpr uint32_t kObjSlot = 0x80000000u;
+
 struct GpuNode2 {
     float a[4];
     float b[4];
@@ -96,9 +101,20 @@ struct BuildResult {
 BuildResult build_bvh2(const float* lo, const float* hi, const float* cen, uint32_t n,
                        uint32_t leaf_size, uint32_t bins, int max_depth);
 
+// Writes leaf slots (12 floats each) for primitives in leaf order: triangles
+// (prim < ntri) as their vertices, objects as a kObjSlot marker; records the
+// slot of every object (its solo-trace leaf).
+struct LeafWriter {
+    const std::vector<float>& verts;
+    const std::vector<uint32_t>& idx;
+    uint32_t ntri;
+    std::vector<uint32_t> obj_slot;
+    void write(std::vector<float>& slots, uint32_t prim);
+};
+
 // Collapse into quantised 8-wide nodes (bvh_wide.cpp); depth = wide-tree depth.
-bool convert_bvh8(const BuildResult& br, const std::vector<float>& verts, const std::vector<uint32_t>& idx, float pad,
-                  std::vector<GpuNode8>& nodes, std::vector<float>& tris, uint64_t& leaves, int& depth);
+bool convert_bvh8(const BuildResult& br, LeafWriter& lw, float pad, std::vector<GpuNode8>& nodes,
+                  std::vector<float>& tris, uint64_t& leaves, int& depth);
 
 struct DeviceBVH {
     uint32_t width = 2;          // 2: GpuNode2 nodes, 8: GpuNode8 nodes
@@ -109,6 +125,7 @@ struct DeviceBVH {
     uint32_t root = kRefEmpty;   // root interior node index (0) or kRefEmpty
     float origin_limit = 0.0f;   // kOriginLimit * scene magnitude
     uint32_t stack_entries = kStackMax;  // per-ray traversal stack bound (incl. sentinel)
+    std::vector<uint32_t> obj_slot;      // leaf slot of each analytic object
 };
 
 // Shading data of the scene, prim order (uploaded with the BVH).  The
@@ -127,8 +144,9 @@ struct GpuMaterial {
     float pad1;
     float cem[3];
     float pad2;
+    float ior, sigma_a, abbe, tilt;   // glass
 };
-static_assert(sizeof(GpuMaterial) == 80, "GpuMaterial must be 80 bytes");
+static_assert(sizeof(GpuMaterial) == 96, "GpuMaterial must be 96 bytes");
 
 namespace spec {
 size_t lut_floats();
@@ -149,6 +167,9 @@ struct SceneDev {
     uint32_t ntri = 0;
     std::vector<GpuMaterial> host_mats;   // host copy (spectral coefficients filled on first spectral render)
     bool spectral_ready = false;
+    float4* objs = nullptr;               // analytic objects, obj::kRecord float4 each (objects.hpp)
+    uint32_t nobj = 0;
+    std::vector<float4> host_objs;
     float box_lo[3] = {0, 0, 0};          // scene bounds (guiding-network inputs)
     float box_inv[3] = {1, 1, 1};         // 1 / extent
 };
@@ -273,6 +294,7 @@ struct ar_ctx {
     std::vector<uint32_t> idx;        // 3 per triangle (global vertex ids)
     std::vector<uint32_t> tri_mat;    // per triangle
     std::vector<ar_material> mats;
+    std::vector<ar_object> objs;      // analytic objects (prims ntri ..)
     bool scene_set = false;
 
     ar::DeviceBVH bvh;
@@ -315,6 +337,8 @@ struct TraceArgs {
     uint32_t threads;
     uint32_t* fetch;     // device ray-refill counter
     const uint32_t* roots = nullptr;   // per-ray root node (batched scenes), else `root`
+    const float4* objs = nullptr;      // analytic object records; NULL: triangle-only scene
+    uint32_t obj_base = 0;             // prim id of object 0 (= triangle count)
 };
 // n_dev (optional): device pointer to the ray count, read by the kernel
 // (then n is only the capacity used to size the grid).

@@ -84,6 +84,7 @@ int batch_merge(ar_ctx* out, ar_ctx* const* scenes, uint32_t n) {
         if (!c->bvh_ready) return set_error(out, AR_E_NOSCENE, "batch: a scene has no BVH");
         if (c->bvh.width != width) return set_error(out, AR_E_INVALID, "batch: scenes must share the BVH width");
         if (!c->batch.empty()) return set_error(out, AR_E_INVALID, "batch: a batch cannot contain a batch");
+        if (c->scene.nobj) return set_error(out, AR_E_UNSUPPORTED, "batch: scenes with analytic objects are not merged");
         nodes += c->bvh.node_count;
         slots += c->bvh.tri_slots;
         tris += c->scene.ntri;

@@ -25,8 +25,7 @@ float area_of(const BuildNode& n) {
 
 struct Collapser {
     const BuildResult& br;
-    const std::vector<float>& verts;
-    const std::vector<uint32_t>& idx;
+    LeafWriter& lw;
     float pad;
     std::vector<GpuNode8> nodes;
     std::vector<float> tris;
@@ -35,18 +34,7 @@ struct Collapser {
 
     uint32_t leaf_ref(const BuildNode& n) {
         const uint32_t first = (uint32_t)(tris.size() / 12);
-        for (uint32_t k = 0; k < n.count; ++k) {
-            const uint32_t prim = br.order[n.first + k];
-            for (int v = 0; v < 3; ++v) {
-                const float* p = &verts[3 * (size_t)idx[3 * (size_t)prim + v]];
-                tris.push_back(p[0]);
-                tris.push_back(p[1]);
-                tris.push_back(p[2]);
-                float w = 0.0f;
-                if (v == 0) std::memcpy(&w, &prim, 4);
-                tris.push_back(w);
-            }
-        }
+        for (uint32_t k = 0; k < n.count; ++k) lw.write(tris, br.order[n.first + k]);
         ++leaves;
         return make_leaf_ref(first, n.count);
     }
@@ -136,9 +124,30 @@ struct Collapser {
 
 } // namespace
 
-bool convert_bvh8(const BuildResult& br, const std::vector<float>& verts, const std::vector<uint32_t>& idx, float pad,
-                  std::vector<GpuNode8>& nodes, std::vector<float>& tris, uint64_t& leaves, int& depth) {
-    Collapser c{br, verts, idx, pad, {}, {}, 0, 0};
+void LeafWriter::write(std::vector<float>& slots, uint32_t prim) {
+    const size_t at = slots.size();
+    slots.resize(at + 12, 0.0f);
+    float* s = &slots[at];
+    if (prim < ntri) {
+        for (int v = 0; v < 3; ++v) {
+            const float* p = &verts[3 * (size_t)idx[3 * (size_t)prim + v]];
+            s[4 * v + 0] = p[0];
+            s[4 * v + 1] = p[1];
+            s[4 * v + 2] = p[2];
+        }
+        std::memcpy(&s[3], &prim, 4);
+    } else {
+        const uint32_t k = prim - ntri;
+        const uint32_t w = kObjSlot | k;
+        std::memcpy(&s[3], &w, 4);
+        if (obj_slot.size() <= k) obj_slot.resize(k + 1, kRefEmpty);
+        obj_slot[k] = (uint32_t)(at / 12);
+    }
+}
+
+bool convert_bvh8(const BuildResult& br, LeafWriter& lw, float pad, std::vector<GpuNode8>& nodes,
+                  std::vector<float>& tris, uint64_t& leaves, int& depth) {
+    Collapser c{br, lw, pad, {}, {}, 0, 0};
     c.nodes.reserve(br.nodes.size() / 4 + 1);
     c.tris.reserve(12 * br.order.size());
     c.build(0, 0);

@@ -21,6 +21,7 @@
 #include "path_common.hpp"
 #include "guide.hpp"
 #include "mlp.hpp"
+#include "objects.hpp"
 
 #include <cfloat>
 #include <cmath>
@@ -1040,7 +1041,7 @@ uint32_t grid_for(size_t n, uint32_t block = kBlock) { return (uint32_t)((n + bl
 void free_scene(ar_ctx* ctx) {
     SceneDev& s = ctx->scene;
     for (void* ptr : {(void*)s.prims, (void*)s.mats, (void*)s.light_tri, (void*)s.light_area, (void*)s.light_prob,
-                      (void*)s.light_cdf})
+                      (void*)s.light_cdf, (void*)s.objs})
         if (ptr) (void)hipFree(ptr);
     s = SceneDev{};
 }
@@ -1100,9 +1101,46 @@ int upload_scene(ar_ctx* ctx) {
         std::memcpy(&prims[12 * (size_t)t + 3], &ctx->tri_mat[t], 4);
         std::memcpy(&prims[12 * (size_t)t + 7], &light, 4);
     }
+    // analytic objects: materials after the mesh materials, lights after the triangles
+    const uint32_t nobj = (uint32_t)ctx->objs.size();
+    std::vector<float4> objs((size_t)obj::kRecord * std::max(nobj, 1u), make_float4(0, 0, 0, 0));
+    for (uint32_t k = 0; k < nobj; ++k) {
+        const ar_object& ob = ctx->objs[k];
+        const float e[3] = {std::fabs(ob.extent[0]), std::fabs(ob.extent[1]), std::fabs(ob.extent[2])};
+        const obj::Basis bs = obj::basis_from_euler(ob.rotation);
+        float plo[3], phi[3];
+        obj::proxy(ob.type, ob.center, e, bs, plo, phi);
+        for (int a = 0; a < 3; ++a) { lo[a] = std::min(lo[a], plo[a]); hi[a] = std::max(hi[a], phi[a]); }
+        const ar_material& m = ob.material;
+        int32_t light = -1;
+        if (m.emission[0] > 0.0f || m.emission[1] > 0.0f || m.emission[2] > 0.0f) {
+            const float ar = obj::area(ob.type, e);
+            const float l = m.emission[0] * 0.2126f + m.emission[1] * 0.7152f + m.emission[2] * 0.0722f;
+            const float w = std::max(ar * l, 1.0e-20f);
+            light = (int32_t)ltri.size();
+            ltri.push_back((int32_t)(n + k));
+            larea.push_back(ar);
+            lprob.push_back(w);
+            total += w;
+        }
+        const uint32_t mat_id = (uint32_t)ctx->mats.size() + k;
+        const uint32_t solo = k < ctx->bvh.obj_slot.size() ? make_leaf_ref(ctx->bvh.obj_slot[k], 1) : kRefEmpty;
+        float4* r = &objs[(size_t)obj::kRecord * k];
+        uint32_t tb = ob.type;
+        r[0] = make_float4(ob.center[0], ob.center[1], ob.center[2], 0.0f);
+        std::memcpy(&r[0].w, &tb, 4);
+        r[1] = make_float4(e[0], e[1], e[2], std::fmax(std::fmax(e[0], e[1]), e[2]));
+        r[2] = make_float4(bs.w2o[0], bs.w2o[1], bs.w2o[2], 0.0f);
+        std::memcpy(&r[2].w, &mat_id, 4);
+        r[3] = make_float4(bs.w2o[3], bs.w2o[4], bs.w2o[5], 0.0f);
+        std::memcpy(&r[3].w, &light, 4);
+        r[4] = make_float4(bs.w2o[6], bs.w2o[7], bs.w2o[8], 0.0f);
+        std::memcpy(&r[4].w, &solo, 4);
+    }
+    const bool any_prim = n + nobj > 0;
     for (int a = 0; a < 3; ++a) {
-        const float ext = n ? hi[a] - lo[a] : 0.0f;
-        s.box_lo[a] = n ? lo[a] : 0.0f;
+        const float ext = any_prim ? hi[a] - lo[a] : 0.0f;
+        s.box_lo[a] = any_prim ? lo[a] : 0.0f;
         s.box_inv[a] = ext > 0.0f ? 1.0f / ext : 1.0f;
     }
     float acc = 0.0f;
@@ -1111,15 +1149,18 @@ int upload_scene(ar_ctx* ctx) {
         acc += lprob[i];
         lcdf.push_back(acc);
     }
-    std::vector<GpuMaterial> mats(ctx->mats.size());
+    std::vector<GpuMaterial> mats(ctx->mats.size() + nobj);
     for (size_t i = 0; i < mats.size(); ++i) {
-        const ar_material& m = ctx->mats[i];
+        const ar_material& m = i < ctx->mats.size() ? ctx->mats[i] : ctx->objs[i - ctx->mats.size()].material;
         GpuMaterial g{};
         for (int k = 0; k < 3; ++k) { g.albedo[k] = m.albedo[k]; g.emission[k] = m.emission[k]; }
         g.specular = m.specular; g.metallic = m.metallic; g.roughness = m.roughness; g.model = m.model;
+        g.ior = m.ior; g.sigma_a = m.sigma_a; g.abbe = m.abbe; g.tilt = m.tilt;
         mats[i] = g;
     }
     s.nlights = (uint32_t)ltri.size();
+    s.nobj = nobj;
+    s.host_objs = objs;
     s.host_mats = mats;
     s.spectral_ready = false;
     auto up = [&](void** dst, const void* src, size_t bytes, const char* what) -> int {
@@ -1135,6 +1176,7 @@ int upload_scene(ar_ctx* ctx) {
     if (rc == AR_OK) rc = up((void**)&s.light_area, larea.empty() ? nullptr : larea.data(), larea.size() * 4, "upload lights");
     if (rc == AR_OK) rc = up((void**)&s.light_prob, lprob.empty() ? nullptr : lprob.data(), lprob.size() * 4, "upload lights");
     if (rc == AR_OK) rc = up((void**)&s.light_cdf, lcdf.empty() ? nullptr : lcdf.data(), lcdf.size() * 4, "upload lights");
+    if (rc == AR_OK) rc = up((void**)&s.objs, objs.data(), objs.size() * sizeof(float4), "upload objects");
     if (rc != AR_OK) free_scene(ctx);
     return rc;
 }
@@ -1303,6 +1345,8 @@ static int make_job(ar_ctx* ctx, uint32_t n, const ar_camera* cams, const ar_ren
         return set_error(ctx, AR_E_INVALID, "spectral must be AR_SPECTRAL_RGB, _SINGLE or _HERO4");
     if (o.spectral != AR_SPECTRAL_RGB && o.sky == AR_SKY_GRADIENT)
         return set_error(ctx, AR_E_UNSUPPORTED, "the gradient sky is RGB-only; use AR_SKY_CONSTANT in spectral modes");
+    if (ctx->scene.nobj)
+        return set_error(ctx, AR_E_UNSUPPORTED, "rendering scenes with analytic objects is not supported yet");
     J.guiding = o.guiding != AR_GUIDING_OFF;
     J.learned = o.guiding == AR_GUIDING_MLP;
     if (J.guiding && n != 1) return set_error(ctx, AR_E_UNSUPPORTED, "guiding renders one scene at a time");

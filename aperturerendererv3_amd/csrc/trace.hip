@@ -1,7 +1,8 @@
 // Closest-hit and any-hit BVH2 traversal for gfx950.
 //
 // Replaces TraverseBVH<bAnyHit> / AnyHitSegmentBVH of the reference
-// (core/render/intersection.cuh:148-310) for triangle meshes.  Design:
+// (core/render/intersection.cuh:148-310) for triangle meshes and the
+// analytic box / sphere objects (objects.hpp).  Design:
 //  * persistent grid: each wave walks 64 consecutive rays at a time, so ray
 //    and hit arrays (SoA) are read and written fully coalesced;
 //  * 64-byte nodes holding both child boxes: one node fetch tests two boxes;
@@ -12,6 +13,7 @@
 //    FMA contraction and with IEEE division, so hit ids, t and barycentrics
 //    are bit-identical to a CPU evaluation of the same formula.
 #include "ar_internal.hpp"
+#include "objects.hpp"
 
 #include <cfloat>
 
@@ -20,6 +22,7 @@ namespace {
 
 struct RayPre {
     float ox, oy, oz;
+    float dx, dy, dz;      // raw direction (analytic-object tests only)
     float idx, idy, idz;   // 1/d, +-FLT_MAX for zero components
     float nox, noy, noz;   // -o/d
     int kx, ky, kz;
@@ -47,6 +50,7 @@ __device__ __forceinline__ RayPre ray_setup(float ox, float oy, float oz, float 
                                             float origin_limit) {
     RayPre p;
     p.ox = ox; p.oy = oy; p.oz = oz;
+    p.dx = dx; p.dy = dy; p.dz = dz;
     p.idx = inv_safe(dx); p.idy = inv_safe(dy); p.idz = inv_safe(dz);
     p.nox = -ox * p.idx; p.noy = -oy * p.idy; p.noz = -oz * p.idz;
     const float ax = fabsf(dx), ay = fabsf(dy), az = fabsf(dz);
@@ -262,18 +266,31 @@ __device__ __forceinline__ uint32_t interior8(const GpuNode8* __restrict__ nodes
     return nchild;
 }
 
-// Leaf: every triangle in it, hit rule "smallest t, then smallest id".
-// Any hit ends the traversal on its first accepted triangle.
-template <bool ANY>
-__device__ __forceinline__ uint32_t leaf(const float4* __restrict__ tris, Trav& s, Stack& st) {
+// Leaf: every primitive in it, hit rule "smallest t, then smallest id".
+// Any hit ends the traversal on its first accepted primitive.  OBJ: the
+// scene has analytic objects, whose slots are tested by the object
+// intersector (prim id = obj_base + object index).
+template <bool ANY, bool OBJ>
+__device__ __forceinline__ uint32_t leaf(const float4* __restrict__ tris, const float4* __restrict__ objs,
+                                         uint32_t obj_base, Trav& s, Stack& st) {
     const uint32_t first = s.ref & kMaxTriSlots;
     const uint32_t cnt = ((s.ref >> 27) & 15u) + 1u;
     for (uint32_t k = 0; k < cnt; ++k) {
         const float4* tp = tris + 3 * (size_t)(first + k);
-        const float4 va = tp[0], vb = tp[1], vc = tp[2];
-        const uint32_t prim = __float_as_uint(va.w);
+        const float4 va = tp[0];
+        uint32_t prim = __float_as_uint(va.w);
         float t;
-        if (!tri_test<false>(s.p, va, vb, vc, t)) continue;
+        if (OBJ && (prim & kObjSlot)) {
+            const uint32_t ok = prim & ~kObjSlot;
+            bool inside;
+            if (!obj::test(objs + (size_t)obj::kRecord * ok, s.p.ox, s.p.oy, s.p.oz, s.p.dx, s.p.dy, s.p.dz, t,
+                           inside))
+                continue;
+            prim = obj_base + ok;
+        } else {
+            const float4 vb = tp[1], vc = tp[2];
+            if (!tri_test<false>(s.p, va, vb, vc, t)) continue;
+        }
         const bool accept = t >= s.tmin &&
             (s.have ? (t < s.best_t || (t == s.best_t && prim < s.best_p)) : t < s.tmax);
         if (accept) {
@@ -294,10 +311,10 @@ constexpr int kStepBudget = 12;   // traversal steps between two ray-refill roun
 // lane whose ray is finished takes the next ray index from a global counter
 // (one atomic per wave and round), so a wave never idles on its slowest ray.
 // ANY: sets bit idx of occ (zeroed beforehand) for occluded rays.
-template <int W, bool ANY, bool COUNT>
+template <int W, bool ANY, bool COUNT, bool OBJ>
 __global__ __launch_bounds__(kTraceBlock) void k_trace(
     const void* __restrict__ nodes, const float4* __restrict__ tris, uint32_t root,
-    const uint32_t* __restrict__ roots, float origin_limit,
+    const uint32_t* __restrict__ roots, float origin_limit, const float4* __restrict__ objs, uint32_t obj_base,
     ar_rays r, size_t n, const uint32_t* __restrict__ n_dev, ar_hits h, uint32_t* __restrict__ occ,
     int* __restrict__ spill, uint32_t threads, uint32_t* __restrict__ cnt_box, uint32_t* __restrict__ cnt_tri,
     uint32_t* __restrict__ fetch) {
@@ -356,7 +373,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(
                         if (COUNT) nb += 2;
                     }
                 } else {
-                    const uint32_t c = leaf<ANY>(tris, s, st);
+                    const uint32_t c = leaf<ANY, OBJ>(tris, objs, obj_base, s, st);
                     if (COUNT) nt += c;
                 }
             }
@@ -371,11 +388,20 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(
                         h.u[i] = 0.0f; h.v[i] = 0.0f; h.flags[i] = 0u;
                     } else {
                         const float4* tp = tris + 3 * (size_t)s.best_slot;
-                        const float4 va = tp[0], vb = tp[1], vc = tp[2];
+                        const float4 va = tp[0];
                         float tt, u = 0.0f, v = 0.0f;
                         uint32_t flags = 0;
-                        tri_test<true>(s.p, va, vb, vc, tt, &u, &v, &flags);
-                        h.prim[i] = __float_as_uint(va.w);
+                        if (OBJ && (__float_as_uint(va.w) & kObjSlot)) {
+                            bool inside = false;
+                            const uint32_t ok = __float_as_uint(va.w) & ~kObjSlot;
+                            obj::test(objs + (size_t)obj::kRecord * ok, s.p.ox, s.p.oy, s.p.oz, s.p.dx, s.p.dy,
+                                      s.p.dz, tt, inside);
+                            flags = inside ? AR_HIT_INSIDE : 0u;
+                        } else {
+                            const float4 vb = tp[1], vc = tp[2];
+                            tri_test<true>(s.p, va, vb, vc, tt, &u, &v, &flags);
+                        }
+                        h.prim[i] = s.best_p;
                         h.t[i] = s.best_t;
                         h.u[i] = u; h.v[i] = v; h.flags[i] = flags;
                     }
@@ -437,10 +463,10 @@ __global__ void k_shadow_rays(ar_rays r, ar_hits h, size_t n, float tx, float ty
 int trace_blocks_per_cu(int width, bool any) {
     int b = 0;
     hipError_t e;
-    if (width == 8) e = any ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_trace<8, true, false>, kTraceBlock, 0)
-                            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_trace<8, false, false>, kTraceBlock, 0);
-    else if (width == 2) e = any ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_trace<2, true, false>, kTraceBlock, 0)
-                                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_trace<2, false, false>, kTraceBlock, 0);
+    if (width == 8) e = any ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_trace<8, true, false, false>, kTraceBlock, 0)
+                            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_trace<8, false, false, false>, kTraceBlock, 0);
+    else if (width == 2) e = any ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_trace<2, true, false, false>, kTraceBlock, 0)
+                                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_trace<2, false, false, false>, kTraceBlock, 0);
     else {
         int m = 1;
         for (int w : {2, 8})
@@ -463,14 +489,17 @@ hipError_t launch_trace_closest(const TraceArgs& ta, const ar_rays& r, size_t n,
     hipError_t e = prepare(ta, nullptr, n, n_dev, s);
     if (e != hipSuccess) return e;
     const void* nd = ta.nodes;
-#define AR_LAUNCH_CLOSEST(W, C)                                                                                  \
-    hipLaunchKernelGGL((k_trace<W, false, C>), dim3(blocks), dim3(kTraceBlock), 0, s, nd, ta.tris, ta.root,     \
-                       ta.roots, ta.origin_limit, r, n, n_dev, h, nullptr, ta.spill, ta.threads, cnt_box, cnt_tri,  \
-                       ta.fetch)
+#define AR_LAUNCH_CLOSEST(W, C, O)                                                                               \
+    hipLaunchKernelGGL((k_trace<W, false, C, O>), dim3(blocks), dim3(kTraceBlock), 0, s, nd, ta.tris, ta.root,  \
+                       ta.roots, ta.origin_limit, ta.objs, ta.obj_base, r, n, n_dev, h, nullptr, ta.spill,      \
+                       ta.threads, cnt_box, cnt_tri, ta.fetch)
+    const bool ob = ta.objs != nullptr;
     if (ta.width == 8) {
-        if (cnt_box) AR_LAUNCH_CLOSEST(8, true); else AR_LAUNCH_CLOSEST(8, false);
+        if (cnt_box) { if (ob) AR_LAUNCH_CLOSEST(8, true, true); else AR_LAUNCH_CLOSEST(8, true, false); }
+        else { if (ob) AR_LAUNCH_CLOSEST(8, false, true); else AR_LAUNCH_CLOSEST(8, false, false); }
     } else {
-        if (cnt_box) AR_LAUNCH_CLOSEST(2, true); else AR_LAUNCH_CLOSEST(2, false);
+        if (cnt_box) { if (ob) AR_LAUNCH_CLOSEST(2, true, true); else AR_LAUNCH_CLOSEST(2, true, false); }
+        else { if (ob) AR_LAUNCH_CLOSEST(2, false, true); else AR_LAUNCH_CLOSEST(2, false, false); }
     }
 #undef AR_LAUNCH_CLOSEST
     return hipGetLastError();
@@ -482,14 +511,14 @@ hipError_t launch_trace_any(const TraceArgs& ta, const ar_rays& r, size_t n, uin
     if (e != hipSuccess) return e;
     const void* nd = ta.nodes;
     const ar_hits none{nullptr, nullptr, nullptr, nullptr, nullptr};
-    if (ta.width == 8)
-        hipLaunchKernelGGL((k_trace<8, true, false>), dim3(blocks), dim3(kTraceBlock), 0, s, nd, ta.tris, ta.root,
-                           ta.roots, ta.origin_limit, r, n, n_dev, none, occ, ta.spill, ta.threads, nullptr, nullptr,
-                           ta.fetch);
-    else
-        hipLaunchKernelGGL((k_trace<2, true, false>), dim3(blocks), dim3(kTraceBlock), 0, s, nd, ta.tris, ta.root,
-                           ta.roots, ta.origin_limit, r, n, n_dev, none, occ, ta.spill, ta.threads, nullptr, nullptr,
-                           ta.fetch);
+#define AR_LAUNCH_ANY(W, O)                                                                                      \
+    hipLaunchKernelGGL((k_trace<W, true, false, O>), dim3(blocks), dim3(kTraceBlock), 0, s, nd, ta.tris, ta.root, \
+                       ta.roots, ta.origin_limit, ta.objs, ta.obj_base, r, n, n_dev, none, occ, ta.spill,       \
+                       ta.threads, nullptr, nullptr, ta.fetch)
+    const bool ob = ta.objs != nullptr;
+    if (ta.width == 8) { if (ob) AR_LAUNCH_ANY(8, true); else AR_LAUNCH_ANY(8, false); }
+    else { if (ob) AR_LAUNCH_ANY(2, true); else AR_LAUNCH_ANY(2, false); }
+#undef AR_LAUNCH_ANY
     return hipGetLastError();
 }
 

@@ -219,6 +219,53 @@ def cornell_box(light_le=(17.0, 12.0, 4.0)):
     return (np.asarray(V, np.float32), np.asarray(F, np.uint32), np.asarray(M, np.uint32), mats)
 
 
+CORNELL_THICK = 0.01
+
+
+def cornell_objects(light_le=(17.0, 12.0, 4.0)):
+    """The Cornell box in analytic objects, as the reference builds it
+    (CreateDiffuseCornellBox, core/helper/scene_helper.h:92-142): five slabs
+    of half-thickness 0.01 outside the unit box (floor, ceiling and back grey
+    0.75, left red (0.63, 0.065, 0.05), right green (0.14, 0.45, 0.09)), a box
+    light of 0.25 x 0.25 footprint just under the ceiling, and the two boxes
+    of cornell_box as rotated OBJ_CUBEs.  Returns a list of object dicts
+    (Context.set_objects)."""
+    t = CORNELL_THICK
+    grey = {"albedo": (0.75, 0.75, 0.75)}
+    return [
+        {"type": "box", "center": (0.0, -0.5 - t, 0.0), "extent": (0.5, t, 0.5), "material": grey},
+        {"type": "box", "center": (0.0, 0.5 + t, 0.0), "extent": (0.5, t, 0.5), "material": grey},
+        {"type": "box", "center": (0.0, 0.5 - t, 0.0), "extent": (0.125, t, 0.125),
+         "material": {"albedo": (0.0, 0.0, 0.0), "emission": tuple(light_le)}},
+        {"type": "box", "center": (0.0, 0.0, -0.5 - t), "extent": (0.5, 0.5, t), "material": grey},
+        {"type": "box", "center": (-0.5 - t, 0.0, 0.0), "extent": (t, 0.5, 0.5),
+         "material": {"albedo": (0.63, 0.065, 0.05)}},
+        {"type": "box", "center": (0.5 + t, 0.0, 0.0), "extent": (t, 0.5, 0.5),
+         "material": {"albedo": (0.14, 0.45, 0.09)}},
+        {"type": "box", "center": (-0.16, -0.2, -0.14), "extent": (0.15, 0.3, 0.15), "rotation": (0.0, 17.0, 0.0),
+         "material": grey},
+        {"type": "box", "center": (0.17, -0.35, 0.12), "extent": (0.15, 0.15, 0.15), "rotation": (0.0, -17.0, 0.0),
+         "material": grey},
+    ]
+
+
+def random_object_scene(n_tris: int = 2000, n_boxes: int = 30, n_spheres: int = 30, seed: int = 1):
+    """Random triangles (edges ~0.1) and rotated boxes / spheres (non-uniform
+    extents) scattered in [-1, 1]^3: the traversal parity workload for
+    analytic objects.  Returns (V f32, F u32, objects)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    c = rng.uniform(-1.0, 1.0, (n_tris, 1, 3))
+    V = (c + rng.normal(0.0, 0.06, (n_tris, 3, 3))).reshape(-1, 3).astype(np.float32)
+    F = np.arange(3 * n_tris, dtype=np.uint32).reshape(-1, 3)
+    objects = []
+    for k in range(n_boxes + n_spheres):
+        objects.append({"type": "box" if k < n_boxes else "sphere",
+                        "center": tuple(float(x) for x in rng.uniform(-1.0, 1.0, 3)),
+                        "extent": tuple(float(x) for x in rng.uniform(0.03, 0.2, 3)),
+                        "rotation": tuple(float(x) for x in rng.uniform(-180.0, 180.0, 3))})
+    return V, F, objects
+
+
 # --------------------------------------------------------------------------
 # Spectral instanced scene (~500k triangles, 64 triangle lights, Lambert/GGX)
 # --------------------------------------------------------------------------

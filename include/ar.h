@@ -43,6 +43,7 @@ extern "C" {
 #define AR_HIT_W1_ZERO 2u
 #define AR_HIT_W2_ZERO 4u
 #define AR_HIT_BACKFACE 8u
+#define AR_HIT_INSIDE 16u      /* analytic object hit from inside (its u = v = 0) */
 
 typedef struct ar_ctx ar_ctx;
 
@@ -51,19 +52,38 @@ typedef struct {
     uint32_t flags;    /* reserved, 0 */
 } ar_device_opts;
 
-/* Material of a triangle (the triangle-mesh counterpart of the reference's
- * GBuffer / Material, core/render/object.cuh:36-150). */
+/* Material of a triangle or analytic object (the reference's GBuffer /
+ * Material, core/render/object.cuh:36-150). */
 typedef struct {
     float albedo[3];
-    float emission[3];   /* radiance; > 0 makes the triangle a light */
+    float emission[3];   /* radiance; > 0 makes the primitive a light (MAT_LIGHT) */
     float specular;      /* UE-style specular (F0 = 0.08*specular for dielectrics) */
     float metallic;
     float roughness;     /* GGX roughness, clamped to [0.01, 1] */
     uint32_t model;      /* AR_MODEL_* */
+    float ior;           /* glass: refractive index n_d at 587.56 nm (GBuffer IOR_, default 1.5) */
+    float sigma_a;       /* glass: absorption scale of the Beer-Lambert term (sigmaA_, default 100) */
+    float abbe;          /* glass: Abbe number V_d, used in spectral modes (abbeNumber_, default 20) */
+    float tilt;          /* glass: spectral tilt of sigma_a, -1..1 (tiltNumber_, default 0) */
 } ar_material;
 
 #define AR_MODEL_LAMBERT 0u
 #define AR_MODEL_GGX 1u      /* GGX + Lambert mix (core/render/material.cuh:47-165) */
+#define AR_MODEL_GLASS 2u    /* smooth dielectric (material.cuh:116-161, 206-257); analytic objects only */
+
+/* Analytic primitives (the reference's SceneObject of type OBJ_CUBE /
+ * OBJ_SPHERE, core/render/object.cuh:207-305; intersectors
+ * core/object/aabb.cuh:9-106 and core/object/sphere.cuh:10-85). */
+#define AR_OBJ_BOX 0u        /* oriented box of half-extents `extent` */
+#define AR_OBJ_SPHERE 1u     /* sphere of radius max(extent) */
+
+typedef struct {
+    uint32_t type;        /* AR_OBJ_* */
+    float center[3];
+    float extent[3];      /* half extents (absolute values are used) */
+    float rotation[3];    /* Euler angles in degrees; objectToWorld = Rz * Ry * Rx */
+    ar_material material;
+} ar_object;
 
 /* A triangle mesh in host memory.  vertices: nv x 3 floats; indices:
  * ntri x 3 u32; tri_material: ntri u32 indices into materials (NULL: all
@@ -85,10 +105,13 @@ typedef struct {
 } ar_rays;
 
 /* Closest-hit output.  prim = AR_MISS on a miss (then t = +inf, u = v = 0,
- * flags = 0).  Otherwise prim is the scene triangle index (meshes are
- * numbered in the order given to ar_scene_set, triangles consecutively),
- * hit point = (1-u-v)*v0 + u*v1 + v*v2.  The hit is the one with the
- * smallest t; among equal t, the smallest prim. */
+ * flags = 0).  Otherwise prim is the scene primitive index: triangles first
+ * (meshes in the order given to ar_scene_set, triangles consecutively), then
+ * the analytic objects of ar_scene_set_objects.  For a triangle the hit point
+ * is (1-u-v)*v0 + u*v1 + v*v2.  For an object t is the distance along
+ * normalize(d) (the reference intersectors normalise the direction), so
+ * scenes with objects are traced with unit-length directions.  The hit is
+ * the one with the smallest t; among equal t, the smallest prim. */
 typedef struct {
     uint32_t* prim;
     float* t;
@@ -128,6 +151,13 @@ int ar_abi_version(void);
  * (core/scene.cu:656-732).  Copies the meshes; the caller keeps ownership of
  * its arrays.  Invalidates the BVH. */
 int ar_scene_set(ar_ctx* ctx, const ar_mesh* meshes, size_t nmesh);
+
+/* Replaces the scene's analytic objects (the box and sphere SceneObjects of
+ * SceneRenderer(const std::vector<SceneObject>&), core/scene.hpp:94-101).
+ * Object k is primitive ntri + k.  An object with emission is a light,
+ * sampled by the facing-facet box sampler or the spherical-cap sampler
+ * (core/nee/nee.cuh:153-310).  Copies the array; invalidates the BVH. */
+int ar_scene_set_objects(ar_ctx* ctx, const ar_object* objects, size_t nobj);
 
 /* --- acceleration structure ----------------------------------------------
  * Replaces BuildBVH (core/bvh/bvh.cuh:142-422) as called from

@@ -96,6 +96,55 @@ void orc_trace_brute(const float* verts, const uint32_t* idx, int ntri,
 void orc_trace_any_brute(const float* verts, const uint32_t* idx, int ntri,
                          orc_rays rays, size_t n, uint32_t* occluded_bits, int threads);
 
+/* ---------------- analytic objects (trace.c) ---------------- */
+
+#define ORC_OBJ_BOX 0u        /* OBJ_CUBE: oriented box of half-extents `extent` */
+#define ORC_OBJ_SPHERE 1u     /* OBJ_SPHERE: radius = largest extent */
+#define ORC_HIT_INSIDE 16u    /* hit flag: the ray started inside the object */
+
+typedef struct {
+    uint32_t type;
+    float center[3];
+    float extent[3];     /* absolute values */
+    float w2o[9];        /* world-to-object rotation, row-major */
+} orc_object;
+
+/* SceneObject(center, extent, material, rotation, type) (core/render/object.cuh:286-305) */
+void orc_object_init(orc_object* ob, uint32_t type, const float center[3], const float extent[3],
+                     const float rot_deg[3]);
+void orc_object_proxy(const orc_object* ob, float pmin[3], float pmax[3], float pcen[3]);
+float orc_object_area(const orc_object* ob);
+/* first crossing with t > 0 along normalize(d); 1 on a hit */
+int orc_object_test(const orc_object* ob, const float o[3], const float d[3], float* t, int* inside);
+void orc_object_surface(const orc_object* ob, const float o[3], const float d[3], float t, int inside,
+                        float pos[3], float n[3]);
+int orc_object_overlap(const orc_object* ob, const float p[3]);
+
+/* Scene geometry: primitives 0..ntri-1 are triangles, ntri..ntri+nobj-1 objects. */
+typedef struct {
+    const float* verts;
+    const uint32_t* idx;
+    uint32_t ntri;
+    const orc_object* objs;
+    uint32_t nobj;
+} orc_geom;
+
+/* proxies of every primitive of g (arrays of (ntri + nobj) x 3 floats) */
+void orc_geom_proxies(const orc_geom* g, float* pmin, float* pmax, float* pcen);
+void orc_trace_closest_g(const orc_node* nodes, int node_count, const int32_t* indices, const orc_geom* g,
+                         orc_rays rays, size_t n, orc_hits hits, int mode,
+                         uint32_t* boxes, uint32_t* tris, int threads);
+void orc_trace_any_g(const orc_node* nodes, int node_count, const int32_t* indices, const orc_geom* g,
+                     orc_rays rays, size_t n, uint32_t* occluded_bits, int mode,
+                     uint32_t* boxes, uint32_t* tris, int threads);
+void orc_trace_brute_g(const orc_geom* g, orc_rays rays, size_t n, orc_hits hits, int threads);
+void orc_trace_any_brute_g(const orc_geom* g, orc_rays rays, size_t n, uint32_t* occluded_bits, int threads);
+int orc_closest1_g(const orc_node* nodes, int node_count, const int32_t* indices, const orc_geom* g,
+                   const float o[3], const float d[3], float tmin, float tmax,
+                   uint32_t* prim, float* t, float* u, float* v, uint32_t* flags);
+int orc_any1_g(const orc_node* nodes, int node_count, const int32_t* indices, const orc_geom* g,
+               const float o[3], const float d[3], float tmin, float tmax);
+
 /* Maximum depth (root = 0) of a built tree. */
 int orc_bvh_depth(const orc_node* nodes, int node_count);
 

@@ -378,11 +378,209 @@ static void tri_verts(const float* verts, const uint32_t* idx, int p,
     *c = &verts[3 * idx[3 * p + 2]];
 }
 
+/* ------------------------------------------------------------------------ */
+/* Analytic objects: oriented box and sphere (core/object/aabb.cuh:9-106,   */
+/* core/object/sphere.cuh:10-85, SceneObject in core/render/object.cuh).    */
+/* ------------------------------------------------------------------------ */
+
+static float dot3(const float a[3], const float b[3])
+{
+    return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+
+/* C = A * B (float3x3::mul, core/vector.cuh: rows of A dotted with columns of B) */
+static void mat3_mul(const float A[9], const float B[9], float C[9])
+{
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c)
+            C[3 * r + c] = A[3 * r] * B[c] + A[3 * r + 1] * B[3 + c] + A[3 * r + 2] * B[6 + c];
+}
+
+/* normalize (core/vector.cuh:229-238): zero below |v|^2 = 1e-20; the
+ * reciprocal square root is taken as 1 / sqrtf (IEEE) */
+static void normalize3(float v[3])
+{
+    const float l2 = dot3(v, v);
+    if (l2 <= 1e-20f) { v[0] = v[1] = v[2] = 0.0f; return; }
+    const float inv = 1.0f / sqrtf(l2);
+    v[0] = v[0] * inv; v[1] = v[1] * inv; v[2] = v[2] * inv;
+}
+
+static float max3f(const float v[3]) { return fmaxf(fmaxf(v[0], v[1]), v[2]); }
+
+/* SceneObject constructor (object.cuh:286-305): objectToWorld = Rz (Ry Rx)
+ * from Euler angles in degrees (float3x3::rotX/Y/Z, vector.cuh:394-425),
+ * worldToObject = its transpose; extents are taken by absolute value. */
+void orc_object_init(orc_object* ob, uint32_t type, const float center[3], const float extent[3],
+                     const float rot_deg[3])
+{
+    const float k = 3.141592654f / 180.0f;
+    const float ax = rot_deg[0] * k, ay = rot_deg[1] * k, az = rot_deg[2] * k;
+    const float cx = cosf(ax), sx = sinf(ax), cy = cosf(ay), sy = sinf(ay), cz = cosf(az), sz = sinf(az);
+    const float Rx[9] = { 1, 0, 0, 0, cx, -sx, 0, sx, cx };
+    const float Ry[9] = { cy, 0, sy, 0, 1, 0, -sy, 0, cy };
+    const float Rz[9] = { cz, -sz, 0, sz, cz, 0, 0, 0, 1 };
+    float RyRx[9], o2w[9];
+    mat3_mul(Ry, Rx, RyRx);
+    mat3_mul(Rz, RyRx, o2w);
+    ob->type = type;
+    for (int a = 0; a < 3; ++a) { ob->center[a] = center[a]; ob->extent[a] = fabsf(extent[a]); }
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) ob->w2o[3 * r + c] = o2w[3 * c + r];
+}
+
+/* UpdateProxy (object.cuh:259-284): box = centre +- sum |axis_k| * extent_k,
+ * sphere = centre +- max extent; the proxy centre is the object centre. */
+void orc_object_proxy(const orc_object* ob, float pmin[3], float pmax[3], float pcen[3])
+{
+    float half[3];
+    if (ob->type == ORC_OBJ_BOX) {
+        const float* ax = &ob->w2o[0];   /* objectToWorld * e_x = row 0 of worldToObject */
+        const float* ay = &ob->w2o[3];
+        const float* az = &ob->w2o[6];
+        for (int a = 0; a < 3; ++a)
+            half[a] = fabsf(ax[a]) * ob->extent[0] + fabsf(ay[a]) * ob->extent[1] + fabsf(az[a]) * ob->extent[2];
+    } else {
+        const float r = max3f(ob->extent);
+        half[0] = half[1] = half[2] = r;
+    }
+    for (int a = 0; a < 3; ++a) {
+        pmin[a] = ob->center[a] - half[a];
+        pmax[a] = ob->center[a] + half[a];
+        pcen[a] = ob->center[a];
+    }
+}
+
+/* GetArea (object.cuh:239-252): the box's 8 (ex ey + ex ez + ey ez); the
+ * sphere's 4 pi extent.x^2 (extent.x, not the intersected max extent) */
+float orc_object_area(const orc_object* ob)
+{
+    const float ex = ob->extent[0], ey = ob->extent[1], ez = ob->extent[2];
+    if (ob->type == ORC_OBJ_BOX) return 8.0f * (ex * ey + ex * ez + ey * ez);
+    return 4.0f * 3.141592654f * ex * ex;
+}
+
+static void obj_local_ray(const orc_object* ob, const float o[3], const float d[3], float lo[3], float ld[3])
+{
+    const float oc[3] = { o[0] - ob->center[0], o[1] - ob->center[1], o[2] - ob->center[2] };
+    for (int r = 0; r < 3; ++r) {
+        lo[r] = dot3(&ob->w2o[3 * r], oc);
+        ld[r] = dot3(&ob->w2o[3 * r], d);
+    }
+    normalize3(ld);
+}
+
+/* IntersectAABBObject / IntersectSphereObject: the first surface crossing
+ * along the normalised local direction with t > 0 (t is measured along
+ * normalize(d)); inside = the ray starts inside the object. */
+int orc_object_test(const orc_object* ob, const float o[3], const float d[3], float* t, int* inside)
+{
+    float lo[3], ld[3];
+    obj_local_ray(ob, o, d, lo, ld);
+    *inside = 0;
+    if (ob->type == ORC_OBJ_BOX) {
+        float te = -INFINITY, tx = INFINITY;
+        for (int a = 0; a < 3; ++a) {
+            const float inv = inv_safe(ld[a]);
+            const float t1 = (-ob->extent[a] - lo[a]) * inv;
+            const float t2 = (ob->extent[a] - lo[a]) * inv;
+            const float mn = fminf(t1, t2), mx = fmaxf(t1, t2);
+            te = a == 0 ? mn : fmaxf(te, mn);
+            tx = a == 0 ? mx : fminf(tx, mx);
+        }
+        if (te > tx) return 0;
+        if (tx <= 0.0f) return 0;
+        *inside = te < 0.0f;
+        *t = *inside ? tx : te;
+        return 1;
+    }
+    const float r = max3f(ob->extent);
+    const float b = dot3(lo, ld);
+    const float c = dot3(lo, lo) - r * r;
+    const float disc = b * b - c;
+    if (disc < 0.0f) return 0;
+    const float s = sqrtf(disc);
+    float tt = -b - s;
+    if (tt <= 0.0f) { tt = -b + s; *inside = 1; }
+    if (tt <= 0.0f) return 0;
+    *t = tt;
+    return 1;
+}
+
+/* Hit point and normal of an object hit (the tail of IntersectAABBObject /
+ * IntersectSphereObject): local point lo + ld t; box normal along the axis of
+ * largest |p_k| / extent_k, sphere normal p / |p|; flipped for a hit from
+ * inside; both rotated to world space and normalised. */
+void orc_object_surface(const orc_object* ob, const float o[3], const float d[3], float t, int inside,
+                        float pos[3], float n[3])
+{
+    float lo[3], ld[3], lp[3], ln[3];
+    obj_local_ray(ob, o, d, lo, ld);
+    for (int a = 0; a < 3; ++a) lp[a] = lo[a] + ld[a] * t;
+    if (ob->type == ORC_OBJ_BOX) {
+        const float rx = fabsf(lp[0]) / ob->extent[0];
+        const float ry = fabsf(lp[1]) / ob->extent[1];
+        const float rz = fabsf(lp[2]) / ob->extent[2];
+        ln[0] = ln[1] = ln[2] = 0.0f;
+        if (rx >= ry && rx >= rz) ln[0] = lp[0] > 0.0f ? 1.0f : -1.0f;
+        else if (ry >= rz) ln[1] = lp[1] > 0.0f ? 1.0f : -1.0f;
+        else ln[2] = lp[2] > 0.0f ? 1.0f : -1.0f;
+        if (inside) { ln[0] = -ln[0]; ln[1] = -ln[1]; ln[2] = -ln[2]; }
+    } else {
+        ln[0] = lp[0]; ln[1] = lp[1]; ln[2] = lp[2];
+        normalize3(ln);
+        const float sg = inside ? -1.0f : 1.0f;
+        ln[0] = ln[0] * sg; ln[1] = ln[1] * sg; ln[2] = ln[2] * sg;
+    }
+    for (int r = 0; r < 3; ++r) {
+        /* objectToWorld row r = column r of worldToObject */
+        const float col[3] = { ob->w2o[r], ob->w2o[3 + r], ob->w2o[6 + r] };
+        pos[r] = dot3(col, lp) + ob->center[r];
+        n[r] = dot3(col, ln);
+    }
+    normalize3(n);
+}
+
+/* OverlapAABBObject / OverlapSphereObject: strict containment of a point */
+int orc_object_overlap(const orc_object* ob, const float p[3])
+{
+    const float oc[3] = { p[0] - ob->center[0], p[1] - ob->center[1], p[2] - ob->center[2] };
+    float lo[3];
+    for (int r = 0; r < 3; ++r) lo[r] = dot3(&ob->w2o[3 * r], oc);
+    if (ob->type == ORC_OBJ_BOX)
+        return -ob->extent[0] < lo[0] && -ob->extent[1] < lo[1] && -ob->extent[2] < lo[2] &&
+               ob->extent[0] > lo[0] && ob->extent[1] > lo[1] && ob->extent[2] > lo[2];
+    const float r = max3f(ob->extent);
+    return dot3(lo, lo) < r * r;
+}
+
+void orc_geom_proxies(const orc_geom* g, float* pmin, float* pmax, float* pcen)
+{
+    if (g->ntri) orc_tri_proxies(g->verts, g->idx, (int)g->ntri, pmin, pmax, pcen);
+    for (uint32_t k = 0; k < g->nobj; ++k) {
+        const size_t i = 3 * ((size_t)g->ntri + k);
+        orc_object_proxy(&g->objs[k], &pmin[i], &pmax[i], &pcen[i]);
+    }
+}
+
+/* Primitive p of the geometry (triangles first, then objects): 1 when the
+ * ray's line (triangle) or the forward ray (object) crosses it, with t. */
+static int prim_test(const orc_geom* g, uint32_t p, const float o[3], const float d[3], const ray_pre* pre,
+                     float* t)
+{
+    if (p < g->ntri) {
+        const float *a, *b, *c;
+        tri_verts(g->verts, g->idx, (int)p, &a, &b, &c);
+        return tri_test_pre(o, pre, a, b, c, t, NULL, NULL, NULL);
+    }
+    int inside;
+    return orc_object_test(&g->objs[p - g->ntri], o, d, t, &inside);
+}
+
 #define ORC_STACK 128
 
 /* One ray.  any = 1: stop at the first accepted hit.  Returns 1 on a hit. */
-static int traverse_one(const orc_node* nodes, int node_count, const int32_t* indices,
-                        const float* verts, const uint32_t* idx,
+static int traverse_one(const orc_node* nodes, int node_count, const int32_t* indices, const orc_geom* g,
                         const float o[3], const float d[3], float tmin, float tmax,
                         int mode, int any,
                         uint32_t* prim_out, float* t_out, uint32_t* nbox, uint32_t* ntri)
@@ -407,19 +605,17 @@ static int traverse_one(const orc_node* nodes, int node_count, const int32_t* in
             if (!box_hit(o, inv, nd->bmin, nd->bmax, tmin, best_t, pad, mode == 0, &te)) continue;
             if (nd->count > 0) {
                 for (int k = 0; k < nd->count; ++k) {
-                    const int p = indices[nd->first + k];
-                    const float *a, *b, *c;
-                    tri_verts(verts, idx, p, &a, &b, &c);
+                    const uint32_t p = (uint32_t)indices[nd->first + k];
                     float t;
                     nt++;
-                    if (!tri_test_pre(o, &pre, a, b, c, &t, NULL, NULL, NULL)) continue;
+                    if (!prim_test(g, p, o, d, &pre, &t)) continue;
                     int accept;
                     if (mode == 0)
-                        accept = t >= tmin && (have ? (t < best_t || (t == best_t && (uint32_t)p < best_p)) : t < tmax);
+                        accept = t >= tmin && (have ? (t < best_t || (t == best_t && p < best_p)) : t < tmax);
                     else
                         accept = t >= tmin && t < best_t;     /* intersection.cuh:73 */
                     if (accept) {
-                        have = 1; best_t = t; best_p = (uint32_t)p;
+                        have = 1; best_t = t; best_p = p;
                         if (any) goto done;
                     }
                 }
@@ -455,20 +651,34 @@ done:
     return have;
 }
 
-static void finish_hit(const float* verts, const uint32_t* idx, const float o[3], const float d[3],
-                       uint32_t p, orc_hits hits, size_t i)
+/* Full hit record: triangles give barycentrics and edge flags, objects
+ * u = v = 0 and ORC_HIT_INSIDE for a hit from inside. */
+static void hit_record(const orc_geom* g, const float o[3], const float d[3], uint32_t p,
+                       float* t, float* u, float* v, uint32_t* flags)
+{
+    *u = 0.0f; *v = 0.0f; *flags = 0u;
+    if (p < g->ntri) {
+        const float *a, *b, *c;
+        tri_verts(g->verts, g->idx, (int)p, &a, &b, &c);
+        const ray_pre pre = ray_precompute(d);
+        tri_test_pre(o, &pre, a, b, c, t, u, v, flags);
+    } else {
+        int inside = 0;
+        orc_object_test(&g->objs[p - g->ntri], o, d, t, &inside);
+        if (inside) *flags = ORC_HIT_INSIDE;
+    }
+}
+
+static void finish_hit(const orc_geom* g, const float o[3], const float d[3], uint32_t p, orc_hits hits, size_t i)
 {
     hits.prim[i] = p;
     if (p == ORC_MISS) {
         hits.t[i] = INFINITY; hits.u[i] = 0.0f; hits.v[i] = 0.0f; hits.flags[i] = 0u;
         return;
     }
-    const float *a, *b, *c;
-    tri_verts(verts, idx, (int)p, &a, &b, &c);
     float t = 0.0f, u = 0.0f, v = 0.0f;
     uint32_t f = 0;
-    const ray_pre pre = ray_precompute(d);
-    tri_test_pre(o, &pre, a, b, c, &t, &u, &v, &f);
+    hit_record(g, o, d, p, &t, &u, &v, &f);
     hits.t[i] = t; hits.u[i] = u; hits.v[i] = v; hits.flags[i] = f;
 }
 
@@ -482,10 +692,15 @@ static int nthreads(int threads)
 #endif
 }
 
-void orc_trace_closest(const orc_node* nodes, int node_count, const int32_t* indices,
-                       const float* verts, const uint32_t* idx,
-                       orc_rays rays, size_t n, orc_hits hits, int mode,
-                       uint32_t* boxes, uint32_t* tris, int threads)
+static orc_geom tri_geom(const float* verts, const uint32_t* idx, int ntri)
+{
+    orc_geom g = { verts, idx, (uint32_t)(ntri > 0 ? ntri : 0), NULL, 0 };
+    return g;
+}
+
+void orc_trace_closest_g(const orc_node* nodes, int node_count, const int32_t* indices, const orc_geom* g,
+                         orc_rays rays, size_t n, orc_hits hits, int mode,
+                         uint32_t* boxes, uint32_t* tris, int threads)
 {
     const int nt = nthreads(threads);
     (void)nt;
@@ -495,17 +710,16 @@ void orc_trace_closest(const orc_node* nodes, int node_count, const int32_t* ind
         float o[3], d[3], tmin, tmax, t;
         uint32_t p, nb, ntr;
         load_ray(rays, i, o, d, &tmin, &tmax);
-        traverse_one(nodes, node_count, indices, verts, idx, o, d, tmin, tmax, mode, 0, &p, &t, &nb, &ntr);
+        traverse_one(nodes, node_count, indices, g, o, d, tmin, tmax, mode, 0, &p, &t, &nb, &ntr);
         if (boxes) boxes[i] = nb;
         if (tris) tris[i] = ntr;
-        finish_hit(verts, idx, o, d, p, hits, i);
+        finish_hit(g, o, d, p, hits, i);
     }
 }
 
-void orc_trace_any(const orc_node* nodes, int node_count, const int32_t* indices,
-                   const float* verts, const uint32_t* idx,
-                   orc_rays rays, size_t n, uint32_t* occluded_bits, int mode,
-                   uint32_t* boxes, uint32_t* tris, int threads)
+void orc_trace_any_g(const orc_node* nodes, int node_count, const int32_t* indices, const orc_geom* g,
+                     orc_rays rays, size_t n, uint32_t* occluded_bits, int mode,
+                     uint32_t* boxes, uint32_t* tris, int threads)
 {
     const int nt = nthreads(threads);
     (void)nt;
@@ -519,7 +733,7 @@ void orc_trace_any(const orc_node* nodes, int node_count, const int32_t* indices
             float o[3], d[3], tmin, tmax, t;
             uint32_t p, nb, ntr;
             load_ray(rays, i, o, d, &tmin, &tmax);
-            if (traverse_one(nodes, node_count, indices, verts, idx, o, d, tmin, tmax, mode, 1, &p, &t, &nb, &ntr))
+            if (traverse_one(nodes, node_count, indices, g, o, d, tmin, tmax, mode, 1, &p, &t, &nb, &ntr))
                 word |= 1u << b;
             if (boxes) boxes[i] = nb;
             if (tris) tris[i] = ntr;
@@ -528,11 +742,11 @@ void orc_trace_any(const orc_node* nodes, int node_count, const int32_t* indices
     }
 }
 
-void orc_trace_brute(const float* verts, const uint32_t* idx, int ntri,
-                     orc_rays rays, size_t n, orc_hits hits, int threads)
+void orc_trace_brute_g(const orc_geom* g, orc_rays rays, size_t n, orc_hits hits, int threads)
 {
     const int nt = nthreads(threads);
     (void)nt;
+    const uint32_t np = g->ntri + g->nobj;
 #pragma omp parallel for schedule(dynamic, 16) num_threads(nt)
     for (long long ii = 0; ii < (long long)n; ++ii) {
         const size_t i = (size_t)ii;
@@ -542,25 +756,23 @@ void orc_trace_brute(const float* verts, const uint32_t* idx, int ntri,
         int have = 0;
         uint32_t best_p = ORC_MISS;
         float best_t = tmax;
-        for (int p = 0; p < ntri; ++p) {
-            const float *a, *b, *c;
-            tri_verts(verts, idx, p, &a, &b, &c);
+        for (uint32_t p = 0; p < np; ++p) {
             float t;
-            if (!tri_test_pre(o, &pre, a, b, c, &t, NULL, NULL, NULL)) continue;
+            if (!prim_test(g, p, o, d, &pre, &t)) continue;
             if (t >= tmin && (have ? t < best_t : t < tmax)) {   /* p ascends: ties keep the smaller id */
-                have = 1; best_t = t; best_p = (uint32_t)p;
+                have = 1; best_t = t; best_p = p;
             }
         }
-        finish_hit(verts, idx, o, d, have ? best_p : ORC_MISS, hits, i);
+        finish_hit(g, o, d, have ? best_p : ORC_MISS, hits, i);
     }
 }
 
-void orc_trace_any_brute(const float* verts, const uint32_t* idx, int ntri,
-                         orc_rays rays, size_t n, uint32_t* occluded_bits, int threads)
+void orc_trace_any_brute_g(const orc_geom* g, orc_rays rays, size_t n, uint32_t* occluded_bits, int threads)
 {
     const int nt = nthreads(threads);
     (void)nt;
     const long long nw = (long long)((n + 31) / 32);
+    const uint32_t np = g->ntri + g->nobj;
 #pragma omp parallel for schedule(dynamic, 4) num_threads(nt)
     for (long long w = 0; w < nw; ++w) {
         uint32_t word = 0;
@@ -570,11 +782,9 @@ void orc_trace_any_brute(const float* verts, const uint32_t* idx, int ntri,
             float o[3], d[3], tmin, tmax;
             load_ray(rays, i, o, d, &tmin, &tmax);
             const ray_pre pre = ray_precompute(d);
-            for (int p = 0; p < ntri; ++p) {
-                const float *a, *bb, *c;
-                tri_verts(verts, idx, p, &a, &bb, &c);
+            for (uint32_t p = 0; p < np; ++p) {
                 float t;
-                if (tri_test_pre(o, &pre, a, bb, c, &t, NULL, NULL, NULL) && t >= tmin && t < tmax) {
+                if (prim_test(g, p, o, d, &pre, &t) && t >= tmin && t < tmax) {
                     word |= 1u << b;
                     break;
                 }
@@ -584,30 +794,74 @@ void orc_trace_any_brute(const float* verts, const uint32_t* idx, int ntri,
     }
 }
 
+void orc_trace_closest(const orc_node* nodes, int node_count, const int32_t* indices,
+                       const float* verts, const uint32_t* idx,
+                       orc_rays rays, size_t n, orc_hits hits, int mode,
+                       uint32_t* boxes, uint32_t* tris, int threads)
+{
+    /* triangle count is not needed by the traversal: every index is a triangle */
+    const orc_geom g = tri_geom(verts, idx, 0x7FFFFFFF);
+    orc_trace_closest_g(nodes, node_count, indices, &g, rays, n, hits, mode, boxes, tris, threads);
+}
+
+void orc_trace_any(const orc_node* nodes, int node_count, const int32_t* indices,
+                   const float* verts, const uint32_t* idx,
+                   orc_rays rays, size_t n, uint32_t* occluded_bits, int mode,
+                   uint32_t* boxes, uint32_t* tris, int threads)
+{
+    const orc_geom g = tri_geom(verts, idx, 0x7FFFFFFF);
+    orc_trace_any_g(nodes, node_count, indices, &g, rays, n, occluded_bits, mode, boxes, tris, threads);
+}
+
+void orc_trace_brute(const float* verts, const uint32_t* idx, int ntri,
+                     orc_rays rays, size_t n, orc_hits hits, int threads)
+{
+    const orc_geom g = tri_geom(verts, idx, ntri);
+    orc_trace_brute_g(&g, rays, n, hits, threads);
+}
+
+void orc_trace_any_brute(const float* verts, const uint32_t* idx, int ntri,
+                         orc_rays rays, size_t n, uint32_t* occluded_bits, int threads)
+{
+    const orc_geom g = tri_geom(verts, idx, ntri);
+    orc_trace_any_brute_g(&g, rays, n, occluded_bits, threads);
+}
+
 /* Single-ray entry points used by the oracle integrator (render.c). */
+int orc_closest1_g(const orc_node* nodes, int node_count, const int32_t* indices, const orc_geom* g,
+                   const float o[3], const float d[3], float tmin, float tmax,
+                   uint32_t* prim, float* t, float* u, float* v, uint32_t* flags)
+{
+    float tt;
+    uint32_t p;
+    const int hit = traverse_one(nodes, node_count, indices, g, o, d, tmin, tmax, 0, 0, &p, &tt, NULL, NULL);
+    *prim = p;
+    *t = tt;
+    *flags = 0u;
+    if (hit) hit_record(g, o, d, p, t, u, v, flags);
+    return hit;
+}
+
+int orc_any1_g(const orc_node* nodes, int node_count, const int32_t* indices, const orc_geom* g,
+               const float o[3], const float d[3], float tmin, float tmax)
+{
+    float tt;
+    uint32_t p;
+    return traverse_one(nodes, node_count, indices, g, o, d, tmin, tmax, 0, 1, &p, &tt, NULL, NULL);
+}
+
 int orc_closest1(const orc_node* nodes, int node_count, const int32_t* indices, const float* verts,
                  const uint32_t* idx, const float o[3], const float d[3], float tmin, float tmax,
                  uint32_t* prim, float* t, float* u, float* v)
 {
-    float tt;
-    uint32_t p;
-    const int hit = traverse_one(nodes, node_count, indices, verts, idx, o, d, tmin, tmax, 0, 0, &p, &tt, NULL, NULL);
-    *prim = p;
-    *t = tt;
-    if (hit) {
-        const float *a, *b, *c;
-        tri_verts(verts, idx, (int)p, &a, &b, &c);
-        const ray_pre pre = ray_precompute(d);
-        uint32_t f;
-        tri_test_pre(o, &pre, a, b, c, t, u, v, &f);
-    }
-    return hit;
+    const orc_geom g = tri_geom(verts, idx, 0x7FFFFFFF);
+    uint32_t f;
+    return orc_closest1_g(nodes, node_count, indices, &g, o, d, tmin, tmax, prim, t, u, v, &f);
 }
 
 int orc_any1(const orc_node* nodes, int node_count, const int32_t* indices, const float* verts,
              const uint32_t* idx, const float o[3], const float d[3], float tmin, float tmax)
 {
-    float tt;
-    uint32_t p;
-    return traverse_one(nodes, node_count, indices, verts, idx, o, d, tmin, tmax, 0, 1, &p, &tt, NULL, NULL);
+    const orc_geom g = tri_geom(verts, idx, 0x7FFFFFFF);
+    return orc_any1_g(nodes, node_count, indices, &g, o, d, tmin, tmax);
 }

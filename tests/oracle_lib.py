@@ -39,7 +39,7 @@ def load():
     global _lib
     if _lib is not None:
         return _lib
-    srcs = [ORACLE_DIR / "trace.c", ORACLE_DIR / "oracle.h"]
+    srcs = [ORACLE_DIR / n for n in ("trace.c", "render.c", "render_impl.h", "guide.c", "mlp.c", "oracle.h")]
     if not LIB.exists() or any(s.stat().st_mtime > LIB.stat().st_mtime for s in srcs if s.exists()):
         subprocess.run(["make", "-s", "-C", str(ORACLE_DIR)], check=True)
     lib = C.CDLL(str(LIB))
@@ -147,6 +147,11 @@ def brute_any(V, F, rays, threads=0):
     return occ
 
 
+def alloc_hits(n):
+    return {"prim": np.empty(n, np.uint32), "t": np.empty(n, np.float32), "u": np.empty(n, np.float32),
+            "v": np.empty(n, np.float32), "flags": np.empty(n, np.uint32)}
+
+
 def _rays(r):
     o = orc_rays()
     for k, name in enumerate(("ox", "oy", "oz", "tmin", "dx", "dy", "dz", "tmax")):
@@ -165,6 +170,118 @@ def unpack_bits(words: np.ndarray, n: int) -> np.ndarray:
     w = np.asarray(words).view(np.uint32)
     bits = (w[:, None] >> np.arange(32, dtype=np.uint32)[None, :]) & 1
     return bits.reshape(-1)[:n].astype(bool)
+
+
+# ---------------------------------------------------------------- analytic objects
+
+OBJ_BOX, OBJ_SPHERE, HIT_INSIDE = 0, 1, 16
+
+
+class orc_object(C.Structure):
+    _fields_ = [("type", C.c_uint32), ("center", C.c_float * 3), ("extent", C.c_float * 3), ("w2o", C.c_float * 9)]
+
+
+class orc_geom(C.Structure):
+    _fields_ = [("verts", C.c_void_p), ("idx", C.c_void_p), ("ntri", C.c_uint32), ("objs", C.c_void_p),
+                ("nobj", C.c_uint32)]
+
+
+def make_objects(objects):
+    """orc_object array from dicts (type 'box' / 'sphere', center, extent,
+    rotation in degrees), initialised by orc_object_init."""
+    lib = load()
+    f = fn("orc_object_init", None, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p])
+    arr = (orc_object * max(1, len(objects)))()
+    for k, o in enumerate(objects):
+        t = o.get("type", "box")
+        t = {"box": OBJ_BOX, "sphere": OBJ_SPHERE}.get(t, t)
+        c = np.asarray(o["center"], np.float32)
+        e = np.asarray(o["extent"], np.float32)
+        r = np.asarray(o.get("rotation", (0, 0, 0)), np.float32)
+        f(C.byref(arr[k]), int(t), c.ctypes.data, e.ctypes.data, r.ctypes.data)
+    del lib
+    return arr
+
+
+class Geom:
+    """Triangles + analytic objects with the oracle BVH over their proxies."""
+
+    def __init__(self, V, F, objects, leaf_size=4, bin_count=12):
+        lib = load()
+        self.V = np.ascontiguousarray(V, np.float32).reshape(-1, 3)
+        self.F = np.ascontiguousarray(F, np.uint32).reshape(-1, 3)
+        self.objects = list(objects)
+        self.objs = make_objects(self.objects)
+        self.g = orc_geom(self.V.ctypes.data, self.F.ctypes.data, self.F.shape[0],
+                          C.cast(self.objs, C.c_void_p), len(self.objects))
+        n = self.F.shape[0] + len(self.objects)
+        self.n = n
+        pmin, pmax, pcen = (np.zeros((max(n, 1), 3), np.float32) for _ in range(3))
+        fn("orc_geom_proxies", None, [C.c_void_p] * 4)(C.byref(self.g), pmin.ctypes.data, pmax.ctypes.data,
+                                                      pcen.ctypes.data)
+        self.pmin, self.pmax = pmin[:n], pmax[:n]
+        self.nodes = np.zeros(max(2 * n, 1), NODE_DTYPE)
+        self.indices = np.zeros(max(n, 1), np.int32)
+        self.count = lib.orc_bvh_build(pmin.ctypes.data, pmax.ctypes.data, pcen.ctypes.data, n, leaf_size,
+                                       bin_count, self.nodes.ctypes.data, self.indices.ctypes.data)
+
+    def closest(self, rays, mode=0, threads=0):
+        rays = np.ascontiguousarray(rays, np.float32)
+        n = rays.shape[1]
+        out = alloc_hits(n)
+        f = fn("orc_trace_closest_g", None, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, orc_rays, C.c_size_t,
+                                             orc_hits, C.c_int, C.c_void_p, C.c_void_p, C.c_int])
+        f(self.nodes.ctypes.data if self.count else None, self.count, self.indices.ctypes.data, C.byref(self.g),
+          _rays(rays), n, _hits(out), mode, None, None, threads)
+        return out
+
+    def any(self, rays, mode=0, threads=0):
+        rays = np.ascontiguousarray(rays, np.float32)
+        n = rays.shape[1]
+        occ = np.zeros((n + 31) // 32, np.uint32)
+        f = fn("orc_trace_any_g", None, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, orc_rays, C.c_size_t,
+                                         C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int])
+        f(self.nodes.ctypes.data if self.count else None, self.count, self.indices.ctypes.data, C.byref(self.g),
+          _rays(rays), n, occ.ctypes.data, mode, None, None, threads)
+        return unpack_bits(occ, n)
+
+    def brute(self, rays, threads=0):
+        rays = np.ascontiguousarray(rays, np.float32)
+        n = rays.shape[1]
+        out = alloc_hits(n)
+        fn("orc_trace_brute_g", None, [C.c_void_p, orc_rays, C.c_size_t, orc_hits, C.c_int])(
+            C.byref(self.g), _rays(rays), n, _hits(out), threads)
+        return out
+
+    def any_brute(self, rays, threads=0):
+        rays = np.ascontiguousarray(rays, np.float32)
+        n = rays.shape[1]
+        occ = np.zeros((n + 31) // 32, np.uint32)
+        fn("orc_trace_any_brute_g", None, [C.c_void_p, orc_rays, C.c_size_t, C.c_void_p, C.c_int])(
+            C.byref(self.g), _rays(rays), n, occ.ctypes.data, threads)
+        return unpack_bits(occ, n)
+
+
+def object_test(obj, o, d):
+    """(hit, t, inside) of one orc_object for one ray."""
+    f = fn("orc_object_test", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_float),
+                                        C.POINTER(C.c_int)])
+    o = np.ascontiguousarray(o, np.float32)
+    d = np.ascontiguousarray(d, np.float32)
+    t, ins = C.c_float(0), C.c_int(0)
+    h = f(C.byref(obj), o.ctypes.data, d.ctypes.data, C.byref(t), C.byref(ins))
+    return bool(h), t.value, bool(ins.value)
+
+
+def object_surface(obj, o, d, t, inside):
+    f = fn("orc_object_surface", None, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_float, C.c_int, C.c_void_p,
+                                        C.c_void_p])
+    o = np.ascontiguousarray(o, np.float32)
+    d = np.ascontiguousarray(d, np.float32)
+    pos = np.zeros(3, np.float32)
+    nrm = np.zeros(3, np.float32)
+    f(C.byref(obj), o.ctypes.data, d.ctypes.data, t, int(inside), pos.ctypes.data, nrm.ctypes.data)
+    return pos, nrm
 
 
 # ---------------------------------------------------------------- integrator
