@@ -200,6 +200,8 @@ struct RenderBuffers {
     uint32_t* scn;
     uint32_t* qroot[2];           // queue ray -> BVH root of its scene
     uint32_t* sroot;              // shadow ray -> BVH root
+    uint32_t* med;                // medium of the path: 0 none, else object index + 1
+    float* mpos[3];               // where the path entered its medium
     float* eps_table;             // INTERSECT_EPS(d) for d < 256
     float* cie;                   // device copies of the CIE 1931 / D65 tables
     float* d65;

@@ -34,6 +34,7 @@ constexpr uint32_t kCameraBounce = 0xFFFFFFFFu;
 constexpr int kBlock = 256;        // default threads per block of the wavefront kernels
 constexpr int kMaxBlock = 512;     // largest block the wavefront kernels accept
 constexpr uint32_t kMetaNee = 1u << 16;
+constexpr uint32_t kMetaHeroOnly = 1u << 17;   // secondary hero wavelengths dropped at a dispersive interface
 
 // ---------------------------------------------------------------- helpers
 struct F3 { float x, y, z; };
@@ -185,11 +186,12 @@ struct Mat {
     float specular, metallic, roughness, escale;
     uint32_t model;
     float ca[3], ce[3];
+    float ior, sigma_a, abbe, tilt;
 };
 
 __device__ __forceinline__ Mat load_mat(const GpuMaterial* __restrict__ mats, uint32_t id) {
     const float4* m4 = reinterpret_cast<const float4*>(mats + id);
-    const float4 a = m4[0], b = m4[1], c = m4[2], d = m4[3], e = m4[4];
+    const float4 a = m4[0], b = m4[1], c = m4[2], d = m4[3], e = m4[4], g = m4[5];
     Mat m;
     m.albedo = f3(a.x, a.y, a.z);
     m.specular = a.w;
@@ -200,6 +202,7 @@ __device__ __forceinline__ Mat load_mat(const GpuMaterial* __restrict__ mats, ui
     m.escale = c.z;
     m.ca[0] = d.x; m.ca[1] = d.y; m.ca[2] = d.z;
     m.ce[0] = e.x; m.ce[1] = e.y; m.ce[2] = e.z;
+    m.ior = g.x; m.sigma_a = g.y; m.abbe = g.z; m.tilt = g.w;
     return m;
 }
 
@@ -339,6 +342,170 @@ __device__ __forceinline__ float power_heuristic(float pa, float pb) {
     return a / maxr(1e-24f, a + b);
 }
 
+// ---------------------------------------------------------------- box / sphere lights
+// Facet f of a box light (nee.cuh:153-183): outward normal +-normalised
+// worldToObject row f/2, centre c +- that axis times the extent, tangent and
+// bitangent axes and half sides.
+struct Facet {
+    F3 n, c, t, b;
+    float s1, s2;
+};
+
+__device__ __forceinline__ Facet box_facet(const float4* __restrict__ ob, int f) {
+    const float4 c4 = ob[0], e = ob[1], r0 = ob[2], r1 = ob[3], r2 = ob[4];
+    const F3 ux = normalize(f3(r0.x, r0.y, r0.z)), uy = normalize(f3(r1.x, r1.y, r1.z)),
+             uz = normalize(f3(r2.x, r2.y, r2.z));
+    const int a = f >> 1;
+    const F3 ax = a == 0 ? ux : (a == 1 ? uy : uz);
+    const float ea = a == 0 ? e.x : (a == 1 ? e.y : e.z);
+    const F3 c = xyz(c4);
+    Facet r;
+    r.n = (f & 1) ? scale(ax, -1.0f) : ax;
+    r.c = (f & 1) ? sub(c, scale(ax, ea)) : add(c, scale(ax, ea));
+    r.t = a == 0 ? uy : ux;
+    r.b = a == 2 ? uy : uz;
+    r.s1 = a == 0 ? e.y : e.x;
+    r.s2 = a == 2 ? e.y : e.z;
+    return r;
+}
+
+// NEEFacingWeight_Box (nee.cuh:55-61)
+__device__ __forceinline__ float facing_weight(const Facet& f, F3 x) {
+    const float area = 4.0f * f.s1 * f.s2;
+    return area * maxr(0.0f, dot(f.n, normalize(sub(x, f.c))));
+}
+
+// AreaToSolidAnglePDF (nee.cuh:35-46)
+__device__ __forceinline__ float area_to_solid(F3 x, F3 y, F3 ny, float pa_total) {
+    const F3 s2l = sub(y, x);
+    const float dist2 = maxr(dot(s2l, s2l), 1e-20f);
+    const F3 dir = scale(s2l, 1.0f / sqrtf(dist2));
+    const float cosy = maxr(0.0f, dot(ny, scale(dir, -1.0f)));
+    if (cosy <= 0.0f) return 0.0f;
+    return pa_total * (dist2 / cosy);
+}
+
+// NEEPDFOmegaFromHit_Box / _Sphere (nee.cuh:63-151, 253-271): solid-angle
+// density with which NEE from x samples the light point y (normal ny),
+// select probability included.
+__device__ __forceinline__ float obj_light_pdf(const float4* __restrict__ ob, float select, F3 x, F3 y, F3 ny) {
+    if (__float_as_uint(ob[0].w) == AR_OBJ_BOX) {
+        int hit = -1;
+        for (int f = 0; f < 6; ++f)
+            if (dot(box_facet(ob, f).n, ny) > 0.999f) { hit = f; break; }
+        if (hit < 0) {
+            float best = 1e30f;
+            for (int f = 0; f < 6; ++f) {
+                const Facet fc = box_facet(ob, f);
+                const float dd = fabsf(dot(fc.n, sub(y, fc.c)));
+                if (dd < best) { best = dd; hit = f; }
+            }
+        }
+        float sum = 0.0f, whit = 0.0f, ahit = 0.0f;
+        for (int f = 0; f < 6; ++f) {
+            const Facet fc = box_facet(ob, f);
+            const float w = facing_weight(fc, x);
+            sum += w;
+            if (f == hit) { whit = w; ahit = 4.0f * fc.s1 * fc.s2; }
+        }
+        if (sum <= 0.0f || whit <= 0.0f) return 0.0f;
+        const float pa = select * (1.0f / ahit) * (whit / sum);
+        return area_to_solid(x, y, ny, pa);
+    }
+    const F3 c = xyz(ob[0]);
+    const float r = ob[1].w;
+    const float dist = length(sub(x, c));
+    if (dist <= r) return 0.0f;
+    const float cosc = clampr(r / dist, 0.0f, 1.0f);
+    const float cap = 2.0f * kPi * r * r * (1.0f - cosc);
+    if (cap <= 1e-20f) return 0.0f;
+    return area_to_solid(x, y, ny, select / cap);
+}
+
+// NEESampleBoxAreaFacing / NEESampleSphereCapArea (nee.cuh:153-251,
+// 273-310): light point y, its normal, its area density (facet or cap,
+// without the select probability).  False when nothing faces x.
+__device__ __forceinline__ bool obj_light_sample(const float4* __restrict__ ob, F3 x, float ub, float u0, float u1,
+                                                 F3& y, F3& ny, float& pa) {
+    if (__float_as_uint(ob[0].w) == AR_OBJ_BOX) {
+        float sum = 0.0f;
+        for (int f = 0; f < 6; ++f) sum += facing_weight(box_facet(ob, f), x);
+        // the valid facets in order; the first whose running sum reaches ub * sum
+        int pick = -1, first = -1;
+        float wpick = 0.0f, acc = 0.0f;
+        const float r = ub * sum;
+        for (int f = 0; f < 6; ++f) {
+            const float w = facing_weight(box_facet(ob, f), x);
+            if (w > 0.0f) {
+                if (first < 0) { first = f; wpick = w; }
+                acc += w;
+                if (pick < 0 && r <= acc) { pick = f; wpick = w; }
+            }
+        }
+        if (first < 0) return false;
+        if (pick < 0) {
+            pick = first;
+            wpick = facing_weight(box_facet(ob, first), x);
+        }
+        const Facet fc = box_facet(ob, pick);
+        const float a1 = 2.0f * u0 - 1.0f, a2 = 2.0f * u1 - 1.0f;
+        y = add(add(fc.c, scale(fc.t, a1 * fc.s1)), scale(fc.b, a2 * fc.s2));
+        ny = fc.n;
+        pa = (wpick / sum) * (1.0f / (4.0f * fc.s1 * fc.s2));
+        return true;
+    }
+    const F3 c = xyz(ob[0]);
+    const float r = ob[1].w;
+    const F3 pc = sub(x, c);
+    const float dist = length(pc);
+    if (dist <= r) return false;
+    const F3 axis = scale(pc, 1.0f / dist);
+    const float cosc = clampr(r / dist, 0.0f, 1.0f);
+    const float cap = 2.0f * kPi * r * r * (1.0f - cosc);
+    const float cost = cosc * (1.0f - u0) + u0;
+    const float sint = sqrtf(maxr(0.0f, 1.0f - cost * cost));
+    const float phi = 2.0f * kPi * u1;
+    const F3 a = (fabsf(axis.z) < 0.999f) ? f3(0, 0, 1) : f3(1, 0, 0);
+    const F3 t = normalize(cross(a, axis));
+    const F3 b = cross(axis, t);
+    const F3 nn = add(add(scale(t, cosf(phi) * sint), scale(b, sinf(phi) * sint)), scale(axis, cost));
+    y = add(c, scale(nn, r));
+    ny = nn;
+    pa = 1.0f / cap;
+    return true;
+}
+
+// ---------------------------------------------------------------- glass
+// IORFromAbbeNM and SigmaAFromTiltNM (core/spectrum/spectrum.cuh:16-54)
+__device__ __forceinline__ float ior_at(float nd, float vd, float lambda) {
+    if (!(lambda > 0.0f)) return nd;
+    const float invf = 1.0f / 486.1327f, invc = 1.0f / 656.2725f, invd = 1.0f / 587.5618f;
+    const float k = vd > 0.0f ? (nd - 1.0f) / (vd * (invf - invc)) : 0.0f;
+    return nd + k * (1.0f / lambda - invd);
+}
+__device__ __forceinline__ float sigma_at(float sa, float lambda, float tilt) {
+    const float invf = 1.0f / 486.1327f, invc = 1.0f / 656.2725f, invd = 1.0f / 587.5618f;
+    const float u = 1.0f / maxr(lambda, 1e-6f);
+    const float du = (u - invd) / (invf - invc);
+    return maxr(0.0f, sa * (1.0f + tilt * du));
+}
+// FresnelDielectricExact (brdf.cuh:25-31), eta = n_in / n_out
+__device__ __forceinline__ float fresnel_dielectric(float cosi, float cost, float eta) {
+    const float inve = 1.0f / eta;
+    const float rpar = (inve * cosi - cost) / (inve * cosi + cost);
+    const float rperp = (cosi - inve * cost) / (cosi + inve * cost);
+    return 0.5f * (rpar * rpar + rperp * rperp);
+}
+__device__ __forceinline__ F3 reflect3(F3 v, F3 n) { return sub(v, scale(n, 2.0f * dot(v, n))); }
+// refract (core/vector.cuh:264-272)
+__device__ __forceinline__ F3 refract3(F3 in, F3 normal, float eta) {
+    const F3 n = dot(normal, in) > 0.0f ? scale(normal, -1.0f) : normal;
+    const float cosi = -dot(n, in);
+    const float k = 1.0f - (eta * eta) * maxr(0.0f, 1.0f - cosi * cosi);
+    const float cost = sqrtf(maxr(0.0f, k));
+    return add(scale(in, eta), scale(n, eta * cosi - cost));
+}
+
 // SkyLighting (core/render/render.cuh:58-73)
 __device__ __forceinline__ F3 sky_gradient(F3 d) {
     const F3 white = f3(1, 1, 1), blue = f3(0.05f, 0.45f, 1.0f);
@@ -358,6 +525,8 @@ struct SceneView {
     uint32_t out_base;           // first pixel of the scene in the accumulation buffer
     uint32_t root;               // BVH root of the scene
     uint32_t light_off, light_cnt;
+    uint32_t medium0;            // object containing the camera + 1 (0: none)
+    uint32_t root0;              // root of the camera rays (the medium's own leaf inside one)
     float cam_pos[3];
     float fwd[3], right[3], up[3];
     float thx, thy;
@@ -407,6 +576,8 @@ struct QueueRays {
 
 struct SceneArgs {
     const float4* __restrict__ prims;
+    const float4* __restrict__ objs;   // analytic objects (obj::kRecord float4 each), prims ntri ..
+    uint32_t ntri;
     const GpuMaterial* __restrict__ mats;
     const int32_t* __restrict__ light_tri;
     const float* __restrict__ light_area;
@@ -426,6 +597,8 @@ struct PathArgs {
     uint32_t* pix;               // pixel of the path
     uint32_t* smp;               // sample index of the path
     uint32_t* scn;               // scene of the path
+    uint32_t* med;               // medium: 0 none, else object index + 1
+    float* mp0; float* mp1; float* mp2;   // where the path entered its medium
 };
 
 // Voxel guiding inputs of the integrator: the probes (read-only during a
@@ -485,8 +658,8 @@ __device__ __forceinline__ void g_endpoint(const GuideArgs& G, uint32_t p, const
 // of this non-delta depth, then scales the open slots by contrib / q.
 __device__ __forceinline__ void g_advance(const GuideArgs& G, uint32_t p, uint32_t non_delta, uint32_t mode,
                                           uint64_t key, int leaf, const S4& contrib, float qrr, bool learned,
-                                          const float x5[5], F3 dir, float pdf) {
-    if (non_delta < 5u) {
+                                          const float* x5, F3 dir, float pdf, bool start) {
+    if (start && non_delta < 5u) {
         const int i = (int)non_delta;
         if (s4zero(gload(G.gm, G.stride, i, p), mode)) {
             gstore(G.gm, G.stride, i, p, s4(1.0f, 1.0f, 1.0f, 1.0f));
@@ -574,7 +747,9 @@ __global__ __launch_bounds__(kMaxBlock) void k_camera(RenderParams P, PathArgs S
     S.pix[p] = pix;
     S.smp[p] = smp;
     S.scn[p] = sg.scene;
-    q.root[p] = V.root;
+    q.root[p] = V.root0;
+    S.med[p] = V.medium0;
+    S.mp0[p] = V.cam_pos[0]; S.mp1[p] = V.cam_pos[1]; S.mp2[p] = V.cam_pos[2];
     q.ox[p] = V.cam_pos[0]; q.oy[p] = V.cam_pos[1]; q.oz[p] = V.cam_pos[2];
     q.tmin[p] = eps_table[0];
     q.dx[p] = d.x; q.dy[p] = d.y; q.dz[p] = d.z;
@@ -600,9 +775,12 @@ __global__ __launch_bounds__(kMaxBlock) void k_camera(RenderParams P, PathArgs S
 // Network inputs of every queued hit (the learned guide's inference batch):
 // the hit point and the shading normal facing the ray, as k_shade computes
 // them.
-__global__ __launch_bounds__(kMaxBlock) void k_mlp_features(const float4* __restrict__ prims, QueueRays cur,
+__global__ __launch_bounds__(kMaxBlock) void k_mlp_features(const float4* __restrict__ prims,
+                                                         const float4* __restrict__ objs, uint32_t ntri, QueueRays cur,
                                                          const uint32_t* __restrict__ hprim,
-                                                         const float* __restrict__ hu, const float* __restrict__ hv,
+                                                         const float* __restrict__ ht, const float* __restrict__ hu,
+                                                         const float* __restrict__ hv,
+                                                         const uint32_t* __restrict__ hflags,
                                                          const uint32_t* __restrict__ counters, float lo0, float lo1,
                                                          float lo2, float inv0, float inv1, float inv2,
                                                          float* __restrict__ feat) {
@@ -610,7 +788,13 @@ __global__ __launch_bounds__(kMaxBlock) void k_mlp_features(const float4* __rest
     if (q >= counters[0]) return;
     float x[5] = {0, 0, 0, 0, 0};
     const uint32_t prim = hprim[q];
-    if (prim != AR_MISS) {
+    if (prim != AR_MISS && prim >= ntri) {
+        float pp[3], np[3];
+        obj::surface(objs + (size_t)obj::kRecord * (prim - ntri), cur.ox[q], cur.oy[q], cur.oz[q], cur.dx[q], cur.dy[q],
+                     cur.dz[q], ht[q], (hflags[q] & AR_HIT_INSIDE) != 0u, pp, np);
+        const float lo[3] = {lo0, lo1, lo2}, inv[3] = {inv0, inv1, inv2};
+        mlp::features(pp[0], pp[1], pp[2], np[0], np[1], np[2], lo, inv, x);
+    } else if (prim != AR_MISS) {
         const F3 a = xyz(prims[3 * (size_t)prim + 0]), b = xyz(prims[3 * (size_t)prim + 1]),
                  c = xyz(prims[3 * (size_t)prim + 2]);
         const float bu = hu[q], bv = hv[q];
@@ -636,7 +820,7 @@ __device__ __forceinline__ void shade_one(
     const RenderParams& P, uint32_t depth, const SceneArgs& sc, const PathArgs& S, const GuideArgs& G,
     const QueueRays& cur, const uint32_t* __restrict__ qcur,
     const uint32_t* __restrict__ hprim, const float* __restrict__ ht, const float* __restrict__ hu,
-    const float* __restrict__ hv,
+    const float* __restrict__ hv, const uint32_t* __restrict__ hflags,
     const QueueRays& nxt, uint32_t* __restrict__ qnext,
     const QueueRays& sh, uint32_t* __restrict__ spath, float* __restrict__ sc0, float* __restrict__ sc1,
     float* __restrict__ sc2, float* __restrict__ sc3, uint32_t* __restrict__ sflags,
@@ -672,8 +856,33 @@ __device__ __forceinline__ void shade_one(
         const F3 d = f3(cur.dx[q], cur.dy[q], cur.dz[q]);
         const float eps = eps_table[non_delta < 255 ? non_delta : 255];
         const uint32_t prim = hprim[q];
-        uint32_t new_meta = non_delta;
-        if (prim == AR_MISS) {
+        uint32_t new_meta = meta & ~kMetaNee;
+        uint32_t medium = S.med[p];
+        if (prim == AR_MISS && medium != 0u) {
+            // inside a medium with no exit found (bMissThinSurfaceInsideMedia_,
+            // render.cuh:107-124): leave the medium and continue straight from
+            // the same origin as a delta vertex of weight 1
+            float qrr = 1.0f;
+            bool alive = true;
+            if (non_delta >= P.rr_min_depth) {
+                qrr = clampr(lum4(mask, P.spectral) * 0.5f + max4(mask, P.spectral) * 0.5f, 0.05f, 0.95f);
+                if (u[9] > qrr) alive = false;
+            }
+            new_meta = meta & ~kMetaNee;
+            if (alive && depth + 1 < P.max_depth) {
+                if (GM != AR_GUIDING_OFF)
+                    g_advance(G, p, non_delta, P.spectral, guide::kNoKey, 0, s4(1, 1, 1, 1), qrr, learned, nullptr, o,
+                              1.0f, false);
+                mask = s4scale(mask, 1.0f / qrr);
+                cont = true;
+                next_o = o;
+                next_d = d;
+                next_tmin = eps;
+                root = V.root;
+                new_meta = meta;
+            }
+            S.med[p] = 0u;
+        } else if (prim == AR_MISS) {
             S4 e = s4(0, 0, 0, 0);
             if (P.sky == AR_SKY_GRADIENT) {
                 const F3 eg = sky_gradient(d);
@@ -694,36 +903,153 @@ __device__ __forceinline__ void shade_one(
             }
             if (GM != AR_GUIDING_OFF) g_endpoint(G, p, e, non_delta, P.spectral);
         } else {
-            const float4 a4 = sc.prims[3 * (size_t)prim + 0];
-            const float4 b4 = sc.prims[3 * (size_t)prim + 1];
-            const float4 c4 = sc.prims[3 * (size_t)prim + 2];
-            const F3 a = xyz(a4), b = xyz(b4), c = xyz(c4);
-            const uint32_t mat_id = __float_as_uint(a4.w);
-            const int light_id = __float_as_int(b4.w);
-            const float bu = hu[q], bv = hv[q];
-            const float bw = 1.0f - bu - bv;
-            F3 pos = add(add(scale(a, bw), scale(b, bu)), scale(c, bv));
-            const F3 ng = normalize(cross(sub(b, a), sub(c, a)));
-            const float ndd = dot(ng, d);
-            const F3 nn = ndd < 0.0f ? ng : scale(ng, -1.0f);
+            const bool is_obj = prim >= sc.ntri;
+            const float4* ob = is_obj ? sc.objs + (size_t)obj::kRecord * (prim - sc.ntri) : nullptr;
+            F3 pos, ng, nn;
+            uint32_t mat_id;
+            int light_id;
+            float ndd;
+            if (is_obj) {
+                // the intersector's normal: outward, or inward for a hit from inside
+                float pp[3], np[3];
+                obj::surface(ob, o.x, o.y, o.z, d.x, d.y, d.z, ht[q], (hflags[q] & AR_HIT_INSIDE) != 0u, pp, np);
+                pos = f3(pp[0], pp[1], pp[2]);
+                ng = f3(np[0], np[1], np[2]);
+                nn = ng;
+                ndd = dot(ng, d);
+                mat_id = __float_as_uint(ob[2].w);
+                light_id = __float_as_int(ob[3].w);
+            } else {
+                const float4 a4 = sc.prims[3 * (size_t)prim + 0];
+                const float4 b4 = sc.prims[3 * (size_t)prim + 1];
+                const float4 c4 = sc.prims[3 * (size_t)prim + 2];
+                const F3 a = xyz(a4), b = xyz(b4), c = xyz(c4);
+                mat_id = __float_as_uint(a4.w);
+                light_id = __float_as_int(b4.w);
+                const float bu = hu[q], bv = hv[q];
+                const float bw = 1.0f - bu - bv;
+                pos = add(add(scale(a, bw), scale(b, bu)), scale(c, bv));
+                ng = normalize(cross(sub(b, a), sub(c, a)));
+                ndd = dot(ng, d);
+                nn = ndd < 0.0f ? ng : scale(ng, -1.0f);
+            }
             const Mat m = load_mat(sc.mats, mat_id);
             if (light_id >= 0) {
-                if (ndd < 0.0f) {
+                // triangle lights emit from the front face, box and sphere
+                // lights from every hit (MAT_LIGHT objects, render.cuh:664-710)
+                if (is_obj || ndd < 0.0f) {
                     float w = 1.0f;
                     if (P.nee && depth > 0 && prev_nee) {
                         const F3 lp = f3(S.lp0[p], S.lp1[p], S.lp2[p]);
-                        const F3 tl = sub(pos, lp);
-                        const float dist2 = maxr(dot(tl, tl), 1e-20f);
-                        const F3 dir = scale(tl, 1.0f / sqrtf(dist2));
-                        const float cosy = maxr(0.0f, -dot(ng, dir));
-                        const float pa = sc.light_prob[light_id] / sc.light_area[light_id];
-                        const float pl = cosy <= 0.0f ? 0.0f : pa * (dist2 / cosy);
+                        float pl;
+                        if (is_obj) {
+                            pl = obj_light_pdf(ob, sc.light_prob[light_id], lp, pos, ng);
+                        } else {
+                            const F3 tl = sub(pos, lp);
+                            const float dist2 = maxr(dot(tl, tl), 1e-20f);
+                            const F3 dir = scale(tl, 1.0f / sqrtf(dist2));
+                            const float cosy = maxr(0.0f, -dot(ng, dir));
+                            const float pa = sc.light_prob[light_id] / sc.light_area[light_id];
+                            pl = cosy <= 0.0f ? 0.0f : pa * (dist2 / cosy);
+                        }
                         const float last_pdf = S.last_pdf[p];
                         if (pl >= 0.0f && last_pdf > 0.0f) w = power_heuristic(last_pdf, pl);
                     }
                     const S4 le = emission_of(m, L, sc.d65);
                     rad = s4add(rad, s4scale(s4mul(mask, le), w));
                     if (GM != AR_GUIDING_OFF) g_endpoint(G, p, s4scale(le, w), non_delta, P.spectral);
+                }
+            } else if (m.model == AR_MODEL_GLASS) {
+                // Fresnel reflection or refraction (SampleBSDF MAT_GLASS,
+                // material.cuh:206-257) weighted by EvaluateBSDFWithCos
+                // (material.cuh:116-161), Beer-Lambert over the path inside
+                const bool ins = medium != 0u;
+                float ior = m.ior;
+                if (P.spectral != AR_SPECTRAL_RGB) ior = ior_at(m.ior, m.abbe, L.l[0]);
+                if (P.spectral == AR_SPECTRAL_HERO4 && m.abbe > 0.0f && !(meta & kMetaHeroOnly)) {
+                    // dispersion: the hero wavelength continues alone with the
+                    // weight of all four
+                    mask = s4(mask.v[0] * 4.0f, 0.0f, 0.0f, 0.0f);
+                    new_meta |= kMetaHeroOnly;
+                }
+                S4 tr = s4(1, 1, 1, 1);
+                if (ins) {
+                    const float dist = length(sub(f3(S.mp0[p], S.mp1[p], S.mp2[p]), pos));
+                    const S4 alb = albedo_of(m, L);
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        const float sa = P.spectral == AR_SPECTRAL_RGB ? m.sigma_a : sigma_at(m.sigma_a, L.l[k], m.tilt);
+                        const float av = minr(maxr(alb.v[k], 1e-12f), 1.0f);
+                        tr.v[k] = expf(dist * sa * logf(av));
+                    }
+                }
+                const F3 nin = dot(ng, d) > 0.0f ? scale(ng, -1.0f) : ng;
+                const float eta = ins ? ior : 1.0f / ior;
+                const float cosi = -dot(nin, d);
+                const float sin2i = maxr(0.0f, 1.0f - cosi * cosi);
+                const float kk = 1.0f - sin2i * (eta * eta);
+                F3 next;
+                float pdf;
+                S4 fcos;
+                uint32_t next_medium = medium;
+                if (kk <= 0.0f) {
+                    next = reflect3(d, nin);
+                    pdf = 1.0f;
+                    fcos = s4scale(tr, 0.99f);
+                } else {
+                    const float cost = sqrtf(kk);
+                    const float fr = fresnel_dielectric(cosi, cost, eta);
+                    const F3 rdir = reflect3(d, nin);
+                    const F3 tdir = refract3(d, nin, eta);
+                    if (u[0] < fr) {
+                        next = rdir;
+                        pdf = fr;
+                    } else {
+                        next = tdir;
+                        pdf = 1.0f - fr;
+                        next_medium = ins ? 0u : prim - sc.ntri + 1u;
+                    }
+                    if (dot(next, rdir) >= dot(next, tdir)) {
+                        fcos = s4scale(tr, fr);
+                    } else {
+                        const float wgt = (eta * eta) * (fabsf(cost) / maxr(1e-6f, cosi));
+                        fcos = s4scale(tr, (1.0f - fr) * wgt);
+                    }
+                }
+                S.mp0[p] = pos.x; S.mp1[p] = pos.y; S.mp2[p] = pos.z;
+                medium = next_medium;
+                S.med[p] = medium;
+                const float off = eps * maxr(1.0f, length(pos)) * (u[4] + 0.5f);
+                const float dn = dot(ng, next);
+                const float sgn = dn > 0.0f ? 1.0f : (dn < 0.0f ? -1.0f : 0.0f);
+                pos = add(pos, scale(ng, sgn * off));
+                S.last_pdf[p] = pdf;
+                S.lp0[p] = pos.x; S.lp1[p] = pos.y; S.lp2[p] = pos.z;
+                const S4 contrib = s4scale(fcos, 1.0f / maxr(pdf, 1e-12f));
+                float qrr = 1.0f;
+                bool alive = true;
+                if (non_delta >= P.rr_min_depth) {
+                    const float base =
+                        clampr(lum4(mask, P.spectral) * 0.5f + max4(mask, P.spectral) * 0.5f, 0.05f, 0.95f);
+                    const float tl = medium != 0u ? 0.99f : 0.0f;
+                    qrr = base * (1.0f - tl) + tl;
+                    if (u[9] > qrr) alive = false;
+                }
+                if (alive && depth + 1 < P.max_depth) {
+                    if (GM != AR_GUIDING_OFF)
+                        g_advance(G, p, non_delta, P.spectral, guide::kNoKey, 0, contrib, qrr, learned, nullptr, next,
+                                  pdf, false);
+                    mask = s4scale(s4mul(mask, contrib), 1.0f / qrr);
+                    cont = true;
+                    next_o = pos;
+                    next_d = next;
+                    next_tmin = eps;
+                    if (medium != 0u) {
+                        const float4* mo = sc.objs + (size_t)obj::kRecord * (medium - 1u);
+                        root = __float_as_uint(mo[4].w);
+                    } else {
+                        root = V.root;
+                    }
                 }
             } else {
                 const S4 alb = albedo_of(m, L);
@@ -734,6 +1060,7 @@ __device__ __forceinline__ void shade_one(
                     S4 fcos = eval_bsdf(m, alb, d, next, nn);
                     float x5[5] = {0, 0, 0, 0, 0};
                     if (learned) mlp::features(pos.x, pos.y, pos.z, nn.x, nn.y, nn.z, G.box_lo, G.box_inv, x5);
+                    S.mp0[p] = pos.x; S.mp1[p] = pos.y; S.mp2[p] = pos.z;
                     const float off = eps * maxr(1.0f, length(pos)) * (u[4] + 0.5f);
                     const float dn = dot(nn, next);
                     const float sgn = dn > 0.0f ? 1.0f : (dn < 0.0f ? -1.0f : 0.0f);
@@ -803,7 +1130,7 @@ __device__ __forceinline__ void shade_one(
                         }
                     }
                     bool nee_now = false;
-                    if (P.nee && V.light_cnt > 0 && depth + 1 < P.max_depth && non_delta < 10 &&
+                    if (P.nee && V.light_cnt > 0 && depth + 1 < P.max_depth && non_delta < 10 && medium == 0u &&
                         u[5] < sqrtf(clampr(lum4(mask, P.spectral), 0.0f, 1.0f))) {
                         // pick by the scene's CDF (first entry with u < cdf; binary search)
                         uint32_t lo = V.light_off, hi = V.light_off + V.light_cnt - 1;
@@ -813,55 +1140,73 @@ __device__ __forceinline__ void shade_one(
                         }
                         const uint32_t li = lo;
                         const uint32_t lp = (uint32_t)sc.light_tri[li];
-                        const float4 la4 = sc.prims[3 * (size_t)lp + 0];
-                        const F3 la = xyz(la4), lb = xyz(sc.prims[3 * (size_t)lp + 1]), lc = xyz(sc.prims[3 * (size_t)lp + 2]);
-                        const float su = sqrtf(u[7]);
-                        const F3 qpt = add(add(scale(la, 1.0f - su), scale(lb, su * (1.0f - u[8]))), scale(lc, su * u[8]));
-                        const F3 ln = normalize(cross(sub(lb, la), sub(lc, la)));
-                        const Mat lm = load_mat(sc.mats, __float_as_uint(la4.w));
-                        const float pa_total = maxr(1e-20f, sc.light_prob[li] / sc.light_area[li]);
-                        const F3 lu = sub(qpt, pos);
-                        const float dist2 = maxr(dot(lu, lu), 1e-12f);
-                        const F3 ldir = scale(lu, 1.0f / sqrtf(dist2));
-                        const float dist = sqrtf(dot(lu, lu));
-                        const float cosx = maxr(0.0f, dot(nn, ldir));
-                        const float cosy = maxr(0.0f, -dot(ln, ldir));
-                        const bool faces = cosx > 0.0f && cosy > 0.0f;
-                        S4 contrib_w = s4(0, 0, 0, 0);
-                        if (faces) {
-                            const S4 f = eval_bsdf(m, alb, d, ldir, nn);
-                            const float g = cosy / dist2;
-                            const S4 contrib = s4scale(s4mul(f, emission_of(lm, L, sc.d65)), g / pa_total);
-                            const float pl = pa_total * (dist2 / cosy);
-                            float pb = maxr(bsdf_pdf(m, pspec, d, ldir, nn), 0.0f);
-                            if (ptex)
-                                pb = (1.0f - P.alpha) * pb +
-                                     P.alpha * guide::hemi_pdf(ptex, ldir.x, ldir.y, ldir.z, nn.x, nn.y, nn.z, gq);
-                            else if (pmix)
-                                pb = (1.0f - P.alpha) * pb +
-                                     P.alpha * mlp::hemi_pdf(pmix, ldir.x, ldir.y, ldir.z, nn.x, nn.y, nn.z);
-                            const float w = power_heuristic(pl, pb);
-                            contrib_w = s4mul(mask, s4scale(contrib, w));
-                        }
-                        // AnyHitSegmentBVH (intersection.cuh:285-310)
-                        float tmx = 0.0f;
-                        F3 sdir = f3(0, 0, 1);
-                        if (dist > 0.0f) {
-                            sdir = scale(lu, 1.0f / dist);
-                            const float seps = eps_table[depth < 255 ? depth : 255] *
-                                               maxr(1.0f, maxr(length(pos), length(qpt)));
-                            tmx = maxr(0.0f, dist - seps);
-                        }
-                        if (tmx > 0.0f) {
-                            shadow = true;
-                            s_o = pos; s_d = sdir; s_tmax = tmx; s_contrib = contrib_w; s_flag = faces ? 1u : 0u;
+                        F3 qpt, ln;
+                        float pa_total = 1.0f;
+                        bool valid = true;
+                        Mat lm;
+                        if (lp < sc.ntri) {
+                            const float4 la4 = sc.prims[3 * (size_t)lp + 0];
+                            const F3 la = xyz(la4), lb = xyz(sc.prims[3 * (size_t)lp + 1]),
+                                     lc = xyz(sc.prims[3 * (size_t)lp + 2]);
+                            const float su = sqrtf(u[7]);
+                            qpt = add(add(scale(la, 1.0f - su), scale(lb, su * (1.0f - u[8]))), scale(lc, su * u[8]));
+                            ln = normalize(cross(sub(lb, la), sub(lc, la)));
+                            lm = load_mat(sc.mats, __float_as_uint(la4.w));
+                            pa_total = maxr(1e-20f, sc.light_prob[li] / sc.light_area[li]);
                         } else {
-                            // nothing to test: visible
-                            if (faces) { rad = s4add(rad, contrib_w); nee_now = true; }
+                            // box facet pick from a fifth group of uniforms
+                            const float4* lo4 = sc.objs + (size_t)obj::kRecord * (lp - sc.ntri);
+                            float ux[4];
+                            uniforms(P.seed, pix, smp, depth, 4, ux);
+                            float pa;
+                            valid = obj_light_sample(lo4, pos, ux[0], u[7], u[8], qpt, ln, pa);
+                            if (valid) pa_total = maxr(1e-20f, pa * sc.light_prob[li]);
+                            lm = load_mat(sc.mats, __float_as_uint(lo4[2].w));
+                        }
+                        if (valid) {
+                            const F3 lu = sub(qpt, pos);
+                            const float dist2 = maxr(dot(lu, lu), 1e-12f);
+                            const F3 ldir = scale(lu, 1.0f / sqrtf(dist2));
+                            const float dist = sqrtf(dot(lu, lu));
+                            const float cosx = maxr(0.0f, dot(nn, ldir));
+                            const float cosy = maxr(0.0f, -dot(ln, ldir));
+                            const bool faces = cosx > 0.0f && cosy > 0.0f;
+                            S4 contrib_w = s4(0, 0, 0, 0);
+                            if (faces) {
+                                const S4 f = eval_bsdf(m, alb, d, ldir, nn);
+                                const float g = cosy / dist2;
+                                const S4 contrib = s4scale(s4mul(f, emission_of(lm, L, sc.d65)), g / pa_total);
+                                const float pl = pa_total * (dist2 / cosy);
+                                float pb = maxr(bsdf_pdf(m, pspec, d, ldir, nn), 0.0f);
+                                if (ptex)
+                                    pb = (1.0f - P.alpha) * pb +
+                                         P.alpha * guide::hemi_pdf(ptex, ldir.x, ldir.y, ldir.z, nn.x, nn.y, nn.z, gq);
+                                else if (pmix)
+                                    pb = (1.0f - P.alpha) * pb +
+                                         P.alpha * mlp::hemi_pdf(pmix, ldir.x, ldir.y, ldir.z, nn.x, nn.y, nn.z);
+                                const float w = power_heuristic(pl, pb);
+                                contrib_w = s4mul(mask, s4scale(contrib, w));
+                            }
+                            // AnyHitSegmentBVH (intersection.cuh:285-310)
+                            float tmx = 0.0f;
+                            F3 sdir = f3(0, 0, 1);
+                            if (dist > 0.0f) {
+                                sdir = scale(lu, 1.0f / dist);
+                                const float seps = eps_table[depth < 255 ? depth : 255] *
+                                                   maxr(1.0f, maxr(length(pos), length(qpt)));
+                                tmx = maxr(0.0f, dist - seps);
+                            }
+                            if (tmx > 0.0f) {
+                                shadow = true;
+                                s_o = pos; s_d = sdir; s_tmax = tmx; s_contrib = contrib_w; s_flag = faces ? 1u : 0u;
+                            } else {
+                                // nothing to test: visible
+                                if (faces) { rad = s4add(rad, contrib_w); nee_now = true; }
+                            }
                         }
                     }
                     // the resolve pass sets the NEE bit for queued shadow segments
-                    new_meta = non_delta | (nee_now ? kMetaNee : 0u);
+                    new_meta = (meta & kMetaHeroOnly) | non_delta | (nee_now ? kMetaNee : 0u);
                     S.last_pdf[p] = pdf;
                     S.lp0[p] = pos.x; S.lp1[p] = pos.y; S.lp2[p] = pos.z;
                     const float inv_pdf = 1.0f / maxr(pdf, 1e-12f);
@@ -870,20 +1215,25 @@ __device__ __forceinline__ void shade_one(
                     bool alive = true;
                     if (non_delta >= P.rr_min_depth) {
                         qrr = clampr(lum4(mask, P.spectral) * 0.5f + max4(mask, P.spectral) * 0.5f, 0.05f, 0.95f);
+                        if (medium != 0u) qrr = qrr * (1.0f - 0.99f) + 0.99f;
                         if (u[9] > qrr) alive = false;
                     }
                     if (alive && depth + 1 < P.max_depth) {
                         if (GM != AR_GUIDING_OFF)
                             g_advance(G, p, non_delta, P.spectral, gkey,
                                       learned ? 0 : guide::leaf_of(next.x, next.y, next.z), contrib, qrr, learned, x5,
-                                      next, pdf);
+                                      next, pdf, medium == 0u);
                         mask = s4scale(s4mul(mask, contrib), 1.0f / qrr);
                         cont = true;
                         next_o = pos;
                         next_d = next;
                         const uint32_t nd = non_delta + 1;
                         next_tmin = eps_table[nd < 255 ? nd : 255];
-                        new_meta = (new_meta & kMetaNee) | nd;
+                        new_meta = (new_meta & (kMetaNee | kMetaHeroOnly)) | nd;
+                        if (medium != 0u) {
+                            const float4* mo = sc.objs + (size_t)obj::kRecord * (medium - 1u);
+                            root = __float_as_uint(mo[4].w);
+                        }
                     }
                 }
             }
@@ -919,13 +1269,13 @@ __global__ __launch_bounds__(kMaxBlock) void k_shade(
     RenderParams P, uint32_t depth, SceneArgs sc, PathArgs S, GuideArgs G,
     QueueRays cur, const uint32_t* __restrict__ qcur,
     const uint32_t* __restrict__ hprim, const float* __restrict__ ht, const float* __restrict__ hu,
-    const float* __restrict__ hv,
+    const float* __restrict__ hv, const uint32_t* __restrict__ hflags,
     QueueRays nxt, uint32_t* __restrict__ qnext,
     QueueRays sh, uint32_t* __restrict__ spath, float* __restrict__ sc0, float* __restrict__ sc1,
     float* __restrict__ sc2, float* __restrict__ sc3, uint32_t* __restrict__ sflags,
     uint32_t* __restrict__ counters, const float* __restrict__ eps_table) {
-    shade_one<GM>(P, depth, sc, S, G, cur, qcur, hprim, ht, hu, hv, nxt, qnext, sh, spath, sc0, sc1, sc2, sc3, sflags,
-                  counters, eps_table, blockIdx.x * blockDim.x + threadIdx.x, counters[0]);
+    shade_one<GM>(P, depth, sc, S, G, cur, qcur, hprim, ht, hu, hv, hflags, nxt, qnext, sh, spath, sc0, sc1, sc2, sc3,
+                  sflags, counters, eps_table, blockIdx.x * blockDim.x + threadIdx.x, counters[0]);
 }
 
 // Shadow results: an occluded segment still counts as a taken NEE sample
@@ -1214,8 +1564,8 @@ static int ensure_render_buffers(ar_ctx* ctx, size_t cap, size_t npix) {
     b.capacity = 0;
     // 4-byte arrays of cap entries: 2x8 queue rays, 2 qpath, 5 hits, 8 shadow rays,
     // spath, 4 scontrib, sflags, 4 mask, 4 rad, 3 last_pos, last_pdf, meta, lambda0,
-    // pix, smp, scn, 2 qroot, sroot = 57
-    const size_t arrays = 57;
+    // pix, smp, scn, 2 qroot, sroot, med, 3 mpos = 61
+    const size_t arrays = 61;
     ctx->buffer_generation += 1;
     const size_t occ_words = (cap + 31) / 32 + 64;
     const size_t bytes = arrays * cap * 4 + occ_words * 4 + 64;
@@ -1248,6 +1598,8 @@ static int ensure_render_buffers(ar_ctx* ctx, size_t cap, size_t npix) {
     b.qroot[0] = (uint32_t*)take(cap);
     b.qroot[1] = (uint32_t*)take(cap);
     b.sroot = (uint32_t*)take(cap);
+    b.med = (uint32_t*)take(cap);
+    for (int k = 0; k < 3; ++k) b.mpos[k] = (float*)take(cap);
     b.occ = (uint32_t*)take(occ_words);
     b.counters = (uint32_t*)take(16);
     b.capacity = cap;
@@ -1306,6 +1658,22 @@ struct RenderJob {
     bool guiding = false, learned = false;
 };
 
+// The first object containing the camera (Overlap, intersection.cuh:312-327;
+// CalculateRadiance:536-538): camera rays start inside it and trace it alone.
+static void camera_medium(const ar_ctx* ctx, const float pos[3], SceneView& v) {
+    v.medium0 = 0;
+    v.root0 = v.root;
+    const SceneDev& s = ctx->scene;
+    for (uint32_t k = 0; k < s.nobj; ++k) {
+        const float4* r = &s.host_objs[(size_t)obj::kRecord * k];
+        if (obj::overlap_host(r, pos)) {
+            v.medium0 = k + 1;
+            std::memcpy(&v.root0, &r[4].w, 4);
+            return;
+        }
+    }
+}
+
 static void camera_basis(const ar_camera* cam, uint32_t width, uint32_t height, SceneView& v) {
     // the reference computes the basis per pixel (core/scene.cu:154-164):
     // FOV = vfov * 3.1415926 / 180
@@ -1345,8 +1713,6 @@ static int make_job(ar_ctx* ctx, uint32_t n, const ar_camera* cams, const ar_ren
         return set_error(ctx, AR_E_INVALID, "spectral must be AR_SPECTRAL_RGB, _SINGLE or _HERO4");
     if (o.spectral != AR_SPECTRAL_RGB && o.sky == AR_SKY_GRADIENT)
         return set_error(ctx, AR_E_UNSUPPORTED, "the gradient sky is RGB-only; use AR_SKY_CONSTANT in spectral modes");
-    if (ctx->scene.nobj)
-        return set_error(ctx, AR_E_UNSUPPORTED, "rendering scenes with analytic objects is not supported yet");
     J.guiding = o.guiding != AR_GUIDING_OFF;
     J.learned = o.guiding == AR_GUIDING_MLP;
     if (J.guiding && n != 1) return set_error(ctx, AR_E_UNSUPPORTED, "guiding renders one scene at a time");
@@ -1370,8 +1736,10 @@ static int make_job(ar_ctx* ctx, uint32_t n, const ar_camera* cams, const ar_ren
         v.out_base = (uint32_t)L.pixels;
         if (ctx->batch.empty()) {
             v.root = ctx->bvh.root; v.light_off = 0; v.light_cnt = ctx->scene.nlights;
+            camera_medium(ctx, cams[i].pos, v);
         } else {
             v.root = ctx->batch[i].root; v.light_off = ctx->batch[i].light_off; v.light_cnt = ctx->batch[i].light_cnt;
+            v.root0 = v.root;
         }
         camera_basis(&cams[i], oi.width, oi.height, v);
         v.out = outs[i];
@@ -1509,9 +1877,10 @@ static int enqueue_job(ar_ctx* ctx, RenderJob& J, hipStream_t stream) {
     RenderBuffers& b = ctx->rb;
     const uint32_t blk = ctx->tile;
     PathArgs S{b.mask[0], b.mask[1], b.mask[2], b.mask[3], b.rad[0], b.rad[1], b.rad[2], b.rad[3],
-               b.last_pos[0], b.last_pos[1], b.last_pos[2], b.last_pdf, b.meta, b.lambda0, b.pix, b.smp, b.scn};
-    SceneArgs sc{ctx->scene.prims, ctx->scene.mats, ctx->scene.light_tri, ctx->scene.light_area,
-                 ctx->scene.light_prob, ctx->scene.light_cdf, b.cie, b.d65};
+               b.last_pos[0], b.last_pos[1], b.last_pos[2], b.last_pdf, b.meta, b.lambda0, b.pix, b.smp, b.scn,
+               b.med, b.mpos[0], b.mpos[1], b.mpos[2]};
+    SceneArgs sc{ctx->scene.prims, ctx->scene.objs, ctx->scene.ntri, ctx->scene.mats, ctx->scene.light_tri,
+                 ctx->scene.light_area, ctx->scene.light_prob, ctx->scene.light_cdf, b.cie, b.d65};
     QueueRays q[2] = {{b.rays[0][0], b.rays[0][1], b.rays[0][2], b.rays[0][3], b.rays[0][4], b.rays[0][5], b.rays[0][6],
                        b.rays[0][7], b.qroot[0]},
                       {b.rays[1][0], b.rays[1][1], b.rays[1][2], b.rays[1][3], b.rays[1][4], b.rays[1][5], b.rays[1][6],
@@ -1558,14 +1927,16 @@ static int enqueue_job(ar_ctx* ctx, RenderJob& J, hipStream_t stream) {
             if (J.learned && P.alpha > 0.0f) {
                 const SceneDev& sd = ctx->scene;
                 hipLaunchKernelGGL(k_mlp_features, dim3(grid_for(P.capacity, blk)), dim3(blk), 0, stream, sd.prims,
-                                   q[c], b.hit_prim, b.hit_u, b.hit_v, b.counters, sd.box_lo[0], sd.box_lo[1],
-                                   sd.box_lo[2], sd.box_inv[0], sd.box_inv[1], sd.box_inv[2], ctx->mlp.feat);
+                                   sd.objs, sd.ntri, q[c], b.hit_prim, b.hit_t, b.hit_u, b.hit_v, b.hit_flags,
+                                   b.counters, sd.box_lo[0], sd.box_lo[1], sd.box_lo[2], sd.box_inv[0], sd.box_inv[1],
+                                   sd.box_inv[2], ctx->mlp.feat);
                 e = mlp_forward(ctx, ctx->mlp.feat, P.capacity, &b.counters[0], ctx->mlp.dec, true, stream);
                 if (e != hipSuccess) return check_hip(ctx, e, "guiding network inference");
             }
 #define AR_SHADE(GM)                                                                                           \
     hipLaunchKernelGGL(k_shade<GM>, dim3(grid_for(P.capacity, blk)), dim3(blk), 0, stream, P, depth, sc, S, G, q[c], \
-                       b.qpath[c], b.hit_prim, b.hit_t, b.hit_u, b.hit_v, q[1 - c], b.qpath[1 - c], shq, b.spath,   \
+                       b.qpath[c], b.hit_prim, b.hit_t, b.hit_u, b.hit_v, b.hit_flags, q[1 - c], b.qpath[1 - c], shq,  \
+                       b.spath,                                                                                     \
                        b.scontrib[0], b.scontrib[1], b.scontrib[2], b.scontrib[3], b.sflags, b.counters, b.eps_table)
             if (P.guiding == AR_GUIDING_VOXEL) AR_SHADE(AR_GUIDING_VOXEL);
             else if (P.guiding == AR_GUIDING_MLP) AR_SHADE(AR_GUIDING_MLP);

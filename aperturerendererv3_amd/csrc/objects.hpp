@@ -24,6 +24,7 @@
 #include <cfloat>
 #include <cmath>
 #include <cstdint>
+#include <cstring>
 
 namespace ar {
 namespace obj {
@@ -76,6 +77,21 @@ inline void proxy(uint32_t type, const float c[3], const float e[3], const Basis
 inline float area(uint32_t type, const float e[3]) {
     if (type == AR_OBJ_BOX) return 8.0f * (e[0] * e[1] + e[0] * e[2] + e[1] * e[2]);
     return 4.0f * 3.141592654f * e[0] * e[0];
+}
+
+// OverlapAABBObject / OverlapSphereObject: strict containment of a point
+// (host side, on the device record layout)
+inline bool overlap_host(const float4* r, const float p[3]) {
+    const float ox = p[0] - r[0].x, oy = p[1] - r[0].y, oz = p[2] - r[0].z;
+    const float lx = r[2].x * ox + r[2].y * oy + r[2].z * oz;
+    const float ly = r[3].x * ox + r[3].y * oy + r[3].z * oz;
+    const float lz = r[4].x * ox + r[4].y * oy + r[4].z * oz;
+    uint32_t type;
+    std::memcpy(&type, &r[0].w, 4);
+    const float4 e = r[1];
+    if (type == AR_OBJ_BOX)
+        return -e.x < lx && -e.y < ly && -e.z < lz && e.x > lx && e.y > ly && e.z > lz;
+    return lx * lx + ly * ly + lz * lz < e.w * e.w;
 }
 
 // ---------------------------------------------------------------- device

@@ -174,7 +174,8 @@ typedef struct {
     float albedo[3];
     float emission[3];
     float specular, metallic, roughness;
-    uint32_t model;       /* 0 Lambert, 1 GGX + Lambert */
+    uint32_t model;       /* 0 Lambert, 1 GGX + Lambert, 2 glass (objects only) */
+    float ior, sigma_a, abbe, tilt;   /* glass: n_d, absorption scale, Abbe number, spectral tilt */
 } orc_material;
 
 typedef struct {
@@ -187,14 +188,18 @@ typedef struct {
     const orc_node* nodes;
     int node_count;
     const int32_t* indices;
-    const int32_t* light_index;   /* ntri: index into the light table or -1 */
+    const int32_t* light_index;   /* ntri + nobj: index into the light table or -1 */
+    const orc_object* objs;       /* analytic objects, prims ntri .. ntri + nobj - 1 */
+    uint32_t nobj;
+    const orc_material* obj_mats; /* material of each object */
+    uint32_t nmats;               /* mesh materials; object k's spectral coefficients are row nmats + k */
     const float* mat_coef;        /* spectral modes: per material calb[3], cem[3], escale, 0 */
     float env_coef[4];            /* spectral constant sky: coefficients, scale */
 } orc_render_scene;
 
 typedef struct {
     int count;
-    const int32_t* tri;    /* light triangle ids */
+    const int32_t* tri;    /* light primitive ids (triangles, then objects at ntri + k) */
     const float* area;
     const float* prob;     /* select probability = w / sum(w), w = max(area * lum(Le), 1e-20) */
     const float* cdf;      /* running FP32 sum of prob */
@@ -271,6 +276,9 @@ typedef struct {
     const float* lut;      /* 3 x 64^3 x 3 sigmoid-coefficient table (spectral modes) */
     int guiding;           /* 1: voxel guiding, one sample per pixel per frame, probes updated per frame */
     orc_guide* guide;      /* probe store used and updated when guiding (kept across calls) */
+    const orc_object* objects;     /* analytic objects (may be NULL) */
+    uint32_t nobj;
+    const orc_material* obj_mats;  /* nobj materials */
 } orc_render_opts;
 
 /* Sigmoid coefficients of an RGB colour (channels clamped to [0, 1]) from the
@@ -284,6 +292,16 @@ float orc_ndtri_f(float u);
 float orc_lambda_pdf_f(float l);
 float orc_sample_lambda_f(float u0, float u1, float u2);
 float orc_sigmoid_f(const float c[3], float l);
+
+/* Box / sphere light sampler (a point y, its normal and area density without
+ * the select probability; 0 when nothing faces x) and the solid-angle
+ * density NEE assigns to a light point, select probability included
+ * (NEESampleBoxAreaFacing / NEESampleSphereCapArea and
+ * NEEPDFOmegaFromHit_Box / _Sphere); FP64 instances. */
+int orc_obj_light_sample_d(const orc_object* ob, const double x[3], double ub, double u0, double u1, double y[3],
+                           double ny[3], double* pa);
+double orc_obj_light_pdf_d(const orc_object* ob, double select, const double x[3], const double y[3],
+                           const double ny[3]);
 
 /* Renders the mean of spp samples per pixel into out (width*height*3,
  * row-major, pixel index = j*width + i).  precision 32: float out, FP32

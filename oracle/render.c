@@ -51,6 +51,45 @@ void orc_philox_uniforms(uint64_t seed, uint32_t pix, uint32_t smp, uint32_t bou
     }
 }
 
+/* ---------------- instantiations ---------------- */
+
+#define RF_CAT(a, b) RF_CAT2(a, b)
+#define RF_CAT2(a, b) a##_##b
+
+#define R float
+#define SUF f32
+#define SQRT sqrtf
+#define SIN sinf
+#define COS cosf
+#define TAN tanf
+#define EXP expf
+#define POW powf
+#define FABS fabsf
+#define PI_R 3.14159265358979323846f
+#define LOG logf
+#define LOG1P log1pf
+#define FMA fmaf
+#define ERFC erfcf
+#define FLOOR floorf
+#include "render_impl.h"
+
+#define R double
+#define SUF f64
+#define SQRT sqrt
+#define SIN sin
+#define COS cos
+#define TAN tan
+#define EXP exp
+#define POW pow
+#define FABS fabs
+#define PI_R 3.14159265358979323846
+#define LOG log
+#define LOG1P log1p
+#define FMA fma
+#define ERFC erfc
+#define FLOOR floor
+#include "render_impl.h"
+
 /* ---------------- FP64 traversal (watertight test in double) ---------------- */
 
 static int tri_test_d(const double o[3], const double d[3], const float* v0, const float* v1, const float* v2,
@@ -95,10 +134,12 @@ static int box_hit_d(const double o[3], const double inv[3], const orc_node* nd,
     return tx >= fmax(0.0, tmin) && te <= tmax && te <= tx;
 }
 
-/* any = 1: first accepted hit.  Canonical rule (min t, then min id). */
+/* any = 1: first accepted hit.  Canonical rule (min t, then min id); objects
+ * by the FP64 instance of the object intersector. */
 static uint32_t trace_d(const orc_render_scene* sc, const double o[3], const double d[3], double tmin, double tmax,
-                        int any, double* tout, double* uout, double* vout)
+                        int any, double* tout, double* uout, double* vout, int* inside)
 {
+    *inside = 0;
     if (sc->node_count == 0 || !(tmax >= tmin)) return ORC_MISS;
     double inv[3];
     for (int a = 0; a < 3; ++a) inv[a] = d[a] == 0 ? (signbit(d[a]) ? -DBL_MAX : DBL_MAX) : 1.0 / d[a];
@@ -113,14 +154,20 @@ static uint32_t trace_d(const orc_render_scene* sc, const double o[3], const dou
         if (nd->count > 0) {
             for (int k = 0; k < nd->count; ++k) {
                 const uint32_t p = (uint32_t)sc->indices[nd->first + k];
-                const float* a = &sc->verts[3 * sc->idx[3 * p]];
-                const float* b = &sc->verts[3 * sc->idx[3 * p + 1]];
-                const float* c = &sc->verts[3 * sc->idx[3 * p + 2]];
-                double t, u, v;
-                if (!tri_test_d(o, d, a, b, c, &t, &u, &v)) continue;
+                double t, u = 0, v = 0;
+                int ins = 0;
+                if (p < sc->ntri) {
+                    const float* a = &sc->verts[3 * sc->idx[3 * p]];
+                    const float* b = &sc->verts[3 * sc->idx[3 * p + 1]];
+                    const float* c = &sc->verts[3 * sc->idx[3 * p + 2]];
+                    if (!tri_test_d(o, d, a, b, c, &t, &u, &v)) continue;
+                } else {
+                    const vec3_f64 oo = { o[0], o[1], o[2] }, dd = { d[0], d[1], d[2] };
+                    if (!obj_test_f64(&sc->objs[p - sc->ntri], oo, dd, &t, &ins)) continue;
+                }
                 const int acc = t >= tmin && (best != ORC_MISS ? (t < bt || (t == bt && p < best)) : t < tmax);
                 if (acc) {
-                    best = p; bt = t; bu = u; bv = v;
+                    best = p; bt = t; bu = u; bv = v; *inside = ins;
                     if (any) { *tout = bt; return best; }
                 }
             }
@@ -134,74 +181,45 @@ static uint32_t trace_d(const orc_render_scene* sc, const double o[3], const dou
     return best;
 }
 
-/* ---------------- instantiations ---------------- */
-
-#define RF_CAT(a, b) RF_CAT2(a, b)
-#define RF_CAT2(a, b) a##_##b
-
-#define R float
-#define SUF f32
-#define SQRT sqrtf
-#define SIN sinf
-#define COS cosf
-#define TAN tanf
-#define EXP expf
-#define POW powf
-#define FABS fabsf
-#define PI_R 3.14159265358979323846f
-#define LOG logf
-#define LOG1P log1pf
-#define FMA fmaf
-#define ERFC erfcf
-#define FLOOR floorf
-#include "render_impl.h"
-
-#define R double
-#define SUF f64
-#define SQRT sqrt
-#define SIN sin
-#define COS cos
-#define TAN tan
-#define EXP exp
-#define POW pow
-#define FABS fabs
-#define PI_R 3.14159265358979323846
-#define LOG log
-#define LOG1P log1p
-#define FMA fma
-#define ERFC erfc
-#define FLOOR floor
-#include "render_impl.h"
+static orc_geom scene_geom(const orc_render_scene* sc)
+{
+    orc_geom g = { sc->verts, sc->idx, sc->ntri, sc->objs, sc->nobj };
+    return g;
+}
 
 static uint32_t closest_f32(const orc_render_scene* sc, vec3_f32 o, vec3_f32 d, float tmin, float tmax,
-                            float* t, float* bu, float* bv)
+                            float* t, float* bu, float* bv, int* inside)
 {
     const float oo[3] = { o.x, o.y, o.z }, dd[3] = { d.x, d.y, d.z };
-    uint32_t p;
+    const orc_geom g = scene_geom(sc);
+    uint32_t p, f = 0;
     *bu = *bv = 0.0f;
-    orc_closest1(sc->nodes, sc->node_count, sc->indices, sc->verts, sc->idx, oo, dd, tmin, tmax, &p, t, bu, bv);
+    orc_closest1_g(sc->nodes, sc->node_count, sc->indices, &g, oo, dd, tmin, tmax, &p, t, bu, bv, &f);
+    *inside = (f & ORC_HIT_INSIDE) != 0;
     return p;
 }
 
 static int occluded_f32(const orc_render_scene* sc, vec3_f32 o, vec3_f32 d, float tmin, float tmax)
 {
     const float oo[3] = { o.x, o.y, o.z }, dd[3] = { d.x, d.y, d.z };
-    return orc_any1(sc->nodes, sc->node_count, sc->indices, sc->verts, sc->idx, oo, dd, tmin, tmax);
+    const orc_geom g = scene_geom(sc);
+    return orc_any1_g(sc->nodes, sc->node_count, sc->indices, &g, oo, dd, tmin, tmax);
 }
 
 static uint32_t closest_f64(const orc_render_scene* sc, vec3_f64 o, vec3_f64 d, double tmin, double tmax,
-                            double* t, double* bu, double* bv)
+                            double* t, double* bu, double* bv, int* inside)
 {
     const double oo[3] = { o.x, o.y, o.z }, dd[3] = { d.x, d.y, d.z };
     *bu = *bv = 0.0;
-    return trace_d(sc, oo, dd, tmin, tmax, 0, t, bu, bv);
+    return trace_d(sc, oo, dd, tmin, tmax, 0, t, bu, bv, inside);
 }
 
 static int occluded_f64(const orc_render_scene* sc, vec3_f64 o, vec3_f64 d, double tmin, double tmax)
 {
     const double oo[3] = { o.x, o.y, o.z }, dd[3] = { d.x, d.y, d.z };
     double t, u, v;
-    return trace_d(sc, oo, dd, tmin, tmax, 1, &t, &u, &v) != ORC_MISS;
+    int ins;
+    return trace_d(sc, oo, dd, tmin, tmax, 1, &t, &u, &v, &ins) != ORC_MISS;
 }
 
 /* ---------------- RGB uplift lookup ---------------- */
@@ -264,28 +282,35 @@ int orc_render(const float* verts, uint32_t nv, const uint32_t* idx, uint32_t nt
     (void)nv;
     orc_render_scene sc;
     memset(&sc, 0, sizeof sc);
+    const uint32_t nobj = opts->objects ? opts->nobj : 0;
+    const uint32_t np = ntri + nobj;
+    const size_t cap = np ? np : 1;
     sc.verts = verts; sc.idx = idx; sc.ntri = ntri; sc.tri_mat = tri_mat; sc.mats = mats;
-    float* pmin = malloc(sizeof(float) * 3 * (ntri ? ntri : 1));
-    float* pmax = malloc(sizeof(float) * 3 * (ntri ? ntri : 1));
-    float* pcen = malloc(sizeof(float) * 3 * (ntri ? ntri : 1));
-    orc_node* nodes = malloc(sizeof(orc_node) * 2 * (ntri ? ntri : 1));
-    int32_t* indices = malloc(sizeof(int32_t) * (ntri ? ntri : 1));
-    int32_t* light_index = malloc(sizeof(int32_t) * (ntri ? ntri : 1));
-    int32_t* ltri = malloc(sizeof(int32_t) * (ntri ? ntri : 1));
-    float* larea = malloc(sizeof(float) * (ntri ? ntri : 1));
-    float* lprob = malloc(sizeof(float) * (ntri ? ntri : 1));
-    float* lcdf = malloc(sizeof(float) * (ntri ? ntri : 1));
-    orc_tri_proxies(verts, idx, (int)ntri, pmin, pmax, pcen);
-    sc.node_count = orc_bvh_build(pmin, pmax, pcen, (int)ntri, 8, 12, nodes, indices);
+    sc.objs = opts->objects; sc.nobj = nobj; sc.obj_mats = opts->obj_mats; sc.nmats = nmats;
+    float* pmin = malloc(sizeof(float) * 3 * cap);
+    float* pmax = malloc(sizeof(float) * 3 * cap);
+    float* pcen = malloc(sizeof(float) * 3 * cap);
+    orc_node* nodes = malloc(sizeof(orc_node) * 2 * cap);
+    int32_t* indices = malloc(sizeof(int32_t) * cap);
+    int32_t* light_index = malloc(sizeof(int32_t) * cap);
+    int32_t* ltri = malloc(sizeof(int32_t) * cap);
+    float* larea = malloc(sizeof(float) * cap);
+    float* lprob = malloc(sizeof(float) * cap);
+    float* lcdf = malloc(sizeof(float) * cap);
+    const orc_geom g = { verts, idx, ntri, opts->objects, nobj };
+    orc_geom_proxies(&g, pmin, pmax, pcen);
+    sc.node_count = orc_bvh_build(pmin, pmax, pcen, (int)np, 8, 12, nodes, indices);
     sc.nodes = nodes;
     sc.indices = indices;
     int nl = 0;
     float total = 0.0f;
-    for (uint32_t p = 0; p < ntri; ++p) {
-        const orc_material* m = &mats[tri_mat ? tri_mat[p] : 0];
+    for (uint32_t p = 0; p < np; ++p) {
+        const orc_material* m = p < ntri ? &mats[tri_mat ? tri_mat[p] : 0] : &opts->obj_mats[p - ntri];
         light_index[p] = -1;
         if (m->emission[0] > 0.0f || m->emission[1] > 0.0f || m->emission[2] > 0.0f) {
-            const float area = tri_area_f(&verts[3 * idx[3 * p]], &verts[3 * idx[3 * p + 1]], &verts[3 * idx[3 * p + 2]]);
+            const float area = p < ntri ? tri_area_f(&verts[3 * idx[3 * p]], &verts[3 * idx[3 * p + 1]],
+                                                     &verts[3 * idx[3 * p + 2]])
+                                        : orc_object_area(&opts->objects[p - ntri]);
             const float lum = m->emission[0] * 0.2126f + m->emission[1] * 0.7152f + m->emission[2] * 0.0722f;
             const float w = fmaxf(area * lum, 1.0e-20f);
             light_index[p] = nl;
@@ -305,10 +330,12 @@ int orc_render(const float* verts, uint32_t nv, const uint32_t* idx, uint32_t nt
     sc.light_index = light_index;
     float* mat_coef = NULL;
     if (opts->spectral != 0) {
-        mat_coef = calloc((size_t)8 * (nmats ? nmats : 1), sizeof(float));
-        for (uint32_t i = 0; i < nmats; ++i) {
-            orc_uplift(opts->lut, mats[i].albedo, &mat_coef[8 * i]);
-            float e[3] = { mats[i].emission[0], mats[i].emission[1], mats[i].emission[2] };
+        const uint32_t nm = nmats + nobj;
+        mat_coef = calloc((size_t)8 * (nm ? nm : 1), sizeof(float));
+        for (uint32_t i = 0; i < nm; ++i) {
+            const orc_material* mi = i < nmats ? &mats[i] : &opts->obj_mats[i - nmats];
+            orc_uplift(opts->lut, mi->albedo, &mat_coef[8 * i]);
+            float e[3] = { mi->emission[0], mi->emission[1], mi->emission[2] };
             const float mx = fmaxf(e[0], fmaxf(e[1], e[2]));
             float scale = 1.0f;
             if (mx > 1.0f) { for (int k = 0; k < 3; ++k) e[k] /= mx; scale = mx; }
@@ -341,3 +368,22 @@ float orc_ndtri_f(float u) { return ndtri_f32(u); }
 float orc_lambda_pdf_f(float l) { return lambda_pdf_f32(l); }
 float orc_sample_lambda_f(float u0, float u1, float u2) { return sample_lambda_f32(u0, u1, u2); }
 float orc_sigmoid_f(const float c[3], float l) { return sigmoid_f32(c, l); }
+
+/* Object light sampler and its density, FP64 instance (for the tests). */
+int orc_obj_light_sample_d(const orc_object* ob, const double x[3], double ub, double u0, double u1, double y[3],
+                           double ny[3], double* pa)
+{
+    const vec3_f64 xv = { x[0], x[1], x[2] };
+    vec3_f64 yv, nv;
+    const int ok = obj_light_sample_f64(ob, xv, ub, u0, u1, &yv, &nv, pa);
+    y[0] = yv.x; y[1] = yv.y; y[2] = yv.z;
+    ny[0] = nv.x; ny[1] = nv.y; ny[2] = nv.z;
+    return ok;
+}
+
+double orc_obj_light_pdf_d(const orc_object* ob, double select, const double x[3], const double y[3],
+                           const double ny[3])
+{
+    const vec3_f64 xv = { x[0], x[1], x[2] }, yv = { y[0], y[1], y[2] }, nv = { ny[0], ny[1], ny[2] };
+    return obj_light_pdf_f64(ob, select, xv, yv, nv);
+}

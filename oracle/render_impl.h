@@ -335,6 +335,7 @@ typedef struct {
     R specular, metallic, roughness, escale;
     uint32_t model;
     float ca[3], ce[3];
+    R ior, sigma_a, abbe, tilt;
 } RF(mat);
 
 static R RF(f0_ue)(R spec) { return (R)0.08 * RF(clampr)(spec, 0, 1); }
@@ -481,8 +482,10 @@ static V3 RF(sky)(V3 d)
 /* Scene view, traversal in R, lights                                      */
 /* ---------------------------------------------------------------------- */
 
-/* closest hit in R: prim (ORC_MISS on a miss), t, barycentric weights of v1, v2 */
-static uint32_t RF(closest)(const orc_render_scene* sc, V3 o, V3 d, R tmin, R tmax, R* t, R* bu, R* bv);
+/* closest hit in R: prim (ORC_MISS on a miss), t, barycentric weights of v1,
+ * v2, and for an object whether the ray started inside it */
+static uint32_t RF(closest)(const orc_render_scene* sc, V3 o, V3 d, R tmin, R tmax, R* t, R* bu, R* bv,
+                            int* inside);
 static int RF(occluded)(const orc_render_scene* sc, V3 o, V3 d, R tmin, R tmax);
 
 static void RF(tri)(const orc_render_scene* sc, uint32_t p, V3* a, V3* b, V3* c)
@@ -494,13 +497,15 @@ static void RF(tri)(const orc_render_scene* sc, uint32_t p, V3* a, V3* b, V3* c)
 
 static RF(mat) RF(material)(const orc_render_scene* sc, uint32_t p)
 {
-    const orc_material* om = &sc->mats[sc->tri_mat ? sc->tri_mat[p] : 0];
+    const int is_obj = p >= sc->ntri;
+    const uint32_t mi = is_obj ? sc->nmats + (p - sc->ntri) : (sc->tri_mat ? sc->tri_mat[p] : 0);
+    const orc_material* om = is_obj ? &sc->obj_mats[p - sc->ntri] : &sc->mats[mi];
     RF(mat) m;
     m.albedo = RF(load3)(om->albedo);
     m.emission = RF(load3)(om->emission);
     m.specular = om->specular; m.metallic = om->metallic; m.roughness = om->roughness;
     m.model = om->model;
-    const uint32_t mi = sc->tri_mat ? sc->tri_mat[p] : 0;
+    m.ior = om->ior; m.sigma_a = om->sigma_a; m.abbe = om->abbe; m.tilt = om->tilt;
     m.escale = 1;
     for (int k = 0; k < 3; ++k) { m.ca[k] = 0.0f; m.ce[k] = 0.0f; }
     if (sc->mat_coef) {
@@ -518,6 +523,287 @@ static int RF(pick_light)(const orc_render_lights* L, R u)
     for (int i = 0; i < L->count; ++i)
         if (u < (R)L->cdf[i]) return i;
     return L->count - 1;
+}
+
+/* ---------------------------------------------------------------------- */
+/* Analytic objects in R (the FP32 instance rounds exactly like trace.c's   */
+/* orc_object_test / orc_object_surface and the GPU kernels)               */
+/* ---------------------------------------------------------------------- */
+
+static V3 RF(w2o_row)(const orc_object* ob, int r) { return RF(load3)(&ob->w2o[3 * r]); }
+
+static void RF(obj_local)(const orc_object* ob, V3 o, V3 d, V3* lo, V3* ld)
+{
+    const V3 oc = RF(sub)(o, RF(load3)(ob->center));
+    const V3 r0 = RF(w2o_row)(ob, 0), r1 = RF(w2o_row)(ob, 1), r2 = RF(w2o_row)(ob, 2);
+    *lo = RF(v3)(RF(dot)(r0, oc), RF(dot)(r1, oc), RF(dot)(r2, oc));
+    *ld = RF(normalize)(RF(v3)(RF(dot)(r0, d), RF(dot)(r1, d), RF(dot)(r2, d)));
+}
+
+static R RF(inv_safe_ref)(R x) { return x == 0 ? (x >= 0 ? (R)FLT_MAX : -(R)FLT_MAX) : (R)1 / x; }
+
+static R RF(obj_radius)(const orc_object* ob)
+{
+    return (R)fmaxf(fmaxf(ob->extent[0], ob->extent[1]), ob->extent[2]);
+}
+
+/* IntersectAABBObject / IntersectSphereObject (aabb.cuh:28-89, sphere.cuh:11-68) */
+static int RF(obj_test)(const orc_object* ob, V3 o, V3 d, R* t, int* inside)
+{
+    V3 lo, ld;
+    RF(obj_local)(ob, o, d, &lo, &ld);
+    *inside = 0;
+    if (ob->type == ORC_OBJ_BOX) {
+        const R lox[3] = { lo.x, lo.y, lo.z }, ldx[3] = { ld.x, ld.y, ld.z };
+        R te = 0, tx = 0;
+        for (int a = 0; a < 3; ++a) {
+            const R inv = RF(inv_safe_ref)(ldx[a]);
+            const R t1 = (-(R)ob->extent[a] - lox[a]) * inv;
+            const R t2 = ((R)ob->extent[a] - lox[a]) * inv;
+            const R mn = t1 < t2 ? t1 : t2, mx = t1 > t2 ? t1 : t2;
+            te = a == 0 ? mn : (te > mn ? te : mn);
+            tx = a == 0 ? mx : (tx < mx ? tx : mx);
+        }
+        if (te > tx) return 0;
+        if (tx <= 0) return 0;
+        *inside = te < 0;
+        *t = *inside ? tx : te;
+        return 1;
+    }
+    const R r = RF(obj_radius)(ob);
+    const R b = RF(dot)(lo, ld);
+    const R c = RF(dot)(lo, lo) - r * r;
+    const R disc = b * b - c;
+    if (disc < 0) return 0;
+    const R sq = SQRT(disc);
+    R tt = -b - sq;
+    if (tt <= 0) { tt = -b + sq; *inside = 1; }
+    if (tt <= 0) return 0;
+    *t = tt;
+    return 1;
+}
+
+/* hit point and world normal (flipped for a hit from inside) */
+static void RF(obj_surface)(const orc_object* ob, V3 o, V3 d, R t, int inside, V3* pos, V3* n)
+{
+    V3 lo, ld;
+    RF(obj_local)(ob, o, d, &lo, &ld);
+    const V3 lp = RF(add)(lo, RF(scale)(ld, t));
+    V3 ln = RF(v3)(0, 0, 0);
+    if (ob->type == ORC_OBJ_BOX) {
+        const R rx = FABS(lp.x) / (R)ob->extent[0], ry = FABS(lp.y) / (R)ob->extent[1],
+                rz = FABS(lp.z) / (R)ob->extent[2];
+        if (rx >= ry && rx >= rz) ln.x = lp.x > 0 ? (R)1 : (R)-1;
+        else if (ry >= rz) ln.y = lp.y > 0 ? (R)1 : (R)-1;
+        else ln.z = lp.z > 0 ? (R)1 : (R)-1;
+        if (inside) ln = RF(scale)(ln, -1);
+    } else {
+        ln = RF(scale)(RF(normalize)(lp), inside ? (R)-1 : (R)1);
+    }
+    const V3 c0 = RF(v3)((R)ob->w2o[0], (R)ob->w2o[3], (R)ob->w2o[6]);
+    const V3 c1 = RF(v3)((R)ob->w2o[1], (R)ob->w2o[4], (R)ob->w2o[7]);
+    const V3 c2 = RF(v3)((R)ob->w2o[2], (R)ob->w2o[5], (R)ob->w2o[8]);
+    *pos = RF(v3)(RF(dot)(c0, lp) + (R)ob->center[0], RF(dot)(c1, lp) + (R)ob->center[1],
+                  RF(dot)(c2, lp) + (R)ob->center[2]);
+    *n = RF(normalize)(RF(v3)(RF(dot)(c0, ln), RF(dot)(c1, ln), RF(dot)(c2, ln)));
+}
+
+/* OverlapAABBObject / OverlapSphereObject (aabb.cuh:91-105, sphere.cuh:70-84) */
+static int RF(obj_overlap)(const orc_object* ob, V3 p)
+{
+    const V3 oc = RF(sub)(p, RF(load3)(ob->center));
+    const V3 lo = RF(v3)(RF(dot)(RF(w2o_row)(ob, 0), oc), RF(dot)(RF(w2o_row)(ob, 1), oc),
+                         RF(dot)(RF(w2o_row)(ob, 2), oc));
+    if (ob->type == ORC_OBJ_BOX) {
+        const R ex = ob->extent[0], ey = ob->extent[1], ez = ob->extent[2];
+        return -ex < lo.x && -ey < lo.y && -ez < lo.z && ex > lo.x && ey > lo.y && ez > lo.z;
+    }
+    const R r = RF(obj_radius)(ob);
+    return RF(dot)(lo, lo) < r * r;
+}
+
+/* The six facets of a box light: outward normal = +-normalised
+ * worldToObject row (GetOBBAxes, nee.cuh:24-31), centre = c +- u e, tangent
+ * and bitangent axes, half sides s1, s2 (nee.cuh:164-183). */
+static void RF(box_facets)(const orc_object* ob, V3 nrm[6], V3 cen[6], V3 tan[6], V3 bit[6], R s1[6], R s2[6])
+{
+    const V3 ux = RF(normalize)(RF(w2o_row)(ob, 0)), uy = RF(normalize)(RF(w2o_row)(ob, 1)),
+             uz = RF(normalize)(RF(w2o_row)(ob, 2));
+    const V3 c = RF(load3)(ob->center);
+    const R ex = ob->extent[0], ey = ob->extent[1], ez = ob->extent[2];
+    const V3 ax[3] = { ux, uy, uz };
+    const R e[3] = { ex, ey, ez };
+    for (int f = 0; f < 6; ++f) {
+        const int a = f >> 1;
+        const R sg = (f & 1) ? (R)-1 : (R)1;
+        nrm[f] = RF(scale)(ax[a], sg);
+        cen[f] = (f & 1) ? RF(sub)(c, RF(scale)(ax[a], e[a])) : RF(add)(c, RF(scale)(ax[a], e[a]));
+    }
+    for (int f = 0; f < 2; ++f) { tan[f] = uy; bit[f] = uz; s1[f] = ey; s2[f] = ez; }
+    for (int f = 2; f < 4; ++f) { tan[f] = ux; bit[f] = uz; s1[f] = ex; s2[f] = ez; }
+    for (int f = 4; f < 6; ++f) { tan[f] = ux; bit[f] = uy; s1[f] = ex; s2[f] = ey; }
+}
+
+/* NEEFacingWeight_Box (nee.cuh:55-61) */
+static R RF(facing_weight)(V3 n, V3 c, V3 x, R area)
+{
+    const V3 l2s = RF(normalize)(RF(sub)(x, c));
+    return area * RF(maxr)((R)0, RF(dot)(n, l2s));
+}
+
+/* AreaToSolidAnglePDF (nee.cuh:35-46) */
+static R RF(area_to_solid)(V3 x, V3 y, V3 ny, R pa_total)
+{
+    const V3 s2l = RF(sub)(y, x);
+    const R dist2 = RF(maxr)(RF(dot)(s2l, s2l), (R)1e-20);
+    const V3 dir = RF(scale)(s2l, (R)1 / SQRT(dist2));
+    const R cosy = RF(maxr)((R)0, RF(dot)(ny, RF(scale)(dir, -1)));
+    if (cosy <= 0) return 0;
+    return pa_total * (dist2 / cosy);
+}
+
+/* NEEPDFOmegaFromHit_Box / _Sphere (nee.cuh:63-151, 253-271): solid-angle
+ * density with which NEE from x would have sampled light point y (normal ny)
+ * of the object, select probability included */
+static R RF(obj_light_pdf)(const orc_object* ob, R select, V3 x, V3 y, V3 ny)
+{
+    if (ob->type == ORC_OBJ_BOX) {
+        V3 nrm[6], cen[6], tan[6], bit[6];
+        R s1[6], s2[6];
+        RF(box_facets)(ob, nrm, cen, tan, bit, s1, s2);
+        int hit = -1;
+        for (int f = 0; f < 6; ++f)
+            if (RF(dot)(nrm[f], ny) > (R)0.999) { hit = f; break; }
+        if (hit < 0) {
+            R best = (R)1e30;
+            for (int f = 0; f < 6; ++f) {
+                const R dd = FABS(RF(dot)(nrm[f], RF(sub)(y, cen[f])));
+                if (dd < best) { best = dd; hit = f; }
+            }
+        }
+        R sum = 0, whit = 0, ahit = 0;
+        for (int f = 0; f < 6; ++f) {
+            const R area = (R)4 * s1[f] * s2[f];
+            const R w = RF(facing_weight)(nrm[f], cen[f], x, area);
+            sum += w;
+            if (f == hit) { whit = w; ahit = area; }
+        }
+        if (sum <= 0 || whit <= 0) return 0;
+        const R pa = select * ((R)1 / ahit) * (whit / sum);
+        return RF(area_to_solid)(x, y, ny, pa);
+    }
+    const V3 c = RF(load3)(ob->center);
+    const R r = RF(obj_radius)(ob);
+    const R dist = RF(length)(RF(sub)(x, c));
+    if (dist <= r) return 0;
+    const R cosc = RF(clampr)(r / dist, 0, 1);
+    const R cap = (R)2 * PI_R * r * r * ((R)1 - cosc);
+    if (cap <= (R)1e-20) return 0;
+    return RF(area_to_solid)(x, y, ny, select / cap);
+}
+
+/* BuildTangentBasis (core/vector.cuh:277-282) */
+static void RF(tangent_basis)(V3 n, V3* t, V3* b)
+{
+    const V3 a = (FABS(n.z) < (R)0.999) ? RF(v3)(0, 0, 1) : RF(v3)(1, 0, 0);
+    *t = RF(normalize)(RF(cross)(a, n));
+    *b = RF(cross)(n, *t);
+}
+
+/* NEESampleBoxAreaFacing / NEESampleSphereCapArea (nee.cuh:153-251,
+ * 273-310): a light point y with normal ny seen from x and its area density
+ * pa (facet or cap, without the select probability).  ub: facet pick (box),
+ * u0, u1: the point.  Returns 0 when nothing is sampleable from x. */
+static int RF(obj_light_sample)(const orc_object* ob, V3 x, R ub, R u0, R u1, V3* y, V3* ny, R* pa)
+{
+    if (ob->type == ORC_OBJ_BOX) {
+        V3 nrm[6], cen[6], tan[6], bit[6];
+        R s1[6], s2[6];
+        RF(box_facets)(ob, nrm, cen, tan, bit, s1, s2);
+        int idx[6], cnt = 0;
+        R w[6], sum = 0;
+        for (int f = 0; f < 6; ++f) {
+            const R area = (R)4 * s1[f] * s2[f];
+            const R wf = RF(facing_weight)(nrm[f], cen[f], x, area);
+            if (wf > 0) { idx[cnt] = f; w[cnt] = wf; sum += wf; ++cnt; }
+        }
+        if (cnt == 0) return 0;
+        int slot = 0;
+        const R r = ub * sum;
+        R acc = 0;
+        for (int k = 0; k < cnt; ++k) {
+            acc += w[k];
+            if (r <= acc) { slot = k; break; }
+        }
+        const int f = idx[slot];
+        const R a1 = (R)2 * u0 - (R)1, a2 = (R)2 * u1 - (R)1;
+        *y = RF(add)(RF(add)(cen[f], RF(scale)(tan[f], a1 * s1[f])), RF(scale)(bit[f], a2 * s2[f]));
+        *ny = nrm[f];
+        *pa = (w[slot] / sum) * ((R)1 / ((R)4 * s1[f] * s2[f]));
+        return 1;
+    }
+    const V3 c = RF(load3)(ob->center);
+    const R r = RF(obj_radius)(ob);
+    const V3 pc = RF(sub)(x, c);
+    const R dist = RF(length)(pc);
+    if (dist <= r) return 0;
+    const V3 axis = RF(scale)(pc, (R)1 / dist);
+    const R cosc = RF(clampr)(r / dist, 0, 1);
+    const R cap = (R)2 * PI_R * r * r * ((R)1 - cosc);
+    const R cost = cosc * ((R)1 - u0) + u0;
+    const R sint = SQRT(RF(maxr)((R)0, (R)1 - cost * cost));
+    const R phi = (R)2 * PI_R * u1;
+    V3 t, b;
+    RF(tangent_basis)(axis, &t, &b);
+    const V3 nn = RF(add)(RF(add)(RF(scale)(t, COS(phi) * sint), RF(scale)(b, SIN(phi) * sint)),
+                          RF(scale)(axis, cost));
+    *y = RF(add)(c, RF(scale)(nn, r));
+    *ny = nn;
+    *pa = (R)1 / cap;
+    return 1;
+}
+
+/* ---------------------------------------------------------------------- */
+/* Glass (core/render/material.cuh:116-161, 206-257; brdf.cuh:15-33;        */
+/* core/spectrum/spectrum.cuh:16-54)                                        */
+/* ---------------------------------------------------------------------- */
+
+/* IORFromAbbeNM: n(lambda) = n_d + K (1/lambda - 1/lambda_d),
+ * K = (n_d - 1) / (V_d (1/lambda_F - 1/lambda_C)); K = 0 for V_d <= 0 */
+static R RF(ior_at)(R nd, R vd, R lambda)
+{
+    if (!(lambda > 0)) return nd;
+    const R invf = (R)1 / (R)486.1327f, invc = (R)1 / (R)656.2725f, invd = (R)1 / (R)587.5618f;
+    const R k = (vd > 0) ? (nd - (R)1) / (vd * (invf - invc)) : (R)0;
+    return nd + k * ((R)1 / lambda - invd);
+}
+
+/* SigmaAFromTiltNM: sigma_a (1 + tilt (1/lambda - 1/lambda_d) / (1/lambda_F - 1/lambda_C)), >= 0 */
+static R RF(sigma_at)(R sa, R lambda, R tilt)
+{
+    const R invf = (R)1 / (R)486.1327f, invc = (R)1 / (R)656.2725f, invd = (R)1 / (R)587.5618f;
+    const R u = (R)1 / RF(maxr)(lambda, (R)1e-6);
+    const R du = (u - invd) / (invf - invc);
+    return RF(maxr)((R)0, sa * ((R)1 + tilt * du));
+}
+
+/* FresnelDielectricExact (brdf.cuh:25-31), eta = n_in / n_out */
+static R RF(fresnel)(R cosi, R cost, R eta)
+{
+    const R inve = (R)1 / eta;
+    const R rpar = (inve * cosi - cost) / (inve * cosi + cost);
+    const R rperp = (cosi - inve * cost) / (cosi + inve * cost);
+    return (R)0.5 * (rpar * rpar + rperp * rperp);
+}
+
+/* refract (core/vector.cuh:264-272) */
+static V3 RF(refract)(V3 in, V3 normal, R eta)
+{
+    const V3 n = RF(dot)(normal, in) > 0 ? RF(scale)(normal, -1) : normal;
+    const R cosi = -RF(dot)(n, in);
+    const R k = (R)1 - (eta * eta) * RF(maxr)((R)0, (R)1 - cosi * cosi);
+    const R cost = SQRT(RF(maxr)((R)0, k));
+    return RF(add)(RF(scale)(in, eta), RF(scale)(n, eta * cosi - cost));
 }
 
 /* ---------------------------------------------------------------------- */
@@ -577,6 +863,35 @@ static void RF(g_endpoint)(S4* gm, S4* gr, int* gval, S4 le, int non_delta, int 
         if (!RF(s4zero)(gm[i], mode)) { gr[i] = RF(s4add)(gr[i], RF(s4mul)(gm[i], le)); gval[i] = 1; }
 }
 
+/* closest hit of the path's next ray: the whole scene, or only the medium
+ * object while inside one (Trace's solo trace, render.cuh:89-92) */
+static uint32_t RF(next_hit)(const orc_render_scene* sc, int medium, V3 o, V3 d, R tmin, R* t, R* bu, R* bv,
+                             int* inside)
+{
+    *inside = 0;
+    *bu = 0; *bv = 0;
+    if (medium >= 0) {
+        R tt;
+        int ins;
+        if (RF(obj_test)(&sc->objs[medium], o, d, &tt, &ins) && tt >= tmin) {
+            *t = tt;
+            *inside = ins;
+            return sc->ntri + (uint32_t)medium;
+        }
+        return ORC_MISS;
+    }
+    return RF(closest)(sc, o, d, tmin, (R)INFINITY, t, bu, bv, inside);
+}
+
+/* The first camera ray starts inside the first object containing the camera
+ * (Overlap, intersection.cuh:312-327; CalculateRadiance:536-538). */
+static int RF(camera_medium)(const orc_render_scene* sc, V3 o)
+{
+    for (uint32_t k = 0; k < sc->nobj; ++k)
+        if (RF(obj_overlap)(&sc->objs[k], o)) return (int)k;
+    return -1;
+}
+
 static V3 RF(trace_path)(const orc_render_scene* sc, const orc_render_lights* L, const orc_render_opts* op,
                          uint32_t pix, uint32_t smp, V3 o, V3 d, R lambda0,
                          const orc_guide* G, float alpha, RF(gsample)* gout, int* ngout)
@@ -588,6 +903,12 @@ static V3 RF(trace_path)(const orc_render_scene* sc, const orc_render_lights* L,
     int prev_nee = 0;
     R last_pdf = 0;
     V3 last_pos = RF(v3)(0, 0, 0);
+    /* media state (MediaContext, material.cuh:12-18): the object the path is
+     * inside (-1 none) and the vertex where it entered; hero wavelengths
+     * dropped at a dispersive interface */
+    int medium = RF(camera_medium)(sc, o);
+    V3 mpos = o;
+    int dropped = 0;
     /* guiding slots: throughput since the slot's vertex, collected radiance,
      * sampled direction and probe key (CollectRadiance, render.cuh:345-410) */
     S4 gm[5], gr[5];
@@ -608,7 +929,28 @@ static V3 RF(trace_path)(const orc_render_scene* sc, const orc_render_lights* L,
         for (int k = 0; k < 12; ++k) u[k] = (R)uf[k];
         const R eps = RF(intersect_eps)(non_delta);
         R t, bu, bv;
-        const uint32_t prim = RF(closest)(sc, o, d, eps, (R)INFINITY, &t, &bu, &bv);
+        int inside = 0;
+        const uint32_t prim = RF(next_hit)(sc, medium, o, d, eps, &t, &bu, &bv, &inside);
+        if (prim == ORC_MISS && medium >= 0) {
+            /* inside a medium with no exit found (bMissThinSurfaceInsideMedia_,
+             * render.cuh:107-124): leave the medium and continue straight from
+             * the same origin as a delta vertex of weight 1 */
+            medium = -1;
+            R qrr = 1;
+            if (non_delta >= op->rr_min_depth) {
+                qrr = RF(clampr)(RF(lum4)(mask, mode) * (R)0.5 + RF(max4)(mask, mode) * (R)0.5, (R)0.05, (R)0.95);
+                if (u[9] > qrr) break;
+            }
+            if (gout) {
+                const R invq = (R)(1.0 / (double)qrr);
+                const int hi = non_delta + 1 < 5 ? non_delta + 1 : 5;
+                for (int i = 0; i < hi; ++i)
+                    if (!RF(s4zero)(gm[i], mode)) gm[i] = RF(s4scale)(gm[i], invq);
+            }
+            mask = RF(s4scale)(mask, (R)1 / qrr);
+            RF(probe_query)(op, &LM, mask, gq);
+            continue;
+        }
         if (prim == ORC_MISS) {
             S4 e = RF(s4)(0, 0, 0, 0);
             if (op->sky == 1) {
@@ -628,26 +970,45 @@ static V3 RF(trace_path)(const orc_render_scene* sc, const orc_render_lights* L,
             RF(g_endpoint)(gm, gr, gval, e, non_delta, mode);
             break;
         }
-        V3 a, b, c;
-        RF(tri)(sc, prim, &a, &b, &c);
-        const R bw = (R)1 - bu - bv;
-        V3 p = RF(add)(RF(add)(RF(scale)(a, bw), RF(scale)(b, bu)), RF(scale)(c, bv));
-        const V3 ng = RF(normalize)(RF(cross)(RF(sub)(b, a), RF(sub)(c, a)));
-        const R ndd = RF(dot)(ng, d);
-        const V3 n = ndd < 0 ? ng : RF(scale)(ng, -1);
+        const int is_obj = prim >= sc->ntri;
+        const orc_object* ob = is_obj ? &sc->objs[prim - sc->ntri] : NULL;
+        V3 p, ng, n;
+        R ndd;
+        if (is_obj) {
+            /* object hits shade with the intersector's normal: outward, or
+             * inward for a hit from inside (aabb.cuh:60-73, sphere.cuh:52-53) */
+            RF(obj_surface)(ob, o, d, t, inside, &p, &ng);
+            ndd = RF(dot)(ng, d);
+            n = ng;
+        } else {
+            V3 a, b, c;
+            RF(tri)(sc, prim, &a, &b, &c);
+            const R bw = (R)1 - bu - bv;
+            p = RF(add)(RF(add)(RF(scale)(a, bw), RF(scale)(b, bu)), RF(scale)(c, bv));
+            ng = RF(normalize)(RF(cross)(RF(sub)(b, a), RF(sub)(c, a)));
+            ndd = RF(dot)(ng, d);
+            n = ndd < 0 ? ng : RF(scale)(ng, -1);
+        }
         const RF(mat) m = RF(material)(sc, prim);
         const int is_light = sc->light_index[prim] >= 0;
         if (is_light) {
-            if (ndd < 0) {
+            /* triangle lights emit from the front face; box and sphere lights
+             * from every hit (MAT_LIGHT objects, render.cuh:664-710) */
+            if (is_obj || ndd < 0) {
                 R w = 1;
                 if (op->nee && depth > 0 && prev_nee) {
                     const int li = sc->light_index[prim];
-                    const V3 tl = RF(sub)(p, last_pos);
-                    const R dist2 = RF(maxr)(RF(dot)(tl, tl), (R)1e-20);
-                    const V3 dir = RF(scale)(tl, (R)1 / SQRT(dist2));
-                    const R cosy = RF(maxr)((R)0, -RF(dot)(ng, dir));
-                    const R pa = (R)L->prob[li] / (R)L->area[li];
-                    const R pl = cosy <= 0 ? (R)0 : pa * (dist2 / cosy);
+                    R pl;
+                    if (is_obj) {
+                        pl = RF(obj_light_pdf)(ob, (R)L->prob[li], last_pos, p, ng);
+                    } else {
+                        const V3 tl = RF(sub)(p, last_pos);
+                        const R dist2 = RF(maxr)(RF(dot)(tl, tl), (R)1e-20);
+                        const V3 dir = RF(scale)(tl, (R)1 / SQRT(dist2));
+                        const R cosy = RF(maxr)((R)0, -RF(dot)(ng, dir));
+                        const R pa = (R)L->prob[li] / (R)L->area[li];
+                        pl = cosy <= 0 ? (R)0 : pa * (dist2 / cosy);
+                    }
                     if (pl >= 0 && last_pdf > 0) w = RF(power_heuristic)(last_pdf, pl);
                 }
                 const S4 le = RF(emission_of)(&m, &LM, op->d65);
@@ -656,12 +1017,99 @@ static V3 RF(trace_path)(const orc_render_scene* sc, const orc_render_lights* L,
             }
             break;
         }
+        if (m.model == 2) {
+            /* glass: Fresnel reflection or refraction (SampleBSDF MAT_GLASS,
+             * material.cuh:206-257) weighted by EvaluateBSDFWithCos
+             * (material.cuh:116-161); Beer-Lambert over the path inside */
+            const int ins = medium >= 0;
+            R ior = m.ior;
+            if (mode != 0) ior = RF(ior_at)(m.ior, m.abbe, LM.l[0]);
+            if (mode == 4 && m.abbe > 0 && !dropped) {
+                /* dispersion: the hero wavelength continues alone with the
+                 * weight of all four (the secondary wavelengths' estimates
+                 * end here) */
+                mask = RF(s4)(mask.v[0] * (R)4, 0, 0, 0);
+                dropped = 1;
+            }
+            S4 tr = RF(s4)(1, 1, 1, 1);
+            if (ins) {
+                const R dist = RF(length)(RF(sub)(mpos, p));
+                const S4 alb = RF(albedo_of)(&m, &LM);
+                for (int k = 0; k < 4; ++k) {
+                    const R sa = mode == 0 ? m.sigma_a : RF(sigma_at)(m.sigma_a, LM.l[k], m.tilt);
+                    const R a = RF(minr)(RF(maxr)(alb.v[k], (R)1e-12f), (R)1);
+                    tr.v[k] = EXP(dist * sa * LOG(a));
+                }
+            }
+            const V3 nin = RF(dot)(ng, d) > 0 ? RF(scale)(ng, -1) : ng;
+            const R eta = ins ? ior : (R)1 / ior;
+            const R cosi = -RF(dot)(nin, d);
+            const R sin2i = RF(maxr)((R)0, (R)1 - cosi * cosi);
+            const R k = (R)1 - sin2i * (eta * eta);
+            V3 next;
+            R pdf;
+            S4 fcos;
+            int next_medium = medium;
+            if (k <= 0) {
+                next = RF(reflect)(d, nin);
+                pdf = 1;
+                fcos = RF(s4scale)(tr, (R)0.99f);
+            } else {
+                const R cost = SQRT(k);
+                const R fr = RF(fresnel)(cosi, cost, eta);
+                const V3 rdir = RF(reflect)(d, nin);
+                const V3 tdir = RF(refract)(d, nin, eta);
+                if (u[0] < fr) {
+                    next = rdir;
+                    pdf = fr;
+                } else {
+                    next = tdir;
+                    pdf = (R)1 - fr;
+                    next_medium = ins ? -1 : (int)(prim - sc->ntri);
+                }
+                if (RF(dot)(next, rdir) >= RF(dot)(next, tdir)) {
+                    fcos = RF(s4scale)(tr, fr);
+                } else {
+                    const R wgt = (eta * eta) * (FABS(cost) / RF(maxr)((R)1e-6f, cosi));
+                    fcos = RF(s4scale)(tr, ((R)1 - fr) * wgt);
+                }
+            }
+            mpos = p;
+            medium = next_medium;
+            const R off = eps * RF(maxr)((R)1, RF(length)(p)) * (u[4] + (R)0.5);
+            const R sgn = RF(dot)(ng, next) > 0 ? (R)1 : (RF(dot)(ng, next) < 0 ? (R)-1 : (R)0);
+            p = RF(add)(p, RF(scale)(ng, sgn * off));
+            prev_nee = 0;
+            last_pdf = pdf;
+            last_pos = p;
+            const S4 contrib = RF(s4scale)(fcos, (R)1 / RF(maxr)(pdf, (R)1e-12));
+            R qrr = 1;
+            if (non_delta >= op->rr_min_depth) {
+                const R base = RF(clampr)(RF(lum4)(mask, mode) * (R)0.5 + RF(max4)(mask, mode) * (R)0.5,
+                                          (R)0.05, (R)0.95);
+                const R tl = medium >= 0 ? (R)0.99f : (R)0;
+                qrr = base * ((R)1 - tl) + tl;
+                if (u[9] > qrr) break;
+            }
+            if (gout) {
+                const R invq = (R)(1.0 / (double)qrr);
+                const int hi = non_delta + 1 < 5 ? non_delta + 1 : 5;
+                for (int i = 0; i < hi; ++i)
+                    if (!RF(s4zero)(gm[i], mode)) gm[i] = RF(s4mul)(gm[i], RF(s4scale)(contrib, invq));
+            }
+            mask = RF(s4scale)(RF(s4mul)(mask, contrib), (R)1 / qrr);
+            RF(probe_query)(op, &LM, mask, gq);
+            o = p;
+            d = next;
+            continue;
+        }
         const S4 alb = RF(albedo_of)(&m, &LM);
         const R pspec = m.model == 0 ? (R)0 : RF(pspec_of)(&m, alb, mode);
         V3 next;
         R pdf;
         if (!RF(sample_bsdf)(&m, pspec, d, n, u, &next, &pdf)) break;
         S4 fcos = RF(eval_bsdf)(&m, alb, d, next, n);
+        mpos = p;
         /* offset along the side of the next direction (render.cuh:155-160) */
         const R off = eps * RF(maxr)((R)1, RF(length)(p)) * (u[4] + (R)0.5);
         const R sgn = RF(dot)(n, next) > 0 ? (R)1 : (RF(dot)(n, next) < 0 ? (R)-1 : (R)0);
@@ -702,48 +1150,62 @@ static V3 RF(trace_path)(const orc_render_scene* sc, const orc_render_lights* L,
             }
         }
         int did_nee = 0;
-        if (op->nee && L->count > 0 && depth < op->max_depth - 1 && non_delta < 10 &&
+        if (op->nee && L->count > 0 && depth < op->max_depth - 1 && non_delta < 10 && medium < 0 &&
             u[5] < SQRT(RF(clampr)(RF(lum4)(mask, mode), 0, 1))) {
             const int li = RF(pick_light)(L, u[6]);
             const uint32_t lp = (uint32_t)L->tri[li];
-            V3 la, lb, lc;
-            RF(tri)(sc, lp, &la, &lb, &lc);
-            const R su = SQRT(u[7]);
-            const V3 q = RF(add)(RF(add)(RF(scale)(la, (R)1 - su), RF(scale)(lb, su * ((R)1 - u[8]))),
-                                 RF(scale)(lc, su * u[8]));
-            const V3 ln = RF(normalize)(RF(cross)(RF(sub)(lb, la), RF(sub)(lc, la)));
-            const RF(mat) lm = RF(material)(sc, lp);
-            const R pa_total = RF(maxr)((R)1e-20, (R)L->prob[li] / (R)L->area[li]);
-            const V3 lu = RF(sub)(q, p);
-            const R dist2 = RF(maxr)(RF(dot)(lu, lu), (R)1e-12);
-            const V3 ldir = RF(scale)(lu, (R)1 / SQRT(dist2));
-            /* AnyHitSegmentBVH (intersection.cuh:285-310) */
-            const R dist = SQRT(RF(dot)(lu, lu));
-            int occl = 0;
-            if (dist > 0) {
-                const V3 sdir = RF(scale)(lu, (R)1 / dist);
-                const R seps = RF(intersect_eps)(depth) * RF(maxr)((R)1, RF(maxr)(RF(length)(p), RF(length)(q)));
-                const R tmx = RF(maxr)((R)0, dist - seps);
-                if (tmx > 0) occl = RF(occluded)(sc, p, sdir, (R)0, tmx);
-            }
-            if (occl) {
-                did_nee = 1;
+            V3 q, ln;
+            R pa_total;
+            int valid = 1;
+            if (lp < sc->ntri) {
+                V3 la, lb, lc;
+                RF(tri)(sc, lp, &la, &lb, &lc);
+                const R su = SQRT(u[7]);
+                q = RF(add)(RF(add)(RF(scale)(la, (R)1 - su), RF(scale)(lb, su * ((R)1 - u[8]))),
+                            RF(scale)(lc, su * u[8]));
+                ln = RF(normalize)(RF(cross)(RF(sub)(lb, la), RF(sub)(lc, la)));
+                pa_total = RF(maxr)((R)1e-20, (R)L->prob[li] / (R)L->area[li]);
             } else {
-                const R cosx = RF(maxr)((R)0, RF(dot)(n, ldir));
-                const R cosy = RF(maxr)((R)0, -RF(dot)(ln, ldir));
-                if (cosx > 0 && cosy > 0) {
-                    const S4 f = RF(eval_bsdf)(&m, alb, d, ldir, n);
-                    const R g = cosy / dist2;
-                    const S4 contrib = RF(s4scale)(RF(s4mul)(f, RF(emission_of)(&lm, &LM, op->d65)), g / pa_total);
-                    const R pl = pa_total * (dist2 / cosy);
-                    R pb = RF(maxr)(RF(bsdf_pdf)(&m, pspec, d, ldir, n), (R)0);
-                    if (ptex) {
-                        const float lf[3] = { (float)ldir.x, (float)ldir.y, (float)ldir.z };
-                        pb = ((R)1 - (R)alpha) * pb + (R)alpha * (R)orc_probe_hemi_pdf(ptex, lf, nf, gq);
-                    }
-                    const R w = RF(power_heuristic)(pl, pb);
-                    rad = RF(s4add)(rad, RF(s4mul)(mask, RF(s4scale)(contrib, w)));
+                /* box facet pick from a fifth group of uniforms */
+                float all[20];
+                orc_philox_uniforms(op->seed, pix, smp, (uint32_t)depth, 5, all);
+                R pa;
+                valid = RF(obj_light_sample)(&sc->objs[lp - sc->ntri], p, (R)all[16], u[7], u[8], &q, &ln, &pa);
+                pa_total = valid ? RF(maxr)((R)1e-20, pa * (R)L->prob[li]) : (R)1;
+            }
+            if (valid) {
+                const RF(mat) lm = RF(material)(sc, lp);
+                const V3 lu = RF(sub)(q, p);
+                const R dist2 = RF(maxr)(RF(dot)(lu, lu), (R)1e-12);
+                const V3 ldir = RF(scale)(lu, (R)1 / SQRT(dist2));
+                /* AnyHitSegmentBVH (intersection.cuh:285-310) */
+                const R dist = SQRT(RF(dot)(lu, lu));
+                int occl = 0;
+                if (dist > 0) {
+                    const V3 sdir = RF(scale)(lu, (R)1 / dist);
+                    const R seps = RF(intersect_eps)(depth) * RF(maxr)((R)1, RF(maxr)(RF(length)(p), RF(length)(q)));
+                    const R tmx = RF(maxr)((R)0, dist - seps);
+                    if (tmx > 0) occl = RF(occluded)(sc, p, sdir, (R)0, tmx);
+                }
+                if (occl) {
                     did_nee = 1;
+                } else {
+                    const R cosx = RF(maxr)((R)0, RF(dot)(n, ldir));
+                    const R cosy = RF(maxr)((R)0, -RF(dot)(ln, ldir));
+                    if (cosx > 0 && cosy > 0) {
+                        const S4 f = RF(eval_bsdf)(&m, alb, d, ldir, n);
+                        const R g = cosy / dist2;
+                        const S4 contrib = RF(s4scale)(RF(s4mul)(f, RF(emission_of)(&lm, &LM, op->d65)), g / pa_total);
+                        const R pl = pa_total * (dist2 / cosy);
+                        R pb = RF(maxr)(RF(bsdf_pdf)(&m, pspec, d, ldir, n), (R)0);
+                        if (ptex) {
+                            const float lf[3] = { (float)ldir.x, (float)ldir.y, (float)ldir.z };
+                            pb = ((R)1 - (R)alpha) * pb + (R)alpha * (R)orc_probe_hemi_pdf(ptex, lf, nf, gq);
+                        }
+                        const R w = RF(power_heuristic)(pl, pb);
+                        rad = RF(s4add)(rad, RF(s4mul)(mask, RF(s4scale)(contrib, w)));
+                        did_nee = 1;
+                    }
                 }
             }
         }
@@ -755,10 +1217,11 @@ static V3 RF(trace_path)(const orc_render_scene* sc, const orc_render_lights* L,
         R qrr = 1;
         if (non_delta >= op->rr_min_depth) {
             qrr = RF(clampr)(RF(lum4)(mask, mode) * (R)0.5 + RF(max4)(mask, mode) * (R)0.5, (R)0.05, (R)0.95);
+            if (medium >= 0) qrr = qrr * ((R)1 - (R)0.99f) + (R)0.99f;
             if (u[9] > qrr) break;
         }
         if (gout) {
-            if (non_delta < 5 && RF(s4zero)(gm[non_delta], mode)) {
+            if (non_delta < 5 && medium < 0 && RF(s4zero)(gm[non_delta], mode)) {
                 const float pf[3] = { (float)p.x, (float)p.y, (float)p.z };
                 const float rf[3] = { (float)rvec.x, (float)rvec.y, (float)rvec.z };
                 gdir[non_delta] = next;

@@ -288,7 +288,8 @@ def object_surface(obj, o, d, t, inside):
 
 class orc_material(C.Structure):
     _fields_ = [("albedo", C.c_float * 3), ("emission", C.c_float * 3), ("specular", C.c_float),
-                ("metallic", C.c_float), ("roughness", C.c_float), ("model", C.c_uint32)]
+                ("metallic", C.c_float), ("roughness", C.c_float), ("model", C.c_uint32),
+                ("ior", C.c_float), ("sigma_a", C.c_float), ("abbe", C.c_float), ("tilt", C.c_float)]
 
 
 class orc_render_opts(C.Structure):
@@ -296,7 +297,8 @@ class orc_render_opts(C.Structure):
                 ("seed", C.c_uint64), ("max_depth", C.c_int), ("rr_min_depth", C.c_int), ("nee", C.c_int),
                 ("sky", C.c_int), ("env", C.c_float * 3), ("cam_pos", C.c_float * 3), ("cam_dir", C.c_float * 3),
                 ("vfov_deg", C.c_float), ("spectral", C.c_int), ("output", C.c_int), ("cie", C.c_void_p),
-                ("d65", C.c_void_p), ("lut", C.c_void_p), ("guiding", C.c_int), ("guide", C.c_void_p)]
+                ("d65", C.c_void_p), ("lut", C.c_void_p), ("guiding", C.c_int), ("guide", C.c_void_p),
+                ("objects", C.c_void_p), ("nobj", C.c_uint32), ("obj_mats", C.c_void_p)]
 
 
 CIE_TABLE = ROOT / "aperturerendererv3_amd" / "data" / "cie1931_d65_1nm.txt"
@@ -377,13 +379,19 @@ def make_materials(mats):
         arr[i].metallic = m.get("metallic", 0.0)
         arr[i].roughness = m.get("roughness", 1.0)
         arr[i].model = m.get("model", 0)
+        arr[i].ior = m.get("ior", 1.5)
+        arr[i].sigma_a = m.get("sigma_a", 100.0)
+        arr[i].abbe = m.get("abbe", 20.0)
+        arr[i].tilt = m.get("tilt", 0.0)
     return arr
 
 
 def render(V, F, tri_mat, mats, width, height, spp, seed=1, max_depth=5, nee=1, sky=0, env=(0, 0, 0),
            cam_pos=(0, 0, 1.1), cam_dir=(0, 0, -1), vfov=60.0, precision=32, rr_min_depth=3, sample_offset=0,
-           threads=0, spectral=0, output=0, guide=None):
-    """Oracle render; returns (height, width, 3) float32 or float64 mean image."""
+           threads=0, spectral=0, output=0, guide=None, objects=None):
+    """Oracle render; returns (height, width, 3) float32 or float64 mean image.
+    objects: analytic objects as dicts (type, center, extent, rotation,
+    material as a dict of ar_material fields)."""
     lib = load()
     V = np.ascontiguousarray(V, np.float32)
     F = np.ascontiguousarray(F, np.uint32)
@@ -403,6 +411,10 @@ def render(V, F, tri_mat, mats, width, height, spp, seed=1, max_depth=5, nee=1, 
         o.lut = uplift_lut().ctypes.data
     if guide is not None:
         o.guiding, o.guide = 1, guide.handle
+    if objects:
+        oarr = make_objects(objects)
+        omats = make_materials([ob.get("material") or {} for ob in objects])
+        o.objects, o.nobj, o.obj_mats = C.cast(oarr, C.c_void_p), len(objects), C.cast(omats, C.c_void_p)
     out = np.zeros((height, width, 3), np.float64 if precision == 64 else np.float32)
     lib.orc_render.restype = C.c_int
     lib.orc_render.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p,
