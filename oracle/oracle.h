@@ -23,6 +23,11 @@
  *                        leaf size, SAH accept rule)
  *   orc_trace_closest    equality with orc_trace_brute on every ray
  *   orc_trace_any        equality with (closest hit t < tmax)
+ *   orc_object_*         FP64 quadratic / slab solutions, sampled surfaces,
+ *                        proxy extreme points (tests/test_oracle_objects.py)
+ *   object lights, glass white / diffuse furnaces, n^2 radiance inside glass,
+ *                        Beer-Lambert series, sampler pairing and solid-angle
+ *                        integrals (tests/test_oracle_render_objects.py)
  *
  * Build: oracle/Makefile (gcc -O2 -ffp-contract=off -fopenmp, no fast-math).
  */
