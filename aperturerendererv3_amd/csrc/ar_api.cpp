@@ -122,10 +122,13 @@ int trace_prologue(ar_ctx* ctx, TraceArgs& ta, size_t n, uint32_t& blocks) {
     if (!ctx->bvh_ready) return set_error(ctx, AR_E_NOSCENE, "no BVH: call ar_scene_set and ar_bvh_build first");
     int rc = ensure_spill(ctx, ctx->bvh.stack_entries);
     if (rc != AR_OK) return rc;
-    const uint32_t full_blocks = std::min(ctx->trace_threads / kTraceBlock,
-                                          (uint32_t)ctx->num_cus * (uint32_t)trace_blocks_per_cu((int)ctx->bvh.width, false));
+    // resident blocks of the closest-hit and any-hit kernels (their register
+    // budgets differ); the spill buffer is laid out for trace_threads threads
+    const uint32_t cap_blocks = ctx->trace_threads / kTraceBlock;
+    ta.full_closest = std::min(cap_blocks, (uint32_t)ctx->num_cus * (uint32_t)trace_blocks_per_cu((int)ctx->bvh.width, false));
+    ta.full_any = std::min(cap_blocks, (uint32_t)ctx->num_cus * (uint32_t)trace_blocks_per_cu((int)ctx->bvh.width, true));
     const size_t need = (n + kTraceBlock - 1) / kTraceBlock;
-    blocks = (uint32_t)std::min<size_t>(full_blocks, std::max<size_t>(need, 1));
+    blocks = (uint32_t)std::min<size_t>(std::max(ta.full_closest, ta.full_any), std::max<size_t>(need, 1));
     ta.width = ctx->bvh.width;
     ta.stack_entries = ctx->bvh.stack_entries;
     ta.nodes = ctx->bvh.nodes;
@@ -133,7 +136,7 @@ int trace_prologue(ar_ctx* ctx, TraceArgs& ta, size_t n, uint32_t& blocks) {
     ta.root = ctx->bvh.root;
     ta.origin_limit = ctx->bvh.origin_limit;
     ta.spill = ctx->spill;
-    ta.threads = blocks * kTraceBlock;
+    ta.threads = ctx->trace_threads;
     ta.fetch = ctx->fetch;
     ta.objs = ctx->scene.nobj ? ctx->scene.objs : nullptr;
     ta.obj_base = (uint32_t)(ctx->idx.size() / 3);

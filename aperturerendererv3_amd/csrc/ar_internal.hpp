@@ -336,7 +336,8 @@ struct TraceArgs {
     uint32_t root;
     float origin_limit;
     int* spill;
-    uint32_t threads;
+    uint32_t threads;    // spill-buffer stride (threads it is laid out for)
+    uint32_t full_closest = 1, full_any = 1;   // resident blocks of each kernel
     uint32_t* fetch;     // device ray-refill counter
     const uint32_t* roots = nullptr;   // per-ray root node (batched scenes), else `root`
     const float4* objs = nullptr;      // analytic object records; NULL: triangle-only scene

@@ -271,10 +271,10 @@ __device__ __forceinline__ uint32_t interior8(const GpuNode8* __restrict__ nodes
 // scene has analytic objects, whose slots are tested by the object
 // intersector (prim id = obj_base + object index).
 template <bool ANY, bool OBJ>
-__device__ __forceinline__ uint32_t leaf(const float4* __restrict__ tris, const float4* __restrict__ objs,
-                                         uint32_t obj_base, Trav& s, Stack& st) {
-    const uint32_t first = s.ref & kMaxTriSlots;
-    const uint32_t cnt = ((s.ref >> 27) & 15u) + 1u;
+__device__ __forceinline__ uint32_t leaf_prims(const float4* __restrict__ tris, const float4* __restrict__ objs,
+                                               uint32_t obj_base, Trav& s, uint32_t lref) {
+    const uint32_t first = lref & kMaxTriSlots;
+    const uint32_t cnt = ((lref >> 27) & 15u) + 1u;
     for (uint32_t k = 0; k < cnt; ++k) {
         const float4* tp = tris + 3 * (size_t)(first + k);
         const float4 va = tp[0];
@@ -301,18 +301,39 @@ __device__ __forceinline__ uint32_t leaf(const float4* __restrict__ tris, const 
             if (ANY) { s.ref = kRefEmpty; return k + 1; }
         }
     }
-    s.ref = st.pop();
     return cnt;
 }
 
-constexpr int kStepBudget = 12;   // traversal steps between two ray-refill rounds
+template <bool ANY, bool OBJ>
+__device__ __forceinline__ uint32_t leaf(const float4* __restrict__ tris, const float4* __restrict__ objs,
+                                         uint32_t obj_base, Trav& s, Stack& st) {
+    const uint32_t c = leaf_prims<ANY, OBJ>(tris, objs, obj_base, s, s.ref);
+    if (!ANY || s.ref != kRefEmpty) s.ref = st.pop();
+    return c;
+}
+
+#ifndef AR_TRACE_STEP_BUDGET
+#define AR_TRACE_STEP_BUDGET 4
+#endif
+constexpr int kStepBudget = AR_TRACE_STEP_BUDGET;   // traversal steps between two ray-refill rounds
+#ifndef AR_TRACE_CHUNK
+#define AR_TRACE_CHUNK 128
+#endif
 
 // Persistent traversal with per-lane ray refill (Aila & Laine, HPG'09): a
 // lane whose ray is finished takes the next ray index from a global counter
 // (one atomic per wave and round), so a wave never idles on its slowest ray.
 // ANY: sets bit idx of occ (zeroed beforehand) for occluded rays.
+#ifndef AR_TRACE_WAVES_CLOSEST
+#define AR_TRACE_WAVES_CLOSEST 6
+#endif
+#ifndef AR_TRACE_WAVES_ANY
+#define AR_TRACE_WAVES_ANY 7
+#endif
+
 template <int W, bool ANY, bool COUNT, bool OBJ>
-__global__ __launch_bounds__(kTraceBlock) void k_trace(
+__global__ __launch_bounds__(kTraceBlock)
+__attribute__((amdgpu_waves_per_eu(ANY ? AR_TRACE_WAVES_ANY : AR_TRACE_WAVES_CLOSEST, 8))) void k_trace(
     const void* __restrict__ nodes, const float4* __restrict__ tris, uint32_t root,
     const uint32_t* __restrict__ roots, float origin_limit, const float4* __restrict__ objs, uint32_t obj_base,
     ar_rays r, size_t n, const uint32_t* __restrict__ n_dev, ar_hits h, uint32_t* __restrict__ occ,
@@ -331,16 +352,37 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(
     s.ref = kRefEmpty;
     bool active = false, exhausted = false;
     uint32_t nb = 0, nt = 0;
+#if AR_TRACE_CHUNK
+    uint32_t pool = 0, pool_end = 0;   // this wave's unclaimed ray indices (wave-uniform)
+#endif
     while (true) {
         const bool need = !active && !exhausted;
         const uint64_t nm = __ballot(need);
         if (nm) {
+#if AR_TRACE_CHUNK
+            // rays come from a wave-private chunk; one atomic per chunk
+            const uint32_t cnt = (uint32_t)__popcll(nm);
+            const uint32_t left = pool_end - pool;
+            uint32_t base2 = 0;
+            if (left < cnt) {
+                const uint32_t leader = (uint32_t)__ffsll((unsigned long long)nm) - 1u;
+                if (lane == leader) base2 = atomicAdd(fetch, (uint32_t)AR_TRACE_CHUNK);
+                base2 = __shfl(base2, (int)leader);
+            }
+            const uint32_t rk = (uint32_t)__popcll(nm & ((1ull << lane) - 1ull));
+            const uint32_t pool0 = pool;
+            if (left < cnt) { pool = base2 + (cnt - left); pool_end = base2 + (uint32_t)AR_TRACE_CHUNK; }
+            else pool += cnt;
+            if (need) {
+                const uint32_t idx = rk < left ? pool0 + rk : base2 + (rk - left);
+#else
             const uint32_t leader = (uint32_t)__ffsll((unsigned long long)nm) - 1u;
             uint32_t base = 0;
             if (lane == leader) base = atomicAdd(fetch, (uint32_t)__popcll(nm));
             base = __shfl(base, (int)leader);
             if (need) {
                 const uint32_t idx = base + (uint32_t)__popcll(nm & ((1ull << lane) - 1ull));
+#endif
                 if (idx < n) {
                     s.idx = idx;
                     s.p = ray_setup(r.ox[idx], r.oy[idx], r.oz[idx], r.dx[idx], r.dy[idx], r.dz[idx], origin_limit);
@@ -489,6 +531,7 @@ hipError_t launch_trace_closest(const TraceArgs& ta, const ar_rays& r, size_t n,
     hipError_t e = prepare(ta, nullptr, n, n_dev, s);
     if (e != hipSuccess) return e;
     const void* nd = ta.nodes;
+    blocks = std::max(1u, std::min(blocks, ta.full_closest));
 #define AR_LAUNCH_CLOSEST(W, C, O)                                                                               \
     hipLaunchKernelGGL((k_trace<W, false, C, O>), dim3(blocks), dim3(kTraceBlock), 0, s, nd, ta.tris, ta.root,  \
                        ta.roots, ta.origin_limit, ta.objs, ta.obj_base, r, n, n_dev, h, nullptr, ta.spill,      \
@@ -511,6 +554,7 @@ hipError_t launch_trace_any(const TraceArgs& ta, const ar_rays& r, size_t n, uin
     if (e != hipSuccess) return e;
     const void* nd = ta.nodes;
     const ar_hits none{nullptr, nullptr, nullptr, nullptr, nullptr};
+    blocks = std::max(1u, std::min(blocks, ta.full_any));
 #define AR_LAUNCH_ANY(W, O)                                                                                      \
     hipLaunchKernelGGL((k_trace<W, true, false, O>), dim3(blocks), dim3(kTraceBlock), 0, s, nd, ta.tris, ta.root, \
                        ta.roots, ta.origin_limit, ta.objs, ta.obj_base, r, n, n_dev, none, occ, ta.spill,       \
