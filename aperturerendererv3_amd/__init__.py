@@ -17,11 +17,11 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import (AR_HIT_INSIDE, AR_MISS, AR_MODEL_GGX, AR_MODEL_GLASS, AR_MODEL_LAMBERT, AR_OBJ_BOX, AR_OBJ_SPHERE, AR_OK, AR_SKY_CONSTANT, AR_SKY_GRADIENT, AR_SKY_NONE,
+from ._lib import (AR_TONE_ACES, AR_TONE_GAMMA, AR_TONE_LINEAR, AR_HIT_INSIDE, AR_MISS, AR_MODEL_GGX, AR_MODEL_GLASS, AR_MODEL_LAMBERT, AR_OBJ_BOX, AR_OBJ_SPHERE, AR_OK, AR_SKY_CONSTANT, AR_SKY_GRADIENT, AR_SKY_NONE,
                    ar_bvh_info, ar_bvh_opts, ar_camera, ar_device_opts, ar_hits, ar_material, ar_mesh, ar_rays,
                    ar_render_opts)
 
-__all__ = ["Context", "ArError", "AR_MISS", "AR_MODEL_LAMBERT", "AR_MODEL_GGX", "AR_MODEL_GLASS", "AR_OBJ_BOX",
+__all__ = ["Context", "ArError", "write_bmp", "AR_TONE_LINEAR", "AR_TONE_GAMMA", "AR_TONE_ACES", "AR_MISS", "AR_MODEL_LAMBERT", "AR_MODEL_GGX", "AR_MODEL_GLASS", "AR_OBJ_BOX",
            "AR_OBJ_SPHERE", "AR_HIT_INSIDE", "AR_SKY_NONE", "AR_SKY_GRADIENT",
            "AR_SKY_CONSTANT", "Material", "RAY_FIELDS"]
 
@@ -400,6 +400,39 @@ class Context:
         return out
 
     # -- integrator ----------------------------------------------------------
+    def render_progressive(self, frame, accum, odd=None, even=None, spp=1, cam_pos=(0.0, 0.0, 1.1),
+                           cam_dir=(0.0, 0.0, -1.0), vfov=60.0, seed=1, max_depth=5, nee=True, sky=AR_SKY_NONE,
+                           env=(0.0, 0.0, 0.0), rr_min_depth=3, spectral=0, output=0, stream=None):
+        """ar_render_progressive: fold spp samples (indices frame ..) into the
+        running means accum (and odd / even), (h, w, 3) float32 device tensors."""
+        h, w = accum.shape[0], accum.shape[1]
+        cam, o = _camera_opts(w, h, spp, cam_pos, cam_dir, vfov, seed, max_depth, nee, sky, env, rr_min_depth, 0, 0,
+                              spectral, output, 0, 0)
+        self._check(self._lib.ar_render_progressive(self._h, C.byref(cam), C.byref(o), int(frame), _ptr(accum),
+                                                    _ptr(odd) if odd is not None else None,
+                                                    _ptr(even) if even is not None else None, _stream_ptr(stream)),
+                    "ar_render_progressive")
+        return accum
+
+    def tonemap(self, hdr, exposure=1.0, tone=0, xyz_input=False, out=None, stream=None):
+        """ar_tonemap: (h, w, 3) float32 device image -> (h, w) int32 tensor of
+        0xAARRGGBB pixels."""
+        import torch
+        h, w = hdr.shape[0], hdr.shape[1]
+        if out is None:
+            out = torch.empty((h, w), dtype=torch.int32, device=hdr.device)
+        self._check(self._lib.ar_tonemap(self._h, _ptr(hdr), _ptr(out), w, h, float(exposure), int(tone),
+                                         1 if xyz_input else 0, _stream_ptr(stream)), "ar_tonemap")
+        return out
+
+    def luminance_mse(self, a, b, stream=None) -> float:
+        """ar_luminance_mse of two (h, w, 3) float32 device images."""
+        res = C.c_double()
+        npix = a.shape[0] * a.shape[1]
+        self._check(self._lib.ar_luminance_mse(self._h, _ptr(a), _ptr(b), npix, C.byref(res), _stream_ptr(stream)),
+                    "ar_luminance_mse")
+        return res.value
+
     def render(self, width, height, spp, cam_pos=(0.0, 0.0, 1.1), cam_dir=(0.0, 0.0, -1.0), vfov=60.0, seed=1,
                max_depth=5, nee=True, sky=AR_SKY_NONE, env=(0.0, 0.0, 0.0), rr_min_depth=3, sample_offset=0,
                max_paths_per_pass=0, spectral=0, output=0, guiding=0, frame_spp=0, out=None, stream=None):
@@ -412,6 +445,17 @@ class Context:
         self._check(self._lib.ar_render(self._h, C.byref(cam), C.byref(o), _ptr(out), _stream_ptr(stream)),
                     "ar_render")
         return out
+
+
+def write_bmp(path, ldr):
+    """ar_write_bmp: (h, w) 0xAARRGGBB pixels (numpy uint32 or int32 torch
+    tensor) to a 32-bit top-down BMP."""
+    import numpy as np
+    a = ldr.cpu().numpy() if hasattr(ldr, "cpu") else np.asarray(ldr)
+    a = np.ascontiguousarray(a).view(np.uint32)
+    rc = _lib.load().ar_write_bmp(str(path).encode(), a.ctypes.data, a.shape[1], a.shape[0])
+    if rc != AR_OK:
+        raise ArError(rc, f"ar_write_bmp({path})")
 
 
 def make_batch(scenes) -> Context:

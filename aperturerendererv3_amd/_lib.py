@@ -24,6 +24,9 @@ AR_MODEL_LAMBERT = 0
 AR_MODEL_GGX = 1
 AR_MODEL_GLASS = 2
 AR_OBJ_BOX = 0
+AR_TONE_LINEAR = 0
+AR_TONE_GAMMA = 1
+AR_TONE_ACES = 2
 AR_OBJ_SPHERE = 1
 AR_HIT_INSIDE = 16
 
@@ -129,6 +132,12 @@ SIGNATURES = {
                                       C.POINTER(C.c_float * 3), C.c_float, C.c_float, C.POINTER(ar_rays),
                                       C.c_void_p]),
     "ar_render": (C.c_int, [C.c_void_p, C.POINTER(ar_camera), C.POINTER(ar_render_opts), C.c_void_p, C.c_void_p]),
+    "ar_render_progressive": (C.c_int, [C.c_void_p, C.POINTER(ar_camera), C.POINTER(ar_render_opts), C.c_uint32,
+                                        C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "ar_tonemap": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_float, C.c_uint32,
+                             C.c_uint32, C.c_void_p]),
+    "ar_luminance_mse": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.POINTER(C.c_double), C.c_void_p]),
+    "ar_write_bmp": (C.c_int, [C.c_char_p, C.c_void_p, C.c_uint32, C.c_uint32]),
     "ar_spectrum_lut_build": (C.c_int, [C.c_void_p, C.c_size_t]),
     "ar_guiding_configure": (C.c_int, [C.c_void_p, C.c_uint64]),
     "ar_batch_create": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_void_p)]),

@@ -198,6 +198,7 @@ int ar_destroy(ar_ctx* ctx) {
     ctx->job = nullptr;
     if (ctx->spill) (void)hipFree(ctx->spill);
     if (ctx->fetch) (void)hipFree(ctx->fetch);
+    if (ctx->reduce) (void)hipFree(ctx->reduce);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return AR_OK;
@@ -465,6 +466,18 @@ int ar_render(ar_ctx* ctx, const ar_camera* cam, const ar_render_opts* opts, flo
     if (opts->sky > AR_SKY_CONSTANT) return set_error(ctx, AR_E_INVALID, "unknown sky mode");
     if (!ctx->bvh_ready) return set_error(ctx, AR_E_NOSCENE, "no BVH: call ar_scene_set and ar_bvh_build first");
     return render(ctx, cam, opts, out_rgb, pick_stream(ctx, stream));
+}
+
+int ar_render_progressive(ar_ctx* ctx, const ar_camera* cam, const ar_render_opts* opts, uint32_t frame,
+                          float* accum, float* odd, float* even, void* stream) {
+    if (!ctx) return AR_E_INVALID;
+    if (!cam || !opts || !accum) return set_error(ctx, AR_E_INVALID, "NULL camera, options or accumulation buffer");
+    if ((odd == nullptr) != (even == nullptr)) return set_error(ctx, AR_E_INVALID, "odd and even go together");
+    if (opts->width == 0 || opts->height == 0 || opts->spp == 0) return set_error(ctx, AR_E_INVALID, "empty image or spp = 0");
+    if (opts->guiding != AR_GUIDING_OFF)
+        return set_error(ctx, AR_E_UNSUPPORTED, "progressive frames are unguided; use ar_render for guided frames");
+    if (!ctx->bvh_ready) return set_error(ctx, AR_E_NOSCENE, "no BVH: call ar_scene_set and ar_bvh_build first");
+    return render_progressive(ctx, cam, opts, frame, accum, odd, even, pick_stream(ctx, stream));
 }
 
 } // extern "C"

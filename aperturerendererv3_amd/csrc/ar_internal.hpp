@@ -321,6 +321,7 @@ struct ar_ctx {
     ar::RenderJob* job = nullptr;         // plan of the last render
     uint32_t tile = 256;                  // threads per block of the wavefront kernels
     uint64_t buffer_generation = 0;       // bumped whenever render buffers are reallocated
+    double* reduce = nullptr;             // partial sums of ar_luminance_mse
 };
 
 namespace ar {
@@ -359,6 +360,8 @@ int upload_scene(ar_ctx* ctx);
 void free_scene(ar_ctx* ctx);
 void free_render_buffers(ar_ctx* ctx);
 int render(ar_ctx* ctx, const ar_camera* cam, const ar_render_opts* opts, float* out, hipStream_t s);
+int render_progressive(ar_ctx* ctx, const ar_camera* cam, const ar_render_opts* opts, uint32_t frame, float* accum,
+                       float* odd, float* even, hipStream_t s);
 int render_batch(ar_ctx* ctx, uint32_t n, const ar_camera* cams, const ar_render_opts* opts, float* const* outs,
                  hipStream_t s);
 int graph_capture(ar_ctx* ctx, uint32_t n, const ar_camera* cams, const ar_render_opts* opts, float* const* outs,

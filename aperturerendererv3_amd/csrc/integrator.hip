@@ -34,6 +34,7 @@ constexpr uint32_t kCameraBounce = 0xFFFFFFFFu;
 constexpr int kBlock = 256;        // default threads per block of the wavefront kernels
 constexpr int kMaxBlock = 512;     // largest block the wavefront kernels accept
 constexpr uint32_t kMetaNee = 1u << 16;
+constexpr uint32_t kAccumFloats = 9;   // per pixel: sums of all, odd-index and even-index samples
 constexpr uint32_t kMetaHeroOnly = 1u << 17;   // secondary hero wavelengths dropped at a dispersive interface
 
 // ---------------------------------------------------------------- helpers
@@ -527,6 +528,9 @@ struct SceneView {
     uint32_t light_off, light_cnt;
     uint32_t medium0;            // object containing the camera + 1 (0: none)
     uint32_t root0;              // root of the camera rays (the medium's own leaf inside one)
+    uint32_t prog;               // progressive: fold the mean into out, the running mean of sample_offset samples
+    float* odd;                  // progressive running means of the odd / even sample indices (or null)
+    float* even;
     float cam_pos[3];
     float fwd[3], right[3], up[3];
     float thx, thy;
@@ -1357,28 +1361,55 @@ __global__ __launch_bounds__(kMaxBlock) void k_accumulate(RenderParams P, SceneA
     const uint32_t pix = t - sg.pixbase;
     const bool first = sg.sample0 == V.sample_offset;
     const bool last = sg.sample0 + sg.count == V.sample_offset + V.spp;
-    float* acc = accum + 3 * ((size_t)V.out_base + pix);
+    // per pixel: sum of all samples, of the odd-index and of the even-index ones
+    float* acc = accum + kAccumFloats * ((size_t)V.out_base + pix);
     float ax = first ? 0.0f : acc[0];
     float ay = first ? 0.0f : acc[1];
     float az = first ? 0.0f : acc[2];
+    float h[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) h[k] = first ? 0.0f : acc[3 + k];
     for (uint32_t s = 0; s < sg.count; ++s) {
         const F3 r = path_value(P, sc, S, sg.base + s * V.npix + pix);
         ax = ax + r.x; ay = ay + r.y; az = az + r.z;
+        if (V.odd) {
+            float* hh = ((sg.sample0 + s) & 1u) ? h : h + 3;
+            hh[0] = hh[0] + r.x; hh[1] = hh[1] + r.y; hh[2] = hh[2] + r.z;
+        }
     }
     if (last) {
-        const float spp_total = (float)V.spp;
-        float x = ax / spp_total, y = ay / spp_total, z = az / spp_total;
-        if (P.spectral != AR_SPECTRAL_RGB && P.output == AR_OUTPUT_SRGB) {
-            const float r = 3.2406f * x + -1.5372f * y + -0.4986f * z;
-            const float g = -0.9689f * x + 1.8758f * y + 0.0415f * z;
-            const float b = 0.0557f * x + -0.204f * y + 1.057f * z;
-            x = maxr(0.0f, r); y = maxr(0.0f, g); z = maxr(0.0f, b);
+        auto finish = [&](float sx, float sy, float sz, float n, float* dst, uint32_t before) {
+            float x = sx / n, y = sy / n, z = sz / n;
+            if (P.spectral != AR_SPECTRAL_RGB && P.output == AR_OUTPUT_SRGB) {
+                const float r = 3.2406f * x + -1.5372f * y + -0.4986f * z;
+                const float g = -0.9689f * x + 1.8758f * y + 0.0415f * z;
+                const float b = 0.0557f * x + -0.204f * y + 1.057f * z;
+                x = maxr(0.0f, r); y = maxr(0.0f, g); z = maxr(0.0f, b);
+            }
+            if (V.prog && before > 0) {
+                // running mean: lerp(target, result, n / (before + n))
+                const float total = (float)before + n;
+                const float wa = (float)before / total, wb = n / total;
+                x = dst[3 * pix + 0] * wa + x * wb;
+                y = dst[3 * pix + 1] * wa + y * wb;
+                z = dst[3 * pix + 2] * wa + z * wb;
+            }
+            dst[3 * pix + 0] = x;
+            dst[3 * pix + 1] = y;
+            dst[3 * pix + 2] = z;
+        };
+        finish(ax, ay, az, (float)V.spp, V.out, V.prog ? V.sample_offset : 0u);
+        if (V.odd) {
+            const uint32_t f = V.sample_offset, e = f + V.spp;
+            const uint32_t odd_before = f / 2, odd_new = e / 2 - f / 2;
+            const uint32_t even_before = (f + 1) / 2, even_new = (e + 1) / 2 - (f + 1) / 2;
+            if (odd_new) finish(h[0], h[1], h[2], (float)odd_new, V.odd, odd_before);
+            if (even_new) finish(h[3], h[4], h[5], (float)even_new, V.even, even_before);
         }
-        V.out[3 * pix + 0] = x;
-        V.out[3 * pix + 1] = y;
-        V.out[3 * pix + 2] = z;
     } else {
         acc[0] = ax; acc[1] = ay; acc[2] = az;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) acc[3 + k] = h[k];
     }
 }
 
@@ -1554,7 +1585,7 @@ static int ensure_render_buffers(ar_ctx* ctx, size_t cap, size_t npix) {
         ctx->buffer_generation += 1;
         if (b.accum) (void)hipFree(b.accum);
         b.accum = nullptr;
-        int rc = check_hip(ctx, hipMalloc((void**)&b.accum, npix * 3 * sizeof(float)), "hipMalloc(accum)");
+        int rc = check_hip(ctx, hipMalloc((void**)&b.accum, npix * kAccumFloats * sizeof(float)), "hipMalloc(accum)");
         if (rc != AR_OK) { b.accum_pixels = 0; return rc; }
         b.accum_pixels = npix;
     }
@@ -1977,6 +2008,24 @@ int render_batch(ar_ctx* ctx, uint32_t n, const ar_camera* cams, const ar_render
 
 int render(ar_ctx* ctx, const ar_camera* cam, const ar_render_opts* opts, float* out, hipStream_t stream) {
     return render_batch(ctx, 1, cam, opts, &out, stream);
+}
+
+int render_progressive(ar_ctx* ctx, const ar_camera* cam, const ar_render_opts* opts, uint32_t frame, float* accum,
+                       float* odd, float* even, hipStream_t stream) {
+    if (!ctx->job) ctx->job = new RenderJob();
+    RenderJob& J = *ctx->job;
+    ar_render_opts o = *opts;
+    o.sample_offset = frame;
+    float* out = accum;
+    int rc = make_job(ctx, 1, cam, &o, &out, J);
+    if (rc != AR_OK) return rc;
+    SceneView& v = J.plan.views[0];
+    v.prog = 1u;
+    v.odd = odd;
+    v.even = even;
+    rc = prepare_job(ctx, J, stream, false);
+    if (rc == AR_OK) rc = enqueue_job(ctx, J, stream);
+    return rc;
 }
 
 void free_job(RenderJob* J) {

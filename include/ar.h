@@ -325,6 +325,43 @@ typedef struct {
 int ar_autotune_batch(ar_ctx* const* scenes, uint32_t n, const ar_camera* cams, const ar_render_opts* opts,
                       uint32_t probe_spp, ar_tune_result* out);
 
+/* --- progressive accumulation and output --------------------------------
+ * ar_render_progressive replaces the interactive SceneRenderer::Render
+ * (core/scene.cu:240-290 with RenderCamera's running mean and odd/even
+ * halves, core/scene.cu:135-189): it renders opts->spp samples per pixel
+ * with sample indices frame .. frame + spp - 1 (opts->sample_offset is not
+ * used) and folds their mean into accum, the running mean of the frame
+ * samples before (accum is overwritten when frame = 0, so it needs no
+ * clearing).  odd / even (both or neither) hold the running means of the
+ * samples with odd / even index.  All three are width*height*3 floats on the
+ * device, in the render's output space (opts->output).  Spectral renders
+ * accumulate CIE XYZ (AR_OUTPUT_XYZ) and convert in ar_tonemap (xyz_input),
+ * as the reference does: AR_OUTPUT_SRGB clamps each frame's mean at zero
+ * before the fold. */
+int ar_render_progressive(ar_ctx* ctx, const ar_camera* cam, const ar_render_opts* opts, uint32_t frame,
+                          float* accum, float* odd, float* even, void* stream);
+
+#define AR_TONE_LINEAR 0u
+#define AR_TONE_GAMMA 1u      /* pow(x, 1/2.2) (core/tonemap.cuh:6-9) */
+#define AR_TONE_ACES 2u       /* ACES fit (core/tonemap.cuh:66-114) */
+
+/* Tonemap kernel (core/scene.cu:22-133): hdr (width*height*3 floats, device;
+ * CIE XYZ when xyz_input, converted by XYZ2SRGBLinearD65, else linear sRGB)
+ * times exposure, the tone curve, then 0xff000000 | r << 16 | g << 8 | b
+ * per pixel into ldr (device). */
+int ar_tonemap(ar_ctx* ctx, const float* hdr, uint32_t* ldr, uint32_t width, uint32_t height, float exposure,
+               uint32_t tone, uint32_t xyz_input, void* stream);
+
+/* DeviceMSELuminance (core/helper/measure.cuh:49-74): mean over npix pixels
+ * of (lum(a) - lum(b))^2 with Rec.709 luminance; the reference's displayed
+ * variance is 1e6 times this for the odd / even halves.  Waits for the
+ * result. */
+int ar_luminance_mse(ar_ctx* ctx, const float* a, const float* b, size_t npix, double* out, void* stream);
+
+/* SaveBMP (core/camera.cu:10-62): writes width*height 0xAARRGGBB pixels
+ * (host memory, row 0 first) as a 32-bit top-down BMP.  Host only. */
+int ar_write_bmp(const char* path, const uint32_t* bgra, uint32_t width, uint32_t height);
+
 /* --- voxel guiding state ------------------------------------------------
  * Probes are keyed by (voxel = floor(p / 0.05), icosahedral face of the
  * reflected direction) and hold an 8x8 equal-area octahedral texture of

@@ -308,6 +308,19 @@ int orc_obj_light_sample_d(const orc_object* ob, const double x[3], double ub, d
 double orc_obj_light_pdf_d(const orc_object* ob, double select, const double x[3], const double y[3],
                            const double ny[3]);
 
+/* ---------------- output stage (output.c) ---------------- */
+
+/* ACES display curve (core/tonemap.cuh:66-114), linear sRGB in */
+void orc_aces(const float in[3], float out[3]);
+/* Tonemap (core/scene.cu:22-133): hdr npix x 3 (XYZ when xyz_input, else
+ * linear sRGB) -> 0xff000000 | r << 16 | g << 8 | b; tone 0 linear, 1 gamma
+ * 2.2, 2 ACES */
+void orc_tonemap(const float* hdr, uint32_t* ldr, uint32_t npix, float exposure, int tone, int xyz_input);
+/* mean over pixels of (lum(a) - lum(b))^2, luminance in float, difference in double */
+double orc_luminance_mse(const float* a, const float* b, size_t npix);
+/* running mean: acc (mean of n_before samples) takes in the mean of n_new more */
+void orc_running_mean(float* acc, const float* mean_new, size_t nfloat, uint32_t n_before, uint32_t n_new);
+
 /* Renders the mean of spp samples per pixel into out (width*height*3,
  * row-major, pixel index = j*width + i).  precision 32: float out, FP32
  * arithmetic with the same triangle test and hit rule as the GPU; 64:

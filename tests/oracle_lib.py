@@ -39,7 +39,7 @@ def load():
     global _lib
     if _lib is not None:
         return _lib
-    srcs = [ORACLE_DIR / n for n in ("trace.c", "render.c", "render_impl.h", "guide.c", "mlp.c", "oracle.h")]
+    srcs = [ORACLE_DIR / n for n in ("trace.c", "render.c", "render_impl.h", "guide.c", "mlp.c", "output.c", "oracle.h")]
     if not LIB.exists() or any(s.stat().st_mtime > LIB.stat().st_mtime for s in srcs if s.exists()):
         subprocess.run(["make", "-s", "-C", str(ORACLE_DIR)], check=True)
     lib = C.CDLL(str(LIB))

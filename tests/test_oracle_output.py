@@ -1,0 +1,105 @@
+"""Output stage on the CPU: the oracle tonemap, luminance MSE and running
+mean against closed forms, and the library's BMP writer (host code) against
+an independent reading of the BMP format.
+
+Pins: linear and gamma tone curves against numpy's floor(255 * sat(x)) and
+x ** (1 / 2.2); byte packing 0xff000000 | r << 16 | g << 8 | b; the
+XYZ -> linear sRGB matrix mapping D65 white to (1, 1, 1); ACES properties (0
+maps to 0, monotone and near-neutral along the grey axis, bounded); the MSE
+against numpy; the folded running mean against the direct mean."""
+import ctypes as C
+import struct
+
+import numpy as np
+import pytest
+
+import oracle_lib as orc
+
+
+def _tonemap(hdr, exposure=1.0, tone=0, xyz=False):
+    f = orc.fn("orc_tonemap", None, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_float, C.c_int, C.c_int])
+    hdr = np.ascontiguousarray(hdr, np.float32).reshape(-1, 3)
+    out = np.zeros(hdr.shape[0], np.uint32)
+    f(hdr.ctypes.data, out.ctypes.data, hdr.shape[0], exposure, tone, int(xyz))
+    return out
+
+
+def _unpack(p):
+    return np.stack([(p >> 16) & 255, (p >> 8) & 255, p & 255], axis=-1).astype(np.int64)
+
+
+def test_linear_and_gamma_curves_and_packing():
+    rng = np.random.default_rng(1)
+    x = rng.uniform(-0.2, 1.5, (5000, 3)).astype(np.float32)
+    lin = _tonemap(x, exposure=1.0, tone=0)
+    assert np.all((lin >> 24) == 255)
+    np.testing.assert_array_equal(_unpack(lin), np.floor(255.0 * np.clip(x.astype(np.float32), 0, 1)).astype(np.int64))
+    g = _unpack(_tonemap(x, exposure=0.8, tone=1))
+    expect = np.floor(255.0 * np.clip(np.clip(x * np.float32(0.8), 0, None).astype(np.float64) ** (1 / 2.2), 0, 1))
+    assert np.abs(g - expect).max() <= 1 and np.mean(g == expect) > 0.99
+    p = _tonemap(np.array([[1.0, 0.5, 0.0]], np.float32))[0]
+    assert p == 0xff000000 | (255 << 16) | (127 << 8) | 0
+
+
+def test_xyz_input_maps_d65_white_to_white():
+    p = _unpack(_tonemap(np.array([[0.95047, 1.0, 1.08883]], np.float32) * 0.999, xyz=True))[0]
+    assert np.all(np.abs(p - 254) <= 1)
+
+
+def test_aces_properties():
+    f = orc.fn("orc_aces", None, [C.c_void_p, C.c_void_p])
+
+    def aces(v):
+        i = np.ascontiguousarray(v, np.float32)
+        o = np.zeros(3, np.float32)
+        f(i.ctypes.data, o.ctypes.data)
+        return o
+
+    np.testing.assert_allclose(aces([0, 0, 0]), 0, atol=1e-6)
+    grey = np.array([aces([v, v, v]) for v in np.geomspace(1e-3, 1e3, 60)])
+    assert np.all(np.diff(grey.mean(axis=1)) > 0)                 # monotone
+    assert np.max(np.abs(grey - grey.mean(axis=1, keepdims=True))) < 0.02   # near-neutral
+    assert grey.max() <= 1.0 and grey[-1].mean() > 0.95          # bounded, approaching white
+    # a saturated red stays red-dominant
+    r = aces([1.0, 0.05, 0.05])
+    assert r[0] > r[1] and r[0] > r[2]
+
+
+def test_luminance_mse_and_running_mean():
+    rng = np.random.default_rng(2)
+    a = rng.random((777, 3), dtype=np.float32)
+    b = rng.random((777, 3), dtype=np.float32)
+    f = orc.fn("orc_luminance_mse", C.c_double, [C.c_void_p, C.c_void_p, C.c_size_t])
+    la = (np.float32(0.2126) * a[:, 0] + np.float32(0.7152) * a[:, 1]) + np.float32(0.0722) * a[:, 2]
+    lb = (np.float32(0.2126) * b[:, 0] + np.float32(0.7152) * b[:, 1]) + np.float32(0.0722) * b[:, 2]
+    expect = np.mean((la.astype(np.float64) - lb.astype(np.float64)) ** 2)
+    assert f(a.ctypes.data, b.ctypes.data, 777) == pytest.approx(expect, rel=1e-12)
+    # running mean over chunks of 1, 3, 2, 5 samples equals the direct mean
+    fold = orc.fn("orc_running_mean", None, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_uint32, C.c_uint32])
+    samples = rng.random((11, 64), dtype=np.float32)
+    acc = np.zeros(64, np.float32)
+    done = 0
+    for k in (1, 3, 2, 5):
+        m = np.ascontiguousarray(samples[done:done + k].mean(axis=0), np.float32)
+        fold(acc.ctypes.data, m.ctypes.data, 64, done, k)
+        done += k
+    np.testing.assert_allclose(acc, samples.mean(axis=0), rtol=1e-6)
+
+
+def test_bmp_writer_matches_the_format(tmp_path):
+    from aperturerendererv3_amd import write_bmp
+    rng = np.random.default_rng(3)
+    h, w = 7, 5
+    px = (0xff000000 | rng.integers(0, 1 << 24, (h, w))).astype(np.uint32)
+    path = tmp_path / "img.bmp"
+    write_bmp(path, px)
+    data = path.read_bytes()
+    assert data[:2] == b"BM"
+    size, _, off = struct.unpack_from("<IIII", data, 2)[:3]
+    assert size == len(data) == 54 + 4 * h * w and off == 54
+    dib, bw, bh, planes, bpp, comp, isize = struct.unpack_from("<IiiHHII", data, 14)
+    assert (dib, bw, bh, planes, bpp, comp, isize) == (40, w, -h, 1, 32, 0, 4 * h * w)
+    pix = np.frombuffer(data, np.uint8, offset=54).reshape(h, w, 4)   # top-down: row 0 first
+    np.testing.assert_array_equal(pix[..., 2], (px >> 16) & 255)     # R
+    np.testing.assert_array_equal(pix[..., 1], (px >> 8) & 255)      # G
+    np.testing.assert_array_equal(pix[..., 0], px & 255)             # B
