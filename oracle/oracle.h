@@ -25,6 +25,11 @@
  *   orc_trace_any        equality with (closest hit t < tmax)
  *   orc_object_*         FP64 quadratic / slab solutions, sampled surfaces,
  *                        proxy extreme points (tests/test_oracle_objects.py)
+ *   SDF objects          the torus's closed-form ray intersection (quartic
+ *                        roots in FP64) and normal, the march against the
+ *                        analytic torus through a baked volume, the bake's
+ *                        grid values against the closed form
+ *                        (tests/test_oracle_sdf.py)
  *   object lights, glass white / diffuse furnaces, n^2 radiance inside glass,
  *                        Beer-Lambert series, sampler pairing and solid-angle
  *                        integrals (tests/test_oracle_render_objects.py)
@@ -72,6 +77,7 @@ int orc_tri_test(const float org[3], const float dir[3],
 /* Ray batch in structure-of-arrays form (same as the library's ar_rays). */
 typedef struct {
     const float *ox, *oy, *oz, *tmin, *dx, *dy, *dz, *tmax;
+    const float* meps;   /* optional SDF march tolerance per ray (NULL: max(tmin, 1e-3) / 2) */
 } orc_rays;
 
 typedef struct {
@@ -105,6 +111,8 @@ void orc_trace_any_brute(const float* verts, const uint32_t* idx, int ntri,
 
 #define ORC_OBJ_BOX 0u        /* OBJ_CUBE: oriented box of half-extents `extent` */
 #define ORC_OBJ_SPHERE 1u     /* OBJ_SPHERE: radius = largest extent */
+#define ORC_OBJ_SDF_TORUS 2u  /* OBJ_SDF + SDF_TORUS: radii sdf[0], sdf[1] in unit space */
+#define ORC_OBJ_SDF_VOLUME 3u /* OBJ_SDF + SDF_VOLUME: distance grid vol (unit-space values) */
 #define ORC_HIT_INSIDE 16u    /* hit flag: the ray started inside the object */
 
 typedef struct {
@@ -112,6 +120,10 @@ typedef struct {
     float center[3];
     float extent[3];     /* absolute values */
     float w2o[9];        /* world-to-object rotation, row-major */
+    float sdf[4];        /* torus radii */
+    const float* vol;    /* volume voxels, x fastest (dims[0] * dims[1] * dims[2]) */
+    uint32_t dims[3];
+    float vmax;          /* voxelSizeMax_: |nearest| above it skips the trilinear lookup */
 } orc_object;
 
 /* SceneObject(center, extent, material, rotation, type) (core/render/object.cuh:286-305) */
@@ -124,6 +136,22 @@ int orc_object_test(const orc_object* ob, const float o[3], const float d[3], fl
 void orc_object_surface(const orc_object* ob, const float o[3], const float d[3], float t, int inside,
                         float pos[3], float n[3]);
 int orc_object_overlap(const orc_object* ob, const float p[3]);
+/* SDF objects: params (torus radii) and the volume of an SDF_VOLUME object;
+ * vmax = |(1/dx, 1/dy, 1/dz)| * 2.5 (SDFVolumeInfo::ComputeVoxelSize) */
+void orc_object_set_sdf(orc_object* ob, const float params[4], const float* vol, const uint32_t dims[3]);
+/* as orc_object_test / _surface with the SDF march tolerance meps (the
+ * reference's SDF_EPS(depth) = INTERSECT_EPS(depth) / 2; unused by boxes and
+ * spheres); the plain forms use meps = 5e-4 */
+int orc_object_test_eps(const orc_object* ob, const float o[3], const float d[3], float meps, float* t, int* inside);
+void orc_object_surface_eps(const orc_object* ob, const float o[3], const float d[3], float t, int inside,
+                            float meps, float pos[3], float n[3]);
+/* unit-space distance of an SDF object's shape at u (|u_k| <= 0.5 inside its box) */
+float orc_sdf_value(const orc_object* ob, const float u[3]);
+/* FillSDFVolume (sdf_cache_manager.cuh:16-37): out[x + dims0 (y + dims1 z)] =
+ * shape(p01 - 0.5), p01 = (x, y, z) / (dims - 1); shape 0 = torus (params R, r),
+ * 1 = diamond (SdfDiamondUnit, sdf_diamond.cuh:21-66; params d, bias, tableZ,
+ * scale, zeros taking the defaults 0.94, 0.5, 0.3, 1) */
+void orc_sdf_bake(uint32_t shape, const float params[4], const uint32_t dims[3], float* out);
 
 /* Scene geometry: primitives 0..ntri-1 are triangles, ntri..ntri+nobj-1 objects. */
 typedef struct {
@@ -149,6 +177,12 @@ int orc_closest1_g(const orc_node* nodes, int node_count, const int32_t* indices
                    uint32_t* prim, float* t, float* u, float* v, uint32_t* flags);
 int orc_any1_g(const orc_node* nodes, int node_count, const int32_t* indices, const orc_geom* g,
                const float o[3], const float d[3], float tmin, float tmax);
+/* the same with an explicit SDF march tolerance */
+int orc_closest1_ge(const orc_node* nodes, int node_count, const int32_t* indices, const orc_geom* g,
+                    const float o[3], const float d[3], float tmin, float tmax, float meps,
+                    uint32_t* prim, float* t, float* u, float* v, uint32_t* flags);
+int orc_any1_ge(const orc_node* nodes, int node_count, const int32_t* indices, const orc_geom* g,
+                const float o[3], const float d[3], float tmin, float tmax, float meps);
 
 /* Maximum depth (root = 0) of a built tree. */
 int orc_bvh_depth(const orc_node* nodes, int node_count);

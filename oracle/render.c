@@ -137,7 +137,7 @@ static int box_hit_d(const double o[3], const double inv[3], const orc_node* nd,
 /* any = 1: first accepted hit.  Canonical rule (min t, then min id); objects
  * by the FP64 instance of the object intersector. */
 static uint32_t trace_d(const orc_render_scene* sc, const double o[3], const double d[3], double tmin, double tmax,
-                        int any, double* tout, double* uout, double* vout, int* inside)
+                        double meps, int any, double* tout, double* uout, double* vout, int* inside)
 {
     *inside = 0;
     if (sc->node_count == 0 || !(tmax >= tmin)) return ORC_MISS;
@@ -163,7 +163,7 @@ static uint32_t trace_d(const orc_render_scene* sc, const double o[3], const dou
                     if (!tri_test_d(o, d, a, b, c, &t, &u, &v)) continue;
                 } else {
                     const vec3_f64 oo = { o[0], o[1], o[2] }, dd = { d[0], d[1], d[2] };
-                    if (!obj_test_f64(&sc->objs[p - sc->ntri], oo, dd, &t, &ins)) continue;
+                    if (!obj_test_f64(&sc->objs[p - sc->ntri], oo, dd, meps, &t, &ins)) continue;
                 }
                 const int acc = t >= tmin && (best != ORC_MISS ? (t < bt || (t == bt && p < best)) : t < tmax);
                 if (acc) {
@@ -194,16 +194,16 @@ static uint32_t closest_f32(const orc_render_scene* sc, vec3_f32 o, vec3_f32 d, 
     const orc_geom g = scene_geom(sc);
     uint32_t p, f = 0;
     *bu = *bv = 0.0f;
-    orc_closest1_g(sc->nodes, sc->node_count, sc->indices, &g, oo, dd, tmin, tmax, &p, t, bu, bv, &f);
+    orc_closest1_ge(sc->nodes, sc->node_count, sc->indices, &g, oo, dd, tmin, tmax, 0.5f * tmin, &p, t, bu, bv, &f);
     *inside = (f & ORC_HIT_INSIDE) != 0;
     return p;
 }
 
-static int occluded_f32(const orc_render_scene* sc, vec3_f32 o, vec3_f32 d, float tmin, float tmax)
+static int occluded_f32(const orc_render_scene* sc, vec3_f32 o, vec3_f32 d, float tmin, float tmax, float meps)
 {
     const float oo[3] = { o.x, o.y, o.z }, dd[3] = { d.x, d.y, d.z };
     const orc_geom g = scene_geom(sc);
-    return orc_any1_g(sc->nodes, sc->node_count, sc->indices, &g, oo, dd, tmin, tmax);
+    return orc_any1_ge(sc->nodes, sc->node_count, sc->indices, &g, oo, dd, tmin, tmax, meps);
 }
 
 static uint32_t closest_f64(const orc_render_scene* sc, vec3_f64 o, vec3_f64 d, double tmin, double tmax,
@@ -211,15 +211,15 @@ static uint32_t closest_f64(const orc_render_scene* sc, vec3_f64 o, vec3_f64 d, 
 {
     const double oo[3] = { o.x, o.y, o.z }, dd[3] = { d.x, d.y, d.z };
     *bu = *bv = 0.0;
-    return trace_d(sc, oo, dd, tmin, tmax, 0, t, bu, bv, inside);
+    return trace_d(sc, oo, dd, tmin, tmax, 0.5 * tmin, 0, t, bu, bv, inside);
 }
 
-static int occluded_f64(const orc_render_scene* sc, vec3_f64 o, vec3_f64 d, double tmin, double tmax)
+static int occluded_f64(const orc_render_scene* sc, vec3_f64 o, vec3_f64 d, double tmin, double tmax, double meps)
 {
     const double oo[3] = { o.x, o.y, o.z }, dd[3] = { d.x, d.y, d.z };
     double t, u, v;
     int ins;
-    return trace_d(sc, oo, dd, tmin, tmax, 1, &t, &u, &v, &ins) != ORC_MISS;
+    return trace_d(sc, oo, dd, tmin, tmax, meps, 1, &t, &u, &v, &ins) != ORC_MISS;
 }
 
 /* ---------------- RGB uplift lookup ---------------- */

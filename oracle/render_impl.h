@@ -486,7 +486,7 @@ static V3 RF(sky)(V3 d)
  * v2, and for an object whether the ray started inside it */
 static uint32_t RF(closest)(const orc_render_scene* sc, V3 o, V3 d, R tmin, R tmax, R* t, R* bu, R* bv,
                             int* inside);
-static int RF(occluded)(const orc_render_scene* sc, V3 o, V3 d, R tmin, R tmax);
+static int RF(occluded)(const orc_render_scene* sc, V3 o, V3 d, R tmin, R tmax, R meps);
 
 static void RF(tri)(const orc_render_scene* sc, uint32_t p, V3* a, V3* b, V3* c)
 {
@@ -547,13 +547,24 @@ static R RF(obj_radius)(const orc_object* ob)
     return (R)fmaxf(fmaxf(ob->extent[0], ob->extent[1]), ob->extent[2]);
 }
 
-/* IntersectAABBObject / IntersectSphereObject (aabb.cuh:28-89, sphere.cuh:11-68) */
-static int RF(obj_test)(const orc_object* ob, V3 o, V3 d, R* t, int* inside)
+/* SDF objects in R (oracle/sdf_impl.h) */
+#define SR(name) RF(name)
+#include "sdf_impl.h"
+#undef SR
+
+static int RF(is_sdf)(const orc_object* ob)
+{
+    return ob->type == ORC_OBJ_SDF_TORUS || ob->type == ORC_OBJ_SDF_VOLUME;
+}
+
+/* IntersectAABBObject / IntersectSphereObject / IntersectSDFObject (aabb.cuh:28-89,
+ * sphere.cuh:11-68, sdf_base.cuh:118-167); meps = the SDF march tolerance */
+static int RF(obj_test)(const orc_object* ob, V3 o, V3 d, R meps, R* t, int* inside)
 {
     V3 lo, ld;
     RF(obj_local)(ob, o, d, &lo, &ld);
     *inside = 0;
-    if (ob->type == ORC_OBJ_BOX) {
+    if (ob->type != ORC_OBJ_SPHERE) {
         const R lox[3] = { lo.x, lo.y, lo.z }, ldx[3] = { ld.x, ld.y, ld.z };
         R te = 0, tx = 0;
         for (int a = 0; a < 3; ++a) {
@@ -566,6 +577,10 @@ static int RF(obj_test)(const orc_object* ob, V3 o, V3 d, R* t, int* inside)
         }
         if (te > tx) return 0;
         if (tx <= 0) return 0;
+        if (RF(is_sdf)(ob)) {
+            const R l3[3] = { lo.x, lo.y, lo.z }, d3[3] = { ld.x, ld.y, ld.z };
+            return RF(sdf_march)(ob, l3, d3, te, tx, meps, t, inside);
+        }
         *inside = te < 0;
         *t = *inside ? tx : te;
         return 1;
@@ -583,14 +598,20 @@ static int RF(obj_test)(const orc_object* ob, V3 o, V3 d, R* t, int* inside)
     return 1;
 }
 
-/* hit point and world normal (flipped for a hit from inside) */
-static void RF(obj_surface)(const orc_object* ob, V3 o, V3 d, R t, int inside, V3* pos, V3* n)
+/* hit point and world normal (box and sphere: flipped for a hit from inside;
+ * SDF: the outward distance gradient) */
+static void RF(obj_surface)(const orc_object* ob, V3 o, V3 d, R t, int inside, R meps, V3* pos, V3* n)
 {
     V3 lo, ld;
     RF(obj_local)(ob, o, d, &lo, &ld);
     const V3 lp = RF(add)(lo, RF(scale)(ld, t));
     V3 ln = RF(v3)(0, 0, 0);
-    if (ob->type == ORC_OBJ_BOX) {
+    if (RF(is_sdf)(ob)) {
+        const R l3[3] = { lp.x, lp.y, lp.z };
+        R n3[3];
+        RF(sdf_normal)(ob, l3, meps, n3);
+        ln = RF(v3)(n3[0], n3[1], n3[2]);
+    } else if (ob->type == ORC_OBJ_BOX) {
         const R rx = FABS(lp.x) / (R)ob->extent[0], ry = FABS(lp.y) / (R)ob->extent[1],
                 rz = FABS(lp.z) / (R)ob->extent[2];
         if (rx >= ry && rx >= rz) ln.x = lp.x > 0 ? (R)1 : (R)-1;
@@ -614,6 +635,12 @@ static int RF(obj_overlap)(const orc_object* ob, V3 p)
     const V3 oc = RF(sub)(p, RF(load3)(ob->center));
     const V3 lo = RF(v3)(RF(dot)(RF(w2o_row)(ob, 0), oc), RF(dot)(RF(w2o_row)(ob, 1), oc),
                          RF(dot)(RF(w2o_row)(ob, 2), oc));
+    if (RF(is_sdf)(ob)) {
+        const R l3[3] = { lo.x, lo.y, lo.z };
+        R u3[3];
+        RF(sdf_to_unit)(ob, l3, u3);
+        return RF(sdf_unit)(ob, u3[0], u3[1], u3[2]) <= 0;
+    }
     if (ob->type == ORC_OBJ_BOX) {
         const R ex = ob->extent[0], ey = ob->extent[1], ez = ob->extent[2];
         return -ex < lo.x && -ey < lo.y && -ez < lo.z && ex > lo.x && ey > lo.y && ez > lo.z;
@@ -873,7 +900,7 @@ static uint32_t RF(next_hit)(const orc_render_scene* sc, int medium, V3 o, V3 d,
     if (medium >= 0) {
         R tt;
         int ins;
-        if (RF(obj_test)(&sc->objs[medium], o, d, &tt, &ins) && tt >= tmin) {
+        if (RF(obj_test)(&sc->objs[medium], o, d, tmin * (R)0.5, &tt, &ins) && tt >= tmin) {
             *t = tt;
             *inside = ins;
             return sc->ntri + (uint32_t)medium;
@@ -977,7 +1004,7 @@ static V3 RF(trace_path)(const orc_render_scene* sc, const orc_render_lights* L,
         if (is_obj) {
             /* object hits shade with the intersector's normal: outward, or
              * inward for a hit from inside (aabb.cuh:60-73, sphere.cuh:52-53) */
-            RF(obj_surface)(ob, o, d, t, inside, &p, &ng);
+            RF(obj_surface)(ob, o, d, t, inside, eps * (R)0.5, &p, &ng);
             ndd = RF(dot)(ng, d);
             n = ng;
         } else {
@@ -1185,7 +1212,7 @@ static V3 RF(trace_path)(const orc_render_scene* sc, const orc_render_lights* L,
                     const V3 sdir = RF(scale)(lu, (R)1 / dist);
                     const R seps = RF(intersect_eps)(depth) * RF(maxr)((R)1, RF(maxr)(RF(length)(p), RF(length)(q)));
                     const R tmx = RF(maxr)((R)0, dist - seps);
-                    if (tmx > 0) occl = RF(occluded)(sc, p, sdir, (R)0, tmx);
+                    if (tmx > 0) occl = RF(occluded)(sc, p, sdir, (R)0, tmx, RF(intersect_eps)(depth) * (R)0.5);
                 }
                 if (occl) {
                     did_nee = 1;

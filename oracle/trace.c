@@ -363,11 +363,16 @@ static float box_pad(const float mn[3], const float mx[3], float omax)
     return m * 0x1p-16f + 1e-30f;
 }
 
-static void load_ray(orc_rays r, size_t i, float o[3], float d[3], float* tmin, float* tmax)
+/* default SDF march tolerance of a ray: SDF_EPS(0) = INTERSECT_EPS(0) / 2 for
+ * tmin <= 1e-3, tmin / 2 above (integrator rays carry tmin = INTERSECT_EPS(depth)) */
+static float default_meps(float tmin) { return 0.5f * fmaxf(tmin, 1e-3f); }
+
+static void load_ray(orc_rays r, size_t i, float o[3], float d[3], float* tmin, float* tmax, float* meps)
 {
     o[0] = r.ox[i]; o[1] = r.oy[i]; o[2] = r.oz[i];
     d[0] = r.dx[i]; d[1] = r.dy[i]; d[2] = r.dz[i];
     *tmin = r.tmin[i]; *tmax = r.tmax[i];
+    *meps = r.meps ? r.meps[i] : default_meps(*tmin);
 }
 
 static void tri_verts(const float* verts, const uint32_t* idx, int p,
@@ -408,6 +413,21 @@ static void normalize3(float v[3])
 
 static float max3f(const float v[3]) { return fmaxf(fmaxf(v[0], v[1]), v[2]); }
 
+/* float instance of the SDF march (oracle/sdf_impl.h) */
+#define R float
+#define SR(name) name##_f32
+#define SQRT sqrtf
+#define FLOOR floorf
+#define FABS fabsf
+#include "sdf_impl.h"
+#undef R
+#undef SR
+#undef SQRT
+#undef FLOOR
+#undef FABS
+
+static int is_sdf(const orc_object* ob) { return ob->type == ORC_OBJ_SDF_TORUS || ob->type == ORC_OBJ_SDF_VOLUME; }
+
 /* SceneObject constructor (object.cuh:286-305): objectToWorld = Rz (Ry Rx)
  * from Euler angles in degrees (float3x3::rotX/Y/Z, vector.cuh:394-425),
  * worldToObject = its transpose; extents are taken by absolute value. */
@@ -423,18 +443,19 @@ void orc_object_init(orc_object* ob, uint32_t type, const float center[3], const
     float RyRx[9], o2w[9];
     mat3_mul(Ry, Rx, RyRx);
     mat3_mul(Rz, RyRx, o2w);
+    memset(ob, 0, sizeof *ob);
     ob->type = type;
     for (int a = 0; a < 3; ++a) { ob->center[a] = center[a]; ob->extent[a] = fabsf(extent[a]); }
     for (int r = 0; r < 3; ++r)
         for (int c = 0; c < 3; ++c) ob->w2o[3 * r + c] = o2w[3 * c + r];
 }
 
-/* UpdateProxy (object.cuh:259-284): box = centre +- sum |axis_k| * extent_k,
- * sphere = centre +- max extent; the proxy centre is the object centre. */
+/* UpdateProxy (object.cuh:259-284): box and SDF = centre +- sum |axis_k| *
+ * extent_k, sphere = centre +- max extent; the proxy centre is the object centre. */
 void orc_object_proxy(const orc_object* ob, float pmin[3], float pmax[3], float pcen[3])
 {
     float half[3];
-    if (ob->type == ORC_OBJ_BOX) {
+    if (ob->type != ORC_OBJ_SPHERE) {
         const float* ax = &ob->w2o[0];   /* objectToWorld * e_x = row 0 of worldToObject */
         const float* ay = &ob->w2o[3];
         const float* az = &ob->w2o[6];
@@ -451,12 +472,13 @@ void orc_object_proxy(const orc_object* ob, float pmin[3], float pmax[3], float 
     }
 }
 
-/* GetArea (object.cuh:239-252): the box's 8 (ex ey + ex ez + ey ez); the
- * sphere's 4 pi extent.x^2 (extent.x, not the intersected max extent) */
+/* GetArea (object.cuh:239-252): the box's (and an SDF object's box's) 8 (ex ey
+ * + ex ez + ey ez); the sphere's 4 pi extent.x^2 (extent.x, not the
+ * intersected max extent) */
 float orc_object_area(const orc_object* ob)
 {
     const float ex = ob->extent[0], ey = ob->extent[1], ez = ob->extent[2];
-    if (ob->type == ORC_OBJ_BOX) return 8.0f * (ex * ey + ex * ez + ey * ez);
+    if (ob->type != ORC_OBJ_SPHERE) return 8.0f * (ex * ey + ex * ez + ey * ez);
     return 4.0f * 3.141592654f * ex * ex;
 }
 
@@ -470,15 +492,17 @@ static void obj_local_ray(const orc_object* ob, const float o[3], const float d[
     normalize3(ld);
 }
 
-/* IntersectAABBObject / IntersectSphereObject: the first surface crossing
- * along the normalised local direction with t > 0 (t is measured along
- * normalize(d)); inside = the ray starts inside the object. */
-int orc_object_test(const orc_object* ob, const float o[3], const float d[3], float* t, int* inside)
+/* IntersectAABBObject / IntersectSphereObject / IntersectSDFObject
+ * (sdf_base.cuh:118-167): the first surface crossing along the normalised
+ * local direction with t > 0 (t is measured along normalize(d)); inside = the
+ * ray starts inside the object.  An SDF object is marched between the
+ * entry and exit of its box. */
+int orc_object_test_eps(const orc_object* ob, const float o[3], const float d[3], float meps, float* t, int* inside)
 {
     float lo[3], ld[3];
     obj_local_ray(ob, o, d, lo, ld);
     *inside = 0;
-    if (ob->type == ORC_OBJ_BOX) {
+    if (ob->type != ORC_OBJ_SPHERE) {
         float te = -INFINITY, tx = INFINITY;
         for (int a = 0; a < 3; ++a) {
             const float inv = inv_safe(ld[a]);
@@ -490,6 +514,7 @@ int orc_object_test(const orc_object* ob, const float o[3], const float d[3], fl
         }
         if (te > tx) return 0;
         if (tx <= 0.0f) return 0;
+        if (is_sdf(ob)) return sdf_march_f32(ob, lo, ld, te, tx, meps, t, inside);
         *inside = te < 0.0f;
         *t = *inside ? tx : te;
         return 1;
@@ -507,17 +532,25 @@ int orc_object_test(const orc_object* ob, const float o[3], const float d[3], fl
     return 1;
 }
 
+int orc_object_test(const orc_object* ob, const float o[3], const float d[3], float* t, int* inside)
+{
+    return orc_object_test_eps(ob, o, d, 5e-4f, t, inside);
+}
+
 /* Hit point and normal of an object hit (the tail of IntersectAABBObject /
- * IntersectSphereObject): local point lo + ld t; box normal along the axis of
- * largest |p_k| / extent_k, sphere normal p / |p|; flipped for a hit from
- * inside; both rotated to world space and normalised. */
-void orc_object_surface(const orc_object* ob, const float o[3], const float d[3], float t, int inside,
-                        float pos[3], float n[3])
+ * IntersectSphereObject / IntersectSDFObject): local point lo + ld t; box
+ * normal along the axis of largest |p_k| / extent_k, sphere normal p / |p|,
+ * both flipped for a hit from inside; SDF normal the distance gradient
+ * (outward, never flipped); rotated to world space and normalised. */
+void orc_object_surface_eps(const orc_object* ob, const float o[3], const float d[3], float t, int inside,
+                            float meps, float pos[3], float n[3])
 {
     float lo[3], ld[3], lp[3], ln[3];
     obj_local_ray(ob, o, d, lo, ld);
     for (int a = 0; a < 3; ++a) lp[a] = lo[a] + ld[a] * t;
-    if (ob->type == ORC_OBJ_BOX) {
+    if (is_sdf(ob)) {
+        sdf_normal_f32(ob, lp, meps, ln);
+    } else if (ob->type == ORC_OBJ_BOX) {
         const float rx = fabsf(lp[0]) / ob->extent[0];
         const float ry = fabsf(lp[1]) / ob->extent[1];
         const float rz = fabsf(lp[2]) / ob->extent[2];
@@ -541,17 +574,99 @@ void orc_object_surface(const orc_object* ob, const float o[3], const float d[3]
     normalize3(n);
 }
 
-/* OverlapAABBObject / OverlapSphereObject: strict containment of a point */
+void orc_object_surface(const orc_object* ob, const float o[3], const float d[3], float t, int inside,
+                        float pos[3], float n[3])
+{
+    orc_object_surface_eps(ob, o, d, t, inside, 5e-4f, pos, n);
+}
+
+/* OverlapAABBObject / OverlapSphereObject: strict containment of a point;
+ * OverlapSDFObject (sdf_base.cuh:169-180): distance <= 0 */
 int orc_object_overlap(const orc_object* ob, const float p[3])
 {
     const float oc[3] = { p[0] - ob->center[0], p[1] - ob->center[1], p[2] - ob->center[2] };
     float lo[3];
     for (int r = 0; r < 3; ++r) lo[r] = dot3(&ob->w2o[3 * r], oc);
+    if (is_sdf(ob)) {
+        float u[3];
+        sdf_to_unit_f32(ob, lo, u);
+        return sdf_unit_f32(ob, u[0], u[1], u[2]) <= 0.0f;
+    }
     if (ob->type == ORC_OBJ_BOX)
         return -ob->extent[0] < lo[0] && -ob->extent[1] < lo[1] && -ob->extent[2] < lo[2] &&
                ob->extent[0] > lo[0] && ob->extent[1] > lo[1] && ob->extent[2] > lo[2];
     const float r = max3f(ob->extent);
     return dot3(lo, lo) < r * r;
+}
+
+void orc_object_set_sdf(orc_object* ob, const float params[4], const float* vol, const uint32_t dims[3])
+{
+    for (int k = 0; k < 4; ++k) ob->sdf[k] = params ? params[k] : 0.0f;
+    ob->vol = vol;
+    for (int k = 0; k < 3; ++k) ob->dims[k] = dims ? dims[k] : 0u;
+    if (dims) {
+        const float ix = 1.0f / fmaxf((float)dims[0], 1.0f), iy = 1.0f / fmaxf((float)dims[1], 1.0f),
+                    iz = 1.0f / fmaxf((float)dims[2], 1.0f);
+        ob->vmax = sqrtf(ix * ix + iy * iy + iz * iz) * 2.5f;
+    }
+}
+
+float orc_sdf_value(const orc_object* ob, const float u[3])
+{
+    return sdf_unit_f32(ob, u[0], u[1], u[2]);
+}
+
+/* PlaneSDF (sdf_diamond.cuh:15-19) */
+static float plane_sdf(const float p[3], float nx, float ny, float nz, float d)
+{
+    const float len = fmaxf(sqrtf(nx * nx + ny * ny + nz * nz), 1e-8f);
+    return (p[0] * nx + p[1] * ny + p[2] * nz - d) / len;
+}
+
+/* SdfDiamondUnit (sdf_diamond.cuh:21-66): a brilliant cut as the maximum of
+ * its facet planes, in the sector of the azimuth of p */
+static float sdf_diamond(const float u[3], const float prm[4])
+{
+    const float dd = prm[0] != 0.0f ? prm[0] : 0.94f;
+    const float bias = prm[1] != 0.0f ? prm[1] : 0.5f;
+    const float table = prm[2] != 0.0f ? prm[2] : 0.3f;
+    const float scale = prm[3] != 0.0f ? prm[3] : 1.0f;
+    const float si = 1.0f / scale;
+    const float p[3] = { u[0] * si, u[1] * si, u[2] * si };
+    const float af2 = 4.0f / 3.14159265358979f;
+    const float s = atan2f(p[1], p[0]);
+    const float sf = floorf(s * af2 + bias) / af2;
+    const float sf2 = floorf(s * af2) / af2;
+    float m = -1e20f;
+    m = fmaxf(m, plane_sdf(p, cosf(sf), sinf(sf), 1.444f, dd));
+    m = fmaxf(m, plane_sdf(p, cosf(sf), sinf(sf), -1.072f, dd));
+    m = fmaxf(m, plane_sdf(p, 0.0f, 0.0f, 1.0f, table));
+    m = fmaxf(m, plane_sdf(p, 0.0f, 0.0f, -1.0f, 0.865f));
+    m = fmaxf(m, plane_sdf(p, cosf(s), sinf(s), 0.0f, 0.911f));
+    m = fmaxf(m, plane_sdf(p, cosf(sf + 0.21f), sinf(sf + 0.21f), -1.02f, 0.9193f));
+    m = fmaxf(m, plane_sdf(p, cosf(sf - 0.21f), sinf(sf - 0.21f), -1.02f, 0.9193f));
+    m = fmaxf(m, plane_sdf(p, cosf(sf - 0.21f), sinf(sf - 0.21f), 1.03f, 0.912f));
+    m = fmaxf(m, plane_sdf(p, cosf(sf + 0.21f), sinf(sf + 0.21f), 1.03f, 0.912f));
+    m = fmaxf(m, plane_sdf(p, cosf(sf2 + 0.393f), sinf(sf2 + 0.393f), 2.21f, 1.131f));
+    return m * scale;
+}
+
+void orc_sdf_bake(uint32_t shape, const float params[4], const uint32_t dims[3], float* out)
+{
+    orc_object tor;
+    memset(&tor, 0, sizeof tor);
+    tor.type = ORC_OBJ_SDF_TORUS;
+    tor.sdf[0] = params[0]; tor.sdf[1] = params[1];
+    for (uint32_t z = 0; z < dims[2]; ++z)
+        for (uint32_t y = 0; y < dims[1]; ++y)
+            for (uint32_t x = 0; x < dims[0]; ++x) {
+                const float p01[3] = { dims[0] > 1 ? (float)x / (float)(dims[0] - 1) : 0.0f,
+                                       dims[1] > 1 ? (float)y / (float)(dims[1] - 1) : 0.0f,
+                                       dims[2] > 1 ? (float)z / (float)(dims[2] - 1) : 0.0f };
+                const float u[3] = { p01[0] - 0.5f, p01[1] - 0.5f, p01[2] - 0.5f };
+                out[((size_t)z * dims[1] + y) * dims[0] + x] =
+                    shape == 0 ? sdf_torus_f32(&tor, u[0], u[1], u[2]) : sdf_diamond(u, params);
+            }
 }
 
 void orc_geom_proxies(const orc_geom* g, float* pmin, float* pmax, float* pcen)
@@ -566,7 +681,7 @@ void orc_geom_proxies(const orc_geom* g, float* pmin, float* pmax, float* pcen)
 /* Primitive p of the geometry (triangles first, then objects): 1 when the
  * ray's line (triangle) or the forward ray (object) crosses it, with t. */
 static int prim_test(const orc_geom* g, uint32_t p, const float o[3], const float d[3], const ray_pre* pre,
-                     float* t)
+                     float meps, float* t)
 {
     if (p < g->ntri) {
         const float *a, *b, *c;
@@ -574,14 +689,14 @@ static int prim_test(const orc_geom* g, uint32_t p, const float o[3], const floa
         return tri_test_pre(o, pre, a, b, c, t, NULL, NULL, NULL);
     }
     int inside;
-    return orc_object_test(&g->objs[p - g->ntri], o, d, t, &inside);
+    return orc_object_test_eps(&g->objs[p - g->ntri], o, d, meps, t, &inside);
 }
 
 #define ORC_STACK 128
 
 /* One ray.  any = 1: stop at the first accepted hit.  Returns 1 on a hit. */
 static int traverse_one(const orc_node* nodes, int node_count, const int32_t* indices, const orc_geom* g,
-                        const float o[3], const float d[3], float tmin, float tmax,
+                        const float o[3], const float d[3], float tmin, float tmax, float meps,
                         int mode, int any,
                         uint32_t* prim_out, float* t_out, uint32_t* nbox, uint32_t* ntri)
 {
@@ -608,7 +723,7 @@ static int traverse_one(const orc_node* nodes, int node_count, const int32_t* in
                     const uint32_t p = (uint32_t)indices[nd->first + k];
                     float t;
                     nt++;
-                    if (!prim_test(g, p, o, d, &pre, &t)) continue;
+                    if (!prim_test(g, p, o, d, &pre, meps, &t)) continue;
                     int accept;
                     if (mode == 0)
                         accept = t >= tmin && (have ? (t < best_t || (t == best_t && p < best_p)) : t < tmax);
@@ -653,7 +768,7 @@ done:
 
 /* Full hit record: triangles give barycentrics and edge flags, objects
  * u = v = 0 and ORC_HIT_INSIDE for a hit from inside. */
-static void hit_record(const orc_geom* g, const float o[3], const float d[3], uint32_t p,
+static void hit_record(const orc_geom* g, const float o[3], const float d[3], float meps, uint32_t p,
                        float* t, float* u, float* v, uint32_t* flags)
 {
     *u = 0.0f; *v = 0.0f; *flags = 0u;
@@ -664,12 +779,13 @@ static void hit_record(const orc_geom* g, const float o[3], const float d[3], ui
         tri_test_pre(o, &pre, a, b, c, t, u, v, flags);
     } else {
         int inside = 0;
-        orc_object_test(&g->objs[p - g->ntri], o, d, t, &inside);
+        orc_object_test_eps(&g->objs[p - g->ntri], o, d, meps, t, &inside);
         if (inside) *flags = ORC_HIT_INSIDE;
     }
 }
 
-static void finish_hit(const orc_geom* g, const float o[3], const float d[3], uint32_t p, orc_hits hits, size_t i)
+static void finish_hit(const orc_geom* g, const float o[3], const float d[3], float meps, uint32_t p,
+                       orc_hits hits, size_t i)
 {
     hits.prim[i] = p;
     if (p == ORC_MISS) {
@@ -678,7 +794,7 @@ static void finish_hit(const orc_geom* g, const float o[3], const float d[3], ui
     }
     float t = 0.0f, u = 0.0f, v = 0.0f;
     uint32_t f = 0;
-    hit_record(g, o, d, p, &t, &u, &v, &f);
+    hit_record(g, o, d, meps, p, &t, &u, &v, &f);
     hits.t[i] = t; hits.u[i] = u; hits.v[i] = v; hits.flags[i] = f;
 }
 
@@ -707,13 +823,13 @@ void orc_trace_closest_g(const orc_node* nodes, int node_count, const int32_t* i
 #pragma omp parallel for schedule(dynamic, 1024) num_threads(nt)
     for (long long ii = 0; ii < (long long)n; ++ii) {
         const size_t i = (size_t)ii;
-        float o[3], d[3], tmin, tmax, t;
+        float o[3], d[3], tmin, tmax, meps, t;
         uint32_t p, nb, ntr;
-        load_ray(rays, i, o, d, &tmin, &tmax);
-        traverse_one(nodes, node_count, indices, g, o, d, tmin, tmax, mode, 0, &p, &t, &nb, &ntr);
+        load_ray(rays, i, o, d, &tmin, &tmax, &meps);
+        traverse_one(nodes, node_count, indices, g, o, d, tmin, tmax, meps, mode, 0, &p, &t, &nb, &ntr);
         if (boxes) boxes[i] = nb;
         if (tris) tris[i] = ntr;
-        finish_hit(g, o, d, p, hits, i);
+        finish_hit(g, o, d, meps, p, hits, i);
     }
 }
 
@@ -730,10 +846,10 @@ void orc_trace_any_g(const orc_node* nodes, int node_count, const int32_t* indic
         for (int b = 0; b < 32; ++b) {
             const size_t i = (size_t)w * 32 + (size_t)b;
             if (i >= n) break;
-            float o[3], d[3], tmin, tmax, t;
+            float o[3], d[3], tmin, tmax, meps, t;
             uint32_t p, nb, ntr;
-            load_ray(rays, i, o, d, &tmin, &tmax);
-            if (traverse_one(nodes, node_count, indices, g, o, d, tmin, tmax, mode, 1, &p, &t, &nb, &ntr))
+            load_ray(rays, i, o, d, &tmin, &tmax, &meps);
+            if (traverse_one(nodes, node_count, indices, g, o, d, tmin, tmax, meps, mode, 1, &p, &t, &nb, &ntr))
                 word |= 1u << b;
             if (boxes) boxes[i] = nb;
             if (tris) tris[i] = ntr;
@@ -750,20 +866,20 @@ void orc_trace_brute_g(const orc_geom* g, orc_rays rays, size_t n, orc_hits hits
 #pragma omp parallel for schedule(dynamic, 16) num_threads(nt)
     for (long long ii = 0; ii < (long long)n; ++ii) {
         const size_t i = (size_t)ii;
-        float o[3], d[3], tmin, tmax;
-        load_ray(rays, i, o, d, &tmin, &tmax);
+        float o[3], d[3], tmin, tmax, meps;
+        load_ray(rays, i, o, d, &tmin, &tmax, &meps);
         const ray_pre pre = ray_precompute(d);
         int have = 0;
         uint32_t best_p = ORC_MISS;
         float best_t = tmax;
         for (uint32_t p = 0; p < np; ++p) {
             float t;
-            if (!prim_test(g, p, o, d, &pre, &t)) continue;
+            if (!prim_test(g, p, o, d, &pre, meps, &t)) continue;
             if (t >= tmin && (have ? t < best_t : t < tmax)) {   /* p ascends: ties keep the smaller id */
                 have = 1; best_t = t; best_p = p;
             }
         }
-        finish_hit(g, o, d, have ? best_p : ORC_MISS, hits, i);
+        finish_hit(g, o, d, meps, have ? best_p : ORC_MISS, hits, i);
     }
 }
 
@@ -779,12 +895,12 @@ void orc_trace_any_brute_g(const orc_geom* g, orc_rays rays, size_t n, uint32_t*
         for (int b = 0; b < 32; ++b) {
             const size_t i = (size_t)w * 32 + (size_t)b;
             if (i >= n) break;
-            float o[3], d[3], tmin, tmax;
-            load_ray(rays, i, o, d, &tmin, &tmax);
+            float o[3], d[3], tmin, tmax, meps;
+            load_ray(rays, i, o, d, &tmin, &tmax, &meps);
             const ray_pre pre = ray_precompute(d);
             for (uint32_t p = 0; p < np; ++p) {
                 float t;
-                if (prim_test(g, p, o, d, &pre, &t) && t >= tmin && t < tmax) {
+                if (prim_test(g, p, o, d, &pre, meps, &t) && t >= tmin && t < tmax) {
                     word |= 1u << b;
                     break;
                 }
@@ -828,26 +944,40 @@ void orc_trace_any_brute(const float* verts, const uint32_t* idx, int ntri,
 }
 
 /* Single-ray entry points used by the oracle integrator (render.c). */
+int orc_closest1_ge(const orc_node* nodes, int node_count, const int32_t* indices, const orc_geom* g,
+                    const float o[3], const float d[3], float tmin, float tmax, float meps,
+                    uint32_t* prim, float* t, float* u, float* v, uint32_t* flags)
+{
+    float tt;
+    uint32_t p;
+    const int hit = traverse_one(nodes, node_count, indices, g, o, d, tmin, tmax, meps, 0, 0, &p, &tt, NULL, NULL);
+    *prim = p;
+    *t = tt;
+    *flags = 0u;
+    if (hit) hit_record(g, o, d, meps, p, t, u, v, flags);
+    return hit;
+}
+
+int orc_any1_ge(const orc_node* nodes, int node_count, const int32_t* indices, const orc_geom* g,
+                const float o[3], const float d[3], float tmin, float tmax, float meps)
+{
+    float tt;
+    uint32_t p;
+    return traverse_one(nodes, node_count, indices, g, o, d, tmin, tmax, meps, 0, 1, &p, &tt, NULL, NULL);
+}
+
 int orc_closest1_g(const orc_node* nodes, int node_count, const int32_t* indices, const orc_geom* g,
                    const float o[3], const float d[3], float tmin, float tmax,
                    uint32_t* prim, float* t, float* u, float* v, uint32_t* flags)
 {
-    float tt;
-    uint32_t p;
-    const int hit = traverse_one(nodes, node_count, indices, g, o, d, tmin, tmax, 0, 0, &p, &tt, NULL, NULL);
-    *prim = p;
-    *t = tt;
-    *flags = 0u;
-    if (hit) hit_record(g, o, d, p, t, u, v, flags);
-    return hit;
+    return orc_closest1_ge(nodes, node_count, indices, g, o, d, tmin, tmax, default_meps(tmin), prim, t, u, v,
+                           flags);
 }
 
 int orc_any1_g(const orc_node* nodes, int node_count, const int32_t* indices, const orc_geom* g,
                const float o[3], const float d[3], float tmin, float tmax)
 {
-    float tt;
-    uint32_t p;
-    return traverse_one(nodes, node_count, indices, g, o, d, tmin, tmax, 0, 1, &p, &tt, NULL, NULL);
+    return orc_any1_ge(nodes, node_count, indices, g, o, d, tmin, tmax, default_meps(tmin));
 }
 
 int orc_closest1(const orc_node* nodes, int node_count, const int32_t* indices, const float* verts,

@@ -22,7 +22,7 @@ class orc_node(C.Structure):
 
 
 class orc_rays(C.Structure):
-    _fields_ = [(k, C.c_void_p) for k in ("ox", "oy", "oz", "tmin", "dx", "dy", "dz", "tmax")]
+    _fields_ = [(k, C.c_void_p) for k in ("ox", "oy", "oz", "tmin", "dx", "dy", "dz", "tmax", "meps")]
 
 
 class orc_hits(C.Structure):
@@ -39,7 +39,7 @@ def load():
     global _lib
     if _lib is not None:
         return _lib
-    srcs = [ORACLE_DIR / n for n in ("trace.c", "render.c", "render_impl.h", "guide.c", "mlp.c", "output.c", "oracle.h")]
+    srcs = [ORACLE_DIR / n for n in ("trace.c", "render.c", "render_impl.h", "guide.c", "mlp.c", "output.c", "oracle.h", "sdf_impl.h")]
     if not LIB.exists() or any(s.stat().st_mtime > LIB.stat().st_mtime for s in srcs if s.exists()):
         subprocess.run(["make", "-s", "-C", str(ORACLE_DIR)], check=True)
     lib = C.CDLL(str(LIB))
@@ -153,9 +153,12 @@ def alloc_hits(n):
 
 
 def _rays(r):
+    """orc_rays of an (8, n) or (9, n) float32 array; row 8, when present,
+    is the per-ray SDF march tolerance."""
     o = orc_rays()
     for k, name in enumerate(("ox", "oy", "oz", "tmin", "dx", "dy", "dz", "tmax")):
         setattr(o, name, r[k].ctypes.data)
+    o.meps = r[8].ctypes.data if r.shape[0] > 8 else None
     return o
 
 
@@ -178,7 +181,13 @@ OBJ_BOX, OBJ_SPHERE, HIT_INSIDE = 0, 1, 16
 
 
 class orc_object(C.Structure):
-    _fields_ = [("type", C.c_uint32), ("center", C.c_float * 3), ("extent", C.c_float * 3), ("w2o", C.c_float * 9)]
+    _fields_ = [("type", C.c_uint32), ("center", C.c_float * 3), ("extent", C.c_float * 3), ("w2o", C.c_float * 9),
+                ("sdf", C.c_float * 4), ("vol", C.c_void_p), ("dims", C.c_uint32 * 3), ("vmax", C.c_float)]
+
+
+OBJ_SDF_TORUS, OBJ_SDF_VOLUME = 2, 3
+OBJ_TYPES = {"box": 0, "sphere": 1, "torus": 2, "sdf_torus": 2, "volume": 3, "sdf_volume": 3}
+TORUS_DEFAULT = (0.375, 0.125, 0.0, 0.0)    # CreateTorus(scale 1, R 0.375, r 0.125)
 
 
 class orc_geom(C.Structure):
@@ -187,20 +196,41 @@ class orc_geom(C.Structure):
 
 
 def make_objects(objects):
-    """orc_object array from dicts (type 'box' / 'sphere', center, extent,
-    rotation in degrees), initialised by orc_object_init."""
+    """orc_object array from dicts (type 'box' / 'sphere' / 'torus' /
+    'volume', center, extent, rotation in degrees; a torus's 'sdf' radii, a
+    volume's 'volume_data' as a (nz, ny, nx) float32 array), initialised by
+    orc_object_init and orc_object_set_sdf.  The array keeps the volumes alive."""
     lib = load()
     f = fn("orc_object_init", None, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p])
+    fs = fn("orc_object_set_sdf", None, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p])
     arr = (orc_object * max(1, len(objects)))()
+    keep = []
     for k, o in enumerate(objects):
         t = o.get("type", "box")
-        t = {"box": OBJ_BOX, "sphere": OBJ_SPHERE}.get(t, t)
+        t = OBJ_TYPES.get(t, t)
         c = np.asarray(o["center"], np.float32)
         e = np.asarray(o["extent"], np.float32)
         r = np.asarray(o.get("rotation", (0, 0, 0)), np.float32)
         f(C.byref(arr[k]), int(t), c.ctypes.data, e.ctypes.data, r.ctypes.data)
+        if t in (OBJ_SDF_TORUS, OBJ_SDF_VOLUME):
+            prm = np.zeros(4, np.float32)
+            sp = o.get("sdf", TORUS_DEFAULT if t == OBJ_SDF_TORUS else (0, 0, 0, 0))
+            prm[:len(sp)] = sp
+            vol = dims = None
+            if t == OBJ_SDF_VOLUME:
+                vd = np.ascontiguousarray(o["volume_data"], np.float32)
+                keep.append(vd)
+                vol = vd.ctypes.data
+                dims = np.array(vd.shape[::-1], np.uint32)
+                keep.append(dims)
+                dims = dims.ctypes.data
+            fs(C.byref(arr[k]), prm.ctypes.data, vol, dims)
     del lib
+    arr_keep[id(arr)] = keep
     return arr
+
+
+arr_keep = {}   # volumes referenced by live orc_object arrays (by id; entries are small and tests are short)
 
 
 class Geom:
@@ -262,26 +292,43 @@ class Geom:
         return unpack_bits(occ, n)
 
 
-def object_test(obj, o, d):
-    """(hit, t, inside) of one orc_object for one ray."""
-    f = fn("orc_object_test", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_float),
-                                        C.POINTER(C.c_int)])
+def object_test(obj, o, d, meps=5e-4):
+    """(hit, t, inside) of one orc_object for one ray (meps: SDF march tolerance)."""
+    f = fn("orc_object_test_eps", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_float, C.POINTER(C.c_float),
+                                            C.POINTER(C.c_int)])
     o = np.ascontiguousarray(o, np.float32)
     d = np.ascontiguousarray(d, np.float32)
     t, ins = C.c_float(0), C.c_int(0)
-    h = f(C.byref(obj), o.ctypes.data, d.ctypes.data, C.byref(t), C.byref(ins))
+    h = f(C.byref(obj), o.ctypes.data, d.ctypes.data, meps, C.byref(t), C.byref(ins))
     return bool(h), t.value, bool(ins.value)
 
 
-def object_surface(obj, o, d, t, inside):
-    f = fn("orc_object_surface", None, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_float, C.c_int, C.c_void_p,
-                                        C.c_void_p])
+def object_surface(obj, o, d, t, inside, meps=5e-4):
+    f = fn("orc_object_surface_eps", None, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_float, C.c_int, C.c_float,
+                                            C.c_void_p, C.c_void_p])
     o = np.ascontiguousarray(o, np.float32)
     d = np.ascontiguousarray(d, np.float32)
     pos = np.zeros(3, np.float32)
     nrm = np.zeros(3, np.float32)
-    f(C.byref(obj), o.ctypes.data, d.ctypes.data, t, int(inside), pos.ctypes.data, nrm.ctypes.data)
+    f(C.byref(obj), o.ctypes.data, d.ctypes.data, t, int(inside), meps, pos.ctypes.data, nrm.ctypes.data)
     return pos, nrm
+
+
+def sdf_bake(shape, params, dims):
+    """orc_sdf_bake: (nz, ny, nx) float32 grid of shape 'torus' (0) or 'diamond' (1)."""
+    shape = {"torus": 0, "diamond": 1}.get(shape, shape)
+    prm = np.zeros(4, np.float32)
+    prm[:len(params)] = params
+    d = np.asarray(dims, np.uint32)
+    out = np.zeros((int(d[2]), int(d[1]), int(d[0])), np.float32)
+    fn("orc_sdf_bake", None, [C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p])(int(shape), prm.ctypes.data,
+                                                                             d.ctypes.data, out.ctypes.data)
+    return out
+
+
+def sdf_value(obj, u):
+    u = np.ascontiguousarray(u, np.float32)
+    return fn("orc_sdf_value", C.c_float, [C.c_void_p, C.c_void_p])(C.byref(obj), u.ctypes.data)
 
 
 # ---------------------------------------------------------------- integrator
