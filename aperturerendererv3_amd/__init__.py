@@ -17,7 +17,7 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import (AR_TONE_ACES, AR_TONE_GAMMA, AR_TONE_LINEAR, AR_HIT_INSIDE, AR_MISS, AR_MODEL_GGX, AR_MODEL_GLASS, AR_MODEL_LAMBERT, AR_OBJ_BOX, AR_OBJ_SPHERE, AR_OK, AR_SKY_CONSTANT, AR_SKY_GRADIENT, AR_SKY_NONE,
+from ._lib import (AR_OBJ_SDF_TORUS, AR_OBJ_SDF_VOLUME, AR_TONE_ACES, AR_TONE_GAMMA, AR_TONE_LINEAR, AR_HIT_INSIDE, AR_MISS, AR_MODEL_GGX, AR_MODEL_GLASS, AR_MODEL_LAMBERT, AR_OBJ_BOX, AR_OBJ_SPHERE, AR_OK, AR_SKY_CONSTANT, AR_SKY_GRADIENT, AR_SKY_NONE,
                    ar_bvh_info, ar_bvh_opts, ar_camera, ar_device_opts, ar_hits, ar_material, ar_mesh, ar_rays,
                    ar_render_opts)
 
@@ -58,16 +58,25 @@ class Material:
         return m
 
 
-_OBJ_TYPES = {"box": AR_OBJ_BOX, "sphere": AR_OBJ_SPHERE, AR_OBJ_BOX: AR_OBJ_BOX, AR_OBJ_SPHERE: AR_OBJ_SPHERE}
+_OBJ_TYPES = {"box": AR_OBJ_BOX, "sphere": AR_OBJ_SPHERE, "torus": _lib.AR_OBJ_SDF_TORUS,
+              "sdf_torus": _lib.AR_OBJ_SDF_TORUS, "volume": _lib.AR_OBJ_SDF_VOLUME,
+              "sdf_volume": _lib.AR_OBJ_SDF_VOLUME}
+_OBJ_TYPES.update({v: v for v in list(_OBJ_TYPES.values())})
+TORUS_DEFAULT = (0.375, 0.125)   # CreateTorus(scale 1): ring and tube radius in unit space
 
 
 def object_array(objects):
-    """ctypes ar_object array from dicts: type ('box' / 'sphere'), center,
-    extent (half extents; sphere radius = max), rotation (Euler degrees,
-    objectToWorld = Rz Ry Rx), material (Material or its keyword dict)."""
+    """ctypes ar_object array from dicts: type ('box' / 'sphere' / 'torus' /
+    'volume'), center, extent (half extents; sphere radius = max), rotation
+    (Euler degrees, objectToWorld = Rz Ry Rx), material (Material or its
+    keyword dict); a torus's 'sdf' radii (default 0.375, 0.125), a volume's
+    'volume' index."""
     arr = (_lib.ar_object * max(1, len(objects)))()
     for k, o in enumerate(objects):
         arr[k].type = _OBJ_TYPES[o.get("type", "box")]
+        sdf = list(o.get("sdf", TORUS_DEFAULT if arr[k].type == _lib.AR_OBJ_SDF_TORUS else ()))
+        arr[k].sdf[:] = [float(x) for x in (sdf + [0.0] * 4)[:4]]
+        arr[k].sdf_volume = int(o.get("volume", -1))
         arr[k].center[:] = [float(x) for x in o["center"]]
         arr[k].extent[:] = [float(x) for x in o["extent"]]
         arr[k].rotation[:] = [float(x) for x in o.get("rotation", (0.0, 0.0, 0.0))]
@@ -334,16 +343,49 @@ class Context:
         self._check(self._lib.ar_bvh_info_get(self._h, C.byref(info)), "ar_bvh_info_get")
         return {k: getattr(info, k) for k, _ in info._fields_}
 
+    # -- signed-distance volumes ----------------------------------------------
+    def sdf_volume_bake(self, shape, dims, params=(0.0, 0.0, 0.0, 0.0)) -> int:
+        """ar_sdf_volume_bake: shape 'torus' (params R, r) or 'diamond'; dims (nx, ny, nz)."""
+        shape = {"torus": _lib.AR_SDF_SHAPE_TORUS, "diamond": _lib.AR_SDF_SHAPE_DIAMOND}.get(shape, shape)
+        prm = (C.c_float * 4)(*[float(x) for x in (list(params) + [0.0] * 4)[:4]])
+        dm = (C.c_uint32 * 3)(*[int(x) for x in dims])
+        idx = C.c_uint32()
+        self._check(self._lib.ar_sdf_volume_bake(self._h, int(shape), prm, dm, C.byref(idx)), "ar_sdf_volume_bake")
+        return idx.value
+
+    def sdf_volume_add(self, voxels) -> int:
+        """ar_sdf_volume_add of a (nz, ny, nx) float32 array."""
+        import numpy as np
+        v = np.ascontiguousarray(voxels, np.float32)
+        dm = (C.c_uint32 * 3)(v.shape[2], v.shape[1], v.shape[0])
+        idx = C.c_uint32()
+        self._check(self._lib.ar_sdf_volume_add(self._h, v.ctypes.data, dm, C.byref(idx)), "ar_sdf_volume_add")
+        return idx.value
+
+    def sdf_volume_read(self, index):
+        """ar_sdf_volume_read: the volume as a (nz, ny, nx) float32 array."""
+        import numpy as np
+        dm = (C.c_uint32 * 3)()
+        self._check(self._lib.ar_sdf_volume_read(self._h, int(index), None, dm), "ar_sdf_volume_read")
+        out = np.zeros((dm[2], dm[1], dm[0]), np.float32)
+        self._check(self._lib.ar_sdf_volume_read(self._h, int(index), out.ctypes.data, None), "ar_sdf_volume_read")
+        return out
+
+    def sdf_volume_clear(self):
+        self._check(self._lib.ar_sdf_volume_clear(self._h), "ar_sdf_volume_clear")
+
     # -- traversal -----------------------------------------------------------
     @staticmethod
     def _rays_c(rays) -> ar_rays:
-        """rays: float32 CUDA tensor of shape (8, n) with rows RAY_FIELDS."""
-        assert rays.dim() == 2 and rays.shape[0] == 8 and rays.is_contiguous() and rays.is_cuda
+        """rays: float32 CUDA tensor of shape (8, n) with rows RAY_FIELDS, or
+        (9, n) with the per-ray SDF march tolerance in row 8."""
+        assert rays.dim() == 2 and rays.shape[0] in (8, 9) and rays.is_contiguous() and rays.is_cuda
         import torch
         assert rays.dtype == torch.float32
         r = ar_rays()
         for k, name in enumerate(RAY_FIELDS):
             setattr(r, name, _ptr(rays[k]))
+        r.meps = _ptr(rays[8]) if rays.shape[0] == 9 else None
         return r
 
     @staticmethod

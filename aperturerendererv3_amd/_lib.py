@@ -24,6 +24,10 @@ AR_MODEL_LAMBERT = 0
 AR_MODEL_GGX = 1
 AR_MODEL_GLASS = 2
 AR_OBJ_BOX = 0
+AR_OBJ_SDF_TORUS = 2
+AR_OBJ_SDF_VOLUME = 3
+AR_SDF_SHAPE_TORUS = 0
+AR_SDF_SHAPE_DIAMOND = 1
 AR_TONE_LINEAR = 0
 AR_TONE_GAMMA = 1
 AR_TONE_ACES = 2
@@ -46,7 +50,8 @@ class ar_material(C.Structure):
 
 class ar_object(C.Structure):
     _fields_ = [("type", C.c_uint32), ("center", C.c_float * 3), ("extent", C.c_float * 3),
-                ("rotation", C.c_float * 3), ("material", ar_material)]
+                ("rotation", C.c_float * 3), ("material", ar_material), ("sdf", C.c_float * 4),
+                ("sdf_volume", C.c_int32)]
 
 
 class ar_mesh(C.Structure):
@@ -55,7 +60,7 @@ class ar_mesh(C.Structure):
 
 
 class ar_rays(C.Structure):
-    _fields_ = [(k, C.c_void_p) for k in ("ox", "oy", "oz", "tmin", "dx", "dy", "dz", "tmax")]
+    _fields_ = [(k, C.c_void_p) for k in ("ox", "oy", "oz", "tmin", "dx", "dy", "dz", "tmax", "meps")]
 
 
 class ar_hits(C.Structure):
@@ -138,6 +143,10 @@ SIGNATURES = {
                              C.c_uint32, C.c_void_p]),
     "ar_luminance_mse": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.POINTER(C.c_double), C.c_void_p]),
     "ar_write_bmp": (C.c_int, [C.c_char_p, C.c_void_p, C.c_uint32, C.c_uint32]),
+    "ar_sdf_volume_bake": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.POINTER(C.c_uint32)]),
+    "ar_sdf_volume_add": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_uint32)]),
+    "ar_sdf_volume_read": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]),
+    "ar_sdf_volume_clear": (C.c_int, [C.c_void_p]),
     "ar_spectrum_lut_build": (C.c_int, [C.c_void_p, C.c_size_t]),
     "ar_guiding_configure": (C.c_int, [C.c_void_p, C.c_uint64]),
     "ar_batch_create": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_void_p)]),

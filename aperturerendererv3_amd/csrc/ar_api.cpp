@@ -140,6 +140,7 @@ int trace_prologue(ar_ctx* ctx, TraceArgs& ta, size_t n, uint32_t& blocks) {
     ta.fetch = ctx->fetch;
     ta.objs = ctx->scene.nobj ? ctx->scene.objs : nullptr;
     ta.obj_base = (uint32_t)(ctx->idx.size() / 3);
+    ta.vol = ctx->sdf.dev;
     return AR_OK;
 }
 
@@ -199,6 +200,7 @@ int ar_destroy(ar_ctx* ctx) {
     if (ctx->spill) (void)hipFree(ctx->spill);
     if (ctx->fetch) (void)hipFree(ctx->fetch);
     if (ctx->reduce) (void)hipFree(ctx->reduce);
+    sdf_free(ctx);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return AR_OK;
@@ -279,13 +281,20 @@ int ar_scene_set_objects(ar_ctx* ctx, const ar_object* objects, size_t nobj) {
     for (size_t k = 0; k < nobj; ++k) {
         const ar_object& o = objects[k];
         const std::string who = "object " + std::to_string(k);
-        if (o.type != AR_OBJ_BOX && o.type != AR_OBJ_SPHERE) return set_error(ctx, AR_E_INVALID, who + ": unknown type");
+        if (o.type > AR_OBJ_SDF_VOLUME) return set_error(ctx, AR_E_INVALID, who + ": unknown type");
         for (int a = 0; a < 3; ++a)
             if (!std::isfinite(o.center[a]) || !std::isfinite(o.extent[a]) || !std::isfinite(o.rotation[a]))
                 return set_error(ctx, AR_E_INVALID, who + ": non-finite center, extent or rotation");
         const float ex = std::fabs(o.extent[0]), ey = std::fabs(o.extent[1]), ez = std::fabs(o.extent[2]);
-        if (o.type == AR_OBJ_BOX ? !(ex > 0.0f && ey > 0.0f && ez > 0.0f) : !(std::max(ex, std::max(ey, ez)) > 0.0f))
+        if (o.type != AR_OBJ_SPHERE ? !(ex > 0.0f && ey > 0.0f && ez > 0.0f) : !(std::max(ex, std::max(ey, ez)) > 0.0f))
             return set_error(ctx, AR_E_INVALID, who + ": empty extent");
+        if (o.type == AR_OBJ_SDF_TORUS && !(std::isfinite(o.sdf[0]) && std::isfinite(o.sdf[1])))
+            return set_error(ctx, AR_E_INVALID, who + ": non-finite torus radii");
+        if (o.type == AR_OBJ_SDF_VOLUME && (o.sdf_volume < 0 || (size_t)o.sdf_volume >= ctx->sdf.off.size()))
+            return set_error(ctx, AR_E_INVALID, who + ": no such SDF volume");
+        if ((o.type == AR_OBJ_SDF_TORUS || o.type == AR_OBJ_SDF_VOLUME) &&
+            (o.material.emission[0] > 0.0f || o.material.emission[1] > 0.0f || o.material.emission[2] > 0.0f))
+            return set_error(ctx, AR_E_UNSUPPORTED, who + ": SDF objects cannot be lights");
         if (o.material.model > AR_MODEL_GLASS) return set_error(ctx, AR_E_INVALID, who + ": unknown material model");
         if (o.material.model == AR_MODEL_GLASS && !(o.material.ior > 0.0f))
             return set_error(ctx, AR_E_INVALID, who + ": glass needs ior > 0");

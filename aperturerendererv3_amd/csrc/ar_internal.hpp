@@ -169,9 +169,21 @@ struct SceneDev {
     bool spectral_ready = false;
     float4* objs = nullptr;               // analytic objects, obj::kRecord float4 each (objects.hpp)
     uint32_t nobj = 0;
+    uint32_t nsdf = 0;                    // SDF objects among them
     std::vector<float4> host_objs;
     float box_lo[3] = {0, 0, 0};          // scene bounds (guiding-network inputs)
     float box_inv[3] = {1, 1, 1};         // 1 / extent
+};
+
+// Signed-distance volumes: every volume's voxels back to back (host copy,
+// the source of truth), uploaded to `dev` when the scene is uploaded.
+struct SdfPool {
+    std::vector<float> host;
+    std::vector<uint32_t> off;        // first voxel of each volume
+    std::vector<uint32_t> dims;       // 3 per volume
+    float* dev = nullptr;
+    size_t dev_floats = 0;
+    bool dirty = false;               // host changed since the last upload
 };
 
 // Wavefront state of the integrator, sized for `capacity` paths.
@@ -182,6 +194,7 @@ struct RenderBuffers {
     uint32_t* qpath[2];           // queue slot -> path id
     uint32_t* hit_prim; float* hit_t; float* hit_u; float* hit_v; uint32_t* hit_flags;
     float* srays[8];              // shadow rays
+    float* smeps;                 // shadow ray SDF march tolerance (scenes with SDF objects)
     uint32_t* spath;
     float* scontrib[4];
     uint32_t* sflags;             // bit0: the sampled light point faced the surface (both cosines > 0)
@@ -322,6 +335,7 @@ struct ar_ctx {
     uint32_t tile = 256;                  // threads per block of the wavefront kernels
     uint64_t buffer_generation = 0;       // bumped whenever render buffers are reallocated
     double* reduce = nullptr;             // partial sums of ar_luminance_mse
+    ar::SdfPool sdf;                      // signed-distance volumes
 };
 
 namespace ar {
@@ -343,6 +357,7 @@ struct TraceArgs {
     const uint32_t* roots = nullptr;   // per-ray root node (batched scenes), else `root`
     const float4* objs = nullptr;      // analytic object records; NULL: triangle-only scene
     uint32_t obj_base = 0;             // prim id of object 0 (= triangle count)
+    const float* vol = nullptr;        // SDF voxel pool (objects.hpp)
 };
 // n_dev (optional): device pointer to the ray count, read by the kernel
 // (then n is only the capacity used to size the grid).
@@ -357,6 +372,9 @@ int trace_blocks_per_cu(int width = 0, bool any = false);
 
 // integrator.hip
 int upload_scene(ar_ctx* ctx);
+// sdf.hip: uploads the voxel pool when it changed
+int sdf_upload(ar_ctx* ctx);
+void sdf_free(ar_ctx* ctx);
 void free_scene(ar_ctx* ctx);
 void free_render_buffers(ar_ctx* ctx);
 int render(ar_ctx* ctx, const ar_camera* cam, const ar_render_opts* opts, float* out, hipStream_t s);

@@ -576,6 +576,7 @@ __device__ __forceinline__ uint32_t segment_of(const Segment* segs, uint32_t nse
 struct QueueRays {
     float *ox, *oy, *oz, *tmin, *dx, *dy, *dz, *tmax;
     uint32_t* root;              // BVH root of the ray's scene
+    float* meps;                 // SDF march tolerance (shadow queue of scenes with SDF objects; else null)
 };
 
 struct SceneArgs {
@@ -589,6 +590,8 @@ struct SceneArgs {
     const float* __restrict__ light_cdf;
     const float* __restrict__ cie;
     const float* __restrict__ d65;
+    const float* __restrict__ vol;     // SDF voxel pool
+    uint32_t sdf;                      // the scene has SDF objects
 };
 
 struct PathArgs {
@@ -780,7 +783,8 @@ __global__ __launch_bounds__(kMaxBlock) void k_camera(RenderParams P, PathArgs S
 // the hit point and the shading normal facing the ray, as k_shade computes
 // them.
 __global__ __launch_bounds__(kMaxBlock) void k_mlp_features(const float4* __restrict__ prims,
-                                                         const float4* __restrict__ objs, uint32_t ntri, QueueRays cur,
+                                                         const float4* __restrict__ objs, const float* __restrict__ vol,
+                                                         uint32_t ntri, QueueRays cur,
                                                          const uint32_t* __restrict__ hprim,
                                                          const float* __restrict__ ht, const float* __restrict__ hu,
                                                          const float* __restrict__ hv,
@@ -795,7 +799,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_mlp_features(const float4* __rest
     if (prim != AR_MISS && prim >= ntri) {
         float pp[3], np[3];
         obj::surface(objs + (size_t)obj::kRecord * (prim - ntri), cur.ox[q], cur.oy[q], cur.oz[q], cur.dx[q], cur.dy[q],
-                     cur.dz[q], ht[q], (hflags[q] & AR_HIT_INSIDE) != 0u, pp, np);
+                     cur.dz[q], ht[q], (hflags[q] & AR_HIT_INSIDE) != 0u, 0.5f * cur.tmin[q], vol, pp, np);
         const float lo[3] = {lo0, lo1, lo2}, inv[3] = {inv0, inv1, inv2};
         mlp::features(pp[0], pp[1], pp[2], np[0], np[1], np[2], lo, inv, x);
     } else if (prim != AR_MISS) {
@@ -914,9 +918,12 @@ __device__ __forceinline__ void shade_one(
             int light_id;
             float ndd;
             if (is_obj) {
-                // the intersector's normal: outward, or inward for a hit from inside
+                // the intersector's normal: outward, or inward for a box or sphere hit
+                // from inside; an SDF normal is the outward gradient, its
+                // difference step the march tolerance of the ray (SDF_EPS of its depth)
                 float pp[3], np[3];
-                obj::surface(ob, o.x, o.y, o.z, d.x, d.y, d.z, ht[q], (hflags[q] & AR_HIT_INSIDE) != 0u, pp, np);
+                obj::surface(ob, o.x, o.y, o.z, d.x, d.y, d.z, ht[q], (hflags[q] & AR_HIT_INSIDE) != 0u,
+                             0.5f * cur.tmin[q], sc.vol, pp, np);
                 pos = f3(pp[0], pp[1], pp[2]);
                 ng = f3(np[0], np[1], np[2]);
                 nn = ng;
@@ -1259,6 +1266,8 @@ __device__ __forceinline__ void shade_one(
     if (shadow) {
         sh.ox[ss] = s_o.x; sh.oy[ss] = s_o.y; sh.oz[ss] = s_o.z; sh.tmin[ss] = 0.0f;
         sh.dx[ss] = s_d.x; sh.dy[ss] = s_d.y; sh.dz[ss] = s_d.z; sh.tmax[ss] = s_tmax;
+        // SegmentTrace(depth, ...): SDF_EPS of the bounce index
+        if (sc.sdf) sh.meps[ss] = 0.5f * eps_table[depth < 255 ? depth : 255];
         spath[ss] = p;
         sh.root[ss] = root;
         sc0[ss] = s_contrib.v[0]; sc1[ss] = s_contrib.v[1]; sc2[ss] = s_contrib.v[2]; sc3[ss] = s_contrib.v[3];
@@ -1484,6 +1493,7 @@ int upload_scene(ar_ctx* ctx) {
     }
     // analytic objects: materials after the mesh materials, lights after the triangles
     const uint32_t nobj = (uint32_t)ctx->objs.size();
+    uint32_t nsdf = 0;
     std::vector<float4> objs((size_t)obj::kRecord * std::max(nobj, 1u), make_float4(0, 0, 0, 0));
     for (uint32_t k = 0; k < nobj; ++k) {
         const ar_object& ob = ctx->objs[k];
@@ -1517,6 +1527,24 @@ int upload_scene(ar_ctx* ctx) {
         std::memcpy(&r[3].w, &light, 4);
         r[4] = make_float4(bs.w2o[6], bs.w2o[7], bs.w2o[8], 0.0f);
         std::memcpy(&r[4].w, &solo, 4);
+        r[5] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (ob.type == AR_OBJ_SDF_TORUS) {
+            r[5] = make_float4(ob.sdf[0], ob.sdf[1], 0.0f, 0.0f);
+            nsdf += 1;
+        } else if (ob.type == AR_OBJ_SDF_VOLUME) {
+            // validated by ar_scene_set_objects
+            const uint32_t v = (uint32_t)ob.sdf_volume;
+            const uint32_t* dm = &ctx->sdf.dims[3 * (size_t)v];
+            const uint32_t nxy = dm[0] | (dm[1] << 16);
+            // voxelSizeMax_ = |1 / max(dims, 1)| * SDF_VOLUME_USE_NEAREST_SCALE (sdf_volume.cuh:23-45)
+            const float ix = 1.0f / std::fmax((float)dm[0], 1.0f), iy = 1.0f / std::fmax((float)dm[1], 1.0f),
+                        iz = 1.0f / std::fmax((float)dm[2], 1.0f);
+            r[5] = make_float4(0.0f, 0.0f, 0.0f, std::sqrt(ix * ix + iy * iy + iz * iz) * 2.5f);
+            std::memcpy(&r[5].x, &ctx->sdf.off[v], 4);
+            std::memcpy(&r[5].y, &nxy, 4);
+            std::memcpy(&r[5].z, &dm[2], 4);
+            nsdf += 1;
+        }
     }
     const bool any_prim = n + nobj > 0;
     for (int a = 0; a < 3; ++a) {
@@ -1541,6 +1569,7 @@ int upload_scene(ar_ctx* ctx) {
     }
     s.nlights = (uint32_t)ltri.size();
     s.nobj = nobj;
+    s.nsdf = nsdf;
     s.host_objs = objs;
     s.host_mats = mats;
     s.spectral_ready = false;
@@ -1558,6 +1587,7 @@ int upload_scene(ar_ctx* ctx) {
     if (rc == AR_OK) rc = up((void**)&s.light_prob, lprob.empty() ? nullptr : lprob.data(), lprob.size() * 4, "upload lights");
     if (rc == AR_OK) rc = up((void**)&s.light_cdf, lcdf.empty() ? nullptr : lcdf.data(), lcdf.size() * 4, "upload lights");
     if (rc == AR_OK) rc = up((void**)&s.objs, objs.data(), objs.size() * sizeof(float4), "upload objects");
+    if (rc == AR_OK) rc = sdf_upload(ctx);
     if (rc != AR_OK) free_scene(ctx);
     return rc;
 }
@@ -1595,8 +1625,8 @@ static int ensure_render_buffers(ar_ctx* ctx, size_t cap, size_t npix) {
     b.capacity = 0;
     // 4-byte arrays of cap entries: 2x8 queue rays, 2 qpath, 5 hits, 8 shadow rays,
     // spath, 4 scontrib, sflags, 4 mask, 4 rad, 3 last_pos, last_pdf, meta, lambda0,
-    // pix, smp, scn, 2 qroot, sroot, med, 3 mpos = 61
-    const size_t arrays = 61;
+    // pix, smp, scn, 2 qroot, sroot, med, 3 mpos, smeps = 62
+    const size_t arrays = 62;
     ctx->buffer_generation += 1;
     const size_t occ_words = (cap + 31) / 32 + 64;
     const size_t bytes = arrays * cap * 4 + occ_words * 4 + 64;
@@ -1631,6 +1661,7 @@ static int ensure_render_buffers(ar_ctx* ctx, size_t cap, size_t npix) {
     b.sroot = (uint32_t*)take(cap);
     b.med = (uint32_t*)take(cap);
     for (int k = 0; k < 3; ++k) b.mpos[k] = (float*)take(cap);
+    b.smeps = (float*)take(cap);
     b.occ = (uint32_t*)take(occ_words);
     b.counters = (uint32_t*)take(16);
     b.capacity = cap;
@@ -1697,7 +1728,7 @@ static void camera_medium(const ar_ctx* ctx, const float pos[3], SceneView& v) {
     const SceneDev& s = ctx->scene;
     for (uint32_t k = 0; k < s.nobj; ++k) {
         const float4* r = &s.host_objs[(size_t)obj::kRecord * k];
-        if (obj::overlap_host(r, pos)) {
+        if (obj::overlap_host(r, pos, ctx->sdf.host.data())) {
             v.medium0 = k + 1;
             std::memcpy(&v.root0, &r[4].w, 4);
             return;
@@ -1910,14 +1941,16 @@ static int enqueue_job(ar_ctx* ctx, RenderJob& J, hipStream_t stream) {
     PathArgs S{b.mask[0], b.mask[1], b.mask[2], b.mask[3], b.rad[0], b.rad[1], b.rad[2], b.rad[3],
                b.last_pos[0], b.last_pos[1], b.last_pos[2], b.last_pdf, b.meta, b.lambda0, b.pix, b.smp, b.scn,
                b.med, b.mpos[0], b.mpos[1], b.mpos[2]};
+    const bool sdf = ctx->scene.nsdf > 0;
     SceneArgs sc{ctx->scene.prims, ctx->scene.objs, ctx->scene.ntri, ctx->scene.mats, ctx->scene.light_tri,
-                 ctx->scene.light_area, ctx->scene.light_prob, ctx->scene.light_cdf, b.cie, b.d65};
+                 ctx->scene.light_area, ctx->scene.light_prob, ctx->scene.light_cdf, b.cie, b.d65, ctx->sdf.dev,
+                 sdf ? 1u : 0u};
     QueueRays q[2] = {{b.rays[0][0], b.rays[0][1], b.rays[0][2], b.rays[0][3], b.rays[0][4], b.rays[0][5], b.rays[0][6],
-                       b.rays[0][7], b.qroot[0]},
+                       b.rays[0][7], b.qroot[0], nullptr},
                       {b.rays[1][0], b.rays[1][1], b.rays[1][2], b.rays[1][3], b.rays[1][4], b.rays[1][5], b.rays[1][6],
-                       b.rays[1][7], b.qroot[1]}};
+                       b.rays[1][7], b.qroot[1], nullptr}};
     QueueRays shq{b.srays[0], b.srays[1], b.srays[2], b.srays[3], b.srays[4], b.srays[5], b.srays[6], b.srays[7],
-                  b.sroot};
+                  b.sroot, sdf ? b.smeps : nullptr};
     ar_hits hits{b.hit_prim, b.hit_t, b.hit_u, b.hit_v, b.hit_flags};
     GuideDev& gd = ctx->guide;
     GuideArgs G{};
@@ -1950,7 +1983,7 @@ static int enqueue_job(ar_ctx* ctx, RenderJob& J, hipStream_t stream) {
                            b.counters, b.eps_table);
         int c = 0;
         for (uint32_t depth = 0; depth < max_depth; ++depth) {
-            ar_rays rr{q[c].ox, q[c].oy, q[c].oz, q[c].tmin, q[c].dx, q[c].dy, q[c].dz, q[c].tmax};
+            ar_rays rr{q[c].ox, q[c].oy, q[c].oz, q[c].tmin, q[c].dx, q[c].dy, q[c].dz, q[c].tmax, nullptr};
             TraceArgs tq = ta;
             tq.roots = q[c].root;
             hipError_t e = launch_trace_closest(tq, rr, P.capacity, hits, nullptr, nullptr, tblocks, stream, &b.counters[0]);
@@ -1958,7 +1991,7 @@ static int enqueue_job(ar_ctx* ctx, RenderJob& J, hipStream_t stream) {
             if (J.learned && P.alpha > 0.0f) {
                 const SceneDev& sd = ctx->scene;
                 hipLaunchKernelGGL(k_mlp_features, dim3(grid_for(P.capacity, blk)), dim3(blk), 0, stream, sd.prims,
-                                   sd.objs, sd.ntri, q[c], b.hit_prim, b.hit_t, b.hit_u, b.hit_v, b.hit_flags,
+                                   sd.objs, ctx->sdf.dev, sd.ntri, q[c], b.hit_prim, b.hit_t, b.hit_u, b.hit_v, b.hit_flags,
                                    b.counters, sd.box_lo[0], sd.box_lo[1], sd.box_lo[2], sd.box_inv[0], sd.box_inv[1],
                                    sd.box_inv[2], ctx->mlp.feat);
                 e = mlp_forward(ctx, ctx->mlp.feat, P.capacity, &b.counters[0], ctx->mlp.dec, true, stream);
@@ -1974,7 +2007,7 @@ static int enqueue_job(ar_ctx* ctx, RenderJob& J, hipStream_t stream) {
             else AR_SHADE(AR_GUIDING_OFF);
 #undef AR_SHADE
             if (P.nee) {
-                ar_rays sr{shq.ox, shq.oy, shq.oz, shq.tmin, shq.dx, shq.dy, shq.dz, shq.tmax};
+                ar_rays sr{shq.ox, shq.oy, shq.oz, shq.tmin, shq.dx, shq.dy, shq.dz, shq.tmax, shq.meps};
                 TraceArgs ts = ta;
                 ts.roots = shq.root;
                 e = launch_trace_any(ts, sr, P.capacity, b.occ, tblocks, stream, &b.counters[2]);

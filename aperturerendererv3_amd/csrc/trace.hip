@@ -161,8 +161,10 @@ struct Stack {
 struct Trav {
     RayPre p;
     float tmin, tmax, best_t;
+    float meps;            // SDF march tolerance (object scenes)
     uint32_t ref, best_p, best_slot, idx;
     bool have;
+    bool best_in;          // the best hit is an object the ray started inside
 };
 
 __device__ __forceinline__ float byte_f(uint32_t w, int k) { return (float)((w >> (8 * k)) & 0xFFu); }
@@ -269,10 +271,11 @@ __device__ __forceinline__ uint32_t interior8(const GpuNode8* __restrict__ nodes
 // Leaf: every primitive in it, hit rule "smallest t, then smallest id".
 // Any hit ends the traversal on its first accepted primitive.  OBJ: the
 // scene has analytic objects, whose slots are tested by the object
-// intersector (prim id = obj_base + object index).
+// intersector (prim id = obj_base + object index; vol = SDF voxel pool).
 template <bool ANY, bool OBJ>
 __device__ __forceinline__ uint32_t leaf_prims(const float4* __restrict__ tris, const float4* __restrict__ objs,
-                                               uint32_t obj_base, Trav& s, uint32_t lref) {
+                                               uint32_t obj_base, const float* __restrict__ vol, Trav& s,
+                                               uint32_t lref) {
     const uint32_t first = lref & kMaxTriSlots;
     const uint32_t cnt = ((lref >> 27) & 15u) + 1u;
     for (uint32_t k = 0; k < cnt; ++k) {
@@ -280,11 +283,11 @@ __device__ __forceinline__ uint32_t leaf_prims(const float4* __restrict__ tris, 
         const float4 va = tp[0];
         uint32_t prim = __float_as_uint(va.w);
         float t;
+        bool inside = false;
         if (OBJ && (prim & kObjSlot)) {
             const uint32_t ok = prim & ~kObjSlot;
-            bool inside;
-            if (!obj::test(objs + (size_t)obj::kRecord * ok, s.p.ox, s.p.oy, s.p.oz, s.p.dx, s.p.dy, s.p.dz, t,
-                           inside))
+            if (!obj::test(objs + (size_t)obj::kRecord * ok, s.p.ox, s.p.oy, s.p.oz, s.p.dx, s.p.dy, s.p.dz, s.meps,
+                           vol, t, inside))
                 continue;
             prim = obj_base + ok;
         } else {
@@ -298,6 +301,7 @@ __device__ __forceinline__ uint32_t leaf_prims(const float4* __restrict__ tris, 
             s.best_t = t;
             s.best_p = prim;
             s.best_slot = first + k;
+            if (OBJ) s.best_in = inside;
             if (ANY) { s.ref = kRefEmpty; return k + 1; }
         }
     }
@@ -306,8 +310,8 @@ __device__ __forceinline__ uint32_t leaf_prims(const float4* __restrict__ tris, 
 
 template <bool ANY, bool OBJ>
 __device__ __forceinline__ uint32_t leaf(const float4* __restrict__ tris, const float4* __restrict__ objs,
-                                         uint32_t obj_base, Trav& s, Stack& st) {
-    const uint32_t c = leaf_prims<ANY, OBJ>(tris, objs, obj_base, s, s.ref);
+                                         uint32_t obj_base, const float* __restrict__ vol, Trav& s, Stack& st) {
+    const uint32_t c = leaf_prims<ANY, OBJ>(tris, objs, obj_base, vol, s, s.ref);
     if (!ANY || s.ref != kRefEmpty) s.ref = st.pop();
     return c;
 }
@@ -336,7 +340,7 @@ __global__ __launch_bounds__(kTraceBlock)
 __attribute__((amdgpu_waves_per_eu(ANY ? AR_TRACE_WAVES_ANY : AR_TRACE_WAVES_CLOSEST, 8))) void k_trace(
     const void* __restrict__ nodes, const float4* __restrict__ tris, uint32_t root,
     const uint32_t* __restrict__ roots, float origin_limit, const float4* __restrict__ objs, uint32_t obj_base,
-    ar_rays r, size_t n, const uint32_t* __restrict__ n_dev, ar_hits h, uint32_t* __restrict__ occ,
+    const float* __restrict__ vol, ar_rays r, size_t n, const uint32_t* __restrict__ n_dev, ar_hits h, uint32_t* __restrict__ occ,
     int* __restrict__ spill, uint32_t threads, uint32_t* __restrict__ cnt_box, uint32_t* __restrict__ cnt_tri,
     uint32_t* __restrict__ fetch) {
     __shared__ int lds_stack[kLdsStack * kTraceBlock];
@@ -392,6 +396,10 @@ __attribute__((amdgpu_waves_per_eu(ANY ? AR_TRACE_WAVES_ANY : AR_TRACE_WAVES_CLO
                     s.best_p = kRefEmpty;
                     s.best_slot = kRefEmpty;
                     s.have = false;
+                    if (OBJ) {
+                        s.best_in = false;
+                        s.meps = r.meps ? r.meps[idx] : 0.5f * fmaxf(s.tmin, 1e-3f);
+                    }
                     st.sp = 0;
                     st.push(kRefEmpty);
                     s.ref = (s.p.valid && s.tmax >= s.tmin) ? (roots ? roots[idx] : root) : kRefEmpty;
@@ -415,7 +423,7 @@ __attribute__((amdgpu_waves_per_eu(ANY ? AR_TRACE_WAVES_ANY : AR_TRACE_WAVES_CLO
                         if (COUNT) nb += 2;
                     }
                 } else {
-                    const uint32_t c = leaf<ANY, OBJ>(tris, objs, obj_base, s, st);
+                    const uint32_t c = leaf<ANY, OBJ>(tris, objs, obj_base, vol, s, st);
                     if (COUNT) nt += c;
                 }
             }
@@ -434,11 +442,7 @@ __attribute__((amdgpu_waves_per_eu(ANY ? AR_TRACE_WAVES_ANY : AR_TRACE_WAVES_CLO
                         float tt, u = 0.0f, v = 0.0f;
                         uint32_t flags = 0;
                         if (OBJ && (__float_as_uint(va.w) & kObjSlot)) {
-                            bool inside = false;
-                            const uint32_t ok = __float_as_uint(va.w) & ~kObjSlot;
-                            obj::test(objs + (size_t)obj::kRecord * ok, s.p.ox, s.p.oy, s.p.oz, s.p.dx, s.p.dy,
-                                      s.p.dz, tt, inside);
-                            flags = inside ? AR_HIT_INSIDE : 0u;
+                            flags = s.best_in ? AR_HIT_INSIDE : 0u;
                         } else {
                             const float4 vb = tp[1], vc = tp[2];
                             tri_test<true>(s.p, va, vb, vc, tt, &u, &v, &flags);
@@ -534,7 +538,7 @@ hipError_t launch_trace_closest(const TraceArgs& ta, const ar_rays& r, size_t n,
     blocks = std::max(1u, std::min(blocks, ta.full_closest));
 #define AR_LAUNCH_CLOSEST(W, C, O)                                                                               \
     hipLaunchKernelGGL((k_trace<W, false, C, O>), dim3(blocks), dim3(kTraceBlock), 0, s, nd, ta.tris, ta.root,  \
-                       ta.roots, ta.origin_limit, ta.objs, ta.obj_base, r, n, n_dev, h, nullptr, ta.spill,      \
+                       ta.roots, ta.origin_limit, ta.objs, ta.obj_base, ta.vol, r, n, n_dev, h, nullptr, ta.spill, \
                        ta.threads, cnt_box, cnt_tri, ta.fetch)
     const bool ob = ta.objs != nullptr;
     if (ta.width == 8) {
@@ -557,7 +561,7 @@ hipError_t launch_trace_any(const TraceArgs& ta, const ar_rays& r, size_t n, uin
     blocks = std::max(1u, std::min(blocks, ta.full_any));
 #define AR_LAUNCH_ANY(W, O)                                                                                      \
     hipLaunchKernelGGL((k_trace<W, true, false, O>), dim3(blocks), dim3(kTraceBlock), 0, s, nd, ta.tris, ta.root, \
-                       ta.roots, ta.origin_limit, ta.objs, ta.obj_base, r, n, n_dev, none, occ, ta.spill,       \
+                       ta.roots, ta.origin_limit, ta.objs, ta.obj_base, ta.vol, r, n, n_dev, none, occ, ta.spill, \
                        ta.threads, nullptr, nullptr, ta.fetch)
     const bool ob = ta.objs != nullptr;
     if (ta.width == 8) { if (ob) AR_LAUNCH_ANY(8, true); else AR_LAUNCH_ANY(8, false); }

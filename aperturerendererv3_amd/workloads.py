@@ -266,6 +266,60 @@ def random_object_scene(n_tris: int = 2000, n_boxes: int = 30, n_spheres: int = 
     return V, F, objects
 
 
+def _hsv(h, s=1.0, v=1.0):
+    """HSV -> RGB (h in [0, 1))."""
+    import colorsys
+    return colorsys.hsv_to_rgb(float(h) % 1.0, s, v)
+
+
+def glass_sdf_grid(kind="torus", half=2, scale=1.0, object_scale=1.0, position=(0.0, 0.0, 0.0), volume=-1,
+                   seed=1):
+    """The glass stress grids of the reference: CreateGlassTorus
+    (core/helper/scene_helper.h:376-410, kind 'torus') and CreateGlassSDFVolume
+    (:412-445, kind 'volume', `volume` = an SDF volume index): (2 half + 1)^3
+    glass SDF objects on a lattice of spacing 0.25 scale / half around
+    `position`, half extent 0.05 scale object_scale, random rotations, hue and
+    the reference's IOR / absorption / dispersion ramps over the lattice
+    coordinates.  The reference draws its random numbers with Hash11f; here a
+    seeded numpy generator does.  Returns object dicts."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    objs = []
+    for x in range(-half, half + 1):
+        for y in range(-half, half + 1):
+            for z in range(-half, half + 1):
+                nx, ny, nz = ((v + half) / half * 0.5 for v in (x, y, z))
+                loc = tuple(float(position[k] + v * 0.25 * scale / half) for k, v in enumerate((x, y, z)))
+                rot = tuple(float(a) for a in rng.uniform(0.0, 360.0, 3))
+                col = np.asarray(_hsv(rng.random()), np.float64)
+                if kind == "torus":
+                    col = col * (1 - nx * 0.7) + nx * 0.7
+                    mat = {"albedo": tuple(col), "model": 2, "ior": 1.3 + 2.0 * ny, "sigma_a": 5.0 + nz * 50.0,
+                           "abbe": 1.0, "tilt": 0.1}
+                    ob = {"type": "torus", "sdf": (0.375, 0.125)}
+                else:
+                    col = col * (1 - nz * 0.7) + nz * 0.7
+                    mat = {"albedo": tuple(col), "model": 2, "ior": 1.6 + 2.0 * ny, "sigma_a": 5.0 + nx * 25.0,
+                           "abbe": 1.0, "tilt": 0.1}
+                    ob = {"type": "volume", "volume": int(volume)}
+                e = 0.05 * scale * object_scale
+                ob.update({"center": loc, "extent": (e, e, e), "rotation": rot, "material": mat})
+                objs.append(ob)
+    return objs
+
+
+def sdf_cornell(kind="torus", volume=-1, glass=True, half=1):
+    """The object Cornell box (cornell_objects without its two boxes) holding
+    a glass_sdf_grid of tori or volumes at 1.6x the reference's object size
+    (so that the objects are several pixels wide at 32-64 px); glass=False
+    gives the same grid in white Lambert."""
+    objs = cornell_objects()[:6]
+    grid = glass_sdf_grid(kind, half=half, scale=1.2, object_scale=1.6, position=(0.0, -0.1, 0.0), volume=volume)
+    if not glass:
+        for ob in grid:
+            ob["material"] = {"albedo": (0.8, 0.8, 0.8)}
+    return objs + grid
+
+
 # --------------------------------------------------------------------------
 # Spectral instanced scene (~500k triangles, 64 triangle lights, Lambert/GGX)
 # --------------------------------------------------------------------------

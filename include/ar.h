@@ -72,10 +72,17 @@ typedef struct {
 #define AR_MODEL_GLASS 2u    /* smooth dielectric (material.cuh:116-161, 206-257); analytic objects only */
 
 /* Analytic primitives (the reference's SceneObject of type OBJ_CUBE /
- * OBJ_SPHERE, core/render/object.cuh:207-305; intersectors
- * core/object/aabb.cuh:9-106 and core/object/sphere.cuh:10-85). */
-#define AR_OBJ_BOX 0u        /* oriented box of half-extents `extent` */
-#define AR_OBJ_SPHERE 1u     /* sphere of radius max(extent) */
+ * OBJ_SPHERE / OBJ_SDF, core/render/object.cuh:207-305; intersectors
+ * core/object/aabb.cuh:9-106, core/object/sphere.cuh:10-85 and
+ * core/object/sdf/sdf_base.cuh:54-180).  A signed-distance object lives in
+ * its box: unit space u = local / (2 extent) in [-0.5, 0.5]^3, distances
+ * scaled by 2 min(extent).  It is sphere-traced (at most 512 steps) with the
+ * ray's march tolerance (ar_rays.meps), its normal is the distance gradient,
+ * and it cannot be a light. */
+#define AR_OBJ_BOX 0u          /* oriented box of half-extents `extent` */
+#define AR_OBJ_SPHERE 1u       /* sphere of radius max(extent) */
+#define AR_OBJ_SDF_TORUS 2u    /* SDF_TORUS: radii sdf[0] (ring), sdf[1] (tube) in unit space (CreateTorus: 0.375, 0.125) */
+#define AR_OBJ_SDF_VOLUME 3u   /* SDF_VOLUME: baked distance grid sdf_volume (ar_sdf_volume_bake / _add) */
 
 typedef struct {
     uint32_t type;        /* AR_OBJ_* */
@@ -83,6 +90,8 @@ typedef struct {
     float extent[3];      /* half extents (absolute values are used) */
     float rotation[3];    /* Euler angles in degrees; objectToWorld = Rz * Ry * Rx */
     ar_material material;
+    float sdf[4];         /* AR_OBJ_SDF_TORUS: ring and tube radius (sdf[2], sdf[3] unused) */
+    int32_t sdf_volume;   /* AR_OBJ_SDF_VOLUME: volume index */
 } ar_object;
 
 /* A triangle mesh in host memory.  vertices: nv x 3 floats; indices:
@@ -102,6 +111,10 @@ typedef struct {
  * per ray.  A ray is valid for t in [tmin, tmax). */
 typedef struct {
     const float *ox, *oy, *oz, *tmin, *dx, *dy, *dz, *tmax;
+    /* optional: the SDF march tolerance of each ray (the reference's
+     * SDF_EPS(depth) = INTERSECT_EPS(depth) / 2); NULL: max(tmin, 1e-3) / 2,
+     * which is SDF_EPS(depth) for rays with tmin = INTERSECT_EPS(depth) */
+    const float* meps;
 } ar_rays;
 
 /* Closest-hit output.  prim = AR_MISS on a miss (then t = +inf, u = v = 0,
@@ -152,12 +165,37 @@ int ar_abi_version(void);
  * its arrays.  Invalidates the BVH. */
 int ar_scene_set(ar_ctx* ctx, const ar_mesh* meshes, size_t nmesh);
 
-/* Replaces the scene's analytic objects (the box and sphere SceneObjects of
- * SceneRenderer(const std::vector<SceneObject>&), core/scene.hpp:94-101).
- * Object k is primitive ntri + k.  An object with emission is a light,
- * sampled by the facing-facet box sampler or the spherical-cap sampler
- * (core/nee/nee.cuh:153-310).  Copies the array; invalidates the BVH. */
+/* Replaces the scene's analytic objects (the box, sphere and SDF
+ * SceneObjects of SceneRenderer(const std::vector<SceneObject>&),
+ * core/scene.hpp:94-101).  Object k is primitive ntri + k.  A box or sphere
+ * with emission is a light, sampled by the facing-facet box sampler or the
+ * spherical-cap sampler (core/nee/nee.cuh:153-310); an emissive SDF object
+ * is rejected (AR_E_UNSUPPORTED).  Copies the array; invalidates the BVH. */
 int ar_scene_set_objects(ar_ctx* ctx, const ar_object* objects, size_t nobj);
+
+/* --- signed-distance volumes ---------------------------------------------
+ * Replace SDFCacheManager (core/object/sdf/sdf_cache_manager.cuh:39-114,
+ * SceneRenderer::AddVolumeToCacheManager, core/scene.hpp:118-126).  A volume
+ * is a grid of dims[0] x dims[1] x dims[2] unit-space distances (x fastest)
+ * over the unit cube [-0.5, 0.5]^3; AR_OBJ_SDF_VOLUME objects refer to it by
+ * index.  Volumes must exist before ar_scene_set_objects names them. */
+#define AR_SDF_SHAPE_TORUS 0u      /* SdfTorus, params (R, r) */
+#define AR_SDF_SHAPE_DIAMOND 1u    /* SdfDiamondUnit (sdf_diamond.cuh:21-66), params (d, bias, tableZ, scale);
+                                      zeros take 0.94, 0.5, 0.3, 1 (CreateDiamond passes scale 0.5) */
+
+/* FillSDFVolume (sdf_cache_manager.cuh:16-37) on the GPU: voxel (x, y, z) =
+ * shape(p01 - 0.5), p01 = (x, y, z) / (dims - 1).  dims: 1..4096 each.
+ * Waits for the result. */
+int ar_sdf_volume_bake(ar_ctx* ctx, uint32_t shape, const float params[4], const uint32_t dims[3],
+                       uint32_t* index);
+/* A volume from host data (dims[0] * dims[1] * dims[2] floats, x fastest). */
+int ar_sdf_volume_add(ar_ctx* ctx, const float* voxels, const uint32_t dims[3], uint32_t* index);
+/* Copies volume `index` to host memory (its dims[0] * dims[1] * dims[2] floats);
+ * dims (optional) receives its size. */
+int ar_sdf_volume_read(ar_ctx* ctx, uint32_t index, float* voxels, uint32_t dims[3]);
+/* Drops every volume (SDFCacheManager::Clear); AR_E_INVALID while the
+ * scene's objects refer to one. */
+int ar_sdf_volume_clear(ar_ctx* ctx);
 
 /* --- acceleration structure ----------------------------------------------
  * Replaces BuildBVH (core/bvh/bvh.cuh:142-422) as called from

@@ -27,3 +27,37 @@ def test_null_arguments_rejected():
     assert lib.ar_bvh_build(None, None) == _lib.AR_E_INVALID
     assert lib.ar_trace_closest(None, None, 0, None, None) == _lib.AR_E_INVALID
     assert lib.ar_last_error(None) == b"null context"
+
+
+def test_ctypes_structs_match_the_header(tmp_path):
+    """Every ctypes mirror has the size and field offsets gcc gives the
+    struct of include/ar.h."""
+    import pathlib
+    import shutil
+    import subprocess
+    import pytest
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc not available")
+    structs = [n for n in dir(_lib) if n.startswith("ar_") and isinstance(getattr(_lib, n), type)
+               and issubclass(getattr(_lib, n), C.Structure)]
+    assert {"ar_object", "ar_material", "ar_rays", "ar_render_opts"} <= set(structs)
+    inc = pathlib.Path(_lib.__file__).resolve().parent.parent / "include"
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "ar.h"', "int main(void) {"]
+    for n in structs:
+        lines.append(f'printf("{n} size %zu\\n", sizeof({n}));')
+        for f, _ in getattr(_lib, n)._fields_:
+            lines.append(f'printf("{n} {f} %zu\\n", offsetof({n}, {f}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-I", str(inc), str(src), "-o", str(exe)], check=True)
+    got = {}
+    for line in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.splitlines():
+        n, f, v = line.split()
+        got[(n, f)] = int(v)
+    for n in structs:
+        cls = getattr(_lib, n)
+        assert got[(n, "size")] == C.sizeof(cls), n
+        for f, _ in cls._fields_:
+            assert got[(n, f)] == getattr(cls, f).offset, (n, f)

@@ -1,0 +1,211 @@
+"""GPU signed-distance objects through the C ABI: the volume bake against the
+oracle's, traversal of tori and volumes (with triangles, boxes and spheres)
+bit-exact against the oracle's brute force, and the integrator on SDF scenes
+against the FP32 oracle at equal seeds.
+
+Bars: the torus bake and every hit (prim, t bits, inside flag) are
+bit-identical to the oracle (same float arithmetic: no contraction, IEEE
+division and square root); the diamond bake goes through atan2f / cosf /
+sinf, whose device and glibc results may differ in the last ulp, so 99.9 %
+of its voxels must agree to 1e-5 and all of them to 2e-3 (a flipped azimuth
+sector at a sector edge); images as in test_render_objects_gpu.py."""
+import numpy as np
+import pytest
+
+import oracle_lib as orc
+from aperturerendererv3_amd.workloads import CORNELL_CAMERA, random_object_scene, sdf_cornell
+
+pytestmark = pytest.mark.gpu
+
+EMPTY_V = np.zeros((0, 3), np.float32)
+EMPTY_F = np.zeros((0, 3), np.uint32)
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from aperturerendererv3_amd import Context
+    c = Context(0)
+    yield c
+    c.close()
+
+
+def test_bake_matches_oracle(ctx):
+    dims = (40, 33, 27)
+    print("bake torus", dims, flush=True)
+    k = ctx.sdf_volume_bake("torus", dims, (0.3, 0.1))
+    np.testing.assert_array_equal(ctx.sdf_volume_read(k), orc.sdf_bake("torus", (0.3, 0.1), dims))
+    print("bake diamond 64^3", flush=True)
+    k = ctx.sdf_volume_bake("diamond", (64, 64, 64), (0.0, 0.0, 0.0, 0.5))
+    g, o = ctx.sdf_volume_read(k), orc.sdf_bake("diamond", (0.0, 0.0, 0.0, 0.5), (64, 64, 64))
+    d = np.abs(g - o)
+    print(f"  diamond: {np.mean(d <= 1e-5):.5f} within 1e-5, max {d.max():.2e}", flush=True)
+    assert np.mean(d <= 1e-5) >= 0.999 and d.max() < 2e-3
+    # host volumes round-trip
+    v = np.random.default_rng(1).normal(size=(5, 6, 7)).astype(np.float32)
+    np.testing.assert_array_equal(ctx.sdf_volume_read(ctx.sdf_volume_add(v)), v)
+
+
+def _sdf_scene(ctx, seed):
+    rng = np.random.default_rng(seed)
+    V, F, objs = random_object_scene(n_tris=800, n_boxes=8, n_spheres=8, seed=seed)
+    vol_np = orc.sdf_bake("torus", (0.35, 0.12), (32, 32, 32))
+    vidx = ctx.sdf_volume_add(vol_np)
+    for k in range(24):
+        kind = "torus" if k % 2 == 0 else "volume"
+        ob = {"type": kind, "center": tuple(rng.uniform(-1, 1, 3)), "extent": tuple(rng.uniform(0.08, 0.3, 3)),
+              "rotation": tuple(rng.uniform(-180, 180, 3))}
+        if kind == "torus":
+            ob["sdf"] = (float(rng.uniform(0.25, 0.4)), float(rng.uniform(0.05, 0.1)))
+        else:
+            ob["volume"] = vidx
+        objs.append(ob)
+    # the oracle takes the voxels themselves
+    oobjs = [dict(o, volume_data=vol_np) if o.get("type") == "volume" else o for o in objs]
+    return V, F, objs, oobjs
+
+
+def _rays(rng, objs, n, meps):
+    o = rng.uniform(-1.6, 1.6, (n, 3))
+    tgt = np.array([ob["center"] for ob in objs])[rng.integers(0, len(objs), n)] + rng.normal(size=(n, 3)) * 0.1
+    d = tgt - o
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.zeros((9 if meps else 8, n), np.float32)
+    rays[0:3] = o.T
+    rays[3] = np.where(np.arange(n) % 4 == 0, 0.0, 1e-3)
+    rays[4:7] = d.T
+    rays[7] = np.inf
+    if meps:
+        rays[8] = rng.choice([5e-4, 1e-3, 2.7e-3], n)
+    return rays
+
+
+@pytest.mark.parametrize("width", [2, 8])
+@pytest.mark.parametrize("meps", [False, True])
+def test_sdf_traversal_bit_exact(ctx, width, meps):
+    import torch
+    V, F, objs, oobjs = _sdf_scene(ctx, 3)
+    ctx.set_meshes([{"vertices": V, "indices": F}])
+    ctx.set_objects(objs)
+    ctx.build_bvh(width=width)
+    rng = np.random.default_rng(4)
+    rays = _rays(rng, objs, 40000, meps)
+    print(f"sdf traversal width {width} meps {meps}: {rays.shape[1]} rays", flush=True)
+    r = torch.from_numpy(rays).cuda()
+    h = ctx.trace_closest(r)
+    torch.cuda.synchronize()
+    prim = h["prim"].cpu().numpy().view(np.uint32)
+    t = h["tuv"][0].cpu().numpy()
+    flags = h["flags"].cpu().numpy().view(np.uint32)
+    g = orc.Geom(V, F, oobjs)
+    ref = g.brute(rays)
+    np.testing.assert_array_equal(prim, ref["prim"])
+    np.testing.assert_array_equal(t.view(np.uint32), ref["t"].view(np.uint32))
+    np.testing.assert_array_equal(flags, ref["flags"])
+    nobj_hits = np.sum((prim != 0xFFFFFFFF) & (prim >= F.shape[0] + 16))
+    print(f"  SDF hits {nobj_hits}, inside {np.sum(flags == orc.HIT_INSIDE)}", flush=True)
+    assert nobj_hits > 2000
+    # any hit over segments ending just short of / past the closest hit
+    rays2 = rays.copy()
+    hit = prim != 0xFFFFFFFF
+    rays2[7] = np.where(hit, t * np.where(np.arange(len(t)) % 2 == 0, 0.999, 1.001), 50.0).astype(np.float32)
+    occ = ctx.trace_any(torch.from_numpy(rays2).cuda())
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(orc.unpack_bits(occ.cpu().numpy(), rays.shape[1]), g.any_brute(rays2))
+
+
+def _kw(**extra):
+    c = CORNELL_CAMERA
+    return dict(cam_pos=c["pos"], cam_dir=c["dir"], vfov=c["vfov"], **extra)
+
+
+def compare(g, o, name, rtol, atol, min_frac, max_rel):
+    """relMSE over the pixels within tolerance must be tiny (the bulk of the
+    image is the same arithmetic), the fraction within tolerance high, and
+    relMSE over all pixels below max_rel.  Pixels outside tolerance are
+    paths that diverged: a last-ulp difference between the device's and
+    glibc's sinf / cosf (cosine lobes, light samplers) moves one sample
+    direction, which then crosses a glass edge the other did not; through
+    glass of index up to 3.6 such a path carries a large weight."""
+    rel = orc.rel_mse(g, o)
+    close = (np.abs(g - o) <= rtol * np.abs(o) + atol).all(axis=2)
+    frac = close.mean()
+    rel_in = orc.rel_mse(g[close], o[close])
+    print(f"{name}: relMSE(gpu, oracle) = {rel:.3e} ({rel_in:.3e} within tolerance), pixels within tolerance "
+          f"= {frac:.5f} ({int((~close).sum())} outside)", flush=True)
+    assert rel <= max_rel and frac >= min_frac and rel_in <= 1e-7, (name, rel, frac, rel_in)
+
+
+def _render_pair(ctx, objs, oobjs, w, h, spp, **kw):
+    import torch
+    ctx.set_meshes([])
+    ctx.set_objects(objs)
+    ctx.build_bvh()
+    g = ctx.render(w, h, spp, **kw)
+    torch.cuda.synchronize()
+    o = orc.render(EMPTY_V, EMPTY_F, None, [{}], w, h, spp, precision=32, objects=oobjs, **kw)
+    return g.cpu().numpy(), o
+
+
+def test_lambert_tori_match_oracle(ctx):
+    objs = sdf_cornell("torus", glass=False)
+    print("lambert tori in the object cornell box", flush=True)
+    g, o = _render_pair(ctx, objs, objs, 48, 48, 16, **_kw(seed=5, max_depth=5))
+    assert o.mean() > 0.05
+    compare(g, o, "lambert tori", rtol=1e-3, atol=1e-4, min_frac=0.98, max_rel=1e-4)
+
+
+@pytest.mark.parametrize("kind", ["torus", "volume"])
+def test_glass_sdf_grid_matches_oracle(ctx, kind):
+    vidx = -1
+    vol = None
+    if kind == "volume":
+        vidx = ctx.sdf_volume_bake("diamond", (48, 48, 48), (0.0, 0.0, 0.0, 0.5))
+        vol = ctx.sdf_volume_read(vidx)
+    objs = sdf_cornell(kind, volume=vidx, glass=True)
+    oobjs = [dict(o, volume_data=vol) if o.get("type") == "volume" else o for o in objs]
+    print(f"glass {kind} grid in the object cornell box", flush=True)
+    g, o = _render_pair(ctx, objs, oobjs, 40, 40, 16, **_kw(seed=7, max_depth=12))
+    compare(g, o, f"glass {kind}", rtol=1e-3, atol=1e-4, min_frac=0.98, max_rel=1e-2)
+
+
+def test_spectral_dispersive_tori_match_oracle(ctx):
+    objs = sdf_cornell("torus", glass=True)
+    print("spectral hero glass tori", flush=True)
+    g, o = _render_pair(ctx, objs, objs, 32, 32, 16, **_kw(seed=9, max_depth=12, spectral=4, output=1))
+    compare(g, o, "spectral glass tori", rtol=1e-3, atol=1e-4, min_frac=0.93, max_rel=1e-3)
+
+
+def test_camera_inside_a_glass_torus(ctx):
+    # camera on the core circle of a torus (looking out through the tube
+    # wall): it starts in the torus's medium.  Radii 0.3 / 0.12 keep the tube
+    # off the faces of its box (CreateTorus's 0.375 + 0.125 touches them, and
+    # a march that reaches the box exit on the surface reports no hit).
+    torus = {"type": "torus", "center": (0.0, 0.0, 0.0), "extent": (1.0, 1.0, 1.0), "sdf": (0.3, 0.12),
+             "material": {"albedo": (1.0, 1.0, 1.0), "model": 2, "ior": 1.4, "sigma_a": 0.0}}
+    kw = dict(cam_pos=(0.6, 0.0, 0.0), cam_dir=(1.0, 0.0, 0.0), vfov=50.0, sky=2, env=(0.5, 0.5, 0.5),
+              seed=11, max_depth=64)
+    print("camera inside a glass torus", flush=True)
+    g, o = _render_pair(ctx, [torus], [torus], 24, 24, 16, **kw)
+    compare(g, o, "inside torus", rtol=1e-3, atol=1e-4, min_frac=0.9, max_rel=1e-3)
+    # seen from inside a medium of index n the sky is brighter: n^2 L at
+    # normal incidence, less at oblique exits
+    assert 0.5 * 1.4 ** 2 * 0.9 < g.mean() < 0.5 * 1.4 ** 2 * 1.02
+
+
+def test_sdf_api_errors(ctx):
+    from aperturerendererv3_amd import ArError
+    with pytest.raises(ArError):
+        ctx.set_objects([{"type": "volume", "center": (0, 0, 0), "extent": (1, 1, 1), "volume": 10_000}])
+    with pytest.raises(ArError):
+        ctx.set_objects([{"type": "torus", "center": (0, 0, 0), "extent": (1, 1, 1),
+                          "material": {"albedo": (0, 0, 0), "emission": (1.0, 1.0, 1.0)}}])
+    with pytest.raises(ArError):
+        ctx.sdf_volume_bake("torus", (0, 4, 4))
+    k = ctx.sdf_volume_bake("torus", (8, 8, 8), (0.3, 0.1))
+    ctx.set_objects([{"type": "volume", "center": (0, 0, 0), "extent": (1, 1, 1), "volume": k}])
+    with pytest.raises(ArError):
+        ctx.sdf_volume_clear()            # referenced by the scene
+    ctx.set_objects([])
+    ctx.sdf_volume_clear()
+    with pytest.raises(ArError):
+        ctx.sdf_volume_read(k)
