@@ -61,6 +61,8 @@ def parse_args():
     ap.add_argument("--batch-only", action="store_true", help="run only the batched multi-scene bench")
     ap.add_argument("--batch-scenes", type=int, default=48)
     ap.add_argument("--mlp-only", action="store_true", help="run only the guiding-network kernel bench")
+    ap.add_argument("--output-only", action="store_true", help="run only the output-stage bench")
+    ap.add_argument("--sdf-only", action="store_true", help="run only the SDF-object render bench")
     ap.add_argument("--cpu-sample-stride", type=int, default=2)
     ap.add_argument("--profile", action="store_true",
                     help="timed steps only, plus one 256 MiB device copy used to scale HBM counters")
@@ -363,11 +365,155 @@ def mlp_bench(ctx_cls, reps=10):
                         "achieved_TFLOPs": round(fwd_tf, 2), "frac_bf16_peak": round(fwd_tf / MFMA_BF16_PEAK, 5)}}
 
 
+def output_bench(ctx_cls, reps=20, cpu=True):
+    """Output stage at 3840x2160: the ACES tonemap kernel (12 B read + 4 B
+    written per pixel, HBM-bound), the luminance MSE of two HDR images (24 B
+    read per pixel), and progressive 1-spp frames with odd / even halves on
+    the Cornell box at 1920x1080 against plain 1-spp renders.  CPU baseline:
+    the oracle tonemap on the same pixels (one core)."""
+    import torch
+    from aperturerendererv3_amd import AR_TONE_ACES
+    from aperturerendererv3_amd.workloads import CORNELL_CAMERA, cornell_box
+    w, h = 3840, 2160
+    npix = w * h
+    rng = np.random.default_rng(1)
+    hdr_np = (rng.random((h, w, 3), dtype=np.float32) * 2.0) ** 2
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    out = {"workload": f"{w}x{h} HDR frame (tonemap, luminance MSE); Cornell box 1920x1080 progressive 1-spp frames"}
+    with ctx_cls(0) as ctx:
+        hdr = torch.from_numpy(hdr_np).to(dev)
+        hdr2 = hdr.flip(0).contiguous()
+        ldr = torch.empty((h, w), dtype=torch.int32, device=dev)
+        for _ in range(3):
+            ctx.tonemap(hdr, exposure=1.0, tone=AR_TONE_ACES, out=ldr)
+        torch.cuda.synchronize()
+        e0.record(stream)
+        for _ in range(reps):
+            ctx.tonemap(hdr, exposure=1.0, tone=AR_TONE_ACES, out=ldr)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        tm_ms = e0.elapsed_time(e1) / reps
+        ctx.luminance_mse(hdr, hdr2)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            ctx.luminance_mse(hdr, hdr2)
+        mse_ms = (time.perf_counter() - t0) * 1e3 / reps     # synchronous entry point: wall time per call
+        tm_gbps = npix * 16 / (tm_ms * 1e-3) / 1e9
+        out["tonemap_aces"] = {"ms": round(tm_ms, 4), "bytes_per_pixel": 16, "achieved_GBps": round(tm_gbps, 1),
+                               "hbm_frac": round(tm_gbps / HBM_PEAK_GBPS, 4)}
+        out["luminance_mse"] = {"ms_wall": round(mse_ms, 4), "bytes_per_pixel": 24,
+                                "achieved_GBps": round(npix * 24 / (mse_ms * 1e-3) / 1e9, 1)}
+        V, F, M, mats = cornell_box()
+        cam = CORNELL_CAMERA
+        kw = dict(cam_pos=cam["pos"], cam_dir=cam["dir"], vfov=cam["vfov"], seed=1, max_depth=5)
+        ctx.set_meshes([{"vertices": V, "indices": F, "tri_material": M, "materials": mats}])
+        ctx.build_bvh()
+        fw, fh, frames = 1920, 1080, 16
+        acc, odd, even = (torch.empty((fh, fw, 3), dtype=torch.float32, device=dev) for _ in range(3))
+        img = torch.empty((fh, fw, 3), dtype=torch.float32, device=dev)
+        ctx.render_progressive(0, acc, odd, even, spp=1, **kw)
+        ctx.render(fw, fh, 1, out=img, **kw)
+        torch.cuda.synchronize()
+        e0.record(stream)
+        for f in range(frames):
+            ctx.render_progressive(f, acc, odd, even, spp=1, **kw)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        prog_ms = e0.elapsed_time(e1) / frames
+        e0.record(stream)
+        for f in range(frames):
+            ctx.render(fw, fh, 1, sample_offset=f, out=img, **kw)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        plain_ms = e0.elapsed_time(e1) / frames
+        out["progressive_1080p"] = {"frames": frames, "ms_per_frame": round(prog_ms, 4),
+                                    "plain_1spp_ms": round(plain_ms, 4),
+                                    "paths_per_s": round(fw * fh / (prog_ms * 1e-3), 1),
+                                    "variance_1e6_mse": round(ctx.luminance_mse(odd, even) * 1e6, 3)}
+    if cpu:
+        sys.path.insert(0, str(ROOT / "tests"))
+        import ctypes as C
+        import oracle_lib as orc
+        f = orc.fn("orc_tonemap", None, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_float, C.c_int, C.c_int])
+        o = np.zeros(npix, np.uint32)
+        t0 = time.perf_counter()
+        f(hdr_np.ctypes.data, o.ctypes.data, npix, 1.0, 2, 0)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(npix / dt, 1), "unit": "pixels/s (ACES tonemap)", "cores": 1,
+                               "kind": "port", "sample": f"the full {w}x{h} frame with the oracle tonemap, {dt:.2f} s"}
+        out["tonemap_gpu_pixels_per_s"] = round(npix / (tm_ms * 1e-3), 1)
+    return out
+
+
+def sdf_bench(ctx_cls, reps=3, cpu=True):
+    """Signed-distance objects: the reference's glass stress grids
+    (CreateGlassTorus, CreateGlassSDFVolume with a baked 64^3 diamond; 125
+    glass objects each) inside the object Cornell box, 512x512, 16 spp,
+    max depth 12, RGB, NEE on.  GPU paths/s; CPU baseline: the oracle FP32
+    integrator on a 256x256 frame of the same view at 8 spp."""
+    import torch
+    from aperturerendererv3_amd.workloads import CORNELL_CAMERA, cornell_objects, glass_sdf_grid
+    cam = CORNELL_CAMERA
+    kw = dict(cam_pos=cam["pos"], cam_dir=cam["dir"], vfov=cam["vfov"], seed=1, max_depth=12)
+    w = h = 512
+    spp = 16
+    out = {"workload": "object Cornell box + 125 glass SDF objects (CreateGlassTorus / CreateGlassSDFVolume grids, "
+                       "half extent 0.05), 512x512, 16 spp, max depth 12, NEE on, seed 1"}
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    with ctx_cls(0) as ctx:
+        vidx = ctx.sdf_volume_bake("diamond", (64, 64, 64), (0.0, 0.0, 0.0, 0.5))
+        vol = ctx.sdf_volume_read(vidx)
+        for kind in ("torus", "volume"):
+            objs = cornell_objects()[:6] + glass_sdf_grid(kind, half=2, scale=1.2, position=(0.0, -0.1, 0.0),
+                                                          volume=vidx)
+            ctx.set_meshes([])
+            ctx.set_objects(objs)
+            ctx.build_bvh()
+            img = torch.empty((h, w, 3), dtype=torch.float32, device=dev)
+            ctx.render(w, h, spp, out=img, **kw)
+            torch.cuda.synchronize()
+            e0.record(stream)
+            for _ in range(reps):
+                ctx.render(w, h, spp, out=img, **kw)
+            e1.record(stream)
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / reps
+            res = {"gpu_ms_per_frame": round(ms, 3), "gpu_paths_per_s": round(w * h * spp / (ms * 1e-3), 1)}
+            if cpu:
+                sys.path.insert(0, str(ROOT / "tests"))
+                import oracle_lib as orc
+                cores = len(os.sched_getaffinity(0))
+                oobjs = [dict(o, volume_data=vol) if o.get("type") == "volume" else o for o in objs]
+                cw = ch = 256
+                cs = 8
+                t0 = time.perf_counter()
+                orc.render(np.zeros((0, 3), np.float32), np.zeros((0, 3), np.uint32), None, [{}], cw, ch, cs,
+                           precision=32, threads=cores, objects=oobjs, **kw)
+                dt = time.perf_counter() - t0
+                res["cpu_baseline"] = {"value": round(cw * ch * cs / dt, 1), "unit": "paths/s", "cores": cores,
+                                       "kind": "port", "sample": f"{cw}x{ch} at {cs} spp of the same view with the "
+                                                                 f"oracle FP32 integrator, {dt:.2f} s"}
+            out["glass_" + ("tori" if kind == "torus" else "diamond_volumes")] = res
+    return out
+
+
 def main():
     args = parse_args()
     if args.mlp_only:
         from aperturerendererv3_amd import Context
         print(json.dumps(mlp_bench(Context)), flush=True)
+        return
+    if args.output_only:
+        from aperturerendererv3_amd import Context
+        print(json.dumps(output_bench(Context, cpu=not args.no_cpu_baseline)), flush=True)
+        return
+    if args.sdf_only:
+        from aperturerendererv3_amd import Context
+        print(json.dumps(sdf_bench(Context, cpu=not args.no_cpu_baseline)), flush=True)
         return
     if args.batch_only:
         from aperturerendererv3_amd import Context
@@ -492,6 +638,8 @@ def main():
     batched = batch_bench(Context, count=args.batch_scenes, cpu=not args.no_cpu_baseline)
     guiding = guiding_bench(Context)
     mlp = mlp_bench(Context)
+    output = output_bench(Context, cpu=not args.no_cpu_baseline)
+    sdf = sdf_bench(Context, cpu=not args.no_cpu_baseline)
 
     valu_frac = achieved_tops / VALU_PEAK_TOPS
     out = {
@@ -517,6 +665,8 @@ def main():
         "batched": batched,
         "guiding": guiding,
         "guiding_network": mlp,
+        "output": output,
+        "sdf_objects": sdf,
     }
     print(json.dumps(out), flush=True)
     ctx.close()
