@@ -444,17 +444,31 @@ class Context:
     # -- integrator ----------------------------------------------------------
     def render_progressive(self, frame, accum, odd=None, even=None, spp=1, cam_pos=(0.0, 0.0, 1.1),
                            cam_dir=(0.0, 0.0, -1.0), vfov=60.0, seed=1, max_depth=5, nee=True, sky=AR_SKY_NONE,
-                           env=(0.0, 0.0, 0.0), rr_min_depth=3, spectral=0, output=0, stream=None):
+                           env=(0.0, 0.0, 0.0), rr_min_depth=3, spectral=0, output=0, gbuffer=None, stats=None,
+                           stream=None):
         """ar_render_progressive: fold spp samples (indices frame ..) into the
-        running means accum (and odd / even), (h, w, 3) float32 device tensors."""
+        running means accum (and odd / even), (h, w, 3) float32 device tensors;
+        gbuffer (h, w, 16) / stats (h, w, 12) keep the denoiser's records."""
         h, w = accum.shape[0], accum.shape[1]
         cam, o = _camera_opts(w, h, spp, cam_pos, cam_dir, vfov, seed, max_depth, nee, sky, env, rr_min_depth, 0, 0,
                               spectral, output, 0, 0)
+        opt = lambda t: _ptr(t) if t is not None else None
         self._check(self._lib.ar_render_progressive(self._h, C.byref(cam), C.byref(o), int(frame), _ptr(accum),
-                                                    _ptr(odd) if odd is not None else None,
-                                                    _ptr(even) if even is not None else None, _stream_ptr(stream)),
+                                                    opt(odd), opt(even), opt(gbuffer), opt(stats),
+                                                    _stream_ptr(stream)),
                     "ar_render_progressive")
         return accum
+
+    def denoise(self, hdr, gbuffer, stats, out=None, stream=None):
+        """ar_denoise: (h, w, 3) float32 device image filtered with its
+        (h, w, 16) G-buffer and (h, w, 12) statistics records."""
+        import torch
+        h, w = hdr.shape[0], hdr.shape[1]
+        if out is None:
+            out = torch.empty_like(hdr)
+        self._check(self._lib.ar_denoise(self._h, _ptr(hdr), _ptr(gbuffer), _ptr(stats), _ptr(out), w, h,
+                                         _stream_ptr(stream)), "ar_denoise")
+        return out
 
     def tonemap(self, hdr, exposure=1.0, tone=0, xyz_input=False, out=None, stream=None):
         """ar_tonemap: (h, w, 3) float32 device image -> (h, w) int32 tensor of

@@ -28,6 +28,8 @@ AR_OBJ_SDF_TORUS = 2
 AR_OBJ_SDF_VOLUME = 3
 AR_SDF_SHAPE_TORUS = 0
 AR_SDF_SHAPE_DIAMOND = 1
+AR_GBUFFER_FLOATS = 16
+AR_STATS_FLOATS = 12
 AR_TONE_LINEAR = 0
 AR_TONE_GAMMA = 1
 AR_TONE_ACES = 2
@@ -138,7 +140,9 @@ SIGNATURES = {
                                       C.c_void_p]),
     "ar_render": (C.c_int, [C.c_void_p, C.POINTER(ar_camera), C.POINTER(ar_render_opts), C.c_void_p, C.c_void_p]),
     "ar_render_progressive": (C.c_int, [C.c_void_p, C.POINTER(ar_camera), C.POINTER(ar_render_opts), C.c_uint32,
-                                        C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+                                        C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "ar_denoise": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32,
+                             C.c_void_p]),
     "ar_tonemap": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_float, C.c_uint32,
                              C.c_uint32, C.c_void_p]),
     "ar_luminance_mse": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.POINTER(C.c_double), C.c_void_p]),

@@ -201,6 +201,7 @@ int ar_destroy(ar_ctx* ctx) {
     if (ctx->fetch) (void)hipFree(ctx->fetch);
     if (ctx->reduce) (void)hipFree(ctx->reduce);
     sdf_free(ctx);
+    if (ctx->denoise_feat) (void)hipFree(ctx->denoise_feat);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return AR_OK;
@@ -478,7 +479,7 @@ int ar_render(ar_ctx* ctx, const ar_camera* cam, const ar_render_opts* opts, flo
 }
 
 int ar_render_progressive(ar_ctx* ctx, const ar_camera* cam, const ar_render_opts* opts, uint32_t frame,
-                          float* accum, float* odd, float* even, void* stream) {
+                          float* accum, float* odd, float* even, float* gbuffer, float* stats, void* stream) {
     if (!ctx) return AR_E_INVALID;
     if (!cam || !opts || !accum) return set_error(ctx, AR_E_INVALID, "NULL camera, options or accumulation buffer");
     if ((odd == nullptr) != (even == nullptr)) return set_error(ctx, AR_E_INVALID, "odd and even go together");
@@ -486,7 +487,7 @@ int ar_render_progressive(ar_ctx* ctx, const ar_camera* cam, const ar_render_opt
     if (opts->guiding != AR_GUIDING_OFF)
         return set_error(ctx, AR_E_UNSUPPORTED, "progressive frames are unguided; use ar_render for guided frames");
     if (!ctx->bvh_ready) return set_error(ctx, AR_E_NOSCENE, "no BVH: call ar_scene_set and ar_bvh_build first");
-    return render_progressive(ctx, cam, opts, frame, accum, odd, even, pick_stream(ctx, stream));
+    return render_progressive(ctx, cam, opts, frame, accum, odd, even, gbuffer, stats, pick_stream(ctx, stream));
 }
 
 } // extern "C"

@@ -195,6 +195,8 @@ struct RenderBuffers {
     uint32_t* hit_prim; float* hit_t; float* hit_u; float* hit_v; uint32_t* hit_flags;
     float* srays[8];              // shadow rays
     float* smeps;                 // shadow ray SDF march tolerance (scenes with SDF objects)
+    float* fh = nullptr;          // per-path first-hit records of the denoiser G-buffer (allocated on demand)
+    size_t fh_cap = 0;            // paths fh holds (its array stride)
     uint32_t* spath;
     float* scontrib[4];
     uint32_t* sflags;             // bit0: the sampled light point faced the surface (both cosines > 0)
@@ -336,6 +338,8 @@ struct ar_ctx {
     uint64_t buffer_generation = 0;       // bumped whenever render buffers are reallocated
     double* reduce = nullptr;             // partial sums of ar_luminance_mse
     ar::SdfPool sdf;                      // signed-distance volumes
+    float* denoise_feat = nullptr;        // per-pixel filter features of ar_denoise (29 planes)
+    size_t denoise_floats = 0;
 };
 
 namespace ar {
@@ -379,7 +383,7 @@ void free_scene(ar_ctx* ctx);
 void free_render_buffers(ar_ctx* ctx);
 int render(ar_ctx* ctx, const ar_camera* cam, const ar_render_opts* opts, float* out, hipStream_t s);
 int render_progressive(ar_ctx* ctx, const ar_camera* cam, const ar_render_opts* opts, uint32_t frame, float* accum,
-                       float* odd, float* even, hipStream_t s);
+                       float* odd, float* even, float* gbuffer, float* stats, hipStream_t s);
 int render_batch(ar_ctx* ctx, uint32_t n, const ar_camera* cams, const ar_render_opts* opts, float* const* outs,
                  hipStream_t s);
 int graph_capture(ar_ctx* ctx, uint32_t n, const ar_camera* cams, const ar_render_opts* opts, float* const* outs,

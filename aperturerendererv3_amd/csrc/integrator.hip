@@ -22,6 +22,7 @@
 #include "guide.hpp"
 #include "mlp.hpp"
 #include "objects.hpp"
+#include "denoise.hpp"
 
 #include <cfloat>
 #include <cmath>
@@ -35,6 +36,7 @@ constexpr int kBlock = 256;        // default threads per block of the wavefront
 constexpr int kMaxBlock = 512;     // largest block the wavefront kernels accept
 constexpr uint32_t kMetaNee = 1u << 16;
 constexpr uint32_t kAccumFloats = 9;   // per pixel: sums of all, odd-index and even-index samples
+constexpr size_t kFirstHitFloats = 10; // per path: model bits, albedo[3], normal[3], distance, metallic, roughness
 constexpr uint32_t kMetaHeroOnly = 1u << 17;   // secondary hero wavelengths dropped at a dispersive interface
 
 // ---------------------------------------------------------------- helpers
@@ -531,6 +533,8 @@ struct SceneView {
     uint32_t prog;               // progressive: fold the mean into out, the running mean of sample_offset samples
     float* odd;                  // progressive running means of the odd / even sample indices (or null)
     float* even;
+    float* gbuf;                 // progressive: denoiser G-buffer records (AR_GBUFFER_FLOATS per pixel) or null
+    float* stats;                // progressive: denoiser sample statistics (AR_STATS_FLOATS per pixel) or null
     float cam_pos[3];
     float fwd[3], right[3], up[3];
     float thx, thy;
@@ -606,6 +610,8 @@ struct PathArgs {
     uint32_t* scn;               // scene of the path
     uint32_t* med;               // medium: 0 none, else object index + 1
     float* mp0; float* mp1; float* mp2;   // where the path entered its medium
+    float* fh;                   // first vertex of each path for the denoiser (kFirstHit arrays of fh_stride) or null
+    uint32_t fh_stride;
 };
 
 // Voxel guiding inputs of the integrator: the probes (read-only during a
@@ -866,6 +872,7 @@ __device__ __forceinline__ void shade_one(
         const uint32_t prim = hprim[q];
         uint32_t new_meta = meta & ~kMetaNee;
         uint32_t medium = S.med[p];
+        if (depth == 0 && S.fh && prim == AR_MISS) S.fh[p] = __uint_as_float(4u);   // MAT_MAX: no first hit
         if (prim == AR_MISS && medium != 0u) {
             // inside a medium with no exit found (bMissThinSurfaceInsideMedia_,
             // render.cuh:107-124): leave the medium and continue straight from
@@ -945,6 +952,20 @@ __device__ __forceinline__ void shade_one(
                 nn = ndd < 0.0f ? ng : scale(ng, -1.0f);
             }
             const Mat m = load_mat(sc.mats, mat_id);
+            if (depth == 0 && S.fh) {
+                // RecordScreenGBuffer's inputs (render.cuh:640-643): albedo, the
+                // intersector's normal (facing the ray for triangles), distance,
+                // metallic, roughness and MAT_DIFFUSE / LIGHT / GLASS / GGX
+                const uint32_t model = light_id >= 0 ? 1u : (m.model == AR_MODEL_GLASS ? 2u : (m.model == AR_MODEL_GGX ? 3u : 0u));
+                float* f = S.fh + p;
+                const size_t st = S.fh_stride;
+                f[0] = __uint_as_float(model);
+                f[st * 1] = m.albedo.x; f[st * 2] = m.albedo.y; f[st * 3] = m.albedo.z;
+                f[st * 4] = nn.x; f[st * 5] = nn.y; f[st * 6] = nn.z;
+                f[st * 7] = ht[q];
+                f[st * 8] = m.metallic;
+                f[st * 9] = m.roughness;
+            }
             if (light_id >= 0) {
                 // triangle lights emit from the front face, box and sphere
                 // lights from every hit (MAT_LIGHT objects, render.cuh:664-710)
@@ -1378,8 +1399,30 @@ __global__ __launch_bounds__(kMaxBlock) void k_accumulate(RenderParams P, SceneA
     float h[6];
 #pragma unroll
     for (int k = 0; k < 6; ++k) h[k] = first ? 0.0f : acc[3 + k];
+    // the denoiser's records start empty with the first progressive frame
+    float* gb = V.gbuf ? V.gbuf + kGBufferFloats * (size_t)pix : nullptr;
+    float* stt = V.stats ? V.stats + kStatsFloats * (size_t)pix : nullptr;
+    if (first && V.sample_offset == 0) {
+        if (gb)
+            for (int k = 0; k < kGBufferFloats; ++k) gb[k] = 0.0f;
+        if (stt)
+            for (int k = 0; k < kStatsFloats; ++k) stt[k] = 0.0f;
+    }
     for (uint32_t s = 0; s < sg.count; ++s) {
-        const F3 r = path_value(P, sc, S, sg.base + s * V.npix + pix);
+        const uint32_t pp = sg.base + s * V.npix + pix;
+        const F3 r = path_value(P, sc, S, pp);
+        if (gb) {
+            const size_t st = S.fh_stride;
+            const float* f = S.fh + pp;
+            const uint32_t model = __float_as_uint(f[0]);
+            const float alb[3] = {f[st * 1], f[st * 2], f[st * 3]};
+            const float nrm[3] = {f[st * 4], f[st * 5], f[st * 6]};
+            dn::gbuffer_record(gb, model != 4u, alb, nrm, f[st * 7], f[st * 8], f[st * 9], model);
+        }
+        if (stt) {
+            const float x[3] = {r.x, r.y, r.z};
+            dn::stats_record(stt, x);
+        }
         ax = ax + r.x; ay = ay + r.y; az = az + r.z;
         if (V.odd) {
             float* hh = ((sg.sample0 + s) & 1u) ? h : h + 3;
@@ -1437,6 +1480,9 @@ void free_scene(ar_ctx* ctx) {
 }
 
 void free_render_buffers(ar_ctx* ctx) {
+    if (ctx->rb.fh) (void)hipFree(ctx->rb.fh);
+    ctx->rb.fh = nullptr;
+    ctx->rb.fh_cap = 0;
     if (ctx->rb.block) (void)hipFree(ctx->rb.block);
     if (ctx->rb.accum) (void)hipFree(ctx->rb.accum);
     if (ctx->rb.eps_table) (void)hipFree(ctx->rb.eps_table);
@@ -1869,11 +1915,30 @@ static int make_job(ar_ctx* ctx, uint32_t n, const ar_camera* cams, const ar_ren
 // allocations and uploads; nothing here may be captured into a graph.
 // stream == nullptr with sync: synchronous upload (graphs); else the plan is
 // copied on `stream`, ordered after the previous render on that stream.
+static bool wants_first_hits(const RenderPlan& L) {
+    for (const SceneView& v : L.views)
+        if (v.gbuf) return true;
+    return false;
+}
+
 static int prepare_job(ar_ctx* ctx, RenderJob& J, hipStream_t stream, bool sync) {
     (void)hipSetDevice(ctx->device);
     const RenderPlan& L = J.plan;
     int rc = ensure_render_buffers(ctx, L.cap, L.pixels);
     if (rc != AR_OK) return rc;
+    if (wants_first_hits(L)) {
+        // per-path first-hit records for the denoiser's G-buffer
+        RenderBuffers& b = ctx->rb;
+        if (b.fh_cap < b.capacity) {
+            if (b.fh) (void)hipFree(b.fh);
+            b.fh = nullptr;
+            b.fh_cap = 0;
+            rc = check_hip(ctx, hipMalloc((void**)&b.fh, b.capacity * kFirstHitFloats * sizeof(float)),
+                           "hipMalloc(first hits)");
+            if (rc != AR_OK) return rc;
+            b.fh_cap = b.capacity;
+        }
+    }
     if (J.guiding) {
         rc = guide_ensure(ctx, L.cap, !J.learned);
         if (rc != AR_OK) return rc;
@@ -1938,9 +2003,10 @@ static int enqueue_job(ar_ctx* ctx, RenderJob& J, hipStream_t stream) {
     if (rc != AR_OK) return rc;
     RenderBuffers& b = ctx->rb;
     const uint32_t blk = ctx->tile;
+    const bool fh = wants_first_hits(L);
     PathArgs S{b.mask[0], b.mask[1], b.mask[2], b.mask[3], b.rad[0], b.rad[1], b.rad[2], b.rad[3],
                b.last_pos[0], b.last_pos[1], b.last_pos[2], b.last_pdf, b.meta, b.lambda0, b.pix, b.smp, b.scn,
-               b.med, b.mpos[0], b.mpos[1], b.mpos[2]};
+               b.med, b.mpos[0], b.mpos[1], b.mpos[2], fh ? b.fh : nullptr, (uint32_t)b.fh_cap};
     const bool sdf = ctx->scene.nsdf > 0;
     SceneArgs sc{ctx->scene.prims, ctx->scene.objs, ctx->scene.ntri, ctx->scene.mats, ctx->scene.light_tri,
                  ctx->scene.light_area, ctx->scene.light_prob, ctx->scene.light_cdf, b.cie, b.d65, ctx->sdf.dev,
@@ -2044,7 +2110,7 @@ int render(ar_ctx* ctx, const ar_camera* cam, const ar_render_opts* opts, float*
 }
 
 int render_progressive(ar_ctx* ctx, const ar_camera* cam, const ar_render_opts* opts, uint32_t frame, float* accum,
-                       float* odd, float* even, hipStream_t stream) {
+                       float* odd, float* even, float* gbuffer, float* stats, hipStream_t stream) {
     if (!ctx->job) ctx->job = new RenderJob();
     RenderJob& J = *ctx->job;
     ar_render_opts o = *opts;
@@ -2056,6 +2122,8 @@ int render_progressive(ar_ctx* ctx, const ar_camera* cam, const ar_render_opts* 
     v.prog = 1u;
     v.odd = odd;
     v.even = even;
+    v.gbuf = gbuffer;
+    v.stats = stats;
     rc = prepare_job(ctx, J, stream, false);
     if (rc == AR_OK) rc = enqueue_job(ctx, J, stream);
     return rc;

@@ -377,7 +377,31 @@ int ar_autotune_batch(ar_ctx* const* scenes, uint32_t n, const ar_camera* cams, 
  * as the reference does: AR_OUTPUT_SRGB clamps each frame's mean at zero
  * before the fold. */
 int ar_render_progressive(ar_ctx* ctx, const ar_camera* cam, const ar_render_opts* opts, uint32_t frame,
-                          float* accum, float* odd, float* even, void* stream);
+                          float* accum, float* odd, float* even, float* gbuffer, float* stats, void* stream);
+
+/* --- screen-space denoiser -------------------------------------------------
+ * The denoiser's per-pixel records, kept by ar_render_progressive when
+ * gbuffer / stats are given (NULL: not kept; both restart at frame 0):
+ *   gbuffer, AR_GBUFFER_FLOATS per pixel (RecordScreenGBuffer,
+ *     core/denoise/denoise.cuh:82-163): running means over the samples that
+ *     hit of the first hit's albedo[3], normal[3], distance, metallic and
+ *     roughness, the shading-model frequencies over all samples (diffuse,
+ *     light, glass, GGX, none), then the hit and sample counts (uint32 bits);
+ *   stats, AR_STATS_FLOATS per pixel (RecordScreenStatisticBuffer,
+ *     denoise_statistic.cuh:36-69): sample count (uint32 bits), Welford mean,
+ *     M2 / n and M3 / n of the Box-Cox (lambda 1/2) transformed samples.
+ * Means are kept in float (the reference keeps the G-buffer in half). */
+#define AR_GBUFFER_FLOATS 16
+#define AR_STATS_FLOATS 12
+
+/* Bilateral16x16<7> (denoise.cuh:352-444): each output pixel averages the
+ * 15 x 15 window of hdr in log(1 + c), weighted by the G-buffer similarity
+ * (ComputeSVGFWeight), a spatial Gaussian of sigma 2 and a Welch t-test of
+ * the two pixels' skewness-corrected Box-Cox means at the 0.5 % level
+ * (ComputeTStatisticFinal); borders clamp.  hdr / out: width*height*3
+ * floats (device; out must not alias hdr). */
+int ar_denoise(ar_ctx* ctx, const float* hdr, const float* gbuffer, const float* stats, float* out, uint32_t width,
+               uint32_t height, void* stream);
 
 #define AR_TONE_LINEAR 0u
 #define AR_TONE_GAMMA 1u      /* pow(x, 1/2.2) (core/tonemap.cuh:6-9) */

@@ -318,6 +318,8 @@ typedef struct {
     const orc_object* objects;     /* analytic objects (may be NULL) */
     uint32_t nobj;
     const orc_material* obj_mats;  /* nobj materials */
+    float* gbuffer;                /* optional: 16 floats per pixel, folded per sample (orc_gbuffer_record) */
+    float* stats;                  /* optional: 12 floats per pixel, folded per sample (orc_stats_record) */
 } orc_render_opts;
 
 /* Sigmoid coefficients of an RGB colour (channels clamped to [0, 1]) from the
@@ -354,6 +356,21 @@ void orc_tonemap(const float* hdr, uint32_t* ldr, uint32_t npix, float exposure,
 double orc_luminance_mse(const float* a, const float* b, size_t npix);
 /* running mean: acc (mean of n_before samples) takes in the mean of n_new more */
 void orc_running_mean(float* acc, const float* mean_new, size_t nfloat, uint32_t n_before, uint32_t n_new);
+
+/* ---------------- denoiser (denoise.c) ---------------- */
+
+/* Records per pixel (uint32 counts stored as their bit patterns):
+ *   G-buffer (16 floats): albedo[3] normal[3] depth metallic roughness
+ *     model[5] (diffuse, light, glass, GGX, none) valid-frame count, frame count
+ *   statistics (12 floats): n mean[3] M2/n[3] M3/n[3], 2 unused */
+void orc_gbuffer_record(float* g, int hit, const float albedo[3], const float normal[3], float depth, float metallic,
+                        float roughness, uint32_t model);
+void orc_stats_record(float* st, const float x[3]);
+float orc_t_quantile(int nu);
+float orc_svgf_weight(const float* ga, const float* gb);
+float orc_tstat_gate(const float* sa, const float* sb);
+void orc_bilateral(const float* hdr, const float* gbuf, const float* stats, float* out, uint32_t width,
+                   uint32_t height);
 
 /* Renders the mean of spp samples per pixel into out (width*height*3,
  * row-major, pixel index = j*width + i).  precision 32: float out, FP32

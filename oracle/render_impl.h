@@ -919,10 +919,16 @@ static int RF(camera_medium)(const orc_render_scene* sc, V3 o)
     return -1;
 }
 
+/* fh (optional, 11 floats): the first vertex for the denoiser's G-buffer
+ * (RecordScreenGBuffer at depth 0, render.cuh:640-643): hit, albedo[3],
+ * normal[3] (the intersector's for objects, the one facing the ray for
+ * triangles), distance, metallic, roughness, shading model (0 diffuse,
+ * 1 light, 2 glass, 3 GGX, 4 none) */
 static V3 RF(trace_path)(const orc_render_scene* sc, const orc_render_lights* L, const orc_render_opts* op,
                          uint32_t pix, uint32_t smp, V3 o, V3 d, R lambda0,
-                         const orc_guide* G, float alpha, RF(gsample)* gout, int* ngout)
+                         const orc_guide* G, float alpha, RF(gsample)* gout, int* ngout, float* fh)
 {
+    if (fh) { for (int k = 0; k < 11; ++k) fh[k] = 0.0f; fh[10] = 4.0f; }
     const int mode = op->spectral;
     const RF(lam) LM = RF(make_lam)(mode, lambda0);
     S4 rad = RF(s4)(0, 0, 0, 0), mask = RF(s4)(1, 1, 1, 1);
@@ -1018,6 +1024,15 @@ static V3 RF(trace_path)(const orc_render_scene* sc, const orc_render_lights* L,
         }
         const RF(mat) m = RF(material)(sc, prim);
         const int is_light = sc->light_index[prim] >= 0;
+        if (fh && depth == 0) {
+            fh[0] = 1.0f;
+            fh[1] = (float)m.albedo.x; fh[2] = (float)m.albedo.y; fh[3] = (float)m.albedo.z;
+            fh[4] = (float)n.x; fh[5] = (float)n.y; fh[6] = (float)n.z;
+            fh[7] = (float)t;
+            fh[8] = (float)m.metallic;
+            fh[9] = (float)m.roughness;
+            fh[10] = is_light ? 1.0f : (m.model == 2 ? 2.0f : (m.model == 1 ? 3.0f : 0.0f));
+        }
         if (is_light) {
             /* triangle lights emit from the front face; box and sphere lights
              * from every hit (MAT_LIGHT objects, render.cuh:664-710) */
@@ -1324,7 +1339,7 @@ static V3 RF(camera_dir)(const orc_render_opts* op, uint32_t i, uint32_t j, R u0
 /* one camera sample of pixel px (RenderCamera, core/scene.cu:135-189) */
 static V3 RF(sample_pixel)(const orc_render_scene* sc, const orc_render_lights* L, const orc_render_opts* op,
                            long long px, uint32_t smp, const orc_guide* G, float alpha, RF(gsample)* gout,
-                           int* ngout)
+                           int* ngout, float* fh)
 {
     const uint32_t i = (uint32_t)(px % op->width), j = (uint32_t)(px / op->width);
     float uf[8];
@@ -1337,7 +1352,7 @@ static V3 RF(sample_pixel)(const orc_render_scene* sc, const orc_render_lights* 
         for (int k = 0; k < 4; ++k) w[k] = uf[4 + k];
         l0 = op->spectral == 1 ? RF(sample_lambda)((R)w[0], (R)w[1], (R)w[2]) : (R)380 + (R)400 * (R)w[0];
     }
-    return RF(trace_path)(sc, L, op, (uint32_t)px, smp, RF(load3)(op->cam_pos), d, l0, G, alpha, gout, ngout);
+    return RF(trace_path)(sc, L, op, (uint32_t)px, smp, RF(load3)(op->cam_pos), d, l0, G, alpha, gout, ngout, fh);
 }
 
 static void RF(finish_pixel)(const orc_render_opts* op, V3 acc, R* o)
@@ -1363,8 +1378,21 @@ static void RF(render)(const orc_render_scene* sc, const orc_render_lights* L, c
 #pragma omp parallel for schedule(dynamic, 16) num_threads(threads)
         for (long long px = 0; px < npix; ++px) {
             V3 acc = RF(v3)(0, 0, 0);
-            for (uint32_t s = 0; s < (uint32_t)op->spp; ++s)
-                acc = RF(add)(acc, RF(sample_pixel)(sc, L, op, px, s + op->sample_offset, NULL, 0.0f, NULL, NULL));
+            for (uint32_t s = 0; s < (uint32_t)op->spp; ++s) {
+                float fh[11];
+                const int aux = op->gbuffer || op->stats;
+                const V3 v = RF(sample_pixel)(sc, L, op, px, s + op->sample_offset, NULL, 0.0f, NULL, NULL,
+                                              aux ? fh : NULL);
+                acc = RF(add)(acc, v);
+                /* the denoiser's per-pixel records, folded in sample order */
+                if (op->gbuffer)
+                    orc_gbuffer_record(&op->gbuffer[16 * px], fh[0] != 0.0f, &fh[1], &fh[4], fh[7], fh[8], fh[9],
+                                       (uint32_t)fh[10]);
+                if (op->stats) {
+                    const float x[3] = { (float)v.x, (float)v.y, (float)v.z };
+                    orc_stats_record(&op->stats[12 * px], x);
+                }
+            }
             RF(finish_pixel)(op, acc, &out[3 * px]);
         }
         return;
@@ -1384,7 +1412,8 @@ static void RF(render)(const orc_render_scene* sc, const orc_render_lights* L, c
         const float alpha = orc_guide_alpha(smp);
 #pragma omp parallel for schedule(dynamic, 16) num_threads(threads)
         for (long long px = 0; px < npix; ++px)
-            acc[px] = RF(add)(acc[px], RF(sample_pixel)(sc, L, op, px, smp, op->guide, alpha, &gs[5 * px], &gn[px]));
+            acc[px] = RF(add)(acc[px], RF(sample_pixel)(sc, L, op, px, smp, op->guide, alpha, &gs[5 * px], &gn[px],
+                                                        NULL));
         int ns = 0;
         for (long long px = 0; px < npix; ++px)
             for (int k = 0; k < gn[px]; ++k, ++ns) {

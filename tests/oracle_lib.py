@@ -39,7 +39,7 @@ def load():
     global _lib
     if _lib is not None:
         return _lib
-    srcs = [ORACLE_DIR / n for n in ("trace.c", "render.c", "render_impl.h", "guide.c", "mlp.c", "output.c", "oracle.h", "sdf_impl.h")]
+    srcs = [ORACLE_DIR / n for n in ("trace.c", "render.c", "render_impl.h", "guide.c", "mlp.c", "output.c", "denoise.c", "oracle.h", "sdf_impl.h")]
     if not LIB.exists() or any(s.stat().st_mtime > LIB.stat().st_mtime for s in srcs if s.exists()):
         subprocess.run(["make", "-s", "-C", str(ORACLE_DIR)], check=True)
     lib = C.CDLL(str(LIB))
@@ -345,7 +345,8 @@ class orc_render_opts(C.Structure):
                 ("sky", C.c_int), ("env", C.c_float * 3), ("cam_pos", C.c_float * 3), ("cam_dir", C.c_float * 3),
                 ("vfov_deg", C.c_float), ("spectral", C.c_int), ("output", C.c_int), ("cie", C.c_void_p),
                 ("d65", C.c_void_p), ("lut", C.c_void_p), ("guiding", C.c_int), ("guide", C.c_void_p),
-                ("objects", C.c_void_p), ("nobj", C.c_uint32), ("obj_mats", C.c_void_p)]
+                ("objects", C.c_void_p), ("nobj", C.c_uint32), ("obj_mats", C.c_void_p), ("gbuffer", C.c_void_p),
+                ("stats", C.c_void_p)]
 
 
 CIE_TABLE = ROOT / "aperturerendererv3_amd" / "data" / "cie1931_d65_1nm.txt"
@@ -435,7 +436,7 @@ def make_materials(mats):
 
 def render(V, F, tri_mat, mats, width, height, spp, seed=1, max_depth=5, nee=1, sky=0, env=(0, 0, 0),
            cam_pos=(0, 0, 1.1), cam_dir=(0, 0, -1), vfov=60.0, precision=32, rr_min_depth=3, sample_offset=0,
-           threads=0, spectral=0, output=0, guide=None, objects=None):
+           threads=0, spectral=0, output=0, guide=None, objects=None, gbuffer=None, stats=None):
     """Oracle render; returns (height, width, 3) float32 or float64 mean image.
     objects: analytic objects as dicts (type, center, extent, rotation,
     material as a dict of ar_material fields)."""
@@ -458,6 +459,12 @@ def render(V, F, tri_mat, mats, width, height, spp, seed=1, max_depth=5, nee=1, 
         o.lut = uplift_lut().ctypes.data
     if guide is not None:
         o.guiding, o.guide = 1, guide.handle
+    if gbuffer is not None:
+        assert gbuffer.dtype == np.float32 and gbuffer.size == width * height * 16 and gbuffer.flags["C_CONTIGUOUS"]
+        o.gbuffer = gbuffer.ctypes.data
+    if stats is not None:
+        assert stats.dtype == np.float32 and stats.size == width * height * 12 and stats.flags["C_CONTIGUOUS"]
+        o.stats = stats.ctypes.data
     if objects:
         oarr = make_objects(objects)
         omats = make_materials([ob.get("material") or {} for ob in objects])
@@ -517,3 +524,42 @@ class Guide:
         dirs = np.ascontiguousarray(dirs, np.float32)
         rad = np.ascontiguousarray(rad, np.float32)
         self.lib.orc_guide_update(self.handle, keys.size, keys.ctypes.data, dirs.ctypes.data, rad.ctypes.data)
+
+
+# ---------------------------------------------------------------- denoiser
+
+def gbuffer_record(g, hit, albedo, normal, depth, metallic, roughness, model):
+    f = fn("orc_gbuffer_record", None, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_float, C.c_float, C.c_float,
+                                        C.c_uint32])
+    a = np.ascontiguousarray(albedo, np.float32)
+    n = np.ascontiguousarray(normal, np.float32)
+    f(g.ctypes.data, int(hit), a.ctypes.data, n.ctypes.data, depth, metallic, roughness, int(model))
+
+
+def stats_record(st, x):
+    x = np.ascontiguousarray(x, np.float32)
+    fn("orc_stats_record", None, [C.c_void_p, C.c_void_p])(st.ctypes.data, x.ctypes.data)
+
+
+def tstat_gate(sa, sb):
+    return fn("orc_tstat_gate", C.c_float, [C.c_void_p, C.c_void_p])(sa.ctypes.data, sb.ctypes.data)
+
+
+def svgf_weight(ga, gb):
+    return fn("orc_svgf_weight", C.c_float, [C.c_void_p, C.c_void_p])(ga.ctypes.data, gb.ctypes.data)
+
+
+def t_quantile(nu):
+    return fn("orc_t_quantile", C.c_float, [C.c_int])(int(nu))
+
+
+def bilateral(hdr, gbuf, stats):
+    """orc_bilateral of (h, w, 3) hdr, (h, w, 16) G-buffer and (h, w, 12) statistics."""
+    h, w = hdr.shape[:2]
+    hdr = np.ascontiguousarray(hdr, np.float32)
+    gbuf = np.ascontiguousarray(gbuf, np.float32)
+    stats = np.ascontiguousarray(stats, np.float32)
+    out = np.zeros_like(hdr)
+    fn("orc_bilateral", None, [C.c_void_p] * 4 + [C.c_uint32, C.c_uint32])(
+        hdr.ctypes.data, gbuf.ctypes.data, stats.ctypes.data, out.ctypes.data, w, h)
+    return out
