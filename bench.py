@@ -367,10 +367,12 @@ def mlp_bench(ctx_cls, reps=10):
 
 def output_bench(ctx_cls, reps=20, cpu=True):
     """Output stage at 3840x2160: the ACES tonemap kernel (12 B read + 4 B
-    written per pixel, HBM-bound), the luminance MSE of two HDR images (24 B
-    read per pixel), and progressive 1-spp frames with odd / even halves on
-    the Cornell box at 1920x1080 against plain 1-spp renders.  CPU baseline:
-    the oracle tonemap on the same pixels (one core)."""
+    written per pixel), the luminance MSE of two HDR images (24 B read per
+    pixel), progressive 1-spp frames with odd / even halves on the Cornell
+    box at 1920x1080 against plain 1-spp renders, the same frames keeping
+    the denoiser's records, and the bilateral denoiser on the 1080p frame.
+    CPU baselines: the oracle tonemap on the same pixels (one core) and the
+    oracle filter on a 480x270 crop (all cores)."""
     import torch
     from aperturerendererv3_amd import AR_TONE_ACES
     from aperturerendererv3_amd.workloads import CORNELL_CAMERA, cornell_box
@@ -432,6 +434,31 @@ def output_bench(ctx_cls, reps=20, cpu=True):
                                     "plain_1spp_ms": round(plain_ms, 4),
                                     "paths_per_s": round(fw * fh / (prog_ms * 1e-3), 1),
                                     "variance_1e6_mse": round(ctx.luminance_mse(odd, even) * 1e6, 3)}
+        # denoiser: progressive frames that also keep the G-buffer and sample
+        # statistics, then the bilateral filter of the 1080p frame
+        gb = torch.empty((fh, fw, 16), dtype=torch.float32, device=dev)
+        st = torch.empty((fh, fw, 12), dtype=torch.float32, device=dev)
+        ctx.render_progressive(0, acc, odd, even, spp=1, gbuffer=gb, stats=st, **kw)
+        torch.cuda.synchronize()
+        e0.record(stream)
+        for f in range(frames):
+            ctx.render_progressive(f, acc, odd, even, spp=1, gbuffer=gb, stats=st, **kw)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        rec_ms = e0.elapsed_time(e1) / frames
+        den = torch.empty_like(acc)
+        ctx.denoise(acc, gb, st, out=den)
+        torch.cuda.synchronize()
+        e0.record(stream)
+        for _ in range(reps):
+            ctx.denoise(acc, gb, st, out=den)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        den_ms = e0.elapsed_time(e1) / reps
+        out["denoise_1080p"] = {"frames_with_records_ms": round(rec_ms, 4), "denoise_ms": round(den_ms, 4),
+                                "pixels_per_s": round(fw * fh / (den_ms * 1e-3), 1), "taps_per_pixel": 225}
+        crop = (acc[:270, :480].contiguous().cpu().numpy(), gb[:270, :480].contiguous().cpu().numpy(),
+                st[:270, :480].contiguous().cpu().numpy())
     if cpu:
         sys.path.insert(0, str(ROOT / "tests"))
         import ctypes as C
@@ -444,6 +471,14 @@ def output_bench(ctx_cls, reps=20, cpu=True):
         out["cpu_baseline"] = {"value": round(npix / dt, 1), "unit": "pixels/s (ACES tonemap)", "cores": 1,
                                "kind": "port", "sample": f"the full {w}x{h} frame with the oracle tonemap, {dt:.2f} s"}
         out["tonemap_gpu_pixels_per_s"] = round(npix / (tm_ms * 1e-3), 1)
+        cores = len(os.sched_getaffinity(0))
+        t0 = time.perf_counter()
+        orc.bilateral(*crop)
+        dt = time.perf_counter() - t0
+        out["denoise_1080p"]["cpu_baseline"] = {
+            "value": round(crop[0].shape[0] * crop[0].shape[1] / dt, 1), "unit": "pixels/s (bilateral)",
+            "cores": cores, "kind": "port",
+            "sample": f"a 480x270 crop of the same frame and records with the oracle filter, {dt:.2f} s"}
     return out
 
 
