@@ -77,6 +77,24 @@ struct Collapser {
                 ch.push_back(c.right);
             }
         }
+        // children in ascending order of their centre along the axis of
+        // largest extent of their union (bits 28-29 of meta): traversal pops
+        // the hit children after the nearest in that order or its reverse,
+        // by the sign of the ray's direction on the axis
+        int axis = 0;
+        {
+            float ulo[3] = {INFINITY, INFINITY, INFINITY}, uhi[3] = {-INFINITY, -INFINITY, -INFINITY};
+            for (int c : ch)
+                for (int a = 0; a < 3; ++a) {
+                    ulo[a] = std::min(ulo[a], br.nodes[c].lo[a]);
+                    uhi[a] = std::max(uhi[a], br.nodes[c].hi[a]);
+                }
+            for (int a = 1; a < 3; ++a)
+                if (uhi[a] - ulo[a] > uhi[axis] - ulo[axis]) axis = a;
+            std::stable_sort(ch.begin(), ch.end(), [&](int x, int y) {
+                return br.nodes[x].lo[axis] + br.nodes[x].hi[axis] < br.nodes[y].lo[axis] + br.nodes[y].hi[axis];
+            });
+        }
         const uint32_t gi = (uint32_t)nodes.size();
         nodes.emplace_back();
         uint32_t refs[8];
@@ -100,7 +118,8 @@ struct Collapser {
             eb[a] = exponent_for(hi[a] - lo[a]);
             s[a] = scale_of(eb[a]);
         }
-        g.meta = (uint32_t)eb[0] | ((uint32_t)eb[1] << 8) | ((uint32_t)eb[2] << 16) | ((uint32_t)ch.size() << 24);
+        g.meta = (uint32_t)eb[0] | ((uint32_t)eb[1] << 8) | ((uint32_t)eb[2] << 16) | ((uint32_t)ch.size() << 24) |
+                 ((uint32_t)axis << 28);
         for (int k = 0; k < 8; ++k) g.ref[k] = refs[k];
         for (size_t k = 0; k < ch.size(); ++k) {
             const BuildNode& c = br.nodes[ch[k]];

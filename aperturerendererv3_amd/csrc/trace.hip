@@ -20,6 +20,22 @@
 namespace ar {
 namespace {
 
+// Closest hit pushes the hit children after the nearest in the node's slot
+// order (children are stored along the node's largest axis) rather than
+// sorting them far to near (PUSH_SLOT_ORDER 0): on the 1M-triangle workload
+// the sort costs more VALU than its better culling saves.
+#ifndef AR_TRACE_PUSH_SLOT_ORDER
+#define AR_TRACE_PUSH_SLOT_ORDER 1
+#endif
+// Leaf deferral: a wave runs its leaf branch only when at least 1 / N of its
+// live lanes sit at a leaf (0: every step).
+#ifndef AR_TRACE_LEAF_DEFER_CLOSEST
+#define AR_TRACE_LEAF_DEFER_CLOSEST 3
+#endif
+#ifndef AR_TRACE_LEAF_DEFER_ANY
+#define AR_TRACE_LEAF_DEFER_ANY 3
+#endif
+
 struct RayPre {
     float ox, oy, oz;
     float dx, dy, dz;      // raw direction (analytic-object tests only)
@@ -207,7 +223,7 @@ __device__ __forceinline__ uint32_t interior8(const GpuNode8* __restrict__ nodes
     const uint4 q1 = *reinterpret_cast<const uint4*>(nd + 4);
     const uint4 q2 = *reinterpret_cast<const uint4*>(nd + 5);
     const uint32_t meta = __float_as_uint(h.w);
-    const uint32_t nchild = meta >> 24;
+    const uint32_t nchild = (meta >> 24) & 15u;
     const RayPre& p = s.p;
     const float ax = __uint_as_float((meta & 0xFFu) << 23) * p.idx;
     const float ay = __uint_as_float(((meta >> 8) & 0xFFu) << 23) * p.idy;
@@ -216,21 +232,31 @@ __device__ __forceinline__ uint32_t interior8(const GpuNode8* __restrict__ nodes
     const float by = (h.y - p.oy) * p.idy;
     const float bz = (h.z - p.oz) * p.idz;
     const uint32_t refs[8] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w};
-    const uint32_t lox[2] = {q0.x, q0.y}, loy[2] = {q0.z, q0.w}, loz[2] = {q1.x, q1.y};
-    const uint32_t hix[2] = {q1.z, q1.w}, hiy[2] = {q2.x, q2.y}, hiz[2] = {q2.z, q2.w};
     float tn[8];
     uint32_t mask = 0;
+    // Near and far planes picked once per axis by the sign of the ray's
+    // direction (the quantised lo byte is <= the hi byte, and the scale has
+    // the sign of 1/d), so each child needs no min / max per axis.
+    const bool nxs = p.idx < 0.0f, nys = p.idy < 0.0f, nzs = p.idz < 0.0f;
+    const uint32_t nearx[2] = {nxs ? q1.z : q0.x, nxs ? q1.w : q0.y};
+    const uint32_t farx[2] = {nxs ? q0.x : q1.z, nxs ? q0.y : q1.w};
+    const uint32_t neary[2] = {nys ? q2.x : q0.z, nys ? q2.y : q0.w};
+    const uint32_t fary[2] = {nys ? q0.z : q2.x, nys ? q0.w : q2.y};
+    const uint32_t nearz[2] = {nzs ? q2.z : q1.x, nzs ? q2.w : q1.y};
+    const uint32_t farz[2] = {nzs ? q1.x : q2.z, nzs ? q1.y : q2.w};
+    const float lo_t = s.tmin, hi_t = s.best_t;
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
         const int w = c >> 2, k = c & 3;
-        const float x0 = fmaf(byte_f(lox[w], k), ax, bx), x1 = fmaf(byte_f(hix[w], k), ax, bx);
-        const float y0 = fmaf(byte_f(loy[w], k), ay, by), y1 = fmaf(byte_f(hiy[w], k), ay, by);
-        const float z0 = fmaf(byte_f(loz[w], k), az, bz), z1 = fmaf(byte_f(hiz[w], k), az, bz);
-        const float a = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), s.tmin)) - p.tpad;
-        const float b = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), s.best_t)) + p.tpad;
+        const float xn = fmaf(byte_f(nearx[w], k), ax, bx), xf = fmaf(byte_f(farx[w], k), ax, bx);
+        const float yn = fmaf(byte_f(neary[w], k), ay, by), yf = fmaf(byte_f(fary[w], k), ay, by);
+        const float zn = fmaf(byte_f(nearz[w], k), az, bz), zf = fmaf(byte_f(farz[w], k), az, bz);
+        const float a = fmaxf(fmaxf(xn, yn), fmaxf(zn, lo_t)) - p.tpad;
+        const float b = fminf(fminf(xf, yf), fminf(zf, hi_t)) + p.tpad;
         tn[c] = a;
-        if ((uint32_t)c < nchild && !(a > b)) mask |= 1u << c;   // NaN counts as a hit
+        if (!(a > b)) mask |= 1u << c;   // NaN counts as a hit
     }
+    mask &= (1u << nchild) - 1u;
     if (mask == 0) {
         s.ref = st.pop();
         return nchild;
@@ -253,6 +279,22 @@ __device__ __forceinline__ uint32_t interior8(const GpuNode8* __restrict__ nodes
         for (int c = 0; c < 8; ++c)
             if (((mask >> c) & 1u) && tn[c] < nt) { nt = tn[c]; next = refs[c]; ni = c; }
         mask &= ~(1u << ni);
+#if AR_TRACE_PUSH_SLOT_ORDER
+        // the rest without a sort: children are stored along the node's
+        // largest axis, so popping them in slot order (ray going up that
+        // axis) or reverse slot order (going down) is near to far
+        const uint32_t axis = (meta >> 28) & 3u;
+        const float dax = axis == 0u ? p.idx : (axis == 1u ? p.idy : p.idz);
+        if (dax < 0.0f) {
+#pragma unroll
+            for (int c = 0; c < 8; ++c)
+                if ((mask >> c) & 1u) st.push(refs[c]);
+        } else {
+#pragma unroll
+            for (int c = 7; c >= 0; --c)
+                if ((mask >> c) & 1u) st.push(refs[c]);
+        }
+#else
         while (mask) {
             float ft = -INFINITY;
             uint32_t fr = 0;
@@ -263,6 +305,7 @@ __device__ __forceinline__ uint32_t interior8(const GpuNode8* __restrict__ nodes
             st.push(fr);
             mask &= ~(1u << fi);
         }
+#endif
         s.ref = next;
     }
     return nchild;
@@ -413,8 +456,22 @@ __attribute__((amdgpu_waves_per_eu(ANY ? AR_TRACE_WAVES_ANY : AR_TRACE_WAVES_CLO
         }
         if (__ballot(active) == 0) break;
         if (active) {
-            for (int it = 0; it < kStepBudget && s.ref != kRefEmpty; ++it) {
-                if ((int)s.ref >= 0) {
+            // Lanes that reach a leaf wait while the wave has few of them, so
+            // the wave does not run the leaf branch for one or two lanes every
+            // step; leaves run once at least 1 / kDefer of the live lanes are
+            // at one (always when all are).  kDefer = 0: no waiting.
+            constexpr uint32_t kDefer = ANY ? AR_TRACE_LEAF_DEFER_ANY : AR_TRACE_LEAF_DEFER_CLOSEST;
+            for (int it = 0; it < kStepBudget; ++it) {
+                const bool live = s.ref != kRefEmpty;
+                const bool at_leaf = live && (int)s.ref < 0;
+                const uint64_t am = __ballot(live);
+                if (am == 0) break;
+                bool run_leaves = true;
+                if constexpr (kDefer > 0) {
+                    const uint64_t lm = __ballot(at_leaf);
+                    run_leaves = (uint32_t)__popcll(lm) * kDefer >= (uint32_t)__popcll(am);
+                }
+                if (live && !at_leaf) {
                     if constexpr (W == 8) {
                         const uint32_t c = interior8<ANY>(reinterpret_cast<const GpuNode8*>(nodes), s, st);
                         if (COUNT) nb += c;
@@ -422,7 +479,7 @@ __attribute__((amdgpu_waves_per_eu(ANY ? AR_TRACE_WAVES_ANY : AR_TRACE_WAVES_CLO
                         interior2(reinterpret_cast<const GpuNode2*>(nodes), s, st);
                         if (COUNT) nb += 2;
                     }
-                } else {
+                } else if (at_leaf && run_leaves) {
                     const uint32_t c = leaf<ANY, OBJ>(tris, objs, obj_base, vol, s, st);
                     if (COUNT) nt += c;
                 }
