@@ -11,12 +11,15 @@
 //                       s^2 / n and (s^2 / n)^2 / (n - 1)), 29 floats, SoA.
 //                       These are the same roundings the reference performs
 //                       inside every tap, done once.
-//   k_bilateral         16 x 16 pixels per 256-thread block: the 30 x 30
+//   k_bilateral         16 x 16 pixels per 1024-thread block: the 30 x 30
 //                       feature tile (7-pixel apron, clamped at the border)
-//                       and the t quantile table are staged in LDS (106 KB),
-//                       then each thread walks its 15 x 15 window.  VALU-bound:
-//                       per tap one expf, three divisions and a square root
-//                       per channel for the t statistic.
+//                       and the t quantile table are staged in LDS (106 KB,
+//                       so one block per CU); four threads per pixel walk
+//                       bands of 4, 4, 4 and 3 rows of its 15 x 15 window
+//                       and their sums are added in row order, which gives
+//                       each SIMD four waves to hide the latency of the LDS
+//                       reads and of the IEEE divisions and square roots of
+//                       the t statistic.
 #include "ar_internal.hpp"
 #include "denoise.hpp"
 
@@ -31,6 +34,12 @@ constexpr int kSpan = kTile + 2 * kRadius;     // 30
 constexpr int kSpanPix = kSpan * kSpan;        // 900
 constexpr int kFeat = 29;
 constexpr int kQuantiles = 1024;
+#ifndef AR_DENOISE_SPLIT
+#define AR_DENOISE_SPLIT 4
+#endif
+constexpr int kSplit = AR_DENOISE_SPLIT;        // threads per pixel, each taking a band of window rows
+constexpr int kThreads = kSplit * kTile * kTile;
+static_assert(kThreads <= 1024, "block too large");
 
 // feature planes (each npix floats)
 enum : int {
@@ -51,13 +60,14 @@ __device__ const float kTQuantile[kQuantiles] = {
 #include "t_quantile_table.inc"
 };
 
-// float -> int as the hardware converts: NaN -> 0, saturating
-__device__ __forceinline__ int floor_to_int(float x) {
-    if (x != x) return 0;
-    const float f = floorf(x);
-    if (f >= 2147483648.0f) return 2147483647;
-    if (f < -2147483648.0f) return (-2147483647 - 1);
-    return (int)f;
+// LookupTQuantile005(floorToInt(nu)) index: nu - 1 clamped to the table,
+// NaN and nu < 1 giving 0 (floorToInt of NaN is 0 on the GPU), branch-free
+__device__ __forceinline__ uint32_t quantile_index(float nu) {
+    float f = floorf(nu);
+    f = nu != nu ? 0.0f : f;
+    f = fminf(fmaxf(f, 0.0f), (float)kQuantiles);
+    const int k = (int)f - 1;
+    return (uint32_t)(k < 0 ? 0 : k);
 }
 
 __global__ __launch_bounds__(256) void k_denoise_features(const float* __restrict__ hdr, const float* __restrict__ gbuf,
@@ -98,16 +108,16 @@ __global__ __launch_bounds__(256) void k_denoise_features(const float* __restric
     }
 }
 
-__global__ __launch_bounds__(256) void k_bilateral(const float* __restrict__ feat, float* __restrict__ out,
+__global__ __launch_bounds__(kThreads) void k_bilateral(const float* __restrict__ feat, float* __restrict__ out,
                                                    int width, int height) {
     __shared__ float tile[kFeat][kSpanPix];
     __shared__ float tq[kQuantiles];
     const int tx = threadIdx.x, ty = threadIdx.y;
-    const int tid = ty * kTile + tx;
+    const int tid = ((int)threadIdx.z * kTile + ty) * kTile + tx;
     const int gx0 = blockIdx.x * kTile - kRadius, gy0 = blockIdx.y * kTile - kRadius;
     const size_t npix = (size_t)width * height;
-    for (int k = tid; k < kQuantiles; k += kTile * kTile) tq[k] = kTQuantile[k];
-    for (int s = tid; s < kSpanPix; s += kTile * kTile) {
+    for (int k = tid; k < kQuantiles; k += kThreads) tq[k] = kTQuantile[k];
+    for (int s = tid; s < kSpanPix; s += kThreads) {
         const int sy = s / kSpan, sx = s - sy * kSpan;
         const int gy = min(max(gy0 + sy, 0), height - 1);
         const int gx = min(max(gx0 + sx, 0), width - 1);
@@ -117,7 +127,7 @@ __global__ __launch_bounds__(256) void k_bilateral(const float* __restrict__ fea
     }
     __syncthreads();
     const int ox = blockIdx.x * kTile + tx, oy = blockIdx.y * kTile + ty;
-    if (ox >= width || oy >= height) return;
+    const bool inside = ox < width && oy < height;   // outside pixels still reach both barriers
     const int cs = (ty + kRadius) * kSpan + tx + kRadius;
     float c_alb[3], c_nrm[3], c_hist[5], c_th[3], c_vt[3], c_u[3], c_l[3];
 #pragma unroll
@@ -134,7 +144,17 @@ __global__ __launch_bounds__(256) void k_bilateral(const float* __restrict__ fea
     for (int c = 0; c < 3; ++c) acc[c] = tile[F_LOG + c][cs] * 1.0e-6f;     // CENTER_PIXEL_EPS
     float aw = 1.0e-6f;
     constexpr float kInvModel = 1.0f / (2.0f * 0.1f * 0.1f);                 // FilterParams, in float
-    for (int j = -kRadius; j <= kRadius; ++j) {
+    // the window's rows are split between the two threads of the pixel
+    // (threadIdx.z); the halves are added at the end
+    const int band = (2 * kRadius + 1 + kSplit - 1) / kSplit;
+    const int j0 = -kRadius + (int)threadIdx.z * band;
+    const int j1 = min(j0 + band - 1, kRadius);
+    if (threadIdx.z != 0) {
+        acc[0] = acc[1] = acc[2] = 0.0f;
+        aw = 0.0f;
+    }
+    for (int j = j0; j <= j1; ++j) {
+#pragma unroll 3
         for (int i = -kRadius; i <= kRadius; ++i) {
             const int s = cs + j * kSpan + i;
             // ComputeSVGFWeight
@@ -184,16 +204,28 @@ __global__ __launch_bounds__(256) void k_bilateral(const float* __restrict__ fea
                 const float T = sqrtf(0.5f / (1.0f - asym) - 1.0f);
                 const float uu = c_u[c] + tile[F_U + c][s];
                 const float nu = uu * uu / (c_l[c] + tile[F_L + c][s]);
-                const int q = floor_to_int(nu);
-                uint32_t qi = q <= 0 ? 0u : (uint32_t)(q - 1);
-                qi = qi > (uint32_t)(kQuantiles - 1) ? (uint32_t)(kQuantiles - 1) : qi;
-                pass = pass && (T < tq[qi]);
+                pass = pass & (T < tq[quantile_index(nu)]);
             }
             const float wt = svgf * expf((float)(-(i * i + j * j)) * 0.125f) * (pass ? 1.0f : 0.0f);
 #pragma unroll
             for (int c = 0; c < 3; ++c) acc[c] = acc[c] + tile[F_LOG + c][s] * wt;
             aw = aw + wt;
         }
+    }
+    __syncthreads();                       // the tile is no longer read; its first planes take the bands
+    float* part = &tile[0][0];
+    const int slot = ty * kTile + tx;
+    if (threadIdx.z != 0) {
+        float* q = part + 4 * ((threadIdx.z - 1) * kTile * kTile + slot);
+        q[0] = acc[0]; q[1] = acc[1]; q[2] = acc[2]; q[3] = aw;
+    }
+    __syncthreads();
+    if (threadIdx.z != 0 || !inside) return;
+    for (int z = 1; z < kSplit; ++z) {      // bands added in row order
+        const float* q = part + 4 * ((z - 1) * kTile * kTile + slot);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) acc[c] = acc[c] + q[c];
+        aw = aw + q[3];
     }
     const size_t o = (size_t)oy * width + ox;
 #pragma unroll
@@ -226,7 +258,7 @@ extern "C" int ar_denoise(ar_ctx* ctx, const float* hdr, const float* gbuffer, c
     hipStream_t s = (hipStream_t)stream;
     hipLaunchKernelGGL(k_denoise_features, dim3((uint32_t)((npix + 255) / 256)), dim3(256), 0, s, hdr, gbuffer, stats,
                        ctx->denoise_feat, (uint32_t)npix);
-    hipLaunchKernelGGL(k_bilateral, dim3((width + kTile - 1) / kTile, (height + kTile - 1) / kTile), dim3(kTile, kTile),
-                       0, s, ctx->denoise_feat, out, (int)width, (int)height);
+    hipLaunchKernelGGL(k_bilateral, dim3((width + kTile - 1) / kTile, (height + kTile - 1) / kTile),
+                       dim3(kTile, kTile, kSplit), 0, s, ctx->denoise_feat, out, (int)width, (int)height);
     return check_hip(ctx, hipGetLastError(), "denoise launch");
 }

@@ -283,9 +283,9 @@ __device__ __forceinline__ uint32_t interior8(const GpuNode8* __restrict__ nodes
         // the rest without a sort: children are stored along the node's
         // largest axis, so popping them in slot order (ray going up that
         // axis) or reverse slot order (going down) is near to far
-        const uint32_t axis = (meta >> 28) & 3u;
-        const float dax = axis == 0u ? p.idx : (axis == 1u ? p.idy : p.idz);
-        if (dax < 0.0f) {
+        const uint32_t paxis = (meta >> 28) & 3u;
+        const float pdir = paxis == 0u ? p.idx : (paxis == 1u ? p.idy : p.idz);
+        if (pdir < 0.0f) {
 #pragma unroll
             for (int c = 0; c < 8; ++c)
                 if ((mask >> c) & 1u) st.push(refs[c]);
