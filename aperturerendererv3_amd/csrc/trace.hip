@@ -29,6 +29,11 @@ namespace {
 #endif
 // Leaf deferral: a wave runs its leaf branch only when at least 1 / N of its
 // live lanes sit at a leaf (0: every step).
+// Hit children pushed by rank without a branch per child (Stack::push_mask)
+// rather than one conditional push each.
+#ifndef AR_TRACE_PUSH_MASK
+#define AR_TRACE_PUSH_MASK 1
+#endif
 #ifndef AR_TRACE_LEAF_DEFER_CLOSEST
 #define AR_TRACE_LEAF_DEFER_CLOSEST 3
 #endif
@@ -171,6 +176,33 @@ struct Stack {
         --sp;
         return (uint32_t)(sp < kLdsStack ? lds[sp * kTraceBlock] : spill[(size_t)(sp - kLdsStack) * threads]);
     }
+    // Pushes refs[c] for every set bit c of mask: in slot order 0..7 when
+    // up is false (slot 7 pops first), else 7..0 (slot 0 pops first).  When
+    // all of them fit in LDS every child is written without a branch, to its
+    // rank's entry or, if not set, to the lane's spare entry kLdsStack.
+    __device__ __forceinline__ void push_mask(const uint32_t refs[8], uint32_t mask, bool up) {
+        const int n = __popc(mask);
+        if (sp + n <= kLdsStack) {
+            int rank = 0;
+            const int top = sp + n - 1;
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                const bool hit = (mask >> c) & 1u;
+                const int pos = up ? top - rank : sp + rank;
+                lds[(hit ? pos : kLdsStack) * kTraceBlock] = (int)refs[c];
+                rank += hit ? 1 : 0;
+            }
+            sp += n;
+        } else if (up) {
+#pragma unroll
+            for (int c = 7; c >= 0; --c)
+                if ((mask >> c) & 1u) push(refs[c]);
+        } else {
+#pragma unroll
+            for (int c = 0; c < 8; ++c)
+                if ((mask >> c) & 1u) push(refs[c]);
+        }
+    }
 };
 
 // Per-lane traversal state, kept in registers across refill rounds.
@@ -262,6 +294,15 @@ __device__ __forceinline__ uint32_t interior8(const GpuNode8* __restrict__ nodes
         return nchild;
     }
     if (ANY) {
+#if AR_TRACE_PUSH_MASK
+        // the lowest hit slot next, the others pushed so that the highest pops first
+        const int ni = __ffs(mask) - 1;
+        uint32_t next = refs[0];
+#pragma unroll
+        for (int c = 1; c < 8; ++c)
+            if (c == ni) next = refs[c];
+        st.push_mask(refs, mask & (mask - 1u), false);
+#else
         uint32_t next = kRefEmpty;
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
@@ -270,6 +311,7 @@ __device__ __forceinline__ uint32_t interior8(const GpuNode8* __restrict__ nodes
                 else st.push(refs[c]);
             }
         }
+#endif
         s.ref = next;
     } else {
         float nt = INFINITY;
@@ -285,6 +327,9 @@ __device__ __forceinline__ uint32_t interior8(const GpuNode8* __restrict__ nodes
         // axis) or reverse slot order (going down) is near to far
         const uint32_t paxis = (meta >> 28) & 3u;
         const float pdir = paxis == 0u ? p.idx : (paxis == 1u ? p.idy : p.idz);
+#if AR_TRACE_PUSH_MASK
+        st.push_mask(refs, mask, !(pdir < 0.0f));
+#else
         if (pdir < 0.0f) {
 #pragma unroll
             for (int c = 0; c < 8; ++c)
@@ -294,6 +339,7 @@ __device__ __forceinline__ uint32_t interior8(const GpuNode8* __restrict__ nodes
             for (int c = 7; c >= 0; --c)
                 if ((mask >> c) & 1u) st.push(refs[c]);
         }
+#endif
 #else
         while (mask) {
             float ft = -INFINITY;
@@ -386,7 +432,7 @@ __attribute__((amdgpu_waves_per_eu(ANY ? AR_TRACE_WAVES_ANY : AR_TRACE_WAVES_CLO
     const float* __restrict__ vol, ar_rays r, size_t n, const uint32_t* __restrict__ n_dev, ar_hits h, uint32_t* __restrict__ occ,
     int* __restrict__ spill, uint32_t threads, uint32_t* __restrict__ cnt_box, uint32_t* __restrict__ cnt_tri,
     uint32_t* __restrict__ fetch) {
-    __shared__ int lds_stack[kLdsStack * kTraceBlock];
+    __shared__ int lds_stack[(kLdsStack + 1) * kTraceBlock];   // + one spare entry per lane (push_mask)
     if (n_dev) n = *n_dev;
     const uint32_t gtid = blockIdx.x * kTraceBlock + threadIdx.x;
     const uint32_t lane = __lane_id();
