@@ -380,6 +380,34 @@ int orc_render(const float* verts, uint32_t nv, const uint32_t* idx, uint32_t nt
                const orc_material* mats, uint32_t nmats, const orc_render_opts* opts, int precision,
                void* out, int threads);
 
+/* ---------------- spectral basis network (spectral_mlp.c) ---------------- */
+
+/* 16096 parameters in the reference header's layer order; 1 when entry p is
+ * a weight-matrix entry (weight decay applies), 0 for a bias */
+int orc_smlp_param_count(void);
+int orc_smlp_decays(int p);
+/* FP32 inference in the reference's operation order */
+void orc_smlp_rgb_encode(const float* w, const float rgb[3], float out[4]);
+void orc_smlp_lambda_encode(const float* w, float lam, float out[4]);
+float orc_smlp_interact(const float a[4], const float b[4]);
+void orc_smlp_eval(const float* w, size_t n, const float* x, float* out);
+/* PyTorch evaluation with TF32 matmuls, as the reference made its golden vectors */
+void orc_smlp_eval_torch_tf32(const float* w, size_t n, const float* x, float* out);
+/* table queries at a resolution (entries evaluated directly) */
+void orc_smlp_query_rgb(const float* w, int res, const float rgb[3], float out[4]);
+void orc_smlp_lambda_entry(const float* w, int res, int i, float out[4]);
+void orc_smlp_query_lambda(const float* w, int res, float lam, float out[4]);
+float orc_smlp_lut_interact(const float* w, int rgb_res, int lam_res, const float rgb[3], float lam);
+/* training pieces (FP64) */
+double orc_smlp_normal(uint32_t seed, uint32_t step, uint32_t sample, uint32_t slot);
+double orc_smlp_importance2(double lam);
+double orc_smlp_loss_grad(const float* w, uint32_t B, const float* x, const float* y, int noise, uint32_t seed,
+                          uint32_t step, double* grad);
+double orc_smlp_clip(double* grad, double max_norm);
+void orc_smlp_adamw(double* p, double* m, double* v, const double* g, int t, double lr, double b1, double b2,
+                    double eps, double wd, double* ema, double ema_decay);
+double orc_smlp_lr_factor(int step, int total, double warmup_ratio, double min_ratio);
+
 #ifdef __cplusplus
 }
 #endif

@@ -563,3 +563,113 @@ def bilateral(hdr, gbuf, stats):
     fn("orc_bilateral", None, [C.c_void_p] * 4 + [C.c_uint32, C.c_uint32])(
         hdr.ctypes.data, gbuf.ctypes.data, stats.ctypes.data, out.ctypes.data, w, h)
     return out
+
+
+# ---------------- spectral basis network ----------------
+
+SMLP_PARAMS = 16096
+
+
+def smlp_eval(w, x):
+    """orc_smlp_eval: FP32 reflectances of (n, 4) rows (r, g, b, lambda nm)."""
+    w = np.ascontiguousarray(w, np.float32)
+    x = np.ascontiguousarray(x, np.float32).reshape(-1, 4)
+    out = np.empty(len(x), np.float32)
+    fn("orc_smlp_eval", None, [C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p])(_p(w), len(x), _p(x), _p(out))
+    return out
+
+
+def smlp_eval_torch_tf32(w, x):
+    """orc_smlp_eval_torch_tf32: the PyTorch / TF32 evaluation of the golden vectors."""
+    w = np.ascontiguousarray(w, np.float32)
+    x = np.ascontiguousarray(x, np.float32).reshape(-1, 4)
+    out = np.empty(len(x), np.float32)
+    fn("orc_smlp_eval_torch_tf32", None, [C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p])(_p(w), len(x), _p(x),
+                                                                                            _p(out))
+    return out
+
+
+def smlp_encode(w, rgb=None, lam=None):
+    """RGBEncode of an RGB or LambdaEncode of a wavelength (4 floats)."""
+    w = np.ascontiguousarray(w, np.float32)
+    out = np.empty(4, np.float32)
+    if rgb is not None:
+        v = np.ascontiguousarray(rgb, np.float32)
+        fn("orc_smlp_rgb_encode", None, [C.c_void_p] * 3)(_p(w), _p(v), _p(out))
+    else:
+        fn("orc_smlp_lambda_encode", None, [C.c_void_p, C.c_float, C.c_void_p])(_p(w), float(lam), _p(out))
+    return out
+
+
+def smlp_query_rgb(w, res, rgb):
+    w = np.ascontiguousarray(w, np.float32)
+    v = np.ascontiguousarray(rgb, np.float32)
+    out = np.empty(4, np.float32)
+    fn("orc_smlp_query_rgb", None, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p])(_p(w), int(res), _p(v), _p(out))
+    return out
+
+
+def smlp_lambda_entry(w, res, i):
+    w = np.ascontiguousarray(w, np.float32)
+    out = np.empty(4, np.float32)
+    fn("orc_smlp_lambda_entry", None, [C.c_void_p, C.c_int, C.c_int, C.c_void_p])(_p(w), int(res), int(i), _p(out))
+    return out
+
+
+def smlp_query_lambda(w, res, lam):
+    w = np.ascontiguousarray(w, np.float32)
+    out = np.empty(4, np.float32)
+    fn("orc_smlp_query_lambda", None, [C.c_void_p, C.c_int, C.c_float, C.c_void_p])(_p(w), int(res), float(lam),
+                                                                                     _p(out))
+    return out
+
+
+def smlp_lut_interact(w, rgb_res, lam_res, rgb, lam):
+    w = np.ascontiguousarray(w, np.float32)
+    v = np.ascontiguousarray(rgb, np.float32)
+    return fn("orc_smlp_lut_interact", C.c_float, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_float])(
+        _p(w), int(rgb_res), int(lam_res), _p(v), float(lam))
+
+
+def smlp_loss_grad(w, x, y, noise=False, seed=0, step=0):
+    """(loss, FP64 gradient) of one batch, training-mode noise optional."""
+    w = np.ascontiguousarray(w, np.float32)
+    x = np.ascontiguousarray(x, np.float32).reshape(-1, 4)
+    y = np.ascontiguousarray(y, np.float32)
+    g = np.zeros(SMLP_PARAMS, np.float64)
+    loss = fn("orc_smlp_loss_grad", C.c_double, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_int, C.c_uint32,
+                                                 C.c_uint32, C.c_void_p])(
+        _p(w), len(x), _p(x), _p(y), int(noise), int(seed) & 0xFFFFFFFF, int(step), _p(g))
+    return loss, g
+
+
+def smlp_normal(seed, step, sample, slot):
+    return fn("orc_smlp_normal", C.c_double, [C.c_uint32] * 4)(seed, step, sample, slot)
+
+
+def smlp_importance2(lam):
+    return fn("orc_smlp_importance2", C.c_double, [C.c_double])(float(lam))
+
+
+def smlp_decay_mask():
+    f = fn("orc_smlp_decays", C.c_int, [C.c_int])
+    return np.array([f(p) for p in range(SMLP_PARAMS)], bool)
+
+
+def smlp_clip(g, max_norm):
+    g = np.ascontiguousarray(g, np.float64)
+    norm = fn("orc_smlp_clip", C.c_double, [C.c_void_p, C.c_double])(_p(g), float(max_norm))
+    return norm, g
+
+
+def smlp_adamw(p, m, v, g, t, lr, b1=0.9, b2=0.95, eps=1e-8, wd=1e-3, ema=None, ema_decay=0.999):
+    """One AdamW step in place on FP64 arrays p, m, v (and ema)."""
+    for a in (p, m, v, g) + ((ema,) if ema is not None else ()):
+        assert a.dtype == np.float64 and a.flags.c_contiguous
+    fn("orc_smlp_adamw", None, [C.c_void_p] * 4 + [C.c_int] + [C.c_double] * 5 + [C.c_void_p, C.c_double])(
+        _p(p), _p(m), _p(v), _p(g), int(t), lr, b1, b2, eps, wd, _p(ema) if ema is not None else None, ema_decay)
+
+
+def smlp_lr_factor(step, total, warmup_ratio=0.001, min_ratio=0.01):
+    return fn("orc_smlp_lr_factor", C.c_double, [C.c_int, C.c_int, C.c_double, C.c_double])(
+        int(step), int(total), warmup_ratio, min_ratio)
