@@ -149,6 +149,26 @@ class Graph:
             pass
 
 
+def spectral_mlp_init(seed=0):
+    """Initial parameters of the spectral basis network as the reference's
+    trainer sets them (pytorch/spectrum/train_spectrum.py:103-160): Kaiming
+    normal (fan in, ReLU gain: std sqrt(2 / 32)) for the RGB input layer and
+    every block, zero biases, and torch's default for the two output layers
+    (uniform on +-1 / sqrt(32)).  AR_SMLP_PARAMS float32 values."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    w = np.zeros(_lib.SMLP_PARAMS, np.float32)
+    std = np.sqrt(2.0 / 32.0)
+    w[0:1024] = rng.normal(0.0, std, 1024)
+    for k in range(7):
+        w[1056 + k * 1056:1056 + k * 1056 + 1024] = rng.normal(0.0, std, 1024)
+        w[8576 + k * 1056:8576 + k * 1056 + 1024] = rng.normal(0.0, std, 1024)
+    bound = 1.0 / np.sqrt(32.0)
+    w[8448:8576] = rng.uniform(-bound, bound, 128)
+    w[15968:16096] = rng.uniform(-bound, bound, 128)
+    return w
+
+
 class Context:
     """One device context (ar_ctx).  Thread-compatible, not thread-safe."""
 
@@ -469,6 +489,117 @@ class Context:
         self._check(self._lib.ar_denoise(self._h, _ptr(hdr), _ptr(gbuffer), _ptr(stats), _ptr(out), w, h,
                                          _stream_ptr(stream)), "ar_denoise")
         return out
+
+    # ---- spectral basis network (the reference's RGB / wavelength encoders)
+
+    def spectral_mlp_set(self, params):
+        """ar_spectral_mlp_set: AR_SMLP_PARAMS float32 parameters (host array)."""
+        import numpy as np
+        p = np.ascontiguousarray(params, np.float32).reshape(-1)
+        if p.size != _lib.SMLP_PARAMS:
+            raise ValueError(f"expected {_lib.SMLP_PARAMS} parameters, got {p.size}")
+        self._check(self._lib.ar_spectral_mlp_set(self._h, p.ctypes.data), "ar_spectral_mlp_set")
+
+    def spectral_mlp_init(self, seed=0, stream=None):
+        """Sets spectral_mlp_init(seed) parameters, then makes sure training
+        can start: the reference's clamp passes no gradient where the dot
+        product is outside (0, 1), and a draw whose outputs all sit at 0 (or
+        at 1) would never move.  On 4096 probe rows the wavelength output
+        layer is negated while most outputs are 0 and halved while most are
+        1.  Returns the parameters set."""
+        import numpy as np
+        import torch
+        w = spectral_mlp_init(seed)
+        rng = np.random.default_rng(seed + 1)
+        probe = np.empty((4096, 4), np.float32)
+        probe[:, :3] = rng.random((4096, 3))
+        probe[:, 3] = rng.uniform(360.0, 780.0, 4096)
+        xp = torch.from_numpy(probe).cuda()
+        for _ in range(12):
+            self.spectral_mlp_set(w)
+            out = self.spectral_mlp_eval(xp, stream=stream).cpu().numpy()
+            if np.mean(out <= 0.0) > 0.5:
+                w[15968:16096] *= -1.0
+            elif np.mean(out >= 1.0) > 0.5:
+                w[15968:16096] *= 0.5
+            else:
+                break
+        self.spectral_mlp_set(w)
+        return w
+
+    def spectral_mlp_get(self, ema=False):
+        """ar_spectral_mlp_get: the parameters (or their EMA) as a numpy array."""
+        import numpy as np
+        out = np.empty(_lib.SMLP_PARAMS, np.float32)
+        self._check(self._lib.ar_spectral_mlp_get(self._h, out.ctypes.data, 1 if ema else 0), "ar_spectral_mlp_get")
+        return out
+
+    def spectral_mlp_eval(self, x, out=None, stream=None):
+        """ar_spectral_mlp_eval of (n, 4) float32 device rows (r, g, b, nm)."""
+        import torch
+        x = x.contiguous()
+        if out is None:
+            out = torch.empty(x.shape[0], dtype=torch.float32, device=x.device)
+        self._check(self._lib.ar_spectral_mlp_eval(self._h, _ptr(x), _ptr(out), x.shape[0], _stream_ptr(stream)),
+                    "ar_spectral_mlp_eval")
+        return out
+
+    def spectral_mlp_build_tables(self, rgb_res=512, lambda_res=840, stream=None):
+        """ar_spectral_mlp_build_tables (the reference's resolutions by default)."""
+        self._check(self._lib.ar_spectral_mlp_build_tables(self._h, int(rgb_res), int(lambda_res),
+                                                           _stream_ptr(stream)), "ar_spectral_mlp_build_tables")
+
+    def spectral_mlp_read_tables(self, rgb_res, lambda_res):
+        """ar_spectral_mlp_read_tables: ((res, res, res, 4) [b][g][r], (lambda_res, 4)) numpy arrays."""
+        import numpy as np
+        rgb = np.empty((rgb_res, rgb_res, rgb_res, 4), np.float32)
+        lam = np.empty((lambda_res, 4), np.float32)
+        self._check(self._lib.ar_spectral_mlp_read_tables(self._h, rgb.ctypes.data, lam.ctypes.data),
+                    "ar_spectral_mlp_read_tables")
+        return rgb, lam
+
+    def spectral_mlp_query(self, x, out=None, stream=None):
+        """ar_spectral_mlp_query (Interact through the tables) of (n, 4) device rows."""
+        import torch
+        x = x.contiguous()
+        if out is None:
+            out = torch.empty(x.shape[0], dtype=torch.float32, device=x.device)
+        self._check(self._lib.ar_spectral_mlp_query(self._h, _ptr(x), _ptr(out), x.shape[0], _stream_ptr(stream)),
+                    "ar_spectral_mlp_query")
+        return out
+
+    def spectral_mlp_gradient(self, x, y, batch, step=0, seed=0, noise=False, stream=None):
+        """ar_spectral_mlp_gradient: (gradient, loss) device tensors of one batch."""
+        import torch
+        grad = torch.empty(_lib.SMLP_PARAMS, dtype=torch.float32, device=x.device)
+        loss = torch.empty(1, dtype=torch.float32, device=x.device)
+        self._check(self._lib.ar_spectral_mlp_gradient(self._h, _ptr(x), _ptr(y), x.shape[0], int(batch), int(step),
+                                                       int(seed) & 0xFFFFFFFF, 1 if noise else 0, _ptr(grad),
+                                                       _ptr(loss), _stream_ptr(stream)), "ar_spectral_mlp_gradient")
+        return grad, loss
+
+    def spectral_mlp_train(self, x, y, steps, batch=4096, first_step=0, total_steps=0, seed=0, noise=True,
+                           lr=5e-4, weight_decay=1e-3, beta1=0.9, beta2=0.95, eps=1e-8, clip_norm=1.0,
+                           warmup_ratio=0.001, min_lr_ratio=5e-6, ema_decay=0.999, stream=None):
+        """ar_spectral_mlp_train; defaults are the reference's (config.ini,
+        train_spectrum.py).  Returns the per-step losses (device tensor)."""
+        import torch
+        o = _lib.ar_smlp_train_opts(int(steps), int(batch), int(first_step), int(total_steps),
+                                    int(seed) & 0xFFFFFFFF, 1 if noise else 0, lr, weight_decay, beta1, beta2, eps,
+                                    clip_norm, warmup_ratio, min_lr_ratio, ema_decay)
+        hist = torch.empty(max(1, int(steps)), dtype=torch.float32, device=x.device)
+        self._check(self._lib.ar_spectral_mlp_train(self._h, _ptr(x), _ptr(y), x.shape[0], C.byref(o), _ptr(hist),
+                                                    _stream_ptr(stream)), "ar_spectral_mlp_train")
+        return hist[:int(steps)]
+
+    def spectral_uplift_dataset(self, n, seed=0, device="cuda", stream=None):
+        """ar_spectral_uplift_dataset: (x (n, 4), y (n,)) device tensors."""
+        import torch
+        x = torch.empty((int(n), 4), dtype=torch.float32, device=device)
+        y = torch.empty(int(n), dtype=torch.float32, device=device)
+        self._check(self._lib.ar_spectral_uplift_dataset(self._h, int(n), int(seed) & 0xFFFFFFFF, _ptr(x), _ptr(y),
+                                                         _stream_ptr(stream)), "ar_spectral_uplift_dataset")
+        return x, y
 
     def tonemap(self, hdr, exposure=1.0, tone=0, xyz_input=False, out=None, stream=None):
         """ar_tonemap: (h, w, 3) float32 device image -> (h, w) int32 tensor of

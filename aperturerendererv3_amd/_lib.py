@@ -104,6 +104,16 @@ class ar_tune_result(C.Structure):
     _fields_ = [("width", C.c_uint32), ("tile", C.c_uint32), ("ms", (C.c_float * 4) * 2)]
 
 
+class ar_smlp_train_opts(C.Structure):
+    _fields_ = [("steps", C.c_uint32), ("batch", C.c_uint32), ("first_step", C.c_uint32), ("total_steps", C.c_uint32),
+                ("seed", C.c_uint32), ("noise", C.c_uint32), ("lr", C.c_float), ("weight_decay", C.c_float),
+                ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float), ("clip_norm", C.c_float),
+                ("warmup_ratio", C.c_float), ("min_lr_ratio", C.c_float), ("ema_decay", C.c_float)]
+
+
+SMLP_PARAMS = 16096
+
+
 class ar_guiding_info(C.Structure):
     _fields_ = [("probes", C.c_uint64), ("capacity", C.c_uint64), ("samples", C.c_uint64), ("dropped", C.c_uint64),
                 ("device_bytes", C.c_uint64)]
@@ -177,6 +187,18 @@ SIGNATURES = {
                                       C.POINTER(C.c_uint64)]),
     "ar_guiding_restore": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]),
     "ar_spectrum_uplift": (C.c_int, [C.c_void_p, C.c_size_t, C.c_void_p]),
+    "ar_spectral_mlp_set": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "ar_spectral_mlp_get": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
+    "ar_spectral_mlp_eval": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
+    "ar_spectral_mlp_build_tables": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p]),
+    "ar_spectral_mlp_read_tables": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "ar_spectral_mlp_query": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
+    "ar_spectral_mlp_train": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t,
+                                        C.POINTER(ar_smlp_train_opts), C.c_void_p, C.c_void_p]),
+    "ar_spectral_mlp_gradient": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_uint32, C.c_uint32,
+                                           C.c_uint32, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "ar_spectral_uplift_dataset": (C.c_int, [C.c_void_p, C.c_size_t, C.c_uint32, C.c_void_p, C.c_void_p,
+                                             C.c_void_p]),
 }
 
 _lib = None

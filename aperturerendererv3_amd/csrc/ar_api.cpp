@@ -195,6 +195,7 @@ int ar_destroy(ar_ctx* ctx) {
     free_render_buffers(ctx);
     guide_free(ctx);
     mlp_free(ctx);
+    smlp_free(ctx);
     free_job(ctx->job);
     ctx->job = nullptr;
     if (ctx->spill) (void)hipFree(ctx->spill);

@@ -280,6 +280,27 @@ struct MlpDev {
     uint32_t* rcount = nullptr; // records collected this frame
 };
 
+// Spectral basis network (smlp.hip): parameters, AdamW state, training
+// scratch and the uplift tables built from the parameters.
+struct SmlpDev {
+    float* p = nullptr;          // parameters (one allocation: p, m, v, ema, g)
+    float* m = nullptr;
+    float* v = nullptr;
+    float* ema = nullptr;
+    float* g = nullptr;          // gradient of the last batch (after reduction)
+    bool have = false;           // parameters set
+    uint64_t t = 0;              // AdamW steps taken
+    float* part = nullptr;       // per-tile gradient partials [tiles][params]
+    float* part2 = nullptr;      // per-chunk sums [chunks][params]
+    float* misc = nullptr;       // per-block squared norms, per-tile losses
+    size_t part_tiles = 0;
+    float* rgb_lut = nullptr;    // [res^3][4], x fastest
+    uint32_t rgb_res = 0;
+    float* lam_lut = nullptr;    // [res][4]
+    uint32_t lam_res = 0;
+    uint32_t basis = 0;          // AR_BASIS_* used by spectral renders
+};
+
 // One scene of a batch context (ar_batch_create): its place in the merged arrays.
 struct BatchEntry {
     uint32_t root;         // BVH root node
@@ -339,6 +360,7 @@ struct ar_ctx {
     double* reduce = nullptr;             // partial sums of ar_luminance_mse
     ar::SdfPool sdf;                      // signed-distance volumes
     float* denoise_feat = nullptr;        // per-pixel filter features of ar_denoise (29 planes)
+    ar::SmlpDev smlp;                     // spectral basis network
     size_t denoise_floats = 0;
 };
 
@@ -407,6 +429,8 @@ int mlp_ensure(ar_ctx* ctx);
 int mlp_init(ar_ctx* ctx, uint64_t seed);
 int mlp_set_params(ar_ctx* ctx, const float* host);
 void mlp_free(ar_ctx* ctx);
+// smlp.hip
+void smlp_free(ar_ctx* ctx);
 hipError_t mlp_forward(ar_ctx* ctx, const float* x5, size_t n, const uint32_t* n_dev, float* out, bool decoded,
                        hipStream_t s);
 hipError_t mlp_gradient(ar_ctx* ctx, const float* x5, const float* dir, const float* w, size_t n,

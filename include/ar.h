@@ -501,6 +501,84 @@ int ar_spectrum_lut_build(float* out, size_t count);
  * [0, 1]) from the table (builds it on first use).  No device needed. */
 int ar_spectrum_uplift(const float* rgb, size_t n, float* coeffs);
 
+/* --- spectral basis network -------------------------------------------------
+ * The reference's RGB -> spectrum uplift network (core/spectrum/
+ * spectrum_basis.cuh:2273-2470): RGBEncode (sin / cos features of r, g, b
+ * over their maximum s + 0.001 and of s, Linear 32 -> 32 + ReLU, 7 residual
+ * blocks h += ReLU(W h + b), Linear 32 -> 4, times s) and LambdaEncode
+ * (sin / cos features of (lambda - 360) / 420, 7 residual blocks, Linear
+ * 32 -> 4); reflectance clamp(dot, 0, 1).  Parameters: AR_SMLP_PARAMS
+ * floats in the header's order (RGB_IN_W, RGB_IN_B, RGB_BLKk_W, RGB_BLKk_B
+ * for k = 0..6, RGB_OUT_W, LAM_BLKk_W, LAM_BLKk_B, LAM_OUT_W; matrices
+ * row-major [out][in]).  Evaluated and trained with FP32 MFMA. */
+#define AR_SMLP_PARAMS 16096
+
+/* Sets the parameters (host array) and restarts the optimiser state; the
+ * EMA copy starts at the parameters. */
+int ar_spectral_mlp_set(ar_ctx* ctx, const float* params);
+/* Reads the parameters (ema = 0) or their EMA (ema = 1) into a host array;
+ * waits for the context's work. */
+int ar_spectral_mlp_get(ar_ctx* ctx, float* params, int ema);
+/* SpectralInteract(RGBEncode(rgb), LambdaEncode(lambda)) of n rows
+ * x = (r, g, b, lambda nm) (device, n x 4) into out (device, n). */
+int ar_spectral_mlp_eval(ar_ctx* ctx, const float* x, float* out, size_t n, void* stream);
+/* PrecomputeSpectrumLUTsToDevice (spectrum_lut.cuh:12-78): RGB table of
+ * rgb_res^3 entries at voxel centres (the reference uses 512) and wavelength
+ * table of lambda_res entries (840), kept in the context. */
+int ar_spectral_mlp_build_tables(ar_ctx* ctx, uint32_t rgb_res, uint32_t lambda_res, void* stream);
+/* Copies the tables to host arrays (rgb_res^3 x 4, lambda_res x 4 floats;
+ * either may be NULL); waits for the device. */
+int ar_spectral_mlp_read_tables(ar_ctx* ctx, float* rgb_table, float* lambda_table);
+/* Interact (spectrum_lut.cuh:223-235) of n rows x = (r, g, b, lambda nm)
+ * through the tables: trilinear RGB and linear wavelength queries; an RGB
+ * above 1 is divided by its largest channel and the result scaled back. */
+int ar_spectral_mlp_query(ar_ctx* ctx, const float* x, float* out, size_t n, void* stream);
+
+/* Training (pytorch/spectrum/train_spectrum.py:636-752).  Step k of a call
+ * uses dataset rows ((first_step + k) batch + j) mod n, j < batch, and the
+ * reference's training-mode noise when noise != 0 (standard deviations
+ * 0.025 / 256 on r, g, b and 0.025 / 420 on the normalised wavelength of the
+ * first half of the batch, 0.001 on every block's Linear input; drawn from a
+ * counter hash of seed, step, row in batch and slot).  Loss: mean over the
+ * batch and three channels of ((pred - y)(0.25 + max(cie_c(lambda), 0)))^2
+ * with the analytic CIE fits.  Then gradient clipping to clip_norm (0: none),
+ * AdamW (decoupled weight decay on weight matrices only) at
+ * lr * warmup-cosine(first_step + k of total_steps; 0 = first_step + steps)
+ * and the EMA update. */
+typedef struct {
+    uint32_t steps;          /* optimiser steps of this call */
+    uint32_t batch;          /* multiple of 32, at most 262144 (reference 4096) */
+    uint32_t first_step;     /* global step of the first one (schedule, rows, noise) */
+    uint32_t total_steps;    /* schedule length; 0 = first_step + steps */
+    uint32_t seed;
+    uint32_t noise;          /* training-mode noise on / off */
+    float lr;                /* 5e-4 in the reference's config.ini */
+    float weight_decay;      /* 1e-3 */
+    float beta1, beta2;      /* 0.9, 0.95 */
+    float eps;               /* 1e-8 */
+    float clip_norm;         /* 1.0 */
+    float warmup_ratio;      /* 0.001 */
+    float min_lr_ratio;      /* the reference passes lr * 0.01 = 5e-6 */
+    float ema_decay;         /* 0.999 */
+} ar_smlp_train_opts;
+
+/* Runs opts->steps optimiser steps on the dataset x (device, n x 4: r, g, b,
+ * lambda nm), y (device, n); loss_history (device, steps floats, or NULL)
+ * receives each step's loss. */
+int ar_spectral_mlp_train(ar_ctx* ctx, const float* x, const float* y, size_t n, const ar_smlp_train_opts* opts,
+                          float* loss_history, void* stream);
+/* The loss gradient of one batch (rows as above for global step `step`)
+ * into grad (device, AR_SMLP_PARAMS floats) and the loss into *loss (device,
+ * or NULL); parameters unchanged. */
+int ar_spectral_mlp_gradient(ar_ctx* ctx, const float* x, const float* y, size_t n, uint32_t batch, uint32_t step,
+                             uint32_t seed, int noise, float* grad, float* loss, void* stream);
+/* A training set for the uplift this library ships (no measured spectra
+ * needed): n rows drawn like the reference's synthetic branch
+ * (pytorch/spectrum/spectrum_dataset.py:331-361) with the Jakob-Hanika
+ * sigmoid reflectance of each RGB at its wavelength as the target, written
+ * to x (device, n x 4) and y (device, n); waits for the copies. */
+int ar_spectral_uplift_dataset(ar_ctx* ctx, size_t n, uint32_t seed, float* x, float* y, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
