@@ -527,6 +527,12 @@ class Context:
         self.spectral_mlp_set(w)
         return w
 
+    def spectral_set_basis(self, network):
+        """ar_spectral_set_basis: the basis network's tables (True) or the
+        sigmoid uplift (False) for spectral renders."""
+        self._check(self._lib.ar_spectral_set_basis(self._h, _lib.AR_BASIS_NETWORK if network else
+                                                    _lib.AR_BASIS_SIGMOID), "ar_spectral_set_basis")
+
     def spectral_mlp_get(self, ema=False):
         """ar_spectral_mlp_get: the parameters (or their EMA) as a numpy array."""
         import numpy as np

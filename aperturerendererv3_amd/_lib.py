@@ -112,6 +112,8 @@ class ar_smlp_train_opts(C.Structure):
 
 
 SMLP_PARAMS = 16096
+AR_BASIS_SIGMOID = 0
+AR_BASIS_NETWORK = 1
 
 
 class ar_guiding_info(C.Structure):
@@ -187,6 +189,7 @@ SIGNATURES = {
                                       C.POINTER(C.c_uint64)]),
     "ar_guiding_restore": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]),
     "ar_spectrum_uplift": (C.c_int, [C.c_void_p, C.c_size_t, C.c_void_p]),
+    "ar_spectral_set_basis": (C.c_int, [C.c_void_p, C.c_uint32]),
     "ar_spectral_mlp_set": (C.c_int, [C.c_void_p, C.c_void_p]),
     "ar_spectral_mlp_get": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
     "ar_spectral_mlp_eval": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),

@@ -139,11 +139,9 @@ struct GpuMaterial {
     float roughness;
     uint32_t model;
     float escale;
-    float pad0;
-    float calb[3];
-    float pad1;
-    float cem[3];
-    float pad2;
+    float ascale;        // network basis: the albedo's largest channel when above 1
+    float calb[4];       // albedo: sigmoid coefficients (3), or the network's RGB basis (4)
+    float cem[4];        // emission normalised by escale: likewise
     float ior, sigma_a, abbe, tilt;   // glass
 };
 static_assert(sizeof(GpuMaterial) == 96, "GpuMaterial must be 96 bytes");
@@ -167,6 +165,7 @@ struct SceneDev {
     uint32_t ntri = 0;
     std::vector<GpuMaterial> host_mats;   // host copy (spectral coefficients filled on first spectral render)
     bool spectral_ready = false;
+    uint32_t spectral_basis = 0;    // AR_BASIS_* the material records were prepared for
     float4* objs = nullptr;               // analytic objects, obj::kRecord float4 each (objects.hpp)
     uint32_t nobj = 0;
     uint32_t nsdf = 0;                    // SDF objects among them
@@ -300,6 +299,8 @@ struct SmlpDev {
     uint32_t lam_res = 0;
     uint32_t basis = 0;          // AR_BASIS_* used by spectral renders
 };
+// per-material network bases (calb, cem, ascale) of n materials on the device
+int smlp_material_bases(ar_ctx* ctx, GpuMaterial* mats, uint32_t n);
 
 // One scene of a batch context (ar_batch_create): its place in the merged arrays.
 struct BatchEntry {

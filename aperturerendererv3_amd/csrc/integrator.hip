@@ -17,6 +17,7 @@
 // equal seed the GPU and the oracle trace the same paths except where a
 // transcendental (sinf/cosf) differs in its last bit.
 #include "ar_internal.hpp"
+#include "smlp.hpp"
 #include "spectral.hpp"
 #include "path_common.hpp"
 #include "guide.hpp"
@@ -186,9 +187,9 @@ __device__ __forceinline__ float vndf_pdf(F3 v, F3 n, F3 l, float alpha) {
 // ---------------------------------------------------------------- BSDF
 struct Mat {
     F3 albedo, emission;
-    float specular, metallic, roughness, escale;
+    float specular, metallic, roughness, escale, ascale;
     uint32_t model;
-    float ca[3], ce[3];
+    float ca[4], ce[4];
     float ior, sigma_a, abbe, tilt;
 };
 
@@ -203,32 +204,11 @@ __device__ __forceinline__ Mat load_mat(const GpuMaterial* __restrict__ mats, ui
     m.roughness = c.x;
     m.model = __float_as_uint(c.y);
     m.escale = c.z;
-    m.ca[0] = d.x; m.ca[1] = d.y; m.ca[2] = d.z;
-    m.ce[0] = e.x; m.ce[1] = e.y; m.ce[2] = e.z;
+    m.ascale = c.w;
+    m.ca[0] = d.x; m.ca[1] = d.y; m.ca[2] = d.z; m.ca[3] = d.w;
+    m.ce[0] = e.x; m.ce[1] = e.y; m.ce[2] = e.z; m.ce[3] = e.w;
     m.ior = g.x; m.sigma_a = g.y; m.abbe = g.z; m.tilt = g.w;
     return m;
-}
-
-// Reflectance of an RGB albedo in this path's channels.
-__device__ __forceinline__ S4 albedo_of(const Mat& m, const Lam& L) {
-    if (L.mode == AR_SPECTRAL_RGB) return s4(m.albedo.x, m.albedo.y, m.albedo.z, 0.0f);
-    S4 r;
-#pragma unroll
-    for (int k = 0; k < 4; ++k)
-        r.v[k] = (L.mode == AR_SPECTRAL_HERO4 || k == 0) ? spec::sigmoid_poly(m.ca[0], m.ca[1], m.ca[2], L.l[k]) : 0.0f;
-    return r;
-}
-
-// Emitted radiance (Interact * D65Norm of render.cuh:657-700 in spectral modes)
-__device__ __forceinline__ S4 emission_of(const Mat& m, const Lam& L, const float* __restrict__ d65) {
-    if (L.mode == AR_SPECTRAL_RGB) return s4(m.emission.x, m.emission.y, m.emission.z, 0.0f);
-    S4 r;
-#pragma unroll
-    for (int k = 0; k < 4; ++k)
-        r.v[k] = (L.mode == AR_SPECTRAL_HERO4 || k == 0)
-                     ? spec::sigmoid_poly(m.ce[0], m.ce[1], m.ce[2], L.l[k]) * m.escale * spec::d65_norm(d65, L.l[k])
-                     : 0.0f;
-    return r;
 }
 
 // luminance and largest channel of a throughput (RGB: Rec.709 luminance;
@@ -596,7 +576,43 @@ struct SceneArgs {
     const float* __restrict__ d65;
     const float* __restrict__ vol;     // SDF voxel pool
     uint32_t sdf;                      // the scene has SDF objects
+    uint32_t basis;                    // spectral uplift: 0 sigmoid coefficients, 1 basis network
+    const float* __restrict__ lam_lut; // basis network tables: wavelength [lam_res][4], RGB [rgb_res^3][4]
+    uint32_t lam_res;
+    const float* __restrict__ rgb_lut;
+    uint32_t rgb_res;
 };
+
+// Reflectance of an uplift record at wavelength l: the sigmoid of the
+// coefficients, or (network basis) SpectralInteract of the material's RGB
+// basis with the wavelength table's basis at l (spectrum_lut.cuh:183-235).
+__device__ __forceinline__ float uplift_at(const SceneArgs& sc, const float c[4], float l) {
+    if (!sc.basis) return spec::sigmoid_poly(c[0], c[1], c[2], l);
+    return sb::dot_clamp(make_float4(c[0], c[1], c[2], c[3]), sb::query_lambda(sc.lam_lut, (int)sc.lam_res, l));
+}
+
+// Reflectance of an RGB albedo in this path's channels.
+__device__ __forceinline__ S4 albedo_of(const Mat& m, const Lam& L, const SceneArgs& sc) {
+    if (L.mode == AR_SPECTRAL_RGB) return s4(m.albedo.x, m.albedo.y, m.albedo.z, 0.0f);
+    S4 r;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        r.v[k] = (L.mode == AR_SPECTRAL_HERO4 || k == 0) ? uplift_at(sc, m.ca, L.l[k]) * m.ascale : 0.0f;
+    return r;
+}
+
+// Emitted radiance (Interact * D65Norm of render.cuh:657-700 in spectral modes)
+__device__ __forceinline__ S4 emission_of(const Mat& m, const Lam& L, const SceneArgs& sc) {
+    if (L.mode == AR_SPECTRAL_RGB) return s4(m.emission.x, m.emission.y, m.emission.z, 0.0f);
+    S4 r;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        r.v[k] = (L.mode == AR_SPECTRAL_HERO4 || k == 0)
+                     ? uplift_at(sc, m.ce, L.l[k]) * m.escale * spec::d65_norm(sc.d65, L.l[k])
+                     : 0.0f;
+    return r;
+}
+
 
 struct PathArgs {
     float* mask0; float* mask1; float* mask2; float* mask3;
@@ -908,11 +924,16 @@ __device__ __forceinline__ void shade_one(
                     e = s4(P.env[0], P.env[1], P.env[2], 0.0f);
                 } else {
 #pragma unroll
-                    for (int k = 0; k < 4; ++k)
-                        e.v[k] = (P.spectral == AR_SPECTRAL_HERO4 || k == 0)
-                                     ? spec::sigmoid_poly(P.env_c[0], P.env_c[1], P.env_c[2], L.l[k]) * P.env_scale *
-                                           spec::d65_norm(sc.d65, L.l[k])
-                                     : 0.0f;
+                    for (int k = 0; k < 4; ++k) {
+                        float v = 0.0f;
+                        if (P.spectral == AR_SPECTRAL_HERO4 || k == 0)
+                            v = (sc.basis ? sb::lut_interact(sc.rgb_lut, (int)sc.rgb_res,
+                                                             sb::query_lambda(sc.lam_lut, (int)sc.lam_res, L.l[k]),
+                                                             P.env[0], P.env[1], P.env[2])
+                                          : spec::sigmoid_poly(P.env_c[0], P.env_c[1], P.env_c[2], L.l[k]) * P.env_scale) *
+                                spec::d65_norm(sc.d65, L.l[k]);
+                        e.v[k] = v;
+                    }
                 }
                 rad = s4add(rad, s4mul(mask, e));
             }
@@ -987,7 +1008,7 @@ __device__ __forceinline__ void shade_one(
                         const float last_pdf = S.last_pdf[p];
                         if (pl >= 0.0f && last_pdf > 0.0f) w = power_heuristic(last_pdf, pl);
                     }
-                    const S4 le = emission_of(m, L, sc.d65);
+                    const S4 le = emission_of(m, L, sc);
                     rad = s4add(rad, s4scale(s4mul(mask, le), w));
                     if (GM != AR_GUIDING_OFF) g_endpoint(G, p, s4scale(le, w), non_delta, P.spectral);
                 }
@@ -1007,7 +1028,7 @@ __device__ __forceinline__ void shade_one(
                 S4 tr = s4(1, 1, 1, 1);
                 if (ins) {
                     const float dist = length(sub(f3(S.mp0[p], S.mp1[p], S.mp2[p]), pos));
-                    const S4 alb = albedo_of(m, L);
+                    const S4 alb = albedo_of(m, L, sc);
 #pragma unroll
                     for (int k = 0; k < 4; ++k) {
                         const float sa = P.spectral == AR_SPECTRAL_RGB ? m.sigma_a : sigma_at(m.sigma_a, L.l[k], m.tilt);
@@ -1084,7 +1105,7 @@ __device__ __forceinline__ void shade_one(
                     }
                 }
             } else {
-                const S4 alb = albedo_of(m, L);
+                const S4 alb = albedo_of(m, L, sc);
                 const float pspec = m.model == AR_MODEL_LAMBERT ? 0.0f : pspec_of(m, alb, P.spectral);
                 F3 next;
                 float pdf;
@@ -1207,7 +1228,7 @@ __device__ __forceinline__ void shade_one(
                             if (faces) {
                                 const S4 f = eval_bsdf(m, alb, d, ldir, nn);
                                 const float g = cosy / dist2;
-                                const S4 contrib = s4scale(s4mul(f, emission_of(lm, L, sc.d65)), g / pa_total);
+                                const S4 contrib = s4scale(s4mul(f, emission_of(lm, L, sc)), g / pa_total);
                                 const float pl = pa_total * (dist2 / cosy);
                                 float pb = maxr(bsdf_pdf(m, pspec, d, ldir, nn), 0.0f);
                                 if (ptex)
@@ -1714,27 +1735,48 @@ static int ensure_render_buffers(ar_ctx* ctx, size_t cap, size_t npix) {
     return AR_OK;
 }
 
-// Sigmoid coefficients of every material (albedo; emission normalised by its
-// largest channel when that exceeds 1, as Interact does, spectrum_lut.cuh).
+// Uplift records of every material: sigmoid coefficients (albedo; emission
+// normalised by its largest channel when that exceeds 1, as Interact does,
+// spectrum_lut.cuh:223-235), or with the basis network the RGB bases that
+// QueryRGBBasis returns for them, computed on the device from the tables.
 static int prepare_spectral_materials(ar_ctx* ctx) {
     SceneDev& s = ctx->scene;
-    if (s.spectral_ready) return AR_OK;
-    const std::vector<float>& lut = spec::uplift_lut(spec::cie_xyz_table(), spec::d65_table());
-    for (GpuMaterial& g : s.host_mats) {
-        spec::uplift(lut.data(), g.albedo, g.calb);
-        const float mx = std::max(g.emission[0], std::max(g.emission[1], g.emission[2]));
-        float e[3] = {g.emission[0], g.emission[1], g.emission[2]};
-        g.escale = 1.0f;
-        if (mx > 1.0f) {
-            for (float& x : e) x /= mx;
-            g.escale = mx;
+    const uint32_t basis = ctx->smlp.basis;
+    if (s.spectral_ready && s.spectral_basis == basis) return AR_OK;
+    const size_t bytes = s.host_mats.size() * sizeof(GpuMaterial);
+    if (basis == AR_BASIS_NETWORK) {
+        if (!ctx->smlp.rgb_lut || !ctx->smlp.lam_lut)
+            return set_error(ctx, AR_E_INVALID, "basis network tables not built (ar_spectral_mlp_build_tables)");
+        int rc = check_hip(ctx, hipMemcpy(s.mats, s.host_mats.data(), bytes, hipMemcpyHostToDevice),
+                           "upload materials");
+        if (rc == AR_OK) rc = smlp_material_bases(ctx, s.mats, (uint32_t)s.host_mats.size());
+        if (rc == AR_OK)
+            rc = check_hip(ctx, hipMemcpy(s.host_mats.data(), s.mats, bytes, hipMemcpyDeviceToHost),
+                           "read material bases");
+        if (rc != AR_OK) return rc;
+    } else {
+        const std::vector<float>& lut = spec::uplift_lut(spec::cie_xyz_table(), spec::d65_table());
+        for (GpuMaterial& g : s.host_mats) {
+            spec::uplift(lut.data(), g.albedo, g.calb);
+            g.calb[3] = 0.0f;
+            g.ascale = 1.0f;
+            const float mx = std::max(g.emission[0], std::max(g.emission[1], g.emission[2]));
+            float e[3] = {g.emission[0], g.emission[1], g.emission[2]};
+            g.escale = 1.0f;
+            if (mx > 1.0f) {
+                for (float& x : e) x /= mx;
+                g.escale = mx;
+            }
+            spec::uplift(lut.data(), e, g.cem);
+            g.cem[3] = 0.0f;
         }
-        spec::uplift(lut.data(), e, g.cem);
+        int rc = check_hip(ctx, hipMemcpy(s.mats, s.host_mats.data(), bytes, hipMemcpyHostToDevice),
+                           "upload spectral materials");
+        if (rc != AR_OK) return rc;
     }
-    int rc = check_hip(ctx, hipMemcpy(s.mats, s.host_mats.data(), s.host_mats.size() * sizeof(GpuMaterial),
-                                      hipMemcpyHostToDevice), "upload spectral materials");
-    if (rc == AR_OK) s.spectral_ready = true;
-    return rc;
+    s.spectral_ready = true;
+    s.spectral_basis = basis;
+    return AR_OK;
 }
 
 // ------------------------------------------------------------ render plans
@@ -2008,9 +2050,11 @@ static int enqueue_job(ar_ctx* ctx, RenderJob& J, hipStream_t stream) {
                b.last_pos[0], b.last_pos[1], b.last_pos[2], b.last_pdf, b.meta, b.lambda0, b.pix, b.smp, b.scn,
                b.med, b.mpos[0], b.mpos[1], b.mpos[2], fh ? b.fh : nullptr, (uint32_t)b.fh_cap};
     const bool sdf = ctx->scene.nsdf > 0;
+    const bool net = ctx->smlp.basis == AR_BASIS_NETWORK && J.base.spectral != AR_SPECTRAL_RGB;
     SceneArgs sc{ctx->scene.prims, ctx->scene.objs, ctx->scene.ntri, ctx->scene.mats, ctx->scene.light_tri,
                  ctx->scene.light_area, ctx->scene.light_prob, ctx->scene.light_cdf, b.cie, b.d65, ctx->sdf.dev,
-                 sdf ? 1u : 0u};
+                 sdf ? 1u : 0u, net ? 1u : 0u, ctx->smlp.lam_lut, ctx->smlp.lam_res, ctx->smlp.rgb_lut,
+                 ctx->smlp.rgb_res};
     QueueRays q[2] = {{b.rays[0][0], b.rays[0][1], b.rays[0][2], b.rays[0][3], b.rays[0][4], b.rays[0][5], b.rays[0][6],
                        b.rays[0][7], b.qroot[0], nullptr},
                       {b.rays[1][0], b.rays[1][1], b.rays[1][2], b.rays[1][3], b.rays[1][4], b.rays[1][5], b.rays[1][6],

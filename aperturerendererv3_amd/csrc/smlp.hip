@@ -632,6 +632,33 @@ __global__ __launch_bounds__(kRedBlock) void k_smlp_adamw(float* __restrict__ P,
     ema[p] = ema[p] * a.ema_decay + (1.0f - a.ema_decay) * w;
 }
 
+// per material: the RGB bases of the albedo (divided by its largest channel
+// above 1, which becomes ascale) and of the emission normalised by escale
+__global__ __launch_bounds__(64) void k_material_bases(GpuMaterial* __restrict__ mats, uint32_t n,
+                                                       const float* __restrict__ lut, int res) {
+    const uint32_t k = blockIdx.x * 64 + threadIdx.x;
+    if (k >= n) return;
+    GpuMaterial& m = mats[k];
+    float a[3] = {m.albedo[0], m.albedo[1], m.albedo[2]};
+    const float ma = fmaxf(fmaxf(a[0], a[1]), a[2]);
+    m.ascale = 1.0f;
+    if (ma > 1.0f) {
+        for (float& v : a) v /= ma;
+        m.ascale = ma;
+    }
+    float e[3] = {m.emission[0], m.emission[1], m.emission[2]};
+    const float me = fmaxf(fmaxf(e[0], e[1]), e[2]);
+    m.escale = 1.0f;
+    if (me > 1.0f) {
+        for (float& v : e) v /= me;
+        m.escale = me;
+    }
+    const float4 ka = sb::query_rgb(lut, res, a[0], a[1], a[2]);
+    const float4 ke = sb::query_rgb(lut, res, e[0], e[1], e[2]);
+    m.calb[0] = ka.x; m.calb[1] = ka.y; m.calb[2] = ka.z; m.calb[3] = ka.w;
+    m.cem[0] = ke.x; m.cem[1] = ke.y; m.cem[2] = ke.z; m.cem[3] = ke.w;
+}
+
 int ensure_part(ar_ctx* ctx, uint32_t tiles) {
     SmlpDev& S = ctx->smlp;
     if (S.part && S.part_tiles >= tiles) return AR_OK;
@@ -691,6 +718,15 @@ int need_params(ar_ctx* ctx) {
 
 }  // namespace
 
+int smlp_material_bases(ar_ctx* ctx, GpuMaterial* mats, uint32_t n) {
+    if (n == 0) return AR_OK;
+    const SmlpDev& S = ctx->smlp;
+    hipLaunchKernelGGL(k_material_bases, dim3((n + 63) / 64), dim3(64), 0, nullptr, mats, n, S.rgb_lut, (int)S.rgb_res);
+    int rc = check_hip(ctx, hipGetLastError(), "material bases launch");
+    if (rc == AR_OK) rc = check_hip(ctx, hipDeviceSynchronize(), "material bases");
+    return rc;
+}
+
 void smlp_free(ar_ctx* ctx) {
     SmlpDev& S = ctx->smlp;
     for (float* q : {S.p, S.part, S.part2, S.misc, S.rgb_lut, S.lam_lut})
@@ -725,6 +761,15 @@ extern "C" int ar_spectral_mlp_set(ar_ctx* ctx, const float* params) {
     if (rc != AR_OK) return rc;
     S.have = true;
     S.t = 0;
+    return AR_OK;
+}
+
+extern "C" int ar_spectral_set_basis(ar_ctx* ctx, uint32_t basis) {
+    if (!ctx) return AR_E_INVALID;
+    if (basis != AR_BASIS_SIGMOID && basis != AR_BASIS_NETWORK) return set_error(ctx, AR_E_INVALID, "unknown basis");
+    if (basis == AR_BASIS_NETWORK && (!ctx->smlp.rgb_lut || !ctx->smlp.lam_lut))
+        return set_error(ctx, AR_E_INVALID, "basis network tables not built (ar_spectral_mlp_build_tables)");
+    ctx->smlp.basis = basis;
     return AR_OK;
 }
 

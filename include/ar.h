@@ -513,6 +513,16 @@ int ar_spectrum_uplift(const float* rgb, size_t n, float* coeffs);
  * row-major [out][in]).  Evaluated and trained with FP32 MFMA. */
 #define AR_SMLP_PARAMS 16096
 
+/* Spectral uplift of spectral renders: AR_BASIS_SIGMOID (default) uses the
+ * Jakob-Hanika sigmoid coefficients; AR_BASIS_NETWORK the reference's path
+ * (core/render/material.cuh:69-358, render.cuh:290-738): reflectance and
+ * emission are Interact(QueryLambda(lambda), RGB) through the tables of
+ * ar_spectral_mlp_build_tables, which must exist.  A new table build
+ * refreshes the material bases at the next render. */
+#define AR_BASIS_SIGMOID 0u
+#define AR_BASIS_NETWORK 1u
+int ar_spectral_set_basis(ar_ctx* ctx, uint32_t basis);
+
 /* Sets the parameters (host array) and restarts the optimiser state; the
  * EMA copy starts at the parameters. */
 int ar_spectral_mlp_set(ar_ctx* ctx, const float* params);

@@ -232,8 +232,11 @@ typedef struct {
     uint32_t nobj;
     const orc_material* obj_mats; /* material of each object */
     uint32_t nmats;               /* mesh materials; object k's spectral coefficients are row nmats + k */
-    const float* mat_coef;        /* spectral modes: per material calb[3], cem[3], escale, 0 */
-    float env_coef[4];            /* spectral constant sky: coefficients, scale */
+    const float* mat_coef;        /* spectral modes, 12 per material: albedo coefficients / basis [4],
+                                     emission [4] (normalised by its largest channel above 1), escale, ascale */
+    float env_coef[8];            /* spectral constant sky: coefficients / basis [4], scale */
+    const float* lam_tab;         /* network basis: wavelength table, lam_res x 4 (NULL: sigmoid uplift) */
+    int lam_res;
 } orc_render_scene;
 
 typedef struct {
@@ -320,6 +323,10 @@ typedef struct {
     const orc_material* obj_mats;  /* nobj materials */
     float* gbuffer;                /* optional: 16 floats per pixel, folded per sample (orc_gbuffer_record) */
     float* stats;                  /* optional: 12 floats per pixel, folded per sample (orc_stats_record) */
+    const float* smlp;             /* spectral modes: basis-network parameters (NULL: sigmoid uplift);
+                                      reflectance clamp(dot(QueryRGBBasis, QueryLambda), 0, 1) through
+                                      tables of smlp_rgb_res^3 and smlp_lambda_res entries */
+    int smlp_rgb_res, smlp_lambda_res;
 } orc_render_opts;
 
 /* Sigmoid coefficients of an RGB colour (channels clamped to [0, 1]) from the

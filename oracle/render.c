@@ -329,24 +329,47 @@ int orc_render(const float* verts, uint32_t nv, const uint32_t* idx, uint32_t nt
     }
     sc.light_index = light_index;
     float* mat_coef = NULL;
+    float* lam_tab = NULL;
+    sc.lam_tab = NULL;
+    sc.lam_res = 0;
     if (opts->spectral != 0) {
         const uint32_t nm = nmats + nobj;
-        mat_coef = calloc((size_t)8 * (nm ? nm : 1), sizeof(float));
+        mat_coef = calloc((size_t)12 * (nm ? nm : 1), sizeof(float));
+        const int net = opts->smlp != NULL;
+        if (net) {
+            lam_tab = malloc((size_t)4 * opts->smlp_lambda_res * sizeof(float));
+            for (int i = 0; i < opts->smlp_lambda_res; ++i)
+                orc_smlp_lambda_entry(opts->smlp, opts->smlp_lambda_res, i, &lam_tab[4 * i]);
+            sc.lam_tab = lam_tab;
+            sc.lam_res = opts->smlp_lambda_res;
+        }
         for (uint32_t i = 0; i < nm; ++i) {
             const orc_material* mi = i < nmats ? &mats[i] : &opts->obj_mats[i - nmats];
-            orc_uplift(opts->lut, mi->albedo, &mat_coef[8 * i]);
+            float* mc = &mat_coef[12 * i];
+            float a[3] = { mi->albedo[0], mi->albedo[1], mi->albedo[2] };
+            const float ma = fmaxf(a[0], fmaxf(a[1], a[2]));
+            mc[9] = 1.0f;
             float e[3] = { mi->emission[0], mi->emission[1], mi->emission[2] };
             const float mx = fmaxf(e[0], fmaxf(e[1], e[2]));
             float scale = 1.0f;
             if (mx > 1.0f) { for (int k = 0; k < 3; ++k) e[k] /= mx; scale = mx; }
-            orc_uplift(opts->lut, e, &mat_coef[8 * i + 3]);
-            mat_coef[8 * i + 6] = scale;
+            if (net) {
+                /* Interact: an RGB above 1 is queried divided by its largest channel */
+                if (ma > 1.0f) { for (int k = 0; k < 3; ++k) a[k] /= ma; mc[9] = ma; }
+                orc_smlp_query_rgb(opts->smlp, opts->smlp_rgb_res, a, mc);
+                orc_smlp_query_rgb(opts->smlp, opts->smlp_rgb_res, e, mc + 4);
+            } else {
+                orc_uplift(opts->lut, mi->albedo, mc);
+                orc_uplift(opts->lut, e, mc + 4);
+            }
+            mc[8] = scale;
         }
         float e[3] = { opts->env[0], opts->env[1], opts->env[2] };
         const float mx = fmaxf(e[0], fmaxf(e[1], e[2]));
-        sc.env_coef[3] = 1.0f;
-        if (mx > 1.0f) { for (int k = 0; k < 3; ++k) e[k] /= mx; sc.env_coef[3] = mx; }
-        orc_uplift(opts->lut, e, sc.env_coef);
+        sc.env_coef[4] = 1.0f;
+        if (mx > 1.0f) { for (int k = 0; k < 3; ++k) e[k] /= mx; sc.env_coef[4] = mx; }
+        if (net) orc_smlp_query_rgb(opts->smlp, opts->smlp_rgb_res, e, sc.env_coef);
+        else orc_uplift(opts->lut, e, sc.env_coef);
     }
     sc.mat_coef = mat_coef;
     orc_render_lights L = { nl, ltri, larea, lprob, lcdf };
@@ -359,7 +382,7 @@ int orc_render(const float* verts, uint32_t nv, const uint32_t* idx, uint32_t nt
     if (precision == 64) render_f64(&sc, &L, opts, (double*)out, nt);
     else render_f32(&sc, &L, opts, (float*)out, nt);
     free(pmin); free(pmax); free(pcen); free(nodes); free(indices); free(light_index);
-    free(ltri); free(larea); free(lprob); free(lcdf); free(mat_coef);
+    free(ltri); free(larea); free(lprob); free(lcdf); free(mat_coef); free(lam_tab);
     return 0;
 }
 

@@ -332,9 +332,9 @@ static R RF(vndf_pdf)(V3 v, V3 n, V3 l, R alpha)
 
 typedef struct {
     V3 albedo, emission;
-    R specular, metallic, roughness, escale;
+    R specular, metallic, roughness, escale, ascale;
     uint32_t model;
-    float ca[3], ce[3];
+    float ca[4], ce[4];
     R ior, sigma_a, abbe, tilt;
 } RF(mat);
 
@@ -355,20 +355,46 @@ static R RF(alpha_of)(const RF(mat)* m)
     return r * r;
 }
 
-static S4 RF(albedo_of)(const RF(mat)* m, const RF(lam)* L)
+/* QueryLambda through the scene's wavelength table (spectrum_lut.cuh:183-221) */
+static void RF(lam_basis)(const orc_render_scene* sc, R l, float kl[4])
+{
+    float u = ((float)l - 360.0f) / (780.0f - 360.0f);
+    u = fminf(fmaxf(u, 0.0f), 1.0f);
+    const int res = sc->lam_res;
+    float g = u * (float)res - 0.5f;
+    g = fminf(fmaxf(g, 0.0f), (float)res - 1.0f);
+    const float f = floorf(g);
+    const int i0 = (int)f, i1 = i0 + 1 >= res ? res - 1 : i0 + 1;
+    const float t = i0 == i1 ? 0.0f : g - f;
+    for (int k = 0; k < 4; ++k) kl[k] = sc->lam_tab[4 * i0 + k] * (1.0f - t) + sc->lam_tab[4 * i1 + k] * t;
+}
+
+/* reflectance of an uplift record at l: the sigmoid of the coefficients, or
+ * SpectralInteract of the RGB basis with the wavelength basis (network) */
+static R RF(uplift_at)(const orc_render_scene* sc, const float* c, R l)
+{
+    if (!sc->lam_tab) return RF(sigmoid)(c, l);
+    float kl[4];
+    RF(lam_basis)(sc, l, kl);
+    return (R)orc_smlp_interact(c, kl);
+}
+
+static S4 RF(albedo_of)(const orc_render_scene* sc, const RF(mat)* m, const RF(lam)* L)
 {
     if (L->mode == 0) return RF(s4)(m->albedo.x, m->albedo.y, m->albedo.z, 0);
     S4 r;
-    for (int k = 0; k < 4; ++k) r.v[k] = (L->mode == 4 || k == 0) ? RF(sigmoid)(m->ca, L->l[k]) : (R)0;
+    for (int k = 0; k < 4; ++k)
+        r.v[k] = (L->mode == 4 || k == 0) ? RF(uplift_at)(sc, m->ca, L->l[k]) * m->ascale : (R)0;
     return r;
 }
 
-static S4 RF(emission_of)(const RF(mat)* m, const RF(lam)* L, const float* d65)
+static S4 RF(emission_of)(const orc_render_scene* sc, const RF(mat)* m, const RF(lam)* L, const float* d65)
 {
     if (L->mode == 0) return RF(s4)(m->emission.x, m->emission.y, m->emission.z, 0);
     S4 r;
     for (int k = 0; k < 4; ++k)
-        r.v[k] = (L->mode == 4 || k == 0) ? RF(sigmoid)(m->ce, L->l[k]) * m->escale * RF(d65n)(d65, L->l[k]) : (R)0;
+        r.v[k] = (L->mode == 4 || k == 0) ? RF(uplift_at)(sc, m->ce, L->l[k]) * m->escale * RF(d65n)(d65, L->l[k])
+                                          : (R)0;
     return r;
 }
 
@@ -507,11 +533,13 @@ static RF(mat) RF(material)(const orc_render_scene* sc, uint32_t p)
     m.model = om->model;
     m.ior = om->ior; m.sigma_a = om->sigma_a; m.abbe = om->abbe; m.tilt = om->tilt;
     m.escale = 1;
-    for (int k = 0; k < 3; ++k) { m.ca[k] = 0.0f; m.ce[k] = 0.0f; }
+    m.ascale = 1;
+    for (int k = 0; k < 4; ++k) { m.ca[k] = 0.0f; m.ce[k] = 0.0f; }
     if (sc->mat_coef) {
-        const float* mc = &sc->mat_coef[8 * mi];
-        for (int k = 0; k < 3; ++k) { m.ca[k] = mc[k]; m.ce[k] = mc[3 + k]; }
-        m.escale = (R)mc[6];
+        const float* mc = &sc->mat_coef[12 * mi];
+        for (int k = 0; k < 4; ++k) { m.ca[k] = mc[k]; m.ce[k] = mc[4 + k]; }
+        m.escale = (R)mc[8];
+        m.ascale = (R)mc[9];
     }
     return m;
 }
@@ -994,9 +1022,9 @@ static V3 RF(trace_path)(const orc_render_scene* sc, const orc_render_lights* L,
                 if (mode == 0) e = RF(s4)((R)op->env[0], (R)op->env[1], (R)op->env[2], 0);
                 else {
                     RF(mat) em;
-                    em.escale = (R)sc->env_coef[3];
-                    for (int k = 0; k < 3; ++k) em.ce[k] = sc->env_coef[k];
-                    e = RF(emission_of)(&em, &LM, op->d65);
+                    em.escale = (R)sc->env_coef[4];
+                    for (int k = 0; k < 4; ++k) em.ce[k] = sc->env_coef[k];
+                    e = RF(emission_of)(sc, &em, &LM, op->d65);
                 }
                 rad = RF(s4add)(rad, RF(s4mul)(mask, e));
             }
@@ -1053,7 +1081,7 @@ static V3 RF(trace_path)(const orc_render_scene* sc, const orc_render_lights* L,
                     }
                     if (pl >= 0 && last_pdf > 0) w = RF(power_heuristic)(last_pdf, pl);
                 }
-                const S4 le = RF(emission_of)(&m, &LM, op->d65);
+                const S4 le = RF(emission_of)(sc, &m, &LM, op->d65);
                 rad = RF(s4add)(rad, RF(s4scale)(RF(s4mul)(mask, le), w));
                 RF(g_endpoint)(gm, gr, gval, RF(s4scale)(le, w), non_delta, mode);
             }
@@ -1076,7 +1104,7 @@ static V3 RF(trace_path)(const orc_render_scene* sc, const orc_render_lights* L,
             S4 tr = RF(s4)(1, 1, 1, 1);
             if (ins) {
                 const R dist = RF(length)(RF(sub)(mpos, p));
-                const S4 alb = RF(albedo_of)(&m, &LM);
+                const S4 alb = RF(albedo_of)(sc, &m, &LM);
                 for (int k = 0; k < 4; ++k) {
                     const R sa = mode == 0 ? m.sigma_a : RF(sigma_at)(m.sigma_a, LM.l[k], m.tilt);
                     const R a = RF(minr)(RF(maxr)(alb.v[k], (R)1e-12f), (R)1);
@@ -1145,7 +1173,7 @@ static V3 RF(trace_path)(const orc_render_scene* sc, const orc_render_lights* L,
             d = next;
             continue;
         }
-        const S4 alb = RF(albedo_of)(&m, &LM);
+        const S4 alb = RF(albedo_of)(sc, &m, &LM);
         const R pspec = m.model == 0 ? (R)0 : RF(pspec_of)(&m, alb, mode);
         V3 next;
         R pdf;
@@ -1237,7 +1265,7 @@ static V3 RF(trace_path)(const orc_render_scene* sc, const orc_render_lights* L,
                     if (cosx > 0 && cosy > 0) {
                         const S4 f = RF(eval_bsdf)(&m, alb, d, ldir, n);
                         const R g = cosy / dist2;
-                        const S4 contrib = RF(s4scale)(RF(s4mul)(f, RF(emission_of)(&lm, &LM, op->d65)), g / pa_total);
+                        const S4 contrib = RF(s4scale)(RF(s4mul)(f, RF(emission_of)(sc, &lm, &LM, op->d65)), g / pa_total);
                         const R pl = pa_total * (dist2 / cosy);
                         R pb = RF(maxr)(RF(bsdf_pdf)(&m, pspec, d, ldir, n), (R)0);
                         if (ptex) {

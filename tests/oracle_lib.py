@@ -346,7 +346,8 @@ class orc_render_opts(C.Structure):
                 ("vfov_deg", C.c_float), ("spectral", C.c_int), ("output", C.c_int), ("cie", C.c_void_p),
                 ("d65", C.c_void_p), ("lut", C.c_void_p), ("guiding", C.c_int), ("guide", C.c_void_p),
                 ("objects", C.c_void_p), ("nobj", C.c_uint32), ("obj_mats", C.c_void_p), ("gbuffer", C.c_void_p),
-                ("stats", C.c_void_p)]
+                ("stats", C.c_void_p), ("smlp", C.c_void_p), ("smlp_rgb_res", C.c_int),
+                ("smlp_lambda_res", C.c_int)]
 
 
 CIE_TABLE = ROOT / "aperturerendererv3_amd" / "data" / "cie1931_d65_1nm.txt"
@@ -436,10 +437,12 @@ def make_materials(mats):
 
 def render(V, F, tri_mat, mats, width, height, spp, seed=1, max_depth=5, nee=1, sky=0, env=(0, 0, 0),
            cam_pos=(0, 0, 1.1), cam_dir=(0, 0, -1), vfov=60.0, precision=32, rr_min_depth=3, sample_offset=0,
-           threads=0, spectral=0, output=0, guide=None, objects=None, gbuffer=None, stats=None):
+           threads=0, spectral=0, output=0, guide=None, objects=None, gbuffer=None, stats=None, basis=None):
     """Oracle render; returns (height, width, 3) float32 or float64 mean image.
     objects: analytic objects as dicts (type, center, extent, rotation,
-    material as a dict of ar_material fields)."""
+    material as a dict of ar_material fields).  basis: (parameters,
+    rgb_res, lambda_res) of the spectral basis network, else the sigmoid
+    uplift."""
     lib = load()
     V = np.ascontiguousarray(V, np.float32)
     F = np.ascontiguousarray(F, np.uint32)
@@ -457,6 +460,9 @@ def render(V, F, tri_mat, mats, width, height, spp, seed=1, max_depth=5, nee=1, 
     o.cie, o.d65 = xyz.ctypes.data, d65.ctypes.data
     if spectral:
         o.lut = uplift_lut().ctypes.data
+    if basis is not None:
+        bw = np.ascontiguousarray(basis[0], np.float32)
+        o.smlp, o.smlp_rgb_res, o.smlp_lambda_res = bw.ctypes.data, int(basis[1]), int(basis[2])
     if guide is not None:
         o.guiding, o.guide = 1, guide.handle
     if gbuffer is not None:

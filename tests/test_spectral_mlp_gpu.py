@@ -193,3 +193,54 @@ def test_training_fits_the_uplift_and_is_reproducible(ctx):
         assert np.array_equal(c2.spectral_mlp_get(ema=True), ctx.spectral_mlp_get(ema=True))
     finally:
         c2.close()
+
+
+@pytest.mark.parametrize("mode,output", [(4, 0), (1, 1)])
+def test_network_basis_render_matches_oracle(ctx, mode, output):
+    """Spectral Cornell box with the reference's uplift path: material and
+    light spectra through the basis network's tables (reference weights,
+    32^3 RGB and 840 wavelength entries), against the FP32 oracle at equal
+    seed.  The material bases come from MFMA-built tables (1e-7 relative from
+    direct evaluation), so the bar is the spectral renders' 1e-4 per pixel."""
+    from aperturerendererv3_amd.workloads import CORNELL_CAMERA, cornell_box
+    from test_render_gpu import _kw, assert_close_to_oracle, gpu_render
+    w = _ref()[0]
+    ctx.spectral_mlp_set(w)
+    ctx.spectral_mlp_build_tables(32, 840)
+    ctx.spectral_set_basis(True)
+    try:
+        V, F, M, mats = cornell_box()
+        mats = [dict(m) for m in mats]
+        mats[2].update({"model": 1, "roughness": 0.3, "specular": 0.5, "metallic": 0.5})
+        kw = dict(_kw(CORNELL_CAMERA), seed=4, max_depth=6, spectral=mode, output=output)
+        print(f"network-basis render mode={mode} output={output}", flush=True)
+        g = gpu_render(ctx, V, F, M, mats, 40, 40, 6, **kw)
+        o = orc.render(V, F, M, mats, 40, 40, 6, precision=32, basis=(w, 32, 840), **kw)
+        assert_close_to_oracle(g, o, f"network basis mode={mode}", rtol=1e-4, atol=1e-5, min_frac=0.99)
+        # the sigmoid uplift gives another image
+        s = orc.render(V, F, M, mats, 40, 40, 6, precision=32, **kw)
+        assert orc.rel_mse(s, o) > 1e-4
+    finally:
+        ctx.spectral_set_basis(False)
+
+
+def test_network_basis_sky_above_one(ctx):
+    """Constant sky brighter than 1 (Interact divides by the largest channel
+    and scales back) over a gray floor, against the oracle."""
+    import geom
+    w = _ref()[0]
+    ctx.spectral_mlp_set(w)
+    ctx.spectral_mlp_build_tables(24, 420)
+    ctx.spectral_set_basis(True)
+    try:
+        from test_render_gpu import assert_close_to_oracle, gpu_render
+        V, F, M, mats, _ = geom.floor_and_light(albedo=0.6)
+        F, M, mats = F[:2], M[:2], mats[:1]
+        kw = dict(max_depth=4, nee=False, sky=2, env=(2.5, 1.5, 0.7), cam_pos=(0, 0.5, 0.01), cam_dir=(0, -1, 1e-3),
+                  vfov=20.0, rr_min_depth=100, spectral=4, output=1, seed=3)
+        g = gpu_render(ctx, V, F, M, mats, 32, 32, 8, **kw)
+        o = orc.render(V, F, M, mats, 32, 32, 8, precision=32, basis=(w, 24, 420), **kw)
+        assert_close_to_oracle(g, o, "network basis, sky 2.5", rtol=1e-4, atol=1e-5, min_frac=0.99)
+        assert g.mean() > 0.1
+    finally:
+        ctx.spectral_set_basis(False)
