@@ -63,6 +63,7 @@ def parse_args():
     ap.add_argument("--mlp-only", action="store_true", help="run only the guiding-network kernel bench")
     ap.add_argument("--output-only", action="store_true", help="run only the output-stage bench")
     ap.add_argument("--sdf-only", action="store_true", help="run only the SDF-object render bench")
+    ap.add_argument("--smlp-only", action="store_true", help="run only the spectral basis network bench")
     ap.add_argument("--cpu-sample-stride", type=int, default=2)
     ap.add_argument("--profile", action="store_true",
                     help="timed steps only, plus one 256 MiB device copy used to scale HBM counters")
@@ -536,6 +537,118 @@ def sdf_bench(ctx_cls, reps=3, cpu=True):
     return out
 
 
+SMLP_TRAIN_FLOP = 89600   # per sample: forward, backward-data and weight gradients of both encoders
+SMLP_EVAL_FLOP = 31232    # per row: RGBEncode + LambdaEncode
+SMLP_RGB_FLOP = 16640     # per RGB-table entry: RGBEncode
+FP32_MFMA_PEAK_TFLOPS = 157.3
+
+
+def smlp_bench(ctx_cls, cpu=True):
+    """Spectral basis network (the reference's RGB / wavelength encoders):
+    retraining on the GPU at the reference's batch of 4096 and at 65536,
+    evaluation of 4M rows, the reference-size uplift tables (512^3 RGB,
+    840 wavelengths), table queries, a fit of the library's sigmoid uplift
+    (10000 steps) and the 1080p spectral frame rendered through the tables.
+    FLOP counts are the algorithmic ones (the training kernel's second
+    wavelength forward is not counted).  CPU baseline: the oracle's FP32
+    evaluation (all host cores) and FP64 training gradient (one core)."""
+    import torch
+    from aperturerendererv3_amd.workloads import SPECTRAL_CAMERA, spectral_scene
+    out = {"workload": "spectral basis network 2 x (7 residual 32-wide blocks) + RGB input layer, 16096 parameters, "
+                       "FP32 MFMA; dataset 2^20 rows of the sigmoid uplift"}
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+    def timed(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    with ctx_cls(0) as ctx:
+        x, y = ctx.spectral_uplift_dataset(1 << 20, seed=1)
+        xv, yv = ctx.spectral_uplift_dataset(1 << 16, seed=2)
+        train = {}
+        for batch, steps in ((4096, 200), (65536, 40)):
+            ctx.spectral_mlp_init(1)
+            ms = timed(lambda: ctx.spectral_mlp_train(x, y, steps=steps, batch=batch, seed=1), 2) / steps
+            train[f"batch_{batch}"] = {"ms_per_step": round(ms, 4),
+                                       "samples_per_s": round(batch / (ms * 1e-3), 1),
+                                       "achieved_TFLOPs": round(batch * SMLP_TRAIN_FLOP / (ms * 1e-3) / 1e12, 2)}
+        out["train"] = train
+        # fit of the sigmoid uplift
+        ctx.spectral_mlp_init(7)
+        t0 = time.perf_counter()
+        hist = ctx.spectral_mlp_train(x, y, steps=10000, batch=4096, seed=3, lr=1e-3, warmup_ratio=0.05,
+                                      min_lr_ratio=0.01)
+        torch.cuda.synchronize()
+        fit_s = time.perf_counter() - t0
+        pred = ctx.spectral_mlp_eval(xv)
+        err = (pred - yv).abs()
+        h = hist.cpu().numpy()
+        out["fit"] = {"steps": 10000, "batch": 4096, "seconds": round(fit_s, 3),
+                      "loss_first_100": float(h[:100].mean()), "loss_last_100": float(h[-100:].mean()),
+                      "heldout_mae": round(float(err.mean()), 5), "heldout_p99_abs_err": round(
+                          float(torch.quantile(err[:1 << 16], 0.99)), 5)}
+        # evaluation, tables, queries
+        n = 1 << 22
+        xe = x[:n] if x.shape[0] >= n else x.repeat(n // x.shape[0], 1)
+        oute = torch.empty(n, dtype=torch.float32, device=dev)
+        ms = timed(lambda: ctx.spectral_mlp_eval(xe, out=oute), 5)
+        out["eval"] = {"rows": n, "ms": round(ms, 4), "rows_per_s": round(n / (ms * 1e-3), 1),
+                       "achieved_TFLOPs": round(n * SMLP_EVAL_FLOP / (ms * 1e-3) / 1e12, 2),
+                       "frac_fp32_mfma_peak": round(n * SMLP_EVAL_FLOP / (ms * 1e-3) / 1e12 / FP32_MFMA_PEAK_TFLOPS, 4)}
+        ms = timed(lambda: ctx.spectral_mlp_build_tables(512, 840), 2)
+        entries = 512 ** 3
+        out["tables_512"] = {"ms": round(ms, 3), "entries": entries,
+                             "achieved_TFLOPs": round(entries * SMLP_RGB_FLOP / (ms * 1e-3) / 1e12, 2),
+                             "device_bytes": entries * 16 + 840 * 16}
+        ms = timed(lambda: ctx.spectral_mlp_query(xe, out=oute), 5)
+        out["query"] = {"rows": n, "ms": round(ms, 4), "rows_per_s": round(n / (ms * 1e-3), 1)}
+        # the 1080p spectral frame through the tables vs the sigmoid uplift
+        V, F, M, mats = spectral_scene()
+        cam = SPECTRAL_CAMERA
+        kw = dict(cam_pos=cam["pos"], cam_dir=cam["dir"], vfov=cam["vfov"], seed=1, max_depth=8, spectral=4,
+                  output=0)
+        ctx.set_meshes([{"vertices": V, "indices": F, "tri_material": M, "materials": mats}])
+        ctx.build_bvh()
+        img = torch.empty((1080, 1920, 3), dtype=torch.float32, device=dev)
+        frame = {}
+        for net in (False, True):
+            ctx.spectral_set_basis(net)
+            ms = timed(lambda: ctx.render(1920, 1080, 16, out=img, **kw), 2)
+            frame["network" if net else "sigmoid"] = {"ms": round(ms, 3),
+                                                      "paths_per_s": round(1920 * 1080 * 16 / (ms * 1e-3), 1)}
+        ctx.spectral_set_basis(False)
+        out["spectral_1080p_16spp"] = frame
+    if cpu:
+        sys.path.insert(0, str(ROOT / "tests"))
+        import oracle_lib as orc
+        cores = len(os.sched_getaffinity(0))
+        w0 = np.load(ROOT / "tests" / "golden" / "spectral_mlp_reference.npz")["weights"]
+        rng = np.random.default_rng(0)
+        xr = np.empty((200000, 4), np.float32)
+        xr[:, :3] = rng.random((200000, 3))
+        xr[:, 3] = rng.uniform(360, 780, 200000)
+        t0 = time.perf_counter()
+        orc.smlp_eval(w0, xr)
+        dt = time.perf_counter() - t0
+        yr = rng.random(512).astype(np.float32)
+        t1 = time.perf_counter()
+        orc.smlp_loss_grad(w0, xr[:512], yr, noise=True, seed=1, step=0)
+        dt1 = time.perf_counter() - t1
+        out["cpu_baseline"] = {"value": round(200000 / dt, 1), "unit": "rows/s (evaluation)", "cores": cores,
+                               "kind": "port",
+                               "sample": f"200000 rows through the oracle's FP32 forward, {dt:.2f} s",
+                               "train_samples_per_s_1_core": round(512 / dt1, 1)}
+    return out
+
+
 def main():
     args = parse_args()
     if args.mlp_only:
@@ -545,6 +658,10 @@ def main():
     if args.output_only:
         from aperturerendererv3_amd import Context
         print(json.dumps(output_bench(Context, cpu=not args.no_cpu_baseline)), flush=True)
+        return
+    if args.smlp_only:
+        from aperturerendererv3_amd import Context
+        print(json.dumps(smlp_bench(Context, cpu=not args.no_cpu_baseline)), flush=True)
         return
     if args.sdf_only:
         from aperturerendererv3_amd import Context
@@ -675,6 +792,7 @@ def main():
     mlp = mlp_bench(Context)
     output = output_bench(Context, cpu=not args.no_cpu_baseline)
     sdf = sdf_bench(Context, cpu=not args.no_cpu_baseline)
+    smlp = smlp_bench(Context, cpu=not args.no_cpu_baseline)
 
     valu_frac = achieved_tops / VALU_PEAK_TOPS
     out = {
@@ -702,6 +820,7 @@ def main():
         "guiding_network": mlp,
         "output": output,
         "sdf_objects": sdf,
+        "spectral_network": smlp,
     }
     print(json.dumps(out), flush=True)
     ctx.close()
