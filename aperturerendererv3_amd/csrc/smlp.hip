@@ -43,7 +43,7 @@ constexpr int kParams = 16096;
 constexpr int kNB = 2;                 // 16-sample column blocks per tile
 constexpr int kTile = 16 * kNB;        // samples per wave
 constexpr float kPi = 3.141592654f;    // CUDART_PI_F
-constexpr int kChunk = 64;             // tiles summed per first-level reduction block
+constexpr int kChunk = 16;             // tiles summed per first-level reduction block
 constexpr int kRedBlock = 256;
 constexpr int kGradBlocks = (kParams + kRedBlock - 1) / kRedBlock;
 constexpr uint32_t kMaxBatch = 1u << 18;
