@@ -80,8 +80,9 @@ struct GpuNode8 {
     uint32_t meta;
     uint32_t ref[8];
     uint8_t q[48];
+    uint32_t pad[8];     // to 128 B: one node is one L2 line
 };
-static_assert(sizeof(GpuNode8) == 96, "GpuNode8 must be 96 bytes");
+static_assert(sizeof(GpuNode8) == 128, "GpuNode8 must be 128 bytes");
 
 // Host-side binary tree produced by the builder (pre-order, left = parent+1).
 struct BuildNode {
