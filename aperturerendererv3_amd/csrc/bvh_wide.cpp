@@ -32,7 +32,19 @@ struct Collapser {
     uint64_t leaves = 0;
     int max_depth = 0;
 
+    // A leaf's slots (48 B each) are placed so that they touch no more
+    // 128-byte lines than their size needs (one for 1-2 slots, two for 3):
+    // up to two unused slots are inserted before it.  The slot array starts
+    // on a 256-byte boundary (hipMalloc).
+    static uint32_t lines(size_t slot, uint32_t count) {
+        const size_t a = 48 * slot, b = a + 48 * (size_t)count - 1;
+        return (uint32_t)(b / 128 - a / 128 + 1);
+    }
+
     uint32_t leaf_ref(const BuildNode& n) {
+        const uint32_t need = (48 * n.count + 127) / 128;
+        for (int pad_slots = 0; pad_slots < 2 && lines(tris.size() / 12, n.count) > need; ++pad_slots)
+            tris.resize(tris.size() + 12, 0.0f);
         const uint32_t first = (uint32_t)(tris.size() / 12);
         for (uint32_t k = 0; k < n.count; ++k) lw.write(tris, br.order[n.first + k]);
         ++leaves;

@@ -807,7 +807,7 @@ extern "C" int ar_spectral_mlp_build_tables(ar_ctx* ctx, uint32_t rgb_res, uint3
     hipStream_t st = (hipStream_t)stream;
     const size_t nrgb = (size_t)rgb_res * rgb_res * rgb_res;
     if (S.rgb_res != rgb_res) {
-        (void)hipStreamSynchronize(st);
+        (void)hipDeviceSynchronize();     // renders on any stream may still read the old table
         if (S.rgb_lut) (void)hipFree(S.rgb_lut);
         S.rgb_lut = nullptr;
         S.rgb_res = 0;
@@ -816,7 +816,7 @@ extern "C" int ar_spectral_mlp_build_tables(ar_ctx* ctx, uint32_t rgb_res, uint3
         S.rgb_res = rgb_res;
     }
     if (S.lam_res != lambda_res) {
-        (void)hipStreamSynchronize(st);
+        (void)hipDeviceSynchronize();
         if (S.lam_lut) (void)hipFree(S.lam_lut);
         S.lam_lut = nullptr;
         S.lam_res = 0;
