@@ -262,12 +262,18 @@ __host__ __device__ __forceinline__ uint32_t mix32(uint32_t x) {
     return x;
 }
 
-// standard normal for (seed, step, sample, slot): Box-Muller on two 24-bit uniforms
-__device__ __forceinline__ float normal(uint32_t base, uint32_t slot) {
-    const uint32_t a = mix32(base + 2u * slot), b = mix32(base + 2u * slot + 1u);
+// standard normals of the slot pair (2 pair, 2 pair + 1) of (seed, step,
+// sample): Box-Muller on two 24-bit uniforms, cosine for the even slot, sine
+// for the odd one
+__device__ __forceinline__ void normal2(uint32_t base, uint32_t pair, float& even, float& odd) {
+    const uint32_t a = mix32(base + 2u * pair), b = mix32(base + 2u * pair + 1u);
     const float u1 = (float)((a >> 8) + 1u) * (1.0f / 16777216.0f);
     const float u2 = (float)(b >> 8) * (1.0f / 16777216.0f);
-    return sqrtf(-2.0f * logf(u1)) * cosf(6.283185307f * u2);
+    const float r = sqrtf(-2.0f * logf(u1)), th = 6.283185307f * u2;
+    float sn, cs;
+    sincosf(th, &sn, &cs);
+    even = r * cs;
+    odd = r * sn;
 }
 
 __device__ __forceinline__ uint32_t noise_base(uint32_t seed, uint32_t step, uint32_t j) {
@@ -307,14 +313,18 @@ __device__ __forceinline__ void store_act(float* L, const Act& a, int i, int g) 
 __device__ __forceinline__ void train_block(const float* __restrict__ P, int off, Act& h, float* L, uint32_t& mask,
                                             const uint32_t base[kNB], uint32_t slot0, bool noise, int i, int g) {
     Act xin = h;
-    if (noise) {
+    if (noise) {                       // slot0 is even: units 4g + {0, 1} and 4g + {2, 3} are pairs
 #pragma unroll
         for (int mb = 0; mb < 2; ++mb)
 #pragma unroll
             for (int nb = 0; nb < kNB; ++nb)
 #pragma unroll
-                for (int r = 0; r < 4; ++r)
-                    xin.v[mb][nb][r] += 0.001f * normal(base[nb], slot0 + mb * 16 + 4 * g + r);
+                for (int r = 0; r < 4; r += 2) {
+                    float n0, n1;
+                    normal2(base[nb], (slot0 + mb * 16 + 4 * g + r) >> 1, n0, n1);
+                    xin.v[mb][nb][r] += 0.001f * n0;
+                    xin.v[mb][nb][r + 1] += 0.001f * n1;
+                }
     }
     if (L) store_act(L, xin, i, g);
     Act z;
@@ -433,9 +443,13 @@ __global__ __launch_bounds__(64) void k_smlp_train(TrainArgs A) {
         lam[nb] = v.w;
         lx[nb] = (v.w - 360.0f) / 420.0f;
         if (noise && j < A.batch / 2) {
-#pragma unroll
-            for (int c = 0; c < 3; ++c) rgb[nb][c] += (0.025f / 256.0f) * normal(base[nb], c);
-            lx[nb] += (0.025f / 420.0f) * normal(base[nb], 3);
+            float n0, n1, n2, n3;
+            normal2(base[nb], 0, n0, n1);
+            normal2(base[nb], 1, n2, n3);
+            rgb[nb][0] += (0.025f / 256.0f) * n0;
+            rgb[nb][1] += (0.025f / 256.0f) * n1;
+            rgb[nb][2] += (0.025f / 256.0f) * n2;
+            lx[nb] += (0.025f / 420.0f) * n3;
         }
         s[nb] = fmaxf(fmaxf(rgb[nb][0], rgb[nb][1]), rgb[nb][2]);
     }

@@ -548,9 +548,10 @@ int ar_spectral_mlp_query(ar_ctx* ctx, const float* x, float* out, size_t n, voi
  * uses dataset rows ((first_step + k) batch + j) mod n, j < batch, and the
  * reference's training-mode noise when noise != 0 (standard deviations
  * 0.025 / 256 on r, g, b and 0.025 / 420 on the normalised wavelength of the
- * first half of the batch, 0.001 on every block's Linear input; drawn from a
- * counter hash of seed, step, row in batch and slot).  Loss: mean over the
- * batch and three channels of ((pred - y)(0.25 + max(cie_c(lambda), 0)))^2
+ * first half of the batch, 0.001 on every block's Linear input; Box-Muller
+ * pairs drawn from a counter hash of seed, step, row in batch and slot
+ * pair).  Loss: mean over the batch and three channels of
+ * ((pred - y)(0.25 + max(cie_c(lambda), 0)))^2
  * with the analytic CIE fits.  Then gradient clipping to clip_norm (0: none),
  * AdamW (decoupled weight decay on weight matrices only) at
  * lr * warmup-cosine(first_step + k of total_steps; 0 = first_step + steps)

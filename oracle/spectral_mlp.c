@@ -30,8 +30,9 @@
  *   AdamW with decoupled weight decay on the weight matrices only
  *   (:28-50), torch's update order; warmup-cosine learning rate (:52-60);
  *   EMA of the parameters (:62-73).
- *   The noise is not torch.randn: every normal is drawn by Box-Muller from a
- *   counter hash of (seed, step, sample in batch, slot), the same on the GPU.
+ *   The noise is not torch.randn: normals come in Box-Muller pairs drawn from
+ *   a counter hash of (seed, step, sample in batch, slot pair), the same on
+ *   the GPU.
  *
  * Parity pins (tests/test_oracle_spectral_mlp.py): the reference's golden
  * vectors (inputTest / gtTest) through the FP32 forward with the reference's
@@ -272,13 +273,16 @@ static uint32_t mix32(uint32_t x) {
 }
 
 /* standard normal for (seed, step, sample, slot): Box-Muller on two 24-bit
- * uniforms, u1 in (0, 1] */
+ * uniforms of the slot pair slot / 2, u1 in (0, 1]; an even slot takes the
+ * cosine, an odd slot the sine of the pair */
 double orc_smlp_normal(uint32_t seed, uint32_t step, uint32_t sample, uint32_t slot) {
     const uint32_t base = mix32(mix32(seed + step * 0x9E3779B9u) ^ sample);
-    const uint32_t a = mix32(base + 2u * slot), b = mix32(base + 2u * slot + 1u);
+    const uint32_t pair = slot >> 1;
+    const uint32_t a = mix32(base + 2u * pair), b = mix32(base + 2u * pair + 1u);
     const float u1 = (float)((a >> 8) + 1u) * (1.0f / 16777216.0f);
     const float u2 = (float)(b >> 8) * (1.0f / 16777216.0f);
-    return (double)(sqrtf(-2.0f * logf(u1)) * cosf(6.283185307f * u2));
+    const float r = sqrtf(-2.0f * logf(u1)), th = 6.283185307f * u2;
+    return (double)(r * ((slot & 1u) ? sinf(th) : cosf(th)));
 }
 
 /* ---------------- FP64 training step ---------------- */
