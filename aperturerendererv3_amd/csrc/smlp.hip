@@ -59,25 +59,46 @@ struct Act {
     f32x4 v[2][kNB];
 };
 
-// z = W x (+ b), W row-major [32 out][32 in]; the bias is the accumulator's start
-__device__ __forceinline__ void linear(const float* __restrict__ Wm, const float* __restrict__ b, const Act& x, Act& z,
-                                       int i, int g) {
+// A fragments and bias of one 32 x 32 layer (the training kernel loads them a
+// layer ahead of use, so that the L2 latency overlaps the previous layer's
+// MFMAs)
+struct Frag {
+    float4 w0[2], w1[2];
+    f32x4 b[2];
+};
+
+__device__ __forceinline__ void load_frag(const float* __restrict__ Wm, const float* __restrict__ b, Frag& f, int i,
+                                          int g) {
 #pragma unroll
     for (int mb = 0; mb < 2; ++mb) {
-        const float4 w0 = *reinterpret_cast<const float4*>(Wm + (mb * 16 + i) * kW + 4 * g);
-        const float4 w1 = *reinterpret_cast<const float4*>(Wm + (mb * 16 + i) * kW + 16 + 4 * g);
-        const float a[8] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w};
-        f32x4 c0;
+        f.w0[mb] = *reinterpret_cast<const float4*>(Wm + (mb * 16 + i) * kW + 4 * g);
+        f.w1[mb] = *reinterpret_cast<const float4*>(Wm + (mb * 16 + i) * kW + 16 + 4 * g);
 #pragma unroll
-        for (int r = 0; r < 4; ++r) c0[r] = b ? b[mb * 16 + 4 * g + r] : 0.0f;
+        for (int r = 0; r < 4; ++r) f.b[mb][r] = b ? b[mb * 16 + 4 * g + r] : 0.0f;
+    }
+}
+
+// z = W x (+ b), W row-major [32 out][32 in]; the bias is the accumulator's start
+__device__ __forceinline__ void linear_f(const Frag& f, const Act& x, Act& z) {
+#pragma unroll
+    for (int mb = 0; mb < 2; ++mb) {
+        const float a[8] = {f.w0[mb].x, f.w0[mb].y, f.w0[mb].z, f.w0[mb].w,
+                            f.w1[mb].x, f.w1[mb].y, f.w1[mb].z, f.w1[mb].w};
 #pragma unroll
         for (int nb = 0; nb < kNB; ++nb) {
-            f32x4 c = c0;
+            f32x4 c = f.b[mb];
 #pragma unroll
             for (int kb = 0; kb < 8; ++kb) c = mfma4(a[kb], x.v[kb >> 2][nb][kb & 3], c);
             z.v[mb][nb] = c;
         }
     }
+}
+
+__device__ __forceinline__ void linear(const float* __restrict__ Wm, const float* __restrict__ b, const Act& x, Act& z,
+                                       int i, int g) {
+    Frag f;
+    load_frag(Wm, b, f, i, g);
+    linear_f(f, x, z);
 }
 
 // 32 -> 4 without bias; valid in lanes g == 0: out[nb][r] = K[sample 16 nb + i][r]
@@ -133,15 +154,26 @@ __device__ __forceinline__ void embed_lambda(const float x[kNB], Act& e, int g) 
         }
 }
 
-__device__ __forceinline__ void residual(const float* __restrict__ P, int off, Act& h, int i, int g) {
+__device__ __forceinline__ void residual_f(const Frag& f, Act& h) {
     Act t;
-    linear(P + off, P + off + 1024, h, t, i, g);
+    linear_f(f, h, t);
 #pragma unroll
     for (int mb = 0; mb < 2; ++mb)
 #pragma unroll
         for (int nb = 0; nb < kNB; ++nb)
 #pragma unroll
             for (int r = 0; r < 4; ++r) h.v[mb][nb][r] = h.v[mb][nb][r] + fmaxf(t.v[mb][nb][r], 0.0f);
+}
+
+// the 7 residual blocks starting at parameter offset first (inference: several
+// waves per SIMD hide the fragment loads, which are not loaded ahead here)
+__device__ __forceinline__ void blocks(const float* __restrict__ P, int first, Act& h, int i, int g) {
+#pragma unroll 1
+    for (int k = 0; k < kBlocks; ++k) {
+        Frag f;
+        load_frag(P + first + k * 1056, P + first + k * 1056 + 1024, f, i, g);
+        residual_f(f, h);
+    }
 }
 
 __device__ __forceinline__ void rgb_forward(const float* __restrict__ P, const Act& e, f32x4 K[kNB], int i, int g) {
@@ -153,14 +185,12 @@ __device__ __forceinline__ void rgb_forward(const float* __restrict__ P, const A
         for (int nb = 0; nb < kNB; ++nb)
 #pragma unroll
             for (int r = 0; r < 4; ++r) h.v[mb][nb][r] = fmaxf(h.v[mb][nb][r], 0.0f);
-#pragma unroll 1
-    for (int k = 0; k < kBlocks; ++k) residual(P, rblk(k), h, i, g);
+    blocks(P, rblk(0), h, i, g);
     linear_out(P + kRout, h, K, i, g);
 }
 
 __device__ __forceinline__ void lambda_forward(const float* __restrict__ P, Act& h, f32x4 K[kNB], int i, int g) {
-#pragma unroll 1
-    for (int k = 0; k < kBlocks; ++k) residual(P, lblk(k), h, i, g);
+    blocks(P, lblk(0), h, i, g);
     linear_out(P + kLout, h, K, i, g);
 }
 
@@ -310,7 +340,7 @@ __device__ __forceinline__ void store_act(float* L, const Act& a, int i, int g) 
 
 // block forward in training mode: the Linear sees h + noise, the residual h;
 // records the block input in LDS (when given) and the ReLU mask
-__device__ __forceinline__ void train_block(const float* __restrict__ P, int off, Act& h, float* L, uint32_t& mask,
+__device__ __forceinline__ void train_block(const Frag& f, Act& h, float* L, uint32_t& mask,
                                             const uint32_t base[kNB], uint32_t slot0, bool noise, int i, int g) {
     Act xin = h;
     if (noise) {                       // slot0 is even: units 4g + {0, 1} and 4g + {2, 3} are pairs
@@ -328,7 +358,7 @@ __device__ __forceinline__ void train_block(const float* __restrict__ P, int off
     }
     if (L) store_act(L, xin, i, g);
     Act z;
-    linear(P + off, P + off + 1024, xin, z, i, g);
+    linear_f(f, xin, z);
     mask = 0;
 #pragma unroll
     for (int mb = 0; mb < 2; ++mb)
@@ -374,21 +404,29 @@ __device__ __forceinline__ void weight_grads(const float* dzl, const float* xl, 
     }
 }
 
+// transposed A fragments of one 32 x 32 layer for the backward-data product
+struct BFrag {
+    float a[2][8];
+};
+
+__device__ __forceinline__ void load_bfrag(const float* __restrict__ Wm, BFrag& f, int i, int g) {
+#pragma unroll
+    for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+        for (int kb = 0; kb < 8; ++kb) f.a[mb][kb] = Wm[((kb >> 2) * 16 + 4 * g + (kb & 3)) * kW + mb * 16 + i];
+}
+
 // dh += W^T dz
-__device__ __forceinline__ void backward_data(const float* __restrict__ Wm, const Act& dz, Act& dh, int i, int g) {
+__device__ __forceinline__ void backward_data_f(const BFrag& f, const Act& dz, Act& dh) {
 #pragma unroll
-    for (int mb = 0; mb < 2; ++mb) {
-        float a[8];
-#pragma unroll
-        for (int kb = 0; kb < 8; ++kb) a[kb] = Wm[((kb >> 2) * 16 + 4 * g + (kb & 3)) * kW + mb * 16 + i];
+    for (int mb = 0; mb < 2; ++mb)
 #pragma unroll
         for (int nb = 0; nb < kNB; ++nb) {
             f32x4 c = dh.v[mb][nb];
 #pragma unroll
-            for (int kb = 0; kb < 8; ++kb) c = mfma4(a[kb], dz.v[kb >> 2][nb][kb & 3], c);
+            for (int kb = 0; kb < 8; ++kb) c = mfma4(f.a[mb][kb], dz.v[kb >> 2][nb][kb & 3], c);
             dh.v[mb][nb] = c;
         }
-    }
 }
 
 // output layer backward: dh = Wo^T dK, dWo = dK^T h7 (dK in LDS [s][4], h7 in LDS)
@@ -460,8 +498,16 @@ __global__ __launch_bounds__(64) void k_smlp_train(TrainArgs A) {
         Act h;
         embed_lambda(lx, h, g);
         uint32_t mk;
+        Frag f;
+        load_frag(P + lblk(0), P + lblk(0) + 1024, f, i, g);
 #pragma unroll 1
-        for (int k = 0; k < kBlocks; ++k) train_block(P, lblk(k), h, nullptr, mk, base, lam_slot + k * kW, noise, i, g);
+        for (int k = 0; k < kBlocks; ++k) {
+            const int nx = lblk(k + 1 < kBlocks ? k + 1 : k);
+            Frag nf;
+            load_frag(P + nx, P + nx + 1024, nf, i, g);
+            train_block(f, h, nullptr, mk, base, lam_slot + k * kW, noise, i, g);
+            f = nf;
+        }
         linear_out(P + kLout, h, kl, i, g);
     }
     // RGB encoder forward with records
@@ -483,8 +529,16 @@ __global__ __launch_bounds__(64) void k_smlp_train(TrainArgs A) {
                     h.v[mb][nb][r] = fmaxf(h.v[mb][nb][r], 0.0f);
                 }
         mrgb[0] = m;
+        Frag f;
+        load_frag(P + rblk(0), P + rblk(0) + 1024, f, i, g);
 #pragma unroll 1
-        for (int k = 0; k < kBlocks; ++k) train_block(P, rblk(k), h, L.x[1 + k], mrgb[1 + k], base, 4 + k * kW, noise, i, g);
+        for (int k = 0; k < kBlocks; ++k) {
+            const int nx = rblk(k + 1 < kBlocks ? k + 1 : k);
+            Frag nf;
+            load_frag(P + nx, P + nx + 1024, nf, i, g);
+            train_block(f, h, L.x[1 + k], mrgb[1 + k], base, 4 + k * kW, noise, i, g);
+            f = nf;
+        }
         store_act(L.x[8], h, i, g);
         linear_out(P + kRout, h, kr, i, g);
     }
@@ -522,14 +576,19 @@ __global__ __launch_bounds__(64) void k_smlp_train(TrainArgs A) {
     {
         Act dh, dz;
         out_backward(P + kRout, L.dk, L.x[8], part, kRout, dh, i, g);
+        BFrag bf;
+        load_bfrag(P + rblk(kBlocks - 1), bf, i, g);
 #pragma unroll 1
         for (int k = kBlocks - 1; k >= 0; --k) {
+            BFrag nbf;
+            load_bfrag(P + rblk(k > 0 ? k - 1 : 0), nbf, i, g);
             apply_mask(dh, mrgb[1 + k], dz);
             store_act(L.dz, dz, i, g);
             __syncthreads();
             weight_grads(L.dz, L.x[1 + k], part, rblk(k), true, lane, i, g);
-            backward_data(P + rblk(k), dz, dh, i, g);
+            backward_data_f(bf, dz, dh);
             __syncthreads();
+            bf = nbf;
         }
         apply_mask(dh, mrgb[0], dz);
         store_act(L.dz, dz, i, g);
@@ -542,8 +601,16 @@ __global__ __launch_bounds__(64) void k_smlp_train(TrainArgs A) {
     {
         Act h;
         embed_lambda(lx, h, g);
+        Frag f;
+        load_frag(P + lblk(0), P + lblk(0) + 1024, f, i, g);
 #pragma unroll 1
-        for (int k = 0; k < kBlocks; ++k) train_block(P, lblk(k), h, L.x[1 + k], mlam[k], base, lam_slot + k * kW, noise, i, g);
+        for (int k = 0; k < kBlocks; ++k) {
+            const int nx = lblk(k + 1 < kBlocks ? k + 1 : k);
+            Frag nf;
+            load_frag(P + nx, P + nx + 1024, nf, i, g);
+            train_block(f, h, L.x[1 + k], mlam[k], base, lam_slot + k * kW, noise, i, g);
+            f = nf;
+        }
         store_act(L.x[8], h, i, g);
         if (g == 0)
 #pragma unroll
@@ -551,14 +618,19 @@ __global__ __launch_bounds__(64) void k_smlp_train(TrainArgs A) {
         __syncthreads();
         Act dh, dz;
         out_backward(P + kLout, L.dk, L.x[8], part, kLout, dh, i, g);
+        BFrag bf;
+        load_bfrag(P + lblk(kBlocks - 1), bf, i, g);
 #pragma unroll 1
         for (int k = kBlocks - 1; k >= 0; --k) {
+            BFrag nbf;
+            load_bfrag(P + lblk(k > 0 ? k - 1 : 0), nbf, i, g);
             apply_mask(dh, mlam[k], dz);
             store_act(L.dz, dz, i, g);
             __syncthreads();
             weight_grads(L.dz, L.x[1 + k], part, lblk(k), true, lane, i, g);
-            if (k > 0) backward_data(P + lblk(k), dz, dh, i, g);
+            if (k > 0) backward_data_f(bf, dz, dh);
             __syncthreads();
+            bf = nbf;
         }
     }
 }
