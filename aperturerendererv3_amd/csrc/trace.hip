@@ -405,10 +405,14 @@ __device__ __forceinline__ uint32_t leaf(const float4* __restrict__ tris, const 
     return c;
 }
 
-#ifndef AR_TRACE_STEP_BUDGET
-#define AR_TRACE_STEP_BUDGET 4
+// traversal steps between two ray-refill rounds: closest-hit rays are longer,
+// so more steps per round pay for the refill; any-hit rays end early
+#ifndef AR_TRACE_STEP_BUDGET_CLOSEST
+#define AR_TRACE_STEP_BUDGET_CLOSEST 6
 #endif
-constexpr int kStepBudget = AR_TRACE_STEP_BUDGET;   // traversal steps between two ray-refill rounds
+#ifndef AR_TRACE_STEP_BUDGET_ANY
+#define AR_TRACE_STEP_BUDGET_ANY 4
+#endif
 #ifndef AR_TRACE_CHUNK
 #define AR_TRACE_CHUNK 128
 #endif
@@ -418,15 +422,22 @@ constexpr int kStepBudget = AR_TRACE_STEP_BUDGET;   // traversal steps between t
 // (one atomic per wave and round), so a wave never idles on its slowest ray.
 // ANY: sets bit idx of occ (zeroed beforehand) for occluded rays.
 #ifndef AR_TRACE_WAVES_CLOSEST
-#define AR_TRACE_WAVES_CLOSEST 6
+#define AR_TRACE_WAVES_CLOSEST 7
 #endif
 #ifndef AR_TRACE_WAVES_ANY
 #define AR_TRACE_WAVES_ANY 7
 #endif
+// kernels with analytic and SDF objects hold more state: at 5 waves per SIMD
+// they keep it in registers (at 6 or 7 they spill 70-100 bytes per lane)
+#ifndef AR_TRACE_WAVES_OBJ
+#define AR_TRACE_WAVES_OBJ 5
+#endif
 
 template <int W, bool ANY, bool COUNT, bool OBJ>
 __global__ __launch_bounds__(kTraceBlock)
-__attribute__((amdgpu_waves_per_eu(ANY ? AR_TRACE_WAVES_ANY : AR_TRACE_WAVES_CLOSEST, 8))) void k_trace(
+__attribute__((amdgpu_waves_per_eu(OBJ ? AR_TRACE_WAVES_OBJ
+                                       : ANY ? AR_TRACE_WAVES_ANY : COUNT ? 6 : AR_TRACE_WAVES_CLOSEST,
+                                   8))) void k_trace(
     const void* __restrict__ nodes, const float4* __restrict__ tris, uint32_t root,
     const uint32_t* __restrict__ roots, float origin_limit, const float4* __restrict__ objs, uint32_t obj_base,
     const float* __restrict__ vol, ar_rays r, size_t n, const uint32_t* __restrict__ n_dev, ar_hits h, uint32_t* __restrict__ occ,
@@ -507,6 +518,7 @@ __attribute__((amdgpu_waves_per_eu(ANY ? AR_TRACE_WAVES_ANY : AR_TRACE_WAVES_CLO
             // step; leaves run once at least 1 / kDefer of the live lanes are
             // at one (always when all are).  kDefer = 0: no waiting.
             constexpr uint32_t kDefer = ANY ? AR_TRACE_LEAF_DEFER_ANY : AR_TRACE_LEAF_DEFER_CLOSEST;
+            constexpr int kStepBudget = ANY ? AR_TRACE_STEP_BUDGET_ANY : AR_TRACE_STEP_BUDGET_CLOSEST;
             for (int it = 0; it < kStepBudget; ++it) {
                 const bool live = s.ref != kRefEmpty;
                 const bool at_leaf = live && (int)s.ref < 0;
