@@ -244,3 +244,37 @@ def test_network_basis_sky_above_one(ctx):
         assert g.mean() > 0.1
     finally:
         ctx.spectral_set_basis(False)
+
+
+def test_error_paths():
+    """Argument checks of the spectral-network entry points (no kernel runs
+    for a rejected call)."""
+    import torch
+    from aperturerendererv3_amd import ArError, Context
+    c = Context(0)
+    try:
+        x = torch.zeros((64, 4), device="cuda")
+        y = torch.zeros(64, device="cuda")
+        with pytest.raises(ArError):
+            c.spectral_mlp_eval(x)                          # no parameters yet
+        with pytest.raises(ArError):
+            c.spectral_set_basis(True)                      # no tables yet
+        w = _ref()[0].copy()
+        w[5] = np.nan
+        with pytest.raises(ArError):
+            c.spectral_mlp_set(w)                           # non-finite parameter
+        with pytest.raises(ValueError):
+            c.spectral_mlp_set(w[:100])                     # wrong size
+        c.spectral_mlp_set(_ref()[0])
+        with pytest.raises(ArError):
+            c.spectral_mlp_train(x, y, steps=1, batch=33)   # not a multiple of 32
+        with pytest.raises(ArError):
+            c.spectral_mlp_train(x, y, steps=1, batch=1 << 19)
+        with pytest.raises(ArError):
+            c.spectral_mlp_build_tables(1, 840)
+        assert c.spectral_mlp_eval(torch.zeros((0, 4), device="cuda")).numel() == 0
+        hist = c.spectral_mlp_train(x, y, steps=0, batch=32)
+        assert hist.numel() == 0
+        assert np.array_equal(c.spectral_mlp_get(), _ref()[0])   # nothing moved
+    finally:
+        c.close()
