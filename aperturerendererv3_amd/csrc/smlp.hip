@@ -324,7 +324,7 @@ struct TrainArgs {
 };
 
 struct TrainLds {
-    float x[9][kTile * kW];    // layer inputs [sample][unit]
+    float x[8][kTile * kW];    // block inputs 0..6 and the output layer's input, [sample][unit]
     float dz[kTile * kW];      // output gradient of the layer being differentiated
     float dk[kTile * 4];       // gradient of the 4 encoder outputs
     float red[kTile];
@@ -516,7 +516,6 @@ __global__ __launch_bounds__(64) void k_smlp_train(TrainArgs A) {
     {
         Act e, h;
         embed_rgb(rgb, s, e, g);
-        store_act(L.x[0], e, i, g);
         linear(P + kRinW, P + kRinB, e, h, i, g);
         uint32_t m = 0;
 #pragma unroll
@@ -536,10 +535,10 @@ __global__ __launch_bounds__(64) void k_smlp_train(TrainArgs A) {
             const int nx = rblk(k + 1 < kBlocks ? k + 1 : k);
             Frag nf;
             load_frag(P + nx, P + nx + 1024, nf, i, g);
-            train_block(f, h, L.x[1 + k], mrgb[1 + k], base, 4 + k * kW, noise, i, g);
+            train_block(f, h, L.x[k], mrgb[1 + k], base, 4 + k * kW, noise, i, g);
             f = nf;
         }
-        store_act(L.x[8], h, i, g);
+        store_act(L.x[7], h, i, g);
         linear_out(P + kRout, h, kr, i, g);
     }
     // loss and the gradient of the encoder outputs (lanes g == 0 hold K)
@@ -575,7 +574,7 @@ __global__ __launch_bounds__(64) void k_smlp_train(TrainArgs A) {
     // RGB backward
     {
         Act dh, dz;
-        out_backward(P + kRout, L.dk, L.x[8], part, kRout, dh, i, g);
+        out_backward(P + kRout, L.dk, L.x[7], part, kRout, dh, i, g);
         BFrag bf;
         load_bfrag(P + rblk(kBlocks - 1), bf, i, g);
 #pragma unroll 1
@@ -585,13 +584,20 @@ __global__ __launch_bounds__(64) void k_smlp_train(TrainArgs A) {
             apply_mask(dh, mrgb[1 + k], dz);
             store_act(L.dz, dz, i, g);
             __syncthreads();
-            weight_grads(L.dz, L.x[1 + k], part, rblk(k), true, lane, i, g);
+            weight_grads(L.dz, L.x[k], part, rblk(k), true, lane, i, g);
             backward_data_f(bf, dz, dh);
             __syncthreads();
             bf = nbf;
         }
         apply_mask(dh, mrgb[0], dz);
         store_act(L.dz, dz, i, g);
+        {
+            // the first block's input is no longer needed: its slot takes the
+            // RGB features (recomputed) for the first Linear's weight gradient
+            Act e;
+            embed_rgb(rgb, s, e, g);
+            store_act(L.x[0], e, i, g);
+        }
         __syncthreads();
         weight_grads(L.dz, L.x[0], part, kRinW, true, lane, i, g);
         __syncthreads();
@@ -608,16 +614,16 @@ __global__ __launch_bounds__(64) void k_smlp_train(TrainArgs A) {
             const int nx = lblk(k + 1 < kBlocks ? k + 1 : k);
             Frag nf;
             load_frag(P + nx, P + nx + 1024, nf, i, g);
-            train_block(f, h, L.x[1 + k], mlam[k], base, lam_slot + k * kW, noise, i, g);
+            train_block(f, h, L.x[k], mlam[k], base, lam_slot + k * kW, noise, i, g);
             f = nf;
         }
-        store_act(L.x[8], h, i, g);
+        store_act(L.x[7], h, i, g);
         if (g == 0)
 #pragma unroll
             for (int nb = 0; nb < kNB; ++nb) *reinterpret_cast<f32x4*>(L.dk + (nb * 16 + i) * 4) = dkl[nb];
         __syncthreads();
         Act dh, dz;
-        out_backward(P + kLout, L.dk, L.x[8], part, kLout, dh, i, g);
+        out_backward(P + kLout, L.dk, L.x[7], part, kLout, dh, i, g);
         BFrag bf;
         load_bfrag(P + lblk(kBlocks - 1), bf, i, g);
 #pragma unroll 1
@@ -627,7 +633,7 @@ __global__ __launch_bounds__(64) void k_smlp_train(TrainArgs A) {
             apply_mask(dh, mlam[k], dz);
             store_act(L.dz, dz, i, g);
             __syncthreads();
-            weight_grads(L.dz, L.x[1 + k], part, lblk(k), true, lane, i, g);
+            weight_grads(L.dz, L.x[k], part, lblk(k), true, lane, i, g);
             if (k > 0) backward_data_f(bf, dz, dh);
             __syncthreads();
             bf = nbf;
