@@ -34,6 +34,16 @@
  *                        Beer-Lambert series, sampler pairing and solid-angle
  *                        integrals (tests/test_oracle_render_objects.py)
  *
+ *   output stage         the ACES curve's published rational fit, gamma and
+ *                        packing closed forms, the luminance MSE in FP64
+ *                        (tests/test_oracle_output.py)
+ *   denoiser             numpy moments of the Box-Cox samples, scipy's Welch
+ *                        t-test, the reference's t-quantile table
+ *                        (tests/test_oracle_denoise.py)
+ *   spectral network     the reference's golden vectors under TF32,
+ *                        torch.autograd, torch's clipping and AdamW
+ *                        (tests/test_oracle_spectral_mlp.py)
+ *
  * Build: oracle/Makefile (gcc -O2 -ffp-contract=off -fopenmp, no fast-math).
  */
 #ifndef AR_ORACLE_H
