@@ -34,8 +34,9 @@
  *                        Beer-Lambert series, sampler pairing and solid-angle
  *                        integrals (tests/test_oracle_render_objects.py)
  *
- *   output stage         the ACES curve's published rational fit, gamma and
- *                        packing closed forms, the luminance MSE in FP64
+ *   output stage         linear / gamma curves and byte packing against
+ *                        numpy, the XYZ -> sRGB matrix on D65 white, ACES
+ *                        properties, the MSE and running mean against numpy
  *                        (tests/test_oracle_output.py)
  *   denoiser             numpy moments of the Box-Cox samples, scipy's Welch
  *                        t-test, the reference's t-quantile table
