@@ -324,7 +324,8 @@ struct TrainArgs {
 };
 
 struct TrainLds {
-    float x[8][kTile * kW];    // block inputs 0..6 and the output layer's input, [sample][unit]
+    float x[8][kTile * kW];    // block inputs 0..6 and the output layer's input, [sample][unit];
+                               // slot 0 then takes the RGB features for the first Linear's gradient
     float dz[kTile * kW];      // output gradient of the layer being differentiated
     float dk[kTile * 4];       // gradient of the 4 encoder outputs
     float red[kTile];
