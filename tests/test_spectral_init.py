@@ -17,5 +17,5 @@ def test_layout_and_statistics():
     allm = np.concatenate(mats)
     assert abs(allm.mean()) < 0.01 and abs(allm.std() - np.sqrt(2.0 / 32.0)) < 0.01   # Kaiming normal, fan in 32
     for out in (w[8448:8576], w[15968:16096]):             # torch's default Linear init
-        assert np.abs(out).max() <= 1.0 / np.sqrt(32.0) and out.std() > 0.1
+        assert np.abs(out).max() <= 1.0 / np.sqrt(32.0) and out.std() > 0.07   # uniform: std 0.102
     assert np.array_equal(spectral_mlp_init(3), w) and not np.array_equal(spectral_mlp_init(4), w)
