@@ -3,7 +3,8 @@
 // (spectrum_lut.cuh:12-235) and their training (pytorch/spectrum/
 // train_spectrum.py), on FP32 MFMA.
 //
-// Every layer is 32 wide.  One wave owns a tile of 32 samples and computes
+// Every layer is 32 wide.  One wave owns a tile of 32 samples (training
+// batches up to 16384: 16 samples, for twice the waves) and computes
 // each Linear as v_mfma_f32_16x16x4f32 products with the samples as the N
 // dimension, lane l = 16 g + i:
 //   A: a = A[i][k = g],  B: b = B[k = g][i],  D: d[r] = D[4 g + r][i].
@@ -47,6 +48,9 @@ constexpr int kChunk = 16;             // tiles summed per first-level reduction
 constexpr int kRedBlock = 256;
 constexpr int kGradBlocks = (kParams + kRedBlock - 1) / kRedBlock;
 constexpr uint32_t kMaxBatch = 1u << 18;
+#ifndef AR_SMLP_SMALL_BATCH
+#define AR_SMLP_SMALL_BATCH 16384
+#endif
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
@@ -55,6 +59,7 @@ __device__ __forceinline__ f32x4 mfma4(float a, float b, f32x4 c) {
 }
 
 // D-layout activations of one tile: v[mb][nb][r] = X[sample 16 nb + i][unit 16 mb + 4 g + r]
+template <int kNB>
 struct Act {
     f32x4 v[2][kNB];
 };
@@ -79,7 +84,8 @@ __device__ __forceinline__ void load_frag(const float* __restrict__ Wm, const fl
 }
 
 // z = W x (+ b), W row-major [32 out][32 in]; the bias is the accumulator's start
-__device__ __forceinline__ void linear_f(const Frag& f, const Act& x, Act& z) {
+template <int kNB>
+__device__ __forceinline__ void linear_f(const Frag& f, const Act<kNB>& x, Act<kNB>& z) {
 #pragma unroll
     for (int mb = 0; mb < 2; ++mb) {
         const float a[8] = {f.w0[mb].x, f.w0[mb].y, f.w0[mb].z, f.w0[mb].w,
@@ -94,7 +100,8 @@ __device__ __forceinline__ void linear_f(const Frag& f, const Act& x, Act& z) {
     }
 }
 
-__device__ __forceinline__ void linear(const float* __restrict__ Wm, const float* __restrict__ b, const Act& x, Act& z,
+template <int kNB>
+__device__ __forceinline__ void linear(const float* __restrict__ Wm, const float* __restrict__ b, const Act<kNB>& x, Act<kNB>& z,
                                        int i, int g) {
     Frag f;
     load_frag(Wm, b, f, i, g);
@@ -102,7 +109,8 @@ __device__ __forceinline__ void linear(const float* __restrict__ Wm, const float
 }
 
 // 32 -> 4 without bias; valid in lanes g == 0: out[nb][r] = K[sample 16 nb + i][r]
-__device__ __forceinline__ void linear_out(const float* __restrict__ Wo, const Act& x, f32x4 out[kNB], int i, int g) {
+template <int kNB>
+__device__ __forceinline__ void linear_out(const float* __restrict__ Wo, const Act<kNB>& x, f32x4 out[kNB], int i, int g) {
     float a[8];
     if (i < 4) {
         const float4 w0 = *reinterpret_cast<const float4*>(Wo + i * kW + 4 * g);
@@ -124,7 +132,8 @@ __device__ __forceinline__ void linear_out(const float* __restrict__ Wo, const A
 
 // RGB embedding in B layout: entry k = 16 h + 4 g + r is channel 2 h + g / 2,
 // sine for even g, cosine for odd g, harmonic r + 1 (RGBEncode's phase order)
-__device__ __forceinline__ void embed_rgb(const float rgb[kNB][3], const float s[kNB], Act& e, int g) {
+template <int kNB>
+__device__ __forceinline__ void embed_rgb(const float rgb[kNB][3], const float s[kNB], Act<kNB>& e, int g) {
     const bool cosine = g & 1;
 #pragma unroll
     for (int nb = 0; nb < kNB; ++nb) {
@@ -143,7 +152,8 @@ __device__ __forceinline__ void embed_rgb(const float rgb[kNB][3], const float s
 }
 
 // wavelength embedding: entry 4 g + r is sin, 16 + 4 g + r cos, harmonic 4 g + r + 1
-__device__ __forceinline__ void embed_lambda(const float x[kNB], Act& e, int g) {
+template <int kNB>
+__device__ __forceinline__ void embed_lambda(const float x[kNB], Act<kNB>& e, int g) {
 #pragma unroll
     for (int nb = 0; nb < kNB; ++nb)
 #pragma unroll
@@ -154,8 +164,9 @@ __device__ __forceinline__ void embed_lambda(const float x[kNB], Act& e, int g) 
         }
 }
 
-__device__ __forceinline__ void residual_f(const Frag& f, Act& h) {
-    Act t;
+template <int kNB>
+__device__ __forceinline__ void residual_f(const Frag& f, Act<kNB>& h) {
+    Act<kNB> t;
     linear_f(f, h, t);
 #pragma unroll
     for (int mb = 0; mb < 2; ++mb)
@@ -167,7 +178,8 @@ __device__ __forceinline__ void residual_f(const Frag& f, Act& h) {
 
 // the 7 residual blocks starting at parameter offset first (inference: several
 // waves per SIMD hide the fragment loads, which are not loaded ahead here)
-__device__ __forceinline__ void blocks(const float* __restrict__ P, int first, Act& h, int i, int g) {
+template <int kNB>
+__device__ __forceinline__ void blocks(const float* __restrict__ P, int first, Act<kNB>& h, int i, int g) {
 #pragma unroll 1
     for (int k = 0; k < kBlocks; ++k) {
         Frag f;
@@ -176,8 +188,9 @@ __device__ __forceinline__ void blocks(const float* __restrict__ P, int first, A
     }
 }
 
-__device__ __forceinline__ void rgb_forward(const float* __restrict__ P, const Act& e, f32x4 K[kNB], int i, int g) {
-    Act h;
+template <int kNB>
+__device__ __forceinline__ void rgb_forward(const float* __restrict__ P, const Act<kNB>& e, f32x4 K[kNB], int i, int g) {
+    Act<kNB> h;
     linear(P + kRinW, P + kRinB, e, h, i, g);
 #pragma unroll
     for (int mb = 0; mb < 2; ++mb)
@@ -189,7 +202,8 @@ __device__ __forceinline__ void rgb_forward(const float* __restrict__ P, const A
     linear_out(P + kRout, h, K, i, g);
 }
 
-__device__ __forceinline__ void lambda_forward(const float* __restrict__ P, Act& h, f32x4 K[kNB], int i, int g) {
+template <int kNB>
+__device__ __forceinline__ void lambda_forward(const float* __restrict__ P, Act<kNB>& h, f32x4 K[kNB], int i, int g) {
     blocks(P, lblk(0), h, i, g);
     linear_out(P + kLout, h, K, i, g);
 }
@@ -236,7 +250,7 @@ __global__ __launch_bounds__(64) void k_smlp_encode(const float* __restrict__ P,
             lx[nb] = fminf(fmaxf((l - 360.0f) / (780.0f - 360.0f), 0.0f), 1.0f);
         }
         f32x4 kr[kNB], kl[kNB];
-        Act e;
+        Act<kNB> e;
         if (MODE != kLambdaTable) {
             embed_rgb(rgb, s, e, g);
             rgb_forward(P, e, kr, i, g);
@@ -323,15 +337,17 @@ struct TrainArgs {
     float* loss;         // [tiles]
 };
 
+template <int kNB>
 struct TrainLds {
-    float x[8][kTile * kW];    // block inputs 0..6 and the output layer's input, [sample][unit];
+    float x[8][16 * kNB * kW];    // block inputs 0..6 and the output layer's input, [sample][unit];
                                // slot 0 then takes the RGB features for the first Linear's gradient
-    float dz[kTile * kW];      // output gradient of the layer being differentiated
-    float dk[kTile * 4];       // gradient of the 4 encoder outputs
-    float red[kTile];
+    float dz[16 * kNB * kW];   // output gradient of the layer being differentiated
+    float dk[16 * kNB * 4];    // gradient of the 4 encoder outputs
+    float red[16 * kNB];
 };
 
-__device__ __forceinline__ void store_act(float* L, const Act& a, int i, int g) {
+template <int kNB>
+__device__ __forceinline__ void store_act(float* L, const Act<kNB>& a, int i, int g) {
 #pragma unroll
     for (int mb = 0; mb < 2; ++mb)
 #pragma unroll
@@ -341,9 +357,10 @@ __device__ __forceinline__ void store_act(float* L, const Act& a, int i, int g) 
 
 // block forward in training mode: the Linear sees h + noise, the residual h;
 // records the block input in LDS (when given) and the ReLU mask
-__device__ __forceinline__ void train_block(const Frag& f, Act& h, float* L, uint32_t& mask,
+template <int kNB>
+__device__ __forceinline__ void train_block(const Frag& f, Act<kNB>& h, float* L, uint32_t& mask,
                                             const uint32_t base[kNB], uint32_t slot0, bool noise, int i, int g) {
-    Act xin = h;
+    Act<kNB> xin = h;
     if (noise) {                       // slot0 is even: units 4g + {0, 1} and 4g + {2, 3} are pairs
 #pragma unroll
         for (int mb = 0; mb < 2; ++mb)
@@ -358,7 +375,7 @@ __device__ __forceinline__ void train_block(const Frag& f, Act& h, float* L, uin
                 }
     }
     if (L) store_act(L, xin, i, g);
-    Act z;
+    Act<kNB> z;
     linear_f(f, xin, z);
     mask = 0;
 #pragma unroll
@@ -373,7 +390,8 @@ __device__ __forceinline__ void train_block(const Frag& f, Act& h, float* L, uin
             }
 }
 
-__device__ __forceinline__ void apply_mask(const Act& dh, uint32_t mask, Act& dz) {
+template <int kNB>
+__device__ __forceinline__ void apply_mask(const Act<kNB>& dh, uint32_t mask, Act<kNB>& dz) {
 #pragma unroll
     for (int mb = 0; mb < 2; ++mb)
 #pragma unroll
@@ -385,6 +403,7 @@ __device__ __forceinline__ void apply_mask(const Act& dh, uint32_t mask, Act& dz
 
 // dW[u][v] = sum_s dZ[s][u] X[s][v] and db[u] = sum_s dZ[s][u] from LDS,
 // written to this tile's partials at off (bias at off + 1024 when has_bias)
+template <int kNB>
 __device__ __forceinline__ void weight_grads(const float* dzl, const float* xl, float* __restrict__ part, int off,
                                              bool has_bias, int lane, int i, int g) {
 #pragma unroll
@@ -393,14 +412,14 @@ __device__ __forceinline__ void weight_grads(const float* dzl, const float* xl, 
         for (int nv = 0; nv < 2; ++nv) {
             f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-            for (int kk = 0; kk < kTile / 4; ++kk)
+            for (int kk = 0; kk < (16 * kNB) / 4; ++kk)
                 c = mfma4(dzl[(4 * kk + g) * kW + mb * 16 + i], xl[(4 * kk + g) * kW + nv * 16 + i], c);
 #pragma unroll
             for (int r = 0; r < 4; ++r) part[off + (mb * 16 + 4 * g + r) * kW + nv * 16 + i] = c[r];
         }
     if (has_bias && lane < kW) {
         float s = 0.0f;
-        for (int k = 0; k < kTile; ++k) s += dzl[k * kW + lane];
+        for (int k = 0; k < (16 * kNB); ++k) s += dzl[k * kW + lane];
         part[off + 1024 + lane] = s;
     }
 }
@@ -418,7 +437,8 @@ __device__ __forceinline__ void load_bfrag(const float* __restrict__ Wm, BFrag& 
 }
 
 // dh += W^T dz
-__device__ __forceinline__ void backward_data_f(const BFrag& f, const Act& dz, Act& dh) {
+template <int kNB>
+__device__ __forceinline__ void backward_data_f(const BFrag& f, const Act<kNB>& dz, Act<kNB>& dh) {
 #pragma unroll
     for (int mb = 0; mb < 2; ++mb)
 #pragma unroll
@@ -431,8 +451,9 @@ __device__ __forceinline__ void backward_data_f(const BFrag& f, const Act& dz, A
 }
 
 // output layer backward: dh = Wo^T dK, dWo = dK^T h7 (dK in LDS [s][4], h7 in LDS)
+template <int kNB>
 __device__ __forceinline__ void out_backward(const float* __restrict__ Wo, const float* dk, const float* h7,
-                                             float* __restrict__ part, int off, Act& dh, int i, int g) {
+                                             float* __restrict__ part, int off, Act<kNB>& dh, int i, int g) {
 #pragma unroll
     for (int mb = 0; mb < 2; ++mb) {
         const float a = Wo[g * kW + mb * 16 + i];
@@ -446,7 +467,7 @@ __device__ __forceinline__ void out_backward(const float* __restrict__ Wo, const
     for (int nv = 0; nv < 2; ++nv) {
         f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-        for (int kk = 0; kk < kTile / 4; ++kk)
+        for (int kk = 0; kk < (16 * kNB) / 4; ++kk)
             c = mfma4(i < 4 ? dk[(4 * kk + g) * 4 + i] : 0.0f, h7[(4 * kk + g) * kW + nv * 16 + i], c);
         if (g == 0)
 #pragma unroll
@@ -461,8 +482,10 @@ __device__ __forceinline__ float importance2(float l) {
     return fx * fx + fy * fy + fz * fz;
 }
 
+template <int kNB>
 __global__ __launch_bounds__(64) void k_smlp_train(TrainArgs A) {
-    __shared__ TrainLds L;
+    constexpr int kTile = 16 * kNB;
+    __shared__ TrainLds<kNB> L;
     const int lane = threadIdx.x, i = lane & 15, g = lane >> 4;
     const uint32_t tile = blockIdx.x;
     const float* P = A.P;
@@ -496,7 +519,7 @@ __global__ __launch_bounds__(64) void k_smlp_train(TrainArgs A) {
     // wavelength encoder forward (recomputed with records after the RGB backward)
     f32x4 kl[kNB];
     {
-        Act h;
+        Act<kNB> h;
         embed_lambda(lx, h, g);
         uint32_t mk;
         Frag f;
@@ -515,7 +538,7 @@ __global__ __launch_bounds__(64) void k_smlp_train(TrainArgs A) {
     uint32_t mrgb[kBlocks + 1];
     f32x4 kr[kNB];
     {
-        Act e, h;
+        Act<kNB> e, h;
         embed_rgb(rgb, s, e, g);
         linear(P + kRinW, P + kRinB, e, h, i, g);
         uint32_t m = 0;
@@ -574,7 +597,7 @@ __global__ __launch_bounds__(64) void k_smlp_train(TrainArgs A) {
     }
     // RGB backward
     {
-        Act dh, dz;
+        Act<kNB> dh, dz;
         out_backward(P + kRout, L.dk, L.x[7], part, kRout, dh, i, g);
         BFrag bf;
         load_bfrag(P + rblk(kBlocks - 1), bf, i, g);
@@ -585,7 +608,7 @@ __global__ __launch_bounds__(64) void k_smlp_train(TrainArgs A) {
             apply_mask(dh, mrgb[1 + k], dz);
             store_act(L.dz, dz, i, g);
             __syncthreads();
-            weight_grads(L.dz, L.x[k], part, rblk(k), true, lane, i, g);
+            weight_grads<kNB>(L.dz, L.x[k], part, rblk(k), true, lane, i, g);
             backward_data_f(bf, dz, dh);
             __syncthreads();
             bf = nbf;
@@ -595,18 +618,18 @@ __global__ __launch_bounds__(64) void k_smlp_train(TrainArgs A) {
         {
             // the first block's input is no longer needed: its slot takes the
             // RGB features (recomputed) for the first Linear's weight gradient
-            Act e;
+            Act<kNB> e;
             embed_rgb(rgb, s, e, g);
             store_act(L.x[0], e, i, g);
         }
         __syncthreads();
-        weight_grads(L.dz, L.x[0], part, kRinW, true, lane, i, g);
+        weight_grads<kNB>(L.dz, L.x[0], part, kRinW, true, lane, i, g);
         __syncthreads();
     }
     // wavelength encoder again, with records, then its backward
     uint32_t mlam[kBlocks];
     {
-        Act h;
+        Act<kNB> h;
         embed_lambda(lx, h, g);
         Frag f;
         load_frag(P + lblk(0), P + lblk(0) + 1024, f, i, g);
@@ -623,7 +646,7 @@ __global__ __launch_bounds__(64) void k_smlp_train(TrainArgs A) {
 #pragma unroll
             for (int nb = 0; nb < kNB; ++nb) *reinterpret_cast<f32x4*>(L.dk + (nb * 16 + i) * 4) = dkl[nb];
         __syncthreads();
-        Act dh, dz;
+        Act<kNB> dh, dz;
         out_backward(P + kLout, L.dk, L.x[7], part, kLout, dh, i, g);
         BFrag bf;
         load_bfrag(P + lblk(kBlocks - 1), bf, i, g);
@@ -634,7 +657,7 @@ __global__ __launch_bounds__(64) void k_smlp_train(TrainArgs A) {
             apply_mask(dh, mlam[k], dz);
             store_act(L.dz, dz, i, g);
             __syncthreads();
-            weight_grads(L.dz, L.x[k], part, lblk(k), true, lane, i, g);
+            weight_grads<kNB>(L.dz, L.x[k], part, lblk(k), true, lane, i, g);
             if (k > 0) backward_data_f(bf, dz, dh);
             __syncthreads();
             bf = nbf;
@@ -782,11 +805,16 @@ int check_batch(ar_ctx* ctx, uint32_t batch, size_t n) {
 int gradient(ar_ctx* ctx, const float* x, const float* y, size_t n, uint32_t batch, uint32_t step, uint32_t seed,
              uint32_t noise, float* grad_out, float* loss_out, hipStream_t st) {
     SmlpDev& S = ctx->smlp;
-    const uint32_t tiles = batch / kTile;
+    // small batches take 16-sample tiles: twice the waves, half the chain each
+    const uint32_t nb = batch <= AR_SMLP_SMALL_BATCH ? 1u : 2u;
+    const uint32_t tiles = batch / (16u * nb);
     int rc = ensure_part(ctx, tiles);
     if (rc != AR_OK) return rc;
     TrainArgs A{S.p, x, y, n, batch, step, seed, noise, S.part, S.misc + kGradBlocks};
-    hipLaunchKernelGGL(k_smlp_train, dim3(tiles), dim3(64), 0, st, A);
+    if (nb == 1)
+        hipLaunchKernelGGL(k_smlp_train<1>, dim3(tiles), dim3(64), 0, st, A);
+    else
+        hipLaunchKernelGGL(k_smlp_train<2>, dim3(tiles), dim3(64), 0, st, A);
     const uint32_t chunks = (tiles + kChunk - 1) / kChunk;
     hipLaunchKernelGGL(k_smlp_reduce1, dim3((kParams / 4 + kRedBlock - 1) / kRedBlock, chunks), dim3(kRedBlock), 0,
                        st, S.part, tiles, S.part2);

@@ -133,6 +133,28 @@ def test_gradient_matches_oracle(ctx, noise):
     assert err <= 2e-4 and cos > 0.99999
 
 
+def test_gradient_matches_oracle_32_sample_tiles(ctx):
+    """Batches above 16384 run 32-sample tiles (smaller ones 16-sample
+    tiles): the same bar on a batch of 16416."""
+    import torch
+    w = _ref()[0]
+    ctx.spectral_mlp_set(w)
+    n, batch = 20000, 16416
+    x = _rows(n, 14)
+    y = np.random.default_rng(15).random(n).astype(np.float32)
+    print(f"gradient batch={batch}", flush=True)
+    g, loss = ctx.spectral_mlp_gradient(_cuda(x), _cuda(y), batch, step=1, seed=5, noise=True)
+    torch.cuda.synchronize()
+    rows = (1 * batch + np.arange(batch)) % n
+    ol, og = orc.smlp_loss_grad(w, x[rows], y[rows], noise=True, seed=5, step=1)
+    g = g.cpu().numpy().astype(np.float64)
+    err = np.abs(g - og).max() / np.abs(og).max()
+    cos = float(g @ og / np.linalg.norm(g) / np.linalg.norm(og))
+    print(f"  loss {loss.item():.6e} vs {ol:.6e}; max err / max |g| {err:.3e}, cosine {cos:.9f}", flush=True)
+    assert abs(loss.item() - ol) <= 1e-4 * ol
+    assert err <= 2e-4 and cos > 0.99999
+
+
 def test_adamw_steps_match_oracle(ctx):
     import torch
     w = _ref()[0]
