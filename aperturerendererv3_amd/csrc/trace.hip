@@ -1,14 +1,21 @@
-// Closest-hit and any-hit BVH2 traversal for gfx950.
+// Closest-hit and any-hit BVH traversal for gfx950.
 //
 // Replaces TraverseBVH<bAnyHit> / AnyHitSegmentBVH of the reference
 // (core/render/intersection.cuh:148-310) for triangle meshes and the
-// analytic box / sphere objects (objects.hpp).  Design:
-//  * persistent grid: each wave walks 64 consecutive rays at a time, so ray
-//    and hit arrays (SoA) are read and written fully coalesced;
-//  * 64-byte nodes holding both child boxes: one node fetch tests two boxes;
+// analytic box / sphere / SDF objects (objects.hpp).  Design:
+//  * persistent grid with per-lane refill: each wave owns a private chunk
+//    of 128 ray indices taken with one atomic, and hands the next index of
+//    its chunk to every lane whose ray finished, so lanes stay busy despite
+//    divergent ray lengths; a bounded step budget per refill round;
+//  * default layout: the quantised 8-wide BVH, one 128-byte node (one L2
+//    line) per interior node, children decoded with one FMA per plane and
+//    stored along the node's largest axis so pushes need no sort; the
+//    binary layout (64-byte nodes holding both child boxes) is width 2;
 //  * per-lane traversal stack: the top kLdsStack entries in LDS (indexed
 //    [entry][thread], conflict-free), deeper entries spill to an HBM buffer
 //    laid out [entry][global thread];
+//  * leaf deferral: lanes at a leaf wait until a third of the live lanes
+//    are at one, so triangle tests run with more lanes active;
 //  * watertight triangle test (Woop, Benthin, Wald 2013) compiled without
 //    FMA contraction and with IEEE division, so hit ids, t and barycentrics
 //    are bit-identical to a CPU evaluation of the same formula.
