@@ -439,6 +439,17 @@ class Context:
                         "ar_trace_closest")
         return hits
 
+    def trace_closest_noalloc(self, rays, hits=None, stream=None):
+        """ar_trace_closest_noalloc: closest hit with the library's fixed
+        per-device scratch array (no device allocation in the call)."""
+        n = rays.shape[1]
+        if hits is None:
+            hits = self.alloc_hits(n, rays.device)
+        rc, hc = self._rays_c(rays), self._hits_c(hits)
+        self._check(self._lib.ar_trace_closest_noalloc(self._h, C.byref(rc), n, C.byref(hc), _stream_ptr(stream)),
+                    "ar_trace_closest_noalloc")
+        return hits
+
     def trace_any(self, rays, occluded=None, stream=None):
         import torch
         n = rays.shape[1]

@@ -144,6 +144,8 @@ SIGNATURES = {
     "ar_bvh_build": (C.c_int, [C.c_void_p, C.POINTER(ar_bvh_opts)]),
     "ar_bvh_info_get": (C.c_int, [C.c_void_p, C.POINTER(ar_bvh_info)]),
     "ar_trace_closest": (C.c_int, [C.c_void_p, C.POINTER(ar_rays), C.c_size_t, C.POINTER(ar_hits), C.c_void_p]),
+    "ar_trace_closest_noalloc": (C.c_int, [C.c_void_p, C.POINTER(ar_rays), C.c_size_t, C.POINTER(ar_hits),
+                                           C.c_void_p]),
     "ar_trace_any": (C.c_int, [C.c_void_p, C.POINTER(ar_rays), C.c_size_t, C.c_void_p, C.c_void_p]),
     "ar_trace_closest_counted": (C.c_int, [C.c_void_p, C.POINTER(ar_rays), C.c_size_t, C.POINTER(ar_hits),
                                            C.c_void_p, C.c_void_p, C.c_void_p]),

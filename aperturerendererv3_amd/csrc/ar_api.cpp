@@ -182,6 +182,9 @@ int ar_create(ar_ctx** out, const ar_device_opts* opts) {
         delete ctx;
         return AR_E_DEVICE;
     }
+    // the no-allocation trace's scratch address on this device; a failure
+    // here is reported by that entry point alone
+    (void)noalloc_scratch_setup(ctx->device);
     *out = ctx;
     return AR_OK;
 }

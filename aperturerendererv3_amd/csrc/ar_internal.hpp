@@ -397,6 +397,9 @@ hipError_t launch_trace_any(const TraceArgs& ta, const ar_rays& r, size_t n, uin
 hipError_t launch_shadow_rays(const ar_rays& r, const ar_hits& h, size_t n, float tx, float ty, float tz,
                               float tmin, float tmax_scale, const ar_rays& out, hipStream_t s);
 int trace_blocks_per_cu(int width = 0, bool any = false);
+// device address of ar_trace_closest_noalloc's scratch array on `device`
+// (the current device); resolved once per device
+hipError_t noalloc_scratch_setup(int device);
 
 // integrator.hip
 int upload_scene(ar_ctx* ctx);

@@ -22,7 +22,10 @@
 #include "ar_internal.hpp"
 #include "objects.hpp"
 
+#include <algorithm>
 #include <cfloat>
+#include <mutex>
+#include <string>
 
 namespace ar {
 namespace {
@@ -700,4 +703,94 @@ hipError_t launch_shadow_rays(const ar_rays& r, const ar_hits& h, size_t n, floa
     return hipGetLastError();
 }
 
+// ---- closest hit without allocation ----------------------------------------
+// The traversal's scratch (the 32-bit ray-refill counter and the per-thread
+// stack spill) lives in this fixed array instead of context buffers, for
+// callers that cannot allocate.  Layout: the counter in bytes 0..255, then
+// the spill ints [entry][thread] for the launched grid.  Each device holds its
+// own copy for the life of the process; every call on a device shares it.
+constexpr size_t kNoAllocScratchBytes = size_t(32) << 20;
+constexpr size_t kNoAllocHeader = 256;
+constexpr int kNoAllocMaxDevices = 64;
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kNoAllocMaxDevices] = {};
+hipError_t g_scratch_err[kNoAllocMaxDevices] = {};
+bool g_scratch_done[kNoAllocMaxDevices] = {};
+} // namespace
+
+// Resolves the device address of `scratch` on `device`, which must be the
+// current device; once per device (later calls return the first result).
+hipError_t noalloc_scratch_setup(int device) {
+    if (device < 0 || device >= kNoAllocMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_done[device]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        g_scratch_ptr[device] = e == hipSuccess ? ptr : nullptr;
+        g_scratch_err[device] = e;
+        g_scratch_done[device] = true;
+    }
+    return g_scratch_err[device];
+}
+
+static void* noalloc_scratch(int device) {
+    if (device < 0 || device >= kNoAllocMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return g_scratch_done[device] ? g_scratch_ptr[device] : nullptr;
+}
+
 } // namespace ar
+
+using namespace ar;
+
+extern "C" int ar_trace_closest_noalloc(ar_ctx* ctx, const ar_rays* rays, size_t n, const ar_hits* hits,
+                                        void* stream) {
+    if (!ctx) return AR_E_INVALID;
+    if (n == 0) return AR_OK;
+    if (!rays || !rays->ox || !rays->oy || !rays->oz || !rays->tmin || !rays->dx || !rays->dy || !rays->dz ||
+        !rays->tmax || !hits || !hits->prim || !hits->t || !hits->u || !hits->v || !hits->flags)
+        return set_error(ctx, AR_E_INVALID, "NULL ray or hit array");
+    if (!ctx->bvh_ready) return set_error(ctx, AR_E_NOSCENE, "no BVH: call ar_scene_set and ar_bvh_build first");
+    // ray indices come from the 32-bit refill counter, advanced in chunks
+    if (n > (size_t(1) << 31))
+        return set_error(ctx, AR_E_CAPACITY, "ar_trace_closest_noalloc: at most 2^31 rays per call");
+    unsigned char* base = static_cast<unsigned char*>(noalloc_scratch(ctx->device));
+    if (!base)
+        return set_error(ctx, AR_E_DEVICE, "ar_trace_closest_noalloc: the scratch array's device address is unknown "
+                                           "(hipGetSymbolAddress failed when the context was created)");
+    const uint32_t stack = ctx->bvh.stack_entries;
+    const size_t deep = stack > (uint32_t)kLdsStack ? stack - (uint32_t)kLdsStack : 1u;
+    const size_t per_block = deep * kTraceBlock * sizeof(int);
+    const size_t fit_blocks = (kNoAllocScratchBytes - kNoAllocHeader) / per_block;
+    if (fit_blocks < (size_t)ctx->num_cus)
+        return set_error(ctx, AR_E_CAPACITY,
+                         "ar_trace_closest_noalloc: the BVH's stack (" + std::to_string(stack) +
+                             " entries) needs " + std::to_string(per_block * (size_t)ctx->num_cus + kNoAllocHeader) +
+                             " bytes of scratch for one block per CU; the scratch array holds " +
+                             std::to_string(kNoAllocScratchBytes));
+    TraceArgs ta;
+    ta.width = ctx->bvh.width;
+    ta.stack_entries = stack;
+    ta.nodes = ctx->bvh.nodes;
+    ta.tris = reinterpret_cast<const float4*>(ctx->bvh.tris);
+    ta.root = ctx->bvh.root;
+    ta.origin_limit = ctx->bvh.origin_limit;
+    ta.objs = ctx->scene.nobj ? ctx->scene.objs : nullptr;
+    ta.obj_base = (uint32_t)(ctx->idx.size() / 3);
+    ta.vol = ctx->sdf.dev;
+    const size_t resident = (size_t)ctx->num_cus * (size_t)trace_blocks_per_cu((int)ctx->bvh.width, false);
+    const size_t need = (n + kTraceBlock - 1) / kTraceBlock;
+    const uint32_t blocks = (uint32_t)std::max<size_t>(1, std::min(std::min(resident, fit_blocks), need));
+    ta.full_closest = blocks;
+    ta.full_any = blocks;
+    ta.threads = blocks * (uint32_t)kTraceBlock;   // spill stride = launched threads
+    ta.fetch = reinterpret_cast<uint32_t*>(base);
+    ta.spill = reinterpret_cast<int*>(base + kNoAllocHeader);
+    return check_hip(ctx, launch_trace_closest(ta, *rays, n, *hits, nullptr, nullptr, blocks,
+                                               reinterpret_cast<hipStream_t>(stream)),
+                     "trace_closest_noalloc launch");
+}

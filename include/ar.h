@@ -221,6 +221,19 @@ int ar_trace_any(ar_ctx* ctx, const ar_rays* rays, size_t n, uint32_t* occluded,
 int ar_trace_closest_counted(ar_ctx* ctx, const ar_rays* rays, size_t n, const ar_hits* hits,
                              uint32_t* box_tests, uint32_t* tri_tests, void* stream);
 
+/* ar_trace_closest without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same rays, hits and results as ar_trace_closest, bit for bit.
+ * The traversal's scratch (ray-refill counter and stack spill) is a fixed
+ * 32 MiB array per device, held until the process ends and shared by every
+ * call on that device from any context: calls that overlap on one device
+ * race and must be ordered by the caller (one stream, or events).  The grid
+ * is limited to what that array holds (the spill needs (stack entries - 16)
+ * x 1 KiB per 256-thread block); AR_E_CAPACITY if one block per CU does not
+ * fit (an 8-wide BVH deeper than 19 levels) or n > 2^31. */
+int ar_trace_closest_noalloc(ar_ctx* ctx, const ar_rays* rays, size_t n, const ar_hits* hits, void* stream);
+
 /* Shadow segments toward a point light for every hit of a closest-hit pass
  * (the traversal benchmark's any-hit workload): for ray r with a hit, the
  * segment starts at o + t*d, points at `target` and ends at
