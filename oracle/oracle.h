@@ -34,6 +34,20 @@
  *                        Beer-Lambert series, sampler pairing and solid-angle
  *                        integrals (tests/test_oracle_render_objects.py)
  *
+ *   integrator           Lambert furnace, polygon-irradiance direct light,
+ *                        NEE on / off, FP32 = FP64 paths
+ *                        (tests/test_oracle_render.py)
+ *   GGX BSDF, samplers   D's projected-area integral and tangent form, Smith
+ *                        G's Lambda form, a numpy microfacet model,
+ *                        reciprocity, Schlick limits; bounded-VNDF, cosine and
+ *                        mixture samplers: pdf pairing, chi-square against
+ *                        the pdf, surviving mass, unbiased albedo estimate
+ *   light pick, RR       area * lum(Le) select probabilities and pick
+ *                        chi-square, NEE on / off with unequal lights; RR
+ *                        survivors at exactly 1/q of the RR-free path value,
+ *                        survival fraction q, deep mean unchanged
+ *                        (tests/test_oracle_bsdf.py)
+ *
  *   output stage         linear / gamma curves and byte packing against
  *                        numpy, the XYZ -> sRGB matrix on D65 white, ACES
  *                        properties, the MSE and running mean against numpy
@@ -361,6 +375,33 @@ int orc_obj_light_sample_d(const orc_object* ob, const double x[3], double ub, d
                            double ny[3], double* pa);
 double orc_obj_light_pdf_d(const orc_object* ob, double select, const double x[3], const double y[3],
                            const double ny[3]);
+
+/* Light table of orc_render (core/scene.cu:677-701): every emissive
+ * primitive (triangles, then objects) with select probability area *
+ * lum(Le) / sum, its running FP32 CDF; light_index: per primitive, -1 or its
+ * light.  Returns the light count. */
+int orc_light_table(const float* verts, const uint32_t* idx, uint32_t ntri, const uint32_t* tri_mat,
+                    const orc_material* mats, const orc_object* objects, uint32_t nobj, const orc_material* obj_mats,
+                    int32_t* light_index, int32_t* ltri, float* larea, float* lprob, float* lcdf);
+
+/* FP64 instances of the BSDF pieces (brdf.cuh, sample.cuh, material.cuh; a
+ * material in RGB mode), the light pick of NEESampleLight and the RR
+ * survival probability, for the tests.  rd points toward the surface. */
+void orc_ggx_d_d(double alpha, size_t count, const double* noh, double* out);
+void orc_g_correlated_d(double alpha, size_t count, const double* nol, const double* nov, double* out);
+void orc_sample_vndf_d(const double v[3], const double n[3], double alpha, size_t count, const double* r,
+                       double* l, double* pdf);
+void orc_vndf_pdf_d(const double v[3], const double n[3], double alpha, size_t count, const double* l, double* out);
+void orc_sample_cosine_d(const double n[3], size_t count, const double* e, double* l, double* pdf);
+double orc_pspec_d(const orc_material* m);
+void orc_eval_bsdf_d(const orc_material* m, const double rd[3], const double n[3], size_t count, const double* l,
+                     double* out);
+void orc_bsdf_pdf_d(const orc_material* m, const double rd[3], const double n[3], size_t count, const double* l,
+                    double* out);
+void orc_sample_bsdf_d(const orc_material* m, const double rd[3], const double n[3], size_t count, const double* u,
+                       double* l, double* pdf, int32_t* ok);
+void orc_pick_light_d(const float* cdf, int nlights, size_t count, const double* u, int32_t* out);
+double orc_rr_survival_d(const double mask[3]);
 
 /* ---------------- output stage (output.c) ---------------- */
 

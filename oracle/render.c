@@ -275,6 +275,39 @@ static float tri_area_f(const float* a, const float* b, const float* c)
     return 0.5f * sqrtf(cx * cx + cy * cy + cz * cz);
 }
 
+int orc_light_table(const float* verts, const uint32_t* idx, uint32_t ntri, const uint32_t* tri_mat,
+                    const orc_material* mats, const orc_object* objects, uint32_t nobj, const orc_material* obj_mats,
+                    int32_t* light_index, int32_t* ltri, float* larea, float* lprob, float* lcdf)
+{
+    const uint32_t np = ntri + nobj;
+    int nl = 0;
+    float total = 0.0f;
+    for (uint32_t p = 0; p < np; ++p) {
+        const orc_material* m = p < ntri ? &mats[tri_mat ? tri_mat[p] : 0] : &obj_mats[p - ntri];
+        light_index[p] = -1;
+        if (m->emission[0] > 0.0f || m->emission[1] > 0.0f || m->emission[2] > 0.0f) {
+            const float area = p < ntri ? tri_area_f(&verts[3 * idx[3 * p]], &verts[3 * idx[3 * p + 1]],
+                                                     &verts[3 * idx[3 * p + 2]])
+                                        : orc_object_area(&objects[p - ntri]);
+            const float lum = m->emission[0] * 0.2126f + m->emission[1] * 0.7152f + m->emission[2] * 0.0722f;
+            const float w = fmaxf(area * lum, 1.0e-20f);
+            light_index[p] = nl;
+            ltri[nl] = (int32_t)p;
+            larea[nl] = area;
+            lprob[nl] = w;
+            total += w;
+            ++nl;
+        }
+    }
+    float acc = 0.0f;
+    for (int i = 0; i < nl; ++i) {
+        lprob[i] = lprob[i] / total;
+        acc += lprob[i];
+        lcdf[i] = acc;
+    }
+    return nl;
+}
+
 int orc_render(const float* verts, uint32_t nv, const uint32_t* idx, uint32_t ntri, const uint32_t* tri_mat,
                const orc_material* mats, uint32_t nmats, const orc_render_opts* opts, int precision,
                void* out, int threads)
@@ -302,31 +335,8 @@ int orc_render(const float* verts, uint32_t nv, const uint32_t* idx, uint32_t nt
     sc.node_count = orc_bvh_build(pmin, pmax, pcen, (int)np, 8, 12, nodes, indices);
     sc.nodes = nodes;
     sc.indices = indices;
-    int nl = 0;
-    float total = 0.0f;
-    for (uint32_t p = 0; p < np; ++p) {
-        const orc_material* m = p < ntri ? &mats[tri_mat ? tri_mat[p] : 0] : &opts->obj_mats[p - ntri];
-        light_index[p] = -1;
-        if (m->emission[0] > 0.0f || m->emission[1] > 0.0f || m->emission[2] > 0.0f) {
-            const float area = p < ntri ? tri_area_f(&verts[3 * idx[3 * p]], &verts[3 * idx[3 * p + 1]],
-                                                     &verts[3 * idx[3 * p + 2]])
-                                        : orc_object_area(&opts->objects[p - ntri]);
-            const float lum = m->emission[0] * 0.2126f + m->emission[1] * 0.7152f + m->emission[2] * 0.0722f;
-            const float w = fmaxf(area * lum, 1.0e-20f);
-            light_index[p] = nl;
-            ltri[nl] = (int32_t)p;
-            larea[nl] = area;
-            lprob[nl] = w;
-            total += w;
-            ++nl;
-        }
-    }
-    float acc = 0.0f;
-    for (int i = 0; i < nl; ++i) {
-        lprob[i] = lprob[i] / total;
-        acc += lprob[i];
-        lcdf[i] = acc;
-    }
+    const int nl = orc_light_table(verts, idx, ntri, tri_mat, mats, opts->objects, nobj, opts->obj_mats, light_index,
+                                   ltri, larea, lprob, lcdf);
     sc.light_index = light_index;
     float* mat_coef = NULL;
     float* lam_tab = NULL;
@@ -409,4 +419,98 @@ double orc_obj_light_pdf_d(const orc_object* ob, double select, const double x[3
 {
     const vec3_f64 xv = { x[0], x[1], x[2] }, yv = { y[0], y[1], y[2] }, nv = { ny[0], ny[1], ny[2] };
     return obj_light_pdf_f64(ob, select, xv, yv, nv);
+}
+
+/* BSDF pieces and the light pick, FP64 instances (for the tests).  A
+ * material is evaluated in RGB mode, as a triangle material. */
+static mat_f64 test_material(const orc_material* om)
+{
+    orc_render_scene sc;
+    memset(&sc, 0, sizeof sc);
+    sc.mats = om;
+    sc.ntri = 1;
+    return material_f64(&sc, 0);
+}
+
+static vec3_f64 v3d(const double* p) { return v3_f64(p[0], p[1], p[2]); }
+static void put3(vec3_f64 v, double* p) { p[0] = v.x; p[1] = v.y; p[2] = v.z; }
+
+void orc_ggx_d_d(double alpha, size_t count, const double* noh, double* out)
+{
+    for (size_t i = 0; i < count; ++i) out[i] = ggx_d_f64(alpha, noh[i]);
+}
+
+void orc_g_correlated_d(double alpha, size_t count, const double* nol, const double* nov, double* out)
+{
+    for (size_t i = 0; i < count; ++i) out[i] = g_correlated_f64(alpha, nol[i], nov[i]);
+}
+
+void orc_sample_vndf_d(const double v[3], const double n[3], double alpha, size_t count, const double* r,
+                       double* l, double* pdf)
+{
+    for (size_t i = 0; i < count; ++i)
+        put3(sample_vndf_f64(v3d(v), v3d(n), alpha, r[3 * i], r[3 * i + 1], r[3 * i + 2], &pdf[i]), &l[3 * i]);
+}
+
+void orc_vndf_pdf_d(const double v[3], const double n[3], double alpha, size_t count, const double* l, double* out)
+{
+    for (size_t i = 0; i < count; ++i) out[i] = vndf_pdf_f64(v3d(v), v3d(n), v3d(&l[3 * i]), alpha);
+}
+
+void orc_sample_cosine_d(const double n[3], size_t count, const double* e, double* l, double* pdf)
+{
+    for (size_t i = 0; i < count; ++i)
+        put3(sample_cosine_f64(e[3 * i], e[3 * i + 1], e[3 * i + 2], v3d(n), &pdf[i]), &l[3 * i]);
+}
+
+double orc_pspec_d(const orc_material* om)
+{
+    const mat_f64 m = test_material(om);
+    return pspec_of_f64(&m, s4_f64(0, 0, 0, 0), 0);
+}
+
+void orc_eval_bsdf_d(const orc_material* om, const double rd[3], const double n[3], size_t count, const double* l,
+                     double* out)
+{
+    const mat_f64 m = test_material(om);
+    const s4t_f64 alb = s4_f64(m.albedo.x, m.albedo.y, m.albedo.z, 0);
+    for (size_t i = 0; i < count; ++i) {
+        const s4t_f64 f = eval_bsdf_f64(&m, alb, v3d(rd), v3d(&l[3 * i]), v3d(n));
+        out[3 * i] = f.v[0]; out[3 * i + 1] = f.v[1]; out[3 * i + 2] = f.v[2];
+    }
+}
+
+void orc_bsdf_pdf_d(const orc_material* om, const double rd[3], const double n[3], size_t count, const double* l,
+                    double* out)
+{
+    const mat_f64 m = test_material(om);
+    const double ps = orc_pspec_d(om);
+    for (size_t i = 0; i < count; ++i) out[i] = bsdf_pdf_f64(&m, ps, v3d(rd), v3d(&l[3 * i]), v3d(n));
+}
+
+void orc_sample_bsdf_d(const orc_material* om, const double rd[3], const double n[3], size_t count, const double* u,
+                       double* l, double* pdf, int32_t* ok)
+{
+    const mat_f64 m = test_material(om);
+    const double ps = orc_pspec_d(om);
+    for (size_t i = 0; i < count; ++i) {
+        vec3_f64 next = v3_f64(0, 0, 0);
+        pdf[i] = 0;
+        ok[i] = sample_bsdf_f64(&m, ps, v3d(rd), v3d(n), &u[4 * i], &next, &pdf[i]);
+        put3(next, &l[3 * i]);
+    }
+}
+
+void orc_pick_light_d(const float* cdf, int nlights, size_t count, const double* u, int32_t* out)
+{
+    orc_render_lights L;
+    memset(&L, 0, sizeof L);
+    L.count = nlights;
+    L.cdf = cdf;
+    for (size_t i = 0; i < count; ++i) out[i] = pick_light_f64(&L, u[i]);
+}
+
+double orc_rr_survival_d(const double mask[3])
+{
+    return rr_q_f64(s4_f64(mask[0], mask[1], mask[2], 0), 0, 0);
 }

@@ -216,6 +216,16 @@ static R RF(max4)(S4 c, int mode)
     return a > b ? a : b;
 }
 
+/* Russian-roulette survival probability of a path with throughput `mask`
+ * before the current vertex's BSDF weight (render.cuh:838-852):
+ * clamp(0.5 lum + 0.5 max, 0.05, 0.95), raised toward 1 by 0.99 inside a
+ * medium (DELTA_RR_TERMINATE_REDUCE) */
+static R RF(rr_q)(S4 mask, int mode, int in_medium)
+{
+    const R base = RF(clampr)(RF(lum4)(mask, mode) * (R)0.5 + RF(max4)(mask, mode) * (R)0.5, (R)0.05, (R)0.95);
+    return in_medium ? base * ((R)1 - (R)0.99f) + (R)0.99f : base;
+}
+
 /* all active channels zero (allZero in render.cuh:364-407) */
 static int RF(s4zero)(S4 c, int mode)
 {
@@ -999,7 +1009,7 @@ static V3 RF(trace_path)(const orc_render_scene* sc, const orc_render_lights* L,
             medium = -1;
             R qrr = 1;
             if (non_delta >= op->rr_min_depth) {
-                qrr = RF(clampr)(RF(lum4)(mask, mode) * (R)0.5 + RF(max4)(mask, mode) * (R)0.5, (R)0.05, (R)0.95);
+                qrr = RF(rr_q)(mask, mode, 0);
                 if (u[9] > qrr) break;
             }
             if (gout) {
@@ -1155,10 +1165,7 @@ static V3 RF(trace_path)(const orc_render_scene* sc, const orc_render_lights* L,
             const S4 contrib = RF(s4scale)(fcos, (R)1 / RF(maxr)(pdf, (R)1e-12));
             R qrr = 1;
             if (non_delta >= op->rr_min_depth) {
-                const R base = RF(clampr)(RF(lum4)(mask, mode) * (R)0.5 + RF(max4)(mask, mode) * (R)0.5,
-                                          (R)0.05, (R)0.95);
-                const R tl = medium >= 0 ? (R)0.99f : (R)0;
-                qrr = base * ((R)1 - tl) + tl;
+                qrr = RF(rr_q)(mask, mode, medium >= 0);
                 if (u[9] > qrr) break;
             }
             if (gout) {
@@ -1286,8 +1293,7 @@ static V3 RF(trace_path)(const orc_render_scene* sc, const orc_render_lights* L,
         const S4 contrib = RF(s4scale)(fcos, inv_pdf);
         R qrr = 1;
         if (non_delta >= op->rr_min_depth) {
-            qrr = RF(clampr)(RF(lum4)(mask, mode) * (R)0.5 + RF(max4)(mask, mode) * (R)0.5, (R)0.05, (R)0.95);
-            if (medium >= 0) qrr = qrr * ((R)1 - (R)0.99f) + (R)0.99f;
+            qrr = RF(rr_q)(mask, mode, medium >= 0);
             if (u[9] > qrr) break;
         }
         if (gout) {

@@ -679,3 +679,117 @@ def smlp_adamw(p, m, v, g, t, lr, b1=0.9, b2=0.95, eps=1e-8, wd=1e-3, ema=None, 
 def smlp_lr_factor(step, total, warmup_ratio=0.001, min_ratio=0.01):
     return fn("orc_smlp_lr_factor", C.c_double, [C.c_int, C.c_int, C.c_double, C.c_double])(
         int(step), int(total), warmup_ratio, min_ratio)
+
+
+# ---- BSDF pieces, light table and RR (FP64 instances) --------------------
+
+def _d(a, shape=None):
+    a = np.ascontiguousarray(a, np.float64)
+    return a if shape is None else a.reshape(shape)
+
+
+def ggx_d(alpha, noh):
+    noh = _d(noh).ravel()
+    out = np.empty_like(noh)
+    fn("orc_ggx_d_d", None, [C.c_double, C.c_size_t, C.c_void_p, C.c_void_p])(alpha, noh.size, _p(noh), _p(out))
+    return out
+
+
+def g_correlated(alpha, nol, nov):
+    nol, nov = np.broadcast_arrays(_d(nol).ravel(), _d(nov).ravel())
+    nol, nov = _d(nol), _d(nov)
+    out = np.empty_like(nol)
+    fn("orc_g_correlated_d", None, [C.c_double, C.c_size_t, C.c_void_p, C.c_void_p, C.c_void_p])(
+        alpha, nol.size, _p(nol), _p(nov), _p(out))
+    return out
+
+
+def sample_vndf(v, n, alpha, r):
+    """r: (k, 3) uniforms -> directions (k, 3), pdfs (k,)"""
+    r = _d(r, (-1, 3))
+    v3, n3 = _d(v), _d(n)
+    l, pdf = np.empty_like(r), np.empty(r.shape[0])
+    fn("orc_sample_vndf_d", None, [C.c_void_p, C.c_void_p, C.c_double, C.c_size_t, C.c_void_p, C.c_void_p,
+                                   C.c_void_p])(_p(v3), _p(n3), alpha, r.shape[0], _p(r), _p(l), _p(pdf))
+    return l, pdf
+
+
+def vndf_pdf(v, n, alpha, l):
+    l = _d(l, (-1, 3))
+    v3, n3 = _d(v), _d(n)
+    out = np.empty(l.shape[0])
+    fn("orc_vndf_pdf_d", None, [C.c_void_p, C.c_void_p, C.c_double, C.c_size_t, C.c_void_p, C.c_void_p])(
+        _p(v3), _p(n3), alpha, l.shape[0], _p(l), _p(out))
+    return out
+
+
+def sample_cosine(n, e):
+    e = _d(e, (-1, 3))
+    n3 = _d(n)
+    l, pdf = np.empty_like(e), np.empty(e.shape[0])
+    fn("orc_sample_cosine_d", None, [C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p, C.c_void_p])(
+        _p(n3), e.shape[0], _p(e), _p(l), _p(pdf))
+    return l, pdf
+
+
+def pspec(mat):
+    m = make_materials([mat])
+    return fn("orc_pspec_d", C.c_double, [C.c_void_p])(C.cast(m, C.c_void_p))
+
+
+def eval_bsdf(mat, rd, n, l):
+    """f * cos(theta_l) per RGB channel for directions l (k, 3); rd toward the surface"""
+    l = _d(l, (-1, 3))
+    m = make_materials([mat])
+    rd3, n3 = _d(rd), _d(n)
+    out = np.empty_like(l)
+    fn("orc_eval_bsdf_d", None, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p])(
+        C.cast(m, C.c_void_p), _p(rd3), _p(n3), l.shape[0], _p(l), _p(out))
+    return out
+
+
+def bsdf_pdf(mat, rd, n, l):
+    l = _d(l, (-1, 3))
+    m = make_materials([mat])
+    rd3, n3 = _d(rd), _d(n)
+    out = np.empty(l.shape[0])
+    fn("orc_bsdf_pdf_d", None, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p])(
+        C.cast(m, C.c_void_p), _p(rd3), _p(n3), l.shape[0], _p(l), _p(out))
+    return out
+
+
+def sample_bsdf(mat, rd, n, u):
+    """u: (k, 4) uniforms (lobe, direction) -> (ok (k,) bool, l (k, 3), pdf (k,))"""
+    u = _d(u, (-1, 4))
+    m = make_materials([mat])
+    rd3, n3 = _d(rd), _d(n)
+    l, pdf = np.empty((u.shape[0], 3)), np.empty(u.shape[0])
+    ok = np.empty(u.shape[0], np.int32)
+    fn("orc_sample_bsdf_d", None, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p,
+                                   C.c_void_p, C.c_void_p])(
+        C.cast(m, C.c_void_p), _p(rd3), _p(n3), u.shape[0], _p(u), _p(l), _p(pdf), _p(ok))
+    return ok.astype(bool), l, pdf
+
+
+def light_table(V, F, tri_mat, mats):
+    """The integrator's light table: (light prim ids, areas, select probabilities, FP32 CDF)"""
+    V = np.ascontiguousarray(V, np.float32)
+    F = np.ascontiguousarray(F, np.uint32)
+    tm = np.ascontiguousarray(tri_mat, np.uint32)
+    marr = make_materials(mats)
+    n = F.shape[0]
+    li, lt = np.empty(n, np.int32), np.empty(n, np.int32)
+    la, lp, lc = (np.empty(n, np.float32) for _ in range(3))
+    nl = fn("orc_light_table", C.c_int, [C.c_void_p] * 4 + [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p] +
+            [C.c_void_p] * 5)(_p(V), _p(F), n, _p(tm), C.cast(marr, C.c_void_p), None, 0, None,
+                               _p(li), _p(lt), _p(la), _p(lp), _p(lc))
+    return lt[:nl], la[:nl], lp[:nl], lc[:nl]
+
+
+def pick_light(cdf, u):
+    cdf = np.ascontiguousarray(cdf, np.float32)
+    u = _d(u).ravel()
+    out = np.empty(u.size, np.int32)
+    fn("orc_pick_light_d", None, [C.c_void_p, C.c_int, C.c_size_t, C.c_void_p, C.c_void_p])(
+        _p(cdf), cdf.size, u.size, _p(u), _p(out))
+    return out
