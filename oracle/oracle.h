@@ -352,6 +352,11 @@ typedef struct {
                                       reflectance clamp(dot(QueryRGBBasis, QueryLambda), 0, 1) through
                                       tables of smlp_rgb_res^3 and smlp_lambda_res entries */
     int smlp_rgb_res, smlp_lambda_res;
+    /* optional crop (crop_w > 0, unguided renders): the output holds pixels
+     * [crop_x0, crop_x0 + crop_w) x [crop_y0, crop_y0 + crop_h) of the
+     * width x height image, each traced exactly as in the full render (same
+     * camera ray and random numbers) */
+    uint32_t crop_x0, crop_y0, crop_w, crop_h;
 } orc_render_opts;
 
 /* Sigmoid coefficients of an RGB colour (channels clamped to [0, 1]) from the

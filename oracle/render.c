@@ -312,6 +312,9 @@ int orc_render(const float* verts, uint32_t nv, const uint32_t* idx, uint32_t nt
                const orc_material* mats, uint32_t nmats, const orc_render_opts* opts, int precision,
                void* out, int threads)
 {
+    if (opts->crop_w && (opts->guiding || opts->gbuffer || opts->stats || opts->crop_h == 0 ||
+                         opts->crop_x0 + opts->crop_w > opts->width || opts->crop_y0 + opts->crop_h > opts->height))
+        return -1;
     (void)nv;
     orc_render_scene sc;
     memset(&sc, 0, sizeof sc);

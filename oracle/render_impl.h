@@ -1375,6 +1375,7 @@ static V3 RF(sample_pixel)(const orc_render_scene* sc, const orc_render_lights* 
                            long long px, uint32_t smp, const orc_guide* G, float alpha, RF(gsample)* gout,
                            int* ngout, float* fh)
 {
+    if (op->crop_w) px = (long long)(op->crop_y0 + px / op->crop_w) * op->width + op->crop_x0 + px % op->crop_w;
     const uint32_t i = (uint32_t)(px % op->width), j = (uint32_t)(px / op->width);
     float uf[8];
     orc_philox_uniforms(op->seed, (uint32_t)px, smp, ORC_CAMERA_BOUNCE, 1, uf);
@@ -1407,7 +1408,7 @@ static void RF(finish_pixel)(const orc_render_opts* op, V3 acc, R* o)
 static void RF(render)(const orc_render_scene* sc, const orc_render_lights* L, const orc_render_opts* op,
                        R* out, int threads)
 {
-    const long long npix = (long long)op->width * op->height;
+    const long long npix = op->crop_w ? (long long)op->crop_w * op->crop_h : (long long)op->width * op->height;
     if (!op->guiding || !op->guide) {
 #pragma omp parallel for schedule(dynamic, 16) num_threads(threads)
         for (long long px = 0; px < npix; ++px) {

@@ -347,7 +347,8 @@ class orc_render_opts(C.Structure):
                 ("d65", C.c_void_p), ("lut", C.c_void_p), ("guiding", C.c_int), ("guide", C.c_void_p),
                 ("objects", C.c_void_p), ("nobj", C.c_uint32), ("obj_mats", C.c_void_p), ("gbuffer", C.c_void_p),
                 ("stats", C.c_void_p), ("smlp", C.c_void_p), ("smlp_rgb_res", C.c_int),
-                ("smlp_lambda_res", C.c_int)]
+                ("smlp_lambda_res", C.c_int), ("crop_x0", C.c_uint32), ("crop_y0", C.c_uint32),
+                ("crop_w", C.c_uint32), ("crop_h", C.c_uint32)]
 
 
 CIE_TABLE = ROOT / "aperturerendererv3_amd" / "data" / "cie1931_d65_1nm.txt"
@@ -437,12 +438,14 @@ def make_materials(mats):
 
 def render(V, F, tri_mat, mats, width, height, spp, seed=1, max_depth=5, nee=1, sky=0, env=(0, 0, 0),
            cam_pos=(0, 0, 1.1), cam_dir=(0, 0, -1), vfov=60.0, precision=32, rr_min_depth=3, sample_offset=0,
-           threads=0, spectral=0, output=0, guide=None, objects=None, gbuffer=None, stats=None, basis=None):
+           threads=0, spectral=0, output=0, guide=None, objects=None, gbuffer=None, stats=None, basis=None,
+           crop=None):
     """Oracle render; returns (height, width, 3) float32 or float64 mean image.
     objects: analytic objects as dicts (type, center, extent, rotation,
     material as a dict of ar_material fields).  basis: (parameters,
     rgb_res, lambda_res) of the spectral basis network, else the sigmoid
-    uplift."""
+    uplift.  crop: (x0, y0, w, h) renders only that window of the image,
+    each pixel traced as in the full render."""
     lib = load()
     V = np.ascontiguousarray(V, np.float32)
     F = np.ascontiguousarray(F, np.uint32)
@@ -475,7 +478,11 @@ def render(V, F, tri_mat, mats, width, height, spp, seed=1, max_depth=5, nee=1, 
         oarr = make_objects(objects)
         omats = make_materials([ob.get("material") or {} for ob in objects])
         o.objects, o.nobj, o.obj_mats = C.cast(oarr, C.c_void_p), len(objects), C.cast(omats, C.c_void_p)
-    out = np.zeros((height, width, 3), np.float64 if precision == 64 else np.float32)
+    oh, ow = height, width
+    if crop is not None:
+        o.crop_x0, o.crop_y0, o.crop_w, o.crop_h = (int(x) for x in crop)
+        ow, oh = o.crop_w, o.crop_h
+    out = np.zeros((oh, ow, 3), np.float64 if precision == 64 else np.float32)
     lib.orc_render.restype = C.c_int
     lib.orc_render.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p,
                                C.c_uint32, C.POINTER(orc_render_opts), C.c_int, C.c_void_p, C.c_int]

@@ -4,8 +4,9 @@ Bar: bit-exact prim, t, u, v and flags for closest hit, bit-exact occlusion
 words for any hit, on identical rays.  Small cases are checked against the
 oracle's brute force (every triangle), larger ones against the oracle's BVH
 traversal in its canonical mode (itself pinned to brute force in
-test_oracle_trace.py), the full-size traversal workload on a ray subsample
-plus size-independent properties over all rays.
+test_oracle_trace.py), and the full-size traversal workload (16,777,216
+rays and their shadow segments) on every ray, plus size-independent
+properties.
 """
 import numpy as np
 import pytest
@@ -181,9 +182,20 @@ def test_full_size_traversal_workload(ctx):
     h2 = ctx.trace_closest(r)
     torch.cuda.synchronize()
     assert torch.equal(h2["prim"], h["prim"]) and torch.equal(h2["tuv"], h["tuv"])
-    sub = np.random.default_rng(1).choice(rays.shape[1], 1 << 18, replace=False)
-    sub.sort()
-    ref = orc.OracleBVH(V, F).closest(rays[:, sub].copy())
-    got = {"prim": prim[sub], "t": t[sub], "u": h["tuv"][1].cpu().numpy()[sub],
-           "v": h["tuv"][2].cpu().numpy()[sub], "flags": h["flags"].cpu().numpy().view(np.uint32)[sub]}
-    assert_hits_equal(got, ref, "full workload subsample")
+    # every ray against the oracle's BVH traversal (OpenMP over the host cores)
+    obvh = orc.OracleBVH(V, F)
+    print("oracle closest hit over all rays", flush=True)
+    ref = obvh.closest(rays)
+    got = {"prim": prim, "t": t, "u": h["tuv"][1].cpu().numpy(), "v": h["tuv"][2].cpu().numpy(),
+           "flags": h["flags"].cpu().numpy().view(np.uint32)}
+    assert_hits_equal(got, ref, "full workload, all rays")
+    # the benchmark's shadow segments toward (2, 2, 2): any-hit words against the oracle on every segment
+    sh = ctx.make_shadow_rays(r, h, (2.0, 2.0, 2.0))
+    occ_sh = ctx.trace_any(sh)
+    torch.cuda.synchronize()
+    print("oracle any hit over all shadow segments", flush=True)
+    ref_occ = obvh.any(sh.cpu().numpy())
+    got_occ = occ_sh.cpu().numpy().view(np.uint32)
+    bad = np.nonzero(got_occ != ref_occ)[0]
+    assert bad.size == 0, f"shadow occlusion words differ at {bad[:5]}"
+    assert 0 < orc.unpack_bits(got_occ, rays.shape[1]).sum() < hit.size
