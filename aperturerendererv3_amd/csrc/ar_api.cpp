@@ -17,6 +17,29 @@ int set_error(ar_ctx* ctx, int code, const std::string& msg) {
     return code;
 }
 
+int order_stream(ar_ctx* ctx, hipStream_t s) {
+    if (ctx->have_last_stream && ctx->last_stream != s) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        hipStreamCaptureStatus cl = hipStreamCaptureStatusNone;
+        (void)hipStreamIsCapturing(s, &cs);
+        (void)hipStreamIsCapturing(ctx->last_stream, &cl);
+        // inside a capture the caller's graph orders the work
+        if (cs == hipStreamCaptureStatusNone && cl == hipStreamCaptureStatusNone) {
+            if (!ctx->order_event) {
+                const int rc = check_hip(ctx, hipEventCreateWithFlags(&ctx->order_event, hipEventDisableTiming),
+                                         "hipEventCreate(order)");
+                if (rc != AR_OK) return rc;
+            }
+            int rc = check_hip(ctx, hipEventRecord(ctx->order_event, ctx->last_stream), "hipEventRecord(order)");
+            if (rc == AR_OK) rc = check_hip(ctx, hipStreamWaitEvent(s, ctx->order_event, 0), "hipStreamWaitEvent(order)");
+            if (rc != AR_OK) return rc;
+        }
+    }
+    ctx->last_stream = s;
+    ctx->have_last_stream = true;
+    return AR_OK;
+}
+
 int check_hip(ar_ctx* ctx, hipError_t e, const char* what) {
     if (e == hipSuccess) return AR_OK;
     return set_error(ctx, e == hipErrorOutOfMemory ? AR_E_OOM : AR_E_DEVICE,
@@ -26,6 +49,7 @@ int check_hip(ar_ctx* ctx, hipError_t e, const char* what) {
 namespace {
 
 void free_bvh(ar_ctx* ctx) {
+    invalidate_graphs(ctx);
     if (ctx->bvh.nodes) (void)hipFree(ctx->bvh.nodes);
     if (ctx->bvh.tris) (void)hipFree(ctx->bvh.tris);
     ctx->bvh = DeviceBVH{};
@@ -93,6 +117,7 @@ struct Converter {
 
 int ensure_spill(ar_ctx* ctx, uint32_t stack_entries) {
     if (ctx->spill && ctx->spill_stack >= stack_entries) return AR_OK;
+    invalidate_graphs(ctx);
     if (ctx->spill) (void)hipFree(ctx->spill);
     ctx->spill = nullptr;
     const int bpc = trace_blocks_per_cu();
@@ -192,7 +217,11 @@ int ar_create(ar_ctx** out, const ar_device_opts* opts) {
 int ar_destroy(ar_ctx* ctx) {
     if (!ctx) return AR_E_INVALID;
     (void)hipSetDevice(ctx->device);
-    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipDeviceSynchronize();
+    // captured renders of this context can no longer run
+    for (GraphRec* g : ctx->graphs) g->ctx = nullptr;
+    ctx->graphs.clear();
+    if (ctx->order_event) (void)hipEventDestroy(ctx->order_event);
     free_bvh(ctx);
     free_scene(ctx);
     free_render_buffers(ctx);
@@ -432,6 +461,8 @@ int ar_trace_closest(ar_ctx* ctx, const ar_rays* rays, size_t n, const ar_hits* 
     uint32_t blocks;
     int rc = trace_prologue(ctx, ta, n, blocks);
     if (rc != AR_OK) return rc;
+    rc = order_stream(ctx, pick_stream(ctx, stream));
+    if (rc != AR_OK) return rc;
     return check_hip(ctx, launch_trace_closest(ta, *rays, n, *hits, nullptr, nullptr, blocks, pick_stream(ctx, stream)),
                      "trace_closest launch");
 }
@@ -446,6 +477,8 @@ int ar_trace_closest_counted(ar_ctx* ctx, const ar_rays* rays, size_t n, const a
     uint32_t blocks;
     int rc = trace_prologue(ctx, ta, n, blocks);
     if (rc != AR_OK) return rc;
+    rc = order_stream(ctx, pick_stream(ctx, stream));
+    if (rc != AR_OK) return rc;
     return check_hip(ctx, launch_trace_closest(ta, *rays, n, *hits, box_tests, tri_tests, blocks,
                                                pick_stream(ctx, stream)), "trace_closest_counted launch");
 }
@@ -457,6 +490,8 @@ int ar_trace_any(ar_ctx* ctx, const ar_rays* rays, size_t n, uint32_t* occluded,
     TraceArgs ta;
     uint32_t blocks;
     int rc = trace_prologue(ctx, ta, n, blocks);
+    if (rc != AR_OK) return rc;
+    rc = order_stream(ctx, pick_stream(ctx, stream));
     if (rc != AR_OK) return rc;
     return check_hip(ctx, launch_trace_any(ta, *rays, n, occluded, blocks, pick_stream(ctx, stream)),
                      "trace_any launch");
@@ -479,6 +514,8 @@ int ar_render(ar_ctx* ctx, const ar_camera* cam, const ar_render_opts* opts, flo
     if (opts->max_depth == 0 || opts->max_depth > 1000) return set_error(ctx, AR_E_INVALID, "max_depth must be 1..1000");
     if (opts->sky > AR_SKY_CONSTANT) return set_error(ctx, AR_E_INVALID, "unknown sky mode");
     if (!ctx->bvh_ready) return set_error(ctx, AR_E_NOSCENE, "no BVH: call ar_scene_set and ar_bvh_build first");
+    const int rc = order_stream(ctx, pick_stream(ctx, stream));
+    if (rc != AR_OK) return rc;
     return render(ctx, cam, opts, out_rgb, pick_stream(ctx, stream));
 }
 
@@ -491,6 +528,8 @@ int ar_render_progressive(ar_ctx* ctx, const ar_camera* cam, const ar_render_opt
     if (opts->guiding != AR_GUIDING_OFF)
         return set_error(ctx, AR_E_UNSUPPORTED, "progressive frames are unguided; use ar_render for guided frames");
     if (!ctx->bvh_ready) return set_error(ctx, AR_E_NOSCENE, "no BVH: call ar_scene_set and ar_bvh_build first");
+    const int rc = order_stream(ctx, pick_stream(ctx, stream));
+    if (rc != AR_OK) return rc;
     return render_progressive(ctx, cam, opts, frame, accum, odd, even, gbuffer, stats, pick_stream(ctx, stream));
 }
 

@@ -250,7 +250,7 @@ struct GuideDev {
     uint64_t* gkey = nullptr;       // slot probe key [5][pcap]
     uint32_t* gleaf = nullptr;      // slot leaf texels, 6 bits each
     uint32_t* gflag = nullptr;      // bits 0..4: slot received radiance
-    float* gx = nullptr;            // learned guiding: slot network inputs [5*5][pcap]
+    float* gx = nullptr;            // learned guiding: slot network inputs [5*kRec][pcap] (features, scene)
     float* gw = nullptr;            // slot sampled direction [5*3][pcap]
     float* gp = nullptr;            // slot direction pdf [5][pcap]
 };
@@ -261,20 +261,19 @@ struct MlpDev {
     float* m = nullptr;         // Adam moments
     float* v = nullptr;
     float* g = nullptr;         // gradient
-    float* scal = nullptr;      // [0] weight scale, [1] loss
+    float* scal = nullptr;      // [0] weight scale, [1] loss, [2] Adam steps taken
     uint16_t* wb = nullptr;     // BF16 weights [in][out]
     uint16_t* wt = nullptr;     // BF16 weights [out][in]
-    uint64_t t = 0;             // Adam steps
     float lr = 5e-3f;
     uint32_t batch = 1u << 18;  // path vertices per training step
     uint32_t iters = 1;         // training steps per frame
     uint64_t seed = 1;
     // integrator buffers
     size_t cap = 0;
-    float* feat = nullptr;      // [cap][5] network inputs of the queued vertices
+    float* feat = nullptr;      // [cap][kRec] network inputs of the queued vertices (features, scene)
     float* dec = nullptr;       // [cap][48] decoded mixtures
     size_t rcap = 0;
-    float* rx = nullptr;        // training records: inputs [rcap][5]
+    float* rx = nullptr;        // training records: inputs [rcap][kRec]
     float* rd = nullptr;        // directions [rcap][3]
     float* rw = nullptr;        // weights [rcap]
     uint32_t* rcount = nullptr; // records collected this frame
@@ -309,6 +308,8 @@ struct BatchEntry {
     uint32_t light_off;    // first light
     uint32_t light_cnt;
     uint32_t tri_off;      // first triangle (prim id offset)
+    float box_lo[3];       // the scene's box (learned-guiding inputs are normalised to it)
+    float box_inv[3];
 };
 
 struct RenderJob;           // integrator.hip: a planned render and its device plan
@@ -358,7 +359,11 @@ struct ar_ctx {
     std::vector<ar::BatchEntry> batch;   // non-empty for a batch context
     ar::RenderJob* job = nullptr;         // plan of the last render
     uint32_t tile = 256;                  // threads per block of the wavefront kernels
-    uint64_t buffer_generation = 0;       // bumped whenever render buffers are reallocated
+    uint64_t buffer_generation = 0;       // bumped whenever device memory a captured render uses is freed
+    std::vector<ar::GraphRec*> graphs;    // live captured renders of this context (detached by ar_destroy)
+    hipStream_t last_stream = nullptr;    // stream of the last traversal / render call
+    bool have_last_stream = false;
+    hipEvent_t order_event = nullptr;     // orders a call on a new stream after the last one
     double* reduce = nullptr;             // partial sums of ar_luminance_mse
     ar::SdfPool sdf;                      // signed-distance volumes
     float* denoise_feat = nullptr;        // per-pixel filter features of ar_denoise (29 planes)
@@ -368,6 +373,13 @@ struct ar_ctx {
 
 namespace ar {
 int set_error(ar_ctx* ctx, int code, const std::string& msg);
+// Captured renders read the context's device buffers by address: every free
+// of such a buffer outdates them (ar_graph_launch then refuses to run them).
+inline void invalidate_graphs(ar_ctx* ctx) { ctx->buffer_generation += 1; }
+// Work of one context shares scratch (refill counter, stack spill, path
+// state): a call queued on a stream other than the previous call's is made to
+// wait for the previous call's stream first.
+int order_stream(ar_ctx* ctx, hipStream_t s);
 int check_hip(ar_ctx* ctx, hipError_t e, const char* what);
 
 // trace.hip launchers
@@ -436,9 +448,9 @@ int mlp_set_params(ar_ctx* ctx, const float* host);
 void mlp_free(ar_ctx* ctx);
 // smlp.hip
 void smlp_free(ar_ctx* ctx);
-hipError_t mlp_forward(ar_ctx* ctx, const float* x5, size_t n, const uint32_t* n_dev, float* out, bool decoded,
+hipError_t mlp_forward(ar_ctx* ctx, const float* x5, int xs, size_t n, const uint32_t* n_dev, float* out, bool decoded,
                        hipStream_t s);
-hipError_t mlp_gradient(ar_ctx* ctx, const float* x5, const float* dir, const float* w, size_t n,
+hipError_t mlp_gradient(ar_ctx* ctx, const float* x5, int xs, const float* dir, const float* w, size_t n,
                         const uint32_t* n_dev, bool normalize, hipStream_t s);
 hipError_t mlp_adam_step(ar_ctx* ctx, hipStream_t s);
 int mlp_ensure_buffers(ar_ctx* ctx, size_t paths);

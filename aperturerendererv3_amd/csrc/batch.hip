@@ -153,6 +153,7 @@ int batch_merge(ar_ctx* out, ar_ctx* const* scenes, uint32_t n) {
         be.light_off = (uint32_t)ol;
         be.light_cnt = cl;
         be.tri_off = (uint32_t)ot;
+        for (int a = 0; a < 3; ++a) { be.box_lo[a] = c->scene.box_lo[a]; be.box_inv[a] = c->scene.box_inv[a]; }
         entries.push_back(be);
         hm.insert(hm.end(), c->scene.host_mats.begin(), c->scene.host_mats.end());
         for (int a = 0; a < 3; ++a) {
@@ -237,6 +238,8 @@ int ar_render_batch(ar_ctx* ctx, uint32_t n, const ar_camera* cams, const ar_ren
     if (!ctx->bvh_ready) return set_error(ctx, AR_E_NOSCENE, "no BVH");
     for (uint32_t i = 0; i < n; ++i)
         if (opts[i].sky > AR_SKY_CONSTANT) return set_error(ctx, AR_E_INVALID, "unknown sky mode");
+    const int rc = order_stream(ctx, (hipStream_t)stream);
+    if (rc != AR_OK) return rc;
     return render_batch(ctx, n, cams, opts, outs, (hipStream_t)stream);
 }
 
@@ -255,8 +258,6 @@ int ar_graph_capture(ar_ctx* ctx, uint32_t n, const ar_camera* cams, const ar_re
     if (ctx->batch.empty() ? n != 1 : n != ctx->batch.size())
         return set_error(ctx, AR_E_INVALID, "n must equal the number of scenes of the context");
     if (!ctx->bvh_ready) return set_error(ctx, AR_E_NOSCENE, "no BVH");
-    if (opts[0].guiding != AR_GUIDING_OFF)
-        return set_error(ctx, AR_E_UNSUPPORTED, "guided renders update state between frames; render them directly");
     GraphRec* g = new GraphRec();
     g->ctx = ctx;
     const int rc = graph_capture(ctx, n, cams, opts, outs, g);
@@ -266,6 +267,7 @@ int ar_graph_capture(ar_ctx* ctx, uint32_t n, const ar_camera* cams, const ar_re
         delete g;
         return rc;
     }
+    ctx->graphs.push_back(g);
     *out = reinterpret_cast<ar_graph*>(g);
     return AR_OK;
 }
@@ -273,8 +275,13 @@ int ar_graph_capture(ar_ctx* ctx, uint32_t n, const ar_camera* cams, const ar_re
 int ar_graph_launch(ar_graph* graph, void* stream) {
     GraphRec* g = reinterpret_cast<GraphRec*>(graph);
     if (!g) return AR_E_INVALID;
+    if (!g->ctx) return AR_E_INVALID;   // its context was destroyed
     if (g->generation != g->ctx->buffer_generation)
-        return set_error(g->ctx, AR_E_INVALID, "graph outdated: the context's render buffers were reallocated");
+        return set_error(g->ctx, AR_E_INVALID,
+                         "graph outdated: device memory it uses was freed or reallocated (scene, BVH, render, "
+                         "guiding or table buffers)");
+    const int rc = order_stream(g->ctx, (hipStream_t)stream);
+    if (rc != AR_OK) return rc;
     return check_hip(g->ctx, hipGraphLaunch(g->exec, (hipStream_t)stream), "hipGraphLaunch");
 }
 
@@ -282,6 +289,10 @@ int ar_graph_destroy(ar_graph* graph) {
     GraphRec* g = reinterpret_cast<GraphRec*>(graph);
     if (!g) return AR_E_INVALID;
     (void)hipDeviceSynchronize();
+    if (g->ctx) {
+        std::vector<GraphRec*>& v = g->ctx->graphs;
+        v.erase(std::remove(v.begin(), v.end(), g), v.end());
+    }
     if (g->exec) (void)hipGraphExecDestroy(g->exec);
     free_job(g->job);
     delete g;

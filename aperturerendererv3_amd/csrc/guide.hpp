@@ -148,6 +148,20 @@ AR_HD uint64_t voxel_key(float px, float py, float pz, float dx, float dy, float
     return k | ((uint64_t)face20(dx, dy, dz) << 57);
 }
 
+// The key of a batched scene: (voxel, scene, face) packed 16/16/16/8/_/5
+// bits (scene in bits 48..55, face in 57..61), so that scenes never share a
+// probe; kNoKey beyond +-2^15 voxels or past scene 255.
+AR_HD uint64_t voxel_key_scene(float px, float py, float pz, float dx, float dy, float dz, uint32_t scene) {
+    if (scene > 255u) return kNoKey;
+    const float s[3] = {floorf(px * 20.0f), floorf(py * 20.0f), floorf(pz * 20.0f)};
+    uint64_t k = 0;
+    for (int a = 0; a < 3; ++a) {
+        if (!(s[a] >= -32768.0f && s[a] < 32768.0f)) return kNoKey;
+        k |= ((uint64_t)((int64_t)s[a] + 32768) & 0xFFFFull) << (16 * a);
+    }
+    return k | ((uint64_t)scene << 48) | ((uint64_t)face20(dx, dy, dz) << 57);
+}
+
 AR_HD int leaf_of(float dx, float dy, float dz) {
     normalize3(dx, dy, dz);
     float u, v;

@@ -22,6 +22,7 @@
 // Only touched probes are visited, and nothing is sorted.
 #include "ar_internal.hpp"
 #include "guide.hpp"
+#include "mlp.hpp"
 #include "path_common.hpp"
 #include "spectral.hpp"
 
@@ -303,6 +304,7 @@ void free_pool(GuideDev& g) {
 } // namespace
 
 void guide_free(ar_ctx* ctx) {
+    invalidate_graphs(ctx);
     free_slots(ctx->guide);
     free_pool(ctx->guide);
 }
@@ -322,6 +324,7 @@ int guide_ensure(ar_ctx* ctx, size_t paths, bool pool) {
     GuideDev& g = ctx->guide;
     (void)hipSetDevice(ctx->device);
     if (pool && g.cap != ctx->guide_cap_req) {
+        invalidate_graphs(ctx);
         free_pool(g);
         const uint32_t cap = (uint32_t)ctx->guide_cap_req;
         uint32_t tcap = 1;
@@ -350,6 +353,7 @@ int guide_ensure(ar_ctx* ctx, size_t paths, bool pool) {
         if (hipDeviceSynchronize() != hipSuccess) return set_error(ctx, AR_E_DEVICE, "guiding: clear failed");
     }
     if (g.pcap < paths) {
+        invalidate_graphs(ctx);
         free_slots(g);
         size_t b = 0;
         int rc = AR_OK;
@@ -362,7 +366,7 @@ int guide_ensure(ar_ctx* ctx, size_t paths, bool pool) {
         if (rc == AR_OK) rc = dev_alloc(ctx, &g.gkey, 5 * paths, b);
         if (rc == AR_OK) rc = dev_alloc(ctx, &g.gleaf, paths, b);
         if (rc == AR_OK) rc = dev_alloc(ctx, &g.gflag, paths, b);
-        if (rc == AR_OK) rc = dev_alloc(ctx, &g.gx, 25 * paths, b);
+        if (rc == AR_OK) rc = dev_alloc(ctx, &g.gx, 5 * (size_t)mlp::kRec * paths, b);
         if (rc == AR_OK) rc = dev_alloc(ctx, &g.gw, 15 * paths, b);
         if (rc == AR_OK) rc = dev_alloc(ctx, &g.gp, 5 * paths, b);
         if (rc != AR_OK) {

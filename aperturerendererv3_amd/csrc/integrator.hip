@@ -519,6 +519,7 @@ struct SceneView {
     float fwd[3], right[3], up[3];
     float thx, thy;
     float* out;                  // output image, width*height*3 floats
+    float box_lo[3], box_inv[3]; // the scene's box (learned-guiding inputs are normalised to it)
 };
 
 // Paths [base, base + count * npix) of a pass: samples sample0 .. sample0 +
@@ -539,6 +540,7 @@ struct RenderParams {
     uint32_t spectral;           // AR_SPECTRAL_*
     uint32_t output;             // AR_OUTPUT_*
     uint32_t guiding;            // AR_GUIDING_*
+    uint32_t batched;            // a batch context: voxel-guiding keys carry the scene index
     float alpha;                 // guide/BSDF mixture probability of this frame
     float env[3];
     float env_c[3], env_scale;   // sigmoid coefficients of the constant sky (spectral modes)
@@ -645,10 +647,9 @@ struct GuideArgs {
     uint32_t stride;
     // learned guiding
     const float* dec;              // decoded mixture per queue slot [q][48]
-    float* gx;                     // slot network inputs [5*5][stride]
+    float* gx;                     // slot network inputs [5*kRec][stride] (features, scene index)
     float* gw;                     // slot direction [5*3][stride]
     float* gp;                     // slot direction pdf [5][stride]
-    float box_lo[3], box_inv[3];
 };
 
 __device__ __forceinline__ bool s4zero(const S4& c, uint32_t mode) {
@@ -687,13 +688,13 @@ __device__ __forceinline__ void g_endpoint(const GuideArgs& G, uint32_t p, const
 // of this non-delta depth, then scales the open slots by contrib / q.
 __device__ __forceinline__ void g_advance(const GuideArgs& G, uint32_t p, uint32_t non_delta, uint32_t mode,
                                           uint64_t key, int leaf, const S4& contrib, float qrr, bool learned,
-                                          const float* x5, F3 dir, float pdf, bool start) {
+                                          const float* xr, F3 dir, float pdf, bool start) {
     if (start && non_delta < 5u) {
         const int i = (int)non_delta;
         if (s4zero(gload(G.gm, G.stride, i, p), mode)) {
             gstore(G.gm, G.stride, i, p, s4(1.0f, 1.0f, 1.0f, 1.0f));
             if (learned) {
-                for (int k = 0; k < 5; ++k) G.gx[(size_t)(5 * i + k) * G.stride + p] = x5[k];
+                for (int k = 0; k < mlp::kRec; ++k) G.gx[(size_t)(mlp::kRec * i + k) * G.stride + p] = xr[k];
                 G.gw[(size_t)(3 * i + 0) * G.stride + p] = dir.x;
                 G.gw[(size_t)(3 * i + 1) * G.stride + p] = dir.y;
                 G.gw[(size_t)(3 * i + 2) * G.stride + p] = dir.z;
@@ -802,27 +803,32 @@ __global__ __launch_bounds__(kMaxBlock) void k_camera(RenderParams P, PathArgs S
 }
 
 // Network inputs of every queued hit (the learned guide's inference batch):
-// the hit point and the shading normal facing the ray, as k_shade computes
-// them.
+// the hit point (in the box of the path's scene) and the shading normal
+// facing the ray, as k_shade computes them, and the scene index.
 __global__ __launch_bounds__(kMaxBlock) void k_mlp_features(const float4* __restrict__ prims,
                                                          const float4* __restrict__ objs, const float* __restrict__ vol,
-                                                         uint32_t ntri, QueueRays cur,
+                                                         uint32_t ntri, QueueRays cur, const uint32_t* __restrict__ qcur,
+                                                         const uint32_t* __restrict__ scn,
+                                                         const SceneView* __restrict__ views,
                                                          const uint32_t* __restrict__ hprim,
                                                          const float* __restrict__ ht, const float* __restrict__ hu,
                                                          const float* __restrict__ hv,
                                                          const uint32_t* __restrict__ hflags,
-                                                         const uint32_t* __restrict__ counters, float lo0, float lo1,
-                                                         float lo2, float inv0, float inv1, float inv2,
+                                                         const uint32_t* __restrict__ counters,
                                                          float* __restrict__ feat) {
     const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= counters[0]) return;
-    float x[5] = {0, 0, 0, 0, 0};
+    float x[mlp::kRec] = {0, 0, 0, 0, 0, 0};
+    const uint32_t scene = scn[qcur[q]];
+    const SceneView& V = views[scene];
+    const float* lo = V.box_lo;
+    const float* inv = V.box_inv;
+    x[mlp::kUsedIn] = (float)scene;
     const uint32_t prim = hprim[q];
     if (prim != AR_MISS && prim >= ntri) {
         float pp[3], np[3];
         obj::surface(objs + (size_t)obj::kRecord * (prim - ntri), cur.ox[q], cur.oy[q], cur.oz[q], cur.dx[q], cur.dy[q],
                      cur.dz[q], ht[q], (hflags[q] & AR_HIT_INSIDE) != 0u, 0.5f * cur.tmin[q], vol, pp, np);
-        const float lo[3] = {lo0, lo1, lo2}, inv[3] = {inv0, inv1, inv2};
         mlp::features(pp[0], pp[1], pp[2], np[0], np[1], np[2], lo, inv, x);
     } else if (prim != AR_MISS) {
         const F3 a = xyz(prims[3 * (size_t)prim + 0]), b = xyz(prims[3 * (size_t)prim + 1]),
@@ -833,11 +839,10 @@ __global__ __launch_bounds__(kMaxBlock) void k_mlp_features(const float4* __rest
         const F3 ng = normalize(cross(sub(b, a), sub(c, a)));
         const F3 d = f3(cur.dx[q], cur.dy[q], cur.dz[q]);
         const F3 nn = dot(ng, d) < 0.0f ? ng : scale(ng, -1.0f);
-        const float lo[3] = {lo0, lo1, lo2}, inv[3] = {inv0, inv1, inv2};
         mlp::features(pos.x, pos.y, pos.z, nn.x, nn.y, nn.z, lo, inv, x);
     }
 #pragma unroll
-    for (int k = 0; k < 5; ++k) feat[5 * (size_t)q + k] = x[k];
+    for (int k = 0; k < mlp::kRec; ++k) feat[mlp::kRec * (size_t)q + k] = x[k];
 }
 
 // One bounce of CalculateRadiance for every queued path: emission with MIS
@@ -1111,8 +1116,11 @@ __device__ __forceinline__ void shade_one(
                 float pdf;
                 if (sample_bsdf(m, pspec, d, nn, u, next, pdf)) {
                     S4 fcos = eval_bsdf(m, alb, d, next, nn);
-                    float x5[5] = {0, 0, 0, 0, 0};
-                    if (learned) mlp::features(pos.x, pos.y, pos.z, nn.x, nn.y, nn.z, G.box_lo, G.box_inv, x5);
+                    float xr[mlp::kRec] = {0, 0, 0, 0, 0, 0};
+                    if (learned) {
+                        mlp::features(pos.x, pos.y, pos.z, nn.x, nn.y, nn.z, V.box_lo, V.box_inv, xr);
+                        xr[mlp::kUsedIn] = (float)S.scn[p];
+                    }
                     S.mp0[p] = pos.x; S.mp1[p] = pos.y; S.mp2[p] = pos.z;
                     const float off = eps * maxr(1.0f, length(pos)) * (u[4] + 0.5f);
                     const float dn = dot(nn, next);
@@ -1124,7 +1132,8 @@ __device__ __forceinline__ void shade_one(
                     uint64_t gkey = guide::kNoKey;
                     if (GM == AR_GUIDING_VOXEL) {
                         const F3 rv = sub(d, scale(nn, 2.0f * dot(d, nn)));
-                        gkey = guide::voxel_key(pos.x, pos.y, pos.z, rv.x, rv.y, rv.z);
+                        gkey = P.batched ? guide::voxel_key_scene(pos.x, pos.y, pos.z, rv.x, rv.y, rv.z, S.scn[p])
+                                         : guide::voxel_key(pos.x, pos.y, pos.z, rv.x, rv.y, rv.z);
                     }
                     if (guided && learned) {
                         // the network's mixture for this vertex (decoded by the inference pass)
@@ -1274,7 +1283,7 @@ __device__ __forceinline__ void shade_one(
                     if (alive && depth + 1 < P.max_depth) {
                         if (GM != AR_GUIDING_OFF)
                             g_advance(G, p, non_delta, P.spectral, gkey,
-                                      learned ? 0 : guide::leaf_of(next.x, next.y, next.z), contrib, qrr, learned, x5,
+                                      learned ? 0 : guide::leaf_of(next.x, next.y, next.z), contrib, qrr, learned, xr,
                                       next, pdf, medium == 0u);
                         mask = s4scale(s4mul(mask, contrib), 1.0f / qrr);
                         cont = true;
@@ -1493,6 +1502,7 @@ uint32_t grid_for(size_t n, uint32_t block = kBlock) { return (uint32_t)((n + bl
 // ---------------------------------------------------------------- host side
 
 void free_scene(ar_ctx* ctx) {
+    invalidate_graphs(ctx);
     SceneDev& s = ctx->scene;
     for (void* ptr : {(void*)s.prims, (void*)s.mats, (void*)s.light_tri, (void*)s.light_area, (void*)s.light_prob,
                       (void*)s.light_cdf, (void*)s.objs})
@@ -1501,6 +1511,7 @@ void free_scene(ar_ctx* ctx) {
 }
 
 void free_render_buffers(ar_ctx* ctx) {
+    invalidate_graphs(ctx);
     if (ctx->rb.fh) (void)hipFree(ctx->rb.fh);
     ctx->rb.fh = nullptr;
     ctx->rb.fh_cap = 0;
@@ -1865,7 +1876,11 @@ static int make_job(ar_ctx* ctx, uint32_t n, const ar_camera* cams, const ar_ren
         return set_error(ctx, AR_E_UNSUPPORTED, "the gradient sky is RGB-only; use AR_SKY_CONSTANT in spectral modes");
     J.guiding = o.guiding != AR_GUIDING_OFF;
     J.learned = o.guiding == AR_GUIDING_MLP;
-    if (J.guiding && n != 1) return set_error(ctx, AR_E_UNSUPPORTED, "guiding renders one scene at a time");
+    if (J.guiding && n > mlp::kMaxGuidedScenes)
+        return set_error(ctx, AR_E_CAPACITY, "guided batches hold at most 256 scenes");
+    for (uint32_t i = 1; i < n && J.guiding; ++i)
+        if (opts[i].sample_offset != o.sample_offset)
+            return set_error(ctx, AR_E_INVALID, "guided batch scenes must share sample_offset (their frames run together)");
     const size_t max_paths = o.max_paths_per_pass ? o.max_paths_per_pass : ((size_t)1 << 24);
     RenderPlan& L = J.plan;
     L = RenderPlan{};
@@ -1887,9 +1902,12 @@ static int make_job(ar_ctx* ctx, uint32_t n, const ar_camera* cams, const ar_ren
         if (ctx->batch.empty()) {
             v.root = ctx->bvh.root; v.light_off = 0; v.light_cnt = ctx->scene.nlights;
             camera_medium(ctx, cams[i].pos, v);
+            for (int a = 0; a < 3; ++a) { v.box_lo[a] = ctx->scene.box_lo[a]; v.box_inv[a] = ctx->scene.box_inv[a]; }
         } else {
-            v.root = ctx->batch[i].root; v.light_off = ctx->batch[i].light_off; v.light_cnt = ctx->batch[i].light_cnt;
+            const BatchEntry& be = ctx->batch[i];
+            v.root = be.root; v.light_off = be.light_off; v.light_cnt = be.light_cnt;
             v.root0 = v.root;
+            for (int a = 0; a < 3; ++a) { v.box_lo[a] = be.box_lo[a]; v.box_inv[a] = be.box_inv[a]; }
         }
         camera_basis(&cams[i], oi.width, oi.height, v);
         v.out = outs[i];
@@ -1898,11 +1916,12 @@ static int make_job(ar_ctx* ctx, uint32_t n, const ar_camera* cams, const ar_ren
         max_npix = std::max(max_npix, npix);
         if (L.pixels > 0xFFFFFFFFull) return set_error(ctx, AR_E_CAPACITY, "batch larger than 2^32 pixels");
     }
-    // guiding: frames of frame_spp samples per pixel with a guide update after each
+    // guiding: frames of frame_spp samples per pixel of every scene with a
+    // guide update after each; one pass per frame
     const uint32_t frame_spp = std::max(1u, o.frame_spp);
-    if (J.guiding && max_npix * frame_spp > ((size_t)1 << 26))
+    if (J.guiding && L.pixels * frame_spp > ((size_t)1 << 26))
         return set_error(ctx, AR_E_CAPACITY, "guided frame larger than 2^26 paths");
-    const size_t cap = J.guiding ? max_npix * frame_spp : std::max(max_paths, max_npix);
+    const size_t cap = J.guiding ? (size_t)L.pixels * frame_spp : std::max(max_paths, max_npix);
     RenderPlan::Pass cur{0, 0, 0, 0, 0};
     auto close = [&]() {
         if (cur.nseg) {
@@ -1912,13 +1931,33 @@ static int make_job(ar_ctx* ctx, uint32_t n, const ar_camera* cams, const ar_ren
         }
         cur = RenderPlan::Pass{(uint32_t)L.segs.size(), 0, 0, 0, 0};
     };
-    for (uint32_t i = 0; i < n; ++i) {
+    if (J.guiding) {
+        // frame f: samples f * frame_spp .. of every scene that has them
+        uint32_t max_spp = 0;
+        for (const SceneView& v : L.views) max_spp = std::max(max_spp, v.spp);
+        for (uint32_t done = 0; done < max_spp; done += frame_spp) {
+            for (uint32_t i = 0; i < n; ++i) {
+                const SceneView& v = L.views[i];
+                if (done >= v.spp) continue;
+                const uint32_t k = std::min(frame_spp, v.spp - done);
+                Segment sg{};
+                sg.scene = i; sg.sample0 = v.sample_offset + done; sg.count = k;
+                sg.base = cur.paths; sg.pixbase = cur.pixels;
+                if (cur.nseg == 0) cur.sample_base = sg.sample0;
+                L.segs.push_back(sg);
+                cur.nseg += 1;
+                cur.paths += k * v.npix;
+                cur.pixels += v.npix;
+            }
+            close();
+        }
+    }
+    for (uint32_t i = 0; i < n && !J.guiding; ++i) {
         const SceneView& v = L.views[i];
         uint32_t done = 0;
         while (done < v.spp) {
-            size_t fit = (cap - cur.paths) / v.npix;
-            if (J.guiding) fit = std::min<size_t>(fit, frame_spp);
-            if (fit == 0 || (J.guiding && cur.nseg)) { close(); continue; }
+            const size_t fit = (cap - cur.paths) / v.npix;
+            if (fit == 0) { close(); continue; }
             const uint32_t k = (uint32_t)std::min<size_t>(fit, v.spp - done);
             Segment sg{};
             sg.scene = i; sg.sample0 = v.sample_offset + done; sg.count = k;
@@ -1942,6 +1981,7 @@ static int make_job(ar_ctx* ctx, uint32_t n, const ar_camera* cams, const ar_ren
     P.spectral = o.spectral;
     P.output = o.output;
     P.guiding = o.guiding;
+    P.batched = ctx->batch.empty() ? 0u : 1u;
     float e[3] = {o.env[0], o.env[1], o.env[2]};
     for (int k = 0; k < 3; ++k) P.env[k] = o.env[k];
     const float mx = std::max(e[0], std::max(e[1], e[2]));
@@ -2071,7 +2111,6 @@ static int enqueue_job(ar_ctx* ctx, RenderJob& J, hipStream_t stream) {
         G.stride = (uint32_t)gd.pcap;
         G.dec = ctx->mlp.dec;
         G.gx = gd.gx; G.gw = gd.gw; G.gp = gd.gp;
-        for (int k = 0; k < 3; ++k) { G.box_lo[k] = ctx->scene.box_lo[k]; G.box_inv[k] = ctx->scene.box_inv[k]; }
     }
     RenderParams P = J.base;
     P.views = (const SceneView*)J.dev;
@@ -2101,10 +2140,9 @@ static int enqueue_job(ar_ctx* ctx, RenderJob& J, hipStream_t stream) {
             if (J.learned && P.alpha > 0.0f) {
                 const SceneDev& sd = ctx->scene;
                 hipLaunchKernelGGL(k_mlp_features, dim3(grid_for(P.capacity, blk)), dim3(blk), 0, stream, sd.prims,
-                                   sd.objs, ctx->sdf.dev, sd.ntri, q[c], b.hit_prim, b.hit_t, b.hit_u, b.hit_v, b.hit_flags,
-                                   b.counters, sd.box_lo[0], sd.box_lo[1], sd.box_lo[2], sd.box_inv[0], sd.box_inv[1],
-                                   sd.box_inv[2], ctx->mlp.feat);
-                e = mlp_forward(ctx, ctx->mlp.feat, P.capacity, &b.counters[0], ctx->mlp.dec, true, stream);
+                                   sd.objs, ctx->sdf.dev, sd.ntri, q[c], b.qpath[c], b.scn, P.views, b.hit_prim,
+                                   b.hit_t, b.hit_u, b.hit_v, b.hit_flags, b.counters, ctx->mlp.feat);
+                e = mlp_forward(ctx, ctx->mlp.feat, mlp::kRec, P.capacity, &b.counters[0], ctx->mlp.dec, true, stream);
                 if (e != hipSuccess) return check_hip(ctx, e, "guiding network inference");
             }
 #define AR_SHADE(GM)                                                                                           \

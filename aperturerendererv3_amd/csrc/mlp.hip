@@ -155,10 +155,11 @@ struct FwdLds {
     float o[64 * kOut];
 };
 
-__device__ __forceinline__ void load_inputs(const float* __restrict__ x5, size_t n, size_t s0, uint16_t* xl, int lane) {
+__device__ __forceinline__ void load_inputs(const float* __restrict__ x, int xs, size_t n, size_t s0, uint16_t* xl,
+                                            int lane) {
     const size_t s = s0 + lane;
 #pragma unroll
-    for (int k = 0; k < kIn; ++k) xl[lane * kIn + k] = (k < kUsedIn && s < n) ? f2bf(x5[s * kUsedIn + k]) : (uint16_t)0;
+    for (int k = 0; k < kIn; ++k) xl[lane * kIn + k] = s < n ? f2bf(record_input(x + s * xs, xs, k)) : (uint16_t)0;
 }
 
 __device__ __forceinline__ void forward_tile(const Net& N, FwdLds& L, int lane) {
@@ -217,8 +218,8 @@ __device__ __forceinline__ bf16x4 relu_bf16(f32x4 c, const float b[4]) {
 }
 
 template <bool DECODE>
-__global__ __launch_bounds__(64) void k_mlp_forward(Net N, const float* __restrict__ x5, size_t n, float* __restrict__ out,
-                                                    const uint32_t* __restrict__ n_dev) {
+__global__ __launch_bounds__(64) void k_mlp_forward(Net N, const float* __restrict__ x5, int xs, size_t n,
+                                                    float* __restrict__ out, const uint32_t* __restrict__ n_dev) {
     __shared__ float lo[DECODE ? 64 * kOut : 1];
     const int lane = threadIdx.x;
     const int i = lane & 15, g = lane >> 4;
@@ -251,7 +252,7 @@ __global__ __launch_bounds__(64) void k_mlp_forward(Net N, const float* __restri
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int k = 4 * g + r;
-                xb[r] = (short)((k < kUsedIn && s < nn) ? f2bf(x5[s * kUsedIn + k]) : (uint16_t)0);
+                xb[r] = (short)(s < nn ? f2bf(record_input(x5 + s * xs, xs, k)) : (uint16_t)0);
             }
 #pragma unroll
             for (int mb = 0; mb < 4; ++mb) {
@@ -369,7 +370,7 @@ __device__ __forceinline__ float loss_grad(const float* o, const float w[3], flo
 }
 
 // gradient (and loss) of sum_s weight_s * scale * (-log q(dir_s | x_s))
-__global__ __launch_bounds__(64) void k_mlp_grad(Net N, const float* __restrict__ x5, const float* __restrict__ dir,
+__global__ __launch_bounds__(64) void k_mlp_grad(Net N, const float* __restrict__ x5, int xs, const float* __restrict__ dir,
                                                  const float* __restrict__ weight, size_t n,
                                                  const uint32_t* __restrict__ n_dev, const float* __restrict__ scale,
                                                  float* __restrict__ grad, float* __restrict__ loss) {
@@ -378,7 +379,7 @@ __global__ __launch_bounds__(64) void k_mlp_grad(Net N, const float* __restrict_
     const size_t nn = n_dev ? std::min<size_t>(n, *n_dev) : n;
     const float sc = *scale;
     for (size_t s0 = (size_t)blockIdx.x * 64; s0 < nn; s0 += (size_t)gridDim.x * 64) {
-        load_inputs(x5, nn, s0, L.f.x, lane);
+        load_inputs(x5, xs, nn, s0, L.f.x, lane);
         __syncthreads();
         forward_tile(N, L.f, lane);
         const size_t s = s0 + lane;
@@ -421,8 +422,13 @@ __global__ void k_set(float* v, float x) { v[0] = x; }
 struct Adam {
     float* p; float* m; float* v; float* g;
     uint16_t* wb; uint16_t* wt;
-    float lr, b1, b2, eps, c1, c2;   // c1 = 1 - b1^t, c2 = 1 - b2^t
+    const float* t;                  // step count (device; advanced by k_adam_tick)
+    float lr, b1, b2, eps;
 };
+
+// Advances the Adam step count kept in device memory, so that a captured
+// training step replays with the right bias corrections.
+__global__ void k_adam_tick(float* t) { t[0] += 1.0f; }
 
 // maps a weight index to its transposed position ([out][in] per layer)
 __device__ __forceinline__ int transposed(int i) {
@@ -440,12 +446,14 @@ __global__ void k_adam(Adam A, int apply) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= kParams) return;
     if (apply) {
+        const float t = A.t[0];
+        const float c1 = 1.0f - powf(A.b1, t), c2 = 1.0f - powf(A.b2, t);
         const float g = A.g[i];
         const float m = A.b1 * A.m[i] + (1.0f - A.b1) * g;
         const float v = A.b2 * A.v[i] + (1.0f - A.b2) * g * g;
         A.m[i] = m;
         A.v[i] = v;
-        A.p[i] -= A.lr * (m / A.c1) / (sqrtf(v / A.c2) + A.eps);
+        A.p[i] -= A.lr * (m / c1) / (sqrtf(v / c2) + A.eps);
         A.g[i] = 0.0f;
     }
     const int t = transposed(i);
@@ -465,7 +473,7 @@ uint32_t tiles(size_t n) { return (uint32_t)std::max<size_t>(1, std::min<size_t>
 struct Slots {
     const float* gr;       // [5*4][stride] collected radiance
     const uint32_t* gflag;
-    const float* gx;       // [5*5][stride]
+    const float* gx;       // [5*kRec][stride]
     const float* gw;       // [5*3][stride]
     const float* gp;       // [5][stride]
     uint32_t stride;
@@ -507,7 +515,7 @@ __global__ __launch_bounds__(256) void k_mlp_records(Slots S, uint32_t npaths, u
         const bool ok = w > 0.0f && w < 3.0e38f;
         const uint32_t slot = wave_append(ok, count);
         if (ok && slot < cap) {
-            for (int k = 0; k < 5; ++k) rx[5 * (size_t)slot + k] = S.gx[(size_t)(5 * i + k) * S.stride + p];
+            for (int k = 0; k < kRec; ++k) rx[kRec * (size_t)slot + k] = S.gx[(size_t)(kRec * i + k) * S.stride + p];
             for (int k = 0; k < 3; ++k) rd[3 * (size_t)slot + k] = S.gw[(size_t)(3 * i + k) * S.stride + p];
             rw[slot] = w;
         }
@@ -523,14 +531,15 @@ __global__ void k_clamp_count(uint32_t* count, uint32_t cap) {
 int mlp_ensure_buffers(ar_ctx* ctx, size_t paths) {
     MlpDev& M = ctx->mlp;
     if (M.cap >= paths) return AR_OK;
+    invalidate_graphs(ctx);
     for (float** q : {&M.feat, &M.dec, &M.rx, &M.rd, &M.rw})
         if (*q) { (void)hipFree(*q); *q = nullptr; }
     if (M.rcount) { (void)hipFree(M.rcount); M.rcount = nullptr; }
     M.cap = M.rcap = 0;
     const size_t rcap = 5 * paths;
-    hipError_t e = hipMalloc((void**)&M.feat, paths * 5 * sizeof(float));
+    hipError_t e = hipMalloc((void**)&M.feat, paths * kRec * sizeof(float));
     if (e == hipSuccess) e = hipMalloc((void**)&M.dec, paths * kDecoded * sizeof(float));
-    if (e == hipSuccess) e = hipMalloc((void**)&M.rx, rcap * 5 * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc((void**)&M.rx, rcap * kRec * sizeof(float));
     if (e == hipSuccess) e = hipMalloc((void**)&M.rd, rcap * 3 * sizeof(float));
     if (e == hipSuccess) e = hipMalloc((void**)&M.rw, rcap * sizeof(float));
     if (e == hipSuccess) e = hipMalloc((void**)&M.rcount, 4 * sizeof(uint32_t));
@@ -554,13 +563,14 @@ int mlp_frame_update(ar_ctx* ctx, uint32_t spectral, const float* lambda0, const
     const size_t n = std::min<size_t>(M.rcap, M.batch);
     hipError_t e = hipSuccess;
     for (uint32_t it = 0; it < M.iters && e == hipSuccess; ++it) {
-        e = mlp_gradient(ctx, M.rx, M.rd, M.rw, n, M.rcount, true, s);
+        e = mlp_gradient(ctx, M.rx, kRec, M.rd, M.rw, n, M.rcount, true, s);
         if (e == hipSuccess) e = mlp_adam_step(ctx, s);
     }
     return check_hip(ctx, e, "guiding network training");
 }
 
 void mlp_free(ar_ctx* ctx) {
+    invalidate_graphs(ctx);
     MlpDev& M = ctx->mlp;
     for (float** q : {&M.p, &M.feat, &M.dec, &M.rx, &M.rd, &M.rw})
         if (*q) { (void)hipFree(*q); *q = nullptr; }
@@ -580,7 +590,7 @@ int mlp_ensure(ar_ctx* ctx) {
     M.m = M.p + n;
     M.v = M.m + n;
     M.g = M.v + n;
-    M.scal = M.g + n;                // [0] weight scale, [1] loss, [2..3] spare
+    M.scal = M.g + n;                // [0] weight scale, [1] loss, [2] Adam step count, [3] spare
     M.wb = (uint16_t*)(M.scal + 4);
     M.wt = M.wb + n;
     return mlp_init(ctx, M.seed);
@@ -599,37 +609,35 @@ int mlp_init(ar_ctx* ctx, uint64_t seed) {
         for (int i = 0; i < uin[l]; ++i)
             for (int o = 0; o < uout[l]; ++o) h[off[l] + i * dout[l] + o] = nd(rng) * std::sqrt(2.0f / (float)uin[l]);
     (void)din;
-    M.t = 0;
     return mlp_set_params(ctx, h.data());
 }
 
 int mlp_set_params(ar_ctx* ctx, const float* host) {
     MlpDev& M = ctx->mlp;
     hipError_t e = hipMemcpy(M.p, host, kParams * sizeof(float), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemset(M.m, 0, 3 * kParams * sizeof(float));   // m, v, g
+    if (e == hipSuccess) e = hipMemset(M.m, 0, (3 * kParams + 4) * sizeof(float));   // m, v, g, scalars
     if (e != hipSuccess) return check_hip(ctx, e, "mlp parameters");
-    Adam A{M.p, M.m, M.v, M.g, M.wb, M.wt, 0, 0, 0, 0, 1, 1};
+    Adam A{M.p, M.m, M.v, M.g, M.wb, M.wt, M.scal + 2, 0, 0, 0, 0};
     hipLaunchKernelGGL(k_adam, dim3((kParams + 255) / 256), dim3(256), 0, nullptr, A, 0);
-    M.t = 0;
     return check_hip(ctx, hipDeviceSynchronize(), "mlp parameters");
 }
 
-hipError_t mlp_forward(ar_ctx* ctx, const float* x5, size_t n, const uint32_t* n_dev, float* out, bool decoded,
+hipError_t mlp_forward(ar_ctx* ctx, const float* x5, int xs, size_t n, const uint32_t* n_dev, float* out, bool decoded,
                        hipStream_t s) {
     MlpDev& M = ctx->mlp;
     Net N{M.p, M.wb, M.wt};
     // persistent: one wave per SIMD (the kernel keeps ~280 registers), each looping over 64-sample tiles
     const uint32_t grid = (uint32_t)std::min<size_t>((n + 63) / 64, (size_t)std::max(ctx->num_cus, 1) * 4);
     if (decoded)
-        hipLaunchKernelGGL(k_mlp_forward<true>, dim3(std::max(grid, 1u)), dim3(64), 0, s, N, x5, n, out, n_dev);
+        hipLaunchKernelGGL(k_mlp_forward<true>, dim3(std::max(grid, 1u)), dim3(64), 0, s, N, x5, xs, n, out, n_dev);
     else
-        hipLaunchKernelGGL(k_mlp_forward<false>, dim3(std::max(grid, 1u)), dim3(64), 0, s, N, x5, n, out, n_dev);
+        hipLaunchKernelGGL(k_mlp_forward<false>, dim3(std::max(grid, 1u)), dim3(64), 0, s, N, x5, xs, n, out, n_dev);
     return hipGetLastError();
 }
 
 // gradient into M.g (zeroed first unless accumulate) and loss into M.scal[1];
 // normalize: scale = 1 / sum(weights), else 1 / n
-hipError_t mlp_gradient(ar_ctx* ctx, const float* x5, const float* dir, const float* w, size_t n,
+hipError_t mlp_gradient(ar_ctx* ctx, const float* x5, int xs, const float* dir, const float* w, size_t n,
                         const uint32_t* n_dev, bool normalize, hipStream_t s) {
     MlpDev& M = ctx->mlp;
     Net N{M.p, M.wb, M.wt};
@@ -642,17 +650,15 @@ hipError_t mlp_gradient(ar_ctx* ctx, const float* x5, const float* dir, const fl
     } else {
         hipLaunchKernelGGL(k_set, dim3(1), dim3(1), 0, s, M.scal, 1.0f / (float)std::max<size_t>(n, 1));
     }
-    hipLaunchKernelGGL(k_mlp_grad, dim3(tiles(n)), dim3(64), 0, s, N, x5, dir, w, n, n_dev, M.scal, M.g,
+    hipLaunchKernelGGL(k_mlp_grad, dim3(tiles(n)), dim3(64), 0, s, N, x5, xs, dir, w, n, n_dev, M.scal, M.g,
                        M.scal + 1);
     return hipGetLastError();
 }
 
 hipError_t mlp_adam_step(ar_ctx* ctx, hipStream_t s) {
     MlpDev& M = ctx->mlp;
-    M.t += 1;
-    const float b1 = 0.9f, b2 = 0.999f;
-    Adam A{M.p, M.m, M.v, M.g, M.wb, M.wt, M.lr, b1, b2, 1e-8f,
-           1.0f - std::pow(b1, (float)M.t), 1.0f - std::pow(b2, (float)M.t)};
+    Adam A{M.p, M.m, M.v, M.g, M.wb, M.wt, M.scal + 2, M.lr, 0.9f, 0.999f, 1e-8f};
+    hipLaunchKernelGGL(k_adam_tick, dim3(1), dim3(1), 0, s, M.scal + 2);
     hipLaunchKernelGGL(k_adam, dim3((kParams + 255) / 256), dim3(256), 0, s, A, 1);
     return hipGetLastError();
 }
@@ -700,7 +706,7 @@ int ar_mlp_forward(ar_ctx* ctx, const float* x5, size_t n, float* out40, void* s
     int rc = mlp_ensure(ctx);
     if (rc != AR_OK) return rc;
     if (n == 0) return AR_OK;
-    return check_hip(ctx, mlp_forward(ctx, x5, n, nullptr, out40, false, (hipStream_t)stream), "mlp forward");
+    return check_hip(ctx, mlp_forward(ctx, x5, kUsedIn, n, nullptr, out40, false, (hipStream_t)stream), "mlp forward");
 }
 
 int ar_mlp_gradient(ar_ctx* ctx, const float* x5, const float* dir, const float* weight, size_t n, float* grad,
@@ -709,7 +715,7 @@ int ar_mlp_gradient(ar_ctx* ctx, const float* x5, const float* dir, const float*
     int rc = mlp_ensure(ctx);
     if (rc != AR_OK) return rc;
     hipStream_t s = (hipStream_t)stream;
-    hipError_t e = mlp_gradient(ctx, x5, dir, weight, n, nullptr, false, s);
+    hipError_t e = mlp_gradient(ctx, x5, kUsedIn, dir, weight, n, nullptr, false, s);
     if (e == hipSuccess) e = hipMemcpyAsync(grad, ctx->mlp.g, kParams * sizeof(float), hipMemcpyDeviceToDevice, s);
     if (e == hipSuccess && loss) e = hipMemcpyAsync(loss, ctx->mlp.scal + 1, sizeof(float), hipMemcpyDeviceToDevice, s);
     return check_hip(ctx, e, "mlp gradient");
@@ -723,7 +729,7 @@ int ar_mlp_train(ar_ctx* ctx, const float* x5, const float* dir, const float* we
     hipStream_t s = (hipStream_t)stream;
     hipError_t e = hipSuccess;
     for (uint32_t t = 0; t < steps && e == hipSuccess; ++t) {
-        e = mlp_gradient(ctx, x5, dir, weight, n, nullptr, true, s);
+        e = mlp_gradient(ctx, x5, kUsedIn, dir, weight, n, nullptr, true, s);
         if (e == hipSuccess) e = mlp_adam_step(ctx, s);
     }
     if (e == hipSuccess && loss) e = hipMemcpyAsync(loss, ctx->mlp.scal + 1, sizeof(float), hipMemcpyDeviceToDevice, s);

@@ -4,8 +4,9 @@
 // and the shading normal as (theta / pi, phi / (2 pi) + 1/2).
 //
 // Parameters are FP32 masters in the layout W0 16x64, b0, W1 64x64, b1,
-// W2 64x64, b2, W3 64x48, b3 (W[in][out]; the 5 inputs and 40 outputs are
-// zero-padded to 16 and 48 for 16x16x16 MFMA tiles).  The kernels use BF16
+// W2 64x64, b2, W3 64x48, b3 (W[in][out]; the 5 inputs (13 with the scene
+// bits of a batch) and 40 outputs are zero-padded to 16 and 48 for 16x16x16
+// MFMA tiles).  The kernels use BF16
 // copies of the weights (row-major for the backward pass, transposed for the
 // forward pass) with FP32 accumulation.
 #pragma once
@@ -19,6 +20,19 @@ namespace ar {
 namespace mlp {
 
 constexpr int kIn = 16, kHid = 64, kOut = 48, kUsedIn = 5, kUsedOut = 40, kLobes = 8;
+// Batched renders train one network for all scenes: a vertex record holds
+// the 5 features and the scene index, whose 8 bits (0 / 1) feed inputs
+// 5..12 of the zero-padded first layer.  A scene-0 record therefore equals
+// the plain 5-input network.
+constexpr int kSceneBits = 8, kRec = 6;
+constexpr uint32_t kMaxGuidedScenes = 1u << kSceneBits;
+
+// input k of a record of stride xs (5: features only, scene 0; 6: + scene)
+__host__ __device__ __forceinline__ float record_input(const float* __restrict__ r, int xs, int k) {
+    if (k < kUsedIn) return r[k];
+    if (xs == kRec && k < kUsedIn + kSceneBits) return (float)(((uint32_t)r[kUsedIn] >> (k - kUsedIn)) & 1u);
+    return 0.0f;
+}
 constexpr int OW0 = 0;
 constexpr int OB0 = OW0 + kIn * kHid;
 constexpr int OW1 = OB0 + kHid;

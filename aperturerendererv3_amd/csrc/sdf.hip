@@ -95,6 +95,7 @@ uint32_t append(ar_ctx* ctx, const float* voxels, size_t n, const uint32_t* dims
 int sdf_upload(ar_ctx* ctx) {
     SdfPool& p = ctx->sdf;
     if (!p.dirty) return AR_OK;
+    invalidate_graphs(ctx);
     if (p.dev) (void)hipFree(p.dev);
     p.dev = nullptr;
     p.dev_floats = 0;
@@ -110,6 +111,7 @@ int sdf_upload(ar_ctx* ctx) {
 }
 
 void sdf_free(ar_ctx* ctx) {
+    invalidate_graphs(ctx);
     SdfPool& p = ctx->sdf;
     if (p.dev) (void)hipFree(p.dev);
     p.dev = nullptr;

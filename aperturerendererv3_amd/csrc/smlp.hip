@@ -849,6 +849,7 @@ int smlp_material_bases(ar_ctx* ctx, GpuMaterial* mats, uint32_t n) {
 }
 
 void smlp_free(ar_ctx* ctx) {
+    invalidate_graphs(ctx);
     SmlpDev& S = ctx->smlp;
     for (float* q : {S.p, S.part, S.part2, S.misc, S.rgb_lut, S.lam_lut})
         if (q) (void)hipFree(q);
@@ -929,6 +930,7 @@ extern "C" int ar_spectral_mlp_build_tables(ar_ctx* ctx, uint32_t rgb_res, uint3
     const size_t nrgb = (size_t)rgb_res * rgb_res * rgb_res;
     if (S.rgb_res != rgb_res) {
         (void)hipDeviceSynchronize();     // renders on any stream may still read the old table
+        invalidate_graphs(ctx);
         if (S.rgb_lut) (void)hipFree(S.rgb_lut);
         S.rgb_lut = nullptr;
         S.rgb_res = 0;
@@ -938,6 +940,7 @@ extern "C" int ar_spectral_mlp_build_tables(ar_ctx* ctx, uint32_t rgb_res, uint3
     }
     if (S.lam_res != lambda_res) {
         (void)hipDeviceSynchronize();
+        invalidate_graphs(ctx);
         if (S.lam_lut) (void)hipFree(S.lam_lut);
         S.lam_lut = nullptr;
         S.lam_res = 0;

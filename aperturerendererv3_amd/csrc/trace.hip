@@ -790,6 +790,8 @@ extern "C" int ar_trace_closest_noalloc(ar_ctx* ctx, const ar_rays* rays, size_t
     ta.threads = blocks * (uint32_t)kTraceBlock;   // spill stride = launched threads
     ta.fetch = reinterpret_cast<uint32_t*>(base);
     ta.spill = reinterpret_cast<int*>(base + kNoAllocHeader);
+    const int rc = order_stream(ctx, reinterpret_cast<hipStream_t>(stream));
+    if (rc != AR_OK) return rc;
     return check_hip(ctx, launch_trace_closest(ta, *rays, n, *hits, nullptr, nullptr, blocks,
                                                reinterpret_cast<hipStream_t>(stream)),
                      "trace_closest_noalloc launch");

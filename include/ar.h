@@ -12,7 +12,11 @@
  * Threading: one ar_ctx per host thread (thread-compatible, not
  * thread-safe).  Device work is queued on the stream passed in (a
  * hipStream_t; NULL = the HIP default stream) and is not waited for unless
- * the entry point says so.
+ * the entry point says so.  The traversal and render calls of one context
+ * share device scratch (ray-refill counter, stack spill, path state): a call
+ * on a stream other than the previous call's is queued after the previous
+ * call's stream (an event wait), so two streams never race on one context;
+ * use one context per stream for concurrency.
  */
 #ifndef AR_H
 #define AR_H
@@ -342,8 +346,13 @@ int ar_batch_create(ar_ctx* const* scenes, uint32_t n, ar_ctx** out);
 int ar_batch_size(const ar_ctx* ctx);
 /* cams, opts, outs: n entries (n = the batch's scene count, or 1 for a plain
  * context); per scene: width, height, spp, sample_offset; seed, depth, NEE,
- * sky, spectral, output, guiding and max_paths_per_pass must agree.  Guiding
- * needs n == 1. */
+ * sky, spectral, output, guiding, frame_spp and max_paths_per_pass must
+ * agree.  Guided batches (at most 256 scenes, one sample_offset) run frames
+ * of frame_spp samples of every scene with one guide update per frame over
+ * all of them: voxel probes are keyed by scene as well (a batch context's
+ * key packs the voxel as three 16-bit offsets (+2^15), the scene index in
+ * bits 48..55 and the face in bits 57..61), and the learned network takes
+ * the scene index as 8 input bits (rows 5..12 of W0). */
 int ar_render_batch(ar_ctx* ctx, uint32_t n, const ar_camera* cams, const ar_render_opts* opts, float* const* outs,
                     void* stream);
 
@@ -355,8 +364,12 @@ int ar_set_tile(ar_ctx* ctx, uint32_t threads);
  * ar_graph_launch replays it on `stream` (same seeds: same image).  The graph
  * uses the context's render buffers: launches must not overlap other renders
  * of the context, and a render that reallocates them (a larger image)
- * invalidates the graph (ar_graph_launch then returns AR_E_INVALID).
- * Guided renders cannot be captured. */
+ * invalidates the graph (ar_graph_launch then returns AR_E_INVALID), as
+ * does any call that frees memory the graph reads (a new scene or BVH,
+ * guiding or table reallocation) and ar_destroy of its context.  Guided
+ * renders capture their frames with the guide updates and training steps:
+ * each replay renders further frames with the guide's current state (the
+ * network's Adam step count is kept on the device). */
 int ar_graph_capture(ar_ctx* ctx, uint32_t n, const ar_camera* cams, const ar_render_opts* opts, float* const* outs,
                      ar_graph** out);
 int ar_graph_launch(ar_graph* graph, void* stream);
@@ -476,8 +489,9 @@ int ar_guiding_restore(ar_ctx* ctx, const uint64_t* keys, const float* texels, c
  * normal as theta/pi and phi/(2 pi) + 1/2), three hidden layers of 64 ReLU
  * units, 40 outputs = 8 lobes x (logit, log kappa clamped to [-3, 7], mean
  * direction).  Parameters (ar_mlp_params_*): FP32, W0 16x64, b0 64, W1 64x64,
- * b1 64, W2 64x64, b2 64, W3 64x48, b3 48 (W[in][out], rows 5..15 of W0 and
- * columns 40..47 of W3 unused and zero): 12,720 floats.  Loss of a sample
+ * b1 64, W2 64x64, b2 64, W3 64x48, b3 48 (W[in][out]; rows 5..12 of W0
+ * take the scene index bits of a guided batch and stay zero otherwise, rows
+ * 13..15 of W0 and columns 40..47 of W3 are unused and zero): 12,720 floats.  Loss of a sample
  * (x, w, weight): -weight * log q(w | x). */
 typedef struct {
     float learning_rate;        /* Adam step (default 5e-3) */
