@@ -262,6 +262,12 @@ struct MlpDev {
     float* v = nullptr;
     float* g = nullptr;         // gradient
     float* scal = nullptr;      // [0] weight scale, [1] loss, [2] Adam steps taken
+    float* part = nullptr;      // training partials (one allocation): per-block gradients [part_blocks][params],
+    float* part2 = nullptr;     //   chunk sums, per-block and per-chunk losses, weight-sum partials
+    float* ploss = nullptr;
+    float* ploss2 = nullptr;
+    float* wpart = nullptr;
+    uint32_t part_blocks = 0;   // blocks of a training launch at most
     uint16_t* wb = nullptr;     // BF16 weights [in][out]
     uint16_t* wt = nullptr;     // BF16 weights [out][in]
     float lr = 5e-3f;
@@ -450,8 +456,9 @@ void mlp_free(ar_ctx* ctx);
 void smlp_free(ar_ctx* ctx);
 hipError_t mlp_forward(ar_ctx* ctx, const float* x5, int xs, size_t n, const uint32_t* n_dev, float* out, bool decoded,
                        hipStream_t s);
+// adam: an Adam step follows (advances the device-side step count)
 hipError_t mlp_gradient(ar_ctx* ctx, const float* x5, int xs, const float* dir, const float* w, size_t n,
-                        const uint32_t* n_dev, bool normalize, hipStream_t s);
+                        const uint32_t* n_dev, bool normalize, hipStream_t s, bool adam);
 hipError_t mlp_adam_step(ar_ctx* ctx, hipStream_t s);
 int mlp_ensure_buffers(ar_ctx* ctx, size_t paths);
 // training records from the frame's guiding slots, then the frame's Adam steps

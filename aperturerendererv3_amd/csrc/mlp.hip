@@ -1,20 +1,16 @@
 // Learned guiding: BF16 MFMA forward and backward of the 5-64-64-64-40 MLP,
 // Adam, and the guiding-MLP C ABI.
 //
-// One wave64 owns a tile of 64 samples and keeps the tile's activations in
-// LDS.  Every layer is a product of 16x16x16 BF16 tiles
-// (v_mfma_f32_16x16x16_bf16, FP32 accumulate), computed transposed so that
-// the samples are the N dimension:
-//   forward        Z^T[out x s]  = W^T[out x in] . H^T[in x s]
-//   backward data  dH^T[in x s]  = W[in x out]   . dZ^T[out x s]
-//   weight grads   dW[in x out] += H^T[in x s]   . dZ[s x out]   (K = samples)
-// With lane l, i = l % 16, g = l / 16, the operand layouts are
-//   A: a[r] = A[i][4g + r],  B: b[r] = B[4g + r][i],  D: d[r] = D[4g + r][i],
-// so a sample's activations are contiguous in LDS for both the A (weights,
-// from global memory) and B (activations) fetches of the forward and
-// backward-data products.  The loss gradient of the vMF mixture is computed
-// per sample (lane = sample) in FP32; weight gradients are summed with FP32
-// atomics; Adam updates the FP32 masters and rewrites the BF16 copies.
+// Inference (k_mlp_forward): persistent one-wave blocks holding every
+// weight fragment in registers and chaining a 64-sample tile through the
+// layers in registers (v_mfma_f32_16x16x16_bf16, samples as N).
+// Training (k_mlp_train): 4-wave blocks on v_mfma_f32_16x16x32_bf16, the
+// activations of a 32-sample tile per wave in registers, weight gradients
+// accumulated per block from LDS-transposed activations and reduced in a
+// fixed order (no atomics: two runs give the same bits); see the comment
+// above k_mlp_train.  The loss gradient of the vMF mixture is computed per
+// sample (lane = sample) in FP32; Adam updates the FP32 masters and rewrites
+// the BF16 copies.
 #include "ar_internal.hpp"
 #include "mlp.hpp"
 #include "path_common.hpp"
@@ -51,127 +47,11 @@ __device__ __forceinline__ f32x4 mfma(bf16x4 a, bf16x4 b, f32x4 c) {
 }
 
 __device__ __forceinline__ bf16x4 ld4(const uint16_t* p) { return *reinterpret_cast<const bf16x4*>(p); }
-__device__ __forceinline__ bf16x4 ld4s(const uint16_t* p, int stride) {
-    bf16x4 v;
-    v[0] = (short)p[0];
-    v[1] = (short)p[stride];
-    v[2] = (short)p[2 * stride];
-    v[3] = (short)p[3 * stride];
-    return v;
-}
-
-// Z^T = W^T . H^T for 64 samples; H in LDS [64][IN] bf16, WT global [OUT][IN].
-// RELU: store bf16(relu(z + b)) to Hout [64][OUT]; else store z + b as FP32 to Oout [64][OUT].
-template <int IN, int OUT, bool RELU>
-__device__ __forceinline__ void fwd_layer(const uint16_t* __restrict__ WT, const float* __restrict__ bias,
-                                          const uint16_t* Hin, uint16_t* Hout, float* Oout, int lane) {
-    const int i = lane & 15, g = lane >> 4;
-#pragma unroll 1
-    for (int mb = 0; mb < OUT / 16; ++mb) {
-        float bb[4];
-#pragma unroll
-        for (int r = 0; r < 4; ++r) bb[r] = bias[mb * 16 + 4 * g + r];
-#pragma unroll 1
-        for (int nb = 0; nb < 4; ++nb) {
-            f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-            for (int kb = 0; kb < IN / 16; ++kb)
-                c = mfma(ld4(WT + (mb * 16 + i) * IN + kb * 16 + 4 * g), ld4(Hin + (nb * 16 + i) * IN + kb * 16 + 4 * g), c);
-            const int s = nb * 16 + i, u0 = mb * 16 + 4 * g;
-            if (RELU) {
-                uint16_t* o = Hout + s * OUT + u0;
-#pragma unroll
-                for (int r = 0; r < 4; ++r) o[r] = f2bf(fmaxf(c[r] + bb[r], 0.0f));
-            } else {
-                float* o = Oout + s * OUT + u0;
-#pragma unroll
-                for (int r = 0; r < 4; ++r) o[r] = c[r] + bb[r];
-            }
-        }
-    }
-}
-
-// dH^T = W . dZ^T, masked by the layer input's ReLU (Hin > 0); writes dZ of
-// the layer below as bf16 [64][IN].
-template <int IN, int OUT>
-__device__ __forceinline__ void bwd_data(const uint16_t* __restrict__ W, const uint16_t* dZ, const uint16_t* Hin,
-                                         uint16_t* dZin, int lane) {
-    const int i = lane & 15, g = lane >> 4;
-#pragma unroll 1
-    for (int mb = 0; mb < IN / 16; ++mb) {
-#pragma unroll 1
-        for (int nb = 0; nb < 4; ++nb) {
-            f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-            for (int kb = 0; kb < OUT / 16; ++kb)
-                c = mfma(ld4(W + (mb * 16 + i) * OUT + kb * 16 + 4 * g), ld4(dZ + (nb * 16 + i) * OUT + kb * 16 + 4 * g), c);
-            const int s = nb * 16 + i, u0 = mb * 16 + 4 * g;
-#pragma unroll
-            for (int r = 0; r < 4; ++r)
-                dZin[s * IN + u0 + r] = f2bf(bf2f(Hin[s * IN + u0 + r]) > 0.0f ? c[r] : 0.0f);
-        }
-    }
-}
-
-// grad[W][in][out] += sum_s H[s][in] dZ[s][out]; grad[b][out] += sum_s dZ[s][out]
-// (the bias sum reads dZf when given, else the BF16 dZ)
-template <int IN, int OUT>
-__device__ __forceinline__ void bwd_weights(const uint16_t* H, const uint16_t* dZ, const float* dZf, float* gW, float* gB,
-                                            int lane) {
-    const int i = lane & 15, g = lane >> 4;
-#pragma unroll 1
-    for (int mb = 0; mb < IN / 16; ++mb) {
-#pragma unroll 1
-        for (int nb = 0; nb < OUT / 16; ++nb) {
-            f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-            for (int kb = 0; kb < 4; ++kb)
-                c = mfma(ld4s(H + (kb * 16 + 4 * g) * IN + mb * 16 + i, IN),
-                         ld4s(dZ + (kb * 16 + 4 * g) * OUT + nb * 16 + i, OUT), c);
-#pragma unroll
-            for (int r = 0; r < 4; ++r)
-                if (c[r] != 0.0f) atomicAdd(gW + (mb * 16 + 4 * g + r) * OUT + nb * 16 + i, c[r]);
-        }
-    }
-    if (lane < OUT) {
-        float s = 0.0f;
-        for (int k = 0; k < 64; ++k) s += dZf ? dZf[k * OUT + lane] : bf2f(dZ[k * OUT + lane]);
-        if (s != 0.0f) atomicAdd(gB + lane, s);
-    }
-}
-
 struct Net {
     const float* p;        // FP32 masters (biases are read from here)
     const uint16_t* wb;    // BF16 weights, [in][out] per layer (same offsets as p)
     const uint16_t* wt;    // BF16 weights transposed, [out][in] per layer (same offsets)
 };
-
-// LDS of one forward tile
-struct FwdLds {
-    uint16_t x[64 * kIn];
-    uint16_t h1[64 * kHid];
-    uint16_t h2[64 * kHid];
-    uint16_t h3[64 * kHid];
-    float o[64 * kOut];
-};
-
-__device__ __forceinline__ void load_inputs(const float* __restrict__ x, int xs, size_t n, size_t s0, uint16_t* xl,
-                                            int lane) {
-    const size_t s = s0 + lane;
-#pragma unroll
-    for (int k = 0; k < kIn; ++k) xl[lane * kIn + k] = s < n ? f2bf(record_input(x + s * xs, xs, k)) : (uint16_t)0;
-}
-
-__device__ __forceinline__ void forward_tile(const Net& N, FwdLds& L, int lane) {
-    fwd_layer<kIn, kHid, true>(N.wt + OW0, N.p + OB0, L.x, L.h1, nullptr, lane);
-    __syncthreads();
-    fwd_layer<kHid, kHid, true>(N.wt + OW1, N.p + OB1, L.h1, L.h2, nullptr, lane);
-    __syncthreads();
-    fwd_layer<kHid, kHid, true>(N.wt + OW2, N.p + OB2, L.h2, L.h3, nullptr, lane);
-    __syncthreads();
-    fwd_layer<kHid, kOut, false>(N.wt + OW3, N.p + OB3, L.h3, nullptr, L.o, lane);
-    __syncthreads();
-}
 
 struct Lobes {
     float pi[kLobes], kappa[kLobes], mu[kLobes][3], clen[kLobes];
@@ -217,8 +97,8 @@ __device__ __forceinline__ bf16x4 relu_bf16(f32x4 c, const float b[4]) {
     return h;
 }
 
-template <bool DECODE>
-__global__ __launch_bounds__(64) void k_mlp_forward(Net N, const float* __restrict__ x5, int xs, size_t n,
+template <bool DECODE, int XS>
+__global__ __launch_bounds__(64) void k_mlp_forward(Net N, const float* __restrict__ x5, size_t n,
                                                     float* __restrict__ out, const uint32_t* __restrict__ n_dev) {
     __shared__ float lo[DECODE ? 64 * kOut : 1];
     const int lane = threadIdx.x;
@@ -252,7 +132,7 @@ __global__ __launch_bounds__(64) void k_mlp_forward(Net N, const float* __restri
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int k = 4 * g + r;
-                xb[r] = (short)(s < nn ? f2bf(record_input(x5 + s * xs, xs, k)) : (uint16_t)0);
+                xb[r] = (short)(s < nn ? f2bf(record_input(x5 + s * XS, XS, k)) : (uint16_t)0);
             }
 #pragma unroll
             for (int mb = 0; mb < 4; ++mb) {
@@ -332,91 +212,494 @@ __global__ __launch_bounds__(64) void k_mlp_forward(Net N, const float* __restri
     }
 }
 
-struct TrainLds {
-    FwdLds f;
-    uint16_t dob[64 * kOut];
-    uint16_t dz[64 * kHid];
-    uint16_t dz2[64 * kHid];
+
+// ---------------------------------------------------------------- training
+//
+// One training step's gradient, deterministic (no atomics): blocks of eight
+// waves, each wave a tile of 32 samples per round (256 per block round).
+//  * Forward and backward-data products run on v_mfma_f32_16x16x32_bf16 with
+//    the samples as N.  A layer's D fragment (units 4g..4g+3 of sample i)
+//    after bias, ReLU and BF16 rounding is half of the next layer's B
+//    fragment: two D blocks (units 32j + 4g.. and 32j + 16 + 4g..) form the
+//    8 K values of lane group g, and the weight fragments are loaded in that
+//    same K order, so activations never leave registers.
+//  * Weight gradients take the samples as K: each layer's activations and
+//    output gradients are written transposed (unit-major, padded rows) to
+//    LDS, and every wave accumulates its share of the layer's 16x16
+//    gradient blocks (blocks w and w + 8) over the block's 256 samples in
+//    registers, across all rounds.  Bias gradients are the same product with an all-ones A on
+//    the BF16 output gradients dZ.
+//  * Each block writes its partial gradient once; k_mlp_reduce1/2 add the
+//    partials in block order.
+constexpr int kTT = 32;              // samples per wave tile
+constexpr int kTW = 8;               // waves per block (two per SIMD)
+constexpr int kTB = kTW * 64;        // threads per block
+constexpr int kTR = kTT * kTW;       // samples per block round
+constexpr int kTS = kTR + 8;         // transposed-row stride (shorts): 16 B of padding per row
+constexpr int kRS = 52;              // loss-stage row stride (floats)
+constexpr int kDS = 56;              // output-gradient row stride (shorts)
+
+typedef short s16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ f32x4 mfma32(s16x8 a, s16x8 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c,
+                                                   0, 0, 0);
+}
+__device__ __forceinline__ s16x8 cat(bf16x4 lo, bf16x4 hi) {
+    s16x8 r;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) { r[k] = lo[k]; r[4 + k] = hi[k]; }
+    return r;
+}
+// 8 consecutive BF16 values in LDS (16-byte aligned)
+__device__ __forceinline__ s16x8 lds8(const uint16_t* p) { return *reinterpret_cast<const s16x8*>(p); }
+// the K-permuted weight fragment of a 64-wide K block j: units 32j + 4g .. + 3 and 32j + 16 + 4g .. + 3
+__device__ __forceinline__ s16x8 wfrag(const uint16_t* row, int j, int g) {
+    return cat(ld4(row + 32 * j + 4 * g), ld4(row + 32 * j + 16 + 4 * g));
+}
+__device__ __forceinline__ bf16x4 relu_mask(bf16x4 h, f32x4 c) {
+    bf16x4 r;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) r[k] = (short)f2bf(bf2f((uint16_t)h[k]) > 0.0f ? c[k] : 0.0f);
+    return r;
+}
+// Keeps the loads of one layer next to its products: the scheduler would
+// otherwise issue every layer's weight fragments at once (about 130
+// registers) and spill.
+__device__ __forceinline__ void layer_fence() { asm volatile("" ::: "memory"); }
+
+// orders a wave's LDS writes before its later LDS reads of other lanes' data
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// BF16 weights staged in LDS per block, rows padded by 4 values so that the
+// 16 rows of a fragment fall in different banks: forward W^T [out][in],
+// backward W [in][out] (layers 1-3), then the FP32 biases.
+constexpr int kS16 = kIn + 4, kS64 = kHid + 4, kS48 = kOut + 4;
+constexpr int LT0 = 0, LT1 = LT0 + kHid * kS16, LT2 = LT1 + kHid * kS64, LT3 = LT2 + kHid * kS64;
+constexpr int LB3 = LT3 + kOut * kS64, LB2 = LB3 + kHid * kS48, LB1 = LB2 + kHid * kS64;
+constexpr int kLdsW = LB1 + kHid * kS64;   // BF16 values
+constexpr int kBias = 3 * kHid + kOut;     // b0, b1, b2, b3
+
+struct TrainShared {
+    uint16_t wgt[kLdsW];
+    float bias[kBias];
+    union {
+        struct { uint16_t ht[kHid * kTS]; uint16_t zt[kHid * kTS]; } t;   // transposed activations / gradients
+        float r[kTW][kTT * kRS];                                          // loss stage (per wave)
+    } u;
+    uint16_t dor[kTW][kTT * kDS];   // BF16 output gradient, [sample][unit] per wave
+    float red[kTW];
 };
 
-// loss gradient of one sample w.r.t. the 40 outputs (zero padded to 48)
-__device__ __forceinline__ float loss_grad(const float* o, const float w[3], float wt, float* d) {
-    Lobes L;
-    decode(o, L);
-    float f[kLobes], q = 0.0f;
+// transposed store of a layer's D-layout fragments (units 4g + r of sample i)
+__device__ __forceinline__ void store_t(uint16_t* t, const bf16x4 (&v)[4][2], int col0, int i, int g) {
 #pragma unroll
-    for (int k = 0; k < kLobes; ++k) {
-        const float mw = L.mu[k][0] * w[0] + L.mu[k][1] * w[1] + L.mu[k][2] * w[2];
-        f[k] = vmf_norm(L.kappa[k]) * expf(L.kappa[k] * (mw - 1.0f));
-        q += L.pi[k] * f[k];
-    }
-    q = fmaxf(q, 1e-30f);
+    for (int mb = 0; mb < 4; ++mb)
 #pragma unroll
-    for (int k = 0; k < kLobes; ++k) {
-        const float r = L.pi[k] * f[k] / q;
-        const float kap = L.kappa[k];
-        const float mw = L.mu[k][0] * w[0] + L.mu[k][1] * w[1] + L.mu[k][2] * w[2];
-        d[5 * k] = wt * (L.pi[k] - r);
-        const float e2 = expf(-2.0f * kap);
-        const float dlk = 1.0f / kap - 2.0f * e2 / (1.0f - e2) + (mw - 1.0f);
-        d[5 * k + 1] = L.clamped[k] ? 0.0f : -wt * r * dlk * kap;
+        for (int nb = 0; nb < 2; ++nb)
 #pragma unroll
-        for (int c = 0; c < 3; ++c)
-            d[5 * k + 2 + c] = L.clen[k] > 0.0f ? -wt * r * kap * (w[c] - mw * L.mu[k][c]) / L.clen[k] : 0.0f;
-    }
-#pragma unroll
-    for (int k = kUsedOut; k < kOut; ++k) d[k] = 0.0f;
-    return wt == 0.0f ? 0.0f : -wt * logf(q);
+            for (int r = 0; r < 4; ++r) t[(mb * 16 + 4 * g + r) * kTS + col0 + nb * 16 + i] = (uint16_t)v[mb][nb][r];
 }
 
-// gradient (and loss) of sum_s weight_s * scale * (-log q(dir_s | x_s))
-__global__ __launch_bounds__(64) void k_mlp_grad(Net N, const float* __restrict__ x5, int xs, const float* __restrict__ dir,
-                                                 const float* __restrict__ weight, size_t n,
-                                                 const uint32_t* __restrict__ n_dev, const float* __restrict__ scale,
-                                                 float* __restrict__ grad, float* __restrict__ loss) {
-    __shared__ TrainLds L;
-    const int lane = threadIdx.x;
+// Z of the next layer from a 64-unit layer: D = W^T . H over two K blocks
+__device__ __forceinline__ void fwd_hidden(const uint16_t* wt, const float* bias, const bf16x4 (&hin)[4][2],
+                                           bf16x4 (&hout)[4][2], int i, int g) {
+    layer_fence();
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb) {
+        const uint16_t* row = wt + (mb * 16 + i) * kS64;
+        const s16x8 a0 = wfrag(row, 0, g), a1 = wfrag(row, 1, g);
+        float b[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) b[r] = bias[mb * 16 + 4 * g + r];
+#pragma unroll
+        for (int nb = 0; nb < 2; ++nb) {
+            f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
+            c = mfma32(a0, cat(hin[0][nb], hin[1][nb]), c);
+            c = mfma32(a1, cat(hin[2][nb], hin[3][nb]), c);
+            hout[mb][nb] = relu_bf16(c, b);
+        }
+    }
+}
+
+// dZ of the layer below: D = W . dZ^T over two K blocks, masked by the layer input's ReLU
+__device__ __forceinline__ void bwd_hidden(const uint16_t* wb, const bf16x4 (&dz)[4][2], const bf16x4 (&hin)[4][2],
+                                           bf16x4 (&dzin)[4][2], int i, int g) {
+    layer_fence();
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb) {
+        const uint16_t* row = wb + (mb * 16 + i) * kS64;
+        const s16x8 a0 = wfrag(row, 0, g), a1 = wfrag(row, 1, g);
+#pragma unroll
+        for (int nb = 0; nb < 2; ++nb) {
+            f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
+            c = mfma32(a0, cat(dz[0][nb], dz[1][nb]), c);
+            c = mfma32(a1, cat(dz[2][nb], dz[3][nb]), c);
+            dzin[mb][nb] = relu_mask(hin[mb][nb], c);
+        }
+    }
+}
+
+// accumulates gradient block (rows mb, columns nb) of H^T . dZ over the block's 128 samples
+__device__ __forceinline__ f32x4 wgrad(const uint16_t* ht, const uint16_t* zt, int mb, int nb, int i, int g, f32x4 acc) {
+#pragma unroll 2
+    for (int kb = 0; kb < kTR / 32; ++kb)
+        acc = mfma32(lds8(ht + (mb * 16 + i) * kTS + 32 * kb + 8 * g), lds8(zt + (nb * 16 + i) * kTS + 32 * kb + 8 * g),
+                     acc);
+    return acc;
+}
+__device__ __forceinline__ f32x4 bsum(const uint16_t* zt, int nb, int i, int g, f32x4 acc) {
+    s16x8 ones;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) ones[k] = (short)0x3F80;   // BF16 1.0
+#pragma unroll 2
+    for (int kb = 0; kb < kTR / 32; ++kb) acc = mfma32(ones, lds8(zt + (nb * 16 + i) * kTS + 32 * kb + 8 * g), acc);
+    return acc;
+}
+
+// Loss gradient of one sample from its outputs o (40 floats, LDS), shared by
+// two lanes: lane half h = 0 / 1 (lanes s and s + 32) takes lobes 4h .. 4h+3;
+// the maximum logit, the softmax normaliser and the mixture density are
+// combined across the halves with one exchange each.  Hardware exp and
+// reciprocal square root (about 1 ulp); the result is a training signal
+// compared with the FP64 network at the BF16 tolerance.  Writes the BF16
+// gradient of this half's 20 outputs (half 1 also zeroes the padding) to
+// gout and returns the loss (half 0; half 1 returns 0).
+__device__ __forceinline__ float loss_half(const float* o, const float w[3], float wt, uint16_t* gout, int h) {
+    constexpr float kLog2e = 1.4426950408889634f;
+    float lg[4], pu[4], f[4], kap[4], e2[4], mu[4][3], clen[4];
+    bool cl[4];
+    float mx = -3.0e38f;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        lg[j] = o[5 * (4 * h + j)];
+        mx = fmaxf(mx, lg[j]);
+    }
+    mx = fmaxf(mx, __shfl_xor(mx, 32));
+    float z = 0.0f, qz = 0.0f;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const float* ok = o + 5 * (4 * h + j);
+        pu[j] = exp2f((lg[j] - mx) * kLog2e);
+        const float sk = ok[1];
+        cl[j] = sk < -3.0f || sk > 7.0f;
+        kap[j] = exp2f(fminf(7.0f, fmaxf(-3.0f, sk)) * kLog2e);
+        const float cx = ok[2], cy = ok[3], cz = ok[4];
+        const float l2 = cx * cx + cy * cy + cz * cz;
+        const bool deg = l2 < 1e-24f;
+        const float rl = deg ? 0.0f : __builtin_amdgcn_rsqf(l2);
+        clen[j] = deg ? 0.0f : l2 * rl;
+        mu[j][0] = deg ? 0.0f : cx * rl;
+        mu[j][1] = deg ? 0.0f : cy * rl;
+        mu[j][2] = deg ? 1.0f : cz * rl;
+        const float mw = mu[j][0] * w[0] + mu[j][1] * w[1] + mu[j][2] * w[2];
+        e2[j] = exp2f(-2.0f * kap[j] * kLog2e);
+        f[j] = kap[j] * __builtin_amdgcn_rcpf(kTwoPi * (1.0f - e2[j])) * exp2f(kap[j] * (mw - 1.0f) * kLog2e);
+        z += pu[j];
+        qz += pu[j] * f[j];
+    }
+    z += __shfl_xor(z, 32);
+    qz += __shfl_xor(qz, 32);
+    const float rz = __builtin_amdgcn_rcpf(z);
+    const float q = fmaxf(qz * rz, 1e-30f);
+    const float rq = __builtin_amdgcn_rcpf(q);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int k = 4 * h + j;
+        const float pi = pu[j] * rz;
+        const float r = pi * f[j] * rq;
+        const float mw = mu[j][0] * w[0] + mu[j][1] * w[1] + mu[j][2] * w[2];
+        gout[5 * k] = f2bf(wt * (pi - r));
+        const float dlk = __builtin_amdgcn_rcpf(kap[j]) - 2.0f * e2[j] * __builtin_amdgcn_rcpf(1.0f - e2[j]) + (mw - 1.0f);
+        gout[5 * k + 1] = f2bf(cl[j] ? 0.0f : -wt * r * dlk * kap[j]);
+        const float rc = clen[j] > 0.0f ? -wt * r * kap[j] * __builtin_amdgcn_rcpf(clen[j]) : 0.0f;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) gout[5 * k + 2 + c] = f2bf(rc * (w[c] - mw * mu[j][c]));
+    }
+    if (h == 1) {
+#pragma unroll
+        for (int k = kUsedOut; k < kOut; ++k) gout[k] = 0;
+    }
+    return (h == 1 || wt == 0.0f) ? 0.0f : -wt * logf(q);
+}
+
+// gradient of sum_s weight_s * scale * (-log q(dir_s | x_s)): block partials
+// part[block][kParams] and part_loss[block]
+__global__ __launch_bounds__(kTB) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_mlp_train(Net N, const float* __restrict__ x, int xs,
+                                                   const float* __restrict__ dir, const float* __restrict__ weight,
+                                                   size_t n, const uint32_t* __restrict__ n_dev,
+                                                   const float* __restrict__ scale, float* __restrict__ part,
+                                                   float* __restrict__ part_loss) {
+    __shared__ TrainShared S;
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63, i = lane & 15, g = lane >> 4;
     const size_t nn = n_dev ? std::min<size_t>(n, *n_dev) : n;
     const float sc = *scale;
-    for (size_t s0 = (size_t)blockIdx.x * 64; s0 < nn; s0 += (size_t)gridDim.x * 64) {
-        load_inputs(x5, xs, nn, s0, L.f.x, lane);
-        __syncthreads();
-        forward_tile(N, L.f, lane);
-        const size_t s = s0 + lane;
-        float wv[3] = {0.0f, 0.0f, 1.0f}, wt = 0.0f;
-        if (s < nn) {
-            wv[0] = dir[3 * s]; wv[1] = dir[3 * s + 1]; wv[2] = dir[3 * s + 2];
-            wt = weight[s] * sc;
+    const f32x4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
+    // gradient blocks of this wave: q = w and q = w + 8 of each layer's
+    // 16x16 blocks (W3: 12, W2 and W1: 16, W0: 4), biases on waves 4..7
+    f32x4 aw3[2] = {zero, zero}, aw2[2] = {zero, zero}, aw1[2] = {zero, zero};
+    f32x4 aw0 = zero, ab3 = zero, ab2 = zero, ab1 = zero, ab0 = zero;
+    float aloss = 0.0f;
+    uint16_t* ht = S.u.t.ht;
+    uint16_t* zt = S.u.t.zt;
+    const int col0 = w * kTT;
+    // ---- stage the BF16 weights (padded rows) and the biases in LDS
+    {
+        struct Mat { int src, rows, cols, dst, stride; const uint16_t* base; };
+        const Mat mats[7] = {{OW0, kHid, kIn, LT0, kS16, N.wt}, {OW1, kHid, kHid, LT1, kS64, N.wt},
+                             {OW2, kHid, kHid, LT2, kS64, N.wt}, {OW3, kOut, kHid, LT3, kS64, N.wt},
+                             {OW3, kHid, kOut, LB3, kS48, N.wb}, {OW2, kHid, kHid, LB2, kS64, N.wb},
+                             {OW1, kHid, kHid, LB1, kS64, N.wb}};
+#pragma unroll
+        for (int m = 0; m < 7; ++m) {   // unrolled: a loop over the table would move it to LDS
+            const Mat& M = mats[m];
+            const int quads = M.rows * M.cols / 4;
+            // every thread's loads of a matrix are issued before its stores
+            // (at most 2 per thread), so the copy waits on memory once
+            bf16x4 v[2];
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                const int e = tid + kTB * u;
+                if (e < quads) v[u] = *reinterpret_cast<const bf16x4*>(M.base + M.src + 4 * e);
+            }
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                const int e = tid + kTB * u;
+                if (e < quads) {
+                    const int row = (4 * e) / M.cols, col = (4 * e) % M.cols;
+                    *reinterpret_cast<bf16x4*>(&S.wgt[M.dst + row * M.stride + col]) = v[u];
+                }
+            }
         }
-        float* d = L.f.o + lane * kOut;
-        float l = loss_grad(d, wv, wt, d);
-        for (int k = 0; k < kOut; ++k) L.dob[lane * kOut + k] = f2bf(d[k]);
-        for (int off = 32; off > 0; off >>= 1) l += __shfl_xor(l, off);
-        if (lane == 0 && l != 0.0f) atomicAdd(loss, l);
+        if (tid < kHid) {
+            S.bias[tid] = N.p[OB0 + tid];
+            S.bias[kHid + tid] = N.p[OB1 + tid];
+            S.bias[2 * kHid + tid] = N.p[OB2 + tid];
+            if (tid < kOut) S.bias[3 * kHid + tid] = N.p[OB3 + tid];
+        }
         __syncthreads();
-        bwd_weights<kHid, kOut>(L.f.h3, L.dob, L.f.o, grad + OW3, grad + OB3, lane);
-        bwd_data<kHid, kOut>(N.wb + OW3, L.dob, L.f.h3, L.dz, lane);
+    }
+    for (size_t r0 = (size_t)blockIdx.x * kTR; r0 < nn; r0 += (size_t)gridDim.x * kTR) {
+        const size_t t0 = r0 + (size_t)col0;
+        // ---- forward: inputs (K = 16, 16x16x16 tiles), three hidden layers, outputs
+        bf16x4 xb[2];
+#pragma unroll
+        for (int nb = 0; nb < 2; ++nb) {
+            const size_t s = t0 + nb * 16 + i;
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                xb[nb][r] = (short)(s < nn ? f2bf(record_input(x + s * xs, xs, 4 * g + r)) : (uint16_t)0);
+        }
+        bf16x4 h1[4][2], h2[4][2], h3[4][2];
+#pragma unroll
+        for (int mb = 0; mb < 4; ++mb) {
+            const bf16x4 a = ld4(S.wgt + LT0 + (mb * 16 + i) * kS16 + 4 * g);
+            float b[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) b[r] = S.bias[mb * 16 + 4 * g + r];
+#pragma unroll
+            for (int nb = 0; nb < 2; ++nb) h1[mb][nb] = relu_bf16(mfma(a, xb[nb], zero), b);
+        }
+        fwd_hidden(S.wgt + LT1, S.bias + kHid, h1, h2, i, g);
+        fwd_hidden(S.wgt + LT2, S.bias + 2 * kHid, h2, h3, i, g);
+        float* R = S.u.r[w];
+        layer_fence();
+#pragma unroll
+        for (int mb = 0; mb < 3; ++mb) {
+            const uint16_t* row = S.wgt + LT3 + (mb * 16 + i) * kS64;
+            const s16x8 a0 = wfrag(row, 0, g), a1 = wfrag(row, 1, g);
+            float b[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) b[r] = S.bias[3 * kHid + mb * 16 + 4 * g + r];
+#pragma unroll
+            for (int nb = 0; nb < 2; ++nb) {
+                f32x4 c = mfma32(a0, cat(h3[0][nb], h3[1][nb]), zero);
+                c = mfma32(a1, cat(h3[2][nb], h3[3][nb]), c);
+                *reinterpret_cast<float4*>(&R[(nb * 16 + i) * kRS + mb * 16 + 4 * g]) =
+                    make_float4(c[0] + b[0], c[1] + b[1], c[2] + b[2], c[3] + b[3]);
+            }
+        }
+        // ---- loss gradient, two lanes per sample (lanes s and s + 32)
+        wave_sync();
+        {
+            const int sl = lane & (kTT - 1);
+            const size_t s = t0 + sl;
+            float wv[3] = {0.0f, 0.0f, 1.0f}, wt = 0.0f;
+            if (s < nn) {
+                wv[0] = dir[3 * s]; wv[1] = dir[3 * s + 1]; wv[2] = dir[3 * s + 2];
+                wt = weight[s] * sc;
+            }
+            aloss += loss_half(&R[sl * kRS], wv, wt, &S.dor[w][sl * kDS], lane >> 5);
+        }
+        __syncthreads();                           // the loss stage's LDS (aliasing ht / zt) is free
+        // ---- output layer: dW3 = H3^T . dO over the block's samples; dZ2 = W3 . dO^T masked by H3
+        store_t(ht, h3, col0, i, g);
+        if (lane < kTT) {
+#pragma unroll 4
+            for (int k = 0; k < kOut; ++k) zt[k * kTS + col0 + lane] = S.dor[w][lane * kDS + k];
+        }
         __syncthreads();
-        bwd_weights<kHid, kHid>(L.f.h2, L.dz, nullptr, grad + OW2, grad + OB2, lane);
-        bwd_data<kHid, kHid>(N.wb + OW2, L.dz, L.f.h2, L.dz2, lane);
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            const int q = w + kTW * t;
+            if (q < 12) aw3[t] = wgrad(ht, zt, q / 3, q % 3, i, g, aw3[t]);
+        }
+        if (w >= 4 && w < 7) ab3 = bsum(zt, w - 4, i, g, ab3);
+        bf16x4 dz[4][2];
+        layer_fence();
+#pragma unroll
+        for (int mb = 0; mb < 4; ++mb) {
+            // K = 48 outputs: units 0..31, then units 32..47 as a 16x16x32
+            // product whose upper K half is zero (a 16x16x16 MFMA accumulating
+            // onto this chain's 16x16x32 result loses rows 4g and 4g + 1)
+            const uint16_t* row = S.wgt + LB3 + (mb * 16 + i) * kS48;
+            const bf16x4 z4 = {0, 0, 0, 0};
+            const s16x8 a0 = cat(ld4(row + 8 * g), ld4(row + 8 * g + 4));
+            const s16x8 a1 = cat(ld4(row + 32 + 4 * g), z4);
+#pragma unroll
+            for (int nb = 0; nb < 2; ++nb) {
+                const uint16_t* dr = &S.dor[w][(nb * 16 + i) * kDS];
+                f32x4 c = mfma32(a0, lds8(dr + 8 * g), zero);
+                c = mfma32(a1, cat(*reinterpret_cast<const bf16x4*>(dr + 32 + 4 * g), z4), c);
+                dz[mb][nb] = relu_mask(h3[mb][nb], c);
+            }
+        }
         __syncthreads();
-        bwd_weights<kHid, kHid>(L.f.h1, L.dz2, nullptr, grad + OW1, grad + OB1, lane);
-        bwd_data<kHid, kHid>(N.wb + OW1, L.dz2, L.f.h1, L.dz, lane);
+        // ---- hidden layer 2: dW2, b2; dZ1
+        store_t(ht, h2, col0, i, g);
+        store_t(zt, dz, col0, i, g);
         __syncthreads();
-        bwd_weights<kIn, kHid>(L.f.x, L.dz, nullptr, grad + OW0, grad + OB0, lane);
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            const int q = w + kTW * t;
+            aw2[t] = wgrad(ht, zt, q / 4, q % 4, i, g, aw2[t]);
+        }
+        if (w >= 4) ab2 = bsum(zt, w - 4, i, g, ab2);
+        bf16x4 dz1[4][2];
+        bwd_hidden(S.wgt + LB2, dz, h2, dz1, i, g);
         __syncthreads();
+        // ---- hidden layer 1: dW1, b1; dZ0
+        store_t(ht, h1, col0, i, g);
+        store_t(zt, dz1, col0, i, g);
+        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            const int q = w + kTW * t;
+            aw1[t] = wgrad(ht, zt, q / 4, q % 4, i, g, aw1[t]);
+        }
+        if (w >= 4) ab1 = bsum(zt, w - 4, i, g, ab1);
+        bwd_hidden(S.wgt + LB1, dz1, h1, dz, i, g);
+        __syncthreads();
+        // ---- input layer: dW0 = X^T . dZ0 (16 input rows), b0
+#pragma unroll
+        for (int nb = 0; nb < 2; ++nb)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) ht[(4 * g + r) * kTS + col0 + nb * 16 + i] = (uint16_t)xb[nb][r];
+        store_t(zt, dz, col0, i, g);
+        __syncthreads();
+        if (w < 4) aw0 = wgrad(ht, zt, 0, w, i, g, aw0);
+        else ab0 = bsum(zt, w - 4, i, g, ab0);
+        __syncthreads();
+    }
+    // ---- this block's partial gradient and loss
+    float* P = part + (size_t)blockIdx.x * kParams;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int row = 4 * g + r;
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            const int q = w + kTW * t;
+            if (q < 12) P[OW3 + ((q / 3) * 16 + row) * kOut + (q % 3) * 16 + i] = aw3[t][r];
+            P[OW2 + ((q / 4) * 16 + row) * kHid + (q % 4) * 16 + i] = aw2[t][r];
+            P[OW1 + ((q / 4) * 16 + row) * kHid + (q % 4) * 16 + i] = aw1[t][r];
+        }
+        if (w < 4) P[OW0 + row * kHid + w * 16 + i] = aw0[r];
+    }
+    if (g == 0 && w >= 4) {
+        const int nb = w - 4;
+        if (nb < 3) P[OB3 + nb * 16 + i] = ab3[0];
+        P[OB2 + nb * 16 + i] = ab2[0];
+        P[OB1 + nb * 16 + i] = ab1[0];
+        P[OB0 + nb * 16 + i] = ab0[0];
+    }
+    for (int off = 32; off > 0; off >>= 1) aloss += __shfl_xor(aloss, off);
+    if (lane == 0) S.red[w] = aloss;
+    __syncthreads();
+    if (tid == 0) {
+        float l = 0.0f;
+        for (int k = 0; k < kTW; ++k) l += S.red[k];
+        part_loss[blockIdx.x] = l;
     }
 }
 
-// scale = 1 / max(sum of weights, tiny)
-__global__ void k_weight_sum(const float* __restrict__ w, size_t n, const uint32_t* __restrict__ n_dev, float* sum) {
+// Fixed-order sums of the block partials: chunks of kRedChunk blocks, then the chunk sums.
+constexpr int kRedChunk = 16;
+__global__ void k_mlp_reduce1(const float* __restrict__ part, const float* __restrict__ part_loss, int blocks,
+                              float* __restrict__ part2, float* __restrict__ loss2) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.y;
+    const int b0 = c * kRedChunk, b1 = std::min(blocks, b0 + kRedChunk);
+    if (p < kParams) {
+        float s = 0.0f;
+        for (int b = b0; b < b1; ++b) s += part[(size_t)b * kParams + p];
+        part2[(size_t)c * kParams + p] = s;
+    }
+    if (p == kParams) {
+        float s = 0.0f;
+        for (int b = b0; b < b1; ++b) s += part_loss[b];
+        loss2[c] = s;
+    }
+}
+// tick (optional): the Adam step count, advanced here when an Adam step follows
+__global__ void k_mlp_reduce2(const float* __restrict__ part2, const float* __restrict__ loss2, int chunks,
+                              float* __restrict__ g, float* __restrict__ loss, float* __restrict__ tick) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tick && p == kParams + 1) tick[0] += 1.0f;
+    if (p < kParams) {
+        float s = 0.0f;
+        for (int c = 0; c < chunks; ++c) s += part2[(size_t)c * kParams + p];
+        g[p] = s;
+    }
+    if (p == kParams) {
+        float s = 0.0f;
+        for (int c = 0; c < chunks; ++c) s += loss2[c];
+        *loss = s;
+    }
+}
+
+// Per-block sums of the sample weights in a fixed order (strided thread
+// sums, then an LDS tree); k_inverse adds the block sums in block order and
+// writes scale = 1 / max(sum, tiny).
+constexpr int kWsBlocks = 256;
+__global__ __launch_bounds__(256) void k_weight_sum(const float* __restrict__ w, size_t n,
+                                                    const uint32_t* __restrict__ n_dev, float* __restrict__ partial) {
+    __shared__ float t[256];
     const size_t nn = n_dev ? std::min<size_t>(n, *n_dev) : n;
     float s = 0.0f;
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < nn; i += (size_t)gridDim.x * blockDim.x) s += w[i];
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
-    if ((threadIdx.x & 63) == 0 && s != 0.0f) atomicAdd(sum, s);
+    t[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) t[threadIdx.x] += t[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = t[0];
 }
-__global__ void k_inverse(float* v) { v[0] = 1.0f / fmaxf(v[0], 1e-30f); }
+__global__ __launch_bounds__(256) void k_inverse(const float* __restrict__ partial, int blocks, float* v) {
+    __shared__ float t[256];
+    t[threadIdx.x] = (int)threadIdx.x < blocks ? partial[threadIdx.x] : 0.0f;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) t[threadIdx.x] += t[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) v[0] = 1.0f / fmaxf(t[0], 1e-30f);
+}
 __global__ void k_set(float* v, float x) { v[0] = x; }
 
 struct Adam {
@@ -426,9 +709,7 @@ struct Adam {
     float lr, b1, b2, eps;
 };
 
-// Advances the Adam step count kept in device memory, so that a captured
-// training step replays with the right bias corrections.
-__global__ void k_adam_tick(float* t) { t[0] += 1.0f; }
+
 
 // maps a weight index to its transposed position ([out][in] per layer)
 __device__ __forceinline__ int transposed(int i) {
@@ -468,7 +749,10 @@ __global__ void k_zero(float* p, size_t n) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0.0f;
 }
 
-uint32_t tiles(size_t n) { return (uint32_t)std::max<size_t>(1, std::min<size_t>((n + 63) / 64, 8192)); }
+// blocks of a training launch over n samples (at most the resident blocks the partials are sized for)
+uint32_t train_grid(const MlpDev& M, size_t n) {
+    return (uint32_t)std::max<size_t>(1, std::min<size_t>((n + kTR - 1) / kTR, M.part_blocks));
+}
 
 struct Slots {
     const float* gr;       // [5*4][stride] collected radiance
@@ -563,7 +847,7 @@ int mlp_frame_update(ar_ctx* ctx, uint32_t spectral, const float* lambda0, const
     const size_t n = std::min<size_t>(M.rcap, M.batch);
     hipError_t e = hipSuccess;
     for (uint32_t it = 0; it < M.iters && e == hipSuccess; ++it) {
-        e = mlp_gradient(ctx, M.rx, kRec, M.rd, M.rw, n, M.rcount, true, s);
+        e = mlp_gradient(ctx, M.rx, kRec, M.rd, M.rw, n, M.rcount, true, s, true);
         if (e == hipSuccess) e = mlp_adam_step(ctx, s);
     }
     return check_hip(ctx, e, "guiding network training");
@@ -572,10 +856,12 @@ int mlp_frame_update(ar_ctx* ctx, uint32_t spectral, const float* lambda0, const
 void mlp_free(ar_ctx* ctx) {
     invalidate_graphs(ctx);
     MlpDev& M = ctx->mlp;
-    for (float** q : {&M.p, &M.feat, &M.dec, &M.rx, &M.rd, &M.rw})
+    for (float** q : {&M.p, &M.feat, &M.dec, &M.rx, &M.rd, &M.rw, &M.part})
         if (*q) { (void)hipFree(*q); *q = nullptr; }
     if (M.rcount) { (void)hipFree(M.rcount); M.rcount = nullptr; }
     M.m = M.v = M.g = M.scal = nullptr;
+    M.part2 = M.ploss = M.ploss2 = M.wpart = nullptr;
+    M.part_blocks = 0;
     M.wb = M.wt = nullptr;
     M.cap = M.rcap = 0;
 }
@@ -593,6 +879,18 @@ int mlp_ensure(ar_ctx* ctx) {
     M.scal = M.g + n;                // [0] weight scale, [1] loss, [2] Adam step count, [3] spare
     M.wb = (uint16_t*)(M.scal + 4);
     M.wt = M.wb + n;
+    // training partials: one gradient per resident block of k_mlp_train, chunk sums, losses, weight sums
+    int bpc = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_mlp_train, kTB, 0) != hipSuccess || bpc < 1) bpc = 1;
+    M.part_blocks = (uint32_t)std::max(1, ctx->num_cus) * (uint32_t)std::min(bpc, 4);
+    const size_t chunks = (M.part_blocks + kRedChunk - 1) / kRedChunk;
+    const size_t pf = ((size_t)M.part_blocks + chunks) * kParams + M.part_blocks + chunks + kWsBlocks;
+    e = hipMalloc((void**)&M.part, pf * sizeof(float));
+    if (e != hipSuccess) { mlp_free(ctx); return set_error(ctx, AR_E_OOM, "mlp: device allocation failed"); }
+    M.part2 = M.part + (size_t)M.part_blocks * kParams;
+    M.ploss = M.part2 + chunks * kParams;
+    M.ploss2 = M.ploss + M.part_blocks;
+    M.wpart = M.ploss2 + chunks;
     return mlp_init(ctx, M.seed);
 }
 
@@ -628,37 +926,40 @@ hipError_t mlp_forward(ar_ctx* ctx, const float* x5, int xs, size_t n, const uin
     Net N{M.p, M.wb, M.wt};
     // persistent: one wave per SIMD (the kernel keeps ~280 registers), each looping over 64-sample tiles
     const uint32_t grid = (uint32_t)std::min<size_t>((n + 63) / 64, (size_t)std::max(ctx->num_cus, 1) * 4);
-    if (decoded)
-        hipLaunchKernelGGL(k_mlp_forward<true>, dim3(std::max(grid, 1u)), dim3(64), 0, s, N, x5, xs, n, out, n_dev);
-    else
-        hipLaunchKernelGGL(k_mlp_forward<false>, dim3(std::max(grid, 1u)), dim3(64), 0, s, N, x5, xs, n, out, n_dev);
+    const dim3 g(std::max(grid, 1u));
+    if (decoded && xs == kRec) hipLaunchKernelGGL((k_mlp_forward<true, kRec>), g, dim3(64), 0, s, N, x5, n, out, n_dev);
+    else if (decoded) hipLaunchKernelGGL((k_mlp_forward<true, kUsedIn>), g, dim3(64), 0, s, N, x5, n, out, n_dev);
+    else if (xs == kRec) hipLaunchKernelGGL((k_mlp_forward<false, kRec>), g, dim3(64), 0, s, N, x5, n, out, n_dev);
+    else hipLaunchKernelGGL((k_mlp_forward<false, kUsedIn>), g, dim3(64), 0, s, N, x5, n, out, n_dev);
     return hipGetLastError();
 }
 
 // gradient into M.g (zeroed first unless accumulate) and loss into M.scal[1];
 // normalize: scale = 1 / sum(weights), else 1 / n
 hipError_t mlp_gradient(ar_ctx* ctx, const float* x5, int xs, const float* dir, const float* w, size_t n,
-                        const uint32_t* n_dev, bool normalize, hipStream_t s) {
+                        const uint32_t* n_dev, bool normalize, hipStream_t s, bool adam) {
     MlpDev& M = ctx->mlp;
     Net N{M.p, M.wb, M.wt};
-    hipLaunchKernelGGL(k_zero, dim3(64), dim3(256), 0, s, M.g, (size_t)kParams);
-    hipLaunchKernelGGL(k_set, dim3(1), dim3(1), 0, s, M.scal + 1, 0.0f);
     if (normalize) {
-        hipLaunchKernelGGL(k_set, dim3(1), dim3(1), 0, s, M.scal, 0.0f);
-        hipLaunchKernelGGL(k_weight_sum, dim3(256), dim3(256), 0, s, w, n, n_dev, M.scal);
-        hipLaunchKernelGGL(k_inverse, dim3(1), dim3(1), 0, s, M.scal);
+        hipLaunchKernelGGL(k_weight_sum, dim3(kWsBlocks), dim3(256), 0, s, w, n, n_dev, M.wpart);
+        hipLaunchKernelGGL(k_inverse, dim3(1), dim3(256), 0, s, M.wpart, kWsBlocks, M.scal);
     } else {
         hipLaunchKernelGGL(k_set, dim3(1), dim3(1), 0, s, M.scal, 1.0f / (float)std::max<size_t>(n, 1));
     }
-    hipLaunchKernelGGL(k_mlp_grad, dim3(tiles(n)), dim3(64), 0, s, N, x5, xs, dir, w, n, n_dev, M.scal, M.g,
-                       M.scal + 1);
+    const uint32_t grid = train_grid(M, n);
+    hipLaunchKernelGGL(k_mlp_train, dim3(grid), dim3(kTB), 0, s, N, x5, xs, dir, w, n, n_dev, M.scal, M.part, M.ploss);
+    const uint32_t chunks = (grid + kRedChunk - 1) / kRedChunk;
+    const uint32_t pblocks = (uint32_t)((kParams + 2 + 255) / 256);
+    hipLaunchKernelGGL(k_mlp_reduce1, dim3(pblocks, chunks), dim3(256), 0, s, M.part, M.ploss, (int)grid, M.part2,
+                       M.ploss2);
+    hipLaunchKernelGGL(k_mlp_reduce2, dim3(pblocks), dim3(256), 0, s, M.part2, M.ploss2, (int)chunks, M.g, M.scal + 1,
+                       adam ? M.scal + 2 : nullptr);
     return hipGetLastError();
 }
 
 hipError_t mlp_adam_step(ar_ctx* ctx, hipStream_t s) {
     MlpDev& M = ctx->mlp;
     Adam A{M.p, M.m, M.v, M.g, M.wb, M.wt, M.scal + 2, M.lr, 0.9f, 0.999f, 1e-8f};
-    hipLaunchKernelGGL(k_adam_tick, dim3(1), dim3(1), 0, s, M.scal + 2);
     hipLaunchKernelGGL(k_adam, dim3((kParams + 255) / 256), dim3(256), 0, s, A, 1);
     return hipGetLastError();
 }
@@ -715,7 +1016,7 @@ int ar_mlp_gradient(ar_ctx* ctx, const float* x5, const float* dir, const float*
     int rc = mlp_ensure(ctx);
     if (rc != AR_OK) return rc;
     hipStream_t s = (hipStream_t)stream;
-    hipError_t e = mlp_gradient(ctx, x5, kUsedIn, dir, weight, n, nullptr, false, s);
+    hipError_t e = mlp_gradient(ctx, x5, kUsedIn, dir, weight, n, nullptr, false, s, false);
     if (e == hipSuccess) e = hipMemcpyAsync(grad, ctx->mlp.g, kParams * sizeof(float), hipMemcpyDeviceToDevice, s);
     if (e == hipSuccess && loss) e = hipMemcpyAsync(loss, ctx->mlp.scal + 1, sizeof(float), hipMemcpyDeviceToDevice, s);
     return check_hip(ctx, e, "mlp gradient");
@@ -729,7 +1030,7 @@ int ar_mlp_train(ar_ctx* ctx, const float* x5, const float* dir, const float* we
     hipStream_t s = (hipStream_t)stream;
     hipError_t e = hipSuccess;
     for (uint32_t t = 0; t < steps && e == hipSuccess; ++t) {
-        e = mlp_gradient(ctx, x5, kUsedIn, dir, weight, n, nullptr, true, s);
+        e = mlp_gradient(ctx, x5, kUsedIn, dir, weight, n, nullptr, true, s, true);
         if (e == hipSuccess) e = mlp_adam_step(ctx, s);
     }
     if (e == hipSuccess && loss) e = hipMemcpyAsync(loss, ctx->mlp.scal + 1, sizeof(float), hipMemcpyDeviceToDevice, s);

@@ -335,7 +335,7 @@ MLP_TRAIN_FLOP = 2 * MLP_FWD_FLOP + 2 * (64 * 64 + 64 * 64 + 64 * 40)
 
 def mlp_bench(ctx_cls, reps=10):
     """Guiding-network kernels on synthetic records: one Adam step over 2^18
-    training records (k_mlp_grad + weight reduction + Adam) and inference of
+    training records (k_mlp_train + fixed-order reductions + Adam) and inference of
     2^21 vertices (k_mlp_forward), each timed with HIP events on the current
     stream; bound = BF16 dense MFMA."""
     import torch

@@ -197,7 +197,7 @@ double orc_mlp_loss_grad(const double* P, int n, const float* x5, const float* d
             }
             d3[i] = bf(h3[i] > 0 ? s3 : 0);
         }
-        for (int o2 = 0; o2 < NO; ++o2) grad[OB3 + o2] += d_o[o2];
+        for (int o2 = 0; o2 < NO; ++o2) grad[OB3 + o2] += dob[o2];
         for (int i = 0; i < NH; ++i) {
             double s2 = 0;
             for (int o2 = 0; o2 < NH; ++o2) {

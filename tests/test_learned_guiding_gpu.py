@@ -4,10 +4,10 @@ online on each frame's path vertices and sampled in the BSDF mixture.
 * Unbiased: guided and plain renders of the Cornell box converge to the same
   mean (per-render spread ~0.1 %, bar 0.5 %).
 * Gain: on the occluded-light room (light only through a door gap) the
-  guided render's relMSE against a 8192-spp reference is below 0.85x the plain
-  render's at equal spp, averaged over two seeds (measured 0.56x to 0.76x
-  for single renders at 64 spp: training sums gradients with float atomics,
-  so its order, and the network, vary from run to run).
+  guided render's relMSE against a 8192-spp reference is below 0.8x the plain
+  render's at equal spp, averaged over two seeds (measured 0.63x; training
+  reduces gradients in a fixed order, so the trained network and the ratio
+  are reproducible).
 """
 import numpy as np
 import pytest
@@ -65,6 +65,6 @@ def test_learned_guiding_reduces_error_on_occluded_light(ctx):
         rp += orc.rel_mse(plain, ref) / 2
         rl += orc.rel_mse(learned.cpu().numpy(), ref) / 2
     print(f"relMSE at 64 spp (2 seeds): plain {rp:.4e}, learned guiding {rl:.4e} ({rl / rp:.2f}x)", flush=True)
-    assert rl < 0.85 * rp
+    assert rl < 0.8 * rp
     params = ctx.mlp_params()
     assert np.isfinite(params).all()

@@ -116,3 +116,42 @@ def test_training_fits_a_target_lobe(ctx):
     l1, _ = loss_grad(P1, x[:512], d[:512], wt[:512], False)
     print(f"fp64 loss of the GPU-trained network: {l0:.4f} -> {l1:.4f}", flush=True)
     assert l1 < l0 - 1.5
+
+
+def test_gradient_over_many_rounds_matches_bf16_oracle(ctx):
+    """60,000 samples: every block of the training kernel runs two rounds of
+    128 samples, so the register accumulators carry across rounds."""
+    P = init_params(21).astype(np.float32)
+    ctx.mlp_set_params(P)
+    x, d, wt = batch(22, 60_000)
+    g, loss = ctx.mlp_gradient(_dev(x), _dev(d), _dev(wt))
+    g = g.cpu().numpy().astype(np.float64)
+    lb, gb = loss_grad_bf16(P.astype(np.float64), x, d, wt)
+    print(f"60k: loss gpu {loss.item():.6f} bf16 oracle {lb:.6f}", flush=True)
+    assert abs(loss.item() - lb) < 1e-4 * abs(lb) + 1e-5
+    for name, (off, n) in LAYERS.items():
+        a, b = g[off:off + n], gb[off:off + n]
+        cos = a @ b / (np.linalg.norm(a) * np.linalg.norm(b) + 1e-30)
+        rel = np.linalg.norm(a - b) / (np.linalg.norm(b) + 1e-30)
+        print(f"60k {name}: cos {cos:.6f}, rel L2 {rel:.4f}", flush=True)
+        assert cos > 0.9999 and rel < 0.01, name
+
+
+def test_gradient_and_training_are_bit_reproducible(ctx):
+    """No atomics: the same batch gives the same gradient bits, and the same
+    Adam steps give the same parameters in another context."""
+    from aperturerendererv3_amd import Context
+    P = init_params(31).astype(np.float32)
+    ctx.mlp_set_params(P)
+    x, d, wt = batch(32, 1 << 17)
+    xd, dd, wd = _dev(x), _dev(d), _dev(wt)
+    g1, l1 = ctx.mlp_gradient(xd, dd, wd)
+    g2, l2 = ctx.mlp_gradient(xd, dd, wd)
+    assert np.array_equal(g1.cpu().numpy().view(np.uint32), g2.cpu().numpy().view(np.uint32))
+    assert l1.item() == l2.item()
+    params = []
+    for c in (ctx, Context(0)):
+        c.mlp_configure(learning_rate=3e-3, batch=1 << 16, seed=5)
+        c.mlp_train(xd, dd, wd, 20)
+        params.append(c.mlp_params())
+    assert np.array_equal(params[0].view(np.uint32), params[1].view(np.uint32))
