@@ -1,16 +1,14 @@
 // Learned guiding: BF16 MFMA forward and backward of the 5-64-64-64-40 MLP,
 // Adam, and the guiding-MLP C ABI.
 //
-// Inference (k_mlp_forward): persistent one-wave blocks holding every
-// weight fragment in registers and chaining a 64-sample tile through the
-// layers in registers (v_mfma_f32_16x16x16_bf16, samples as N).
-// Training (k_mlp_train): 4-wave blocks on v_mfma_f32_16x16x32_bf16, the
-// activations of a 32-sample tile per wave in registers, weight gradients
-// accumulated per block from LDS-transposed activations and reduced in a
-// fixed order (no atomics: two runs give the same bits); see the comment
-// above k_mlp_train.  The loss gradient of the vMF mixture is computed per
-// sample (lane = sample) in FP32; Adam updates the FP32 masters and rewrites
-// the BF16 copies.
+// Both kernels run on v_mfma_f32_16x16x32_bf16 with FP32 accumulation, the
+// samples as N, the weights staged in LDS and each wave's 32-sample tile
+// chained through the layers in registers.  Training (k_mlp_train, 8-wave
+// blocks) accumulates weight gradients per block from LDS-transposed
+// activations and reduces them in a fixed order (no atomics: two runs give
+// the same bits); inference (k_mlp_infer, 4-wave blocks) optionally decodes
+// the vMF mixture.  The loss gradient of the mixture is computed per sample
+// in FP32; Adam updates the FP32 masters and rewrites the BF16 copies.
 #include "ar_internal.hpp"
 #include "mlp.hpp"
 #include "path_common.hpp"
@@ -53,165 +51,7 @@ struct Net {
     const uint16_t* wt;    // BF16 weights transposed, [out][in] per layer (same offsets)
 };
 
-struct Lobes {
-    float pi[kLobes], kappa[kLobes], mu[kLobes][3], clen[kLobes];
-    bool clamped[kLobes];
-};
-
-__device__ __forceinline__ void decode(const float* o, Lobes& L) {
-    float mx = -3.0e38f;
-#pragma unroll
-    for (int k = 0; k < kLobes; ++k) mx = fmaxf(mx, o[5 * k]);
-    float z = 0.0f;
-#pragma unroll
-    for (int k = 0; k < kLobes; ++k) { L.pi[k] = expf(o[5 * k] - mx); z += L.pi[k]; }
-#pragma unroll
-    for (int k = 0; k < kLobes; ++k) {
-        L.pi[k] /= z;
-        float s = o[5 * k + 1];
-        L.clamped[k] = s < -3.0f || s > 7.0f;
-        s = fminf(7.0f, fmaxf(-3.0f, s));
-        L.kappa[k] = expf(s);
-        const float cx = o[5 * k + 2], cy = o[5 * k + 3], cz = o[5 * k + 4];
-        const float l = sqrtf(cx * cx + cy * cy + cz * cz);
-        L.clen[k] = l < 1e-12f ? 0.0f : l;
-        if (l < 1e-12f) { L.mu[k][0] = 0.0f; L.mu[k][1] = 0.0f; L.mu[k][2] = 1.0f; }
-        else { L.mu[k][0] = cx / l; L.mu[k][1] = cy / l; L.mu[k][2] = cz / l; }
-    }
-}
-
 __device__ __forceinline__ float vmf_norm(float kap) { return kap / (kTwoPi * (1.0f - expf(-2.0f * kap))); }
-
-// Inference: persistent one-wave blocks.  Each lane keeps its A fragments of
-// all four layers in registers (96 VGPRs of BF16 weights) and each 64-sample
-// tile's activations stay in registers too: with the samples as the N
-// dimension, a layer's D tile (rows = units 4g..4g+3, column = sample i)
-// after bias, ReLU and BF16 rounding is exactly the B operand of the next
-// layer's K block, so hidden layers never touch LDS.  DECODE: the 40 outputs
-// of a sample go through LDS to one lane, which writes the 48 decoded floats;
-// else the 40 raw outputs are stored directly.
-__device__ __forceinline__ bf16x4 relu_bf16(f32x4 c, const float b[4]) {
-    bf16x4 h;
-#pragma unroll
-    for (int r = 0; r < 4; ++r) h[r] = (short)f2bf(fmaxf(c[r] + b[r], 0.0f));
-    return h;
-}
-
-template <bool DECODE, int XS>
-__global__ __launch_bounds__(64) void k_mlp_forward(Net N, const float* __restrict__ x5, size_t n,
-                                                    float* __restrict__ out, const uint32_t* __restrict__ n_dev) {
-    __shared__ float lo[DECODE ? 64 * kOut : 1];
-    const int lane = threadIdx.x;
-    const int i = lane & 15, g = lane >> 4;
-    const size_t nn = n_dev ? std::min<size_t>(n, *n_dev) : n;
-    if ((size_t)blockIdx.x * 64 >= nn) return;
-    bf16x4 a0[4], a1[4][4], a2[4][4], a3[3][4];
-#pragma unroll
-    for (int mb = 0; mb < 4; ++mb) {
-        a0[mb] = ld4(N.wt + OW0 + (mb * 16 + i) * kIn + 4 * g);
-#pragma unroll
-        for (int kb = 0; kb < 4; ++kb) {
-            a1[mb][kb] = ld4(N.wt + OW1 + (mb * 16 + i) * kHid + kb * 16 + 4 * g);
-            a2[mb][kb] = ld4(N.wt + OW2 + (mb * 16 + i) * kHid + kb * 16 + 4 * g);
-        }
-    }
-#pragma unroll
-    for (int mb = 0; mb < 3; ++mb)
-#pragma unroll
-        for (int kb = 0; kb < 4; ++kb) a3[mb][kb] = ld4(N.wt + OW3 + (mb * 16 + i) * kHid + kb * 16 + 4 * g);
-    for (size_t s0 = (size_t)blockIdx.x * 64; s0 < nn; s0 += (size_t)gridDim.x * 64) {
-        // keep the per-tile bias loads inside the loop (hoisting them out would
-        // cost 60 registers and halve the occupancy)
-        asm volatile("" ::: "memory");
-        bf16x4 h[4][4], h2[4][4];
-        // layer 0: B[k = input 4g + r][sample i]
-#pragma unroll
-        for (int nb = 0; nb < 4; ++nb) {
-            const size_t s = s0 + nb * 16 + i;
-            bf16x4 xb;
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int k = 4 * g + r;
-                xb[r] = (short)(s < nn ? f2bf(record_input(x5 + s * XS, XS, k)) : (uint16_t)0);
-            }
-#pragma unroll
-            for (int mb = 0; mb < 4; ++mb) {
-                float b[4];
-#pragma unroll
-                for (int r = 0; r < 4; ++r) b[r] = N.p[OB0 + mb * 16 + 4 * g + r];
-                const f32x4 z = {0.0f, 0.0f, 0.0f, 0.0f};
-                h[mb][nb] = relu_bf16(mfma(a0[mb], xb, z), b);
-            }
-        }
-        // layers 1 and 2
-#pragma unroll
-        for (int mb = 0; mb < 4; ++mb) {
-            float b[4];
-#pragma unroll
-            for (int r = 0; r < 4; ++r) b[r] = N.p[OB1 + mb * 16 + 4 * g + r];
-#pragma unroll
-            for (int nb = 0; nb < 4; ++nb) {
-                f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-                for (int kb = 0; kb < 4; ++kb) c = mfma(a1[mb][kb], h[kb][nb], c);
-                h2[mb][nb] = relu_bf16(c, b);
-            }
-        }
-#pragma unroll
-        for (int mb = 0; mb < 4; ++mb) {
-            float b[4];
-#pragma unroll
-            for (int r = 0; r < 4; ++r) b[r] = N.p[OB2 + mb * 16 + 4 * g + r];
-#pragma unroll
-            for (int nb = 0; nb < 4; ++nb) {
-                f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-                for (int kb = 0; kb < 4; ++kb) c = mfma(a2[mb][kb], h2[kb][nb], c);
-                h[mb][nb] = relu_bf16(c, b);
-            }
-        }
-        // output layer
-#pragma unroll
-        for (int mb = 0; mb < 3; ++mb) {
-            float b[4];
-#pragma unroll
-            for (int r = 0; r < 4; ++r) b[r] = N.p[OB3 + mb * 16 + 4 * g + r];
-#pragma unroll
-            for (int nb = 0; nb < 4; ++nb) {
-                f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-                for (int kb = 0; kb < 4; ++kb) c = mfma(a3[mb][kb], h[kb][nb], c);
-                const int u0 = mb * 16 + 4 * g, sl = nb * 16 + i;
-                const float4 v = make_float4(c[0] + b[0], c[1] + b[1], c[2] + b[2], c[3] + b[3]);
-                if (DECODE) {
-                    *reinterpret_cast<float4*>(&lo[sl * kOut + u0]) = v;
-                } else if (u0 < kUsedOut && s0 + sl < nn) {
-                    *reinterpret_cast<float4*>(&out[(s0 + sl) * kUsedOut + u0]) = v;
-                }
-            }
-        }
-        if (DECODE) {
-            __syncthreads();
-            const size_t s = s0 + lane;
-            if (s < nn) {
-                Lobes D;
-                decode(lo + lane * kOut, D);
-                float* d = out + s * kDecoded;
-#pragma unroll
-                for (int k = 0; k < kLobes; ++k) {
-                    d[6 * k + 0] = D.pi[k];
-                    d[6 * k + 1] = D.pi[k] * vmf_norm(D.kappa[k]);
-                    d[6 * k + 2] = D.kappa[k];
-                    d[6 * k + 3] = D.mu[k][0];
-                    d[6 * k + 4] = D.mu[k][1];
-                    d[6 * k + 5] = D.mu[k][2];
-                }
-            }
-            __syncthreads();
-        }
-    }
-}
-
 
 // ---------------------------------------------------------------- training
 //
@@ -257,6 +97,13 @@ __device__ __forceinline__ s16x8 lds8(const uint16_t* p) { return *reinterpret_c
 // the K-permuted weight fragment of a 64-wide K block j: units 32j + 4g .. + 3 and 32j + 16 + 4g .. + 3
 __device__ __forceinline__ s16x8 wfrag(const uint16_t* row, int j, int g) {
     return cat(ld4(row + 32 * j + 4 * g), ld4(row + 32 * j + 16 + 4 * g));
+}
+// bias, ReLU and BF16 rounding of a D fragment: the next layer's B half
+__device__ __forceinline__ bf16x4 relu_bf16(f32x4 c, const float b[4]) {
+    bf16x4 h;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) h[r] = (short)f2bf(fmaxf(c[r] + b[r], 0.0f));
+    return h;
 }
 __device__ __forceinline__ bf16x4 relu_mask(bf16x4 h, f32x4 c) {
     bf16x4 r;
@@ -637,6 +484,135 @@ __global__ __launch_bounds__(kTB) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     }
 }
 
+// ---------------------------------------------------------------- inference
+//
+// k_mlp_infer: the training kernel's forward on 4-wave blocks (two blocks per
+// CU), forward weights and biases staged in LDS, a 32-sample tile per wave
+// chained through the layers in registers on the 16x16x32 BF16 MFMA.
+// Raw outputs are stored from the D fragments; DECODE sends each tile's
+// outputs through LDS to two lanes per vertex (lobes 0..3 / 4..7), which
+// write the 48 decoded floats per vertex: per lobe (pi, pi * kappa /
+// (2 pi (1 - e^-2kappa)), kappa, mu).
+constexpr int kIW = 4;
+struct InferShared {
+    uint16_t wgt[LB3];                 // forward weights W^T, padded rows (LT0 .. LT3)
+    float bias[kBias];
+    float r[kIW][kTT * kRS];           // output tile per wave (DECODE)
+};
+
+template <bool DECODE, int XS>
+__global__ __launch_bounds__(kIW * 64) void k_mlp_infer(Net N, const float* __restrict__ x, size_t n,
+                                                        float* __restrict__ out, const uint32_t* __restrict__ n_dev) {
+    __shared__ InferShared S;
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63, i = lane & 15, g = lane >> 4;
+    const size_t nn = n_dev ? std::min<size_t>(n, *n_dev) : n;
+    const f32x4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
+    {
+        struct Mat { int src, rows, cols, dst, stride; };
+        const Mat mats[4] = {{OW0, kHid, kIn, LT0, kS16}, {OW1, kHid, kHid, LT1, kS64}, {OW2, kHid, kHid, LT2, kS64},
+                             {OW3, kOut, kHid, LT3, kS64}};
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {   // unrolled: a loop over the table would move it to LDS
+            const Mat& M = mats[m];
+            const int quads = M.rows * M.cols / 4;
+            bf16x4 v[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int e = tid + kIW * 64 * u;
+                if (e < quads) v[u] = *reinterpret_cast<const bf16x4*>(N.wt + M.src + 4 * e);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int e = tid + kIW * 64 * u;
+                if (e < quads) {
+                    const int row = (4 * e) / M.cols, col = (4 * e) % M.cols;
+                    *reinterpret_cast<bf16x4*>(&S.wgt[M.dst + row * M.stride + col]) = v[u];
+                }
+            }
+        }
+        if (tid < kHid) {
+            S.bias[tid] = N.p[OB0 + tid];
+            S.bias[kHid + tid] = N.p[OB1 + tid];
+            S.bias[2 * kHid + tid] = N.p[OB2 + tid];
+            if (tid < kOut) S.bias[3 * kHid + tid] = N.p[OB3 + tid];
+        }
+        __syncthreads();
+    }
+    float* R = S.r[w];
+    for (size_t t0 = ((size_t)blockIdx.x * kIW + w) * kTT; t0 < nn; t0 += (size_t)gridDim.x * kIW * kTT) {
+        bf16x4 xb[2];
+#pragma unroll
+        for (int nb = 0; nb < 2; ++nb) {
+            const size_t s = t0 + nb * 16 + i;
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                xb[nb][r] = (short)(s < nn ? f2bf(record_input(x + s * XS, XS, 4 * g + r)) : (uint16_t)0);
+        }
+        bf16x4 h1[4][2], h2[4][2];
+#pragma unroll
+        for (int mb = 0; mb < 4; ++mb) {
+            const bf16x4 a = ld4(S.wgt + LT0 + (mb * 16 + i) * kS16 + 4 * g);
+            float b[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) b[r] = S.bias[mb * 16 + 4 * g + r];
+#pragma unroll
+            for (int nb = 0; nb < 2; ++nb) h1[mb][nb] = relu_bf16(mfma(a, xb[nb], zero), b);
+        }
+        fwd_hidden(S.wgt + LT1, S.bias + kHid, h1, h2, i, g);
+        fwd_hidden(S.wgt + LT2, S.bias + 2 * kHid, h2, h1, i, g);   // h1 now holds the third layer
+        layer_fence();
+        if (DECODE) wave_sync();                                    // the previous tile's decode has read R
+#pragma unroll
+        for (int mb = 0; mb < 3; ++mb) {
+            const uint16_t* row = S.wgt + LT3 + (mb * 16 + i) * kS64;
+            const s16x8 a0 = wfrag(row, 0, g), a1 = wfrag(row, 1, g);
+            float b[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) b[r] = S.bias[3 * kHid + mb * 16 + 4 * g + r];
+#pragma unroll
+            for (int nb = 0; nb < 2; ++nb) {
+                f32x4 c = mfma32(a0, cat(h1[0][nb], h1[1][nb]), zero);
+                c = mfma32(a1, cat(h1[2][nb], h1[3][nb]), c);
+                const float4 v = make_float4(c[0] + b[0], c[1] + b[1], c[2] + b[2], c[3] + b[3]);
+                const int u0 = mb * 16 + 4 * g, sl = nb * 16 + i;
+                if (DECODE) *reinterpret_cast<float4*>(&R[sl * kRS + u0]) = v;
+                else if (u0 < kUsedOut && t0 + sl < nn) *reinterpret_cast<float4*>(&out[(t0 + sl) * kUsedOut + u0]) = v;
+            }
+        }
+        if (DECODE) {
+            wave_sync();
+            const int sl = lane & (kTT - 1), h = lane >> 5;
+            const size_t s = t0 + sl;
+            const float* o = &R[sl * kRS];
+            float mx = -3.0e38f;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) mx = fmaxf(mx, o[5 * (4 * h + j)]);
+            mx = fmaxf(mx, __shfl_xor(mx, 32));
+            float pu[4], z = 0.0f;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) { pu[j] = expf(o[5 * (4 * h + j)] - mx); z += pu[j]; }
+            z += __shfl_xor(z, 32);
+            if (s < nn) {
+                float* d = out + s * kDecoded;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int k = 4 * h + j;
+                    const float pi = pu[j] / z;
+                    const float kap = expf(fminf(7.0f, fmaxf(-3.0f, o[5 * k + 1])));
+                    const float cx = o[5 * k + 2], cy = o[5 * k + 3], cz = o[5 * k + 4];
+                    const float l = sqrtf(cx * cx + cy * cy + cz * cz);
+                    d[6 * k + 0] = pi;
+                    d[6 * k + 1] = pi * vmf_norm(kap);
+                    d[6 * k + 2] = kap;
+                    d[6 * k + 3] = l < 1e-12f ? 0.0f : cx / l;
+                    d[6 * k + 4] = l < 1e-12f ? 0.0f : cy / l;
+                    d[6 * k + 5] = l < 1e-12f ? 1.0f : cz / l;
+                }
+            }
+        }
+    }
+}
+
 // Fixed-order sums of the block partials: chunks of kRedChunk blocks, then the chunk sums.
 constexpr int kRedChunk = 16;
 __global__ void k_mlp_reduce1(const float* __restrict__ part, const float* __restrict__ part_loss, int blocks,
@@ -924,13 +900,14 @@ hipError_t mlp_forward(ar_ctx* ctx, const float* x5, int xs, size_t n, const uin
                        hipStream_t s) {
     MlpDev& M = ctx->mlp;
     Net N{M.p, M.wb, M.wt};
-    // persistent: one wave per SIMD (the kernel keeps ~280 registers), each looping over 64-sample tiles
-    const uint32_t grid = (uint32_t)std::min<size_t>((n + 63) / 64, (size_t)std::max(ctx->num_cus, 1) * 4);
-    const dim3 g(std::max(grid, 1u));
-    if (decoded && xs == kRec) hipLaunchKernelGGL((k_mlp_forward<true, kRec>), g, dim3(64), 0, s, N, x5, n, out, n_dev);
-    else if (decoded) hipLaunchKernelGGL((k_mlp_forward<true, kUsedIn>), g, dim3(64), 0, s, N, x5, n, out, n_dev);
-    else if (xs == kRec) hipLaunchKernelGGL((k_mlp_forward<false, kRec>), g, dim3(64), 0, s, N, x5, n, out, n_dev);
-    else hipLaunchKernelGGL((k_mlp_forward<false, kUsedIn>), g, dim3(64), 0, s, N, x5, n, out, n_dev);
+    // persistent: two 4-wave blocks per CU, each wave looping over 32-sample tiles
+    const uint32_t grid = (uint32_t)std::min<size_t>((n + kIW * kTT - 1) / (kIW * kTT),
+                                                     (size_t)std::max(ctx->num_cus, 1) * 2);
+    const dim3 g(std::max(grid, 1u)), b(kIW * 64);
+    if (decoded && xs == kRec) hipLaunchKernelGGL((k_mlp_infer<true, kRec>), g, b, 0, s, N, x5, n, out, n_dev);
+    else if (decoded) hipLaunchKernelGGL((k_mlp_infer<true, kUsedIn>), g, b, 0, s, N, x5, n, out, n_dev);
+    else if (xs == kRec) hipLaunchKernelGGL((k_mlp_infer<false, kRec>), g, b, 0, s, N, x5, n, out, n_dev);
+    else hipLaunchKernelGGL((k_mlp_infer<false, kUsedIn>), g, b, 0, s, N, x5, n, out, n_dev);
     return hipGetLastError();
 }
 
