@@ -476,13 +476,14 @@ class Context:
     def render_progressive(self, frame, accum, odd=None, even=None, spp=1, cam_pos=(0.0, 0.0, 1.1),
                            cam_dir=(0.0, 0.0, -1.0), vfov=60.0, seed=1, max_depth=5, nee=True, sky=AR_SKY_NONE,
                            env=(0.0, 0.0, 0.0), rr_min_depth=3, spectral=0, output=0, gbuffer=None, stats=None,
-                           stream=None):
+                           guiding=0, frame_spp=0, stream=None):
         """ar_render_progressive: fold spp samples (indices frame ..) into the
         running means accum (and odd / even), (h, w, 3) float32 device tensors;
-        gbuffer (h, w, 16) / stats (h, w, 12) keep the denoiser's records."""
+        gbuffer (h, w, 16) / stats (h, w, 12) keep the denoiser's records;
+        guiding updates the context's guide after every frame_spp samples."""
         h, w = accum.shape[0], accum.shape[1]
         cam, o = _camera_opts(w, h, spp, cam_pos, cam_dir, vfov, seed, max_depth, nee, sky, env, rr_min_depth, 0, 0,
-                              spectral, output, 0, 0)
+                              spectral, output, guiding, frame_spp)
         opt = lambda t: _ptr(t) if t is not None else None
         self._check(self._lib.ar_render_progressive(self._h, C.byref(cam), C.byref(o), int(frame), _ptr(accum),
                                                     opt(odd), opt(even), opt(gbuffer), opt(stats),
@@ -649,6 +650,38 @@ class Context:
         self._check(self._lib.ar_render(self._h, C.byref(cam), C.byref(o), _ptr(out), _stream_ptr(stream)),
                     "ar_render")
         return out
+
+
+    def present(self, hdr=None, gbuffer=None, stats=None, view=0, exposure=1.0, tone=AR_TONE_ACES, xyz_input=False,
+                width=None, height=None, out=None, stream=None):
+        """ar_present: (h, w) int32 device image of 0xAARRGGBB pixels from
+        hdr (h, w, 3) through the tone curve (view 0), denoised first (view
+        1), or a debug view of the gbuffer (h, w, 16) / stats (h, w, 12)
+        records (views 2..10)."""
+        import torch
+        ref = next(t for t in (hdr, gbuffer, stats) if t is not None)
+        h = height if height is not None else ref.shape[0]
+        w = width if width is not None else ref.shape[1]
+        if out is None:
+            out = torch.empty((h, w), dtype=torch.int32, device=ref.device)
+        opt = lambda t: _ptr(t) if t is not None else None
+        self._check(self._lib.ar_present(self._h, opt(hdr), opt(gbuffer), opt(stats), _ptr(out), w, h,
+                                         float(exposure), int(tone), int(bool(xyz_input)), int(view),
+                                         _stream_ptr(stream)), "ar_present")
+        return out
+
+    def render_host(self, width, height, spp, exposure=1.0, tone=AR_TONE_ACES, hdr=True, ldr=True, **kw):
+        """ar_render_host: renders into the context's own device image and
+        returns (hdr, ldr) numpy arrays ((height, width, 3) float32 and
+        (height, width) uint32 0xAARRGGBB; either may be skipped -> None)."""
+        import numpy as np
+        cam, o = _camera_opts(width, height, spp, **kw)
+        h = np.empty((height, width, 3), np.float32) if hdr else None
+        l = np.empty((height, width), np.uint32) if ldr else None
+        self._check(self._lib.ar_render_host(self._h, C.byref(cam), C.byref(o), h.ctypes.data if hdr else None,
+                                             l.ctypes.data if ldr else None, float(exposure), int(tone)),
+                    "ar_render_host")
+        return h, l
 
 
 def write_bmp(path, ldr):

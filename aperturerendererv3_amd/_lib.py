@@ -33,6 +33,9 @@ AR_STATS_FLOATS = 12
 AR_TONE_LINEAR = 0
 AR_TONE_GAMMA = 1
 AR_TONE_ACES = 2
+AR_VIEW_IMAGE, AR_VIEW_DENOISED, AR_VIEW_ALBEDO, AR_VIEW_NORMAL, AR_VIEW_DEPTH = 0, 1, 2, 3, 4
+AR_VIEW_METALLIC, AR_VIEW_ROUGHNESS, AR_VIEW_STAT_COUNT, AR_VIEW_STAT_MEAN = 5, 6, 7, 8
+AR_VIEW_STAT_M2, AR_VIEW_STAT_M3 = 9, 10
 AR_OBJ_SPHERE = 1
 AR_HIT_INSIDE = 16
 
@@ -153,6 +156,10 @@ SIGNATURES = {
                                       C.POINTER(C.c_float * 3), C.c_float, C.c_float, C.POINTER(ar_rays),
                                       C.c_void_p]),
     "ar_render": (C.c_int, [C.c_void_p, C.POINTER(ar_camera), C.POINTER(ar_render_opts), C.c_void_p, C.c_void_p]),
+    "ar_present": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32,
+                             C.c_float, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p]),
+    "ar_render_host": (C.c_int, [C.c_void_p, C.POINTER(ar_camera), C.POINTER(ar_render_opts), C.c_void_p, C.c_void_p,
+                                 C.c_float, C.c_uint32]),
     "ar_render_progressive": (C.c_int, [C.c_void_p, C.POINTER(ar_camera), C.POINTER(ar_render_opts), C.c_uint32,
                                         C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "ar_denoise": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32,

@@ -235,6 +235,8 @@ int ar_destroy(ar_ctx* ctx) {
     if (ctx->reduce) (void)hipFree(ctx->reduce);
     sdf_free(ctx);
     if (ctx->denoise_feat) (void)hipFree(ctx->denoise_feat);
+    if (ctx->host_out) (void)hipFree(ctx->host_out);
+    if (ctx->present_buf) (void)hipFree(ctx->present_buf);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return AR_OK;
@@ -525,8 +527,6 @@ int ar_render_progressive(ar_ctx* ctx, const ar_camera* cam, const ar_render_opt
     if (!cam || !opts || !accum) return set_error(ctx, AR_E_INVALID, "NULL camera, options or accumulation buffer");
     if ((odd == nullptr) != (even == nullptr)) return set_error(ctx, AR_E_INVALID, "odd and even go together");
     if (opts->width == 0 || opts->height == 0 || opts->spp == 0) return set_error(ctx, AR_E_INVALID, "empty image or spp = 0");
-    if (opts->guiding != AR_GUIDING_OFF)
-        return set_error(ctx, AR_E_UNSUPPORTED, "progressive frames are unguided; use ar_render for guided frames");
     if (!ctx->bvh_ready) return set_error(ctx, AR_E_NOSCENE, "no BVH: call ar_scene_set and ar_bvh_build first");
     const int rc = order_stream(ctx, pick_stream(ctx, stream));
     if (rc != AR_OK) return rc;

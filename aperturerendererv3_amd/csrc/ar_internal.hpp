@@ -282,7 +282,8 @@ struct MlpDev {
     float* rx = nullptr;        // training records: inputs [rcap][kRec]
     float* rd = nullptr;        // directions [rcap][3]
     float* rw = nullptr;        // weights [rcap]
-    uint32_t* rcount = nullptr; // records collected this frame
+    uint32_t* rcount = nullptr; // [0] records trained on this frame (min(T, batch)), [1] records T of the frame
+    uint32_t* rblk = nullptr;   // per 256-path block: record count, then its offset
 };
 
 // Spectral basis network (smlp.hip): parameters, AdamW state, training
@@ -373,6 +374,10 @@ struct ar_ctx {
     double* reduce = nullptr;             // partial sums of ar_luminance_mse
     ar::SdfPool sdf;                      // signed-distance volumes
     float* denoise_feat = nullptr;        // per-pixel filter features of ar_denoise (29 planes)
+    float* host_out = nullptr;            // ar_render_host's device image (3 floats HDR + 1 word LDR per pixel)
+    size_t host_out_pixels = 0;
+    float* present_buf = nullptr;         // ar_present's denoised image (3 floats per pixel)
+    size_t present_pixels = 0;
     ar::SmlpDev smlp;                     // spectral basis network
     size_t denoise_floats = 0;
 };

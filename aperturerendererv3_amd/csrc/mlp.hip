@@ -741,16 +741,21 @@ struct Slots {
 
 // Training records of a frame: every slot that received radiance gives
 // (inputs, direction, luminance(L) / pdf(direction)), the Monte-Carlo weight
-// of the KL divergence from the incident radiance to the guide.
-__global__ __launch_bounds__(256) void k_mlp_records(Slots S, uint32_t npaths, uint32_t spectral,
-                                                     const float* __restrict__ lambda0, const float* __restrict__ cie,
-                                                     float* rx, float* rd, float* rw, uint32_t* count, uint32_t cap) {
-    const uint32_t p = blockIdx.x * 256 + threadIdx.x;
+// of the KL divergence from the incident radiance to the guide.  The records
+// are numbered in (path, slot) order by a block count, a scan and a write
+// pass, so a frame's records and the trained network do not depend on
+// scheduling.  When a frame has T records and the training batch is B < T,
+// the batch is records floor(j T / B), j < B: evenly strided over the frame
+// (over the image), not its first B.
+__device__ __forceinline__ uint32_t record_weights(const Slots& S, uint32_t p, uint32_t npaths, uint32_t spectral,
+                                                   const float* __restrict__ lambda0, const float* __restrict__ cie,
+                                                   float w[5]) {
     const bool live = p < npaths;
     const uint32_t flags = live ? S.gflag[p] : 0u;
     const Lam L = make_lam(spectral, live && spectral != AR_SPECTRAL_RGB ? lambda0[p] : 0.0f);
+    uint32_t mask = 0;
     for (int i = 0; i < 5; ++i) {
-        float w = 0.0f;
+        w[i] = 0.0f;
         if ((flags >> i) & 1u) {
             const float* r = S.gr + (size_t)(4 * i) * S.stride + p;
             float lum;
@@ -770,20 +775,91 @@ __global__ __launch_bounds__(256) void k_mlp_records(Slots S, uint32_t npaths, u
                 lum = lum / (4.0f / 400.0f);
             }
             const float pdf = S.gp[(size_t)i * S.stride + p];
-            w = pdf > 0.0f ? lum / pdf : 0.0f;
+            w[i] = pdf > 0.0f ? lum / pdf : 0.0f;
         }
-        const bool ok = w > 0.0f && w < 3.0e38f;
-        const uint32_t slot = wave_append(ok, count);
-        if (ok && slot < cap) {
-            for (int k = 0; k < kRec; ++k) rx[kRec * (size_t)slot + k] = S.gx[(size_t)(kRec * i + k) * S.stride + p];
-            for (int k = 0; k < 3; ++k) rd[3 * (size_t)slot + k] = S.gw[(size_t)(3 * i + k) * S.stride + p];
-            rw[slot] = w;
-        }
+        if (w[i] > 0.0f && w[i] < 3.0e38f) mask |= 1u << i;
+    }
+    return mask;
+}
+
+constexpr uint32_t kRecBlock = 256;
+
+// in-block exclusive scan of c (Hillis-Steele over LDS); returns the prefix, *total the block sum
+__device__ __forceinline__ uint32_t block_exclusive(uint32_t* sc, uint32_t c, uint32_t* total) {
+    const uint32_t t = threadIdx.x;
+    sc[t] = c;
+    __syncthreads();
+    for (uint32_t off = 1; off < blockDim.x; off <<= 1) {
+        const uint32_t v = t >= off ? sc[t - off] : 0u;
+        __syncthreads();
+        sc[t] += v;
+        __syncthreads();
+    }
+    *total = sc[blockDim.x - 1];
+    return sc[t] - c;
+}
+
+__global__ __launch_bounds__(kRecBlock) void k_rec_count(Slots S, uint32_t npaths, uint32_t spectral,
+                                                         const float* __restrict__ lambda0,
+                                                         const float* __restrict__ cie, uint32_t* __restrict__ blk) {
+    __shared__ uint32_t sc[kRecBlock];
+    float w[5];
+    const uint32_t m = record_weights(S, blockIdx.x * kRecBlock + threadIdx.x, npaths, spectral, lambda0, cie, w);
+    uint32_t total;
+    (void)block_exclusive(sc, (uint32_t)__popc(m), &total);
+    if (threadIdx.x == 0) blk[blockIdx.x] = total;
+}
+
+// exclusive offsets of the block counts in place; count[0] = min(T, batch), count[1] = T
+__global__ __launch_bounds__(1024) void k_rec_scan(uint32_t* __restrict__ blk, uint32_t nblocks, uint32_t batch,
+                                                   uint32_t* __restrict__ count) {
+    __shared__ uint32_t sc[1024];
+    const uint32_t t = threadIdx.x, per = (nblocks + 1023u) / 1024u;
+    const uint32_t b0 = min(nblocks, t * per), b1 = min(nblocks, b0 + per);
+    uint32_t sum = 0;
+    for (uint32_t b = b0; b < b1; ++b) sum += blk[b];
+    uint32_t total;
+    uint32_t run = block_exclusive(sc, sum, &total);
+    for (uint32_t b = b0; b < b1; ++b) {
+        const uint32_t c = blk[b];
+        blk[b] = run;
+        run += c;
+    }
+    if (t == 0) {
+        count[0] = min(total, batch);
+        count[1] = total;
     }
 }
 
-__global__ void k_clamp_count(uint32_t* count, uint32_t cap) {
-    if (count[0] > cap) count[0] = cap;
+__global__ __launch_bounds__(kRecBlock) void k_mlp_records(Slots S, uint32_t npaths, uint32_t spectral,
+                                                           const float* __restrict__ lambda0,
+                                                           const float* __restrict__ cie,
+                                                           const uint32_t* __restrict__ blk,
+                                                           const uint32_t* __restrict__ count, float* rx, float* rd,
+                                                           float* rw) {
+    __shared__ uint32_t sc[kRecBlock];
+    const uint32_t p = blockIdx.x * kRecBlock + threadIdx.x;
+    float w[5];
+    const uint32_t m = record_weights(S, p, npaths, spectral, lambda0, cie, w);
+    uint32_t total;
+    uint32_t r = blk[blockIdx.x] + block_exclusive(sc, (uint32_t)__popc(m), &total);
+    const uint32_t B = count[0], T = count[1];
+    for (int i = 0; i < 5; ++i) {
+        if (!((m >> i) & 1u)) continue;
+        // record r is batch slot j = ceil(r B / T) when floor(j T / B) = r
+        uint64_t j = r;
+        bool keep = true;
+        if (T > B) {
+            j = ((uint64_t)r * B + T - 1) / T;
+            keep = j < B && (j * T) / B == r;
+        }
+        if (keep) {
+            for (int k = 0; k < kRec; ++k) rx[kRec * j + k] = S.gx[(size_t)(kRec * i + k) * S.stride + p];
+            for (int k = 0; k < 3; ++k) rd[3 * j + k] = S.gw[(size_t)(3 * i + k) * S.stride + p];
+            rw[j] = w[i];
+        }
+        ++r;
+    }
 }
 
 } // namespace
@@ -795,6 +871,7 @@ int mlp_ensure_buffers(ar_ctx* ctx, size_t paths) {
     for (float** q : {&M.feat, &M.dec, &M.rx, &M.rd, &M.rw})
         if (*q) { (void)hipFree(*q); *q = nullptr; }
     if (M.rcount) { (void)hipFree(M.rcount); M.rcount = nullptr; }
+    if (M.rblk) { (void)hipFree(M.rblk); M.rblk = nullptr; }
     M.cap = M.rcap = 0;
     const size_t rcap = 5 * paths;
     hipError_t e = hipMalloc((void**)&M.feat, paths * kRec * sizeof(float));
@@ -803,24 +880,25 @@ int mlp_ensure_buffers(ar_ctx* ctx, size_t paths) {
     if (e == hipSuccess) e = hipMalloc((void**)&M.rd, rcap * 3 * sizeof(float));
     if (e == hipSuccess) e = hipMalloc((void**)&M.rw, rcap * sizeof(float));
     if (e == hipSuccess) e = hipMalloc((void**)&M.rcount, 4 * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc((void**)&M.rblk, ((paths + kRecBlock - 1) / kRecBlock) * sizeof(uint32_t));
     if (e != hipSuccess) return set_error(ctx, AR_E_OOM, "mlp: device allocation failed");
     M.cap = paths;
     M.rcap = rcap;
     return AR_OK;
 }
 
-__global__ void k_zero_u32(uint32_t* p) { p[0] = 0u; }
-
 int mlp_frame_update(ar_ctx* ctx, uint32_t spectral, const float* lambda0, const float* cie, size_t paths,
                      hipStream_t s) {
     MlpDev& M = ctx->mlp;
     const GuideDev& g = ctx->guide;
     Slots S{g.gr, g.gflag, g.gx, g.gw, g.gp, (uint32_t)g.pcap};
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, M.rcount);
-    hipLaunchKernelGGL(k_mlp_records, dim3((uint32_t)((paths + 255) / 256)), dim3(256), 0, s, S, (uint32_t)paths,
-                       spectral, lambda0, cie, M.rx, M.rd, M.rw, M.rcount, (uint32_t)M.rcap);
-    hipLaunchKernelGGL(k_clamp_count, dim3(1), dim3(1), 0, s, M.rcount, (uint32_t)M.rcap);
     const size_t n = std::min<size_t>(M.rcap, M.batch);
+    const uint32_t nb = (uint32_t)((paths + kRecBlock - 1) / kRecBlock);
+    hipLaunchKernelGGL(k_rec_count, dim3(nb), dim3(kRecBlock), 0, s, S, (uint32_t)paths, spectral, lambda0, cie,
+                       M.rblk);
+    hipLaunchKernelGGL(k_rec_scan, dim3(1), dim3(1024), 0, s, M.rblk, nb, (uint32_t)n, M.rcount);
+    hipLaunchKernelGGL(k_mlp_records, dim3(nb), dim3(kRecBlock), 0, s, S, (uint32_t)paths, spectral, lambda0, cie,
+                       M.rblk, M.rcount, M.rx, M.rd, M.rw);
     hipError_t e = hipSuccess;
     for (uint32_t it = 0; it < M.iters && e == hipSuccess; ++it) {
         e = mlp_gradient(ctx, M.rx, kRec, M.rd, M.rw, n, M.rcount, true, s, true);
@@ -835,6 +913,7 @@ void mlp_free(ar_ctx* ctx) {
     for (float** q : {&M.p, &M.feat, &M.dec, &M.rx, &M.rd, &M.rw, &M.part})
         if (*q) { (void)hipFree(*q); *q = nullptr; }
     if (M.rcount) { (void)hipFree(M.rcount); M.rcount = nullptr; }
+    if (M.rblk) { (void)hipFree(M.rblk); M.rblk = nullptr; }
     M.m = M.v = M.g = M.scal = nullptr;
     M.part2 = M.ploss = M.ploss2 = M.wpart = nullptr;
     M.part_blocks = 0;

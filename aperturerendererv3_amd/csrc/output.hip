@@ -5,6 +5,9 @@
 //                       XYZ2SRGBLinearD65 (core/tonemap.cuh:116-126) first
 //   ar_luminance_mse    DeviceMSELuminance (core/helper/measure.cuh:49-74), a
 //                       two-stage reduction in double with a fixed order
+//   ar_present          the output stage of SceneRenderer::Render (core/scene.cu:
+//                       291-365): tonemap, denoise + tonemap, or a debug view
+//   ar_render_host      the offline SceneRenderer::Render (core/scene.cu:215-237)
 //   ar_write_bmp        SaveBMP (core/camera.cu:10-62): 32-bit BGRA, top-down
 //
 // One thread per pixel for the tonemap (HBM-bound: 12 B in, 4 B out).
@@ -102,6 +105,53 @@ __global__ __launch_bounds__(kOutBlock) void k_tonemap(const float* __restrict__
     ldr[i] = 0xff000000u | (r << 16) | (g << 8) | b;
 }
 
+// The Tonemap kernel's debug views (core/scene.cu:35-107) over the
+// denoiser's per-pixel records (G-buffer: albedo[3] normal[3] depth metallic
+// roughness ...; statistics: n mean[3] M2/n[3] M3/n[3]).
+// HeatMapViridis (core/render/color.cuh:99-140): matplotlib's viridis map at
+// t = 0, 1/4, 1/2, 3/4, 1, linear in between.  At t = 1 the reference's
+// segment search lands on the last segment with weight 0 and so returns the
+// 3/4 sample; kept so the views match.
+__constant__ float kViridis[15] = {0.267004f, 0.004874f, 0.329415f, 0.229739f, 0.322361f, 0.545706f,
+                                   0.127568f, 0.566949f, 0.550556f, 0.369214f, 0.788888f, 0.382914f,
+                                   0.993248f, 0.906157f, 0.143936f};
+
+__device__ __forceinline__ void viridis(float t, float o[3]) {
+    t = fminf(fmaxf(t, 0.0f), 1.0f);
+    const float a = t * 4.0f;
+    const int i = (int)floorf(a);
+    const float f = a - (float)i;
+    const int s = i <= 0 ? 0 : (i >= 3 ? 3 : i);
+    for (int k = 0; k < 3; ++k) o[k] = kViridis[3 * s + k] + (kViridis[3 * s + 3 + k] - kViridis[3 * s + k]) * f;
+}
+
+__global__ __launch_bounds__(kOutBlock) void k_view(const float* __restrict__ gbuf, const float* __restrict__ stats,
+                                                    uint32_t* __restrict__ ldr, size_t npix, uint32_t view) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npix) return;
+    const float* g = gbuf ? gbuf + (size_t)AR_GBUFFER_FLOATS * i : nullptr;
+    const float* st = stats ? stats + (size_t)AR_STATS_FLOATS * i : nullptr;
+    float v[3] = {0.0f, 0.0f, 0.0f};
+    switch (view) {
+    case AR_VIEW_ALBEDO: v[0] = g[0]; v[1] = g[1]; v[2] = g[2]; break;
+    case AR_VIEW_NORMAL:
+        for (int k = 0; k < 3; ++k) v[k] = g[3 + k] * 0.5f + 0.5f;
+        break;
+    case AR_VIEW_DEPTH: viridis(1.0f / (g[6] + 1.0f), v); break;
+    case AR_VIEW_METALLIC: viridis(g[7], v); break;
+    case AR_VIEW_ROUGHNESS: viridis(g[8], v); break;
+    case AR_VIEW_STAT_COUNT: v[0] = (float)log((double)__float_as_uint(st[0]) + 1.0) * 0.01f; break;
+    default: {   // AR_VIEW_STAT_MEAN / _M2 / _M3: |moment| / 4
+        const int at = 1 + 3 * (int)(view - AR_VIEW_STAT_MEAN);
+        for (int k = 0; k < 3; ++k) v[k] = fabsf(st[at + k]) * 0.25f;
+    }
+    }
+    const uint32_t r = (uint32_t)(255.0f * sat(v[0]));
+    const uint32_t gg = (uint32_t)(255.0f * sat(v[1]));
+    const uint32_t b = (uint32_t)(255.0f * sat(v[2]));
+    ldr[i] = 0xff000000u | (r << 16) | (gg << 8) | b;
+}
+
 // first stage: each block sums a grid-stride share in a fixed order
 __global__ __launch_bounds__(kOutBlock) void k_lum_mse_partial(const float* __restrict__ a, const float* __restrict__ b,
                                                                size_t npix, double* __restrict__ part) {
@@ -154,6 +204,77 @@ int ar_tonemap(ar_ctx* ctx, const float* hdr, uint32_t* ldr, uint32_t width, uin
     hipLaunchKernelGGL(k_tonemap, dim3((uint32_t)((npix + kOutBlock - 1) / kOutBlock)), dim3(kOutBlock), 0,
                        (hipStream_t)stream, hdr, ldr, npix, exposure, tone, xyz_input ? 1u : 0u);
     return check_hip(ctx, hipGetLastError(), "tonemap launch");
+}
+
+int ar_present(ar_ctx* ctx, const float* hdr, const float* gbuffer, const float* stats, uint32_t* ldr, uint32_t width,
+               uint32_t height, float exposure, uint32_t tone, uint32_t xyz_input, uint32_t view, void* stream) {
+    if (!ctx) return AR_E_INVALID;
+    if (!ldr) return set_error(ctx, AR_E_INVALID, "NULL ldr");
+    if (tone > AR_TONE_ACES) return set_error(ctx, AR_E_INVALID, "unknown tone curve");
+    if (view > AR_VIEW_STAT_M3) return set_error(ctx, AR_E_INVALID, "unknown view");
+    if (view <= AR_VIEW_DENOISED && !hdr) return set_error(ctx, AR_E_INVALID, "NULL hdr");
+    if ((view == AR_VIEW_DENOISED || (view >= AR_VIEW_ALBEDO && view <= AR_VIEW_ROUGHNESS)) && !gbuffer)
+        return set_error(ctx, AR_E_INVALID, "this view needs the G-buffer");
+    if ((view == AR_VIEW_DENOISED || view >= AR_VIEW_STAT_COUNT) && !stats)
+        return set_error(ctx, AR_E_INVALID, "this view needs the sample statistics");
+    const size_t npix = (size_t)width * height;
+    if (npix == 0) return AR_OK;
+    (void)hipSetDevice(ctx->device);
+    const dim3 grid((uint32_t)((npix + kOutBlock - 1) / kOutBlock));
+    if (view >= AR_VIEW_ALBEDO) {
+        hipLaunchKernelGGL(k_view, grid, dim3(kOutBlock), 0, (hipStream_t)stream, gbuffer, stats, ldr, npix, view);
+        return check_hip(ctx, hipGetLastError(), "view launch");
+    }
+    const float* src = hdr;
+    if (view == AR_VIEW_DENOISED) {
+        if (ctx->present_pixels < npix) {
+            invalidate_graphs(ctx);
+            if (ctx->present_buf) (void)hipFree(ctx->present_buf);
+            ctx->present_buf = nullptr;
+            ctx->present_pixels = 0;
+            const int rc = check_hip(ctx, hipMalloc((void**)&ctx->present_buf, npix * 3 * sizeof(float)),
+                                     "hipMalloc(denoised image)");
+            if (rc != AR_OK) return rc;
+            ctx->present_pixels = npix;
+        }
+        const int rc = ar_denoise(ctx, hdr, gbuffer, stats, ctx->present_buf, width, height, stream);
+        if (rc != AR_OK) return rc;
+        src = ctx->present_buf;
+    }
+    hipLaunchKernelGGL(k_tonemap, grid, dim3(kOutBlock), 0, (hipStream_t)stream, src, ldr, npix, exposure, tone,
+                       xyz_input ? 1u : 0u);
+    return check_hip(ctx, hipGetLastError(), "tonemap launch");
+}
+
+int ar_render_host(ar_ctx* ctx, const ar_camera* cam, const ar_render_opts* opts, float* hdr, uint32_t* ldr,
+                   float exposure, uint32_t tone) {
+    if (!ctx) return AR_E_INVALID;
+    if (!cam || !opts || (!hdr && !ldr)) return set_error(ctx, AR_E_INVALID, "NULL camera, options or outputs");
+    if (tone > AR_TONE_ACES) return set_error(ctx, AR_E_INVALID, "unknown tone curve");
+    if (opts->width == 0 || opts->height == 0) return set_error(ctx, AR_E_INVALID, "empty image");
+    const size_t npix = (size_t)opts->width * opts->height;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->host_out_pixels < npix) {
+        invalidate_graphs(ctx);
+        if (ctx->host_out) (void)hipFree(ctx->host_out);
+        ctx->host_out = nullptr;
+        ctx->host_out_pixels = 0;
+        const int rc = check_hip(ctx, hipMalloc((void**)&ctx->host_out, npix * 4 * sizeof(float)), "hipMalloc(render target)");
+        if (rc != AR_OK) return rc;
+        ctx->host_out_pixels = npix;
+    }
+    float* d_hdr = ctx->host_out;
+    uint32_t* d_ldr = reinterpret_cast<uint32_t*>(ctx->host_out + 3 * npix);
+    int rc = ar_render(ctx, cam, opts, d_hdr, ctx->stream);
+    if (rc == AR_OK && ldr)
+        rc = ar_tonemap(ctx, d_hdr, d_ldr, opts->width, opts->height, exposure, tone,
+                        opts->spectral != AR_SPECTRAL_RGB && opts->output == AR_OUTPUT_XYZ, ctx->stream);
+    if (rc == AR_OK) rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "render");
+    if (rc == AR_OK && hdr)
+        rc = check_hip(ctx, hipMemcpy(hdr, d_hdr, npix * 3 * sizeof(float), hipMemcpyDeviceToHost), "copy render");
+    if (rc == AR_OK && ldr)
+        rc = check_hip(ctx, hipMemcpy(ldr, d_ldr, npix * sizeof(uint32_t), hipMemcpyDeviceToHost), "copy image");
+    return rc;
 }
 
 int ar_luminance_mse(ar_ctx* ctx, const float* a, const float* b, size_t npix, double* out, void* stream) {

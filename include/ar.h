@@ -138,7 +138,7 @@ typedef struct {
 } ar_hits;
 
 typedef struct {
-    uint32_t width;      /* branching factor: 2 (64 B nodes) or 8 (quantised 96 B nodes, default) */
+    uint32_t width;      /* branching factor: 2 (64 B nodes) or 8 (quantised 128 B nodes, default) */
     uint32_t leaf_size;  /* max triangles per leaf, 1..15 (default 3) */
     uint32_t bin_count;  /* SAH bins, 4..64 */
     uint32_t max_depth;  /* tree depth limit, <= 64 */
@@ -256,8 +256,9 @@ int ar_make_shadow_rays(ar_ctx* ctx, const ar_rays* rays, const ar_hits* hits, s
  * Camera: the reference convention (core/scene.cu:145-164): right =
  * normalize(dir x +Y), up = normalize(dir x right), pixel (i, j) jittered
  * and mapped to ndc ((u*2-1) * tan(fov/2) * aspect, ((1-v)*2-1) * tan(fov/2)).
- * Row j = 0 therefore looks along -up; the reference's BMP writer stores it
- * as the bottom row.
+ * Row j = 0 therefore looks along +up = normalize(dir x right), which is
+ * world -Y for a level camera; the reference's BMP writer (negative height:
+ * top-down) stores it as the top row, and so does ar_write_bmp.
  *
  * Random numbers: Philox4x32-10, key = seed, counter = (pixel, sample,
  * bounce, group), so a render is a pure function of its options and
@@ -389,6 +390,16 @@ typedef struct {
 int ar_autotune_batch(ar_ctx* const* scenes, uint32_t n, const ar_camera* cams, const ar_render_opts* opts,
                       uint32_t probe_spp, ar_tune_result* out);
 
+/* Replaces the offline SceneRenderer::Render(size, origin, dir, spp)
+ * (core/scene.cu:215-237), which allocates its own target and returns a host
+ * vector: renders into a device image the context owns and copies it to host
+ * memory (waits).  hdr (width*height*3 floats, host, or NULL) receives the
+ * image as ar_render defines it; ldr (width*height words, host, or NULL) the
+ * ar_tonemap of it with `exposure` and `tone` (spectral XYZ output is
+ * converted to sRGB first). */
+int ar_render_host(ar_ctx* ctx, const ar_camera* cam, const ar_render_opts* opts, float* hdr, uint32_t* ldr,
+                   float exposure, uint32_t tone);
+
 /* --- progressive accumulation and output --------------------------------
  * ar_render_progressive replaces the interactive SceneRenderer::Render
  * (core/scene.cu:240-290 with RenderCamera's running mean and odd/even
@@ -401,7 +412,9 @@ int ar_autotune_batch(ar_ctx* const* scenes, uint32_t n, const ar_camera* cams, 
  * device, in the render's output space (opts->output).  Spectral renders
  * accumulate CIE XYZ (AR_OUTPUT_XYZ) and convert in ar_tonemap (xyz_input),
  * as the reference does: AR_OUTPUT_SRGB clamps each frame's mean at zero
- * before the fold. */
+ * before the fold.  Guided frames (opts->guiding) update the guide after
+ * every frame_spp samples, as SceneRenderer::Render does after each frame
+ * when bUsePathGuiding_ is set (core/scene.cu:240-290). */
 int ar_render_progressive(ar_ctx* ctx, const ar_camera* cam, const ar_render_opts* opts, uint32_t frame,
                           float* accum, float* odd, float* even, float* gbuffer, float* stats, void* stream);
 
@@ -439,6 +452,34 @@ int ar_denoise(ar_ctx* ctx, const float* hdr, const float* gbuffer, const float*
  * per pixel into ldr (device). */
 int ar_tonemap(ar_ctx* ctx, const float* hdr, uint32_t* ldr, uint32_t width, uint32_t height, float exposure,
                uint32_t tone, uint32_t xyz_input, void* stream);
+
+/* The output stage of the interactive SceneRenderer::Render (core/scene.cu:
+ * 291-365): ldr (width*height words, device) receives one of
+ *   AR_VIEW_IMAGE      hdr through exposure and the tone curve (as ar_tonemap)
+ *   AR_VIEW_DENOISED   ar_denoise of hdr (into a device image the context
+ *                      owns), then the tone curve (sceneSetting_.bDenoise_)
+ *   AR_VIEW_ALBEDO .. AR_VIEW_STAT_M3   the debug views of the Tonemap kernel
+ *                      (bDebugAlbedo_ .. bDebugStatisticsM3_, core/scene.cu:
+ *                      35-107) over the G-buffer / statistics records kept by
+ *                      ar_render_progressive: albedo, normal * 0.5 + 0.5, the
+ *                      viridis heat map of 1 / (depth + 1), of metallic and of
+ *                      roughness (core/render/color.cuh:99-140), log(n + 1) / 100
+ *                      in red, and |mean|, |M2/n|, |M3/n| / 4.
+ * Each view reads only what it needs (NULL elsewhere is fine).  Queued on
+ * `stream`; the call does not wait. */
+#define AR_VIEW_IMAGE 0u
+#define AR_VIEW_DENOISED 1u
+#define AR_VIEW_ALBEDO 2u
+#define AR_VIEW_NORMAL 3u
+#define AR_VIEW_DEPTH 4u
+#define AR_VIEW_METALLIC 5u
+#define AR_VIEW_ROUGHNESS 6u
+#define AR_VIEW_STAT_COUNT 7u
+#define AR_VIEW_STAT_MEAN 8u
+#define AR_VIEW_STAT_M2 9u
+#define AR_VIEW_STAT_M3 10u
+int ar_present(ar_ctx* ctx, const float* hdr, const float* gbuffer, const float* stats, uint32_t* ldr, uint32_t width,
+               uint32_t height, float exposure, uint32_t tone, uint32_t xyz_input, uint32_t view, void* stream);
 
 /* DeviceMSELuminance (core/helper/measure.cuh:49-74): mean over npix pixels
  * of (lum(a) - lum(b))^2 with Rec.709 luminance; the reference's displayed

@@ -420,6 +420,12 @@ void orc_tonemap(const float* hdr, uint32_t* ldr, uint32_t npix, float exposure,
 double orc_luminance_mse(const float* a, const float* b, size_t npix);
 /* running mean: acc (mean of n_before samples) takes in the mean of n_new more */
 void orc_running_mean(float* acc, const float* mean_new, size_t nfloat, uint32_t n_before, uint32_t n_new);
+/* HeatMapViridis (core/render/color.cuh:99-140) */
+void orc_viridis(float t, float out[3]);
+/* debug views 2..10 of ar_present (AR_VIEW_ALBEDO .. AR_VIEW_STAT_M3) over
+ * G-buffer (16 floats) / statistics (12 floats) records, packed as
+ * orc_tonemap packs */
+void orc_view(const float* gbuf, const float* stats, uint32_t* ldr, uint32_t npix, int view);
 
 /* ---------------- denoiser (denoise.c) ---------------- */
 

@@ -11,14 +11,19 @@
  *   orc_running_mean     the lerp(target, result, 1 / (1 + frame)) running mean
  *                        of RenderCamera (core/scene.cu:176-189) folded over k
  *                        samples at once
+ *   orc_viridis,         the Tonemap kernel's debug views (core/scene.cu:35-107)
+ *   orc_view             with HeatMapViridis (core/render/color.cuh:99-140)
  *
  * Pins: tests/test_oracle_output.py (gamma closed form, ACES properties,
  * quantisation and packing, the MSE against numpy, the running mean against
- * the direct mean).
+ * the direct mean; the views: knots and midpoints of the heat map, t = 1
+ * landing on the 3/4 sample as the reference's segment search does, each view's
+ * channel formula on hand-made records).
  */
 #include "oracle.h"
 
 #include <math.h>
+#include <string.h>
 
 static float sat(float x) { return x < 0.0f ? 0.0f : (x > 1.0f ? 1.0f : x); }
 
@@ -121,4 +126,58 @@ void orc_running_mean(float* acc, const float* mean_new, size_t nfloat, uint32_t
     const float total = (float)n_before + (float)n_new;
     const float wa = (float)n_before / total, wb = (float)n_new / total;
     for (size_t i = 0; i < nfloat; ++i) acc[i] = n_before == 0 ? mean_new[i] : acc[i] * wa + mean_new[i] * wb;
+}
+
+/* matplotlib's viridis at t = 0, 1/4, 1/2, 3/4, 1 */
+static const float viridis_knots[15] = {
+    0.267004f, 0.004874f, 0.329415f, 0.229739f, 0.322361f, 0.545706f, 0.127568f, 0.566949f, 0.550556f,
+    0.369214f, 0.788888f, 0.382914f, 0.993248f, 0.906157f, 0.143936f,
+};
+
+/* Linear between the knots; segment floor(4t) clamped to 0..3 with the
+ * weight 4t - floor(4t), so t = 1 gives the 3/4 knot (the reference's search
+ * ends on the last segment with weight 0). */
+void orc_viridis(float t, float out[3])
+{
+    t = fminf(fmaxf(t, 0.0f), 1.0f);
+    const float a = t * 4.0f;
+    const int i = (int)floorf(a);
+    const float f = a - (float)i;
+    const int s = i <= 0 ? 0 : (i >= 3 ? 3 : i);
+    for (int k = 0; k < 3; ++k)
+        out[k] = viridis_knots[3 * s + k] + (viridis_knots[3 * s + 3 + k] - viridis_knots[3 * s + k]) * f;
+}
+
+static uint32_t pack(const float v[3])
+{
+    const uint32_t r = (uint32_t)(255.0f * sat(v[0]));
+    const uint32_t g = (uint32_t)(255.0f * sat(v[1]));
+    const uint32_t b = (uint32_t)(255.0f * sat(v[2]));
+    return 0xff000000u | (r << 16) | (g << 8) | b;
+}
+
+void orc_view(const float* gbuf, const float* stats, uint32_t* ldr, uint32_t npix, int view)
+{
+    for (uint32_t i = 0; i < npix; ++i) {
+        const float* g = gbuf ? gbuf + 16 * (size_t)i : NULL;
+        const float* st = stats ? stats + 12 * (size_t)i : NULL;
+        float v[3] = { 0.0f, 0.0f, 0.0f };
+        if (view == 2) {
+            for (int k = 0; k < 3; ++k) v[k] = g[k];
+        } else if (view == 3) {
+            for (int k = 0; k < 3; ++k) v[k] = g[3 + k] * 0.5f + 0.5f;
+        } else if (view == 4) {
+            orc_viridis(1.0f / (g[6] + 1.0f), v);
+        } else if (view == 5 || view == 6) {
+            orc_viridis(g[view + 2], v);
+        } else if (view == 7) {
+            uint32_t n;
+            memcpy(&n, st, 4);
+            v[0] = (float)log((double)n + 1.0) * 0.01f;
+        } else {
+            const int at = 1 + 3 * (view - 8);
+            for (int k = 0; k < 3; ++k) v[k] = fabsf(st[at + k]) * 0.25f;
+        }
+        ldr[i] = pack(v);
+    }
 }

@@ -566,6 +566,47 @@ def t_quantile(nu):
     return fn("orc_t_quantile", C.c_float, [C.c_int])(int(nu))
 
 
+def tonemap(hdr, exposure=1.0, tone=0, xyz=False):
+    """orc_tonemap of (..., 3) float32 hdr -> uint32 0xAARRGGBB of shape hdr.shape[:-1]."""
+    hdr = np.ascontiguousarray(hdr, np.float32)
+    out = np.zeros(hdr.shape[:-1], np.uint32)
+    fn("orc_tonemap", None, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_float, C.c_int, C.c_int])(
+        hdr.ctypes.data, out.ctypes.data, out.size, exposure, tone, int(bool(xyz)))
+    return out
+
+
+def running_mean(acc, new, n_before, n_new):
+    """orc_running_mean in place on float32 acc (the mean of n_before samples)."""
+    assert acc.dtype == np.float32 and acc.flags["C_CONTIGUOUS"]
+    new = np.ascontiguousarray(new, np.float32)
+    fn("orc_running_mean", None, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_uint32, C.c_uint32])(
+        acc.ctypes.data, new.ctypes.data, acc.size, n_before, n_new)
+    return acc
+
+
+def luminance_mse(a, b):
+    a = np.ascontiguousarray(a, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    return fn("orc_luminance_mse", C.c_double, [C.c_void_p, C.c_void_p, C.c_size_t])(
+        a.ctypes.data, b.ctypes.data, a.size // 3)
+
+
+def viridis(t):
+    out = np.zeros(3, np.float32)
+    fn("orc_viridis", None, [C.c_float, C.c_void_p])(float(t), out.ctypes.data)
+    return out
+
+
+def view(gbuf, stats, view_id, npix):
+    """orc_view: debug view 2..10 of (npix, 16) G-buffer / (npix, 12) statistics records."""
+    g = None if gbuf is None else np.ascontiguousarray(gbuf, np.float32)
+    st = None if stats is None else np.ascontiguousarray(stats, np.float32)
+    out = np.zeros(npix, np.uint32)
+    fn("orc_view", None, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_int])(
+        None if g is None else g.ctypes.data, None if st is None else st.ctypes.data, out.ctypes.data, npix, view_id)
+    return out
+
+
 def bilateral(hdr, gbuf, stats):
     """orc_bilateral of (h, w, 3) hdr, (h, w, 16) G-buffer and (h, w, 12) statistics."""
     h, w = hdr.shape[:2]

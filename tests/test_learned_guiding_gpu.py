@@ -5,9 +5,10 @@ online on each frame's path vertices and sampled in the BSDF mixture.
   mean (per-render spread ~0.1 %, bar 0.5 %).
 * Gain: on the occluded-light room (light only through a door gap) the
   guided render's relMSE against a 8192-spp reference is below 0.8x the plain
-  render's at equal spp, averaged over two seeds (measured 0.63x; training
-  reduces gradients in a fixed order, so the trained network and the ratio
-  are reproducible).
+  render's at equal spp, averaged over two seeds.
+* Reproducible: a frame's training records are numbered in path order (no
+  atomics) and gradients are reduced in a fixed order, so two guided renders
+  with the same seeds give the same image and network, bit for bit.
 """
 import numpy as np
 import pytest
@@ -68,3 +69,8 @@ def test_learned_guiding_reduces_error_on_occluded_light(ctx):
     assert rl < 0.8 * rp
     params = ctx.mlp_params()
     assert np.isfinite(params).all()
+    ctx.mlp_configure(learning_rate=5e-3, iters_per_frame=4, seed=6)
+    again = ctx.render(96, 96, 64, seed=6, guiding=2, **kw)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(again.cpu().numpy().view(np.uint32), learned.cpu().numpy().view(np.uint32))
+    np.testing.assert_array_equal(ctx.mlp_params().view(np.uint32), params.view(np.uint32))

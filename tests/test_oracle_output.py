@@ -103,3 +103,52 @@ def test_bmp_writer_matches_the_format(tmp_path):
     np.testing.assert_array_equal(pix[..., 2], (px >> 16) & 255)     # R
     np.testing.assert_array_equal(pix[..., 1], (px >> 8) & 255)      # G
     np.testing.assert_array_equal(pix[..., 0], px & 255)             # B
+
+
+# matplotlib's viridis table (matplotlib/_cm_listed.py, _viridis_data), entries 0, 64, 128, 191 and 255
+VIRIDIS = np.array([[0.267004, 0.004874, 0.329415], [0.229739, 0.322361, 0.545706], [0.127568, 0.566949, 0.550556],
+                    [0.369214, 0.788888, 0.382914], [0.993248, 0.906157, 0.143936]], np.float32)
+
+
+def test_viridis_knots_midpoints_and_end():
+    for k in range(4):
+        np.testing.assert_allclose(orc.viridis(k / 4), VIRIDIS[k], rtol=0, atol=1e-7)
+        np.testing.assert_allclose(orc.viridis(k / 4 + 0.125), 0.5 * (VIRIDIS[k] + VIRIDIS[k + 1]), atol=2e-7)
+    np.testing.assert_allclose(orc.viridis(0.9999), VIRIDIS[4], atol=5e-4)
+    # t = 1 lands on the last segment with weight 0: the 3/4 entry, as the reference's search does
+    np.testing.assert_allclose(orc.viridis(1.0), VIRIDIS[3], atol=1e-7)
+    np.testing.assert_allclose(orc.viridis(-3.0), VIRIDIS[0], atol=1e-7)
+
+
+def _px(v):
+    v = np.clip(np.asarray(v, np.float32), 0, 1)
+    q = np.floor(np.float32(255.0) * v).astype(np.uint32)
+    return 0xFF000000 | (q[0] << 16) | (q[1] << 8) | q[2]
+
+
+def test_debug_views_on_hand_made_records():
+    g = np.zeros((3, 16), np.float32)
+    g[0, 0:3] = (0.5, 1.2, -0.1)
+    g[0, 3:6] = (-1.0, 0.0, 1.0)
+    g[0, 6:9] = (0.0, 0.25, 1.0)
+    g[1, 0:3] = (0.2, 0.4, 0.6)
+    g[1, 3:6] = (0.6, -0.8, 0.0)
+    g[1, 6:9] = (3.0, 0.6, 0.05)
+    st = np.zeros((3, 12), np.float32)
+    st[0, 0] = np.array([100], np.uint32).view(np.float32)[0]
+    st[0, 1:10] = (-2.0, 0.1, 8.0, 0.4, -0.4, 0.0, 1.0, 2.0, -3.0)
+    st[1, 0] = np.array([0], np.uint32).view(np.float32)[0]
+    for i in range(2):
+        assert orc.view(g, st, 2, 3)[i] == _px(g[i, 0:3])
+        assert orc.view(g, st, 3, 3)[i] == _px(g[i, 3:6] * np.float32(0.5) + np.float32(0.5))
+        assert orc.view(g, st, 4, 3)[i] == _px(orc.viridis(np.float32(1.0) / (g[i, 6] + np.float32(1.0))))
+        assert orc.view(g, st, 5, 3)[i] == _px(orc.viridis(g[i, 7]))
+        assert orc.view(g, st, 6, 3)[i] == _px(orc.viridis(g[i, 8]))
+    # sample count: log(n + 1) / 100 in red only
+    n0 = orc.view(g, st, 7, 3)
+    assert n0[0] == 0xFF000000 | (int(np.floor(255.0 * np.float32(np.log(101.0) * 0.01))) << 16)
+    assert n0[1] == 0xFF000000
+    # moments: |x| / 4
+    assert orc.view(g, st, 8, 3)[0] == _px([0.5, 0.025, 1.0])
+    assert orc.view(g, st, 9, 3)[0] == _px([0.1, 0.1, 0.0])
+    assert orc.view(g, st, 10, 3)[0] == _px([0.25, 0.5, 0.75])
