@@ -722,4 +722,4 @@ def autotune(scenes, views, probe_spp=1, **shared) -> dict:
         msg = lib.ar_last_error(scenes[0]._h)
         raise ArError(rc, f"ar_autotune_batch: {msg.decode() if msg else ''}")
     return {"width": res.width, "tile": res.tile,
-            "ms": {f"w{w}_t{t}": res.ms[i][j] for i, w in enumerate((2, 8)) for j, t in enumerate((64, 128, 256, 512))}}
+            "ms": {f"w{w}_t{t}": res.ms[i][j] for i, w in enumerate((2, 4, 8)) for j, t in enumerate((64, 128, 256, 512))}}

@@ -104,7 +104,7 @@ MLP_PARAMS = 16 * 64 + 64 + 64 * 64 + 64 + 64 * 64 + 64 + 64 * 48 + 48
 
 
 class ar_tune_result(C.Structure):
-    _fields_ = [("width", C.c_uint32), ("tile", C.c_uint32), ("ms", (C.c_float * 4) * 2)]
+    _fields_ = [("width", C.c_uint32), ("tile", C.c_uint32), ("ms", (C.c_float * 4) * 3)]
 
 
 class ar_smlp_train_opts(C.Structure):

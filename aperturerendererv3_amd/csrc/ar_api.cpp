@@ -352,7 +352,7 @@ int ar_bvh_build(ar_ctx* ctx, const ar_bvh_opts* opts) {
     if (!ctx->scene_set) return set_error(ctx, AR_E_NOSCENE, "no scene: call ar_scene_set first");
     ar_bvh_opts o{8, 3, 16, (uint32_t)kMaxDepth};
     if (opts) o = *opts;
-    if (o.width != 2 && o.width != 8) return set_error(ctx, AR_E_UNSUPPORTED, "BVH width must be 2 or 8");
+    if (o.width != 2 && o.width != 4 && o.width != 8) return set_error(ctx, AR_E_UNSUPPORTED, "BVH width must be 2, 4 or 8");
     if (o.leaf_size < 1 || o.leaf_size > kMaxLeafTris) return set_error(ctx, AR_E_INVALID, "leaf_size must be 1..15");
     if (o.bin_count < 4 || o.bin_count > 64) return set_error(ctx, AR_E_INVALID, "bin_count must be 4..64");
     if (o.max_depth == 0 || o.max_depth > (uint32_t)kMaxDepth) o.max_depth = kMaxDepth;
@@ -399,6 +399,7 @@ int ar_bvh_build(ar_ctx* ctx, const ar_bvh_opts* opts) {
         return set_error(ctx, AR_E_CAPACITY, "BVH deeper than the traversal stack allows");
     const float pad = mag * kPadRel + 1e-30f;
     std::vector<GpuNode8> nodes8;
+    std::vector<GpuNode4> nodes4;
     std::vector<float> tris;
     size_t node_bytes = 0;
     uint64_t leaves = 0;
@@ -412,6 +413,12 @@ int ar_bvh_build(ar_ctx* ctx, const ar_bvh_opts* opts) {
         node_bytes = nodes8.size() * sizeof(GpuNode8);
         ctx->bvh.stack_entries = 7u * (uint32_t)(depth + 1) + 2u;
         ctx->info.node_count = nodes8.size();
+    } else if (o.width == 4) {
+        convert_bvh4(br, lw, pad, nodes4, tris, leaves, depth);
+        node_src = nodes4.data();
+        node_bytes = nodes4.size() * sizeof(GpuNode4);
+        ctx->bvh.stack_entries = 3u * (uint32_t)(depth + 1) + 2u;
+        ctx->info.node_count = nodes4.size();
     } else {
         cv.nodes.reserve(br.nodes.size() / 2 + 1);
         cv.tris.reserve(12 * (size_t)n);

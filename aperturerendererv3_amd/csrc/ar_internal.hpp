@@ -62,9 +62,9 @@ struct GpuNode2 {
 static_assert(sizeof(GpuNode2) == 64, "GpuNode2 must be 64 bytes");
 
 // ---------------------------------------------------------------------------
-// Device BVH8 layout (quantised 8-wide node, 96 bytes = 6 x 16 B).
+// Device BVH8 layout (quantised 8-wide node, 96 bytes = 6 x 16 B of data, padded to 128 B).
 //
-//   float4 h  = (p.x, p.y, p.z, bits: e.x | e.y << 8 | e.z << 16 | nchild << 24)
+//   float4 h  = (p.x, p.y, p.z, bits: e.x | e.y << 8 | e.z << 16 | nchild << 24 | axis << 28)
 //   uint4 r0  = ref[0..3],  uint4 r1 = ref[4..7]   (same encoding as BVH2 refs)
 //   uint4 q0  = lo.x[0..7] bytes, lo.y[0..7] bytes
 //   uint4 q1  = lo.z[0..7] bytes, hi.x[0..7] bytes
@@ -83,6 +83,23 @@ struct GpuNode8 {
     uint32_t pad[8];     // to 128 B: one node is one L2 line
 };
 static_assert(sizeof(GpuNode8) == 128, "GpuNode8 must be 128 bytes");
+
+// Quantised 4-wide node, 64 bytes = 4 x 16 B (half an L2 line):
+//
+//   float4 h  = (p.x, p.y, p.z, bits: e.x | e.y << 8 | e.z << 16 | nchild << 24 | axis << 28)
+//   uint4 r   = ref[0..3]
+//   uint4 q0  = lo.x[0..3] bytes, lo.y[0..3], lo.z[0..3], hi.x[0..3]
+//   uint4 q1  = hi.y[0..3] bytes, hi.z[0..3], 0, 0
+//
+// with the 8-wide layout's grid, rounding and child order.
+struct GpuNode4 {
+    float p[3];
+    uint32_t meta;
+    uint32_t ref[4];
+    uint8_t q[24];
+    uint32_t pad[2];
+};
+static_assert(sizeof(GpuNode4) == 64, "GpuNode4 must be 64 bytes");
 
 // Host-side binary tree produced by the builder (pre-order, left = parent+1).
 struct BuildNode {
@@ -113,8 +130,10 @@ struct LeafWriter {
     void write(std::vector<float>& slots, uint32_t prim);
 };
 
-// Collapse into quantised 8-wide nodes (bvh_wide.cpp); depth = wide-tree depth.
+// Collapse into quantised 8- / 4-wide nodes (bvh_wide.cpp); depth = wide-tree depth.
 bool convert_bvh8(const BuildResult& br, LeafWriter& lw, float pad, std::vector<GpuNode8>& nodes,
+                  std::vector<float>& tris, uint64_t& leaves, int& depth);
+bool convert_bvh4(const BuildResult& br, LeafWriter& lw, float pad, std::vector<GpuNode4>& nodes,
                   std::vector<float>& tris, uint64_t& leaves, int& depth);
 
 struct DeviceBVH {

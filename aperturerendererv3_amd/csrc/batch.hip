@@ -32,6 +32,13 @@ __global__ void k_rebase_nodes2(GpuNode2* n, uint32_t count, uint32_t node_off, 
     n[i].d[1] = rebase_ref(n[i].d[1], node_off, slot_off);
 }
 
+__global__ void k_rebase_nodes4(GpuNode4* n, uint32_t count, uint32_t node_off, uint32_t slot_off) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) n[i].ref[k] = rebase_ref(n[i].ref[k], node_off, slot_off);
+}
+
 __global__ void k_rebase_nodes8(GpuNode8* n, uint32_t count, uint32_t node_off, uint32_t slot_off) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= count) return;
@@ -95,7 +102,7 @@ int batch_merge(ar_ctx* out, ar_ctx* const* scenes, uint32_t n) {
     }
     if (slots >= (1u << 27) || nodes >= (1u << 31) || tris >= (1u << 31))
         return set_error(out, AR_E_CAPACITY, "batch: more than 2^27 triangle slots");
-    const size_t nb = width == 8 ? sizeof(GpuNode8) : sizeof(GpuNode2);
+    const size_t nb = width == 8 ? sizeof(GpuNode8) : width == 4 ? sizeof(GpuNode4) : sizeof(GpuNode2);
     char* dnodes = nullptr;
     float4* dtris = nullptr;
     float4* dprims = nullptr;
@@ -139,6 +146,9 @@ int batch_merge(ar_ctx* out, ar_ctx* const* scenes, uint32_t n) {
             if (width == 8)
                 hipLaunchKernelGGL(k_rebase_nodes8, dim3(blocks(cn)), dim3(256), 0, nullptr,
                                    (GpuNode8*)(dnodes + on * nb), cn, (uint32_t)on, (uint32_t)os);
+            else if (width == 4)
+                hipLaunchKernelGGL(k_rebase_nodes4, dim3(blocks(cn)), dim3(256), 0, nullptr,
+                                   (GpuNode4*)(dnodes + on * nb), cn, (uint32_t)on, (uint32_t)os);
             else
                 hipLaunchKernelGGL(k_rebase_nodes2, dim3(blocks(cn)), dim3(256), 0, nullptr,
                                    (GpuNode2*)(dnodes + on * nb), cn, (uint32_t)on, (uint32_t)os);
@@ -311,13 +321,13 @@ int ar_autotune_batch(ar_ctx* const* scenes, uint32_t n, const ar_camera* cams, 
     for (uint32_t i = 0; i < n && rc == AR_OK; ++i)
         rc = check_hip(c0, hipMalloc((void**)&outs[i], (size_t)po[i].width * po[i].height * 3 * sizeof(float)),
                        "autotune outputs");
-    const uint32_t widths[2] = {2, 8};
+    const uint32_t widths[3] = {2, 4, 8};
     const uint32_t tiles[4] = {64, 128, 256, 512};
     float best = INFINITY;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (rc == AR_OK) rc = check_hip(c0, hipEventCreate(&e0), "event");
     if (rc == AR_OK) rc = check_hip(c0, hipEventCreate(&e1), "event");
-    for (int wi = 0; wi < 2 && rc == AR_OK; ++wi) {
+    for (int wi = 0; wi < 3 && rc == AR_OK; ++wi) {
         for (uint32_t i = 0; i < n && rc == AR_OK; ++i) {
             ar_bvh_opts bo{widths[wi], 3, 12, 0};
             rc = ar_bvh_build(scenes[i], &bo);

@@ -1,8 +1,9 @@
-// Collapse of the binary SAH tree into the quantised 8-wide device layout.
+// Collapse of the binary SAH tree into the quantised 4- or 8-wide device
+// layouts.
 //
 // Each wide node takes a binary node's two children and repeatedly opens the
 // interior child of largest surface area (its two children replace it) until
-// eight children or only leaves remain, as in the wide-BVH construction of
+// four / eight children or only leaves remain, as in the wide-BVH construction of
 // Wald et al. (RT'08) and Ylitie et al. (HPG'17).  Child boxes are padded
 // by the scene pad, then quantised outward to 8 bits on a per-node grid
 // p + q * 2^e; every decoded bound is checked in float against the padded
@@ -23,11 +24,24 @@ float area_of(const BuildNode& n) {
     return 2.0f * (dx * dy + dy * dz + dz * dx);
 }
 
+// quantised child bounds: lo.x..lo.z then hi.x..hi.z, K bytes each
+template <int K> struct Quant;
+template <> struct Quant<8> {
+    using Node = GpuNode8;
+    static void set(Node& g, int a, int k, uint8_t lo, uint8_t hi) { g.q[8 * a + k] = lo; g.q[24 + 8 * a + k] = hi; }
+};
+template <> struct Quant<4> {
+    using Node = GpuNode4;
+    static void set(Node& g, int a, int k, uint8_t lo, uint8_t hi) { g.q[4 * a + k] = lo; g.q[12 + 4 * a + k] = hi; }
+};
+
+template <int K>
 struct Collapser {
+    using Node = typename Quant<K>::Node;
     const BuildResult& br;
     LeafWriter& lw;
     float pad;
-    std::vector<GpuNode8> nodes;
+    std::vector<Node> nodes;
     std::vector<float> tris;
     uint64_t leaves = 0;
     int max_depth = 0;
@@ -76,7 +90,7 @@ struct Collapser {
         } else {
             ch.push_back(n.left);
             ch.push_back(n.right);
-            while (ch.size() < 8) {
+            while (ch.size() < (size_t)K) {
                 int best = -1;
                 float best_a = -1.0f;
                 for (size_t k = 0; k < ch.size(); ++k) {
@@ -109,8 +123,8 @@ struct Collapser {
         }
         const uint32_t gi = (uint32_t)nodes.size();
         nodes.emplace_back();
-        uint32_t refs[8];
-        for (size_t k = 0; k < 8; ++k) refs[k] = kRefEmpty;
+        uint32_t refs[K];
+        for (size_t k = 0; k < (size_t)K; ++k) refs[k] = kRefEmpty;
         for (size_t k = 0; k < ch.size(); ++k) {
             const BuildNode& c = br.nodes[ch[k]];
             refs[k] = c.left < 0 ? leaf_ref(c) : build(ch[k], depth + 1);
@@ -122,7 +136,7 @@ struct Collapser {
                 lo[a] = std::min(lo[a], br.nodes[c].lo[a] - pad);
                 hi[a] = std::max(hi[a], br.nodes[c].hi[a] + pad);
             }
-        GpuNode8 g{};
+        Node g{};
         int eb[3];
         float s[3];
         for (int a = 0; a < 3; ++a) {
@@ -132,7 +146,7 @@ struct Collapser {
         }
         g.meta = (uint32_t)eb[0] | ((uint32_t)eb[1] << 8) | ((uint32_t)eb[2] << 16) | ((uint32_t)ch.size() << 24) |
                  ((uint32_t)axis << 28);
-        for (int k = 0; k < 8; ++k) g.ref[k] = refs[k];
+        for (int k = 0; k < K; ++k) g.ref[k] = refs[k];
         for (size_t k = 0; k < ch.size(); ++k) {
             const BuildNode& c = br.nodes[ch[k]];
             for (int a = 0; a < 3; ++a) {
@@ -143,9 +157,7 @@ struct Collapser {
                 qh = std::min(255L, std::max(0L, qh));
                 while (ql > 0 && std::fma((float)ql, s[a], g.p[a]) > clo) --ql;
                 while (qh < 255 && std::fma((float)qh, s[a], g.p[a]) < chi) ++qh;
-                // byte positions: lo.x 0-7, lo.y 8-15, lo.z 16-23, hi.x 24-31, hi.y 32-39, hi.z 40-47
-                g.q[8 * a + k] = (uint8_t)ql;
-                g.q[24 + 8 * a + k] = (uint8_t)qh;
+                Quant<K>::set(g, a, (int)k, (uint8_t)ql, (uint8_t)qh);
             }
         }
         nodes[gi] = g;
@@ -176,16 +188,28 @@ void LeafWriter::write(std::vector<float>& slots, uint32_t prim) {
     }
 }
 
-bool convert_bvh8(const BuildResult& br, LeafWriter& lw, float pad, std::vector<GpuNode8>& nodes,
-                  std::vector<float>& tris, uint64_t& leaves, int& depth) {
-    Collapser c{br, lw, pad, {}, {}, 0, 0};
-    c.nodes.reserve(br.nodes.size() / 4 + 1);
+template <int K, class Node>
+static void convert(const BuildResult& br, LeafWriter& lw, float pad, std::vector<Node>& nodes,
+                    std::vector<float>& tris, uint64_t& leaves, int& depth) {
+    Collapser<K> c{br, lw, pad, {}, {}, 0, 0};
+    c.nodes.reserve(br.nodes.size() / (K / 2) + 1);
     c.tris.reserve(12 * br.order.size());
     c.build(0, 0);
     nodes.swap(c.nodes);
     tris.swap(c.tris);
     leaves = c.leaves;
     depth = c.max_depth;
+}
+
+bool convert_bvh8(const BuildResult& br, LeafWriter& lw, float pad, std::vector<GpuNode8>& nodes,
+                  std::vector<float>& tris, uint64_t& leaves, int& depth) {
+    convert<8>(br, lw, pad, nodes, tris, leaves, depth);
+    return true;
+}
+
+bool convert_bvh4(const BuildResult& br, LeafWriter& lw, float pad, std::vector<GpuNode4>& nodes,
+                  std::vector<float>& tris, uint64_t& leaves, int& depth) {
+    convert<4>(br, lw, pad, nodes, tris, leaves, depth);
     return true;
 }
 

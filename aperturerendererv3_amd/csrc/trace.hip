@@ -10,7 +10,8 @@
 //  * default layout: the quantised 8-wide BVH, one 128-byte node (one L2
 //    line) per interior node, children decoded with one FMA per plane and
 //    stored along the node's largest axis so pushes need no sort; the
-//    binary layout (64-byte nodes holding both child boxes) is width 2;
+//    quantised 4-wide layout (64-byte nodes) is width 4 and the binary
+//    layout (64-byte nodes holding both child boxes) width 2;
 //  * per-lane traversal stack: the top kLdsStack entries in LDS (indexed
 //    [entry][thread], conflict-free), deeper entries spill to an HBM buffer
 //    laid out [entry][global thread];
@@ -186,17 +187,18 @@ struct Stack {
         --sp;
         return (uint32_t)(sp < kLdsStack ? lds[sp * kTraceBlock] : spill[(size_t)(sp - kLdsStack) * threads]);
     }
-    // Pushes refs[c] for every set bit c of mask: in slot order 0..7 when
-    // up is false (slot 7 pops first), else 7..0 (slot 0 pops first).  When
-    // all of them fit in LDS every child is written without a branch, to its
-    // rank's entry or, if not set, to the lane's spare entry kLdsStack.
-    __device__ __forceinline__ void push_mask(const uint32_t refs[8], uint32_t mask, bool up) {
+    // Pushes refs[c] for every set bit c of mask: in slot order 0..N-1 when
+    // up is false (slot N-1 pops first), else N-1..0 (slot 0 pops first).
+    // When all of them fit in LDS every child is written without a branch, to
+    // its rank's entry or, if not set, to the lane's spare entry kLdsStack.
+    template <int N>
+    __device__ __forceinline__ void push_mask(const uint32_t (&refs)[N], uint32_t mask, bool up) {
         const int n = __popc(mask);
         if (sp + n <= kLdsStack) {
             int rank = 0;
             const int top = sp + n - 1;
 #pragma unroll
-            for (int c = 0; c < 8; ++c) {
+            for (int c = 0; c < N; ++c) {
                 const bool hit = (mask >> c) & 1u;
                 const int pos = up ? top - rank : sp + rank;
                 lds[(hit ? pos : kLdsStack) * kTraceBlock] = (int)refs[c];
@@ -205,11 +207,11 @@ struct Stack {
             sp += n;
         } else if (up) {
 #pragma unroll
-            for (int c = 7; c >= 0; --c)
+            for (int c = N - 1; c >= 0; --c)
                 if ((mask >> c) & 1u) push(refs[c]);
         } else {
 #pragma unroll
-            for (int c = 0; c < 8; ++c)
+            for (int c = 0; c < N; ++c)
                 if ((mask >> c) & 1u) push(refs[c]);
         }
     }
@@ -362,6 +364,71 @@ __device__ __forceinline__ uint32_t interior8(const GpuNode8* __restrict__ nodes
             mask &= ~(1u << fi);
         }
 #endif
+        s.ref = next;
+    }
+    return nchild;
+}
+
+// Interior node of the quantised 4-wide layout: four child boxes from one
+// 64 B node, otherwise as interior8.
+template <bool ANY>
+__device__ __forceinline__ uint32_t interior4(const GpuNode4* __restrict__ nodes, Trav& s, Stack& st) {
+    const float4* nd = reinterpret_cast<const float4*>(nodes + s.ref);
+    const float4 h = nd[0];
+    const uint4 r = *reinterpret_cast<const uint4*>(nd + 1);
+    const uint4 q0 = *reinterpret_cast<const uint4*>(nd + 2);
+    const uint4 q1 = *reinterpret_cast<const uint4*>(nd + 3);
+    const uint32_t meta = __float_as_uint(h.w);
+    const uint32_t nchild = (meta >> 24) & 15u;
+    const RayPre& p = s.p;
+    const float ax = __uint_as_float((meta & 0xFFu) << 23) * p.idx;
+    const float ay = __uint_as_float(((meta >> 8) & 0xFFu) << 23) * p.idy;
+    const float az = __uint_as_float(((meta >> 16) & 0xFFu) << 23) * p.idz;
+    const float bx = (h.x - p.ox) * p.idx;
+    const float by = (h.y - p.oy) * p.idy;
+    const float bz = (h.z - p.oz) * p.idz;
+    const uint32_t refs[4] = {r.x, r.y, r.z, r.w};
+    float tn[4];
+    uint32_t mask = 0;
+    const bool nxs = p.idx < 0.0f, nys = p.idy < 0.0f, nzs = p.idz < 0.0f;
+    const uint32_t nearx = nxs ? q0.w : q0.x, farx = nxs ? q0.x : q0.w;
+    const uint32_t neary = nys ? q1.x : q0.y, fary = nys ? q0.y : q1.x;
+    const uint32_t nearz = nzs ? q1.y : q0.z, farz = nzs ? q0.z : q1.y;
+    const float lo_t = s.tmin, hi_t = s.best_t;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const float xn = fmaf(byte_f(nearx, c), ax, bx), xf = fmaf(byte_f(farx, c), ax, bx);
+        const float yn = fmaf(byte_f(neary, c), ay, by), yf = fmaf(byte_f(fary, c), ay, by);
+        const float zn = fmaf(byte_f(nearz, c), az, bz), zf = fmaf(byte_f(farz, c), az, bz);
+        const float a = fmaxf(fmaxf(xn, yn), fmaxf(zn, lo_t)) - p.tpad;
+        const float b = fminf(fminf(xf, yf), fminf(zf, hi_t)) + p.tpad;
+        tn[c] = a;
+        if (!(a > b)) mask |= 1u << c;   // NaN counts as a hit
+    }
+    mask &= (1u << nchild) - 1u;
+    if (mask == 0) {
+        s.ref = st.pop();
+        return nchild;
+    }
+    if (ANY) {
+        const int ni = __ffs(mask) - 1;
+        uint32_t next = refs[0];
+#pragma unroll
+        for (int c = 1; c < 4; ++c)
+            if (c == ni) next = refs[c];
+        st.push_mask(refs, mask & (mask - 1u), false);
+        s.ref = next;
+    } else {
+        float nt = INFINITY;
+        uint32_t next = kRefEmpty;
+        int ni = 0;
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+            if (((mask >> c) & 1u) && tn[c] < nt) { nt = tn[c]; next = refs[c]; ni = c; }
+        mask &= ~(1u << ni);
+        const uint32_t paxis = (meta >> 28) & 3u;
+        const float pdir = paxis == 0u ? p.idx : (paxis == 1u ? p.idy : p.idz);
+        st.push_mask(refs, mask, !(pdir < 0.0f));
         s.ref = next;
     }
     return nchild;
@@ -543,6 +610,9 @@ __attribute__((amdgpu_waves_per_eu(OBJ ? AR_TRACE_WAVES_OBJ
                     if constexpr (W == 8) {
                         const uint32_t c = interior8<ANY>(reinterpret_cast<const GpuNode8*>(nodes), s, st);
                         if (COUNT) nb += c;
+                    } else if constexpr (W == 4) {
+                        const uint32_t c = interior4<ANY>(reinterpret_cast<const GpuNode4*>(nodes), s, st);
+                        if (COUNT) nb += c;
                     } else {
                         interior2(reinterpret_cast<const GpuNode2*>(nodes), s, st);
                         if (COUNT) nb += 2;
@@ -636,11 +706,13 @@ int trace_blocks_per_cu(int width, bool any) {
     hipError_t e;
     if (width == 8) e = any ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_trace<8, true, false, false>, kTraceBlock, 0)
                             : hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_trace<8, false, false, false>, kTraceBlock, 0);
+    else if (width == 4) e = any ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_trace<4, true, false, false>, kTraceBlock, 0)
+                                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_trace<4, false, false, false>, kTraceBlock, 0);
     else if (width == 2) e = any ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_trace<2, true, false, false>, kTraceBlock, 0)
                                  : hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_trace<2, false, false, false>, kTraceBlock, 0);
     else {
         int m = 1;
-        for (int w : {2, 8})
+        for (int w : {2, 4, 8})
             for (bool a : {false, true}) m = std::max(m, trace_blocks_per_cu(w, a));
         return m;
     }
@@ -669,6 +741,9 @@ hipError_t launch_trace_closest(const TraceArgs& ta, const ar_rays& r, size_t n,
     if (ta.width == 8) {
         if (cnt_box) { if (ob) AR_LAUNCH_CLOSEST(8, true, true); else AR_LAUNCH_CLOSEST(8, true, false); }
         else { if (ob) AR_LAUNCH_CLOSEST(8, false, true); else AR_LAUNCH_CLOSEST(8, false, false); }
+    } else if (ta.width == 4) {
+        if (cnt_box) { if (ob) AR_LAUNCH_CLOSEST(4, true, true); else AR_LAUNCH_CLOSEST(4, true, false); }
+        else { if (ob) AR_LAUNCH_CLOSEST(4, false, true); else AR_LAUNCH_CLOSEST(4, false, false); }
     } else {
         if (cnt_box) { if (ob) AR_LAUNCH_CLOSEST(2, true, true); else AR_LAUNCH_CLOSEST(2, true, false); }
         else { if (ob) AR_LAUNCH_CLOSEST(2, false, true); else AR_LAUNCH_CLOSEST(2, false, false); }
@@ -690,6 +765,7 @@ hipError_t launch_trace_any(const TraceArgs& ta, const ar_rays& r, size_t n, uin
                        ta.threads, nullptr, nullptr, ta.fetch)
     const bool ob = ta.objs != nullptr;
     if (ta.width == 8) { if (ob) AR_LAUNCH_ANY(8, true); else AR_LAUNCH_ANY(8, false); }
+    else if (ta.width == 4) { if (ob) AR_LAUNCH_ANY(4, true); else AR_LAUNCH_ANY(4, false); }
     else { if (ob) AR_LAUNCH_ANY(2, true); else AR_LAUNCH_ANY(2, false); }
 #undef AR_LAUNCH_ANY
     return hipGetLastError();

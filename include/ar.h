@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define AR_ABI_VERSION 1
+#define AR_ABI_VERSION 2
 
 #define AR_OK 0
 #define AR_E_INVALID (-1)     /* bad argument */
@@ -138,7 +138,8 @@ typedef struct {
 } ar_hits;
 
 typedef struct {
-    uint32_t width;      /* branching factor: 2 (64 B nodes) or 8 (quantised 128 B nodes, default) */
+    uint32_t width;      /* branching factor: 2 (64 B nodes), 4 (quantised 64 B nodes) or 8 (quantised 128 B
+                            nodes, default) */
     uint32_t leaf_size;  /* max triangles per leaf, 1..15 (default 3) */
     uint32_t bin_count;  /* SAH bins, 4..64 */
     uint32_t max_depth;  /* tree depth limit, <= 64 */
@@ -384,7 +385,7 @@ int ar_graph_destroy(ar_graph* graph);
 typedef struct {
     uint32_t width;       /* chosen BVH width */
     uint32_t tile;        /* chosen threads per block */
-    float ms[2][4];       /* [width 2, 8][tile 64, 128, 256, 512] */
+    float ms[3][4];       /* [width 2, 4, 8][tile 64, 128, 256, 512] */
 } ar_tune_result;
 
 int ar_autotune_batch(ar_ctx* const* scenes, uint32_t n, const ar_camera* cams, const ar_render_opts* opts,

@@ -16,7 +16,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_abi_version():
     lib = _lib.load()
-    assert lib.ar_abi_version() == 1
+    assert lib.ar_abi_version() == 2
 
 
 def test_null_arguments_rejected():
@@ -73,7 +73,7 @@ def test_cpu_backend_exports_every_declared_symbol():
     lib = C.CDLL(str(oracle / "libar_cpu.so"))
     missing = [s for s in _lib.declared_symbols() if not hasattr(lib, s)]
     assert not missing, f"declared in ar.h but not in libar_cpu.so: {missing}"
-    assert lib.ar_abi_version() == 1
+    assert lib.ar_abi_version() == 2
     assert lib.ar_create(None, None) == _lib.AR_E_INVALID
 
 

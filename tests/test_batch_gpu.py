@@ -84,7 +84,7 @@ def test_autotune_picks_a_measured_configuration(scenes):
     ctxs, views = scenes
     res = autotune(ctxs, views, probe_spp=1, max_depth=4, seed=1)
     print(res, flush=True)
-    assert res["width"] in (2, 8) and res["tile"] in (64, 128, 256, 512)
+    assert res["width"] in (2, 4, 8) and res["tile"] in (64, 128, 256, 512)
     assert all(v > 0 for v in res["ms"].values())
     assert res["ms"][f"w{res['width']}_t{res['tile']}"] == min(res["ms"].values())
     assert all(c.bvh_info()["width"] == res["width"] for c in ctxs)

@@ -49,7 +49,7 @@ def _gpu(ctx, V, F, objs, rays, width):
     return out, orc.unpack_bits(occ.cpu().numpy(), rays.shape[1])
 
 
-@pytest.mark.parametrize("width", [2, 8])
+@pytest.mark.parametrize("width", [2, 4, 8])
 def test_mixed_scene_matches_oracle(ctx, width):
     V, F, objs = random_object_scene(n_tris=1500, n_boxes=25, n_spheres=25, seed=4)
     objs.append(dict(objs[-1]))

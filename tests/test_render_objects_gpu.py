@@ -52,7 +52,7 @@ def compare(g, o, name, rtol, atol, min_frac, max_rel=1e-6):
     assert rel <= max_rel and frac >= min_frac, (name, rel, frac)
 
 
-@pytest.mark.parametrize("width", [2, 8])
+@pytest.mark.parametrize("width", [2, 4, 8])
 def test_object_cornell_matches_oracle(ctx, width):
     objs = cornell_objects()
     kw = dict(_kw(CORNELL_CAMERA), seed=3, max_depth=5)

@@ -79,7 +79,7 @@ def _rays(rng, objs, n, meps):
     return rays
 
 
-@pytest.mark.parametrize("width", [2, 8])
+@pytest.mark.parametrize("width", [2, 4, 8])
 @pytest.mark.parametrize("meps", [False, True])
 def test_sdf_traversal_bit_exact(ctx, width, meps):
     import torch

@@ -66,7 +66,7 @@ CASES = [
 ]
 
 
-@pytest.mark.parametrize("width", [2, 8])
+@pytest.mark.parametrize("width", [2, 4, 8])
 @pytest.mark.parametrize("case", range(len(CASES)), ids=[c[0] for c in CASES])
 def test_closest_and_any_match_brute_force(ctx, case, width):
     name, mk_mesh, mk_rays = CASES[case]
@@ -121,7 +121,7 @@ def test_scaled_traversal_workload(ctx):
     from aperturerendererv3_amd.workloads import traversal_mesh, traversal_rays
     V, F = traversal_mesh(levels=6)
     rays = traversal_rays(V, F, res=512)
-    for width in (2, 8):
+    for width in (2, 4, 8):
         got, _ = gpu_closest(ctx, V, F, rays, width=width)
         obvh = orc.OracleBVH(V, F)
         ref = obvh.closest(rays)

@@ -53,7 +53,7 @@ def _two_calls(ctx, rays_np, name):
         _assert_same(hg, hr, f"{name}: {tag}")
 
 
-@pytest.mark.parametrize("width", [2, 8])
+@pytest.mark.parametrize("width", [2, 4, 8])
 def test_noalloc_equals_trace_closest_mesh(ctx, width):
     V, F = traversal_mesh(levels=7)
     rays = traversal_rays(V, F, res=512)
@@ -62,7 +62,7 @@ def test_noalloc_equals_trace_closest_mesh(ctx, width):
     _two_calls(ctx, rays, f"mesh width {width}")
 
 
-@pytest.mark.parametrize("width", [2, 8])
+@pytest.mark.parametrize("width", [2, 4, 8])
 def test_noalloc_equals_trace_closest_objects(ctx, width):
     V, F, objs = random_object_scene(n_tris=1500, n_boxes=25, n_spheres=25, seed=4)
     ctx.set_meshes([{"vertices": V, "indices": F}])
