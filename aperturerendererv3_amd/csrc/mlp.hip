@@ -950,7 +950,6 @@ int mlp_ensure(ar_ctx* ctx) {
 }
 
 int mlp_init(ar_ctx* ctx, uint64_t seed) {
-    MlpDev& M = ctx->mlp;
     // He-normal weights on the used rows/columns, zero padding and biases
     std::vector<float> h(kParams, 0.0f);
     std::mt19937_64 rng(seed);

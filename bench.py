@@ -64,6 +64,7 @@ def parse_args():
     ap.add_argument("--output-only", action="store_true", help="run only the output-stage bench")
     ap.add_argument("--sdf-only", action="store_true", help="run only the SDF-object render bench")
     ap.add_argument("--smlp-only", action="store_true", help="run only the spectral basis network bench")
+    ap.add_argument("--guiding-only", action="store_true", help="run only the equal-time guiding comparison")
     ap.add_argument("--cpu-sample-stride", type=int, default=2)
     ap.add_argument("--profile", action="store_true",
                     help="timed steps only, plus one 256 MiB device copy used to scale HBM counters")
@@ -276,54 +277,126 @@ def batch_bench(ctx_cls, count=48, cpu=True, reps=2):
     return res
 
 
-def guiding_bench(ctx_cls):
-    """Equal-time comparison on the occluded-light room (light only through a
-    0.5 m door gap; 196,770 triangles): 64 x 64 pixels of its camera, depth
-    8, scored by relMSE against the committed FP64 oracle reference at 16,384
-    spp.  Plain rendering at 1024 and 4096 spp; learned guiding (MLP, frames
-    of 64 spp with 4 Adam steps each, and frames of 256 spp with one) and
-    voxel guiding (frames of 16 spp) at 1024 and 4096 spp; wall time per
-    render with the GPU synchronised."""
+GUIDE_FIXTURES = {
+    "room": ("guiding_room_fp64_320x180.npz", (320, 180), None),
+    "spectral": ("guiding_spectral_crop_fp64_320x180.npz", (1920, 1080), (800, 450, 320, 180)),
+}
+
+
+def guiding_bench(ctx_cls, room_spp=(256, 1024), spectral_spp=(32, 128)):
+    """Learned guiding against voxel guiding and plain rendering at equal
+    time, scored by relMSE against the FP64 oracle fixtures at 16,384 spp
+    (tests/golden/make_guiding_references.py): the occluded-light room
+    (196,770 triangles, light only through a 0.5 m door gap) at 320x180, and
+    the 500k-triangle spectral scene rendered at 1920x1080 and scored on its
+    320x180 centre window.  Both: 4 hero wavelengths, linear sRGB, depth 8,
+    NEE on.  Each relMSE has the fixture's own noise (its per-pixel variance
+    estimate) subtracted.  Wall time per render with the GPU synchronised.
+
+    Equal-time ratio of a method: t_m e_m / (t_plain e_plain) at equal spp
+    (error ~ 1 / time for an unbiased estimator): the time the method needs
+    for the plain render's error, as a fraction of plain's.  The combined
+    form renders both scenes as one batch with one guiding network (scene
+    index input) and is compared with the plain renders of the two scenes:
+    t_batch / sum_k(t_plain,k e_plain,k / e_batch,k)."""
     import torch
     sys.path.insert(0, str(ROOT / "tests"))
     import oracle_lib as orc
-    from aperturerendererv3_amd.workloads import OCCLUDED_CAMERA as C, occluded_room
-    ref = np.load(ROOT / "tests" / "golden" / "occluded_room_fp64_64x64.npy")
-    V, F, M, mats = occluded_room()
-    kw = dict(cam_pos=C["pos"], cam_dir=C["dir"], vfov=C["vfov"], max_depth=8)
-    out = {"workload": f"occluded-light room, {F.shape[0]} triangles, 64x64, max depth 8, NEE on; relMSE against "
-                       "the FP64 oracle at 16384 spp (tests/golden/occluded_room_fp64_64x64.npy)"}
-    with ctx_cls(0) as ctx:
-        ctx.set_meshes([{"vertices": V, "indices": F, "tri_material": M, "materials": mats}])
-        ctx.build_bvh()
+    from aperturerendererv3_amd import make_batch
+    from aperturerendererv3_amd.workloads import OCCLUDED_CAMERA, SPECTRAL_CAMERA, occluded_room, spectral_scene
+    scenes, refs = {}, {}
+    for name, (fname, size, crop) in GUIDE_FIXTURES.items():
+        path = ROOT / "tests" / "golden" / fname
+        if not path.exists():
+            continue
+        d = np.load(path)
+        ref = d["mean"].astype(np.float64)
+        floor = float(np.mean(d["var_mean"].astype(np.float64) / (ref ** 2 + 0.01)))
+        refs[name] = (ref, floor)
+        V, F, M, mats = occluded_room() if name == "room" else spectral_scene()
+        cam = OCCLUDED_CAMERA if name == "room" else SPECTRAL_CAMERA
+        scenes[name] = (V, F, M, mats, cam, size, crop)
+    out = {"workload": "occluded-light room 320x180 and spectral scene 1920x1080 (scored on its 320x180 centre "
+                       "window), 4 hero wavelengths, sRGB, depth 8, NEE on; relMSE against the FP64 oracle at "
+                       "16384 spp minus the fixture's own noise",
+           "reference_noise": {k: round(v[1], 6) for k, v in refs.items()}}
+    ctxs = {}
+    for name, (V, F, M, mats, cam, size, crop) in scenes.items():
+        c = ctx_cls(0)
+        c.set_meshes([{"vertices": V, "indices": F, "tri_material": M, "materials": mats}])
+        c.build_bvh()
+        ctxs[name] = c
+    kws = {name: dict(cam_pos=cam["pos"], cam_dir=cam["dir"], vfov=cam["vfov"], max_depth=8, spectral=4, output=1)
+           for name, (V, F, M, mats, cam, size, crop) in scenes.items()}
 
-        def run(spp, seed, prep=None, **extra):
-            if prep:
-                prep()
-            torch.cuda.synchronize()
-            t = time.perf_counter()
-            img = ctx.render(64, 64, spp, seed=seed, **kw, **extra)
-            torch.cuda.synchronize()
-            return {"spp": spp, "ms": round((time.perf_counter() - t) * 1e3, 2),
-                    "relmse": orc.rel_mse(img.cpu().numpy(), ref)}
+    def score(name, img):
+        ref, floor = refs[name]
+        crop = scenes[name][6]
+        if crop is not None:
+            x0, y0, w, h = crop
+            img = img[y0:y0 + h, x0:x0 + w]
+        return orc.rel_mse(img, ref) - floor
 
-        run(64, 99)
-        out["plain"] = [run(s, 1) for s in (1024, 4096)]
-        mlp = lambda: ctx.mlp_configure(learning_rate=5e-3, iters_per_frame=4, seed=1)
-        out["learned"] = [run(s, 7, prep=mlp, guiding=2, frame_spp=64) for s in (1024, 4096)]
-        # fewer, longer frames with one Adam step each: the cheapest setting
-        # of a sweep over frame size, steps per frame and learning rate
-        fast = lambda: ctx.mlp_configure(learning_rate=2e-2, iters_per_frame=1, seed=1)
-        out["learned_fast"] = [run(s, 7, prep=fast, guiding=2, frame_spp=256) for s in (1024, 4096)]
-        out["voxel"] = [run(s, 7, prep=ctx.guiding_reset, guiding=1, frame_spp=16) for s in (1024, 4096)]
-    # time the plain integrator needs for the learned render's error (relMSE ~ 1/spp)
-    p = out["plain"][-1]
-    ratios = {}
-    for name in ("learned", "learned_fast"):
-        l = out[name][-1]
-        ratios[name] = round(l["ms"] / (p["ms"] * p["relmse"] / l["relmse"]), 3)
-    out["time_ratio_at_equal_relmse"] = ratios
-    out["learned_vs_plain_time_at_equal_relmse"] = min(ratios.values())
+    def run(name, spp, seed, prep=None, **extra):
+        c = ctxs[name]
+        w, h = scenes[name][5]
+        if prep:
+            prep(c)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        img = c.render(w, h, spp, seed=seed, **kws[name], **extra)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t) * 1e3
+        return {"spp": spp, "ms": round(ms, 2), "relmse": score(name, img.cpu().numpy())}
+
+    mlp_cfg = dict(learning_rate=5e-3, iters_per_frame=4)
+    methods = {
+        "plain": (None, {}),
+        "voxel": (lambda c: c.guiding_reset(), dict(guiding=1)),
+        "learned": (lambda c: c.mlp_configure(seed=1, **mlp_cfg), dict(guiding=2)),
+    }
+    frame = {"room": 64, "spectral": 16}
+    for name in scenes:
+        levels = room_spp if name == "room" else spectral_spp
+        run(name, 4, 99)
+        res = {}
+        for mname, (prep, extra) in methods.items():
+            ex = dict(extra, frame_spp=frame[name]) if extra else {}
+            res[mname] = [run(name, s, 7 if extra else 1, prep=prep, **ex) for s in levels]
+        ratios = {}
+        for mname in ("voxel", "learned"):
+            p, m = res["plain"][-1], res[mname][-1]
+            ratios[mname] = round(m["ms"] * m["relmse"] / (p["ms"] * p["relmse"]), 3)
+        res["time_ratio_at_equal_relmse"] = ratios
+        out[name] = res
+    # the combined form: both scenes in one batch, one network
+    if len(scenes) == 2:
+        names = list(scenes)
+        b = make_batch([ctxs[n] for n in names])
+        views = []
+        for n in names:
+            w, h = scenes[n][5]
+            kw = kws[n]
+            views.append(dict(width=w, height=h, spp=(room_spp if n == "room" else spectral_spp)[-1],
+                              cam_pos=kw["cam_pos"], cam_dir=kw["cam_dir"], vfov=kw["vfov"]))
+        shared = dict(max_depth=8, spectral=4, output=1, seed=7)
+        b.render_views([dict(v, spp=4) for v in views], **shared)
+        b.mlp_configure(seed=1, **mlp_cfg)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        imgs = b.render_views(views, guiding=2, frame_spp=16, **shared)
+        torch.cuda.synchronize()
+        bms = (time.perf_counter() - t) * 1e3
+        errs = {n: score(n, im.cpu().numpy()) for n, im in zip(names, imgs)}
+        plain_equiv = sum(out[n]["plain"][-1]["ms"] * out[n]["plain"][-1]["relmse"] / errs[n] for n in names)
+        out["batched_learned"] = {"scenes": names, "spp": [v["spp"] for v in views], "frame_spp": 16,
+                                  "ms": round(bms, 2), "relmse": {n: errs[n] for n in names},
+                                  "plain_ms": round(sum(out[n]["plain"][-1]["ms"] for n in names), 2),
+                                  "time_ratio_at_equal_relmse": round(bms / plain_equiv, 3)}
+        out["learned_vs_plain_time_at_equal_relmse"] = out["batched_learned"]["time_ratio_at_equal_relmse"]
+        b.close()
+    for c in ctxs.values():
+        c.close()
     return out
 
 
@@ -663,6 +736,10 @@ def main():
     if args.mlp_only:
         from aperturerendererv3_amd import Context
         print(json.dumps(mlp_bench(Context)), flush=True)
+        return
+    if args.guiding_only:
+        from aperturerendererv3_amd import Context
+        print(json.dumps(guiding_bench(Context)), flush=True)
         return
     if args.output_only:
         from aperturerendererv3_amd import Context
