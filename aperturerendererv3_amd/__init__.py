@@ -670,6 +670,19 @@ class Context:
                                          _stream_ptr(stream)), "ar_present")
         return out
 
+    def render_counters(self, enable=True):
+        """ar_render_counters: count box / triangle tests and shaded vertices
+        of this context's renders (measurement only; slower)."""
+        self._check(self._lib.ar_render_counters(self._h, 1 if enable else 0), "ar_render_counters")
+
+    def render_counters_read(self) -> dict:
+        """ar_render_counters_read: totals since the last read (waits)."""
+        import numpy as np
+        out = np.zeros(5, np.uint64)
+        self._check(self._lib.ar_render_counters_read(self._h, out.ctypes.data), "ar_render_counters_read")
+        keys = ("closest_box_tests", "closest_tri_tests", "vertices", "shadow_box_tests", "shadow_tri_tests")
+        return {k: int(v) for k, v in zip(keys, out)}
+
     def render_host(self, width, height, spp, exposure=1.0, tone=AR_TONE_ACES, hdr=True, ldr=True, **kw):
         """ar_render_host: renders into the context's own device image and
         returns (hdr, ldr) numpy arrays ((height, width, 3) float32 and

@@ -158,6 +158,8 @@ SIGNATURES = {
     "ar_render": (C.c_int, [C.c_void_p, C.POINTER(ar_camera), C.POINTER(ar_render_opts), C.c_void_p, C.c_void_p]),
     "ar_present": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32,
                              C.c_float, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p]),
+    "ar_render_counters": (C.c_int, [C.c_void_p, C.c_int]),
+    "ar_render_counters_read": (C.c_int, [C.c_void_p, C.c_void_p]),
     "ar_render_host": (C.c_int, [C.c_void_p, C.POINTER(ar_camera), C.POINTER(ar_render_opts), C.c_void_p, C.c_void_p,
                                  C.c_float, C.c_uint32]),
     "ar_render_progressive": (C.c_int, [C.c_void_p, C.POINTER(ar_camera), C.POINTER(ar_render_opts), C.c_uint32,

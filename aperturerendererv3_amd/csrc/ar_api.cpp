@@ -237,6 +237,8 @@ int ar_destroy(ar_ctx* ctx) {
     if (ctx->denoise_feat) (void)hipFree(ctx->denoise_feat);
     if (ctx->host_out) (void)hipFree(ctx->host_out);
     if (ctx->present_buf) (void)hipFree(ctx->present_buf);
+    if (ctx->cnt_box) (void)hipFree(ctx->cnt_box);
+    if (ctx->cnt_totals) (void)hipFree(ctx->cnt_totals);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return AR_OK;
@@ -538,6 +540,39 @@ int ar_render_progressive(ar_ctx* ctx, const ar_camera* cam, const ar_render_opt
     const int rc = order_stream(ctx, pick_stream(ctx, stream));
     if (rc != AR_OK) return rc;
     return render_progressive(ctx, cam, opts, frame, accum, odd, even, gbuffer, stats, pick_stream(ctx, stream));
+}
+
+int ar_render_counters(ar_ctx* ctx, int enable) {
+    if (!ctx) return AR_E_INVALID;
+    (void)hipSetDevice(ctx->device);
+    if (!enable) {
+        ctx->count_on = false;
+        return AR_OK;
+    }
+    if (!ctx->cnt_totals) {
+        const int rc = check_hip(ctx, hipMalloc((void**)&ctx->cnt_totals, 5 * sizeof(unsigned long long)),
+                                 "hipMalloc(test totals)");
+        if (rc != AR_OK) return rc;
+    }
+    const unsigned long long zero[5] = {0, 0, 0, 0, 0};
+    const int rc = check_hip(ctx, hipMemcpy(ctx->cnt_totals, zero, sizeof zero, hipMemcpyHostToDevice), "zero totals");
+    if (rc != AR_OK) return rc;
+    invalidate_graphs(ctx);   // graphs captured before counting do not count
+    ctx->count_on = true;
+    return AR_OK;
+}
+
+int ar_render_counters_read(ar_ctx* ctx, uint64_t out[5]) {
+    if (!ctx || !out) return AR_E_INVALID;
+    if (!ctx->cnt_totals) return set_error(ctx, AR_E_INVALID, "counting was never enabled (ar_render_counters)");
+    (void)hipSetDevice(ctx->device);
+    unsigned long long h[5];
+    int rc = check_hip(ctx, hipDeviceSynchronize(), "counters");
+    if (rc == AR_OK) rc = check_hip(ctx, hipMemcpy(h, ctx->cnt_totals, sizeof h, hipMemcpyDeviceToHost), "read totals");
+    if (rc != AR_OK) return rc;
+    for (int k = 0; k < 5; ++k) out[k] = (uint64_t)h[k];
+    const unsigned long long zero[5] = {0, 0, 0, 0, 0};
+    return check_hip(ctx, hipMemcpy(ctx->cnt_totals, zero, sizeof zero, hipMemcpyHostToDevice), "zero totals");
 }
 
 } // extern "C"

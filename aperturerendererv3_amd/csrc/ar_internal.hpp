@@ -395,6 +395,14 @@ struct ar_ctx {
     float* denoise_feat = nullptr;        // per-pixel filter features of ar_denoise (29 planes)
     float* host_out = nullptr;            // ar_render_host's device image (3 floats HDR + 1 word LDR per pixel)
     size_t host_out_pixels = 0;
+    // ar_render_counters: per-ray box / triangle test counts of the integrator's
+    // traversals and their totals (closest box, closest tri, any box, any tri,
+    // shaded vertices) while counting is on
+    bool count_on = false;
+    uint32_t* cnt_box = nullptr;
+    uint32_t* cnt_tri = nullptr;
+    size_t cnt_cap = 0;
+    unsigned long long* cnt_totals = nullptr;
     float* present_buf = nullptr;         // ar_present's denoised image (3 floats per pixel)
     size_t present_pixels = 0;
     ar::SmlpDev smlp;                     // spectral basis network
@@ -435,7 +443,8 @@ hipError_t launch_trace_closest(const TraceArgs& ta, const ar_rays& r, size_t n,
                                 uint32_t* cnt_box, uint32_t* cnt_tri, uint32_t blocks, hipStream_t s,
                                 const uint32_t* n_dev = nullptr);
 hipError_t launch_trace_any(const TraceArgs& ta, const ar_rays& r, size_t n, uint32_t* occ,
-                            uint32_t blocks, hipStream_t s, const uint32_t* n_dev = nullptr);
+                            uint32_t blocks, hipStream_t s, const uint32_t* n_dev = nullptr,
+                            uint32_t* cnt_box = nullptr, uint32_t* cnt_tri = nullptr);
 hipError_t launch_shadow_rays(const ar_rays& r, const ar_hits& h, size_t n, float tx, float ty, float tz,
                               float tmin, float tmax_scale, const ar_rays& out, hipStream_t s);
 int trace_blocks_per_cu(int width = 0, bool any = false);

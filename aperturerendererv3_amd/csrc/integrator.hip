@@ -1367,6 +1367,31 @@ __global__ __launch_bounds__(kMaxBlock) void k_resolve(PathArgs S, const uint32_
 }
 
 // Next bounce: the appended queue becomes the current one.
+// Ops-model counting (ar_render_counters): adds the box and triangle tests of
+// the n_dev[0] rays just traced to tot[0], tot[1] and, when vert, the ray
+// count (the vertices shaded next) to tot[2].  Integer sums, so the atomic
+// order does not matter.
+__global__ __launch_bounds__(256) void k_count_sum(const uint32_t* __restrict__ box, const uint32_t* __restrict__ tri,
+                                                   const uint32_t* __restrict__ n_dev, unsigned long long* tot,
+                                                   int vert) {
+    __shared__ unsigned long long sb[256], st[256];
+    const uint32_t n = *n_dev;
+    unsigned long long a = 0, c = 0;
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) { a += box[i]; c += tri[i]; }
+    sb[threadIdx.x] = a;
+    st[threadIdx.x] = c;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) { sb[threadIdx.x] += sb[threadIdx.x + off]; st[threadIdx.x] += st[threadIdx.x + off]; }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        atomicAdd(&tot[0], sb[0]);
+        atomicAdd(&tot[1], st[0]);
+        if (vert && blockIdx.x == 0) atomicAdd(&tot[2], (unsigned long long)n);
+    }
+}
+
 __global__ void k_advance(uint32_t* counters) {
     counters[0] = counters[1];
     counters[1] = 0;
@@ -2008,6 +2033,16 @@ static int prepare_job(ar_ctx* ctx, RenderJob& J, hipStream_t stream, bool sync)
     const RenderPlan& L = J.plan;
     int rc = ensure_render_buffers(ctx, L.cap, L.pixels);
     if (rc != AR_OK) return rc;
+    if (ctx->count_on && ctx->cnt_cap < L.cap) {
+        invalidate_graphs(ctx);
+        if (ctx->cnt_box) (void)hipFree(ctx->cnt_box);
+        ctx->cnt_box = ctx->cnt_tri = nullptr;
+        ctx->cnt_cap = 0;
+        rc = check_hip(ctx, hipMalloc((void**)&ctx->cnt_box, 2 * L.cap * sizeof(uint32_t)), "hipMalloc(test counts)");
+        if (rc != AR_OK) return rc;
+        ctx->cnt_tri = ctx->cnt_box + L.cap;
+        ctx->cnt_cap = L.cap;
+    }
     if (wants_first_hits(L)) {
         // per-path first-hit records for the denoiser's G-buffer
         RenderBuffers& b = ctx->rb;
@@ -2135,8 +2170,13 @@ static int enqueue_job(ar_ctx* ctx, RenderJob& J, hipStream_t stream) {
             ar_rays rr{q[c].ox, q[c].oy, q[c].oz, q[c].tmin, q[c].dx, q[c].dy, q[c].dz, q[c].tmax, nullptr};
             TraceArgs tq = ta;
             tq.roots = q[c].root;
-            hipError_t e = launch_trace_closest(tq, rr, P.capacity, hits, nullptr, nullptr, tblocks, stream, &b.counters[0]);
+            const bool count = ctx->count_on && ctx->cnt_cap >= P.capacity;
+            hipError_t e = launch_trace_closest(tq, rr, P.capacity, hits, count ? ctx->cnt_box : nullptr,
+                                                count ? ctx->cnt_tri : nullptr, tblocks, stream, &b.counters[0]);
             if (e != hipSuccess) return check_hip(ctx, e, "integrator trace");
+            if (count)
+                hipLaunchKernelGGL(k_count_sum, dim3(stride_grid), dim3(256), 0, stream, ctx->cnt_box, ctx->cnt_tri,
+                                   &b.counters[0], ctx->cnt_totals, 1);
             if (J.learned && P.alpha > 0.0f) {
                 const SceneDev& sd = ctx->scene;
                 hipLaunchKernelGGL(k_mlp_features, dim3(grid_for(P.capacity, blk)), dim3(blk), 0, stream, sd.prims,
@@ -2158,8 +2198,12 @@ static int enqueue_job(ar_ctx* ctx, RenderJob& J, hipStream_t stream) {
                 ar_rays sr{shq.ox, shq.oy, shq.oz, shq.tmin, shq.dx, shq.dy, shq.dz, shq.tmax, shq.meps};
                 TraceArgs ts = ta;
                 ts.roots = shq.root;
-                e = launch_trace_any(ts, sr, P.capacity, b.occ, tblocks, stream, &b.counters[2]);
+                e = launch_trace_any(ts, sr, P.capacity, b.occ, tblocks, stream, &b.counters[2],
+                                     count ? ctx->cnt_box : nullptr, count ? ctx->cnt_tri : nullptr);
                 if (e != hipSuccess) return check_hip(ctx, e, "integrator shadow trace");
+                if (count)
+                    hipLaunchKernelGGL(k_count_sum, dim3(stride_grid), dim3(256), 0, stream, ctx->cnt_box, ctx->cnt_tri,
+                                       &b.counters[2], ctx->cnt_totals + 3, 0);
                 hipLaunchKernelGGL(k_resolve, dim3(stride_grid), dim3(blk), 0, stream, S, b.spath,
                                    b.scontrib[0], b.scontrib[1], b.scontrib[2], b.scontrib[3], b.sflags, b.occ,
                                    b.counters);

@@ -626,6 +626,7 @@ __attribute__((amdgpu_waves_per_eu(OBJ ? AR_TRACE_WAVES_OBJ
                 const uint32_t i = s.idx;
                 if constexpr (ANY) {
                     if (s.have) atomicOr(&occ[i >> 5], 1u << (i & 31u));
+                    if (COUNT) { cnt_box[i] = nb; cnt_tri[i] = nt; }
                 } else {
                     if (!s.have) {
                         h.prim[i] = AR_MISS;
@@ -753,20 +754,24 @@ hipError_t launch_trace_closest(const TraceArgs& ta, const ar_rays& r, size_t n,
 }
 
 hipError_t launch_trace_any(const TraceArgs& ta, const ar_rays& r, size_t n, uint32_t* occ, uint32_t blocks,
-                            hipStream_t s, const uint32_t* n_dev) {
+                            hipStream_t s, const uint32_t* n_dev, uint32_t* cnt_box, uint32_t* cnt_tri) {
     hipError_t e = prepare(ta, occ, n, n_dev, s);
     if (e != hipSuccess) return e;
     const void* nd = ta.nodes;
     const ar_hits none{nullptr, nullptr, nullptr, nullptr, nullptr};
     blocks = std::max(1u, std::min(blocks, ta.full_any));
-#define AR_LAUNCH_ANY(W, O)                                                                                      \
-    hipLaunchKernelGGL((k_trace<W, true, false, O>), dim3(blocks), dim3(kTraceBlock), 0, s, nd, ta.tris, ta.root, \
+#define AR_LAUNCH_ANY(W, C, O)                                                                                   \
+    hipLaunchKernelGGL((k_trace<W, true, C, O>), dim3(blocks), dim3(kTraceBlock), 0, s, nd, ta.tris, ta.root,     \
                        ta.roots, ta.origin_limit, ta.objs, ta.obj_base, ta.vol, r, n, n_dev, none, occ, ta.spill, \
-                       ta.threads, nullptr, nullptr, ta.fetch)
+                       ta.threads, cnt_box, cnt_tri, ta.fetch)
+#define AR_LAUNCH_ANY_W(W)                                                                                       \
+    if (cnt_box) { if (ob) AR_LAUNCH_ANY(W, true, true); else AR_LAUNCH_ANY(W, true, false); }                   \
+    else { if (ob) AR_LAUNCH_ANY(W, false, true); else AR_LAUNCH_ANY(W, false, false); }
     const bool ob = ta.objs != nullptr;
-    if (ta.width == 8) { if (ob) AR_LAUNCH_ANY(8, true); else AR_LAUNCH_ANY(8, false); }
-    else if (ta.width == 4) { if (ob) AR_LAUNCH_ANY(4, true); else AR_LAUNCH_ANY(4, false); }
-    else { if (ob) AR_LAUNCH_ANY(2, true); else AR_LAUNCH_ANY(2, false); }
+    if (ta.width == 8) { AR_LAUNCH_ANY_W(8) }
+    else if (ta.width == 4) { AR_LAUNCH_ANY_W(4) }
+    else { AR_LAUNCH_ANY_W(2) }
+#undef AR_LAUNCH_ANY_W
 #undef AR_LAUNCH_ANY
     return hipGetLastError();
 }

@@ -91,10 +91,48 @@ def cpu_baseline(V, F, rays_np, stride, shadow_np):
     bvh.closest(sub, mode=0, threads=cores)
     bvh.any(sh, mode=0, threads=cores)
     dt = time.perf_counter() - t0
+    # one core, and the reference's own rule (FastAABBHit as written, first hit in traversal order)
+    small = rays_np[:, ::stride * 64].copy()
+    t0 = time.perf_counter()
+    bvh.closest(small, mode=0, threads=1)
+    dt1 = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    bvh.closest(small, mode=1, threads=1)
+    dt1r = time.perf_counter() - t0
     return {"value": (sub.shape[1] + n_any) / dt / 1e9, "unit": "Grays/s", "cores": cores, "kind": "port",
             "sample": f"every {stride}th ray slot of the same workload: {sub.shape[1]} closest-hit rays + "
                       f"{n_any} shadow any-hit segments, oracle BVH2 traversal (reference builder: binned SAH, "
-                      f"12 bins, leaf 8; sorted 128-entry stack; strict slab test), OpenMP, {dt:.2f} s"}
+                      f"12 bins, leaf 8; sorted 128-entry stack; strict slab test), OpenMP, {dt:.2f} s",
+            "one_core": {"value": small.shape[1] / dt1 / 1e9, "cores": 1,
+                         "sample": f"every {stride * 64}th ray slot, {small.shape[1]} closest-hit rays, {dt1:.2f} s"},
+            "one_core_reference_rule": {"value": small.shape[1] / dt1r / 1e9, "cores": 1,
+                                        "sample": "the same rays with the reference's FastAABBHit as written and "
+                                                  f"its first-hit-in-order rule, {dt1r:.2f} s"}}
+
+
+SHADE_OPS_PER_VERTEX = 200
+
+
+def integrator_roofline(counts, ms):
+    """Ops model of an integrator run (16 per box test, 40 per triangle test
+    of both traversals, 200 per shaded vertex) over its wall time, against the
+    non-packed VALU lane-op peak."""
+    ops = 16 * (counts["closest_box_tests"] + counts["shadow_box_tests"]) + \
+        40 * (counts["closest_tri_tests"] + counts["shadow_tri_tests"]) + SHADE_OPS_PER_VERTEX * counts["vertices"]
+    achieved = ops / (ms * 1e-3) / 1e12
+    return {"bound": "valu", "achieved": round(achieved, 4), "peak": round(VALU_PEAK_TOPS, 2), "unit": "Tops/s",
+            "frac": round(achieved / VALU_PEAK_TOPS, 5), "traffic": None, "ops": ops, "counts": counts,
+            "model": "16 per box test + 40 per triangle test (closest and shadow traversals) + 200 per shaded "
+                     "vertex, counted by the counting render (ar_render_counters) of the same frame"}
+
+
+def counted(ctx, fn):
+    """Runs fn() once with the context's test counters on; returns the totals."""
+    ctx.render_counters(True)
+    fn()
+    counts = ctx.render_counters_read()
+    ctx.render_counters(False)
+    return counts
 
 
 def integrator_bench(ctx_cls, reps=5, cpu=True):
@@ -124,9 +162,11 @@ def integrator_bench(ctx_cls, reps=5, cpu=True):
         torch.cuda.synchronize()
         ms = e0.elapsed_time(e1) / reps
         img = out.cpu().numpy()
+        counts = counted(ctx, lambda: ctx.render(w, h, spp, out=torch.empty_like(out), **kw))
     paths = w * h * spp
     res = {"workload": "Cornell box, 32 triangles, 256x256, 16 spp, max depth 5, NEE on, seed 1",
-           "gpu_ms_per_frame": round(ms, 4), "gpu_paths_per_s": round(paths / (ms * 1e-3), 1)}
+           "gpu_ms_per_frame": round(ms, 4), "gpu_paths_per_s": round(paths / (ms * 1e-3), 1),
+           "roofline": integrator_roofline(counts, ms)}
     if cpu:
         sys.path.insert(0, str(ROOT / "tests"))
         import oracle_lib as orc
@@ -134,8 +174,13 @@ def integrator_bench(ctx_cls, reps=5, cpu=True):
         t0 = time.perf_counter()
         ref = orc.render(V, F, M, mats, w, h, spp, precision=32, threads=cores, **kw)
         dt = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        orc.render(V, F, M, mats, w, h, 2, precision=32, threads=1, **kw)
+        dt1 = time.perf_counter() - t0
         res["cpu_baseline"] = {"value": round(paths / dt, 1), "unit": "paths/s", "cores": cores, "kind": "port",
-                               "sample": f"the full frame ({paths} paths) with the oracle FP32 integrator, {dt:.2f} s"}
+                               "sample": f"the full frame ({paths} paths) with the oracle FP32 integrator, {dt:.2f} s",
+                               "one_core": {"value": round(w * h * 2 / dt1, 1), "cores": 1,
+                                            "sample": f"the frame at 2 spp ({w * h * 2} paths), {dt1:.2f} s"}}
         res["relmse_vs_oracle_same_seed"] = orc.rel_mse(img, ref)
     return res
 
@@ -169,12 +214,29 @@ def spectral_bench(ctx_cls, frames=1, spp=64, cpu=True):
         torch.cuda.synchronize()
         ms = e0.elapsed_time(e1) / frames
         mean = float(out.mean().item())
+        counts = counted(ctx, lambda: ctx.render(w, h, spp, out=torch.empty_like(out), **kw))
+        # accuracy: the same frame in linear sRGB against the FP64 oracle fixture of its 320x180 centre window
+        fix = ROOT / "tests" / "golden" / "guiding_spectral_crop_fp64_320x180.npz"
+        acc = None
+        if fix.exists():
+            sys.path.insert(0, str(ROOT / "tests"))
+            import oracle_lib as orc
+            d = np.load(fix)
+            refw = d["mean"].astype(np.float64)
+            floor = float(np.mean(d["var_mean"].astype(np.float64) / (refw ** 2 + 0.01)))
+            srgb = ctx.render(w, h, spp, **dict(kw, output=1)).cpu().numpy()[450:630, 800:1120]
+            acc = {"relmse_vs_fp64_window": orc.rel_mse(srgb, refw) - floor, "reference_noise": floor,
+                   "window": "320x180 centre window (pixels 800..1119 x 450..629), linear sRGB, FP64 oracle at "
+                             "16384 spp (tests/golden/guiding_spectral_crop_fp64_320x180.npz), its own noise "
+                             "subtracted"}
     paths = w * h * spp
     res = {"workload": f"instanced spectral scene, {F.shape[0]} triangles, 64 triangle lights, Lambert/GGX, "
                        f"{w}x{h}, {spp} spp, max depth 8, 4 hero wavelengths, NEE on, sky off, seed 1",
            "bvh_build_s": round(build_s, 3), "bvh": {"nodes": info["node_count"], "depth": info["depth"]},
            "gpu_ms_per_frame": round(ms, 2), "gpu_paths_per_s": round(paths / (ms * 1e-3), 1),
-           "image_mean_Y": mean}
+           "image_mean_Y": mean, "roofline": integrator_roofline(counts, ms)}
+    if acc is not None:
+        res["accuracy"] = acc
     if cpu:
         sys.path.insert(0, str(ROOT / "tests"))
         import oracle_lib as orc
@@ -246,13 +308,15 @@ def batch_bench(ctx_cls, count=48, cpu=True, reps=2):
     torch.cuda.synchronize()
     batch_ms = e0.elapsed_time(e1) / reps
     same = all(np.array_equal(a.view(np.uint32), o.cpu().numpy().view(np.uint32)) for a, o in zip(seq_img, g.outputs))
+    counts = counted(b, lambda: b.render_views(views, **shared))
     res = {"workload": f"{count} instanced spectral scenes, {tris} triangles in total (log-uniform 1e3..2e6 per scene), "
                        "resolutions in {256^2, 512^2, 1024^2, 1920x1080}, spp in {4, 16, 64}, max depth 6, NEE on, "
                        "sky off, RGB",
            "paths": paths, "autotune": tune, "setup_s": round(setup_s, 2), "autotune_s": round(tune_s, 2),
            "batched_graph_ms": round(batch_ms, 2), "batched_paths_per_s": round(paths / (batch_ms * 1e-3), 1),
            "sequential_ms": round(seq_ms, 2), "sequential_paths_per_s": round(paths / (seq_ms * 1e-3), 1),
-           "speedup_vs_sequential": round(seq_ms / batch_ms, 3), "images_identical_to_sequential": bool(same)}
+           "speedup_vs_sequential": round(seq_ms / batch_ms, 3), "images_identical_to_sequential": bool(same),
+           "roofline": integrator_roofline(counts, batch_ms)}
     if cpu:
         sys.path.insert(0, str(ROOT / "tests"))
         import oracle_lib as orc

@@ -333,6 +333,17 @@ typedef struct {
  * Queued on `stream`; the call does not wait. */
 int ar_render(ar_ctx* ctx, const ar_camera* cam, const ar_render_opts* opts, float* out_rgb, void* stream);
 
+/* Measurement of the integrator's ops model (no counterpart in the
+ * reference): while enabled, every render of the context (ar_render,
+ * ar_render_batch, ar_render_progressive, graphs captured afterwards) counts
+ * the box and triangle tests of its closest-hit and shadow traversals (the
+ * counting traversal variant: slower) and the path vertices it shades.
+ * ar_render_counters_read waits for the context's work and returns then
+ * clears {closest box tests, closest triangle tests, shaded vertices, shadow
+ * box tests, shadow triangle tests}. */
+int ar_render_counters(ar_ctx* ctx, int enable);
+int ar_render_counters_read(ar_ctx* ctx, uint64_t out[5]);
+
 /* --- batched rendering ----------------------------------------------------
  * No counterpart in the reference (one SceneRenderer renders one scene).
  * ar_batch_create merges scene contexts (each with its scene set and BVH

@@ -796,6 +796,9 @@ int ar_guiding_restore(ar_ctx* c, const uint64_t* keys, const float* texels, con
     return UNSUPPORTED(c);
 }
 
+int ar_render_counters(ar_ctx* c, int enable) { (void)enable; return UNSUPPORTED(c); }
+int ar_render_counters_read(ar_ctx* c, uint64_t out[5]) { (void)out; return UNSUPPORTED(c); }
+
 int ar_mlp_configure(ar_ctx* c, const ar_mlp_opts* opts) { (void)opts; return UNSUPPORTED(c); }
 int ar_mlp_params_get(ar_ctx* c, float* host_out, size_t count) { (void)host_out; (void)count; return UNSUPPORTED(c); }
 int ar_mlp_params_set(ar_ctx* c, const float* host_in, size_t count) { (void)host_in; (void)count; return UNSUPPORTED(c); }
