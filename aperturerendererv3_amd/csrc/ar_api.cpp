@@ -177,6 +177,19 @@ hipStream_t pick_stream(ar_ctx*, void* stream) {
     return reinterpret_cast<hipStream_t>(stream);
 }
 
+// Every float of a material is finite.  The reference takes any value and
+// scrubs non-finite path radiance to 0 per sample (core/vector.cuh:189-221);
+// a non-finite emission would also poison the light CDF, so it is refused
+// here instead.
+bool material_finite(const ar_material& m) {
+    const float f[] = {m.albedo[0], m.albedo[1], m.albedo[2], m.emission[0], m.emission[1], m.emission[2],
+                       m.specular,  m.metallic,  m.roughness, m.ior,         m.sigma_a,     m.abbe,
+                       m.tilt};
+    for (float v : f)
+        if (!std::isfinite(v)) return false;
+    return true;
+}
+
 } // namespace
 } // namespace ar
 
@@ -280,6 +293,9 @@ int ar_scene_set(ar_ctx* ctx, const ar_mesh* meshes, size_t nmesh) {
         if (me.nmaterials) {
             if (!me.materials) return set_error(ctx, AR_E_INVALID, "materials is NULL");
             for (uint32_t k = 0; k < me.nmaterials; ++k) {
+                if (!material_finite(me.materials[k]))
+                    return set_error(ctx, AR_E_INVALID,
+                                     "mesh " + std::to_string(m) + ": material " + std::to_string(k) + " has a non-finite value");
                 if (me.materials[k].model > AR_MODEL_GGX)
                     return set_error(ctx, me.materials[k].model == AR_MODEL_GLASS ? AR_E_UNSUPPORTED : AR_E_INVALID,
                                      "mesh " + std::to_string(m) +
@@ -333,6 +349,7 @@ int ar_scene_set_objects(ar_ctx* ctx, const ar_object* objects, size_t nobj) {
         if ((o.type == AR_OBJ_SDF_TORUS || o.type == AR_OBJ_SDF_VOLUME) &&
             (o.material.emission[0] > 0.0f || o.material.emission[1] > 0.0f || o.material.emission[2] > 0.0f))
             return set_error(ctx, AR_E_UNSUPPORTED, who + ": SDF objects cannot be lights");
+        if (!material_finite(o.material)) return set_error(ctx, AR_E_INVALID, who + ": non-finite material value");
         if (o.material.model > AR_MODEL_GLASS) return set_error(ctx, AR_E_INVALID, who + ": unknown material model");
         if (o.material.model == AR_MODEL_GLASS && !(o.material.ior > 0.0f))
             return set_error(ctx, AR_E_INVALID, who + ": glass needs ior > 0");

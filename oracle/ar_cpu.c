@@ -86,6 +86,13 @@ static int set_error(ar_ctx* c, int code, const char* fmt, ...)
 
 static int finite3(const float v[3]) { return isfinite(v[0]) && isfinite(v[1]) && isfinite(v[2]); }
 
+/* every float of a material is finite (the HIP library refuses the rest) */
+static int material_finite(const ar_material* m)
+{
+    return finite3(m->albedo) && finite3(m->emission) && isfinite(m->specular) && isfinite(m->metallic) &&
+           isfinite(m->roughness) && isfinite(m->ior) && isfinite(m->sigma_a) && isfinite(m->abbe) && isfinite(m->tilt);
+}
+
 /* ------------------------------------------------------------------ context */
 
 int ar_abi_version(void) { return AR_ABI_VERSION; }
@@ -156,10 +163,13 @@ int ar_scene_set(ar_ctx* c, const ar_mesh* meshes, size_t nmesh)
                 return set_error(c, AR_E_INVALID, "mesh %zu: material out of range", m);
         }
         if (me->nmaterials && !me->materials) return set_error(c, AR_E_INVALID, "materials is NULL");
-        for (uint32_t k = 0; k < me->nmaterials; ++k)
+        for (uint32_t k = 0; k < me->nmaterials; ++k) {
+            if (!material_finite(&me->materials[k]))
+                return set_error(c, AR_E_INVALID, "mesh %zu: material %u has a non-finite value", m, k);
             if (me->materials[k].model > AR_MODEL_GGX)
                 return set_error(c, me->materials[k].model == AR_MODEL_GLASS ? AR_E_UNSUPPORTED : AR_E_INVALID,
                                  "mesh %zu: triangle materials are Lambert or GGX (glass is for analytic objects)", m);
+        }
         nv += me->nv;
         nt += me->ntri;
         nm += me->nmaterials;
@@ -240,6 +250,7 @@ int ar_scene_set_objects(ar_ctx* c, const ar_object* objects, size_t nobj)
         const float* e = o->material.emission;
         if ((o->type == AR_OBJ_SDF_TORUS || o->type == AR_OBJ_SDF_VOLUME) && (e[0] > 0.0f || e[1] > 0.0f || e[2] > 0.0f))
             return set_error(c, AR_E_UNSUPPORTED, "object %zu: SDF objects cannot be lights", k);
+        if (!material_finite(&o->material)) return set_error(c, AR_E_INVALID, "object %zu: non-finite material value", k);
         if (o->material.model > AR_MODEL_GLASS) return set_error(c, AR_E_INVALID, "object %zu: unknown material model", k);
         if (o->material.model == AR_MODEL_GLASS && !(o->material.ior > 0.0f))
             return set_error(c, AR_E_INVALID, "object %zu: glass needs ior > 0", k);
