@@ -5,6 +5,7 @@ Test infrastructure only: the shipped library never imports this module.
 from __future__ import annotations
 
 import ctypes as C
+import os
 import subprocess
 from pathlib import Path
 
@@ -12,7 +13,9 @@ import numpy as np
 
 ROOT = Path(__file__).resolve().parent.parent
 ORACLE_DIR = ROOT / "oracle"
-LIB = ORACLE_DIR / "liboracle.so"
+# AR_ORACLE_LIB names another build of the same sources in oracle/ (the
+# sanitizer build, made by its own make target)
+LIB = ORACLE_DIR / os.environ.get("AR_ORACLE_LIB", "liboracle.so")
 MISS = 0xFFFFFFFF
 
 
@@ -41,7 +44,7 @@ def load():
         return _lib
     srcs = [ORACLE_DIR / n for n in ("trace.c", "render.c", "render_impl.h", "guide.c", "mlp.c", "output.c", "denoise.c", "oracle.h", "sdf_impl.h")]
     if not LIB.exists() or any(s.stat().st_mtime > LIB.stat().st_mtime for s in srcs if s.exists()):
-        subprocess.run(["make", "-s", "-C", str(ORACLE_DIR)], check=True)
+        subprocess.run(["make", "-s", "-C", str(ORACLE_DIR), LIB.name], check=True)
     lib = C.CDLL(str(LIB))
     vp = C.c_void_p
     lib.orc_bvh_build.restype = C.c_int

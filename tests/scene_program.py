@@ -1,7 +1,8 @@
 """Builds and runs tests/cxx/scene_renderer_main.cpp, a C++ program written
 against the SceneRenderer / Camera wrapper (include/ar/scene_renderer.hpp),
-linked against libar_hip.so ("hip") or the oracle's libar_cpu.so ("cpu"),
-and checks what it rendered against the oracle.
+linked against libar_hip.so ("hip") or the oracle's libar_cpu.so ("cpu";
+"cpu-san": program and library under AddressSanitizer and UBSan), and
+checks what it rendered against the oracle.
 
 The binaries go to tests/cxx/build/scene_renderer_<backend>; they are rebuilt
 when the program, the headers or the library are newer.  `python
@@ -31,7 +32,9 @@ SRC = HERE / "cxx" / "scene_renderer_main.cpp"
 HEADERS = [ROOT / "include" / "ar.h", ROOT / "include" / "ar" / "scene_renderer.hpp"]
 BUILD = HERE / "cxx" / "build"
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
-LIBS = {"hip": (ROOT / "aperturerendererv3_amd", "ar_hip"), "cpu": (ROOT / "oracle", "ar_cpu")}
+LIBS = {"hip": (ROOT / "aperturerendererv3_amd", "ar_hip"), "cpu": (ROOT / "oracle", "ar_cpu"),
+        "cpu-san": (ROOT / "oracle", "ar_cpu_san")}
+SANITIZE = ["-g", "-fno-omit-frame-pointer", "-fsanitize=address,undefined", "-fno-sanitize-recover=undefined"]
 W = H = 48
 
 
@@ -40,6 +43,8 @@ def compile_command(backend: str, out: Path) -> list[str]:
     rel = os.path.relpath(libdir, BUILD)
     cmd = ["g++", "-std=c++17", "-O1", "-Wall", "-Wextra", "-Werror", f"-I{ROOT / 'include'}", str(SRC), "-o",
            str(out), f"-L{libdir}", f"-l{lib}", f"-Wl,-rpath,$ORIGIN/{rel}"]
+    if backend == "cpu-san":   # the wrapper and the program under ASan / UBSan, on the sanitized CPU backend
+        cmd += SANITIZE
     if backend == "hip":
         cmd += ["-DAR_TEST_HIP", "-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include", f"-L{ROCM}/lib", "-lamdhip64",
                 f"-Wl,-rpath,{ROCM}/lib"]
