@@ -164,7 +164,8 @@ int trace_prologue(ar_ctx* ctx, TraceArgs& ta, size_t n, uint32_t& blocks) {
     ta.threads = ctx->trace_threads;
     ta.fetch = ctx->fetch;
     ta.objs = ctx->scene.nobj ? ctx->scene.objs : nullptr;
-    ta.obj_base = (uint32_t)(ctx->idx.size() / 3);
+    // objects follow the triangles (a batch's merged triangles: scene.ntri)
+    ta.obj_base = ctx->batch.empty() ? (uint32_t)(ctx->idx.size() / 3) : ctx->scene.ntri;
     ta.vol = ctx->sdf.dev;
     return AR_OK;
 }

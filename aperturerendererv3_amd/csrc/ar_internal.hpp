@@ -334,6 +334,8 @@ struct BatchEntry {
     uint32_t light_off;    // first light
     uint32_t light_cnt;
     uint32_t tri_off;      // first triangle (prim id offset)
+    uint32_t obj_off;      // first analytic object of the scene in the merged object records
+    uint32_t obj_cnt;
     float box_lo[3];       // the scene's box (learned-guiding inputs are normalised to it)
     float box_inv[3];
 };

@@ -2,14 +2,18 @@
 // render, and the tile/BVH-width autotuner.
 //
 // ar_batch_create merges the device arrays of scene contexts (BVH nodes,
-// leaf triangles, per-triangle shading data, materials, lights) into one
-// batch context, rebasing every reference on the GPU: node and leaf-slot
-// references of the BVH, prim ids of leaf triangles, material and light ids
-// of the shading data, and triangle ids of the light table.  A batch render
-// then traces the paths of all scenes through the same queues; each ray
-// carries its scene's BVH root and each path its scene's camera and light
-// range.
+// leaf triangles, per-triangle shading data, materials, lights, analytic
+// objects and their signed-distance voxels) into one batch context,
+// rebasing every reference: node and leaf-slot references of the BVH, prim
+// ids of leaf triangles and object ids of object leaf slots, material and
+// light ids of the shading data and of the object records, the object
+// records' own leaf slots and voxel offsets, and the prim ids of the light
+// table (triangles first: objects follow all triangles of the batch, as in
+// one scene).  A batch render then traces the paths of all scenes through
+// the same queues; each ray carries its scene's BVH root and each path its
+// scene's camera, light range and starting medium.
 #include "ar_internal.hpp"
+#include "objects.hpp"
 
 #include <algorithm>
 #include <chrono>
@@ -46,11 +50,22 @@ __global__ void k_rebase_nodes8(GpuNode8* n, uint32_t count, uint32_t node_off, 
     for (int k = 0; k < 8; ++k) n[i].ref[k] = rebase_ref(n[i].ref[k], node_off, slot_off);
 }
 
-// leaf triangles: (v0, prim id bits), (v1, 0), (v2, 0)
-__global__ void k_rebase_tris(float4* t, uint32_t slots, uint32_t tri_off) {
+// leaf slots: triangles (v0, prim id bits), (v1, 0), (v2, 0); objects
+// (0, kObjSlot | object id bits), (0), (0)
+__global__ void k_rebase_tris(float4* t, uint32_t slots, uint32_t tri_off, uint32_t obj_off) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= slots) return;
-    t[3 * (size_t)i].w = __uint_as_float(__float_as_uint(t[3 * (size_t)i].w) + tri_off);
+    const uint32_t w = __float_as_uint(t[3 * (size_t)i].w);
+    t[3 * (size_t)i].w = __uint_as_float(w + ((w & kObjSlot) ? obj_off : tri_off));
+}
+
+// light table prim ids: the scene's triangles (< ntri) move by tri_off, its
+// objects (ntri + k) to obj_prim + k
+__global__ void k_rebase_lights(int32_t* v, uint32_t n, uint32_t ntri, uint32_t tri_off, uint32_t obj_prim) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t l = (uint32_t)v[i];
+    v[i] = (int32_t)(l < ntri ? l + tri_off : l - ntri + obj_prim);
 }
 
 // shading triangles: (v0, material bits), (v1, light index bits or -1), (v2, 0)
@@ -60,11 +75,6 @@ __global__ void k_rebase_prims(float4* p, uint32_t tris, uint32_t mat_off, uint3
     p[3 * (size_t)i].w = __uint_as_float(__float_as_uint(p[3 * (size_t)i].w) + mat_off);
     const int l = __float_as_int(p[3 * (size_t)i + 1].w);
     if (l >= 0) p[3 * (size_t)i + 1].w = __int_as_float(l + (int)light_off);
-}
-
-__global__ void k_add_i32(int32_t* v, uint32_t n, int32_t off) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) v[i] += off;
 }
 
 uint32_t blocks(size_t n) { return (uint32_t)std::max<size_t>(1, (n + 255) / 256); }
@@ -81,27 +91,34 @@ int alloc(ar_ctx* ctx, T** p, size_t n) {
 
 } // namespace
 
+// an object record's own leaf slot (a one-slot leaf reference) in the merged slots
+uint32_t rebase_leaf(uint32_t r, uint32_t slot_off) { return r == kRefEmpty ? r : r + slot_off; }
+
 int batch_merge(ar_ctx* out, ar_ctx* const* scenes, uint32_t n) {
     const uint32_t width = scenes[0]->bvh.width;
-    size_t nodes = 0, slots = 0, tris = 0, mats = 0, lights = 0;
-    uint32_t stack = 0;
+    size_t nodes = 0, slots = 0, tris = 0, mats = 0, lights = 0, nobjs = 0, voxels = 0;
+    uint32_t stack = 0, nsdf = 0;
     float olim = INFINITY;
     for (uint32_t i = 0; i < n; ++i) {
         const ar_ctx* c = scenes[i];
         if (!c->bvh_ready) return set_error(out, AR_E_NOSCENE, "batch: a scene has no BVH");
         if (c->bvh.width != width) return set_error(out, AR_E_INVALID, "batch: scenes must share the BVH width");
         if (!c->batch.empty()) return set_error(out, AR_E_INVALID, "batch: a batch cannot contain a batch");
-        if (c->scene.nobj) return set_error(out, AR_E_UNSUPPORTED, "batch: scenes with analytic objects are not merged");
         nodes += c->bvh.node_count;
         slots += c->bvh.tri_slots;
         tris += c->scene.ntri;
         mats += c->scene.host_mats.size();
         lights += c->scene.nlights;
+        nobjs += c->scene.nobj;
+        nsdf += c->scene.nsdf;
+        voxels += c->sdf.host.size();
         stack = std::max(stack, c->bvh.stack_entries);
         if (c->bvh.root != kRefEmpty) olim = std::min(olim, c->bvh.origin_limit);
     }
-    if (slots >= (1u << 27) || nodes >= (1u << 31) || tris >= (1u << 31))
+    if (slots >= (1u << 27) || nodes >= (1u << 31) || tris + nobjs >= (1u << 31))
         return set_error(out, AR_E_CAPACITY, "batch: more than 2^27 triangle slots");
+    if (nobjs >= kObjSlot || voxels > (size_t)UINT32_MAX)
+        return set_error(out, AR_E_CAPACITY, "batch: too many objects or SDF voxels");
     const size_t nb = width == 8 ? sizeof(GpuNode8) : width == 4 ? sizeof(GpuNode4) : sizeof(GpuNode2);
     char* dnodes = nullptr;
     float4* dtris = nullptr;
@@ -109,6 +126,7 @@ int batch_merge(ar_ctx* out, ar_ctx* const* scenes, uint32_t n) {
     GpuMaterial* dmats = nullptr;
     int32_t* ltri = nullptr;
     float *larea = nullptr, *lprob = nullptr, *lcdf = nullptr;
+    float4* dobjs = nullptr;
     int rc = alloc(out, &dnodes, nodes * nb);
     if (rc == AR_OK) rc = alloc(out, &dtris, 3 * slots);
     if (rc == AR_OK) rc = alloc(out, &dprims, 3 * tris);
@@ -117,21 +135,56 @@ int batch_merge(ar_ctx* out, ar_ctx* const* scenes, uint32_t n) {
     if (rc == AR_OK) rc = alloc(out, &larea, lights);
     if (rc == AR_OK) rc = alloc(out, &lprob, lights);
     if (rc == AR_OK) rc = alloc(out, &lcdf, lights);
-    if (rc != AR_OK) {
+    if (rc == AR_OK && nobjs) rc = alloc(out, &dobjs, (size_t)obj::kRecord * nobjs);
+    auto free_all = [&]() {
         for (void* p : {(void*)dnodes, (void*)dtris, (void*)dprims, (void*)dmats, (void*)ltri, (void*)larea,
-                        (void*)lprob, (void*)lcdf})
+                        (void*)lprob, (void*)lcdf, (void*)dobjs})
             if (p) (void)hipFree(p);
+    };
+    if (rc != AR_OK) {
+        free_all();
         return rc;
     }
     std::vector<GpuMaterial> hm;
     hm.reserve(mats);
+    std::vector<float4> ho;      // merged object records (host copy)
+    ho.reserve((size_t)obj::kRecord * nobjs);
+    SdfPool pool;                // merged voxels
     std::vector<BatchEntry> entries;
     float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-    size_t on = 0, os = 0, ot = 0, om = 0, ol = 0;
+    size_t on = 0, os = 0, ot = 0, om = 0, ol = 0, oo = 0;
     hipError_t e = hipSuccess;
     for (uint32_t i = 0; i < n && e == hipSuccess; ++i) {
         const ar_ctx* c = scenes[i];
         const uint32_t cn = c->bvh.node_count, cs = c->bvh.tri_slots, ct = c->scene.ntri, cl = c->scene.nlights;
+        const uint32_t co = c->scene.nobj;
+        const uint32_t vox_off = (uint32_t)pool.host.size();
+        for (uint32_t k = 0; k < co; ++k) {
+            float4 r[obj::kRecord];
+            std::memcpy(r, &c->scene.host_objs[(size_t)obj::kRecord * k], sizeof r);
+            uint32_t type, mat, solo;
+            int32_t light;
+            std::memcpy(&type, &r[0].w, 4);
+            std::memcpy(&mat, &r[2].w, 4);
+            std::memcpy(&light, &r[3].w, 4);
+            std::memcpy(&solo, &r[4].w, 4);
+            mat += (uint32_t)om;
+            if (light >= 0) light += (int32_t)ol;
+            solo = rebase_leaf(solo, (uint32_t)os);
+            std::memcpy(&r[2].w, &mat, 4);
+            std::memcpy(&r[3].w, &light, 4);
+            std::memcpy(&r[4].w, &solo, 4);
+            if (type == AR_OBJ_SDF_VOLUME) {
+                uint32_t first;
+                std::memcpy(&first, &r[5].x, 4);
+                first += vox_off;
+                std::memcpy(&r[5].x, &first, 4);
+            }
+            ho.insert(ho.end(), r, r + obj::kRecord);
+        }
+        pool.host.insert(pool.host.end(), c->sdf.host.begin(), c->sdf.host.end());
+        for (uint32_t v : c->sdf.off) pool.off.push_back(v + vox_off);
+        pool.dims.insert(pool.dims.end(), c->sdf.dims.begin(), c->sdf.dims.end());
         if (cn) e = hipMemcpy(dnodes + on * nb, c->bvh.nodes, cn * nb, hipMemcpyDeviceToDevice);
         if (e == hipSuccess && cs) e = hipMemcpy(dtris + 3 * os, c->bvh.tris, 3 * cs * sizeof(float4), hipMemcpyDeviceToDevice);
         if (e == hipSuccess && ct) e = hipMemcpy(dprims + 3 * ot, c->scene.prims, 3 * ct * sizeof(float4), hipMemcpyDeviceToDevice);
@@ -153,16 +206,22 @@ int batch_merge(ar_ctx* out, ar_ctx* const* scenes, uint32_t n) {
                 hipLaunchKernelGGL(k_rebase_nodes2, dim3(blocks(cn)), dim3(256), 0, nullptr,
                                    (GpuNode2*)(dnodes + on * nb), cn, (uint32_t)on, (uint32_t)os);
         }
-        if (cs) hipLaunchKernelGGL(k_rebase_tris, dim3(blocks(cs)), dim3(256), 0, nullptr, dtris + 3 * os, cs, (uint32_t)ot);
+        if (cs)
+            hipLaunchKernelGGL(k_rebase_tris, dim3(blocks(cs)), dim3(256), 0, nullptr, dtris + 3 * os, cs, (uint32_t)ot,
+                               (uint32_t)oo);
         if (ct)
             hipLaunchKernelGGL(k_rebase_prims, dim3(blocks(ct)), dim3(256), 0, nullptr, dprims + 3 * ot, ct,
                                (uint32_t)om, (uint32_t)ol);
-        if (cl) hipLaunchKernelGGL(k_add_i32, dim3(blocks(cl)), dim3(256), 0, nullptr, ltri + ol, cl, (int32_t)ot);
+        if (cl)
+            hipLaunchKernelGGL(k_rebase_lights, dim3(blocks(cl)), dim3(256), 0, nullptr, ltri + ol, cl, ct,
+                               (uint32_t)ot, (uint32_t)(tris + oo));
         BatchEntry be{};
         be.root = c->bvh.root == kRefEmpty ? kRefEmpty : (uint32_t)(c->bvh.root + on);
         be.light_off = (uint32_t)ol;
         be.light_cnt = cl;
         be.tri_off = (uint32_t)ot;
+        be.obj_off = (uint32_t)oo;
+        be.obj_cnt = co;
         for (int a = 0; a < 3; ++a) { be.box_lo[a] = c->scene.box_lo[a]; be.box_inv[a] = c->scene.box_inv[a]; }
         entries.push_back(be);
         hm.insert(hm.end(), c->scene.host_mats.begin(), c->scene.host_mats.end());
@@ -172,15 +231,23 @@ int batch_merge(ar_ctx* out, ar_ctx* const* scenes, uint32_t n) {
                 hi[a] = std::max(hi[a], c->scene.box_lo[a] + 1.0f / c->scene.box_inv[a]);
             }
         }
-        on += cn; os += cs; ot += ct; om += c->scene.host_mats.size(); ol += cl;
+        on += cn; os += cs; ot += ct; om += c->scene.host_mats.size(); ol += cl; oo += co;
     }
     if (e == hipSuccess && !hm.empty()) e = hipMemcpy(dmats, hm.data(), hm.size() * sizeof(GpuMaterial), hipMemcpyHostToDevice);
+    if (e == hipSuccess && nobjs) e = hipMemcpy(dobjs, ho.data(), ho.size() * sizeof(float4), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) {
-        for (void* p : {(void*)dnodes, (void*)dtris, (void*)dprims, (void*)dmats, (void*)ltri, (void*)larea,
-                        (void*)lprob, (void*)lcdf})
-            if (p) (void)hipFree(p);
+        free_all();
         return check_hip(out, e, "batch merge");
+    }
+    out->sdf.host.swap(pool.host);
+    out->sdf.off.swap(pool.off);
+    out->sdf.dims.swap(pool.dims);
+    out->sdf.dirty = !out->sdf.host.empty();
+    rc = sdf_upload(out);
+    if (rc != AR_OK) {
+        free_all();
+        return rc;
     }
     out->bvh.width = width;
     out->bvh.nodes = (GpuNode2*)dnodes;
@@ -203,6 +270,10 @@ int batch_merge(ar_ctx* out, ar_ctx* const* scenes, uint32_t n) {
     s.light_cdf = lcdf;
     s.nlights = (uint32_t)lights;
     s.ntri = (uint32_t)tris;
+    s.objs = dobjs;
+    s.nobj = (uint32_t)nobjs;
+    s.nsdf = nsdf;
+    s.host_objs.swap(ho);
     s.host_mats = hm;
     s.spectral_ready = false;
     for (int a = 0; a < 3; ++a) {

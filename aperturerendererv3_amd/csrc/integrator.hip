@@ -1844,13 +1844,14 @@ struct RenderJob {
     bool guiding = false, learned = false;
 };
 
-// The first object containing the camera (Overlap, intersection.cuh:312-327;
-// CalculateRadiance:536-538): camera rays start inside it and trace it alone.
-static void camera_medium(const ar_ctx* ctx, const float pos[3], SceneView& v) {
+// The first object of [k0, k1) containing the camera (Overlap,
+// intersection.cuh:312-327; CalculateRadiance:536-538): camera rays start
+// inside it and trace it alone.
+static void camera_medium(const ar_ctx* ctx, const float pos[3], uint32_t k0, uint32_t k1, SceneView& v) {
     v.medium0 = 0;
     v.root0 = v.root;
     const SceneDev& s = ctx->scene;
-    for (uint32_t k = 0; k < s.nobj; ++k) {
+    for (uint32_t k = k0; k < k1 && k < s.nobj; ++k) {
         const float4* r = &s.host_objs[(size_t)obj::kRecord * k];
         if (obj::overlap_host(r, pos, ctx->sdf.host.data())) {
             v.medium0 = k + 1;
@@ -1926,12 +1927,12 @@ static int make_job(ar_ctx* ctx, uint32_t n, const ar_camera* cams, const ar_ren
         v.out_base = (uint32_t)L.pixels;
         if (ctx->batch.empty()) {
             v.root = ctx->bvh.root; v.light_off = 0; v.light_cnt = ctx->scene.nlights;
-            camera_medium(ctx, cams[i].pos, v);
+            camera_medium(ctx, cams[i].pos, 0, ctx->scene.nobj, v);
             for (int a = 0; a < 3; ++a) { v.box_lo[a] = ctx->scene.box_lo[a]; v.box_inv[a] = ctx->scene.box_inv[a]; }
         } else {
             const BatchEntry& be = ctx->batch[i];
             v.root = be.root; v.light_off = be.light_off; v.light_cnt = be.light_cnt;
-            v.root0 = v.root;
+            camera_medium(ctx, cams[i].pos, be.obj_off, be.obj_off + be.obj_cnt, v);
             for (int a = 0; a < 3; ++a) { v.box_lo[a] = be.box_lo[a]; v.box_inv[a] = be.box_inv[a]; }
         }
         camera_basis(&cams[i], oi.width, oi.height, v);

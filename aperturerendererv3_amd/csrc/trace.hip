@@ -861,7 +861,8 @@ extern "C" int ar_trace_closest_noalloc(ar_ctx* ctx, const ar_rays* rays, size_t
     ta.root = ctx->bvh.root;
     ta.origin_limit = ctx->bvh.origin_limit;
     ta.objs = ctx->scene.nobj ? ctx->scene.objs : nullptr;
-    ta.obj_base = (uint32_t)(ctx->idx.size() / 3);
+    // objects follow the triangles (a batch's merged triangles: scene.ntri)
+    ta.obj_base = ctx->batch.empty() ? (uint32_t)(ctx->idx.size() / 3) : ctx->scene.ntri;
     ta.vol = ctx->sdf.dev;
     const size_t resident = (size_t)ctx->num_cus * (size_t)trace_blocks_per_cu((int)ctx->bvh.width, false);
     const size_t need = (n + kTraceBlock - 1) / kTraceBlock;

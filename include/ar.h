@@ -349,8 +349,10 @@ int ar_render_counters_read(ar_ctx* ctx, uint64_t out[5]);
  * ar_batch_create merges scene contexts (each with its scene set and BVH
  * built, all at the same BVH width) into a batch context whose render traces
  * the paths of every scene through one set of wavefront queues: each ray
- * carries its scene's BVH root, each path its scene's camera, image and
- * lights.  The scene contexts stay valid and independent.  Scenes are packed
+ * carries its scene's BVH root, each path its scene's camera, image,
+ * lights and starting medium.  Scenes may hold triangles and analytic
+ * objects (boxes, spheres, SDF tori and volumes; each scene's voxel pool is
+ * merged).  The scene contexts stay valid and independent.  Scenes are packed
  * into passes in order, whole samples at a time, so each scene's image is
  * bit-identical to rendering it alone with the same options. */
 typedef struct ar_graph ar_graph;

@@ -126,18 +126,3 @@ def test_glass_sphere_furnace_and_camera_inside(ctx):
     inside = gpu_render(ctx, [], [sphere], 16, 16, 4, cam_pos=(0.0, 0.0, 0.0), cam_dir=(0.0, 0.0, -1.0), vfov=60.0,
                         sky=2, env=(0.8, 0.8, 0.8), max_depth=64)
     np.testing.assert_allclose(inside, 0.8 * 1.5 ** 2, rtol=1e-4)
-
-
-def test_render_batch_rejects_object_scenes():
-    from aperturerendererv3_amd import ArError, Context, make_batch
-    a, b = Context(0), Context(0)
-    try:
-        for c in (a, b):
-            c.set_meshes([])
-            c.set_objects(cornell_objects())
-            c.build_bvh()
-        with pytest.raises(ArError):
-            make_batch([a, b])
-    finally:
-        a.close()
-        b.close()
