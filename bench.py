@@ -44,6 +44,12 @@ HBM_PEAK_GBPS = 8000.0
 MFMA_BF16_PEAK = 2500.0                          # dense BF16 MFMA, TFLOP/s
 LIGHT = (2.0, 2.0, 2.0)
 TRAFFIC_FILE = ROOT / "profiles" / "k_trace_closest_bvh8_traversal_1M_tris_16M_rays.md"
+_T0 = time.perf_counter()
+
+
+def log(msg):
+    """Progress line on stderr (stdout carries only the JSON line)."""
+    print(f"[bench {time.perf_counter() - _T0:7.1f} s] {msg}", file=sys.stderr, flush=True)
 
 
 def parse_args():
@@ -91,13 +97,17 @@ def cpu_baseline(V, F, rays_np, stride, shadow_np):
     bvh.closest(sub, mode=0, threads=cores)
     bvh.any(sh, mode=0, threads=cores)
     dt = time.perf_counter() - t0
-    # one core, and the reference's own rule (FastAABBHit as written, first hit in traversal order)
+    log(f"traversal CPU baseline: {cores} cores {dt:.1f} s; one core")
+    # one core, and the reference's own rule (FastAABBHit as written, first
+    # hit in traversal order: ~1e5 box tests per ray here, so a small sample)
     small = rays_np[:, ::stride * 64].copy()
     t0 = time.perf_counter()
     bvh.closest(small, mode=0, threads=1)
     dt1 = time.perf_counter() - t0
+    log(f"traversal CPU baseline: one core {dt1:.1f} s; reference rule")
+    tiny = rays_np[:, ::stride * 32768].copy()
     t0 = time.perf_counter()
-    bvh.closest(small, mode=1, threads=1)
+    bvh.closest(tiny, mode=1, threads=1)
     dt1r = time.perf_counter() - t0
     return {"value": (sub.shape[1] + n_any) / dt / 1e9, "unit": "Grays/s", "cores": cores, "kind": "port",
             "sample": f"every {stride}th ray slot of the same workload: {sub.shape[1]} closest-hit rays + "
@@ -105,9 +115,10 @@ def cpu_baseline(V, F, rays_np, stride, shadow_np):
                       f"12 bins, leaf 8; sorted 128-entry stack; strict slab test), OpenMP, {dt:.2f} s",
             "one_core": {"value": small.shape[1] / dt1 / 1e9, "cores": 1,
                          "sample": f"every {stride * 64}th ray slot, {small.shape[1]} closest-hit rays, {dt1:.2f} s"},
-            "one_core_reference_rule": {"value": small.shape[1] / dt1r / 1e9, "cores": 1,
-                                        "sample": "the same rays with the reference's FastAABBHit as written and "
-                                                  f"its first-hit-in-order rule, {dt1r:.2f} s"}}
+            "one_core_reference_rule": {"value": tiny.shape[1] / dt1r / 1e9, "cores": 1,
+                                        "sample": f"every {stride * 32768}th ray slot, {tiny.shape[1]} closest-hit "
+                                                  "rays, with the reference's FastAABBHit as written and its "
+                                                  f"first-hit-in-order rule, {dt1r:.2f} s"}}
 
 
 SHADE_OPS_PER_VERTEX = 200
@@ -163,11 +174,29 @@ def integrator_bench(ctx_cls, reps=5, cpu=True):
         ms = e0.elapsed_time(e1) / reps
         img = out.cpu().numpy()
         counts = counted(ctx, lambda: ctx.render(w, h, spp, out=torch.empty_like(out), **kw))
+        # accuracy against the FP64 oracle at 65,536 spp (its own noise subtracted)
+        fix = ROOT / "tests" / "golden" / "cornell_fp64_256x256.npz"
+        acc = None
+        if fix.exists():
+            sys.path.insert(0, str(ROOT / "tests"))
+            import oracle_lib as orc
+            d = np.load(fix)
+            ref64 = d["mean"].astype(np.float64)
+            floor = float(np.mean(d["var_mean"].astype(np.float64) / (ref64 ** 2 + 0.01)))
+            acc = {"reference": "FP64 oracle, 65536 spp (tests/golden/cornell_fp64_256x256.npz)",
+                   "reference_noise": floor, f"relmse_{spp}spp": orc.rel_mse(img, ref64) - floor}
+            for hs in (1024, 8192):
+                hi = ctx.render(w, h, hs, **dict(kw, seed=2)).cpu().numpy()
+                acc[f"relmse_{hs}spp"] = orc.rel_mse(hi, ref64) - floor
+            acc["mean_rel_diff_8192spp"] = float(abs(hi.mean() - ref64.mean()) / ref64.mean())
     paths = w * h * spp
     res = {"workload": "Cornell box, 32 triangles, 256x256, 16 spp, max depth 5, NEE on, seed 1",
            "gpu_ms_per_frame": round(ms, 4), "gpu_paths_per_s": round(paths / (ms * 1e-3), 1),
            "roofline": integrator_roofline(counts, ms)}
+    if acc is not None:
+        res["accuracy_vs_fp64"] = acc
     if cpu:
+        log("integrator_bench: CPU baseline")
         sys.path.insert(0, str(ROOT / "tests"))
         import oracle_lib as orc
         cores = len(os.sched_getaffinity(0))
@@ -238,6 +267,7 @@ def spectral_bench(ctx_cls, frames=1, spp=64, cpu=True):
     if acc is not None:
         res["accuracy"] = acc
     if cpu:
+        log("spectral_bench: CPU baseline")
         sys.path.insert(0, str(ROOT / "tests"))
         import oracle_lib as orc
         cores = len(os.sched_getaffinity(0))
@@ -318,6 +348,7 @@ def batch_bench(ctx_cls, count=48, cpu=True, reps=2):
            "speedup_vs_sequential": round(seq_ms / batch_ms, 3), "images_identical_to_sequential": bool(same),
            "roofline": integrator_roofline(counts, batch_ms)}
     if cpu:
+        log("batch_bench: CPU baseline")
         sys.path.insert(0, str(ROOT / "tests"))
         import oracle_lib as orc
         cores = len(os.sched_getaffinity(0))
@@ -425,6 +456,7 @@ def guiding_bench(ctx_cls, room_spp=(256, 1024), spectral_spp=(32, 128)):
         run(name, 4, 99)
         res = {}
         for mname, (prep, extra) in methods.items():
+            log(f"guiding: {name} {mname} at {levels} spp")
             ex = dict(extra, frame_spp=frame[name]) if extra else {}
             res[mname] = [run(name, s, 7 if extra else 1, prep=prep, **ex) for s in levels]
         ratios = {}
@@ -444,6 +476,7 @@ def guiding_bench(ctx_cls, room_spp=(256, 1024), spectral_spp=(32, 128)):
             views.append(dict(width=w, height=h, spp=(room_spp if n == "room" else spectral_spp)[-1],
                               cam_pos=kw["cam_pos"], cam_dir=kw["cam_dir"], vfov=kw["vfov"]))
         shared = dict(max_depth=8, spectral=4, output=1, seed=7)
+        log("guiding: both scenes batched, one network")
         b.render_views([dict(v, spp=4) for v in views], **shared)
         b.mlp_configure(seed=1, **mlp_cfg)
         torch.cuda.synchronize()
@@ -607,6 +640,7 @@ def output_bench(ctx_cls, reps=20, cpu=True):
         crop = (acc[:270, :480].contiguous().cpu().numpy(), gb[:270, :480].contiguous().cpu().numpy(),
                 st[:270, :480].contiguous().cpu().numpy())
     if cpu:
+        log("output_bench: CPU baseline")
         sys.path.insert(0, str(ROOT / "tests"))
         import ctypes as C
         import oracle_lib as orc
@@ -666,6 +700,7 @@ def sdf_bench(ctx_cls, reps=3, cpu=True):
             ms = e0.elapsed_time(e1) / reps
             res = {"gpu_ms_per_frame": round(ms, 3), "gpu_paths_per_s": round(w * h * spp / (ms * 1e-3), 1)}
             if cpu:
+                log("sdf_bench: CPU baseline")
                 sys.path.insert(0, str(ROOT / "tests"))
                 import oracle_lib as orc
                 cores = len(os.sched_getaffinity(0))
@@ -773,6 +808,7 @@ def smlp_bench(ctx_cls, cpu=True):
         ctx.spectral_set_basis(False)
         out["spectral_1080p_16spp"] = frame
     if cpu:
+        log("smlp_bench: CPU baseline")
         sys.path.insert(0, str(ROOT / "tests"))
         import oracle_lib as orc
         cores = len(os.sched_getaffinity(0))
@@ -932,17 +968,24 @@ def main():
     breakdown["box_tests_per_ray"] = boxes / n
     breakdown["tri_tests_per_ray"] = tris / n
 
+    log(f"traversal: {value:.3f} Grays/s, closest {closest_ms:.3f} ms, any {any_ms:.3f} ms")
     cpu = None
     if not args.no_cpu_baseline:
+        log("traversal CPU baseline (oracle)")
         cpu = cpu_baseline(V, F, rays_np, args.cpu_sample_stride, shadow.cpu().numpy())
-    integ = integrator_bench(Context, cpu=not args.no_cpu_baseline)
-    spectral = spectral_bench(Context, frames=2, spp=args.spectral_spp, cpu=not args.no_cpu_baseline)
-    batched = batch_bench(Context, count=args.batch_scenes, cpu=not args.no_cpu_baseline)
-    guiding = guiding_bench(Context)
-    mlp = mlp_bench(Context)
-    output = output_bench(Context, cpu=not args.no_cpu_baseline)
-    sdf = sdf_bench(Context, cpu=not args.no_cpu_baseline)
-    smlp = smlp_bench(Context, cpu=not args.no_cpu_baseline)
+    blocks = {}
+    for key, fn in (("integrator", lambda: integrator_bench(Context, cpu=not args.no_cpu_baseline)),
+                    ("spectral_1080p", lambda: spectral_bench(Context, frames=2, spp=args.spectral_spp,
+                                                              cpu=not args.no_cpu_baseline)),
+                    ("batched", lambda: batch_bench(Context, count=args.batch_scenes, cpu=not args.no_cpu_baseline)),
+                    ("guiding", lambda: guiding_bench(Context)),
+                    ("guiding_network", lambda: mlp_bench(Context)),
+                    ("output", lambda: output_bench(Context, cpu=not args.no_cpu_baseline)),
+                    ("sdf_objects", lambda: sdf_bench(Context, cpu=not args.no_cpu_baseline)),
+                    ("spectral_network", lambda: smlp_bench(Context, cpu=not args.no_cpu_baseline))):
+        log(f"{key}: start")
+        blocks[key] = fn()
+        log(f"{key}: done")
 
     valu_frac = achieved_tops / VALU_PEAK_TOPS
     out = {
@@ -963,14 +1006,7 @@ def main():
                      "ops_per_launch": ops, "avg_launch_ms": round(closest_ms, 4)},
         "cpu_baseline": cpu,
         "breakdown": {k: round(v, 4) for k, v in breakdown.items()},
-        "integrator": integ,
-        "spectral_1080p": spectral,
-        "batched": batched,
-        "guiding": guiding,
-        "guiding_network": mlp,
-        "output": output,
-        "sdf_objects": sdf,
-        "spectral_network": smlp,
+        **blocks,
     }
     print(json.dumps(out), flush=True)
     ctx.close()
