@@ -248,7 +248,7 @@ class Context:
         arr = object_array(objects)
         self._check(self._lib.ar_scene_set_objects(self._h, arr, len(objects)), "ar_scene_set_objects")
 
-    def build_bvh(self, width=8, leaf_size=3, bin_count=16, max_depth=60) -> dict:
+    def build_bvh(self, width=4, leaf_size=2, bin_count=16, max_depth=60) -> dict:
         o = ar_bvh_opts(width, leaf_size, bin_count, max_depth)
         self._check(self._lib.ar_bvh_build(self._h, C.byref(o)), "ar_bvh_build")
         return self.bvh_info()

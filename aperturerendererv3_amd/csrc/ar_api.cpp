@@ -370,7 +370,7 @@ int ar_scene_set_objects(ar_ctx* ctx, const ar_object* objects, size_t nobj) {
 int ar_bvh_build(ar_ctx* ctx, const ar_bvh_opts* opts) {
     if (!ctx) return AR_E_INVALID;
     if (!ctx->scene_set) return set_error(ctx, AR_E_NOSCENE, "no scene: call ar_scene_set first");
-    ar_bvh_opts o{8, 3, 16, (uint32_t)kMaxDepth};
+    ar_bvh_opts o{4, 2, 16, (uint32_t)kMaxDepth};
     if (opts) o = *opts;
     if (o.width != 2 && o.width != 4 && o.width != 8) return set_error(ctx, AR_E_UNSUPPORTED, "BVH width must be 2, 4 or 8");
     if (o.leaf_size < 1 || o.leaf_size > kMaxLeafTris) return set_error(ctx, AR_E_INVALID, "leaf_size must be 1..15");

@@ -43,7 +43,7 @@ VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12      # 39.3 Tops/s
 HBM_PEAK_GBPS = 8000.0
 MFMA_BF16_PEAK = 2500.0                          # dense BF16 MFMA, TFLOP/s
 LIGHT = (2.0, 2.0, 2.0)
-TRAFFIC_FILE = ROOT / "profiles" / "k_trace_closest_bvh8_traversal_1M_tris_16M_rays.md"
+TRAFFIC_FILE = ROOT / "profiles" / "k_trace_closest_bvh4_traversal_1M_tris_16M_rays.md"
 _T0 = time.perf_counter()
 
 
@@ -59,8 +59,9 @@ def parse_args():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--levels", type=int, default=9, help="subdivision levels (9 = 1,048,576 triangles)")
     ap.add_argument("--res", type=int, default=4096, help="ray grid side (4096 = 16,777,216 slots)")
-    ap.add_argument("--leaf-size", type=int, default=3)
-    ap.add_argument("--width", type=int, default=8, help="BVH branching factor (2 or 8)")
+    ap.add_argument("--leaf-size", type=int, default=2)
+    ap.add_argument("--bins", type=int, default=16, help="SAH bins")
+    ap.add_argument("--width", type=int, default=4, help="BVH branching factor (2, 4 or 8)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--spectral-only", action="store_true", help="run only the spectral 1080p render bench")
     ap.add_argument("--spectral-spp", type=int, default=64)
@@ -875,7 +876,7 @@ def main():
     n = rays_np.shape[1]
     ctx = Context(0)
     ctx.set_meshes([{"vertices": V, "indices": F}])
-    info = ctx.build_bvh(width=args.width, leaf_size=args.leaf_size)
+    info = ctx.build_bvh(width=args.width, leaf_size=args.leaf_size, bin_count=args.bins)
     dev = torch.device("cuda:0")
     rays = torch.from_numpy(rays_np).to(dev)
     hits = ctx.alloc_hits(n, dev)

@@ -138,9 +138,9 @@ typedef struct {
 } ar_hits;
 
 typedef struct {
-    uint32_t width;      /* branching factor: 2 (64 B nodes), 4 (quantised 64 B nodes) or 8 (quantised 128 B
-                            nodes, default) */
-    uint32_t leaf_size;  /* max triangles per leaf, 1..15 (default 3) */
+    uint32_t width;      /* branching factor: 2 (64 B nodes), 4 (quantised 64 B nodes, default) or 8
+                            (quantised 128 B nodes) */
+    uint32_t leaf_size;  /* max triangles per leaf, 1..15 (default 2) */
     uint32_t bin_count;  /* SAH bins, 4..64 */
     uint32_t max_depth;  /* tree depth limit, <= 64 */
 } ar_bvh_opts;

@@ -312,7 +312,7 @@ struct RenderConfig {
     uint32_t spectral = AR_SPECTRAL_RGB;    // USE_SPECTRUM_RENDERING: AR_SPECTRAL_SINGLE or AR_SPECTRAL_HERO4
     uint32_t output = AR_OUTPUT_XYZ;        // spectral output space (XYZ, converted by the tonemap)
     uint32_t guiding = AR_GUIDING_VOXEL;    // the guide bUsePathGuiding_ turns on
-    ar_bvh_opts bvh = {8, 3, 16, 64};
+    ar_bvh_opts bvh = {4, 2, 16, 64};
 };
 
 // ---------------------------------------------------------------- SceneRenderer (core/scene.hpp:18-135)

@@ -10,7 +10,7 @@
  *
  * Implemented (same arguments, results and error codes as libar_hip.so):
  *   context, scene (meshes, analytic objects, SDF volumes), ar_bvh_build (the
- *   oracle's binned-SAH BVH2: width 2 and 8 are accepted, the tree is binary),
+ *   oracle's binned-SAH BVH2: widths 2, 4 and 8 are accepted, the tree is binary),
  *   ar_trace_closest / _counted / _noalloc and ar_trace_any (canonical hit
  *   rule: the same hits as the GPU, bit for bit), ar_render, ar_render_host
  *   and ar_render_progressive in RGB (the FP32 oracle integrator; voxel
@@ -356,9 +356,9 @@ int ar_bvh_build(ar_ctx* c, const ar_bvh_opts* opts)
 {
     if (!c) return AR_E_INVALID;
     if (!c->scene_set) return set_error(c, AR_E_NOSCENE, "no scene: call ar_scene_set first");
-    ar_bvh_opts o = { 8, 3, 16, 64 };
+    ar_bvh_opts o = { 4, 2, 16, 64 };
     if (opts) o = *opts;
-    if (o.width != 2 && o.width != 8) return set_error(c, AR_E_UNSUPPORTED, "BVH width must be 2 or 8");
+    if (o.width != 2 && o.width != 4 && o.width != 8) return set_error(c, AR_E_UNSUPPORTED, "BVH width must be 2, 4 or 8");
     if (o.leaf_size < 1 || o.leaf_size > 15) return set_error(c, AR_E_INVALID, "leaf_size must be 1..15");
     if (o.bin_count < 4 || o.bin_count > 64) return set_error(c, AR_E_INVALID, "bin_count must be 4..64");
     free_bvh(c);
