@@ -98,7 +98,7 @@ def _stream_ptr(stream) -> int | None:
 
 def _camera_opts(width, height, spp, cam_pos=(0.0, 0.0, 1.1), cam_dir=(0.0, 0.0, -1.0), vfov=60.0, seed=1,
                  max_depth=5, nee=True, sky=AR_SKY_NONE, env=(0.0, 0.0, 0.0), rr_min_depth=3, sample_offset=0,
-                 max_paths_per_pass=0, spectral=0, output=0, guiding=0, frame_spp=0):
+                 max_paths_per_pass=0, spectral=0, output=0, guiding=0, frame_spp=0, frame_growth=0):
     cam = ar_camera()
     cam.pos[:] = [float(x) for x in cam_pos]
     cam.dir[:] = [float(x) for x in cam_dir]
@@ -109,6 +109,7 @@ def _camera_opts(width, height, spp, cam_pos=(0.0, 0.0, 1.1), cam_dir=(0.0, 0.0,
     o.env[:] = [float(x) for x in env]
     o.max_paths_per_pass = max_paths_per_pass
     o.spectral, o.output, o.guiding, o.frame_spp = spectral, output, guiding, frame_spp
+    o.frame_growth = frame_growth
     return cam, o
 
 
@@ -291,6 +292,20 @@ class Context:
         p = np.zeros(_lib.MLP_PARAMS, np.float32)
         self._check(self._lib.ar_mlp_params_get(self._h, p.ctypes.data, p.size), "ar_mlp_params_get")
         return p
+
+    def mlp_records(self, cap=1 << 18):
+        """The last learned-guided frame's training records (ar_mlp_records_get):
+        dict of x (n, 6), dir (n, 3), weight (n,) and total (records before the
+        batch was drawn)."""
+        import numpy as np
+        x = np.zeros((cap, 6), np.float32)
+        d = np.zeros((cap, 3), np.float32)
+        w = np.zeros(cap, np.float32)
+        n, t = C.c_uint32(0), C.c_uint32(0)
+        self._check(self._lib.ar_mlp_records_get(self._h, x.ctypes.data, d.ctypes.data, w.ctypes.data, cap,
+                                                 C.byref(n), C.byref(t)), "ar_mlp_records_get")
+        k = min(n.value, cap)
+        return {"x": x[:k], "dir": d[:k], "weight": w[:k], "total": t.value}
 
     def mlp_set_params(self, p):
         import numpy as np
@@ -476,14 +491,15 @@ class Context:
     def render_progressive(self, frame, accum, odd=None, even=None, spp=1, cam_pos=(0.0, 0.0, 1.1),
                            cam_dir=(0.0, 0.0, -1.0), vfov=60.0, seed=1, max_depth=5, nee=True, sky=AR_SKY_NONE,
                            env=(0.0, 0.0, 0.0), rr_min_depth=3, spectral=0, output=0, gbuffer=None, stats=None,
-                           guiding=0, frame_spp=0, stream=None):
+                           guiding=0, frame_spp=0, frame_growth=0, stream=None):
         """ar_render_progressive: fold spp samples (indices frame ..) into the
         running means accum (and odd / even), (h, w, 3) float32 device tensors;
         gbuffer (h, w, 16) / stats (h, w, 12) keep the denoiser's records;
-        guiding updates the context's guide after every frame_spp samples."""
+        guiding updates the context's guide after every frame (frame_spp
+        samples, times frame_growth from one frame to the next)."""
         h, w = accum.shape[0], accum.shape[1]
         cam, o = _camera_opts(w, h, spp, cam_pos, cam_dir, vfov, seed, max_depth, nee, sky, env, rr_min_depth, 0, 0,
-                              spectral, output, guiding, frame_spp)
+                              spectral, output, guiding, frame_spp, frame_growth)
         opt = lambda t: _ptr(t) if t is not None else None
         self._check(self._lib.ar_render_progressive(self._h, C.byref(cam), C.byref(o), int(frame), _ptr(accum),
                                                     opt(odd), opt(even), opt(gbuffer), opt(stats),
@@ -640,11 +656,13 @@ class Context:
 
     def render(self, width, height, spp, cam_pos=(0.0, 0.0, 1.1), cam_dir=(0.0, 0.0, -1.0), vfov=60.0, seed=1,
                max_depth=5, nee=True, sky=AR_SKY_NONE, env=(0.0, 0.0, 0.0), rr_min_depth=3, sample_offset=0,
-               max_paths_per_pass=0, spectral=0, output=0, guiding=0, frame_spp=0, out=None, stream=None):
+               max_paths_per_pass=0, spectral=0, output=0, guiding=0, frame_spp=0, frame_growth=0, out=None,
+               stream=None):
         """Mean radiance image (height, width, 3) float32 on the GPU (ar_render)."""
         import torch
         cam, o = _camera_opts(width, height, spp, cam_pos, cam_dir, vfov, seed, max_depth, nee, sky, env,
-                              rr_min_depth, sample_offset, max_paths_per_pass, spectral, output, guiding, frame_spp)
+                              rr_min_depth, sample_offset, max_paths_per_pass, spectral, output, guiding, frame_spp,
+                              frame_growth)
         if out is None:
             out = torch.empty((height, width, 3), dtype=torch.float32, device=f"cuda:{self.device}")
         self._check(self._lib.ar_render(self._h, C.byref(cam), C.byref(o), _ptr(out), _stream_ptr(stream)),

@@ -92,7 +92,7 @@ class ar_render_opts(C.Structure):
                 ("seed", C.c_uint64), ("max_depth", C.c_uint32), ("rr_min_depth", C.c_uint32), ("nee", C.c_uint32),
                 ("sky", C.c_uint32), ("env", C.c_float * 3), ("max_paths_per_pass", C.c_uint32),
                 ("spectral", C.c_uint32), ("output", C.c_uint32), ("guiding", C.c_uint32),
-                ("frame_spp", C.c_uint32)]
+                ("frame_spp", C.c_uint32), ("frame_growth", C.c_uint32)]
 
 
 class ar_mlp_opts(C.Structure):
@@ -194,6 +194,8 @@ SIGNATURES = {
                                   C.c_void_p, C.c_void_p]),
     "ar_mlp_train": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_uint32,
                                C.c_void_p, C.c_void_p]),
+    "ar_mlp_records_get": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t,
+                                     C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     "ar_guiding_reset": (C.c_int, [C.c_void_p]),
     "ar_guiding_info_get": (C.c_int, [C.c_void_p, C.POINTER(ar_guiding_info)]),
     "ar_guiding_snapshot": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64,

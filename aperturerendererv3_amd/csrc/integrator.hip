@@ -27,6 +27,7 @@
 
 #include <cfloat>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 namespace ar {
@@ -632,10 +633,16 @@ struct PathArgs {
     uint32_t fh_stride;
 };
 
-// Voxel guiding inputs of the integrator: the probes (read-only during a
-// frame) and the per-path collection slots (CollectRadiance,
+// Guiding inputs of the integrator.  Voxel guiding: the probes (read-only
+// during a frame) and the per-path collection slots (CollectRadiance,
 // render.cuh:345-410): throughput since the slot's vertex (gm), collected
 // radiance (gr), probe key and leaf texel of the slot's sampled direction.
+// Learned guiding: a slot is opened at a non-delta vertex (inputs, sampled
+// direction, its pdf) and completed when the path's next vertex is shaded,
+// with the path's throughput there (gm) and its radiance before anything
+// that vertex adds, less the vertex's own emission without MIS weight (gr);
+// at the end of the frame (radiance - gr) / gm is the radiance that arrived
+// along the slot's direction: emission, NEE and every later bounce.
 struct GuideArgs {
     guide::Probes pr;
     const uint32_t* probe_count;   // device counter; 0 = no probes yet
@@ -643,7 +650,8 @@ struct GuideArgs {
     float* gr;                     // [5*4][stride]
     uint64_t* gkey;                // [5][stride]
     uint32_t* gleaf;               // 6 bits per slot
-    uint32_t* gflag;               // bit i: slot i received radiance
+    uint32_t* gflag;               // bit i: slot i received radiance (voxel) / is complete (learned);
+                                   // bit 8 + i (learned): slot i awaits its next vertex
     uint32_t stride;
     // learned guiding
     const float* dec;              // decoded mixture per queue slot [q][48]
@@ -687,22 +695,13 @@ __device__ __forceinline__ void g_endpoint(const GuideArgs& G, uint32_t p, const
 // CollectRadiance at a surviving vertex (render.cuh:378-409): opens the slot
 // of this non-delta depth, then scales the open slots by contrib / q.
 __device__ __forceinline__ void g_advance(const GuideArgs& G, uint32_t p, uint32_t non_delta, uint32_t mode,
-                                          uint64_t key, int leaf, const S4& contrib, float qrr, bool learned,
-                                          const float* xr, F3 dir, float pdf, bool start) {
+                                          uint64_t key, int leaf, const S4& contrib, float qrr, bool start) {
     if (start && non_delta < 5u) {
         const int i = (int)non_delta;
         if (s4zero(gload(G.gm, G.stride, i, p), mode)) {
             gstore(G.gm, G.stride, i, p, s4(1.0f, 1.0f, 1.0f, 1.0f));
-            if (learned) {
-                for (int k = 0; k < mlp::kRec; ++k) G.gx[(size_t)(mlp::kRec * i + k) * G.stride + p] = xr[k];
-                G.gw[(size_t)(3 * i + 0) * G.stride + p] = dir.x;
-                G.gw[(size_t)(3 * i + 1) * G.stride + p] = dir.y;
-                G.gw[(size_t)(3 * i + 2) * G.stride + p] = dir.z;
-                G.gp[(size_t)i * G.stride + p] = pdf;
-            } else {
-                G.gkey[(size_t)i * G.stride + p] = key;
-                G.gleaf[p] = (G.gleaf[p] & ~(63u << (6 * i))) | ((uint32_t)leaf << (6 * i));
-            }
+            G.gkey[(size_t)i * G.stride + p] = key;
+            G.gleaf[p] = (G.gleaf[p] & ~(63u << (6 * i))) | ((uint32_t)leaf << (6 * i));
         }
     }
     const float invq = (float)(1.0 / (double)qrr);
@@ -712,6 +711,21 @@ __device__ __forceinline__ void g_advance(const GuideArgs& G, uint32_t p, uint32
         const S4 m = gload(G.gm, G.stride, i, p);
         if (!s4zero(m, mode)) gstore(G.gm, G.stride, i, p, s4mul(m, c));
     }
+}
+
+// Learned guiding: opens slot i at a non-delta vertex; it is completed when
+// the path's next vertex is shaded (see GuideArgs).
+__device__ __forceinline__ uint32_t g_open_learned(const GuideArgs& G, uint32_t p, uint32_t non_delta, uint32_t flags,
+                                                   const float* xr, F3 dir, float pdf) {
+    if (non_delta >= 5u) return flags;
+    const int i = (int)non_delta;
+    if ((flags >> i) & 0x101u) return flags;
+    for (int k = 0; k < mlp::kRec; ++k) G.gx[(size_t)(mlp::kRec * i + k) * G.stride + p] = xr[k];
+    G.gw[(size_t)(3 * i + 0) * G.stride + p] = dir.x;
+    G.gw[(size_t)(3 * i + 1) * G.stride + p] = dir.y;
+    G.gw[(size_t)(3 * i + 2) * G.stride + p] = dir.z;
+    G.gp[(size_t)i * G.stride + p] = pdf;
+    return flags | (1u << (8 + i));
 }
 
 // maskForProbeQuery (render.cuh:893-900): max(w, 1e-6) / (sum + 1e-6) with w
@@ -791,15 +805,15 @@ __global__ __launch_bounds__(kMaxBlock) void k_camera(RenderParams P, PathArgs S
     S.lp0[p] = 0.0f; S.lp1[p] = 0.0f; S.lp2[p] = 0.0f;
     S.meta[p] = 0u;
     S.lambda0[p] = l0;
-    if (P.guiding) {
+    if (P.guiding == AR_GUIDING_VOXEL) {
         for (int i = 0; i < guide::kSlots; ++i) {
             gstore(G.gm, G.stride, i, p, s4(0, 0, 0, 0));
             gstore(G.gr, G.stride, i, p, s4(0, 0, 0, 0));
             G.gkey[(size_t)i * G.stride + p] = guide::kNoKey;
         }
         G.gleaf[p] = 0u;
-        G.gflag[p] = 0u;
     }
+    if (P.guiding) G.gflag[p] = 0u;
 }
 
 // Network inputs of every queued hit (the learned guide's inference batch):
@@ -884,7 +898,14 @@ __device__ __forceinline__ void shade_one(
         S4 mask = load4(S.mask0, S.mask1, S.mask2, S.mask3, p);
         S4 rad = load4(S.rad0, S.rad1, S.rad2, S.rad3, p);
         constexpr bool learned = GM == AR_GUIDING_MLP;
-        const bool guided = learned ? P.alpha > 0.0f : (GM != AR_GUIDING_OFF && *G.probe_count > 0u);
+        // learned guiding: the slot opened at the previous non-delta vertex is
+        // completed here with the entry throughput and radiance
+        const uint32_t lflags_in = learned ? G.gflag[p] : 0u;
+        uint32_t lflags = lflags_in;
+        const S4 mask_in = mask, rad_in = rad;
+        S4 le_unw = s4(0, 0, 0, 0), le_add = s4(0, 0, 0, 0);   // this vertex's emission: unweighted, as added
+        const bool guided = learned ? P.alpha > 0.0f && depth + 1 < P.max_depth
+                                    : (GM != AR_GUIDING_OFF && *G.probe_count > 0u);
         float gq[3] = {1.0f, 1.0f, 1.0f};
         if (GM != AR_GUIDING_OFF && depth > 0) probe_query(mask, L, sc.cie, gq);
         const F3 o = f3(cur.ox[q], cur.oy[q], cur.oz[q]);
@@ -906,9 +927,8 @@ __device__ __forceinline__ void shade_one(
             }
             new_meta = meta & ~kMetaNee;
             if (alive && depth + 1 < P.max_depth) {
-                if (GM != AR_GUIDING_OFF)
-                    g_advance(G, p, non_delta, P.spectral, guide::kNoKey, 0, s4(1, 1, 1, 1), qrr, learned, nullptr, o,
-                              1.0f, false);
+                if (GM == AR_GUIDING_VOXEL)
+                    g_advance(G, p, non_delta, P.spectral, guide::kNoKey, 0, s4(1, 1, 1, 1), qrr, false);
                 mask = s4scale(mask, 1.0f / qrr);
                 cont = true;
                 next_o = o;
@@ -942,7 +962,8 @@ __device__ __forceinline__ void shade_one(
                 }
                 rad = s4add(rad, s4mul(mask, e));
             }
-            if (GM != AR_GUIDING_OFF) g_endpoint(G, p, e, non_delta, P.spectral);
+            if (GM == AR_GUIDING_VOXEL) g_endpoint(G, p, e, non_delta, P.spectral);
+            if (learned && P.sky != AR_SKY_NONE) { le_unw = e; le_add = s4mul(mask, e); }
         } else {
             const bool is_obj = prim >= sc.ntri;
             const float4* ob = is_obj ? sc.objs + (size_t)obj::kRecord * (prim - sc.ntri) : nullptr;
@@ -1015,7 +1036,8 @@ __device__ __forceinline__ void shade_one(
                     }
                     const S4 le = emission_of(m, L, sc);
                     rad = s4add(rad, s4scale(s4mul(mask, le), w));
-                    if (GM != AR_GUIDING_OFF) g_endpoint(G, p, s4scale(le, w), non_delta, P.spectral);
+                    if (GM == AR_GUIDING_VOXEL) g_endpoint(G, p, s4scale(le, w), non_delta, P.spectral);
+                    if (learned) { le_unw = le; le_add = s4scale(s4mul(mask, le), w); }
                 }
             } else if (m.model == AR_MODEL_GLASS) {
                 // Fresnel reflection or refraction (SampleBSDF MAT_GLASS,
@@ -1094,9 +1116,8 @@ __device__ __forceinline__ void shade_one(
                     if (u[9] > qrr) alive = false;
                 }
                 if (alive && depth + 1 < P.max_depth) {
-                    if (GM != AR_GUIDING_OFF)
-                        g_advance(G, p, non_delta, P.spectral, guide::kNoKey, 0, contrib, qrr, learned, nullptr, next,
-                                  pdf, false);
+                    if (GM == AR_GUIDING_VOXEL)
+                        g_advance(G, p, non_delta, P.spectral, guide::kNoKey, 0, contrib, qrr, false);
                     mask = s4scale(s4mul(mask, contrib), 1.0f / qrr);
                     cont = true;
                     next_o = pos;
@@ -1114,8 +1135,33 @@ __device__ __forceinline__ void shade_one(
                 const float pspec = m.model == AR_MODEL_LAMBERT ? 0.0f : pspec_of(m, alb, P.spectral);
                 F3 next;
                 float pdf;
-                if (sample_bsdf(m, pspec, d, nn, u, next, pdf)) {
+                // the learned guide: choose the guide (probability alpha) or the
+                // BSDF first and sample only the chosen one, so the direction's
+                // density is exactly the one-sample mixture (1 - alpha) p_bsdf +
+                // alpha p_guide however sharp the guide's lobes are
+                const float* pmix = nullptr;
+                bool via_guide = false, sampled;
+                if (learned && guided) {
+                    pmix = G.dec + (size_t)q * mlp::kDecoded;
+                    via_guide = u[10] < P.alpha;
+                }
+                if (via_guide) {
+                    float ug[4];
+                    uniforms(P.seed, pix, smp, depth, 3, ug);
+                    float gx, gy, gz;
+                    mlp::hemi_sample(pmix, nn.x, nn.y, nn.z, ug, gx, gy, gz, pdf);
+                    next = f3(gx, gy, gz);
+                    sampled = dot(next, nn) > 0.0f && pdf > 0.0f;
+                } else {
+                    sampled = sample_bsdf(m, pspec, d, nn, u, next, pdf);
+                }
+                if (sampled) {
                     S4 fcos = eval_bsdf(m, alb, d, next, nn);
+                    if (pmix) {
+                        const float pb = via_guide ? bsdf_pdf(m, pspec, d, next, nn) : pdf;
+                        const float pg = via_guide ? pdf : mlp::hemi_pdf(pmix, next.x, next.y, next.z, nn.x, nn.y, nn.z);
+                        pdf = (1.0f - P.alpha) * pb + P.alpha * pg;
+                    }
                     float xr[mlp::kRec] = {0, 0, 0, 0, 0, 0};
                     if (learned) {
                         mlp::features(pos.x, pos.y, pos.z, nn.x, nn.y, nn.z, V.box_lo, V.box_inv, xr);
@@ -1126,42 +1172,15 @@ __device__ __forceinline__ void shade_one(
                     const float dn = dot(nn, next);
                     const float sgn = dn > 0.0f ? 1.0f : (dn < 0.0f ? -1.0f : 0.0f);
                     pos = add(pos, scale(nn, sgn * off));
-                    // guide/BSDF one-sample mixture (render.cuh:553-638)
+                    // voxel guide/BSDF one-sample mixture (render.cuh:553-638)
                     const float4* ptex = nullptr;
-                    const float* pmix = nullptr;
                     uint64_t gkey = guide::kNoKey;
                     if (GM == AR_GUIDING_VOXEL) {
                         const F3 rv = sub(d, scale(nn, 2.0f * dot(d, nn)));
                         gkey = P.batched ? guide::voxel_key_scene(pos.x, pos.y, pos.z, rv.x, rv.y, rv.z, S.scn[p])
                                          : guide::voxel_key(pos.x, pos.y, pos.z, rv.x, rv.y, rv.z);
                     }
-                    if (guided && learned) {
-                        // the network's mixture for this vertex (decoded by the inference pass)
-                        pmix = G.dec + (size_t)q * mlp::kDecoded;
-                        const float pb = pdf;
-                        const float pp = mlp::hemi_pdf(pmix, next.x, next.y, next.z, nn.x, nn.y, nn.z);
-                        if (pb > 1e-12f && pp > 1e-12f) {
-                            if (u[10] < P.alpha) {
-                                float ug[4];
-                                uniforms(P.seed, pix, smp, depth, 3, ug);
-                                float gx, gy, gz, gp;
-                                mlp::hemi_sample(pmix, nn.x, nn.y, nn.z, ug, gx, gy, gz, gp);
-                                const F3 gv = f3(gx, gy, gz);
-                                if (dot(gv, nn) > 1e-12f) {
-                                    const float pbs = bsdf_pdf(m, pspec, d, gv, nn);
-                                    next = gv;
-                                    fcos = eval_bsdf(m, alb, d, gv, nn);
-                                    pdf = (1.0f - P.alpha) * maxr(pbs, 1e-12f) + P.alpha * gp;
-                                } else {
-                                    pmix = nullptr;
-                                }
-                            } else {
-                                pdf = (1.0f - P.alpha) * pb + P.alpha * pp;
-                            }
-                        } else {
-                            pmix = nullptr;
-                        }
-                    } else if (guided) {
+                    if (GM == AR_GUIDING_VOXEL && guided) {
                         const uint32_t pi = guide::find(G.pr, gkey);
                         if (pi != guide::kNoProbe) {
                             ptex = G.pr.tex + (size_t)pi * guide::kTexels;
@@ -1281,10 +1300,10 @@ __device__ __forceinline__ void shade_one(
                         if (u[9] > qrr) alive = false;
                     }
                     if (alive && depth + 1 < P.max_depth) {
-                        if (GM != AR_GUIDING_OFF)
-                            g_advance(G, p, non_delta, P.spectral, gkey,
-                                      learned ? 0 : guide::leaf_of(next.x, next.y, next.z), contrib, qrr, learned, xr,
-                                      next, pdf, medium == 0u);
+                        if (GM == AR_GUIDING_VOXEL)
+                            g_advance(G, p, non_delta, P.spectral, gkey, guide::leaf_of(next.x, next.y, next.z),
+                                      contrib, qrr, medium == 0u);
+                        if (learned && medium == 0u) lflags = g_open_learned(G, p, non_delta, lflags, xr, next, pdf);
                         mask = s4scale(s4mul(mask, contrib), 1.0f / qrr);
                         cont = true;
                         next_o = pos;
@@ -1299,6 +1318,19 @@ __device__ __forceinline__ void shade_one(
                     }
                 }
             }
+        }
+        if (learned) {
+            const uint32_t pend = (lflags_in >> 8) & 31u;
+            if (pend) {
+                const int j = __ffs(pend) - 1;
+                const S4 me = s4mul(mask_in, le_unw);
+                const S4 base = s4(rad_in.v[0] + le_add.v[0] - me.v[0], rad_in.v[1] + le_add.v[1] - me.v[1],
+                                   rad_in.v[2] + le_add.v[2] - me.v[2], rad_in.v[3] + le_add.v[3] - me.v[3]);
+                gstore(G.gr, G.stride, j, p, base);
+                gstore(G.gm, G.stride, j, p, mask_in);
+                lflags = (lflags & ~(1u << (8 + j))) | (1u << j);
+            }
+            if (lflags != lflags_in) G.gflag[p] = lflags;
         }
         S.mask0[p] = mask.v[0]; S.mask1[p] = mask.v[1]; S.mask2[p] = mask.v[2]; S.mask3[p] = mask.v[3];
         S.rad0[p] = rad.v[0]; S.rad1[p] = rad.v[1]; S.rad2[p] = rad.v[2]; S.rad3[p] = rad.v[3];
@@ -1888,7 +1920,7 @@ static bool same_settings(const ar_render_opts& a, const ar_render_opts& b) {
     return a.seed == b.seed && a.max_depth == b.max_depth && a.rr_min_depth == b.rr_min_depth && a.nee == b.nee &&
            a.sky == b.sky && a.env[0] == b.env[0] && a.env[1] == b.env[1] && a.env[2] == b.env[2] &&
            a.spectral == b.spectral && a.output == b.output && a.guiding == b.guiding && a.frame_spp == b.frame_spp &&
-           a.max_paths_per_pass == b.max_paths_per_pass;
+           a.frame_growth == b.frame_growth && a.max_paths_per_pass == b.max_paths_per_pass;
 }
 
 static int make_job(ar_ctx* ctx, uint32_t n, const ar_camera* cams, const ar_render_opts* opts, float* const* outs,
@@ -1942,10 +1974,13 @@ static int make_job(ar_ctx* ctx, uint32_t n, const ar_camera* cams, const ar_ren
         max_npix = std::max(max_npix, npix);
         if (L.pixels > 0xFFFFFFFFull) return set_error(ctx, AR_E_CAPACITY, "batch larger than 2^32 pixels");
     }
-    // guiding: frames of frame_spp samples per pixel of every scene with a
-    // guide update after each; one pass per frame
+    // guiding: frames of frame_spp samples per pixel of every scene (times
+    // frame_growth from one frame to the next) with a guide update after
+    // each; one pass per frame, of at most 2^26 paths
     const uint32_t frame_spp = std::max(1u, o.frame_spp);
-    if (J.guiding && L.pixels * frame_spp > ((size_t)1 << 26))
+    const uint32_t growth = std::max(1u, o.frame_growth);
+    constexpr size_t kMaxFramePaths = (size_t)1 << 26;
+    if (J.guiding && L.pixels * frame_spp > kMaxFramePaths)
         return set_error(ctx, AR_E_CAPACITY, "guided frame larger than 2^26 paths");
     const size_t cap = J.guiding ? (size_t)L.pixels * frame_spp : std::max(max_paths, max_npix);
     RenderPlan::Pass cur{0, 0, 0, 0, 0};
@@ -1961,11 +1996,16 @@ static int make_job(ar_ctx* ctx, uint32_t n, const ar_camera* cams, const ar_ren
         // frame f: samples f * frame_spp .. of every scene that has them
         uint32_t max_spp = 0;
         for (const SceneView& v : L.views) max_spp = std::max(max_spp, v.spp);
-        for (uint32_t done = 0; done < max_spp; done += frame_spp) {
+        uint32_t fs = frame_spp;
+        for (uint32_t done = 0; done < max_spp;) {
+            size_t px = 0;   // pixels of the scenes this frame renders
+            for (const SceneView& v : L.views)
+                if (done < v.spp) px += v.npix;
+            const uint32_t f = (uint32_t)std::max<size_t>(frame_spp, std::min<size_t>(fs, kMaxFramePaths / px));
             for (uint32_t i = 0; i < n; ++i) {
                 const SceneView& v = L.views[i];
                 if (done >= v.spp) continue;
-                const uint32_t k = std::min(frame_spp, v.spp - done);
+                const uint32_t k = std::min(f, v.spp - done);
                 Segment sg{};
                 sg.scene = i; sg.sample0 = v.sample_offset + done; sg.count = k;
                 sg.base = cur.paths; sg.pixbase = cur.pixels;
@@ -1976,6 +2016,8 @@ static int make_job(ar_ctx* ctx, uint32_t n, const ar_camera* cams, const ar_ren
                 cur.pixels += v.npix;
             }
             close();
+            done += f;
+            fs = (uint32_t)std::min<size_t>((size_t)fs * growth, kMaxFramePaths);
         }
     }
     for (uint32_t i = 0; i < n && !J.guiding; ++i) {
@@ -2158,9 +2200,11 @@ static int enqueue_job(ar_ctx* ctx, RenderJob& J, hipStream_t stream) {
         P.capacity = pass.paths;
         P.npixels = pass.pixels;
         if (J.guiding) {
-            // GetPathGuidingProb (render.cuh:195-200): lerp(0, 0.5, min(1, frame/64)^0.5)
+            // GetPathGuidingProb (render.cuh:195-200): lerp(0, 0.5, min(1, frame/64)^0.5);
+            // the learned guide ramps to kGuideProb instead
             const float t = std::pow(std::fmin(1.0f, (float)pass.sample_base / 64.0f), 0.5f);
-            P.alpha = std::fmin(std::fmax(0.0f * (1.0f - t) + 0.5f * t, 0.0f), 1.0f);
+            const float amax = J.learned ? mlp::kGuideProb : 0.5f;
+            P.alpha = std::fmin(std::fmax(0.0f * (1.0f - t) + amax * t, 0.0f), 1.0f);
         }
         // grid-stride kernels: enough blocks to fill every CU several times over
         const uint32_t stride_grid = std::min<uint32_t>(grid_for(P.capacity, blk), (uint32_t)ctx->num_cus * (2048u / blk) * 2u);
@@ -2178,7 +2222,8 @@ static int enqueue_job(ar_ctx* ctx, RenderJob& J, hipStream_t stream) {
             if (count)
                 hipLaunchKernelGGL(k_count_sum, dim3(stride_grid), dim3(256), 0, stream, ctx->cnt_box, ctx->cnt_tri,
                                    &b.counters[0], ctx->cnt_totals, 1);
-            if (J.learned && P.alpha > 0.0f) {
+            if (J.learned && P.alpha > 0.0f && depth + 1 < max_depth) {
+                // (the last bounce samples no direction and takes no NEE sample)
                 const SceneDev& sd = ctx->scene;
                 hipLaunchKernelGGL(k_mlp_features, dim3(grid_for(P.capacity, blk)), dim3(blk), 0, stream, sd.prims,
                                    sd.objs, ctx->sdf.dev, sd.ntri, q[c], b.qpath[c], b.scn, P.views, b.hit_prim,

@@ -731,17 +731,24 @@ uint32_t train_grid(const MlpDev& M, size_t n) {
 }
 
 struct Slots {
-    const float* gr;       // [5*4][stride] collected radiance
-    const uint32_t* gflag;
+    const float* gr;       // [5*4][stride] path radiance where the slot was completed, less that vertex's emission
+    const float* gm;       // [5*4][stride] path throughput there
+    const uint32_t* gflag; // bit i: slot i complete
     const float* gx;       // [5*kRec][stride]
     const float* gw;       // [5*3][stride]
     const float* gp;       // [5][stride]
+    const float* rad[4];   // the paths' radiance at the end of the frame
     uint32_t stride;
 };
 
-// Training records of a frame: every slot that received radiance gives
-// (inputs, direction, luminance(L) / pdf(direction)), the Monte-Carlo weight
-// of the KL divergence from the incident radiance to the guide.  The records
+// Training records of a frame: every complete slot gives (inputs, direction,
+// luminance(L) / pdf(direction)), with L = (radiance at the end of the path -
+// gr) / gm the radiance that arrived along the slot's direction (the next
+// vertex's unweighted emission, its NEE and every later bounce): the
+// Monte-Carlo weight of the KL divergence from the incident radiance to the
+// guide (Muller et al. 2019).  Weighting by the cosine as well (the
+// integrand of a diffuse vertex) trained a worse guide for the one-sample
+// mixture with the BSDF, which supplies the cosine.  The records
 // are numbered in (path, slot) order by a block count, a scan and a write
 // pass, so a frame's records and the trained network do not depend on
 // scheduling.  When a frame has T records and the training batch is B < T,
@@ -757,10 +764,19 @@ __device__ __forceinline__ uint32_t record_weights(const Slots& S, uint32_t p, u
     for (int i = 0; i < 5; ++i) {
         w[i] = 0.0f;
         if ((flags >> i) & 1u) {
-            const float* r = S.gr + (size_t)(4 * i) * S.stride + p;
+            const float* base = S.gr + (size_t)(4 * i) * S.stride + p;
+            const float* thr = S.gm + (size_t)(4 * i) * S.stride + p;
+            float li[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const float m = thr[(size_t)k * S.stride];
+                li[k] = m > 0.0f ? fmaxf(0.0f, (S.rad[k][p] - base[(size_t)k * S.stride]) / m) : 0.0f;
+            }
+            const float* r = li;
+            const size_t rs = 1;
             float lum;
             if (spectral == AR_SPECTRAL_RGB) {
-                lum = r[0] * 0.2126f + r[S.stride] * 0.7152f + r[2 * (size_t)S.stride] * 0.0722f;
+                lum = r[0] * 0.2126f + r[rs] * 0.7152f + r[2 * rs] * 0.0722f;
             } else if (spectral == AR_SPECTRAL_SINGLE) {
                 float cx, cy, cz;
                 spec::cie_xyz(cie, L.l[0], cx, cy, cz);
@@ -770,7 +786,7 @@ __device__ __forceinline__ uint32_t record_weights(const Slots& S, uint32_t p, u
                 for (int k = 0; k < 4; ++k) {
                     float cx, cy, cz;
                     spec::cie_xyz(cie, L.l[k], cx, cy, cz);
-                    lum += cy * r[(size_t)k * S.stride];
+                    lum += cy * r[k * rs];
                 }
                 lum = lum / (4.0f / 400.0f);
             }
@@ -783,6 +799,18 @@ __device__ __forceinline__ uint32_t record_weights(const Slots& S, uint32_t p, u
 }
 
 constexpr uint32_t kRecBlock = 256;
+
+// Path j of the nsel a frame's records come from: one per stratum of
+// npaths / nsel consecutive paths, at a hashed offset inside it (a fixed
+// function of j, so the records stay a deterministic function of the frame).
+__device__ __forceinline__ uint32_t record_path(uint32_t j, uint32_t nsel, uint32_t npaths) {
+    const uint64_t a = (uint64_t)j * npaths / nsel, b = (uint64_t)(j + 1) * npaths / nsel;
+    uint32_t h = j * 0x9E3779B1u;
+    h ^= h >> 16;
+    h *= 0x85EBCA6Bu;
+    h ^= h >> 13;
+    return (uint32_t)(a + h % (uint32_t)(b - a));
+}
 
 // in-block exclusive scan of c (Hillis-Steele over LDS); returns the prefix, *total the block sum
 __device__ __forceinline__ uint32_t block_exclusive(uint32_t* sc, uint32_t c, uint32_t* total) {
@@ -799,12 +827,14 @@ __device__ __forceinline__ uint32_t block_exclusive(uint32_t* sc, uint32_t c, ui
     return sc[t] - c;
 }
 
-__global__ __launch_bounds__(kRecBlock) void k_rec_count(Slots S, uint32_t npaths, uint32_t spectral,
+__global__ __launch_bounds__(kRecBlock) void k_rec_count(Slots S, uint32_t npaths, uint32_t nsel, uint32_t spectral,
                                                          const float* __restrict__ lambda0,
                                                          const float* __restrict__ cie, uint32_t* __restrict__ blk) {
     __shared__ uint32_t sc[kRecBlock];
     float w[5];
-    const uint32_t m = record_weights(S, blockIdx.x * kRecBlock + threadIdx.x, npaths, spectral, lambda0, cie, w);
+    const uint32_t j = blockIdx.x * kRecBlock + threadIdx.x;
+    const uint32_t m = record_weights(S, j < nsel ? record_path(j, nsel, npaths) : npaths, npaths, spectral, lambda0,
+                                      cie, w);
     uint32_t total;
     (void)block_exclusive(sc, (uint32_t)__popc(m), &total);
     if (threadIdx.x == 0) blk[blockIdx.x] = total;
@@ -831,14 +861,15 @@ __global__ __launch_bounds__(1024) void k_rec_scan(uint32_t* __restrict__ blk, u
     }
 }
 
-__global__ __launch_bounds__(kRecBlock) void k_mlp_records(Slots S, uint32_t npaths, uint32_t spectral,
+__global__ __launch_bounds__(kRecBlock) void k_mlp_records(Slots S, uint32_t npaths, uint32_t nsel, uint32_t spectral,
                                                            const float* __restrict__ lambda0,
                                                            const float* __restrict__ cie,
                                                            const uint32_t* __restrict__ blk,
                                                            const uint32_t* __restrict__ count, float* rx, float* rd,
                                                            float* rw) {
     __shared__ uint32_t sc[kRecBlock];
-    const uint32_t p = blockIdx.x * kRecBlock + threadIdx.x;
+    const uint32_t j0 = blockIdx.x * kRecBlock + threadIdx.x;
+    const uint32_t p = j0 < nsel ? record_path(j0, nsel, npaths) : npaths;
     float w[5];
     const uint32_t m = record_weights(S, p, npaths, spectral, lambda0, cie, w);
     uint32_t total;
@@ -866,14 +897,16 @@ __global__ __launch_bounds__(kRecBlock) void k_mlp_records(Slots S, uint32_t npa
 
 int mlp_ensure_buffers(ar_ctx* ctx, size_t paths) {
     MlpDev& M = ctx->mlp;
-    if (M.cap >= paths) return AR_OK;
+    // a frame's records fill at most one training batch
+    const size_t rcap = std::min<size_t>(5 * paths, M.batch);
+    if (M.cap >= paths && M.rcap >= rcap) return AR_OK;
+    paths = std::max(paths, M.cap);
     invalidate_graphs(ctx);
     for (float** q : {&M.feat, &M.dec, &M.rx, &M.rd, &M.rw})
         if (*q) { (void)hipFree(*q); *q = nullptr; }
     if (M.rcount) { (void)hipFree(M.rcount); M.rcount = nullptr; }
     if (M.rblk) { (void)hipFree(M.rblk); M.rblk = nullptr; }
     M.cap = M.rcap = 0;
-    const size_t rcap = 5 * paths;
     hipError_t e = hipMalloc((void**)&M.feat, paths * kRec * sizeof(float));
     if (e == hipSuccess) e = hipMalloc((void**)&M.dec, paths * kDecoded * sizeof(float));
     if (e == hipSuccess) e = hipMalloc((void**)&M.rx, rcap * kRec * sizeof(float));
@@ -891,14 +924,18 @@ int mlp_frame_update(ar_ctx* ctx, uint32_t spectral, const float* lambda0, const
                      hipStream_t s) {
     MlpDev& M = ctx->mlp;
     const GuideDev& g = ctx->guide;
-    Slots S{g.gr, g.gflag, g.gx, g.gw, g.gp, (uint32_t)g.pcap};
+    const RenderBuffers& b = ctx->rb;
+    Slots S{g.gr, g.gm, g.gflag, g.gx, g.gw, g.gp, {b.rad[0], b.rad[1], b.rad[2], b.rad[3]}, (uint32_t)g.pcap};
     const size_t n = std::min<size_t>(M.rcap, M.batch);
-    const uint32_t nb = (uint32_t)((paths + kRecBlock - 1) / kRecBlock);
-    hipLaunchKernelGGL(k_rec_count, dim3(nb), dim3(kRecBlock), 0, s, S, (uint32_t)paths, spectral, lambda0, cie,
+    // records come from one path per batch slot (each gives up to 5), spread
+    // evenly over the frame's paths
+    const uint32_t nsel = (uint32_t)std::min<size_t>(paths, n);
+    const uint32_t nb = (nsel + kRecBlock - 1) / kRecBlock;
+    hipLaunchKernelGGL(k_rec_count, dim3(nb), dim3(kRecBlock), 0, s, S, (uint32_t)paths, nsel, spectral, lambda0, cie,
                        M.rblk);
     hipLaunchKernelGGL(k_rec_scan, dim3(1), dim3(1024), 0, s, M.rblk, nb, (uint32_t)n, M.rcount);
-    hipLaunchKernelGGL(k_mlp_records, dim3(nb), dim3(kRecBlock), 0, s, S, (uint32_t)paths, spectral, lambda0, cie,
-                       M.rblk, M.rcount, M.rx, M.rd, M.rw);
+    hipLaunchKernelGGL(k_mlp_records, dim3(nb), dim3(kRecBlock), 0, s, S, (uint32_t)paths, nsel, spectral, lambda0,
+                       cie, M.rblk, M.rcount, M.rx, M.rd, M.rw);
     hipError_t e = hipSuccess;
     for (uint32_t it = 0; it < M.iters && e == hipSuccess; ++it) {
         e = mlp_gradient(ctx, M.rx, kRec, M.rd, M.rw, n, M.rcount, true, s, true);
@@ -1055,6 +1092,26 @@ int ar_mlp_params_set(ar_ctx* ctx, const float* in, size_t count) {
     if (rc != AR_OK) return rc;
     (void)hipDeviceSynchronize();
     return mlp_set_params(ctx, in);
+}
+
+int ar_mlp_records_get(ar_ctx* ctx, float* x, float* dir, float* weight, size_t cap, uint32_t* count,
+                       uint32_t* total) {
+    if (!ctx) return AR_E_INVALID;
+    const MlpDev& M = ctx->mlp;
+    uint32_t c[2] = {0, 0};
+    int rc = check_hip(ctx, hipDeviceSynchronize(), "mlp records");
+    if (rc == AR_OK && M.rcount)
+        rc = check_hip(ctx, hipMemcpy(c, M.rcount, sizeof c, hipMemcpyDeviceToHost), "mlp records");
+    if (rc != AR_OK) return rc;
+    if (count) *count = c[0];
+    if (total) *total = c[1];
+    const size_t n = std::min<size_t>(c[0], cap);
+    if (n && x) rc = check_hip(ctx, hipMemcpy(x, M.rx, n * kRec * sizeof(float), hipMemcpyDeviceToHost), "mlp records");
+    if (rc == AR_OK && n && dir)
+        rc = check_hip(ctx, hipMemcpy(dir, M.rd, n * 3 * sizeof(float), hipMemcpyDeviceToHost), "mlp records");
+    if (rc == AR_OK && n && weight)
+        rc = check_hip(ctx, hipMemcpy(weight, M.rw, n * sizeof(float), hipMemcpyDeviceToHost), "mlp records");
+    return rc;
 }
 
 int ar_mlp_forward(ar_ctx* ctx, const float* x5, size_t n, float* out40, void* stream) {

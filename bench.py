@@ -445,11 +445,14 @@ def guiding_bench(ctx_cls, room_spp=(256, 1024), spectral_spp=(32, 128)):
         ms = (time.perf_counter() - t) * 1e3
         return {"spp": spp, "ms": round(ms, 2), "relmse": score(name, img.cpu().numpy())}
 
-    mlp_cfg = dict(learning_rate=5e-3, iters_per_frame=4)
+    # learned: frames of 16, 32, 64, ... spp (growth 2), 32 Adam steps of
+    # 2^17 records after each; voxel: the reference's probes in frames of
+    # 64 (room) / 16 (spectral) spp
+    mlp_cfg = dict(learning_rate=1e-2, iters_per_frame=32, batch=1 << 17)
     methods = {
         "plain": (None, {}),
         "voxel": (lambda c: c.guiding_reset(), dict(guiding=1)),
-        "learned": (lambda c: c.mlp_configure(seed=1, **mlp_cfg), dict(guiding=2)),
+        "learned": (lambda c: c.mlp_configure(seed=1, **mlp_cfg), dict(guiding=2, frame_spp=16, frame_growth=2)),
     }
     frame = {"room": 64, "spectral": 16}
     for name in scenes:
@@ -458,7 +461,7 @@ def guiding_bench(ctx_cls, room_spp=(256, 1024), spectral_spp=(32, 128)):
         res = {}
         for mname, (prep, extra) in methods.items():
             log(f"guiding: {name} {mname} at {levels} spp")
-            ex = dict(extra, frame_spp=frame[name]) if extra else {}
+            ex = dict(dict(frame_spp=frame[name]), **extra) if extra else {}
             res[mname] = [run(name, s, 7 if extra else 1, prep=prep, **ex) for s in levels]
         ratios = {}
         for mname in ("voxel", "learned"):
@@ -482,12 +485,12 @@ def guiding_bench(ctx_cls, room_spp=(256, 1024), spectral_spp=(32, 128)):
         b.mlp_configure(seed=1, **mlp_cfg)
         torch.cuda.synchronize()
         t = time.perf_counter()
-        imgs = b.render_views(views, guiding=2, frame_spp=16, **shared)
+        imgs = b.render_views(views, guiding=2, frame_spp=16, frame_growth=2, **shared)
         torch.cuda.synchronize()
         bms = (time.perf_counter() - t) * 1e3
         errs = {n: score(n, im.cpu().numpy()) for n, im in zip(names, imgs)}
         plain_equiv = sum(out[n]["plain"][-1]["ms"] * out[n]["plain"][-1]["relmse"] / errs[n] for n in names)
-        out["batched_learned"] = {"scenes": names, "spp": [v["spp"] for v in views], "frame_spp": 16,
+        out["batched_learned"] = {"scenes": names, "spp": [v["spp"] for v in views], "frame_spp": 16, "frame_growth": 2,
                                   "ms": round(bms, 2), "relmse": {n: errs[n] for n in names},
                                   "plain_ms": round(sum(out[n]["plain"][-1]["ms"] for n in names), 2),
                                   "time_ratio_at_equal_relmse": round(bms / plain_equiv, 3)}

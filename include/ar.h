@@ -290,6 +290,11 @@ typedef struct {
     uint32_t guiding;             /* AR_GUIDING_* */
     uint32_t frame_spp;           /* guided renders: samples per pixel per frame, i.e. between guide
                                      updates (0: 1, the reference's frame) */
+    uint32_t frame_growth;        /* guided renders: each frame has this many times the samples of the
+                                     one before (0 or 1: every frame has frame_spp; 2: frame_spp,
+                                     2 frame_spp, 4 frame_spp, ...), the last frame what remains, and no
+                                     frame more paths than 2^26 allow (no counterpart in the reference,
+                                     whose frames have one sample) */
 } ar_render_opts;
 
 /* Path guiding.  VOXEL: the reference's voxel guiding (core/guiding,
@@ -569,6 +574,15 @@ int ar_mlp_gradient(ar_ctx* ctx, const float* x5, const float* dir, const float*
  * loss (device, may be NULL) receives the last step's loss. */
 int ar_mlp_train(ar_ctx* ctx, const float* x5, const float* dir, const float* weight, size_t n, uint32_t steps,
                  float* loss, void* stream);
+/* The training records of the last learned-guided frame, copied to host
+ * after the device is idle (no counterpart in the reference; for inspecting
+ * what the guide is trained on): *count = the records trained on (at most
+ * cap are copied), *total = the frame's records before the batch was drawn;
+ * x: count x 6 (the network inputs and the scene index), dir: count x 3,
+ * weight: count (luminance of the radiance that arrived along dir, over the
+ * pdf dir was sampled with).  Any output may be NULL. */
+int ar_mlp_records_get(ar_ctx* ctx, float* x, float* dir, float* weight, size_t cap, uint32_t* count,
+                       uint32_t* total);
 
 /* --- spectral uplift table --------------------------------------------------
  * Replaces PrecomputeRGBLUTKernel / SetupSpectrumLUT (core/spectrum/

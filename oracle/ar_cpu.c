@@ -479,7 +479,7 @@ static int render_opts(ar_ctx* c, const ar_camera* cam, const ar_render_opts* o,
         return set_error(c, AR_E_UNSUPPORTED, "the CPU backend renders RGB only");
     if (o->guiding == AR_GUIDING_MLP) return set_error(c, AR_E_UNSUPPORTED, "the CPU backend has no learned guiding");
     if (o->guiding > AR_GUIDING_MLP) return set_error(c, AR_E_INVALID, "unknown guiding mode");
-    if (o->guiding == AR_GUIDING_VOXEL && o->frame_spp > 1)
+    if (o->guiding == AR_GUIDING_VOXEL && (o->frame_spp > 1 || o->frame_growth > 1))
         return set_error(c, AR_E_UNSUPPORTED, "the CPU backend guides frames of one sample");
     if (o->sky > AR_SKY_CONSTANT) return set_error(c, AR_E_INVALID, "unknown sky");
     memset(r, 0, sizeof *r);
@@ -813,6 +813,13 @@ int ar_render_counters_read(ar_ctx* c, uint64_t out[5]) { (void)out; return UNSU
 int ar_mlp_configure(ar_ctx* c, const ar_mlp_opts* opts) { (void)opts; return UNSUPPORTED(c); }
 int ar_mlp_params_get(ar_ctx* c, float* host_out, size_t count) { (void)host_out; (void)count; return UNSUPPORTED(c); }
 int ar_mlp_params_set(ar_ctx* c, const float* host_in, size_t count) { (void)host_in; (void)count; return UNSUPPORTED(c); }
+
+int ar_mlp_records_get(ar_ctx* c, float* x, float* dir, float* weight, size_t cap, uint32_t* count,
+                       uint32_t* total)
+{
+    (void)x; (void)dir; (void)weight; (void)cap; (void)count; (void)total;
+    return UNSUPPORTED(c);
+}
 
 int ar_mlp_forward(ar_ctx* c, const float* x5, size_t n, float* out40, void* stream)
 {

@@ -11,9 +11,10 @@ variance estimate).
   plus the fixture's, and at least 1 %.  The spectral window is dark (mean
   5.7e-5, lit mostly by caustic paths to 64 small lights), so its image
   mean carries 2.3e-3 relative noise at 16,384 spp and about 2.7 % at 128.
-- On the room, the learned guide (frames of 64 spp, 4 Adam steps each)
-  lowers the error at equal spp below 0.9x plain (measured 0.71-0.81x) and
-  voxel guiding below plain (measured 0.85-0.88x).
+- On the room, the learned guide (frames of 16, 32, 64, ... spp, 32 Adam
+  steps of 2^17 records after each) lowers the error at equal spp below 0.6x
+  plain and voxel guiding (frames of 64 spp) below plain (measured
+  0.85-0.88x).
 - On the spectral scene (64 small lights, mostly direct light) guiding has
   little to learn and takes up to half of the BSDF samples: it may raise the
   error, but by less than 50 % (measured at 128 spp: voxel 1.29x, learned
@@ -76,8 +77,8 @@ def test_guiding_against_fp64_fixture(name):
     imgs = {"plain": render(1)}
     c.guiding_reset()
     imgs["voxel"] = render(7, guiding=1, frame_spp=frame)
-    c.mlp_configure(seed=1, learning_rate=5e-3, iters_per_frame=4)
-    imgs["learned"] = render(7, guiding=2, frame_spp=frame)
+    c.mlp_configure(seed=1, learning_rate=1e-2, iters_per_frame=32, batch=1 << 17)
+    imgs["learned"] = render(7, guiding=2, frame_spp=16, frame_growth=2)
     err = {k: orc.rel_mse(v, ref) - floor for k, v in imgs.items()}
     for k, v in imgs.items():
         rel_mean = abs(v.mean() - ref.mean()) / ref.mean()
@@ -86,7 +87,7 @@ def test_guiding_against_fp64_fixture(name):
         assert np.isfinite(v).all()
         assert rel_mean < tol, (k, rel_mean, tol)
     if name == "room":
-        assert err["learned"] < 0.9 * err["plain"]
+        assert err["learned"] < 0.6 * err["plain"]
         assert err["voxel"] < err["plain"]
     else:
         assert err["learned"] < 1.5 * err["plain"] and err["voxel"] < 1.5 * err["plain"]

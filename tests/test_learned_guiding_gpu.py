@@ -8,7 +8,13 @@ online on each frame's path vertices and sampled in the BSDF mixture.
   render's at equal spp, averaged over two seeds.
 * Reproducible: a frame's training records are numbered in path order (no
   atomics) and gradients are reduced in a fixed order, so two guided renders
-  with the same seeds give the same image and network, bit for bit.
+  with the same seeds give the same image and network, bit for bit; also with
+  growing frames.
+* Records: on a floor lit by one area light (no sky), a frame rendered before
+  the guide is sampled (cosine-sampled directions) gives records only for
+  directions that reach the light, each of weight lum(Le) pi / cos: the
+  radiance that arrived (the light's emission without MIS weight, without the
+  floor's own NEE sample or BSDF factor) over the direction's pdf.
 """
 import numpy as np
 import pytest
@@ -74,3 +80,50 @@ def test_learned_guiding_reduces_error_on_occluded_light(ctx):
     torch.cuda.synchronize()
     np.testing.assert_array_equal(again.cpu().numpy().view(np.uint32), learned.cpu().numpy().view(np.uint32))
     np.testing.assert_array_equal(ctx.mlp_params().view(np.uint32), params.view(np.uint32))
+
+
+@pytest.mark.parametrize("growth", [1, 2])
+def test_learned_guiding_reproducible_with_frame_growth(ctx, growth):
+    import torch
+    _load(ctx, cornell_box())
+    kw = dict(_cam(CORNELL_CAMERA), max_depth=5)
+    out = []
+    for _ in range(2):
+        ctx.mlp_configure(learning_rate=1e-2, iters_per_frame=8, batch=1 << 16, seed=3)
+        g = ctx.render(96, 96, 120, guiding=2, seed=9, frame_spp=8, frame_growth=growth, **kw)
+        torch.cuda.synchronize()
+        out.append((g.cpu().numpy(), ctx.mlp_params()))
+    np.testing.assert_array_equal(out[0][0].view(np.uint32), out[1][0].view(np.uint32))
+    np.testing.assert_array_equal(out[0][1].view(np.uint32), out[1][1].view(np.uint32))
+    p = ctx.render(96, 96, 1024, seed=10, **kw).reshape(-1, 3).mean(0).cpu().numpy()
+    a = out[0][0].reshape(-1, 3).mean(0)
+    print(f"growth {growth}: guided mean {a}, plain mean {p}", flush=True)
+    assert np.all(np.abs(a - p) / p < 0.01), (a, p)
+
+
+def test_learned_records_are_incident_radiance(ctx):
+    le = (4.0, 2.0, 1.0)
+    V = np.array([[-1, 0, -1], [-1, 0, 1], [1, 0, 1], [1, 0, -1],
+                  [-0.3, 1, -0.3], [0.3, 1, -0.3], [0.3, 1, 0.3], [-0.3, 1, 0.3]], np.float32)
+    F = np.array([[0, 1, 2], [0, 2, 3], [4, 5, 6], [4, 6, 7]], np.uint32)
+    M = np.array([0, 0, 1, 1], np.uint32)
+    mats = [{"albedo": (0.5, 0.5, 0.5)}, {"albedo": (0.0, 0.0, 0.0), "emission": le}]
+    ctx.set_meshes([{"vertices": V, "indices": F, "tri_material": M, "materials": mats}])
+    ctx.build_bvh()
+    ctx.mlp_configure(learning_rate=1e-2, iters_per_frame=1, batch=1 << 18, seed=1)
+    # one frame of 8 samples: the guide is not sampled yet (probability 0 before sample 1)
+    ctx.render(64, 64, 8, cam_pos=(0.0, 0.6, 1.6), cam_dir=(0.0, -0.6, -1.6), vfov=60.0, seed=5, max_depth=5,
+               guiding=2, frame_spp=8)
+    r = ctx.mlp_records()
+    n = r["weight"].shape[0]
+    print(f"{n} records of {r['total']}", flush=True)
+    assert n == r["total"] and n > 1000
+    th = r["x"][:, 3] * np.pi
+    ph = (r["x"][:, 4] - 0.5) * 2.0 * np.pi
+    nrm = np.stack([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)], 1)
+    np.testing.assert_allclose(nrm, np.broadcast_to([0.0, 1.0, 0.0], nrm.shape), atol=1e-5)
+    d = r["dir"].astype(np.float64)
+    assert (d[:, 1] > 0).all()
+    cos = d[:, 1]
+    lum = 0.2126 * le[0] + 0.7152 * le[1] + 0.0722 * le[2]
+    np.testing.assert_allclose(r["weight"] * cos / np.pi, lum, rtol=2e-4)
