@@ -450,6 +450,15 @@ int ar_bvh_build(ar_ctx* ctx, const ar_bvh_opts* opts) {
         ctx->bvh.stack_entries = (uint32_t)br.depth + 2u;
         ctx->info.node_count = cv.nodes.size();
     }
+    // the root node (node 0) carries the tree's origin limit in a spare word,
+    // so that a batch (whose context limit is the smallest of its scenes')
+    // gives every ray its own scene's limit
+    const float olim = kOriginLimit * mag;
+    uint32_t olim_bits;
+    std::memcpy(&olim_bits, &olim, 4);
+    if (o.width == 8 && !nodes8.empty()) nodes8[0].pad[0] = olim_bits;
+    else if (o.width == 4 && !nodes4.empty()) nodes4[0].pad[0] = olim_bits;
+    else if (o.width == 2 && !cv.nodes.empty()) cv.nodes[0].d[2] = olim_bits;
     int rc = check_hip(ctx, hipMalloc((void**)&ctx->bvh.nodes, node_bytes), "hipMalloc(nodes)");
     if (rc != AR_OK) { ctx->bvh.nodes = nullptr; return rc; }
     rc = check_hip(ctx, hipMalloc(&ctx->bvh.tris, tris.size() * sizeof(float)), "hipMalloc(tris)");
@@ -464,7 +473,7 @@ int ar_bvh_build(ar_ctx* ctx, const ar_bvh_opts* opts) {
     ctx->bvh.node_count = (uint32_t)ctx->info.node_count;
     ctx->bvh.tri_slots = (uint32_t)(tris.size() / 12);
     ctx->bvh.root = 0;
-    ctx->bvh.origin_limit = kOriginLimit * mag;
+    ctx->bvh.origin_limit = olim;
     ctx->bvh_ready = true;
     const auto t1 = std::chrono::steady_clock::now();
     ctx->info.leaf_count = leaves;

@@ -57,7 +57,7 @@ struct GpuNode2 {
     float a[4];
     float b[4];
     float c[4];
-    uint32_t d[4];
+    uint32_t d[4];       // ref0, ref1, 0 (the root: the tree's origin limit, float), 0
 };
 static_assert(sizeof(GpuNode2) == 64, "GpuNode2 must be 64 bytes");
 
@@ -80,7 +80,7 @@ struct GpuNode8 {
     uint32_t meta;
     uint32_t ref[8];
     uint8_t q[48];
-    uint32_t pad[8];     // to 128 B: one node is one L2 line
+    uint32_t pad[8];     // to 128 B: one node is one L2 line; pad[0] of the root: the tree's origin limit (float)
 };
 static_assert(sizeof(GpuNode8) == 128, "GpuNode8 must be 128 bytes");
 
@@ -97,7 +97,7 @@ struct GpuNode4 {
     uint32_t meta;
     uint32_t ref[4];
     uint8_t q[24];
-    uint32_t pad[2];
+    uint32_t pad[2];     // pad[0] of the root: the tree's origin limit (float)
 };
 static_assert(sizeof(GpuNode4) == 64, "GpuNode4 must be 64 bytes");
 
@@ -302,7 +302,8 @@ struct MlpDev {
     float* rd = nullptr;        // directions [rcap][3]
     float* rw = nullptr;        // weights [rcap]
     uint32_t* rcount = nullptr; // [0] records trained on this frame (min(T, batch)), [1] records T of the frame
-    uint32_t* rblk = nullptr;   // per 256-path block: record count, then its offset
+    uint32_t* rblk = nullptr;   // per block of 256 selected paths: record count, then its offset
+    size_t rblocks = 0;         // blocks rblk holds
 };
 
 // Spectral basis network (smlp.hip): parameters, AdamW state, training
@@ -498,5 +499,5 @@ hipError_t mlp_adam_step(ar_ctx* ctx, hipStream_t s);
 int mlp_ensure_buffers(ar_ctx* ctx, size_t paths);
 // training records from the frame's guiding slots, then the frame's Adam steps
 int mlp_frame_update(ar_ctx* ctx, uint32_t spectral, const float* lambda0, const float* cie, size_t paths,
-                     hipStream_t s);
+                     const uint32_t* ranges, uint32_t nseg, hipStream_t s);
 } // namespace ar

@@ -96,6 +96,13 @@ uint32_t rebase_leaf(uint32_t r, uint32_t slot_off) { return r == kRefEmpty ? r 
 
 int batch_merge(ar_ctx* out, ar_ctx* const* scenes, uint32_t n) {
     const uint32_t width = scenes[0]->bvh.width;
+    const size_t nb = width == 8 ? sizeof(GpuNode8) : width == 4 ? sizeof(GpuNode4) : sizeof(GpuNode2);
+    // Each scene's nodes and leaf slots start where its own arrays did
+    // relative to the 128-byte lines (the builder places leaves to touch the
+    // fewest lines): node offsets in whole lines, slot offsets in multiples
+    // of 8 slots (8 x 48 B = 3 lines).  The gaps are never referenced.
+    const size_t node_align = 128 / nb, slot_align = 8;
+    auto up = [](size_t x, size_t a) { return (x + a - 1) / a * a; };
     size_t nodes = 0, slots = 0, tris = 0, mats = 0, lights = 0, nobjs = 0, voxels = 0;
     uint32_t stack = 0, nsdf = 0;
     float olim = INFINITY;
@@ -104,8 +111,8 @@ int batch_merge(ar_ctx* out, ar_ctx* const* scenes, uint32_t n) {
         if (!c->bvh_ready) return set_error(out, AR_E_NOSCENE, "batch: a scene has no BVH");
         if (c->bvh.width != width) return set_error(out, AR_E_INVALID, "batch: scenes must share the BVH width");
         if (!c->batch.empty()) return set_error(out, AR_E_INVALID, "batch: a batch cannot contain a batch");
-        nodes += c->bvh.node_count;
-        slots += c->bvh.tri_slots;
+        nodes = up(nodes, node_align) + c->bvh.node_count;
+        slots = up(slots, slot_align) + c->bvh.tri_slots;
         tris += c->scene.ntri;
         mats += c->scene.host_mats.size();
         lights += c->scene.nlights;
@@ -119,7 +126,6 @@ int batch_merge(ar_ctx* out, ar_ctx* const* scenes, uint32_t n) {
         return set_error(out, AR_E_CAPACITY, "batch: more than 2^27 triangle slots");
     if (nobjs >= kObjSlot || voxels > (size_t)UINT32_MAX)
         return set_error(out, AR_E_CAPACITY, "batch: too many objects or SDF voxels");
-    const size_t nb = width == 8 ? sizeof(GpuNode8) : width == 4 ? sizeof(GpuNode4) : sizeof(GpuNode2);
     char* dnodes = nullptr;
     float4* dtris = nullptr;
     float4* dprims = nullptr;
@@ -145,6 +151,11 @@ int batch_merge(ar_ctx* out, ar_ctx* const* scenes, uint32_t n) {
         free_all();
         return rc;
     }
+    // the alignment gaps hold zeros
+    if (hipMemset(dnodes, 0, nodes * nb) != hipSuccess || hipMemset(dtris, 0, 3 * slots * sizeof(float4)) != hipSuccess) {
+        free_all();
+        return set_error(out, AR_E_DEVICE, "batch: hipMemset failed");
+    }
     std::vector<GpuMaterial> hm;
     hm.reserve(mats);
     std::vector<float4> ho;      // merged object records (host copy)
@@ -156,6 +167,8 @@ int batch_merge(ar_ctx* out, ar_ctx* const* scenes, uint32_t n) {
     hipError_t e = hipSuccess;
     for (uint32_t i = 0; i < n && e == hipSuccess; ++i) {
         const ar_ctx* c = scenes[i];
+        on = up(on, node_align);
+        os = up(os, slot_align);
         const uint32_t cn = c->bvh.node_count, cs = c->bvh.tri_slots, ct = c->scene.ntri, cl = c->scene.nlights;
         const uint32_t co = c->scene.nobj;
         const uint32_t vox_off = (uint32_t)pool.host.size();

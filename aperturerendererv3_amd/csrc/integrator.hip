@@ -2117,7 +2117,15 @@ static int prepare_job(ar_ctx* ctx, RenderJob& J, hipStream_t stream, bool sync)
     rc = trace_prologue(ctx, ta, L.cap, tblocks);
     if (rc != AR_OK) return rc;
     const size_t vb = L.views.size() * sizeof(SceneView);
-    const size_t bytes = vb + L.segs.size() * sizeof(Segment);
+    const size_t sb = L.segs.size() * sizeof(Segment);
+    // per segment: its first path and path count (the learned guide's record selection)
+    std::vector<uint32_t> ranges(2 * L.segs.size());
+    for (size_t k = 0; k < L.segs.size(); ++k) {
+        ranges[2 * k] = L.segs[k].base;
+        ranges[2 * k + 1] = L.segs[k].count * L.views[L.segs[k].scene].npix;
+    }
+    const size_t rb = ranges.size() * sizeof(uint32_t);
+    const size_t bytes = vb + sb + rb;
     if (J.dev_bytes < bytes) {
         if (J.dev) (void)hipFree(J.dev);
         J.dev = nullptr;
@@ -2129,8 +2137,11 @@ static int prepare_job(ar_ctx* ctx, RenderJob& J, hipStream_t stream, bool sync)
     if (sync) {
         rc = check_hip(ctx, hipMemcpy(J.dev, L.views.data(), vb, hipMemcpyHostToDevice), "upload render plan");
         if (rc == AR_OK)
-            rc = check_hip(ctx, hipMemcpy((char*)J.dev + vb, L.segs.data(), L.segs.size() * sizeof(Segment),
-                                          hipMemcpyHostToDevice), "upload render plan");
+            rc = check_hip(ctx, hipMemcpy((char*)J.dev + vb, L.segs.data(), sb, hipMemcpyHostToDevice),
+                           "upload render plan");
+        if (rc == AR_OK)
+            rc = check_hip(ctx, hipMemcpy((char*)J.dev + vb + sb, ranges.data(), rb, hipMemcpyHostToDevice),
+                           "upload render plan");
         return rc;
     }
     if (!J.uploaded) {
@@ -2148,7 +2159,8 @@ static int prepare_job(ar_ctx* ctx, RenderJob& J, hipStream_t stream, bool sync)
         J.host_bytes = bytes;
     }
     std::memcpy(J.host, L.views.data(), vb);
-    std::memcpy((char*)J.host + vb, L.segs.data(), L.segs.size() * sizeof(Segment));
+    std::memcpy((char*)J.host + vb, L.segs.data(), sb);
+    std::memcpy((char*)J.host + vb + sb, ranges.data(), rb);
     rc = check_hip(ctx, hipMemcpyAsync(J.dev, J.host, bytes, hipMemcpyHostToDevice, stream), "upload render plan");
     if (rc == AR_OK) rc = check_hip(ctx, hipEventRecord(J.uploaded, stream), "hipEventRecord");
     return rc;
@@ -2193,6 +2205,7 @@ static int enqueue_job(ar_ctx* ctx, RenderJob& J, hipStream_t stream) {
     RenderParams P = J.base;
     P.views = (const SceneView*)J.dev;
     const Segment* segs = (const Segment*)((const char*)J.dev + L.views.size() * sizeof(SceneView));
+    const uint32_t* ranges = (const uint32_t*)(segs + L.segs.size());
     const uint32_t max_depth = P.max_depth;
     for (const RenderPlan::Pass& pass : L.passes) {
         P.segs = segs + pass.seg0;
@@ -2258,7 +2271,8 @@ static int enqueue_job(ar_ctx* ctx, RenderJob& J, hipStream_t stream) {
             c = 1 - c;
         }
         if (J.guiding) {
-            rc = J.learned ? mlp_frame_update(ctx, P.spectral, b.lambda0, b.cie, P.capacity, stream)
+            rc = J.learned ? mlp_frame_update(ctx, P.spectral, b.lambda0, b.cie, P.capacity, ranges + 2 * pass.seg0,
+                                              pass.nseg, stream)
                            : guide_update(ctx, P.spectral, b.lambda0, b.cie, P.capacity, stream);
             if (rc != AR_OK) return rc;
         }

@@ -800,16 +800,26 @@ __device__ __forceinline__ uint32_t record_weights(const Slots& S, uint32_t p, u
 
 constexpr uint32_t kRecBlock = 256;
 
-// Path j of the nsel a frame's records come from: one per stratum of
-// npaths / nsel consecutive paths, at a hashed offset inside it (a fixed
-// function of j, so the records stay a deterministic function of the frame).
-__device__ __forceinline__ uint32_t record_path(uint32_t j, uint32_t nsel, uint32_t npaths) {
-    const uint64_t a = (uint64_t)j * npaths / nsel, b = (uint64_t)(j + 1) * npaths / nsel;
+// The paths a frame's records come from: `quota` per segment (scene) of the
+// pass, so that every scene of a guided batch trains the network alike
+// whatever its size; in a segment of m paths, path k < min(quota, m) is one
+// per stratum of m / min(quota, m) consecutive paths, at a hashed offset
+// inside it (a fixed function of k, so the records stay a deterministic
+// function of the frame).  ranges: (first path, path count) per segment.
+// Returns npaths (no path) for unused selection slots.
+__device__ __forceinline__ uint32_t record_path(uint32_t j, uint32_t quota, const uint32_t* __restrict__ ranges,
+                                               uint32_t nseg, uint32_t npaths) {
+    const uint32_t sg = j / quota, k = j - sg * quota;
+    if (sg >= nseg) return npaths;
+    const uint32_t base = ranges[2 * sg], m = ranges[2 * sg + 1];
+    const uint32_t sel = min(quota, m);
+    if (k >= sel) return npaths;
+    const uint64_t a = (uint64_t)k * m / sel, b = (uint64_t)(k + 1) * m / sel;
     uint32_t h = j * 0x9E3779B1u;
     h ^= h >> 16;
     h *= 0x85EBCA6Bu;
     h ^= h >> 13;
-    return (uint32_t)(a + h % (uint32_t)(b - a));
+    return base + (uint32_t)(a + h % (uint32_t)(b - a));
 }
 
 // in-block exclusive scan of c (Hillis-Steele over LDS); returns the prefix, *total the block sum
@@ -827,14 +837,15 @@ __device__ __forceinline__ uint32_t block_exclusive(uint32_t* sc, uint32_t c, ui
     return sc[t] - c;
 }
 
-__global__ __launch_bounds__(kRecBlock) void k_rec_count(Slots S, uint32_t npaths, uint32_t nsel, uint32_t spectral,
-                                                         const float* __restrict__ lambda0,
+__global__ __launch_bounds__(kRecBlock) void k_rec_count(Slots S, uint32_t npaths, uint32_t quota,
+                                                         const uint32_t* __restrict__ ranges, uint32_t nseg,
+                                                         uint32_t spectral, const float* __restrict__ lambda0,
                                                          const float* __restrict__ cie, uint32_t* __restrict__ blk) {
     __shared__ uint32_t sc[kRecBlock];
     float w[5];
     const uint32_t j = blockIdx.x * kRecBlock + threadIdx.x;
-    const uint32_t m = record_weights(S, j < nsel ? record_path(j, nsel, npaths) : npaths, npaths, spectral, lambda0,
-                                      cie, w);
+    const uint32_t m = record_weights(S, record_path(j, quota, ranges, nseg, npaths), npaths, spectral, lambda0, cie,
+                                      w);
     uint32_t total;
     (void)block_exclusive(sc, (uint32_t)__popc(m), &total);
     if (threadIdx.x == 0) blk[blockIdx.x] = total;
@@ -861,15 +872,15 @@ __global__ __launch_bounds__(1024) void k_rec_scan(uint32_t* __restrict__ blk, u
     }
 }
 
-__global__ __launch_bounds__(kRecBlock) void k_mlp_records(Slots S, uint32_t npaths, uint32_t nsel, uint32_t spectral,
-                                                           const float* __restrict__ lambda0,
+__global__ __launch_bounds__(kRecBlock) void k_mlp_records(Slots S, uint32_t npaths, uint32_t quota,
+                                                           const uint32_t* __restrict__ ranges, uint32_t nseg,
+                                                           uint32_t spectral, const float* __restrict__ lambda0,
                                                            const float* __restrict__ cie,
                                                            const uint32_t* __restrict__ blk,
                                                            const uint32_t* __restrict__ count, float* rx, float* rd,
                                                            float* rw) {
     __shared__ uint32_t sc[kRecBlock];
-    const uint32_t j0 = blockIdx.x * kRecBlock + threadIdx.x;
-    const uint32_t p = j0 < nsel ? record_path(j0, nsel, npaths) : npaths;
+    const uint32_t p = record_path(blockIdx.x * kRecBlock + threadIdx.x, quota, ranges, nseg, npaths);
     float w[5];
     const uint32_t m = record_weights(S, p, npaths, spectral, lambda0, cie, w);
     uint32_t total;
@@ -906,36 +917,42 @@ int mlp_ensure_buffers(ar_ctx* ctx, size_t paths) {
         if (*q) { (void)hipFree(*q); *q = nullptr; }
     if (M.rcount) { (void)hipFree(M.rcount); M.rcount = nullptr; }
     if (M.rblk) { (void)hipFree(M.rblk); M.rblk = nullptr; }
-    M.cap = M.rcap = 0;
+    M.cap = M.rcap = M.rblocks = 0;
     hipError_t e = hipMalloc((void**)&M.feat, paths * kRec * sizeof(float));
     if (e == hipSuccess) e = hipMalloc((void**)&M.dec, paths * kDecoded * sizeof(float));
     if (e == hipSuccess) e = hipMalloc((void**)&M.rx, rcap * kRec * sizeof(float));
     if (e == hipSuccess) e = hipMalloc((void**)&M.rd, rcap * 3 * sizeof(float));
     if (e == hipSuccess) e = hipMalloc((void**)&M.rw, rcap * sizeof(float));
     if (e == hipSuccess) e = hipMalloc((void**)&M.rcount, 4 * sizeof(uint32_t));
-    if (e == hipSuccess) e = hipMalloc((void**)&M.rblk, ((paths + kRecBlock - 1) / kRecBlock) * sizeof(uint32_t));
+    // record selection: at most batch + kMaxGuidedScenes threads (one per slot of each scene's quota)
+    const size_t rblocks = (std::max<size_t>(paths, M.batch) + kMaxGuidedScenes + kRecBlock - 1) / kRecBlock;
+    if (e == hipSuccess) e = hipMalloc((void**)&M.rblk, rblocks * sizeof(uint32_t));
     if (e != hipSuccess) return set_error(ctx, AR_E_OOM, "mlp: device allocation failed");
     M.cap = paths;
     M.rcap = rcap;
+    M.rblocks = rblocks;
     return AR_OK;
 }
 
 int mlp_frame_update(ar_ctx* ctx, uint32_t spectral, const float* lambda0, const float* cie, size_t paths,
-                     hipStream_t s) {
+                     const uint32_t* ranges, uint32_t nseg, hipStream_t s) {
     MlpDev& M = ctx->mlp;
     const GuideDev& g = ctx->guide;
     const RenderBuffers& b = ctx->rb;
     Slots S{g.gr, g.gm, g.gflag, g.gx, g.gw, g.gp, {b.rad[0], b.rad[1], b.rad[2], b.rad[3]}, (uint32_t)g.pcap};
     const size_t n = std::min<size_t>(M.rcap, M.batch);
-    // records come from one path per batch slot (each gives up to 5), spread
-    // evenly over the frame's paths
-    const uint32_t nsel = (uint32_t)std::min<size_t>(paths, n);
-    const uint32_t nb = (nsel + kRecBlock - 1) / kRecBlock;
-    hipLaunchKernelGGL(k_rec_count, dim3(nb), dim3(kRecBlock), 0, s, S, (uint32_t)paths, nsel, spectral, lambda0, cie,
-                       M.rblk);
+    // records come from one path per batch slot (each gives up to 5), the
+    // slots shared equally by the frame's scenes and spread evenly over each
+    // scene's paths
+    nseg = std::max(1u, nseg);
+    uint32_t quota = (uint32_t)std::max<size_t>(1, (n + nseg - 1) / nseg);
+    quota = (uint32_t)std::min<size_t>(quota, M.rblocks * kRecBlock / nseg);   // fits the block counts
+    const uint32_t nb = (uint32_t)(((size_t)quota * nseg + kRecBlock - 1) / kRecBlock);
+    hipLaunchKernelGGL(k_rec_count, dim3(nb), dim3(kRecBlock), 0, s, S, (uint32_t)paths, quota, ranges, nseg,
+                       spectral, lambda0, cie, M.rblk);
     hipLaunchKernelGGL(k_rec_scan, dim3(1), dim3(1024), 0, s, M.rblk, nb, (uint32_t)n, M.rcount);
-    hipLaunchKernelGGL(k_mlp_records, dim3(nb), dim3(kRecBlock), 0, s, S, (uint32_t)paths, nsel, spectral, lambda0,
-                       cie, M.rblk, M.rcount, M.rx, M.rd, M.rw);
+    hipLaunchKernelGGL(k_mlp_records, dim3(nb), dim3(kRecBlock), 0, s, S, (uint32_t)paths, quota, ranges, nseg,
+                       spectral, lambda0, cie, M.rblk, M.rcount, M.rx, M.rd, M.rw);
     hipError_t e = hipSuccess;
     for (uint32_t it = 0; it < M.iters && e == hipSuccess; ++it) {
         e = mlp_gradient(ctx, M.rx, kRec, M.rd, M.rw, n, M.rcount, true, s, true);
@@ -955,7 +972,7 @@ void mlp_free(ar_ctx* ctx) {
     M.part2 = M.ploss = M.ploss2 = M.wpart = nullptr;
     M.part_blocks = 0;
     M.wb = M.wt = nullptr;
-    M.cap = M.rcap = 0;
+    M.cap = M.rcap = M.rblocks = 0;
 }
 
 int mlp_ensure(ar_ctx* ctx) {

@@ -105,6 +105,19 @@ __device__ __forceinline__ RayPre ray_setup(float ox, float oy, float oz, float 
     return p;
 }
 
+// The origin limit of a ray whose traversal starts at ref: its tree's own
+// (kept by the root node; in a batch the context's limit is the smallest of
+// its scenes'), or the context's when the ray starts at a leaf.
+template <int W>
+__device__ __forceinline__ float root_origin_limit(const void* __restrict__ nodes, uint32_t ref, float ctx_limit) {
+    if ((int)ref < 0) return ctx_limit;
+    uint32_t bits;
+    if constexpr (W == 8) bits = reinterpret_cast<const GpuNode8*>(nodes)[ref].pad[0];
+    else if constexpr (W == 4) bits = reinterpret_cast<const GpuNode4*>(nodes)[ref].pad[0];
+    else bits = reinterpret_cast<const GpuNode2*>(nodes)[ref].d[2];
+    return fmaxf(ctx_limit, __uint_as_float(bits));
+}
+
 // Watertight test; returns true when the ray's line crosses the triangle and
 // writes t.  With full = true also writes u, v (weights of v1, v2) and flags.
 // The arithmetic is written to round exactly like the CPU formula: no
@@ -575,7 +588,9 @@ __attribute__((amdgpu_waves_per_eu(OBJ ? AR_TRACE_WAVES_OBJ
 #endif
                 if (idx < n) {
                     s.idx = idx;
-                    s.p = ray_setup(r.ox[idx], r.oy[idx], r.oz[idx], r.dx[idx], r.dy[idx], r.dz[idx], origin_limit);
+                    const uint32_t rt = roots ? roots[idx] : root;
+                    s.p = ray_setup(r.ox[idx], r.oy[idx], r.oz[idx], r.dx[idx], r.dy[idx], r.dz[idx],
+                                    root_origin_limit<W>(nodes, rt, origin_limit));
                     s.tmin = r.tmin[idx];
                     s.tmax = r.tmax[idx];
                     s.best_t = s.tmax;
@@ -588,7 +603,7 @@ __attribute__((amdgpu_waves_per_eu(OBJ ? AR_TRACE_WAVES_OBJ
                     }
                     st.sp = 0;
                     st.push(kRefEmpty);
-                    s.ref = (s.p.valid && s.tmax >= s.tmin) ? (roots ? roots[idx] : root) : kRefEmpty;
+                    s.ref = (s.p.valid && s.tmax >= s.tmin) ? rt : kRefEmpty;
                     nb = 0;
                     nt = 0;
                     active = true;

@@ -379,7 +379,7 @@ GUIDE_FIXTURES = {
 }
 
 
-def guiding_bench(ctx_cls, room_spp=(256, 1024), spectral_spp=(32, 128)):
+def guiding_bench(ctx_cls, room_spp=1024, spectral_spp=128):
     """Learned guiding against voxel guiding and plain rendering at equal
     time, scored by relMSE against the FP64 oracle fixtures at 16,384 spp
     (tests/golden/make_guiding_references.py): the occluded-light room
@@ -437,35 +437,44 @@ def guiding_bench(ctx_cls, room_spp=(256, 1024), spectral_spp=(32, 128)):
         c = ctxs[name]
         w, h = scenes[name][5]
         if prep:
-            prep(c)
+            prep(c, seed)
         torch.cuda.synchronize()
         t = time.perf_counter()
         img = c.render(w, h, spp, seed=seed, **kws[name], **extra)
         torch.cuda.synchronize()
         ms = (time.perf_counter() - t) * 1e3
-        return {"spp": spp, "ms": round(ms, 2), "relmse": score(name, img.cpu().numpy())}
+        return ms, score(name, img.cpu().numpy())
+
+    def summary(runs, spp):
+        ms = [r[0] for r in runs]
+        e = [r[1] for r in runs]
+        return {"spp": spp, "ms": round(float(np.mean(ms)), 2), "relmse": float(np.mean(e)),
+                "relmse_per_seed": [float(x) for x in e]}
 
     # learned: frames of 16, 32, 64, ... spp (growth 2), 32 Adam steps of
     # 2^17 records after each; voxel: the reference's probes in frames of
-    # 64 (room) / 16 (spectral) spp
+    # 64 (room) / 16 (spectral) spp.  Each method: three seeds (the network
+    # seeded alike), means of time and relMSE.
     mlp_cfg = dict(learning_rate=1e-2, iters_per_frame=32, batch=1 << 17)
     methods = {
         "plain": (None, {}),
-        "voxel": (lambda c: c.guiding_reset(), dict(guiding=1)),
-        "learned": (lambda c: c.mlp_configure(seed=1, **mlp_cfg), dict(guiding=2, frame_spp=16, frame_growth=2)),
+        "voxel": (lambda c, s: c.guiding_reset(), dict(guiding=1)),
+        "learned": (lambda c, s: c.mlp_configure(seed=s, **mlp_cfg), dict(guiding=2, frame_spp=16, frame_growth=2)),
     }
     frame = {"room": 64, "spectral": 16}
+    spp_of = {"room": room_spp, "spectral": spectral_spp}
+    seeds = {"plain": (1, 2, 3), "voxel": (7, 8, 9), "learned": (7, 8, 9)}
     for name in scenes:
-        levels = room_spp if name == "room" else spectral_spp
+        spp = spp_of[name]
         run(name, 4, 99)
         res = {}
         for mname, (prep, extra) in methods.items():
-            log(f"guiding: {name} {mname} at {levels} spp")
+            log(f"guiding: {name} {mname} at {spp} spp, seeds {seeds[mname]}")
             ex = dict(dict(frame_spp=frame[name]), **extra) if extra else {}
-            res[mname] = [run(name, s, 7 if extra else 1, prep=prep, **ex) for s in levels]
+            res[mname] = summary([run(name, spp, sd, prep=prep, **ex) for sd in seeds[mname]], spp)
         ratios = {}
         for mname in ("voxel", "learned"):
-            p, m = res["plain"][-1], res[mname][-1]
+            p, m = res["plain"], res[mname]
             ratios[mname] = round(m["ms"] * m["relmse"] / (p["ms"] * p["relmse"]), 3)
         res["time_ratio_at_equal_relmse"] = ratios
         out[name] = res
@@ -477,22 +486,26 @@ def guiding_bench(ctx_cls, room_spp=(256, 1024), spectral_spp=(32, 128)):
         for n in names:
             w, h = scenes[n][5]
             kw = kws[n]
-            views.append(dict(width=w, height=h, spp=(room_spp if n == "room" else spectral_spp)[-1],
-                              cam_pos=kw["cam_pos"], cam_dir=kw["cam_dir"], vfov=kw["vfov"]))
-        shared = dict(max_depth=8, spectral=4, output=1, seed=7)
+            views.append(dict(width=w, height=h, spp=spp_of[n], cam_pos=kw["cam_pos"], cam_dir=kw["cam_dir"],
+                              vfov=kw["vfov"]))
+        shared = dict(max_depth=8, spectral=4, output=1)
         log("guiding: both scenes batched, one network")
-        b.render_views([dict(v, spp=4) for v in views], **shared)
-        b.mlp_configure(seed=1, **mlp_cfg)
-        torch.cuda.synchronize()
-        t = time.perf_counter()
-        imgs = b.render_views(views, guiding=2, frame_spp=16, frame_growth=2, **shared)
-        torch.cuda.synchronize()
-        bms = (time.perf_counter() - t) * 1e3
-        errs = {n: score(n, im.cpu().numpy()) for n, im in zip(names, imgs)}
-        plain_equiv = sum(out[n]["plain"][-1]["ms"] * out[n]["plain"][-1]["relmse"] / errs[n] for n in names)
+        b.render_views([dict(v, spp=4) for v in views], seed=99, **shared)
+        runs = []
+        for sd in seeds["learned"]:
+            b.mlp_configure(seed=sd, **mlp_cfg)
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            imgs = b.render_views(views, guiding=2, frame_spp=16, frame_growth=2, seed=sd, **shared)
+            torch.cuda.synchronize()
+            runs.append(((time.perf_counter() - t) * 1e3, [score(n, im.cpu().numpy()) for n, im in zip(names, imgs)]))
+        bms = float(np.mean([r[0] for r in runs]))
+        errs = {n: float(np.mean([r[1][k] for r in runs])) for k, n in enumerate(names)}
+        plain_equiv = sum(out[n]["plain"]["ms"] * out[n]["plain"]["relmse"] / errs[n] for n in names)
         out["batched_learned"] = {"scenes": names, "spp": [v["spp"] for v in views], "frame_spp": 16, "frame_growth": 2,
-                                  "ms": round(bms, 2), "relmse": {n: errs[n] for n in names},
-                                  "plain_ms": round(sum(out[n]["plain"][-1]["ms"] for n in names), 2),
+                                  "seeds": list(seeds["learned"]), "ms": round(bms, 2), "relmse": errs,
+                                  "relmse_per_seed": {n: [float(r[1][k]) for r in runs] for k, n in enumerate(names)},
+                                  "plain_ms": round(sum(out[n]["plain"]["ms"] for n in names), 2),
                                   "time_ratio_at_equal_relmse": round(bms / plain_equiv, 3)}
         out["learned_vs_plain_time_at_equal_relmse"] = out["batched_learned"]["time_ratio_at_equal_relmse"]
         b.close()
