@@ -309,10 +309,14 @@ typedef struct {
 #define AR_GUIDING_VOXEL 1u
 /* MLP: learned guiding (no counterpart in the reference): an MLP maps the
  * vertex (position in the scene box, shading normal) to an 8-lobe vMF
- * mixture over directions, sampled in the same one-sample mixture and
- * hemisphere fold as the voxel probes.  After each frame it is trained on the
- * frame's path vertices (collected like the voxel samples) with BF16 MFMA
- * forward/backward and Adam; see ar_mlp_configure. */
+ * mixture over directions, sampled in a one-sample mixture with the BSDF
+ * (guide or BSDF chosen first, probability 0.75 * min(1, frame/64)^0.5 of
+ * the guide) in the voxel probes' hemisphere fold.  After each frame it is
+ * trained with BF16 MFMA forward/backward and Adam on records of the
+ * frame's path vertices: the direction sampled at a non-delta vertex and the
+ * radiance that arrived along it (the next vertex's emission without MIS
+ * weight, its NEE and every later bounce) over the direction's pdf; see
+ * ar_mlp_configure and ar_mlp_records_get. */
 #define AR_GUIDING_MLP 2u
 
 /* Spectral modes.  RGB: three colour channels throughout (the reference with
@@ -555,7 +559,9 @@ int ar_guiding_restore(ar_ctx* ctx, const uint64_t* keys, const float* texels, c
  * (x, w, weight): -weight * log q(w | x). */
 typedef struct {
     float learning_rate;        /* Adam step (default 5e-3) */
-    uint32_t batch;             /* path vertices per training step (default 2^18) */
+    uint32_t batch;             /* records per training step (default 2^18); a frame's records
+                                   come from one path per record slot, the slots shared equally by
+                                   the frame's scenes */
     uint32_t iters_per_frame;   /* training steps after each frame (default 1) */
     uint64_t seed;              /* He-normal initialisation seed */
 } ar_mlp_opts;

@@ -15,10 +15,10 @@ variance estimate).
   steps of 2^17 records after each) lowers the error at equal spp below 0.6x
   plain and voxel guiding (frames of 64 spp) below plain (measured
   0.85-0.88x).
-- On the spectral scene (64 small lights, mostly direct light) guiding has
-  little to learn and takes up to half of the BSDF samples: it may raise the
-  error, but by less than 50 % (measured at 128 spp: voxel 1.29x, learned
-  1.20x plain).
+- On the spectral scene (64 small lights, a dark window lit mostly by
+  caustic paths, whose relMSE a few fireflies dominate) neither guide may
+  raise the error by 50 % or more (measured at 128 spp, one seed: voxel
+  1.29x, learned 0.31x plain).
 The learned renders are bit-reproducible, so the ratios are fixed numbers
 for a given build; voxel probes accumulate with float atomics."""
 from pathlib import Path
