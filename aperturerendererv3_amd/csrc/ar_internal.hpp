@@ -492,10 +492,13 @@ void mlp_free(ar_ctx* ctx);
 void smlp_free(ar_ctx* ctx);
 hipError_t mlp_forward(ar_ctx* ctx, const float* x5, int xs, size_t n, const uint32_t* n_dev, float* out, bool decoded,
                        hipStream_t s);
-// adam: an Adam step follows (advances the device-side step count)
+// norm: the loss is normalised by 1/n (kNormCount), by 1/sum(w) computed
+// here (kNormSum) or by the normaliser the previous call computed from the
+// same weights (kNormKeep); adam: an Adam step is applied with the gradient
+// (and the device-side step count advanced)
+enum { kNormCount = 0, kNormSum = 1, kNormKeep = 2 };
 hipError_t mlp_gradient(ar_ctx* ctx, const float* x5, int xs, const float* dir, const float* w, size_t n,
-                        const uint32_t* n_dev, bool normalize, hipStream_t s, bool adam);
-hipError_t mlp_adam_step(ar_ctx* ctx, hipStream_t s);
+                        const uint32_t* n_dev, int norm, hipStream_t s, bool adam);
 int mlp_ensure_buffers(ar_ctx* ctx, size_t paths);
 // training records from the frame's guiding slots, then the frame's Adam steps
 int mlp_frame_update(ar_ctx* ctx, uint32_t spectral, const float* lambda0, const float* cie, size_t paths,

@@ -582,7 +582,6 @@ __global__ __launch_bounds__(kIW * 64) void k_mlp_infer(Net N, const float* __re
         if (DECODE) {
             wave_sync();
             const int sl = lane & (kTT - 1), h = lane >> 5;
-            const size_t s = t0 + sl;
             const float* o = &R[sl * kRS];
             float mx = -3.0e38f;
 #pragma unroll
@@ -592,22 +591,35 @@ __global__ __launch_bounds__(kIW * 64) void k_mlp_infer(Net N, const float* __re
 #pragma unroll
             for (int j = 0; j < 4; ++j) { pu[j] = expf(o[5 * (4 * h + j)] - mx); z += pu[j]; }
             z += __shfl_xor(z, 32);
-            if (s < nn) {
-                float* d = out + s * kDecoded;
+            float dv[24];
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    const int k = 4 * h + j;
-                    const float pi = pu[j] / z;
-                    const float kap = expf(fminf(7.0f, fmaxf(-3.0f, o[5 * k + 1])));
-                    const float cx = o[5 * k + 2], cy = o[5 * k + 3], cz = o[5 * k + 4];
-                    const float l = sqrtf(cx * cx + cy * cy + cz * cz);
-                    d[6 * k + 0] = pi;
-                    d[6 * k + 1] = pi * vmf_norm(kap);
-                    d[6 * k + 2] = kap;
-                    d[6 * k + 3] = l < 1e-12f ? 0.0f : cx / l;
-                    d[6 * k + 4] = l < 1e-12f ? 0.0f : cy / l;
-                    d[6 * k + 5] = l < 1e-12f ? 1.0f : cz / l;
-                }
+            for (int j = 0; j < 4; ++j) {
+                const int k = 4 * h + j;
+                const float pi = pu[j] / z;
+                const float kap = expf(fminf(7.0f, fmaxf(-3.0f, o[5 * k + 1])));
+                const float cx = o[5 * k + 2], cy = o[5 * k + 3], cz = o[5 * k + 4];
+                const float l = sqrtf(cx * cx + cy * cy + cz * cz);
+                dv[6 * j + 0] = pi;
+                dv[6 * j + 1] = pi * vmf_norm(kap);
+                dv[6 * j + 2] = kap;
+                dv[6 * j + 3] = l < 1e-12f ? 0.0f : cx / l;
+                dv[6 * j + 4] = l < 1e-12f ? 0.0f : cy / l;
+                dv[6 * j + 5] = l < 1e-12f ? 1.0f : cz / l;
+            }
+            // the decoded tile (32 vertices x 48 floats) goes back through
+            // R, then out in whole 16-byte pieces of consecutive vertices
+            wave_sync();                                            // every lane has read its raw outputs
+#pragma unroll
+            for (int q = 0; q < 6; ++q)
+                *reinterpret_cast<float4*>(&R[sl * kRS + 24 * h + 4 * q]) =
+                    make_float4(dv[4 * q], dv[4 * q + 1], dv[4 * q + 2], dv[4 * q + 3]);
+            wave_sync();
+#pragma unroll
+            for (int q = 0; q < 6; ++q) {
+                const int c = lane + 64 * q, v = c / 12, part = c - 12 * v;
+                if (t0 + v < nn)
+                    *reinterpret_cast<float4*>(&out[(t0 + v) * kDecoded + 4 * part]) =
+                        *reinterpret_cast<const float4*>(&R[v * kRS + 4 * part]);
             }
         }
     }
@@ -615,10 +627,13 @@ __global__ __launch_bounds__(kIW * 64) void k_mlp_infer(Net N, const float* __re
 
 // Fixed-order sums of the block partials: chunks of kRedChunk blocks, then the chunk sums.
 constexpr int kRedChunk = 16;
+// tick (optional): the Adam step count, advanced here when an Adam step
+// follows (k_mlp_reduce2_adam reads it after this kernel has finished)
 __global__ void k_mlp_reduce1(const float* __restrict__ part, const float* __restrict__ part_loss, int blocks,
-                              float* __restrict__ part2, float* __restrict__ loss2) {
+                              float* __restrict__ part2, float* __restrict__ loss2, float* __restrict__ tick) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     const int c = blockIdx.y;
+    if (tick && p == 0 && c == 0) tick[0] += 1.0f;
     const int b0 = c * kRedChunk, b1 = std::min(blocks, b0 + kRedChunk);
     if (p < kParams) {
         float s = 0.0f;
@@ -631,11 +646,9 @@ __global__ void k_mlp_reduce1(const float* __restrict__ part, const float* __res
         loss2[c] = s;
     }
 }
-// tick (optional): the Adam step count, advanced here when an Adam step follows
 __global__ void k_mlp_reduce2(const float* __restrict__ part2, const float* __restrict__ loss2, int chunks,
-                              float* __restrict__ g, float* __restrict__ loss, float* __restrict__ tick) {
+                              float* __restrict__ g, float* __restrict__ loss) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (tick && p == kParams + 1) tick[0] += 1.0f;
     if (p < kParams) {
         float s = 0.0f;
         for (int c = 0; c < chunks; ++c) s += part2[(size_t)c * kParams + p];
@@ -681,7 +694,7 @@ __global__ void k_set(float* v, float x) { v[0] = x; }
 struct Adam {
     float* p; float* m; float* v; float* g;
     uint16_t* wb; uint16_t* wt;
-    const float* t;                  // step count (device; advanced by k_adam_tick)
+    const float* t;                  // step count (device; advanced by k_mlp_reduce1)
     float lr, b1, b2, eps;
 };
 
@@ -718,6 +731,37 @@ __global__ void k_adam(Adam A, int apply) {
         const uint16_t b = f2bf(A.p[i]);
         A.wb[i] = b;
         A.wt[t] = b;
+    }
+}
+
+// k_mlp_reduce2 and k_adam in one pass: the gradient of each parameter is
+// summed over the chunks in the same order and applied at once (the same
+// arithmetic as the two kernels; the gradient buffer is left zero as k_adam
+// leaves it)
+__global__ void k_mlp_reduce2_adam(const float* __restrict__ part2, const float* __restrict__ loss2, int chunks,
+                                   float* __restrict__ loss, Adam A) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == kParams) {
+        float s = 0.0f;
+        for (int c = 0; c < chunks; ++c) s += loss2[c];
+        *loss = s;
+    }
+    if (i >= kParams) return;
+    float g = 0.0f;
+    for (int c = 0; c < chunks; ++c) g += part2[(size_t)c * kParams + i];
+    const float t = A.t[0];
+    const float c1 = 1.0f - powf(A.b1, t), c2 = 1.0f - powf(A.b2, t);
+    const float m = A.b1 * A.m[i] + (1.0f - A.b1) * g;
+    const float v = A.b2 * A.v[i] + (1.0f - A.b2) * g * g;
+    A.m[i] = m;
+    A.v[i] = v;
+    A.p[i] -= A.lr * (m / c1) / (sqrtf(v / c2) + A.eps);
+    A.g[i] = 0.0f;
+    const int tr = transposed(i);
+    if (tr >= 0) {
+        const uint16_t b = f2bf(A.p[i]);
+        A.wb[i] = b;
+        A.wt[tr] = b;
     }
 }
 
@@ -954,10 +998,9 @@ int mlp_frame_update(ar_ctx* ctx, uint32_t spectral, const float* lambda0, const
     hipLaunchKernelGGL(k_mlp_records, dim3(nb), dim3(kRecBlock), 0, s, S, (uint32_t)paths, quota, ranges, nseg,
                        spectral, lambda0, cie, M.rblk, M.rcount, M.rx, M.rd, M.rw);
     hipError_t e = hipSuccess;
-    for (uint32_t it = 0; it < M.iters && e == hipSuccess; ++it) {
-        e = mlp_gradient(ctx, M.rx, kRec, M.rd, M.rw, n, M.rcount, true, s, true);
-        if (e == hipSuccess) e = mlp_adam_step(ctx, s);
-    }
+    // the weight normaliser depends only on the records: computed by the first step
+    for (uint32_t it = 0; it < M.iters && e == hipSuccess; ++it)
+        e = mlp_gradient(ctx, M.rx, kRec, M.rd, M.rw, n, M.rcount, it == 0 ? kNormSum : kNormKeep, s, true);
     return check_hip(ctx, e, "guiding network training");
 }
 
@@ -1046,13 +1089,13 @@ hipError_t mlp_forward(ar_ctx* ctx, const float* x5, int xs, size_t n, const uin
 // gradient into M.g (zeroed first unless accumulate) and loss into M.scal[1];
 // normalize: scale = 1 / sum(weights), else 1 / n
 hipError_t mlp_gradient(ar_ctx* ctx, const float* x5, int xs, const float* dir, const float* w, size_t n,
-                        const uint32_t* n_dev, bool normalize, hipStream_t s, bool adam) {
+                        const uint32_t* n_dev, int norm, hipStream_t s, bool adam) {
     MlpDev& M = ctx->mlp;
     Net N{M.p, M.wb, M.wt};
-    if (normalize) {
+    if (norm == kNormSum) {
         hipLaunchKernelGGL(k_weight_sum, dim3(kWsBlocks), dim3(256), 0, s, w, n, n_dev, M.wpart);
         hipLaunchKernelGGL(k_inverse, dim3(1), dim3(256), 0, s, M.wpart, kWsBlocks, M.scal);
-    } else {
+    } else if (norm == kNormCount) {
         hipLaunchKernelGGL(k_set, dim3(1), dim3(1), 0, s, M.scal, 1.0f / (float)std::max<size_t>(n, 1));
     }
     const uint32_t grid = train_grid(M, n);
@@ -1060,16 +1103,15 @@ hipError_t mlp_gradient(ar_ctx* ctx, const float* x5, int xs, const float* dir, 
     const uint32_t chunks = (grid + kRedChunk - 1) / kRedChunk;
     const uint32_t pblocks = (uint32_t)((kParams + 2 + 255) / 256);
     hipLaunchKernelGGL(k_mlp_reduce1, dim3(pblocks, chunks), dim3(256), 0, s, M.part, M.ploss, (int)grid, M.part2,
-                       M.ploss2);
-    hipLaunchKernelGGL(k_mlp_reduce2, dim3(pblocks), dim3(256), 0, s, M.part2, M.ploss2, (int)chunks, M.g, M.scal + 1,
-                       adam ? M.scal + 2 : nullptr);
-    return hipGetLastError();
-}
-
-hipError_t mlp_adam_step(ar_ctx* ctx, hipStream_t s) {
-    MlpDev& M = ctx->mlp;
-    Adam A{M.p, M.m, M.v, M.g, M.wb, M.wt, M.scal + 2, M.lr, 0.9f, 0.999f, 1e-8f};
-    hipLaunchKernelGGL(k_adam, dim3((kParams + 255) / 256), dim3(256), 0, s, A, 1);
+                       M.ploss2, adam ? M.scal + 2 : nullptr);
+    if (adam) {
+        Adam A{M.p, M.m, M.v, M.g, M.wb, M.wt, M.scal + 2, M.lr, 0.9f, 0.999f, 1e-8f};
+        hipLaunchKernelGGL(k_mlp_reduce2_adam, dim3(pblocks), dim3(256), 0, s, M.part2, M.ploss2, (int)chunks,
+                           M.scal + 1, A);
+    } else {
+        hipLaunchKernelGGL(k_mlp_reduce2, dim3(pblocks), dim3(256), 0, s, M.part2, M.ploss2, (int)chunks, M.g,
+                           M.scal + 1);
+    }
     return hipGetLastError();
 }
 
@@ -1145,7 +1187,7 @@ int ar_mlp_gradient(ar_ctx* ctx, const float* x5, const float* dir, const float*
     int rc = mlp_ensure(ctx);
     if (rc != AR_OK) return rc;
     hipStream_t s = (hipStream_t)stream;
-    hipError_t e = mlp_gradient(ctx, x5, kUsedIn, dir, weight, n, nullptr, false, s, false);
+    hipError_t e = mlp_gradient(ctx, x5, kUsedIn, dir, weight, n, nullptr, kNormCount, s, false);
     if (e == hipSuccess) e = hipMemcpyAsync(grad, ctx->mlp.g, kParams * sizeof(float), hipMemcpyDeviceToDevice, s);
     if (e == hipSuccess && loss) e = hipMemcpyAsync(loss, ctx->mlp.scal + 1, sizeof(float), hipMemcpyDeviceToDevice, s);
     return check_hip(ctx, e, "mlp gradient");
@@ -1158,10 +1200,8 @@ int ar_mlp_train(ar_ctx* ctx, const float* x5, const float* dir, const float* we
     if (rc != AR_OK) return rc;
     hipStream_t s = (hipStream_t)stream;
     hipError_t e = hipSuccess;
-    for (uint32_t t = 0; t < steps && e == hipSuccess; ++t) {
-        e = mlp_gradient(ctx, x5, kUsedIn, dir, weight, n, nullptr, true, s, true);
-        if (e == hipSuccess) e = mlp_adam_step(ctx, s);
-    }
+    for (uint32_t t = 0; t < steps && e == hipSuccess; ++t)
+        e = mlp_gradient(ctx, x5, kUsedIn, dir, weight, n, nullptr, t == 0 ? kNormSum : kNormKeep, s, true);
     if (e == hipSuccess && loss) e = hipMemcpyAsync(loss, ctx->mlp.scal + 1, sizeof(float), hipMemcpyDeviceToDevice, s);
     return check_hip(ctx, e, "mlp train");
 }
