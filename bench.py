@@ -72,6 +72,9 @@ def parse_args():
     ap.add_argument("--sdf-only", action="store_true", help="run only the SDF-object render bench")
     ap.add_argument("--smlp-only", action="store_true", help="run only the spectral basis network bench")
     ap.add_argument("--guiding-only", action="store_true", help="run only the equal-time guiding comparison")
+    ap.add_argument("--guided-render", choices=["room_plain", "room_learned", "batch_learned"],
+                    help="one render of the guiding workload (for kernel traces): the room at 1024 spp plain or "
+                         "learned-guided, or the room and the spectral scene as one learned-guided batch")
     ap.add_argument("--cpu-sample-stride", type=int, default=2)
     ap.add_argument("--profile", action="store_true",
                     help="timed steps only, plus one 256 MiB device copy used to scale HBM counters")
@@ -514,6 +517,41 @@ def guiding_bench(ctx_cls, room_spp=1024, spectral_spp=128):
     return out
 
 
+def guided_render(ctx_cls, which):
+    """One render of the guiding workload after a 4-spp warm-up, with the
+    settings of guiding_bench (seed 7): prints its wall time."""
+    import torch
+    from aperturerendererv3_amd import make_batch
+    from aperturerendererv3_amd.workloads import OCCLUDED_CAMERA, SPECTRAL_CAMERA, occluded_room, spectral_scene
+    scenes = [(occluded_room(), OCCLUDED_CAMERA, 320, 180, 1024)]
+    if which == "batch_learned":
+        scenes.append((spectral_scene(), SPECTRAL_CAMERA, 1920, 1080, 128))
+    ctxs = []
+    for (V, F, M, mats), _, _, _, _ in scenes:
+        c = ctx_cls(0)
+        c.set_meshes([{"vertices": V, "indices": F, "tri_material": M, "materials": mats}])
+        c.build_bvh()
+        ctxs.append(c)
+    views = [dict(width=w, height=h, spp=spp, cam_pos=cam["pos"], cam_dir=cam["dir"], vfov=cam["vfov"])
+             for _, cam, w, h, spp in scenes]
+    shared = dict(max_depth=8, spectral=4, output=1)
+    r = make_batch(ctxs) if which == "batch_learned" else ctxs[0]
+    r.render_views([dict(v, spp=4) for v in views], seed=99, **shared)
+    extra = {}
+    if which != "room_plain":
+        r.mlp_configure(seed=7, learning_rate=1e-2, iters_per_frame=32, batch=1 << 17)
+        extra = dict(guiding=2, frame_spp=16, frame_growth=2)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    r.render_views(views, seed=7, **shared, **extra)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t) * 1e3
+    for c in ([r] if r is not ctxs[0] else []) + ctxs:
+        c.close()
+    return {"guided_render": which, "ms": round(ms, 2),
+            "paths": int(sum(v["width"] * v["height"] * v["spp"] for v in views))}
+
+
 # algorithmic FLOPs per vertex of the guiding network 5 -> 64 -> 64 -> 64 -> 40
 MLP_FWD_FLOP = 2 * (5 * 64 + 64 * 64 + 64 * 64 + 64 * 40)
 # training: forward, backward through layers 3..1 (no input gradient), weight gradients
@@ -857,6 +895,10 @@ def main():
     if args.guiding_only:
         from aperturerendererv3_amd import Context
         print(json.dumps(guiding_bench(Context)), flush=True)
+        return
+    if args.guided_render:
+        from aperturerendererv3_amd import Context
+        print(json.dumps(guided_render(Context, args.guided_render)), flush=True)
         return
     if args.output_only:
         from aperturerendererv3_amd import Context
