@@ -373,7 +373,72 @@ def batch_bench(ctx_cls, count=48, cpu=True, reps=2):
     b.close()
     for c in ctxs:
         c.close()
+    res["small_scenes"] = small_batch_bench(ctx_cls)
     return res
+
+
+def small_batch_bench(ctx_cls, count=256, reps=3):
+    """Where batching pays: `count` small instanced spectral scenes
+    (triangles log-uniform in [1e3, 2e4], scene k seeded 1001 + k) at 128 x 128
+    and 4 spp, depth 6, RGB, NEE on, sky off: 65,536 paths per scene, too few
+    to fill the GPU alone.  The whole batch as one HIP graph against the
+    scenes one after another, as ar_render calls and as one captured graph
+    per scene (no host work between launches); the same images."""
+    import math
+    import torch
+    from aperturerendererv3_amd import make_batch
+    from aperturerendererv3_amd.workloads import SPECTRAL_CAMERA, spectral_scene
+    cam = dict(cam_pos=SPECTRAL_CAMERA["pos"], cam_dir=SPECTRAL_CAMERA["dir"], vfov=SPECTRAL_CAMERA["vfov"])
+    ctxs, views, tris = [], [], 0
+    lo, hi = math.log(1e3), math.log(2e4)
+    for k in range(count):
+        rng = np.random.Generator(np.random.PCG64(1001 + k))
+        V, F, M, mats = spectral_scene(target_tris=int(round(math.exp(lo + (hi - lo) * rng.random()))), seed=1001 + k)
+        c = ctx_cls(0)
+        c.set_meshes([{"vertices": V, "indices": F, "tri_material": M, "materials": mats}])
+        c.build_bvh()
+        ctxs.append(c)
+        views.append(dict(width=128, height=128, spp=4, **cam))
+        tris += F.shape[0]
+    shared = dict(max_depth=6, seed=1)
+    paths = sum(v["width"] * v["height"] * v["spp"] for v in views)
+    outs = [torch.empty((v["height"], v["width"], 3), dtype=torch.float32, device="cuda:0") for v in views]
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    seq_ms = timed(lambda: [c.render_views([v], outs=[o], **shared) for c, v, o in zip(ctxs, views, outs)])
+    seq_img = [o.cpu().numpy() for o in outs]
+    solo_graphs = [c.capture([v], **shared) for c, v in zip(ctxs, views)]
+    seq_graph_ms = timed(lambda: [gr.launch() for gr in solo_graphs])
+    same_solo = all(np.array_equal(a.view(np.uint32), gr.outputs[0].cpu().numpy().view(np.uint32))
+                    for a, gr in zip(seq_img, solo_graphs))
+    for gr in solo_graphs:
+        gr.close()
+    b = make_batch(ctxs)
+    g = b.capture(views, **shared)
+    batch_ms = timed(g.launch)
+    same = all(np.array_equal(a.view(np.uint32), o.cpu().numpy().view(np.uint32)) for a, o in zip(seq_img, g.outputs))
+    g.close()
+    b.close()
+    for c in ctxs:
+        c.close()
+    return {"workload": f"{count} instanced spectral scenes, {tris} triangles in total (log-uniform 1e3..2e4 per "
+                        "scene), 128x128, 4 spp, max depth 6, NEE on, sky off, RGB",
+            "paths": paths, "batched_graph_ms": round(batch_ms, 3),
+            "batched_paths_per_s": round(paths / (batch_ms * 1e-3), 1),
+            "sequential_ms": round(seq_ms, 3), "sequential_graphs_ms": round(seq_graph_ms, 3),
+            "speedup_vs_sequential": round(seq_ms / batch_ms, 2),
+            "speedup_vs_sequential_graphs": round(seq_graph_ms / batch_ms, 2),
+            "images_identical_to_sequential": bool(same and same_solo)}
 
 
 GUIDE_FIXTURES = {
