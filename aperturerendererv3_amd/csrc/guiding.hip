@@ -140,7 +140,7 @@ __global__ __launch_bounds__(kBlock) void k_guide_collect(SlotArgs S, uint32_t n
                 if (slot == kNoProbe) { atomicAdd(&counters[kCDropped], 1u); ok = false; }
             }
         }
-        const uint32_t w = wave_append(ok, &counters[kCSamples]);
+        const uint32_t w = block_append<kBlock / 64>(ok, &counters[kCSamples]);
         if (ok && w < scap) {
             s_slot[w] = slot;
             s_leaf[w] = (leaves >> (6 * i)) & 63u;

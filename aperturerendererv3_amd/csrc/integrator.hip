@@ -1336,7 +1336,8 @@ __device__ __forceinline__ void shade_one(
         S.rad0[p] = rad.v[0]; S.rad1[p] = rad.v[1]; S.rad2[p] = rad.v[2]; S.rad3[p] = rad.v[3];
         S.meta[p] = new_meta;
     }
-    const uint32_t slot = wave_append(cont, &counters[1]);
+    uint32_t slot, ss;
+    block_append2<kMaxBlock / 64>(cont, shadow, &counters[1], &counters[2], slot, ss);
     if (cont) {
         nxt.ox[slot] = next_o.x; nxt.oy[slot] = next_o.y; nxt.oz[slot] = next_o.z;
         nxt.tmin[slot] = next_tmin;
@@ -1345,7 +1346,6 @@ __device__ __forceinline__ void shade_one(
         nxt.root[slot] = root;
         qnext[slot] = p;
     }
-    const uint32_t ss = wave_append(shadow, &counters[2]);
     if (shadow) {
         sh.ox[ss] = s_o.x; sh.oy[ss] = s_o.y; sh.oz[ss] = s_o.z; sh.tmin[ss] = 0.0f;
         sh.dx[ss] = s_d.x; sh.dy[ss] = s_d.y; sh.dz[ss] = s_d.z; sh.tmax[ss] = s_tmax;
