@@ -28,11 +28,10 @@ using namespace mlp;
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ uint16_t f2bf(float x) {
-    uint32_t u = __float_as_uint(x);
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
-}
+// FP32 -> BF16, round to nearest even (gfx950's v_cvt_pk_bf16_f32, which
+// converts two values per instruction; for finite values the same bits as
+// adding 0x7FFF + lsb and truncating)
+__device__ __forceinline__ uint16_t f2bf(float x) { return __builtin_bit_cast(uint16_t, (__bf16)x); }
 __host__ __device__ __forceinline__ float bf2f(uint16_t h) {
     uint32_t u = (uint32_t)h << 16;
     float f;
@@ -539,14 +538,37 @@ __global__ __launch_bounds__(kIW * 64) void k_mlp_infer(Net N, const float* __re
         __syncthreads();
     }
     float* R = S.r[w];
-    for (size_t t0 = ((size_t)blockIdx.x * kIW + w) * kTT; t0 < nn; t0 += (size_t)gridDim.x * kIW * kTT) {
+    // The inputs of a tile are loaded one tile ahead, so their latency
+    // overlaps the previous tile's products: raw[nb][r] is the record value
+    // behind input 4g + r (a feature, or the scene index for its bits).
+    const size_t tstride = (size_t)gridDim.x * kIW * kTT;
+    auto load_raw = [&](size_t t, float raw[2][4]) {
+#pragma unroll
+        for (int nb = 0; nb < 2; ++nb) {
+            const size_t s = t + nb * 16 + i;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int k = 4 * g + r;
+                const bool used = k < kUsedIn || (XS == kRec && k < kUsedIn + kSceneBits);
+                raw[nb][r] = (s < nn && used) ? x[s * XS + (k < kUsedIn ? k : kUsedIn)] : 0.0f;
+            }
+        }
+    };
+    float cur[2][4];
+    load_raw(((size_t)blockIdx.x * kIW + w) * kTT, cur);
+    for (size_t t0 = ((size_t)blockIdx.x * kIW + w) * kTT; t0 < nn; t0 += tstride) {
+        float nxt[2][4];
+        load_raw(t0 + tstride, nxt);
         bf16x4 xb[2];
 #pragma unroll
         for (int nb = 0; nb < 2; ++nb) {
-            const size_t s = t0 + nb * 16 + i;
 #pragma unroll
-            for (int r = 0; r < 4; ++r)
-                xb[nb][r] = (short)(s < nn ? f2bf(record_input(x + s * XS, XS, 4 * g + r)) : (uint16_t)0);
+            for (int r = 0; r < 4; ++r) {
+                const int k = 4 * g + r;
+                const float v = k < kUsedIn ? cur[nb][r]
+                                            : (float)(((uint32_t)cur[nb][r] >> (k < kUsedIn + kSceneBits ? k - kUsedIn : 0)) & 1u);
+                xb[nb][r] = (short)f2bf(k < kUsedIn || (XS == kRec && k < kUsedIn + kSceneBits) ? v : 0.0f);
+            }
         }
         bf16x4 h1[4][2], h2[4][2];
 #pragma unroll
@@ -622,6 +644,10 @@ __global__ __launch_bounds__(kIW * 64) void k_mlp_infer(Net N, const float* __re
                         *reinterpret_cast<const float4*>(&R[v * kRS + 4 * part]);
             }
         }
+#pragma unroll
+        for (int nb = 0; nb < 2; ++nb)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) cur[nb][r] = nxt[nb][r];
     }
 }
 
