@@ -534,11 +534,11 @@ def guiding_bench(ctx_cls, room_spp=1024, spectral_spp=128):
     seeds = {"plain": (1, 2, 3), "voxel": (7, 8, 9), "learned": (7, 8, 9)}
     for name in scenes:
         spp = spp_of[name]
-        run(name, 4, 99)
         res = {}
         for mname, (prep, extra) in methods.items():
             log(f"guiding: {name} {mname} at {spp} spp, seeds {seeds[mname]}")
             ex = dict(dict(frame_spp=frame[name]), **extra) if extra else {}
+            run(name, spp, 99, prep=prep, **ex)   # untimed: allocates the method's buffers
             res[mname] = summary([run(name, spp, sd, prep=prep, **ex) for sd in seeds[mname]], spp)
         ratios = {}
         for mname in ("voxel", "learned"):
@@ -558,7 +558,8 @@ def guiding_bench(ctx_cls, room_spp=1024, spectral_spp=128):
                               vfov=kw["vfov"]))
         shared = dict(max_depth=8, spectral=4, output=1)
         log("guiding: both scenes batched, one network")
-        b.render_views([dict(v, spp=4) for v in views], seed=99, **shared)
+        b.mlp_configure(seed=99, **mlp_cfg)
+        b.render_views(views, guiding=2, frame_spp=16, frame_growth=2, seed=99, **shared)   # untimed
         runs = []
         for sd in seeds["learned"]:
             b.mlp_configure(seed=sd, **mlp_cfg)
@@ -583,8 +584,10 @@ def guiding_bench(ctx_cls, room_spp=1024, spectral_spp=128):
 
 
 def guided_render(ctx_cls, which):
-    """One render of the guiding workload after a 4-spp warm-up, with the
-    settings of guiding_bench (seed 7): prints its wall time."""
+    """One render of the guiding workload with the settings of guiding_bench
+    (seed 7), after the same render untimed (so that no buffer is allocated
+    in the timed one; the network is re-initialised in between): prints its
+    wall time."""
     import torch
     from aperturerendererv3_amd import make_batch
     from aperturerendererv3_amd.workloads import OCCLUDED_CAMERA, SPECTRAL_CAMERA, occluded_room, spectral_scene
@@ -601,11 +604,14 @@ def guided_render(ctx_cls, which):
              for _, cam, w, h, spp in scenes]
     shared = dict(max_depth=8, spectral=4, output=1)
     r = make_batch(ctxs) if which == "batch_learned" else ctxs[0]
-    r.render_views([dict(v, spp=4) for v in views], seed=99, **shared)
     extra = {}
     if which != "room_plain":
-        r.mlp_configure(seed=7, learning_rate=1e-2, iters_per_frame=32, batch=1 << 17)
         extra = dict(guiding=2, frame_spp=16, frame_growth=2)
+        r.mlp_configure(seed=7, learning_rate=1e-2, iters_per_frame=32, batch=1 << 17)
+    # the same render untimed first: every buffer is allocated before the timed one
+    r.render_views(views, seed=99, **shared, **extra)
+    if extra:
+        r.mlp_configure(seed=7, learning_rate=1e-2, iters_per_frame=32, batch=1 << 17)
     torch.cuda.synchronize()
     t = time.perf_counter()
     r.render_views(views, seed=7, **shared, **extra)
