@@ -655,6 +655,7 @@ struct GuideArgs {
     uint32_t stride;
     // learned guiding
     const float* dec;              // decoded mixture per queue slot [q][48]
+    const float* feat;             // network inputs per queue slot [q][kRec] (k_mlp_features)
     float* gx;                     // slot network inputs [5*kRec][stride] (features, scene index)
     float* gw;                     // slot direction [5*3][stride]
     float* gp;                     // slot direction pdf [5][stride]
@@ -858,6 +859,12 @@ __global__ __launch_bounds__(kMaxBlock) void k_mlp_features(const float4* __rest
 #pragma unroll
     for (int k = 0; k < mlp::kRec; ++k) feat[mlp::kRec * (size_t)q + k] = x[k];
 }
+
+// k_shade<AR_GUIDING_MLP>'s dynamic LDS: one row of kMixRow floats per
+// thread for its vertex's decoded mixture
+constexpr int kMixRow = mlp::kDecoded + 1;
+extern __shared__ float shade_mix_lds[];
+__device__ __forceinline__ float* shade_mix_row() { return shade_mix_lds + threadIdx.x * kMixRow; }
 
 // One bounce of CalculateRadiance for every queued path: emission with MIS
 // (render.cuh:664-710), BSDF sampling and the hit-point offset
@@ -1139,10 +1146,21 @@ __device__ __forceinline__ void shade_one(
                 // BSDF first and sample only the chosen one, so the direction's
                 // density is exactly the one-sample mixture (1 - alpha) p_bsdf +
                 // alpha p_guide however sharp the guide's lobes are
+                // the vertex's decoded mixture, copied to this lane's own LDS
+                // row (kMixRow floats, an odd stride: no bank conflicts): the
+                // evaluations below read it several times, and 12 waves per CU
+                // of 192-byte records do not stay in the L1
                 const float* pmix = nullptr;
                 bool via_guide = false, sampled;
                 if (learned && guided) {
-                    pmix = G.dec + (size_t)q * mlp::kDecoded;
+                    float* row = shade_mix_row();
+                    const float4* src = reinterpret_cast<const float4*>(G.dec + (size_t)q * mlp::kDecoded);
+#pragma unroll
+                    for (int k = 0; k < mlp::kDecoded / 4; ++k) {
+                        const float4 v = src[k];
+                        row[4 * k] = v.x; row[4 * k + 1] = v.y; row[4 * k + 2] = v.z; row[4 * k + 3] = v.w;
+                    }
+                    pmix = row;
                     via_guide = u[10] < P.alpha;
                 }
                 if (via_guide) {
@@ -1164,8 +1182,14 @@ __device__ __forceinline__ void shade_one(
                     }
                     float xr[mlp::kRec] = {0, 0, 0, 0, 0, 0};
                     if (learned) {
-                        mlp::features(pos.x, pos.y, pos.z, nn.x, nn.y, nn.z, V.box_lo, V.box_inv, xr);
-                        xr[mlp::kUsedIn] = (float)S.scn[p];
+                        if (guided) {
+                            // k_mlp_features took them from the same hit, the same way
+#pragma unroll
+                            for (int k = 0; k < mlp::kRec; ++k) xr[k] = G.feat[(size_t)mlp::kRec * q + k];
+                        } else {
+                            mlp::features(pos.x, pos.y, pos.z, nn.x, nn.y, nn.z, V.box_lo, V.box_inv, xr);
+                            xr[mlp::kUsedIn] = (float)S.scn[p];
+                        }
                     }
                     S.mp0[p] = pos.x; S.mp1[p] = pos.y; S.mp2[p] = pos.z;
                     const float off = eps * maxr(1.0f, length(pos)) * (u[4] + 0.5f);
@@ -2112,6 +2136,13 @@ static int prepare_job(ar_ctx* ctx, RenderJob& J, hipStream_t stream, bool sync)
         rc = mlp_ensure(ctx);
         if (rc == AR_OK) rc = mlp_ensure_buffers(ctx, L.cap);
         if (rc != AR_OK) return rc;
+        // the learned shading kernel's LDS rows: above 64 KB for blocks of more than 334 threads
+        rc = check_hip(ctx,
+                       hipFuncSetAttribute(reinterpret_cast<const void*>(&k_shade<AR_GUIDING_MLP>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)(kMaxBlock * kMixRow * sizeof(float))),
+                       "hipFuncSetAttribute(k_shade)");
+        if (rc != AR_OK) return rc;
     }
     if (J.base.spectral != AR_SPECTRAL_RGB) {
         rc = prepare_spectral_materials(ctx);
@@ -2205,6 +2236,7 @@ static int enqueue_job(ar_ctx* ctx, RenderJob& J, hipStream_t stream) {
         G.gm = gd.gm; G.gr = gd.gr; G.gkey = gd.gkey; G.gleaf = gd.gleaf; G.gflag = gd.gflag;
         G.stride = (uint32_t)gd.pcap;
         G.dec = ctx->mlp.dec;
+        G.feat = ctx->mlp.feat;
         G.gx = gd.gx; G.gw = gd.gw; G.gp = gd.gp;
     }
     RenderParams P = J.base;
@@ -2250,7 +2282,8 @@ static int enqueue_job(ar_ctx* ctx, RenderJob& J, hipStream_t stream) {
                 if (e != hipSuccess) return check_hip(ctx, e, "guiding network inference");
             }
 #define AR_SHADE(GM)                                                                                           \
-    hipLaunchKernelGGL(k_shade<GM>, dim3(grid_for(P.capacity, blk)), dim3(blk), 0, stream, P, depth, sc, S, G, q[c], \
+    hipLaunchKernelGGL(k_shade<GM>, dim3(grid_for(P.capacity, blk)), dim3(blk),                                 \
+                       GM == AR_GUIDING_MLP ? blk * kMixRow * sizeof(float) : 0, stream, P, depth, sc, S, G, q[c], \
                        b.qpath[c], b.hit_prim, b.hit_t, b.hit_u, b.hit_v, b.hit_flags, q[1 - c], b.qpath[1 - c], shq,  \
                        b.spath,                                                                                     \
                        b.scontrib[0], b.scontrib[1], b.scontrib[2], b.scontrib[3], b.sflags, b.counters, b.eps_table)
