@@ -97,19 +97,36 @@ __device__ __forceinline__ s16x8 lds8(const uint16_t* p) { return *reinterpret_c
 __device__ __forceinline__ s16x8 wfrag(const uint16_t* row, int j, int g) {
     return cat(ld4(row + 32 * j + 4 * g), ld4(row + 32 * j + 16 + 4 * g));
 }
-// bias, ReLU and BF16 rounding of a D fragment: the next layer's B half
-__device__ __forceinline__ bf16x4 relu_bf16(f32x4 c, const float b[4]) {
-    bf16x4 h;
-#pragma unroll
-    for (int r = 0; r < 4; ++r) h[r] = (short)f2bf(fmaxf(c[r] + b[r], 0.0f));
-    return h;
+typedef short s16x2 __attribute__((ext_vector_type(2)));
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ s16x2 bf16_pair(float a, float b) {
+    return __builtin_bit_cast(s16x2, __builtin_convertvector((f32x2){a, b}, bf16x2));
 }
+// ReLU and BF16 rounding of a D fragment (whose accumulator started at the
+// bias): the next layer's B half.  BF16 of max(c, 0) for every c but NaN:
+// each pair is rounded to BF16 first, then kept or zeroed by a max with 0 as
+// signed 16-bit integers (a negative value, -0 included, has its sign bit
+// set and so is a negative integer), so the result is +0 or positive.
+__device__ __forceinline__ bf16x4 relu_pk(f32x4 c) {
+    const s16x2 z = {0, 0};
+    const s16x2 lo = __builtin_elementwise_max(bf16_pair(c[0], c[1]), z);
+    const s16x2 hi = __builtin_elementwise_max(bf16_pair(c[2], c[3]), z);
+    return (bf16x4){lo[0], lo[1], hi[0], hi[1]};
+}
+// BF16 of c where the layer's ReLU output h (from relu_pk: +0 or positive)
+// is non-zero, else +0: the mask is 0 - min(h, 1) per 16-bit half.
 __device__ __forceinline__ bf16x4 relu_mask(bf16x4 h, f32x4 c) {
-    bf16x4 r;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) r[k] = (short)f2bf(bf2f((uint16_t)h[k]) > 0.0f ? c[k] : 0.0f);
-    return r;
+    const u16x2 one = {1, 1}, zero = {0, 0};
+    const u16x2 mlo = zero - __builtin_elementwise_min(__builtin_bit_cast(u16x2, (s16x2){h[0], h[1]}), one);
+    const u16x2 mhi = zero - __builtin_elementwise_min(__builtin_bit_cast(u16x2, (s16x2){h[2], h[3]}), one);
+    const u16x2 lo = __builtin_bit_cast(u16x2, bf16_pair(c[0], c[1])) & mlo;
+    const u16x2 hi = __builtin_bit_cast(u16x2, bf16_pair(c[2], c[3])) & mhi;
+    return (bf16x4){(short)lo[0], (short)lo[1], (short)hi[0], (short)hi[1]};
 }
+// a layer's four biases of lane group g, the initial accumulator of its D fragment
+__device__ __forceinline__ f32x4 bias4(const float* b) { return *reinterpret_cast<const f32x4*>(b); }
 // Keeps the loads of one layer next to its products: the scheduler would
 // otherwise issue every layer's weight fragments at once (about 130
 // registers) and spill.
@@ -133,7 +150,7 @@ constexpr int kBias = 3 * kHid + kOut;     // b0, b1, b2, b3
 
 struct TrainShared {
     uint16_t wgt[kLdsW];
-    float bias[kBias];
+    alignas(16) float bias[kBias];   // read four at a time (bias4)
     union {
         struct { uint16_t ht[kHid * kTS]; uint16_t zt[kHid * kTS]; } t;   // transposed activations / gradients
         float r[kTW][kTT * kRS];                                          // loss stage (per wave)
@@ -152,23 +169,26 @@ __device__ __forceinline__ void store_t(uint16_t* t, const bf16x4 (&v)[4][2], in
             for (int r = 0; r < 4; ++r) t[(mb * 16 + 4 * g + r) * kTS + col0 + nb * 16 + i] = (uint16_t)v[mb][nb][r];
 }
 
-// Z of the next layer from a 64-unit layer: D = W^T . H over two K blocks
+// the next layer's activations from a 64-unit layer: D = b + W^T . H over two
+// K blocks.  BIAS_INIT: the bias is the accumulator's initial value (the
+// inference kernel); else it is added to the product (the training kernel,
+// whose gradient accumulators leave no registers for the bias fragments
+// across the products).
+template <bool BIAS_INIT>
 __device__ __forceinline__ void fwd_hidden(const uint16_t* wt, const float* bias, const bf16x4 (&hin)[4][2],
                                            bf16x4 (&hout)[4][2], int i, int g) {
     layer_fence();
+    const f32x4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
     for (int mb = 0; mb < 4; ++mb) {
         const uint16_t* row = wt + (mb * 16 + i) * kS64;
         const s16x8 a0 = wfrag(row, 0, g), a1 = wfrag(row, 1, g);
-        float b[4];
-#pragma unroll
-        for (int r = 0; r < 4; ++r) b[r] = bias[mb * 16 + 4 * g + r];
 #pragma unroll
         for (int nb = 0; nb < 2; ++nb) {
-            f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
-            c = mfma32(a0, cat(hin[0][nb], hin[1][nb]), c);
+            f32x4 c = mfma32(a0, cat(hin[0][nb], hin[1][nb]), BIAS_INIT ? bias4(bias + mb * 16 + 4 * g) : zero);
             c = mfma32(a1, cat(hin[2][nb], hin[3][nb]), c);
-            hout[mb][nb] = relu_bf16(c, b);
+            if (!BIAS_INIT) c += bias4(bias + mb * 16 + 4 * g);
+            hout[mb][nb] = relu_pk(c);
         }
     }
 }
@@ -345,29 +365,23 @@ __global__ __launch_bounds__(kTB) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 #pragma unroll
         for (int mb = 0; mb < 4; ++mb) {
             const bf16x4 a = ld4(S.wgt + LT0 + (mb * 16 + i) * kS16 + 4 * g);
-            float b[4];
 #pragma unroll
-            for (int r = 0; r < 4; ++r) b[r] = S.bias[mb * 16 + 4 * g + r];
-#pragma unroll
-            for (int nb = 0; nb < 2; ++nb) h1[mb][nb] = relu_bf16(mfma(a, xb[nb], zero), b);
+            for (int nb = 0; nb < 2; ++nb) h1[mb][nb] = relu_pk(mfma(a, xb[nb], zero) + bias4(S.bias + mb * 16 + 4 * g));
         }
-        fwd_hidden(S.wgt + LT1, S.bias + kHid, h1, h2, i, g);
-        fwd_hidden(S.wgt + LT2, S.bias + 2 * kHid, h2, h3, i, g);
+        fwd_hidden<false>(S.wgt + LT1, S.bias + kHid, h1, h2, i, g);
+        fwd_hidden<false>(S.wgt + LT2, S.bias + 2 * kHid, h2, h3, i, g);
         float* R = S.u.r[w];
         layer_fence();
 #pragma unroll
         for (int mb = 0; mb < 3; ++mb) {
             const uint16_t* row = S.wgt + LT3 + (mb * 16 + i) * kS64;
             const s16x8 a0 = wfrag(row, 0, g), a1 = wfrag(row, 1, g);
-            float b[4];
-#pragma unroll
-            for (int r = 0; r < 4; ++r) b[r] = S.bias[3 * kHid + mb * 16 + 4 * g + r];
 #pragma unroll
             for (int nb = 0; nb < 2; ++nb) {
                 f32x4 c = mfma32(a0, cat(h3[0][nb], h3[1][nb]), zero);
                 c = mfma32(a1, cat(h3[2][nb], h3[3][nb]), c);
-                *reinterpret_cast<float4*>(&R[(nb * 16 + i) * kRS + mb * 16 + 4 * g]) =
-                    make_float4(c[0] + b[0], c[1] + b[1], c[2] + b[2], c[3] + b[3]);
+                c += bias4(S.bias + 3 * kHid + mb * 16 + 4 * g);
+                *reinterpret_cast<float4*>(&R[(nb * 16 + i) * kRS + mb * 16 + 4 * g]) = make_float4(c[0], c[1], c[2], c[3]);
             }
         }
         // ---- loss gradient, two lanes per sample (lanes s and s + 32)
@@ -487,25 +501,30 @@ __global__ __launch_bounds__(kTB) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 //
 // k_mlp_infer: the training kernel's forward on 4-wave blocks (two blocks per
 // CU), forward weights and biases staged in LDS, a 32-sample tile per wave
-// chained through the layers in registers on the 16x16x32 BF16 MFMA.
-// Raw outputs are stored from the D fragments; DECODE sends each tile's
-// outputs through LDS to two lanes per vertex (lobes 0..3 / 4..7), which
-// write the 48 decoded floats per vertex: per lobe (pi, pi * kappa /
-// (2 pi (1 - e^-2kappa)), kappa, mu).
+// chained through the layers in registers on the 16x16x32 BF16 MFMA.  Each
+// layer's bias is the accumulator's initial value and its ReLU acts on the
+// BF16 pairs (relu_pk).  Raw outputs are stored from the D fragments; DECODE
+// sends each tile's outputs through LDS to two lanes per vertex (lobes 0..3 /
+// 4..7), which write the 48 decoded floats per vertex: per lobe (pi, pi *
+// kappa / (2 pi (1 - e^-2kappa)), kappa, mu), with hardware exp, reciprocal
+// and reciprocal square root (about 1 ulp each; sampling and densities read
+// the same decoded values, so these roundings change the guide, not the
+// consistency of its samples and densities).  Offsets are 32-bit: one launch
+// covers at most kInferMax vertices.
 constexpr int kIW = 4;
+constexpr size_t kInferMax = (size_t)1 << 26;
 struct InferShared {
     uint16_t wgt[LB3];                 // forward weights W^T, padded rows (LT0 .. LT3)
-    float bias[kBias];
+    alignas(16) float bias[kBias];   // read four at a time (bias4)
     float r[kIW][kTT * kRS];           // output tile per wave (DECODE)
 };
 
 template <bool DECODE, int XS>
-__global__ __launch_bounds__(kIW * 64) void k_mlp_infer(Net N, const float* __restrict__ x, size_t n,
+__global__ __launch_bounds__(kIW * 64) void k_mlp_infer(Net N, const float* __restrict__ x, uint32_t n,
                                                         float* __restrict__ out, const uint32_t* __restrict__ n_dev) {
     __shared__ InferShared S;
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63, i = lane & 15, g = lane >> 4;
-    const size_t nn = n_dev ? std::min<size_t>(n, *n_dev) : n;
-    const f32x4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
+    const uint32_t nn = n_dev ? std::min(n, *n_dev) : n;
     {
         struct Mat { int src, rows, cols, dst, stride; };
         const Mat mats[4] = {{OW0, kHid, kIn, LT0, kS16}, {OW1, kHid, kHid, LT1, kS64}, {OW2, kHid, kHid, LT2, kS64},
@@ -541,11 +560,11 @@ __global__ __launch_bounds__(kIW * 64) void k_mlp_infer(Net N, const float* __re
     // The inputs of a tile are loaded one tile ahead, so their latency
     // overlaps the previous tile's products: raw[nb][r] is the record value
     // behind input 4g + r (a feature, or the scene index for its bits).
-    const size_t tstride = (size_t)gridDim.x * kIW * kTT;
-    auto load_raw = [&](size_t t, float raw[2][4]) {
+    const uint32_t tstride = gridDim.x * kIW * kTT;
+    auto load_raw = [&](uint32_t t, float raw[2][4]) {
 #pragma unroll
         for (int nb = 0; nb < 2; ++nb) {
-            const size_t s = t + nb * 16 + i;
+            const uint32_t s = t + nb * 16 + i;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int k = 4 * g + r;
@@ -555,8 +574,8 @@ __global__ __launch_bounds__(kIW * 64) void k_mlp_infer(Net N, const float* __re
         }
     };
     float cur[2][4];
-    load_raw(((size_t)blockIdx.x * kIW + w) * kTT, cur);
-    for (size_t t0 = ((size_t)blockIdx.x * kIW + w) * kTT; t0 < nn; t0 += tstride) {
+    load_raw((blockIdx.x * kIW + w) * kTT, cur);
+    for (uint32_t t0 = (blockIdx.x * kIW + w) * kTT; t0 < nn; t0 += tstride) {
         float nxt[2][4];
         load_raw(t0 + tstride, nxt);
         bf16x4 xb[2];
@@ -574,34 +593,31 @@ __global__ __launch_bounds__(kIW * 64) void k_mlp_infer(Net N, const float* __re
 #pragma unroll
         for (int mb = 0; mb < 4; ++mb) {
             const bf16x4 a = ld4(S.wgt + LT0 + (mb * 16 + i) * kS16 + 4 * g);
-            float b[4];
+            const f32x4 b = bias4(S.bias + mb * 16 + 4 * g);
 #pragma unroll
-            for (int r = 0; r < 4; ++r) b[r] = S.bias[mb * 16 + 4 * g + r];
-#pragma unroll
-            for (int nb = 0; nb < 2; ++nb) h1[mb][nb] = relu_bf16(mfma(a, xb[nb], zero), b);
+            for (int nb = 0; nb < 2; ++nb) h1[mb][nb] = relu_pk(mfma(a, xb[nb], b));
         }
-        fwd_hidden(S.wgt + LT1, S.bias + kHid, h1, h2, i, g);
-        fwd_hidden(S.wgt + LT2, S.bias + 2 * kHid, h2, h1, i, g);   // h1 now holds the third layer
+        fwd_hidden<true>(S.wgt + LT1, S.bias + kHid, h1, h2, i, g);
+        fwd_hidden<true>(S.wgt + LT2, S.bias + 2 * kHid, h2, h1, i, g);   // h1 now holds the third layer
         layer_fence();
-        if (DECODE) wave_sync();                                    // the previous tile's decode has read R
+        if (DECODE) wave_sync();                                        // the previous tile's decode has read R
 #pragma unroll
         for (int mb = 0; mb < 3; ++mb) {
             const uint16_t* row = S.wgt + LT3 + (mb * 16 + i) * kS64;
             const s16x8 a0 = wfrag(row, 0, g), a1 = wfrag(row, 1, g);
-            float b[4];
-#pragma unroll
-            for (int r = 0; r < 4; ++r) b[r] = S.bias[3 * kHid + mb * 16 + 4 * g + r];
+            const f32x4 b = bias4(S.bias + 3 * kHid + mb * 16 + 4 * g);
 #pragma unroll
             for (int nb = 0; nb < 2; ++nb) {
-                f32x4 c = mfma32(a0, cat(h1[0][nb], h1[1][nb]), zero);
+                f32x4 c = mfma32(a0, cat(h1[0][nb], h1[1][nb]), b);
                 c = mfma32(a1, cat(h1[2][nb], h1[3][nb]), c);
-                const float4 v = make_float4(c[0] + b[0], c[1] + b[1], c[2] + b[2], c[3] + b[3]);
-                const int u0 = mb * 16 + 4 * g, sl = nb * 16 + i;
+                const float4 v = make_float4(c[0], c[1], c[2], c[3]);
+                const uint32_t u0 = mb * 16 + 4 * g, sl = nb * 16 + i;
                 if (DECODE) *reinterpret_cast<float4*>(&R[sl * kRS + u0]) = v;
                 else if (u0 < kUsedOut && t0 + sl < nn) *reinterpret_cast<float4*>(&out[(t0 + sl) * kUsedOut + u0]) = v;
             }
         }
         if (DECODE) {
+            constexpr float kLog2e = 1.4426950408889634f;
             wave_sync();
             const int sl = lane & (kTT - 1), h = lane >> 5;
             const float* o = &R[sl * kRS];
@@ -611,22 +627,26 @@ __global__ __launch_bounds__(kIW * 64) void k_mlp_infer(Net N, const float* __re
             mx = fmaxf(mx, __shfl_xor(mx, 32));
             float pu[4], z = 0.0f;
 #pragma unroll
-            for (int j = 0; j < 4; ++j) { pu[j] = expf(o[5 * (4 * h + j)] - mx); z += pu[j]; }
+            for (int j = 0; j < 4; ++j) { pu[j] = __builtin_amdgcn_exp2f((o[5 * (4 * h + j)] - mx) * kLog2e); z += pu[j]; }
             z += __shfl_xor(z, 32);
+            const float iz = __builtin_amdgcn_rcpf(z);
             float dv[24];
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const int k = 4 * h + j;
-                const float pi = pu[j] / z;
-                const float kap = expf(fminf(7.0f, fmaxf(-3.0f, o[5 * k + 1])));
+                const float pi = pu[j] * iz;
+                const float kap = __builtin_amdgcn_exp2f(fminf(7.0f, fmaxf(-3.0f, o[5 * k + 1])) * kLog2e);
+                const float e2 = __builtin_amdgcn_exp2f(-2.0f * kLog2e * kap);
                 const float cx = o[5 * k + 2], cy = o[5 * k + 3], cz = o[5 * k + 4];
-                const float l = sqrtf(cx * cx + cy * cy + cz * cz);
+                const float l2 = cx * cx + cy * cy + cz * cz;
+                const float il = __builtin_amdgcn_rsqf(l2);
+                const bool deg = !(l2 >= 1e-24f);
                 dv[6 * j + 0] = pi;
-                dv[6 * j + 1] = pi * vmf_norm(kap);
+                dv[6 * j + 1] = pi * kap * __builtin_amdgcn_rcpf(kTwoPi * (1.0f - e2));
                 dv[6 * j + 2] = kap;
-                dv[6 * j + 3] = l < 1e-12f ? 0.0f : cx / l;
-                dv[6 * j + 4] = l < 1e-12f ? 0.0f : cy / l;
-                dv[6 * j + 5] = l < 1e-12f ? 1.0f : cz / l;
+                dv[6 * j + 3] = deg ? 0.0f : cx * il;
+                dv[6 * j + 4] = deg ? 0.0f : cy * il;
+                dv[6 * j + 5] = deg ? 1.0f : cz * il;
             }
             // the decoded tile (32 vertices x 48 floats) goes back through
             // R, then out in whole 16-byte pieces of consecutive vertices
@@ -638,7 +658,7 @@ __global__ __launch_bounds__(kIW * 64) void k_mlp_infer(Net N, const float* __re
             wave_sync();
 #pragma unroll
             for (int q = 0; q < 6; ++q) {
-                const int c = lane + 64 * q, v = c / 12, part = c - 12 * v;
+                const uint32_t c = lane + 64 * q, v = c / 12, part = c - 12 * v;
                 if (t0 + v < nn)
                     *reinterpret_cast<float4*>(&out[(t0 + v) * kDecoded + 4 * part]) =
                         *reinterpret_cast<const float4*>(&R[v * kRS + 4 * part]);
@@ -1101,15 +1121,26 @@ hipError_t mlp_forward(ar_ctx* ctx, const float* x5, int xs, size_t n, const uin
                        hipStream_t s) {
     MlpDev& M = ctx->mlp;
     Net N{M.p, M.wb, M.wt};
-    // persistent: two 4-wave blocks per CU, each wave looping over 32-sample tiles
-    const uint32_t grid = (uint32_t)std::min<size_t>((n + kIW * kTT - 1) / (kIW * kTT),
-                                                     (size_t)std::max(ctx->num_cus, 1) * 2);
-    const dim3 g(std::max(grid, 1u)), b(kIW * 64);
-    if (decoded && xs == kRec) hipLaunchKernelGGL((k_mlp_infer<true, kRec>), g, b, 0, s, N, x5, n, out, n_dev);
-    else if (decoded) hipLaunchKernelGGL((k_mlp_infer<true, kUsedIn>), g, b, 0, s, N, x5, n, out, n_dev);
-    else if (xs == kRec) hipLaunchKernelGGL((k_mlp_infer<false, kRec>), g, b, 0, s, N, x5, n, out, n_dev);
-    else hipLaunchKernelGGL((k_mlp_infer<false, kUsedIn>), g, b, 0, s, N, x5, n, out, n_dev);
-    return hipGetLastError();
+    // a device count bounds one launch (the integrator's queues hold at most
+    // kInferMax vertices); a host count is split into launches of kInferMax
+    if (n_dev && n > kInferMax) return hipErrorInvalidValue;
+    const int ow = decoded ? kDecoded : kUsedOut;
+    for (size_t first = 0; first < n; first += kInferMax) {
+        const uint32_t m = (uint32_t)std::min(n - first, kInferMax);
+        const float* xf = x5 + first * xs;
+        float* of = out + first * ow;
+        // persistent: two 4-wave blocks per CU, each wave looping over 32-sample tiles
+        const uint32_t grid = (uint32_t)std::min<size_t>((m + kIW * kTT - 1) / (kIW * kTT),
+                                                         (size_t)std::max(ctx->num_cus, 1) * 2);
+        const dim3 g(std::max(grid, 1u)), b(kIW * 64);
+        if (decoded && xs == kRec) hipLaunchKernelGGL((k_mlp_infer<true, kRec>), g, b, 0, s, N, xf, m, of, n_dev);
+        else if (decoded) hipLaunchKernelGGL((k_mlp_infer<true, kUsedIn>), g, b, 0, s, N, xf, m, of, n_dev);
+        else if (xs == kRec) hipLaunchKernelGGL((k_mlp_infer<false, kRec>), g, b, 0, s, N, xf, m, of, n_dev);
+        else hipLaunchKernelGGL((k_mlp_infer<false, kUsedIn>), g, b, 0, s, N, xf, m, of, n_dev);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 // gradient into M.g (zeroed first unless accumulate) and loss into M.scal[1];

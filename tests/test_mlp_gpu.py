@@ -75,6 +75,28 @@ def test_forward_matches_fp64_oracle(ctx):
     assert errb < 1e-3 * scale
 
 
+def test_forward_splits_host_counts_above_2_26(ctx):
+    """A host count above 2^26 vertices runs as launches of at most 2^26 (the
+    kernel's offsets are 32-bit): vertices on both sides of the split get
+    the same bits as a call on them alone (each vertex's outputs depend on
+    its own inputs only)."""
+    import torch
+    P = init_params(14).astype(np.float32)
+    ctx.mlp_set_params(P)
+    n = (1 << 26) + 1000
+    gen = torch.Generator(device="cuda").manual_seed(5)
+    x = torch.rand((n, 5), device="cuda", generator=gen)
+    out = ctx.mlp_forward(x)
+    lo = (1 << 26) - 1000
+    part = ctx.mlp_forward(x[lo:].contiguous())
+    torch.cuda.synchronize()
+    a, b = out[lo:].cpu().numpy(), part.cpu().numpy()
+    del out, x
+    torch.cuda.empty_cache()
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    assert np.all(np.isfinite(b)) and np.abs(b).max() > 0
+
+
 def test_gradient_matches_fp64_oracle(ctx):
     P = init_params(14).astype(np.float32)
     ctx.mlp_set_params(P)
