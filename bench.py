@@ -521,8 +521,9 @@ def guiding_bench(ctx_cls, room_spp=1024, spectral_spp=128):
 
     # learned: frames of 16, 32, 64, ... spp (growth 2), 32 Adam steps of
     # 2^17 records after each; voxel: the reference's probes in frames of
-    # 64 (room) / 16 (spectral) spp.  Each method: three seeds (the network
-    # seeded alike), means of time and relMSE.
+    # 64 (room) / 16 (spectral) spp.  Each method: eight seeds (the network
+    # seeded alike), means of time and relMSE: the spectral window's error is
+    # dominated by a few caustic fireflies and varies severalfold by seed.
     mlp_cfg = dict(learning_rate=1e-2, iters_per_frame=32, batch=1 << 17)
     methods = {
         "plain": (None, {}),
@@ -531,7 +532,7 @@ def guiding_bench(ctx_cls, room_spp=1024, spectral_spp=128):
     }
     frame = {"room": 64, "spectral": 16}
     spp_of = {"room": room_spp, "spectral": spectral_spp}
-    seeds = {"plain": (1, 2, 3), "voxel": (7, 8, 9), "learned": (7, 8, 9)}
+    seeds = {"plain": tuple(range(1, 9)), "voxel": tuple(range(11, 19)), "learned": tuple(range(11, 19))}
     for name in scenes:
         spp = spp_of[name]
         res = {}
