@@ -654,7 +654,7 @@ struct GuideArgs {
                                    // bit 8 + i (learned): slot i awaits its next vertex
     uint32_t stride;
     // learned guiding
-    const float* dec;              // decoded mixture per queue slot [q][48]
+    const float* dec;              // mixture per queue slot [q][kPacked]: per lobe (pi, kappa, octahedral mu)
     const float* feat;             // network inputs per queue slot [q][kRec] (k_mlp_features)
     float* gx;                     // slot network inputs [5*kRec][stride] (features, scene index)
     float* gw;                     // slot direction [5*3][stride]
@@ -1146,19 +1146,23 @@ __device__ __forceinline__ void shade_one(
                 // BSDF first and sample only the chosen one, so the direction's
                 // density is exactly the one-sample mixture (1 - alpha) p_bsdf +
                 // alpha p_guide however sharp the guide's lobes are
-                // the vertex's decoded mixture, copied to this lane's own LDS
-                // row (kMixRow floats, an odd stride: no bank conflicts): the
-                // evaluations below read it several times, and 12 waves per CU
-                // of 192-byte records do not stay in the L1
+                // the vertex's mixture (kPacked words from inference),
+                // unpacked once into this lane's own LDS row (kMixRow floats,
+                // an odd stride: no bank conflicts) that the evaluations below
+                // read several times
                 const float* pmix = nullptr;
                 bool via_guide = false, sampled;
                 if (learned && guided) {
                     float* row = shade_mix_row();
-                    const float4* src = reinterpret_cast<const float4*>(G.dec + (size_t)q * mlp::kDecoded);
+                    const float4* src = reinterpret_cast<const float4*>(G.dec + (size_t)q * mlp::kPacked);
 #pragma unroll
-                    for (int k = 0; k < mlp::kDecoded / 4; ++k) {
-                        const float4 v = src[k];
-                        row[4 * k] = v.x; row[4 * k + 1] = v.y; row[4 * k + 2] = v.z; row[4 * k + 3] = v.w;
+                    for (int k = 0; k < mlp::kPacked / 12; ++k) {   // four lobes per three 16-byte loads
+                        const float4 a = src[3 * k], b = src[3 * k + 1], c = src[3 * k + 2];
+                        float* l = row + 24 * k;
+                        mlp::unpack_lobe(a.x, a.y, __float_as_uint(a.z), l);
+                        mlp::unpack_lobe(a.w, b.x, __float_as_uint(b.y), l + 6);
+                        mlp::unpack_lobe(b.z, b.w, __float_as_uint(c.x), l + 12);
+                        mlp::unpack_lobe(c.y, c.z, __float_as_uint(c.w), l + 18);
                     }
                     pmix = row;
                     via_guide = u[10] < P.alpha;

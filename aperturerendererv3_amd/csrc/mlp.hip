@@ -505,12 +505,12 @@ __global__ __launch_bounds__(kTB) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 // layer's bias is the accumulator's initial value and its ReLU acts on the
 // BF16 pairs (relu_pk).  Raw outputs are stored from the D fragments; DECODE
 // sends each tile's outputs through LDS to two lanes per vertex (lobes 0..3 /
-// 4..7), which write the 48 decoded floats per vertex: per lobe (pi, pi *
-// kappa / (2 pi (1 - e^-2kappa)), kappa, mu), with hardware exp, reciprocal
-// and reciprocal square root (about 1 ulp each; sampling and densities read
-// the same decoded values, so these roundings change the guide, not the
-// consistency of its samples and densities).  Offsets are 32-bit: one launch
-// covers at most kInferMax vertices.
+// 4..7), which write kPacked words per vertex: per lobe (pi, kappa, mu as
+// 16-bit octahedral coordinates), with hardware exp and reciprocal (about
+// 1 ulp each; sampling and densities both read these stored values, so such
+// roundings change the guide, not the agreement of its samples with its
+// densities).  Offsets are 32-bit: one launch covers at most kInferMax
+// vertices.
 constexpr int kIW = 4;
 constexpr size_t kInferMax = (size_t)1 << 26;
 struct InferShared {
@@ -630,37 +630,29 @@ __global__ __launch_bounds__(kIW * 64) void k_mlp_infer(Net N, const float* __re
             for (int j = 0; j < 4; ++j) { pu[j] = __builtin_amdgcn_exp2f((o[5 * (4 * h + j)] - mx) * kLog2e); z += pu[j]; }
             z += __shfl_xor(z, 32);
             const float iz = __builtin_amdgcn_rcpf(z);
-            float dv[24];
+            float dv[12];
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const int k = 4 * h + j;
-                const float pi = pu[j] * iz;
-                const float kap = __builtin_amdgcn_exp2f(fminf(7.0f, fmaxf(-3.0f, o[5 * k + 1])) * kLog2e);
-                const float e2 = __builtin_amdgcn_exp2f(-2.0f * kLog2e * kap);
                 const float cx = o[5 * k + 2], cy = o[5 * k + 3], cz = o[5 * k + 4];
-                const float l2 = cx * cx + cy * cy + cz * cz;
-                const float il = __builtin_amdgcn_rsqf(l2);
-                const bool deg = !(l2 >= 1e-24f);
-                dv[6 * j + 0] = pi;
-                dv[6 * j + 1] = pi * kap * __builtin_amdgcn_rcpf(kTwoPi * (1.0f - e2));
-                dv[6 * j + 2] = kap;
-                dv[6 * j + 3] = deg ? 0.0f : cx * il;
-                dv[6 * j + 4] = deg ? 0.0f : cy * il;
-                dv[6 * j + 5] = deg ? 1.0f : cz * il;
+                const bool deg = !(cx * cx + cy * cy + cz * cz >= 1e-24f);
+                dv[3 * j + 0] = pu[j] * iz;
+                dv[3 * j + 1] = __builtin_amdgcn_exp2f(fminf(7.0f, fmaxf(-3.0f, o[5 * k + 1])) * kLog2e);
+                dv[3 * j + 2] = __builtin_bit_cast(float, deg ? oct_pack(0.0f, 0.0f, 1.0f) : oct_pack(cx, cy, cz));
             }
-            // the decoded tile (32 vertices x 48 floats) goes back through
-            // R, then out in whole 16-byte pieces of consecutive vertices
+            // the stored tile (32 vertices x 24 words) goes back through R,
+            // then out in whole 16-byte pieces of consecutive vertices
             wave_sync();                                            // every lane has read its raw outputs
 #pragma unroll
-            for (int q = 0; q < 6; ++q)
-                *reinterpret_cast<float4*>(&R[sl * kRS + 24 * h + 4 * q]) =
+            for (int q = 0; q < 3; ++q)
+                *reinterpret_cast<float4*>(&R[sl * kRS + 12 * h + 4 * q]) =
                     make_float4(dv[4 * q], dv[4 * q + 1], dv[4 * q + 2], dv[4 * q + 3]);
             wave_sync();
 #pragma unroll
-            for (int q = 0; q < 6; ++q) {
-                const uint32_t c = lane + 64 * q, v = c / 12, part = c - 12 * v;
+            for (int q = 0; q < 3; ++q) {
+                const uint32_t c = lane + 64 * q, v = c / 6, part = c - 6 * v;
                 if (t0 + v < nn)
-                    *reinterpret_cast<float4*>(&out[(t0 + v) * kDecoded + 4 * part]) =
+                    *reinterpret_cast<float4*>(&out[(t0 + v) * kPacked + 4 * part]) =
                         *reinterpret_cast<const float4*>(&R[v * kRS + 4 * part]);
             }
         }
@@ -1009,7 +1001,7 @@ int mlp_ensure_buffers(ar_ctx* ctx, size_t paths) {
     if (M.rblk) { (void)hipFree(M.rblk); M.rblk = nullptr; }
     M.cap = M.rcap = M.rblocks = 0;
     hipError_t e = hipMalloc((void**)&M.feat, paths * kRec * sizeof(float));
-    if (e == hipSuccess) e = hipMalloc((void**)&M.dec, paths * kDecoded * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc((void**)&M.dec, paths * kPacked * sizeof(float));
     if (e == hipSuccess) e = hipMalloc((void**)&M.rx, rcap * kRec * sizeof(float));
     if (e == hipSuccess) e = hipMalloc((void**)&M.rd, rcap * 3 * sizeof(float));
     if (e == hipSuccess) e = hipMalloc((void**)&M.rw, rcap * sizeof(float));
@@ -1124,7 +1116,7 @@ hipError_t mlp_forward(ar_ctx* ctx, const float* x5, int xs, size_t n, const uin
     // a device count bounds one launch (the integrator's queues hold at most
     // kInferMax vertices); a host count is split into launches of kInferMax
     if (n_dev && n > kInferMax) return hipErrorInvalidValue;
-    const int ow = decoded ? kDecoded : kUsedOut;
+    const int ow = decoded ? kPacked : kUsedOut;
     for (size_t first = 0; first < n; first += kInferMax) {
         const uint32_t m = (uint32_t)std::min(n - first, kInferMax);
         const float* xf = x5 + first * xs;

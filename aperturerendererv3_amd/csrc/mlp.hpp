@@ -42,7 +42,10 @@ constexpr int OB2 = OW2 + kHid * kHid;
 constexpr int OW3 = OB2 + kHid;
 constexpr int OB3 = OW3 + kHid * kOut;
 constexpr int kParams = OB3 + kOut;
-constexpr int kDecoded = 48;   // per vertex: 8 lobes x (pi, norm, kappa, mu xyz)
+constexpr int kDecoded = 48;   // per vertex, evaluation form: 8 lobes x (pi, norm, kappa, mu xyz)
+// per vertex as inference stores it: 8 lobes x (pi, kappa, mu as two 16-bit
+// octahedral coordinates); unpack_mixture turns it into the evaluation form
+constexpr int kPacked = 24;
 // largest probability of sampling the learned guide in the one-sample
 // mixture with the BSDF, reached after 64 samples per pixel as the voxel
 // guide's 0.5 is (measured on the occluded-light room: 0.75 lowers the error
@@ -94,6 +97,46 @@ __device__ __forceinline__ void mixture_sample(const float* __restrict__ d, floa
     wx = st * cp * tx + st * sp * bx + ct * mx;
     wy = st * cp * ty + st * sp * by + ct * my;
     wz = st * cp * tz + st * sp * bz + ct * mz;
+}
+
+// A direction as two signed-normalised 16-bit octahedral coordinates
+// (Cigolle et al. 2014): the point (x, y) / (|x| + |y| + |z|), with the lower
+// hemisphere folded over the square's diagonals.  w need not be unit length.
+__device__ __forceinline__ uint32_t oct_pack(float x, float y, float z) {
+    const float s = __builtin_amdgcn_rcpf(fabsf(x) + fabsf(y) + fabsf(z));
+    float u = x * s, v = y * s;
+    if (z < 0.0f) {
+        const float fu = (1.0f - fabsf(v)) * (u >= 0.0f ? 1.0f : -1.0f);
+        v = (1.0f - fabsf(u)) * (v >= 0.0f ? 1.0f : -1.0f);
+        u = fu;
+    }
+    const int qu = __float2int_rn(fminf(1.0f, fmaxf(-1.0f, u)) * 32767.0f);
+    const int qv = __float2int_rn(fminf(1.0f, fmaxf(-1.0f, v)) * 32767.0f);
+    return ((uint32_t)qu & 0xFFFFu) | ((uint32_t)qv << 16);
+}
+// the unit direction of a packed pair
+__device__ __forceinline__ void oct_unpack(uint32_t w, float& x, float& y, float& z) {
+    x = (float)(int16_t)(w & 0xFFFFu) * (1.0f / 32767.0f);
+    y = (float)(int16_t)(w >> 16) * (1.0f / 32767.0f);
+    z = 1.0f - fabsf(x) - fabsf(y);
+    const float t = fmaxf(-z, 0.0f);
+    x += x >= 0.0f ? -t : t;
+    y += y >= 0.0f ? -t : t;
+    const float il = __builtin_amdgcn_rsqf(x * x + y * y + z * z);
+    x *= il;
+    y *= il;
+    z *= il;
+}
+// evaluation form (kDecoded floats) of a stored mixture (kPacked words):
+// the normaliser pi kappa / (2 pi (1 - e^-2kappa)) and unit mean directions
+// are computed here, once per vertex, from the stored values that sampling
+// also reads, so samples and densities describe the same distribution.
+__device__ __forceinline__ void unpack_lobe(float pi, float kap, uint32_t oct, float* __restrict__ l) {
+    constexpr float kLog2e = 1.4426950408889634f;
+    l[0] = pi;
+    l[1] = pi * kap * __builtin_amdgcn_rcpf(kTwoPi * (1.0f - __builtin_amdgcn_exp2f(-2.0f * kLog2e * kap)));
+    l[2] = kap;
+    oct_unpack(oct, l[3], l[4], l[5]);
 }
 
 // hemisphere fold of the mixture, as for the voxel probes (octahedron.cuh:
