@@ -43,11 +43,17 @@ def _sources():
     return sorted(CSRC.glob("*.hip")) + sorted(CSRC.glob("*.cpp"))
 
 
+# Per-source device flags.  mlp.hip: MFMA accumulators in VGPRs, not AGPRs
+# (gfx950 allows either), which saves the guiding network's inference kernel
+# its 96 accumulator-to-VGPR copies per 32-vertex tile.
+SOURCE_FLAGS = {"mlp.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
+
+
 def _compile(src: Path) -> Path:
     obj = BUILD_DIR / (src.name + ".o")
     cmd = [_hipcc(), "-c", str(src), "-o", str(obj)] + COMMON + DEVICE
     if src.suffix == ".hip":
-        cmd = [_hipcc(), "-x", "hip", "-c", str(src), "-o", str(obj)] + COMMON + DEVICE
+        cmd = [_hipcc(), "-x", "hip", "-c", str(src), "-o", str(obj)] + COMMON + DEVICE + SOURCE_FLAGS.get(src.name, [])
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"compile failed: {src.name}\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
