@@ -46,7 +46,7 @@ def _sources():
 # Per-source device flags.  mlp.hip: MFMA accumulators in VGPRs, not AGPRs
 # (gfx950 allows either), which saves the guiding network's inference kernel
 # its 96 accumulator-to-VGPR copies per 32-vertex tile.
-SOURCE_FLAGS = {"mlp.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
+SOURCE_FLAGS = {src: ["-mllvm", "-amdgpu-mfma-vgpr-form"] for src in ("mlp.hip", "smlp.hip")}
 
 
 def _compile(src: Path) -> Path:
